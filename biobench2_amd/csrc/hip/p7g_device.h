@@ -1,0 +1,118 @@
+// Device-side data structures shared by the kernels and the runtime.
+#pragma once
+#include <stdint.h>
+#include <stddef.h>
+
+// Residue code used for rows past the end of a target and for the tail
+// padding of each sequence in the HBM layout: every score against it is
+// "impossible", so extra rows never change a result.
+#define P7G_PADCODE 31
+
+// Number of transition rows in the device Viterbi and Forward/Backward tables.
+#define P7G_VF_NT 8    // BM MM IM DM (into k), MDs (M_{k-1}->D_k), MI II (out of k), DDs (D_{k-1}->D_k)
+#define P7G_FB_NT 13   // the 8 above + MMn IMn DMn (into k+1), MD DD (out of k)
+enum { VT_BM = 0, VT_MM, VT_IM, VT_DM, VT_MDS, VT_MI, VT_II, VT_DDS,
+       FT_MMN = 8, FT_IMN, FT_DMN, FT_MD, FT_DD };
+
+struct DevProfile {
+  int M, Kp;
+  // MSV stage, ungapped (SSV) pass: int16 pairs of (bias - rbv), 16 or 64 lanes per target
+  int   ssv_G, ssv_ND, ssv_ND4;
+  int  *ssv_tbl;     size_t ssv_dwords;     // [Kp+1 rows incl. pad][G][ND4]
+  // MSV stage, full recurrence with the J state: same scores, 64 lanes
+  int   mf_ND, mf_ND4;
+  int  *mf_tbl;      size_t mf_dwords;      // [rows][64][ND4]
+  // Viterbi filter: int16 pairs, 64 lanes, ND dwords per lane
+  int   vf_ND;
+  int  *vf_tbl;      size_t vf_dwords;      // [(rows) + 8][64][ND]
+  // Forward/Backward: float, 64 lanes, B positions per lane
+  int   fb_B;
+  float *fb_tbl;     size_t fb_floats;      // [(rows) + 13][64][B]
+  int   nrows;       // Kp rounded up to include P7G_PADCODE (32)
+  // per-length constants, L = 0..Lmax
+  int      Lmax;
+  uint8_t *tjb;
+  int16_t *xw_move;
+  float   *xf_move, *xf_loop, *nullsc, *flen_a, *flen_b;
+  // bias-filter HMM odds, [32 codes][2 states]
+  float   *eo;
+  // scalars
+  int   tbm, tec, base, bias;
+  float scale_b;
+  int   base_w, ddbound_w, xwE_move, xwE_loop, xwN_loop, xwC_loop, xwJ_loop;
+  float scale_w;
+  float xfE_move, xfE_loop;
+  float nj;
+  float ft1_0, ft1_1;     // filter HMM state-1 transitions
+  float pi0, pi1;
+  void *blob;             // single device allocation holding everything above
+};
+
+// A device-resident target set: residues grouped by length (longest first),
+// each sequence starting on a 16-byte boundary and padded with P7G_PADCODE.
+struct DevSeqs {
+  const uint8_t *res;
+  const int64_t *off;    // [slot]
+  const int32_t *len;    // [slot]
+  int64_t nslot;
+};
+
+// Per-slot results written by the stage kernels.
+struct DevResults {
+  uint16_t *zmax;      // SSV pass: best ungapped score above the B floor
+  int16_t  *xj;        // MSV: final xJ, -1 on overflow
+  float    *nullsc, *usc, *filtersc, *vfsc, *fwdsc, *bcksc;
+  uint8_t  *stage;
+  int      *vstatus;   // Viterbi status (0 ok, 16 overflow)
+  int      *fstatus;   // Forward/Backward status
+};
+
+// Work lists: counts live in device memory so that stages chain without
+// host round trips.
+struct DevLists {
+  int32_t *full, *bias, *vit, *fwd, *bck;
+  int32_t *counts;     // [0] full, [1] bias, [2] vit, [3] fwd, [4] bck, [6] scale-pool overflow flag
+  unsigned long long *cursor;   // next free float in the Forward scale-factor pool
+};
+
+// Length model of one call. on == 0: every target uses its own length
+// (hmmsearch's per-target ReconfigLength); on == 1: the caller's current
+// oprofile/bg scalars are used as they are (single-target drop-ins).
+struct LenCfg {
+  int   on;
+  int   tjb, xmove;
+  float floop, fmove;
+  float p1, q1;          // bias filter HMM t[0][0], t[0][1]
+};
+
+struct GateThresh {
+  float T1m;   // MSV/bias gate at F1 (MSV Gumbel)
+  float T2m;   // "P <= F2" at the bias stage: skip the Viterbi filter
+  float T2v;   // Viterbi gate at F2 (Viterbi Gumbel)
+  float T3f;   // Forward gate at F3 (exponential tail)
+  int   do_bias, do_max;
+};
+
+#ifdef __HIPCC__
+#include <hip/hip_runtime.h>
+// launchers (p7g_kernels.hip)
+hipError_t launch_ssv(const DevProfile &p, const DevSeqs &s, const int32_t *slots, int64_t n, uint16_t *zmax,
+                      hipStream_t st);
+hipError_t launch_msv_classify(const DevProfile &p, const DevSeqs &s, const int32_t *slots, int64_t n,
+                               const uint16_t *zmax, int16_t *xj, int32_t *full_list, int32_t *full_count,
+                               LenCfg lc, hipStream_t st);
+hipError_t launch_msv_full(const DevProfile &p, const DevSeqs &s, const int32_t *list, const int32_t *count,
+                           int64_t maxn, int16_t *xj, LenCfg lc, hipStream_t st);
+hipError_t launch_msv_gate(const DevProfile &p, const DevSeqs &s, int64_t n, const int16_t *xj, DevResults r,
+                           DevLists l, GateThresh g, hipStream_t st);
+hipError_t launch_bias(const DevProfile &p, const DevSeqs &s, const int32_t *list, const int32_t *count, int64_t maxn,
+                       DevResults r, DevLists l, GateThresh g, int gating, LenCfg lc, hipStream_t st);
+hipError_t launch_vit(const DevProfile &p, const DevSeqs &s, const int32_t *list, const int32_t *count, int64_t maxn,
+                      DevResults r, DevLists l, GateThresh g, int gating, LenCfg lc, hipStream_t st);
+hipError_t launch_fwd(const DevProfile &p, const DevSeqs &s, const int32_t *list, const int32_t *count, int64_t maxn,
+                      DevResults r, DevLists l, GateThresh g, int gating, LenCfg lc,
+                      float *scale_pool, int64_t pool_size, int64_t *scale_off, float *xmx_out, hipStream_t st);
+hipError_t launch_bck(const DevProfile &p, const DevSeqs &s, const int32_t *list, const int32_t *count, int64_t maxn,
+                      DevResults r, const float *scale_pool, const int64_t *scale_off, const float *fwd_xmx,
+                      float *xmx_out, LenCfg lc, hipStream_t st);
+#endif

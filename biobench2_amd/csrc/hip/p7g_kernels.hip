@@ -1,0 +1,893 @@
+// gfx950 kernels of the HMMER3 filter pipeline.
+//
+// Layout shared by all DP kernels: a target's model positions k = 1..M are
+// split into contiguous blocks, one block per lane of a G-lane group
+// (G = 16 or 64). Lane z owns positions z*B + 1 .. z*B + B, so the row-to-row
+// dependency M(i,k) <- M(i-1,k-1) stays inside a lane except at block starts,
+// where one DPP lane shift (row_shr / wave_shr) brings the neighbour's last
+// value. The integer stages hold two positions per dword as int16 halves:
+// dword d carries positions j = d (low half) and j = d + ND (high half), so a
+// whole row shifts with one DPP move plus one v_alignbit on the last dword.
+//
+// MSV stage (HMMER/src/impl_sse/msvfilter.c:76-210): the uchar recurrence
+//   sv = subs_u8(adds_u8(max(M(i-1,k-1), xB), bias), rbv[x][k])
+// is evaluated exactly, in two passes:
+//   1. k_ssv: while the J state is unused xB is a per-target constant, and with
+//      Z = max(M, xB) - xB the row update becomes Z(i,k) = max(Z(i-1,k-1) + r, 0),
+//      r = bias - rbv. Holding Z - 32768 in int16, the floor at 0 is exactly the
+//      saturation of v_pk_add_i16 ... clamp, so a cell costs one packed add and
+//      one packed max (the running maximum). The pass records max Z per target.
+//   2. k_msv_classify decides per target: overflow (some row reaches
+//      xE + bias >= 255), exact score (J never above base, xJ = max xE - tec), or
+//      "needs the J state" -> k_msv_full runs the row-serial recurrence.
+// Viterbi filter (vitfilter.c:84-249): int16 pairs with v_pk_add_i16 clamp =
+// _mm_adds_epi16 and v_pk_max_i16 = _mm_max_epi16; the lazy-F condition
+// (Dmax + ddbound > xB) is evaluated per row exactly as the reference does, and
+// when it holds the D->D closure is a max-plus prefix scan across the wave.
+// Forward/Backward parsers (fwdback.c:258-735): fp32, the D->D chain as an
+// affine prefix scan across the wave, sparse rescaling when xE > 1e4.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <math.h>
+#include "p7g_device.h"
+
+typedef short s2v __attribute__((ext_vector_type(2)));
+
+#define NEG_PAIR 0x80008000u     // (-32768, -32768)
+#define LOG2_D   0.69314718055994529
+
+__device__ __forceinline__ int   s2i(s2v v) { return __builtin_bit_cast(int, v); }
+__device__ __forceinline__ s2v   i2s(int v) { return __builtin_bit_cast(s2v, v); }
+__device__ __forceinline__ int   adds2(int a, int b) { return s2i(__builtin_elementwise_add_sat(i2s(a), i2s(b))); }
+__device__ __forceinline__ int   max2(int a, int b)  { return s2i(__builtin_elementwise_max(i2s(a), i2s(b))); }
+__device__ __forceinline__ int   lo16(int v) { return (int)(short)(v & 0xffff); }
+__device__ __forceinline__ int   hi16(int v) { return (int)(short)((unsigned)v >> 16); }
+__device__ __forceinline__ int   pair16(int v) { return (int)(((unsigned)v & 0xffffu) | ((unsigned)v << 16)); }
+
+// DPP lane moves. Lanes without a source receive `fill`.
+__device__ __forceinline__ int wave_shr1(int v, int fill) { return __builtin_amdgcn_update_dpp(fill, v, 0x138, 0xf, 0xf, false); }
+__device__ __forceinline__ int wave_shl1(int v, int fill) { return __builtin_amdgcn_update_dpp(fill, v, 0x130, 0xf, 0xf, false); }
+__device__ __forceinline__ int row_shr1(int v, int fill)  { return __builtin_amdgcn_update_dpp(fill, v, 0x111, 0xf, 0xf, false); }
+__device__ __forceinline__ float wave_shr1f(float v) { return __builtin_bit_cast(float, wave_shr1(__builtin_bit_cast(int, v), 0)); }
+__device__ __forceinline__ float wave_shl1f(float v) { return __builtin_bit_cast(float, wave_shl1(__builtin_bit_cast(int, v), 0)); }
+
+// Row shift of a packed block: the dword that becomes the new position pair 0.
+template <int G>
+__device__ __forceinline__ int shift_in(int last, int fill)
+{
+  int nb = (G == 16) ? row_shr1(last, fill) : wave_shr1(last, fill);
+  return (int) __builtin_amdgcn_alignbit((unsigned) last, (unsigned) nb, 16);
+}
+
+__device__ __forceinline__ int wave_max_i(int v)
+{
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
+  return v;
+}
+__device__ __forceinline__ float wave_sum_f(float v)
+{
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+__device__ __forceinline__ int   uni(int v)   { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ float unif(float v){ return __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, v))); }
+
+__device__ __forceinline__ int64_t wave_id()  { return (int64_t) blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); }
+__device__ __forceinline__ int64_t wave_cnt() { return (int64_t) gridDim.x * (blockDim.x >> 6); }
+
+template <typename T>
+__device__ __forceinline__ void stage_table(T *lds, const T *__restrict__ g, size_t n)
+{
+  for (size_t i = threadIdx.x; i < n; i += blockDim.x) lds[i] = g[i];
+  __syncthreads();
+}
+
+// Residue i (0-based) of a target whose residues start 16-byte aligned.
+__device__ __forceinline__ int residue_at(const uint8_t *sq, int i) { return sq[i]; }
+
+/*****************************************************************
+ * 1. MSV stage, ungapped pass (constant xB)
+ *****************************************************************/
+template <int G, int ND>
+__global__ __launch_bounds__(256)
+void k_ssv(const int *__restrict__ tbl, int tbl_dwords, int ND4,
+           const uint8_t *__restrict__ res, const int64_t *__restrict__ off, const int32_t *__restrict__ len,
+           const int32_t *__restrict__ slots, int64_t n, uint16_t *__restrict__ zmax)
+{
+  extern __shared__ int lds[];
+  stage_table(lds, tbl, (size_t) tbl_dwords);
+
+  constexpr int TPW = 64 / G;
+  const int lane = threadIdx.x & 63;
+  const int grp  = lane / G;
+  const int z    = lane % G;
+  const int zoff = z * ND4 * 4;
+  const int rowbytes = G * ND4 * 4;
+  const char *ldsb = (const char *) lds;
+
+  for (int64_t w = wave_id(); w * TPW < n; w += wave_cnt()) {
+    const int64_t t = w * TPW + grp;
+    const bool valid = t < n;
+    const int64_t slot = valid ? (slots ? (int64_t) slots[t] : t) : 0;
+    const int L = valid ? len[slot] : 0;
+    const uint8_t *sq = res + (valid ? off[slot] : 0);
+    int Lw = L;
+    if (G == 16) { Lw = max(Lw, __shfl_xor(Lw, 16, 64)); Lw = max(Lw, __shfl_xor(Lw, 32, 64)); }
+    Lw = uni(Lw);
+
+    int zr[ND];
+#pragma unroll
+    for (int d = 0; d < ND; d++) zr[d] = (int) NEG_PAIR;
+    int acc = (int) NEG_PAIR;
+
+    for (int i0 = 0; i0 < Lw; i0 += 16) {
+      uint4 ch;
+      if (i0 < L) ch = *(const uint4 *) (sq + i0);
+      else        ch = make_uint4(0x1f1f1f1fu, 0x1f1f1f1fu, 0x1f1f1f1fu, 0x1f1f1f1fu);
+      auto step = [&](int x) {
+        const int *row = (const int *) (ldsb + x * rowbytes + zoff);
+        const int s0 = shift_in<G>(zr[ND-1], (int) NEG_PAIR);
+#pragma unroll
+        for (int d = ND-1; d >= 1; d--) { zr[d] = adds2(zr[d-1], row[d]); acc = max2(acc, zr[d]); }
+        zr[0] = adds2(s0, row[0]);
+        acc = max2(acc, zr[0]);
+      };
+      if (i0 + 16 <= Lw) {
+#pragma unroll
+        for (int u = 0; u < 16; u++) {
+          const unsigned wd = (u < 4) ? ch.x : (u < 8) ? ch.y : (u < 12) ? ch.z : ch.w;
+          step((int) ((wd >> (8 * (u & 3))) & 0xffu));
+        }
+      } else {
+        const int nrow = Lw - i0;
+        for (int u = 0; u < nrow; u++) {
+          const unsigned wd = (u < 4) ? ch.x : (u < 8) ? ch.y : (u < 12) ? ch.z : ch.w;
+          step((int) ((wd >> (8 * (u & 3))) & 0xffu));
+        }
+      }
+    }
+    int a = max(lo16(acc), hi16(acc));
+#pragma unroll
+    for (int o = G/2; o > 0; o >>= 1) a = max(a, __shfl_xor(a, o, 64));
+    if (valid && z == 0) zmax[slot] = (uint16_t) (a + 32768);
+  }
+}
+
+/*****************************************************************
+ * 2. MSV stage: per-target decision after the ungapped pass
+ *****************************************************************/
+__global__ void k_msv_classify(const DevProfile p, const int32_t *__restrict__ len, const int32_t *__restrict__ slots,
+                               int64_t n, const uint16_t *__restrict__ zmax, int16_t *__restrict__ xj,
+                               int32_t *__restrict__ full_list, int32_t *__restrict__ full_count, LenCfg lc)
+{
+  for (int64_t t = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (int64_t) gridDim.x * blockDim.x) {
+    const int64_t slot = slots ? (int64_t) slots[t] : t;
+    const int L    = len[slot];
+    const int tjbm = ((lc.on ? lc.tjb : (int) p.tjb[L]) + p.tbm) & 0xff;
+    const int xB   = max(p.base - tjbm, 0);
+    const int zm   = zmax[slot];
+    int v;
+    if (zm == 0)                          v = -2;          // no cell above the B floor: rescore exactly
+    else if (xB + zm + p.bias >= 255)     v = -1;          // some row overflows (MSV values only grow with xB)
+    else if (xB + zm > p.base + p.tec)    v = -2;          // the J state would raise xB
+    else                                  v = max(xB + zm - p.tec, 0);
+    if (v == -2) { int at = atomicAdd(full_count, 1); full_list[at] = (int32_t) slot; v = 0; }
+    xj[slot] = (int16_t) v;
+  }
+}
+
+/*****************************************************************
+ * 3. MSV stage, full recurrence (one target per wave, 64 lanes)
+ *****************************************************************/
+template <int ND>
+__global__ __launch_bounds__(256)
+void k_msv_full(const DevProfile p, const int *__restrict__ tbl, int tbl_dwords, int ND4,
+                const uint8_t *__restrict__ res, const int64_t *__restrict__ off, const int32_t *__restrict__ len,
+                const int32_t *__restrict__ list, const int32_t *__restrict__ count, int16_t *__restrict__ xj, LenCfg lc)
+{
+  if ((int64_t) blockIdx.x * (blockDim.x >> 6) >= *count) return;
+  extern __shared__ int lds[];
+  stage_table(lds, tbl, (size_t) tbl_dwords);
+  const int z = threadIdx.x & 63;
+  const int zoff = z * ND4 * 4;
+  const int rowbytes = 64 * ND4 * 4;
+  const char *ldsb = (const char *) lds;
+  const int64_t n = *count;
+
+  for (int64_t t = wave_id(); t < n; t += wave_cnt()) {
+    const int slot = uni(list[t]);
+    const int L = uni(len[slot]);
+    const uint8_t *sq = res + off[slot];
+    const int tjbm = ((lc.on ? lc.tjb : (int) p.tjb[L]) + p.tbm) & 0xff;
+    int xJ = 0;
+    int xB = max(p.base - tjbm, 0);
+    int mp[ND];
+#pragma unroll
+    for (int d = 0; d < ND; d++) mp[d] = 0;
+    bool overflow = false;
+    for (int i = 0; i < L; i++) {
+      const int x = uni(residue_at(sq, i));
+      const int *row = (const int *) (ldsb + x * rowbytes + zoff);
+      const int s0 = shift_in<64>(mp[ND-1], 0);
+      const int xB2 = pair16(xB);
+      int E = 0;
+#pragma unroll
+      for (int d = ND-1; d >= 0; d--) {
+        int src = (d > 0) ? mp[d-1] : s0;
+        int tv = max2(src, xB2);
+        tv = adds2(tv, row[d]);
+        tv = max2(tv, 0);
+        E  = max2(E, tv);
+        mp[d] = tv;
+      }
+      const int xE = uni(wave_max_i(max(lo16(E), hi16(E))));
+      if (xE + p.bias >= 255) { overflow = true; break; }
+      xJ = max(xJ, max(xE - p.tec, 0));
+      xB = max(max(p.base, xJ) - tjbm, 0);
+    }
+    if (z == 0) xj[slot] = (int16_t) (overflow ? -1 : xJ);
+  }
+}
+
+/*****************************************************************
+ * 4. MSV gate: score, null score, F1 (and F2 when the bias filter is off)
+ *****************************************************************/
+__global__ void k_msv_gate(const DevProfile p, const int32_t *__restrict__ len, int64_t n,
+                           const int16_t *__restrict__ xj, DevResults r, DevLists l, GateThresh g)
+{
+#pragma clang fp contract(off)
+  for (int64_t slot = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; slot < n; slot += (int64_t) gridDim.x * blockDim.x) {
+    const int L = len[slot];
+    const int v = xj[slot];
+    const float nullsc = p.nullsc[L];
+    float usc;
+    if (v < 0) usc = INFINITY;
+    else {
+      usc = __fdiv_rn((float) (v - (int) p.tjb[L]) - (float) p.base, p.scale_b);
+      usc = (float) ((double) usc - 3.0);
+    }
+    r.usc[slot] = usc;
+    r.nullsc[slot] = nullsc;
+    uint8_t stage = 0;
+    if (L > 0) {
+      const float seq = (float) __ddiv_rn((double) (usc - nullsc), LOG2_D);
+      if (seq >= g.T1m) {
+        stage = 1;
+        if (g.do_bias) {
+          int at = atomicAdd(&l.counts[1], 1); l.bias[at] = (int32_t) slot;
+        } else {
+          r.filtersc[slot] = nullsc;
+          if (seq >= g.T2m) { int at = atomicAdd(&l.counts[3], 1); l.fwd[at] = (int32_t) slot; stage = 3; }
+          else              { int at = atomicAdd(&l.counts[2], 1); l.vit[at] = (int32_t) slot; stage = 2; }
+        }
+      }
+    }
+    r.stage[slot] = stage;
+  }
+}
+
+/*****************************************************************
+ * 5. Bias-composition filter: 2-state HMM Forward with per-row max
+ *    normalisation (easel/esl_hmm.c:351-409, HMMER/src/p7_bg.c:473-484).
+ *    One target per thread; float arithmetic in the reference's order.
+ *****************************************************************/
+__global__ void k_bias(const DevProfile p, const uint8_t *__restrict__ res, const int64_t *__restrict__ off,
+                       const int32_t *__restrict__ len, const int32_t *__restrict__ list, const int32_t *__restrict__ count,
+                       DevResults r, DevLists l, GateThresh g, int gating, LenCfg lc)
+{
+#pragma clang fp contract(off)
+  const int64_t n = *count;
+  for (int64_t t = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (int64_t) gridDim.x * blockDim.x) {
+    const int slot = list[t];
+    const int L = len[slot];
+    const uint8_t *sq = res + off[slot];
+    const float p1  = lc.on ? lc.p1 : __fdiv_rn((float) L, (float) (L + 1));
+    const float t00 = p1, t01 = lc.on ? lc.q1 : 1.0f - p1;
+    const float t10 = p.ft1_0, t11 = p.ft1_1;
+    float logsc = 0.0f;
+    float d0, d1, mx;
+    int x = sq[0];
+    d0 = p.eo[x*2 + 0] * p.pi0;
+    d1 = p.eo[x*2 + 1] * p.pi1;
+    mx = 0.0f;
+    mx = (d0 > mx) ? d0 : mx;
+    mx = (d1 > mx) ? d1 : mx;
+    d0 = __fdiv_rn(d0, mx); d1 = __fdiv_rn(d1, mx);
+    logsc += (float) log((double) mx);
+    for (int i = 1; i < L; i++) {
+      x = sq[i];
+      float n0 = 0.0f, n1 = 0.0f;
+      n0 += d0 * t00; n0 += d1 * t10; n0 *= p.eo[x*2 + 0];
+      mx = 0.0f; mx = (n0 > mx) ? n0 : mx;
+      n1 += d0 * t01; n1 += d1 * t11; n1 *= p.eo[x*2 + 1];
+      mx = (n1 > mx) ? n1 : mx;
+      d0 = __fdiv_rn(n0, mx); d1 = __fdiv_rn(n1, mx);
+      logsc += (float) log((double) mx);
+    }
+    float s = 0.0f;
+    s += d0 * 1.0f; s += d1 * 1.0f;
+    logsc += (float) log((double) s);
+    // with a caller-supplied length model the host adds the length terms
+    const float filtersc = lc.on ? logsc : (logsc + p.flen_a[L]) + p.flen_b[L];
+    r.filtersc[slot] = filtersc;
+    if (gating) {
+      const float seq = (float) __ddiv_rn((double) (r.usc[slot] - filtersc), LOG2_D);
+      if (seq < g.T1m) r.stage[slot] = 1;
+      else if (seq >= g.T2m) { int at = atomicAdd(&l.counts[3], 1); l.fwd[at] = slot; r.stage[slot] = 3; }
+      else                   { int at = atomicAdd(&l.counts[2], 1); l.vit[at] = slot; r.stage[slot] = 2; }
+    }
+  }
+}
+
+/*****************************************************************
+ * 6. Viterbi filter (one target per wave, 64 lanes, int16 pairs)
+ *****************************************************************/
+template <int ND>
+__device__ __forceinline__ void dd_closure(int (&D)[ND], const int *__restrict__ tdd, int z)
+{
+  // D(k) = max(D0(k), D(k-1) + tDD(k-1)), exact in int32 because every tDD <= 0
+  constexpr int NB = 2 * ND;
+  const int NEG = -(1 << 28);
+  int v[NB], a[NB];
+#pragma unroll
+  for (int d = 0; d < ND; d++) { v[d] = lo16(D[d]); v[d+ND] = hi16(D[d]); a[d] = lo16(tdd[d]); a[d+ND] = hi16(tdd[d]); }
+  int cur = NEG, A = 0;
+#pragma unroll
+  for (int j = 0; j < NB; j++) { cur = max(v[j], cur + a[j]); A += a[j]; }
+  int C = cur;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    int A2 = __shfl_up(A, o, 64), C2 = __shfl_up(C, o, 64);
+    if (z >= o) { C = max(C, C2 + A); A = A + A2; A = max(A, NEG); C = max(C, NEG); }
+  }
+  int cin = __shfl_up(C, 1, 64);
+  if (z == 0) cin = NEG;
+  cur = cin;
+#pragma unroll
+  for (int j = 0; j < NB; j++) { cur = max(v[j], cur + a[j]); v[j] = cur; }
+#pragma unroll
+  for (int d = 0; d < ND; d++) D[d] = (int) (((unsigned) v[d] & 0xffffu) | ((unsigned) v[d+ND] << 16));
+}
+
+template <int ND>
+__global__ __launch_bounds__(256)
+void k_vit(const DevProfile p, const uint8_t *__restrict__ res, const int64_t *__restrict__ off,
+           const int32_t *__restrict__ len, const int32_t *__restrict__ list, const int32_t *__restrict__ count,
+           DevResults r, DevLists l, GateThresh g, int gating, LenCfg lc)
+{
+#pragma clang fp contract(off)
+  if ((int64_t) blockIdx.x * (blockDim.x >> 6) >= *count) return;
+  extern __shared__ int lds[];
+  const size_t rowd = (size_t) 64 * ND;
+  stage_table(lds, p.vf_tbl, p.vf_dwords);
+  const int z = threadIdx.x & 63;
+  const int *tr = lds + (size_t) p.nrows * rowd + z * ND;
+  int tBM[ND], tMM[ND], tIM[ND], tDM[ND], tMDS[ND], tMI[ND], tII[ND];
+#pragma unroll
+  for (int d = 0; d < ND; d++) {
+    tBM[d] = tr[VT_BM*rowd + d]; tMM[d] = tr[VT_MM*rowd + d]; tIM[d] = tr[VT_IM*rowd + d]; tDM[d] = tr[VT_DM*rowd + d];
+    tMDS[d] = tr[VT_MDS*rowd + d]; tMI[d] = tr[VT_MI*rowd + d]; tII[d] = tr[VT_II*rowd + d];
+  }
+  const int64_t n = *count;
+
+  for (int64_t t = wave_id(); t < n; t += wave_cnt()) {
+    const int slot = uni(list[t]);
+    const int L = uni(len[slot]);
+    const uint8_t *sq = res + off[slot];
+    const int xmove = lc.on ? lc.xmove : (int) p.xw_move[L];
+    int xN = p.base_w;
+    int xB = (int16_t) (xN + xmove);
+    int xJ = -32768, xC = -32768;
+    int M[ND], I[ND], D[ND];
+#pragma unroll
+    for (int d = 0; d < ND; d++) { M[d] = I[d] = D[d] = (int) NEG_PAIR; }
+    int status = 0;
+
+    for (int i = 0; i < L; i++) {
+      const int x = uni(residue_at(sq, i));
+      const int *e = lds + (size_t) x * rowd + z * ND;
+      const int ms0 = shift_in<64>(M[ND-1], (int) NEG_PAIR);
+      const int is0 = shift_in<64>(I[ND-1], (int) NEG_PAIR);
+      const int ds0 = shift_in<64>(D[ND-1], (int) NEG_PAIR);
+      const int xB2 = pair16(xB);
+      int xE = (int) NEG_PAIR;
+#pragma unroll
+      for (int d = ND-1; d >= 0; d--) {
+        const int mp = d ? M[d-1] : ms0, ip = d ? I[d-1] : is0, dp = d ? D[d-1] : ds0;
+        int sv = adds2(xB2, tBM[d]);
+        sv = max2(sv, adds2(mp, tMM[d]));
+        sv = max2(sv, adds2(ip, tIM[d]));
+        sv = max2(sv, adds2(dp, tDM[d]));
+        sv = adds2(sv, e[d]);
+        xE = max2(xE, sv);
+        I[d] = max2(adds2(M[d], tMI[d]), adds2(I[d], tII[d]));
+        M[d] = sv;
+      }
+      // D(k) <- M(k-1) + tMD(k-1) on the new row
+      const int sm0 = shift_in<64>(M[ND-1], (int) NEG_PAIR);
+      int Dm = (int) NEG_PAIR;
+#pragma unroll
+      for (int d = 0; d < ND; d++) { D[d] = adds2(d ? M[d-1] : sm0, tMDS[d]); Dm = max2(Dm, D[d]); }
+      const int xEs = uni(wave_max_i(max(lo16(xE), hi16(xE))));
+      if (xEs >= 32767) { status = 16; break; }
+      xN = (int16_t) (xN + p.xwN_loop);
+      xC = (int16_t) max(xC + p.xwC_loop, xEs + p.xwE_move);
+      xJ = (int16_t) max(xJ + p.xwJ_loop, xEs + p.xwE_loop);
+      xB = (int16_t) max(xJ + xmove, xN + xmove);
+      const int Dmax = uni(wave_max_i(max(lo16(Dm), hi16(Dm))));
+      if (Dmax + p.ddbound_w > xB) dd_closure<ND>(D, tr + VT_DDS*rowd, z);
+    }
+    if (z == 0) {
+      float sc;
+      if (status == 16) sc = INFINITY;
+      else if (xC > -32768) {
+        sc = (float) xC + (float) xmove - (float) p.base_w;
+        sc = __fdiv_rn(sc, p.scale_w);
+        sc = (float) ((double) sc - 3.0);
+      } else sc = -INFINITY;
+      r.vfsc[slot] = sc;
+      r.vstatus[slot] = status;
+      if (gating) {
+        const float seq = (float) __ddiv_rn((double) (sc - r.filtersc[slot]), LOG2_D);
+        if (seq >= g.T2v) { int at = atomicAdd(&l.counts[3], 1); l.fwd[at] = slot; r.stage[slot] = 3; }
+        else r.stage[slot] = 2;
+      }
+    }
+  }
+}
+
+/*****************************************************************
+ * 7. Forward parser (one target per wave, fp32, B positions per lane)
+ *****************************************************************/
+template <int B>
+__device__ __forceinline__ void affine_fwd_scan(float (&D)[B], const float (&a)[B], int z)
+{
+  // D(j) = D0(j) + a(j) * D(j-1), carried across lanes in increasing k.
+  float P = 1.0f, C = 0.0f;
+#pragma unroll
+  for (int j = 0; j < B; j++) { C = D[j] + a[j] * C; P *= a[j]; }
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    float P2 = __shfl_up(P, o, 64), C2 = __shfl_up(C, o, 64);
+    if (z >= o) { C = C + P * C2; P = P * P2; }
+  }
+  float cin = __shfl_up(C, 1, 64);
+  if (z == 0) cin = 0.0f;
+  float cur = cin;
+#pragma unroll
+  for (int j = 0; j < B; j++) { cur = D[j] + a[j] * cur; D[j] = cur; }
+}
+
+template <int B>
+__device__ __forceinline__ void affine_bck_scan(float (&D)[B], const float (&a)[B], int z)
+{
+  // D(j) = c(j) + a(j) * D(j+1), carried across lanes in decreasing k.
+  float P = 1.0f, C = 0.0f;
+#pragma unroll
+  for (int j = B-1; j >= 0; j--) { C = D[j] + a[j] * C; P *= a[j]; }
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    float P2 = __shfl_down(P, o, 64), C2 = __shfl_down(C, o, 64);
+    if (z + o < 64) { C = C + P * C2; P = P * P2; }
+  }
+  float cin = __shfl_down(C, 1, 64);
+  if (z == 63) cin = 0.0f;
+  float cur = cin;
+#pragma unroll
+  for (int j = B-1; j >= 0; j--) { cur = D[j] + a[j] * cur; D[j] = cur; }
+}
+
+template <int B, bool LDS>
+__global__ __launch_bounds__(256)
+void k_fwd(const DevProfile p, const uint8_t *__restrict__ res, const int64_t *__restrict__ off,
+           const int32_t *__restrict__ len, const int32_t *__restrict__ list, const int32_t *__restrict__ count,
+           DevResults r, DevLists l, GateThresh g, int gating, LenCfg lc,
+           float *__restrict__ pool, int64_t pool_size, int64_t *__restrict__ scale_off, float *__restrict__ xmx_out)
+{
+  if ((int64_t) blockIdx.x * (blockDim.x >> 6) >= *count) return;
+  extern __shared__ float ldsf[];
+  const size_t rowf = (size_t) 64 * B;
+  const float *tbl = p.fb_tbl;
+  if (LDS) { stage_table(ldsf, p.fb_tbl, p.fb_floats); tbl = ldsf; }
+  const int z = threadIdx.x & 63;
+  const float *tr = tbl + (size_t) p.nrows * rowf + z * B;
+  float tBM[B], tMM[B], tIM[B], tDM[B], tMDS[B], tMI[B], tII[B], tDDS[B];
+#pragma unroll
+  for (int j = 0; j < B; j++) {
+    tBM[j] = tr[VT_BM*rowf + j]; tMM[j] = tr[VT_MM*rowf + j]; tIM[j] = tr[VT_IM*rowf + j]; tDM[j] = tr[VT_DM*rowf + j];
+    tMDS[j] = tr[VT_MDS*rowf + j]; tMI[j] = tr[VT_MI*rowf + j]; tII[j] = tr[VT_II*rowf + j]; tDDS[j] = tr[VT_DDS*rowf + j];
+  }
+  const int64_t n = *count;
+
+  for (int64_t t = wave_id(); t < n; t += wave_cnt()) {
+    const int slot = uni(list[t]);
+    const int L = uni(len[slot]);
+    const uint8_t *sq = res + off[slot];
+    const float floop = lc.on ? lc.floop : p.xf_loop[L], fmove = lc.on ? lc.fmove : p.xf_move[L];
+    float *scl = nullptr;
+    if (pool) {
+      unsigned long long a64 = 0;
+      if (z == 0) a64 = atomicAdd(l.cursor, (unsigned long long) (L + 1));
+      const int64_t at = ((int64_t) (unsigned) __shfl((int) (a64 >> 32), 0, 64) << 32) | (unsigned) __shfl((int) a64, 0, 64);
+      if (at + L + 1 <= pool_size) { scl = pool + at; if (z == 0) scale_off[slot] = at; }
+      else { if (z == 0) { scale_off[slot] = -1; atomicExch(&l.counts[6], 1); } }
+    }
+    float M[B], I[B], D[B];
+#pragma unroll
+    for (int j = 0; j < B; j++) M[j] = I[j] = D[j] = 0.0f;
+    float xN = 1.0f, xJ = 0.0f, xC = 0.0f, xB = fmove, xE = 0.0f;
+    float totscale = 0.0f;
+    if (xmx_out && z == 0) {
+      xmx_out[0] = 0.0f; xmx_out[1] = 1.0f; xmx_out[2] = 0.0f; xmx_out[3] = xB; xmx_out[4] = 0.0f; xmx_out[5] = 1.0f;
+    }
+    if (scl && z == 0) scl[0] = 1.0f;
+
+    for (int i = 1; i <= L; i++) {
+      const int x = uni(residue_at(sq, i-1));
+      const float *e = tbl + (size_t) x * rowf + z * B;
+      const float mL = wave_shr1f(M[B-1]), iL = wave_shr1f(I[B-1]), dL = wave_shr1f(D[B-1]);
+      float xEl = 0.0f;
+#pragma unroll
+      for (int j = B-1; j >= 0; j--) {
+        const float mp = j ? M[j-1] : mL, ip = j ? I[j-1] : iL, dp = j ? D[j-1] : dL;
+        const float In = M[j] * tMI[j] + I[j] * tII[j];
+        float sv = xB * tBM[j] + mp * tMM[j] + ip * tIM[j] + dp * tDM[j];
+        sv = sv * e[j];
+        M[j] = sv; I[j] = In;
+        xEl += sv;
+      }
+      const float mS = wave_shr1f(M[B-1]);
+#pragma unroll
+      for (int j = 0; j < B; j++) D[j] = (j ? M[j-1] : mS) * tMDS[j];
+      affine_fwd_scan<B>(D, tDDS, z);
+#pragma unroll
+      for (int j = 0; j < B; j++) xEl += D[j];
+      xE = unif(wave_sum_f(xEl));
+      xN = xN * floop;
+      xC = (xC * floop) + (xE * p.xfE_move);
+      xJ = (xJ * floop) + (xE * p.xfE_loop);
+      xB = (xJ * fmove) + (xN * fmove);
+      float scale = 1.0f;
+      if (xE > 1.0e4f) {
+        xN = xN / xE; xC = xC / xE; xJ = xJ / xE; xB = xB / xE;
+        const float inv = (float) (1.0 / (double) xE);
+#pragma unroll
+        for (int j = 0; j < B; j++) { M[j] *= inv; D[j] *= inv; I[j] *= inv; }
+        scale = xE;
+        totscale = (float) ((double) totscale + log((double) xE));
+        xE = 1.0f;
+      }
+      if (scl && z == 0) scl[i] = scale;
+      if (xmx_out && z == 0) {
+        float *xr = xmx_out + (size_t) i * 6;
+        xr[0] = xE; xr[1] = xN; xr[2] = xJ; xr[3] = xB; xr[4] = xC; xr[5] = scale;
+      }
+    }
+    if (z == 0) {
+      int status = 0;
+      float sc;
+      if (isnan(xC) || (L > 0 && xC == 0.0f) || isinf(xC)) { status = 16; sc = NAN; }
+      else sc = (float) ((double) totscale + log((double) (xC * fmove)));
+      r.fwdsc[slot] = sc;
+      r.fstatus[slot] = status;
+      if (gating) {
+        const float seq = (float) __ddiv_rn((double) (sc - r.filtersc[slot]), LOG2_D);
+        if (status == 0 && seq >= g.T3f) { int at = atomicAdd(&l.counts[4], 1); l.bck[at] = slot; r.stage[slot] = 4; }
+        else r.stage[slot] = 3;
+      }
+    }
+  }
+}
+
+/*****************************************************************
+ * 8. Backward parser (one target per wave), with Forward's scale factors
+ *****************************************************************/
+template <int B, bool LDS>
+__global__ __launch_bounds__(256)
+void k_bck(const DevProfile p, const uint8_t *__restrict__ res, const int64_t *__restrict__ off,
+           const int32_t *__restrict__ len, const int32_t *__restrict__ list, const int32_t *__restrict__ count,
+           DevResults r, const float *__restrict__ pool, const int64_t *__restrict__ scale_off,
+           const float *__restrict__ fwd_xmx, float *__restrict__ xmx_out, LenCfg lc)
+{
+  if ((int64_t) blockIdx.x * (blockDim.x >> 6) >= *count) return;
+  extern __shared__ float ldsf[];
+  const size_t rowf = (size_t) 64 * B;
+  const float *tbl = p.fb_tbl;
+  if (LDS) { stage_table(ldsf, p.fb_tbl, p.fb_floats); tbl = ldsf; }
+  const int z = threadIdx.x & 63;
+  const float *tr = tbl + (size_t) p.nrows * rowf + z * B;
+  float tBM[B], tMMN[B], tIMN[B], tDMN[B], tMD[B], tDD[B], tMI[B], tII[B];
+#pragma unroll
+  for (int j = 0; j < B; j++) {
+    tBM[j] = tr[VT_BM*rowf + j]; tMMN[j] = tr[FT_MMN*rowf + j]; tIMN[j] = tr[FT_IMN*rowf + j]; tDMN[j] = tr[FT_DMN*rowf + j];
+    tMD[j] = tr[FT_MD*rowf + j]; tDD[j] = tr[FT_DD*rowf + j]; tMI[j] = tr[VT_MI*rowf + j]; tII[j] = tr[VT_II*rowf + j];
+  }
+  const int64_t n = *count;
+
+  for (int64_t t = wave_id(); t < n; t += wave_cnt()) {
+    const int slot = uni(list[t]);
+    const int L = uni(len[slot]);
+    const uint8_t *sq = res + off[slot];
+    const float floop = lc.on ? lc.floop : p.xf_loop[L], fmove = lc.on ? lc.fmove : p.xf_move[L];
+    const float *fsc = nullptr;       // forward row scale factors, fsc[i], i = 0..L
+    int fstride = 1;
+    if (fwd_xmx) { fsc = fwd_xmx + 5; fstride = 6; }
+    else {
+      const int64_t so = scale_off[slot];
+      if (so < 0) { if (z == 0) { r.bcksc[slot] = NAN; r.fstatus[slot] = 5; } continue; }
+      fsc = pool + so;
+    }
+    float M[B], I[B], D[B];
+    float xJ = 0.0f, xB = 0.0f, xN = 0.0f;
+    float xC = fmove;
+    float xE = xC * p.xfE_move;
+#pragma unroll
+    for (int j = 0; j < B; j++) { M[j] = xE; D[j] = xE; I[j] = 0.0f; }
+    affine_bck_scan<B>(D, tDD, z);
+    {
+      const float dS = wave_shl1f(D[0]);
+#pragma unroll
+      for (int j = 0; j < B; j++) M[j] = M[j] + (j < B-1 ? D[j+1] : dS) * tMD[j];
+    }
+    int own = 0;
+    float totscale;
+    {
+      const float s = fsc[(size_t) L * fstride];
+      if (s > 1.0f) {
+        xE = xE / s; xN = xN / s; xC = xC / s; xJ = xJ / s; xB = xB / s;
+        const float inv = (float) (1.0 / (double) s);
+#pragma unroll
+        for (int j = 0; j < B; j++) { M[j] *= inv; D[j] *= inv; I[j] *= inv; }
+      }
+      totscale = (float) log((double) s);
+      if (xmx_out && z == 0) {
+        float *xr = xmx_out + (size_t) L * 6;
+        xr[0] = xE; xr[1] = xN; xr[2] = xJ; xr[3] = xB; xr[4] = xC; xr[5] = s;
+      }
+    }
+    for (int i = L-1; i >= 1; i--) {
+      const int x = uni(residue_at(sq, i));           // residue i+1
+      const float *e = tbl + (size_t) x * rowf + z * B;
+      float mn[B];
+      float xBl = 0.0f;
+#pragma unroll
+      for (int j = 0; j < B; j++) { mn[j] = M[j] * e[j]; xBl += mn[j] * tBM[j]; }
+      const float mS = wave_shl1f(mn[0]);
+#pragma unroll
+      for (int j = 0; j < B; j++) {
+        const float ms = (j < B-1) ? mn[j+1] : mS;
+        const float ip = I[j];
+        I[j] = ip * tII[j] + ms * tIMN[j];
+        D[j] = ms * tDMN[j];
+        M[j] = ip * tMI[j] + ms * tMMN[j];
+      }
+      xB = unif(wave_sum_f(xBl));
+      xC = xC * floop;
+      xJ = (xB * fmove) + (xJ * floop);
+      xN = (xB * fmove) + (xN * floop);
+      xE = (xC * p.xfE_move) + (xJ * p.xfE_loop);
+#pragma unroll
+      for (int j = 0; j < B; j++) D[j] = D[j] + xE;
+      affine_bck_scan<B>(D, tDD, z);
+      const float dS = wave_shl1f(D[0]);
+#pragma unroll
+      for (int j = 0; j < B; j++) M[j] = M[j] + xE + (j < B-1 ? D[j+1] : dS) * tMD[j];
+      if (xB > 1.0e16f) own = 1;
+      const float s = own ? ((xB > 1.0e4f) ? xB : 1.0f) : fsc[(size_t) i * fstride];
+      if (s > 1.0f) {
+        xE /= s; xN /= s; xJ /= s; xB /= s; xC /= s;
+        const float inv = (float) (1.0 / (double) s);
+#pragma unroll
+        for (int j = 0; j < B; j++) { M[j] *= inv; D[j] *= inv; I[j] *= inv; }
+        totscale = (float) ((double) totscale + log((double) s));
+      }
+      if (xmx_out && z == 0) {
+        float *xr = xmx_out + (size_t) i * 6;
+        xr[0] = xE; xr[1] = xN; xr[2] = xJ; xr[3] = xB; xr[4] = xC; xr[5] = s;
+      }
+    }
+    {
+      const int x = uni(residue_at(sq, 0));
+      const float *e = tbl + (size_t) x * rowf + z * B;
+      float xBl = 0.0f;
+#pragma unroll
+      for (int j = 0; j < B; j++) xBl += (M[j] * e[j]) * tBM[j];
+      xB = unif(wave_sum_f(xBl));
+      xN = (xB * fmove) + (xN * floop);
+    }
+    if (z == 0) {
+      if (xmx_out) {
+        xmx_out[0] = 0.0f; xmx_out[1] = xN; xmx_out[2] = 0.0f; xmx_out[3] = xB; xmx_out[4] = 0.0f; xmx_out[5] = 1.0f;
+      }
+      int status = 0;
+      float sc;
+      if (isnan(xN) || (L > 0 && xN == 0.0f) || isinf(xN)) { status = 16; sc = NAN; }
+      else sc = (float) ((double) totscale + log((double) xN));
+      r.bcksc[slot] = sc;
+      r.fstatus[slot] = status | (own ? 0x100 : 0);
+    }
+  }
+}
+
+/*****************************************************************
+ * Launchers
+ *****************************************************************/
+static int g_ncu = 0, g_lds_block = 0, g_lds_cu = 0;
+static void dev_props()
+{
+  if (g_ncu) return;
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) dev = 0;
+  if (hipDeviceGetAttribute(&g_ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || g_ncu <= 0) g_ncu = 256;
+  if (hipDeviceGetAttribute(&g_lds_block, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess || g_lds_block <= 0)
+    g_lds_block = 64 * 1024;
+  if (hipDeviceGetAttribute(&g_lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev) != hipSuccess || g_lds_cu <= 0)
+    g_lds_cu = g_lds_block;
+}
+static int ncu() { dev_props(); return g_ncu; }
+static int lds_block_limit() { dev_props(); return g_lds_block; }
+
+// Launch with dynamic LDS; raises the per-kernel limit when a table needs more than 64 KB.
+static hipError_t launch(void *k, unsigned grid, unsigned block, void **args, size_t lds, hipStream_t st)
+{
+  if (lds > (size_t) lds_block_limit()) return hipErrorInvalidValue;
+  if (lds > 64 * 1024) (void) hipFuncSetAttribute((const void *) k, hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds);
+  return hipLaunchKernel(k, dim3(grid), dim3(block), args, lds, st);
+}
+
+static unsigned grid_for_waves(int64_t nwaves, size_t lds_bytes)
+{
+  // persistent-style grid: enough 4-wave workgroups to fill the chip, bounded by the work
+  int per_cu = 8;
+  dev_props();
+  if (lds_bytes > 0) { per_cu = (int) ((size_t) g_lds_cu / lds_bytes); if (per_cu < 1) per_cu = 1; if (per_cu > 8) per_cu = 8; }
+  int64_t g = (int64_t) ncu() * per_cu;
+  int64_t need = (nwaves + 3) / 4;
+  if (need < g) g = need;
+  if (g < 1) g = 1;
+  return (unsigned) g;
+}
+
+#define SSV16_CASE(N) case N: k = (void *) k_ssv<16, N>; break;
+#define SSV64_CASE(N) case N: k = (void *) k_ssv<64, N>; break;
+
+hipError_t launch_ssv(const DevProfile &p, const DevSeqs &s, const int32_t *slots, int64_t n, uint16_t *zmax,
+                      hipStream_t st)
+{
+  if (n <= 0) return hipSuccess;
+  void *k = nullptr;
+  if (p.ssv_G == 16) {
+    switch (p.ssv_ND) {
+      SSV16_CASE(1) SSV16_CASE(2) SSV16_CASE(3) SSV16_CASE(4) SSV16_CASE(5) SSV16_CASE(6) SSV16_CASE(7) SSV16_CASE(8)
+      SSV16_CASE(9) SSV16_CASE(10) SSV16_CASE(11) SSV16_CASE(12) SSV16_CASE(13) SSV16_CASE(14) SSV16_CASE(15) SSV16_CASE(16)
+      SSV16_CASE(17) SSV16_CASE(18) SSV16_CASE(19) SSV16_CASE(20) SSV16_CASE(21) SSV16_CASE(22) SSV16_CASE(23) SSV16_CASE(24)
+      SSV16_CASE(25) SSV16_CASE(26) SSV16_CASE(27) SSV16_CASE(28) SSV16_CASE(29) SSV16_CASE(30) SSV16_CASE(31) SSV16_CASE(32)
+      default: return hipErrorInvalidValue;
+    }
+  } else {
+    switch (p.ssv_ND) {
+      SSV64_CASE(8) SSV64_CASE(9) SSV64_CASE(10) SSV64_CASE(11) SSV64_CASE(12) SSV64_CASE(13) SSV64_CASE(14) SSV64_CASE(15)
+      SSV64_CASE(16) SSV64_CASE(18) SSV64_CASE(20) SSV64_CASE(22) SSV64_CASE(24) SSV64_CASE(26) SSV64_CASE(28) SSV64_CASE(30)
+      SSV64_CASE(32) SSV64_CASE(36) SSV64_CASE(40)
+      default: return hipErrorInvalidValue;
+    }
+  }
+  const size_t lds = p.ssv_dwords * 4;
+  const int64_t nwaves = (n + (64 / p.ssv_G) - 1) / (64 / p.ssv_G);
+  const unsigned grid = grid_for_waves(nwaves, lds);
+  int tbl_dwords = (int) p.ssv_dwords;
+  int ND4 = p.ssv_ND4;
+  void *args[] = { (void *) &p.ssv_tbl, &tbl_dwords, &ND4, (void *) &s.res, (void *) &s.off, (void *) &s.len,
+                   (void *) &slots, &n, &zmax };
+  return launch(k, grid, 256, args, lds, st);
+}
+
+hipError_t launch_msv_classify(const DevProfile &p, const DevSeqs &s, const int32_t *slots, int64_t n,
+                               const uint16_t *zmax, int16_t *xj, int32_t *full_list, int32_t *full_count,
+                               LenCfg lc, hipStream_t st)
+{
+  if (n <= 0) return hipSuccess;
+  unsigned grid = (unsigned) ((n + 255) / 256); if (grid > 65536) grid = 65536;
+  hipLaunchKernelGGL(k_msv_classify, dim3(grid), dim3(256), 0, st, p, s.len, slots, n, zmax, xj, full_list, full_count, lc);
+  return hipGetLastError();
+}
+
+#define MF_CASE(N) case N: k = (void *) k_msv_full<N>; break;
+hipError_t launch_msv_full(const DevProfile &p, const DevSeqs &s, const int32_t *list, const int32_t *count,
+                           int64_t maxn, int16_t *xj, LenCfg lc, hipStream_t st)
+{
+  if (maxn <= 0) return hipSuccess;
+  void *k = nullptr;
+  switch (p.mf_ND) {
+    MF_CASE(1) MF_CASE(2) MF_CASE(3) MF_CASE(4) MF_CASE(5) MF_CASE(6) MF_CASE(8) MF_CASE(10) MF_CASE(12)
+    MF_CASE(16) MF_CASE(20) MF_CASE(24) MF_CASE(32) MF_CASE(40)
+    default: return hipErrorInvalidValue;
+  }
+  const size_t lds = p.mf_dwords * 4;
+  const unsigned grid = grid_for_waves(maxn, lds);
+  int tbl_dwords = (int) p.mf_dwords, ND4 = p.mf_ND4;
+  void *args[] = { (void *) &p, (void *) &p.mf_tbl, &tbl_dwords, &ND4, (void *) &s.res, (void *) &s.off, (void *) &s.len,
+                   (void *) &list, (void *) &count, &xj, &lc };
+  return launch(k, grid, 256, args, lds, st);
+}
+
+hipError_t launch_msv_gate(const DevProfile &p, const DevSeqs &s, int64_t n, const int16_t *xj, DevResults r,
+                           DevLists l, GateThresh g, hipStream_t st)
+{
+  if (n <= 0) return hipSuccess;
+  unsigned grid = (unsigned) ((n + 255) / 256); if (grid > 65536) grid = 65536;
+  hipLaunchKernelGGL(k_msv_gate, dim3(grid), dim3(256), 0, st, p, s.len, n, xj, r, l, g);
+  return hipGetLastError();
+}
+
+hipError_t launch_bias(const DevProfile &p, const DevSeqs &s, const int32_t *list, const int32_t *count, int64_t maxn,
+                       DevResults r, DevLists l, GateThresh g, int gating, LenCfg lc, hipStream_t st)
+{
+  if (maxn <= 0) return hipSuccess;
+  unsigned grid = (unsigned) ((maxn + 63) / 64); if (grid > 8192) grid = 8192;
+  hipLaunchKernelGGL(k_bias, dim3(grid), dim3(64), 0, st, p, s.res, s.off, s.len, list, count, r, l, g, gating, lc);
+  return hipGetLastError();
+}
+
+#define VIT_CASE(N) case N: k = (void *) k_vit<N>; break;
+hipError_t launch_vit(const DevProfile &p, const DevSeqs &s, const int32_t *list, const int32_t *count, int64_t maxn,
+                      DevResults r, DevLists l, GateThresh g, int gating, LenCfg lc, hipStream_t st)
+{
+  if (maxn <= 0) return hipSuccess;
+  void *k = nullptr;
+  switch (p.vf_ND) {
+    VIT_CASE(1) VIT_CASE(2) VIT_CASE(3) VIT_CASE(4) VIT_CASE(5) VIT_CASE(6) VIT_CASE(8) VIT_CASE(10) VIT_CASE(12)
+    VIT_CASE(16) VIT_CASE(20) VIT_CASE(24) VIT_CASE(32) VIT_CASE(40)
+    default: return hipErrorInvalidValue;
+  }
+  const size_t lds = p.vf_dwords * 4;
+  const unsigned grid = grid_for_waves(maxn, lds);
+  void *args[] = { (void *) &p, (void *) &s.res, (void *) &s.off, (void *) &s.len, (void *) &list, (void *) &count,
+                   &r, &l, &g, &gating, &lc };
+  return launch(k, grid, 256, args, lds, st);
+}
+
+#define FB_LDS_LIMIT (150 * 1024)
+#define FWD_CASE(N) case N: k = lds_ok ? (void *) k_fwd<N, true> : (void *) k_fwd<N, false>; break;
+hipError_t launch_fwd(const DevProfile &p, const DevSeqs &s, const int32_t *list, const int32_t *count, int64_t maxn,
+                      DevResults r, DevLists l, GateThresh g, int gating, LenCfg lc,
+                      float *scale_pool, int64_t pool_size, int64_t *scale_off, float *xmx_out, hipStream_t st)
+{
+  if (maxn <= 0) return hipSuccess;
+  const bool lds_ok = p.fb_floats * 4 <= FB_LDS_LIMIT && p.fb_floats * 4 <= (size_t) lds_block_limit();
+  void *k = nullptr;
+  switch (p.fb_B) {
+    FWD_CASE(1) FWD_CASE(2) FWD_CASE(3) FWD_CASE(4) FWD_CASE(5) FWD_CASE(6) FWD_CASE(7) FWD_CASE(8)
+    FWD_CASE(10) FWD_CASE(12) FWD_CASE(14) FWD_CASE(16) FWD_CASE(20) FWD_CASE(24) FWD_CASE(28) FWD_CASE(32)
+    FWD_CASE(40) FWD_CASE(48) FWD_CASE(64) FWD_CASE(80)
+    default: return hipErrorInvalidValue;
+  }
+  const size_t lds = lds_ok ? p.fb_floats * 4 : 0;
+  const unsigned grid = grid_for_waves(maxn, lds);
+  void *args[] = { (void *) &p, (void *) &s.res, (void *) &s.off, (void *) &s.len, (void *) &list, (void *) &count,
+                   &r, &l, &g, &gating, &lc, &scale_pool, &pool_size, &scale_off, &xmx_out };
+  return launch(k, grid, 256, args, lds, st);
+}
+
+#define BCK_CASE(N) case N: k = lds_ok ? (void *) k_bck<N, true> : (void *) k_bck<N, false>; break;
+hipError_t launch_bck(const DevProfile &p, const DevSeqs &s, const int32_t *list, const int32_t *count, int64_t maxn,
+                      DevResults r, const float *scale_pool, const int64_t *scale_off, const float *fwd_xmx,
+                      float *xmx_out, LenCfg lc, hipStream_t st)
+{
+  if (maxn <= 0) return hipSuccess;
+  const bool lds_ok = p.fb_floats * 4 <= FB_LDS_LIMIT && p.fb_floats * 4 <= (size_t) lds_block_limit();
+  void *k = nullptr;
+  switch (p.fb_B) {
+    BCK_CASE(1) BCK_CASE(2) BCK_CASE(3) BCK_CASE(4) BCK_CASE(5) BCK_CASE(6) BCK_CASE(7) BCK_CASE(8)
+    BCK_CASE(10) BCK_CASE(12) BCK_CASE(14) BCK_CASE(16) BCK_CASE(20) BCK_CASE(24) BCK_CASE(28) BCK_CASE(32)
+    BCK_CASE(40) BCK_CASE(48) BCK_CASE(64) BCK_CASE(80)
+    default: return hipErrorInvalidValue;
+  }
+  const size_t lds = lds_ok ? p.fb_floats * 4 : 0;
+  const unsigned grid = grid_for_waves(maxn, lds);
+  void *args[] = { (void *) &p, (void *) &s.res, (void *) &s.off, (void *) &s.len, (void *) &list, (void *) &count,
+                   &r, (void *) &scale_pool, (void *) &scale_off, (void *) &fwd_xmx, &xmx_out, &lc };
+  return launch(k, grid, 256, args, lds, st);
+}

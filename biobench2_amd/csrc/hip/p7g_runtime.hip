@@ -1,0 +1,968 @@
+// C ABI of libp7gpu: device setup, device copies of profiles and target
+// databases, batched filter stages, the device pipeline, and the
+// single-target drop-ins. One HIP stream per process, default priority.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <math.h>
+#include <mutex>
+#include <vector>
+#include <algorithm>
+
+#include "p7_gpu.h"
+#include "p7g_internal.h"
+#include "p7g_device.h"
+
+extern "C" const char *p7g_host_error(void);
+
+static hipStream_t g_stream = nullptr;
+static int         g_device = -1;
+static std::mutex  g_lock;     // serialises calls that share the stream and scratch buffers
+
+#define HIPCHK(call) do { hipError_t e_ = (call); if (e_ != hipSuccess) { \
+      p7g_set_error("HIP error %s at %s:%d (%s)", hipGetErrorString(e_), __FILE__, __LINE__, #call); \
+      return p7G_EDEVICE; } } while (0)
+
+extern "C" int
+p7_gpu_Init(int device)
+{
+  std::lock_guard<std::mutex> lk(g_lock);
+  if (g_stream && device == g_device) return p7G_OK;
+  int n = 0;
+  HIPCHK(hipGetDeviceCount(&n));
+  if (device < 0 || device >= n) { p7g_set_error("no HIP device %d (%d present)", device, n); return p7G_EINVAL; }
+  HIPCHK(hipSetDevice(device));
+  if (g_stream) { (void) hipStreamDestroy(g_stream); g_stream = nullptr; }
+  HIPCHK(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking));
+  g_device = device;
+  return p7G_OK;
+}
+
+static int
+ensure_init(void)
+{
+  if (g_stream) return p7G_OK;
+  return p7_gpu_Init(0);
+}
+
+extern "C" int
+p7_gpu_Synchronize(void)
+{
+  if (! g_stream) return p7G_OK;
+  HIPCHK(hipStreamSynchronize(g_stream));
+  return p7G_OK;
+}
+
+extern "C" const char *p7_gpu_LastError(void) { return p7g_host_error(); }
+
+/*****************************************************************
+ * Device profile: re-laid-out tables for 16/64-lane blocks, per-length
+ * constants, and the bias-filter HMM, in one allocation.
+ *****************************************************************/
+struct p7g_dev_profile_s {
+  DevProfile p;
+  int        Lmax;
+};
+
+extern "C" void
+p7g_dev_profile_free(struct p7g_dev_profile_s *d)
+{
+  if (! d) return;
+  if (d->p.blob) (void) hipFree(d->p.blob);
+  delete d;
+}
+
+static const int vf_nd_set[] = { 1, 2, 3, 4, 5, 6, 8, 10, 12, 16, 20, 24, 32, 40 };
+static const int fb_b_set[]  = { 1, 2, 3, 4, 5, 6, 7, 8, 10, 12, 14, 16, 20, 24, 28, 32, 40, 48, 64, 80 };
+static const int ssv64_set[] = { 8, 9, 10, 11, 12, 13, 14, 15, 16, 18, 20, 22, 24, 26, 28, 30, 32, 36, 40 };
+
+static int pick(const int *set, int nset, int need)
+{
+  for (int i = 0; i < nset; i++) if (set[i] >= need) return set[i];
+  return -1;
+}
+static int cdiv(int a, int b) { return (a + b - 1) / b; }
+// per-lane stride in dwords: a multiple of 4 whose quarter is odd, so that
+// 16-byte LDS reads of consecutive lanes land on different bank groups
+static int odd_quad_stride(int nd) { int s = ((nd + 3) / 4) * 4; if (((s / 4) & 1) == 0) s += 4; return s; }
+
+static inline int pack2(int lo, int hi) { return (int) (((unsigned) lo & 0xffffu) | ((unsigned) hi << 16)); }
+
+static int
+dev_profile_build(P7_GPU_OPROFILE *om, int Lmax_need, struct p7g_dev_profile_s **ret)
+{
+  const int M = om->M, Kp = om->abc->Kp;
+  const int NR = 32;                                 // residue rows incl. P7G_PADCODE
+  auto *d = new p7g_dev_profile_s();
+  DevProfile &p = d->p;
+  memset(&p, 0, sizeof(p));
+  p.M = M; p.Kp = Kp; p.nrows = NR;
+
+  // ---- SSV / MSV-full tables: int16 (bias - rbv) ----
+  auto rsv = [&](int x, int k) -> int {               // score at node k (1-based) or "impossible"
+    if (x >= Kp || k < 1 || k > M) return -32768;
+    return (int) om->bias_b - (int) om->rbv[(size_t) x * M + k - 1];
+  };
+  if (M <= 1024) { p.ssv_G = 16; p.ssv_ND = std::max(1, cdiv(cdiv(M, 16), 2)); }
+  else {
+    p.ssv_G = 64; p.ssv_ND = pick(ssv64_set, (int) (sizeof(ssv64_set)/sizeof(int)), cdiv(cdiv(M, 64), 2));
+    if (p.ssv_ND < 0) { delete d; p7g_set_error("model too long for the device MSV stage (M=%d)", M); return p7G_EINVAL; }
+  }
+  p.ssv_ND4 = odd_quad_stride(p.ssv_ND);
+  p.ssv_dwords = (size_t) NR * p.ssv_G * p.ssv_ND4;
+  std::vector<int> ssv(p.ssv_dwords, (int) 0x80008000u);
+  {
+    const int G = p.ssv_G, ND = p.ssv_ND, B = 2 * ND;
+    for (int x = 0; x < NR; x++)
+      for (int z = 0; z < G; z++)
+        for (int dd = 0; dd < ND; dd++)
+          ssv[((size_t) x * G + z) * p.ssv_ND4 + dd] = pack2(rsv(x, z*B + dd + 1), rsv(x, z*B + dd + ND + 1));
+  }
+  p.mf_ND = pick(vf_nd_set, (int) (sizeof(vf_nd_set)/sizeof(int)), std::max(1, cdiv(cdiv(M, 64), 2)));
+  if (p.mf_ND < 0) { delete d; p7g_set_error("model too long for the device MSV stage (M=%d)", M); return p7G_EINVAL; }
+  p.mf_ND4 = p.mf_ND;
+  p.mf_dwords = (size_t) NR * 64 * p.mf_ND4;
+  std::vector<int> mf(p.mf_dwords);
+  {
+    const int ND = p.mf_ND, B = 2 * ND;
+    for (int x = 0; x < NR; x++)
+      for (int z = 0; z < 64; z++)
+        for (int dd = 0; dd < ND; dd++)
+          mf[((size_t) x * 64 + z) * ND + dd] = pack2(rsv(x, z*B + dd + 1), rsv(x, z*B + dd + ND + 1));
+  }
+
+  // ---- Viterbi tables: int16 pairs, capacity >= M+1 ----
+  p.vf_ND = pick(vf_nd_set, (int) (sizeof(vf_nd_set)/sizeof(int)), cdiv(cdiv(M + 1, 64), 2));
+  if (p.vf_ND < 0) { delete d; p7g_set_error("model too long for the device Viterbi stage (M=%d)", M); return p7G_EINVAL; }
+  p.vf_dwords = (size_t) (NR + P7G_VF_NT) * 64 * p.vf_ND;
+  std::vector<int> vf(p.vf_dwords);
+  {
+    const int ND = p.vf_ND, B = 2 * ND;
+    auto tw = [&](int t, int k) -> int { return (k >= 1 && k <= M) ? (int) om->twv[(size_t) t * M + k - 1] : -32768; };
+    auto ew = [&](int x, int k) -> int { return (x < Kp && k >= 1 && k <= M) ? (int) om->rwv[(size_t) x * M + k - 1] : -32768; };
+    auto tv = [&](int row, int k) -> int {
+      switch (row) {
+      case VT_BM: return tw(p7G_BM, k);
+      case VT_MM: return tw(p7G_MM, k);
+      case VT_IM: return tw(p7G_IM, k);
+      case VT_DM: return tw(p7G_DM, k);
+      case VT_MDS: return tw(p7G_MD, k - 1);      // M_{k-1} -> D_k
+      case VT_MI: return tw(p7G_MI, k);
+      case VT_II: return tw(p7G_II, k);
+      case VT_DDS: return tw(p7G_DD, k - 1);      // D_{k-1} -> D_k
+      }
+      return -32768;
+    };
+    for (int r = 0; r < NR + P7G_VF_NT; r++)
+      for (int z = 0; z < 64; z++)
+        for (int dd = 0; dd < ND; dd++) {
+          const int k0 = z*B + dd + 1, k1 = z*B + dd + ND + 1;
+          int a, b;
+          if (r < NR) { a = ew(r, k0); b = ew(r, k1); }
+          else        { a = tv(r - NR, k0); b = tv(r - NR, k1); }
+          vf[((size_t) r * 64 + z) * ND + dd] = pack2(a, b);
+        }
+  }
+
+  // ---- Forward/Backward tables: float, capacity >= M+1 ----
+  p.fb_B = pick(fb_b_set, (int) (sizeof(fb_b_set)/sizeof(int)), cdiv(M + 1, 64));
+  if (p.fb_B < 0) { delete d; p7g_set_error("model too long for the device Forward stage (M=%d)", M); return p7G_EINVAL; }
+  p.fb_floats = (size_t) (NR + P7G_FB_NT) * 64 * p.fb_B;
+  std::vector<float> fb(p.fb_floats);
+  {
+    const int B = p.fb_B;
+    auto tf = [&](int t, int k) -> float { return (k >= 1 && k <= M) ? om->tfv[(size_t) t * M + k - 1] : 0.0f; };
+    auto ef = [&](int x, int k) -> float { return (x < Kp && k >= 1 && k <= M) ? om->rfv[(size_t) x * M + k - 1] : 0.0f; };
+    auto tv = [&](int row, int k) -> float {
+      switch (row) {
+      case VT_BM: return tf(p7G_BM, k);
+      case VT_MM: return tf(p7G_MM, k);
+      case VT_IM: return tf(p7G_IM, k);
+      case VT_DM: return tf(p7G_DM, k);
+      case VT_MDS: return tf(p7G_MD, k - 1);
+      case VT_MI: return tf(p7G_MI, k);
+      case VT_II: return tf(p7G_II, k);
+      case VT_DDS: return tf(p7G_DD, k - 1);
+      case FT_MMN: return tf(p7G_MM, k + 1);     // M_k -> M_{k+1}
+      case FT_IMN: return tf(p7G_IM, k + 1);
+      case FT_DMN: return tf(p7G_DM, k + 1);
+      case FT_MD:  return tf(p7G_MD, k);         // M_k -> D_{k+1}
+      case FT_DD:  return tf(p7G_DD, k);
+      }
+      return 0.0f;
+    };
+    for (int r = 0; r < NR + P7G_FB_NT; r++)
+      for (int z = 0; z < 64; z++)
+        for (int j = 0; j < B; j++) {
+          const int k = z*B + j + 1;
+          fb[((size_t) r * 64 + z) * B + j] = (r < NR) ? ef(r, k) : tv(r - NR, k);
+        }
+  }
+
+  // ---- per-length constants ----
+  P7G_LENCFG lc;
+  const int Lmax = std::max(Lmax_need, 65536);
+  if (p7g_lencfg_build(om, Lmax, &lc) != p7G_OK) { delete d; return p7G_EMEM; }
+  p.Lmax = Lmax;
+
+  // ---- bias filter HMM: odds ratios of the two states (p7_bg_SetFilter) ----
+  std::vector<float> eo((size_t) NR * 2, 1.0f);
+  {
+    P7_GPU_BG *bg = p7_gpu_bg_Create(om->abc->type);
+    if (! bg) { p7g_lencfg_free(&lc); delete d; return p7G_EMEM; }
+    float compo[P7G_MAXABET];
+    for (int x = 0; x < om->abc->K; x++) compo[x] = om->compo[x];
+    p7_gpu_bg_SetFilter(bg, M, compo);
+    for (int x = 0; x < Kp; x++) { eo[x*2] = bg->eo[x][0]; eo[x*2+1] = bg->eo[x][1]; }
+    p.ft1_0 = bg->ft[1][0]; p.ft1_1 = bg->ft[1][1]; p.pi0 = bg->pi[0]; p.pi1 = bg->pi[1];
+    p7_gpu_bg_Destroy(bg);
+  }
+
+  // ---- scalars ----
+  p.tbm = om->tbm_b; p.tec = om->tec_b; p.base = om->base_b; p.bias = om->bias_b; p.scale_b = om->scale_b;
+  p.base_w = om->base_w; p.ddbound_w = om->ddbound_w; p.scale_w = om->scale_w;
+  p.xwE_move = om->xw[p7G_XE][p7G_MOVE]; p.xwE_loop = om->xw[p7G_XE][p7G_LOOP];
+  p.xwN_loop = om->xw[p7G_XN][p7G_LOOP]; p.xwC_loop = om->xw[p7G_XC][p7G_LOOP]; p.xwJ_loop = om->xw[p7G_XJ][p7G_LOOP];
+  p.xfE_move = om->xf[p7G_XE][p7G_MOVE]; p.xfE_loop = om->xf[p7G_XE][p7G_LOOP];
+  p.nj = om->nj;
+
+  // ---- one allocation ----
+  const size_t L1 = (size_t) Lmax + 1;
+  size_t sz[] = { p.ssv_dwords * 4, p.mf_dwords * 4, p.vf_dwords * 4, p.fb_floats * 4,
+                  L1, L1 * 2, L1 * 4, L1 * 4, L1 * 4, L1 * 4, L1 * 4, eo.size() * 4 };
+  const void *src[] = { ssv.data(), mf.data(), vf.data(), fb.data(), lc.tjb_b, lc.xw_move, lc.xf_move, lc.xf_loop,
+                        lc.nullsc, lc.flen_a, lc.flen_b, eo.data() };
+  const int NB = (int) (sizeof(sz) / sizeof(sz[0]));
+  size_t offs[16], total = 0;
+  for (int i = 0; i < NB; i++) { offs[i] = total; total += (sz[i] + 255) & ~(size_t) 255; }
+  char *blob = nullptr;
+  hipError_t e = hipMalloc((void **) &blob, total);
+  if (e == hipSuccess) {
+    for (int i = 0; i < NB && e == hipSuccess; i++) e = hipMemcpyAsync(blob + offs[i], src[i], sz[i], hipMemcpyHostToDevice, g_stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(g_stream);
+  }
+  p7g_lencfg_free(&lc);
+  if (e != hipSuccess) {
+    if (blob) (void) hipFree(blob);
+    delete d;
+    p7g_set_error("device profile upload failed: %s", hipGetErrorString(e));
+    return p7G_EDEVICE;
+  }
+  p.blob = blob;
+  p.ssv_tbl = (int *) (blob + offs[0]); p.mf_tbl = (int *) (blob + offs[1]);
+  p.vf_tbl = (int *) (blob + offs[2]);  p.fb_tbl = (float *) (blob + offs[3]);
+  p.tjb = (uint8_t *) (blob + offs[4]); p.xw_move = (int16_t *) (blob + offs[5]);
+  p.xf_move = (float *) (blob + offs[6]); p.xf_loop = (float *) (blob + offs[7]);
+  p.nullsc = (float *) (blob + offs[8]); p.flen_a = (float *) (blob + offs[9]); p.flen_b = (float *) (blob + offs[10]);
+  p.eo = (float *) (blob + offs[11]);
+  d->Lmax = Lmax;
+  *ret = d;
+  return p7G_OK;
+}
+
+static int
+dev_profile(const P7_GPU_OPROFILE *om_c, int Lmax_need, DevProfile **ret)
+{
+  P7_GPU_OPROFILE *om = (P7_GPU_OPROFILE *) om_c;      // the device copy is a cache of the const tables
+  int status;
+  if (om->M <= 0) { p7g_set_error("oprofile has no model"); return p7G_EINVAL; }
+  if (om->mode != p7G_LOCAL && om->mode != p7G_UNILOCAL) { p7g_set_error("the filters need a local-mode profile"); return p7G_EINVAL; }
+  if (om->dev && om->dev->Lmax < Lmax_need) { p7g_dev_profile_free(om->dev); om->dev = nullptr; }
+  if (! om->dev && (status = dev_profile_build(om, Lmax_need, &om->dev)) != p7G_OK) return status;
+  *ret = &om->dev->p;
+  return p7G_OK;
+}
+
+/*****************************************************************
+ * Target databases in HBM
+ *****************************************************************/
+struct p7g_seqdb_s {
+  int64_t  nseq, nres, bytes;
+  int      Lmax, maxcode;
+  int32_t *h_len;          // [slot]
+  int64_t *h_perm;         // slot -> caller index
+  int32_t *h_inv;          // caller index -> slot
+  uint8_t *d_res;
+  int64_t *d_off;
+  int32_t *d_len;
+};
+
+extern "C" void
+p7_gpu_seqdb_Destroy(P7_GPU_SEQDB *db)
+{
+  if (! db) return;
+  if (db->d_res) (void) hipFree(db->d_res);
+  if (db->d_off) (void) hipFree(db->d_off);
+  if (db->d_len) (void) hipFree(db->d_len);
+  free(db->h_len); free(db->h_perm); free(db->h_inv);
+  free(db);
+}
+
+extern "C" int
+p7_gpu_seqdb_Create(const uint8_t *residues, const int64_t *offset, const int32_t *length, int64_t nseq, P7_GPU_SEQDB **ret_db)
+{
+  int status;
+  *ret_db = nullptr;
+  if ((status = ensure_init()) != p7G_OK) return status;
+  if (nseq <= 0 || nseq > INT32_MAX - 1) { p7g_set_error("bad database size %lld", (long long) nseq); return p7G_EINVAL; }
+  P7_GPU_SEQDB *db = (P7_GPU_SEQDB *) calloc(1, sizeof(P7_GPU_SEQDB));
+  if (! db) return p7G_EMEM;
+  db->nseq = nseq;
+  db->h_len  = (int32_t *) malloc(sizeof(int32_t) * (size_t) nseq);
+  db->h_perm = (int64_t *) malloc(sizeof(int64_t) * (size_t) nseq);
+  db->h_inv  = (int32_t *) malloc(sizeof(int32_t) * (size_t) nseq);
+  int64_t *h_off = (int64_t *) malloc(sizeof(int64_t) * (size_t) nseq);
+  if (! db->h_len || ! db->h_perm || ! db->h_inv || ! h_off) { free(h_off); p7_gpu_seqdb_Destroy(db); return p7G_EMEM; }
+
+  for (int64_t i = 0; i < nseq; i++) {
+    if (length[i] < 0 || length[i] > 100000000) { free(h_off); p7_gpu_seqdb_Destroy(db); p7g_set_error("bad length %d for sequence %lld", length[i], (long long) i); return p7G_EINVAL; }
+    db->h_perm[i] = i;
+  }
+  // longest first; equal lengths keep the caller's order
+  std::stable_sort(db->h_perm, db->h_perm + nseq, [&](int64_t a, int64_t b) { return length[a] > length[b]; });
+  int64_t pos = 0;
+  db->Lmax = 0;
+  for (int64_t s = 0; s < nseq; s++) {
+    const int64_t i = db->h_perm[s];
+    db->h_inv[i] = (int32_t) s;
+    db->h_len[s] = length[i];
+    h_off[s] = pos;
+    pos += ((int64_t) length[i] + 15) & ~(int64_t) 15;
+    db->nres += length[i];
+    if (length[i] > db->Lmax) db->Lmax = length[i];
+  }
+  db->bytes = pos + 16;
+  uint8_t *h_res = (uint8_t *) malloc((size_t) db->bytes);
+  if (! h_res) { free(h_off); p7_gpu_seqdb_Destroy(db); return p7G_EMEM; }
+  memset(h_res, P7G_PADCODE, (size_t) db->bytes);
+  int maxcode = 0;
+  for (int64_t s = 0; s < nseq; s++) {
+    const int64_t i = db->h_perm[s];
+    const uint8_t *src = residues + offset[i];
+    uint8_t *dst = h_res + h_off[s];
+    for (int32_t k = 0; k < length[i]; k++) { const int c = src[k]; if (c > maxcode) maxcode = c; dst[k] = (uint8_t) c; }
+  }
+  db->maxcode = maxcode;
+  if (maxcode >= 29) { free(h_res); free(h_off); p7_gpu_seqdb_Destroy(db); p7g_set_error("residue code %d is not a digital residue", maxcode); return p7G_EINVAL; }
+
+  hipError_t e = hipMalloc((void **) &db->d_res, (size_t) db->bytes);
+  if (e == hipSuccess) e = hipMalloc((void **) &db->d_off, sizeof(int64_t) * (size_t) nseq);
+  if (e == hipSuccess) e = hipMalloc((void **) &db->d_len, sizeof(int32_t) * (size_t) nseq);
+  if (e == hipSuccess) e = hipMemcpyAsync(db->d_res, h_res, (size_t) db->bytes, hipMemcpyHostToDevice, g_stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(db->d_off, h_off, sizeof(int64_t) * (size_t) nseq, hipMemcpyHostToDevice, g_stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(db->d_len, db->h_len, sizeof(int32_t) * (size_t) nseq, hipMemcpyHostToDevice, g_stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(g_stream);
+  free(h_res); free(h_off);
+  if (e != hipSuccess) { p7g_set_error("database upload failed: %s", hipGetErrorString(e)); p7_gpu_seqdb_Destroy(db); return p7G_EDEVICE; }
+  *ret_db = db;
+  return p7G_OK;
+}
+
+extern "C" int64_t p7_gpu_seqdb_NSeq(const P7_GPU_SEQDB *db) { return db ? db->nseq : 0; }
+extern "C" int64_t p7_gpu_seqdb_NRes(const P7_GPU_SEQDB *db) { return db ? db->nres : 0; }
+
+static DevSeqs seqs_of(const P7_GPU_SEQDB *db) { DevSeqs s; s.res = db->d_res; s.off = db->d_off; s.len = db->d_len; s.nslot = db->nseq; return s; }
+
+/*****************************************************************
+ * Work buffers: per-slot results and lists, grown on demand
+ *****************************************************************/
+struct Work {
+  int64_t     cap = 0;
+  DevResults  r;
+  DevLists    l;
+  int32_t    *slots = nullptr;      // explicit slot list for batched calls
+  int64_t    *scale_off = nullptr;
+  float      *pool = nullptr;
+  int64_t     pool_size = 0;
+  void       *mem = nullptr;
+  hipEvent_t  ev[8];
+  bool        ev_ready = false;
+};
+static Work g_work;
+
+static int
+work_reserve(int64_t n, int64_t pool_floats)
+{
+  Work &w = g_work;
+  if (! w.ev_ready) { for (int i = 0; i < 8; i++) HIPCHK(hipEventCreate(&w.ev[i])); w.ev_ready = true; }
+  if (n <= w.cap && pool_floats <= w.pool_size) return p7G_OK;
+  if (w.mem) { HIPCHK(hipStreamSynchronize(g_stream)); (void) hipFree(w.mem); w.mem = nullptr; w.cap = 0; }
+  if (w.pool) { (void) hipFree(w.pool); w.pool = nullptr; w.pool_size = 0; }
+  const int64_t cap = std::max<int64_t>(n, 1024);
+  size_t per = 2 + 2 + 4*6 + 1 + 4 + 4 + 4*5 + 4 + 8;   // per-slot bytes of all arrays
+  size_t total = (size_t) cap * per + 4096;
+  char *m = nullptr;
+  HIPCHK(hipMalloc((void **) &m, total));
+  size_t o = 0;
+  auto take = [&](size_t bytes) { char *p = m + o; o += (bytes + 255) & ~(size_t) 255; return (void *) p; };
+  w.r.zmax = (uint16_t *) take(2 * cap); w.r.xj = (int16_t *) take(2 * cap);
+  w.r.nullsc = (float *) take(4 * cap); w.r.usc = (float *) take(4 * cap); w.r.filtersc = (float *) take(4 * cap);
+  w.r.vfsc = (float *) take(4 * cap); w.r.fwdsc = (float *) take(4 * cap); w.r.bcksc = (float *) take(4 * cap);
+  w.r.stage = (uint8_t *) take(cap); w.r.vstatus = (int *) take(4 * cap); w.r.fstatus = (int *) take(4 * cap);
+  w.l.full = (int32_t *) take(4 * cap); w.l.bias = (int32_t *) take(4 * cap); w.l.vit = (int32_t *) take(4 * cap);
+  w.l.fwd = (int32_t *) take(4 * cap); w.l.bck = (int32_t *) take(4 * cap);
+  w.slots = (int32_t *) take(4 * cap);
+  w.scale_off = (int64_t *) take(8 * cap);
+  w.l.counts = (int32_t *) take(64);
+  w.l.cursor = (unsigned long long *) take(64);
+  if (o > total) { p7g_set_error("internal: work buffer overrun"); return p7G_EINVAL; }
+  w.mem = m; w.cap = cap;
+  const int64_t pf = std::max<int64_t>(pool_floats, 1 << 20);
+  HIPCHK(hipMalloc((void **) &w.pool, sizeof(float) * (size_t) pf));
+  w.pool_size = pf;
+  return p7G_OK;
+}
+
+static int
+reset_counts(void)
+{
+  HIPCHK(hipMemsetAsync(g_work.l.counts, 0, 64, g_stream));
+  HIPCHK(hipMemsetAsync(g_work.l.cursor, 0, 64, g_stream));
+  return p7G_OK;
+}
+
+static int
+slots_from_idx(const P7_GPU_SEQDB *db, const int64_t *idx, int64_t n, std::vector<int32_t> &slots)
+{
+  slots.resize((size_t) n);
+  for (int64_t t = 0; t < n; t++) {
+    const int64_t i = idx ? idx[t] : t;
+    if (i < 0 || i >= db->nseq) { p7g_set_error("target index %lld out of range", (long long) i); return p7G_EINVAL; }
+    slots[t] = db->h_inv[i];
+  }
+  return p7G_OK;
+}
+
+static LenCfg lencfg_per_target(void) { LenCfg lc; memset(&lc, 0, sizeof(lc)); return lc; }
+
+static float
+msv_score_from_xj(const P7_GPU_OPROFILE *om, int xj, int tjb)
+{
+  float sc = ((float) (xj - tjb) - (float) om->base_b);
+  sc /= om->scale_b;
+  sc -= 3.0;
+  return sc;
+}
+
+/*****************************************************************
+ * Batched stages (no gating): every listed target, length-configured to
+ * its own L.
+ *****************************************************************/
+extern "C" int
+p7_gpu_MSVFilterBatch(const P7_GPU_OPROFILE *om, const P7_GPU_SEQDB *db, const int64_t *idx, int64_t n,
+                      float *ret_sc, int *ret_status)
+{
+  std::lock_guard<std::mutex> lk(g_lock);
+  int status;
+  DevProfile *p;
+  if ((status = ensure_init()) != p7G_OK) return status;
+  if (db->maxcode >= om->abc->Kp) { p7g_set_error("database holds residue codes outside the model alphabet"); return p7G_EINCOMPAT; }
+  if ((status = dev_profile(om, db->Lmax, &p)) != p7G_OK) return status;
+  if (idx == nullptr) n = db->nseq;
+  if ((status = work_reserve(db->nseq, 0)) != p7G_OK) return status;
+  std::vector<int32_t> slots;
+  if ((status = slots_from_idx(db, idx, n, slots)) != p7G_OK) return status;
+  DevSeqs s = seqs_of(db);
+  Work &w = g_work;
+  LenCfg lc = lencfg_per_target();
+  if ((status = reset_counts()) != p7G_OK) return status;
+  HIPCHK(hipMemcpyAsync(w.slots, slots.data(), sizeof(int32_t) * (size_t) n, hipMemcpyHostToDevice, g_stream));
+  HIPCHK(launch_ssv(*p, s, w.slots, n, w.r.zmax, g_stream));
+  HIPCHK(launch_msv_classify(*p, s, w.slots, n, w.r.zmax, w.r.xj, w.l.full, &w.l.counts[0], lc, g_stream));
+  HIPCHK(launch_msv_full(*p, s, w.l.full, &w.l.counts[0], n, w.r.xj, lc, g_stream));
+  std::vector<int16_t> xj((size_t) db->nseq);
+  HIPCHK(hipMemcpyAsync(xj.data(), w.r.xj, sizeof(int16_t) * (size_t) db->nseq, hipMemcpyDeviceToHost, g_stream));
+  HIPCHK(hipStreamSynchronize(g_stream));
+  P7G_LENCFG lcf;
+  if ((status = p7g_lencfg_build(om, db->Lmax, &lcf)) != p7G_OK) return status;
+  for (int64_t t = 0; t < n; t++) {
+    const int slot = slots[t];
+    const int v = xj[slot];
+    if (v < 0) { ret_sc[t] = INFINITY; if (ret_status) ret_status[t] = p7G_ERANGE; }
+    else { ret_sc[t] = msv_score_from_xj(om, v, lcf.tjb_b[db->h_len[slot]]); if (ret_status) ret_status[t] = p7G_OK; }
+  }
+  p7g_lencfg_free(&lcf);
+  return p7G_OK;
+}
+
+// Shared driver of the list-based stages for a batch of explicit targets.
+enum { ST_BIAS = 1, ST_VIT = 2, ST_FWD = 3, ST_BCK = 4 };
+
+static int
+run_list_stage(int which, const P7_GPU_OPROFILE *om, const P7_GPU_SEQDB *db, const int64_t *idx, int64_t n,
+               float *ret_sc, int *ret_status)
+{
+  std::lock_guard<std::mutex> lk(g_lock);
+  int status;
+  DevProfile *p;
+  if ((status = ensure_init()) != p7G_OK) return status;
+  if (db->maxcode >= om->abc->Kp) { p7g_set_error("database holds residue codes outside the model alphabet"); return p7G_EINCOMPAT; }
+  if ((status = dev_profile(om, db->Lmax, &p)) != p7G_OK) return status;
+  if (idx == nullptr) n = db->nseq;
+  std::vector<int32_t> slots;
+  if ((status = slots_from_idx(db, idx, n, slots)) != p7G_OK) return status;
+  int64_t poolneed = 0;
+  if (which == ST_BCK) for (int64_t t = 0; t < n; t++) poolneed += db->h_len[slots[t]] + 1;
+  if ((status = work_reserve(db->nseq, poolneed)) != p7G_OK) return status;
+  for (int64_t t = 0; t < n; t++) if (db->h_len[slots[t]] == 0) { p7g_set_error("zero-length target in a batched stage"); return p7G_EINVAL; }
+  DevSeqs s = seqs_of(db);
+  Work &w = g_work;
+  LenCfg lc = lencfg_per_target();
+  GateThresh g; memset(&g, 0, sizeof(g));
+  if ((status = reset_counts()) != p7G_OK) return status;
+  int32_t *list = (which == ST_BIAS) ? w.l.bias : (which == ST_VIT) ? w.l.vit : w.l.fwd;
+  int32_t *cnt  = (which == ST_BIAS) ? &w.l.counts[1] : (which == ST_VIT) ? &w.l.counts[2] : &w.l.counts[3];
+  int32_t n32 = (int32_t) n;
+  HIPCHK(hipMemcpyAsync(list, slots.data(), sizeof(int32_t) * (size_t) n, hipMemcpyHostToDevice, g_stream));
+  HIPCHK(hipMemcpyAsync(cnt, &n32, sizeof(int32_t), hipMemcpyHostToDevice, g_stream));
+  if (which == ST_BIAS) {
+    HIPCHK(launch_bias(*p, s, list, cnt, n, w.r, w.l, g, 0, lc, g_stream));
+  } else if (which == ST_VIT) {
+    HIPCHK(launch_vit(*p, s, list, cnt, n, w.r, w.l, g, 0, lc, g_stream));
+  } else {
+    HIPCHK(launch_fwd(*p, s, list, cnt, n, w.r, w.l, g, 0, lc,
+                      which == ST_BCK ? w.pool : nullptr, w.pool_size, w.scale_off, nullptr, g_stream));
+    if (which == ST_BCK)
+      HIPCHK(launch_bck(*p, s, list, cnt, n, w.r, w.pool, w.scale_off, nullptr, nullptr, lc, g_stream));
+  }
+  std::vector<float> sc((size_t) db->nseq);
+  std::vector<int>   st((size_t) db->nseq, 0);
+  float *src = (which == ST_BIAS) ? w.r.filtersc : (which == ST_VIT) ? w.r.vfsc : (which == ST_FWD) ? w.r.fwdsc : w.r.bcksc;
+  HIPCHK(hipMemcpyAsync(sc.data(), src, sizeof(float) * (size_t) db->nseq, hipMemcpyDeviceToHost, g_stream));
+  if (which == ST_VIT) HIPCHK(hipMemcpyAsync(st.data(), w.r.vstatus, sizeof(int) * (size_t) db->nseq, hipMemcpyDeviceToHost, g_stream));
+  if (which >= ST_FWD) HIPCHK(hipMemcpyAsync(st.data(), w.r.fstatus, sizeof(int) * (size_t) db->nseq, hipMemcpyDeviceToHost, g_stream));
+  int32_t counts[16];
+  HIPCHK(hipMemcpyAsync(counts, w.l.counts, sizeof(counts), hipMemcpyDeviceToHost, g_stream));
+  HIPCHK(hipStreamSynchronize(g_stream));
+  if (which == ST_BCK && counts[6]) { p7g_set_error("Forward scale-factor pool exhausted"); return p7G_EMEM; }
+  for (int64_t t = 0; t < n; t++) {
+    ret_sc[t] = sc[slots[t]];
+    if (ret_status) ret_status[t] = (st[slots[t]] & 0xff) == 16 ? p7G_ERANGE : ((st[slots[t]] & 0xff) ? p7G_FAIL : p7G_OK);
+  }
+  return p7G_OK;
+}
+
+extern "C" int p7_gpu_ViterbiFilterBatch(const P7_GPU_OPROFILE *om, const P7_GPU_SEQDB *db, const int64_t *idx, int64_t n, float *ret_sc, int *ret_status)
+{ return run_list_stage(ST_VIT, om, db, idx, n, ret_sc, ret_status); }
+extern "C" int p7_gpu_ForwardParserBatch(const P7_GPU_OPROFILE *om, const P7_GPU_SEQDB *db, const int64_t *idx, int64_t n, float *ret_sc, int *ret_status)
+{ return run_list_stage(ST_FWD, om, db, idx, n, ret_sc, ret_status); }
+extern "C" int p7_gpu_BackwardParserBatch(const P7_GPU_OPROFILE *om, const P7_GPU_SEQDB *db, const int64_t *idx, int64_t n, float *ret_sc, int *ret_status)
+{ return run_list_stage(ST_BCK, om, db, idx, n, ret_sc, ret_status); }
+extern "C" int p7_gpu_FilterScoreBatch(const P7_GPU_OPROFILE *om, const P7_GPU_SEQDB *db, const int64_t *idx, int64_t n, float *ret_sc)
+{ return run_list_stage(ST_BIAS, om, db, idx, n, ret_sc, nullptr); }
+
+/*****************************************************************
+ * Pipeline (p7_pipeline.c:666-735) on the device
+ *****************************************************************/
+struct p7g_pipeline_s {
+  double F1, F2, F3;
+  int    do_biasfilter, do_max;
+  P7_GPU_STATS st;
+};
+
+extern "C" P7_GPU_PIPELINE *
+p7_gpu_pipeline_Create(double F1, double F2, double F3, int do_biasfilter, int do_max)
+{
+  P7_GPU_PIPELINE *pli = (P7_GPU_PIPELINE *) calloc(1, sizeof(P7_GPU_PIPELINE));
+  if (! pli) return nullptr;
+  pli->F1 = F1; pli->F2 = F2; pli->F3 = F3;
+  pli->do_biasfilter = do_biasfilter; pli->do_max = do_max;
+  if (do_max) { pli->F1 = pli->F2 = pli->F3 = 1.0; pli->do_biasfilter = 0; }   // --max (p7_pipeline.c:214-219)
+  return pli;
+}
+
+extern "C" void p7_gpu_pipeline_Destroy(P7_GPU_PIPELINE *pli) { free(pli); }
+
+extern "C" int
+p7_gpu_pipeline_GetStats(const P7_GPU_PIPELINE *pli, P7_GPU_STATS *st)
+{
+  *st = pli->st;
+  return p7G_OK;
+}
+
+extern "C" int
+p7_gpu_Pipeline(P7_GPU_PIPELINE *pli, const P7_GPU_OPROFILE *om, const P7_GPU_SEQDB *db, P7_GPU_HITS *hits)
+{
+  std::lock_guard<std::mutex> lk(g_lock);
+  int status;
+  DevProfile *p;
+  if ((status = ensure_init()) != p7G_OK) return status;
+  if (db->maxcode >= om->abc->Kp) { p7g_set_error("database holds residue codes outside the model alphabet"); return p7G_EINCOMPAT; }
+  for (int z = 0; z < 6; z++)
+    if (om->evparam[z] == -99999.0f) { p7g_set_error("profile has no E-value parameters (STATS)"); return p7G_EINVAL; }
+  if ((status = dev_profile(om, db->Lmax, &p)) != p7G_OK) return status;
+  const int64_t n = db->nseq;
+  // pool for the Forward row scale factors of F3 survivors: bounded by the
+  // total residues of every target that may reach the Forward stage
+  if ((status = work_reserve(n, std::min<int64_t>(db->nres + n, (int64_t) 1 << 28))) != p7G_OK) return status;
+  DevSeqs s = seqs_of(db);
+  Work &w = g_work;
+  LenCfg lc = lencfg_per_target();
+  GateThresh g;
+  g.T1m = p7g_gumbel_threshold(om->evparam[p7G_MMU], om->evparam[p7G_MLAMBDA], pli->F1);
+  g.T2m = p7g_gumbel_threshold(om->evparam[p7G_MMU], om->evparam[p7G_MLAMBDA], pli->F2);
+  g.T2v = p7g_gumbel_threshold(om->evparam[p7G_VMU], om->evparam[p7G_VLAMBDA], pli->F2);
+  g.T3f = p7g_exp_threshold(om->evparam[p7G_FTAU], om->evparam[p7G_FLAMBDA], pli->F3);
+  g.do_bias = pli->do_biasfilter; g.do_max = pli->do_max;
+
+  if ((status = reset_counts()) != p7G_OK) return status;
+  HIPCHK(hipMemsetAsync(w.r.filtersc, 0xff, sizeof(float) * (size_t) n, g_stream));   // NaN where not computed
+  HIPCHK(hipMemsetAsync(w.r.vfsc, 0xff, sizeof(float) * (size_t) n, g_stream));
+  HIPCHK(hipMemsetAsync(w.r.fwdsc, 0xff, sizeof(float) * (size_t) n, g_stream));
+  HIPCHK(hipMemsetAsync(w.r.bcksc, 0xff, sizeof(float) * (size_t) n, g_stream));
+  HIPCHK(hipEventRecord(w.ev[0], g_stream));
+  HIPCHK(launch_ssv(*p, s, nullptr, n, w.r.zmax, g_stream));
+  HIPCHK(launch_msv_classify(*p, s, nullptr, n, w.r.zmax, w.r.xj, w.l.full, &w.l.counts[0], lc, g_stream));
+  HIPCHK(launch_msv_full(*p, s, w.l.full, &w.l.counts[0], n, w.r.xj, lc, g_stream));
+  HIPCHK(launch_msv_gate(*p, s, n, w.r.xj, w.r, w.l, g, g_stream));
+  HIPCHK(hipEventRecord(w.ev[1], g_stream));
+  if (g.do_bias) HIPCHK(launch_bias(*p, s, w.l.bias, &w.l.counts[1], n, w.r, w.l, g, 1, lc, g_stream));
+  HIPCHK(hipEventRecord(w.ev[2], g_stream));
+  HIPCHK(launch_vit(*p, s, w.l.vit, &w.l.counts[2], n, w.r, w.l, g, 1, lc, g_stream));
+  HIPCHK(hipEventRecord(w.ev[3], g_stream));
+  HIPCHK(launch_fwd(*p, s, w.l.fwd, &w.l.counts[3], n, w.r, w.l, g, 1, lc, w.pool, w.pool_size, w.scale_off, nullptr, g_stream));
+  HIPCHK(hipEventRecord(w.ev[4], g_stream));
+  HIPCHK(launch_bck(*p, s, w.l.bck, &w.l.counts[4], n, w.r, w.pool, w.scale_off, nullptr, nullptr, lc, g_stream));
+  HIPCHK(hipEventRecord(w.ev[5], g_stream));
+
+  // results back to the host
+  std::vector<uint8_t> stage((size_t) n);
+  HIPCHK(hipMemcpyAsync(stage.data(), w.r.stage, (size_t) n, hipMemcpyDeviceToHost, g_stream));
+  int32_t counts[16];
+  HIPCHK(hipMemcpyAsync(counts, w.l.counts, sizeof(counts), hipMemcpyDeviceToHost, g_stream));
+  struct { float *dst; const float *src; } fl[] = {
+    { hits ? hits->nullsc : nullptr, w.r.nullsc }, { hits ? hits->usc : nullptr, w.r.usc },
+    { hits ? hits->filtersc : nullptr, w.r.filtersc }, { hits ? hits->vfsc : nullptr, w.r.vfsc },
+    { hits ? hits->fwdsc : nullptr, w.r.fwdsc }, { hits ? hits->bcksc : nullptr, w.r.bcksc } };
+  std::vector<float> tmp[6];
+  for (int a = 0; a < 6; a++) if (fl[a].dst) {
+    tmp[a].resize((size_t) n);
+    HIPCHK(hipMemcpyAsync(tmp[a].data(), fl[a].src, sizeof(float) * (size_t) n, hipMemcpyDeviceToHost, g_stream));
+  }
+  HIPCHK(hipStreamSynchronize(g_stream));
+  if (counts[6]) { p7g_set_error("Forward scale-factor pool exhausted"); return p7G_EMEM; }
+
+  P7_GPU_STATS &st = pli->st;
+  float ms[5], mtot;
+  for (int a = 0; a < 5; a++) HIPCHK(hipEventElapsedTime(&ms[a], w.ev[a], w.ev[a+1]));
+  HIPCHK(hipEventElapsedTime(&mtot, w.ev[0], w.ev[5]));
+  st.ms_total = mtot;
+  st.ms_msv = ms[0]; st.ms_bias = ms[1]; st.ms_vit = ms[2]; st.ms_fwd = ms[3]; st.ms_bck = ms[4];
+  int64_t pm = 0, pb = 0, pv = 0, pf = 0;
+  double cb = 0, cv = 0, cf = 0, ck = 0;
+  for (int64_t slot = 0; slot < n; slot++) {
+    const int sg = stage[slot];
+    const double cells = (double) db->h_len[slot] * om->M;
+    if (sg >= 1) { pm++; cb += db->h_len[slot]; }
+    if (sg >= 2) pb++;
+    if (sg >= 3) pv++;
+    if (sg >= 4) { pf++; ck += cells; }
+    (void) cells;
+  }
+  // cells of the list stages: Viterbi ran on targets that reached it without the F2 shortcut
+  st.cells_msv  = (double) db->nres * om->M;
+  st.cells_bias = cb;
+  st.nmodels += 1; st.nseqs += n; st.nres += db->nres; st.nnodes += om->M;
+  st.n_past_msv += pm; st.n_past_bias += pb; st.n_past_vit += pv; st.n_past_fwd += pf;
+  st.n_msv_full = counts[0];
+  {
+    // Viterbi and Forward cells need the lists' residue sums
+    std::vector<int32_t> lv((size_t) counts[2]), lf((size_t) counts[3]);
+    if (counts[2]) HIPCHK(hipMemcpy(lv.data(), w.l.vit, sizeof(int32_t) * (size_t) counts[2], hipMemcpyDeviceToHost));
+    if (counts[3]) HIPCHK(hipMemcpy(lf.data(), w.l.fwd, sizeof(int32_t) * (size_t) counts[3], hipMemcpyDeviceToHost));
+    for (int32_t sl : lv) cv += db->h_len[sl];
+    for (int32_t sl : lf) cf += db->h_len[sl];
+  }
+  st.cells_vit = cv * om->M; st.cells_fwd = cf * om->M; st.cells_bck = ck;
+
+  if (hits && hits->stage) for (int64_t slot = 0; slot < n; slot++) hits->stage[db->h_perm[slot]] = stage[slot];
+  for (int a = 0; a < 6; a++) if (fl[a].dst) {
+    float *dst = fl[a].dst;
+    const float *srcv = tmp[a].data();
+    for (int64_t slot = 0; slot < n; slot++) dst[db->h_perm[slot]] = srcv[slot];
+  }
+  return p7G_OK;
+}
+
+/*****************************************************************
+ * Single-target drop-ins: one target staged in a scratch database
+ *****************************************************************/
+struct Single {
+  uint8_t *res = nullptr; int64_t cap = 0;
+  int64_t *off = nullptr; int32_t *len = nullptr;
+  float   *xmx = nullptr; int64_t xcap = 0;     // two (L+1)*6 buffers: forward, backward
+};
+static Single g_one;
+
+static int
+single_stage(const uint8_t *dsq, int L, DevSeqs *s)
+{
+  Single &o = g_one;
+  const int64_t need = (((int64_t) L + 15) & ~(int64_t) 15) + 16;
+  if (need > o.cap) {
+    if (o.res) (void) hipFree(o.res);
+    o.res = nullptr;
+    HIPCHK(hipMalloc((void **) &o.res, (size_t) need));
+    o.cap = need;
+  }
+  if (! o.off) { HIPCHK(hipMalloc((void **) &o.off, 64)); HIPCHK(hipMalloc((void **) &o.len, 64)); }
+  std::vector<uint8_t> h((size_t) need, P7G_PADCODE);
+  for (int i = 0; i < L; i++) h[i] = dsq[i + 1];
+  int64_t zero = 0;
+  HIPCHK(hipMemcpyAsync(o.res, h.data(), (size_t) need, hipMemcpyHostToDevice, g_stream));
+  HIPCHK(hipMemcpyAsync(o.off, &zero, sizeof(zero), hipMemcpyHostToDevice, g_stream));
+  HIPCHK(hipMemcpyAsync(o.len, &L, sizeof(int32_t), hipMemcpyHostToDevice, g_stream));
+  // the work buffers' stream-ordered use keeps `h` alive until the copies complete
+  HIPCHK(hipStreamSynchronize(g_stream));
+  s->res = o.res; s->off = o.off; s->len = o.len; s->nslot = 1;
+  return p7G_OK;
+}
+
+static int
+check_dsq(const uint8_t *dsq, int L, const P7_GPU_OPROFILE *om)
+{
+  if (L < 0) { p7g_set_error("negative sequence length"); return p7G_EINVAL; }
+  for (int i = 1; i <= L; i++) if (dsq[i] >= om->abc->Kp) { p7g_set_error("residue %d at position %d is not a digital residue", dsq[i], i); return p7G_EINVAL; }
+  return p7G_OK;
+}
+
+static LenCfg
+lencfg_of(const P7_GPU_OPROFILE *om)
+{
+  LenCfg lc;
+  lc.on = 1;
+  lc.tjb = om->tjb_b;
+  lc.xmove = om->xw[p7G_XC][p7G_MOVE];
+  lc.floop = om->xf[p7G_XC][p7G_LOOP];
+  lc.fmove = om->xf[p7G_XC][p7G_MOVE];
+  lc.p1 = 0.0f; lc.q1 = 0.0f;
+  return lc;
+}
+
+static int
+single_msv(const uint8_t *dsq, int L, const P7_GPU_OPROFILE *om, int *ret_xj)
+{
+  int status;
+  DevProfile *p;
+  if ((status = dev_profile(om, L, &p)) != p7G_OK) return status;
+  if ((status = work_reserve(1, 0)) != p7G_OK) return status;
+  DevSeqs s;
+  if ((status = single_stage(dsq, L, &s)) != p7G_OK) return status;
+  Work &w = g_work;
+  LenCfg lc = lencfg_of(om);
+  if ((status = reset_counts()) != p7G_OK) return status;
+  HIPCHK(launch_ssv(*p, s, nullptr, 1, w.r.zmax, g_stream));
+  HIPCHK(launch_msv_classify(*p, s, nullptr, 1, w.r.zmax, w.r.xj, w.l.full, &w.l.counts[0], lc, g_stream));
+  HIPCHK(launch_msv_full(*p, s, w.l.full, &w.l.counts[0], 1, w.r.xj, lc, g_stream));
+  int16_t xj; int32_t nfull;
+  HIPCHK(hipMemcpyAsync(&xj, w.r.xj, sizeof(xj), hipMemcpyDeviceToHost, g_stream));
+  HIPCHK(hipMemcpyAsync(&nfull, &w.l.counts[0], sizeof(nfull), hipMemcpyDeviceToHost, g_stream));
+  HIPCHK(hipStreamSynchronize(g_stream));
+  *ret_xj = (xj < 0) ? -1 : (nfull ? 1000 + xj : xj);
+  return p7G_OK;
+}
+
+extern "C" int
+p7_gpu_MSVFilter(const uint8_t *dsq, int L, const P7_GPU_OPROFILE *om, P7_GPU_OMX *ox, float *ret_sc)
+{
+  std::lock_guard<std::mutex> lk(g_lock);
+  int status, v;
+  if ((status = ensure_init()) != p7G_OK) return status;
+  if ((status = check_dsq(dsq, L, om)) != p7G_OK) return status;
+  if (ox) ox->M = om->M;
+  if (L == 0) { *ret_sc = msv_score_from_xj(om, 0, om->tjb_b); return p7G_OK; }
+  if ((status = single_msv(dsq, L, om, &v)) != p7G_OK) return status;
+  if (v < 0) { *ret_sc = INFINITY; return p7G_ERANGE; }
+  *ret_sc = msv_score_from_xj(om, v >= 1000 ? v - 1000 : v, om->tjb_b);
+  return p7G_OK;
+}
+
+extern "C" int
+p7_gpu_SSVFilter(const uint8_t *dsq, int L, const P7_GPU_OPROFILE *om, float *ret_sc)
+{
+  std::lock_guard<std::mutex> lk(g_lock);
+  int status, v;
+  if ((status = ensure_init()) != p7G_OK) return status;
+  if ((status = check_dsq(dsq, L, om)) != p7G_OK) return status;
+  if (L == 0) { *ret_sc = msv_score_from_xj(om, 0, om->tjb_b); return p7G_OK; }
+  if ((status = single_msv(dsq, L, om, &v)) != p7G_OK) return status;
+  if (v < 0) { *ret_sc = INFINITY; return p7G_ERANGE; }
+  if (v >= 1000) return p7G_ENORESULT;      // the J state is needed: caller runs the full MSV filter
+  *ret_sc = msv_score_from_xj(om, v, om->tjb_b);
+  return p7G_OK;
+}
+
+extern "C" int
+p7_gpu_ViterbiFilter(const uint8_t *dsq, int L, const P7_GPU_OPROFILE *om, P7_GPU_OMX *ox, float *ret_sc)
+{
+  std::lock_guard<std::mutex> lk(g_lock);
+  int status;
+  DevProfile *p;
+  if ((status = ensure_init()) != p7G_OK) return status;
+  if (om->mode != p7G_LOCAL && om->mode != p7G_UNILOCAL) { p7g_set_error("Fast filter only works for local alignment"); return p7G_EINVAL; }
+  if ((status = check_dsq(dsq, L, om)) != p7G_OK) return status;
+  if (ox) ox->M = om->M;
+  if (L == 0) {                                // no rows: xC stays -infinity
+    *ret_sc = -INFINITY;
+    return p7G_OK;
+  }
+  if ((status = dev_profile(om, L, &p)) != p7G_OK) return status;
+  if ((status = work_reserve(1, 0)) != p7G_OK) return status;
+  DevSeqs s;
+  if ((status = single_stage(dsq, L, &s)) != p7G_OK) return status;
+  Work &w = g_work;
+  GateThresh g; memset(&g, 0, sizeof(g));
+  if ((status = reset_counts()) != p7G_OK) return status;
+  int32_t zero = 0, one = 1;
+  HIPCHK(hipMemcpyAsync(w.l.vit, &zero, sizeof(zero), hipMemcpyHostToDevice, g_stream));
+  HIPCHK(hipMemcpyAsync(&w.l.counts[2], &one, sizeof(one), hipMemcpyHostToDevice, g_stream));
+  HIPCHK(launch_vit(*p, s, w.l.vit, &w.l.counts[2], 1, w.r, w.l, g, 0, lencfg_of(om), g_stream));
+  float sc; int vst;
+  HIPCHK(hipMemcpyAsync(&sc, w.r.vfsc, sizeof(sc), hipMemcpyDeviceToHost, g_stream));
+  HIPCHK(hipMemcpyAsync(&vst, w.r.vstatus, sizeof(vst), hipMemcpyDeviceToHost, g_stream));
+  HIPCHK(hipStreamSynchronize(g_stream));
+  *ret_sc = sc;
+  return vst == 16 ? p7G_ERANGE : p7G_OK;
+}
+
+static int
+single_xmx(int L)
+{
+  Single &o = g_one;
+  const int64_t need = (int64_t) (L + 1) * 6 * 2;
+  if (need > o.xcap) {
+    if (o.xmx) (void) hipFree(o.xmx);
+    o.xmx = nullptr;
+    HIPCHK(hipMalloc((void **) &o.xmx, sizeof(float) * (size_t) need));
+    o.xcap = need;
+  }
+  return p7G_OK;
+}
+
+extern "C" int
+p7_gpu_ForwardParser(const uint8_t *dsq, int L, const P7_GPU_OPROFILE *om, P7_GPU_OMX *fwd, float *opt_sc)
+{
+  std::lock_guard<std::mutex> lk(g_lock);
+  int status;
+  DevProfile *p;
+  if ((status = ensure_init()) != p7G_OK) return status;
+  if ((status = check_dsq(dsq, L, om)) != p7G_OK) return status;
+  if (! fwd) { p7g_set_error("ForwardParser needs a matrix"); return p7G_EINVAL; }
+  if ((status = p7_gpu_omx_GrowTo(fwd, om->M, 0, L)) != p7G_OK) return status;
+  fwd->M = om->M; fwd->L = L; fwd->has_own_scales = 1; fwd->totscale = 0.0f;
+  if (L == 0) {
+    float *x = fwd->xmx;
+    x[0] = 0.0f; x[1] = 1.0f; x[2] = 0.0f; x[3] = om->xf[p7G_XN][p7G_MOVE]; x[4] = 0.0f; x[5] = 1.0f;
+    if (opt_sc) *opt_sc = (float) ((double) 0.0f + log((double) (0.0f * om->xf[p7G_XC][p7G_MOVE])));
+    return p7G_OK;
+  }
+  if ((status = dev_profile(om, L, &p)) != p7G_OK) return status;
+  if ((status = work_reserve(1, 0)) != p7G_OK) return status;
+  if ((status = single_xmx(L)) != p7G_OK) return status;
+  DevSeqs s;
+  if ((status = single_stage(dsq, L, &s)) != p7G_OK) return status;
+  Work &w = g_work;
+  GateThresh g; memset(&g, 0, sizeof(g));
+  if ((status = reset_counts()) != p7G_OK) return status;
+  int32_t zero = 0, one = 1;
+  HIPCHK(hipMemcpyAsync(w.l.fwd, &zero, sizeof(zero), hipMemcpyHostToDevice, g_stream));
+  HIPCHK(hipMemcpyAsync(&w.l.counts[3], &one, sizeof(one), hipMemcpyHostToDevice, g_stream));
+  HIPCHK(launch_fwd(*p, s, w.l.fwd, &w.l.counts[3], 1, w.r, w.l, g, 0, lencfg_of(om), nullptr, 0, nullptr, g_one.xmx, g_stream));
+  float sc; int fst;
+  HIPCHK(hipMemcpyAsync(&sc, w.r.fwdsc, sizeof(sc), hipMemcpyDeviceToHost, g_stream));
+  HIPCHK(hipMemcpyAsync(&fst, w.r.fstatus, sizeof(fst), hipMemcpyDeviceToHost, g_stream));
+  HIPCHK(hipMemcpyAsync(fwd->xmx, g_one.xmx, sizeof(float) * (size_t) (L + 1) * 6, hipMemcpyDeviceToHost, g_stream));
+  HIPCHK(hipStreamSynchronize(g_stream));
+  double ts = 0.0;
+  for (int i = 1; i <= L; i++) { float sf = fwd->xmx[i*6 + 5]; if (sf != 1.0f) ts = (float) (ts + log((double) sf)); }
+  fwd->totscale = (float) ts;
+  if (fst == 16) { p7g_set_error("forward score is NaN, zero or infinite"); return p7G_ERANGE; }
+  if (opt_sc) *opt_sc = sc;
+  return p7G_OK;
+}
+
+extern "C" int
+p7_gpu_BackwardParser(const uint8_t *dsq, int L, const P7_GPU_OPROFILE *om, const P7_GPU_OMX *fwd, P7_GPU_OMX *bck, float *opt_sc)
+{
+  std::lock_guard<std::mutex> lk(g_lock);
+  int status;
+  DevProfile *p;
+  if ((status = ensure_init()) != p7G_OK) return status;
+  if ((status = check_dsq(dsq, L, om)) != p7G_OK) return status;
+  if (! fwd || ! bck) { p7g_set_error("BackwardParser needs two matrices"); return p7G_EINVAL; }
+  if (fwd->L != L) { p7g_set_error("fwd matrix size doesn't agree with length L"); return p7G_EINVAL; }
+  if ((status = p7_gpu_omx_GrowTo(bck, om->M, 0, L)) != p7G_OK) return status;
+  bck->M = om->M; bck->L = L; bck->has_own_scales = 0;
+  if (L == 0) { p7g_set_error("backward score underflow on an empty sequence"); return p7G_ERANGE; }
+  if ((status = dev_profile(om, L, &p)) != p7G_OK) return status;
+  if ((status = work_reserve(1, 0)) != p7G_OK) return status;
+  if ((status = single_xmx(L)) != p7G_OK) return status;
+  DevSeqs s;
+  if ((status = single_stage(dsq, L, &s)) != p7G_OK) return status;
+  Work &w = g_work;
+  if ((status = reset_counts()) != p7G_OK) return status;
+  int32_t zero = 0, one = 1;
+  float *d_fx = g_one.xmx, *d_bx = g_one.xmx + (size_t) (L + 1) * 6;
+  HIPCHK(hipMemcpyAsync(d_fx, fwd->xmx, sizeof(float) * (size_t) (L + 1) * 6, hipMemcpyHostToDevice, g_stream));
+  HIPCHK(hipMemcpyAsync(w.l.bck, &zero, sizeof(zero), hipMemcpyHostToDevice, g_stream));
+  HIPCHK(hipMemcpyAsync(&w.l.counts[4], &one, sizeof(one), hipMemcpyHostToDevice, g_stream));
+  HIPCHK(launch_bck(*p, s, w.l.bck, &w.l.counts[4], 1, w.r, nullptr, nullptr, d_fx, d_bx, lencfg_of(om), g_stream));
+  float sc; int fst;
+  HIPCHK(hipMemcpyAsync(&sc, w.r.bcksc, sizeof(sc), hipMemcpyDeviceToHost, g_stream));
+  HIPCHK(hipMemcpyAsync(&fst, w.r.fstatus, sizeof(fst), hipMemcpyDeviceToHost, g_stream));
+  HIPCHK(hipMemcpyAsync(bck->xmx, d_bx, sizeof(float) * (size_t) (L + 1) * 6, hipMemcpyDeviceToHost, g_stream));
+  HIPCHK(hipStreamSynchronize(g_stream));
+  bck->has_own_scales = (fst & 0x100) ? 1 : 0;
+  {
+    double ts = log((double) bck->xmx[(size_t) L * 6 + 5]);
+    float tsf = (float) ts;
+    for (int i = L - 1; i >= 1; i--) { float sf = bck->xmx[(size_t) i * 6 + 5]; if (sf > 1.0f) tsf = (float) ((double) tsf + log((double) sf)); }
+    bck->totscale = tsf;
+  }
+  if ((fst & 0xff) == 16) { p7g_set_error("backward score is NaN, zero or infinite"); return p7G_ERANGE; }
+  if (opt_sc) *opt_sc = sc;
+  return p7G_OK;
+}
+
+/*****************************************************************
+ * Bias filter drop-in (p7_bg_FilterScore): the 2-state Forward on the GPU,
+ * length terms on the host with the bg's own p1.
+ *****************************************************************/
+extern "C" int
+p7_gpu_bg_FilterScore(P7_GPU_BG *bg, const uint8_t *dsq, int L, float *ret_sc)
+{
+  std::lock_guard<std::mutex> lk(g_lock);
+  int status;
+  if ((status = ensure_init()) != p7G_OK) return status;
+  if (! bg->filter_set) { p7g_set_error("p7_gpu_bg_SetFilter() must be called first"); return p7G_EINVAL; }
+  for (int i = 1; i <= L; i++) if (dsq[i] >= bg->abc->Kp) { p7g_set_error("bad residue code"); return p7G_EINVAL; }
+  float logsc;
+  if (L == 0) logsc = (float) log((double) bg->pi[2]);
+  else {
+    // a throwaway device profile carries only the filter HMM
+    DevProfile p; memset(&p, 0, sizeof(p));
+    std::vector<float> eo(64, 1.0f);
+    for (int x = 0; x < bg->abc->Kp; x++) { eo[x*2] = bg->eo[x][0]; eo[x*2+1] = bg->eo[x][1]; }
+    float *d_eo = nullptr;
+    HIPCHK(hipMalloc((void **) &d_eo, sizeof(float) * 64));
+    HIPCHK(hipMemcpyAsync(d_eo, eo.data(), sizeof(float) * 64, hipMemcpyHostToDevice, g_stream));
+    p.eo = d_eo; p.ft1_0 = bg->ft[1][0]; p.ft1_1 = bg->ft[1][1]; p.pi0 = bg->pi[0]; p.pi1 = bg->pi[1];
+    if ((status = work_reserve(1, 0)) != p7G_OK) { (void) hipFree(d_eo); return status; }
+    DevSeqs s;
+    if ((status = single_stage(dsq, L, &s)) != p7G_OK) { (void) hipFree(d_eo); return status; }
+    Work &w = g_work;
+    GateThresh g; memset(&g, 0, sizeof(g));
+    LenCfg lc; memset(&lc, 0, sizeof(lc)); lc.on = 1; lc.p1 = bg->ft[0][0]; lc.q1 = bg->ft[0][1];
+    int32_t zero = 0, one = 1;
+    HIPCHK(hipMemcpyAsync(w.l.bias, &zero, sizeof(zero), hipMemcpyHostToDevice, g_stream));
+    HIPCHK(hipMemcpyAsync(&w.l.counts[1], &one, sizeof(one), hipMemcpyHostToDevice, g_stream));
+    HIPCHK(launch_bias(p, s, w.l.bias, &w.l.counts[1], 1, w.r, w.l, g, 0, lc, g_stream));
+    HIPCHK(hipMemcpyAsync(&logsc, w.r.filtersc, sizeof(float), hipMemcpyDeviceToHost, g_stream));
+    HIPCHK(hipStreamSynchronize(g_stream));
+    (void) hipFree(d_eo);
+  }
+  *ret_sc = logsc + (float) L * logf(bg->p1) + logf(1.-bg->p1);
+  return p7G_OK;
+}

@@ -1,0 +1,437 @@
+"""ctypes binding of libp7gpu.so, the C ABI declared in include/p7_gpu.h.
+
+The classes mirror the HMMER objects the C entry points stand in for
+(P7_HMM, P7_BG, P7_PROFILE, P7_OPROFILE, P7_OMX, P7_PIPELINE); every call goes
+through the HIP library. There is no CPU fallback: if the shared library is
+missing, importing this module raises.
+"""
+import ctypes
+import os
+from ctypes import POINTER, c_double, c_float, c_int, c_int16, c_int32, c_int64, c_uint8, c_void_p, c_char_p
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(_HERE, "libp7gpu.so")
+
+OK, FAIL, EOF, EMEM, ENOTFOUND, EFORMAT, EINCOMPAT, EINVAL, ESYS, ERANGE, ENORESULT, EDEVICE = \
+    0, 1, 3, 5, 6, 7, 10, 11, 12, 16, 19, 64
+RNA, DNA, AMINO = 1, 2, 3
+LOCAL, GLOCAL, UNILOCAL = 1, 2, 3
+XE, XN, XJ, XC = 0, 1, 2, 3
+MOVE, LOOP = 0, 1
+BM, MM, IM, DM, MD, MI, II, DD = range(8)
+KP = {AMINO: 29, DNA: 18, RNA: 18}
+K = {AMINO: 20, DNA: 4, RNA: 4}
+
+
+class P7GpuError(RuntimeError):
+    def __init__(self, status, where):
+        self.status = status
+        super().__init__(f"{where}: status {status}: {last_error()}")
+
+
+class Tables(ctypes.Structure):
+    """P7_GPU_TABLES (include/p7_gpu.h)."""
+    _fields_ = [
+        ("M", c_int), ("Kp", c_int), ("abctype", c_int),
+        ("rbv", POINTER(c_uint8)),
+        ("tbm_b", c_uint8), ("tec_b", c_uint8), ("tjb_b", c_uint8), ("base_b", c_uint8), ("bias_b", c_uint8),
+        ("scale_b", c_float),
+        ("rwv", POINTER(c_int16)), ("twv", POINTER(c_int16)),
+        ("xw", (c_int16 * 2) * 4), ("scale_w", c_float), ("base_w", c_int16), ("ddbound_w", c_int16),
+        ("rfv", POINTER(c_float)), ("tfv", POINTER(c_float)), ("xf", (c_float * 2) * 4),
+        ("L", c_int), ("mode", c_int), ("nj", c_float), ("evparam", c_float * 6), ("compo", c_float * 20),
+    ]
+
+
+class Hits(ctypes.Structure):
+    _fields_ = [(n, POINTER(c_float)) for n in ("nullsc", "usc", "filtersc", "vfsc", "fwdsc", "bcksc")] + \
+               [("stage", POINTER(c_uint8))]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [(n, c_int64) for n in ("nmodels", "nseqs", "nres", "nnodes", "n_past_msv", "n_past_bias",
+                                       "n_past_vit", "n_past_fwd", "n_msv_full")] + \
+               [(n, c_double) for n in ("cells_msv", "cells_bias", "cells_vit", "cells_fwd", "cells_bck",
+                                        "ms_msv", "ms_bias", "ms_vit", "ms_fwd", "ms_bck", "ms_total")]
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+
+def _load():
+    if not os.path.exists(LIBPATH):
+        raise ImportError(f"{LIBPATH} is missing: build it with __graft_entry__.build() "
+                          "(make -C biobench2_amd/csrc); there is no CPU fallback")
+    lib = ctypes.CDLL(LIBPATH)
+    vp, i, f, d, i64 = c_void_p, c_int, c_float, c_double, c_int64
+    pu8, pf, pi, pi64, pi32 = POINTER(c_uint8), POINTER(c_float), POINTER(c_int), POINTER(c_int64), POINTER(c_int32)
+    sig = {
+        "p7_gpu_Init": (i, [i]), "p7_gpu_Synchronize": (i, []), "p7_gpu_LastError": (c_char_p, []),
+        "p7_gpu_hmmfile_Read": (i, [c_char_p, i, POINTER(vp)]),
+        "p7_gpu_hmm_Info": (i, [vp, pi, pi, c_char_p, i]), "p7_gpu_hmm_Destroy": (None, [vp]),
+        "p7_gpu_bg_Create": (vp, [i]), "p7_gpu_bg_Destroy": (None, [vp]),
+        "p7_gpu_bg_SetLength": (i, [vp, i]), "p7_gpu_bg_NullOne": (i, [vp, pu8, i, pf]),
+        "p7_gpu_bg_SetFilter": (i, [vp, i, pf]), "p7_gpu_bg_FilterScore": (i, [vp, pu8, i, pf]),
+        "p7_gpu_ProfileConfig": (i, [vp, vp, i, i, POINTER(vp)]),
+        "p7_gpu_profile_ReconfigLength": (i, [vp, i]),
+        "p7_gpu_profile_GetScores": (i, [vp, pf, pf, pf]), "p7_gpu_profile_Destroy": (None, [vp]),
+        "p7_gpu_oprofile_Create": (vp, [i, i]), "p7_gpu_oprofile_Convert": (i, [vp, vp]),
+        "p7_gpu_oprofile_Load": (i, [vp, POINTER(Tables)]), "p7_gpu_oprofile_GetTables": (i, [vp, POINTER(Tables)]),
+        "p7_gpu_oprofile_ReconfigLength": (i, [vp, i]), "p7_gpu_oprofile_ReconfigMSVLength": (i, [vp, i]),
+        "p7_gpu_oprofile_ReconfigRestLength": (i, [vp, i]), "p7_gpu_oprofile_Destroy": (None, [vp]),
+        "p7_gpu_omx_Create": (vp, [i, i, i]), "p7_gpu_omx_GrowTo": (i, [vp, i, i, i]),
+        "p7_gpu_omx_Xmx": (pf, [vp, pi, pf, pi]), "p7_gpu_omx_Destroy": (None, [vp]),
+        "p7_gpu_MSVFilter": (i, [pu8, i, vp, vp, pf]), "p7_gpu_SSVFilter": (i, [pu8, i, vp, pf]),
+        "p7_gpu_ViterbiFilter": (i, [pu8, i, vp, vp, pf]), "p7_gpu_ForwardParser": (i, [pu8, i, vp, vp, pf]),
+        "p7_gpu_BackwardParser": (i, [pu8, i, vp, vp, vp, pf]),
+        "p7_gpu_seqdb_Create": (i, [pu8, pi64, pi32, i64, POINTER(vp)]),
+        "p7_gpu_seqdb_NSeq": (i64, [vp]), "p7_gpu_seqdb_NRes": (i64, [vp]), "p7_gpu_seqdb_Destroy": (None, [vp]),
+        "p7_gpu_MSVFilterBatch": (i, [vp, vp, pi64, i64, pf, pi]),
+        "p7_gpu_ViterbiFilterBatch": (i, [vp, vp, pi64, i64, pf, pi]),
+        "p7_gpu_ForwardParserBatch": (i, [vp, vp, pi64, i64, pf, pi]),
+        "p7_gpu_BackwardParserBatch": (i, [vp, vp, pi64, i64, pf, pi]),
+        "p7_gpu_FilterScoreBatch": (i, [vp, vp, pi64, i64, pf]),
+        "p7_gpu_pipeline_Create": (vp, [d, d, d, i, i]), "p7_gpu_pipeline_Destroy": (None, [vp]),
+        "p7_gpu_Pipeline": (i, [vp, vp, vp, POINTER(Hits)]),
+        "p7_gpu_pipeline_GetStats": (i, [vp, POINTER(Stats)]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    return lib
+
+
+lib = _load()
+
+
+def last_error():
+    s = lib.p7_gpu_LastError()
+    return s.decode() if s else ""
+
+
+def _check(status, where, ok=(OK,)):
+    if status not in ok:
+        raise P7GpuError(status, where)
+    return status
+
+
+def _ptr(a, ctype):
+    return a.ctypes.data_as(POINTER(ctype))
+
+
+def init(device=0):
+    _check(lib.p7_gpu_Init(device), "p7_gpu_Init")
+
+
+def dsq_of(residues):
+    """Digital sequence with sentinels: dsq[0] = dsq[L+1] = 255, residues at 1..L."""
+    r = np.asarray(residues, dtype=np.uint8)
+    out = np.empty(r.size + 2, dtype=np.uint8)
+    out[0] = out[-1] = 255
+    out[1:-1] = r
+    return out
+
+
+class HMM:
+    def __init__(self, path, which=0):
+        h = c_void_p()
+        _check(lib.p7_gpu_hmmfile_Read(str(path).encode(), which, ctypes.byref(h)), f"read {path}")
+        self.h = h
+        M, t = c_int(), c_int()
+        name = ctypes.create_string_buffer(256)
+        lib.p7_gpu_hmm_Info(h, ctypes.byref(M), ctypes.byref(t), name, 256)
+        self.M, self.abctype, self.name = M.value, t.value, name.value.decode()
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib.p7_gpu_hmm_Destroy(self.h)
+            self.h = None
+
+
+class Background:
+    def __init__(self, abctype):
+        self.h = lib.p7_gpu_bg_Create(abctype)
+        if not self.h:
+            raise P7GpuError(EINVAL, "p7_gpu_bg_Create")
+        self.abctype = abctype
+
+    def set_length(self, L):
+        _check(lib.p7_gpu_bg_SetLength(self.h, L), "bg_SetLength")
+
+    def null_one(self, L):
+        sc = c_float()
+        _check(lib.p7_gpu_bg_NullOne(self.h, None, L, ctypes.byref(sc)), "bg_NullOne")
+        return sc.value
+
+    def set_filter(self, M, compo):
+        c = np.ascontiguousarray(compo, dtype=np.float32)
+        _check(lib.p7_gpu_bg_SetFilter(self.h, M, _ptr(c, c_float)), "bg_SetFilter")
+
+    def filter_score(self, residues):
+        dsq = dsq_of(residues)
+        sc = c_float()
+        _check(lib.p7_gpu_bg_FilterScore(self.h, _ptr(dsq, c_uint8), dsq.size - 2, ctypes.byref(sc)), "bg_FilterScore")
+        return sc.value
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib.p7_gpu_bg_Destroy(self.h)
+            self.h = None
+
+
+class Profile:
+    """P7_PROFILE: float log-odds scores from p7_gpu_ProfileConfig."""
+
+    def __init__(self, hmm, bg, L=100, mode=LOCAL):
+        h = c_void_p()
+        _check(lib.p7_gpu_ProfileConfig(hmm.h, bg.h, L, mode, ctypes.byref(h)), "ProfileConfig")
+        self.h, self.M, self.abctype, self.mode = h, hmm.M, hmm.abctype, mode
+
+    def reconfig_length(self, L):
+        _check(lib.p7_gpu_profile_ReconfigLength(self.h, L), "profile_ReconfigLength")
+
+    def scores(self):
+        """tsc[M,8] (MM,IM,DM,BM,MD,DD,MI,II), rsc[Kp,M+1,2] (MSC,ISC), xsc[4,2] ([E,N,J,C][LOOP,MOVE])."""
+        Kp = KP[self.abctype]
+        tsc = np.zeros((self.M, 8), np.float32)
+        rsc = np.zeros((Kp, self.M + 1, 2), np.float32)
+        xsc = np.zeros((4, 2), np.float32)
+        _check(lib.p7_gpu_profile_GetScores(self.h, _ptr(tsc, c_float), _ptr(rsc, c_float), _ptr(xsc, c_float)), "GetScores")
+        return tsc, rsc, xsc
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib.p7_gpu_profile_Destroy(self.h)
+            self.h = None
+
+
+class OProfile:
+    """P7_OPROFILE: quantised tables; the device copy is built on first use."""
+
+    def __init__(self, allocM, abctype):
+        self.h = lib.p7_gpu_oprofile_Create(allocM, abctype)
+        if not self.h:
+            raise P7GpuError(EINVAL, "oprofile_Create")
+        self.abctype = abctype
+
+    @classmethod
+    def from_profile(cls, gm):
+        om = cls(gm.M, gm.abctype)
+        _check(lib.p7_gpu_oprofile_Convert(gm.h, om.h), "oprofile_Convert")
+        return om
+
+    @classmethod
+    def from_file(cls, path, which=0, L=100):
+        """hmmsearch's per-query preparation: read, configure local multihit with L, convert."""
+        hmm = HMM(path, which)
+        bg = Background(hmm.abctype)
+        gm = Profile(hmm, bg, L, LOCAL)
+        om = cls.from_profile(gm)
+        om.name, om.hmm, om.gm, om.bg = hmm.name, hmm, gm, bg
+        return om
+
+    def tables(self):
+        t = Tables()
+        _check(lib.p7_gpu_oprofile_GetTables(self.h, ctypes.byref(t)), "GetTables")
+        M, Kp = t.M, t.Kp
+        d = {k: getattr(t, k) for k in ("M", "Kp", "abctype", "tbm_b", "tec_b", "tjb_b", "base_b", "bias_b", "scale_b",
+                                         "scale_w", "base_w", "ddbound_w", "L", "mode", "nj")}
+        d["rbv"] = np.ctypeslib.as_array(t.rbv, (Kp, M)).copy()
+        d["rwv"] = np.ctypeslib.as_array(t.rwv, (Kp, M)).copy()
+        d["twv"] = np.ctypeslib.as_array(t.twv, (8, M)).copy()
+        d["rfv"] = np.ctypeslib.as_array(t.rfv, (Kp, M)).copy()
+        d["tfv"] = np.ctypeslib.as_array(t.tfv, (8, M)).copy()
+        d["xw"] = np.array([[t.xw[s][m] for m in range(2)] for s in range(4)], np.int16)
+        d["xf"] = np.array([[t.xf[s][m] for m in range(2)] for s in range(4)], np.float32)
+        d["evparam"] = np.array(list(t.evparam), np.float32)
+        d["compo"] = np.array(list(t.compo), np.float32)
+        return d
+
+    def load(self, d):
+        """Import tables (same keys as tables())."""
+        keep = {}
+        t = Tables()
+        for k in ("M", "Kp", "abctype", "tbm_b", "tec_b", "tjb_b", "base_b", "bias_b", "scale_b",
+                  "scale_w", "base_w", "ddbound_w", "L", "mode", "nj"):
+            setattr(t, k, d[k])
+        for k, ct, dt in (("rbv", c_uint8, np.uint8), ("rwv", c_int16, np.int16), ("twv", c_int16, np.int16),
+                          ("rfv", c_float, np.float32), ("tfv", c_float, np.float32)):
+            keep[k] = np.ascontiguousarray(d[k], dtype=dt)
+            setattr(t, k, _ptr(keep[k], ct))
+        for s in range(4):
+            for m in range(2):
+                t.xw[s][m] = int(d["xw"][s][m])
+                t.xf[s][m] = float(d["xf"][s][m])
+        for z in range(6):
+            t.evparam[z] = float(d["evparam"][z])
+        for z in range(20):
+            t.compo[z] = float(d["compo"][z])
+        _check(lib.p7_gpu_oprofile_Load(self.h, ctypes.byref(t)), "oprofile_Load")
+
+    def reconfig_length(self, L):
+        _check(lib.p7_gpu_oprofile_ReconfigLength(self.h, L), "oprofile_ReconfigLength")
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib.p7_gpu_oprofile_Destroy(self.h)
+            self.h = None
+
+
+class OMX:
+    def __init__(self, allocM=1, allocL=0, allocXL=0):
+        self.h = lib.p7_gpu_omx_Create(allocM, allocL, allocXL)
+        if not self.h:
+            raise P7GpuError(EMEM, "omx_Create")
+
+    def xmx(self):
+        L, ts, own = c_int(), c_float(), c_int()
+        p = lib.p7_gpu_omx_Xmx(self.h, ctypes.byref(L), ctypes.byref(ts), ctypes.byref(own))
+        return np.ctypeslib.as_array(p, ((L.value + 1) * 6,)).reshape(L.value + 1, 6).copy(), ts.value, own.value
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib.p7_gpu_omx_Destroy(self.h)
+            self.h = None
+
+
+def _single(fn, residues, om, *extra):
+    dsq = dsq_of(residues)
+    sc = c_float()
+    st = fn(_ptr(dsq, c_uint8), dsq.size - 2, om.h, *extra, ctypes.byref(sc))
+    return st, sc.value
+
+
+def msv_filter(residues, om):
+    st, sc = _single(lib.p7_gpu_MSVFilter, residues, om, None)
+    _check(st, "MSVFilter", (OK, ERANGE))
+    return st, sc
+
+
+def ssv_filter(residues, om):
+    st, sc = _single(lib.p7_gpu_SSVFilter, residues, om)
+    _check(st, "SSVFilter", (OK, ERANGE, ENORESULT))
+    return st, sc
+
+
+def viterbi_filter(residues, om):
+    st, sc = _single(lib.p7_gpu_ViterbiFilter, residues, om, None)
+    _check(st, "ViterbiFilter", (OK, ERANGE))
+    return st, sc
+
+
+def forward_parser(residues, om, fwd=None):
+    fwd = fwd or OMX()
+    st, sc = _single(lib.p7_gpu_ForwardParser, residues, om, fwd.h)
+    _check(st, "ForwardParser", (OK, ERANGE))
+    return st, sc, fwd
+
+
+def backward_parser(residues, om, fwd, bck=None):
+    bck = bck or OMX()
+    st, sc = _single(lib.p7_gpu_BackwardParser, residues, om, fwd.h, bck.h)
+    _check(st, "BackwardParser", (OK, ERANGE))
+    return st, sc, bck
+
+
+class SeqDB:
+    """HBM-resident digital database. residues: uint8 concatenation; offsets/lengths per sequence."""
+
+    def __init__(self, residues, offsets, lengths):
+        self.residues = np.ascontiguousarray(residues, np.uint8)
+        self.offsets = np.ascontiguousarray(offsets, np.int64)
+        self.lengths = np.ascontiguousarray(lengths, np.int32)
+        h = c_void_p()
+        _check(lib.p7_gpu_seqdb_Create(_ptr(self.residues, c_uint8), _ptr(self.offsets, c_int64),
+                                       _ptr(self.lengths, c_int32), self.lengths.size, ctypes.byref(h)), "seqdb_Create")
+        self.h = h
+        self.nseq = int(self.lengths.size)
+        self.nres = int(lib.p7_gpu_seqdb_NRes(h))
+
+    @classmethod
+    def from_list(cls, seqs):
+        lengths = np.array([len(s) for s in seqs], np.int32)
+        offsets = np.zeros(len(seqs), np.int64)
+        if len(seqs) > 1:
+            offsets[1:] = np.cumsum(lengths[:-1])
+        res = np.concatenate([np.asarray(s, np.uint8) for s in seqs]) if len(seqs) else np.zeros(0, np.uint8)
+        if res.size == 0:
+            res = np.zeros(1, np.uint8)
+        return cls(res, offsets, lengths)
+
+    def seq(self, i):
+        o, L = int(self.offsets[i]), int(self.lengths[i])
+        return self.residues[o:o + L]
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib.p7_gpu_seqdb_Destroy(self.h)
+            self.h = None
+
+
+def _batch(fn, om, db, idx, name, with_status=True):
+    if idx is None:
+        n = db.nseq
+        ip = None
+    else:
+        idx = np.ascontiguousarray(idx, np.int64)
+        n = idx.size
+        ip = _ptr(idx, c_int64)
+    sc = np.zeros(n, np.float32)
+    if with_status:
+        st = np.zeros(n, np.int32)
+        _check(fn(om.h, db.h, ip, n, _ptr(sc, c_float), _ptr(st, c_int)), name)
+        return sc, st
+    _check(fn(om.h, db.h, ip, n, _ptr(sc, c_float)), name)
+    return sc
+
+
+def msv_batch(om, db, idx=None):
+    return _batch(lib.p7_gpu_MSVFilterBatch, om, db, idx, "MSVFilterBatch")
+
+
+def viterbi_batch(om, db, idx=None):
+    return _batch(lib.p7_gpu_ViterbiFilterBatch, om, db, idx, "ViterbiFilterBatch")
+
+
+def forward_batch(om, db, idx=None):
+    return _batch(lib.p7_gpu_ForwardParserBatch, om, db, idx, "ForwardParserBatch")
+
+
+def backward_batch(om, db, idx=None):
+    return _batch(lib.p7_gpu_BackwardParserBatch, om, db, idx, "BackwardParserBatch")
+
+
+def filter_score_batch(om, db, idx=None):
+    return _batch(lib.p7_gpu_FilterScoreBatch, om, db, idx, "FilterScoreBatch", with_status=False)
+
+
+class Pipeline:
+    """P7_PIPELINE thresholds and counters; run() is p7_Pipeline over a whole database."""
+
+    def __init__(self, F1=0.02, F2=1e-3, F3=1e-5, do_biasfilter=True, do_max=False):
+        self.h = lib.p7_gpu_pipeline_Create(F1, F2, F3, int(do_biasfilter), int(do_max))
+
+    def run(self, om, db, scores=True):
+        n = db.nseq
+        out = {"stage": np.zeros(n, np.uint8)}
+        hits = Hits()
+        hits.stage = _ptr(out["stage"], c_uint8)
+        if scores:
+            for k in ("nullsc", "usc", "filtersc", "vfsc", "fwdsc", "bcksc"):
+                out[k] = np.zeros(n, np.float32)
+                setattr(hits, k, _ptr(out[k], c_float))
+        _check(lib.p7_gpu_Pipeline(self.h, om.h, db.h, ctypes.byref(hits)), "Pipeline")
+        return out
+
+    def stats(self):
+        st = Stats()
+        _check(lib.p7_gpu_pipeline_GetStats(self.h, ctypes.byref(st)), "GetStats")
+        return st.as_dict()
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib.p7_gpu_pipeline_Destroy(self.h)
+            self.h = None

@@ -1,0 +1,228 @@
+/* p7_gpu.h -- C ABI of the MI355X (gfx950) HMMER3 filter pipeline.
+ *
+ * This is the drop-in boundary. Every entry point takes plain pointers and
+ * sizes, returns an Easel status code, and keeps the argument meaning of the
+ * HMMER 3.1b2 function it stands in for. Citations are to the reference tree
+ * (paths relative to the biobench2 root).
+ *
+ * Conventions shared with HMMER:
+ *   - digital sequences are uint8_t (ESL_DSQ), residues dsq[1..L]; dsq[0] and
+ *     dsq[L+1] are sentinels that are never read;
+ *   - scores are in nats; a filter that overflows its limited range returns
+ *     p7G_ERANGE with *ret_sc = +infinity (a high-scoring hit);
+ *   - the quantised tables in P7_GPU_TABLES are the P7_OPROFILE tables
+ *     (HMMER/src/impl_sse/impl_sse.h:76-143) with the Farrar striping removed:
+ *     node k (1..M) of residue x sits at [x*M + k-1].
+ */
+#ifndef P7_GPU_INCLUDED
+#define P7_GPU_INCLUDED
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes: numerically identical to Easel's (HMMER/easel/easel.h:95-123). */
+#define p7G_OK          0
+#define p7G_FAIL        1
+#define p7G_EOF         3
+#define p7G_EMEM        5
+#define p7G_ENOTFOUND   6
+#define p7G_EFORMAT     7
+#define p7G_EINCOMPAT  10
+#define p7G_EINVAL     11
+#define p7G_ESYS       12
+#define p7G_ERANGE     16
+#define p7G_ENORESULT  19
+#define p7G_EDEVICE    64   /* HIP runtime failure (no Easel counterpart)     */
+
+/* Alphabet types, as Easel numbers them (HMMER/easel/esl_alphabet.h). */
+#define p7G_RNA    1
+#define p7G_DNA    2
+#define p7G_AMINO  3
+
+/* Profile modes (HMMER/src/hmmer.h:60-63). The filters need a local mode. */
+#define p7G_LOCAL     1
+#define p7G_GLOCAL    2
+#define p7G_UNILOCAL  3
+
+/* Special-state and transition indices of P7_OPROFILE
+ * (HMMER/src/impl_sse/impl_sse.h:82-86). */
+enum { p7G_XE = 0, p7G_XN = 1, p7G_XJ = 2, p7G_XC = 3 };
+enum { p7G_MOVE = 0, p7G_LOOP = 1 };
+enum { p7G_BM = 0, p7G_MM = 1, p7G_IM = 2, p7G_DM = 3,
+       p7G_MD = 4, p7G_MI = 5, p7G_II = 6, p7G_DD = 7 };
+
+/* Parser special cells per row (HMMER/src/impl_sse/impl_sse.h:173-175). */
+enum { p7G_X_E = 0, p7G_X_N = 1, p7G_X_J = 2, p7G_X_B = 3, p7G_X_C = 4, p7G_X_SCALE = 5 };
+#define p7G_NXCELLS 6
+
+/* Evalue parameters (HMMER/src/hmmer.h:72). */
+enum { p7G_MMU = 0, p7G_MLAMBDA = 1, p7G_VMU = 2, p7G_VLAMBDA = 3, p7G_FTAU = 4, p7G_FLAMBDA = 5 };
+
+typedef struct p7g_hmm_s       P7_GPU_HMM;       /* probability model from a HMMER3 save file       */
+typedef struct p7g_bg_s        P7_GPU_BG;        /* null model + bias-composition filter HMM (P7_BG) */
+typedef struct p7g_profile_s   P7_GPU_PROFILE;   /* float log-odds profile (P7_PROFILE)             */
+typedef struct p7g_oprofile_s  P7_GPU_OPROFILE;  /* quantised tables, host + device (P7_OPROFILE)   */
+typedef struct p7g_omx_s       P7_GPU_OMX;       /* DP scratch and parser specials (P7_OMX)          */
+typedef struct p7g_seqdb_s     P7_GPU_SEQDB;     /* HBM-resident digital target database             */
+typedef struct p7g_pipeline_s  P7_GPU_PIPELINE;  /* thresholds + counters (P7_PIPELINE)             */
+
+/* The quantised P7_OPROFILE tables, unstriped. Used to import a profile that
+ * HMMER already converted (de-striped by the caller) and to export ours. */
+typedef struct p7g_tables_s {
+  int       M;                 /* model length                                           */
+  int       Kp;                /* alphabet size incl. degenerate/gap/missing codes       */
+  int       abctype;           /* p7G_AMINO, p7G_DNA, p7G_RNA                             */
+  /* MSVFilter: biased uchar costs (p7_oprofile.c:764-804)                                */
+  const uint8_t *rbv;          /* [Kp][M]                                                */
+  uint8_t   tbm_b, tec_b, tjb_b, base_b, bias_b;
+  float     scale_b;
+  /* ViterbiFilter: int16 scores (p7_oprofile.c:817-912)                                 */
+  const int16_t *rwv;          /* [Kp][M]                                                */
+  const int16_t *twv;          /* [8][M]: BM,MM,IM,DM into node k; MD,MI,II,DD out of k  */
+  int16_t   xw[4][2];          /* [E,N,J,C][MOVE,LOOP]                                   */
+  float     scale_w;
+  int16_t   base_w, ddbound_w;
+  /* Forward/Backward: odds ratios (p7_oprofile.c:920-985)                                */
+  const float *rfv;            /* [Kp][M]                                                */
+  const float *tfv;            /* [8][M], same ordering as twv                           */
+  float     xf[4][2];
+  /* configuration                                                                          */
+  int       L;                 /* currently configured target length                     */
+  int       mode;              /* p7G_LOCAL or p7G_UNILOCAL                               */
+  float     nj;                /* expected J uses: 1 multihit, 0 unihit                  */
+  float     evparam[6];
+  float     compo[20];         /* model composition for the bias filter (K entries)      */
+} P7_GPU_TABLES;
+
+/* Per-target outputs of a batched pipeline run. Every pointer may be NULL;
+ * otherwise it holds one entry per database sequence, in database order.
+ * Scores that a stage did not compute are NaN. */
+typedef struct p7g_hits_s {
+  float   *nullsc;     /* p7_bg_NullOne                                    */
+  float   *usc;        /* MSV score (nats); +inf when it overflowed          */
+  float   *filtersc;   /* bias filter score (MSV survivors)                 */
+  float   *vfsc;       /* Viterbi filter score (when the pipeline ran it)  */
+  float   *fwdsc;      /* Forward parser score                               */
+  float   *bcksc;      /* Backward parser score (F3 survivors)              */
+  uint8_t *stage;      /* 0 failed MSV, 1 failed bias, 2 failed Vit,
+                          3 failed Fwd, 4 passed all filters               */
+} P7_GPU_HITS;
+
+/* Counters of P7_PIPELINE (HMMER/src/hmmer.h:1223-1237) plus device timings. */
+typedef struct p7g_stats_s {
+  int64_t nmodels, nseqs, nres, nnodes;
+  int64_t n_past_msv, n_past_bias, n_past_vit, n_past_fwd;
+  int64_t n_msv_full;          /* targets the MSV stage rescored with the J state   */
+  double  cells_msv, cells_bias, cells_vit, cells_fwd, cells_bck;
+  double  ms_msv, ms_bias, ms_vit, ms_fwd, ms_bck, ms_total;   /* device time, last call */
+} P7_GPU_STATS;
+
+/* ---- device ------------------------------------------------------------- */
+int         p7_gpu_Init(int device);               /* select device, create the stream  */
+int         p7_gpu_Synchronize(void);
+const char *p7_gpu_LastError(void);                /* text of the last failure           */
+
+/* ---- models (host) ------------------------------------------------------- */
+/* Replaces p7_hmmfile_Read (HMMER/src/p7_hmmfile.c:1180, ascii 3/e and 3/f at :1294). */
+int   p7_gpu_hmmfile_Read(const char *path, int which, P7_GPU_HMM **ret_hmm);
+int   p7_gpu_hmm_Info(const P7_GPU_HMM *hmm, int *ret_M, int *ret_abctype, char *name, int namelen);
+void  p7_gpu_hmm_Destroy(P7_GPU_HMM *hmm);
+
+/* Replaces p7_bg_Create (HMMER/src/p7_bg.c:55). */
+P7_GPU_BG *p7_gpu_bg_Create(int abctype);
+void       p7_gpu_bg_Destroy(P7_GPU_BG *bg);
+/* Replaces p7_bg_SetLength (p7_bg.c:191). */
+int   p7_gpu_bg_SetLength(P7_GPU_BG *bg, int L);
+/* Replaces p7_bg_NullOne (p7_bg.c:359). */
+int   p7_gpu_bg_NullOne(const P7_GPU_BG *bg, const uint8_t *dsq, int L, float *ret_sc);
+/* Replaces p7_bg_SetFilter (p7_bg.c:431). */
+int   p7_gpu_bg_SetFilter(P7_GPU_BG *bg, int M, const float *compo);
+/* Replaces p7_bg_FilterScore (p7_bg.c:473); runs the 2-state Forward on the GPU. */
+int   p7_gpu_bg_FilterScore(P7_GPU_BG *bg, const uint8_t *dsq, int L, float *ret_sc);
+
+/* Replaces p7_ProfileConfig (HMMER/src/modelconfig.c:49). */
+int   p7_gpu_ProfileConfig(const P7_GPU_HMM *hmm, const P7_GPU_BG *bg, int L, int mode, P7_GPU_PROFILE **ret_gm);
+/* Replaces p7_ReconfigLength (modelconfig.c:209). */
+int   p7_gpu_profile_ReconfigLength(P7_GPU_PROFILE *gm, int L);
+/* Copies out the float scores: tsc[M*8] (P7_PROFILE order MM,IM,DM,BM,MD,DD,MI,II),
+ * rsc[Kp*(M+1)*2] ([x][k][MSC,ISC]), xsc[4*2] ([E,N,J,C][LOOP,MOVE]). */
+int   p7_gpu_profile_GetScores(const P7_GPU_PROFILE *gm, float *tsc, float *rsc, float *xsc);
+void  p7_gpu_profile_Destroy(P7_GPU_PROFILE *gm);
+
+/* Replaces p7_oprofile_Create (HMMER/src/impl_sse/p7_oprofile.c:50). */
+P7_GPU_OPROFILE *p7_gpu_oprofile_Create(int allocM, int abctype);
+/* Replaces p7_oprofile_Convert (p7_oprofile.c:1005). */
+int   p7_gpu_oprofile_Convert(const P7_GPU_PROFILE *gm, P7_GPU_OPROFILE *om);
+/* Imports tables of an already converted P7_OPROFILE (de-striped by the caller). */
+int   p7_gpu_oprofile_Load(P7_GPU_OPROFILE *om, const P7_GPU_TABLES *t);
+/* Exports the tables; pointers stay valid until the next Load/Convert/Destroy. */
+int   p7_gpu_oprofile_GetTables(const P7_GPU_OPROFILE *om, P7_GPU_TABLES *t);
+/* Replace p7_oprofile_ReconfigLength / MSVLength / RestLength (p7_oprofile.c:1062-1127). */
+int   p7_gpu_oprofile_ReconfigLength(P7_GPU_OPROFILE *om, int L);
+int   p7_gpu_oprofile_ReconfigMSVLength(P7_GPU_OPROFILE *om, int L);
+int   p7_gpu_oprofile_ReconfigRestLength(P7_GPU_OPROFILE *om, int L);
+void  p7_gpu_oprofile_Destroy(P7_GPU_OPROFILE *om);
+
+/* Replaces p7_omx_Create / GrowTo / Destroy (HMMER/src/impl_sse/p7_omx.c:64,140). */
+P7_GPU_OMX *p7_gpu_omx_Create(int allocM, int allocL, int allocXL);
+int   p7_gpu_omx_GrowTo(P7_GPU_OMX *ox, int allocM, int allocL, int allocXL);
+/* The parser specials xmx[(L+1)*6] ([i*6 + p7G_X_*]), totscale and has_own_scales. */
+const float *p7_gpu_omx_Xmx(const P7_GPU_OMX *ox, int *ret_L, float *ret_totscale, int *ret_has_own_scales);
+void  p7_gpu_omx_Destroy(P7_GPU_OMX *ox);
+
+/* ---- single-target drop-ins (same signatures as impl_sse.h:308-344) -------- */
+/* p7_MSVFilter (msvfilter.c:76). */
+int   p7_gpu_MSVFilter(const uint8_t *dsq, int L, const P7_GPU_OPROFILE *om, P7_GPU_OMX *ox, float *ret_sc);
+/* p7_SSVFilter (ssvfilter.c:878): p7G_ENORESULT when the J state would be used. */
+int   p7_gpu_SSVFilter(const uint8_t *dsq, int L, const P7_GPU_OPROFILE *om, float *ret_sc);
+/* p7_ViterbiFilter (vitfilter.c:84). */
+int   p7_gpu_ViterbiFilter(const uint8_t *dsq, int L, const P7_GPU_OPROFILE *om, P7_GPU_OMX *ox, float *ret_sc);
+/* p7_ForwardParser (fwdback.c:134). Fills fwd's specials for the Backward pass. */
+int   p7_gpu_ForwardParser(const uint8_t *dsq, int L, const P7_GPU_OPROFILE *om, P7_GPU_OMX *fwd, float *opt_sc);
+/* p7_BackwardParser (fwdback.c:238), using fwd's row scale factors. */
+int   p7_gpu_BackwardParser(const uint8_t *dsq, int L, const P7_GPU_OPROFILE *om, const P7_GPU_OMX *fwd,
+                            P7_GPU_OMX *bck, float *opt_sc);
+
+/* ---- HBM-resident database and batched stages ----------------------------- */
+/* residues: concatenation; seq i is residues[offset[i] .. offset[i]+length[i]-1]
+ * (digital codes 0..Kp-1, no sentinels). The library keeps its own copy in HBM,
+ * grouped by length; results are always reported in the caller's order. */
+int     p7_gpu_seqdb_Create(const uint8_t *residues, const int64_t *offset, const int32_t *length,
+                            int64_t nseq, P7_GPU_SEQDB **ret_db);
+int64_t p7_gpu_seqdb_NSeq(const P7_GPU_SEQDB *db);
+int64_t p7_gpu_seqdb_NRes(const P7_GPU_SEQDB *db);
+void    p7_gpu_seqdb_Destroy(P7_GPU_SEQDB *db);
+
+/* Each target is scored with the profile length-configured to its own L, as
+ * hmmsearch does (hmmsearch.c:1404-1405). idx == NULL means every sequence;
+ * outputs are indexed like idx (or like the database when idx is NULL). */
+int   p7_gpu_MSVFilterBatch    (const P7_GPU_OPROFILE *om, const P7_GPU_SEQDB *db, const int64_t *idx, int64_t n,
+                                float *ret_sc, int *ret_status);
+int   p7_gpu_ViterbiFilterBatch(const P7_GPU_OPROFILE *om, const P7_GPU_SEQDB *db, const int64_t *idx, int64_t n,
+                                float *ret_sc, int *ret_status);
+int   p7_gpu_ForwardParserBatch(const P7_GPU_OPROFILE *om, const P7_GPU_SEQDB *db, const int64_t *idx, int64_t n,
+                                float *ret_sc, int *ret_status);
+int   p7_gpu_BackwardParserBatch(const P7_GPU_OPROFILE *om, const P7_GPU_SEQDB *db, const int64_t *idx, int64_t n,
+                                 float *ret_sc, int *ret_status);
+int   p7_gpu_FilterScoreBatch  (const P7_GPU_OPROFILE *om, const P7_GPU_SEQDB *db, const int64_t *idx, int64_t n,
+                                float *ret_sc);
+
+/* ---- pipeline (p7_pipeline.c:99, :666-731) -------------------------------- */
+/* F1/F2/F3 are P-value thresholds (defaults 0.02, 1e-3, 1e-5; p7_pipeline.c:210-212);
+ * do_biasfilter mirrors --nobias; do_max mirrors --max (all filters off). */
+P7_GPU_PIPELINE *p7_gpu_pipeline_Create(double F1, double F2, double F3, int do_biasfilter, int do_max);
+void  p7_gpu_pipeline_Destroy(P7_GPU_PIPELINE *pli);
+/* Runs MSV -> bias -> Viterbi -> Forward -> Backward over every database
+ * sequence on the device, with the stage gates of p7_Pipeline. Accumulates
+ * the counters like p7_pli_NewModel/NewSeq/p7_Pipeline. */
+int   p7_gpu_Pipeline(P7_GPU_PIPELINE *pli, const P7_GPU_OPROFILE *om, const P7_GPU_SEQDB *db, P7_GPU_HITS *hits);
+int   p7_gpu_pipeline_GetStats(const P7_GPU_PIPELINE *pli, P7_GPU_STATS *st);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* P7_GPU_INCLUDED */

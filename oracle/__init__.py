@@ -1,0 +1,324 @@
+"""CPU oracle of the HMMER3 filter path -- TEST INFRASTRUCTURE.
+
+Only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg use this
+package, and only as the checker. It wraps oracle/_build/libp7oracle.so, the
+plain-C restatement in oracle/p7_oracle.c (each function there cites the
+reference routine it restates).
+
+Tables are the unstriped P7_GPU_TABLES layout as returned by
+biobench2_amd.p7gpu.OProfile.tables().
+"""
+import ctypes
+import os
+import subprocess
+from ctypes import POINTER, c_double, c_float, c_int, c_int16, c_int32, c_int64, c_uint8
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(_HERE, "_build", "libp7oracle.so")
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def _load():
+    if not os.path.exists(LIBPATH):
+        build()
+    return ctypes.CDLL(LIBPATH)
+
+
+lib = _load()
+pu8, pf, pi, pi16, pd = POINTER(c_uint8), POINTER(c_float), POINTER(c_int), POINTER(c_int16), POINTER(c_double)
+
+
+class OrcProfile(ctypes.Structure):
+    _fields_ = [
+        ("M", c_int), ("Kp", c_int),
+        ("rbv", pu8), ("tbm", c_int), ("tec", c_int), ("base", c_int), ("bias", c_int), ("scale_b", c_float),
+        ("rwv", pi16), ("twv", pi16), ("xw", c_int16 * 8), ("scale_w", c_float), ("base_w", c_int), ("ddbound_w", c_int),
+        ("rfv", pf), ("tfv", pf), ("xf", c_float * 8),
+        ("tjb_by_L", pu8), ("xwmove_by_L", pi16), ("xfmove_by_L", pf), ("xfloop_by_L", pf),
+        ("eo", pf), ("t10", c_float), ("t11", c_float), ("pi0", c_float), ("pi1", c_float),
+        ("evparam", c_double * 6), ("F1", c_double), ("F2", c_double), ("F3", c_double), ("do_bias", c_int),
+    ]
+
+
+class OrcHit(ctypes.Structure):
+    _fields_ = [(n, c_float) for n in ("usc", "nullsc", "filtersc", "vfsc", "fwdsc", "bcksc")] + \
+               [(n, c_int) for n in ("stage", "msv_status", "vit_status", "fwd_status")]
+
+
+lib.orc_msv_filter.argtypes = [pu8, c_int, c_int, pu8, c_int, c_int, c_int, c_int, c_int, c_float, pf, pi]
+lib.orc_vit_filter.argtypes = [pu8, c_int, c_int, pi16, pi16, pi16, c_float, c_int, c_int, pf, pi]
+lib.orc_fwd_parser.argtypes = [pu8, c_int, c_int, c_int, pf, pf, pf, pf, pf, pf]
+lib.orc_bck_parser.argtypes = [pu8, c_int, c_int, c_int, pf, pf, pf, pf, pf, pi, pf]
+lib.orc_filter_score.argtypes = [pu8, c_int, pf, c_float, c_float, c_float, c_float, c_float, c_float, c_float]
+lib.orc_filter_score.restype = c_float
+lib.orc_null_one.argtypes = [c_int]
+lib.orc_null_one.restype = c_float
+lib.orc_generic_viterbi.argtypes = [pu8, c_int, c_int, pf, pf, pf, c_int, pf]
+lib.orc_forward_log64.argtypes = [pu8, c_int, c_int, pf, pf, pf, pd]
+lib.orc_backward_log64.argtypes = [pu8, c_int, c_int, pf, pf, pf, pd]
+lib.orc_brute_force.argtypes = [pu8, c_int, c_int, pf, pf, pf, pd, pd]
+lib.orc_gumbel_surv.argtypes = [c_double, c_double, c_double]
+lib.orc_gumbel_surv.restype = c_double
+lib.orc_exp_surv.argtypes = [c_double, c_double, c_double]
+lib.orc_exp_surv.restype = c_double
+lib.orc_pipeline_db.argtypes = [POINTER(OrcProfile), pu8, POINTER(c_int64), POINTER(c_int32), c_int64, c_int, POINTER(OrcHit)]
+lib.orc_msv_db.argtypes = [POINTER(OrcProfile), pu8, POINTER(c_int64), POINTER(c_int32), c_int64, c_int, pf]
+
+
+def _p(a, t):
+    return a.ctypes.data_as(POINTER(t))
+
+
+def dsq_of(residues):
+    r = np.asarray(residues, np.uint8)
+    out = np.empty(r.size + 2, np.uint8)
+    out[0] = out[-1] = 255
+    out[1:-1] = r
+    return out
+
+
+def msv_filter(residues, t, tjb=None):
+    """(status, score, xJ) of p7_MSVFilter with table dict t; tjb defaults to t['tjb_b']."""
+    dsq = dsq_of(residues)
+    rbv = np.ascontiguousarray(t["rbv"], np.uint8)
+    sc, xj = c_float(), c_int()
+    st = lib.orc_msv_filter(_p(dsq, c_uint8), dsq.size - 2, t["M"], _p(rbv, c_uint8), t["tbm_b"], t["tec_b"],
+                            t["tjb_b"] if tjb is None else tjb, t["base_b"], t["bias_b"], t["scale_b"],
+                            ctypes.byref(sc), ctypes.byref(xj))
+    return st, sc.value, xj.value
+
+
+def vit_filter(residues, t, xw=None):
+    dsq = dsq_of(residues)
+    rwv = np.ascontiguousarray(t["rwv"], np.int16)
+    twv = np.ascontiguousarray(t["twv"], np.int16)
+    xwa = np.ascontiguousarray(t["xw"] if xw is None else xw, np.int16).reshape(8)
+    sc, xc = c_float(), c_int()
+    st = lib.orc_vit_filter(_p(dsq, c_uint8), dsq.size - 2, t["M"], _p(rwv, c_int16), _p(twv, c_int16),
+                            _p(xwa, c_int16), t["scale_w"], t["base_w"], t["ddbound_w"], ctypes.byref(sc), ctypes.byref(xc))
+    return st, sc.value, xc.value
+
+
+def fwd_parser(residues, t, xf=None):
+    """(status, score, xmx[L+1,6], totscale) of p7_ForwardParser, fp32 as the reference computes it."""
+    dsq = dsq_of(residues)
+    L = dsq.size - 2
+    rfv = np.ascontiguousarray(t["rfv"], np.float32)
+    tfv = np.ascontiguousarray(t["tfv"], np.float32)
+    xfa = np.ascontiguousarray(t["xf"] if xf is None else xf, np.float32).reshape(8)
+    xmx = np.zeros((L + 1) * 6, np.float32)
+    ts, sc = c_float(), c_float()
+    st = lib.orc_fwd_parser(_p(dsq, c_uint8), L, t["M"], t["Kp"], _p(rfv, c_float), _p(tfv, c_float), _p(xfa, c_float),
+                            _p(xmx, c_float), ctypes.byref(ts), ctypes.byref(sc))
+    return st, sc.value, xmx.reshape(L + 1, 6), ts.value
+
+
+def bck_parser(residues, t, fwd_xmx, xf=None):
+    dsq = dsq_of(residues)
+    L = dsq.size - 2
+    rfv = np.ascontiguousarray(t["rfv"], np.float32)
+    tfv = np.ascontiguousarray(t["tfv"], np.float32)
+    xfa = np.ascontiguousarray(t["xf"] if xf is None else xf, np.float32).reshape(8)
+    fx = np.ascontiguousarray(fwd_xmx, np.float32).reshape(-1)
+    xmx = np.zeros((L + 1) * 6, np.float32)
+    own, sc = c_int(), c_float()
+    st = lib.orc_bck_parser(_p(dsq, c_uint8), L, t["M"], t["Kp"], _p(rfv, c_float), _p(tfv, c_float), _p(xfa, c_float),
+                            _p(fx, c_float), _p(xmx, c_float), ctypes.byref(own), ctypes.byref(sc))
+    return st, sc.value, xmx.reshape(L + 1, 6), own.value
+
+
+def null_one(L):
+    return lib.orc_null_one(L)
+
+
+def filter_score(residues, eo, t00, t01, t10, t11, pi0, pi1, p1):
+    dsq = dsq_of(residues)
+    e = np.ascontiguousarray(eo, np.float32).reshape(-1)
+    return lib.orc_filter_score(_p(dsq, c_uint8), dsq.size - 2, _p(e, c_float), t00, t01, t10, t11, pi0, pi1, p1)
+
+
+def _gm(tsc, rsc, xsc):
+    return (np.ascontiguousarray(tsc, np.float32), np.ascontiguousarray(rsc, np.float32),
+            np.ascontiguousarray(xsc, np.float32))
+
+
+def generic_viterbi(residues, tsc, rsc, xsc, local=True):
+    dsq = dsq_of(residues)
+    tsc, rsc, xsc = _gm(tsc, rsc, xsc)
+    sc = c_float()
+    lib.orc_generic_viterbi(_p(dsq, c_uint8), dsq.size - 2, tsc.shape[0], _p(tsc, c_float), _p(rsc, c_float),
+                            _p(xsc, c_float), int(local), ctypes.byref(sc))
+    return sc.value
+
+
+def forward_log64(residues, tsc, rsc, xsc):
+    dsq = dsq_of(residues)
+    tsc, rsc, xsc = _gm(tsc, rsc, xsc)
+    sc = c_double()
+    lib.orc_forward_log64(_p(dsq, c_uint8), dsq.size - 2, tsc.shape[0], _p(tsc, c_float), _p(rsc, c_float),
+                          _p(xsc, c_float), ctypes.byref(sc))
+    return sc.value
+
+
+def backward_log64(residues, tsc, rsc, xsc):
+    dsq = dsq_of(residues)
+    tsc, rsc, xsc = _gm(tsc, rsc, xsc)
+    sc = c_double()
+    lib.orc_backward_log64(_p(dsq, c_uint8), dsq.size - 2, tsc.shape[0], _p(tsc, c_float), _p(rsc, c_float),
+                           _p(xsc, c_float), ctypes.byref(sc))
+    return sc.value
+
+
+def brute_force(residues, tsc, rsc, xsc):
+    dsq = dsq_of(residues)
+    tsc, rsc, xsc = _gm(tsc, rsc, xsc)
+    f, v = c_double(), c_double()
+    lib.orc_brute_force(_p(dsq, c_uint8), dsq.size - 2, tsc.shape[0], _p(tsc, c_float), _p(rsc, c_float),
+                        _p(xsc, c_float), ctypes.byref(f), ctypes.byref(v))
+    return f.value, v.value
+
+
+def gumbel_surv(x, mu, lam):
+    return lib.orc_gumbel_surv(x, mu, lam)
+
+
+def exp_surv(x, mu, lam):
+    return lib.orc_exp_surv(x, mu, lam)
+
+
+# --- per-length constants (p7_oprofile.c:1086-1121), from libm through ctypes so
+# they match the C library's roundf/logf bit for bit
+_libm = ctypes.CDLL("libm.so.6")
+_libm.logf.argtypes = [c_float]
+_libm.logf.restype = c_float
+_libm.roundf.argtypes = [c_float]
+_libm.roundf.restype = c_float
+
+
+def _unbiased_byteify(scale_b, sc):
+    v = -1.0 * _libm.roundf(np.float32(scale_b) * np.float32(sc))
+    v = float(np.float32(v))
+    return 255 if v > 255.0 else int(v)
+
+
+def _wordify(scale_w, sc):
+    v = float(_libm.roundf(float(np.float32(np.float32(scale_w) * np.float32(sc)))))
+    if v >= 32767.0:
+        return 32767
+    if v <= -32768.0:
+        return -32768
+    return int(v)
+
+
+class LengthModel:
+    """tjb_b, xw move and xf move/loop for every L in 0..Lmax, as ReconfigLength sets them."""
+
+    def __init__(self, t, Lmax):
+        nj = np.float32(t["nj"])
+        self.tjb = np.zeros(Lmax + 1, np.uint8)
+        self.xwmove = np.zeros(Lmax + 1, np.int16)
+        self.xfmove = np.zeros(Lmax + 1, np.float32)
+        self.xfloop = np.zeros(Lmax + 1, np.float32)
+        for L in range(Lmax + 1):
+            pmove = np.float32(np.float32(2.0) + nj) / np.float32(np.float32(L) + np.float32(2.0) + nj)
+            ploop = np.float32(1.0) - pmove
+            self.tjb[L] = _unbiased_byteify(t["scale_b"], _libm.logf(np.float32(3.0) / np.float32(L + 3)))
+            self.xwmove[L] = _wordify(t["scale_w"], _libm.logf(pmove))
+            self.xfmove[L] = pmove
+            self.xfloop[L] = ploop
+
+
+def make_profile(t, bias_hmm, evparam, F1=0.02, F2=1e-3, F3=1e-5, do_bias=True, Lmax=6000):
+    """ORC_PROFILE for orc_pipeline_db/orc_msv_db. bias_hmm: dict eo[Kp,2], t10, t11, pi0, pi1."""
+    lm = LengthModel(t, Lmax)
+    p = OrcProfile()
+    keep = {"lm": lm}
+    for k, ct, dt in (("rbv", c_uint8, np.uint8), ("rwv", c_int16, np.int16), ("twv", c_int16, np.int16),
+                      ("rfv", c_float, np.float32), ("tfv", c_float, np.float32)):
+        keep[k] = np.ascontiguousarray(t[k], dt)
+        setattr(p, k, _p(keep[k], ct))
+    p.M, p.Kp = t["M"], t["Kp"]
+    p.tbm, p.tec, p.base, p.bias, p.scale_b = t["tbm_b"], t["tec_b"], t["base_b"], t["bias_b"], t["scale_b"]
+    for i, v in enumerate(np.asarray(t["xw"], np.int16).reshape(8)):
+        p.xw[i] = int(v)
+    for i, v in enumerate(np.asarray(t["xf"], np.float32).reshape(8)):
+        p.xf[i] = float(v)
+    p.scale_w, p.base_w, p.ddbound_w = t["scale_w"], t["base_w"], t["ddbound_w"]
+    p.tjb_by_L = _p(lm.tjb, c_uint8)
+    p.xwmove_by_L = _p(lm.xwmove, c_int16)
+    p.xfmove_by_L = _p(lm.xfmove, c_float)
+    p.xfloop_by_L = _p(lm.xfloop, c_float)
+    keep["eo"] = np.ascontiguousarray(bias_hmm["eo"], np.float32).reshape(-1)
+    p.eo = _p(keep["eo"], c_float)
+    p.t10, p.t11, p.pi0, p.pi1 = bias_hmm["t10"], bias_hmm["t11"], bias_hmm["pi0"], bias_hmm["pi1"]
+    for i in range(6):
+        p.evparam[i] = float(evparam[i])
+    p.F1, p.F2, p.F3, p.do_bias = F1, F2, F3, int(do_bias)
+    return p, keep
+
+
+def pipeline_db(prof, residues, offsets, lengths, nthreads=1):
+    """Stage and scores of p7_Pipeline for every sequence; returns a dict of arrays."""
+    res = np.ascontiguousarray(residues, np.uint8)
+    off = np.ascontiguousarray(offsets, np.int64)
+    lens = np.ascontiguousarray(lengths, np.int32)
+    n = lens.size
+    out = (OrcHit * n)()
+    lib.orc_pipeline_db(ctypes.byref(prof), _p(res, c_uint8), _p(off, c_int64), _p(lens, c_int32), n, nthreads, out)
+    arr = np.frombuffer(out, dtype=np.dtype([(nm, np.float32) for nm in ("usc", "nullsc", "filtersc", "vfsc", "fwdsc", "bcksc")] +
+                                            [(nm, np.int32) for nm in ("stage", "msv_status", "vit_status", "fwd_status")]))
+    return {nm: arr[nm].copy() for nm in arr.dtype.names}
+
+
+def msv_db(prof, residues, offsets, lengths, nthreads=1):
+    res = np.ascontiguousarray(residues, np.uint8)
+    off = np.ascontiguousarray(offsets, np.int64)
+    lens = np.ascontiguousarray(lengths, np.int32)
+    out = np.zeros(lens.size, np.float32)
+    lib.orc_msv_db(ctypes.byref(prof), _p(res, c_uint8), _p(off, c_int64), _p(lens, c_int32), lens.size, nthreads,
+                   _p(out, c_float))
+    return out
+
+
+AMINO_BG = np.array([0.0787945, 0.0151600, 0.0535222, 0.0668298, 0.0397062, 0.0695071, 0.0229198,
+                     0.0590092, 0.0594422, 0.0963728, 0.0237718, 0.0414386, 0.0482904, 0.0395639,
+                     0.0540978, 0.0683364, 0.0540687, 0.0673417, 0.0114135, 0.0304133], np.float32)
+AMINO_DEGEN = {21: "ND", 22: "IL", 23: "QE", 24: "K", 25: "C"}     # B J Z O U; 26 = X (any)
+AMINO_SYM = "ACDEFGHIKLMNPQRSTVWY"
+
+
+def bias_hmm(t):
+    """Two-state filter HMM of p7_bg_SetFilter (p7_bg.c:431-453) + esl_hmm_Configure (esl_hmm.c:116-153)."""
+    f32 = np.float32
+    K, Kp = (20, 29) if t["abctype"] == 3 else (4, 18)
+    f = AMINO_BG if t["abctype"] == 3 else np.full(K, f32(1.0) / f32(K), np.float32)
+    e = [f.astype(np.float32), np.asarray(t["compo"][:K], np.float32)]
+    eo = np.ones((Kp, 2), np.float32)
+    for x in range(K):
+        for s in range(2):
+            eo[x, s] = f32(e[s][x]) / f32(f[x])
+    degen = {}
+    if t["abctype"] == 3:
+        for x, letters in AMINO_DEGEN.items():
+            degen[x] = [AMINO_SYM.index(c) for c in letters]
+        degen[26] = list(range(20))
+    else:
+        dna = {5: "AG", 6: "CT", 7: "AC", 8: "GT", 9: "CG", 10: "AT", 11: "ACT", 12: "CGT", 13: "ACG", 14: "AGT"}
+        for x, letters in dna.items():
+            degen[x] = ["ACGT".index(c) for c in letters]
+        degen[15] = list(range(4))
+    for x, ys in degen.items():
+        for s in range(2):
+            num, den = f32(0.0), f32(0.0)
+            for y in ys:
+                num = f32(num + e[s][y])
+                den = f32(den + f[y])
+            eo[x, s] = f32(num / den) if den > 0 else f32(0.0)
+    L1 = f32(f32(t["M"]) / f32(8.0))
+    return {"eo": eo, "t10": float(f32(1.0) / f32(L1 + f32(1.0))), "t11": float(f32(L1 / f32(L1 + f32(1.0)))),
+            "pi0": float(f32(0.999)), "pi1": float(f32(0.001))}

@@ -1,0 +1,840 @@
+/* p7_oracle.c -- CPU oracle of the HMMER3 filter path. TEST INFRASTRUCTURE.
+ *
+ * This file is the checker against which the GPU library is tested. Only the
+ * test suite, the smoke check and the CPU-baseline leg of the benchmark load
+ * it; the product never links or calls it.
+ *
+ * Each function restates one reference routine (paths relative to the
+ * biobench2 root) in plain C, on the unstriped tables that the product's
+ * P7_GPU_TABLES also uses ([x*M + k-1], [t*M + k-1]):
+ *   orc_msv_filter     HMMER/src/impl_sse/msvfilter.c:76-210   (uchar, saturating)
+ *   orc_vit_filter     HMMER/src/impl_sse/vitfilter.c:84-249   (int16, lazy-F)
+ *   orc_fwd_parser     HMMER/src/impl_sse/fwdback.c:258-465    (fp32, 4-lane striped emulation)
+ *   orc_bck_parser     HMMER/src/impl_sse/fwdback.c:470-735    (fp32, 4-lane striped emulation)
+ *   orc_filter_score   easel/esl_hmm.c:351-409 + HMMER/src/p7_bg.c:473-484
+ *   orc_null_one       HMMER/src/p7_bg.c:359-363
+ *   orc_generic_viterbi HMMER/src/generic_viterbi.c:66-158 (float, full matrix)
+ *   orc_forward_log64 / orc_backward_log64: textbook log-space Forward/Backward
+ *                      in double over the same profile (generic_fwdback.c:50-253
+ *                      structure, exact log-sum-exp instead of the table)
+ *   orc_brute_force    enumeration of every state path (itest_brute.c idea)
+ *   orc_pipeline_db    the stage gates of p7_Pipeline (p7_pipeline.c:687-731)
+ * The Forward/Backward restatements keep the reference's four-lane striping,
+ * operation order and FTZ/DAZ mode, so they reproduce its fp32 values.
+ */
+#include <math.h>
+#include <float.h>
+#include <pthread.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#include <xmmintrin.h>     /* MXCSR flush-to-zero / denormals-are-zero control only */
+
+#define LOG2_D 0.69314718055994529
+
+/* om-style special indices: [E,N,J,C][MOVE,LOOP] */
+#define XO(s,t) ((s)*2 + (t))
+enum { XE = 0, XN = 1, XJ = 2, XC = 3 };
+enum { MOVE = 0, LOOP = 1 };
+enum { T_BM = 0, T_MM, T_IM, T_DM, T_MD, T_MI, T_II, T_DD };
+/* gm-style: tsc[k*8 + P_*], xsc[s*2 + LOOP/MOVE] with LOOP=0, MOVE=1 */
+enum { P_MM = 0, P_IM = 1, P_DM = 2, P_BM = 3, P_MD = 4, P_DD = 5, P_MI = 6, P_II = 7 };
+enum { GL = 0, GM = 1 };
+
+static inline int u8_adds(int a, int b) { int s = a + b; return s > 255 ? 255 : s; }
+static inline int u8_subs(int a, int b) { int s = a - b; return s < 0 ? 0 : s; }
+static inline int i16_adds(int a, int b) { int s = a + b; return s > 32767 ? 32767 : (s < -32768 ? -32768 : s); }
+static inline int imax(int a, int b) { return a > b ? a : b; }
+
+/*****************************************************************
+ * MSV filter (msvfilter.c:76-210). The striped SSE loop works element-wise
+ * on M(i-1,k-1), so a loop over k in order gives the same values.
+ *****************************************************************/
+int
+orc_msv_filter(const uint8_t *dsq, int L, int M, const uint8_t *rbv, int tbm, int tec, int tjb,
+               int base, int bias, float scale_b, float *ret_sc, int *ret_xj)
+{
+  int  *dp = calloc((size_t) M + 1, sizeof(int));
+  int   tjbm = (uint8_t) ((int8_t) tjb + (int8_t) tbm);      /* set1_epi8 keeps the low byte */
+  int   xJ = 0, xB = u8_subs(base, tjbm);
+  int   i, k;
+  if (! dp) return 5;
+  for (i = 1; i <= L; i++) {
+    const uint8_t *rsc = rbv + (size_t) dsq[i] * M;
+    int xE = 0, mpv = 0, sv;
+    for (k = 1; k <= M; k++) {
+      sv  = imax(mpv, xB);
+      sv  = u8_adds(sv, bias);
+      sv  = u8_subs(sv, rsc[k-1]);
+      xE  = imax(xE, sv);
+      mpv = dp[k];
+      dp[k] = sv;
+    }
+    if (u8_adds(xE, bias) == 255) {            /* overflow: some element saturated */
+      free(dp);
+      *ret_sc = INFINITY;
+      if (ret_xj) *ret_xj = -1;
+      return 16;
+    }
+    xE = u8_subs(xE, tec);
+    xJ = imax(xJ, xE);
+    xB = u8_subs(imax(base, xJ), tjbm);
+  }
+  free(dp);
+  if (ret_xj) *ret_xj = xJ;
+  *ret_sc  = ((float) (xJ - tjb) - (float) base);
+  *ret_sc /= scale_b;
+  *ret_sc -= 3.0;
+  return 0;
+}
+
+/*****************************************************************
+ * Viterbi filter (vitfilter.c:84-249), with the reference's lazy-F test:
+ * D->D paths are only closed when max_k D(k) + ddbound > xB. Dmax covers
+ * node M+1 (M(M) + tMD(M)), which the striped layout always holds.
+ *****************************************************************/
+int
+orc_vit_filter(const uint8_t *dsq, int L, int M, const int16_t *rwv, const int16_t *twv, const int16_t *xw,
+               float scale_w, int base_w, int ddbound_w, float *ret_sc, int *ret_xc)
+{
+  int *Mp = malloc(sizeof(int) * (size_t) (M + 2)), *Ip = malloc(sizeof(int) * (size_t) (M + 2)), *Dp = malloc(sizeof(int) * (size_t) (M + 2));
+  int *Mc = malloc(sizeof(int) * (size_t) (M + 2)), *Ic = malloc(sizeof(int) * (size_t) (M + 2)), *Dc = malloc(sizeof(int) * (size_t) (M + 2));
+  int16_t xN, xB, xJ, xC, xE;
+  int i, k;
+#define TW(t,k) ((int) twv[(size_t)(t) * M + (k) - 1])
+  if (! Mp || ! Ip || ! Dp || ! Mc || ! Ic || ! Dc) return 5;
+  for (k = 0; k <= M + 1; k++) Mp[k] = Ip[k] = Dp[k] = -32768;
+  xN = (int16_t) base_w;
+  xB = (int16_t) (xN + xw[XO(XN,MOVE)]);
+  xJ = -32768; xC = -32768; xE = -32768;
+
+  for (i = 1; i <= L; i++) {
+    const int16_t *rsc = rwv + (size_t) dsq[i] * M;
+    int xEm = -32768, Dmax = -32768, sv;
+    Mc[0] = Ic[0] = Dc[0] = -32768;
+    Dc[1] = -32768;
+    for (k = 1; k <= M; k++) {
+      sv = i16_adds(xB, TW(T_BM, k));
+      sv = imax(sv, i16_adds(Mp[k-1], TW(T_MM, k)));
+      sv = imax(sv, i16_adds(Ip[k-1], TW(T_IM, k)));
+      sv = imax(sv, i16_adds(Dp[k-1], TW(T_DM, k)));
+      sv = i16_adds(sv, rsc[k-1]);
+      xEm = imax(xEm, sv);
+      Mc[k] = sv;
+      Dc[k+1] = i16_adds(sv, TW(T_MD, k));
+      Dmax = imax(Dmax, Dc[k+1]);
+      Ic[k] = imax(i16_adds(Mp[k], TW(T_MI, k)), i16_adds(Ip[k], TW(T_II, k)));
+    }
+    xE = (int16_t) xEm;
+    if (xE >= 32767) {
+      free(Mp); free(Ip); free(Dp); free(Mc); free(Ic); free(Dc);
+      *ret_sc = INFINITY;
+      if (ret_xc) *ret_xc = 32767;
+      return 16;
+    }
+    xN = (int16_t) (xN + xw[XO(XN,LOOP)]);
+    xC = (int16_t) imax(xC + xw[XO(XC,LOOP)], xE + xw[XO(XE,MOVE)]);
+    xJ = (int16_t) imax(xJ + xw[XO(XJ,LOOP)], xE + xw[XO(XE,LOOP)]);
+    xB = (int16_t) imax(xJ + xw[XO(XJ,MOVE)], xN + xw[XO(XN,MOVE)]);
+    if (Dmax + ddbound_w > xB)
+      for (k = 2; k <= M; k++) Dc[k] = imax(Dc[k], i16_adds(Dc[k-1], TW(T_DD, k-1)));
+    { int *t; t = Mp; Mp = Mc; Mc = t; t = Ip; Ip = Ic; Ic = t; t = Dp; Dp = Dc; Dc = t; }
+    Mp[M+1] = Ip[M+1] = Dp[M+1] = -32768;
+  }
+#undef TW
+  free(Mp); free(Ip); free(Dp); free(Mc); free(Ic); free(Dc);
+  if (ret_xc) *ret_xc = xC;
+  if (xC > -32768) {
+    *ret_sc = (float) xC + (float) xw[XO(XC,MOVE)] - (float) base_w;
+    *ret_sc /= scale_w;
+    *ret_sc -= 3.0;
+  } else *ret_sc = -INFINITY;
+  return 0;
+}
+
+/*****************************************************************
+ * Forward / Backward parsers: four-lane striped emulation of fwdback.c
+ *****************************************************************/
+typedef struct { float v[4]; } V4;
+static inline V4 v4_set(float a) { V4 r = {{ a, a, a, a }}; return r; }
+static inline V4 v4_add(V4 a, V4 b) { V4 r; for (int z = 0; z < 4; z++) r.v[z] = a.v[z] + b.v[z]; return r; }
+static inline V4 v4_mul(V4 a, V4 b) { V4 r; for (int z = 0; z < 4; z++) r.v[z] = a.v[z] * b.v[z]; return r; }
+static inline V4 v4_rshift(V4 a)    { V4 r = {{ 0.0f, a.v[0], a.v[1], a.v[2] }}; return r; }   /* esl_sse_rightshift_ps(a, 0) */
+static inline V4 v4_lshift(V4 a)    { V4 r = {{ a.v[1], a.v[2], a.v[3], 0.0f }}; return r; }   /* move_ss + shuffle(0,3,2,1) */
+static inline float v4_hsum(V4 a)   { return (a.v[0] + a.v[1]) + (a.v[2] + a.v[3]); }
+static inline int v4_anygt(V4 a, V4 b) { for (int z = 0; z < 4; z++) if (a.v[z] > b.v[z]) return 1; return 0; }
+
+typedef struct { int Q; V4 *rf; V4 *tf; } STRIPED;
+
+static int
+stripe(int M, int Kp, const float *rfv, const float *tfv, STRIPED *s)
+{
+  int Q = (M - 1) / 4 + 1, x, q, z, t;
+  if (Q < 2) Q = 2;
+  s->Q = Q;
+  s->rf = calloc((size_t) Kp * Q, sizeof(V4));
+  s->tf = calloc((size_t) 8 * Q, sizeof(V4));
+  if (! s->rf || ! s->tf) return 5;
+  for (x = 0; x < Kp; x++)
+    for (q = 0; q < Q; q++)
+      for (z = 0; z < 4; z++) { int k = q + 1 + z*Q; s->rf[(size_t) x*Q + q].v[z] = (k <= M) ? rfv[(size_t) x*M + k-1] : 0.0f; }
+  for (q = 0; q < Q; q++)
+    for (t = 0; t < 7; t++)
+      for (z = 0; z < 4; z++) { int k = q + 1 + z*Q; s->tf[(size_t) q*7 + t].v[z] = (k <= M) ? tfv[(size_t) t*M + k-1] : 0.0f; }
+  for (q = 0; q < Q; q++)
+    for (z = 0; z < 4; z++) { int k = q + 1 + z*Q; s->tf[(size_t) 7*Q + q].v[z] = (k <= M) ? tfv[(size_t) T_DD*M + k-1] : 0.0f; }
+  return 0;
+}
+static void unstripe(STRIPED *s) { free(s->rf); free(s->tf); }
+
+int
+orc_fwd_parser(const uint8_t *dsq, int L, int M, int Kp, const float *rfv, const float *tfv, const float *xf,
+               float *xmx, float *ret_totscale, float *ret_sc)
+{
+  STRIPED s;
+  V4 *MMO, *DMO, *IMO, mpv, dpv, ipv, sv, dcv, xEv, xBv, zerov = v4_set(0.0f);
+  float xN, xE, xB, xC, xJ, totscale = 0.0f;
+  int   i, q, j, Q, status = 0;
+  unsigned csr = _mm_getcsr();
+  _mm_setcsr(csr | 0x8040);                      /* FTZ + DAZ, as impl_Init() sets */
+  if (stripe(M, Kp, rfv, tfv, &s)) return 5;
+  Q = s.Q;
+  MMO = calloc((size_t) Q, sizeof(V4)); DMO = calloc((size_t) Q, sizeof(V4)); IMO = calloc((size_t) Q, sizeof(V4));
+  xE = 0.; xN = 1.; xJ = 0.; xB = xf[XO(XN,MOVE)]; xC = 0.;
+  if (xmx) { xmx[0] = xE; xmx[1] = xN; xmx[2] = xJ; xmx[3] = xB; xmx[4] = xC; xmx[5] = 1.0f; }
+
+  for (i = 1; i <= L; i++) {
+    const V4 *rp = s.rf + (size_t) dsq[i] * Q;
+    const V4 *tp = s.tf;
+    dcv = zerov; xEv = zerov; xBv = v4_set(xB);
+    mpv = v4_rshift(MMO[Q-1]); dpv = v4_rshift(DMO[Q-1]); ipv = v4_rshift(IMO[Q-1]);
+    for (q = 0; q < Q; q++) {
+      sv  =            v4_mul(xBv, *tp); tp++;
+      sv  = v4_add(sv, v4_mul(mpv, *tp)); tp++;
+      sv  = v4_add(sv, v4_mul(ipv, *tp)); tp++;
+      sv  = v4_add(sv, v4_mul(dpv, *tp)); tp++;
+      sv  = v4_mul(sv, rp[q]);
+      xEv = v4_add(xEv, sv);
+      mpv = MMO[q]; dpv = DMO[q]; ipv = IMO[q];
+      MMO[q] = sv; DMO[q] = dcv;
+      dcv = v4_mul(sv, *tp); tp++;
+      sv  =               v4_mul(mpv, *tp);  tp++;
+      IMO[q] = v4_add(sv, v4_mul(ipv, *tp)); tp++;
+    }
+    dcv = v4_rshift(dcv);
+    DMO[0] = zerov;
+    tp = s.tf + 7*Q;
+    for (q = 0; q < Q; q++) { DMO[q] = v4_add(dcv, DMO[q]); dcv = v4_mul(DMO[q], *tp); tp++; }
+    if (M < 100) {
+      for (j = 1; j < 4; j++) {
+        dcv = v4_rshift(dcv);
+        tp = s.tf + 7*Q;
+        for (q = 0; q < Q; q++) { DMO[q] = v4_add(dcv, DMO[q]); dcv = v4_mul(dcv, *tp); tp++; }
+      }
+    } else {
+      for (j = 1; j < 4; j++) {
+        int changed = 0;
+        dcv = v4_rshift(dcv);
+        tp = s.tf + 7*Q;
+        for (q = 0; q < Q; q++) {
+          sv = v4_add(dcv, DMO[q]);
+          changed |= v4_anygt(sv, DMO[q]);
+          DMO[q] = sv;
+          dcv = v4_mul(dcv, *tp); tp++;
+        }
+        if (! changed) break;
+      }
+    }
+    for (q = 0; q < Q; q++) xEv = v4_add(DMO[q], xEv);
+    { V4 a = xEv, b; b.v[0] = a.v[1]; b.v[1] = a.v[2]; b.v[2] = a.v[3]; b.v[3] = a.v[0];
+      a = v4_add(a, b); b.v[0] = a.v[2]; b.v[1] = a.v[3]; b.v[2] = a.v[0]; b.v[3] = a.v[1];
+      a = v4_add(a, b); xE = a.v[0]; }
+    xN =  xN * xf[XO(XN,LOOP)];
+    xC = (xC * xf[XO(XC,LOOP)]) + (xE * xf[XO(XE,MOVE)]);
+    xJ = (xJ * xf[XO(XJ,LOOP)]) + (xE * xf[XO(XE,LOOP)]);
+    xB = (xJ * xf[XO(XJ,MOVE)]) + (xN * xf[XO(XN,MOVE)]);
+    float scale = 1.0f;
+    if (xE > 1.0e4) {
+      xN = xN / xE; xC = xC / xE; xJ = xJ / xE; xB = xB / xE;
+      xEv = v4_set((float) (1.0 / xE));
+      for (q = 0; q < Q; q++) { MMO[q] = v4_mul(MMO[q], xEv); DMO[q] = v4_mul(DMO[q], xEv); IMO[q] = v4_mul(IMO[q], xEv); }
+      scale = xE;
+      totscale += log(xE);
+      xE = 1.0;
+    }
+    if (xmx) { float *r = xmx + (size_t) i*6; r[0] = xE; r[1] = xN; r[2] = xJ; r[3] = xB; r[4] = xC; r[5] = scale; }
+  }
+  free(MMO); free(DMO); free(IMO); unstripe(&s);
+  if (ret_totscale) *ret_totscale = totscale;
+  if      (isnan(xC))          status = 16;
+  else if (L > 0 && xC == 0.0) status = 16;
+  else if (isinf(xC) == 1)     status = 16;
+  if (ret_sc) *ret_sc = status ? NAN : totscale + log(xC * xf[XO(XC,MOVE)]);
+  _mm_setcsr(csr);
+  return status;
+}
+
+int
+orc_bck_parser(const uint8_t *dsq, int L, int M, int Kp, const float *rfv, const float *tfv, const float *xf,
+               const float *fwd_xmx, float *xmx, int *ret_own, float *ret_sc)
+{
+  STRIPED s;
+  V4 *MMO, *DMO, *IMO, mpv, ipv, dpv, mcv, dcv, tmmv, timv, tdmv, xBv, xEv, zerov = v4_set(0.0f);
+  float xN, xE, xB, xC, xJ, totscale;
+  int   i, q, j, Q, own = 0, status = 0;
+  const V4 *tp, *rp;
+  unsigned csr = _mm_getcsr();
+  _mm_setcsr(csr | 0x8040);
+  if (L < 1) return 11;
+  if (stripe(M, Kp, rfv, tfv, &s)) return 5;
+  Q = s.Q;
+  MMO = calloc((size_t) Q, sizeof(V4)); DMO = calloc((size_t) Q, sizeof(V4)); IMO = calloc((size_t) Q, sizeof(V4));
+
+  xJ = 0.0; xB = 0.0; xN = 0.0;
+  xC = xf[XO(XC,MOVE)];
+  xE = xC * xf[XO(XE,MOVE)];
+  xEv = v4_set(xE);
+  dcv = zerov;
+  for (q = 0; q < Q; q++) MMO[q] = DMO[q] = xEv;
+  for (q = 0; q < Q; q++) IMO[q] = zerov;
+  tp  = s.tf + 8*Q - 1;
+  dpv = v4_lshift(DMO[Q-1]);
+  for (q = Q-1; q >= 0; q--) { dcv = v4_mul(dpv, *tp); tp--; DMO[q] = v4_add(DMO[q], dcv); dpv = DMO[q]; }
+  for (j = 1; j < 4; j++) {
+    tp  = s.tf + 8*Q - 1;
+    dcv = v4_lshift(dcv);
+    for (q = Q-1; q >= 0; q--) { dcv = v4_mul(dcv, *tp); tp--; DMO[q] = v4_add(DMO[q], dcv); }
+  }
+  tp  = s.tf + 7*Q - 3;
+  dcv = v4_lshift(DMO[0]);
+  for (q = Q-1; q >= 0; q--) { MMO[q] = v4_add(MMO[q], v4_mul(dcv, *tp)); tp -= 7; dcv = DMO[q]; }
+  {
+    const float sL = fwd_xmx[(size_t) L*6 + 5];
+    if (sL > 1.0) {
+      xE = xE / sL; xN = xN / sL; xC = xC / sL; xJ = xJ / sL; xB = xB / sL;
+      xEv = v4_set((float) (1.0 / sL));
+      for (q = 0; q < Q; q++) { MMO[q] = v4_mul(MMO[q], xEv); DMO[q] = v4_mul(DMO[q], xEv); IMO[q] = v4_mul(IMO[q], xEv); }
+    }
+    totscale = log(sL);
+    if (xmx) { float *r = xmx + (size_t) L*6; r[0] = xE; r[1] = xN; r[2] = xJ; r[3] = xB; r[4] = xC; r[5] = sL; }
+  }
+
+  for (i = L-1; i >= 1; i--) {
+    rp = s.rf + (size_t) dsq[i+1] * Q + Q - 1;
+    tp = s.tf + 7*Q - 1;
+    tmmv = v4_lshift(s.tf[1]); timv = v4_lshift(s.tf[2]); tdmv = v4_lshift(s.tf[3]);
+    mpv = v4_mul(MMO[0], s.rf[(size_t) dsq[i+1] * Q]);
+    mpv = v4_lshift(mpv);
+    xBv = zerov;
+    for (q = Q-1; q >= 0; q--) {
+      ipv = IMO[q];
+      IMO[q] = v4_add(v4_mul(ipv, *tp), v4_mul(mpv, timv)); tp--;
+      DMO[q] =                          v4_mul(mpv, tdmv);
+      mcv    = v4_add(v4_mul(ipv, *tp), v4_mul(mpv, tmmv)); tp -= 2;
+      mpv    = v4_mul(MMO[q], *rp); rp--;
+      MMO[q] = mcv;
+      tdmv = *tp; tp--;
+      timv = *tp; tp--;
+      tmmv = *tp; tp--;
+      xBv = v4_add(xBv, v4_mul(mpv, *tp)); tp--;
+    }
+    { V4 a = xBv, b; b.v[0] = a.v[1]; b.v[1] = a.v[2]; b.v[2] = a.v[3]; b.v[3] = a.v[0];
+      a = v4_add(a, b); b.v[0] = a.v[2]; b.v[1] = a.v[3]; b.v[2] = a.v[0]; b.v[3] = a.v[1];
+      a = v4_add(a, b); xB = a.v[0]; }
+    xC =  xC * xf[XO(XC,LOOP)];
+    xJ = (xB * xf[XO(XJ,MOVE)]) + (xJ * xf[XO(XJ,LOOP)]);
+    xN = (xB * xf[XO(XN,MOVE)]) + (xN * xf[XO(XN,LOOP)]);
+    xE = (xC * xf[XO(XE,MOVE)]) + (xJ * xf[XO(XE,LOOP)]);
+    xEv = v4_set(xE);
+
+    tp  = s.tf + 8*Q - 1;
+    dpv = v4_add(DMO[0], xEv);
+    dpv = v4_lshift(dpv);
+    for (q = Q-1; q >= 0; q--) {
+      dcv    = v4_mul(dpv, *tp); tp--;
+      DMO[q] = v4_add(DMO[q], v4_add(dcv, xEv));
+      dpv    = DMO[q];
+      MMO[q] = v4_add(MMO[q], xEv);
+    }
+    for (j = 1; j < 4; j++) {
+      dcv = v4_lshift(dcv);
+      tp  = s.tf + 8*Q - 1;
+      for (q = Q-1; q >= 0; q--) { dcv = v4_mul(dcv, *tp); tp--; DMO[q] = v4_add(DMO[q], dcv); }
+    }
+    dcv = v4_lshift(DMO[0]);
+    tp  = s.tf + 7*Q - 3;
+    for (q = Q-1; q >= 0; q--) { MMO[q] = v4_add(MMO[q], v4_mul(dcv, *tp)); tp -= 7; dcv = DMO[q]; }
+
+    if (xB > 1.0e16) own = 1;
+    float sc = own ? ((xB > 1.0e4) ? xB : 1.0) : fwd_xmx[(size_t) i*6 + 5];
+    if (sc > 1.0) {
+      xE /= sc; xN /= sc; xJ /= sc; xB /= sc; xC /= sc;
+      xBv = v4_set((float) (1.0 / sc));
+      for (q = 0; q < Q; q++) { MMO[q] = v4_mul(MMO[q], xBv); DMO[q] = v4_mul(DMO[q], xBv); IMO[q] = v4_mul(IMO[q], xBv); }
+      totscale += log(sc);
+    }
+    if (xmx) { float *r = xmx + (size_t) i*6; r[0] = xE; r[1] = xN; r[2] = xJ; r[3] = xB; r[4] = xC; r[5] = sc; }
+  }
+
+  tp  = s.tf;
+  rp  = s.rf + (size_t) dsq[1] * Q;
+  xBv = zerov;
+  for (q = 0; q < Q; q++) {
+    mpv = v4_mul(MMO[q], *rp); rp++;
+    mpv = v4_mul(mpv, *tp);    tp += 7;
+    xBv = v4_add(xBv, mpv);
+  }
+  { V4 a = xBv, b; b.v[0] = a.v[1]; b.v[1] = a.v[2]; b.v[2] = a.v[3]; b.v[3] = a.v[0];
+    a = v4_add(a, b); b.v[0] = a.v[2]; b.v[1] = a.v[3]; b.v[2] = a.v[0]; b.v[3] = a.v[1];
+    a = v4_add(a, b); xB = a.v[0]; }
+  xN = (xB * xf[XO(XN,MOVE)]) + (xN * xf[XO(XN,LOOP)]);
+  if (xmx) { xmx[0] = 0.0; xmx[1] = xN; xmx[2] = 0.0; xmx[3] = xB; xmx[4] = 0.0; xmx[5] = 1.0; }
+  free(MMO); free(DMO); free(IMO); unstripe(&s);
+  if      (isnan(xN))          status = 16;
+  else if (L > 0 && xN == 0.0) status = 16;
+  else if (isinf(xN) == 1)     status = 16;
+  if (ret_own) *ret_own = own;
+  if (ret_sc) *ret_sc = status ? NAN : totscale + log(xN);
+  _mm_setcsr(csr);
+  return status;
+}
+
+/*****************************************************************
+ * Null model and bias-composition filter
+ *****************************************************************/
+float
+orc_null_one(int L)
+{
+  float p1 = (float) L / (float) (L+1);
+  return (float) L * log(p1) + log(1.-p1);
+}
+
+/* eo[x*2 + state]; t00/t01/t10/t11 the 2x2 transitions; the E column is 1.0 */
+float
+orc_filter_score(const uint8_t *dsq, int L, const float *eo, float t00, float t01, float t10, float t11,
+                 float pi0, float pi1, float p1)
+{
+  float logsc = 0.0f, d0, d1, n0, n1, mx, s;
+  int   i;
+  if (L == 0) return NAN;
+  d0 = eo[dsq[1]*2 + 0] * pi0;
+  d1 = eo[dsq[1]*2 + 1] * pi1;
+  mx = 0.0f; mx = (d0 > mx) ? d0 : mx; mx = (d1 > mx) ? d1 : mx;
+  d0 /= mx; d1 /= mx;
+  logsc += (float) log(mx);
+  for (i = 2; i <= L; i++) {
+    const int x = dsq[i];
+    n0 = 0.0f; n0 += d0 * t00; n0 += d1 * t10; n0 *= eo[x*2 + 0];
+    mx = 0.0f; mx = (n0 > mx) ? n0 : mx;
+    n1 = 0.0f; n1 += d0 * t01; n1 += d1 * t11; n1 *= eo[x*2 + 1];
+    mx = (n1 > mx) ? n1 : mx;
+    d0 = n0 / mx; d1 = n1 / mx;
+    logsc += (float) log(mx);
+  }
+  s = 0.0f; s += d0 * 1.0f; s += d1 * 1.0f;
+  logsc += (float) log(s);
+  return logsc + (float) L * logf(p1) + logf(1.-p1);
+}
+
+/*****************************************************************
+ * Generic float Viterbi over a P7_PROFILE (generic_viterbi.c:66-158)
+ *****************************************************************/
+#define GTSC(k,s)  (tsc[(size_t)(k)*8 + (s)])
+#define GMSC(k)    (rs[(size_t)(k)*2 + 0])
+#define GISC(k)    (rs[(size_t)(k)*2 + 1])
+static inline float fmaxf_(float a, float b) { return a > b ? a : b; }
+
+int
+orc_generic_viterbi(const uint8_t *dsq, int L, int M, const float *tsc, const float *rsc, const float *xsc,
+                    int local, float *ret_sc)
+{
+  float *mm = malloc(sizeof(float) * (size_t) (M+1)), *im = malloc(sizeof(float) * (size_t) (M+1)), *dm = malloc(sizeof(float) * (size_t) (M+1));
+  float *mc = malloc(sizeof(float) * (size_t) (M+1)), *ic = malloc(sizeof(float) * (size_t) (M+1)), *dc = malloc(sizeof(float) * (size_t) (M+1));
+  float xN = 0, xB = xsc[XN*2 + GM], xE, xC = -INFINITY, xJ = -INFINITY, sc;
+  float esc = local ? 0 : -INFINITY;
+  int i, k;
+  for (k = 0; k <= M; k++) mm[k] = im[k] = dm[k] = -INFINITY;
+  for (i = 1; i <= L; i++) {
+    const float *rs = rsc + (size_t) dsq[i] * (M+1) * 2;
+    mc[0] = ic[0] = dc[0] = -INFINITY;
+    xE = -INFINITY;
+    for (k = 1; k < M; k++) {
+      sc = fmaxf_(mm[k-1] + GTSC(k-1, P_MM), im[k-1] + GTSC(k-1, P_IM));
+      sc = fmaxf_(sc, dm[k-1] + GTSC(k-1, P_DM));
+      sc = fmaxf_(sc, xB + GTSC(k-1, P_BM));
+      mc[k] = sc + GMSC(k);
+      xE = fmaxf_(xE, mc[k] + esc);
+      sc = fmaxf_(mm[k] + GTSC(k, P_MI), im[k] + GTSC(k, P_II));
+      ic[k] = sc + GISC(k);
+      dc[k] = fmaxf_(mc[k-1] + GTSC(k-1, P_MD), dc[k-1] + GTSC(k-1, P_DD));
+    }
+    sc = fmaxf_(mm[M-1] + GTSC(M-1, P_MM), im[M-1] + GTSC(M-1, P_IM));
+    sc = fmaxf_(sc, dm[M-1] + GTSC(M-1, P_DM));
+    sc = fmaxf_(sc, xB + GTSC(M-1, P_BM));
+    mc[M] = sc + GMSC(M);
+    dc[M] = fmaxf_(mc[M-1] + GTSC(M-1, P_MD), dc[M-1] + GTSC(M-1, P_DD));
+    sc = fmaxf_(xE, mc[M]);
+    xE = fmaxf_(sc, dc[M]);
+    ic[M] = -INFINITY;
+    xJ = fmaxf_(xJ + xsc[XJ*2 + GL], xE + xsc[XE*2 + GL]);
+    xC = fmaxf_(xC + xsc[XC*2 + GL], xE + xsc[XE*2 + GM]);
+    xN = xN + xsc[XN*2 + GL];
+    xB = fmaxf_(xN + xsc[XN*2 + GM], xJ + xsc[XJ*2 + GM]);
+    { float *t; t = mm; mm = mc; mc = t; t = im; im = ic; ic = t; t = dm; dm = dc; dc = t; }
+  }
+  if (M == 1) { /* the k-loop above is empty for M = 1; the unrolled node handles it */ }
+  *ret_sc = xC + xsc[XC*2 + GM];
+  free(mm); free(im); free(dm); free(mc); free(ic); free(dc);
+  return 0;
+}
+
+/*****************************************************************
+ * FP64 log-space Forward/Backward (textbook; exact log-sum-exp)
+ *****************************************************************/
+static inline double lse(double a, double b)
+{
+  if (a == -INFINITY) return b;
+  if (b == -INFINITY) return a;
+  return (a > b) ? a + log1p(exp(b - a)) : b + log1p(exp(a - b));
+}
+
+int
+orc_forward_log64(const uint8_t *dsq, int L, int M, const float *tsc, const float *rsc, const float *xsc,
+                  double *ret_sc)
+{
+  double *mm = malloc(sizeof(double) * (size_t) (M+1)), *im = malloc(sizeof(double) * (size_t) (M+1)), *dm = malloc(sizeof(double) * (size_t) (M+1));
+  double *mc = malloc(sizeof(double) * (size_t) (M+1)), *ic = malloc(sizeof(double) * (size_t) (M+1)), *dc = malloc(sizeof(double) * (size_t) (M+1));
+  double xN = 0, xB = xsc[XN*2 + GM], xE, xC = -INFINITY, xJ = -INFINITY, sc;
+  int i, k;
+  for (k = 0; k <= M; k++) mm[k] = im[k] = dm[k] = -INFINITY;
+  for (i = 1; i <= L; i++) {
+    const float *rs = rsc + (size_t) dsq[i] * (M+1) * 2;
+    mc[0] = ic[0] = dc[0] = -INFINITY;
+    xE = -INFINITY;
+    for (k = 1; k <= M; k++) {
+      sc = lse(lse(mm[k-1] + GTSC(k-1, P_MM), im[k-1] + GTSC(k-1, P_IM)),
+               lse(xB + GTSC(k-1, P_BM),      dm[k-1] + GTSC(k-1, P_DM)));
+      mc[k] = sc + GMSC(k);
+      if (k < M) ic[k] = lse(mm[k] + GTSC(k, P_MI), im[k] + GTSC(k, P_II)) + GISC(k);
+      else       ic[k] = -INFINITY;
+      dc[k] = lse(mc[k-1] + (k > 1 ? GTSC(k-1, P_MD) : -INFINITY), dc[k-1] + (k > 1 ? GTSC(k-1, P_DD) : -INFINITY));
+      xE = lse(xE, lse(mc[k], dc[k]));
+    }
+    xJ = lse(xJ + xsc[XJ*2 + GL], xE + xsc[XE*2 + GL]);
+    xC = lse(xC + xsc[XC*2 + GL], xE + xsc[XE*2 + GM]);
+    xN = xN + xsc[XN*2 + GL];
+    xB = lse(xN + xsc[XN*2 + GM], xJ + xsc[XJ*2 + GM]);
+    { double *t; t = mm; mm = mc; mc = t; t = im; im = ic; ic = t; t = dm; dm = dc; dc = t; }
+  }
+  *ret_sc = xC + xsc[XC*2 + GM];
+  free(mm); free(im); free(dm); free(mc); free(ic); free(dc);
+  return 0;
+}
+
+int
+orc_backward_log64(const uint8_t *dsq, int L, int M, const float *tsc, const float *rsc, const float *xsc,
+                   double *ret_sc)
+{
+  /* B(i,s): log-probability of emitting x_{i+1..L} from state s after row i */
+  double *mn = malloc(sizeof(double) * (size_t) (M+2)), *in = malloc(sizeof(double) * (size_t) (M+2)), *dn = malloc(sizeof(double) * (size_t) (M+2));
+  double *mc = malloc(sizeof(double) * (size_t) (M+2)), *ic = malloc(sizeof(double) * (size_t) (M+2)), *dc = malloc(sizeof(double) * (size_t) (M+2));
+  double xN, xJ, xC, xB, xE;
+  int i, k;
+  /* row L: only C->T and E->C->T, M_k->E, D_k->E remain */
+  xC = xsc[XC*2 + GM];
+  xJ = -INFINITY; xN = -INFINITY; xB = -INFINITY;
+  xE = xC + xsc[XE*2 + GM];
+  mn[M+1] = dn[M+1] = in[M+1] = -INFINITY;
+  for (k = M; k >= 1; k--) {
+    dn[k] = lse(xE, (k < M) ? dn[k+1] + GTSC(k, P_DD) : -INFINITY);
+    mn[k] = lse(xE, (k < M) ? dn[k+1] + GTSC(k, P_MD) : -INFINITY);
+    in[k] = -INFINITY;
+  }
+  for (i = L-1; i >= 0; i--) {
+    const float *rs = rsc + (size_t) dsq[i+1] * (M+1) * 2;
+    /* B state at row i: enter M_k at row i+1 */
+    xB = -INFINITY;
+    for (k = 1; k <= M; k++) xB = lse(xB, GTSC(k-1, P_BM) + GMSC(k) + mn[k]);
+    xC = xC + xsc[XC*2 + GL];
+    xJ = lse(xJ + xsc[XJ*2 + GL], xB + xsc[XJ*2 + GM]);
+    xN = lse(xN + xsc[XN*2 + GL], xB + xsc[XN*2 + GM]);
+    if (i == 0) break;
+    xE = lse(xC + xsc[XE*2 + GM], xJ + xsc[XE*2 + GL]);
+    mc[M+1] = dc[M+1] = ic[M+1] = -INFINITY;
+    for (k = M; k >= 1; k--) {
+      double nm = (k < M) ? GMSC(k+1) + mn[k+1] : -INFINITY;       /* M_{k+1} at row i+1 */
+      ic[k] = (k < M) ? lse(GTSC(k, P_IM) + nm, GTSC(k, P_II) + GISC(k) + in[k]) : -INFINITY;
+      dc[k] = lse(xE, (k < M) ? lse(GTSC(k, P_DM) + nm, GTSC(k, P_DD) + dc[k+1]) : -INFINITY);
+      mc[k] = lse(xE, (k < M) ? lse(lse(GTSC(k, P_MM) + nm, GTSC(k, P_MI) + GISC(k) + in[k]), GTSC(k, P_MD) + dc[k+1]) : -INFINITY);
+    }
+    { double *t; t = mn; mn = mc; mc = t; t = in; in = ic; ic = t; t = dn; dn = dc; dc = t; }
+  }
+  *ret_sc = xN;
+  free(mn); free(in); free(dn); free(mc); free(ic); free(dc);
+  return 0;
+}
+
+/*****************************************************************
+ * Brute force: enumerate every state path of the profile for tiny M, L.
+ * States: S N B M1..MM I1..IM-1 D1..DM E J C T; emissions on M, I and on the
+ * N/C/J loops (score 0 relative to the null). Returns the log of the summed
+ * path probabilities and the best path score.
+ *****************************************************************/
+typedef struct { const uint8_t *dsq; int L, M; const float *tsc, *rsc, *xsc; double sum; double best; } BRUTE;
+
+static void
+brute_walk(BRUTE *b, int st, int k, int i, double sc)
+{
+  /* st: 0 N, 1 B, 2 M, 3 I, 4 D, 5 E, 6 J, 7 C ; i = residues emitted so far */
+  const float *tsc = b->tsc, *xsc = b->xsc;
+  const int M = b->M, L = b->L;
+  if (sc == -INFINITY) return;
+  switch (st) {
+  case 0: /* N */
+    if (i < L) brute_walk(b, 0, 0, i+1, sc + xsc[XN*2 + GL]);
+    brute_walk(b, 1, 0, i, sc + xsc[XN*2 + GM]);
+    break;
+  case 6: /* J */
+    if (i < L) brute_walk(b, 6, 0, i+1, sc + xsc[XJ*2 + GL]);
+    brute_walk(b, 1, 0, i, sc + xsc[XJ*2 + GM]);
+    break;
+  case 7: /* C */
+    if (i < L) brute_walk(b, 7, 0, i+1, sc + xsc[XC*2 + GL]);
+    if (i == L) {
+      double t = sc + xsc[XC*2 + GM];
+      b->sum = lse(b->sum, t);
+      if (t > b->best) b->best = t;
+    }
+    break;
+  case 1: /* B -> M_k (local entry) */
+    if (i < L) for (int kk = 1; kk <= M; kk++) {
+      const float *rs = b->rsc + (size_t) b->dsq[i+1] * (M+1) * 2;
+      brute_walk(b, 2, kk, i+1, sc + GTSC(kk-1, P_BM) + rs[kk*2]);
+    }
+    break;
+  case 2: /* M_k */
+    brute_walk(b, 5, 0, i, sc);                                   /* M_k -> E (local exit, 0) */
+    if (k < M) {
+      if (i < L) {
+        const float *rs = b->rsc + (size_t) b->dsq[i+1] * (M+1) * 2;
+        brute_walk(b, 2, k+1, i+1, sc + GTSC(k, P_MM) + rs[(k+1)*2]);
+        brute_walk(b, 3, k,   i+1, sc + GTSC(k, P_MI) + rs[k*2+1]);
+      }
+      brute_walk(b, 4, k+1, i, sc + GTSC(k, P_MD));
+    }
+    break;
+  case 3: /* I_k */
+    if (i < L) {
+      const float *rs = b->rsc + (size_t) b->dsq[i+1] * (M+1) * 2;
+      brute_walk(b, 2, k+1, i+1, sc + GTSC(k, P_IM) + rs[(k+1)*2]);
+      brute_walk(b, 3, k,   i+1, sc + GTSC(k, P_II) + rs[k*2+1]);
+    }
+    break;
+  case 4: /* D_k */
+    brute_walk(b, 5, 0, i, sc);                                   /* D_k -> E (local exit, 0) */
+    if (k < M) {
+      if (i < L) {
+        const float *rs = b->rsc + (size_t) b->dsq[i+1] * (M+1) * 2;
+        brute_walk(b, 2, k+1, i+1, sc + GTSC(k, P_DM) + rs[(k+1)*2]);
+      }
+      brute_walk(b, 4, k+1, i, sc + GTSC(k, P_DD));
+    }
+    break;
+  case 5: /* E */
+    brute_walk(b, 7, 0, i, sc + xsc[XE*2 + GM]);
+    brute_walk(b, 6, 0, i, sc + xsc[XE*2 + GL]);
+    break;
+  }
+}
+
+int
+orc_brute_force(const uint8_t *dsq, int L, int M, const float *tsc, const float *rsc, const float *xsc,
+                double *ret_fwd, double *ret_vit)
+{
+  BRUTE b;
+  b.dsq = dsq; b.L = L; b.M = M; b.tsc = tsc; b.rsc = rsc; b.xsc = xsc;
+  b.sum = -INFINITY; b.best = -INFINITY;
+  brute_walk(&b, 0, 0, 0, 0.0);
+  *ret_fwd = b.sum;
+  *ret_vit = b.best;
+  return 0;
+}
+
+/*****************************************************************
+ * Statistics and the stage gates of p7_Pipeline
+ *****************************************************************/
+double
+orc_gumbel_surv(double x, double mu, double lambda)
+{
+  double y  = lambda*(x-mu);
+  double ey = -exp(-y);
+  if (fabs(ey) < 5e-9) return -ey;
+  else                 return 1 - exp(ey);
+}
+
+double
+orc_exp_surv(double x, double mu, double lambda)
+{
+  if (x < mu) return 1.0;
+  return exp(-lambda * (x-mu));
+}
+
+/* Everything the gates need for one profile. Per-length scalars are passed
+ * as arrays indexed by L (the product's ReconfigLength values). */
+typedef struct {
+  int M, Kp;
+  const uint8_t *rbv; int tbm, tec, base, bias; float scale_b;
+  const int16_t *rwv, *twv; int16_t xw[8]; float scale_w; int base_w, ddbound_w;
+  const float *rfv, *tfv; float xf[8];
+  const uint8_t *tjb_by_L;      /* [Lmax+1] */
+  const int16_t *xwmove_by_L;   /* [Lmax+1] */
+  const float   *xfmove_by_L, *xfloop_by_L;
+  const float   *eo; float t10, t11, pi0, pi1;
+  double evparam[6];
+  double F1, F2, F3;
+  int    do_bias;
+} ORC_PROFILE;
+
+typedef struct {
+  float usc, nullsc, filtersc, vfsc, fwdsc, bcksc;
+  int   stage, msv_status, vit_status, fwd_status;
+} ORC_HIT;
+
+static void
+orc_pipeline_one(const ORC_PROFILE *p, const uint8_t *dsq, int L, ORC_HIT *h)
+{
+  double P;
+  float  seq_score, filtersc;
+  int    xj;
+  h->stage = 0; h->filtersc = h->vfsc = h->fwdsc = h->bcksc = NAN;
+  h->msv_status = h->vit_status = h->fwd_status = 0;
+  h->nullsc = orc_null_one(L);
+  if (L == 0) { h->usc = NAN; return; }
+  h->msv_status = orc_msv_filter(dsq, L, p->M, p->rbv, p->tbm, p->tec, p->tjb_by_L[L], p->base, p->bias, p->scale_b, &h->usc, &xj);
+  seq_score = (h->usc - h->nullsc) / LOG2_D;
+  P = orc_gumbel_surv(seq_score, p->evparam[0], p->evparam[1]);
+  if (P > p->F1) return;
+  h->stage = 1;
+  if (p->do_bias) {
+    float pl = (float) L / (float) (L+1);
+    filtersc = orc_filter_score(dsq, L, p->eo, pl, 1.0f - pl, p->t10, p->t11, p->pi0, p->pi1, pl);
+    h->filtersc = filtersc;
+    seq_score = (h->usc - filtersc) / LOG2_D;
+    P = orc_gumbel_surv(seq_score, p->evparam[0], p->evparam[1]);
+    if (P > p->F1) return;
+  } else { filtersc = h->nullsc; h->filtersc = filtersc; }
+  h->stage = 2;
+  if (P > p->F2) {
+    int16_t xw[8];
+    memcpy(xw, p->xw, sizeof(xw));
+    xw[XO(XN,MOVE)] = xw[XO(XC,MOVE)] = xw[XO(XJ,MOVE)] = p->xwmove_by_L[L];
+    h->vit_status = orc_vit_filter(dsq, L, p->M, p->rwv, p->twv, xw, p->scale_w, p->base_w, p->ddbound_w, &h->vfsc, NULL);
+    seq_score = (h->vfsc - filtersc) / LOG2_D;
+    P = orc_gumbel_surv(seq_score, p->evparam[2], p->evparam[3]);
+    if (P > p->F2) return;
+  }
+  h->stage = 3;
+  {
+    float xf[8], *xmx = malloc(sizeof(float) * (size_t) (L+1) * 6), ts;
+    memcpy(xf, p->xf, sizeof(xf));
+    xf[XO(XN,MOVE)] = xf[XO(XC,MOVE)] = xf[XO(XJ,MOVE)] = p->xfmove_by_L[L];
+    xf[XO(XN,LOOP)] = xf[XO(XC,LOOP)] = xf[XO(XJ,LOOP)] = p->xfloop_by_L[L];
+    h->fwd_status = orc_fwd_parser(dsq, L, p->M, p->Kp, p->rfv, p->tfv, xf, xmx, &ts, &h->fwdsc);
+    seq_score = (h->fwdsc - filtersc) / LOG2_D;
+    P = orc_exp_surv(seq_score, p->evparam[4], p->evparam[5]);
+    if (h->fwd_status == 0 && P <= p->F3) {
+      h->stage = 4;
+      orc_bck_parser(dsq, L, p->M, p->Kp, p->rfv, p->tfv, xf, xmx, NULL, NULL, &h->bcksc);
+    }
+    free(xmx);
+  }
+}
+
+typedef struct {
+  const ORC_PROFILE *p; const uint8_t *res; const int64_t *off; const int32_t *len;
+  int64_t n; ORC_HIT *out; int64_t next; pthread_mutex_t mu; uint8_t *buf_unused;
+} ORC_JOB;
+
+static void *
+orc_worker(void *arg)
+{
+  ORC_JOB *j = (ORC_JOB *) arg;
+  uint8_t *dsq = NULL; size_t cap = 0;
+  for (;;) {
+    int64_t a, b;
+    pthread_mutex_lock(&j->mu);
+    a = j->next; j->next += 256;
+    pthread_mutex_unlock(&j->mu);
+    if (a >= j->n) break;
+    b = a + 256 < j->n ? a + 256 : j->n;
+    for (int64_t t = a; t < b; t++) {
+      const int L = j->len[t];
+      if ((size_t) L + 2 > cap) { free(dsq); cap = (size_t) L + 2; dsq = malloc(cap); }
+      dsq[0] = 255; memcpy(dsq + 1, j->res + j->off[t], (size_t) L); dsq[L+1] = 255;
+      orc_pipeline_one(j->p, dsq, L, &j->out[t]);
+    }
+  }
+  free(dsq);
+  return NULL;
+}
+
+int
+orc_pipeline_db(const ORC_PROFILE *p, const uint8_t *res, const int64_t *off, const int32_t *len, int64_t n,
+                int nthreads, ORC_HIT *out)
+{
+  ORC_JOB j;
+  pthread_t th[256];
+  int t;
+  if (nthreads < 1) nthreads = 1;
+  if (nthreads > 256) nthreads = 256;
+  j.p = p; j.res = res; j.off = off; j.len = len; j.n = n; j.out = out; j.next = 0;
+  pthread_mutex_init(&j.mu, NULL);
+  for (t = 0; t < nthreads; t++) pthread_create(&th[t], NULL, orc_worker, &j);
+  for (t = 0; t < nthreads; t++) pthread_join(th[t], NULL);
+  pthread_mutex_destroy(&j.mu);
+  return 0;
+}
+
+/* MSV only, over a database: the throughput-dominant stage, for the CPU baseline. */
+typedef struct { const ORC_PROFILE *p; const uint8_t *res; const int64_t *off; const int32_t *len; int64_t n; float *out; int64_t next; pthread_mutex_t mu; } MSV_JOB;
+
+static void *
+msv_worker(void *arg)
+{
+  MSV_JOB *j = (MSV_JOB *) arg;
+  uint8_t *dsq = NULL; size_t cap = 0;
+  for (;;) {
+    int64_t a, b;
+    pthread_mutex_lock(&j->mu);
+    a = j->next; j->next += 256;
+    pthread_mutex_unlock(&j->mu);
+    if (a >= j->n) break;
+    b = a + 256 < j->n ? a + 256 : j->n;
+    for (int64_t t = a; t < b; t++) {
+      const int L = j->len[t];
+      if ((size_t) L + 2 > cap) { free(dsq); cap = (size_t) L + 2; dsq = malloc(cap); }
+      dsq[0] = 255; memcpy(dsq + 1, j->res + j->off[t], (size_t) L); dsq[L+1] = 255;
+      orc_msv_filter(dsq, L, j->p->M, j->p->rbv, j->p->tbm, j->p->tec, j->p->tjb_by_L[L], j->p->base, j->p->bias,
+                     j->p->scale_b, &j->out[t], NULL);
+    }
+  }
+  free(dsq);
+  return NULL;
+}
+
+int
+orc_msv_db(const ORC_PROFILE *p, const uint8_t *res, const int64_t *off, const int32_t *len, int64_t n,
+           int nthreads, float *out)
+{
+  MSV_JOB j;
+  pthread_t th[256];
+  int t;
+  if (nthreads < 1) nthreads = 1;
+  if (nthreads > 256) nthreads = 256;
+  j.p = p; j.res = res; j.off = off; j.len = len; j.n = n; j.out = out; j.next = 0;
+  pthread_mutex_init(&j.mu, NULL);
+  for (t = 0; t < nthreads; t++) pthread_create(&th[t], NULL, msv_worker, &j);
+  for (t = 0; t < nthreads; t++) pthread_join(th[t], NULL);
+  pthread_mutex_destroy(&j.mu);
+  return 0;
+}
