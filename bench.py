@@ -1,0 +1,145 @@
+"""Benchmark: the hmmsearch filter pipeline (MSV -> bias -> Viterbi -> Forward
+-> Backward) of one M=400 query over a resident 2M-sequence synthetic database
+on one MI355X.
+
+A step is one p7_gpu_Pipeline call over the whole database, inputs already in
+HBM. value = nres * M / seconds per step, in GCUPS (the reference's Mc/sec
+convention, HMMER/src/p7_pipeline.c:1791-1792, divided by 1000). Prints one
+JSON line.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PROFILE = os.path.join(ROOT, "tests", "golden", "synth", "synth_M400.hmm")
+PEAK_PK16_TOPS = 78.6      # MI355X packed 16-bit VALU ops/s (256 CU x 64 lanes x 2 x 2.4 GHz)
+PEAK_HBM_GBS = 8000.0
+SSV_OPS_PER_CELL = 2       # the ungapped pass: one saturating add + one max per cell (ssvfilter.c:433-435)
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(om, db, M, seconds_target=15.0, max_threads=16):
+    """The oracle's pipeline on a bounded sample, on the host cores; GCUPS."""
+    import oracle
+    t = om.tables()
+    prof, keep = oracle.make_profile(t, oracle.bias_hmm(t), t["evparam"], Lmax=int(db.lengths.max()) + 1)
+    nthreads = max(1, min(max_threads, os.cpu_count() or 1))
+    n = 2000
+    while True:
+        sub = db.subset(np.arange(n))
+        t0 = time.perf_counter()
+        oracle.pipeline_db(prof, sub.residues, sub.offsets, sub.lengths, nthreads)
+        dt = time.perf_counter() - t0
+        if dt >= seconds_target / 4 or n >= db.nseq:
+            break
+        n = min(db.nseq, int(n * max(2.0, seconds_target / 4 / max(dt, 1e-3))))
+    cells = float(sub.nres) * M
+    return {"value": cells / dt / 1e9, "unit": "GCUPS", "cores": nthreads, "kind": "port",
+            "sample": f"first {n} of {db.nseq} database sequences ({sub.nres} residues), full filter pipeline "
+                      f"(oracle/p7_oracle.c restatement, scalar C), {nthreads} threads, {dt:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--nseq", type=int, default=2_000_000)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--traffic", default=None, help="JSON with HBM bytes per ungapped-pass launch (from rocprofv3 --pmc)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU: --gpus {args.gpus} is not supported by this build "
+              "(the database shards by sequence ranges across GPUs; see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    from biobench2_amd import p7gpu, synth
+    p7gpu.init(0)
+    om = p7gpu.OProfile.from_file(PROFILE)
+    M = om.tables()["M"]
+    hmm = synth.read_hmm_probs(PROFILE)
+    t0 = time.time()
+    db = synth.make_db(args.nseq, [hmm], frac_homolog=0.01, seed_db=43, seed_plant=44)
+    log(f"db: {db.nseq} sequences, {db.nres} residues, generated in {time.time() - t0:.1f} s")
+    sdb = p7gpu.SeqDB(db.residues, db.offsets, db.lengths)
+    pli = p7gpu.Pipeline()
+
+    for w in range(args.warmup):
+        pli.run(om, sdb, scores=False)
+        log(f"warmup {w + 1}/{args.warmup} done")
+    p7gpu.lib.p7_gpu_Synchronize()
+    ms_ssv, ms_stage = [], {k: [] for k in ("ms_msv", "ms_bias", "ms_vit", "ms_fwd", "ms_bck", "ms_total")}
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        out = pli.run(om, sdb, scores=False)
+        st = pli.stats()
+        ms_ssv.append(st["ms_ssv"])
+        for key in ms_stage:
+            ms_stage[key].append(st[key])
+    p7gpu.lib.p7_gpu_Synchronize()
+    t1 = time.perf_counter()
+    log(f"timed {args.steps} steps")
+
+    ms_per_step = (t1 - t0) * 1000.0 / args.steps
+    cells = float(db.nres) * M
+    value = cells / (ms_per_step / 1000.0) / 1e9
+    stage = out["stage"]
+    ssv_ms = float(np.mean(ms_ssv))
+    ssv_tops = cells * SSV_OPS_PER_CELL / (ssv_ms / 1000.0) / 1e12
+    per_stage = {}
+    for key, cname in (("ms_msv", "cells_msv"), ("ms_bias", "cells_bias"), ("ms_vit", "cells_vit"),
+                       ("ms_fwd", "cells_fwd"), ("ms_bck", "cells_bck")):
+        ms = float(np.mean(ms_stage[key]))
+        per_stage[key[3:]] = {"ms": round(ms, 4), "cells": st[cname],
+                              "gcups": round(st[cname] / (ms / 1000.0) / 1e9, 2) if ms > 0 else None}
+    traffic = None
+    if args.traffic and os.path.exists(args.traffic):
+        traffic = json.load(open(args.traffic)).get("bytes_per_launch")
+    line = {
+        "metric": "filter GCUPS (MSV/Vit/Fwd) at M=400 vs CPU; served-query p50/p99 latency",
+        "value": round(value, 2),
+        "unit": "GCUPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u8/i16/f32",
+        "data": "synthetic (seeded profile generator M=400; lognormal-length i.i.d. database, 1% planted homologs)",
+        "config": {"workload": "hmmsearch filter pipeline, one query M=400 vs 2M-sequence database",
+                   "nseq": db.nseq, "nres": db.nres, "M": M, "F1": 0.02, "F2": 1e-3, "F3": 1e-5},
+        "roofline": {"bound": "valu", "kernel": "k_ssv (ungapped MSV pass, packed int16)",
+                     "achieved": round(ssv_tops, 3), "peak": PEAK_PK16_TOPS, "unit": "TOP/s",
+                     "frac": round(ssv_tops / PEAK_PK16_TOPS, 4), "traffic": traffic,
+                     "ms_per_launch": round(ssv_ms, 4), "ops_per_cell": SSV_OPS_PER_CELL,
+                     "residue_stream_GBs": round(db.nres / (ssv_ms / 1000.0) / 1e9, 2)},
+        "stages": per_stage,
+        "pass_counts": {"msv": int((stage >= 1).sum()), "bias": int((stage >= 2).sum()),
+                        "vit": int((stage >= 3).sum()), "fwd": int((stage >= 4).sum()),
+                        "msv_full_rescored": st["n_msv_full"]},
+    }
+    if not args.no_cpu:
+        log("cpu baseline ...")
+        line["cpu_baseline"] = cpu_baseline(om, db, M, args.cpu_seconds)
+    else:
+        line["cpu_baseline"] = None
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

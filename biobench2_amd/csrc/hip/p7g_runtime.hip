@@ -614,6 +614,7 @@ p7_gpu_Pipeline(P7_GPU_PIPELINE *pli, const P7_GPU_OPROFILE *om, const P7_GPU_SE
   HIPCHK(hipMemsetAsync(w.r.bcksc, 0xff, sizeof(float) * (size_t) n, g_stream));
   HIPCHK(hipEventRecord(w.ev[0], g_stream));
   HIPCHK(launch_ssv(*p, s, nullptr, n, w.r.zmax, g_stream));
+  HIPCHK(hipEventRecord(w.ev[6], g_stream));
   HIPCHK(launch_msv_classify(*p, s, nullptr, n, w.r.zmax, w.r.xj, w.l.full, &w.l.counts[0], lc, g_stream));
   HIPCHK(launch_msv_full(*p, s, w.l.full, &w.l.counts[0], n, w.r.xj, lc, g_stream));
   HIPCHK(launch_msv_gate(*p, s, n, w.r.xj, w.r, w.l, g, g_stream));
@@ -649,6 +650,8 @@ p7_gpu_Pipeline(P7_GPU_PIPELINE *pli, const P7_GPU_OPROFILE *om, const P7_GPU_SE
   for (int a = 0; a < 5; a++) HIPCHK(hipEventElapsedTime(&ms[a], w.ev[a], w.ev[a+1]));
   HIPCHK(hipEventElapsedTime(&mtot, w.ev[0], w.ev[5]));
   st.ms_total = mtot;
+  HIPCHK(hipEventElapsedTime(&mtot, w.ev[0], w.ev[6]));
+  st.ms_ssv = mtot;
   st.ms_msv = ms[0]; st.ms_bias = ms[1]; st.ms_vit = ms[2]; st.ms_fwd = ms[3]; st.ms_bck = ms[4];
   int64_t pm = 0, pb = 0, pv = 0, pf = 0;
   double cb = 0, cv = 0, cf = 0, ck = 0;

@@ -55,17 +55,23 @@ static float
 threshold(int which, double mu, double lambda, double P)
 {
   /* pass(s) := surv(s) <= P, false at -inf-like scores, true at +inf */
-  int32_t lo = f2ord(-1e30f), hi = f2ord(1e30f);
+  int64_t lo = f2ord(-1e30f), hi = f2ord(1e30f);      /* 64-bit: the span exceeds INT32_MAX */
 #define PASS(s) (((which) == 0 ? p7g_gumbel_surv((double)(s), mu, lambda) : p7g_exp_surv((double)(s), mu, lambda)) <= P)
-  if (PASS(ord2f(lo))) return -INFINITY;
-  if (! PASS(ord2f(hi))) return INFINITY;
+  if (PASS(ord2f((int32_t) lo))) return -INFINITY;
+  if (! PASS(ord2f((int32_t) hi))) return INFINITY;
   while (hi - lo > 1) {
-    int32_t mid = lo + (hi - lo) / 2;
-    if (PASS(ord2f(mid))) hi = mid; else lo = mid;
+    int64_t mid = lo + (hi - lo) / 2;
+    if (PASS(ord2f((int32_t) mid))) hi = mid; else lo = mid;
   }
 #undef PASS
-  return ord2f(hi);
+  return ord2f((int32_t) hi);
 }
 
 float p7g_gumbel_threshold(double mu, double lambda, double P) { return threshold(0, mu, lambda, P); }
 float p7g_exp_threshold   (double mu, double lambda, double P) { return threshold(1, mu, lambda, P); }
+
+float
+p7_gpu_ScoreThreshold(int which, double mu, double lambda, double P)
+{
+  return which == 0 ? p7g_gumbel_threshold(mu, lambda, P) : p7g_exp_threshold(mu, lambda, P);
+}

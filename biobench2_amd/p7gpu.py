@@ -54,7 +54,7 @@ class Stats(ctypes.Structure):
     _fields_ = [(n, c_int64) for n in ("nmodels", "nseqs", "nres", "nnodes", "n_past_msv", "n_past_bias",
                                        "n_past_vit", "n_past_fwd", "n_msv_full")] + \
                [(n, c_double) for n in ("cells_msv", "cells_bias", "cells_vit", "cells_fwd", "cells_bck",
-                                        "ms_msv", "ms_bias", "ms_vit", "ms_fwd", "ms_bck", "ms_total")]
+                                        "ms_msv", "ms_bias", "ms_vit", "ms_fwd", "ms_bck", "ms_total", "ms_ssv")]
 
     def as_dict(self):
         return {n: getattr(self, n) for n, _ in self._fields_}
@@ -96,6 +96,7 @@ def _load():
         "p7_gpu_pipeline_Create": (vp, [d, d, d, i, i]), "p7_gpu_pipeline_Destroy": (None, [vp]),
         "p7_gpu_Pipeline": (i, [vp, vp, vp, POINTER(Hits)]),
         "p7_gpu_pipeline_GetStats": (i, [vp, POINTER(Stats)]),
+        "p7_gpu_ScoreThreshold": (f, [i, d, d, d]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -120,6 +121,11 @@ def _check(status, where, ok=(OK,)):
 
 def _ptr(a, ctype):
     return a.ctypes.data_as(POINTER(ctype))
+
+
+def score_threshold(which, mu, lam, P):
+    """Smallest float bit score whose Gumbel (which=0) or exponential (which=1) P-value is <= P."""
+    return lib.p7_gpu_ScoreThreshold(which, mu, lam, P)
 
 
 def init(device=0):

@@ -119,6 +119,7 @@ typedef struct p7g_stats_s {
   int64_t n_msv_full;          /* targets the MSV stage rescored with the J state   */
   double  cells_msv, cells_bias, cells_vit, cells_fwd, cells_bck;
   double  ms_msv, ms_bias, ms_vit, ms_fwd, ms_bck, ms_total;   /* device time, last call */
+  double  ms_ssv;              /* the ungapped MSV pass alone (k_ssv), last call    */
 } P7_GPU_STATS;
 
 /* ---- device ------------------------------------------------------------- */
@@ -221,6 +222,11 @@ void  p7_gpu_pipeline_Destroy(P7_GPU_PIPELINE *pli);
  * the counters like p7_pli_NewModel/NewSeq/p7_Pipeline. */
 int   p7_gpu_Pipeline(P7_GPU_PIPELINE *pli, const P7_GPU_OPROFILE *om, const P7_GPU_SEQDB *db, P7_GPU_HITS *hits);
 int   p7_gpu_pipeline_GetStats(const P7_GPU_PIPELINE *pli, P7_GPU_STATS *st);
+/* Score threshold of a stage gate: the smallest float bit score s whose
+ * P-value is <= P under the Gumbel (which = 0, easel/esl_gumbel.c:129) or
+ * exponential (which = 1, easel/esl_exponential.c:127) tail, so the gate
+ * "P > F: reject" (p7_pipeline.c:692, 702, 720, 729) is "s < threshold". */
+float p7_gpu_ScoreThreshold(int which, double mu, double lambda, double P);
 
 #ifdef __cplusplus
 }

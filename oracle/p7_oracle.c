@@ -554,7 +554,7 @@ orc_backward_log64(const uint8_t *dsq, int L, int M, const float *tsc, const flo
     const float *rs = rsc + (size_t) dsq[i+1] * (M+1) * 2;
     /* B state at row i: enter M_k at row i+1 */
     xB = -INFINITY;
-    for (k = 1; k <= M; k++) xB = lse(xB, GTSC(k-1, P_BM) + GMSC(k) + mn[k]);
+    for (k = 1; k <= M; k++) xB = lse(xB, (double) GTSC(k-1, P_BM) + (double) GMSC(k) + mn[k]);
     xC = xC + xsc[XC*2 + GL];
     xJ = lse(xJ + xsc[XJ*2 + GL], xB + xsc[XJ*2 + GM]);
     xN = lse(xN + xsc[XN*2 + GL], xB + xsc[XN*2 + GM]);
@@ -562,10 +562,10 @@ orc_backward_log64(const uint8_t *dsq, int L, int M, const float *tsc, const flo
     xE = lse(xC + xsc[XE*2 + GM], xJ + xsc[XE*2 + GL]);
     mc[M+1] = dc[M+1] = ic[M+1] = -INFINITY;
     for (k = M; k >= 1; k--) {
-      double nm = (k < M) ? GMSC(k+1) + mn[k+1] : -INFINITY;       /* M_{k+1} at row i+1 */
-      ic[k] = (k < M) ? lse(GTSC(k, P_IM) + nm, GTSC(k, P_II) + GISC(k) + in[k]) : -INFINITY;
+      double nm = (k < M) ? (double) GMSC(k+1) + mn[k+1] : -INFINITY;       /* M_{k+1} at row i+1 */
+      ic[k] = (k < M) ? lse(GTSC(k, P_IM) + nm, (double) GTSC(k, P_II) + (double) GISC(k) + in[k]) : -INFINITY;
       dc[k] = lse(xE, (k < M) ? lse(GTSC(k, P_DM) + nm, GTSC(k, P_DD) + dc[k+1]) : -INFINITY);
-      mc[k] = lse(xE, (k < M) ? lse(lse(GTSC(k, P_MM) + nm, GTSC(k, P_MI) + GISC(k) + in[k]), GTSC(k, P_MD) + dc[k+1]) : -INFINITY);
+      mc[k] = lse(xE, (k < M) ? lse(lse(GTSC(k, P_MM) + nm, (double) GTSC(k, P_MI) + (double) GISC(k) + in[k]), GTSC(k, P_MD) + dc[k+1]) : -INFINITY);
     }
     { double *t; t = mn; mn = mc; mc = t; t = in; in = ic; ic = t; t = dn; dn = dc; dc = t; }
   }

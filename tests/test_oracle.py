@@ -1,0 +1,187 @@
+"""Pins the CPU oracle before it is trusted as the checker.
+
+Each check is one the reference's own tests use, or one the mathematics fixes:
+  * MSV filter == generic Viterbi on the profile rounded like the MSV tables,
+    / scale_b - 3  (utest_msv_filter, HMMER/src/impl_sse/msvfilter.c:623-660);
+  * Viterbi filter == generic Viterbi on the profile rounded like the int16
+    tables, / scale_w - 3 (utest_viterbi_filter, vitfilter.c:653-695);
+  * generic Viterbi and the FP64 Forward == brute-force enumeration of every
+    state path on tiny models (the idea of HMMER/src/itest_brute.c);
+  * fp32 Forward parser == FP64 log-space Forward, Forward == Backward
+    (utest in fwdback.c:895-948: 1e-4);
+  * the tutorial's fn3 vs 7LESS_DROME run (tutorial/fn3.out) passes every filter;
+  * P-value tails and the bias filter against closed forms.
+"""
+import math
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from biobench2_amd import p7gpu, seqio, synth
+from conftest import golden, random_seqs, synth_profile
+
+LOG2 = 0.69314718055994529
+MODELS = [(golden("minifam"), 0), (golden("minifam"), 1), (golden("minifam"), 2), (golden("LuxC.hmm"), 0),
+          (golden("M1.hmm"), 0), (golden("20aa.hmm"), 0), (golden("RRM_1.hmm"), 0), (synth_profile("synth_M50"), 0),
+          (synth_profile("synth_M200"), 0)]
+f32 = np.float32
+
+
+def same_as_mf(om, gm):
+    """Generic profile rounded like the MSV tables (p7_profile_SameAsMF, p7_oprofile.c:1888-1915)."""
+    t = om.tables()
+    tsc, rsc, xsc = gm.scores()
+    M, sb = t["M"], f32(t["scale_b"])
+    tbm = f32(np.round(sb * f32(math.log(2.0 / (f32(M) * f32(M + 1))))))
+    tm = np.full_like(tsc, -np.inf)
+    tm[1:, 0] = 0.0
+    tm[:, 3] = tbm
+    rm = rsc.copy()
+    fin = np.isfinite(rm[:, :, 0])
+    rm[:, :, 0] = np.where(fin, np.round(sb * rm[:, :, 0]), -np.inf)
+    rm[:, :, 1] = 0.0
+    xm = np.where(np.isfinite(xsc), np.round(sb * xsc), -np.inf).astype(np.float32)
+    xm[1, 0] = xm[2, 0] = xm[3, 0] = 0.0
+    return tm, rm, xm, sb
+
+
+def same_as_vf(om, gm):
+    """Generic profile rounded like the int16 tables (p7_profile_SameAsVF, p7_oprofile.c:1942-1980)."""
+    t = om.tables()
+    tsc, rsc, xsc = gm.scores()
+    sw = f32(t["scale_w"])
+    tm = np.where(np.isfinite(tsc), np.round(sw * tsc), -np.inf).astype(np.float32)
+    tm[:, 7] = np.where(tm[:, 7] == 0.0, -1.0, tm[:, 7])
+    rm = rsc.copy()
+    fin = np.isfinite(rm[:, :, 0])
+    rm[:, :, 0] = np.where(fin, np.round(sw * rm[:, :, 0]), -np.inf)
+    rm[:, :, 1] = 0.0
+    xm = np.where(np.isfinite(xsc), np.round(sw * xsc), -np.inf).astype(np.float32)
+    xm[1, 0] = xm[2, 0] = xm[3, 0] = 0.0
+    return tm, rm, xm, sw
+
+
+def configured(path, which, L):
+    om = p7gpu.OProfile.from_file(path, which, L=L)
+    return om, om.gm
+
+
+@pytest.mark.parametrize("path,which", MODELS)
+def test_msv_oracle_equals_rounded_generic_viterbi(path, which):
+    rng = np.random.default_rng(42 + which)
+    for L in (1, 2, 17, 200):
+        om, gm = configured(path, which, L)
+        tm, rm, xm, sb = same_as_mf(om, gm)
+        t = om.tables()
+        for s in random_seqs(rng, [L] * 6):
+            st, sc, _ = oracle.msv_filter(s, t)
+            if st == p7gpu.ERANGE:
+                continue
+            ref = f32(oracle.generic_viterbi(s, tm, rm, xm) / sb - 3.0)
+            assert abs(sc - ref) <= 0.001, (L, sc, ref)
+
+
+@pytest.mark.parametrize("path,which", MODELS)
+def test_vit_oracle_equals_rounded_generic_viterbi(path, which):
+    rng = np.random.default_rng(7 + which)
+    for L in (1, 3, 64, 200):
+        om, gm = configured(path, which, L)
+        tm, rm, xm, sw = same_as_vf(om, gm)
+        t = om.tables()
+        for s in random_seqs(rng, [L] * 6):
+            st, sc, _ = oracle.vit_filter(s, t)
+            if st == p7gpu.ERANGE:
+                continue
+            ref = f32(oracle.generic_viterbi(s, tm, rm, xm) / sw - 3.0)
+            assert abs(sc - ref) <= 0.001, (L, sc, ref)
+
+
+def tiny_models(tmp_path):
+    paths = []
+    for M, seed in ((1, 3), (2, 4), (3, 5)):
+        h = synth.random_profile(M, np.random.default_rng(seed), f"tiny{M}")
+        p = os.path.join(tmp_path, f"tiny{M}.hmm")
+        synth.write_hmm(p, [h])
+        paths.append(p)
+    return paths + [golden("M1.hmm")]
+
+
+def test_generic_oracles_equal_brute_force(tmp_path):
+    rng = np.random.default_rng(11)
+    for path in tiny_models(str(tmp_path)):
+        for L in (1, 2, 3, 4):
+            om, gm = configured(path, 0, L)
+            tsc, rsc, xsc = gm.scores()
+            for s in random_seqs(rng, [L] * 3):
+                bf, bv = oracle.brute_force(s, tsc, rsc, xsc)
+                assert abs(oracle.forward_log64(s, tsc, rsc, xsc) - bf) < 1e-9
+                assert abs(oracle.backward_log64(s, tsc, rsc, xsc) - bf) < 1e-9
+                assert abs(oracle.generic_viterbi(s, tsc, rsc, xsc) - bv) < 1e-4
+
+
+@pytest.mark.parametrize("path,which", MODELS)
+def test_fp32_parsers_against_fp64(path, which):
+    rng = np.random.default_rng(5)
+    for L in (1, 10, 150, 600):
+        om, gm = configured(path, which, L)
+        t = om.tables()
+        tsc, rsc, xsc = gm.scores()
+        for s in random_seqs(rng, [L] * 2):
+            st, fsc, xmx, _ = oracle.fwd_parser(s, t)
+            st2, bsc, _, _ = oracle.bck_parser(s, t, xmx)
+            ref = oracle.forward_log64(s, tsc, rsc, xsc)
+            assert st == 0 and st2 == 0
+            assert abs(fsc - bsc) < 1e-4
+            assert abs(fsc - ref) < 1e-4
+
+
+def test_homologs_overflow_the_byte_and_word_filters():
+    om, gm = configured(golden("minifam"), 0, 150)
+    t = om.tables()
+    for name, s in seqio.read_fasta(golden("globins45.fa"))[:5]:
+        st, sc, xj = oracle.msv_filter(s, t)
+        assert st == p7gpu.ERANGE and sc == np.inf and xj == -1
+        st, sc, _ = oracle.vit_filter(s, t)
+        assert st == p7gpu.ERANGE and sc == np.inf
+
+
+def test_fn3_tutorial_run_passes_every_filter():
+    """tutorial/fn3.out: fn3 vs 7LESS_DROME passes MSV, bias, Vit and Fwd, full-sequence score 178.0
+    bits with a null2 bias of 0.4 bits, i.e. (fwd - null) / ln2 = 178.4 bits to the printed precision."""
+    (name, s), = seqio.read_uniprot(golden("7LESS_DROME"))
+    L = len(s)
+    assert L == 2554
+    om = p7gpu.OProfile.from_file(golden("fn3.hmm"))
+    om.reconfig_length(L)
+    t = om.tables()
+    nullsc = oracle.null_one(L)
+    st, usc, _ = oracle.msv_filter(s, t)
+    assert oracle.gumbel_surv(f32((usc - nullsc) / LOG2), t["evparam"][0], t["evparam"][1]) <= 0.02
+    st, fsc, _, _ = oracle.fwd_parser(s, t)
+    bits = (fsc - nullsc) / LOG2
+    assert abs(bits - 178.4) < 0.06, bits
+    assert oracle.exp_surv(f32(bits), t["evparam"][4], t["evparam"][5]) <= 1e-5
+
+
+def test_survival_functions_closed_forms():
+    assert abs(oracle.gumbel_surv(2.0, 2.0, 0.7) - (1 - math.exp(-1.0))) < 1e-15
+    x = 30.0
+    assert oracle.gumbel_surv(x, 0.0, 1.0) == pytest.approx(math.exp(-x), rel=1e-9)      # small-tail branch
+    assert oracle.exp_surv(1.0, 2.0, 0.7) == 1.0
+    assert oracle.exp_surv(4.0, 2.0, 0.7) == pytest.approx(math.exp(-1.4), rel=1e-15)
+
+
+def test_bias_filter_is_the_null_model_when_composition_is_background():
+    """With the model composition equal to the background, both filter states emit with odds 1 and the
+    two-state Forward sums to 1, so the filter score is the null score up to float rounding."""
+    t = {"abctype": 3, "M": 100, "compo": synth.AMINO_BG.astype(np.float32)}
+    bh = oracle.bias_hmm(t)
+    assert np.allclose(bh["eo"][:20], 1.0, atol=1e-6)
+    rng = np.random.default_rng(3)
+    for L in (1, 50, 1000):
+        s = random_seqs(rng, [L])[0]
+        p1 = f32(L) / f32(L + 1)
+        fs = oracle.filter_score(s, bh["eo"], p1, f32(1) - p1, bh["t10"], bh["t11"], bh["pi0"], bh["pi1"], p1)
+        assert abs(fs - oracle.null_one(L)) < 1e-3 * max(1.0, L / 100)
