@@ -10,7 +10,8 @@ Each check is one the reference's own tests use, or one the mathematics fixes:
   * fp32 Forward parser == FP64 log-space Forward, Forward == Backward
     (utest in fwdback.c:895-948: 1e-4);
   * the tutorial's fn3 vs 7LESS_DROME run (tutorial/fn3.out) passes every filter;
-  * P-value tails and the bias filter against closed forms.
+  * P-value tails against closed forms; the bias filter against brute-force
+    enumeration of its state paths and the background-composition closed form.
 """
 import math
 import os
@@ -185,3 +186,49 @@ def test_bias_filter_is_the_null_model_when_composition_is_background():
         p1 = f32(L) / f32(L + 1)
         fs = oracle.filter_score(s, bh["eo"], p1, f32(1) - p1, bh["t10"], bh["t11"], bh["pi0"], bh["pi1"], p1)
         assert abs(fs - oracle.null_one(L)) < 1e-3 * max(1.0, L / 100)
+
+
+def test_bias_filter_equals_brute_force_over_state_paths():
+    """esl_hmm_Forward (esl_hmm.c:351-409) is the sum over every state path of the two-state HMM,
+    pi[s1] eo[x1][s1] prod_i t[s_i-1][s_i] eo[x_i][s_i], times an end transition of 1 (p7_bg.c:438,444);
+    p7_bg_FilterScore adds L log p1 + log(1 - p1) (p7_bg.c:478-481). Enumerating all 2^L paths in
+    float64, with asymmetric transitions and odds, fixes the transition orientation and every term."""
+    rng = np.random.default_rng(11)
+    for trial in range(6):
+        eo = rng.uniform(0.2, 3.0, size=(29, 2)).astype(np.float32)
+        t00, t10 = f32(rng.uniform(0.6, 0.99)), f32(rng.uniform(0.01, 0.4))
+        t01, t11 = f32(1) - t00, f32(1) - t10
+        pi0 = f32(rng.uniform(0.5, 0.99))
+        pi1 = f32(1) - pi0
+        L = int(rng.integers(1, 11))
+        s = rng.integers(0, 20, size=L).astype(np.uint8)
+        p1 = f32(L) / f32(L + 1)
+        T = np.array([[t00, t01], [t10, t11]], np.float64)
+        tot = 0.0
+        for path in range(1 << L):
+            st = [(path >> i) & 1 for i in range(L)]
+            pr = float((pi0, pi1)[st[0]]) * float(eo[s[0], st[0]])
+            for i in range(1, L):
+                pr *= T[st[i - 1], st[i]] * float(eo[s[i], st[i]])
+            tot += pr
+        want = math.log(tot) + L * math.log(float(p1)) + math.log(float(f32(1) - p1))
+        got = oracle.filter_score(s, eo, t00, t01, t10, t11, pi0, pi1, p1)
+        assert abs(got - want) < 2e-5 * max(1.0, abs(want)), (trial, L, got, want)
+
+
+def test_bias_filter_hmm_parameters():
+    """p7_bg_SetFilter (p7_bg.c:431-453): state-1 mean length M/8, start 0.999/0.001, odds = e / f,
+    with gap/nonresidue/missing codes at odds 1 and a degenerate code sum(e)/sum(f) over its residues
+    (esl_hmm.c:116-153)."""
+    compo = synth.AMINO_BG.astype(np.float32)[::-1].copy()
+    compo /= compo.sum()
+    for M in (8, 100, 400):
+        bh = oracle.bias_hmm({"abctype": 3, "M": M, "compo": compo})
+        L1 = f32(M) / f32(8)
+        assert bh["t10"] == f32(1) / (L1 + f32(1)) and bh["t11"] == L1 / (L1 + f32(1))
+        assert bh["pi0"] == f32(0.999) and bh["pi1"] == f32(0.001)
+        np.testing.assert_allclose(bh["eo"][:20, 1], compo / synth.AMINO_BG, rtol=1e-6)
+        np.testing.assert_array_equal(bh["eo"][[20, 27, 28]], 1.0)
+        D, N = 2, 11                                          # B = D or N (code 21)
+        want = (compo[D] + compo[N]) / (synth.AMINO_BG[D] + synth.AMINO_BG[N])
+        assert abs(bh["eo"][21, 1] - want) < 1e-6 * want
