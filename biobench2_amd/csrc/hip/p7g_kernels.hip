@@ -35,6 +35,9 @@ typedef short s2v __attribute__((ext_vector_type(2)));
 
 #define NEG_PAIR 0x80008000u     // (-32768, -32768)
 #define LOG2_D   0.69314718055994529
+#ifndef SSV_BLOCK
+#define SSV_BLOCK 512
+#endif
 
 __device__ __forceinline__ int   s2i(s2v v) { return __builtin_bit_cast(int, v); }
 __device__ __forceinline__ s2v   i2s(int v) { return __builtin_bit_cast(s2v, v); }
@@ -59,17 +62,37 @@ __device__ __forceinline__ int shift_in(int last, int fill)
   return (int) __builtin_amdgcn_alignbit((unsigned) last, (unsigned) nb, 16);
 }
 
+// DPP lane move with an identity for lanes that have no source lane.
+template <int CTRL, int RM = 0xf>
+__device__ __forceinline__ int dppi(int old, int v) { return __builtin_amdgcn_update_dpp(old, v, CTRL, RM, 0xf, false); }
+template <int CTRL, int RM = 0xf>
+__device__ __forceinline__ float dppf(float old, float v)
+{
+  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(__builtin_bit_cast(int, old), __builtin_bit_cast(int, v), CTRL, RM, 0xf, false));
+}
+__device__ __forceinline__ float readlanef(float v, int l) { return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), l)); }
+enum { ROW_SHR1 = 0x111, ROW_SHR2 = 0x112, ROW_SHR4 = 0x114, ROW_SHR8 = 0x118,
+       ROW_SHL1 = 0x101, ROW_SHL2 = 0x102, ROW_SHL4 = 0x104, ROW_SHL8 = 0x108,
+       WAVE_SHL1 = 0x130, WAVE_SHR1 = 0x138 };
+
+// Wave reductions: 16-lane row scans with DPP, then the four row results
+// from lanes 15/31/47/63. The result is wave-uniform.
 __device__ __forceinline__ int wave_max_i(int v)
 {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
-  return v;
+  v = max(v, dppi<ROW_SHR1>(v, v));
+  v = max(v, dppi<ROW_SHR2>(v, v));
+  v = max(v, dppi<ROW_SHR4>(v, v));
+  v = max(v, dppi<ROW_SHR8>(v, v));
+  return max(max(__builtin_amdgcn_readlane(v, 15), __builtin_amdgcn_readlane(v, 31)),
+             max(__builtin_amdgcn_readlane(v, 47), __builtin_amdgcn_readlane(v, 63)));
 }
 __device__ __forceinline__ float wave_sum_f(float v)
 {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  return v;
+  v += dppf<ROW_SHR1>(0.0f, v);
+  v += dppf<ROW_SHR2>(0.0f, v);
+  v += dppf<ROW_SHR4>(0.0f, v);
+  v += dppf<ROW_SHR8>(0.0f, v);
+  return (readlanef(v, 15) + readlanef(v, 31)) + (readlanef(v, 47) + readlanef(v, 63));
 }
 __device__ __forceinline__ int   uni(int v)   { return __builtin_amdgcn_readfirstlane(v); }
 __device__ __forceinline__ float unif(float v){ return __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, v))); }
@@ -90,22 +113,29 @@ __device__ __forceinline__ int residue_at(const uint8_t *sq, int i) { return sq[
 /*****************************************************************
  * 1. MSV stage, ungapped pass (constant xB)
  *****************************************************************/
+// Per-lane stride (dwords) of the ungapped-pass score table: a multiple of 4
+// (16-byte LDS reads) whose quarter is odd, so consecutive lanes' 16-byte
+// reads fall on different bank groups.
+__host__ __device__ constexpr int ssv_stride(int nd) { return ((((nd + 3) / 4) & 1) ? ((nd + 3) / 4) * 4 : ((nd + 3) / 4) * 4 + 4); }
+
 template <int G, int ND>
-__global__ __launch_bounds__(256)
-void k_ssv(const int *__restrict__ tbl, int tbl_dwords, int ND4,
+__global__ __launch_bounds__(SSV_BLOCK)
+void k_ssv(const int *__restrict__ tbl, int tbl_dwords,
            const uint8_t *__restrict__ res, const int64_t *__restrict__ off, const int32_t *__restrict__ len,
            const int32_t *__restrict__ slots, int64_t n, uint16_t *__restrict__ zmax)
 {
-  extern __shared__ int lds[];
-  stage_table(lds, tbl, (size_t) tbl_dwords);
+  constexpr int ND4  = ssv_stride(ND);
+  constexpr int NQ   = (ND + 3) / 4;                  // 16-byte reads per row
+  constexpr int ROW4 = G * ND4 / 4;                   // one residue row, in int4 units
+  constexpr int TPW  = 64 / G;
+  constexpr int NACC = 4;
+  extern __shared__ int4 lds4[];
+  stage_table((int *) lds4, tbl, (size_t) tbl_dwords);
 
-  constexpr int TPW = 64 / G;
   const int lane = threadIdx.x & 63;
   const int grp  = lane / G;
   const int z    = lane % G;
-  const int zoff = z * ND4 * 4;
-  const int rowbytes = G * ND4 * 4;
-  const char *ldsb = (const char *) lds;
+  const int zq   = z * (ND4 / 4);
 
   for (int64_t w = wave_id(); w * TPW < n; w += wave_cnt()) {
     const int64_t t = w * TPW + grp;
@@ -120,38 +150,64 @@ void k_ssv(const int *__restrict__ tbl, int tbl_dwords, int ND4,
     int zr[ND];
 #pragma unroll
     for (int d = 0; d < ND; d++) zr[d] = (int) NEG_PAIR;
-    int acc = (int) NEG_PAIR;
+    int acc[NACC];
+#pragma unroll
+    for (int a = 0; a < NACC; a++) acc[a] = (int) NEG_PAIR;
+    int nb = (int) NEG_PAIR;          // DPP destination: lanes without a left neighbour keep this fill
 
-    for (int i0 = 0; i0 < Lw; i0 += 16) {
-      uint4 ch;
-      if (i0 < L) ch = *(const uint4 *) (sq + i0);
-      else        ch = make_uint4(0x1f1f1f1fu, 0x1f1f1f1fu, 0x1f1f1f1fu, 0x1f1f1f1fu);
-      auto step = [&](int x) {
-        const int *row = (const int *) (ldsb + x * rowbytes + zoff);
-        const int s0 = shift_in<G>(zr[ND-1], (int) NEG_PAIR);
+    const uint4 padchunk = make_uint4(0x1f1f1f1fu, 0x1f1f1f1fu, 0x1f1f1f1fu, 0x1f1f1f1fu);
+    const int4 *lrow = (const int4 *) __builtin_assume_aligned(lds4, 16) + zq;
+    // one DP row with the scores in `cur`, while the row of residue xn is fetched into `nxt`
+#define SSV_ROW(cur, nxt, xn) do {                                                            \
+      const int4 *r_ = lrow + (xn) * ROW4;                                                    \
+      _Pragma("unroll") for (int q = 0; q < NQ; q++) nxt[q] = r_[q];                          \
+      nb = __builtin_amdgcn_update_dpp(nb, zr[ND-1], (G == 16) ? 0x111 : 0x138, 0xf, 0xf, false); \
+      const int s0_ = (int) __builtin_amdgcn_alignbit((unsigned) zr[ND-1], (unsigned) nb, 16); \
+      _Pragma("unroll") for (int d = ND-1; d >= 1; d--) {                                     \
+        const int4 c_ = cur[d / 4];                                                           \
+        const int rv_ = (d % 4 == 0) ? c_.x : (d % 4 == 1) ? c_.y : (d % 4 == 2) ? c_.z : c_.w; \
+        zr[d] = adds2(zr[d-1], rv_);                                                          \
+        acc[d % NACC] = max2(acc[d % NACC], zr[d]);                                           \
+      }                                                                                       \
+      zr[0] = adds2(s0_, cur[0].x);                                                           \
+      acc[0] = max2(acc[0], zr[0]);                                                           \
+    } while (0)
+#define RES(w, u) ((int) (((w) >> (8 * ((u) & 3))) & 0xffu))
+
+    uint4 ch = (0 < L) ? *(const uint4 *) sq : padchunk;
+    int4 bufA[NQ], bufB[NQ];
+    {
+      const int4 *r0 = lrow + RES(ch.x, 0) * ROW4;
 #pragma unroll
-        for (int d = ND-1; d >= 1; d--) { zr[d] = adds2(zr[d-1], row[d]); acc = max2(acc, zr[d]); }
-        zr[0] = adds2(s0, row[0]);
-        acc = max2(acc, zr[0]);
-      };
-      if (i0 + 16 <= Lw) {
-#pragma unroll
-        for (int u = 0; u < 16; u++) {
-          const unsigned wd = (u < 4) ? ch.x : (u < 8) ? ch.y : (u < 12) ? ch.z : ch.w;
-          step((int) ((wd >> (8 * (u & 3))) & 0xffu));
-        }
-      } else {
-        const int nrow = Lw - i0;
-        for (int u = 0; u < nrow; u++) {
-          const unsigned wd = (u < 4) ? ch.x : (u < 8) ? ch.y : (u < 12) ? ch.z : ch.w;
-          step((int) ((wd >> (8 * (u & 3))) & 0xffu));
-        }
-      }
+      for (int q = 0; q < NQ; q++) bufA[q] = r0[q];
     }
-    int a = max(lo16(acc), hi16(acc));
+    int i0 = 0;
+    for (; i0 + 16 <= Lw; i0 += 16) {
+      const uint4 nch = (i0 + 16 < L) ? *(const uint4 *) (sq + i0 + 16) : padchunk;
+      SSV_ROW(bufA, bufB, RES(ch.x, 1));  SSV_ROW(bufB, bufA, RES(ch.x, 2));
+      SSV_ROW(bufA, bufB, RES(ch.x, 3));  SSV_ROW(bufB, bufA, RES(ch.y, 0));
+      SSV_ROW(bufA, bufB, RES(ch.y, 1));  SSV_ROW(bufB, bufA, RES(ch.y, 2));
+      SSV_ROW(bufA, bufB, RES(ch.y, 3));  SSV_ROW(bufB, bufA, RES(ch.z, 0));
+      SSV_ROW(bufA, bufB, RES(ch.z, 1));  SSV_ROW(bufB, bufA, RES(ch.z, 2));
+      SSV_ROW(bufA, bufB, RES(ch.z, 3));  SSV_ROW(bufB, bufA, RES(ch.w, 0));
+      SSV_ROW(bufA, bufB, RES(ch.w, 1));  SSV_ROW(bufB, bufA, RES(ch.w, 2));
+      SSV_ROW(bufA, bufB, RES(ch.w, 3));  SSV_ROW(bufB, bufA, RES(nch.x, 0));
+      ch = nch;
+    }
+    for (int u = 0; i0 + u < Lw; u++) {              // the last rows, fewer than 16
+      const unsigned wd = (u + 1 < 4) ? ch.x : (u + 1 < 8) ? ch.y : (u + 1 < 12) ? ch.z : ch.w;
+      SSV_ROW(bufA, bufB, RES(wd, u + 1));
 #pragma unroll
-    for (int o = G/2; o > 0; o >>= 1) a = max(a, __shfl_xor(a, o, 64));
-    if (valid && z == 0) zmax[slot] = (uint16_t) (a + 32768);
+      for (int q = 0; q < NQ; q++) bufA[q] = bufB[q];
+    }
+#undef SSV_ROW
+#undef RES
+#pragma unroll
+    for (int a = 1; a < NACC; a++) acc[0] = max2(acc[0], acc[a]);
+    int m = max(lo16(acc[0]), hi16(acc[0]));
+#pragma unroll
+    for (int o = G/2; o > 0; o >>= 1) m = max(m, __shfl_xor(m, o, 64));
+    if (valid && z == 0) zmax[slot] = (uint16_t) (m + 32768);
   }
 }
 
@@ -337,13 +393,19 @@ __device__ __forceinline__ void dd_closure(int (&D)[ND], const int *__restrict__
 #pragma unroll
   for (int j = 0; j < NB; j++) { cur = max(v[j], cur + a[j]); A += a[j]; }
   int C = cur;
-#pragma unroll
-  for (int o = 1; o < 64; o <<= 1) {
-    int A2 = __shfl_up(A, o, 64), C2 = __shfl_up(C, o, 64);
-    if (z >= o) { C = max(C, C2 + A); A = A + A2; A = max(A, NEG); C = max(C, NEG); }
+  // lane maps f(x) = max(x + A, C), composed across lanes in increasing k
+#define MP_STEP(CTRL) { const int Ap = dppi<CTRL>(0, A), Cp = dppi<CTRL>(NEG, C); C = max(C, Cp + A); A = A + Ap; }
+  MP_STEP(ROW_SHR1) MP_STEP(ROW_SHR2) MP_STEP(ROW_SHR4) MP_STEP(ROW_SHR8)
+#undef MP_STEP
+  {
+    const int c1 = __builtin_amdgcn_readlane(C, 15);
+    const int c2 = max(c1 + __builtin_amdgcn_readlane(A, 31), __builtin_amdgcn_readlane(C, 31));
+    const int c3 = max(c2 + __builtin_amdgcn_readlane(A, 47), __builtin_amdgcn_readlane(C, 47));
+    const int row = z >> 4;
+    const int rc = (row == 0) ? NEG : (row == 1) ? c1 : (row == 2) ? c2 : c3;
+    C = max(C, rc + A);
   }
-  int cin = __shfl_up(C, 1, 64);
-  if (z == 0) cin = NEG;
+  int cin = dppi<WAVE_SHR1>(NEG, C);
   cur = cin;
 #pragma unroll
   for (int j = 0; j < NB; j++) { cur = max(v[j], cur + a[j]); v[j] = cur; }
@@ -384,13 +446,17 @@ void k_vit(const DevProfile p, const uint8_t *__restrict__ res, const int64_t *_
 #pragma unroll
     for (int d = 0; d < ND; d++) { M[d] = I[d] = D[d] = (int) NEG_PAIR; }
     int status = 0;
+    // DPP destinations: lane 0 never has a source and keeps the -infinity fill
+    int nbM = (int) NEG_PAIR, nbI = (int) NEG_PAIR, nbD = (int) NEG_PAIR;
+    int ms0 = (int) NEG_PAIR;                     // M(i-1, k-1) for the lane's first position
 
     for (int i = 0; i < L; i++) {
       const int x = uni(residue_at(sq, i));
       const int *e = lds + (size_t) x * rowd + z * ND;
-      const int ms0 = shift_in<64>(M[ND-1], (int) NEG_PAIR);
-      const int is0 = shift_in<64>(I[ND-1], (int) NEG_PAIR);
-      const int ds0 = shift_in<64>(D[ND-1], (int) NEG_PAIR);
+      nbI = dppi<WAVE_SHR1>(nbI, I[ND-1]);
+      nbD = dppi<WAVE_SHR1>(nbD, D[ND-1]);
+      const int is0 = (int) __builtin_amdgcn_alignbit((unsigned) I[ND-1], (unsigned) nbI, 16);
+      const int ds0 = (int) __builtin_amdgcn_alignbit((unsigned) D[ND-1], (unsigned) nbD, 16);
       const int xB2 = pair16(xB);
       int xE = (int) NEG_PAIR;
 #pragma unroll
@@ -405,11 +471,12 @@ void k_vit(const DevProfile p, const uint8_t *__restrict__ res, const int64_t *_
         I[d] = max2(adds2(M[d], tMI[d]), adds2(I[d], tII[d]));
         M[d] = sv;
       }
-      // D(k) <- M(k-1) + tMD(k-1) on the new row
-      const int sm0 = shift_in<64>(M[ND-1], (int) NEG_PAIR);
+      // D(k) <- M(k-1) + tMD(k-1) on the new row; the shifted row is also the next row's M(i-1, k-1)
+      nbM = dppi<WAVE_SHR1>(nbM, M[ND-1]);
+      ms0 = (int) __builtin_amdgcn_alignbit((unsigned) M[ND-1], (unsigned) nbM, 16);
       int Dm = (int) NEG_PAIR;
 #pragma unroll
-      for (int d = 0; d < ND; d++) { D[d] = adds2(d ? M[d-1] : sm0, tMDS[d]); Dm = max2(Dm, D[d]); }
+      for (int d = 0; d < ND; d++) { D[d] = adds2(d ? M[d-1] : ms0, tMDS[d]); Dm = max2(Dm, D[d]); }
       const int xEs = uni(wave_max_i(max(lo16(xE), hi16(xE))));
       if (xEs >= 32767) { status = 16; break; }
       xN = (int16_t) (xN + p.xwN_loop);
@@ -444,17 +511,22 @@ void k_vit(const DevProfile p, const uint8_t *__restrict__ res, const int64_t *_
 template <int B>
 __device__ __forceinline__ void affine_fwd_scan(float (&D)[B], const float (&a)[B], int z)
 {
-  // D(j) = D0(j) + a(j) * D(j-1), carried across lanes in increasing k.
+  // D(j) = D0(j) + a(j) * D(j-1), carried across lanes in increasing k:
+  // lane maps f(x) = P x + C, scanned over 16-lane rows with DPP, row carries by readlane.
   float P = 1.0f, C = 0.0f;
 #pragma unroll
   for (int j = 0; j < B; j++) { C = D[j] + a[j] * C; P *= a[j]; }
-#pragma unroll
-  for (int o = 1; o < 64; o <<= 1) {
-    float P2 = __shfl_up(P, o, 64), C2 = __shfl_up(C, o, 64);
-    if (z >= o) { C = C + P * C2; P = P * P2; }
+#define AF_STEP(CTRL) { const float Pp = dppf<CTRL>(1.0f, P), Cp = dppf<CTRL>(0.0f, C); C = C + P * Cp; P = P * Pp; }
+  AF_STEP(ROW_SHR1) AF_STEP(ROW_SHR2) AF_STEP(ROW_SHR4) AF_STEP(ROW_SHR8)
+  {
+    const float c1 = readlanef(C, 15);
+    const float c2 = readlanef(P, 31) * c1 + readlanef(C, 31);
+    const float c3 = readlanef(P, 47) * c2 + readlanef(C, 47);
+    const int row = z >> 4;
+    const float rc = (row == 0) ? 0.0f : (row == 1) ? c1 : (row == 2) ? c2 : c3;
+    C = C + P * rc;
   }
-  float cin = __shfl_up(C, 1, 64);
-  if (z == 0) cin = 0.0f;
+  const float cin = dppf<WAVE_SHR1>(0.0f, C);
   float cur = cin;
 #pragma unroll
   for (int j = 0; j < B; j++) { cur = D[j] + a[j] * cur; D[j] = cur; }
@@ -467,13 +539,17 @@ __device__ __forceinline__ void affine_bck_scan(float (&D)[B], const float (&a)[
   float P = 1.0f, C = 0.0f;
 #pragma unroll
   for (int j = B-1; j >= 0; j--) { C = D[j] + a[j] * C; P *= a[j]; }
-#pragma unroll
-  for (int o = 1; o < 64; o <<= 1) {
-    float P2 = __shfl_down(P, o, 64), C2 = __shfl_down(C, o, 64);
-    if (z + o < 64) { C = C + P * C2; P = P * P2; }
+  AF_STEP(ROW_SHL1) AF_STEP(ROW_SHL2) AF_STEP(ROW_SHL4) AF_STEP(ROW_SHL8)
+#undef AF_STEP
+  {
+    const float c2 = readlanef(C, 48);
+    const float c1 = readlanef(P, 32) * c2 + readlanef(C, 32);
+    const float c0 = readlanef(P, 16) * c1 + readlanef(C, 16);
+    const int row = z >> 4;
+    const float rc = (row == 3) ? 0.0f : (row == 2) ? c2 : (row == 1) ? c1 : c0;
+    C = C + P * rc;
   }
-  float cin = __shfl_down(C, 1, 64);
-  if (z == 63) cin = 0.0f;
+  const float cin = dppf<WAVE_SHL1>(0.0f, C);
   float cur = cin;
 #pragma unroll
   for (int j = B-1; j >= 0; j--) { cur = D[j] + a[j] * cur; D[j] = cur; }
@@ -524,10 +600,11 @@ void k_fwd(const DevProfile p, const uint8_t *__restrict__ res, const int64_t *_
     }
     if (scl && z == 0) scl[0] = 1.0f;
 
+    float mL = 0.0f;                            // M(i-1, k-1) of the lane's first position
     for (int i = 1; i <= L; i++) {
       const int x = uni(residue_at(sq, i-1));
       const float *e = tbl + (size_t) x * rowf + z * B;
-      const float mL = wave_shr1f(M[B-1]), iL = wave_shr1f(I[B-1]), dL = wave_shr1f(D[B-1]);
+      const float iL = dppf<WAVE_SHR1>(0.0f, I[B-1]), dL = dppf<WAVE_SHR1>(0.0f, D[B-1]);
       float xEl = 0.0f;
 #pragma unroll
       for (int j = B-1; j >= 0; j--) {
@@ -538,7 +615,7 @@ void k_fwd(const DevProfile p, const uint8_t *__restrict__ res, const int64_t *_
         M[j] = sv; I[j] = In;
         xEl += sv;
       }
-      const float mS = wave_shr1f(M[B-1]);
+      const float mS = dppf<WAVE_SHR1>(0.0f, M[B-1]);
 #pragma unroll
       for (int j = 0; j < B; j++) D[j] = (j ? M[j-1] : mS) * tMDS[j];
       affine_fwd_scan<B>(D, tDDS, z);
@@ -559,6 +636,7 @@ void k_fwd(const DevProfile p, const uint8_t *__restrict__ res, const int64_t *_
         totscale = (float) ((double) totscale + log((double) xE));
         xE = 1.0f;
       }
+      mL = dppf<WAVE_SHR1>(0.0f, M[B-1]);
       if (scl && z == 0) scl[i] = scale;
       if (xmx_out && z == 0) {
         float *xr = xmx_out + (size_t) i * 6;
@@ -737,6 +815,21 @@ static hipError_t launch(void *k, unsigned grid, unsigned block, void **args, si
   return hipLaunchKernel(k, dim3(grid), dim3(block), args, lds, st);
 }
 
+// Persistent-style grid of blocks of `wpb` waves: enough to fill every CU as
+// far as LDS allows, never more than the work needs.
+static unsigned grid_for_waves_blk(int64_t nwaves, size_t lds_bytes, int wpb)
+{
+  dev_props();
+  int per_cu = 32 / wpb;                                   // 32 waves per CU at most
+  if (lds_bytes > 0) { int byl = (int) ((size_t) g_lds_cu / lds_bytes); if (byl < per_cu) per_cu = byl; }
+  if (per_cu < 1) per_cu = 1;
+  int64_t g = (int64_t) ncu() * per_cu;
+  int64_t need = (nwaves + wpb - 1) / wpb;
+  if (need < g) g = need;
+  if (g < 1) g = 1;
+  return (unsigned) g;
+}
+
 static unsigned grid_for_waves(int64_t nwaves, size_t lds_bytes)
 {
   // persistent-style grid: enough 4-wave workgroups to fill the chip, bounded by the work
@@ -776,12 +869,12 @@ hipError_t launch_ssv(const DevProfile &p, const DevSeqs &s, const int32_t *slot
   }
   const size_t lds = p.ssv_dwords * 4;
   const int64_t nwaves = (n + (64 / p.ssv_G) - 1) / (64 / p.ssv_G);
-  const unsigned grid = grid_for_waves(nwaves, lds);
+  const unsigned grid = grid_for_waves_blk(nwaves, lds, SSV_BLOCK / 64);
   int tbl_dwords = (int) p.ssv_dwords;
-  int ND4 = p.ssv_ND4;
-  void *args[] = { (void *) &p.ssv_tbl, &tbl_dwords, &ND4, (void *) &s.res, (void *) &s.off, (void *) &s.len,
+  if (p.ssv_ND4 != ssv_stride(p.ssv_ND)) return hipErrorInvalidValue;
+  void *args[] = { (void *) &p.ssv_tbl, &tbl_dwords, (void *) &s.res, (void *) &s.off, (void *) &s.len,
                    (void *) &slots, &n, &zmax };
-  return launch(k, grid, 256, args, lds, st);
+  return launch(k, grid, SSV_BLOCK, args, lds, st);
 }
 
 hipError_t launch_msv_classify(const DevProfile &p, const DevSeqs &s, const int32_t *slots, int64_t n,
