@@ -51,6 +51,18 @@ def cpu_baseline(om, db, M, seconds_target=15.0, max_threads=16):
                       f"(oracle/p7_oracle.c restatement, scalar C), {nthreads} threads, {dt:.1f} s"}
 
 
+def calibrate_copy(mib=256, reps=3):
+    """Plain device-to-device copies of a known size: what FETCH_SIZE and WRITE_SIZE
+    report for them gives the factor from counter KiB to HBM bytes."""
+    import torch
+    a = torch.ones(mib * 1024 * 1024 // 4, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(reps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": mib * 1024 * 1024, "copy_bytes_written": mib * 1024 * 1024, "reps": reps}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -59,11 +71,18 @@ def main():
     ap.add_argument("--nseq", type=int, default=2_000_000)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--traffic", default=None, help="JSON with HBM bytes per ungapped-pass launch (from rocprofv3 --pmc)")
+    ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "k_ssv_traffic.json"),
+                    help="JSON with the HBM bytes of one k_ssv launch, from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run three plain 256 MiB device copies (to scale the FETCH_SIZE/WRITE_SIZE counters)")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        calibrate_copy()
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: --gpus {args.gpus} is not supported by this build "
-              "(the database shards by sequence ranges across GPUs; see DESIGN.md)", file=sys.stderr)
+              "(the database would shard by sequence ranges across GPUs; only the one-GPU path is built)",
+              file=sys.stderr)
         sys.exit(2)
 
     from biobench2_amd import p7gpu, synth

@@ -7,6 +7,8 @@
 // padding of each sequence in the HBM layout: every score against it is
 // "impossible", so extra rows never change a result.
 #define P7G_PADCODE 31
+// Residue rows of every device score table: codes 0..Kp-1 and the pad code.
+#define P7G_NROWS 32
 
 // Number of transition rows in the device Viterbi and Forward/Backward tables.
 #define P7G_VF_NT 8    // BM MM IM DM (into k), MDs (M_{k-1}->D_k), MI II (out of k), DDs (D_{k-1}->D_k)
@@ -16,18 +18,22 @@ enum { VT_BM = 0, VT_MM, VT_IM, VT_DM, VT_MDS, VT_MI, VT_II, VT_DDS,
 
 struct DevProfile {
   int M, Kp;
-  // MSV stage, ungapped (SSV) pass: int16 pairs of (bias - rbv), 16 or 64 lanes per target
-  int   ssv_G, ssv_ND, ssv_ND4;
-  int  *ssv_tbl;     size_t ssv_dwords;     // [Kp+1 rows incl. pad][G][ND4]
+  // Every table is lane-minor: the values one read instruction fetches for
+  // consecutive lanes are consecutive in memory, so LDS reads are free of
+  // bank conflicts for any per-lane block size.
+  // MSV stage, ungapped (SSV) pass: int16 pairs of (bias - rbv), 16 or 64 lanes
+  // per target; per residue row, ND/4 quads [q][G] of int4, then ND%4 dwords [t][G].
+  int   ssv_G, ssv_ND;
+  int  *ssv_tbl;     size_t ssv_dwords;     // [rows][G * ND]
   // MSV stage, full recurrence with the J state: same scores, 64 lanes
-  int   mf_ND, mf_ND4;
-  int  *mf_tbl;      size_t mf_dwords;      // [rows][64][ND4]
+  int   mf_ND;
+  int  *mf_tbl;      size_t mf_dwords;      // [rows][ND][64]
   // Viterbi filter: int16 pairs, 64 lanes, ND dwords per lane
   int   vf_ND;
-  int  *vf_tbl;      size_t vf_dwords;      // [(rows) + 8][64][ND]
+  int  *vf_tbl;      size_t vf_dwords;      // [(rows) + 8][ND][64]
   // Forward/Backward: float, 64 lanes, B positions per lane
   int   fb_B;
-  float *fb_tbl;     size_t fb_floats;      // [(rows) + 13][64][B]
+  float *fb_tbl;     size_t fb_floats;      // [(rows) + 13][B][64]
   int   nrows;       // Kp rounded up to include P7G_PADCODE (32)
   // per-length constants, L = 0..Lmax
   int      Lmax;
@@ -54,6 +60,7 @@ struct DevSeqs {
   const uint8_t *res;
   const int64_t *off;    // [slot]
   const int32_t *len;    // [slot]
+  const int32_t *perm;   // [slot] -> the caller's index
   int64_t nslot;
 };
 
@@ -112,6 +119,14 @@ hipError_t launch_vit(const DevProfile &p, const DevSeqs &s, const int32_t *list
 hipError_t launch_fwd(const DevProfile &p, const DevSeqs &s, const int32_t *list, const int32_t *count, int64_t maxn,
                       DevResults r, DevLists l, GateThresh g, int gating, LenCfg lc,
                       float *scale_pool, int64_t pool_size, int64_t *scale_off, float *xmx_out, hipStream_t st);
+// Pipeline counters: out[0..3] targets with stage >= 1..4, out[4] residues with
+// stage >= 1, out[5] residues with stage 4, out[6] and out[7] residues over the
+// Viterbi and Forward lists. out[] must be zero on entry.
+hipError_t launch_stats(const DevSeqs &s, const uint8_t *stage, int64_t n, const int32_t *vit, const int32_t *fwd,
+                        const int32_t *counts, unsigned long long *out, hipStream_t st);
+// dst[perm[slot]] = src[slot] for 4-byte and 1-byte per-slot arrays.
+hipError_t launch_scatter32(const DevSeqs &s, const void *src, void *dst, int64_t n, hipStream_t st);
+hipError_t launch_scatter8(const DevSeqs &s, const uint8_t *src, uint8_t *dst, int64_t n, hipStream_t st);
 hipError_t launch_bck(const DevProfile &p, const DevSeqs &s, const int32_t *list, const int32_t *count, int64_t maxn,
                       DevResults r, const float *scale_pool, const int64_t *scale_off, const float *fwd_xmx,
                       float *xmx_out, LenCfg lc, hipStream_t st);

@@ -29,6 +29,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include <math.h>
+#include <algorithm>
 #include "p7g_device.h"
 
 typedef short s2v __attribute__((ext_vector_type(2)));
@@ -38,6 +39,9 @@ typedef short s2v __attribute__((ext_vector_type(2)));
 #ifndef SSV_BLOCK
 #define SSV_BLOCK 512
 #endif
+// Workgroup size of the one-target-per-wave list stages: 8 waves share one LDS
+// copy of the profile.
+#define LIST_BLOCK 512
 
 __device__ __forceinline__ int   s2i(s2v v) { return __builtin_bit_cast(int, v); }
 __device__ __forceinline__ s2v   i2s(int v) { return __builtin_bit_cast(s2v, v); }
@@ -73,18 +77,22 @@ __device__ __forceinline__ float dppf(float old, float v)
 __device__ __forceinline__ float readlanef(float v, int l) { return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), l)); }
 enum { ROW_SHR1 = 0x111, ROW_SHR2 = 0x112, ROW_SHR4 = 0x114, ROW_SHR8 = 0x118,
        ROW_SHL1 = 0x101, ROW_SHL2 = 0x102, ROW_SHL4 = 0x104, ROW_SHL8 = 0x108,
-       WAVE_SHL1 = 0x130, WAVE_SHR1 = 0x138 };
+       WAVE_SHL1 = 0x130, WAVE_SHR1 = 0x138, ROW_BCAST15 = 0x142, ROW_BCAST31 = 0x143 };
+#define INT_NEG ((int) 0x80000000)
 
-// Wave reductions: 16-lane row scans with DPP, then the four row results
-// from lanes 15/31/47/63. The result is wave-uniform.
+// Wave reductions: inclusive scans within each 16-lane row, then row_bcast:15
+// (rows 1 and 3 take lane 15 of the row before) and row_bcast:31 (rows 2 and 3
+// take lane 31), so lane 63 holds the result. Lanes a move does not write keep
+// the identity, which lets the compiler fuse each move into its ALU op.
 __device__ __forceinline__ int wave_max_i(int v)
 {
-  v = max(v, dppi<ROW_SHR1>(v, v));
-  v = max(v, dppi<ROW_SHR2>(v, v));
-  v = max(v, dppi<ROW_SHR4>(v, v));
-  v = max(v, dppi<ROW_SHR8>(v, v));
-  return max(max(__builtin_amdgcn_readlane(v, 15), __builtin_amdgcn_readlane(v, 31)),
-             max(__builtin_amdgcn_readlane(v, 47), __builtin_amdgcn_readlane(v, 63)));
+  v = max(v, dppi<ROW_SHR1>(INT_NEG, v));
+  v = max(v, dppi<ROW_SHR2>(INT_NEG, v));
+  v = max(v, dppi<ROW_SHR4>(INT_NEG, v));
+  v = max(v, dppi<ROW_SHR8>(INT_NEG, v));
+  v = max(v, dppi<ROW_BCAST15, 0xa>(INT_NEG, v));
+  v = max(v, dppi<ROW_BCAST31, 0xc>(INT_NEG, v));
+  return __builtin_amdgcn_readlane(v, 63);
 }
 __device__ __forceinline__ float wave_sum_f(float v)
 {
@@ -92,7 +100,9 @@ __device__ __forceinline__ float wave_sum_f(float v)
   v += dppf<ROW_SHR2>(0.0f, v);
   v += dppf<ROW_SHR4>(0.0f, v);
   v += dppf<ROW_SHR8>(0.0f, v);
-  return (readlanef(v, 15) + readlanef(v, 31)) + (readlanef(v, 47) + readlanef(v, 63));
+  v += dppf<ROW_BCAST15, 0xa>(0.0f, v);
+  v += dppf<ROW_BCAST31, 0xc>(0.0f, v);
+  return readlanef(v, 63);
 }
 __device__ __forceinline__ int   uni(int v)   { return __builtin_amdgcn_readfirstlane(v); }
 __device__ __forceinline__ float unif(float v){ return __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, v))); }
@@ -107,16 +117,73 @@ __device__ __forceinline__ void stage_table(T *lds, const T *__restrict__ g, siz
   __syncthreads();
 }
 
-// Residue i (0-based) of a target whose residues start 16-byte aligned.
-__device__ __forceinline__ int residue_at(const uint8_t *sq, int i) { return sq[i]; }
+// Appends `val` of every active lane with `pred` set to list[], with one
+// atomic per wave; the order inside the list is unspecified.
+__device__ __forceinline__ void list_append(bool pred, int32_t val, int32_t *__restrict__ list, int32_t *count)
+{
+  const unsigned long long m = __ballot(pred);
+  if (m == 0ull) return;
+  const int lane = __lane_id();
+  const int leader = __ffsll((long long) m) - 1;
+  int base = 0;
+  if (lane == leader) base = atomicAdd(count, __popcll(m));
+  base = __shfl(base, leader, 64);
+  if (pred) list[base + __popcll(m & ((1ull << lane) - 1ull))] = val;
+}
+
+// Wave-uniform residue readers. A target's residues start on a 16-byte boundary
+// and every target is followed by at least 16 readable bytes (its padding or
+// the next target, and 16 spare bytes after the last one), so the chunk after
+// the current one can always be fetched ahead. Residue i is 0-based.
+__device__ __forceinline__ int chunk_byte(const uint4 &c, int i)
+{
+  const int q = (i >> 2) & 3;
+  const unsigned w = (q == 0) ? c.x : (q == 1) ? c.y : (q == 2) ? c.z : c.w;
+  return (int) ((w >> ((i & 3) * 8)) & 0xffu);
+}
+struct ResAsc {                       // i = 0, 1, 2, ...
+  const uint4 *p; uint4 cur, nxt; int c;
+  __device__ __forceinline__ void init(const uint8_t *sq) { p = (const uint4 *) sq; c = 0; cur = p[0]; nxt = p[1]; }
+  __device__ __forceinline__ int at(int i)
+  {
+    if ((i >> 4) != c) { c = i >> 4; cur = nxt; nxt = p[c + 1]; }
+    return chunk_byte(cur, i);
+  }
+};
+struct ResDesc {                      // i = L-1, L-2, ..., 0
+  const uint4 *p; uint4 cur, nxt; int c;
+  __device__ __forceinline__ void init(const uint8_t *sq, int L)
+  {
+    p = (const uint4 *) sq; c = (L - 1) >> 4; cur = p[c]; nxt = p[c > 0 ? c - 1 : 0];
+  }
+  __device__ __forceinline__ int at(int i)
+  {
+    if ((i >> 4) != c) { c = i >> 4; cur = nxt; nxt = p[c > 0 ? c - 1 : 0]; }
+    return chunk_byte(cur, i);
+  }
+};
 
 /*****************************************************************
  * 1. MSV stage, ungapped pass (constant xB)
  *****************************************************************/
-// Per-lane stride (dwords) of the ungapped-pass score table: a multiple of 4
-// (16-byte LDS reads) whose quarter is odd, so consecutive lanes' 16-byte
-// reads fall on different bank groups.
-__host__ __device__ constexpr int ssv_stride(int nd) { return ((((nd + 3) / 4) & 1) ? ((nd + 3) / 4) * 4 : ((nd + 3) / 4) * 4 + 4); }
+// Running maximum of Z over a row, two packed pairs per instruction. A value
+// Z - 32768 held as an int16 has the bit pattern 0x8000 + Z (Z < 32768), which
+// read as a half-precision number is -(Z * 2^-24) while Z < 1024 (zero and
+// subnormals, kept exact: the kernels run with fp16 denormals preserved) and
+// keeps decreasing as Z grows. So the IEEE minimum of those halves is the
+// pattern of the largest Z, and v_pk_minimum3_f16 folds two pairs into the
+// accumulator at once. Values that do not order this way (Z >= 31745: NaN or
+// positive patterns) only occur after Z has passed every overflow threshold on
+// the same diagonal, and the minimum then still decodes to an overflowing Z.
+// (Written as one instruction: the compiler would otherwise regroup the
+// minimum into pairwise trees, one and a half instructions per two pairs.)
+__device__ __forceinline__ int zmax3(int acc, int a, int b)
+{
+  int r;
+  asm("v_pk_minimum3_f16 %0, %1, %2, %3" : "=v"(r) : "v"(acc), "v"(a), "v"(b));
+  return r;
+}
+__device__ __forceinline__ int zmax2(int acc, int a) { return zmax3(acc, a, a); }
 
 template <int G, int ND>
 __global__ __launch_bounds__(SSV_BLOCK)
@@ -124,9 +191,10 @@ void k_ssv(const int *__restrict__ tbl, int tbl_dwords,
            const uint8_t *__restrict__ res, const int64_t *__restrict__ off, const int32_t *__restrict__ len,
            const int32_t *__restrict__ slots, int64_t n, uint16_t *__restrict__ zmax)
 {
-  constexpr int ND4  = ssv_stride(ND);
-  constexpr int NQ   = (ND + 3) / 4;                  // 16-byte reads per row
-  constexpr int ROW4 = G * ND4 / 4;                   // one residue row, in int4 units
+  constexpr int NQ   = ND / 4;                        // 16-byte reads per row
+  constexpr int NT   = ND % 4;                        // 4-byte reads per row
+  constexpr int NQa  = NQ ? NQ : 1, NTa = NT ? NT : 1;
+  constexpr int TAIL0 = P7G_NROWS * NQ * G * 4;       // dword offset of the tail region
   constexpr int TPW  = 64 / G;
   constexpr int NACC = 4;
   extern __shared__ int4 lds4[];
@@ -135,7 +203,8 @@ void k_ssv(const int *__restrict__ tbl, int tbl_dwords,
   const int lane = threadIdx.x & 63;
   const int grp  = lane / G;
   const int z    = lane % G;
-  const int zq   = z * (ND4 / 4);
+  const int4 *qrow = (const int4 *) __builtin_assume_aligned(lds4, 16) + z;     // + (x*NQ + q)*G
+  const int  *trow = (const int *) lds4 + TAIL0 + z;                           // + (x*NT + t)*G
 
   for (int64_t w = wave_id(); w * TPW < n; w += wave_cnt()) {
     const int64_t t = w * TPW + grp;
@@ -156,55 +225,64 @@ void k_ssv(const int *__restrict__ tbl, int tbl_dwords,
     int nb = (int) NEG_PAIR;          // DPP destination: lanes without a left neighbour keep this fill
 
     const uint4 padchunk = make_uint4(0x1f1f1f1fu, 0x1f1f1f1fu, 0x1f1f1f1fu, 0x1f1f1f1fu);
-    const int4 *lrow = (const int4 *) __builtin_assume_aligned(lds4, 16) + zq;
-    // one DP row with the scores in `cur`, while the row of residue xn is fetched into `nxt`
-#define SSV_ROW(cur, nxt, xn) do {                                                            \
-      const int4 *r_ = lrow + (xn) * ROW4;                                                    \
-      _Pragma("unroll") for (int q = 0; q < NQ; q++) nxt[q] = r_[q];                          \
+    // one DP row with the scores in (cq, ct), while the row of residue xn is fetched into (nq, nt)
+#define SSV_ROW(cq, ct, nq, nt, xn) do {                                                      \
+      const int4 *rq_ = qrow + (xn) * (NQ * G);                                               \
+      const int  *rt_ = trow + (xn) * (NT * G);                                               \
+      _Pragma("unroll") for (int q = 0; q < NQ; q++) nq[q] = rq_[q * G];                      \
+      _Pragma("unroll") for (int u = 0; u < NT; u++) nt[u] = rt_[u * G];                      \
       nb = __builtin_amdgcn_update_dpp(nb, zr[ND-1], (G == 16) ? 0x111 : 0x138, 0xf, 0xf, false); \
       const int s0_ = (int) __builtin_amdgcn_alignbit((unsigned) zr[ND-1], (unsigned) nb, 16); \
-      _Pragma("unroll") for (int d = ND-1; d >= 1; d--) {                                     \
-        const int4 c_ = cur[d / 4];                                                           \
-        const int rv_ = (d % 4 == 0) ? c_.x : (d % 4 == 1) ? c_.y : (d % 4 == 2) ? c_.z : c_.w; \
-        zr[d] = adds2(zr[d-1], rv_);                                                          \
-        acc[d % NACC] = max2(acc[d % NACC], zr[d]);                                           \
+      _Pragma("unroll") for (int d = ND-1; d >= 0; d--) {                                     \
+        int rv_;                                                                              \
+        if (d < 4 * NQ) {                                                                     \
+          const int4 c_ = cq[d / 4];                                                          \
+          rv_ = (d % 4 == 0) ? c_.x : (d % 4 == 1) ? c_.y : (d % 4 == 2) ? c_.z : c_.w;       \
+        } else rv_ = ct[d - 4 * NQ];                                                          \
+        zr[d] = adds2(d ? zr[d-1] : s0_, rv_);                                                \
       }                                                                                       \
-      zr[0] = adds2(s0_, cur[0].x);                                                           \
-      acc[0] = max2(acc[0], zr[0]);                                                           \
+      _Pragma("unroll") for (int d = 0; d + 1 < ND; d += 2)                                   \
+        acc[(d / 2) % NACC] = zmax3(acc[(d / 2) % NACC], zr[d], zr[d+1]);                     \
+      if (ND & 1) acc[((ND - 1) / 2) % NACC] = zmax2(acc[((ND - 1) / 2) % NACC], zr[ND-1]);   \
     } while (0)
 #define RES(w, u) ((int) (((w) >> (8 * ((u) & 3))) & 0xffu))
 
     uint4 ch = (0 < L) ? *(const uint4 *) sq : padchunk;
-    int4 bufA[NQ], bufB[NQ];
+    int4 qA[NQa], qB[NQa];
+    int  tA[NTa], tB[NTa];
     {
-      const int4 *r0 = lrow + RES(ch.x, 0) * ROW4;
+      const int x0 = RES(ch.x, 0);
 #pragma unroll
-      for (int q = 0; q < NQ; q++) bufA[q] = r0[q];
+      for (int q = 0; q < NQ; q++) qA[q] = qrow[(x0 * NQ + q) * G];
+#pragma unroll
+      for (int u = 0; u < NT; u++) tA[u] = trow[(x0 * NT + u) * G];
     }
     int i0 = 0;
     for (; i0 + 16 <= Lw; i0 += 16) {
       const uint4 nch = (i0 + 16 < L) ? *(const uint4 *) (sq + i0 + 16) : padchunk;
-      SSV_ROW(bufA, bufB, RES(ch.x, 1));  SSV_ROW(bufB, bufA, RES(ch.x, 2));
-      SSV_ROW(bufA, bufB, RES(ch.x, 3));  SSV_ROW(bufB, bufA, RES(ch.y, 0));
-      SSV_ROW(bufA, bufB, RES(ch.y, 1));  SSV_ROW(bufB, bufA, RES(ch.y, 2));
-      SSV_ROW(bufA, bufB, RES(ch.y, 3));  SSV_ROW(bufB, bufA, RES(ch.z, 0));
-      SSV_ROW(bufA, bufB, RES(ch.z, 1));  SSV_ROW(bufB, bufA, RES(ch.z, 2));
-      SSV_ROW(bufA, bufB, RES(ch.z, 3));  SSV_ROW(bufB, bufA, RES(ch.w, 0));
-      SSV_ROW(bufA, bufB, RES(ch.w, 1));  SSV_ROW(bufB, bufA, RES(ch.w, 2));
-      SSV_ROW(bufA, bufB, RES(ch.w, 3));  SSV_ROW(bufB, bufA, RES(nch.x, 0));
+      SSV_ROW(qA, tA, qB, tB, RES(ch.x, 1));  SSV_ROW(qB, tB, qA, tA, RES(ch.x, 2));
+      SSV_ROW(qA, tA, qB, tB, RES(ch.x, 3));  SSV_ROW(qB, tB, qA, tA, RES(ch.y, 0));
+      SSV_ROW(qA, tA, qB, tB, RES(ch.y, 1));  SSV_ROW(qB, tB, qA, tA, RES(ch.y, 2));
+      SSV_ROW(qA, tA, qB, tB, RES(ch.y, 3));  SSV_ROW(qB, tB, qA, tA, RES(ch.z, 0));
+      SSV_ROW(qA, tA, qB, tB, RES(ch.z, 1));  SSV_ROW(qB, tB, qA, tA, RES(ch.z, 2));
+      SSV_ROW(qA, tA, qB, tB, RES(ch.z, 3));  SSV_ROW(qB, tB, qA, tA, RES(ch.w, 0));
+      SSV_ROW(qA, tA, qB, tB, RES(ch.w, 1));  SSV_ROW(qB, tB, qA, tA, RES(ch.w, 2));
+      SSV_ROW(qA, tA, qB, tB, RES(ch.w, 3));  SSV_ROW(qB, tB, qA, tA, RES(nch.x, 0));
       ch = nch;
     }
     for (int u = 0; i0 + u < Lw; u++) {              // the last rows, fewer than 16
       const unsigned wd = (u + 1 < 4) ? ch.x : (u + 1 < 8) ? ch.y : (u + 1 < 12) ? ch.z : ch.w;
-      SSV_ROW(bufA, bufB, RES(wd, u + 1));
+      SSV_ROW(qA, tA, qB, tB, RES(wd, u + 1));
 #pragma unroll
-      for (int q = 0; q < NQ; q++) bufA[q] = bufB[q];
+      for (int q = 0; q < NQ; q++) qA[q] = qB[q];
+#pragma unroll
+      for (int v = 0; v < NT; v++) tA[v] = tB[v];
     }
 #undef SSV_ROW
 #undef RES
+    int m = -32768;
 #pragma unroll
-    for (int a = 1; a < NACC; a++) acc[0] = max2(acc[0], acc[a]);
-    int m = max(lo16(acc[0]), hi16(acc[0]));
+    for (int a = 0; a < NACC; a++) m = max(m, max(lo16(acc[a]), hi16(acc[a])));
 #pragma unroll
     for (int o = G/2; o > 0; o >>= 1) m = max(m, __shfl_xor(m, o, 64));
     if (valid && z == 0) zmax[slot] = (uint16_t) (m + 32768);
@@ -229,8 +307,8 @@ __global__ void k_msv_classify(const DevProfile p, const int32_t *__restrict__ l
     else if (xB + zm + p.bias >= 255)     v = -1;          // some row overflows (MSV values only grow with xB)
     else if (xB + zm > p.base + p.tec)    v = -2;          // the J state would raise xB
     else                                  v = max(xB + zm - p.tec, 0);
-    if (v == -2) { int at = atomicAdd(full_count, 1); full_list[at] = (int32_t) slot; v = 0; }
-    xj[slot] = (int16_t) v;
+    list_append(v == -2, (int32_t) slot, full_list, full_count);
+    xj[slot] = (int16_t) (v == -2 ? 0 : v);
   }
 }
 
@@ -238,8 +316,8 @@ __global__ void k_msv_classify(const DevProfile p, const int32_t *__restrict__ l
  * 3. MSV stage, full recurrence (one target per wave, 64 lanes)
  *****************************************************************/
 template <int ND>
-__global__ __launch_bounds__(256)
-void k_msv_full(const DevProfile p, const int *__restrict__ tbl, int tbl_dwords, int ND4,
+__global__ __launch_bounds__(512)
+void k_msv_full(const DevProfile p, const int *__restrict__ tbl, int tbl_dwords,
                 const uint8_t *__restrict__ res, const int64_t *__restrict__ off, const int32_t *__restrict__ len,
                 const int32_t *__restrict__ list, const int32_t *__restrict__ count, int16_t *__restrict__ xj, LenCfg lc)
 {
@@ -247,9 +325,6 @@ void k_msv_full(const DevProfile p, const int *__restrict__ tbl, int tbl_dwords,
   extern __shared__ int lds[];
   stage_table(lds, tbl, (size_t) tbl_dwords);
   const int z = threadIdx.x & 63;
-  const int zoff = z * ND4 * 4;
-  const int rowbytes = 64 * ND4 * 4;
-  const char *ldsb = (const char *) lds;
   const int64_t n = *count;
 
   for (int64_t t = wave_id(); t < n; t += wave_cnt()) {
@@ -263,9 +338,11 @@ void k_msv_full(const DevProfile p, const int *__restrict__ tbl, int tbl_dwords,
 #pragma unroll
     for (int d = 0; d < ND; d++) mp[d] = 0;
     bool overflow = false;
+    ResAsc rd;
+    if (L > 0) rd.init(sq);
     for (int i = 0; i < L; i++) {
-      const int x = uni(residue_at(sq, i));
-      const int *row = (const int *) (ldsb + x * rowbytes + zoff);
+      const int x = rd.at(i);
+      const int *row = lds + x * (ND * 64) + z;
       const int s0 = shift_in<64>(mp[ND-1], 0);
       const int xB2 = pair16(xB);
       int E = 0;
@@ -273,7 +350,7 @@ void k_msv_full(const DevProfile p, const int *__restrict__ tbl, int tbl_dwords,
       for (int d = ND-1; d >= 0; d--) {
         int src = (d > 0) ? mp[d-1] : s0;
         int tv = max2(src, xB2);
-        tv = adds2(tv, row[d]);
+        tv = adds2(tv, row[d * 64]);
         tv = max2(tv, 0);
         E  = max2(E, tv);
         mp[d] = tv;
@@ -307,19 +384,22 @@ __global__ void k_msv_gate(const DevProfile p, const int32_t *__restrict__ len, 
     r.usc[slot] = usc;
     r.nullsc[slot] = nullsc;
     uint8_t stage = 0;
+    bool to_bias = false, to_vit = false, to_fwd = false;
     if (L > 0) {
       const float seq = (float) __ddiv_rn((double) (usc - nullsc), LOG2_D);
       if (seq >= g.T1m) {
         stage = 1;
-        if (g.do_bias) {
-          int at = atomicAdd(&l.counts[1], 1); l.bias[at] = (int32_t) slot;
-        } else {
+        if (g.do_bias) to_bias = true;
+        else {
           r.filtersc[slot] = nullsc;
-          if (seq >= g.T2m) { int at = atomicAdd(&l.counts[3], 1); l.fwd[at] = (int32_t) slot; stage = 3; }
-          else              { int at = atomicAdd(&l.counts[2], 1); l.vit[at] = (int32_t) slot; stage = 2; }
+          if (seq >= g.T2m) { to_fwd = true; stage = 3; }
+          else              { to_vit = true; stage = 2; }
         }
       }
     }
+    list_append(to_bias, (int32_t) slot, l.bias, &l.counts[1]);
+    list_append(to_vit,  (int32_t) slot, l.vit,  &l.counts[2]);
+    list_append(to_fwd,  (int32_t) slot, l.fwd,  &l.counts[3]);
     r.stage[slot] = stage;
   }
 }
@@ -344,7 +424,11 @@ __global__ void k_bias(const DevProfile p, const uint8_t *__restrict__ res, cons
     const float t10 = p.ft1_0, t11 = p.ft1_1;
     float logsc = 0.0f;
     float d0, d1, mx;
-    int x = sq[0];
+    // residues 16 at a time, the next chunk fetched ahead (targets are 16-byte
+    // aligned and followed by at least 16 readable bytes)
+    const uint4 *pc = (const uint4 *) sq;
+    uint4 cur = pc[0], nxt = pc[1];
+    int x = chunk_byte(cur, 0);
     d0 = p.eo[x*2 + 0] * p.pi0;
     d1 = p.eo[x*2 + 1] * p.pi1;
     mx = 0.0f;
@@ -353,7 +437,8 @@ __global__ void k_bias(const DevProfile p, const uint8_t *__restrict__ res, cons
     d0 = __fdiv_rn(d0, mx); d1 = __fdiv_rn(d1, mx);
     logsc += (float) log((double) mx);
     for (int i = 1; i < L; i++) {
-      x = sq[i];
+      if ((i & 15) == 0) { cur = nxt; nxt = pc[(i >> 4) + 1]; }
+      x = chunk_byte(cur, i);
       float n0 = 0.0f, n1 = 0.0f;
       n0 += d0 * t00; n0 += d1 * t10; n0 *= p.eo[x*2 + 0];
       mx = 0.0f; mx = (n0 > mx) ? n0 : mx;
@@ -368,53 +453,93 @@ __global__ void k_bias(const DevProfile p, const uint8_t *__restrict__ res, cons
     // with a caller-supplied length model the host adds the length terms
     const float filtersc = lc.on ? logsc : (logsc + p.flen_a[L]) + p.flen_b[L];
     r.filtersc[slot] = filtersc;
+    bool to_vit = false, to_fwd = false;
     if (gating) {
       const float seq = (float) __ddiv_rn((double) (r.usc[slot] - filtersc), LOG2_D);
       if (seq < g.T1m) r.stage[slot] = 1;
-      else if (seq >= g.T2m) { int at = atomicAdd(&l.counts[3], 1); l.fwd[at] = slot; r.stage[slot] = 3; }
-      else                   { int at = atomicAdd(&l.counts[2], 1); l.vit[at] = slot; r.stage[slot] = 2; }
+      else if (seq >= g.T2m) { to_fwd = true; r.stage[slot] = 3; }
+      else                   { to_vit = true; r.stage[slot] = 2; }
     }
+    list_append(to_vit, slot, l.vit, &l.counts[2]);
+    list_append(to_fwd, slot, l.fwd, &l.counts[3]);
   }
 }
 
 /*****************************************************************
  * 6. Viterbi filter (one target per wave, 64 lanes, int16 pairs)
  *****************************************************************/
+// D->D closure of one Viterbi row: D(k) = max(D0(k), D(k-1) + tDD(k-1)).
+// Every tDD <= 0 and the int16 floor at -32768 absorbs, so a saturating chain
+// equals the int32 maximum over paths floored at -32768. Inside a lane both
+// halves run as packed int16 chains; across lanes the lane maps
+// x -> max(C, x + A) are composed by an int32 max-plus scan whose window sums
+// depend only on the profile and are set up once per kernel.
 template <int ND>
-__device__ __forceinline__ void dd_closure(int (&D)[ND], const int *__restrict__ tdd, int z)
+struct DDConst {
+  int t[ND];                   // tDD pairs: position d (low half), position d+ND (high half)
+  int alo, ahi;                // int32 sums of the two halves
+  int k1, k2, k4, k8;          // row-scan window sums; -2^28 where the step has no source lane
+  int arow;                    // sum of lane totals from the row start to this lane
+  int arow31, arow47;          // the same at lanes 31 and 47 (wave-uniform)
+};
+
+template <int ND>
+__device__ __forceinline__ void dd_setup(DDConst<ND> &c, const int *__restrict__ tdd, int z)
 {
-  // D(k) = max(D0(k), D(k-1) + tDD(k-1)), exact in int32 because every tDD <= 0
-  constexpr int NB = 2 * ND;
   const int NEG = -(1 << 28);
-  int v[NB], a[NB];
+  int alo = 0, ahi = 0;
 #pragma unroll
-  for (int d = 0; d < ND; d++) { v[d] = lo16(D[d]); v[d+ND] = hi16(D[d]); a[d] = lo16(tdd[d]); a[d+ND] = hi16(tdd[d]); }
-  int cur = NEG, A = 0;
+  for (int d = 0; d < ND; d++) { c.t[d] = tdd[d * 64]; alo += lo16(c.t[d]); ahi += hi16(c.t[d]); }
+  c.alo = alo; c.ahi = ahi;
+  const int r = z & 15;
+  int pre = alo + ahi;                      // inclusive prefix over the lanes of this row
 #pragma unroll
-  for (int j = 0; j < NB; j++) { cur = max(v[j], cur + a[j]); A += a[j]; }
-  int C = cur;
-  // lane maps f(x) = max(x + A, C), composed across lanes in increasing k
-#define MP_STEP(CTRL) { const int Ap = dppi<CTRL>(0, A), Cp = dppi<CTRL>(NEG, C); C = max(C, Cp + A); A = A + Ap; }
-  MP_STEP(ROW_SHR1) MP_STEP(ROW_SHR2) MP_STEP(ROW_SHR4) MP_STEP(ROW_SHR8)
-#undef MP_STEP
+  for (int s = 1; s < 16; s <<= 1) { const int v = __shfl_up(pre, s, 16); if (r >= s) pre += v; }
+  auto back = [&](int s) { const int v = __shfl_up(pre, s, 16); return (r >= s) ? pre - v : NEG; };
+  c.k1 = back(1); c.k2 = back(2); c.k4 = back(4); c.k8 = back(8);
+  c.arow = pre;
+  c.arow31 = __builtin_amdgcn_readlane(pre, 31);
+  c.arow47 = __builtin_amdgcn_readlane(pre, 47);
+}
+
+template <int CTRL>
+__device__ __forceinline__ int dpp0(int v) { return __builtin_amdgcn_update_dpp(0, v, CTRL, 0xf, 0xf, true); }
+
+template <int ND>
+__device__ __forceinline__ void dd_closure(int (&D)[ND], const DDConst<ND> &c, int z)
+{
+  const int NEG = -(1 << 28);
+  // chains inside the lane, no carry in
+  int P[ND];
+  P[0] = D[0];
+#pragma unroll
+  for (int d = 1; d < ND; d++) P[d] = max2(D[d], adds2(P[d-1], c.t[d]));
+  const int plo = lo16(P[ND-1]), phi = hi16(P[ND-1]);
+  // the lane as x -> max(C, x + A), A = alo + ahi; scanned over 16-lane rows
+  int C = max(phi, plo + c.ahi);
+  C = max(C, dpp0<ROW_SHR1>(C) + c.k1);
+  C = max(C, dpp0<ROW_SHR2>(C) + c.k2);
+  C = max(C, dpp0<ROW_SHR4>(C) + c.k4);
+  C = max(C, dpp0<ROW_SHR8>(C) + c.k8);
   {
     const int c1 = __builtin_amdgcn_readlane(C, 15);
-    const int c2 = max(c1 + __builtin_amdgcn_readlane(A, 31), __builtin_amdgcn_readlane(C, 31));
-    const int c3 = max(c2 + __builtin_amdgcn_readlane(A, 47), __builtin_amdgcn_readlane(C, 47));
+    const int c2 = max(c1 + c.arow31, __builtin_amdgcn_readlane(C, 31));
+    const int c3 = max(c2 + c.arow47, __builtin_amdgcn_readlane(C, 47));
     const int row = z >> 4;
     const int rc = (row == 0) ? NEG : (row == 1) ? c1 : (row == 2) ? c2 : c3;
-    C = max(C, rc + A);
+    C = max(C, rc + c.arow);
   }
-  int cin = dppi<WAVE_SHR1>(NEG, C);
-  cur = cin;
+  const int cin = dppi<WAVE_SHR1>(NEG, C);
+  // carries into the low half (from the lane before) and the high half (end of the low half)
+  const int clo = max(cin, -32768);
+  const int chi = max(plo, max(cin + c.alo, -32768));
+  int Q = (int) (((unsigned) clo & 0xffffu) | ((unsigned) chi << 16));
 #pragma unroll
-  for (int j = 0; j < NB; j++) { cur = max(v[j], cur + a[j]); v[j] = cur; }
-#pragma unroll
-  for (int d = 0; d < ND; d++) D[d] = (int) (((unsigned) v[d] & 0xffffu) | ((unsigned) v[d+ND] << 16));
+  for (int d = 0; d < ND; d++) { Q = adds2(Q, c.t[d]); D[d] = max2(P[d], Q); }
 }
 
 template <int ND>
-__global__ __launch_bounds__(256)
+__global__ __launch_bounds__(512)
 void k_vit(const DevProfile p, const uint8_t *__restrict__ res, const int64_t *__restrict__ off,
            const int32_t *__restrict__ len, const int32_t *__restrict__ list, const int32_t *__restrict__ count,
            DevResults r, DevLists l, GateThresh g, int gating, LenCfg lc)
@@ -425,13 +550,15 @@ void k_vit(const DevProfile p, const uint8_t *__restrict__ res, const int64_t *_
   const size_t rowd = (size_t) 64 * ND;
   stage_table(lds, p.vf_tbl, p.vf_dwords);
   const int z = threadIdx.x & 63;
-  const int *tr = lds + (size_t) p.nrows * rowd + z * ND;
+  const int *tr = lds + (size_t) p.nrows * rowd + z;
   int tBM[ND], tMM[ND], tIM[ND], tDM[ND], tMDS[ND], tMI[ND], tII[ND];
 #pragma unroll
   for (int d = 0; d < ND; d++) {
-    tBM[d] = tr[VT_BM*rowd + d]; tMM[d] = tr[VT_MM*rowd + d]; tIM[d] = tr[VT_IM*rowd + d]; tDM[d] = tr[VT_DM*rowd + d];
-    tMDS[d] = tr[VT_MDS*rowd + d]; tMI[d] = tr[VT_MI*rowd + d]; tII[d] = tr[VT_II*rowd + d];
+    tBM[d] = tr[VT_BM*rowd + d*64]; tMM[d] = tr[VT_MM*rowd + d*64]; tIM[d] = tr[VT_IM*rowd + d*64]; tDM[d] = tr[VT_DM*rowd + d*64];
+    tMDS[d] = tr[VT_MDS*rowd + d*64]; tMI[d] = tr[VT_MI*rowd + d*64]; tII[d] = tr[VT_II*rowd + d*64];
   }
+  DDConst<ND> ddc;
+  dd_setup<ND>(ddc, tr + VT_DDS*rowd, z);
   const int64_t n = *count;
 
   for (int64_t t = wave_id(); t < n; t += wave_cnt()) {
@@ -449,10 +576,12 @@ void k_vit(const DevProfile p, const uint8_t *__restrict__ res, const int64_t *_
     // DPP destinations: lane 0 never has a source and keeps the -infinity fill
     int nbM = (int) NEG_PAIR, nbI = (int) NEG_PAIR, nbD = (int) NEG_PAIR;
     int ms0 = (int) NEG_PAIR;                     // M(i-1, k-1) for the lane's first position
+    ResAsc rd;
+    if (L > 0) rd.init(sq);
 
     for (int i = 0; i < L; i++) {
-      const int x = uni(residue_at(sq, i));
-      const int *e = lds + (size_t) x * rowd + z * ND;
+      const int x = rd.at(i);
+      const int *e = lds + (size_t) x * rowd + z;
       nbI = dppi<WAVE_SHR1>(nbI, I[ND-1]);
       nbD = dppi<WAVE_SHR1>(nbD, D[ND-1]);
       const int is0 = (int) __builtin_amdgcn_alignbit((unsigned) I[ND-1], (unsigned) nbI, 16);
@@ -466,7 +595,7 @@ void k_vit(const DevProfile p, const uint8_t *__restrict__ res, const int64_t *_
         sv = max2(sv, adds2(mp, tMM[d]));
         sv = max2(sv, adds2(ip, tIM[d]));
         sv = max2(sv, adds2(dp, tDM[d]));
-        sv = adds2(sv, e[d]);
+        sv = adds2(sv, e[d * 64]);
         xE = max2(xE, sv);
         I[d] = max2(adds2(M[d], tMI[d]), adds2(I[d], tII[d]));
         M[d] = sv;
@@ -484,7 +613,7 @@ void k_vit(const DevProfile p, const uint8_t *__restrict__ res, const int64_t *_
       xJ = (int16_t) max(xJ + p.xwJ_loop, xEs + p.xwE_loop);
       xB = (int16_t) max(xJ + xmove, xN + xmove);
       const int Dmax = uni(wave_max_i(max(lo16(Dm), hi16(Dm))));
-      if (Dmax + p.ddbound_w > xB) dd_closure<ND>(D, tr + VT_DDS*rowd, z);
+      if (Dmax + p.ddbound_w > xB) dd_closure<ND>(D, ddc, z);
     }
     if (z == 0) {
       float sc;
@@ -556,7 +685,7 @@ __device__ __forceinline__ void affine_bck_scan(float (&D)[B], const float (&a)[
 }
 
 template <int B, bool LDS>
-__global__ __launch_bounds__(256)
+__global__ __launch_bounds__(512)
 void k_fwd(const DevProfile p, const uint8_t *__restrict__ res, const int64_t *__restrict__ off,
            const int32_t *__restrict__ len, const int32_t *__restrict__ list, const int32_t *__restrict__ count,
            DevResults r, DevLists l, GateThresh g, int gating, LenCfg lc,
@@ -568,12 +697,12 @@ void k_fwd(const DevProfile p, const uint8_t *__restrict__ res, const int64_t *_
   const float *tbl = p.fb_tbl;
   if (LDS) { stage_table(ldsf, p.fb_tbl, p.fb_floats); tbl = ldsf; }
   const int z = threadIdx.x & 63;
-  const float *tr = tbl + (size_t) p.nrows * rowf + z * B;
+  const float *tr = tbl + (size_t) p.nrows * rowf + z;
   float tBM[B], tMM[B], tIM[B], tDM[B], tMDS[B], tMI[B], tII[B], tDDS[B];
 #pragma unroll
   for (int j = 0; j < B; j++) {
-    tBM[j] = tr[VT_BM*rowf + j]; tMM[j] = tr[VT_MM*rowf + j]; tIM[j] = tr[VT_IM*rowf + j]; tDM[j] = tr[VT_DM*rowf + j];
-    tMDS[j] = tr[VT_MDS*rowf + j]; tMI[j] = tr[VT_MI*rowf + j]; tII[j] = tr[VT_II*rowf + j]; tDDS[j] = tr[VT_DDS*rowf + j];
+    tBM[j] = tr[VT_BM*rowf + j*64]; tMM[j] = tr[VT_MM*rowf + j*64]; tIM[j] = tr[VT_IM*rowf + j*64]; tDM[j] = tr[VT_DM*rowf + j*64];
+    tMDS[j] = tr[VT_MDS*rowf + j*64]; tMI[j] = tr[VT_MI*rowf + j*64]; tII[j] = tr[VT_II*rowf + j*64]; tDDS[j] = tr[VT_DDS*rowf + j*64];
   }
   const int64_t n = *count;
 
@@ -601,9 +730,11 @@ void k_fwd(const DevProfile p, const uint8_t *__restrict__ res, const int64_t *_
     if (scl && z == 0) scl[0] = 1.0f;
 
     float mL = 0.0f;                            // M(i-1, k-1) of the lane's first position
+    ResAsc rd;
+    if (L > 0) rd.init(sq);
     for (int i = 1; i <= L; i++) {
-      const int x = uni(residue_at(sq, i-1));
-      const float *e = tbl + (size_t) x * rowf + z * B;
+      const int x = rd.at(i-1);
+      const float *e = tbl + (size_t) x * rowf + z;
       const float iL = dppf<WAVE_SHR1>(0.0f, I[B-1]), dL = dppf<WAVE_SHR1>(0.0f, D[B-1]);
       float xEl = 0.0f;
 #pragma unroll
@@ -611,7 +742,7 @@ void k_fwd(const DevProfile p, const uint8_t *__restrict__ res, const int64_t *_
         const float mp = j ? M[j-1] : mL, ip = j ? I[j-1] : iL, dp = j ? D[j-1] : dL;
         const float In = M[j] * tMI[j] + I[j] * tII[j];
         float sv = xB * tBM[j] + mp * tMM[j] + ip * tIM[j] + dp * tDM[j];
-        sv = sv * e[j];
+        sv = sv * e[j * 64];
         M[j] = sv; I[j] = In;
         xEl += sv;
       }
@@ -663,7 +794,7 @@ void k_fwd(const DevProfile p, const uint8_t *__restrict__ res, const int64_t *_
  * 8. Backward parser (one target per wave), with Forward's scale factors
  *****************************************************************/
 template <int B, bool LDS>
-__global__ __launch_bounds__(256)
+__global__ __launch_bounds__(512)
 void k_bck(const DevProfile p, const uint8_t *__restrict__ res, const int64_t *__restrict__ off,
            const int32_t *__restrict__ len, const int32_t *__restrict__ list, const int32_t *__restrict__ count,
            DevResults r, const float *__restrict__ pool, const int64_t *__restrict__ scale_off,
@@ -675,12 +806,12 @@ void k_bck(const DevProfile p, const uint8_t *__restrict__ res, const int64_t *_
   const float *tbl = p.fb_tbl;
   if (LDS) { stage_table(ldsf, p.fb_tbl, p.fb_floats); tbl = ldsf; }
   const int z = threadIdx.x & 63;
-  const float *tr = tbl + (size_t) p.nrows * rowf + z * B;
+  const float *tr = tbl + (size_t) p.nrows * rowf + z;
   float tBM[B], tMMN[B], tIMN[B], tDMN[B], tMD[B], tDD[B], tMI[B], tII[B];
 #pragma unroll
   for (int j = 0; j < B; j++) {
-    tBM[j] = tr[VT_BM*rowf + j]; tMMN[j] = tr[FT_MMN*rowf + j]; tIMN[j] = tr[FT_IMN*rowf + j]; tDMN[j] = tr[FT_DMN*rowf + j];
-    tMD[j] = tr[FT_MD*rowf + j]; tDD[j] = tr[FT_DD*rowf + j]; tMI[j] = tr[VT_MI*rowf + j]; tII[j] = tr[VT_II*rowf + j];
+    tBM[j] = tr[VT_BM*rowf + j*64]; tMMN[j] = tr[FT_MMN*rowf + j*64]; tIMN[j] = tr[FT_IMN*rowf + j*64]; tDMN[j] = tr[FT_DMN*rowf + j*64];
+    tMD[j] = tr[FT_MD*rowf + j*64]; tDD[j] = tr[FT_DD*rowf + j*64]; tMI[j] = tr[VT_MI*rowf + j*64]; tII[j] = tr[VT_II*rowf + j*64];
   }
   const int64_t n = *count;
 
@@ -725,13 +856,15 @@ void k_bck(const DevProfile p, const uint8_t *__restrict__ res, const int64_t *_
         xr[0] = xE; xr[1] = xN; xr[2] = xJ; xr[3] = xB; xr[4] = xC; xr[5] = s;
       }
     }
+    ResDesc rd;
+    if (L > 0) rd.init(sq, L);
     for (int i = L-1; i >= 1; i--) {
-      const int x = uni(residue_at(sq, i));           // residue i+1
-      const float *e = tbl + (size_t) x * rowf + z * B;
+      const int x = rd.at(i);                         // residue i+1 (1-based)
+      const float *e = tbl + (size_t) x * rowf + z;
       float mn[B];
       float xBl = 0.0f;
 #pragma unroll
-      for (int j = 0; j < B; j++) { mn[j] = M[j] * e[j]; xBl += mn[j] * tBM[j]; }
+      for (int j = 0; j < B; j++) { mn[j] = M[j] * e[j * 64]; xBl += mn[j] * tBM[j]; }
       const float mS = wave_shl1f(mn[0]);
 #pragma unroll
       for (int j = 0; j < B; j++) {
@@ -767,11 +900,11 @@ void k_bck(const DevProfile p, const uint8_t *__restrict__ res, const int64_t *_
       }
     }
     {
-      const int x = uni(residue_at(sq, 0));
-      const float *e = tbl + (size_t) x * rowf + z * B;
+      const int x = (L > 0) ? rd.at(0) : P7G_PADCODE;
+      const float *e = tbl + (size_t) x * rowf + z;
       float xBl = 0.0f;
 #pragma unroll
-      for (int j = 0; j < B; j++) xBl += (M[j] * e[j]) * tBM[j];
+      for (int j = 0; j < B; j++) xBl += (M[j] * e[j * 64]) * tBM[j];
       xB = unif(wave_sum_f(xBl));
       xN = (xB * fmove) + (xN * floop);
     }
@@ -830,18 +963,6 @@ static unsigned grid_for_waves_blk(int64_t nwaves, size_t lds_bytes, int wpb)
   return (unsigned) g;
 }
 
-static unsigned grid_for_waves(int64_t nwaves, size_t lds_bytes)
-{
-  // persistent-style grid: enough 4-wave workgroups to fill the chip, bounded by the work
-  int per_cu = 8;
-  dev_props();
-  if (lds_bytes > 0) { per_cu = (int) ((size_t) g_lds_cu / lds_bytes); if (per_cu < 1) per_cu = 1; if (per_cu > 8) per_cu = 8; }
-  int64_t g = (int64_t) ncu() * per_cu;
-  int64_t need = (nwaves + 3) / 4;
-  if (need < g) g = need;
-  if (g < 1) g = 1;
-  return (unsigned) g;
-}
 
 #define SSV16_CASE(N) case N: k = (void *) k_ssv<16, N>; break;
 #define SSV64_CASE(N) case N: k = (void *) k_ssv<64, N>; break;
@@ -871,7 +992,7 @@ hipError_t launch_ssv(const DevProfile &p, const DevSeqs &s, const int32_t *slot
   const int64_t nwaves = (n + (64 / p.ssv_G) - 1) / (64 / p.ssv_G);
   const unsigned grid = grid_for_waves_blk(nwaves, lds, SSV_BLOCK / 64);
   int tbl_dwords = (int) p.ssv_dwords;
-  if (p.ssv_ND4 != ssv_stride(p.ssv_ND)) return hipErrorInvalidValue;
+  if (p.ssv_dwords != (size_t) P7G_NROWS * p.ssv_G * p.ssv_ND) return hipErrorInvalidValue;
   void *args[] = { (void *) &p.ssv_tbl, &tbl_dwords, (void *) &s.res, (void *) &s.off, (void *) &s.len,
                    (void *) &slots, &n, &zmax };
   return launch(k, grid, SSV_BLOCK, args, lds, st);
@@ -899,11 +1020,11 @@ hipError_t launch_msv_full(const DevProfile &p, const DevSeqs &s, const int32_t 
     default: return hipErrorInvalidValue;
   }
   const size_t lds = p.mf_dwords * 4;
-  const unsigned grid = grid_for_waves(maxn, lds);
-  int tbl_dwords = (int) p.mf_dwords, ND4 = p.mf_ND4;
-  void *args[] = { (void *) &p, (void *) &p.mf_tbl, &tbl_dwords, &ND4, (void *) &s.res, (void *) &s.off, (void *) &s.len,
+  const unsigned grid = grid_for_waves_blk(maxn, lds, LIST_BLOCK / 64);
+  int tbl_dwords = (int) p.mf_dwords;
+  void *args[] = { (void *) &p, (void *) &p.mf_tbl, &tbl_dwords, (void *) &s.res, (void *) &s.off, (void *) &s.len,
                    (void *) &list, (void *) &count, &xj, &lc };
-  return launch(k, grid, 256, args, lds, st);
+  return launch(k, grid, LIST_BLOCK, args, lds, st);
 }
 
 hipError_t launch_msv_gate(const DevProfile &p, const DevSeqs &s, int64_t n, const int16_t *xj, DevResults r,
@@ -936,10 +1057,10 @@ hipError_t launch_vit(const DevProfile &p, const DevSeqs &s, const int32_t *list
     default: return hipErrorInvalidValue;
   }
   const size_t lds = p.vf_dwords * 4;
-  const unsigned grid = grid_for_waves(maxn, lds);
+  const unsigned grid = grid_for_waves_blk(maxn, lds, LIST_BLOCK / 64);
   void *args[] = { (void *) &p, (void *) &s.res, (void *) &s.off, (void *) &s.len, (void *) &list, (void *) &count,
                    &r, &l, &g, &gating, &lc };
-  return launch(k, grid, 256, args, lds, st);
+  return launch(k, grid, LIST_BLOCK, args, lds, st);
 }
 
 #define FB_LDS_LIMIT (150 * 1024)
@@ -958,10 +1079,10 @@ hipError_t launch_fwd(const DevProfile &p, const DevSeqs &s, const int32_t *list
     default: return hipErrorInvalidValue;
   }
   const size_t lds = lds_ok ? p.fb_floats * 4 : 0;
-  const unsigned grid = grid_for_waves(maxn, lds);
+  const unsigned grid = grid_for_waves_blk(maxn, lds, LIST_BLOCK / 64);
   void *args[] = { (void *) &p, (void *) &s.res, (void *) &s.off, (void *) &s.len, (void *) &list, (void *) &count,
                    &r, &l, &g, &gating, &lc, &scale_pool, &pool_size, &scale_off, &xmx_out };
-  return launch(k, grid, 256, args, lds, st);
+  return launch(k, grid, LIST_BLOCK, args, lds, st);
 }
 
 #define BCK_CASE(N) case N: k = lds_ok ? (void *) k_bck<N, true> : (void *) k_bck<N, false>; break;
@@ -979,8 +1100,78 @@ hipError_t launch_bck(const DevProfile &p, const DevSeqs &s, const int32_t *list
     default: return hipErrorInvalidValue;
   }
   const size_t lds = lds_ok ? p.fb_floats * 4 : 0;
-  const unsigned grid = grid_for_waves(maxn, lds);
+  const unsigned grid = grid_for_waves_blk(maxn, lds, LIST_BLOCK / 64);
   void *args[] = { (void *) &p, (void *) &s.res, (void *) &s.off, (void *) &s.len, (void *) &list, (void *) &count,
                    &r, (void *) &scale_pool, (void *) &scale_off, (void *) &fwd_xmx, &xmx_out, &lc };
-  return launch(k, grid, 256, args, lds, st);
+  return launch(k, grid, LIST_BLOCK, args, lds, st);
+}
+
+/*****************************************************************
+ * Pipeline counters and caller-order outputs
+ *****************************************************************/
+__global__ __launch_bounds__(256)
+void k_stats(const int32_t *__restrict__ len, const uint8_t *__restrict__ stage, int64_t n,
+             const int32_t *__restrict__ vit, const int32_t *__restrict__ fwd, const int32_t *__restrict__ counts,
+             unsigned long long *__restrict__ out)
+{
+  unsigned long long a[8] = { 0, 0, 0, 0, 0, 0, 0, 0 };
+  const int64_t i0 = (int64_t) blockIdx.x * blockDim.x + threadIdx.x, step = (int64_t) gridDim.x * blockDim.x;
+  for (int64_t slot = i0; slot < n; slot += step) {
+    const int sg = stage[slot];
+    const unsigned long long L = (unsigned long long) len[slot];
+    a[0] += sg >= 1; a[1] += sg >= 2; a[2] += sg >= 3; a[3] += sg >= 4;
+    a[4] += sg >= 1 ? L : 0ull;
+    a[5] += sg >= 4 ? L : 0ull;
+  }
+  const int64_t nv = counts[2], nf = counts[3];
+  for (int64_t t = i0; t < nv; t += step) a[6] += (unsigned long long) len[vit[t]];
+  for (int64_t t = i0; t < nf; t += step) a[7] += (unsigned long long) len[fwd[t]];
+  __shared__ unsigned long long red[8][4];
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+#pragma unroll
+  for (int k = 0; k < 8; k++) {
+    unsigned long long v = a[k];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    if (lane == 0) red[k][wv] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < 8) {
+    const unsigned long long v = red[threadIdx.x][0] + red[threadIdx.x][1] + red[threadIdx.x][2] + red[threadIdx.x][3];
+    if (v) atomicAdd(&out[threadIdx.x], v);
+  }
+}
+
+__global__ void k_scatter32(const int32_t *__restrict__ perm, const uint32_t *__restrict__ src, uint32_t *__restrict__ dst, int64_t n)
+{
+  for (int64_t s = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; s < n; s += (int64_t) gridDim.x * blockDim.x)
+    dst[perm[s]] = src[s];
+}
+__global__ void k_scatter8(const int32_t *__restrict__ perm, const uint8_t *__restrict__ src, uint8_t *__restrict__ dst, int64_t n)
+{
+  for (int64_t s = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; s < n; s += (int64_t) gridDim.x * blockDim.x)
+    dst[perm[s]] = src[s];
+}
+
+hipError_t launch_stats(const DevSeqs &s, const uint8_t *stage, int64_t n, const int32_t *vit, const int32_t *fwd,
+                        const int32_t *counts, unsigned long long *out, hipStream_t st)
+{
+  if (n <= 0) return hipSuccess;
+  unsigned grid = (unsigned) std::min<int64_t>((n + 255) / 256, (int64_t) ncu() * 2);
+  hipLaunchKernelGGL(k_stats, dim3(grid), dim3(256), 0, st, s.len, stage, n, vit, fwd, counts, out);
+  return hipGetLastError();
+}
+hipError_t launch_scatter32(const DevSeqs &s, const void *src, void *dst, int64_t n, hipStream_t st)
+{
+  if (n <= 0) return hipSuccess;
+  unsigned grid = (unsigned) std::min<int64_t>((n + 255) / 256, 65536);
+  hipLaunchKernelGGL(k_scatter32, dim3(grid), dim3(256), 0, st, s.perm, (const uint32_t *) src, (uint32_t *) dst, n);
+  return hipGetLastError();
+}
+hipError_t launch_scatter8(const DevSeqs &s, const uint8_t *src, uint8_t *dst, int64_t n, hipStream_t st)
+{
+  if (n <= 0) return hipSuccess;
+  unsigned grid = (unsigned) std::min<int64_t>((n + 255) / 256, 65536);
+  hipLaunchKernelGGL(k_scatter8, dim3(grid), dim3(256), 0, st, s.perm, src, dst, n);
+  return hipGetLastError();
 }

@@ -1,6 +1,9 @@
 // C ABI of libp7gpu: device setup, device copies of profiles and target
 // databases, batched filter stages, the device pipeline, and the
-// single-target drop-ins. One HIP stream per process, default priority.
+// single-target drop-ins. Every pipeline object owns a HIP stream and its
+// device work area, so pipelines driven from different host threads overlap
+// on the device; the single-target and batched calls share one library
+// stream. All streams are created at the default priority.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include <stdio.h>
@@ -19,7 +22,27 @@ extern "C" const char *p7g_host_error(void);
 
 static hipStream_t g_stream = nullptr;
 static int         g_device = -1;
-static std::mutex  g_lock;     // serialises calls that share the stream and scratch buffers
+static std::mutex  g_lock;     // serialises calls that share the library stream and its work area
+static std::mutex  g_devlock;  // guards building the device copy of a profile (innermost lock)
+
+// Device work area of one stream: per-slot results, survivor lists, the
+// Forward scale-factor pool, timing events.
+struct Work {
+  hipStream_t st = nullptr;
+  int64_t     cap = 0;
+  DevResults  r;
+  DevLists    l;
+  int32_t    *slots = nullptr;      // explicit slot list for batched calls
+  int64_t    *scale_off = nullptr;
+  void       *outbuf = nullptr;     // one per-slot array in the caller's order
+  unsigned long long *stats = nullptr;
+  float      *pool = nullptr;
+  int64_t     pool_size = 0;
+  void       *mem = nullptr;
+  hipEvent_t  ev[8];
+  bool        ev_ready = false;
+};
+static Work g_work;
 
 #define HIPCHK(call) do { hipError_t e_ = (call); if (e_ != hipSuccess) { \
       p7g_set_error("HIP error %s at %s:%d (%s)", hipGetErrorString(e_), __FILE__, __LINE__, #call); \
@@ -36,9 +59,12 @@ p7_gpu_Init(int device)
   HIPCHK(hipSetDevice(device));
   if (g_stream) { (void) hipStreamDestroy(g_stream); g_stream = nullptr; }
   HIPCHK(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking));
+  g_work.st = g_stream;
   g_device = device;
   return p7G_OK;
 }
+
+static int ensure_init(void);
 
 static int
 ensure_init(void)
@@ -64,14 +90,18 @@ extern "C" const char *p7_gpu_LastError(void) { return p7g_host_error(); }
 struct p7g_dev_profile_s {
   DevProfile p;
   int        Lmax;
+  struct p7g_dev_profile_s *older;   // copies replaced by a longer length range, kept until the profile goes
 };
 
 extern "C" void
 p7g_dev_profile_free(struct p7g_dev_profile_s *d)
 {
-  if (! d) return;
-  if (d->p.blob) (void) hipFree(d->p.blob);
-  delete d;
+  while (d) {
+    struct p7g_dev_profile_s *o = d->older;
+    if (d->p.blob) (void) hipFree(d->p.blob);
+    delete d;
+    d = o;
+  }
 }
 
 static const int vf_nd_set[] = { 1, 2, 3, 4, 5, 6, 8, 10, 12, 16, 20, 24, 32, 40 };
@@ -84,9 +114,6 @@ static int pick(const int *set, int nset, int need)
   return -1;
 }
 static int cdiv(int a, int b) { return (a + b - 1) / b; }
-// per-lane stride in dwords: a multiple of 4 whose quarter is odd, so that
-// 16-byte LDS reads of consecutive lanes land on different bank groups
-static int odd_quad_stride(int nd) { int s = ((nd + 3) / 4) * 4; if (((s / 4) & 1) == 0) s += 4; return s; }
 
 static inline int pack2(int lo, int hi) { return (int) (((unsigned) lo & 0xffffu) | ((unsigned) hi << 16)); }
 
@@ -94,8 +121,9 @@ static int
 dev_profile_build(P7_GPU_OPROFILE *om, int Lmax_need, struct p7g_dev_profile_s **ret)
 {
   const int M = om->M, Kp = om->abc->Kp;
-  const int NR = 32;                                 // residue rows incl. P7G_PADCODE
+  const int NR = P7G_NROWS;                          // residue rows incl. P7G_PADCODE
   auto *d = new p7g_dev_profile_s();
+  d->older = nullptr;
   DevProfile &p = d->p;
   memset(&p, 0, sizeof(p));
   p.M = M; p.Kp = Kp; p.nrows = NR;
@@ -110,27 +138,30 @@ dev_profile_build(P7_GPU_OPROFILE *om, int Lmax_need, struct p7g_dev_profile_s *
     p.ssv_G = 64; p.ssv_ND = pick(ssv64_set, (int) (sizeof(ssv64_set)/sizeof(int)), cdiv(cdiv(M, 64), 2));
     if (p.ssv_ND < 0) { delete d; p7g_set_error("model too long for the device MSV stage (M=%d)", M); return p7G_EINVAL; }
   }
-  p.ssv_ND4 = odd_quad_stride(p.ssv_ND);
-  p.ssv_dwords = (size_t) NR * p.ssv_G * p.ssv_ND4;
+  p.ssv_dwords = (size_t) NR * p.ssv_G * p.ssv_ND;
   std::vector<int> ssv(p.ssv_dwords, (int) 0x80008000u);
   {
-    const int G = p.ssv_G, ND = p.ssv_ND, B = 2 * ND;
+    // quads: [x][q][z][4] for dwords 0..4*NQ-1; tail: [x][t][z] for dwords 4*NQ + t
+    const int G = p.ssv_G, ND = p.ssv_ND, B = 2 * ND, NQ = ND / 4, NT = ND % 4;
+    const size_t tail0 = (size_t) NR * NQ * G * 4;
     for (int x = 0; x < NR; x++)
       for (int z = 0; z < G; z++)
-        for (int dd = 0; dd < ND; dd++)
-          ssv[((size_t) x * G + z) * p.ssv_ND4 + dd] = pack2(rsv(x, z*B + dd + 1), rsv(x, z*B + dd + ND + 1));
+        for (int dd = 0; dd < ND; dd++) {
+          const int v = pack2(rsv(x, z*B + dd + 1), rsv(x, z*B + dd + ND + 1));
+          if (dd < 4 * NQ) ssv[(((size_t) x * NQ + dd / 4) * G + z) * 4 + dd % 4] = v;
+          else             ssv[tail0 + ((size_t) x * NT + (dd - 4 * NQ)) * G + z] = v;
+        }
   }
   p.mf_ND = pick(vf_nd_set, (int) (sizeof(vf_nd_set)/sizeof(int)), std::max(1, cdiv(cdiv(M, 64), 2)));
   if (p.mf_ND < 0) { delete d; p7g_set_error("model too long for the device MSV stage (M=%d)", M); return p7G_EINVAL; }
-  p.mf_ND4 = p.mf_ND;
-  p.mf_dwords = (size_t) NR * 64 * p.mf_ND4;
+  p.mf_dwords = (size_t) NR * 64 * p.mf_ND;
   std::vector<int> mf(p.mf_dwords);
   {
     const int ND = p.mf_ND, B = 2 * ND;
     for (int x = 0; x < NR; x++)
       for (int z = 0; z < 64; z++)
         for (int dd = 0; dd < ND; dd++)
-          mf[((size_t) x * 64 + z) * ND + dd] = pack2(rsv(x, z*B + dd + 1), rsv(x, z*B + dd + ND + 1));
+          mf[((size_t) x * ND + dd) * 64 + z] = pack2(rsv(x, z*B + dd + 1), rsv(x, z*B + dd + ND + 1));
   }
 
   // ---- Viterbi tables: int16 pairs, capacity >= M+1 ----
@@ -162,7 +193,7 @@ dev_profile_build(P7_GPU_OPROFILE *om, int Lmax_need, struct p7g_dev_profile_s *
           int a, b;
           if (r < NR) { a = ew(r, k0); b = ew(r, k1); }
           else        { a = tv(r - NR, k0); b = tv(r - NR, k1); }
-          vf[((size_t) r * 64 + z) * ND + dd] = pack2(a, b);
+          vf[((size_t) r * ND + dd) * 64 + z] = pack2(a, b);
         }
   }
 
@@ -197,7 +228,7 @@ dev_profile_build(P7_GPU_OPROFILE *om, int Lmax_need, struct p7g_dev_profile_s *
       for (int z = 0; z < 64; z++)
         for (int j = 0; j < B; j++) {
           const int k = z*B + j + 1;
-          fb[((size_t) r * 64 + z) * B + j] = (r < NR) ? ef(r, k) : tv(r - NR, k);
+          fb[((size_t) r * B + j) * 64 + z] = (r < NR) ? ef(r, k) : tv(r - NR, k);
         }
   }
 
@@ -269,8 +300,14 @@ dev_profile(const P7_GPU_OPROFILE *om_c, int Lmax_need, DevProfile **ret)
   int status;
   if (om->M <= 0) { p7g_set_error("oprofile has no model"); return p7G_EINVAL; }
   if (om->mode != p7G_LOCAL && om->mode != p7G_UNILOCAL) { p7g_set_error("the filters need a local-mode profile"); return p7G_EINVAL; }
-  if (om->dev && om->dev->Lmax < Lmax_need) { p7g_dev_profile_free(om->dev); om->dev = nullptr; }
-  if (! om->dev && (status = dev_profile_build(om, Lmax_need, &om->dev)) != p7G_OK) return status;
+  std::lock_guard<std::mutex> lk(g_devlock);
+  if (! om->dev || om->dev->Lmax < Lmax_need) {
+    // a pipeline on another stream may still read the current copy: retire it, do not free it
+    struct p7g_dev_profile_s *d = nullptr;
+    if ((status = dev_profile_build(om, Lmax_need, &d)) != p7G_OK) return status;
+    d->older = om->dev;
+    om->dev = d;
+  }
   *ret = &om->dev->p;
   return p7G_OK;
 }
@@ -287,6 +324,7 @@ struct p7g_seqdb_s {
   uint8_t *d_res;
   int64_t *d_off;
   int32_t *d_len;
+  int32_t *d_perm;         // slot -> caller index, on the device
 };
 
 extern "C" void
@@ -296,6 +334,7 @@ p7_gpu_seqdb_Destroy(P7_GPU_SEQDB *db)
   if (db->d_res) (void) hipFree(db->d_res);
   if (db->d_off) (void) hipFree(db->d_off);
   if (db->d_len) (void) hipFree(db->d_len);
+  if (db->d_perm) (void) hipFree(db->d_perm);
   free(db->h_len); free(db->h_perm); free(db->h_inv);
   free(db);
 }
@@ -347,12 +386,16 @@ p7_gpu_seqdb_Create(const uint8_t *residues, const int64_t *offset, const int32_
   db->maxcode = maxcode;
   if (maxcode >= 29) { free(h_res); free(h_off); p7_gpu_seqdb_Destroy(db); p7g_set_error("residue code %d is not a digital residue", maxcode); return p7G_EINVAL; }
 
+  std::vector<int32_t> h_perm32((size_t) nseq);
+  for (int64_t s = 0; s < nseq; s++) h_perm32[(size_t) s] = (int32_t) db->h_perm[s];
   hipError_t e = hipMalloc((void **) &db->d_res, (size_t) db->bytes);
   if (e == hipSuccess) e = hipMalloc((void **) &db->d_off, sizeof(int64_t) * (size_t) nseq);
   if (e == hipSuccess) e = hipMalloc((void **) &db->d_len, sizeof(int32_t) * (size_t) nseq);
+  if (e == hipSuccess) e = hipMalloc((void **) &db->d_perm, sizeof(int32_t) * (size_t) nseq);
   if (e == hipSuccess) e = hipMemcpyAsync(db->d_res, h_res, (size_t) db->bytes, hipMemcpyHostToDevice, g_stream);
   if (e == hipSuccess) e = hipMemcpyAsync(db->d_off, h_off, sizeof(int64_t) * (size_t) nseq, hipMemcpyHostToDevice, g_stream);
   if (e == hipSuccess) e = hipMemcpyAsync(db->d_len, db->h_len, sizeof(int32_t) * (size_t) nseq, hipMemcpyHostToDevice, g_stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(db->d_perm, h_perm32.data(), sizeof(int32_t) * (size_t) nseq, hipMemcpyHostToDevice, g_stream);
   if (e == hipSuccess) e = hipStreamSynchronize(g_stream);
   free(h_res); free(h_off);
   if (e != hipSuccess) { p7g_set_error("database upload failed: %s", hipGetErrorString(e)); p7_gpu_seqdb_Destroy(db); return p7G_EDEVICE; }
@@ -363,36 +406,32 @@ p7_gpu_seqdb_Create(const uint8_t *residues, const int64_t *offset, const int32_
 extern "C" int64_t p7_gpu_seqdb_NSeq(const P7_GPU_SEQDB *db) { return db ? db->nseq : 0; }
 extern "C" int64_t p7_gpu_seqdb_NRes(const P7_GPU_SEQDB *db) { return db ? db->nres : 0; }
 
-static DevSeqs seqs_of(const P7_GPU_SEQDB *db) { DevSeqs s; s.res = db->d_res; s.off = db->d_off; s.len = db->d_len; s.nslot = db->nseq; return s; }
+static DevSeqs seqs_of(const P7_GPU_SEQDB *db)
+{
+  DevSeqs s; s.res = db->d_res; s.off = db->d_off; s.len = db->d_len; s.perm = db->d_perm; s.nslot = db->nseq;
+  return s;
+}
 
 /*****************************************************************
  * Work buffers: per-slot results and lists, grown on demand
  *****************************************************************/
-struct Work {
-  int64_t     cap = 0;
-  DevResults  r;
-  DevLists    l;
-  int32_t    *slots = nullptr;      // explicit slot list for batched calls
-  int64_t    *scale_off = nullptr;
-  float      *pool = nullptr;
-  int64_t     pool_size = 0;
-  void       *mem = nullptr;
-  hipEvent_t  ev[8];
-  bool        ev_ready = false;
-};
-static Work g_work;
+
+// bytes per database slot of all per-slot arrays of a work area
+#define WORK_BYTES_PER_SLOT (2 + 2 + 4*6 + 1 + 4 + 4 + 4*5 + 4 + 8 + 4 + 256)
+// Forward scale-factor pool of a pipeline: one float per row of every target
+// that may reach Forward, at most 2^28 floats
+static int64_t pipeline_pool_floats(int64_t nseq, int64_t nres) { return std::min<int64_t>(nres + nseq, (int64_t) 1 << 28); }
 
 static int
-work_reserve(int64_t n, int64_t pool_floats)
+work_reserve(Work &w, int64_t n, int64_t pool_floats)
 {
-  Work &w = g_work;
   if (! w.ev_ready) { for (int i = 0; i < 8; i++) HIPCHK(hipEventCreate(&w.ev[i])); w.ev_ready = true; }
   if (n <= w.cap && pool_floats <= w.pool_size) return p7G_OK;
-  if (w.mem) { HIPCHK(hipStreamSynchronize(g_stream)); (void) hipFree(w.mem); w.mem = nullptr; w.cap = 0; }
+  if (w.mem || w.pool) HIPCHK(hipStreamSynchronize(w.st));
+  if (w.mem) { (void) hipFree(w.mem); w.mem = nullptr; w.cap = 0; }
   if (w.pool) { (void) hipFree(w.pool); w.pool = nullptr; w.pool_size = 0; }
   const int64_t cap = std::max<int64_t>(n, 1024);
-  size_t per = 2 + 2 + 4*6 + 1 + 4 + 4 + 4*5 + 4 + 8;   // per-slot bytes of all arrays
-  size_t total = (size_t) cap * per + 4096;
+  size_t total = (size_t) cap * WORK_BYTES_PER_SLOT + 4096;
   char *m = nullptr;
   HIPCHK(hipMalloc((void **) &m, total));
   size_t o = 0;
@@ -405,8 +444,10 @@ work_reserve(int64_t n, int64_t pool_floats)
   w.l.fwd = (int32_t *) take(4 * cap); w.l.bck = (int32_t *) take(4 * cap);
   w.slots = (int32_t *) take(4 * cap);
   w.scale_off = (int64_t *) take(8 * cap);
+  w.outbuf = take(4 * cap);
   w.l.counts = (int32_t *) take(64);
   w.l.cursor = (unsigned long long *) take(64);
+  w.stats = (unsigned long long *) take(64);
   if (o > total) { p7g_set_error("internal: work buffer overrun"); return p7G_EINVAL; }
   w.mem = m; w.cap = cap;
   const int64_t pf = std::max<int64_t>(pool_floats, 1 << 20);
@@ -416,11 +457,21 @@ work_reserve(int64_t n, int64_t pool_floats)
 }
 
 static int
-reset_counts(void)
+reset_counts(Work &w)
 {
-  HIPCHK(hipMemsetAsync(g_work.l.counts, 0, 64, g_stream));
-  HIPCHK(hipMemsetAsync(g_work.l.cursor, 0, 64, g_stream));
+  HIPCHK(hipMemsetAsync(w.l.counts, 0, 64, w.st));
+  HIPCHK(hipMemsetAsync(w.l.cursor, 0, 64, w.st));
   return p7G_OK;
+}
+
+static void
+work_release(Work &w)
+{
+  if (w.st) (void) hipStreamSynchronize(w.st);
+  if (w.mem) (void) hipFree(w.mem);
+  if (w.pool) (void) hipFree(w.pool);
+  if (w.ev_ready) for (int i = 0; i < 8; i++) (void) hipEventDestroy(w.ev[i]);
+  w.mem = nullptr; w.pool = nullptr; w.cap = 0; w.pool_size = 0; w.ev_ready = false;
 }
 
 static int
@@ -461,13 +512,13 @@ p7_gpu_MSVFilterBatch(const P7_GPU_OPROFILE *om, const P7_GPU_SEQDB *db, const i
   if (db->maxcode >= om->abc->Kp) { p7g_set_error("database holds residue codes outside the model alphabet"); return p7G_EINCOMPAT; }
   if ((status = dev_profile(om, db->Lmax, &p)) != p7G_OK) return status;
   if (idx == nullptr) n = db->nseq;
-  if ((status = work_reserve(db->nseq, 0)) != p7G_OK) return status;
+  if ((status = work_reserve(g_work, db->nseq, 0)) != p7G_OK) return status;
   std::vector<int32_t> slots;
   if ((status = slots_from_idx(db, idx, n, slots)) != p7G_OK) return status;
   DevSeqs s = seqs_of(db);
   Work &w = g_work;
   LenCfg lc = lencfg_per_target();
-  if ((status = reset_counts()) != p7G_OK) return status;
+  if ((status = reset_counts(g_work)) != p7G_OK) return status;
   HIPCHK(hipMemcpyAsync(w.slots, slots.data(), sizeof(int32_t) * (size_t) n, hipMemcpyHostToDevice, g_stream));
   HIPCHK(launch_ssv(*p, s, w.slots, n, w.r.zmax, g_stream));
   HIPCHK(launch_msv_classify(*p, s, w.slots, n, w.r.zmax, w.r.xj, w.l.full, &w.l.counts[0], lc, g_stream));
@@ -505,13 +556,13 @@ run_list_stage(int which, const P7_GPU_OPROFILE *om, const P7_GPU_SEQDB *db, con
   if ((status = slots_from_idx(db, idx, n, slots)) != p7G_OK) return status;
   int64_t poolneed = 0;
   if (which == ST_BCK) for (int64_t t = 0; t < n; t++) poolneed += db->h_len[slots[t]] + 1;
-  if ((status = work_reserve(db->nseq, poolneed)) != p7G_OK) return status;
+  if ((status = work_reserve(g_work, db->nseq, poolneed)) != p7G_OK) return status;
   for (int64_t t = 0; t < n; t++) if (db->h_len[slots[t]] == 0) { p7g_set_error("zero-length target in a batched stage"); return p7G_EINVAL; }
   DevSeqs s = seqs_of(db);
   Work &w = g_work;
   LenCfg lc = lencfg_per_target();
   GateThresh g; memset(&g, 0, sizeof(g));
-  if ((status = reset_counts()) != p7G_OK) return status;
+  if ((status = reset_counts(g_work)) != p7G_OK) return status;
   int32_t *list = (which == ST_BIAS) ? w.l.bias : (which == ST_VIT) ? w.l.vit : w.l.fwd;
   int32_t *cnt  = (which == ST_BIAS) ? &w.l.counts[1] : (which == ST_VIT) ? &w.l.counts[2] : &w.l.counts[3];
   int32_t n32 = (int32_t) n;
@@ -560,20 +611,66 @@ struct p7g_pipeline_s {
   double F1, F2, F3;
   int    do_biasfilter, do_max;
   P7_GPU_STATS st;
+  Work       w;             // this pipeline's stream and device work area
+  std::mutex mu;            // one call at a time per pipeline object
 };
 
 extern "C" P7_GPU_PIPELINE *
 p7_gpu_pipeline_Create(double F1, double F2, double F3, int do_biasfilter, int do_max)
 {
-  P7_GPU_PIPELINE *pli = (P7_GPU_PIPELINE *) calloc(1, sizeof(P7_GPU_PIPELINE));
+  P7_GPU_PIPELINE *pli = new (std::nothrow) p7g_pipeline_s();
   if (! pli) return nullptr;
+  memset(&pli->st, 0, sizeof(pli->st));
   pli->F1 = F1; pli->F2 = F2; pli->F3 = F3;
   pli->do_biasfilter = do_biasfilter; pli->do_max = do_max;
   if (do_max) { pli->F1 = pli->F2 = pli->F3 = 1.0; pli->do_biasfilter = 0; }   // --max (p7_pipeline.c:214-219)
   return pli;
 }
 
-extern "C" void p7_gpu_pipeline_Destroy(P7_GPU_PIPELINE *pli) { free(pli); }
+extern "C" size_t
+p7g_pipeline_work_bytes(int64_t nseq, int64_t nres)
+{
+  const int64_t cap = std::max<int64_t>(nseq, 1024);
+  return (size_t) cap * WORK_BYTES_PER_SLOT + 4096 + sizeof(float) * (size_t) std::max<int64_t>(pipeline_pool_floats(nseq, nres), 1 << 20);
+}
+
+extern "C" int
+p7g_pipeline_Reserve(P7_GPU_PIPELINE *pli, const P7_GPU_SEQDB *db)
+{
+  std::lock_guard<std::mutex> lk(pli->mu);
+  int status;
+  if ((status = ensure_init()) != p7G_OK) return status;
+  if (! pli->w.st) HIPCHK(hipStreamCreateWithFlags(&pli->w.st, hipStreamNonBlocking));
+  return work_reserve(pli->w, db->nseq, pipeline_pool_floats(db->nseq, db->nres));
+}
+
+extern "C" void
+p7g_pipeline_ResetStats(P7_GPU_PIPELINE *pli)
+{
+  std::lock_guard<std::mutex> lk(pli->mu);
+  memset(&pli->st, 0, sizeof(pli->st));
+}
+
+extern "C" int p7g_current_device(void) { return g_device >= 0 ? g_device : 0; }
+
+extern "C" int
+p7g_device_free_bytes(size_t *ret_bytes)
+{
+  size_t fr = 0, tot = 0;
+  *ret_bytes = 0;
+  HIPCHK(hipMemGetInfo(&fr, &tot));
+  *ret_bytes = fr;
+  return p7G_OK;
+}
+
+extern "C" void
+p7_gpu_pipeline_Destroy(P7_GPU_PIPELINE *pli)
+{
+  if (! pli) return;
+  work_release(pli->w);
+  if (pli->w.st) (void) hipStreamDestroy(pli->w.st);
+  delete pli;
+}
 
 extern "C" int
 p7_gpu_pipeline_GetStats(const P7_GPU_PIPELINE *pli, P7_GPU_STATS *st)
@@ -585,10 +682,11 @@ p7_gpu_pipeline_GetStats(const P7_GPU_PIPELINE *pli, P7_GPU_STATS *st)
 extern "C" int
 p7_gpu_Pipeline(P7_GPU_PIPELINE *pli, const P7_GPU_OPROFILE *om, const P7_GPU_SEQDB *db, P7_GPU_HITS *hits)
 {
-  std::lock_guard<std::mutex> lk(g_lock);
+  std::lock_guard<std::mutex> lk(pli->mu);
   int status;
   DevProfile *p;
   if ((status = ensure_init()) != p7G_OK) return status;
+  if (! pli->w.st) HIPCHK(hipStreamCreateWithFlags(&pli->w.st, hipStreamNonBlocking));
   if (db->maxcode >= om->abc->Kp) { p7g_set_error("database holds residue codes outside the model alphabet"); return p7G_EINCOMPAT; }
   for (int z = 0; z < 6; z++)
     if (om->evparam[z] == -99999.0f) { p7g_set_error("profile has no E-value parameters (STATS)"); return p7G_EINVAL; }
@@ -596,9 +694,9 @@ p7_gpu_Pipeline(P7_GPU_PIPELINE *pli, const P7_GPU_OPROFILE *om, const P7_GPU_SE
   const int64_t n = db->nseq;
   // pool for the Forward row scale factors of F3 survivors: bounded by the
   // total residues of every target that may reach the Forward stage
-  if ((status = work_reserve(n, std::min<int64_t>(db->nres + n, (int64_t) 1 << 28))) != p7G_OK) return status;
+  Work &w = pli->w;
+  if ((status = work_reserve(w, n, pipeline_pool_floats(n, db->nres))) != p7G_OK) return status;
   DevSeqs s = seqs_of(db);
-  Work &w = g_work;
   LenCfg lc = lencfg_per_target();
   GateThresh g;
   g.T1m = p7g_gumbel_threshold(om->evparam[p7G_MMU], om->evparam[p7G_MLAMBDA], pli->F1);
@@ -607,42 +705,48 @@ p7_gpu_Pipeline(P7_GPU_PIPELINE *pli, const P7_GPU_OPROFILE *om, const P7_GPU_SE
   g.T3f = p7g_exp_threshold(om->evparam[p7G_FTAU], om->evparam[p7G_FLAMBDA], pli->F3);
   g.do_bias = pli->do_biasfilter; g.do_max = pli->do_max;
 
-  if ((status = reset_counts()) != p7G_OK) return status;
-  HIPCHK(hipMemsetAsync(w.r.filtersc, 0xff, sizeof(float) * (size_t) n, g_stream));   // NaN where not computed
-  HIPCHK(hipMemsetAsync(w.r.vfsc, 0xff, sizeof(float) * (size_t) n, g_stream));
-  HIPCHK(hipMemsetAsync(w.r.fwdsc, 0xff, sizeof(float) * (size_t) n, g_stream));
-  HIPCHK(hipMemsetAsync(w.r.bcksc, 0xff, sizeof(float) * (size_t) n, g_stream));
-  HIPCHK(hipEventRecord(w.ev[0], g_stream));
-  HIPCHK(launch_ssv(*p, s, nullptr, n, w.r.zmax, g_stream));
-  HIPCHK(hipEventRecord(w.ev[6], g_stream));
-  HIPCHK(launch_msv_classify(*p, s, nullptr, n, w.r.zmax, w.r.xj, w.l.full, &w.l.counts[0], lc, g_stream));
-  HIPCHK(launch_msv_full(*p, s, w.l.full, &w.l.counts[0], n, w.r.xj, lc, g_stream));
-  HIPCHK(launch_msv_gate(*p, s, n, w.r.xj, w.r, w.l, g, g_stream));
-  HIPCHK(hipEventRecord(w.ev[1], g_stream));
-  if (g.do_bias) HIPCHK(launch_bias(*p, s, w.l.bias, &w.l.counts[1], n, w.r, w.l, g, 1, lc, g_stream));
-  HIPCHK(hipEventRecord(w.ev[2], g_stream));
-  HIPCHK(launch_vit(*p, s, w.l.vit, &w.l.counts[2], n, w.r, w.l, g, 1, lc, g_stream));
-  HIPCHK(hipEventRecord(w.ev[3], g_stream));
-  HIPCHK(launch_fwd(*p, s, w.l.fwd, &w.l.counts[3], n, w.r, w.l, g, 1, lc, w.pool, w.pool_size, w.scale_off, nullptr, g_stream));
-  HIPCHK(hipEventRecord(w.ev[4], g_stream));
-  HIPCHK(launch_bck(*p, s, w.l.bck, &w.l.counts[4], n, w.r, w.pool, w.scale_off, nullptr, nullptr, lc, g_stream));
-  HIPCHK(hipEventRecord(w.ev[5], g_stream));
+  if ((status = reset_counts(w)) != p7G_OK) return status;
+  HIPCHK(hipMemsetAsync(w.r.filtersc, 0xff, sizeof(float) * (size_t) n, w.st));   // NaN where not computed
+  HIPCHK(hipMemsetAsync(w.r.vfsc, 0xff, sizeof(float) * (size_t) n, w.st));
+  HIPCHK(hipMemsetAsync(w.r.fwdsc, 0xff, sizeof(float) * (size_t) n, w.st));
+  HIPCHK(hipMemsetAsync(w.r.bcksc, 0xff, sizeof(float) * (size_t) n, w.st));
+  HIPCHK(hipEventRecord(w.ev[0], w.st));
+  HIPCHK(launch_ssv(*p, s, nullptr, n, w.r.zmax, w.st));
+  HIPCHK(hipEventRecord(w.ev[6], w.st));
+  HIPCHK(launch_msv_classify(*p, s, nullptr, n, w.r.zmax, w.r.xj, w.l.full, &w.l.counts[0], lc, w.st));
+  HIPCHK(launch_msv_full(*p, s, w.l.full, &w.l.counts[0], n, w.r.xj, lc, w.st));
+  HIPCHK(launch_msv_gate(*p, s, n, w.r.xj, w.r, w.l, g, w.st));
+  HIPCHK(hipEventRecord(w.ev[1], w.st));
+  if (g.do_bias) HIPCHK(launch_bias(*p, s, w.l.bias, &w.l.counts[1], n, w.r, w.l, g, 1, lc, w.st));
+  HIPCHK(hipEventRecord(w.ev[2], w.st));
+  HIPCHK(launch_vit(*p, s, w.l.vit, &w.l.counts[2], n, w.r, w.l, g, 1, lc, w.st));
+  HIPCHK(hipEventRecord(w.ev[3], w.st));
+  HIPCHK(launch_fwd(*p, s, w.l.fwd, &w.l.counts[3], n, w.r, w.l, g, 1, lc, w.pool, w.pool_size, w.scale_off, nullptr, w.st));
+  HIPCHK(hipEventRecord(w.ev[4], w.st));
+  HIPCHK(launch_bck(*p, s, w.l.bck, &w.l.counts[4], n, w.r, w.pool, w.scale_off, nullptr, nullptr, lc, w.st));
+  HIPCHK(hipEventRecord(w.ev[5], w.st));
 
-  // results back to the host
-  std::vector<uint8_t> stage((size_t) n);
-  HIPCHK(hipMemcpyAsync(stage.data(), w.r.stage, (size_t) n, hipMemcpyDeviceToHost, g_stream));
+  // counters on the device, then the requested per-target arrays in the caller's order
+  HIPCHK(hipMemsetAsync(w.stats, 0, 64, w.st));
+  HIPCHK(launch_stats(s, w.r.stage, n, w.l.vit, w.l.fwd, w.l.counts, w.stats, w.st));
+  unsigned long long sums[8];
   int32_t counts[16];
-  HIPCHK(hipMemcpyAsync(counts, w.l.counts, sizeof(counts), hipMemcpyDeviceToHost, g_stream));
+  HIPCHK(hipMemcpyAsync(sums, w.stats, sizeof(sums), hipMemcpyDeviceToHost, w.st));
+  HIPCHK(hipMemcpyAsync(counts, w.l.counts, sizeof(counts), hipMemcpyDeviceToHost, w.st));
+  if (hits && hits->stage) {
+    HIPCHK(launch_scatter8(s, w.r.stage, (uint8_t *) w.outbuf, n, w.st));
+    HIPCHK(hipMemcpyAsync(hits->stage, w.outbuf, (size_t) n, hipMemcpyDeviceToHost, w.st));
+  }
   struct { float *dst; const float *src; } fl[] = {
     { hits ? hits->nullsc : nullptr, w.r.nullsc }, { hits ? hits->usc : nullptr, w.r.usc },
     { hits ? hits->filtersc : nullptr, w.r.filtersc }, { hits ? hits->vfsc : nullptr, w.r.vfsc },
     { hits ? hits->fwdsc : nullptr, w.r.fwdsc }, { hits ? hits->bcksc : nullptr, w.r.bcksc } };
-  std::vector<float> tmp[6];
   for (int a = 0; a < 6; a++) if (fl[a].dst) {
-    tmp[a].resize((size_t) n);
-    HIPCHK(hipMemcpyAsync(tmp[a].data(), fl[a].src, sizeof(float) * (size_t) n, hipMemcpyDeviceToHost, g_stream));
+    // stream order keeps the staging buffer's next use behind this copy
+    HIPCHK(launch_scatter32(s, fl[a].src, w.outbuf, n, w.st));
+    HIPCHK(hipMemcpyAsync(fl[a].dst, w.outbuf, sizeof(float) * (size_t) n, hipMemcpyDeviceToHost, w.st));
   }
-  HIPCHK(hipStreamSynchronize(g_stream));
+  HIPCHK(hipStreamSynchronize(w.st));
   if (counts[6]) { p7g_set_error("Forward scale-factor pool exhausted"); return p7G_EMEM; }
 
   P7_GPU_STATS &st = pli->st;
@@ -653,39 +757,17 @@ p7_gpu_Pipeline(P7_GPU_PIPELINE *pli, const P7_GPU_OPROFILE *om, const P7_GPU_SE
   HIPCHK(hipEventElapsedTime(&mtot, w.ev[0], w.ev[6]));
   st.ms_ssv = mtot;
   st.ms_msv = ms[0]; st.ms_bias = ms[1]; st.ms_vit = ms[2]; st.ms_fwd = ms[3]; st.ms_bck = ms[4];
-  int64_t pm = 0, pb = 0, pv = 0, pf = 0;
-  double cb = 0, cv = 0, cf = 0, ck = 0;
-  for (int64_t slot = 0; slot < n; slot++) {
-    const int sg = stage[slot];
-    const double cells = (double) db->h_len[slot] * om->M;
-    if (sg >= 1) { pm++; cb += db->h_len[slot]; }
-    if (sg >= 2) pb++;
-    if (sg >= 3) pv++;
-    if (sg >= 4) { pf++; ck += cells; }
-    (void) cells;
-  }
-  // cells of the list stages: Viterbi ran on targets that reached it without the F2 shortcut
-  st.cells_msv  = (double) db->nres * om->M;
-  st.cells_bias = cb;
   st.nmodels += 1; st.nseqs += n; st.nres += db->nres; st.nnodes += om->M;
-  st.n_past_msv += pm; st.n_past_bias += pb; st.n_past_vit += pv; st.n_past_fwd += pf;
+  st.n_past_msv += (int64_t) sums[0]; st.n_past_bias += (int64_t) sums[1];
+  st.n_past_vit += (int64_t) sums[2]; st.n_past_fwd += (int64_t) sums[3];
   st.n_msv_full = counts[0];
-  {
-    // Viterbi and Forward cells need the lists' residue sums
-    std::vector<int32_t> lv((size_t) counts[2]), lf((size_t) counts[3]);
-    if (counts[2]) HIPCHK(hipMemcpy(lv.data(), w.l.vit, sizeof(int32_t) * (size_t) counts[2], hipMemcpyDeviceToHost));
-    if (counts[3]) HIPCHK(hipMemcpy(lf.data(), w.l.fwd, sizeof(int32_t) * (size_t) counts[3], hipMemcpyDeviceToHost));
-    for (int32_t sl : lv) cv += db->h_len[sl];
-    for (int32_t sl : lf) cf += db->h_len[sl];
-  }
-  st.cells_vit = cv * om->M; st.cells_fwd = cf * om->M; st.cells_bck = ck;
-
-  if (hits && hits->stage) for (int64_t slot = 0; slot < n; slot++) hits->stage[db->h_perm[slot]] = stage[slot];
-  for (int a = 0; a < 6; a++) if (fl[a].dst) {
-    float *dst = fl[a].dst;
-    const float *srcv = tmp[a].data();
-    for (int64_t slot = 0; slot < n; slot++) dst[db->h_perm[slot]] = srcv[slot];
-  }
+  // cells of the last call: MSV over every target, the bias filter over MSV
+  // survivors (when on), Viterbi and Forward over their lists, Backward over F3 survivors
+  st.cells_msv  = (double) db->nres * om->M;
+  st.cells_bias = pli->do_biasfilter ? (double) sums[4] : 0.0;
+  st.cells_vit  = (double) sums[6] * om->M;
+  st.cells_fwd  = (double) sums[7] * om->M;
+  st.cells_bck  = (double) sums[5] * om->M;
   return p7G_OK;
 }
 
@@ -750,12 +832,12 @@ single_msv(const uint8_t *dsq, int L, const P7_GPU_OPROFILE *om, int *ret_xj)
   int status;
   DevProfile *p;
   if ((status = dev_profile(om, L, &p)) != p7G_OK) return status;
-  if ((status = work_reserve(1, 0)) != p7G_OK) return status;
+  if ((status = work_reserve(g_work, 1, 0)) != p7G_OK) return status;
   DevSeqs s;
   if ((status = single_stage(dsq, L, &s)) != p7G_OK) return status;
   Work &w = g_work;
   LenCfg lc = lencfg_of(om);
-  if ((status = reset_counts()) != p7G_OK) return status;
+  if ((status = reset_counts(g_work)) != p7G_OK) return status;
   HIPCHK(launch_ssv(*p, s, nullptr, 1, w.r.zmax, g_stream));
   HIPCHK(launch_msv_classify(*p, s, nullptr, 1, w.r.zmax, w.r.xj, w.l.full, &w.l.counts[0], lc, g_stream));
   HIPCHK(launch_msv_full(*p, s, w.l.full, &w.l.counts[0], 1, w.r.xj, lc, g_stream));
@@ -812,12 +894,12 @@ p7_gpu_ViterbiFilter(const uint8_t *dsq, int L, const P7_GPU_OPROFILE *om, P7_GP
     return p7G_OK;
   }
   if ((status = dev_profile(om, L, &p)) != p7G_OK) return status;
-  if ((status = work_reserve(1, 0)) != p7G_OK) return status;
+  if ((status = work_reserve(g_work, 1, 0)) != p7G_OK) return status;
   DevSeqs s;
   if ((status = single_stage(dsq, L, &s)) != p7G_OK) return status;
   Work &w = g_work;
   GateThresh g; memset(&g, 0, sizeof(g));
-  if ((status = reset_counts()) != p7G_OK) return status;
+  if ((status = reset_counts(g_work)) != p7G_OK) return status;
   int32_t zero = 0, one = 1;
   HIPCHK(hipMemcpyAsync(w.l.vit, &zero, sizeof(zero), hipMemcpyHostToDevice, g_stream));
   HIPCHK(hipMemcpyAsync(&w.l.counts[2], &one, sizeof(one), hipMemcpyHostToDevice, g_stream));
@@ -862,13 +944,13 @@ p7_gpu_ForwardParser(const uint8_t *dsq, int L, const P7_GPU_OPROFILE *om, P7_GP
     return p7G_OK;
   }
   if ((status = dev_profile(om, L, &p)) != p7G_OK) return status;
-  if ((status = work_reserve(1, 0)) != p7G_OK) return status;
+  if ((status = work_reserve(g_work, 1, 0)) != p7G_OK) return status;
   if ((status = single_xmx(L)) != p7G_OK) return status;
   DevSeqs s;
   if ((status = single_stage(dsq, L, &s)) != p7G_OK) return status;
   Work &w = g_work;
   GateThresh g; memset(&g, 0, sizeof(g));
-  if ((status = reset_counts()) != p7G_OK) return status;
+  if ((status = reset_counts(g_work)) != p7G_OK) return status;
   int32_t zero = 0, one = 1;
   HIPCHK(hipMemcpyAsync(w.l.fwd, &zero, sizeof(zero), hipMemcpyHostToDevice, g_stream));
   HIPCHK(hipMemcpyAsync(&w.l.counts[3], &one, sizeof(one), hipMemcpyHostToDevice, g_stream));
@@ -900,12 +982,12 @@ p7_gpu_BackwardParser(const uint8_t *dsq, int L, const P7_GPU_OPROFILE *om, cons
   bck->M = om->M; bck->L = L; bck->has_own_scales = 0;
   if (L == 0) { p7g_set_error("backward score underflow on an empty sequence"); return p7G_ERANGE; }
   if ((status = dev_profile(om, L, &p)) != p7G_OK) return status;
-  if ((status = work_reserve(1, 0)) != p7G_OK) return status;
+  if ((status = work_reserve(g_work, 1, 0)) != p7G_OK) return status;
   if ((status = single_xmx(L)) != p7G_OK) return status;
   DevSeqs s;
   if ((status = single_stage(dsq, L, &s)) != p7G_OK) return status;
   Work &w = g_work;
-  if ((status = reset_counts()) != p7G_OK) return status;
+  if ((status = reset_counts(g_work)) != p7G_OK) return status;
   int32_t zero = 0, one = 1;
   float *d_fx = g_one.xmx, *d_bx = g_one.xmx + (size_t) (L + 1) * 6;
   HIPCHK(hipMemcpyAsync(d_fx, fwd->xmx, sizeof(float) * (size_t) (L + 1) * 6, hipMemcpyHostToDevice, g_stream));
@@ -952,7 +1034,7 @@ p7_gpu_bg_FilterScore(P7_GPU_BG *bg, const uint8_t *dsq, int L, float *ret_sc)
     HIPCHK(hipMalloc((void **) &d_eo, sizeof(float) * 64));
     HIPCHK(hipMemcpyAsync(d_eo, eo.data(), sizeof(float) * 64, hipMemcpyHostToDevice, g_stream));
     p.eo = d_eo; p.ft1_0 = bg->ft[1][0]; p.ft1_1 = bg->ft[1][1]; p.pi0 = bg->pi[0]; p.pi1 = bg->pi[1];
-    if ((status = work_reserve(1, 0)) != p7G_OK) { (void) hipFree(d_eo); return status; }
+    if ((status = work_reserve(g_work, 1, 0)) != p7G_OK) { (void) hipFree(d_eo); return status; }
     DevSeqs s;
     if ((status = single_stage(dsq, L, &s)) != p7G_OK) { (void) hipFree(d_eo); return status; }
     Work &w = g_work;

@@ -129,8 +129,15 @@ float  p7g_exp_threshold(double mu, double lambda, double P);
 /* error text */
 void p7g_set_error(const char *fmt, ...);
 
-/* device layer (p7g_device.hip) */
-void p7g_dev_profile_free(struct p7g_dev_profile_s *d);
+/* device layer (hip/p7g_runtime.hip) */
+void   p7g_dev_profile_free(struct p7g_dev_profile_s *d);
+int    p7g_current_device(void);
+int    p7g_device_free_bytes(size_t *ret_bytes);
+/* Device bytes one pipeline's work area takes for a database of this size. */
+size_t p7g_pipeline_work_bytes(int64_t nseq, int64_t nres);
+/* Creates the pipeline's stream and allocates its work area for `db` now. */
+int    p7g_pipeline_Reserve(P7_GPU_PIPELINE *pli, const P7_GPU_SEQDB *db);
+void   p7g_pipeline_ResetStats(P7_GPU_PIPELINE *pli);
 
 #ifdef __cplusplus
 }

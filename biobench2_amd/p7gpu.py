@@ -97,6 +97,11 @@ def _load():
         "p7_gpu_Pipeline": (i, [vp, vp, vp, POINTER(Hits)]),
         "p7_gpu_pipeline_GetStats": (i, [vp, POINTER(Stats)]),
         "p7_gpu_ScoreThreshold": (f, [i, d, d, d]),
+        "p7_gpu_server_Create": (vp, [vp, i, ctypes.c_size_t, d, d, d, i]),
+        "p7_gpu_server_Slots": (i, [vp]),
+        "p7_gpu_server_Submit": (i, [vp, vp, POINTER(Hits), POINTER(c_int64)]),
+        "p7_gpu_server_Wait": (i, [vp, c_int64, POINTER(Stats), POINTER(c_double)]),
+        "p7_gpu_server_Destroy": (None, [vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -414,6 +419,54 @@ def filter_score_batch(om, db, idx=None):
     return _batch(lib.p7_gpu_FilterScoreBatch, om, db, idx, "FilterScoreBatch", with_status=False)
 
 
+def _hits(n, scores):
+    out = {"stage": np.zeros(n, np.uint8)}
+    hits = Hits()
+    hits.stage = _ptr(out["stage"], c_uint8)
+    if scores:
+        for k in ("nullsc", "usc", "filtersc", "vfsc", "fwdsc", "bcksc"):
+            out[k] = np.zeros(n, np.float32)
+            setattr(hits, k, _ptr(out[k], c_float))
+    return out, hits
+
+
+class Server:
+    """Queries against a resident database, admitted as they arrive (p7_gpu_server_*)."""
+
+    def __init__(self, db, nslots=4, hbm_budget=0, F1=0.02, F2=1e-3, F3=1e-5, do_biasfilter=True):
+        self.db = db
+        self.h = lib.p7_gpu_server_Create(db.h, int(nslots), int(hbm_budget), F1, F2, F3, int(do_biasfilter))
+        if not self.h:
+            raise P7GpuError(EMEM, "server_Create")
+        self.pending = {}
+
+    @property
+    def slots(self):
+        return lib.p7_gpu_server_Slots(self.h)
+
+    def submit(self, om, scores=True):
+        out, hits = _hits(self.db.nseq, scores)
+        t = c_int64()
+        _check(lib.p7_gpu_server_Submit(self.h, om.h, ctypes.byref(hits), ctypes.byref(t)), "server_Submit")
+        self.pending[t.value] = (om, out, hits)      # keeps the arrays and the profile alive
+        return t.value
+
+    def wait(self, ticket):
+        st, lat = Stats(), c_double()
+        status = lib.p7_gpu_server_Wait(self.h, ticket, ctypes.byref(st), ctypes.byref(lat))
+        om, out, hits = self.pending.pop(ticket)
+        _check(status, "server_Wait")
+        return out, st.as_dict(), lat.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib.p7_gpu_server_Destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+
 class Pipeline:
     """P7_PIPELINE thresholds and counters; run() is p7_Pipeline over a whole database."""
 
@@ -421,14 +474,7 @@ class Pipeline:
         self.h = lib.p7_gpu_pipeline_Create(F1, F2, F3, int(do_biasfilter), int(do_max))
 
     def run(self, om, db, scores=True):
-        n = db.nseq
-        out = {"stage": np.zeros(n, np.uint8)}
-        hits = Hits()
-        hits.stage = _ptr(out["stage"], c_uint8)
-        if scores:
-            for k in ("nullsc", "usc", "filtersc", "vfsc", "fwdsc", "bcksc"):
-                out[k] = np.zeros(n, np.float32)
-                setattr(hits, k, _ptr(out[k], c_float))
+        out, hits = _hits(db.nseq, scores)
         _check(lib.p7_gpu_Pipeline(self.h, om.h, db.h, ctypes.byref(hits)), "Pipeline")
         return out
 

@@ -228,6 +228,27 @@ int   p7_gpu_pipeline_GetStats(const P7_GPU_PIPELINE *pli, P7_GPU_STATS *st);
  * "P > F: reject" (p7_pipeline.c:692, 702, 720, 729) is "s < threshold". */
 float p7_gpu_ScoreThreshold(int which, double mu, double lambda, double P);
 
+/* ---- serving: queries against a resident database, admitted as they arrive ----
+ * Stands in for hmmpgmd's search loop (HMMER/src/hmmdmstr.c:701-790,
+ * hmmdwrkr.c:535-644), which serves one query at a time: here up to `nslots`
+ * queries run at once, each in its own pipeline (stream and device work
+ * area), taken from a FIFO as slots free up. The slot count is cut to what
+ * `hbm_budget` bytes of device memory hold (0: 90% of the free memory).
+ * Each query's result equals a lone p7_gpu_Pipeline call with the same
+ * thresholds. Every call may come from any host thread. */
+typedef struct p7g_server_s P7_GPU_SERVER;
+P7_GPU_SERVER *p7_gpu_server_Create(const P7_GPU_SEQDB *db, int nslots, size_t hbm_budget,
+                                    double F1, double F2, double F3, int do_biasfilter);
+int   p7_gpu_server_Slots(const P7_GPU_SERVER *srv);
+/* Queues a query and returns at once; `om` and `hits` (may be NULL) must stay
+ * valid until the matching Wait returns. */
+int   p7_gpu_server_Submit(P7_GPU_SERVER *srv, const P7_GPU_OPROFILE *om, P7_GPU_HITS *hits, int64_t *ret_ticket);
+/* Blocks until the query is done: its pipeline counters, and the time from
+ * Submit to completion in milliseconds. Returns the query's status. */
+int   p7_gpu_server_Wait(P7_GPU_SERVER *srv, int64_t ticket, P7_GPU_STATS *ret_stats, double *ret_latency_ms);
+/* Finishes the queued queries, then frees the slots. */
+void  p7_gpu_server_Destroy(P7_GPU_SERVER *srv);
+
 #ifdef __cplusplus
 }
 #endif

@@ -46,8 +46,15 @@ def test_pipeline_config1_shape(gpu):
     got, ref, st, db = run_both(gpu, "synth_M200", 20000, 0.01)
     check(got, ref)
     stage = got["stage"]
-    assert st["n_past_msv"] == int((stage >= 1).sum()) and st["n_past_fwd"] == int((stage >= 4).sum())
+    for k, s in (("n_past_msv", 1), ("n_past_bias", 2), ("n_past_vit", 3), ("n_past_fwd", 4)):
+        assert st[k] == int((stage >= s).sum()), k
     assert st["nseqs"] == db.nseq and st["nres"] == db.nres
+    L = db.lengths.astype(np.int64)
+    assert st["cells_bias"] == float(L[stage >= 1].sum())
+    assert st["cells_bck"] == float(L[stage == 4].sum()) * 200
+    # Viterbi runs on bias survivors that did not take the F2 shortcut: they have a Viterbi score
+    assert st["cells_vit"] == float(L[~np.isnan(got["vfsc"])].sum()) * 200
+    assert st["cells_fwd"] == float(L[stage >= 3].sum()) * 200
     assert int((stage >= 4).sum()) >= int(0.9 * (db.homolog >= 0).sum())      # planted homologs are found
 
 
