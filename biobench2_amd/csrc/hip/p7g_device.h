@@ -78,7 +78,10 @@ struct DevResults {
 // host round trips.
 struct DevLists {
   int32_t *full, *bias, *vit, *fwd, *bck;
-  int32_t *counts;     // [0] full, [1] bias, [2] vit, [3] fwd, [4] bck, [6] scale-pool overflow flag
+  // [0] full, [1] bias, [2] vit, [3] fwd, [4] bck: list lengths; [6] scale-pool
+  // overflow flag; the work-queue cursor of list k sits at [k + 8]; [13] is the
+  // cursor of the ungapped pass over all targets. Zeroed before every call.
+  int32_t *counts;
   unsigned long long *cursor;   // next free float in the Forward scale-factor pool
 };
 
@@ -103,8 +106,10 @@ struct GateThresh {
 #ifdef __HIPCC__
 #include <hip/hip_runtime.h>
 // launchers (p7g_kernels.hip)
-hipError_t launch_ssv(const DevProfile &p, const DevSeqs &s, const int32_t *slots, int64_t n, uint16_t *zmax,
+hipError_t launch_ssv(const DevProfile &p, const DevSeqs &s, const int32_t *slots, int64_t n, uint16_t *zmax, int32_t *queue,
                       hipStream_t st);
+// With slots == NULL (every slot) targets that need the full recurrence are
+// marked xj = -2 for launch_compact_i16; with a slot list they are appended.
 hipError_t launch_msv_classify(const DevProfile &p, const DevSeqs &s, const int32_t *slots, int64_t n,
                                const uint16_t *zmax, int16_t *xj, int32_t *full_list, int32_t *full_count,
                                LenCfg lc, hipStream_t st);
@@ -119,6 +124,16 @@ hipError_t launch_vit(const DevProfile &p, const DevSeqs &s, const int32_t *list
 hipError_t launch_fwd(const DevProfile &p, const DevSeqs &s, const int32_t *list, const int32_t *count, int64_t maxn,
                       DevResults r, DevLists l, GateThresh g, int gating, LenCfg lc,
                       float *scale_pool, int64_t pool_size, int64_t *scale_off, float *xmx_out, hipStream_t st);
+#define P7G_Q_SSV 13        // counts[] index of the ungapped pass's work-queue cursor
+#define P7G_QOFF  8         // a list's work-queue cursor is counts[list + P7G_QOFF]
+
+// Order-preserving compaction: list = every slot s (ascending) with
+// flags[s] == value (uint8) or xj[s] == value (int16); *count = its length.
+// tilecnt needs (n + 4095) / 4096 ints.
+hipError_t launch_compact_u8(const uint8_t *flags, int value, int64_t n, int32_t *list, int32_t *count,
+                             int32_t *tilecnt, hipStream_t st);
+hipError_t launch_compact_i16(const int16_t *flags, int value, int64_t n, int32_t *list, int32_t *count,
+                              int32_t *tilecnt, hipStream_t st);
 // Pipeline counters: out[0..3] targets with stage >= 1..4, out[4] residues with
 // stage >= 1, out[5] residues with stage 4, out[6] and out[7] residues over the
 // Viterbi and Forward lists. out[] must be zero on entry.

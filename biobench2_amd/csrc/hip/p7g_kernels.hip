@@ -131,6 +131,16 @@ __device__ __forceinline__ void list_append(bool pred, int32_t val, int32_t *__r
   if (pred) list[base + __popcll(m & ((1ull << lane) - 1ull))] = val;
 }
 
+// Next item of a device work queue shared by the waves of one launch
+// (wave-uniform). Lists are in slot order, longest target first, so waves
+// taking items in queue order schedule the longest work first.
+__device__ __forceinline__ int queue_next(int32_t *q, int step)
+{
+  int v = 0;
+  if (__lane_id() == 0) v = atomicAdd(q, step);
+  return uni(__shfl(v, 0, 64));
+}
+
 // Wave-uniform residue readers. A target's residues start on a 16-byte boundary
 // and every target is followed by at least 16 readable bytes (its padding or
 // the next target, and 16 spare bytes after the last one), so the chunk after
@@ -141,12 +151,18 @@ __device__ __forceinline__ int chunk_byte(const uint4 &c, int i)
   const unsigned w = (q == 0) ? c.x : (q == 1) ? c.y : (q == 2) ? c.z : c.w;
   return (int) ((w >> ((i & 3) * 8)) & 0xffu);
 }
+// The chunk in use is moved to scalar registers; only the one in flight
+// occupies vector registers.
+__device__ __forceinline__ uint4 uni4(const uint4 &v)
+{
+  return make_uint4((unsigned) uni((int) v.x), (unsigned) uni((int) v.y), (unsigned) uni((int) v.z), (unsigned) uni((int) v.w));
+}
 struct ResAsc {                       // i = 0, 1, 2, ...
   const uint4 *p; uint4 cur, nxt; int c;
-  __device__ __forceinline__ void init(const uint8_t *sq) { p = (const uint4 *) sq; c = 0; cur = p[0]; nxt = p[1]; }
+  __device__ __forceinline__ void init(const uint8_t *sq) { p = (const uint4 *) sq; c = 0; cur = uni4(p[0]); nxt = p[1]; }
   __device__ __forceinline__ int at(int i)
   {
-    if ((i >> 4) != c) { c = i >> 4; cur = nxt; nxt = p[c + 1]; }
+    if ((i >> 4) != c) { c = i >> 4; cur = uni4(nxt); nxt = p[c + 1]; }
     return chunk_byte(cur, i);
   }
 };
@@ -154,11 +170,11 @@ struct ResDesc {                      // i = L-1, L-2, ..., 0
   const uint4 *p; uint4 cur, nxt; int c;
   __device__ __forceinline__ void init(const uint8_t *sq, int L)
   {
-    p = (const uint4 *) sq; c = (L - 1) >> 4; cur = p[c]; nxt = p[c > 0 ? c - 1 : 0];
+    p = (const uint4 *) sq; c = (L - 1) >> 4; cur = uni4(p[c]); nxt = p[c > 0 ? c - 1 : 0];
   }
   __device__ __forceinline__ int at(int i)
   {
-    if ((i >> 4) != c) { c = i >> 4; cur = nxt; nxt = p[c > 0 ? c - 1 : 0]; }
+    if ((i >> 4) != c) { c = i >> 4; cur = uni4(nxt); nxt = p[c > 0 ? c - 1 : 0]; }
     return chunk_byte(cur, i);
   }
 };
@@ -189,7 +205,7 @@ template <int G, int ND>
 __global__ __launch_bounds__(SSV_BLOCK)
 void k_ssv(const int *__restrict__ tbl, int tbl_dwords,
            const uint8_t *__restrict__ res, const int64_t *__restrict__ off, const int32_t *__restrict__ len,
-           const int32_t *__restrict__ slots, int64_t n, uint16_t *__restrict__ zmax)
+           const int32_t *__restrict__ slots, int64_t n, uint16_t *__restrict__ zmax, int32_t *queue)
 {
   constexpr int NQ   = ND / 4;                        // 16-byte reads per row
   constexpr int NT   = ND % 4;                        // 4-byte reads per row
@@ -206,7 +222,9 @@ void k_ssv(const int *__restrict__ tbl, int tbl_dwords,
   const int4 *qrow = (const int4 *) __builtin_assume_aligned(lds4, 16) + z;     // + (x*NQ + q)*G
   const int  *trow = (const int *) lds4 + TAIL0 + z;                           // + (x*NT + t)*G
 
-  for (int64_t w = wave_id(); w * TPW < n; w += wave_cnt()) {
+  for (;;) {
+    const int64_t w = queue_next(queue, 1);           // one group of TPW targets per grab
+    if (w * TPW >= n) break;
     const int64_t t = w * TPW + grp;
     const bool valid = t < n;
     const int64_t slot = valid ? (slots ? (int64_t) slots[t] : t) : 0;
@@ -307,8 +325,11 @@ __global__ void k_msv_classify(const DevProfile p, const int32_t *__restrict__ l
     else if (xB + zm + p.bias >= 255)     v = -1;          // some row overflows (MSV values only grow with xB)
     else if (xB + zm > p.base + p.tec)    v = -2;          // the J state would raise xB
     else                                  v = max(xB + zm - p.tec, 0);
-    list_append(v == -2, (int32_t) slot, full_list, full_count);
-    xj[slot] = (int16_t) (v == -2 ? 0 : v);
+    if (slots) {                                   // explicit slot list: append here
+      list_append(v == -2, (int32_t) slot, full_list, full_count);
+      if (v == -2) v = 0;
+    }
+    xj[slot] = (int16_t) v;                        // -2 marks "full recurrence" for the compaction
   }
 }
 
@@ -326,8 +347,9 @@ void k_msv_full(const DevProfile p, const int *__restrict__ tbl, int tbl_dwords,
   stage_table(lds, tbl, (size_t) tbl_dwords);
   const int z = threadIdx.x & 63;
   const int64_t n = *count;
+  int32_t *queue = (int32_t *) count + P7G_QOFF;
 
-  for (int64_t t = wave_id(); t < n; t += wave_cnt()) {
+  for (int64_t t = queue_next(queue, 1); t < n; t = queue_next(queue, 1)) {
     const int slot = uni(list[t]);
     const int L = uni(len[slot]);
     const uint8_t *sq = res + off[slot];
@@ -397,9 +419,7 @@ __global__ void k_msv_gate(const DevProfile p, const int32_t *__restrict__ len, 
         }
       }
     }
-    list_append(to_bias, (int32_t) slot, l.bias, &l.counts[1]);
-    list_append(to_vit,  (int32_t) slot, l.vit,  &l.counts[2]);
-    list_append(to_fwd,  (int32_t) slot, l.fwd,  &l.counts[3]);
+    (void) to_bias; (void) to_vit; (void) to_fwd;   // the next lists are compacted from stage[]
     r.stage[slot] = stage;
   }
 }
@@ -414,6 +434,10 @@ __global__ void k_bias(const DevProfile p, const uint8_t *__restrict__ res, cons
                        DevResults r, DevLists l, GateThresh g, int gating, LenCfg lc)
 {
 #pragma clang fp contract(off)
+  // the two states' odds of every residue code, in LDS
+  __shared__ float2 eo2[P7G_NROWS];
+  for (int x = threadIdx.x; x < P7G_NROWS; x += blockDim.x) eo2[x] = make_float2(p.eo[2*x], p.eo[2*x + 1]);
+  __syncthreads();
   const int64_t n = *count;
   for (int64_t t = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (int64_t) gridDim.x * blockDim.x) {
     const int slot = list[t];
@@ -425,24 +449,30 @@ __global__ void k_bias(const DevProfile p, const uint8_t *__restrict__ res, cons
     float logsc = 0.0f;
     float d0, d1, mx;
     // residues 16 at a time, the next chunk fetched ahead (targets are 16-byte
-    // aligned and followed by at least 16 readable bytes)
+    // aligned and followed by at least 16 readable bytes); the odds of row
+    // i+1 are read while row i computes
     const uint4 *pc = (const uint4 *) sq;
     uint4 cur = pc[0], nxt = pc[1];
-    int x = chunk_byte(cur, 0);
-    d0 = p.eo[x*2 + 0] * p.pi0;
-    d1 = p.eo[x*2 + 1] * p.pi1;
+    const float2 e0 = eo2[chunk_byte(cur, 0)];
+    float2 en = eo2[L > 1 ? chunk_byte(cur, 1) : 0];
+    d0 = e0.x * p.pi0;
+    d1 = e0.y * p.pi1;
     mx = 0.0f;
     mx = (d0 > mx) ? d0 : mx;
     mx = (d1 > mx) ? d1 : mx;
     d0 = __fdiv_rn(d0, mx); d1 = __fdiv_rn(d1, mx);
     logsc += (float) log((double) mx);
     for (int i = 1; i < L; i++) {
-      if ((i & 15) == 0) { cur = nxt; nxt = pc[(i >> 4) + 1]; }
-      x = chunk_byte(cur, i);
+      const float2 e = en;
+      const int j = i + 1;
+      if (j < L) {
+        if ((j & 15) == 0) { cur = nxt; nxt = pc[(j >> 4) + 1]; }
+        en = eo2[chunk_byte(cur, j)];
+      }
       float n0 = 0.0f, n1 = 0.0f;
-      n0 += d0 * t00; n0 += d1 * t10; n0 *= p.eo[x*2 + 0];
+      n0 += d0 * t00; n0 += d1 * t10; n0 *= e.x;
       mx = 0.0f; mx = (n0 > mx) ? n0 : mx;
-      n1 += d0 * t01; n1 += d1 * t11; n1 *= p.eo[x*2 + 1];
+      n1 += d0 * t01; n1 += d1 * t11; n1 *= e.y;
       mx = (n1 > mx) ? n1 : mx;
       d0 = __fdiv_rn(n0, mx); d1 = __fdiv_rn(n1, mx);
       logsc += (float) log((double) mx);
@@ -460,8 +490,7 @@ __global__ void k_bias(const DevProfile p, const uint8_t *__restrict__ res, cons
       else if (seq >= g.T2m) { to_fwd = true; r.stage[slot] = 3; }
       else                   { to_vit = true; r.stage[slot] = 2; }
     }
-    list_append(to_vit, slot, l.vit, &l.counts[2]);
-    list_append(to_fwd, slot, l.fwd, &l.counts[3]);
+    (void) to_vit; (void) to_fwd;                    // the next lists are compacted from stage[]
   }
 }
 
@@ -561,7 +590,8 @@ void k_vit(const DevProfile p, const uint8_t *__restrict__ res, const int64_t *_
   dd_setup<ND>(ddc, tr + VT_DDS*rowd, z);
   const int64_t n = *count;
 
-  for (int64_t t = wave_id(); t < n; t += wave_cnt()) {
+  int32_t *queue = (int32_t *) count + P7G_QOFF;
+  for (int64_t t = queue_next(queue, 1); t < n; t = queue_next(queue, 1)) {
     const int slot = uni(list[t]);
     const int L = uni(len[slot]);
     const uint8_t *sq = res + off[slot];
@@ -627,7 +657,7 @@ void k_vit(const DevProfile p, const uint8_t *__restrict__ res, const int64_t *_
       r.vstatus[slot] = status;
       if (gating) {
         const float seq = (float) __ddiv_rn((double) (sc - r.filtersc[slot]), LOG2_D);
-        if (seq >= g.T2v) { int at = atomicAdd(&l.counts[3], 1); l.fwd[at] = slot; r.stage[slot] = 3; }
+        if (seq >= g.T2v) r.stage[slot] = 3;
         else r.stage[slot] = 2;
       }
     }
@@ -698,15 +728,16 @@ void k_fwd(const DevProfile p, const uint8_t *__restrict__ res, const int64_t *_
   if (LDS) { stage_table(ldsf, p.fb_tbl, p.fb_floats); tbl = ldsf; }
   const int z = threadIdx.x & 63;
   const float *tr = tbl + (size_t) p.nrows * rowf + z;
-  float tBM[B], tMM[B], tIM[B], tDM[B], tMDS[B], tMI[B], tII[B], tDDS[B];
+  float tBM[B], tMM[B], tIM[B], tDM[B], tMDS[B], tMI[B], tII[B];
 #pragma unroll
   for (int j = 0; j < B; j++) {
     tBM[j] = tr[VT_BM*rowf + j*64]; tMM[j] = tr[VT_MM*rowf + j*64]; tIM[j] = tr[VT_IM*rowf + j*64]; tDM[j] = tr[VT_DM*rowf + j*64];
-    tMDS[j] = tr[VT_MDS*rowf + j*64]; tMI[j] = tr[VT_MI*rowf + j*64]; tII[j] = tr[VT_II*rowf + j*64]; tDDS[j] = tr[VT_DDS*rowf + j*64];
+    tMDS[j] = tr[VT_MDS*rowf + j*64]; tMI[j] = tr[VT_MI*rowf + j*64]; tII[j] = tr[VT_II*rowf + j*64];
   }
   const int64_t n = *count;
 
-  for (int64_t t = wave_id(); t < n; t += wave_cnt()) {
+  int32_t *queue = (int32_t *) count + P7G_QOFF;
+  for (int64_t t = queue_next(queue, 1); t < n; t = queue_next(queue, 1)) {
     const int slot = uni(list[t]);
     const int L = uni(len[slot]);
     const uint8_t *sq = res + off[slot];
@@ -749,7 +780,14 @@ void k_fwd(const DevProfile p, const uint8_t *__restrict__ res, const int64_t *_
       const float mS = dppf<WAVE_SHR1>(0.0f, M[B-1]);
 #pragma unroll
       for (int j = 0; j < B; j++) D[j] = (j ? M[j-1] : mS) * tMDS[j];
-      affine_fwd_scan<B>(D, tDDS, z);
+      {
+        // the D->D odds are read per row, not pinned in registers: this keeps
+        // the kernel at 128 VGPRs, four waves per SIMD
+        float tDDS[B];
+#pragma unroll
+        for (int j = 0; j < B; j++) tDDS[j] = tr[VT_DDS*rowf + j*64];
+        affine_fwd_scan<B>(D, tDDS, z);
+      }
 #pragma unroll
       for (int j = 0; j < B; j++) xEl += D[j];
       xE = unif(wave_sum_f(xEl));
@@ -783,7 +821,7 @@ void k_fwd(const DevProfile p, const uint8_t *__restrict__ res, const int64_t *_
       r.fstatus[slot] = status;
       if (gating) {
         const float seq = (float) __ddiv_rn((double) (sc - r.filtersc[slot]), LOG2_D);
-        if (status == 0 && seq >= g.T3f) { int at = atomicAdd(&l.counts[4], 1); l.bck[at] = slot; r.stage[slot] = 4; }
+        if (status == 0 && seq >= g.T3f) r.stage[slot] = 4;
         else r.stage[slot] = 3;
       }
     }
@@ -815,7 +853,8 @@ void k_bck(const DevProfile p, const uint8_t *__restrict__ res, const int64_t *_
   }
   const int64_t n = *count;
 
-  for (int64_t t = wave_id(); t < n; t += wave_cnt()) {
+  int32_t *queue = (int32_t *) count + P7G_QOFF;
+  for (int64_t t = queue_next(queue, 1); t < n; t = queue_next(queue, 1)) {
     const int slot = uni(list[t]);
     const int L = uni(len[slot]);
     const uint8_t *sq = res + off[slot];
@@ -948,14 +987,20 @@ static hipError_t launch(void *k, unsigned grid, unsigned block, void **args, si
   return hipLaunchKernel(k, dim3(grid), dim3(block), args, lds, st);
 }
 
-// Persistent-style grid of blocks of `wpb` waves: enough to fill every CU as
-// far as LDS allows, never more than the work needs.
-static unsigned grid_for_waves_blk(int64_t nwaves, size_t lds_bytes, int wpb)
+// Persistent grid: as many blocks of `wpb` waves as can be resident at once
+// (registers, LDS and wave slots, from the occupancy calculator), never more
+// than the work needs. A block that could not be resident would only start
+// when another finishes and run its share alone at the end.
+static unsigned grid_for_waves_blk(const void *kernel, int64_t nwaves, size_t lds_bytes, int wpb)
 {
   dev_props();
-  int per_cu = 32 / wpb;                                   // 32 waves per CU at most
-  if (lds_bytes > 0) { int byl = (int) ((size_t) g_lds_cu / lds_bytes); if (byl < per_cu) per_cu = byl; }
-  if (per_cu < 1) per_cu = 1;
+  int per_cu = 0;
+  if (lds_bytes > 64 * 1024) (void) hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds_bytes);
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, wpb * 64, lds_bytes) != hipSuccess || per_cu < 1) {
+    per_cu = 32 / wpb;
+    if (lds_bytes > 0) { int byl = (int) ((size_t) g_lds_cu / lds_bytes); if (byl < per_cu) per_cu = byl; }
+    if (per_cu < 1) per_cu = 1;
+  }
   int64_t g = (int64_t) ncu() * per_cu;
   int64_t need = (nwaves + wpb - 1) / wpb;
   if (need < g) g = need;
@@ -967,7 +1012,7 @@ static unsigned grid_for_waves_blk(int64_t nwaves, size_t lds_bytes, int wpb)
 #define SSV16_CASE(N) case N: k = (void *) k_ssv<16, N>; break;
 #define SSV64_CASE(N) case N: k = (void *) k_ssv<64, N>; break;
 
-hipError_t launch_ssv(const DevProfile &p, const DevSeqs &s, const int32_t *slots, int64_t n, uint16_t *zmax,
+hipError_t launch_ssv(const DevProfile &p, const DevSeqs &s, const int32_t *slots, int64_t n, uint16_t *zmax, int32_t *queue,
                       hipStream_t st)
 {
   if (n <= 0) return hipSuccess;
@@ -990,11 +1035,11 @@ hipError_t launch_ssv(const DevProfile &p, const DevSeqs &s, const int32_t *slot
   }
   const size_t lds = p.ssv_dwords * 4;
   const int64_t nwaves = (n + (64 / p.ssv_G) - 1) / (64 / p.ssv_G);
-  const unsigned grid = grid_for_waves_blk(nwaves, lds, SSV_BLOCK / 64);
+  const unsigned grid = grid_for_waves_blk(k, nwaves, lds, SSV_BLOCK / 64);
   int tbl_dwords = (int) p.ssv_dwords;
   if (p.ssv_dwords != (size_t) P7G_NROWS * p.ssv_G * p.ssv_ND) return hipErrorInvalidValue;
   void *args[] = { (void *) &p.ssv_tbl, &tbl_dwords, (void *) &s.res, (void *) &s.off, (void *) &s.len,
-                   (void *) &slots, &n, &zmax };
+                   (void *) &slots, &n, &zmax, &queue };
   return launch(k, grid, SSV_BLOCK, args, lds, st);
 }
 
@@ -1020,7 +1065,7 @@ hipError_t launch_msv_full(const DevProfile &p, const DevSeqs &s, const int32_t 
     default: return hipErrorInvalidValue;
   }
   const size_t lds = p.mf_dwords * 4;
-  const unsigned grid = grid_for_waves_blk(maxn, lds, LIST_BLOCK / 64);
+  const unsigned grid = grid_for_waves_blk(k, maxn, lds, LIST_BLOCK / 64);
   int tbl_dwords = (int) p.mf_dwords;
   void *args[] = { (void *) &p, (void *) &p.mf_tbl, &tbl_dwords, (void *) &s.res, (void *) &s.off, (void *) &s.len,
                    (void *) &list, (void *) &count, &xj, &lc };
@@ -1057,7 +1102,7 @@ hipError_t launch_vit(const DevProfile &p, const DevSeqs &s, const int32_t *list
     default: return hipErrorInvalidValue;
   }
   const size_t lds = p.vf_dwords * 4;
-  const unsigned grid = grid_for_waves_blk(maxn, lds, LIST_BLOCK / 64);
+  const unsigned grid = grid_for_waves_blk(k, maxn, lds, LIST_BLOCK / 64);
   void *args[] = { (void *) &p, (void *) &s.res, (void *) &s.off, (void *) &s.len, (void *) &list, (void *) &count,
                    &r, &l, &g, &gating, &lc };
   return launch(k, grid, LIST_BLOCK, args, lds, st);
@@ -1079,7 +1124,7 @@ hipError_t launch_fwd(const DevProfile &p, const DevSeqs &s, const int32_t *list
     default: return hipErrorInvalidValue;
   }
   const size_t lds = lds_ok ? p.fb_floats * 4 : 0;
-  const unsigned grid = grid_for_waves_blk(maxn, lds, LIST_BLOCK / 64);
+  const unsigned grid = grid_for_waves_blk(k, maxn, lds, LIST_BLOCK / 64);
   void *args[] = { (void *) &p, (void *) &s.res, (void *) &s.off, (void *) &s.len, (void *) &list, (void *) &count,
                    &r, &l, &g, &gating, &lc, &scale_pool, &pool_size, &scale_off, &xmx_out };
   return launch(k, grid, LIST_BLOCK, args, lds, st);
@@ -1100,7 +1145,7 @@ hipError_t launch_bck(const DevProfile &p, const DevSeqs &s, const int32_t *list
     default: return hipErrorInvalidValue;
   }
   const size_t lds = lds_ok ? p.fb_floats * 4 : 0;
-  const unsigned grid = grid_for_waves_blk(maxn, lds, LIST_BLOCK / 64);
+  const unsigned grid = grid_for_waves_blk(k, maxn, lds, LIST_BLOCK / 64);
   void *args[] = { (void *) &p, (void *) &s.res, (void *) &s.off, (void *) &s.len, (void *) &list, (void *) &count,
                    &r, (void *) &scale_pool, (void *) &scale_off, (void *) &fwd_xmx, &xmx_out, &lc };
   return launch(k, grid, LIST_BLOCK, args, lds, st);
@@ -1174,4 +1219,88 @@ hipError_t launch_scatter8(const DevSeqs &s, const uint8_t *src, uint8_t *dst, i
   unsigned grid = (unsigned) std::min<int64_t>((n + 255) / 256, 65536);
   hipLaunchKernelGGL(k_scatter8, dim3(grid), dim3(256), 0, st, s.perm, src, dst, n);
   return hipGetLastError();
+}
+
+/*****************************************************************
+ * Order-preserving compaction of the survivor lists
+ *****************************************************************/
+#define CT_TILE 4096     // slots per block: 256 threads x 16 consecutive slots
+
+template <typename T>
+__device__ __forceinline__ int tile_flags(const T *__restrict__ f, int value, int64_t base, int64_t n, unsigned &mask)
+{
+  mask = 0u;
+#pragma unroll
+  for (int u = 0; u < 16; u++) {
+    const int64_t s = base + u;
+    if (s < n && (int) f[s] == value) mask |= 1u << u;
+  }
+  return __popc(mask);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256)
+void k_compact_count(const T *__restrict__ f, int value, int64_t n, int32_t *__restrict__ tilecnt)
+{
+  __shared__ int part[4];
+  unsigned m;
+  int c = tile_flags(f, value, (int64_t) blockIdx.x * CT_TILE + threadIdx.x * 16, n, m);
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, 64);
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = c;
+  __syncthreads();
+  if (threadIdx.x == 0) tilecnt[blockIdx.x] = part[0] + part[1] + part[2] + part[3];
+}
+
+template <typename T>
+__global__ __launch_bounds__(256)
+void k_compact_scatter(const T *__restrict__ f, int value, int64_t n, const int32_t *__restrict__ tilecnt,
+                       int32_t *__restrict__ list, int32_t *__restrict__ count)
+{
+  __shared__ int part[4];
+  __shared__ int wsum[4];
+  // start of this tile's output: the counts of the tiles before it
+  int before = 0;
+  for (int b = threadIdx.x; b < (int) blockIdx.x; b += blockDim.x) before += tilecnt[b];
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) before += __shfl_xor(before, o, 64);
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = before;
+  // this thread's 16 slots, and an exclusive scan of the counts over the block
+  unsigned m;
+  const int64_t base = (int64_t) blockIdx.x * CT_TILE + threadIdx.x * 16;
+  const int c = tile_flags(f, value, base, n, m);
+  const int lane = threadIdx.x & 63;
+  int incl = c;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) { const int v = __shfl_up(incl, o, 64); if (lane >= o) incl += v; }
+  if (lane == 63) wsum[threadIdx.x >> 6] = incl;
+  __syncthreads();
+  const int start = part[0] + part[1] + part[2] + part[3];
+  int woff = 0;
+  for (int w = 0; w < (int) (threadIdx.x >> 6); w++) woff += wsum[w];
+  int at = start + woff + incl - c;
+#pragma unroll
+  for (int u = 0; u < 16; u++) if (m & (1u << u)) list[at++] = (int32_t) (base + u);
+  if (blockIdx.x == gridDim.x - 1 && threadIdx.x == blockDim.x - 1) *count = start + woff + incl;
+}
+
+template <typename T>
+static hipError_t launch_compact(const T *f, int value, int64_t n, int32_t *list, int32_t *count, int32_t *tilecnt,
+                                 hipStream_t st)
+{
+  if (n <= 0) return hipMemsetAsync(count, 0, sizeof(int32_t), st);
+  const unsigned tiles = (unsigned) ((n + CT_TILE - 1) / CT_TILE);
+  hipLaunchKernelGGL(k_compact_count<T>, dim3(tiles), dim3(256), 0, st, f, value, n, tilecnt);
+  hipLaunchKernelGGL(k_compact_scatter<T>, dim3(tiles), dim3(256), 0, st, f, value, n, (const int32_t *) tilecnt, list, count);
+  return hipGetLastError();
+}
+hipError_t launch_compact_u8(const uint8_t *flags, int value, int64_t n, int32_t *list, int32_t *count,
+                             int32_t *tilecnt, hipStream_t st)
+{
+  return launch_compact<uint8_t>(flags, value, n, list, count, tilecnt, st);
+}
+hipError_t launch_compact_i16(const int16_t *flags, int value, int64_t n, int32_t *list, int32_t *count,
+                              int32_t *tilecnt, hipStream_t st)
+{
+  return launch_compact<int16_t>(flags, value, n, list, count, tilecnt, st);
 }

@@ -35,6 +35,7 @@ struct Work {
   int32_t    *slots = nullptr;      // explicit slot list for batched calls
   int64_t    *scale_off = nullptr;
   void       *outbuf = nullptr;     // one per-slot array in the caller's order
+  int32_t    *tilecnt = nullptr;    // per-tile counts of the list compaction
   unsigned long long *stats = nullptr;
   float      *pool = nullptr;
   int64_t     pool_size = 0;
@@ -417,7 +418,8 @@ static DevSeqs seqs_of(const P7_GPU_SEQDB *db)
  *****************************************************************/
 
 // bytes per database slot of all per-slot arrays of a work area
-#define WORK_BYTES_PER_SLOT (2 + 2 + 4*6 + 1 + 4 + 4 + 4*5 + 4 + 8 + 4 + 256)
+#define WORK_BYTES_PER_SLOT (2 + 2 + 4*6 + 1 + 4 + 4 + 4*5 + 4 + 8 + 4 + 1)
+#define WORK_BYTES_FIXED    (64 * 1024)     // alignment of the arrays, counters, compaction tiles
 // Forward scale-factor pool of a pipeline: one float per row of every target
 // that may reach Forward, at most 2^28 floats
 static int64_t pipeline_pool_floats(int64_t nseq, int64_t nres) { return std::min<int64_t>(nres + nseq, (int64_t) 1 << 28); }
@@ -431,7 +433,7 @@ work_reserve(Work &w, int64_t n, int64_t pool_floats)
   if (w.mem) { (void) hipFree(w.mem); w.mem = nullptr; w.cap = 0; }
   if (w.pool) { (void) hipFree(w.pool); w.pool = nullptr; w.pool_size = 0; }
   const int64_t cap = std::max<int64_t>(n, 1024);
-  size_t total = (size_t) cap * WORK_BYTES_PER_SLOT + 4096;
+  size_t total = (size_t) cap * WORK_BYTES_PER_SLOT + WORK_BYTES_FIXED;
   char *m = nullptr;
   HIPCHK(hipMalloc((void **) &m, total));
   size_t o = 0;
@@ -445,6 +447,7 @@ work_reserve(Work &w, int64_t n, int64_t pool_floats)
   w.slots = (int32_t *) take(4 * cap);
   w.scale_off = (int64_t *) take(8 * cap);
   w.outbuf = take(4 * cap);
+  w.tilecnt = (int32_t *) take(4 * (size_t) (cap / 4096 + 1));
   w.l.counts = (int32_t *) take(64);
   w.l.cursor = (unsigned long long *) take(64);
   w.stats = (unsigned long long *) take(64);
@@ -520,7 +523,7 @@ p7_gpu_MSVFilterBatch(const P7_GPU_OPROFILE *om, const P7_GPU_SEQDB *db, const i
   LenCfg lc = lencfg_per_target();
   if ((status = reset_counts(g_work)) != p7G_OK) return status;
   HIPCHK(hipMemcpyAsync(w.slots, slots.data(), sizeof(int32_t) * (size_t) n, hipMemcpyHostToDevice, g_stream));
-  HIPCHK(launch_ssv(*p, s, w.slots, n, w.r.zmax, g_stream));
+  HIPCHK(launch_ssv(*p, s, w.slots, n, w.r.zmax, &w.l.counts[P7G_Q_SSV], g_stream));
   HIPCHK(launch_msv_classify(*p, s, w.slots, n, w.r.zmax, w.r.xj, w.l.full, &w.l.counts[0], lc, g_stream));
   HIPCHK(launch_msv_full(*p, s, w.l.full, &w.l.counts[0], n, w.r.xj, lc, g_stream));
   std::vector<int16_t> xj((size_t) db->nseq);
@@ -575,8 +578,11 @@ run_list_stage(int which, const P7_GPU_OPROFILE *om, const P7_GPU_SEQDB *db, con
   } else {
     HIPCHK(launch_fwd(*p, s, list, cnt, n, w.r, w.l, g, 0, lc,
                       which == ST_BCK ? w.pool : nullptr, w.pool_size, w.scale_off, nullptr, g_stream));
-    if (which == ST_BCK)
-      HIPCHK(launch_bck(*p, s, list, cnt, n, w.r, w.pool, w.scale_off, nullptr, nullptr, lc, g_stream));
+    if (which == ST_BCK) {
+      // its own count, hence its own work-queue cursor (Forward used up the one of cnt)
+      HIPCHK(hipMemcpyAsync(&w.l.counts[4], &n32, sizeof(int32_t), hipMemcpyHostToDevice, g_stream));
+      HIPCHK(launch_bck(*p, s, list, &w.l.counts[4], n, w.r, w.pool, w.scale_off, nullptr, nullptr, lc, g_stream));
+    }
   }
   std::vector<float> sc((size_t) db->nseq);
   std::vector<int>   st((size_t) db->nseq, 0);
@@ -631,7 +637,7 @@ extern "C" size_t
 p7g_pipeline_work_bytes(int64_t nseq, int64_t nres)
 {
   const int64_t cap = std::max<int64_t>(nseq, 1024);
-  return (size_t) cap * WORK_BYTES_PER_SLOT + 4096 + sizeof(float) * (size_t) std::max<int64_t>(pipeline_pool_floats(nseq, nres), 1 << 20);
+  return (size_t) cap * WORK_BYTES_PER_SLOT + WORK_BYTES_FIXED + sizeof(float) * (size_t) std::max<int64_t>(pipeline_pool_floats(nseq, nres), 1 << 20);
 }
 
 extern "C" int
@@ -711,18 +717,27 @@ p7_gpu_Pipeline(P7_GPU_PIPELINE *pli, const P7_GPU_OPROFILE *om, const P7_GPU_SE
   HIPCHK(hipMemsetAsync(w.r.fwdsc, 0xff, sizeof(float) * (size_t) n, w.st));
   HIPCHK(hipMemsetAsync(w.r.bcksc, 0xff, sizeof(float) * (size_t) n, w.st));
   HIPCHK(hipEventRecord(w.ev[0], w.st));
-  HIPCHK(launch_ssv(*p, s, nullptr, n, w.r.zmax, w.st));
+  // Each stage's list is compacted from the per-slot marks in slot order
+  // (longest target first); the list kernels take items from a work queue.
+  HIPCHK(launch_ssv(*p, s, nullptr, n, w.r.zmax, &w.l.counts[P7G_Q_SSV], w.st));
   HIPCHK(hipEventRecord(w.ev[6], w.st));
   HIPCHK(launch_msv_classify(*p, s, nullptr, n, w.r.zmax, w.r.xj, w.l.full, &w.l.counts[0], lc, w.st));
+  HIPCHK(launch_compact_i16(w.r.xj, -2, n, w.l.full, &w.l.counts[0], w.tilecnt, w.st));
   HIPCHK(launch_msv_full(*p, s, w.l.full, &w.l.counts[0], n, w.r.xj, lc, w.st));
   HIPCHK(launch_msv_gate(*p, s, n, w.r.xj, w.r, w.l, g, w.st));
   HIPCHK(hipEventRecord(w.ev[1], w.st));
-  if (g.do_bias) HIPCHK(launch_bias(*p, s, w.l.bias, &w.l.counts[1], n, w.r, w.l, g, 1, lc, w.st));
+  if (g.do_bias) {
+    HIPCHK(launch_compact_u8(w.r.stage, 1, n, w.l.bias, &w.l.counts[1], w.tilecnt, w.st));
+    HIPCHK(launch_bias(*p, s, w.l.bias, &w.l.counts[1], n, w.r, w.l, g, 1, lc, w.st));
+  }
   HIPCHK(hipEventRecord(w.ev[2], w.st));
+  HIPCHK(launch_compact_u8(w.r.stage, 2, n, w.l.vit, &w.l.counts[2], w.tilecnt, w.st));
   HIPCHK(launch_vit(*p, s, w.l.vit, &w.l.counts[2], n, w.r, w.l, g, 1, lc, w.st));
   HIPCHK(hipEventRecord(w.ev[3], w.st));
+  HIPCHK(launch_compact_u8(w.r.stage, 3, n, w.l.fwd, &w.l.counts[3], w.tilecnt, w.st));
   HIPCHK(launch_fwd(*p, s, w.l.fwd, &w.l.counts[3], n, w.r, w.l, g, 1, lc, w.pool, w.pool_size, w.scale_off, nullptr, w.st));
   HIPCHK(hipEventRecord(w.ev[4], w.st));
+  HIPCHK(launch_compact_u8(w.r.stage, 4, n, w.l.bck, &w.l.counts[4], w.tilecnt, w.st));
   HIPCHK(launch_bck(*p, s, w.l.bck, &w.l.counts[4], n, w.r, w.pool, w.scale_off, nullptr, nullptr, lc, w.st));
   HIPCHK(hipEventRecord(w.ev[5], w.st));
 
@@ -838,8 +853,9 @@ single_msv(const uint8_t *dsq, int L, const P7_GPU_OPROFILE *om, int *ret_xj)
   Work &w = g_work;
   LenCfg lc = lencfg_of(om);
   if ((status = reset_counts(g_work)) != p7G_OK) return status;
-  HIPCHK(launch_ssv(*p, s, nullptr, 1, w.r.zmax, g_stream));
+  HIPCHK(launch_ssv(*p, s, nullptr, 1, w.r.zmax, &w.l.counts[P7G_Q_SSV], g_stream));
   HIPCHK(launch_msv_classify(*p, s, nullptr, 1, w.r.zmax, w.r.xj, w.l.full, &w.l.counts[0], lc, g_stream));
+  HIPCHK(launch_compact_i16(w.r.xj, -2, 1, w.l.full, &w.l.counts[0], w.tilecnt, g_stream));
   HIPCHK(launch_msv_full(*p, s, w.l.full, &w.l.counts[0], 1, w.r.xj, lc, g_stream));
   int16_t xj; int32_t nfull;
   HIPCHK(hipMemcpyAsync(&xj, w.r.xj, sizeof(xj), hipMemcpyDeviceToHost, g_stream));
