@@ -51,6 +51,73 @@ def cpu_baseline(om, db, M, seconds_target=15.0, max_threads=16):
                       f"(oracle/p7_oracle.c restatement, scalar C), {nthreads} threads, {dt:.1f} s"}
 
 
+def serving_leg(p7gpu, synth, sdb, nq, rate, nslots, seed=11):
+    """Queries arrive as a Poisson stream against the resident database and are
+    served by p7_gpu_server_* (several in flight); the same queries run one by
+    one through a single pipeline give the plain per-query rate. Reports
+    p50/p99 latency, aggregate GCUPS of both, and whether every query's pass
+    set is identical in both."""
+    import tempfile
+    import threading
+    tmp = tempfile.mkdtemp(prefix="p7q_")
+    paths = synth.query_family(p7gpu, tmp, nprof=8, seed=seed)
+    oms = [p7gpu.OProfile.from_file(pth) for pth in paths]
+    Ms = [om.tables()["M"] for om in oms]
+    rng = np.random.default_rng(seed)
+    which = rng.integers(0, len(oms), nq)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, nq))
+    cells = float(sum(Ms[q] for q in which)) * sdb.nres
+
+    # warm the device copies of every profile, then the plain per-query path
+    pli = p7gpu.Pipeline()
+    for om in oms:
+        pli.run(om, sdb, scores=False)
+    p7gpu.lib.p7_gpu_Synchronize()
+    t0 = time.perf_counter()
+    plain = [pli.run(oms[q], sdb, scores=False)["stage"] for q in which]
+    t_plain = time.perf_counter() - t0
+
+    srv = p7gpu.Server(sdb, nslots=nslots)
+    tickets = [None] * nq
+    def feeder():
+        start = time.perf_counter()
+        for i in range(nq):
+            dt = arrivals[i] - (time.perf_counter() - start)
+            if dt > 0:
+                time.sleep(dt)
+            tickets[i] = srv.submit(oms[which[i]], scores=False)
+    t0 = time.perf_counter()
+    th = threading.Thread(target=feeder)
+    th.start()
+    lat, same = [], True
+    for i in range(nq):
+        while tickets[i] is None:
+            time.sleep(0.0005)
+        out, st, ms = srv.wait(tickets[i])
+        lat.append(ms)
+        same &= bool(np.array_equal(out["stage"], plain[i]))
+    th.join()
+    t_served = time.perf_counter() - t0
+    # saturation: every query queued at once
+    t0 = time.perf_counter()
+    tk = [srv.submit(oms[q], scores=False) for q in which]
+    for i, t in enumerate(tk):
+        out, st, ms = srv.wait(t)
+        same &= bool(np.array_equal(out["stage"], plain[i]))
+    t_sat = time.perf_counter() - t0
+    slots = srv.slots
+    srv.close()
+    import shutil
+    shutil.rmtree(tmp, ignore_errors=True)
+    lat = np.array(lat)
+    return {"queries": nq, "arrival_rate_qps": rate, "slots": slots, "profiles_M": Ms,
+            "p50_ms": round(float(np.percentile(lat, 50)), 3), "p99_ms": round(float(np.percentile(lat, 99)), 3),
+            "gcups_served_stream": round(cells / t_served / 1e9, 1),
+            "gcups_served_saturated": round(cells / t_sat / 1e9, 1),
+            "gcups_plain_per_query": round(cells / t_plain / 1e9, 1),
+            "identical_pass_sets": same}
+
+
 def calibrate_copy(mib=256, reps=3):
     """Plain device-to-device copies of a known size: what FETCH_SIZE and WRITE_SIZE
     report for them gives the factor from counter KiB to HBM bytes."""
@@ -73,6 +140,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "k_ssv_traffic.json"),
                     help="JSON with the HBM bytes of one k_ssv launch, from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE")
+    ap.add_argument("--serve-queries", type=int, default=32, help="queries in the serving leg (0: skip it)")
+    ap.add_argument("--serve-rate", type=float, default=100.0, help="Poisson arrival rate of the serving leg, queries/s")
+    ap.add_argument("--serve-slots", type=int, default=4)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run three plain 256 MiB device copies (to scale the FETCH_SIZE/WRITE_SIZE counters)")
     args = ap.parse_args()
@@ -152,6 +222,9 @@ def main():
                         "vit": int((stage >= 3).sum()), "fwd": int((stage >= 4).sum()),
                         "msv_full_rescored": st["n_msv_full"]},
     }
+    if args.serve_queries > 0:
+        log("serving leg ...")
+        line["serving"] = serving_leg(p7gpu, synth, sdb, args.serve_queries, args.serve_rate, args.serve_slots)
     if not args.no_cpu:
         log("cpu baseline ...")
         line["cpu_baseline"] = cpu_baseline(om, db, M, args.cpu_seconds)

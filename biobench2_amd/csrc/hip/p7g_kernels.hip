@@ -42,6 +42,8 @@ typedef short s2v __attribute__((ext_vector_type(2)));
 // Workgroup size of the one-target-per-wave list stages: 8 waves share one LDS
 // copy of the profile.
 #define LIST_BLOCK 512
+// Target groups the ungapped pass takes from its work queue per atomic.
+#define SSV_GRAB 4
 
 __device__ __forceinline__ int   s2i(s2v v) { return __builtin_bit_cast(int, v); }
 __device__ __forceinline__ s2v   i2s(int v) { return __builtin_bit_cast(s2v, v); }
@@ -222,10 +224,12 @@ void k_ssv(const int *__restrict__ tbl, int tbl_dwords,
   const int4 *qrow = (const int4 *) __builtin_assume_aligned(lds4, 16) + z;     // + (x*NQ + q)*G
   const int  *trow = (const int *) lds4 + TAIL0 + z;                           // + (x*NT + t)*G
 
+  int64_t w = 0, wend = 0;                            // groups of TPW targets, SSV_GRAB groups per grab
   for (;;) {
-    const int64_t w = queue_next(queue, 1);           // one group of TPW targets per grab
+    if (w == wend) { w = (int64_t) queue_next(queue, SSV_GRAB); wend = w + SSV_GRAB; }
     if (w * TPW >= n) break;
     const int64_t t = w * TPW + grp;
+    w++;
     const bool valid = t < n;
     const int64_t slot = valid ? (slots ? (int64_t) slots[t] : t) : 0;
     const int L = valid ? len[slot] : 0;

@@ -277,3 +277,85 @@ def make_db(nseq, hmms, frac_homolog=0.01, seed_db=43, seed_plant=44, median=300
         o = int(offsets[i]) + a
         residues[o:o + s.size] = s
     return SynthDB(residues, offsets, lengths, homolog)
+
+
+# ---- E-value calibration (the recipe of p7_Calibrate, HMMER/src/evalues.c:60-466) ----
+LOG2 = 0.69314718055994529
+
+
+def gumbel_fit(x):
+    """Maximum-likelihood Gumbel (mu, lambda) of a sample (Lawless; as esl_gumbel_FitComplete)."""
+    x = np.asarray(x, float)
+    lam = math.pi / (math.sqrt(6.0) * x.std())
+    for _ in range(100):
+        e = np.exp(-lam * x)
+        s0, s1, s2 = e.sum(), (x * e).sum(), (x * x * e).sum()
+        f = 1.0 / lam - x.mean() + s1 / s0
+        df = -1.0 / lam ** 2 - (s2 / s0 - (s1 / s0) ** 2)
+        step = f / df
+        lam -= step
+        if abs(step) < 1e-10:
+            break
+    mu = -math.log(np.exp(-lam * x).mean()) / lam
+    return mu, lam
+
+
+def evparams(hmm, msv_bits, vit_bits, fwd_bits, tailp=0.04):
+    """(mmu, lambda, vmu, lambda, tau, lambda) from bit scores of i.i.d. sequences:
+    lambda = ln2 + 1.44 / (M * H) with H the mean match relative entropy in bits
+    (evalues.c:186-191); mu by fixed-lambda Gumbel fits; tau from a Gumbel fit of
+    the Forward scores shifted to the tail (evalues.c:430-466)."""
+    f = AMINO_BG / AMINO_BG.sum()
+    H = float(np.mean([(p[p > 0] * np.log2(p[p > 0] / f[p > 0])).sum() for p in hmm.mat[1:]]))
+    lam = LOG2 + 1.44 / (hmm.M * H)
+    mmu = -math.log(np.mean(np.exp(-lam * np.asarray(msv_bits)))) / lam
+    vmu = -math.log(np.mean(np.exp(-lam * np.asarray(vit_bits)))) / lam
+    gmu, glam = gumbel_fit(fwd_bits)
+    tau = gmu - math.log(-math.log(1.0 - tailp)) / glam + math.log(tailp) / lam
+    return (mmu, lam, vmu, lam, tau, lam)
+
+
+def calibration_seqs(nseq=200, seed=42):
+    """The i.i.d. samples of a calibration: nseq sequences of L = 200 (MSV, Viterbi)
+    and nseq of L = 100 (Forward)."""
+    rng = np.random.default_rng(seed)
+    return [iid_residues(200, rng) for _ in range(nseq)], [iid_residues(100, rng) for _ in range(nseq)]
+
+
+def calibrate_with(gpu, path, nseq=200, seed=42):
+    """E-value parameters of the profile in `path` from this library's own MSV,
+    Viterbi (L = 200) and Forward (L = 100) scores of i.i.d. sequences."""
+    om = gpu.OProfile.from_file(path, 0)
+    t = om.tables()
+    hmm = read_hmm_probs(path)
+    s200, s100 = calibration_seqs(nseq, seed)
+    bg = gpu.Background(gpu.AMINO)
+    bg.set_length(200)                                   # p7_bg_SetLength before p7_bg_NullOne
+    n200 = float(bg.null_one(200))
+    bg.set_length(100)
+    n100 = float(bg.null_one(100))
+    maxm = (255 - t["base_b"]) / t["scale_b"]           # an overflowing MSV score, as p7_MSVMu counts it
+    db200, db100 = gpu.SeqDB.from_list(s200), gpu.SeqDB.from_list(s100)
+    msc, _ = gpu.msv_batch(om, db200)
+    vsc, _ = gpu.viterbi_batch(om, db200)
+    fsc, _ = gpu.forward_batch(om, db100)
+    msc = np.where(np.isinf(msc), maxm, msc.astype(np.float64))
+    return evparams(hmm, (msc - n200) / LOG2, (vsc.astype(np.float64) - n200) / LOG2,
+                    (fsc.astype(np.float64) - n100) / LOG2)
+
+
+def query_family(gpu, dirpath, nprof=8, median_M=250, sigma=0.6, Mmin=50, Mmax=3000, seed=7):
+    """Calibrated query profiles with M ~ lognormal(median, sigma) clipped to [Mmin, Mmax],
+    written to dirpath; returns their paths."""
+    import os
+    rng = np.random.default_rng(seed)
+    paths = []
+    for q in range(nprof):
+        M = int(np.clip(round(math.exp(math.log(median_M) + sigma * rng.standard_normal())), Mmin, Mmax))
+        h = random_profile(M, rng, f"query{q}_M{M}")
+        path = os.path.join(dirpath, h.name + ".hmm")
+        write_hmm(path, [h])
+        h.stats = calibrate_with(gpu, path, seed=seed + q)
+        write_hmm(path, [h])
+        paths.append(path)
+    return paths

@@ -60,3 +60,38 @@ def test_admission_is_bounded_by_the_memory_budget(setup):
     out, st, _ = big.wait(big.submit(oms[0], scores=False))
     assert st["nseqs"] == sdb.nseq and out["stage"].shape == (sdb.nseq,)
     big.close()
+
+
+def test_calibration_with_the_library_equals_the_oracle_calibration(gpu):
+    """synth.calibrate_with fits the E-value parameters to the library's own MSV,
+    Viterbi and Forward scores; fitted to the oracle's scores of the same
+    sequences they must agree: exactly for the MSV and Viterbi locations (the
+    scores are bit-exact), lambda is a closed form of the profile, and tau moves
+    by no more than the Forward scores' 1e-4-nat difference allows."""
+    import oracle
+    path = synth_profile("synth_M200")
+    got = synth.calibrate_with(gpu, path)
+    om = gpu.OProfile.from_file(path)
+    t = om.tables()
+    hmm = synth.read_hmm_probs(path)
+    s200, s100 = synth.calibration_seqs()
+    lm200, lm100 = oracle.LengthModel(t, 200), oracle.LengthModel(t, 100)
+    xw = t["xw"].copy(); xw[1, 0] = xw[2, 0] = xw[3, 0] = lm200.xwmove[200]
+    xf = t["xf"].copy()
+    xf[1, 0] = xf[2, 0] = xf[3, 0] = lm100.xfmove[100]
+    xf[1, 1] = xf[2, 1] = xf[3, 1] = lm100.xfloop[100]
+    maxm = (255 - t["base_b"]) / t["scale_b"]
+    n200, n100 = float(oracle.null_one(200)), float(oracle.null_one(100))
+    xm, xv, xs = [], [], []
+    for s in s200:
+        st, sc, _ = oracle.msv_filter(s, t, tjb=int(lm200.tjb[200]))
+        xm.append(((maxm if st == p7gpu.ERANGE else float(np.float32(sc))) - n200) / synth.LOG2)
+        st, sc, _ = oracle.vit_filter(s, t, xw)
+        xv.append((float(np.float32(sc)) - n200) / synth.LOG2)
+    for s in s100:
+        st, sc, _, _ = oracle.fwd_parser(s, t, xf)
+        xs.append((float(np.float32(sc)) - n100) / synth.LOG2)
+    ref = synth.evparams(hmm, xm, xv, xs)
+    assert abs(got[0] - ref[0]) < 1e-9 and abs(got[2] - ref[2]) < 1e-9, (got, ref)
+    assert got[1] == ref[1] and got[3] == ref[3] and got[5] == ref[5]
+    assert abs(got[4] - ref[4]) < 1e-2, (got[4], ref[4])
