@@ -117,8 +117,12 @@ hipError_t launch_msv_full(const DevProfile &p, const DevSeqs &s, const int32_t 
                            int64_t maxn, int16_t *xj, LenCfg lc, hipStream_t st);
 hipError_t launch_msv_gate(const DevProfile &p, const DevSeqs &s, int64_t n, const int16_t *xj, DevResults r,
                            DevLists l, GateThresh g, hipStream_t st);
+// The bias filter; with `rowmax` (room for L+1 floats per listed target, taken
+// from l.cursor, which is zero again on return) the logs leave the serial
+// recurrence. Targets that find no room are finished in the first pass.
 hipError_t launch_bias(const DevProfile &p, const DevSeqs &s, const int32_t *list, const int32_t *count, int64_t maxn,
-                       DevResults r, DevLists l, GateThresh g, int gating, LenCfg lc, hipStream_t st);
+                       DevResults r, DevLists l, GateThresh g, int gating, LenCfg lc,
+                       float *rowmax, int64_t rowmax_size, int64_t *defer_off, hipStream_t st);
 hipError_t launch_vit(const DevProfile &p, const DevSeqs &s, const int32_t *list, const int32_t *count, int64_t maxn,
                       DevResults r, DevLists l, GateThresh g, int gating, LenCfg lc, hipStream_t st);
 hipError_t launch_fwd(const DevProfile &p, const DevSeqs &s, const int32_t *list, const int32_t *count, int64_t maxn,

@@ -431,11 +431,32 @@ __global__ void k_msv_gate(const DevProfile p, const int32_t *__restrict__ len, 
 /*****************************************************************
  * 5. Bias-composition filter: 2-state HMM Forward with per-row max
  *    normalisation (easel/esl_hmm.c:351-409, HMMER/src/p7_bg.c:473-484).
- *    One target per thread; float arithmetic in the reference's order.
+ *    Float arithmetic in the reference's order, in two passes:
+ *    k_bias runs the recurrence, one target per thread, and stores each
+ *    row's normaliser; k_bias_sum takes the logs of a target's normalisers
+ *    on all lanes at once and adds them in row order, as the reference does.
+ *    The log is off the recurrence's serial chain, which bounds the stage.
  *****************************************************************/
-__global__ void k_bias(const DevProfile p, const uint8_t *__restrict__ res, const int64_t *__restrict__ off,
-                       const int32_t *__restrict__ len, const int32_t *__restrict__ list, const int32_t *__restrict__ count,
-                       DevResults r, DevLists l, GateThresh g, int gating, LenCfg lc)
+__device__ __forceinline__ void bias_finish(const DevProfile &p, DevResults &r, const GateThresh &g, int gating,
+                                            const LenCfg &lc, int slot, int L, float logsc)
+{
+#pragma clang fp contract(off)
+  // with a caller-supplied length model the host adds the length terms
+  const float filtersc = lc.on ? logsc : (logsc + p.flen_a[L]) + p.flen_b[L];
+  r.filtersc[slot] = filtersc;
+  if (gating) {
+    const float seq = (float) __ddiv_rn((double) (r.usc[slot] - filtersc), LOG2_D);
+    if (seq < g.T1m)       r.stage[slot] = 1;
+    else if (seq >= g.T2m) r.stage[slot] = 3;       // the lists of the next stages are compacted from stage[]
+    else                   r.stage[slot] = 2;
+  }
+}
+
+__global__ __launch_bounds__(64)
+void k_bias(const DevProfile p, const uint8_t *__restrict__ res, const int64_t *__restrict__ off,
+            const int32_t *__restrict__ len, const int32_t *__restrict__ list, const int32_t *__restrict__ count,
+            DevResults r, GateThresh g, int gating, LenCfg lc,
+            float *__restrict__ rowmax, int64_t rowmax_size, unsigned long long *cursor, int64_t *__restrict__ defer_off)
 {
 #pragma clang fp contract(off)
   // the two states' odds of every residue code, in LDS
@@ -443,13 +464,31 @@ __global__ void k_bias(const DevProfile p, const uint8_t *__restrict__ res, cons
   for (int x = threadIdx.x; x < P7G_NROWS; x += blockDim.x) eo2[x] = make_float2(p.eo[2*x], p.eo[2*x + 1]);
   __syncthreads();
   const int64_t n = *count;
-  for (int64_t t = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (int64_t) gridDim.x * blockDim.x) {
-    const int slot = list[t];
-    const int L = len[slot];
+  const int lane = threadIdx.x & 63;
+  for (int64_t t0 = (int64_t) blockIdx.x * blockDim.x; t0 < n; t0 += (int64_t) gridDim.x * blockDim.x) {
+    const int64_t t = t0 + threadIdx.x;
+    const bool valid = t < n;
+    const int slot = valid ? list[t] : 0;
+    const int L = valid ? len[slot] : 0;
+    // room for L row normalisers and the final sum: one bump of the cursor per wave
+    int64_t base = -1;
+    if (rowmax) {
+      const int64_t need = valid ? (((int64_t) L + 1 + 3) & ~(int64_t) 3) : 0;   // 16-byte aligned blocks
+      int64_t incl = need;
+#pragma unroll
+      for (int o = 1; o < 64; o <<= 1) { const int64_t v = __shfl_up(incl, o, 64); if (lane >= o) incl += v; }
+      unsigned long long b0 = 0;
+      if (lane == 63) b0 = atomicAdd(cursor, (unsigned long long) incl);
+      b0 = __shfl(b0, 63, 64);
+      const int64_t mine = (int64_t) b0 + incl - need;
+      if (valid && mine + need <= rowmax_size) base = mine;
+    }
+    if (! valid) continue;
     const uint8_t *sq = res + off[slot];
     const float p1  = lc.on ? lc.p1 : __fdiv_rn((float) L, (float) (L + 1));
     const float t00 = p1, t01 = lc.on ? lc.q1 : 1.0f - p1;
     const float t10 = p.ft1_0, t11 = p.ft1_1;
+    float *rm = (base >= 0) ? rowmax + base : nullptr;
     float logsc = 0.0f;
     float d0, d1, mx;
     // residues 16 at a time, the next chunk fetched ahead (targets are 16-byte
@@ -465,7 +504,7 @@ __global__ void k_bias(const DevProfile p, const uint8_t *__restrict__ res, cons
     mx = (d0 > mx) ? d0 : mx;
     mx = (d1 > mx) ? d1 : mx;
     d0 = __fdiv_rn(d0, mx); d1 = __fdiv_rn(d1, mx);
-    logsc += (float) log((double) mx);
+    if (rm) rm[0] = mx; else logsc += (float) log((double) mx);
     for (int i = 1; i < L; i++) {
       const float2 e = en;
       const int j = i + 1;
@@ -479,22 +518,57 @@ __global__ void k_bias(const DevProfile p, const uint8_t *__restrict__ res, cons
       n1 += d0 * t01; n1 += d1 * t11; n1 *= e.y;
       mx = (n1 > mx) ? n1 : mx;
       d0 = __fdiv_rn(n0, mx); d1 = __fdiv_rn(n1, mx);
-      logsc += (float) log((double) mx);
+      if (rm) rm[i] = mx; else logsc += (float) log((double) mx);
     }
-    float s = 0.0f;
-    s += d0 * 1.0f; s += d1 * 1.0f;
-    logsc += (float) log((double) s);
-    // with a caller-supplied length model the host adds the length terms
-    const float filtersc = lc.on ? logsc : (logsc + p.flen_a[L]) + p.flen_b[L];
-    r.filtersc[slot] = filtersc;
-    bool to_vit = false, to_fwd = false;
-    if (gating) {
-      const float seq = (float) __ddiv_rn((double) (r.usc[slot] - filtersc), LOG2_D);
-      if (seq < g.T1m) r.stage[slot] = 1;
-      else if (seq >= g.T2m) { to_fwd = true; r.stage[slot] = 3; }
-      else                   { to_vit = true; r.stage[slot] = 2; }
+    float sfin = 0.0f;
+    sfin += d0 * 1.0f; sfin += d1 * 1.0f;
+    if (rm) { rm[L] = sfin; defer_off[slot] = base; continue; }   // k_bias_log + k_bias_sum finish it
+    if (rowmax) defer_off[slot] = -1;
+    logsc += (float) log((double) sfin);
+    bias_finish(p, r, g, gating, lc, slot, L, logsc);
+  }
+}
+
+// Every stored value replaced by float(log(double(v))), in parallel.
+__global__ __launch_bounds__(256)
+void k_bias_log(float *__restrict__ rowmax, const unsigned long long *__restrict__ used, int64_t cap)
+{
+  const int64_t n = min((int64_t) *used, cap);
+  for (int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t) gridDim.x * blockDim.x)
+    rowmax[i] = (float) log((double) rowmax[i]);
+}
+
+// One target per thread: its L+1 logs added in row order, as esl_hmm_Forward
+// sums sc[1..L+1] (esl_hmm.c:403-405), four at a time from aligned loads
+// issued a block ahead.
+__global__ __launch_bounds__(64)
+void k_bias_sum(const DevProfile p, const int32_t *__restrict__ len, const int32_t *__restrict__ list,
+                const int32_t *__restrict__ count, DevResults r, GateThresh g, int gating, LenCfg lc,
+                const float *__restrict__ rowmax, const int64_t *__restrict__ defer_off)
+{
+#pragma clang fp contract(off)
+  const int64_t n = *count;
+  for (int64_t t = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (int64_t) gridDim.x * blockDim.x) {
+    const int slot = list[t];
+    const int64_t o = defer_off[slot];
+    if (o < 0) continue;                       // finished in the first pass
+    const int L = len[slot];
+    const int nv = L + 1;                      // L row terms and the closing sum
+    const float4 *q = (const float4 *) (rowmax + o);
+    const int nq = (nv + 3) >> 2;
+    float logsc = 0.0f;
+    float4 a = q[0], b = (nq > 1) ? q[1] : a;
+    for (int k = 0; k < nq; k++) {
+      const float4 v = a;
+      a = b;
+      if (k + 2 < nq) b = q[k + 2];
+      const int rem = nv - 4 * k;
+      logsc += v.x;
+      if (rem > 1) logsc += v.y;
+      if (rem > 2) logsc += v.z;
+      if (rem > 3) logsc += v.w;
     }
-    (void) to_vit; (void) to_fwd;                    // the next lists are compacted from stage[]
+    bias_finish(p, r, g, gating, lc, slot, L, logsc);
   }
 }
 
@@ -1086,12 +1160,23 @@ hipError_t launch_msv_gate(const DevProfile &p, const DevSeqs &s, int64_t n, con
 }
 
 hipError_t launch_bias(const DevProfile &p, const DevSeqs &s, const int32_t *list, const int32_t *count, int64_t maxn,
-                       DevResults r, DevLists l, GateThresh g, int gating, LenCfg lc, hipStream_t st)
+                       DevResults r, DevLists l, GateThresh g, int gating, LenCfg lc,
+                       float *rowmax, int64_t rowmax_size, int64_t *defer_off, hipStream_t st)
 {
   if (maxn <= 0) return hipSuccess;
   unsigned grid = (unsigned) ((maxn + 63) / 64); if (grid > 8192) grid = 8192;
-  hipLaunchKernelGGL(k_bias, dim3(grid), dim3(64), 0, st, p, s.res, s.off, s.len, list, count, r, l, g, gating, lc);
-  return hipGetLastError();
+  hipLaunchKernelGGL(k_bias, dim3(grid), dim3(64), 0, st, p, s.res, s.off, s.len, list, count, r, g, gating, lc,
+                     rowmax, rowmax_size, l.cursor, defer_off);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess || ! rowmax) return e;
+  dev_props();
+  hipLaunchKernelGGL(k_bias_log, dim3(ncu() * 8), dim3(256), 0, st, rowmax, (const unsigned long long *) l.cursor, rowmax_size);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  hipLaunchKernelGGL(k_bias_sum, dim3(grid), dim3(64), 0, st, p, s.len, list, count, r, g, gating, lc,
+                     (const float *) rowmax, (const int64_t *) defer_off);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  // the scale-factor pool's cursor starts from zero again for the Forward stage
+  return hipMemsetAsync(l.cursor, 0, sizeof(unsigned long long), st);
 }
 
 #define VIT_CASE(N) case N: k = (void *) k_vit<N>; break;

@@ -572,7 +572,7 @@ run_list_stage(int which, const P7_GPU_OPROFILE *om, const P7_GPU_SEQDB *db, con
   HIPCHK(hipMemcpyAsync(list, slots.data(), sizeof(int32_t) * (size_t) n, hipMemcpyHostToDevice, g_stream));
   HIPCHK(hipMemcpyAsync(cnt, &n32, sizeof(int32_t), hipMemcpyHostToDevice, g_stream));
   if (which == ST_BIAS) {
-    HIPCHK(launch_bias(*p, s, list, cnt, n, w.r, w.l, g, 0, lc, g_stream));
+    HIPCHK(launch_bias(*p, s, list, cnt, n, w.r, w.l, g, 0, lc, w.pool, w.pool_size, w.scale_off, g_stream));
   } else if (which == ST_VIT) {
     HIPCHK(launch_vit(*p, s, list, cnt, n, w.r, w.l, g, 0, lc, g_stream));
   } else {
@@ -728,7 +728,7 @@ p7_gpu_Pipeline(P7_GPU_PIPELINE *pli, const P7_GPU_OPROFILE *om, const P7_GPU_SE
   HIPCHK(hipEventRecord(w.ev[1], w.st));
   if (g.do_bias) {
     HIPCHK(launch_compact_u8(w.r.stage, 1, n, w.l.bias, &w.l.counts[1], w.tilecnt, w.st));
-    HIPCHK(launch_bias(*p, s, w.l.bias, &w.l.counts[1], n, w.r, w.l, g, 1, lc, w.st));
+    HIPCHK(launch_bias(*p, s, w.l.bias, &w.l.counts[1], n, w.r, w.l, g, 1, lc, w.pool, w.pool_size, w.scale_off, w.st));
   }
   HIPCHK(hipEventRecord(w.ev[2], w.st));
   HIPCHK(launch_compact_u8(w.r.stage, 2, n, w.l.vit, &w.l.counts[2], w.tilecnt, w.st));
@@ -1059,7 +1059,7 @@ p7_gpu_bg_FilterScore(P7_GPU_BG *bg, const uint8_t *dsq, int L, float *ret_sc)
     int32_t zero = 0, one = 1;
     HIPCHK(hipMemcpyAsync(w.l.bias, &zero, sizeof(zero), hipMemcpyHostToDevice, g_stream));
     HIPCHK(hipMemcpyAsync(&w.l.counts[1], &one, sizeof(one), hipMemcpyHostToDevice, g_stream));
-    HIPCHK(launch_bias(p, s, w.l.bias, &w.l.counts[1], 1, w.r, w.l, g, 0, lc, g_stream));
+    HIPCHK(launch_bias(p, s, w.l.bias, &w.l.counts[1], 1, w.r, w.l, g, 0, lc, nullptr, 0, nullptr, g_stream));
     HIPCHK(hipMemcpyAsync(&logsc, w.r.filtersc, sizeof(float), hipMemcpyDeviceToHost, g_stream));
     HIPCHK(hipStreamSynchronize(g_stream));
     (void) hipFree(d_eo);
