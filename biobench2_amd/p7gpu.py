@@ -97,6 +97,7 @@ def _load():
         "p7_gpu_Pipeline": (i, [vp, vp, vp, POINTER(Hits)]),
         "p7_gpu_pipeline_GetStats": (i, [vp, POINTER(Stats)]),
         "p7_gpu_ScoreThreshold": (f, [i, d, d, d]),
+        "p7_gpu_oprofile_ReadPressed": (i, [c_char_p, c_char_p, i, POINTER(vp)]),
         "p7_gpu_server_Create": (vp, [vp, i, ctypes.c_size_t, d, d, d, i]),
         "p7_gpu_server_Slots": (i, [vp]),
         "p7_gpu_server_Submit": (i, [vp, vp, POINTER(Hits), POINTER(c_int64)]),
@@ -242,6 +243,17 @@ class OProfile:
         gm = Profile(hmm, bg, L, LOCAL)
         om = cls.from_profile(gm)
         om.name, om.hmm, om.gm, om.bg = hmm.name, hmm, gm, bg
+        return om
+
+    @classmethod
+    def from_pressed(cls, h3f_path, h3p_path, which=0):
+        """Model `which` of a pressed database (.h3f/.h3p, p7_oprofile_ReadMSV + ReadRest)."""
+        om = cls.__new__(cls)
+        h = c_void_p()
+        _check(lib.p7_gpu_oprofile_ReadPressed(str(h3f_path).encode(), str(h3p_path).encode(), which,
+                                               ctypes.byref(h)), "oprofile_ReadPressed")
+        om.h = h.value
+        om.abctype = om.tables()["abctype"]
         return om
 
     def tables(self):

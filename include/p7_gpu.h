@@ -160,6 +160,11 @@ P7_GPU_OPROFILE *p7_gpu_oprofile_Create(int allocM, int abctype);
 int   p7_gpu_oprofile_Convert(const P7_GPU_PROFILE *gm, P7_GPU_OPROFILE *om);
 /* Imports tables of an already converted P7_OPROFILE (de-striped by the caller). */
 int   p7_gpu_oprofile_Load(P7_GPU_OPROFILE *om, const P7_GPU_TABLES *t);
+/* Reads model `which` (0-based) of a pressed profile database: the .h3f and
+ * .h3p files hmmpress writes (format 3/f), as p7_oprofile_ReadMSV +
+ * p7_oprofile_ReadRest do (HMMER/src/impl_sse/io.c:233-338, 493-581), with
+ * the SSE striping removed. p7G_EOF when the files hold fewer models. */
+int   p7_gpu_oprofile_ReadPressed(const char *h3f_path, const char *h3p_path, int which, P7_GPU_OPROFILE **ret_om);
 /* Exports the tables; pointers stay valid until the next Load/Convert/Destroy. */
 int   p7_gpu_oprofile_GetTables(const P7_GPU_OPROFILE *om, P7_GPU_TABLES *t);
 /* Replace p7_oprofile_ReconfigLength / MSVLength / RestLength (p7_oprofile.c:1062-1127). */
