@@ -326,6 +326,8 @@ struct p7g_seqdb_s {
   int64_t *d_off;
   int32_t *d_len;
   int32_t *d_perm;         // slot -> caller index, on the device
+  char    *names;          // sequence names in caller order (when read from a file), NUL-separated
+  int64_t *name_off;       // [caller index] offset into names
 };
 
 extern "C" void
@@ -337,6 +339,7 @@ p7_gpu_seqdb_Destroy(P7_GPU_SEQDB *db)
   if (db->d_len) (void) hipFree(db->d_len);
   if (db->d_perm) (void) hipFree(db->d_perm);
   free(db->h_len); free(db->h_perm); free(db->h_inv);
+  free(db->names); free(db->name_off);
   free(db);
 }
 
@@ -399,6 +402,188 @@ p7_gpu_seqdb_Create(const uint8_t *residues, const int64_t *offset, const int32_
   if (e == hipSuccess) e = hipMemcpyAsync(db->d_perm, h_perm32.data(), sizeof(int32_t) * (size_t) nseq, hipMemcpyHostToDevice, g_stream);
   if (e == hipSuccess) e = hipStreamSynchronize(g_stream);
   free(h_res); free(h_off);
+  if (e != hipSuccess) { p7g_set_error("database upload failed: %s", hipGetErrorString(e)); p7_gpu_seqdb_Destroy(db); return p7G_EDEVICE; }
+  *ret_db = db;
+  return p7G_OK;
+}
+
+/*****************************************************************
+ * Resident databases from files: FASTA (as hmmpgmd's sequence cache loads
+ * it, HMMER/src/cachedb.c:41-280) and a binary image of the device layout
+ *****************************************************************/
+extern "C" int
+p7_gpu_seqdb_ReadFasta(const char *path, int abctype, P7_GPU_SEQDB **ret_db)
+{
+  *ret_db = nullptr;
+  const P7G_ALPHABET *abc = p7g_alphabet(abctype);
+  if (! abc) { p7g_set_error("unknown alphabet type %d", abctype); return p7G_EINVAL; }
+  FILE *fp = fopen(path, "rb");
+  if (! fp) { p7g_set_error("can't open %s", path); return p7G_ENOTFOUND; }
+  std::vector<uint8_t> res;
+  std::vector<int64_t> off;
+  std::vector<int32_t> len;
+  std::vector<char>    names;
+  std::vector<int64_t> noff;
+  std::vector<char> buf(1 << 20);
+  int64_t lineno = 0;
+  // in_header: reading the name (first word) of a '>' line; each header adds one NUL-terminated name
+  bool in_header = false, at_line_start = true, first_line = true, skip_line = false;
+  int status = p7G_OK;
+  size_t got;
+  while (status == p7G_OK && (got = fread(buf.data(), 1, buf.size(), fp)) > 0) {
+    for (size_t i = 0; i < got; i++) {
+      const unsigned char c = (unsigned char) buf[i];
+      if (c == '\n') {
+        if (in_header) { names.push_back('\0'); in_header = false; }
+        lineno++; at_line_start = true; first_line = false; skip_line = false;
+        continue;
+      }
+      if (skip_line) continue;
+      if (at_line_start) {
+        at_line_start = false;
+        // the cache format starts with a '#' line of counts (cachedb.c:79-95): skip it
+        if (first_line && c == '#') { skip_line = true; continue; }
+        if (c == '>') {
+          if (! len.empty()) len.back() = (int32_t) ((int64_t) res.size() - off.back());
+          off.push_back((int64_t) res.size()); len.push_back(0);
+          noff.push_back((int64_t) names.size());
+          in_header = true;
+          continue;
+        }
+      }
+      if (in_header) {
+        if (c == ' ' || c == '\t' || c == '\r') {
+          if ((int64_t) names.size() > noff.back()) { names.push_back('\0'); in_header = false; skip_line = true; }   // the description is dropped
+          continue;
+        }
+        names.push_back((char) c);
+        continue;
+      }
+      if (c == ' ' || c == '\t' || c == '\r') continue;
+      if (off.empty()) { p7g_set_error("%s line %lld: residues before the first '>' header", path, (long long) lineno + 1); status = p7G_EFORMAT; break; }
+      const int x = (c < 128) ? abc->inmap[c] : -1;
+      if (x < 0) { p7g_set_error("%s line %lld: illegal residue '%c'", path, (long long) lineno + 1, c); status = p7G_EFORMAT; break; }
+      res.push_back((uint8_t) x);
+    }
+  }
+  fclose(fp);
+  if (status != p7G_OK) return status;
+  if (in_header) names.push_back('\0');
+  if (off.empty()) { p7g_set_error("%s holds no sequences", path); return p7G_EFORMAT; }
+  len.back() = (int32_t) ((int64_t) res.size() - off.back());
+  if (res.empty()) res.push_back(0);
+  P7_GPU_SEQDB *db = nullptr;
+  if ((status = p7_gpu_seqdb_Create(res.data(), off.data(), len.data(), (int64_t) off.size(), &db)) != p7G_OK) return status;
+  db->names = (char *) malloc(names.size() ? names.size() : 1);
+  db->name_off = (int64_t *) malloc(sizeof(int64_t) * noff.size());
+  if (! db->names || ! db->name_off) { p7_gpu_seqdb_Destroy(db); return p7G_EMEM; }
+  if (! names.empty()) memcpy(db->names, names.data(), names.size()); else db->names[0] = '\0';
+  memcpy(db->name_off, noff.data(), sizeof(int64_t) * noff.size());
+  *ret_db = db;
+  return p7G_OK;
+}
+
+extern "C" const char *
+p7_gpu_seqdb_Name(const P7_GPU_SEQDB *db, int64_t i)
+{
+  if (! db || ! db->names || i < 0 || i >= db->nseq) return nullptr;
+  return db->names + db->name_off[i];
+}
+
+// Binary image: magic, nseq, nres, bytes, Lmax, maxcode, h_len[nseq], h_perm[nseq],
+// the padded residues as laid out in HBM, then the names (size, bytes, offsets).
+#define P7G_DBMAGIC 0x50374744u        // "DG7P"
+
+extern "C" int
+p7_gpu_seqdb_Save(const P7_GPU_SEQDB *db, const char *path)
+{
+  std::lock_guard<std::mutex> lk(g_lock);
+  int status;
+  if ((status = ensure_init()) != p7G_OK) return status;
+  std::vector<uint8_t> h((size_t) db->bytes);
+  HIPCHK(hipMemcpy(h.data(), db->d_res, (size_t) db->bytes, hipMemcpyDeviceToHost));
+  FILE *fp = fopen(path, "wb");
+  if (! fp) { p7g_set_error("can't write %s", path); return p7G_ESYS; }
+  const uint32_t magic = P7G_DBMAGIC;
+  int64_t namebytes = 0;                 // 0: no names
+  if (db->names && db->name_off)
+    for (int64_t i = 0; i < db->nseq; i++)
+      namebytes = std::max<int64_t>(namebytes, db->name_off[i] + (int64_t) strlen(db->names + db->name_off[i]) + 1);
+  bool ok = fwrite(&magic, 4, 1, fp) == 1 && fwrite(&db->nseq, 8, 1, fp) == 1 && fwrite(&db->nres, 8, 1, fp) == 1 &&
+            fwrite(&db->bytes, 8, 1, fp) == 1 && fwrite(&db->Lmax, 4, 1, fp) == 1 && fwrite(&db->maxcode, 4, 1, fp) == 1 &&
+            fwrite(db->h_len, 4, (size_t) db->nseq, fp) == (size_t) db->nseq &&
+            fwrite(db->h_perm, 8, (size_t) db->nseq, fp) == (size_t) db->nseq &&
+            fwrite(h.data(), 1, (size_t) db->bytes, fp) == (size_t) db->bytes &&
+            fwrite(&namebytes, 8, 1, fp) == 1;
+  if (ok && namebytes) ok = fwrite(db->names, 1, (size_t) namebytes, fp) == (size_t) namebytes &&
+                            fwrite(db->name_off, 8, (size_t) db->nseq, fp) == (size_t) db->nseq;
+  ok = ok && fwrite(&magic, 4, 1, fp) == 1;
+  if (fclose(fp) != 0) ok = false;
+  if (! ok) { p7g_set_error("failed writing %s", path); return p7G_ESYS; }
+  return p7G_OK;
+}
+
+extern "C" int
+p7_gpu_seqdb_Load(const char *path, P7_GPU_SEQDB **ret_db)
+{
+  int status;
+  *ret_db = nullptr;
+  if ((status = ensure_init()) != p7G_OK) return status;
+  FILE *fp = fopen(path, "rb");
+  if (! fp) { p7g_set_error("can't open %s", path); return p7G_ENOTFOUND; }
+  P7_GPU_SEQDB *db = (P7_GPU_SEQDB *) calloc(1, sizeof(P7_GPU_SEQDB));
+  uint32_t magic = 0;
+  int64_t namebytes = 0;
+  std::vector<uint8_t> h;
+  std::vector<int64_t> h_off;
+  bool ok = db && fread(&magic, 4, 1, fp) == 1 && magic == P7G_DBMAGIC &&
+            fread(&db->nseq, 8, 1, fp) == 1 && fread(&db->nres, 8, 1, fp) == 1 && fread(&db->bytes, 8, 1, fp) == 1 &&
+            fread(&db->Lmax, 4, 1, fp) == 1 && fread(&db->maxcode, 4, 1, fp) == 1 &&
+            db->nseq > 0 && db->nseq < INT32_MAX && db->bytes > 0 && db->maxcode >= 0 && db->maxcode < 29;
+  if (ok) {
+    db->h_len = (int32_t *) malloc(4 * (size_t) db->nseq);
+    db->h_perm = (int64_t *) malloc(8 * (size_t) db->nseq);
+    db->h_inv = (int32_t *) malloc(4 * (size_t) db->nseq);
+    ok = db->h_len && db->h_perm && db->h_inv &&
+         fread(db->h_len, 4, (size_t) db->nseq, fp) == (size_t) db->nseq &&
+         fread(db->h_perm, 8, (size_t) db->nseq, fp) == (size_t) db->nseq;
+  }
+  if (ok) {
+    // the layout follows from the lengths: each target 16-byte aligned, padded to 16
+    h_off.resize((size_t) db->nseq);
+    int64_t pos = 0, nres = 0;
+    for (int64_t s = 0; s < db->nseq && ok; s++) {
+      const int64_t i = db->h_perm[s];
+      ok = db->h_len[s] >= 0 && i >= 0 && i < db->nseq;
+      if (ok) { db->h_inv[i] = (int32_t) s; h_off[(size_t) s] = pos; pos += ((int64_t) db->h_len[s] + 15) & ~(int64_t) 15; nres += db->h_len[s]; }
+    }
+    ok = ok && pos + 16 == db->bytes && nres == db->nres;
+  }
+  if (ok) { h.resize((size_t) db->bytes); ok = fread(h.data(), 1, (size_t) db->bytes, fp) == (size_t) db->bytes; }
+  if (ok) ok = fread(&namebytes, 8, 1, fp) == 1 && namebytes >= 0;
+  if (ok && namebytes > 0) {
+    db->names = (char *) malloc((size_t) namebytes);
+    db->name_off = (int64_t *) malloc(8 * (size_t) db->nseq);
+    ok = db->names && db->name_off && fread(db->names, 1, (size_t) namebytes, fp) == (size_t) namebytes &&
+         fread(db->name_off, 8, (size_t) db->nseq, fp) == (size_t) db->nseq;
+    for (int64_t i = 0; ok && i < db->nseq; i++) ok = db->name_off[i] >= 0 && db->name_off[i] < namebytes;
+    ok = ok && db->names[namebytes - 1] == '\0';
+  }
+  if (ok) ok = fread(&magic, 4, 1, fp) == 1 && magic == P7G_DBMAGIC;
+  fclose(fp);
+  if (! ok) { p7g_set_error("%s is not a valid database image", path); p7_gpu_seqdb_Destroy(db); return p7G_EFORMAT; }
+  for (int64_t b = 0; b < db->bytes; b++) if (h[(size_t) b] >= 32) { p7g_set_error("%s: bad residue code", path); p7_gpu_seqdb_Destroy(db); return p7G_EFORMAT; }
+  std::vector<int32_t> perm32((size_t) db->nseq);
+  for (int64_t s = 0; s < db->nseq; s++) perm32[(size_t) s] = (int32_t) db->h_perm[s];
+  hipError_t e = hipMalloc((void **) &db->d_res, (size_t) db->bytes);
+  if (e == hipSuccess) e = hipMalloc((void **) &db->d_off, sizeof(int64_t) * (size_t) db->nseq);
+  if (e == hipSuccess) e = hipMalloc((void **) &db->d_len, sizeof(int32_t) * (size_t) db->nseq);
+  if (e == hipSuccess) e = hipMalloc((void **) &db->d_perm, sizeof(int32_t) * (size_t) db->nseq);
+  if (e == hipSuccess) e = hipMemcpyAsync(db->d_res, h.data(), (size_t) db->bytes, hipMemcpyHostToDevice, g_stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(db->d_off, h_off.data(), sizeof(int64_t) * (size_t) db->nseq, hipMemcpyHostToDevice, g_stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(db->d_len, db->h_len, sizeof(int32_t) * (size_t) db->nseq, hipMemcpyHostToDevice, g_stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(db->d_perm, perm32.data(), sizeof(int32_t) * (size_t) db->nseq, hipMemcpyHostToDevice, g_stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(g_stream);
   if (e != hipSuccess) { p7g_set_error("database upload failed: %s", hipGetErrorString(e)); p7_gpu_seqdb_Destroy(db); return p7G_EDEVICE; }
   *ret_db = db;
   return p7G_OK;

@@ -98,6 +98,10 @@ def _load():
         "p7_gpu_pipeline_GetStats": (i, [vp, POINTER(Stats)]),
         "p7_gpu_ScoreThreshold": (f, [i, d, d, d]),
         "p7_gpu_oprofile_ReadPressed": (i, [c_char_p, c_char_p, i, POINTER(vp)]),
+        "p7_gpu_seqdb_ReadFasta": (i, [c_char_p, i, POINTER(vp)]),
+        "p7_gpu_seqdb_Name": (c_char_p, [vp, i64]),
+        "p7_gpu_seqdb_Save": (i, [vp, c_char_p]),
+        "p7_gpu_seqdb_Load": (i, [c_char_p, POINTER(vp)]),
         "p7_gpu_server_Create": (vp, [vp, i, ctypes.c_size_t, d, d, d, i]),
         "p7_gpu_server_Slots": (i, [vp]),
         "p7_gpu_server_Submit": (i, [vp, vp, POINTER(Hits), POINTER(c_int64)]),
@@ -383,6 +387,36 @@ class SeqDB:
         if res.size == 0:
             res = np.zeros(1, np.uint8)
         return cls(res, offsets, lengths)
+
+    @classmethod
+    def _wrap(cls, h):
+        db = cls.__new__(cls)
+        db.h = h
+        db.nseq = int(lib.p7_gpu_seqdb_NSeq(h))
+        db.nres = int(lib.p7_gpu_seqdb_NRes(h))
+        db.residues = db.offsets = db.lengths = None       # resident on the device only
+        return db
+
+    @classmethod
+    def from_fasta(cls, path, abctype=AMINO):
+        """Native FASTA reader straight into HBM (p7_gpu_seqdb_ReadFasta)."""
+        h = c_void_p()
+        _check(lib.p7_gpu_seqdb_ReadFasta(str(path).encode(), abctype, ctypes.byref(h)), "seqdb_ReadFasta")
+        return cls._wrap(h)
+
+    @classmethod
+    def load(cls, path):
+        """A database image written by save()."""
+        h = c_void_p()
+        _check(lib.p7_gpu_seqdb_Load(str(path).encode(), ctypes.byref(h)), "seqdb_Load")
+        return cls._wrap(h)
+
+    def save(self, path):
+        _check(lib.p7_gpu_seqdb_Save(self.h, str(path).encode()), "seqdb_Save")
+
+    def name(self, i):
+        s = lib.p7_gpu_seqdb_Name(self.h, i)
+        return s.decode() if s is not None else None
 
     def seq(self, i):
         o, L = int(self.offsets[i]), int(self.lengths[i])

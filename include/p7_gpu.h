@@ -199,6 +199,18 @@ int   p7_gpu_BackwardParser(const uint8_t *dsq, int L, const P7_GPU_OPROFILE *om
  * grouped by length; results are always reported in the caller's order. */
 int     p7_gpu_seqdb_Create(const uint8_t *residues, const int64_t *offset, const int32_t *length,
                             int64_t nseq, P7_GPU_SEQDB **ret_db);
+/* Reads a FASTA file (first word of each '>' line is the name; a first line
+ * starting with '#', the summary line of hmmpgmd's cache format, is skipped)
+ * into the HBM database, as p7_seqcache_Open (HMMER/src/cachedb.c:41-280)
+ * loads hmmpgmd's resident cache. Residues follow Easel's digital alphabets. */
+int     p7_gpu_seqdb_ReadFasta(const char *path, int abctype, P7_GPU_SEQDB **ret_db);
+/* Name of sequence i (file order), or NULL when the database has no names. */
+const char *p7_gpu_seqdb_Name(const P7_GPU_SEQDB *db, int64_t i);
+/* Writes / reads an image of the device layout (length-sorted, padded residues,
+ * names), so a resident database comes back without parsing or sorting.
+ * No counterpart in the reference. */
+int     p7_gpu_seqdb_Save(const P7_GPU_SEQDB *db, const char *path);
+int     p7_gpu_seqdb_Load(const char *path, P7_GPU_SEQDB **ret_db);
 int64_t p7_gpu_seqdb_NSeq(const P7_GPU_SEQDB *db);
 int64_t p7_gpu_seqdb_NRes(const P7_GPU_SEQDB *db);
 void    p7_gpu_seqdb_Destroy(P7_GPU_SEQDB *db);
