@@ -44,6 +44,8 @@ typedef short s2v __attribute__((ext_vector_type(2)));
 #define LIST_BLOCK 512
 // Target groups the ungapped pass takes from its work queue per atomic.
 #define SSV_GRAB 4
+// 16-byte loads the bias filter's ordered sum keeps in flight.
+#define BIAS_AHEAD 8
 
 __device__ __forceinline__ int   s2i(s2v v) { return __builtin_bit_cast(int, v); }
 __device__ __forceinline__ s2v   i2s(int v) { return __builtin_bit_cast(s2v, v); }
@@ -557,16 +559,23 @@ void k_bias_sum(const DevProfile p, const int32_t *__restrict__ len, const int32
     const float4 *q = (const float4 *) (rowmax + o);
     const int nq = (nv + 3) >> 2;
     float logsc = 0.0f;
-    float4 a = q[0], b = (nq > 1) ? q[1] : a;
-    for (int k = 0; k < nq; k++) {
-      const float4 v = a;
-      a = b;
-      if (k + 2 < nq) b = q[k + 2];
-      const int rem = nv - 4 * k;
-      logsc += v.x;
-      if (rem > 1) logsc += v.y;
-      if (rem > 2) logsc += v.z;
-      if (rem > 3) logsc += v.w;
+    // a ring of BIAS_AHEAD loads in flight: the ordered sum itself is a chain of dependent adds
+    float4 ring[BIAS_AHEAD];
+#pragma unroll
+    for (int u = 0; u < BIAS_AHEAD; u++) ring[u] = (u < nq) ? q[u] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    for (int k0 = 0; k0 < nq; k0 += BIAS_AHEAD) {
+#pragma unroll
+      for (int u = 0; u < BIAS_AHEAD; u++) {
+        const int k = k0 + u;
+        if (k >= nq) break;
+        const float4 v = ring[u];
+        if (k + BIAS_AHEAD < nq) ring[u] = q[k + BIAS_AHEAD];
+        const int rem = nv - 4 * k;
+        logsc += v.x;
+        if (rem > 1) logsc += v.y;
+        if (rem > 2) logsc += v.z;
+        if (rem > 3) logsc += v.w;
+      }
     }
     bias_finish(p, r, g, gating, lc, slot, L, logsc);
   }

@@ -168,7 +168,8 @@ dev_profile_build(P7_GPU_OPROFILE *om, int Lmax_need, struct p7g_dev_profile_s *
   // ---- Viterbi tables: int16 pairs, capacity >= M+1 ----
   p.vf_ND = pick(vf_nd_set, (int) (sizeof(vf_nd_set)/sizeof(int)), cdiv(cdiv(M + 1, 64), 2));
   if (p.vf_ND < 0) { delete d; p7g_set_error("model too long for the device Viterbi stage (M=%d)", M); return p7G_EINVAL; }
-  p.vf_dwords = (size_t) (NR + P7G_VF_NT) * 64 * p.vf_ND;
+  const int G = 64;
+  p.vf_dwords = (size_t) (NR + P7G_VF_NT) * G * p.vf_ND;
   std::vector<int> vf(p.vf_dwords);
   {
     const int ND = p.vf_ND, B = 2 * ND;
@@ -188,13 +189,13 @@ dev_profile_build(P7_GPU_OPROFILE *om, int Lmax_need, struct p7g_dev_profile_s *
       return -32768;
     };
     for (int r = 0; r < NR + P7G_VF_NT; r++)
-      for (int z = 0; z < 64; z++)
+      for (int z = 0; z < G; z++)
         for (int dd = 0; dd < ND; dd++) {
           const int k0 = z*B + dd + 1, k1 = z*B + dd + ND + 1;
           int a, b;
           if (r < NR) { a = ew(r, k0); b = ew(r, k1); }
           else        { a = tv(r - NR, k0); b = tv(r - NR, k1); }
-          vf[((size_t) r * ND + dd) * 64 + z] = pack2(a, b);
+          vf[((size_t) r * ND + dd) * G + z] = pack2(a, b);
         }
   }
 
