@@ -754,23 +754,61 @@ void k_vit(const DevProfile p, const uint8_t *__restrict__ res, const int64_t *_
 /*****************************************************************
  * 7. Forward parser (one target per wave, fp32, B positions per lane)
  *****************************************************************/
+// The D->D chain of a parser row as an affine scan: lane maps f(x) = P x + C
+// composed over 16-lane rows with DPP, rows joined by readlane carries. The
+// products P depend only on the profile's D->D odds: they are computed once
+// per kernel (AffConst, with the same multiplications in the same order as a
+// per-row computation would make), so a row costs the C recurrence alone.
+struct AffConst {
+  float w1, w2, w4, w8;      // window products the four row steps multiply by
+  float prow;                // product over the row part the lane's window covers after the steps
+  float pa, pb;              // wave-uniform row products used by the row carries
+};
+
 template <int B>
-__device__ __forceinline__ void affine_fwd_scan(float (&D)[B], const float (&a)[B], int z)
+__device__ __forceinline__ void aff_fwd_setup(AffConst &k, const float (&a)[B])
 {
-  // D(j) = D0(j) + a(j) * D(j-1), carried across lanes in increasing k:
-  // lane maps f(x) = P x + C, scanned over 16-lane rows with DPP, row carries by readlane.
-  float P = 1.0f, C = 0.0f;
+  float P = 1.0f;
 #pragma unroll
-  for (int j = 0; j < B; j++) { C = D[j] + a[j] * C; P *= a[j]; }
-#define AF_STEP(CTRL) { const float Pp = dppf<CTRL>(1.0f, P), Cp = dppf<CTRL>(0.0f, C); C = C + P * Cp; P = P * Pp; }
-  AF_STEP(ROW_SHR1) AF_STEP(ROW_SHR2) AF_STEP(ROW_SHR4) AF_STEP(ROW_SHR8)
+  for (int j = 0; j < B; j++) P *= a[j];
+  k.w1 = P; P = P * dppf<ROW_SHR1>(1.0f, P);
+  k.w2 = P; P = P * dppf<ROW_SHR2>(1.0f, P);
+  k.w4 = P; P = P * dppf<ROW_SHR4>(1.0f, P);
+  k.w8 = P; P = P * dppf<ROW_SHR8>(1.0f, P);
+  k.prow = P; k.pa = readlanef(P, 31); k.pb = readlanef(P, 47);
+}
+
+template <int B>
+__device__ __forceinline__ void aff_bck_setup(AffConst &k, const float (&a)[B])
+{
+  float P = 1.0f;
+#pragma unroll
+  for (int j = B-1; j >= 0; j--) P *= a[j];
+  k.w1 = P; P = P * dppf<ROW_SHL1>(1.0f, P);
+  k.w2 = P; P = P * dppf<ROW_SHL2>(1.0f, P);
+  k.w4 = P; P = P * dppf<ROW_SHL4>(1.0f, P);
+  k.w8 = P; P = P * dppf<ROW_SHL8>(1.0f, P);
+  k.prow = P; k.pa = readlanef(P, 32); k.pb = readlanef(P, 16);
+}
+
+template <int B>
+__device__ __forceinline__ void affine_fwd_scan(float (&D)[B], const float (&a)[B], const AffConst &k, int z)
+{
+  // D(j) = D0(j) + a(j) * D(j-1), carried across lanes in increasing k
+  float C = 0.0f;
+#pragma unroll
+  for (int j = 0; j < B; j++) C = D[j] + a[j] * C;
+  C = C + k.w1 * dppf<ROW_SHR1>(0.0f, C);
+  C = C + k.w2 * dppf<ROW_SHR2>(0.0f, C);
+  C = C + k.w4 * dppf<ROW_SHR4>(0.0f, C);
+  C = C + k.w8 * dppf<ROW_SHR8>(0.0f, C);
   {
     const float c1 = readlanef(C, 15);
-    const float c2 = readlanef(P, 31) * c1 + readlanef(C, 31);
-    const float c3 = readlanef(P, 47) * c2 + readlanef(C, 47);
+    const float c2 = k.pa * c1 + readlanef(C, 31);
+    const float c3 = k.pb * c2 + readlanef(C, 47);
     const int row = z >> 4;
     const float rc = (row == 0) ? 0.0f : (row == 1) ? c1 : (row == 2) ? c2 : c3;
-    C = C + P * rc;
+    C = C + k.prow * rc;
   }
   const float cin = dppf<WAVE_SHR1>(0.0f, C);
   float cur = cin;
@@ -779,21 +817,23 @@ __device__ __forceinline__ void affine_fwd_scan(float (&D)[B], const float (&a)[
 }
 
 template <int B>
-__device__ __forceinline__ void affine_bck_scan(float (&D)[B], const float (&a)[B], int z)
+__device__ __forceinline__ void affine_bck_scan(float (&D)[B], const float (&a)[B], const AffConst &k, int z)
 {
-  // D(j) = c(j) + a(j) * D(j+1), carried across lanes in decreasing k.
-  float P = 1.0f, C = 0.0f;
+  // D(j) = c(j) + a(j) * D(j+1), carried across lanes in decreasing k
+  float C = 0.0f;
 #pragma unroll
-  for (int j = B-1; j >= 0; j--) { C = D[j] + a[j] * C; P *= a[j]; }
-  AF_STEP(ROW_SHL1) AF_STEP(ROW_SHL2) AF_STEP(ROW_SHL4) AF_STEP(ROW_SHL8)
-#undef AF_STEP
+  for (int j = B-1; j >= 0; j--) C = D[j] + a[j] * C;
+  C = C + k.w1 * dppf<ROW_SHL1>(0.0f, C);
+  C = C + k.w2 * dppf<ROW_SHL2>(0.0f, C);
+  C = C + k.w4 * dppf<ROW_SHL4>(0.0f, C);
+  C = C + k.w8 * dppf<ROW_SHL8>(0.0f, C);
   {
     const float c2 = readlanef(C, 48);
-    const float c1 = readlanef(P, 32) * c2 + readlanef(C, 32);
-    const float c0 = readlanef(P, 16) * c1 + readlanef(C, 16);
+    const float c1 = k.pa * c2 + readlanef(C, 32);
+    const float c0 = k.pb * c1 + readlanef(C, 16);
     const int row = z >> 4;
     const float rc = (row == 3) ? 0.0f : (row == 2) ? c2 : (row == 1) ? c1 : c0;
-    C = C + P * rc;
+    C = C + k.prow * rc;
   }
   const float cin = dppf<WAVE_SHL1>(0.0f, C);
   float cur = cin;
@@ -820,6 +860,13 @@ void k_fwd(const DevProfile p, const uint8_t *__restrict__ res, const int64_t *_
   for (int j = 0; j < B; j++) {
     tBM[j] = tr[VT_BM*rowf + j*64]; tMM[j] = tr[VT_MM*rowf + j*64]; tIM[j] = tr[VT_IM*rowf + j*64]; tDM[j] = tr[VT_DM*rowf + j*64];
     tMDS[j] = tr[VT_MDS*rowf + j*64]; tMI[j] = tr[VT_MI*rowf + j*64]; tII[j] = tr[VT_II*rowf + j*64];
+  }
+  AffConst akc;
+  {
+    float tdd[B];
+#pragma unroll
+    for (int j = 0; j < B; j++) tdd[j] = tr[VT_DDS*rowf + j*64];
+    aff_fwd_setup<B>(akc, tdd);
   }
   const int64_t n = *count;
 
@@ -873,7 +920,7 @@ void k_fwd(const DevProfile p, const uint8_t *__restrict__ res, const int64_t *_
         float tDDS[B];
 #pragma unroll
         for (int j = 0; j < B; j++) tDDS[j] = tr[VT_DDS*rowf + j*64];
-        affine_fwd_scan<B>(D, tDDS, z);
+        affine_fwd_scan<B>(D, tDDS, akc, z);
       }
 #pragma unroll
       for (int j = 0; j < B; j++) xEl += D[j];
@@ -938,6 +985,8 @@ void k_bck(const DevProfile p, const uint8_t *__restrict__ res, const int64_t *_
     tBM[j] = tr[VT_BM*rowf + j*64]; tMMN[j] = tr[FT_MMN*rowf + j*64]; tIMN[j] = tr[FT_IMN*rowf + j*64]; tDMN[j] = tr[FT_DMN*rowf + j*64];
     tMD[j] = tr[FT_MD*rowf + j*64]; tDD[j] = tr[FT_DD*rowf + j*64]; tMI[j] = tr[VT_MI*rowf + j*64]; tII[j] = tr[VT_II*rowf + j*64];
   }
+  AffConst akc;
+  aff_bck_setup<B>(akc, tDD);
   const int64_t n = *count;
 
   int32_t *queue = (int32_t *) count + P7G_QOFF;
@@ -960,7 +1009,7 @@ void k_bck(const DevProfile p, const uint8_t *__restrict__ res, const int64_t *_
     float xE = xC * p.xfE_move;
 #pragma unroll
     for (int j = 0; j < B; j++) { M[j] = xE; D[j] = xE; I[j] = 0.0f; }
-    affine_bck_scan<B>(D, tDD, z);
+    affine_bck_scan<B>(D, tDD, akc, z);
     {
       const float dS = wave_shl1f(D[0]);
 #pragma unroll
@@ -1007,7 +1056,7 @@ void k_bck(const DevProfile p, const uint8_t *__restrict__ res, const int64_t *_
       xE = (xC * p.xfE_move) + (xJ * p.xfE_loop);
 #pragma unroll
       for (int j = 0; j < B; j++) D[j] = D[j] + xE;
-      affine_bck_scan<B>(D, tDD, z);
+      affine_bck_scan<B>(D, tDD, akc, z);
       const float dS = wave_shl1f(D[0]);
 #pragma unroll
       for (int j = 0; j < B; j++) M[j] = M[j] + xE + (j < B-1 ? D[j+1] : dS) * tMD[j];
