@@ -16,15 +16,25 @@
 enum { VT_BM = 0, VT_MM, VT_IM, VT_DM, VT_MDS, VT_MI, VT_II, VT_DDS,
        FT_MMN = 8, FT_IMN, FT_DMN, FT_MD, FT_DD };
 
+// One shape of the ungapped MSV pass: G lanes per target (64/G targets per
+// wave), ND dwords (2*ND model positions) per lane. Its table holds int16
+// pairs of (bias - rbv); per residue row, ND/4 quads [q][G] of int4, then
+// ND%4 dwords [t][G]: [rows][G * ND] in all.
+struct SsvVariant {
+  int    G, ND;          // G == 0: the model does not fit this lane count
+  int   *tbl;
+  size_t dwords;
+};
+#define P7G_SSV_NVAR 3   // G = 16, 32, 64
+
 struct DevProfile {
   int M, Kp;
   // Every table is lane-minor: the values one read instruction fetches for
   // consecutive lanes are consecutive in memory, so LDS reads are free of
   // bank conflicts for any per-lane block size.
-  // MSV stage, ungapped (SSV) pass: int16 pairs of (bias - rbv), 16 or 64 lanes
-  // per target; per residue row, ND/4 quads [q][G] of int4, then ND%4 dwords [t][G].
-  int   ssv_G, ssv_ND;
-  int  *ssv_tbl;     size_t ssv_dwords;     // [rows][G * ND]
+  // MSV stage, ungapped pass: the variants with 16, 32 and 64 lanes per target
+  SsvVariant ssv[P7G_SSV_NVAR];
+  int   ssv_def;         // variant used when no tuned choice applies
   // MSV stage, full recurrence with the J state: same scores, 64 lanes
   int   mf_ND;
   int  *mf_tbl;      size_t mf_dwords;      // [rows][ND][64]
@@ -103,11 +113,31 @@ struct GateThresh {
   int   do_bias, do_max;
 };
 
+// Tables of the profiles one multi-profile launch of the ungapped pass
+// serves, passed by value (kernel arguments).
+#define P7G_SSV_MAXJOBS 192
+struct SsvJobs { const int *tbl[P7G_SSV_MAXJOBS]; };
+
 #ifdef __HIPCC__
 #include <hip/hip_runtime.h>
+// launchers (p7g_ssv.hip)
+// The ungapped pass of variant v of one profile over slots[0..n) (slots ==
+// NULL: every slot); waves take `grab` groups of targets per queue atomic.
+hipError_t launch_ssv(const DevProfile &p, int v, const DevSeqs &s, const int32_t *slots, int64_t n, uint16_t *zmax,
+                      int32_t *queue, int grab, hipStream_t st);
+// The ungapped pass of njobs profiles that share one (G, ND) shape over every
+// slot: one persistent launch per P7G_SSV_MAXJOBS profiles, whose blocks move
+// from profile to profile as work queues drain. Profile j reads tbl[j], writes
+// zmax + j * zstride and takes work from queues[j] (zero on entry).
+hipError_t launch_ssv_multi(int G, int ND, const int *const *tbl, int njobs, const DevSeqs &s, int64_t n,
+                            uint16_t *zmax, int64_t zstride, int32_t *queues, int grab, hipStream_t st);
+// 1 if a kernel of this shape is built
+int ssv_shape_built(int G, int ND);
 // launchers (p7g_kernels.hip)
-hipError_t launch_ssv(const DevProfile &p, const DevSeqs &s, const int32_t *slots, int64_t n, uint16_t *zmax, int32_t *queue,
-                      hipStream_t st);
+// p[0..nwords) = v, as a kernel (so that a captured graph holds no memset node)
+hipError_t launch_fill32(void *p, uint32_t v, int64_t nwords, hipStream_t st);
+// dst[0..nwords) = src[0..nwords), device to device, as a kernel
+hipError_t launch_copy32(void *dst, const void *src, int64_t nwords, hipStream_t st);
 // With slots == NULL (every slot) targets that need the full recurrence are
 // marked xj = -2 for launch_compact_i16; with a slot list they are appended.
 hipError_t launch_msv_classify(const DevProfile &p, const DevSeqs &s, const int32_t *slots, int64_t n,

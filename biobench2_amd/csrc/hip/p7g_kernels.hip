@@ -26,292 +26,7 @@
 // when it holds the D->D closure is a max-plus prefix scan across the wave.
 // Forward/Backward parsers (fwdback.c:258-735): fp32, the D->D chain as an
 // affine prefix scan across the wave, sparse rescaling when xE > 1e4.
-#include <hip/hip_runtime.h>
-#include <stdint.h>
-#include <math.h>
-#include <algorithm>
-#include "p7g_device.h"
-
-typedef short s2v __attribute__((ext_vector_type(2)));
-
-#define NEG_PAIR 0x80008000u     // (-32768, -32768)
-#define LOG2_D   0.69314718055994529
-#ifndef SSV_BLOCK
-#define SSV_BLOCK 512
-#endif
-// Workgroup size of the one-target-per-wave list stages: 8 waves share one LDS
-// copy of the profile.
-#define LIST_BLOCK 512
-// Target groups the ungapped pass takes from its work queue per atomic.
-#define SSV_GRAB 4
-// 16-byte loads the bias filter's ordered sum keeps in flight.
-#define BIAS_AHEAD 8
-
-__device__ __forceinline__ int   s2i(s2v v) { return __builtin_bit_cast(int, v); }
-__device__ __forceinline__ s2v   i2s(int v) { return __builtin_bit_cast(s2v, v); }
-__device__ __forceinline__ int   adds2(int a, int b) { return s2i(__builtin_elementwise_add_sat(i2s(a), i2s(b))); }
-__device__ __forceinline__ int   max2(int a, int b)  { return s2i(__builtin_elementwise_max(i2s(a), i2s(b))); }
-__device__ __forceinline__ int   lo16(int v) { return (int)(short)(v & 0xffff); }
-__device__ __forceinline__ int   hi16(int v) { return (int)(short)((unsigned)v >> 16); }
-__device__ __forceinline__ int   pair16(int v) { return (int)(((unsigned)v & 0xffffu) | ((unsigned)v << 16)); }
-
-// DPP lane moves. Lanes without a source receive `fill`.
-__device__ __forceinline__ int wave_shr1(int v, int fill) { return __builtin_amdgcn_update_dpp(fill, v, 0x138, 0xf, 0xf, false); }
-__device__ __forceinline__ int wave_shl1(int v, int fill) { return __builtin_amdgcn_update_dpp(fill, v, 0x130, 0xf, 0xf, false); }
-__device__ __forceinline__ int row_shr1(int v, int fill)  { return __builtin_amdgcn_update_dpp(fill, v, 0x111, 0xf, 0xf, false); }
-__device__ __forceinline__ float wave_shr1f(float v) { return __builtin_bit_cast(float, wave_shr1(__builtin_bit_cast(int, v), 0)); }
-__device__ __forceinline__ float wave_shl1f(float v) { return __builtin_bit_cast(float, wave_shl1(__builtin_bit_cast(int, v), 0)); }
-
-// Row shift of a packed block: the dword that becomes the new position pair 0.
-template <int G>
-__device__ __forceinline__ int shift_in(int last, int fill)
-{
-  int nb = (G == 16) ? row_shr1(last, fill) : wave_shr1(last, fill);
-  return (int) __builtin_amdgcn_alignbit((unsigned) last, (unsigned) nb, 16);
-}
-
-// DPP lane move with an identity for lanes that have no source lane.
-template <int CTRL, int RM = 0xf>
-__device__ __forceinline__ int dppi(int old, int v) { return __builtin_amdgcn_update_dpp(old, v, CTRL, RM, 0xf, false); }
-template <int CTRL, int RM = 0xf>
-__device__ __forceinline__ float dppf(float old, float v)
-{
-  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(__builtin_bit_cast(int, old), __builtin_bit_cast(int, v), CTRL, RM, 0xf, false));
-}
-__device__ __forceinline__ float readlanef(float v, int l) { return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), l)); }
-enum { ROW_SHR1 = 0x111, ROW_SHR2 = 0x112, ROW_SHR4 = 0x114, ROW_SHR8 = 0x118,
-       ROW_SHL1 = 0x101, ROW_SHL2 = 0x102, ROW_SHL4 = 0x104, ROW_SHL8 = 0x108,
-       WAVE_SHL1 = 0x130, WAVE_SHR1 = 0x138, ROW_BCAST15 = 0x142, ROW_BCAST31 = 0x143 };
-#define INT_NEG ((int) 0x80000000)
-
-// Wave reductions: inclusive scans within each 16-lane row, then row_bcast:15
-// (rows 1 and 3 take lane 15 of the row before) and row_bcast:31 (rows 2 and 3
-// take lane 31), so lane 63 holds the result. Lanes a move does not write keep
-// the identity, which lets the compiler fuse each move into its ALU op.
-__device__ __forceinline__ int wave_max_i(int v)
-{
-  v = max(v, dppi<ROW_SHR1>(INT_NEG, v));
-  v = max(v, dppi<ROW_SHR2>(INT_NEG, v));
-  v = max(v, dppi<ROW_SHR4>(INT_NEG, v));
-  v = max(v, dppi<ROW_SHR8>(INT_NEG, v));
-  v = max(v, dppi<ROW_BCAST15, 0xa>(INT_NEG, v));
-  v = max(v, dppi<ROW_BCAST31, 0xc>(INT_NEG, v));
-  return __builtin_amdgcn_readlane(v, 63);
-}
-__device__ __forceinline__ float wave_sum_f(float v)
-{
-  v += dppf<ROW_SHR1>(0.0f, v);
-  v += dppf<ROW_SHR2>(0.0f, v);
-  v += dppf<ROW_SHR4>(0.0f, v);
-  v += dppf<ROW_SHR8>(0.0f, v);
-  v += dppf<ROW_BCAST15, 0xa>(0.0f, v);
-  v += dppf<ROW_BCAST31, 0xc>(0.0f, v);
-  return readlanef(v, 63);
-}
-__device__ __forceinline__ int   uni(int v)   { return __builtin_amdgcn_readfirstlane(v); }
-__device__ __forceinline__ float unif(float v){ return __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, v))); }
-
-__device__ __forceinline__ int64_t wave_id()  { return (int64_t) blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); }
-__device__ __forceinline__ int64_t wave_cnt() { return (int64_t) gridDim.x * (blockDim.x >> 6); }
-
-template <typename T>
-__device__ __forceinline__ void stage_table(T *lds, const T *__restrict__ g, size_t n)
-{
-  for (size_t i = threadIdx.x; i < n; i += blockDim.x) lds[i] = g[i];
-  __syncthreads();
-}
-
-// Appends `val` of every active lane with `pred` set to list[], with one
-// atomic per wave; the order inside the list is unspecified.
-__device__ __forceinline__ void list_append(bool pred, int32_t val, int32_t *__restrict__ list, int32_t *count)
-{
-  const unsigned long long m = __ballot(pred);
-  if (m == 0ull) return;
-  const int lane = __lane_id();
-  const int leader = __ffsll((long long) m) - 1;
-  int base = 0;
-  if (lane == leader) base = atomicAdd(count, __popcll(m));
-  base = __shfl(base, leader, 64);
-  if (pred) list[base + __popcll(m & ((1ull << lane) - 1ull))] = val;
-}
-
-// Next item of a device work queue shared by the waves of one launch
-// (wave-uniform). Lists are in slot order, longest target first, so waves
-// taking items in queue order schedule the longest work first.
-__device__ __forceinline__ int queue_next(int32_t *q, int step)
-{
-  int v = 0;
-  if (__lane_id() == 0) v = atomicAdd(q, step);
-  return uni(__shfl(v, 0, 64));
-}
-
-// Wave-uniform residue readers. A target's residues start on a 16-byte boundary
-// and every target is followed by at least 16 readable bytes (its padding or
-// the next target, and 16 spare bytes after the last one), so the chunk after
-// the current one can always be fetched ahead. Residue i is 0-based.
-__device__ __forceinline__ int chunk_byte(const uint4 &c, int i)
-{
-  const int q = (i >> 2) & 3;
-  const unsigned w = (q == 0) ? c.x : (q == 1) ? c.y : (q == 2) ? c.z : c.w;
-  return (int) ((w >> ((i & 3) * 8)) & 0xffu);
-}
-// The chunk in use is moved to scalar registers; only the one in flight
-// occupies vector registers.
-__device__ __forceinline__ uint4 uni4(const uint4 &v)
-{
-  return make_uint4((unsigned) uni((int) v.x), (unsigned) uni((int) v.y), (unsigned) uni((int) v.z), (unsigned) uni((int) v.w));
-}
-struct ResAsc {                       // i = 0, 1, 2, ...
-  const uint4 *p; uint4 cur, nxt; int c;
-  __device__ __forceinline__ void init(const uint8_t *sq) { p = (const uint4 *) sq; c = 0; cur = uni4(p[0]); nxt = p[1]; }
-  __device__ __forceinline__ int at(int i)
-  {
-    if ((i >> 4) != c) { c = i >> 4; cur = uni4(nxt); nxt = p[c + 1]; }
-    return chunk_byte(cur, i);
-  }
-};
-struct ResDesc {                      // i = L-1, L-2, ..., 0
-  const uint4 *p; uint4 cur, nxt; int c;
-  __device__ __forceinline__ void init(const uint8_t *sq, int L)
-  {
-    p = (const uint4 *) sq; c = (L - 1) >> 4; cur = uni4(p[c]); nxt = p[c > 0 ? c - 1 : 0];
-  }
-  __device__ __forceinline__ int at(int i)
-  {
-    if ((i >> 4) != c) { c = i >> 4; cur = uni4(nxt); nxt = p[c > 0 ? c - 1 : 0]; }
-    return chunk_byte(cur, i);
-  }
-};
-
-/*****************************************************************
- * 1. MSV stage, ungapped pass (constant xB)
- *****************************************************************/
-// Running maximum of Z over a row, two packed pairs per instruction. A value
-// Z - 32768 held as an int16 has the bit pattern 0x8000 + Z (Z < 32768), which
-// read as a half-precision number is -(Z * 2^-24) while Z < 1024 (zero and
-// subnormals, kept exact: the kernels run with fp16 denormals preserved) and
-// keeps decreasing as Z grows. So the IEEE minimum of those halves is the
-// pattern of the largest Z, and v_pk_minimum3_f16 folds two pairs into the
-// accumulator at once. Values that do not order this way (Z >= 31745: NaN or
-// positive patterns) only occur after Z has passed every overflow threshold on
-// the same diagonal, and the minimum then still decodes to an overflowing Z.
-// (Written as one instruction: the compiler would otherwise regroup the
-// minimum into pairwise trees, one and a half instructions per two pairs.)
-__device__ __forceinline__ int zmax3(int acc, int a, int b)
-{
-  int r;
-  asm("v_pk_minimum3_f16 %0, %1, %2, %3" : "=v"(r) : "v"(acc), "v"(a), "v"(b));
-  return r;
-}
-__device__ __forceinline__ int zmax2(int acc, int a) { return zmax3(acc, a, a); }
-
-template <int G, int ND>
-__global__ __launch_bounds__(SSV_BLOCK)
-void k_ssv(const int *__restrict__ tbl, int tbl_dwords,
-           const uint8_t *__restrict__ res, const int64_t *__restrict__ off, const int32_t *__restrict__ len,
-           const int32_t *__restrict__ slots, int64_t n, uint16_t *__restrict__ zmax, int32_t *queue)
-{
-  constexpr int NQ   = ND / 4;                        // 16-byte reads per row
-  constexpr int NT   = ND % 4;                        // 4-byte reads per row
-  constexpr int NQa  = NQ ? NQ : 1, NTa = NT ? NT : 1;
-  constexpr int TAIL0 = P7G_NROWS * NQ * G * 4;       // dword offset of the tail region
-  constexpr int TPW  = 64 / G;
-  constexpr int NACC = 4;
-  extern __shared__ int4 lds4[];
-  stage_table((int *) lds4, tbl, (size_t) tbl_dwords);
-
-  const int lane = threadIdx.x & 63;
-  const int grp  = lane / G;
-  const int z    = lane % G;
-  const int4 *qrow = (const int4 *) __builtin_assume_aligned(lds4, 16) + z;     // + (x*NQ + q)*G
-  const int  *trow = (const int *) lds4 + TAIL0 + z;                           // + (x*NT + t)*G
-
-  int64_t w = 0, wend = 0;                            // groups of TPW targets, SSV_GRAB groups per grab
-  for (;;) {
-    if (w == wend) { w = (int64_t) queue_next(queue, SSV_GRAB); wend = w + SSV_GRAB; }
-    if (w * TPW >= n) break;
-    const int64_t t = w * TPW + grp;
-    w++;
-    const bool valid = t < n;
-    const int64_t slot = valid ? (slots ? (int64_t) slots[t] : t) : 0;
-    const int L = valid ? len[slot] : 0;
-    const uint8_t *sq = res + (valid ? off[slot] : 0);
-    int Lw = L;
-    if (G == 16) { Lw = max(Lw, __shfl_xor(Lw, 16, 64)); Lw = max(Lw, __shfl_xor(Lw, 32, 64)); }
-    Lw = uni(Lw);
-
-    int zr[ND];
-#pragma unroll
-    for (int d = 0; d < ND; d++) zr[d] = (int) NEG_PAIR;
-    int acc[NACC];
-#pragma unroll
-    for (int a = 0; a < NACC; a++) acc[a] = (int) NEG_PAIR;
-    int nb = (int) NEG_PAIR;          // DPP destination: lanes without a left neighbour keep this fill
-
-    const uint4 padchunk = make_uint4(0x1f1f1f1fu, 0x1f1f1f1fu, 0x1f1f1f1fu, 0x1f1f1f1fu);
-    // one DP row with the scores in (cq, ct), while the row of residue xn is fetched into (nq, nt)
-#define SSV_ROW(cq, ct, nq, nt, xn) do {                                                      \
-      const int4 *rq_ = qrow + (xn) * (NQ * G);                                               \
-      const int  *rt_ = trow + (xn) * (NT * G);                                               \
-      _Pragma("unroll") for (int q = 0; q < NQ; q++) nq[q] = rq_[q * G];                      \
-      _Pragma("unroll") for (int u = 0; u < NT; u++) nt[u] = rt_[u * G];                      \
-      nb = __builtin_amdgcn_update_dpp(nb, zr[ND-1], (G == 16) ? 0x111 : 0x138, 0xf, 0xf, false); \
-      const int s0_ = (int) __builtin_amdgcn_alignbit((unsigned) zr[ND-1], (unsigned) nb, 16); \
-      _Pragma("unroll") for (int d = ND-1; d >= 0; d--) {                                     \
-        int rv_;                                                                              \
-        if (d < 4 * NQ) {                                                                     \
-          const int4 c_ = cq[d / 4];                                                          \
-          rv_ = (d % 4 == 0) ? c_.x : (d % 4 == 1) ? c_.y : (d % 4 == 2) ? c_.z : c_.w;       \
-        } else rv_ = ct[d - 4 * NQ];                                                          \
-        zr[d] = adds2(d ? zr[d-1] : s0_, rv_);                                                \
-      }                                                                                       \
-      _Pragma("unroll") for (int d = 0; d + 1 < ND; d += 2)                                   \
-        acc[(d / 2) % NACC] = zmax3(acc[(d / 2) % NACC], zr[d], zr[d+1]);                     \
-      if (ND & 1) acc[((ND - 1) / 2) % NACC] = zmax2(acc[((ND - 1) / 2) % NACC], zr[ND-1]);   \
-    } while (0)
-#define RES(w, u) ((int) (((w) >> (8 * ((u) & 3))) & 0xffu))
-
-    uint4 ch = (0 < L) ? *(const uint4 *) sq : padchunk;
-    int4 qA[NQa], qB[NQa];
-    int  tA[NTa], tB[NTa];
-    {
-      const int x0 = RES(ch.x, 0);
-#pragma unroll
-      for (int q = 0; q < NQ; q++) qA[q] = qrow[(x0 * NQ + q) * G];
-#pragma unroll
-      for (int u = 0; u < NT; u++) tA[u] = trow[(x0 * NT + u) * G];
-    }
-    int i0 = 0;
-    for (; i0 + 16 <= Lw; i0 += 16) {
-      const uint4 nch = (i0 + 16 < L) ? *(const uint4 *) (sq + i0 + 16) : padchunk;
-      SSV_ROW(qA, tA, qB, tB, RES(ch.x, 1));  SSV_ROW(qB, tB, qA, tA, RES(ch.x, 2));
-      SSV_ROW(qA, tA, qB, tB, RES(ch.x, 3));  SSV_ROW(qB, tB, qA, tA, RES(ch.y, 0));
-      SSV_ROW(qA, tA, qB, tB, RES(ch.y, 1));  SSV_ROW(qB, tB, qA, tA, RES(ch.y, 2));
-      SSV_ROW(qA, tA, qB, tB, RES(ch.y, 3));  SSV_ROW(qB, tB, qA, tA, RES(ch.z, 0));
-      SSV_ROW(qA, tA, qB, tB, RES(ch.z, 1));  SSV_ROW(qB, tB, qA, tA, RES(ch.z, 2));
-      SSV_ROW(qA, tA, qB, tB, RES(ch.z, 3));  SSV_ROW(qB, tB, qA, tA, RES(ch.w, 0));
-      SSV_ROW(qA, tA, qB, tB, RES(ch.w, 1));  SSV_ROW(qB, tB, qA, tA, RES(ch.w, 2));
-      SSV_ROW(qA, tA, qB, tB, RES(ch.w, 3));  SSV_ROW(qB, tB, qA, tA, RES(nch.x, 0));
-      ch = nch;
-    }
-    for (int u = 0; i0 + u < Lw; u++) {              // the last rows, fewer than 16
-      const unsigned wd = (u + 1 < 4) ? ch.x : (u + 1 < 8) ? ch.y : (u + 1 < 12) ? ch.z : ch.w;
-      SSV_ROW(qA, tA, qB, tB, RES(wd, u + 1));
-#pragma unroll
-      for (int q = 0; q < NQ; q++) qA[q] = qB[q];
-#pragma unroll
-      for (int v = 0; v < NT; v++) tA[v] = tB[v];
-    }
-#undef SSV_ROW
-#undef RES
-    int m = -32768;
-#pragma unroll
-    for (int a = 0; a < NACC; a++) m = max(m, max(lo16(acc[a]), hi16(acc[a])));
-#pragma unroll
-    for (int o = G/2; o > 0; o >>= 1) m = max(m, __shfl_xor(m, o, 64));
-    if (valid && z == 0) zmax[slot] = (uint16_t) (m + 32768);
-  }
-}
+#include "p7g_devfn.h"
 
 /*****************************************************************
  * 2. MSV stage: per-target decision after the ungapped pass
@@ -1097,86 +812,33 @@ void k_bck(const DevProfile p, const uint8_t *__restrict__ res, const int64_t *_
   }
 }
 
+__global__ void k_fill32(uint32_t *__restrict__ p, uint32_t v, int64_t n)
+{
+  for (int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t) gridDim.x * blockDim.x) p[i] = v;
+}
+
+__global__ void k_copy32(uint32_t *__restrict__ d, const uint32_t *__restrict__ s, int64_t n)
+{
+  for (int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t) gridDim.x * blockDim.x) d[i] = s[i];
+}
+
 /*****************************************************************
  * Launchers
  *****************************************************************/
-static int g_ncu = 0, g_lds_block = 0, g_lds_cu = 0;
-static void dev_props()
+hipError_t launch_fill32(void *p, uint32_t v, int64_t nwords, hipStream_t st)
 {
-  if (g_ncu) return;
-  int dev = 0;
-  if (hipGetDevice(&dev) != hipSuccess) dev = 0;
-  if (hipDeviceGetAttribute(&g_ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || g_ncu <= 0) g_ncu = 256;
-  if (hipDeviceGetAttribute(&g_lds_block, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess || g_lds_block <= 0)
-    g_lds_block = 64 * 1024;
-  if (hipDeviceGetAttribute(&g_lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev) != hipSuccess || g_lds_cu <= 0)
-    g_lds_cu = g_lds_block;
-}
-static int ncu() { dev_props(); return g_ncu; }
-static int lds_block_limit() { dev_props(); return g_lds_block; }
-
-// Launch with dynamic LDS; raises the per-kernel limit when a table needs more than 64 KB.
-static hipError_t launch(void *k, unsigned grid, unsigned block, void **args, size_t lds, hipStream_t st)
-{
-  if (lds > (size_t) lds_block_limit()) return hipErrorInvalidValue;
-  if (lds > 64 * 1024) (void) hipFuncSetAttribute((const void *) k, hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds);
-  return hipLaunchKernel(k, dim3(grid), dim3(block), args, lds, st);
+  if (nwords <= 0) return hipSuccess;
+  const unsigned grid = (unsigned) std::min<int64_t>((nwords + 255) / 256, 4096);
+  hipLaunchKernelGGL(k_fill32, dim3(grid), dim3(256), 0, st, (uint32_t *) p, v, nwords);
+  return hipGetLastError();
 }
 
-// Persistent grid: as many blocks of `wpb` waves as can be resident at once
-// (registers, LDS and wave slots, from the occupancy calculator), never more
-// than the work needs. A block that could not be resident would only start
-// when another finishes and run its share alone at the end.
-static unsigned grid_for_waves_blk(const void *kernel, int64_t nwaves, size_t lds_bytes, int wpb)
+hipError_t launch_copy32(void *dst, const void *src, int64_t nwords, hipStream_t st)
 {
-  dev_props();
-  int per_cu = 0;
-  if (lds_bytes > 64 * 1024) (void) hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds_bytes);
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, wpb * 64, lds_bytes) != hipSuccess || per_cu < 1) {
-    per_cu = 32 / wpb;
-    if (lds_bytes > 0) { int byl = (int) ((size_t) g_lds_cu / lds_bytes); if (byl < per_cu) per_cu = byl; }
-    if (per_cu < 1) per_cu = 1;
-  }
-  int64_t g = (int64_t) ncu() * per_cu;
-  int64_t need = (nwaves + wpb - 1) / wpb;
-  if (need < g) g = need;
-  if (g < 1) g = 1;
-  return (unsigned) g;
-}
-
-
-#define SSV16_CASE(N) case N: k = (void *) k_ssv<16, N>; break;
-#define SSV64_CASE(N) case N: k = (void *) k_ssv<64, N>; break;
-
-hipError_t launch_ssv(const DevProfile &p, const DevSeqs &s, const int32_t *slots, int64_t n, uint16_t *zmax, int32_t *queue,
-                      hipStream_t st)
-{
-  if (n <= 0) return hipSuccess;
-  void *k = nullptr;
-  if (p.ssv_G == 16) {
-    switch (p.ssv_ND) {
-      SSV16_CASE(1) SSV16_CASE(2) SSV16_CASE(3) SSV16_CASE(4) SSV16_CASE(5) SSV16_CASE(6) SSV16_CASE(7) SSV16_CASE(8)
-      SSV16_CASE(9) SSV16_CASE(10) SSV16_CASE(11) SSV16_CASE(12) SSV16_CASE(13) SSV16_CASE(14) SSV16_CASE(15) SSV16_CASE(16)
-      SSV16_CASE(17) SSV16_CASE(18) SSV16_CASE(19) SSV16_CASE(20) SSV16_CASE(21) SSV16_CASE(22) SSV16_CASE(23) SSV16_CASE(24)
-      SSV16_CASE(25) SSV16_CASE(26) SSV16_CASE(27) SSV16_CASE(28) SSV16_CASE(29) SSV16_CASE(30) SSV16_CASE(31) SSV16_CASE(32)
-      default: return hipErrorInvalidValue;
-    }
-  } else {
-    switch (p.ssv_ND) {
-      SSV64_CASE(8) SSV64_CASE(9) SSV64_CASE(10) SSV64_CASE(11) SSV64_CASE(12) SSV64_CASE(13) SSV64_CASE(14) SSV64_CASE(15)
-      SSV64_CASE(16) SSV64_CASE(18) SSV64_CASE(20) SSV64_CASE(22) SSV64_CASE(24) SSV64_CASE(26) SSV64_CASE(28) SSV64_CASE(30)
-      SSV64_CASE(32) SSV64_CASE(36) SSV64_CASE(40)
-      default: return hipErrorInvalidValue;
-    }
-  }
-  const size_t lds = p.ssv_dwords * 4;
-  const int64_t nwaves = (n + (64 / p.ssv_G) - 1) / (64 / p.ssv_G);
-  const unsigned grid = grid_for_waves_blk(k, nwaves, lds, SSV_BLOCK / 64);
-  int tbl_dwords = (int) p.ssv_dwords;
-  if (p.ssv_dwords != (size_t) P7G_NROWS * p.ssv_G * p.ssv_ND) return hipErrorInvalidValue;
-  void *args[] = { (void *) &p.ssv_tbl, &tbl_dwords, (void *) &s.res, (void *) &s.off, (void *) &s.len,
-                   (void *) &slots, &n, &zmax, &queue };
-  return launch(k, grid, SSV_BLOCK, args, lds, st);
+  if (nwords <= 0) return hipSuccess;
+  const unsigned grid = (unsigned) std::min<int64_t>((nwords + 255) / 256, 4096);
+  hipLaunchKernelGGL(k_copy32, dim3(grid), dim3(256), 0, st, (uint32_t *) dst, (const uint32_t *) src, nwords);
+  return hipGetLastError();
 }
 
 hipError_t launch_msv_classify(const DevProfile &p, const DevSeqs &s, const int32_t *slots, int64_t n,
@@ -1234,7 +896,7 @@ hipError_t launch_bias(const DevProfile &p, const DevSeqs &s, const int32_t *lis
                      (const float *) rowmax, (const int64_t *) defer_off);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   // the scale-factor pool's cursor starts from zero again for the Forward stage
-  return hipMemsetAsync(l.cursor, 0, sizeof(unsigned long long), st);
+  return launch_fill32(l.cursor, 0u, 2, st);
 }
 
 #define VIT_CASE(N) case N: k = (void *) k_vit<N>; break;
@@ -1435,7 +1097,7 @@ template <typename T>
 static hipError_t launch_compact(const T *f, int value, int64_t n, int32_t *list, int32_t *count, int32_t *tilecnt,
                                  hipStream_t st)
 {
-  if (n <= 0) return hipMemsetAsync(count, 0, sizeof(int32_t), st);
+  if (n <= 0) return launch_fill32(count, 0u, 1, st);
   const unsigned tiles = (unsigned) ((n + CT_TILE - 1) / CT_TILE);
   hipLaunchKernelGGL(k_compact_count<T>, dim3(tiles), dim3(256), 0, st, f, value, n, tilecnt);
   hipLaunchKernelGGL(k_compact_scatter<T>, dim3(tiles), dim3(256), 0, st, f, value, n, (const int32_t *) tilecnt, list, count);

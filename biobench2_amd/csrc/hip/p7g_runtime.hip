@@ -11,6 +11,9 @@
 #include <string.h>
 #include <math.h>
 #include <mutex>
+#include <map>
+#include <atomic>
+#include <set>
 #include <vector>
 #include <algorithm>
 
@@ -88,9 +91,12 @@ extern "C" const char *p7_gpu_LastError(void) { return p7g_host_error(); }
  * Device profile: re-laid-out tables for 16/64-lane blocks, per-length
  * constants, and the bias-filter HMM, in one allocation.
  *****************************************************************/
+static std::atomic<unsigned long long> g_serial(0);   // identifies device profile copies and databases
+
 struct p7g_dev_profile_s {
   DevProfile p;
   int        Lmax;
+  unsigned long long serial;
   struct p7g_dev_profile_s *older;   // copies replaced by a longer length range, kept until the profile goes
 };
 
@@ -107,6 +113,7 @@ p7g_dev_profile_free(struct p7g_dev_profile_s *d)
 
 static const int vf_nd_set[] = { 1, 2, 3, 4, 5, 6, 8, 10, 12, 16, 20, 24, 32, 40 };
 static const int fb_b_set[]  = { 1, 2, 3, 4, 5, 6, 7, 8, 10, 12, 14, 16, 20, 24, 28, 32, 40, 48, 64, 80 };
+static const int ssv32_set[] = { 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15, 16, 18, 20, 22, 24, 26, 28, 30, 32 };
 static const int ssv64_set[] = { 8, 9, 10, 11, 12, 13, 14, 15, 16, 18, 20, 22, 24, 26, 28, 30, 32, 36, 40 };
 
 static int pick(const int *set, int nset, int need)
@@ -134,25 +141,33 @@ dev_profile_build(P7_GPU_OPROFILE *om, int Lmax_need, struct p7g_dev_profile_s *
     if (x >= Kp || k < 1 || k > M) return -32768;
     return (int) om->bias_b - (int) om->rbv[(size_t) x * M + k - 1];
   };
-  if (M <= 1024) { p.ssv_G = 16; p.ssv_ND = std::max(1, cdiv(cdiv(M, 16), 2)); }
-  else {
-    p.ssv_G = 64; p.ssv_ND = pick(ssv64_set, (int) (sizeof(ssv64_set)/sizeof(int)), cdiv(cdiv(M, 64), 2));
-    if (p.ssv_ND < 0) { delete d; p7g_set_error("model too long for the device MSV stage (M=%d)", M); return p7G_EINVAL; }
-  }
-  p.ssv_dwords = (size_t) NR * p.ssv_G * p.ssv_ND;
-  std::vector<int> ssv(p.ssv_dwords, (int) 0x80008000u);
-  {
+  // the shapes of the ungapped pass this model fits: 16, 32 and 64 lanes per target
+  const int ssvG[P7G_SSV_NVAR] = { 16, 32, 64 };
+  std::vector<int> ssv[P7G_SSV_NVAR];
+  for (int v = 0; v < P7G_SSV_NVAR; v++) {
+    SsvVariant &sv = p.ssv[v];
+    const int G = ssvG[v];
+    int ND = -1;
+    if (G == 16)      ND = (M <= 1024) ? std::max(1, cdiv(M, 32)) : -1;
+    else if (G == 32) ND = (M <= 2048) ? pick(ssv32_set, (int) (sizeof(ssv32_set)/sizeof(int)), cdiv(M, 64)) : -1;
+    else              ND = pick(ssv64_set, (int) (sizeof(ssv64_set)/sizeof(int)), cdiv(M, 128));
+    if (ND < 0) { sv.G = 0; sv.ND = 0; sv.dwords = 0; continue; }
+    sv.G = G; sv.ND = ND;
+    sv.dwords = (size_t) NR * G * ND;
+    ssv[v].assign(sv.dwords, (int) 0x80008000u);
     // quads: [x][q][z][4] for dwords 0..4*NQ-1; tail: [x][t][z] for dwords 4*NQ + t
-    const int G = p.ssv_G, ND = p.ssv_ND, B = 2 * ND, NQ = ND / 4, NT = ND % 4;
+    const int B = 2 * ND, NQ = ND / 4, NT = ND % 4;
     const size_t tail0 = (size_t) NR * NQ * G * 4;
     for (int x = 0; x < NR; x++)
       for (int z = 0; z < G; z++)
         for (int dd = 0; dd < ND; dd++) {
-          const int v = pack2(rsv(x, z*B + dd + 1), rsv(x, z*B + dd + ND + 1));
-          if (dd < 4 * NQ) ssv[(((size_t) x * NQ + dd / 4) * G + z) * 4 + dd % 4] = v;
-          else             ssv[tail0 + ((size_t) x * NT + (dd - 4 * NQ)) * G + z] = v;
+          const int val = pack2(rsv(x, z*B + dd + 1), rsv(x, z*B + dd + ND + 1));
+          if (dd < 4 * NQ) ssv[v][(((size_t) x * NQ + dd / 4) * G + z) * 4 + dd % 4] = val;
+          else             ssv[v][tail0 + ((size_t) x * NT + (dd - 4 * NQ)) * G + z] = val;
         }
   }
+  if (p.ssv[2].G == 0) { delete d; p7g_set_error("model too long for the device MSV stage (M=%d)", M); return p7G_EINVAL; }
+  p.ssv_def = (p.ssv[0].G != 0) ? 0 : 2;            // 16 lanes up to M = 1024, else 64
   p.mf_ND = pick(vf_nd_set, (int) (sizeof(vf_nd_set)/sizeof(int)), std::max(1, cdiv(cdiv(M, 64), 2)));
   if (p.mf_ND < 0) { delete d; p7g_set_error("model too long for the device MSV stage (M=%d)", M); return p7G_EINVAL; }
   p.mf_dwords = (size_t) NR * 64 * p.mf_ND;
@@ -263,17 +278,18 @@ dev_profile_build(P7_GPU_OPROFILE *om, int Lmax_need, struct p7g_dev_profile_s *
 
   // ---- one allocation ----
   const size_t L1 = (size_t) Lmax + 1;
-  size_t sz[] = { p.ssv_dwords * 4, p.mf_dwords * 4, p.vf_dwords * 4, p.fb_floats * 4,
-                  L1, L1 * 2, L1 * 4, L1 * 4, L1 * 4, L1 * 4, L1 * 4, eo.size() * 4 };
-  const void *src[] = { ssv.data(), mf.data(), vf.data(), fb.data(), lc.tjb_b, lc.xw_move, lc.xf_move, lc.xf_loop,
-                        lc.nullsc, lc.flen_a, lc.flen_b, eo.data() };
+  size_t sz[] = { p.ssv[0].dwords * 4, p.ssv[1].dwords * 4, p.ssv[2].dwords * 4, p.mf_dwords * 4, p.vf_dwords * 4,
+                  p.fb_floats * 4, L1, L1 * 2, L1 * 4, L1 * 4, L1 * 4, L1 * 4, L1 * 4, eo.size() * 4 };
+  const void *src[] = { ssv[0].data(), ssv[1].data(), ssv[2].data(), mf.data(), vf.data(), fb.data(), lc.tjb_b,
+                        lc.xw_move, lc.xf_move, lc.xf_loop, lc.nullsc, lc.flen_a, lc.flen_b, eo.data() };
   const int NB = (int) (sizeof(sz) / sizeof(sz[0]));
   size_t offs[16], total = 0;
   for (int i = 0; i < NB; i++) { offs[i] = total; total += (sz[i] + 255) & ~(size_t) 255; }
   char *blob = nullptr;
   hipError_t e = hipMalloc((void **) &blob, total);
   if (e == hipSuccess) {
-    for (int i = 0; i < NB && e == hipSuccess; i++) e = hipMemcpyAsync(blob + offs[i], src[i], sz[i], hipMemcpyHostToDevice, g_stream);
+    for (int i = 0; i < NB && e == hipSuccess; i++)
+      if (sz[i]) e = hipMemcpyAsync(blob + offs[i], src[i], sz[i], hipMemcpyHostToDevice, g_stream);
     if (e == hipSuccess) e = hipStreamSynchronize(g_stream);
   }
   p7g_lencfg_free(&lc);
@@ -284,13 +300,15 @@ dev_profile_build(P7_GPU_OPROFILE *om, int Lmax_need, struct p7g_dev_profile_s *
     return p7G_EDEVICE;
   }
   p.blob = blob;
-  p.ssv_tbl = (int *) (blob + offs[0]); p.mf_tbl = (int *) (blob + offs[1]);
-  p.vf_tbl = (int *) (blob + offs[2]);  p.fb_tbl = (float *) (blob + offs[3]);
-  p.tjb = (uint8_t *) (blob + offs[4]); p.xw_move = (int16_t *) (blob + offs[5]);
-  p.xf_move = (float *) (blob + offs[6]); p.xf_loop = (float *) (blob + offs[7]);
-  p.nullsc = (float *) (blob + offs[8]); p.flen_a = (float *) (blob + offs[9]); p.flen_b = (float *) (blob + offs[10]);
-  p.eo = (float *) (blob + offs[11]);
+  for (int v = 0; v < P7G_SSV_NVAR; v++) p.ssv[v].tbl = p.ssv[v].G ? (int *) (blob + offs[v]) : nullptr;
+  p.mf_tbl = (int *) (blob + offs[3]);
+  p.vf_tbl = (int *) (blob + offs[4]);  p.fb_tbl = (float *) (blob + offs[5]);
+  p.tjb = (uint8_t *) (blob + offs[6]); p.xw_move = (int16_t *) (blob + offs[7]);
+  p.xf_move = (float *) (blob + offs[8]); p.xf_loop = (float *) (blob + offs[9]);
+  p.nullsc = (float *) (blob + offs[10]); p.flen_a = (float *) (blob + offs[11]); p.flen_b = (float *) (blob + offs[12]);
+  p.eo = (float *) (blob + offs[13]);
   d->Lmax = Lmax;
+  d->serial = ++g_serial;
   *ret = d;
   return p7G_OK;
 }
@@ -329,6 +347,7 @@ struct p7g_seqdb_s {
   int32_t *d_perm;         // slot -> caller index, on the device
   char    *names;          // sequence names in caller order (when read from a file), NUL-separated
   int64_t *name_off;       // [caller index] offset into names
+  unsigned long long serial;
 };
 
 extern "C" void
@@ -352,6 +371,7 @@ p7_gpu_seqdb_Create(const uint8_t *residues, const int64_t *offset, const int32_
   if ((status = ensure_init()) != p7G_OK) return status;
   if (nseq <= 0 || nseq > INT32_MAX - 1) { p7g_set_error("bad database size %lld", (long long) nseq); return p7G_EINVAL; }
   P7_GPU_SEQDB *db = (P7_GPU_SEQDB *) calloc(1, sizeof(P7_GPU_SEQDB));
+  if (db) db->serial = ++g_serial;
   if (! db) return p7G_EMEM;
   db->nseq = nseq;
   db->h_len  = (int32_t *) malloc(sizeof(int32_t) * (size_t) nseq);
@@ -533,6 +553,7 @@ p7_gpu_seqdb_Load(const char *path, P7_GPU_SEQDB **ret_db)
   FILE *fp = fopen(path, "rb");
   if (! fp) { p7g_set_error("can't open %s", path); return p7G_ENOTFOUND; }
   P7_GPU_SEQDB *db = (P7_GPU_SEQDB *) calloc(1, sizeof(P7_GPU_SEQDB));
+  if (db) db->serial = ++g_serial;
   uint32_t magic = 0;
   int64_t namebytes = 0;
   std::vector<uint8_t> h;
@@ -600,6 +621,37 @@ static DevSeqs seqs_of(const P7_GPU_SEQDB *db)
 }
 
 /*****************************************************************
+ * Tuned shape of the ungapped MSV pass
+ *****************************************************************/
+#define SSV_GRAB_DEFAULT 4          // target groups per work-queue atomic
+// A model's length class, M rounded up to a multiple of 32, fixes every shape
+// of the ungapped pass the model fits, so the models of one class share one
+// measurement.
+static int tune_class(int M) { return cdiv(M, 32); }
+struct TuneEntry { int v, grab; float us[P7G_SSV_NVAR]; };
+static std::mutex g_tunelock;
+static std::map<int, TuneEntry> g_tune;
+
+// The variant and queue grab the ungapped pass of this profile runs with.
+static void ssv_choice(const DevProfile &p, int *v, int *grab)
+{
+  *v = p.ssv_def; *grab = SSV_GRAB_DEFAULT;
+  std::lock_guard<std::mutex> lk(g_tunelock);
+  auto it = g_tune.find(tune_class(p.M));
+  if (it != g_tune.end() && it->second.v >= 0 && it->second.v < P7G_SSV_NVAR && p.ssv[it->second.v].G) {
+    *v = it->second.v; *grab = it->second.grab;
+  }
+}
+
+static hipError_t launch_ssv_chosen(const DevProfile &p, const DevSeqs &s, const int32_t *slots, int64_t n,
+                                    uint16_t *zmax, int32_t *queue, hipStream_t st)
+{
+  int v, grab;
+  ssv_choice(p, &v, &grab);
+  return launch_ssv(p, v, s, slots, n, zmax, queue, grab, st);
+}
+
+/*****************************************************************
  * Work buffers: per-slot results and lists, grown on demand
  *****************************************************************/
 
@@ -645,11 +697,13 @@ work_reserve(Work &w, int64_t n, int64_t pool_floats)
   return p7G_OK;
 }
 
+// Lists, queue cursors and the pool cursor start from zero (fill kernels, so
+// that a captured graph holds no memset node).
 static int
 reset_counts(Work &w)
 {
-  HIPCHK(hipMemsetAsync(w.l.counts, 0, 64, w.st));
-  HIPCHK(hipMemsetAsync(w.l.cursor, 0, 64, w.st));
+  HIPCHK(launch_fill32(w.l.counts, 0u, 16, w.st));
+  HIPCHK(launch_fill32(w.l.cursor, 0u, 16, w.st));
   return p7G_OK;
 }
 
@@ -709,7 +763,7 @@ p7_gpu_MSVFilterBatch(const P7_GPU_OPROFILE *om, const P7_GPU_SEQDB *db, const i
   LenCfg lc = lencfg_per_target();
   if ((status = reset_counts(g_work)) != p7G_OK) return status;
   HIPCHK(hipMemcpyAsync(w.slots, slots.data(), sizeof(int32_t) * (size_t) n, hipMemcpyHostToDevice, g_stream));
-  HIPCHK(launch_ssv(*p, s, w.slots, n, w.r.zmax, &w.l.counts[P7G_Q_SSV], g_stream));
+  HIPCHK(launch_ssv_chosen(*p, s, w.slots, n, w.r.zmax, &w.l.counts[P7G_Q_SSV], g_stream));
   HIPCHK(launch_msv_classify(*p, s, w.slots, n, w.r.zmax, w.r.xj, w.l.full, &w.l.counts[0], lc, g_stream));
   HIPCHK(launch_msv_full(*p, s, w.l.full, &w.l.counts[0], n, w.r.xj, lc, g_stream));
   std::vector<int16_t> xj((size_t) db->nseq);
@@ -799,11 +853,15 @@ extern "C" int p7_gpu_FilterScoreBatch(const P7_GPU_OPROFILE *om, const P7_GPU_S
 /*****************************************************************
  * Pipeline (p7_pipeline.c:666-735) on the device
  *****************************************************************/
+struct Batch;
+static void batch_free(Batch *b);
+
 struct p7g_pipeline_s {
   double F1, F2, F3;
   int    do_biasfilter, do_max;
   P7_GPU_STATS st;
   Work       w;             // this pipeline's stream and device work area
+  Batch     *b = nullptr;   // work areas and graph of p7_gpu_PipelineBatch, made on first use
   std::mutex mu;            // one call at a time per pipeline object
 };
 
@@ -859,6 +917,7 @@ extern "C" void
 p7_gpu_pipeline_Destroy(P7_GPU_PIPELINE *pli)
 {
   if (! pli) return;
+  batch_free(pli->b);
   work_release(pli->w);
   if (pli->w.st) (void) hipStreamDestroy(pli->w.st);
   delete pli;
@@ -871,6 +930,110 @@ p7_gpu_pipeline_GetStats(const P7_GPU_PIPELINE *pli, P7_GPU_STATS *st)
   return p7G_OK;
 }
 
+static int
+check_query(const P7_GPU_OPROFILE *om, const P7_GPU_SEQDB *db)
+{
+  if (db->maxcode >= om->abc->Kp) { p7g_set_error("database holds residue codes outside the model alphabet"); return p7G_EINCOMPAT; }
+  for (int z = 0; z < 6; z++)
+    if (om->evparam[z] == -99999.0f) { p7g_set_error("profile has no E-value parameters (STATS)"); return p7G_EINVAL; }
+  return p7G_OK;
+}
+
+static GateThresh
+gates_of(const P7_GPU_PIPELINE *pli, const P7_GPU_OPROFILE *om)
+{
+  GateThresh g;
+  g.T1m = p7g_gumbel_threshold(om->evparam[p7G_MMU], om->evparam[p7G_MLAMBDA], pli->F1);
+  g.T2m = p7g_gumbel_threshold(om->evparam[p7G_MMU], om->evparam[p7G_MLAMBDA], pli->F2);
+  g.T2v = p7g_gumbel_threshold(om->evparam[p7G_VMU], om->evparam[p7G_VLAMBDA], pli->F2);
+  g.T3f = p7g_exp_threshold(om->evparam[p7G_FTAU], om->evparam[p7G_FLAMBDA], pli->F3);
+  g.do_bias = pli->do_biasfilter; g.do_max = pli->do_max;
+  return g;
+}
+
+// Counters, queue cursors and the NaN marks of scores not computed, before a
+// run over n slots.
+static int
+chain_reset(Work &w, int64_t n)
+{
+  int status;
+  if ((status = reset_counts(w)) != p7G_OK) return status;
+  HIPCHK(launch_fill32(w.r.filtersc, 0xffffffffu, n, w.st));
+  HIPCHK(launch_fill32(w.r.vfsc, 0xffffffffu, n, w.st));
+  HIPCHK(launch_fill32(w.r.fwdsc, 0xffffffffu, n, w.st));
+  HIPCHK(launch_fill32(w.r.bcksc, 0xffffffffu, n, w.st));
+  return p7G_OK;
+}
+
+// Every stage after the ungapped pass for one profile over all n slots, on the
+// work area's stream: the MSV decision and full recurrence, then the bias
+// filter, Viterbi, Forward and Backward over lists compacted from the per-slot
+// marks in slot order (longest target first), each list kernel taking items
+// from a work queue. With `timed`, events 1..5 close the MSV, bias, Viterbi,
+// Forward and Backward stages.
+static int
+chain_stages(Work &w, const DevProfile &p, const DevSeqs &s, int64_t n, const uint16_t *zmax, const GateThresh &g,
+             bool timed)
+{
+  LenCfg lc = lencfg_per_target();
+  HIPCHK(launch_msv_classify(p, s, nullptr, n, zmax, w.r.xj, w.l.full, &w.l.counts[0], lc, w.st));
+  HIPCHK(launch_compact_i16(w.r.xj, -2, n, w.l.full, &w.l.counts[0], w.tilecnt, w.st));
+  HIPCHK(launch_msv_full(p, s, w.l.full, &w.l.counts[0], n, w.r.xj, lc, w.st));
+  HIPCHK(launch_msv_gate(p, s, n, w.r.xj, w.r, w.l, g, w.st));
+  if (timed) HIPCHK(hipEventRecord(w.ev[1], w.st));
+  if (g.do_bias) {
+    HIPCHK(launch_compact_u8(w.r.stage, 1, n, w.l.bias, &w.l.counts[1], w.tilecnt, w.st));
+    HIPCHK(launch_bias(p, s, w.l.bias, &w.l.counts[1], n, w.r, w.l, g, 1, lc, w.pool, w.pool_size, w.scale_off, w.st));
+  }
+  if (timed) HIPCHK(hipEventRecord(w.ev[2], w.st));
+  HIPCHK(launch_compact_u8(w.r.stage, 2, n, w.l.vit, &w.l.counts[2], w.tilecnt, w.st));
+  HIPCHK(launch_vit(p, s, w.l.vit, &w.l.counts[2], n, w.r, w.l, g, 1, lc, w.st));
+  if (timed) HIPCHK(hipEventRecord(w.ev[3], w.st));
+  HIPCHK(launch_compact_u8(w.r.stage, 3, n, w.l.fwd, &w.l.counts[3], w.tilecnt, w.st));
+  HIPCHK(launch_fwd(p, s, w.l.fwd, &w.l.counts[3], n, w.r, w.l, g, 1, lc, w.pool, w.pool_size, w.scale_off, nullptr, w.st));
+  if (timed) HIPCHK(hipEventRecord(w.ev[4], w.st));
+  HIPCHK(launch_compact_u8(w.r.stage, 4, n, w.l.bck, &w.l.counts[4], w.tilecnt, w.st));
+  HIPCHK(launch_bck(p, s, w.l.bck, &w.l.counts[4], n, w.r, w.pool, w.scale_off, nullptr, nullptr, lc, w.st));
+  if (timed) HIPCHK(hipEventRecord(w.ev[5], w.st));
+  return p7G_OK;
+}
+
+// Pipeline counters of one profile's run from the device sums (launch_stats)
+// and list counts.
+static void
+stats_of_run(P7_GPU_STATS &st, const unsigned long long *sums, const int32_t *counts, const P7_GPU_OPROFILE *om,
+             const P7_GPU_SEQDB *db, int do_biasfilter)
+{
+  memset(&st, 0, sizeof(st));
+  st.nmodels = 1; st.nseqs = db->nseq; st.nres = db->nres; st.nnodes = om->M;
+  st.n_past_msv = (int64_t) sums[0]; st.n_past_bias = (int64_t) sums[1];
+  st.n_past_vit = (int64_t) sums[2]; st.n_past_fwd = (int64_t) sums[3];
+  st.n_msv_full = counts[0];
+  // MSV over every target, the bias filter over MSV survivors (when on),
+  // Viterbi and Forward over their lists, Backward over F3 survivors
+  st.cells_msv  = (double) db->nres * om->M;
+  st.cells_bias = do_biasfilter ? (double) sums[4] : 0.0;
+  st.cells_vit  = (double) sums[6] * om->M;
+  st.cells_fwd  = (double) sums[7] * om->M;
+  st.cells_bck  = (double) sums[5] * om->M;
+}
+
+// Adds one run's counters to a pipeline's totals (p7_pipeline_Merge-like);
+// cells and times are those of the last call.
+static void
+stats_accumulate(P7_GPU_STATS &tot, const P7_GPU_STATS &one, bool first_of_call)
+{
+  tot.nmodels += one.nmodels; tot.nseqs += one.nseqs; tot.nres += one.nres; tot.nnodes += one.nnodes;
+  tot.n_past_msv += one.n_past_msv; tot.n_past_bias += one.n_past_bias;
+  tot.n_past_vit += one.n_past_vit; tot.n_past_fwd += one.n_past_fwd;
+  if (first_of_call) {
+    tot.n_msv_full = 0; tot.cells_msv = tot.cells_bias = tot.cells_vit = tot.cells_fwd = tot.cells_bck = 0.0;
+  }
+  tot.n_msv_full += one.n_msv_full;
+  tot.cells_msv += one.cells_msv; tot.cells_bias += one.cells_bias; tot.cells_vit += one.cells_vit;
+  tot.cells_fwd += one.cells_fwd; tot.cells_bck += one.cells_bck;
+}
+
 extern "C" int
 p7_gpu_Pipeline(P7_GPU_PIPELINE *pli, const P7_GPU_OPROFILE *om, const P7_GPU_SEQDB *db, P7_GPU_HITS *hits)
 {
@@ -879,9 +1042,7 @@ p7_gpu_Pipeline(P7_GPU_PIPELINE *pli, const P7_GPU_OPROFILE *om, const P7_GPU_SE
   DevProfile *p;
   if ((status = ensure_init()) != p7G_OK) return status;
   if (! pli->w.st) HIPCHK(hipStreamCreateWithFlags(&pli->w.st, hipStreamNonBlocking));
-  if (db->maxcode >= om->abc->Kp) { p7g_set_error("database holds residue codes outside the model alphabet"); return p7G_EINCOMPAT; }
-  for (int z = 0; z < 6; z++)
-    if (om->evparam[z] == -99999.0f) { p7g_set_error("profile has no E-value parameters (STATS)"); return p7G_EINVAL; }
+  if ((status = check_query(om, db)) != p7G_OK) return status;
   if ((status = dev_profile(om, db->Lmax, &p)) != p7G_OK) return status;
   const int64_t n = db->nseq;
   // pool for the Forward row scale factors of F3 survivors: bounded by the
@@ -889,46 +1050,16 @@ p7_gpu_Pipeline(P7_GPU_PIPELINE *pli, const P7_GPU_OPROFILE *om, const P7_GPU_SE
   Work &w = pli->w;
   if ((status = work_reserve(w, n, pipeline_pool_floats(n, db->nres))) != p7G_OK) return status;
   DevSeqs s = seqs_of(db);
-  LenCfg lc = lencfg_per_target();
-  GateThresh g;
-  g.T1m = p7g_gumbel_threshold(om->evparam[p7G_MMU], om->evparam[p7G_MLAMBDA], pli->F1);
-  g.T2m = p7g_gumbel_threshold(om->evparam[p7G_MMU], om->evparam[p7G_MLAMBDA], pli->F2);
-  g.T2v = p7g_gumbel_threshold(om->evparam[p7G_VMU], om->evparam[p7G_VLAMBDA], pli->F2);
-  g.T3f = p7g_exp_threshold(om->evparam[p7G_FTAU], om->evparam[p7G_FLAMBDA], pli->F3);
-  g.do_bias = pli->do_biasfilter; g.do_max = pli->do_max;
+  const GateThresh g = gates_of(pli, om);
 
-  if ((status = reset_counts(w)) != p7G_OK) return status;
-  HIPCHK(hipMemsetAsync(w.r.filtersc, 0xff, sizeof(float) * (size_t) n, w.st));   // NaN where not computed
-  HIPCHK(hipMemsetAsync(w.r.vfsc, 0xff, sizeof(float) * (size_t) n, w.st));
-  HIPCHK(hipMemsetAsync(w.r.fwdsc, 0xff, sizeof(float) * (size_t) n, w.st));
-  HIPCHK(hipMemsetAsync(w.r.bcksc, 0xff, sizeof(float) * (size_t) n, w.st));
+  if ((status = chain_reset(w, n)) != p7G_OK) return status;
   HIPCHK(hipEventRecord(w.ev[0], w.st));
-  // Each stage's list is compacted from the per-slot marks in slot order
-  // (longest target first); the list kernels take items from a work queue.
-  HIPCHK(launch_ssv(*p, s, nullptr, n, w.r.zmax, &w.l.counts[P7G_Q_SSV], w.st));
+  HIPCHK(launch_ssv_chosen(*p, s, nullptr, n, w.r.zmax, &w.l.counts[P7G_Q_SSV], w.st));
   HIPCHK(hipEventRecord(w.ev[6], w.st));
-  HIPCHK(launch_msv_classify(*p, s, nullptr, n, w.r.zmax, w.r.xj, w.l.full, &w.l.counts[0], lc, w.st));
-  HIPCHK(launch_compact_i16(w.r.xj, -2, n, w.l.full, &w.l.counts[0], w.tilecnt, w.st));
-  HIPCHK(launch_msv_full(*p, s, w.l.full, &w.l.counts[0], n, w.r.xj, lc, w.st));
-  HIPCHK(launch_msv_gate(*p, s, n, w.r.xj, w.r, w.l, g, w.st));
-  HIPCHK(hipEventRecord(w.ev[1], w.st));
-  if (g.do_bias) {
-    HIPCHK(launch_compact_u8(w.r.stage, 1, n, w.l.bias, &w.l.counts[1], w.tilecnt, w.st));
-    HIPCHK(launch_bias(*p, s, w.l.bias, &w.l.counts[1], n, w.r, w.l, g, 1, lc, w.pool, w.pool_size, w.scale_off, w.st));
-  }
-  HIPCHK(hipEventRecord(w.ev[2], w.st));
-  HIPCHK(launch_compact_u8(w.r.stage, 2, n, w.l.vit, &w.l.counts[2], w.tilecnt, w.st));
-  HIPCHK(launch_vit(*p, s, w.l.vit, &w.l.counts[2], n, w.r, w.l, g, 1, lc, w.st));
-  HIPCHK(hipEventRecord(w.ev[3], w.st));
-  HIPCHK(launch_compact_u8(w.r.stage, 3, n, w.l.fwd, &w.l.counts[3], w.tilecnt, w.st));
-  HIPCHK(launch_fwd(*p, s, w.l.fwd, &w.l.counts[3], n, w.r, w.l, g, 1, lc, w.pool, w.pool_size, w.scale_off, nullptr, w.st));
-  HIPCHK(hipEventRecord(w.ev[4], w.st));
-  HIPCHK(launch_compact_u8(w.r.stage, 4, n, w.l.bck, &w.l.counts[4], w.tilecnt, w.st));
-  HIPCHK(launch_bck(*p, s, w.l.bck, &w.l.counts[4], n, w.r, w.pool, w.scale_off, nullptr, nullptr, lc, w.st));
-  HIPCHK(hipEventRecord(w.ev[5], w.st));
+  if ((status = chain_stages(w, *p, s, n, w.r.zmax, g, true)) != p7G_OK) return status;
 
   // counters on the device, then the requested per-target arrays in the caller's order
-  HIPCHK(hipMemsetAsync(w.stats, 0, 64, w.st));
+  HIPCHK(launch_fill32(w.stats, 0u, 16, w.st));
   HIPCHK(launch_stats(s, w.r.stage, n, w.l.vit, w.l.fwd, w.l.counts, w.stats, w.st));
   unsigned long long sums[8];
   int32_t counts[16];
@@ -950,6 +1081,9 @@ p7_gpu_Pipeline(P7_GPU_PIPELINE *pli, const P7_GPU_OPROFILE *om, const P7_GPU_SE
   HIPCHK(hipStreamSynchronize(w.st));
   if (counts[6]) { p7g_set_error("Forward scale-factor pool exhausted"); return p7G_EMEM; }
 
+  P7_GPU_STATS one;
+  stats_of_run(one, sums, counts, om, db, pli->do_biasfilter);
+  stats_accumulate(pli->st, one, true);
   P7_GPU_STATS &st = pli->st;
   float ms[5], mtot;
   for (int a = 0; a < 5; a++) HIPCHK(hipEventElapsedTime(&ms[a], w.ev[a], w.ev[a+1]));
@@ -958,17 +1092,459 @@ p7_gpu_Pipeline(P7_GPU_PIPELINE *pli, const P7_GPU_OPROFILE *om, const P7_GPU_SE
   HIPCHK(hipEventElapsedTime(&mtot, w.ev[0], w.ev[6]));
   st.ms_ssv = mtot;
   st.ms_msv = ms[0]; st.ms_bias = ms[1]; st.ms_vit = ms[2]; st.ms_fwd = ms[3]; st.ms_bck = ms[4];
-  st.nmodels += 1; st.nseqs += n; st.nres += db->nres; st.nnodes += om->M;
-  st.n_past_msv += (int64_t) sums[0]; st.n_past_bias += (int64_t) sums[1];
-  st.n_past_vit += (int64_t) sums[2]; st.n_past_fwd += (int64_t) sums[3];
-  st.n_msv_full = counts[0];
-  // cells of the last call: MSV over every target, the bias filter over MSV
-  // survivors (when on), Viterbi and Forward over their lists, Backward over F3 survivors
-  st.cells_msv  = (double) db->nres * om->M;
-  st.cells_bias = pli->do_biasfilter ? (double) sums[4] : 0.0;
-  st.cells_vit  = (double) sums[6] * om->M;
-  st.cells_fwd  = (double) sums[7] * om->M;
-  st.cells_bck  = (double) sums[5] * om->M;
+  return p7G_OK;
+}
+
+/*****************************************************************
+ * Batches: many profiles against one database in one call
+ *****************************************************************/
+#define P7G_BATCH_CHAINS 4          // concurrent per-profile stage chains
+
+struct Batch {
+  Work     ch[P7G_BATCH_CHAINS];   // each chain's stream and work area
+  char    *mem = nullptr;
+  int      nprof_cap = 0;
+  int64_t  n_cap = 0;
+  unsigned mask_cap = 0;
+  uint16_t *zmax = nullptr;              // [row][n] ungapped-pass results, rows in shape order
+  int32_t  *queues = nullptr;            // [row] ungapped-pass work-queue cursors
+  unsigned long long *sums = nullptr;    // [profile][8] launch_stats sums
+  int32_t  *counts = nullptr;            // [profile][16] list counts at the end of the run
+  uint8_t  *o_stage = nullptr;           // [profile][n] outputs in the caller's order, as asked for
+  float    *o_f[6] = { nullptr, nullptr, nullptr, nullptr, nullptr, nullptr };
+  std::vector<hipEvent_t> evs;           // fork (one per shape) and join (one per chain) events
+  hipEvent_t t[2];
+  bool     t_ready = false;
+  hipGraph_t     graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  std::vector<unsigned long long> key;   // what the instantiated graph was captured for
+  int      last_graph = 0;
+};
+
+struct BatchItem {                 // one profile of a batch, in shape order
+  int prof;                        // index in the caller's array
+  const P7_GPU_OPROFILE *om;
+  DevProfile *p;
+  int v, grab, shape;
+  GateThresh g;
+};
+
+static void
+batch_drop_graph(Batch &b)
+{
+  if (b.exec) (void) hipGraphExecDestroy(b.exec);
+  if (b.graph) (void) hipGraphDestroy(b.graph);
+  b.exec = nullptr; b.graph = nullptr; b.key.clear();
+}
+
+static void
+batch_free(Batch *b)
+{
+  if (! b) return;
+  batch_drop_graph(*b);
+  for (int k = 0; k < P7G_BATCH_CHAINS; k++) {
+    work_release(b->ch[k]);
+    if (b->ch[k].st) (void) hipStreamDestroy(b->ch[k].st);
+  }
+  if (b->mem) (void) hipFree(b->mem);
+  for (hipEvent_t e : b->evs) (void) hipEventDestroy(e);
+  if (b->t_ready) { (void) hipEventDestroy(b->t[0]); (void) hipEventDestroy(b->t[1]); }
+  delete b;
+}
+
+// Output arrays a call asks for: bit 0 stage, bits 1..6 nullsc, usc,
+// filtersc, vfsc, fwdsc, bcksc.
+static unsigned
+hits_mask(const P7_GPU_HITS *hits, int nprof)
+{
+  unsigned m = 0;
+  for (int j = 0; hits && j < nprof; j++) {
+    const P7_GPU_HITS &h = hits[j];
+    if (h.stage) m |= 1u;
+    const void *f[6] = { h.nullsc, h.usc, h.filtersc, h.vfsc, h.fwdsc, h.bcksc };
+    for (int a = 0; a < 6; a++) if (f[a]) m |= 2u << a;
+  }
+  return m;
+}
+
+static int
+batch_reserve(Batch &b, int nprof, int64_t n, unsigned mask)
+{
+  if (nprof <= b.nprof_cap && n <= b.n_cap && (mask & ~b.mask_cap) == 0) return p7G_OK;
+  batch_drop_graph(b);
+  if (b.mem) { (void) hipFree(b.mem); b.mem = nullptr; }
+  const int64_t np = std::max(nprof, b.nprof_cap), nn = std::max(n, b.n_cap);
+  mask |= b.mask_cap;
+  size_t o = 0;
+  auto take = [&](size_t bytes) { size_t at = o; o += (bytes + 255) & ~(size_t) 255; return at; };
+  const size_t ozm = take(2 * (size_t) (np * nn)), oq = take(4 * (size_t) np), osu = take(64 * (size_t) np),
+               oco = take(64 * (size_t) np), ost = (mask & 1u) ? take((size_t) (np * nn)) : 0;
+  size_t of[6];
+  for (int a = 0; a < 6; a++) of[a] = (mask & (2u << a)) ? take(4 * (size_t) (np * nn)) : 0;
+  char *m = nullptr;
+  HIPCHK(hipMalloc((void **) &m, o));
+  b.mem = m; b.nprof_cap = (int) np; b.n_cap = nn; b.mask_cap = mask;
+  b.zmax = (uint16_t *) (m + ozm); b.queues = (int32_t *) (m + oq);
+  b.sums = (unsigned long long *) (m + osu); b.counts = (int32_t *) (m + oco);
+  b.o_stage = (mask & 1u) ? (uint8_t *) (m + ost) : nullptr;
+  for (int a = 0; a < 6; a++) b.o_f[a] = (mask & (2u << a)) ? (float *) (m + of[a]) : nullptr;
+  return p7G_OK;
+}
+
+// Enqueues one batch on st0 and the chain streams: the ungapped pass of every
+// profile, one persistent launch per kernel shape on st0, each followed by a
+// fork event; then every profile's later stages on chain (row mod nchain),
+// which waits only for its own shape's launch; then st0 joins every chain.
+static int
+batch_enqueue(Batch &b, hipStream_t st0, int nchain, const std::vector<BatchItem> &it, int nshape,
+              const DevSeqs &s, int64_t n, unsigned mask)
+{
+  const int nit = (int) it.size();
+  HIPCHK(launch_fill32(b.queues, 0u, nit, st0));
+  for (int r0 = 0; r0 < nit; ) {
+    int r1 = r0;
+    while (r1 < nit && it[r1].shape == it[r0].shape) r1++;
+    std::vector<const int *> tbl;
+    for (int r = r0; r < r1; r++) tbl.push_back(it[r].p->ssv[it[r].v].tbl);
+    const SsvVariant &sv = it[r0].p->ssv[it[r0].v];
+    HIPCHK(launch_ssv_multi(sv.G, sv.ND, tbl.data(), r1 - r0, s, n, b.zmax + (size_t) r0 * n, n, b.queues + r0,
+                            it[r0].grab, st0));
+    HIPCHK(hipEventRecord(b.evs[it[r0].shape], st0));
+    r0 = r1;
+  }
+  int status;
+  for (int r = 0; r < nit; r++) {
+    Work &w = b.ch[r % nchain];
+    const int j = it[r].prof;
+    HIPCHK(hipStreamWaitEvent(w.st, b.evs[it[r].shape], 0));
+    if ((status = chain_reset(w, n)) != p7G_OK) return status;
+    if ((status = chain_stages(w, *it[r].p, s, n, b.zmax + (size_t) r * n, it[r].g, false)) != p7G_OK) return status;
+    HIPCHK(launch_fill32(b.sums + 8 * (size_t) j, 0u, 16, w.st));
+    HIPCHK(launch_stats(s, w.r.stage, n, w.l.vit, w.l.fwd, w.l.counts, b.sums + 8 * (size_t) j, w.st));
+    HIPCHK(launch_copy32(b.counts + 16 * (size_t) j, w.l.counts, 16, w.st));
+    if (mask & 1u) HIPCHK(launch_scatter8(s, w.r.stage, b.o_stage + (size_t) j * n, n, w.st));
+    const float *src[6] = { w.r.nullsc, w.r.usc, w.r.filtersc, w.r.vfsc, w.r.fwdsc, w.r.bcksc };
+    for (int a = 0; a < 6; a++)
+      if (mask & (2u << a)) HIPCHK(launch_scatter32(s, src[a], b.o_f[a] + (size_t) j * n, n, w.st));
+  }
+  for (int k = 0; k < nchain; k++) {
+    HIPCHK(hipEventRecord(b.evs[nshape + k], b.ch[k].st));
+    HIPCHK(hipStreamWaitEvent(st0, b.evs[nshape + k], 0));
+  }
+  return p7G_OK;
+}
+
+extern "C" int
+p7_gpu_PipelineBatch(P7_GPU_PIPELINE *pli, const P7_GPU_OPROFILE *const *oms, int nprof, const P7_GPU_SEQDB *db,
+                     P7_GPU_HITS *hits, P7_GPU_STATS *stats, int flags)
+{
+  std::lock_guard<std::mutex> lk(pli->mu);
+  int status;
+  if (nprof <= 0 || ! oms || ! db) { p7g_set_error("empty batch"); return p7G_EINVAL; }
+  if ((status = ensure_init()) != p7G_OK) return status;
+  if (! pli->w.st) HIPCHK(hipStreamCreateWithFlags(&pli->w.st, hipStreamNonBlocking));
+  if (! pli->b) { pli->b = new (std::nothrow) Batch(); if (! pli->b) return p7G_EMEM; }
+  Batch &b = *pli->b;
+  const int64_t n = db->nseq;
+  hipStream_t st0 = pli->w.st;
+
+  // profiles, their device copies and chosen shapes (all before any capture)
+  std::vector<BatchItem> it((size_t) nprof);
+  for (int j = 0; j < nprof; j++) {
+    if (! oms[j]) { p7g_set_error("batch profile %d is NULL", j); return p7G_EINVAL; }
+    if ((status = check_query(oms[j], db)) != p7G_OK) return status;
+    BatchItem &x = it[j];
+    x.prof = j; x.om = oms[j];
+    if ((status = dev_profile(oms[j], db->Lmax, &x.p)) != p7G_OK) return status;
+    ssv_choice(*x.p, &x.v, &x.grab);
+    x.g = gates_of(pli, oms[j]);
+  }
+  // rows grouped by shape (G, ND, grab), longest shapes first; input order within a shape
+  std::stable_sort(it.begin(), it.end(), [](const BatchItem &a, const BatchItem &c) {
+    const SsvVariant &x = a.p->ssv[a.v], &y = c.p->ssv[c.v];
+    const long kx = (long) x.G * x.ND, ky = (long) y.G * y.ND;
+    if (kx != ky) return kx > ky;
+    if (x.G != y.G) return x.G > y.G;
+    return a.grab > c.grab;
+  });
+  int nshape = 0;
+  for (int r = 0; r < nprof; r++) {
+    if (r > 0) {
+      const SsvVariant &x = it[r].p->ssv[it[r].v], &y = it[r-1].p->ssv[it[r-1].v];
+      if (x.G != y.G || x.ND != y.ND || it[r].grab != it[r-1].grab) nshape++;
+    }
+    it[r].shape = nshape;
+  }
+  nshape++;
+  const int nchain = std::min(nprof, P7G_BATCH_CHAINS);
+
+  // work areas
+  const unsigned mask = hits_mask(hits, nprof);
+  for (int k = 0; k < nchain; k++) {
+    Work &w = b.ch[k];
+    if (! w.st) HIPCHK(hipStreamCreateWithFlags(&w.st, hipStreamNonBlocking));
+    if ((status = work_reserve(w, n, pipeline_pool_floats(n, db->nres))) != p7G_OK) return status;
+  }
+  if ((status = batch_reserve(b, nprof, n, mask)) != p7G_OK) return status;
+  while ((int) b.evs.size() < nshape + nchain) {
+    hipEvent_t e;
+    HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    b.evs.push_back(e);
+  }
+  if (! b.t_ready) { HIPCHK(hipEventCreate(&b.t[0])); HIPCHK(hipEventCreate(&b.t[1])); b.t_ready = true; }
+  DevSeqs s = seqs_of(db);
+
+  // the graph is replayed while everything it was captured with is the same
+  std::vector<unsigned long long> key = { (unsigned long long) (uintptr_t) b.mem, db->serial, (unsigned long long) n,
+                                          (unsigned long long) nprof, (unsigned long long) nchain, mask };
+  for (int k = 0; k < nchain; k++) {
+    key.push_back((unsigned long long) (uintptr_t) b.ch[k].mem);
+    key.push_back((unsigned long long) (uintptr_t) b.ch[k].pool);
+  }
+  for (const BatchItem &x : it) {
+    const p7g_dev_profile_s *d = (const p7g_dev_profile_s *) ((const char *) x.p - offsetof(p7g_dev_profile_s, p));
+    auto bits = [](float f) { uint32_t u; memcpy(&u, &f, 4); return (unsigned long long) u; };
+    key.insert(key.end(), { (unsigned long long) x.prof, d->serial, (unsigned long long) x.v, (unsigned long long) x.grab,
+                            bits(x.g.T1m), bits(x.g.T2m), bits(x.g.T2v), bits(x.g.T3f),
+                            (unsigned long long) (x.g.do_bias * 2 + x.g.do_max) });
+  }
+  const bool eager = (flags & P7_GPU_BATCH_EAGER) != 0;
+  if (! eager && (! b.exec || key != b.key)) {
+    batch_drop_graph(b);
+    HIPCHK(hipStreamBeginCapture(st0, hipStreamCaptureModeThreadLocal));
+    status = batch_enqueue(b, st0, nchain, it, nshape, s, n, mask);
+    hipGraph_t gr = nullptr;
+    const hipError_t ec = hipStreamEndCapture(st0, &gr);
+    if (status != p7G_OK) { if (gr) (void) hipGraphDestroy(gr); return status; }
+    HIPCHK(ec);
+    b.graph = gr;
+    HIPCHK(hipGraphInstantiateWithFlags(&b.exec, b.graph, 0));
+    b.key = key;
+  }
+
+  HIPCHK(hipEventRecord(b.t[0], st0));
+  if (eager) { if ((status = batch_enqueue(b, st0, nchain, it, nshape, s, n, mask)) != p7G_OK) return status; }
+  else HIPCHK(hipGraphLaunch(b.exec, st0));
+  HIPCHK(hipEventRecord(b.t[1], st0));
+  b.last_graph = eager ? 0 : 1;
+
+  std::vector<unsigned long long> sums((size_t) nprof * 8);
+  std::vector<int32_t> counts((size_t) nprof * 16);
+  HIPCHK(hipMemcpyAsync(sums.data(), b.sums, sizeof(unsigned long long) * sums.size(), hipMemcpyDeviceToHost, st0));
+  HIPCHK(hipMemcpyAsync(counts.data(), b.counts, sizeof(int32_t) * counts.size(), hipMemcpyDeviceToHost, st0));
+  for (int j = 0; hits && j < nprof; j++) {
+    if (hits[j].stage) HIPCHK(hipMemcpyAsync(hits[j].stage, b.o_stage + (size_t) j * n, (size_t) n, hipMemcpyDeviceToHost, st0));
+    float *dst[6] = { hits[j].nullsc, hits[j].usc, hits[j].filtersc, hits[j].vfsc, hits[j].fwdsc, hits[j].bcksc };
+    for (int a = 0; a < 6; a++)
+      if (dst[a]) HIPCHK(hipMemcpyAsync(dst[a], b.o_f[a] + (size_t) j * n, sizeof(float) * (size_t) n, hipMemcpyDeviceToHost, st0));
+  }
+  HIPCHK(hipStreamSynchronize(st0));
+  float ms = 0.0f;
+  HIPCHK(hipEventElapsedTime(&ms, b.t[0], b.t[1]));
+  for (int j = 0; j < nprof; j++)
+    if (counts[(size_t) j * 16 + 6]) { p7g_set_error("Forward scale-factor pool exhausted (batch profile %d)", j); return p7G_EMEM; }
+  for (int j = 0; j < nprof; j++) {
+    P7_GPU_STATS one;
+    stats_of_run(one, &sums[(size_t) j * 8], &counts[(size_t) j * 16], oms[j], db, pli->do_biasfilter);
+    one.ms_total = ms;
+    if (stats) stats[j] = one;
+    stats_accumulate(pli->st, one, j == 0);
+  }
+  pli->st.ms_total = ms;
+  pli->st.ms_msv = pli->st.ms_bias = pli->st.ms_vit = pli->st.ms_fwd = pli->st.ms_bck = pli->st.ms_ssv = 0.0;
+  return p7G_OK;
+}
+
+extern "C" int
+p7_gpu_pipeline_BatchGraphed(const P7_GPU_PIPELINE *pli)
+{
+  return (pli && pli->b) ? pli->b->last_graph : 0;
+}
+
+/*****************************************************************
+ * Tuner of the ungapped MSV pass
+ *****************************************************************/
+static int
+arch_name(char *buf, size_t len)
+{
+  hipDeviceProp_t prop;
+  HIPCHK(hipGetDeviceProperties(&prop, g_device >= 0 ? g_device : 0));
+  snprintf(buf, len, "%s", prop.gcnArchName);
+  for (char *c = buf; *c; c++) if (*c == ' ' || *c == '\n') *c = '_';
+  return p7G_OK;
+}
+
+// Times the ungapped pass of variant v with queue grab `grab` over the sample;
+// the best of three runs after one untimed run, in microseconds.
+static int
+tune_time(const DevProfile &p, int v, int grab, const DevSeqs &s, const int32_t *d_slots, int64_t ns, uint16_t *zmax,
+          int32_t *queue, hipStream_t st, hipEvent_t e0, hipEvent_t e1, float *ret_us)
+{
+  float best = 1e30f;
+  for (int rep = 0; rep < 4; rep++) {
+    HIPCHK(launch_fill32(queue, 0u, 1, st));
+    HIPCHK(hipEventRecord(e0, st));
+    HIPCHK(launch_ssv(p, v, s, d_slots, ns, zmax, queue, grab, st));
+    HIPCHK(hipEventRecord(e1, st));
+    HIPCHK(hipEventSynchronize(e1));
+    float ms = 0.0f;
+    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+    if (rep > 0) best = std::min(best, ms * 1000.0f);
+  }
+  *ret_us = best;
+  return p7G_OK;
+}
+
+struct TuneScratch {
+  hipStream_t st = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  void *mem = nullptr;
+  ~TuneScratch()
+  {
+    if (st) (void) hipStreamSynchronize(st);
+    if (mem) (void) hipFree(mem);
+    if (e0) (void) hipEventDestroy(e0);
+    if (e1) (void) hipEventDestroy(e1);
+    if (st) (void) hipStreamDestroy(st);
+  }
+};
+
+extern "C" int
+p7_gpu_Autotune(const P7_GPU_OPROFILE *const *oms, int nprof, const P7_GPU_SEQDB *db)
+{
+  int status;
+  if (nprof < 0 || (nprof > 0 && ! oms) || ! db) { p7g_set_error("bad tuner arguments"); return p7G_EINVAL; }
+  if ((status = ensure_init()) != p7G_OK) return status;
+  TuneScratch ts;
+  HIPCHK(hipStreamCreateWithFlags(&ts.st, hipStreamNonBlocking));
+  HIPCHK(hipEventCreate(&ts.e0));
+  HIPCHK(hipEventCreate(&ts.e1));
+  const int64_t n = db->nseq;
+  char *m = nullptr;
+  const size_t zbytes = ((2 * (size_t) n + 255) & ~(size_t) 255);
+  HIPCHK(hipMalloc((void **) &m, zbytes + 256 + 4 * (size_t) n));
+  ts.mem = m;
+  uint16_t *zmax = (uint16_t *) m;
+  int32_t *queue = (int32_t *) (m + zbytes);
+  int32_t *d_slots = (int32_t *) (m + zbytes + 256);
+  DevSeqs s = seqs_of(db);
+  static const int grabs[] = { 1, 2, 4, 8 };
+  std::set<int> done;
+  std::vector<int32_t> sample;
+  for (int j = 0; j < nprof; j++) {
+    DevProfile *p;
+    if (! oms[j]) { p7g_set_error("tuner profile %d is NULL", j); return p7G_EINVAL; }
+    if ((status = check_query(oms[j], db)) != p7G_OK) return status;
+    if ((status = dev_profile(oms[j], db->Lmax, &p)) != p7G_OK) return status;
+    const int cls = tune_class(p->M);
+    if (done.count(cls)) continue;
+    // every k-th slot (slots run longest first, so the sample keeps the length
+    // mix), k chosen for about 3e10 cells per timed run
+    const int64_t k = std::max<int64_t>(1, (int64_t) ((double) db->nres * p->M / 3e10));
+    sample.clear();
+    for (int64_t t = 0; t < n; t += k) sample.push_back((int32_t) t);
+    const int64_t ns = (int64_t) sample.size();
+    HIPCHK(hipMemcpyAsync(d_slots, sample.data(), sizeof(int32_t) * (size_t) ns, hipMemcpyHostToDevice, ts.st));
+    // shapes that compute more than twice the positions of the tightest one are not tried
+    int minpos = INT32_MAX;
+    for (int v = 0; v < P7G_SSV_NVAR; v++) if (p->ssv[v].G) minpos = std::min(minpos, 2 * p->ssv[v].G * p->ssv[v].ND);
+    TuneEntry e;
+    e.v = p->ssv_def; e.grab = SSV_GRAB_DEFAULT;
+    float best = 1e30f;
+    for (int v = 0; v < P7G_SSV_NVAR; v++) {
+      e.us[v] = -1.0f;
+      if (! p->ssv[v].G || 2 * p->ssv[v].G * p->ssv[v].ND > 2 * minpos) continue;
+      for (int gi = 0; gi < 4; gi++) {
+        float us;
+        if ((status = tune_time(*p, v, grabs[gi], s, d_slots, ns, zmax, queue, ts.st, ts.e0, ts.e1, &us)) != p7G_OK) return status;
+        if (e.us[v] < 0.0f || us < e.us[v]) e.us[v] = us;
+        if (us < best) { best = us; e.v = v; e.grab = grabs[gi]; }
+      }
+    }
+    {
+      std::lock_guard<std::mutex> lk(g_tunelock);
+      g_tune[cls] = e;
+    }
+    done.insert(cls);
+  }
+  return p7G_OK;
+}
+
+extern "C" int
+p7_gpu_tune_Get(int M, int *ret_lanes, int *ret_grab, float *ret_us)
+{
+  const int G[P7G_SSV_NVAR] = { 16, 32, 64 };
+  int lanes = (M <= 1024) ? 16 : 64, grab = SSV_GRAB_DEFAULT, tuned = 0;
+  if (ret_us) for (int v = 0; v < P7G_SSV_NVAR; v++) ret_us[v] = -1.0f;
+  {
+    std::lock_guard<std::mutex> lk(g_tunelock);
+    auto f = g_tune.find(tune_class(M));
+    if (M > 0 && f != g_tune.end()) {
+      lanes = G[f->second.v]; grab = f->second.grab; tuned = 1;
+      if (ret_us) for (int v = 0; v < P7G_SSV_NVAR; v++) ret_us[v] = f->second.us[v];
+    }
+  }
+  if (ret_lanes) *ret_lanes = lanes;
+  if (ret_grab) *ret_grab = grab;
+  return tuned;
+}
+
+extern "C" void
+p7_gpu_tune_Clear(void)
+{
+  std::lock_guard<std::mutex> lk(g_tunelock);
+  g_tune.clear();
+}
+
+extern "C" int
+p7_gpu_tune_Save(const char *path)
+{
+  int status;
+  char arch[256];
+  if ((status = ensure_init()) != p7G_OK) return status;
+  if ((status = arch_name(arch, sizeof(arch))) != p7G_OK) return status;
+  FILE *fp = fopen(path, "w");
+  if (! fp) { p7g_set_error("cannot write %s", path); return p7G_EFORMAT; }
+  fprintf(fp, "p7gpu-tune 1 %s\n", arch);
+  {
+    std::lock_guard<std::mutex> lk(g_tunelock);
+    for (const auto &kv : g_tune)
+      fprintf(fp, "%d %d %d %.3f %.3f %.3f\n", kv.first, kv.second.v, kv.second.grab,
+              kv.second.us[0], kv.second.us[1], kv.second.us[2]);
+  }
+  if (fclose(fp) != 0) { p7g_set_error("cannot write %s", path); return p7G_EFORMAT; }
+  return p7G_OK;
+}
+
+extern "C" int
+p7_gpu_tune_Load(const char *path)
+{
+  int status;
+  char arch[256], farch[256];
+  if ((status = ensure_init()) != p7G_OK) return status;
+  if ((status = arch_name(arch, sizeof(arch))) != p7G_OK) return status;
+  FILE *fp = fopen(path, "r");
+  if (! fp) { p7g_set_error("cannot open %s", path); return p7G_ENOTFOUND; }
+  int ver = 0;
+  if (fscanf(fp, "p7gpu-tune %d %255s", &ver, farch) != 2 || ver != 1) {
+    fclose(fp); p7g_set_error("%s is not a tuning file", path); return p7G_EFORMAT;
+  }
+  if (strcmp(arch, farch) != 0) {
+    fclose(fp); p7g_set_error("%s was measured on %s, this device is %s", path, farch, arch); return p7G_EINCOMPAT;
+  }
+  std::map<int, TuneEntry> got;
+  int cls, v, grab;
+  float u0, u1, u2;
+  int nf;
+  while ((nf = fscanf(fp, "%d %d %d %f %f %f", &cls, &v, &grab, &u0, &u1, &u2)) == 6) {
+    if (cls < 1 || v < 0 || v >= P7G_SSV_NVAR || (grab != 1 && grab != 2 && grab != 4 && grab != 8)) { nf = -2; break; }
+    TuneEntry e; e.v = v; e.grab = grab; e.us[0] = u0; e.us[1] = u1; e.us[2] = u2;
+    got[cls] = e;
+  }
+  fclose(fp);
+  if (nf != EOF) { p7g_set_error("%s: bad tuning line", path); return p7G_EFORMAT; }
+  std::lock_guard<std::mutex> lk(g_tunelock);
+  for (const auto &kv : got) g_tune[kv.first] = kv.second;
   return p7G_OK;
 }
 
@@ -1039,7 +1615,7 @@ single_msv(const uint8_t *dsq, int L, const P7_GPU_OPROFILE *om, int *ret_xj)
   Work &w = g_work;
   LenCfg lc = lencfg_of(om);
   if ((status = reset_counts(g_work)) != p7G_OK) return status;
-  HIPCHK(launch_ssv(*p, s, nullptr, 1, w.r.zmax, &w.l.counts[P7G_Q_SSV], g_stream));
+  HIPCHK(launch_ssv_chosen(*p, s, nullptr, 1, w.r.zmax, &w.l.counts[P7G_Q_SSV], g_stream));
   HIPCHK(launch_msv_classify(*p, s, nullptr, 1, w.r.zmax, w.r.xj, w.l.full, &w.l.counts[0], lc, g_stream));
   HIPCHK(launch_compact_i16(w.r.xj, -2, 1, w.l.full, &w.l.counts[0], w.tilecnt, g_stream));
   HIPCHK(launch_msv_full(*p, s, w.l.full, &w.l.counts[0], 1, w.r.xj, lc, g_stream));

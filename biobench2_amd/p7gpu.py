@@ -18,6 +18,7 @@ OK, FAIL, EOF, EMEM, ENOTFOUND, EFORMAT, EINCOMPAT, EINVAL, ESYS, ERANGE, ENORES
     0, 1, 3, 5, 6, 7, 10, 11, 12, 16, 19, 64
 RNA, DNA, AMINO = 1, 2, 3
 LOCAL, GLOCAL, UNILOCAL = 1, 2, 3
+BATCH_EAGER = 1              # p7_gpu_PipelineBatch flag: launch directly, no graph
 XE, XN, XJ, XC = 0, 1, 2, 3
 MOVE, LOOP = 0, 1
 BM, MM, IM, DM, MD, MI, II, DD = range(8)
@@ -107,6 +108,13 @@ def _load():
         "p7_gpu_server_Submit": (i, [vp, vp, POINTER(Hits), POINTER(c_int64)]),
         "p7_gpu_server_Wait": (i, [vp, c_int64, POINTER(Stats), POINTER(c_double)]),
         "p7_gpu_server_Destroy": (None, [vp]),
+        "p7_gpu_PipelineBatch": (i, [vp, POINTER(vp), i, vp, POINTER(Hits), POINTER(Stats), i]),
+        "p7_gpu_pipeline_BatchGraphed": (i, [vp]),
+        "p7_gpu_Autotune": (i, [POINTER(vp), i, vp]),
+        "p7_gpu_tune_Get": (i, [i, POINTER(c_int), POINTER(c_int), pf]),
+        "p7_gpu_tune_Clear": (None, []),
+        "p7_gpu_tune_Save": (i, [c_char_p]),
+        "p7_gpu_tune_Load": (i, [c_char_p]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -524,6 +532,26 @@ class Pipeline:
         _check(lib.p7_gpu_Pipeline(self.h, om.h, db.h, ctypes.byref(hits)), "Pipeline")
         return out
 
+    def run_batch(self, oms, db, scores=True, eager=False):
+        """p7_gpu_PipelineBatch: every profile of `oms` over the database in one
+        call (one launch per kernel shape for the ungapped pass, concurrent
+        per-profile chains, replayed from a captured HIP graph unless eager).
+        Returns one output dict and one stats dict per profile."""
+        n = len(oms)
+        arr = (c_void_p * n)(*[om.h for om in oms])
+        outs, hits = [], (Hits * n)()
+        for j in range(n):
+            o, h = _hits(db.nseq, scores)
+            outs.append(o)
+            hits[j] = h
+        st = (Stats * n)()
+        _check(lib.p7_gpu_PipelineBatch(self.h, arr, n, db.h, hits, st, BATCH_EAGER if eager else 0), "PipelineBatch")
+        return outs, [st[j].as_dict() for j in range(n)]
+
+    @property
+    def batch_graphed(self):
+        return bool(lib.p7_gpu_pipeline_BatchGraphed(self.h))
+
     def stats(self):
         st = Stats()
         _check(lib.p7_gpu_pipeline_GetStats(self.h, ctypes.byref(st)), "GetStats")
@@ -533,3 +561,29 @@ class Pipeline:
         if getattr(self, "h", None):
             lib.p7_gpu_pipeline_Destroy(self.h)
             self.h = None
+
+
+def autotune(oms, db):
+    """p7_gpu_Autotune: measure the ungapped-pass shapes for the models' length classes."""
+    arr = (c_void_p * len(oms))(*[om.h for om in oms])
+    _check(lib.p7_gpu_Autotune(arr, len(oms), db.h), "Autotune")
+
+
+def tune_get(M):
+    """(lanes per target, queue grab, best us per shape [16, 32, 64 lanes], tuned?) for models of length M."""
+    lanes, grab = c_int(), c_int()
+    us = np.zeros(3, np.float32)
+    t = lib.p7_gpu_tune_Get(int(M), ctypes.byref(lanes), ctypes.byref(grab), _ptr(us, c_float))
+    return lanes.value, grab.value, us, bool(t)
+
+
+def tune_clear():
+    lib.p7_gpu_tune_Clear()
+
+
+def tune_save(path):
+    _check(lib.p7_gpu_tune_Save(str(path).encode()), "tune_Save")
+
+
+def tune_load(path):
+    _check(lib.p7_gpu_tune_Load(str(path).encode()), "tune_Load")

@@ -239,6 +239,45 @@ void  p7_gpu_pipeline_Destroy(P7_GPU_PIPELINE *pli);
  * the counters like p7_pli_NewModel/NewSeq/p7_Pipeline. */
 int   p7_gpu_Pipeline(P7_GPU_PIPELINE *pli, const P7_GPU_OPROFILE *om, const P7_GPU_SEQDB *db, P7_GPU_HITS *hits);
 int   p7_gpu_pipeline_GetStats(const P7_GPU_PIPELINE *pli, P7_GPU_STATS *st);
+/* ---- batches: many profiles against one resident database ----------------
+ * Runs the p7_gpu_Pipeline filter chain of nprof profiles over every database
+ * sequence in one call. The ungapped MSV pass of all profiles that share a
+ * kernel shape runs as one persistent launch, with a device work queue per
+ * profile; each profile's later stages (MSV decision, bias filter, Viterbi,
+ * Forward, Backward) follow on one of up to four concurrent streams as soon as
+ * its shape's launch is done. The whole sequence is captured once as a HIP
+ * graph and replayed while the same profiles, database, thresholds and output
+ * arrays are asked for (flags P7_GPU_BATCH_EAGER: launch it directly).
+ * hits and stats are arrays of nprof entries; either may be NULL, and so may
+ * any pointer in hits. Each profile's results equal a p7_gpu_Pipeline call
+ * with the same pipeline. Counters add to the pipeline's totals; ms_total of
+ * every stats entry, and of the totals, is the device time of the whole batch.
+ * No counterpart in the reference, whose hmmsearch runs one query at a time. */
+#define P7_GPU_BATCH_EAGER 1
+int   p7_gpu_PipelineBatch(P7_GPU_PIPELINE *pli, const P7_GPU_OPROFILE *const *oms, int nprof,
+                           const P7_GPU_SEQDB *db, P7_GPU_HITS *hits, P7_GPU_STATS *stats, int flags);
+/* 1 if the last batch of this pipeline ran from the captured graph. */
+int   p7_gpu_pipeline_BatchGraphed(const P7_GPU_PIPELINE *pli);
+
+/* ---- tuner of the MSV stage's ungapped pass --------------------------------
+ * For the length class of each model (M rounded up to a multiple of 32), times
+ * every shape of the ungapped pass the model fits -- 16, 32 or 64 lanes per
+ * target, i.e. 4, 2 or 1 targets per wavefront, each with its
+ * positions-per-lane tile -- with work-queue grabs of 1, 2, 4 and 8 target
+ * groups, on every k-th sequence of db (so the sample keeps the length mix),
+ * and keeps the fastest for this device. Shapes that compute more than twice
+ * the positions of the tightest one are not tried. Pipelines, batches and the
+ * server use the kept choice from then on; no result depends on it. */
+int   p7_gpu_Autotune(const P7_GPU_OPROFILE *const *oms, int nprof, const P7_GPU_SEQDB *db);
+/* The choice for models of length M: lanes per target, queue grab, and the
+ * best time of each shape (16, 32, 64 lanes; -1 if not tried). Returns 1 if
+ * M's class was measured, 0 if the built-in default applies. */
+int   p7_gpu_tune_Get(int M, int *ret_lanes, int *ret_grab, float *ret_us3);
+void  p7_gpu_tune_Clear(void);
+/* A text file of the measured choices, for the device's architecture. */
+int   p7_gpu_tune_Save(const char *path);
+int   p7_gpu_tune_Load(const char *path);
+
 /* Score threshold of a stage gate: the smallest float bit score s whose
  * P-value is <= P under the Gumbel (which = 0, easel/esl_gumbel.c:129) or
  * exponential (which = 1, easel/esl_exponential.c:127) tail, so the gate
