@@ -118,6 +118,69 @@ def serving_leg(p7gpu, synth, sdb, nq, rate, nslots, seed=11):
             "identical_pass_sets": same}
 
 
+def batched_leg(p7gpu, synth, nprof, nseq, reps=3, seed=42):
+    """Many profiles of uneven M against one database with a long-sequence tail:
+    the tuner picks each length class's ungapped-pass shape, then the same
+    profiles run one by one through a pipeline (plain per-query drop-in) and
+    as one p7_gpu_PipelineBatch call (one persistent launch per kernel shape,
+    concurrent stage chains, replayed HIP graph). Reports both aggregate GCUPS
+    and whether every profile's pass set is the same in both."""
+    import tempfile
+    import shutil
+    tmp = tempfile.mkdtemp(prefix="p7b_")
+    t0 = time.perf_counter()
+    paths = synth.batch_family(p7gpu, tmp, nprof=nprof, seed=seed)
+    oms = [p7gpu.OProfile.from_file(pth) for pth in paths]
+    Ms = [om.tables()["M"] for om in oms]
+    hmms = [synth.read_hmm_probs(pth) for pth in paths[:8]]
+    t_prof = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    db = synth.make_db(nseq, hmms, frac_homolog=0.01, seed_db=seed + 1, seed_plant=seed + 2,
+                       long_frac=0.02, long_range=(5000, 35000))
+    sdb = p7gpu.SeqDB(db.residues, db.offsets, db.lengths)
+    t_db = time.perf_counter() - t0
+    log(f"batched leg: {nprof} profiles (M {min(Ms)}..{max(Ms)}) in {t_prof:.1f} s, "
+        f"db {db.nseq} sequences / {db.nres} residues in {t_db:.1f} s")
+    cells = float(sum(Ms)) * db.nres
+
+    t0 = time.perf_counter()
+    p7gpu.autotune(oms, sdb)
+    t_tune = time.perf_counter() - t0
+    shapes = {}
+    for M in Ms:
+        lanes, grab, _, _ = p7gpu.tune_get(M)
+        shapes[f"{lanes}x{grab}"] = shapes.get(f"{lanes}x{grab}", 0) + 1
+
+    pli = p7gpu.Pipeline()
+    for om in oms:                                   # device copies of every profile
+        pli.run(om, sdb, scores=False)
+    p7gpu.lib.p7_gpu_Synchronize()
+    t0 = time.perf_counter()
+    plain = [pli.run(om, sdb, scores=False)["stage"] for om in oms]
+    t_plain = time.perf_counter() - t0
+    log(f"batched leg: plain per-query pass {t_plain:.3f} s")
+
+    pb = p7gpu.Pipeline()
+    t0 = time.perf_counter()
+    pb.run_batch(oms, sdb, scores=False)             # capture + first replay
+    t_first = time.perf_counter() - t0
+    same = True
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        outs, sts = pb.run_batch(oms, sdb, scores=False)
+    t_batch = (time.perf_counter() - t0) / reps
+    for j in range(nprof):
+        same &= bool(np.array_equal(outs[j]["stage"], plain[j]))
+    shutil.rmtree(tmp, ignore_errors=True)
+    return {"profiles": nprof, "M_min": min(Ms), "M_max": max(Ms), "M_mean": round(float(np.mean(Ms)), 1),
+            "nseq": db.nseq, "nres": db.nres, "long_tail": "2% of sequences with L in [5000, 35000]",
+            "cells": cells, "tune_s": round(t_tune, 3), "tuned_shapes_lanes_x_grab": shapes,
+            "graph_capture_and_first_run_s": round(t_first, 3), "graphed": pb.batch_graphed,
+            "ms_batch": round(t_batch * 1000.0, 2), "ms_plain_per_query_total": round(t_plain * 1000.0, 2),
+            "gcups_batched": round(cells / t_batch / 1e9, 1), "gcups_plain_per_query": round(cells / t_plain / 1e9, 1),
+            "identical_pass_sets": same}
+
+
 def calibrate_copy(mib=256, reps=3):
     """Plain device-to-device copies of a known size: what FETCH_SIZE and WRITE_SIZE
     report for them gives the factor from counter KiB to HBM bytes."""
@@ -143,6 +206,8 @@ def main():
     ap.add_argument("--serve-queries", type=int, default=32, help="queries in the serving leg (0: skip it)")
     ap.add_argument("--serve-rate", type=float, default=100.0, help="Poisson arrival rate of the serving leg, queries/s")
     ap.add_argument("--serve-slots", type=int, default=4)
+    ap.add_argument("--batch-profiles", type=int, default=64, help="profiles in the batched leg (0: skip it)")
+    ap.add_argument("--batch-nseq", type=int, default=500_000, help="database sequences of the batched leg")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run three plain 256 MiB device copies (to scale the FETCH_SIZE/WRITE_SIZE counters)")
     args = ap.parse_args()
@@ -165,6 +230,10 @@ def main():
     log(f"db: {db.nseq} sequences, {db.nres} residues, generated in {time.time() - t0:.1f} s")
     sdb = p7gpu.SeqDB(db.residues, db.offsets, db.lengths)
     pli = p7gpu.Pipeline()
+    t0 = time.perf_counter()
+    p7gpu.autotune([om], sdb)                     # the ungapped pass's shape for M = 400 on this device
+    lanes, grab, us, _ = p7gpu.tune_get(M)
+    log(f"tuner: {lanes} lanes per target, grab {grab} ({time.perf_counter() - t0:.2f} s; us per shape {us.tolist()})")
 
     for w in range(args.warmup):
         pli.run(om, sdb, scores=False)
@@ -212,7 +281,7 @@ def main():
         "data": "synthetic (seeded profile generator M=400; lognormal-length i.i.d. database, 1% planted homologs)",
         "config": {"workload": "hmmsearch filter pipeline, one query M=400 vs 2M-sequence database",
                    "nseq": db.nseq, "nres": db.nres, "M": M, "F1": 0.02, "F2": 1e-3, "F3": 1e-5},
-        "roofline": {"bound": "valu", "kernel": "k_ssv (ungapped MSV pass, packed int16)",
+        "roofline": {"bound": "valu", "kernel": f"k_ssv (ungapped MSV pass, packed int16, {lanes} lanes per target)",
                      "achieved": round(ssv_tops, 3), "peak": PEAK_PK16_TOPS, "unit": "TOP/s",
                      "frac": round(ssv_tops / PEAK_PK16_TOPS, 4), "traffic": traffic,
                      "ms_per_launch": round(ssv_ms, 4), "ops_per_cell": SSV_OPS_PER_CELL,
@@ -222,6 +291,9 @@ def main():
                         "vit": int((stage >= 3).sum()), "fwd": int((stage >= 4).sum()),
                         "msv_full_rescored": st["n_msv_full"]},
     }
+    if args.batch_profiles > 0:
+        log("batched leg ...")
+        line["batched"] = batched_leg(p7gpu, synth, args.batch_profiles, args.batch_nseq)
     if args.serve_queries > 0:
         log("serving leg ...")
         line["serving"] = serving_leg(p7gpu, synth, sdb, args.serve_queries, args.serve_rate, args.serve_slots)

@@ -26,6 +26,8 @@ struct SsvVariant {
   size_t dwords;
 };
 #define P7G_SSV_NVAR 3   // G = 16, 32, 64
+#define P7G_LDS_MAX (160 * 1024)   // LDS of one workgroup on gfx950
+#define P7G_SSV_MAXCHUNK 3
 
 struct DevProfile {
   int M, Kp;
@@ -34,7 +36,11 @@ struct DevProfile {
   // bank conflicts for any per-lane block size.
   // MSV stage, ungapped pass: the variants with 16, 32 and 64 lanes per target
   SsvVariant ssv[P7G_SSV_NVAR];
-  int   ssv_def;         // variant used when no tuned choice applies
+  int   ssv_def;         // variant used when no tuned choice applies; -1: chunked
+  // models longer than one LDS table holds (M > 2432): two or three chunks of
+  // 64 lanes per target, chunk c holding positions c*128*ND+1 .. (c+1)*128*ND
+  int   ssv_nchunk;      // 0, 2 or 3
+  SsvVariant ssvc[P7G_SSV_MAXCHUNK];
   // MSV stage, full recurrence with the J state: same scores, 64 lanes
   int   mf_ND;
   int  *mf_tbl;      size_t mf_dwords;      // [rows][ND][64]
@@ -90,7 +96,8 @@ struct DevLists {
   int32_t *full, *bias, *vit, *fwd, *bck;
   // [0] full, [1] bias, [2] vit, [3] fwd, [4] bck: list lengths; [6] scale-pool
   // overflow flag; the work-queue cursor of list k sits at [k + 8]; [13] is the
-  // cursor of the ungapped pass over all targets. Zeroed before every call.
+  // cursor of the ungapped pass over all targets ([14], [15]: the later
+  // launches of a chunked model). Zeroed before every call.
   int32_t *counts;
   unsigned long long *cursor;   // next free float in the Forward scale-factor pool
 };
@@ -131,8 +138,15 @@ hipError_t launch_ssv(const DevProfile &p, int v, const DevSeqs &s, const int32_
 // zmax + j * zstride and takes work from queues[j] (zero on entry).
 hipError_t launch_ssv_multi(int G, int ND, const int *const *tbl, int njobs, const DevSeqs &s, int64_t n,
                             uint16_t *zmax, int64_t zstride, int32_t *queues, int grab, hipStream_t st);
+// The ungapped pass of a chunked model: one launch per chunk, each reading
+// the boundary column the one before wrote. bnd holds two columns of
+// bnd_stride entries (at least one per residue byte of the database);
+// queues[0 .. nchunk-1] are the launches' work-queue cursors (zero on entry).
+hipError_t launch_ssv_chunked(const DevProfile &p, const DevSeqs &s, const int32_t *slots, int64_t n, uint16_t *zmax,
+                              int32_t *queues, int grab, uint16_t *bnd, int64_t bnd_stride, hipStream_t st);
 // 1 if a kernel of this shape is built
 int ssv_shape_built(int G, int ND);
+int ssv_chunk_built(int ND);
 // launchers (p7g_kernels.hip)
 // p[0..nwords) = v, as a kernel (so that a captured graph holds no memset node)
 hipError_t launch_fill32(void *p, uint32_t v, int64_t nwords, hipStream_t st);

@@ -57,7 +57,9 @@ __global__ void k_msv_classify(const DevProfile p, const int32_t *__restrict__ l
 /*****************************************************************
  * 3. MSV stage, full recurrence (one target per wave, 64 lanes)
  *****************************************************************/
-template <int ND>
+// LDS: the score table is staged in LDS; otherwise (models whose table does
+// not fit, ND >= 24) the rows are read from global memory.
+template <int ND, bool LDS>
 __global__ __launch_bounds__(512)
 void k_msv_full(const DevProfile p, const int *__restrict__ tbl, int tbl_dwords,
                 const uint8_t *__restrict__ res, const int64_t *__restrict__ off, const int32_t *__restrict__ len,
@@ -65,7 +67,8 @@ void k_msv_full(const DevProfile p, const int *__restrict__ tbl, int tbl_dwords,
 {
   if ((int64_t) blockIdx.x * (blockDim.x >> 6) >= *count) return;
   extern __shared__ int lds[];
-  stage_table(lds, tbl, (size_t) tbl_dwords);
+  const int *rows = tbl;
+  if (LDS) { stage_table(lds, tbl, (size_t) tbl_dwords); rows = lds; }
   const int z = threadIdx.x & 63;
   const int64_t n = *count;
   int32_t *queue = (int32_t *) count + P7G_QOFF;
@@ -85,7 +88,7 @@ void k_msv_full(const DevProfile p, const int *__restrict__ tbl, int tbl_dwords,
     if (L > 0) rd.init(sq);
     for (int i = 0; i < L; i++) {
       const int x = rd.at(i);
-      const int *row = lds + x * (ND * 64) + z;
+      const int *row = rows + x * (ND * 64) + z;
       const int s0 = shift_in<64>(mp[ND-1], 0);
       const int xB2 = pair16(xB);
       int E = 0;
@@ -369,24 +372,55 @@ __device__ __forceinline__ void dd_closure(int (&D)[ND], const DDConst<ND> &c, i
   for (int d = 0; d < ND; d++) { Q = adds2(Q, c.t[d]); D[d] = max2(P[d], Q); }
 }
 
-template <int ND>
-__global__ __launch_bounds__(512)
+// Where a list kernel finds its profile tables (template argument TM):
+//   TM_LDS   the whole table in LDS, transitions pinned in registers
+//   TM_PIN   the table in global memory (L2-resident), transitions pinned
+//   TM_TLDS  transitions in LDS, read per row; emission rows from global memory
+//   TM_GLOB  the table in global memory, transitions read per row
+// Large models take the last two: B positions of every transition pinned in
+// registers would not fit the register file, and the whole table not in LDS.
+// The per-row reads go through an offset the compiler cannot see through, so
+// that they are not hoisted out of the row loop into registers again.
+enum { TM_LDS = 0, TM_PIN = 1, TM_TLDS = 2, TM_GLOB = 3 };
+
+__device__ __forceinline__ int opaque_zero()
+{
+  int v = 0;
+  asm volatile("" : "+v"(v));
+  return v;
+}
+
+// Workgroup sizes of the list kernels: 8 waves, or 4 (one per SIMD, so that a
+// wave may hold 512 registers with the accumulation registers) where the
+// per-lane state of large models does not fit in 256.
+#define VIT_THREADS(ND) ((ND) >= 40 ? 256 : 512)
+#define FWD_THREADS(B)  ((B) >= 64 ? 256 : 512)
+#define BCK_THREADS(B)  ((B) >= 28 ? 256 : 512)
+
+template <int ND, int TM>
+__global__ __launch_bounds__(VIT_THREADS(ND))
 void k_vit(const DevProfile p, const uint8_t *__restrict__ res, const int64_t *__restrict__ off,
            const int32_t *__restrict__ len, const int32_t *__restrict__ list, const int32_t *__restrict__ count,
            DevResults r, DevLists l, GateThresh g, int gating, LenCfg lc)
 {
 #pragma clang fp contract(off)
+  static_assert(TM == TM_LDS || TM == TM_TLDS, "Viterbi tables are in LDS, whole or transitions only");
+  constexpr bool PIN = (TM == TM_LDS);
   if ((int64_t) blockIdx.x * (blockDim.x >> 6) >= *count) return;
   extern __shared__ int lds[];
   const size_t rowd = (size_t) 64 * ND;
-  stage_table(lds, p.vf_tbl, p.vf_dwords);
+  const int *tbl = p.vf_tbl;                             // emission rows
+  if (TM == TM_LDS) { stage_table(lds, p.vf_tbl, p.vf_dwords); tbl = lds; }
+  else stage_table(lds, p.vf_tbl + (size_t) p.nrows * rowd, (size_t) P7G_VF_NT * rowd);
   const int z = threadIdx.x & 63;
-  const int *tr = lds + (size_t) p.nrows * rowd + z;
+  const int *tr = ((TM == TM_LDS) ? lds + (size_t) p.nrows * rowd : lds) + z;
   int tBM[ND], tMM[ND], tIM[ND], tDM[ND], tMDS[ND], tMI[ND], tII[ND];
+  if (PIN) {
 #pragma unroll
-  for (int d = 0; d < ND; d++) {
-    tBM[d] = tr[VT_BM*rowd + d*64]; tMM[d] = tr[VT_MM*rowd + d*64]; tIM[d] = tr[VT_IM*rowd + d*64]; tDM[d] = tr[VT_DM*rowd + d*64];
-    tMDS[d] = tr[VT_MDS*rowd + d*64]; tMI[d] = tr[VT_MI*rowd + d*64]; tII[d] = tr[VT_II*rowd + d*64];
+    for (int d = 0; d < ND; d++) {
+      tBM[d] = tr[VT_BM*rowd + d*64]; tMM[d] = tr[VT_MM*rowd + d*64]; tIM[d] = tr[VT_IM*rowd + d*64]; tDM[d] = tr[VT_DM*rowd + d*64];
+      tMDS[d] = tr[VT_MDS*rowd + d*64]; tMI[d] = tr[VT_MI*rowd + d*64]; tII[d] = tr[VT_II*rowd + d*64];
+    }
   }
   DDConst<ND> ddc;
   dd_setup<ND>(ddc, tr + VT_DDS*rowd, z);
@@ -413,7 +447,9 @@ void k_vit(const DevProfile p, const uint8_t *__restrict__ res, const int64_t *_
 
     for (int i = 0; i < L; i++) {
       const int x = rd.at(i);
-      const int *e = lds + (size_t) x * rowd + z;
+      const int *e = tbl + (size_t) x * rowd + z;
+      const int *trr = tr + (PIN ? 0 : opaque_zero());
+#define VTR(A, T, d) (PIN ? A[d] : trr[(T) * rowd + (d) * 64])
       nbI = dppi<WAVE_SHR1>(nbI, I[ND-1]);
       nbD = dppi<WAVE_SHR1>(nbD, D[ND-1]);
       const int is0 = (int) __builtin_amdgcn_alignbit((unsigned) I[ND-1], (unsigned) nbI, 16);
@@ -423,13 +459,13 @@ void k_vit(const DevProfile p, const uint8_t *__restrict__ res, const int64_t *_
 #pragma unroll
       for (int d = ND-1; d >= 0; d--) {
         const int mp = d ? M[d-1] : ms0, ip = d ? I[d-1] : is0, dp = d ? D[d-1] : ds0;
-        int sv = adds2(xB2, tBM[d]);
-        sv = max2(sv, adds2(mp, tMM[d]));
-        sv = max2(sv, adds2(ip, tIM[d]));
-        sv = max2(sv, adds2(dp, tDM[d]));
+        int sv = adds2(xB2, VTR(tBM, VT_BM, d));
+        sv = max2(sv, adds2(mp, VTR(tMM, VT_MM, d)));
+        sv = max2(sv, adds2(ip, VTR(tIM, VT_IM, d)));
+        sv = max2(sv, adds2(dp, VTR(tDM, VT_DM, d)));
         sv = adds2(sv, e[d * 64]);
         xE = max2(xE, sv);
-        I[d] = max2(adds2(M[d], tMI[d]), adds2(I[d], tII[d]));
+        I[d] = max2(adds2(M[d], VTR(tMI, VT_MI, d)), adds2(I[d], VTR(tII, VT_II, d)));
         M[d] = sv;
       }
       // D(k) <- M(k-1) + tMD(k-1) on the new row; the shifted row is also the next row's M(i-1, k-1)
@@ -437,7 +473,8 @@ void k_vit(const DevProfile p, const uint8_t *__restrict__ res, const int64_t *_
       ms0 = (int) __builtin_amdgcn_alignbit((unsigned) M[ND-1], (unsigned) nbM, 16);
       int Dm = (int) NEG_PAIR;
 #pragma unroll
-      for (int d = 0; d < ND; d++) { D[d] = adds2(d ? M[d-1] : ms0, tMDS[d]); Dm = max2(Dm, D[d]); }
+      for (int d = 0; d < ND; d++) { D[d] = adds2(d ? M[d-1] : ms0, VTR(tMDS, VT_MDS, d)); Dm = max2(Dm, D[d]); }
+#undef VTR
       const int xEs = uni(wave_max_i(max(lo16(xE), hi16(xE))));
       if (xEs >= 32767) { status = 16; break; }
       xN = (int16_t) (xN + p.xwN_loop);
@@ -531,13 +568,15 @@ __device__ __forceinline__ void affine_fwd_scan(float (&D)[B], const float (&a)[
   for (int j = 0; j < B; j++) { cur = D[j] + a[j] * cur; D[j] = cur; }
 }
 
-template <int B>
-__device__ __forceinline__ void affine_bck_scan(float (&D)[B], const float (&a)[B], const AffConst &k, int z)
+// a(j) gives the D->D odds of the lane's position j (read twice per call, so
+// that large models need not hold them in registers through the scan).
+template <int B, typename A>
+__device__ __forceinline__ void affine_bck_scan(float (&D)[B], A a, const AffConst &k, int z)
 {
   // D(j) = c(j) + a(j) * D(j+1), carried across lanes in decreasing k
   float C = 0.0f;
 #pragma unroll
-  for (int j = B-1; j >= 0; j--) C = D[j] + a[j] * C;
+  for (int j = B-1; j >= 0; j--) C = D[j] + a(j) * C;
   C = C + k.w1 * dppf<ROW_SHL1>(0.0f, C);
   C = C + k.w2 * dppf<ROW_SHL2>(0.0f, C);
   C = C + k.w4 * dppf<ROW_SHL4>(0.0f, C);
@@ -553,28 +592,32 @@ __device__ __forceinline__ void affine_bck_scan(float (&D)[B], const float (&a)[
   const float cin = dppf<WAVE_SHL1>(0.0f, C);
   float cur = cin;
 #pragma unroll
-  for (int j = B-1; j >= 0; j--) { cur = D[j] + a[j] * cur; D[j] = cur; }
+  for (int j = B-1; j >= 0; j--) { cur = D[j] + a(j) * cur; D[j] = cur; }
 }
 
-template <int B, bool LDS>
-__global__ __launch_bounds__(512)
+template <int B, int TM>
+__global__ __launch_bounds__(FWD_THREADS(B))
 void k_fwd(const DevProfile p, const uint8_t *__restrict__ res, const int64_t *__restrict__ off,
            const int32_t *__restrict__ len, const int32_t *__restrict__ list, const int32_t *__restrict__ count,
            DevResults r, DevLists l, GateThresh g, int gating, LenCfg lc,
            float *__restrict__ pool, int64_t pool_size, int64_t *__restrict__ scale_off, float *__restrict__ xmx_out)
 {
+  constexpr bool PIN = (TM == TM_LDS || TM == TM_PIN);
   if ((int64_t) blockIdx.x * (blockDim.x >> 6) >= *count) return;
   extern __shared__ float ldsf[];
   const size_t rowf = (size_t) 64 * B;
-  const float *tbl = p.fb_tbl;
-  if (LDS) { stage_table(ldsf, p.fb_tbl, p.fb_floats); tbl = ldsf; }
+  const float *tbl = p.fb_tbl;                           // emission rows
+  if (TM == TM_LDS) { stage_table(ldsf, p.fb_tbl, p.fb_floats); tbl = ldsf; }
+  if (TM == TM_TLDS) stage_table(ldsf, p.fb_tbl + (size_t) p.nrows * rowf, (size_t) P7G_FB_NT * rowf);
   const int z = threadIdx.x & 63;
-  const float *tr = tbl + (size_t) p.nrows * rowf + z;
+  const float *tr = ((TM == TM_TLDS) ? ldsf : tbl + (size_t) p.nrows * rowf) + z;
   float tBM[B], tMM[B], tIM[B], tDM[B], tMDS[B], tMI[B], tII[B];
+  if (PIN) {
 #pragma unroll
-  for (int j = 0; j < B; j++) {
-    tBM[j] = tr[VT_BM*rowf + j*64]; tMM[j] = tr[VT_MM*rowf + j*64]; tIM[j] = tr[VT_IM*rowf + j*64]; tDM[j] = tr[VT_DM*rowf + j*64];
-    tMDS[j] = tr[VT_MDS*rowf + j*64]; tMI[j] = tr[VT_MI*rowf + j*64]; tII[j] = tr[VT_II*rowf + j*64];
+    for (int j = 0; j < B; j++) {
+      tBM[j] = tr[VT_BM*rowf + j*64]; tMM[j] = tr[VT_MM*rowf + j*64]; tIM[j] = tr[VT_IM*rowf + j*64]; tDM[j] = tr[VT_DM*rowf + j*64];
+      tMDS[j] = tr[VT_MDS*rowf + j*64]; tMI[j] = tr[VT_MI*rowf + j*64]; tII[j] = tr[VT_II*rowf + j*64];
+    }
   }
   AffConst akc;
   {
@@ -615,26 +658,29 @@ void k_fwd(const DevProfile p, const uint8_t *__restrict__ res, const int64_t *_
     for (int i = 1; i <= L; i++) {
       const int x = rd.at(i-1);
       const float *e = tbl + (size_t) x * rowf + z;
+      const float *trr = tr + (PIN ? 0 : opaque_zero());
+#define FTR(A, T, j) (PIN ? A[j] : trr[(T) * rowf + (j) * 64])
       const float iL = dppf<WAVE_SHR1>(0.0f, I[B-1]), dL = dppf<WAVE_SHR1>(0.0f, D[B-1]);
       float xEl = 0.0f;
 #pragma unroll
       for (int j = B-1; j >= 0; j--) {
         const float mp = j ? M[j-1] : mL, ip = j ? I[j-1] : iL, dp = j ? D[j-1] : dL;
-        const float In = M[j] * tMI[j] + I[j] * tII[j];
-        float sv = xB * tBM[j] + mp * tMM[j] + ip * tIM[j] + dp * tDM[j];
+        const float In = M[j] * FTR(tMI, VT_MI, j) + I[j] * FTR(tII, VT_II, j);
+        float sv = xB * FTR(tBM, VT_BM, j) + mp * FTR(tMM, VT_MM, j) + ip * FTR(tIM, VT_IM, j) + dp * FTR(tDM, VT_DM, j);
         sv = sv * e[j * 64];
         M[j] = sv; I[j] = In;
         xEl += sv;
       }
       const float mS = dppf<WAVE_SHR1>(0.0f, M[B-1]);
 #pragma unroll
-      for (int j = 0; j < B; j++) D[j] = (j ? M[j-1] : mS) * tMDS[j];
+      for (int j = 0; j < B; j++) D[j] = (j ? M[j-1] : mS) * FTR(tMDS, VT_MDS, j);
+#undef FTR
       {
         // the D->D odds are read per row, not pinned in registers: this keeps
         // the kernel at 128 VGPRs, four waves per SIMD
         float tDDS[B];
 #pragma unroll
-        for (int j = 0; j < B; j++) tDDS[j] = tr[VT_DDS*rowf + j*64];
+        for (int j = 0; j < B; j++) tDDS[j] = trr[VT_DDS*rowf + j*64];
         affine_fwd_scan<B>(D, tDDS, akc, z);
       }
 #pragma unroll
@@ -680,28 +726,38 @@ void k_fwd(const DevProfile p, const uint8_t *__restrict__ res, const int64_t *_
 /*****************************************************************
  * 8. Backward parser (one target per wave), with Forward's scale factors
  *****************************************************************/
-template <int B, bool LDS>
-__global__ __launch_bounds__(512)
+template <int B, int TM>
+__global__ __launch_bounds__(BCK_THREADS(B))
 void k_bck(const DevProfile p, const uint8_t *__restrict__ res, const int64_t *__restrict__ off,
            const int32_t *__restrict__ len, const int32_t *__restrict__ list, const int32_t *__restrict__ count,
            DevResults r, const float *__restrict__ pool, const int64_t *__restrict__ scale_off,
            const float *__restrict__ fwd_xmx, float *__restrict__ xmx_out, LenCfg lc)
 {
+  constexpr bool PIN = (TM == TM_LDS || TM == TM_PIN);
   if ((int64_t) blockIdx.x * (blockDim.x >> 6) >= *count) return;
   extern __shared__ float ldsf[];
   const size_t rowf = (size_t) 64 * B;
-  const float *tbl = p.fb_tbl;
-  if (LDS) { stage_table(ldsf, p.fb_tbl, p.fb_floats); tbl = ldsf; }
+  const float *tbl = p.fb_tbl;                           // emission rows
+  if (TM == TM_LDS) { stage_table(ldsf, p.fb_tbl, p.fb_floats); tbl = ldsf; }
+  if (TM == TM_TLDS) stage_table(ldsf, p.fb_tbl + (size_t) p.nrows * rowf, (size_t) P7G_FB_NT * rowf);
   const int z = threadIdx.x & 63;
-  const float *tr = tbl + (size_t) p.nrows * rowf + z;
+  const float *tr = ((TM == TM_TLDS) ? ldsf : tbl + (size_t) p.nrows * rowf) + z;
   float tBM[B], tMMN[B], tIMN[B], tDMN[B], tMD[B], tDD[B], tMI[B], tII[B];
+  if (PIN) {
 #pragma unroll
-  for (int j = 0; j < B; j++) {
-    tBM[j] = tr[VT_BM*rowf + j*64]; tMMN[j] = tr[FT_MMN*rowf + j*64]; tIMN[j] = tr[FT_IMN*rowf + j*64]; tDMN[j] = tr[FT_DMN*rowf + j*64];
-    tMD[j] = tr[FT_MD*rowf + j*64]; tDD[j] = tr[FT_DD*rowf + j*64]; tMI[j] = tr[VT_MI*rowf + j*64]; tII[j] = tr[VT_II*rowf + j*64];
+    for (int j = 0; j < B; j++) {
+      tBM[j] = tr[VT_BM*rowf + j*64]; tMMN[j] = tr[FT_MMN*rowf + j*64]; tIMN[j] = tr[FT_IMN*rowf + j*64]; tDMN[j] = tr[FT_DMN*rowf + j*64];
+      tMD[j] = tr[FT_MD*rowf + j*64]; tDD[j] = tr[FT_DD*rowf + j*64]; tMI[j] = tr[VT_MI*rowf + j*64]; tII[j] = tr[VT_II*rowf + j*64];
+    }
   }
   AffConst akc;
-  aff_bck_setup<B>(akc, tDD);
+  {
+    float tdd[B];
+#pragma unroll
+    for (int j = 0; j < B; j++) tdd[j] = tr[FT_DD*rowf + j*64];
+    aff_bck_setup<B>(akc, tdd);
+  }
+#define BTR(A, T, j) (PIN ? A[j] : trr[(T) * rowf + (j) * 64])
   const int64_t n = *count;
 
   int32_t *queue = (int32_t *) count + P7G_QOFF;
@@ -724,11 +780,12 @@ void k_bck(const DevProfile p, const uint8_t *__restrict__ res, const int64_t *_
     float xE = xC * p.xfE_move;
 #pragma unroll
     for (int j = 0; j < B; j++) { M[j] = xE; D[j] = xE; I[j] = 0.0f; }
-    affine_bck_scan<B>(D, tDD, akc, z);
     {
+      const float *trr = tr + (PIN ? 0 : opaque_zero());
+      affine_bck_scan<B>(D, [&](int j) { return BTR(tDD, FT_DD, j); }, akc, z);
       const float dS = wave_shl1f(D[0]);
 #pragma unroll
-      for (int j = 0; j < B; j++) M[j] = M[j] + (j < B-1 ? D[j+1] : dS) * tMD[j];
+      for (int j = 0; j < B; j++) M[j] = M[j] + (j < B-1 ? D[j+1] : dS) * BTR(tMD, FT_MD, j);
     }
     int own = 0;
     float totscale;
@@ -751,18 +808,21 @@ void k_bck(const DevProfile p, const uint8_t *__restrict__ res, const int64_t *_
     for (int i = L-1; i >= 1; i--) {
       const int x = rd.at(i);                         // residue i+1 (1-based)
       const float *e = tbl + (size_t) x * rowf + z;
-      float mn[B];
+      const float *trr = tr + (PIN ? 0 : opaque_zero());
+      // mn(j) = M(j) * e(j), formed one position ahead of the update that
+      // overwrites M(j); xBl sums mn(j) * tBM(j) in increasing j
+      float mn = M[0] * e[0];
       float xBl = 0.0f;
-#pragma unroll
-      for (int j = 0; j < B; j++) { mn[j] = M[j] * e[j * 64]; xBl += mn[j] * tBM[j]; }
-      const float mS = wave_shl1f(mn[0]);
+      xBl += mn * BTR(tBM, VT_BM, 0);
+      const float mS = wave_shl1f(mn);
 #pragma unroll
       for (int j = 0; j < B; j++) {
-        const float ms = (j < B-1) ? mn[j+1] : mS;
+        float ms = mS;
+        if (j < B-1) { ms = M[j+1] * e[(j+1) * 64]; xBl += ms * BTR(tBM, VT_BM, j+1); }
         const float ip = I[j];
-        I[j] = ip * tII[j] + ms * tIMN[j];
-        D[j] = ms * tDMN[j];
-        M[j] = ip * tMI[j] + ms * tMMN[j];
+        I[j] = ip * BTR(tII, VT_II, j) + ms * BTR(tIMN, FT_IMN, j);
+        D[j] = ms * BTR(tDMN, FT_DMN, j);
+        M[j] = ip * BTR(tMI, VT_MI, j) + ms * BTR(tMMN, FT_MMN, j);
       }
       xB = unif(wave_sum_f(xBl));
       xC = xC * floop;
@@ -771,10 +831,10 @@ void k_bck(const DevProfile p, const uint8_t *__restrict__ res, const int64_t *_
       xE = (xC * p.xfE_move) + (xJ * p.xfE_loop);
 #pragma unroll
       for (int j = 0; j < B; j++) D[j] = D[j] + xE;
-      affine_bck_scan<B>(D, tDD, akc, z);
+      affine_bck_scan<B>(D, [&](int j) { return BTR(tDD, FT_DD, j); }, akc, z);
       const float dS = wave_shl1f(D[0]);
 #pragma unroll
-      for (int j = 0; j < B; j++) M[j] = M[j] + xE + (j < B-1 ? D[j+1] : dS) * tMD[j];
+      for (int j = 0; j < B; j++) M[j] = M[j] + xE + (j < B-1 ? D[j+1] : dS) * BTR(tMD, FT_MD, j);
       if (xB > 1.0e16f) own = 1;
       const float s = own ? ((xB > 1.0e4f) ? xB : 1.0f) : fsc[(size_t) i * fstride];
       if (s > 1.0f) {
@@ -792,9 +852,10 @@ void k_bck(const DevProfile p, const uint8_t *__restrict__ res, const int64_t *_
     {
       const int x = (L > 0) ? rd.at(0) : P7G_PADCODE;
       const float *e = tbl + (size_t) x * rowf + z;
+      const float *trr = tr + (PIN ? 0 : opaque_zero());
       float xBl = 0.0f;
 #pragma unroll
-      for (int j = 0; j < B; j++) xBl += (M[j] * e[j * 64]) * tBM[j];
+      for (int j = 0; j < B; j++) xBl += (M[j] * e[j * 64]) * BTR(tBM, VT_BM, j);
       xB = unif(wave_sum_f(xBl));
       xN = (xB * fmove) + (xN * floop);
     }
@@ -811,6 +872,7 @@ void k_bck(const DevProfile p, const uint8_t *__restrict__ res, const int64_t *_
     }
   }
 }
+#undef BTR
 
 __global__ void k_fill32(uint32_t *__restrict__ p, uint32_t v, int64_t n)
 {
@@ -851,18 +913,19 @@ hipError_t launch_msv_classify(const DevProfile &p, const DevSeqs &s, const int3
   return hipGetLastError();
 }
 
-#define MF_CASE(N) case N: k = (void *) k_msv_full<N>; break;
+#define MF_CASE(N, LDS) case N: k = (void *) k_msv_full<N, LDS>; break;
 hipError_t launch_msv_full(const DevProfile &p, const DevSeqs &s, const int32_t *list, const int32_t *count,
                            int64_t maxn, int16_t *xj, LenCfg lc, hipStream_t st)
 {
   if (maxn <= 0) return hipSuccess;
   void *k = nullptr;
   switch (p.mf_ND) {
-    MF_CASE(1) MF_CASE(2) MF_CASE(3) MF_CASE(4) MF_CASE(5) MF_CASE(6) MF_CASE(8) MF_CASE(10) MF_CASE(12)
-    MF_CASE(16) MF_CASE(20) MF_CASE(24) MF_CASE(32) MF_CASE(40)
+    MF_CASE(1, true) MF_CASE(2, true) MF_CASE(3, true) MF_CASE(4, true) MF_CASE(5, true) MF_CASE(6, true)
+    MF_CASE(8, true) MF_CASE(10, true) MF_CASE(12, true) MF_CASE(16, true) MF_CASE(20, true)
+    MF_CASE(24, false) MF_CASE(32, false) MF_CASE(40, false)
     default: return hipErrorInvalidValue;
   }
-  const size_t lds = p.mf_dwords * 4;
+  const size_t lds = (p.mf_ND <= 20) ? p.mf_dwords * 4 : 0;
   const unsigned grid = grid_for_waves_blk(k, maxn, lds, LIST_BLOCK / 64);
   int tbl_dwords = (int) p.mf_dwords;
   void *args[] = { (void *) &p, (void *) &p.mf_tbl, &tbl_dwords, (void *) &s.res, (void *) &s.off, (void *) &s.len,
@@ -899,65 +962,88 @@ hipError_t launch_bias(const DevProfile &p, const DevSeqs &s, const int32_t *lis
   return launch_fill32(l.cursor, 0u, 2, st);
 }
 
-#define VIT_CASE(N) case N: k = (void *) k_vit<N>; break;
+// Viterbi: the whole table in LDS while it fits (ND <= 16 at 160 KB), else the
+// transitions in LDS and emission rows from global memory.
+#define VIT_CASE(N, T) case N: k = (void *) k_vit<N, T>; lds = (T == TM_LDS) ? p.vf_dwords * 4 : (size_t) P7G_VF_NT * 64 * N * 4; \
+  nthr = VIT_THREADS(N); break;
 hipError_t launch_vit(const DevProfile &p, const DevSeqs &s, const int32_t *list, const int32_t *count, int64_t maxn,
                       DevResults r, DevLists l, GateThresh g, int gating, LenCfg lc, hipStream_t st)
 {
   if (maxn <= 0) return hipSuccess;
   void *k = nullptr;
+  size_t lds = 0;
+  int nthr = LIST_BLOCK;
   switch (p.vf_ND) {
-    VIT_CASE(1) VIT_CASE(2) VIT_CASE(3) VIT_CASE(4) VIT_CASE(5) VIT_CASE(6) VIT_CASE(8) VIT_CASE(10) VIT_CASE(12)
-    VIT_CASE(16) VIT_CASE(20) VIT_CASE(24) VIT_CASE(32) VIT_CASE(40)
+    VIT_CASE(1, TM_LDS) VIT_CASE(2, TM_LDS) VIT_CASE(3, TM_LDS) VIT_CASE(4, TM_LDS) VIT_CASE(5, TM_LDS)
+    VIT_CASE(6, TM_LDS) VIT_CASE(8, TM_LDS) VIT_CASE(10, TM_LDS) VIT_CASE(12, TM_LDS) VIT_CASE(16, TM_LDS)
+    VIT_CASE(20, TM_TLDS) VIT_CASE(24, TM_TLDS) VIT_CASE(32, TM_TLDS) VIT_CASE(40, TM_TLDS)
     default: return hipErrorInvalidValue;
   }
-  const size_t lds = p.vf_dwords * 4;
-  const unsigned grid = grid_for_waves_blk(k, maxn, lds, LIST_BLOCK / 64);
+  const unsigned grid = grid_for_waves_blk(k, maxn, lds, nthr / 64);
   void *args[] = { (void *) &p, (void *) &s.res, (void *) &s.off, (void *) &s.len, (void *) &list, (void *) &count,
                    &r, &l, &g, &gating, &lc };
-  return launch(k, grid, LIST_BLOCK, args, lds, st);
+  return launch(k, grid, nthr, args, lds, st);
 }
 
+// Forward/Backward: the whole table in LDS up to B = 12 (150 KB); B = 14 reads
+// the table from global memory with transitions pinned in registers;
+// B = 16..48 keep the transitions in LDS (at most 160 KB) and read them per
+// row; B = 64, 80 read everything from global memory per row.
 #define FB_LDS_LIMIT (150 * 1024)
-#define FWD_CASE(N) case N: k = lds_ok ? (void *) k_fwd<N, true> : (void *) k_fwd<N, false>; break;
+static int fb_mode(const DevProfile &p)
+{
+  if (p.fb_floats * 4 <= FB_LDS_LIMIT && p.fb_floats * 4 <= (size_t) lds_block_limit()) return TM_LDS;
+  if (p.fb_B <= 14) return TM_PIN;
+  if ((size_t) P7G_FB_NT * 64 * p.fb_B * 4 <= (size_t) lds_block_limit()) return TM_TLDS;
+  return TM_GLOB;
+}
+static size_t fb_lds(const DevProfile &p, int tm)
+{
+  return (tm == TM_LDS) ? p.fb_floats * 4 : (tm == TM_TLDS) ? (size_t) P7G_FB_NT * 64 * p.fb_B * 4 : 0;
+}
+#define FB_CASES(X) \
+  X(1, TM_LDS) X(2, TM_LDS) X(3, TM_LDS) X(4, TM_LDS) X(5, TM_LDS) X(6, TM_LDS) X(7, TM_LDS) X(8, TM_LDS) \
+  X(10, TM_LDS) X(12, TM_LDS) X(14, TM_PIN) X(16, TM_TLDS) X(20, TM_TLDS) X(24, TM_TLDS) X(28, TM_TLDS) \
+  X(32, TM_TLDS) X(40, TM_TLDS) X(48, TM_TLDS) X(64, TM_GLOB) X(80, TM_GLOB)
+#define FWD_CASE(N, T) case N * 4 + T: k = (void *) k_fwd<N, T>; nthr = FWD_THREADS(N); break;
+#define BCK_CASE(N, T) case N * 4 + T: k = (void *) k_bck<N, T>; nthr = BCK_THREADS(N); break;
+
 hipError_t launch_fwd(const DevProfile &p, const DevSeqs &s, const int32_t *list, const int32_t *count, int64_t maxn,
                       DevResults r, DevLists l, GateThresh g, int gating, LenCfg lc,
                       float *scale_pool, int64_t pool_size, int64_t *scale_off, float *xmx_out, hipStream_t st)
 {
   if (maxn <= 0) return hipSuccess;
-  const bool lds_ok = p.fb_floats * 4 <= FB_LDS_LIMIT && p.fb_floats * 4 <= (size_t) lds_block_limit();
+  const int tm = fb_mode(p);
   void *k = nullptr;
-  switch (p.fb_B) {
-    FWD_CASE(1) FWD_CASE(2) FWD_CASE(3) FWD_CASE(4) FWD_CASE(5) FWD_CASE(6) FWD_CASE(7) FWD_CASE(8)
-    FWD_CASE(10) FWD_CASE(12) FWD_CASE(14) FWD_CASE(16) FWD_CASE(20) FWD_CASE(24) FWD_CASE(28) FWD_CASE(32)
-    FWD_CASE(40) FWD_CASE(48) FWD_CASE(64) FWD_CASE(80)
+  int nthr = LIST_BLOCK;
+  switch (p.fb_B * 4 + tm) {
+    FB_CASES(FWD_CASE)
     default: return hipErrorInvalidValue;
   }
-  const size_t lds = lds_ok ? p.fb_floats * 4 : 0;
-  const unsigned grid = grid_for_waves_blk(k, maxn, lds, LIST_BLOCK / 64);
+  const size_t lds = fb_lds(p, tm);
+  const unsigned grid = grid_for_waves_blk(k, maxn, lds, nthr / 64);
   void *args[] = { (void *) &p, (void *) &s.res, (void *) &s.off, (void *) &s.len, (void *) &list, (void *) &count,
                    &r, &l, &g, &gating, &lc, &scale_pool, &pool_size, &scale_off, &xmx_out };
-  return launch(k, grid, LIST_BLOCK, args, lds, st);
+  return launch(k, grid, nthr, args, lds, st);
 }
 
-#define BCK_CASE(N) case N: k = lds_ok ? (void *) k_bck<N, true> : (void *) k_bck<N, false>; break;
 hipError_t launch_bck(const DevProfile &p, const DevSeqs &s, const int32_t *list, const int32_t *count, int64_t maxn,
                       DevResults r, const float *scale_pool, const int64_t *scale_off, const float *fwd_xmx,
                       float *xmx_out, LenCfg lc, hipStream_t st)
 {
   if (maxn <= 0) return hipSuccess;
-  const bool lds_ok = p.fb_floats * 4 <= FB_LDS_LIMIT && p.fb_floats * 4 <= (size_t) lds_block_limit();
+  const int tm = fb_mode(p);
   void *k = nullptr;
-  switch (p.fb_B) {
-    BCK_CASE(1) BCK_CASE(2) BCK_CASE(3) BCK_CASE(4) BCK_CASE(5) BCK_CASE(6) BCK_CASE(7) BCK_CASE(8)
-    BCK_CASE(10) BCK_CASE(12) BCK_CASE(14) BCK_CASE(16) BCK_CASE(20) BCK_CASE(24) BCK_CASE(28) BCK_CASE(32)
-    BCK_CASE(40) BCK_CASE(48) BCK_CASE(64) BCK_CASE(80)
+  int nthr = LIST_BLOCK;
+  switch (p.fb_B * 4 + tm) {
+    FB_CASES(BCK_CASE)
     default: return hipErrorInvalidValue;
   }
-  const size_t lds = lds_ok ? p.fb_floats * 4 : 0;
-  const unsigned grid = grid_for_waves_blk(k, maxn, lds, LIST_BLOCK / 64);
+  const size_t lds = fb_lds(p, tm);
+  const unsigned grid = grid_for_waves_blk(k, maxn, lds, nthr / 64);
   void *args[] = { (void *) &p, (void *) &s.res, (void *) &s.off, (void *) &s.len, (void *) &list, (void *) &count,
                    &r, (void *) &scale_pool, (void *) &scale_off, (void *) &fwd_xmx, &xmx_out, &lc };
-  return launch(k, grid, LIST_BLOCK, args, lds, st);
+  return launch(k, grid, nthr, args, lds, st);
 }
 
 /*****************************************************************

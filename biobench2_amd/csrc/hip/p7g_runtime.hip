@@ -42,6 +42,8 @@ struct Work {
   unsigned long long *stats = nullptr;
   float      *pool = nullptr;
   int64_t     pool_size = 0;
+  uint16_t   *bnd = nullptr;        // boundary column of the chunked ungapped pass
+  int64_t     bnd_cap = 0;          // its length in entries (residue bytes)
   void       *mem = nullptr;
   hipEvent_t  ev[8];
   bool        ev_ready = false;
@@ -115,6 +117,7 @@ static const int vf_nd_set[] = { 1, 2, 3, 4, 5, 6, 8, 10, 12, 16, 20, 24, 32, 40
 static const int fb_b_set[]  = { 1, 2, 3, 4, 5, 6, 7, 8, 10, 12, 14, 16, 20, 24, 28, 32, 40, 48, 64, 80 };
 static const int ssv32_set[] = { 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15, 16, 18, 20, 22, 24, 26, 28, 30, 32 };
 static const int ssv64_set[] = { 8, 9, 10, 11, 12, 13, 14, 15, 16, 18, 20, 22, 24, 26, 28, 30, 32, 36, 40 };
+static const int ssvc_set[]  = { 11, 12, 13, 14, 15, 16, 18 };   // chunk shapes (launch_ssv_chunked)
 
 static int pick(const int *set, int nset, int need)
 {
@@ -151,6 +154,7 @@ dev_profile_build(P7_GPU_OPROFILE *om, int Lmax_need, struct p7g_dev_profile_s *
     if (G == 16)      ND = (M <= 1024) ? std::max(1, cdiv(M, 32)) : -1;
     else if (G == 32) ND = (M <= 2048) ? pick(ssv32_set, (int) (sizeof(ssv32_set)/sizeof(int)), cdiv(M, 64)) : -1;
     else              ND = pick(ssv64_set, (int) (sizeof(ssv64_set)/sizeof(int)), cdiv(M, 128));
+    if (ND > 0 && (size_t) NR * G * ND * 4 + 256 > P7G_LDS_MAX) ND = -1;   // the table (and a word) must fit in LDS
     if (ND < 0) { sv.G = 0; sv.ND = 0; sv.dwords = 0; continue; }
     sv.G = G; sv.ND = ND;
     sv.dwords = (size_t) NR * G * ND;
@@ -166,8 +170,32 @@ dev_profile_build(P7_GPU_OPROFILE *om, int Lmax_need, struct p7g_dev_profile_s *
           else             ssv[v][tail0 + ((size_t) x * NT + (dd - 4 * NQ)) * G + z] = val;
         }
   }
-  if (p.ssv[2].G == 0) { delete d; p7g_set_error("model too long for the device MSV stage (M=%d)", M); return p7G_EINVAL; }
-  p.ssv_def = (p.ssv[0].G != 0) ? 0 : 2;            // 16 lanes up to M = 1024, else 64
+  // longer models: two or three chunks of 64 lanes, each table in LDS (profile-chunk tiling)
+  std::vector<int> ssvc[P7G_SSV_MAXCHUNK];
+  p.ssv_nchunk = 0;
+  if (p.ssv[2].G == 0) {
+    const int nset = (int) (sizeof(ssvc_set)/sizeof(int));
+    int nc = 2, ND = pick(ssvc_set, nset, cdiv(M, 2 * 128));
+    if (ND < 0) { nc = 3; ND = pick(ssvc_set, nset, cdiv(M, 3 * 128)); }
+    if (ND < 0) { delete d; p7g_set_error("model too long for the device MSV stage (M=%d)", M); return p7G_EINVAL; }
+    p.ssv_nchunk = nc;
+    for (int c = 0; c < nc; c++) {
+      SsvVariant &sv = p.ssvc[c];
+      const int G = 64, B = 2 * ND, NQ = ND / 4, NT = ND % 4, k0 = c * G * B;
+      sv.G = G; sv.ND = ND; sv.dwords = (size_t) NR * G * ND;
+      ssvc[c].assign(sv.dwords, (int) 0x80008000u);
+      const size_t tail0 = (size_t) NR * NQ * G * 4;
+      for (int x = 0; x < NR; x++)
+        for (int z = 0; z < G; z++)
+          for (int dd = 0; dd < ND; dd++) {
+            const int val = pack2(rsv(x, k0 + z*B + dd + 1), rsv(x, k0 + z*B + dd + ND + 1));
+            if (dd < 4 * NQ) ssvc[c][(((size_t) x * NQ + dd / 4) * G + z) * 4 + dd % 4] = val;
+            else             ssvc[c][tail0 + ((size_t) x * NT + (dd - 4 * NQ)) * G + z] = val;
+          }
+    }
+  }
+  // 16 lanes up to M = 1024, else 64; chunked beyond one LDS table
+  p.ssv_def = (p.ssv[0].G != 0) ? 0 : (p.ssv[2].G != 0) ? 2 : -1;
   p.mf_ND = pick(vf_nd_set, (int) (sizeof(vf_nd_set)/sizeof(int)), std::max(1, cdiv(cdiv(M, 64), 2)));
   if (p.mf_ND < 0) { delete d; p7g_set_error("model too long for the device MSV stage (M=%d)", M); return p7G_EINVAL; }
   p.mf_dwords = (size_t) NR * 64 * p.mf_ND;
@@ -279,11 +307,14 @@ dev_profile_build(P7_GPU_OPROFILE *om, int Lmax_need, struct p7g_dev_profile_s *
   // ---- one allocation ----
   const size_t L1 = (size_t) Lmax + 1;
   size_t sz[] = { p.ssv[0].dwords * 4, p.ssv[1].dwords * 4, p.ssv[2].dwords * 4, p.mf_dwords * 4, p.vf_dwords * 4,
-                  p.fb_floats * 4, L1, L1 * 2, L1 * 4, L1 * 4, L1 * 4, L1 * 4, L1 * 4, eo.size() * 4 };
+                  p.fb_floats * 4, L1, L1 * 2, L1 * 4, L1 * 4, L1 * 4, L1 * 4, L1 * 4, eo.size() * 4,
+                  p.ssv_nchunk > 0 ? p.ssvc[0].dwords * 4 : 0, p.ssv_nchunk > 1 ? p.ssvc[1].dwords * 4 : 0,
+                  p.ssv_nchunk > 2 ? p.ssvc[2].dwords * 4 : 0 };
   const void *src[] = { ssv[0].data(), ssv[1].data(), ssv[2].data(), mf.data(), vf.data(), fb.data(), lc.tjb_b,
-                        lc.xw_move, lc.xf_move, lc.xf_loop, lc.nullsc, lc.flen_a, lc.flen_b, eo.data() };
+                        lc.xw_move, lc.xf_move, lc.xf_loop, lc.nullsc, lc.flen_a, lc.flen_b, eo.data(),
+                        ssvc[0].data(), ssvc[1].data(), ssvc[2].data() };
   const int NB = (int) (sizeof(sz) / sizeof(sz[0]));
-  size_t offs[16], total = 0;
+  size_t offs[20], total = 0;
   for (int i = 0; i < NB; i++) { offs[i] = total; total += (sz[i] + 255) & ~(size_t) 255; }
   char *blob = nullptr;
   hipError_t e = hipMalloc((void **) &blob, total);
@@ -307,6 +338,7 @@ dev_profile_build(P7_GPU_OPROFILE *om, int Lmax_need, struct p7g_dev_profile_s *
   p.xf_move = (float *) (blob + offs[8]); p.xf_loop = (float *) (blob + offs[9]);
   p.nullsc = (float *) (blob + offs[10]); p.flen_a = (float *) (blob + offs[11]); p.flen_b = (float *) (blob + offs[12]);
   p.eo = (float *) (blob + offs[13]);
+  for (int c = 0; c < P7G_SSV_MAXCHUNK; c++) p.ssvc[c].tbl = (c < p.ssv_nchunk) ? (int *) (blob + offs[14 + c]) : nullptr;
   d->Lmax = Lmax;
   d->serial = ++g_serial;
   *ret = d;
@@ -632,10 +664,12 @@ struct TuneEntry { int v, grab; float us[P7G_SSV_NVAR]; };
 static std::mutex g_tunelock;
 static std::map<int, TuneEntry> g_tune;
 
-// The variant and queue grab the ungapped pass of this profile runs with.
+// The variant and queue grab the ungapped pass of this profile runs with
+// (v = -1: the chunked pass).
 static void ssv_choice(const DevProfile &p, int *v, int *grab)
 {
   *v = p.ssv_def; *grab = SSV_GRAB_DEFAULT;
+  if (p.ssv_nchunk) return;
   std::lock_guard<std::mutex> lk(g_tunelock);
   auto it = g_tune.find(tune_class(p.M));
   if (it != g_tune.end() && it->second.v >= 0 && it->second.v < P7G_SSV_NVAR && p.ssv[it->second.v].G) {
@@ -643,11 +677,15 @@ static void ssv_choice(const DevProfile &p, int *v, int *grab)
   }
 }
 
+// The ungapped pass in the chosen shape; a chunked model takes one queue
+// cursor per chunk from queue[] and the boundary columns of a work area
+// (work_bnd, bnd_cap entries each).
 static hipError_t launch_ssv_chosen(const DevProfile &p, const DevSeqs &s, const int32_t *slots, int64_t n,
-                                    uint16_t *zmax, int32_t *queue, hipStream_t st)
+                                    uint16_t *zmax, int32_t *queue, uint16_t *bnd, int64_t bnd_cap, hipStream_t st)
 {
   int v, grab;
   ssv_choice(p, &v, &grab);
+  if (v < 0) return launch_ssv_chunked(p, s, slots, n, zmax, queue, grab, bnd, bnd_cap, st);
   return launch_ssv(p, v, s, slots, n, zmax, queue, grab, st);
 }
 
@@ -707,12 +745,27 @@ reset_counts(Work &w)
   return p7G_OK;
 }
 
+// The boundary columns of the chunked ungapped pass over a residue array of
+// `bytes` bytes (two columns of one uint16 per byte), made when a chunked
+// model first runs.
+static int
+work_bnd(Work &w, const DevProfile &p, int64_t bytes)
+{
+  if (! p.ssv_nchunk || w.bnd_cap >= bytes) return p7G_OK;
+  if (w.bnd) { HIPCHK(hipStreamSynchronize(w.st)); (void) hipFree(w.bnd); w.bnd = nullptr; w.bnd_cap = 0; }
+  HIPCHK(hipMalloc((void **) &w.bnd, 2 * 2 * (size_t) bytes));
+  w.bnd_cap = bytes;
+  return p7G_OK;
+}
+
 static void
 work_release(Work &w)
 {
   if (w.st) (void) hipStreamSynchronize(w.st);
   if (w.mem) (void) hipFree(w.mem);
   if (w.pool) (void) hipFree(w.pool);
+  if (w.bnd) (void) hipFree(w.bnd);
+  w.bnd = nullptr; w.bnd_cap = 0;
   if (w.ev_ready) for (int i = 0; i < 8; i++) (void) hipEventDestroy(w.ev[i]);
   w.mem = nullptr; w.pool = nullptr; w.cap = 0; w.pool_size = 0; w.ev_ready = false;
 }
@@ -763,7 +816,8 @@ p7_gpu_MSVFilterBatch(const P7_GPU_OPROFILE *om, const P7_GPU_SEQDB *db, const i
   LenCfg lc = lencfg_per_target();
   if ((status = reset_counts(g_work)) != p7G_OK) return status;
   HIPCHK(hipMemcpyAsync(w.slots, slots.data(), sizeof(int32_t) * (size_t) n, hipMemcpyHostToDevice, g_stream));
-  HIPCHK(launch_ssv_chosen(*p, s, w.slots, n, w.r.zmax, &w.l.counts[P7G_Q_SSV], g_stream));
+  if ((status = work_bnd(w, *p, db->bytes)) != p7G_OK) return status;
+  HIPCHK(launch_ssv_chosen(*p, s, w.slots, n, w.r.zmax, &w.l.counts[P7G_Q_SSV], w.bnd, w.bnd_cap, g_stream));
   HIPCHK(launch_msv_classify(*p, s, w.slots, n, w.r.zmax, w.r.xj, w.l.full, &w.l.counts[0], lc, g_stream));
   HIPCHK(launch_msv_full(*p, s, w.l.full, &w.l.counts[0], n, w.r.xj, lc, g_stream));
   std::vector<int16_t> xj((size_t) db->nseq);
@@ -1054,7 +1108,8 @@ p7_gpu_Pipeline(P7_GPU_PIPELINE *pli, const P7_GPU_OPROFILE *om, const P7_GPU_SE
 
   if ((status = chain_reset(w, n)) != p7G_OK) return status;
   HIPCHK(hipEventRecord(w.ev[0], w.st));
-  HIPCHK(launch_ssv_chosen(*p, s, nullptr, n, w.r.zmax, &w.l.counts[P7G_Q_SSV], w.st));
+  if ((status = work_bnd(w, *p, db->bytes)) != p7G_OK) return status;
+  HIPCHK(launch_ssv_chosen(*p, s, nullptr, n, w.r.zmax, &w.l.counts[P7G_Q_SSV], w.bnd, w.bnd_cap, w.st));
   HIPCHK(hipEventRecord(w.ev[6], w.st));
   if ((status = chain_stages(w, *p, s, n, w.r.zmax, g, true)) != p7G_OK) return status;
 
@@ -1107,7 +1162,7 @@ struct Batch {
   int64_t  n_cap = 0;
   unsigned mask_cap = 0;
   uint16_t *zmax = nullptr;              // [row][n] ungapped-pass results, rows in shape order
-  int32_t  *queues = nullptr;            // [row] ungapped-pass work-queue cursors
+  int32_t  *queues = nullptr;            // [row] ungapped-pass work-queue cursors, then one per chunk of each row
   unsigned long long *sums = nullptr;    // [profile][8] launch_stats sums
   int32_t  *counts = nullptr;            // [profile][16] list counts at the end of the run
   uint8_t  *o_stage = nullptr;           // [profile][n] outputs in the caller's order, as asked for
@@ -1177,7 +1232,8 @@ batch_reserve(Batch &b, int nprof, int64_t n, unsigned mask)
   mask |= b.mask_cap;
   size_t o = 0;
   auto take = [&](size_t bytes) { size_t at = o; o += (bytes + 255) & ~(size_t) 255; return at; };
-  const size_t ozm = take(2 * (size_t) (np * nn)), oq = take(4 * (size_t) np), osu = take(64 * (size_t) np),
+  const size_t ozm = take(2 * (size_t) (np * nn)), oq = take(4 * (1 + P7G_SSV_MAXCHUNK) * (size_t) np),
+               osu = take(64 * (size_t) np),
                oco = take(64 * (size_t) np), ost = (mask & 1u) ? take((size_t) (np * nn)) : 0;
   size_t of[6];
   for (int a = 0; a < 6; a++) of[a] = (mask & (2u << a)) ? take(4 * (size_t) (np * nn)) : 0;
@@ -1197,13 +1253,20 @@ batch_reserve(Batch &b, int nprof, int64_t n, unsigned mask)
 // which waits only for its own shape's launch; then st0 joins every chain.
 static int
 batch_enqueue(Batch &b, hipStream_t st0, int nchain, const std::vector<BatchItem> &it, int nshape,
-              const DevSeqs &s, int64_t n, unsigned mask)
+              const DevSeqs &s, int64_t n, unsigned mask, uint16_t *bnd, int64_t bnd_stride)
 {
   const int nit = (int) it.size();
-  HIPCHK(launch_fill32(b.queues, 0u, nit, st0));
+  HIPCHK(launch_fill32(b.queues, 0u, (1 + P7G_SSV_MAXCHUNK) * (int64_t) nit, st0));
   for (int r0 = 0; r0 < nit; ) {
     int r1 = r0;
     while (r1 < nit && it[r1].shape == it[r0].shape) r1++;
+    if (it[r0].v < 0) {                  // a chunked model: its own two launches (one row per shape)
+      HIPCHK(launch_ssv_chunked(*it[r0].p, s, nullptr, n, b.zmax + (size_t) r0 * n,
+                                b.queues + nit + P7G_SSV_MAXCHUNK * r0, it[r0].grab, bnd, bnd_stride, st0));
+      HIPCHK(hipEventRecord(b.evs[it[r0].shape], st0));
+      r0 = r1;
+      continue;
+    }
     std::vector<const int *> tbl;
     for (int r = r0; r < r1; r++) tbl.push_back(it[r].p->ssv[it[r].v].tbl);
     const SsvVariant &sv = it[r0].p->ssv[it[r0].v];
@@ -1259,23 +1322,30 @@ p7_gpu_PipelineBatch(P7_GPU_PIPELINE *pli, const P7_GPU_OPROFILE *const *oms, in
     ssv_choice(*x.p, &x.v, &x.grab);
     x.g = gates_of(pli, oms[j]);
   }
-  // rows grouped by shape (G, ND, grab), longest shapes first; input order within a shape
-  std::stable_sort(it.begin(), it.end(), [](const BatchItem &a, const BatchItem &c) {
-    const SsvVariant &x = a.p->ssv[a.v], &y = c.p->ssv[c.v];
-    const long kx = (long) x.G * x.ND, ky = (long) y.G * y.ND;
-    if (kx != ky) return kx > ky;
-    if (x.G != y.G) return x.G > y.G;
+  // rows grouped by shape (G, ND, grab), longest shapes first, chunked models
+  // (one shape each) before all; input order within a shape
+  auto area = [](const BatchItem &a) -> long {
+    return a.v < 0 ? (long) 1 << 40 : (long) a.p->ssv[a.v].G * a.p->ssv[a.v].ND;
+  };
+  auto lanes = [](const BatchItem &a) -> int { return a.v < 0 ? 64 : a.p->ssv[a.v].G; };
+  std::stable_sort(it.begin(), it.end(), [&](const BatchItem &a, const BatchItem &c) {
+    if (area(a) != area(c)) return area(a) > area(c);
+    if (lanes(a) != lanes(c)) return lanes(a) > lanes(c);
     return a.grab > c.grab;
   });
   int nshape = 0;
   for (int r = 0; r < nprof; r++) {
     if (r > 0) {
-      const SsvVariant &x = it[r].p->ssv[it[r].v], &y = it[r-1].p->ssv[it[r-1].v];
-      if (x.G != y.G || x.ND != y.ND || it[r].grab != it[r-1].grab) nshape++;
+      const bool same_shape = it[r].v >= 0 && it[r-1].v >= 0 &&
+        it[r].p->ssv[it[r].v].G == it[r-1].p->ssv[it[r-1].v].G &&
+        it[r].p->ssv[it[r].v].ND == it[r-1].p->ssv[it[r-1].v].ND && it[r].grab == it[r-1].grab;
+      if (! same_shape) nshape++;
     }
     it[r].shape = nshape;
   }
   nshape++;
+  bool any_chunked = false;
+  for (const BatchItem &x : it) any_chunked |= (x.v < 0);
   const int nchain = std::min(nprof, P7G_BATCH_CHAINS);
 
   // work areas
@@ -1286,6 +1356,9 @@ p7_gpu_PipelineBatch(P7_GPU_PIPELINE *pli, const P7_GPU_OPROFILE *const *oms, in
     if ((status = work_reserve(w, n, pipeline_pool_floats(n, db->nres))) != p7G_OK) return status;
   }
   if ((status = batch_reserve(b, nprof, n, mask)) != p7G_OK) return status;
+  for (const BatchItem &x : it)
+    if (x.v < 0 && (status = work_bnd(pli->w, *x.p, db->bytes)) != p7G_OK) return status;
+  uint16_t *bnd = any_chunked ? pli->w.bnd : nullptr;
   while ((int) b.evs.size() < nshape + nchain) {
     hipEvent_t e;
     HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
@@ -1296,7 +1369,8 @@ p7_gpu_PipelineBatch(P7_GPU_PIPELINE *pli, const P7_GPU_OPROFILE *const *oms, in
 
   // the graph is replayed while everything it was captured with is the same
   std::vector<unsigned long long> key = { (unsigned long long) (uintptr_t) b.mem, db->serial, (unsigned long long) n,
-                                          (unsigned long long) nprof, (unsigned long long) nchain, mask };
+                                          (unsigned long long) nprof, (unsigned long long) nchain, mask,
+                                          (unsigned long long) (uintptr_t) bnd, (unsigned long long) pli->w.bnd_cap };
   for (int k = 0; k < nchain; k++) {
     key.push_back((unsigned long long) (uintptr_t) b.ch[k].mem);
     key.push_back((unsigned long long) (uintptr_t) b.ch[k].pool);
@@ -1312,7 +1386,7 @@ p7_gpu_PipelineBatch(P7_GPU_PIPELINE *pli, const P7_GPU_OPROFILE *const *oms, in
   if (! eager && (! b.exec || key != b.key)) {
     batch_drop_graph(b);
     HIPCHK(hipStreamBeginCapture(st0, hipStreamCaptureModeThreadLocal));
-    status = batch_enqueue(b, st0, nchain, it, nshape, s, n, mask);
+    status = batch_enqueue(b, st0, nchain, it, nshape, s, n, mask, bnd, pli->w.bnd_cap);
     hipGraph_t gr = nullptr;
     const hipError_t ec = hipStreamEndCapture(st0, &gr);
     if (status != p7G_OK) { if (gr) (void) hipGraphDestroy(gr); return status; }
@@ -1323,7 +1397,7 @@ p7_gpu_PipelineBatch(P7_GPU_PIPELINE *pli, const P7_GPU_OPROFILE *const *oms, in
   }
 
   HIPCHK(hipEventRecord(b.t[0], st0));
-  if (eager) { if ((status = batch_enqueue(b, st0, nchain, it, nshape, s, n, mask)) != p7G_OK) return status; }
+  if (eager) { if ((status = batch_enqueue(b, st0, nchain, it, nshape, s, n, mask, bnd, pli->w.bnd_cap)) != p7G_OK) return status; }
   else HIPCHK(hipGraphLaunch(b.exec, st0));
   HIPCHK(hipEventRecord(b.t[1], st0));
   b.last_graph = eager ? 0 : 1;
@@ -1437,7 +1511,7 @@ p7_gpu_Autotune(const P7_GPU_OPROFILE *const *oms, int nprof, const P7_GPU_SEQDB
     if ((status = check_query(oms[j], db)) != p7G_OK) return status;
     if ((status = dev_profile(oms[j], db->Lmax, &p)) != p7G_OK) return status;
     const int cls = tune_class(p->M);
-    if (done.count(cls)) continue;
+    if (done.count(cls) || p->ssv_nchunk) continue;     // chunked models have one shape only
     // every k-th slot (slots run longest first, so the sample keeps the length
     // mix), k chosen for about 3e10 cells per timed run
     const int64_t k = std::max<int64_t>(1, (int64_t) ((double) db->nres * p->M / 3e10));
@@ -1615,7 +1689,8 @@ single_msv(const uint8_t *dsq, int L, const P7_GPU_OPROFILE *om, int *ret_xj)
   Work &w = g_work;
   LenCfg lc = lencfg_of(om);
   if ((status = reset_counts(g_work)) != p7G_OK) return status;
-  HIPCHK(launch_ssv_chosen(*p, s, nullptr, 1, w.r.zmax, &w.l.counts[P7G_Q_SSV], g_stream));
+  if ((status = work_bnd(w, *p, g_one.cap)) != p7G_OK) return status;
+  HIPCHK(launch_ssv_chosen(*p, s, nullptr, 1, w.r.zmax, &w.l.counts[P7G_Q_SSV], w.bnd, w.bnd_cap, g_stream));
   HIPCHK(launch_msv_classify(*p, s, nullptr, 1, w.r.zmax, w.r.xj, w.l.full, &w.l.counts[0], lc, g_stream));
   HIPCHK(launch_compact_i16(w.r.xj, -2, 1, w.l.full, &w.l.counts[0], w.tilecnt, g_stream));
   HIPCHK(launch_msv_full(*p, s, w.l.full, &w.l.counts[0], 1, w.r.xj, lc, g_stream));

@@ -58,13 +58,21 @@ __device__ __forceinline__ int lane_in(int fill, int v)
   return __builtin_amdgcn_update_dpp(b, v, ROW_SHR1, 0xf, 0xf, false);
 }
 
-template <int G, int ND>
+// BND: one chunk of a model too long for one LDS table (G = 64, one target
+// per wave). The chunk's first position takes its diagonal predecessor from
+// bin (bin[off + r] = Z of the previous chunk's last position after row r;
+// row 0 takes the floor), and the chunk's last position after row r goes to
+// bout[off + r]. Both are indexed like the residues, so each target owns the
+// entries of its rows. The chunk's maximum is folded into zmax when bin is
+// given.
+template <int G, int ND, bool BND>
 __global__ __launch_bounds__(SSV_BLOCK)
 void k_ssv(const SsvJobs jobs, int njobs, int tbl_dwords,
            const uint8_t *__restrict__ res, const int64_t *__restrict__ off, const int32_t *__restrict__ len,
            const int32_t *__restrict__ slots, int64_t n, uint16_t *__restrict__ zmax, int64_t zstride,
-           int32_t *queues, int grab)
+           int32_t *queues, int grab, const uint16_t *__restrict__ bin, uint16_t *__restrict__ bout)
 {
+  static_assert(! BND || G == 64, "chunked models take one target per wave");
   constexpr int NQ   = ND / 4;                        // 16-byte reads per row
   constexpr int NT   = ND % 4;                        // 4-byte reads per row
   constexpr int NQa  = NQ ? NQ : 1, NTa = NT ? NT : 1;
@@ -117,13 +125,34 @@ void k_ssv(const SsvJobs jobs, int njobs, int tbl_dwords,
       for (int a = 0; a < NACC; a++) acc[a] = (int) NEG_PAIR;
       int nb = (int) NEG_PAIR;          // DPP destination: lanes without a left neighbour keep this fill
 
+      const uint16_t *bi = nullptr;
+      uint16_t *bo = nullptr;
+      if (BND) {
+        if (bin) bi = bin + off[slot];
+        if (bout) bo = bout + off[slot];
+      }
+      // boundary values of rows i0 .. i0+15 (bq) and of row i0-1 (pv); the floor before row 0
+      uint4 bq0 = make_uint4(0x80008000u, 0x80008000u, 0x80008000u, 0x80008000u), bq1 = bq0;
+      unsigned pv = 0x8000u;
+#define BQ_WORD(e) (((e) < 8) ? (((e) & 6) == 0 ? bq0.x : ((e) & 6) == 2 ? bq0.y : ((e) & 6) == 4 ? bq0.z : bq0.w) \
+                              : (((e) & 6) == 0 ? bq1.x : ((e) & 6) == 2 ? bq1.y : ((e) & 6) == 4 ? bq1.z : bq1.w))
+#define BQ_HALF(e) ((BQ_WORD(e) >> (((e) & 1) * 16)) & 0xffffu)
+      // row i0+u's predecessor in at lane 0 before the row's shift; the row's last position out from lane 63
+#define BND_IN(u) do { if (BND && (u) < 16) {                                                   \
+        const unsigned v_ = ((u) == 0) ? pv : BQ_HALF((u) - 1);                                  \
+        nb = (z == 0) ? (int) (v_ << 16) : nb; } } while (0)
+#define BND_OUT(r) do { if (BND && bo && z == 63) bo[r] = (uint16_t) ((unsigned) zr[ND-1] >> 16); } while (0)
+#define BND_LOAD(i) do { if (BND && bi) {                                                        \
+        pv = bq1.w >> 16; bq0 = *(const uint4 *) (bi + (i)); bq1 = *(const uint4 *) (bi + (i) + 8); } } while (0)
+
       const uint4 padchunk = make_uint4(0x1f1f1f1fu, 0x1f1f1f1fu, 0x1f1f1f1fu, 0x1f1f1f1fu);
       // one DP row with the scores in (cq, ct), while the row of residue xn is fetched into (nq, nt)
-#define SSV_ROW(cq, ct, nq, nt, xn) do {                                                      \
+#define SSV_ROW(cq, ct, nq, nt, xn, u, r) do {                                                \
         const int4 *rq_ = qrow + (xn) * (NQ * G);                                               \
         const int  *rt_ = trow + (xn) * (NT * G);                                               \
         _Pragma("unroll") for (int q = 0; q < NQ; q++) nq[q] = rq_[q * G];                      \
-        _Pragma("unroll") for (int u = 0; u < NT; u++) nt[u] = rt_[u * G];                      \
+        _Pragma("unroll") for (int u_ = 0; u_ < NT; u_++) nt[u_] = rt_[u_ * G];                 \
+        BND_IN(u);                                                                              \
         nb = lane_in<G>(nb, zr[ND-1]);                                                          \
         const int s0_ = (int) __builtin_amdgcn_alignbit((unsigned) zr[ND-1], (unsigned) nb, 16); \
         _Pragma("unroll") for (int d = ND-1; d >= 0; d--) {                                     \
@@ -137,6 +166,7 @@ void k_ssv(const SsvJobs jobs, int njobs, int tbl_dwords,
         _Pragma("unroll") for (int d = 0; d + 1 < ND; d += 2)                                   \
           acc[(d / 2) % NACC] = zmax3(acc[(d / 2) % NACC], zr[d], zr[d+1]);                     \
         if (ND & 1) acc[((ND - 1) / 2) % NACC] = zmax2(acc[((ND - 1) / 2) % NACC], zr[ND-1]);   \
+        BND_OUT(r);                                                                             \
       } while (0)
 #define RES(w, u) ((int) (((w) >> (8 * ((u) & 3))) & 0xffu))
 
@@ -153,19 +183,28 @@ void k_ssv(const SsvJobs jobs, int njobs, int tbl_dwords,
       int i0 = 0;
       for (; i0 + 16 <= Lw; i0 += 16) {
         const uint4 nch = (i0 + 16 < L) ? *(const uint4 *) (sq + i0 + 16) : padchunk;
-        SSV_ROW(qA, tA, qB, tB, RES(ch.x, 1));  SSV_ROW(qB, tB, qA, tA, RES(ch.x, 2));
-        SSV_ROW(qA, tA, qB, tB, RES(ch.x, 3));  SSV_ROW(qB, tB, qA, tA, RES(ch.y, 0));
-        SSV_ROW(qA, tA, qB, tB, RES(ch.y, 1));  SSV_ROW(qB, tB, qA, tA, RES(ch.y, 2));
-        SSV_ROW(qA, tA, qB, tB, RES(ch.y, 3));  SSV_ROW(qB, tB, qA, tA, RES(ch.z, 0));
-        SSV_ROW(qA, tA, qB, tB, RES(ch.z, 1));  SSV_ROW(qB, tB, qA, tA, RES(ch.z, 2));
-        SSV_ROW(qA, tA, qB, tB, RES(ch.z, 3));  SSV_ROW(qB, tB, qA, tA, RES(ch.w, 0));
-        SSV_ROW(qA, tA, qB, tB, RES(ch.w, 1));  SSV_ROW(qB, tB, qA, tA, RES(ch.w, 2));
-        SSV_ROW(qA, tA, qB, tB, RES(ch.w, 3));  SSV_ROW(qB, tB, qA, tA, RES(nch.x, 0));
+        BND_LOAD(i0);
+        SSV_ROW(qA, tA, qB, tB, RES(ch.x, 1), 0, i0);       SSV_ROW(qB, tB, qA, tA, RES(ch.x, 2), 1, i0 + 1);
+        SSV_ROW(qA, tA, qB, tB, RES(ch.x, 3), 2, i0 + 2);   SSV_ROW(qB, tB, qA, tA, RES(ch.y, 0), 3, i0 + 3);
+        SSV_ROW(qA, tA, qB, tB, RES(ch.y, 1), 4, i0 + 4);   SSV_ROW(qB, tB, qA, tA, RES(ch.y, 2), 5, i0 + 5);
+        SSV_ROW(qA, tA, qB, tB, RES(ch.y, 3), 6, i0 + 6);   SSV_ROW(qB, tB, qA, tA, RES(ch.z, 0), 7, i0 + 7);
+        SSV_ROW(qA, tA, qB, tB, RES(ch.z, 1), 8, i0 + 8);   SSV_ROW(qB, tB, qA, tA, RES(ch.z, 2), 9, i0 + 9);
+        SSV_ROW(qA, tA, qB, tB, RES(ch.z, 3), 10, i0 + 10); SSV_ROW(qB, tB, qA, tA, RES(ch.w, 0), 11, i0 + 11);
+        SSV_ROW(qA, tA, qB, tB, RES(ch.w, 1), 12, i0 + 12); SSV_ROW(qB, tB, qA, tA, RES(ch.w, 2), 13, i0 + 13);
+        SSV_ROW(qA, tA, qB, tB, RES(ch.w, 3), 14, i0 + 14); SSV_ROW(qB, tB, qA, tA, RES(nch.x, 0), 15, i0 + 15);
         ch = nch;
       }
+      if (i0 < Lw) BND_LOAD(i0);
       for (int u = 0; i0 + u < Lw; u++) {              // the last rows, fewer than 16
         const unsigned wd = (u + 1 < 4) ? ch.x : (u + 1 < 8) ? ch.y : (u + 1 < 12) ? ch.z : ch.w;
-        SSV_ROW(qA, tA, qB, tB, RES(wd, u + 1));
+        if (BND && bi) {                               // row i0+u's predecessor, by a uniform select
+          const int e = u - 1;
+          const unsigned bw = (e < 2) ? bq0.x : (e < 4) ? bq0.y : (e < 6) ? bq0.z : (e < 8) ? bq0.w
+                            : (e < 10) ? bq1.x : (e < 12) ? bq1.y : (e < 14) ? bq1.z : bq1.w;
+          const unsigned v = (u == 0) ? pv : ((bw >> ((e & 1) * 16)) & 0xffffu);
+          nb = (z == 0) ? (int) (v << 16) : nb;
+        }
+        SSV_ROW(qA, tA, qB, tB, RES(wd, u + 1), 16, i0 + u);
 #pragma unroll
         for (int q = 0; q < NQ; q++) qA[q] = qB[q];
 #pragma unroll
@@ -173,12 +212,20 @@ void k_ssv(const SsvJobs jobs, int njobs, int tbl_dwords,
       }
 #undef SSV_ROW
 #undef RES
+#undef BND_IN
+#undef BND_OUT
+#undef BND_LOAD
+#undef BQ_HALF
+#undef BQ_WORD
       int m = -32768;
 #pragma unroll
       for (int a = 0; a < NACC; a++) m = max(m, max(lo16(acc[a]), hi16(acc[a])));
 #pragma unroll
       for (int o = G/2; o > 0; o >>= 1) m = max(m, __shfl_xor(m, o, 64));
-      if (valid && z == 0) zm[slot] = (uint16_t) (m + 32768);
+      if (valid && z == 0) {
+        const uint16_t v = (uint16_t) (m + 32768);
+        zm[slot] = (BND && bin) ? (uint16_t) max((int) zm[slot], (int) v) : v;
+      }
     }
     __syncthreads();                                  // every wave is done with this table
   }
@@ -194,29 +241,38 @@ void k_ssv(const SsvJobs jobs, int njobs, int tbl_dwords,
   X(32,14) X(32,15) X(32,16) X(32,18) X(32,20) X(32,22) X(32,24) X(32,26) X(32,28) X(32,30) X(32,32)
 #define SSV_ND64(X) X(64,8) X(64,9) X(64,10) X(64,11) X(64,12) X(64,13) X(64,14) X(64,15) X(64,16) X(64,18) \
   X(64,20) X(64,22) X(64,24) X(64,26) X(64,28) X(64,30) X(64,32) X(64,36) X(64,40)
-#define SSV_CASE(G_, N_) case (G_) * 256 + (N_): return (void *) k_ssv<G_, N_>;
+// chunk shapes of models longer than one LDS table holds (G = 64)
+#define SSV_CHUNK(X) X(64,11) X(64,12) X(64,13) X(64,14) X(64,15) X(64,16) X(64,18)
+#define SSV_CASE(G_, N_) case (G_) * 256 + (N_): return (void *) k_ssv<G_, N_, false>;
+#define SSV_BCASE(G_, N_) case (G_) * 256 + (N_): return (void *) k_ssv<G_, N_, true>;
 
-static void *ssv_kernel(int G, int ND)
+static void *ssv_kernel(int G, int ND, bool bnd)
 {
+  if (bnd) {
+    switch (G * 256 + ND) { SSV_CHUNK(SSV_BCASE) }
+    return nullptr;
+  }
   switch (G * 256 + ND) {
     SSV_ND16(SSV_CASE) SSV_ND32(SSV_CASE) SSV_ND64(SSV_CASE)
   }
   return nullptr;
 }
 
-int ssv_shape_built(int G, int ND) { return ssv_kernel(G, ND) != nullptr; }
+int ssv_shape_built(int G, int ND) { return ssv_kernel(G, ND, false) != nullptr; }
+int ssv_chunk_built(int ND) { return ssv_kernel(64, ND, true) != nullptr; }
 
 static hipError_t ssv_run(int G, int ND, const SsvJobs &jobs, int njobs, const DevSeqs &s, const int32_t *slots,
-                          int64_t n, uint16_t *zmax, int64_t zstride, int32_t *queues, int grab, hipStream_t st)
+                          int64_t n, uint16_t *zmax, int64_t zstride, int32_t *queues, int grab,
+                          const uint16_t *bin, uint16_t *bout, bool bnd, hipStream_t st)
 {
-  void *k = ssv_kernel(G, ND);
-  if (! k || njobs < 1 || njobs > P7G_SSV_MAXJOBS || grab < 1) return hipErrorInvalidValue;
+  void *k = ssv_kernel(G, ND, bnd);
+  if (! k || njobs < 1 || njobs > P7G_SSV_MAXJOBS || grab < 1 || (bnd && njobs != 1)) return hipErrorInvalidValue;
   const size_t lds = (size_t) P7G_NROWS * G * ND * 4;
   const int64_t nwaves = (int64_t) njobs * ((n + (64 / G) - 1) / (64 / G));
   const unsigned grid = grid_for_waves_blk(k, nwaves, lds, SSV_BLOCK / 64);
   int tbl_dwords = (int) (lds / 4);
   void *args[] = { (void *) &jobs, &njobs, &tbl_dwords, (void *) &s.res, (void *) &s.off, (void *) &s.len,
-                   (void *) &slots, &n, &zmax, &zstride, &queues, &grab };
+                   (void *) &slots, &n, &zmax, &zstride, &queues, &grab, (void *) &bin, (void *) &bout };
   return launch(k, grid, SSV_BLOCK, args, lds, st);
 }
 
@@ -230,7 +286,28 @@ hipError_t launch_ssv(const DevProfile &p, int v, const DevSeqs &s, const int32_
   SsvJobs jobs;
   memset(&jobs, 0, sizeof(jobs));
   jobs.tbl[0] = sv.tbl;
-  return ssv_run(sv.G, sv.ND, jobs, 1, s, slots, n, zmax, 0, queue, grab, st);
+  return ssv_run(sv.G, sv.ND, jobs, 1, s, slots, n, zmax, 0, queue, grab, nullptr, nullptr, false, st);
+}
+
+hipError_t launch_ssv_chunked(const DevProfile &p, const DevSeqs &s, const int32_t *slots, int64_t n, uint16_t *zmax,
+                              int32_t *queues, int grab, uint16_t *bnd, int64_t bnd_stride, hipStream_t st)
+{
+  if (n <= 0) return hipSuccess;
+  const int nc = p.ssv_nchunk;
+  if (nc < 2 || nc > P7G_SSV_MAXCHUNK || ! bnd) return hipErrorInvalidValue;
+  for (int c = 0; c < nc; c++) {
+    const SsvVariant &sv = p.ssvc[c];
+    if (sv.G != 64 || sv.dwords != (size_t) P7G_NROWS * 64 * sv.ND) return hipErrorInvalidValue;
+    SsvJobs jobs;
+    memset(&jobs, 0, sizeof(jobs));
+    jobs.tbl[0] = sv.tbl;
+    // boundary columns alternate between the two halves of bnd
+    const uint16_t *bin = c ? bnd + (size_t) ((c - 1) & 1) * bnd_stride : nullptr;
+    uint16_t *bout = (c < nc - 1) ? bnd + (size_t) (c & 1) * bnd_stride : nullptr;
+    hipError_t e = ssv_run(64, sv.ND, jobs, 1, s, slots, n, zmax, 0, queues + c, grab, bin, bout, true, st);
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
 }
 
 hipError_t launch_ssv_multi(int G, int ND, const int *const *tbl, int njobs, const DevSeqs &s, int64_t n,
@@ -242,7 +319,8 @@ hipError_t launch_ssv_multi(int G, int ND, const int *const *tbl, int njobs, con
     SsvJobs jobs;
     memset(&jobs, 0, sizeof(jobs));
     for (int j = 0; j < nj; j++) jobs.tbl[j] = tbl[j0 + j];
-    hipError_t e = ssv_run(G, ND, jobs, nj, s, nullptr, n, zmax + (size_t) j0 * zstride, zstride, queues + j0, grab, st);
+    hipError_t e = ssv_run(G, ND, jobs, nj, s, nullptr, n, zmax + (size_t) j0 * zstride, zstride, queues + j0, grab,
+                           nullptr, nullptr, false, st);
     if (e != hipSuccess) return e;
   }
   return hipSuccess;

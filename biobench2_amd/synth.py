@@ -359,3 +359,20 @@ def query_family(gpu, dirpath, nprof=8, median_M=250, sigma=0.6, Mmin=50, Mmax=3
         write_hmm(path, [h])
         paths.append(path)
     return paths
+
+
+def batch_family(gpu, dirpath, nprof=64, Mmin=50, Mmax=2000, seed=42):
+    """Calibrated profiles with M log-uniform in [Mmin, Mmax] (a batch of uneven
+    models), written to dirpath; returns their paths."""
+    import os
+    rng = np.random.default_rng(seed)
+    paths = []
+    for q in range(nprof):
+        M = int(round(math.exp(rng.uniform(math.log(Mmin), math.log(Mmax)))))
+        h = random_profile(M, rng, f"batch{q}_M{M}")
+        path = os.path.join(dirpath, h.name + ".hmm")
+        write_hmm(path, [h])
+        h.stats = calibrate_with(gpu, path, seed=seed + q)
+        write_hmm(path, [h])
+        paths.append(path)
+    return paths

@@ -1,0 +1,148 @@
+"""Models longer than one LDS table holds (M = 2100, 2600 and 5000, as in the
+precision stress: long targets, indel-rich homologs).
+
+For these models the ungapped MSV pass runs in two chunks of model positions,
+joined by a boundary column in HBM (above M = 2560); Viterbi keeps its
+transitions in LDS and reads emission rows from global memory (above
+M = 2047); Forward and Backward read their transitions per row (B >= 16).
+MSV and Viterbi must stay bit-exact against the oracle, Forward and Backward
+within 1e-4 nats of the oracle's fp32 restatement, and the whole pipeline
+must give the oracle's pass sets. The error of the fp32 Forward against the
+FP64 log-space oracle is bounded too, for targets of up to 3000 residues.
+
+The STATS lines of these generated models are fixed values, not a
+calibration: the gates only need both sides to use the same parameters."""
+import numpy as np
+import pytest
+
+import oracle
+from biobench2_amd import p7gpu, synth
+from conftest import random_seqs
+
+pytestmark = pytest.mark.gpu
+
+LARGE = [2100, 2600, 5000]
+STATS = (-9.0, 0.6931, -9.5, 0.6931, -5.0, 0.6931)
+
+
+@pytest.fixture(scope="module")
+def large(gpu, tmp_path_factory):
+    d = tmp_path_factory.mktemp("large")
+    rng = np.random.default_rng(77)
+    out = {}
+    for M in LARGE:
+        h = synth.random_profile(M, rng, f"large_M{M}")
+        h.stats = STATS
+        path = str(d / f"large_M{M}.hmm")
+        synth.write_hmm(path, [h])
+        hh = synth.read_hmm_probs(path)
+        seqs = random_seqs(rng, [1, 16, 17, 300, 1000, 2047, 2600, 5000, 12000])
+        for core in synth.emit_homologs(hh, 3, rng, indel_rich=True):
+            flank = random_seqs(rng, [50])[0]
+            seqs.append(np.concatenate([flank, core, flank]))
+        out[M] = (path, hh, seqs, gpu.SeqDB.from_list(seqs))
+    return out
+
+
+def per_target_config(t, L, lm):
+    xw = t["xw"].copy()
+    xw[1, 0] = xw[2, 0] = xw[3, 0] = lm.xwmove[L]
+    xf = t["xf"].copy()
+    xf[1, 0] = xf[2, 0] = xf[3, 0] = lm.xfmove[L]
+    xf[1, 1] = xf[2, 1] = xf[3, 1] = lm.xfloop[L]
+    return xw, xf
+
+
+def same(a, b):
+    return a == b or (np.isinf(a) and np.isinf(b) and np.sign(a) == np.sign(b))
+
+
+@pytest.mark.parametrize("M", LARGE)
+def test_large_msv_and_viterbi_bit_exact(large, M):
+    path, hh, seqs, db = large[M]
+    om = p7gpu.OProfile.from_file(path)
+    t = om.tables()
+    assert t["M"] == M
+    lm = oracle.LengthModel(t, max(len(s) for s in seqs) + 1)
+    msc, mst = p7gpu.msv_batch(om, db)
+    vsc, vst = p7gpu.viterbi_batch(om, db)
+    for i, s in enumerate(seqs):
+        L = len(s)
+        st, sc, _ = oracle.msv_filter(s, t, tjb=int(lm.tjb[L]))
+        assert st == mst[i] and same(sc, msc[i]), ("msv", M, i, L, st, sc, mst[i], msc[i])
+        xw, _ = per_target_config(t, L, lm)
+        st, sc, _ = oracle.vit_filter(s, t, xw)
+        assert st == vst[i] and same(sc, vsc[i]), ("vit", M, i, L, st, sc, vst[i], vsc[i])
+
+
+@pytest.mark.parametrize("M", LARGE)
+def test_large_forward_backward_within_1e4_nats(large, M):
+    path, hh, seqs, db = large[M]
+    om = p7gpu.OProfile.from_file(path)
+    t = om.tables()
+    lm = oracle.LengthModel(t, max(len(s) for s in seqs) + 1)
+    fsc, fst = p7gpu.forward_batch(om, db)
+    bsc, bst = p7gpu.backward_batch(om, db)
+    err_f = err_b = err64 = 0.0
+    for i, s in enumerate(seqs):
+        L = len(s)
+        _, xf = per_target_config(t, L, lm)
+        st, sc, xmx, _ = oracle.fwd_parser(s, t, xf)
+        assert st == fst[i] == 0
+        st2, sc2, _, _ = oracle.bck_parser(s, t, xmx, xf)
+        assert st2 == bst[i] == 0
+        err_f = max(err_f, abs(sc - fsc[i]))
+        err_b = max(err_b, abs(sc2 - bsc[i]))
+        if L <= 3000:
+            om.gm.reconfig_length(L)
+            tsc, rsc, xsc = om.gm.scores()
+            err64 = max(err64, abs(fsc[i] - oracle.forward_log64(s, tsc, rsc, xsc)))
+    assert err_f <= 1e-4 and err_b <= 1e-4, (M, err_f, err_b)
+    assert err64 <= 2e-3, (M, err64)
+
+
+@pytest.mark.parametrize("M", [2600, 5000])
+def test_large_pipeline_pass_sets(large, M):
+    """Loose thresholds, so that every stage runs on many targets, over a
+    database with a long-sequence tail and indel-rich homologs."""
+    path, hh, _, _ = large[M]
+    om = p7gpu.OProfile.from_file(path)
+    t = om.tables()
+    db = synth.make_db(1500, [hh], frac_homolog=0.1, seed_db=M, seed_plant=M + 1, long_frac=0.01,
+                       long_range=(5000, 12000), indel_rich=True)
+    sdb = p7gpu.SeqDB(db.residues, db.offsets, db.lengths)
+    F = (0.5, 0.2, 0.05)
+    got = p7gpu.Pipeline(*F).run(om, sdb)
+    prof, keep = oracle.make_profile(t, oracle.bias_hmm(t), t["evparam"], F1=F[0], F2=F[1], F3=F[2],
+                                     Lmax=int(db.lengths.max()) + 1)
+    ref = oracle.pipeline_db(prof, db.residues, db.offsets, db.lengths, nthreads=8)
+    assert np.array_equal(got["stage"], ref["stage"])
+    assert np.array_equal(got["usc"], ref["usc"])
+    m = ~np.isnan(ref["vfsc"])
+    assert np.array_equal(np.isnan(got["vfsc"]), ~m) and np.array_equal(got["vfsc"][m], ref["vfsc"][m])
+    for k in ("fwdsc", "bcksc"):
+        m = ~np.isnan(ref[k])
+        assert np.array_equal(np.isnan(got[k]), ~m), k
+        if m.any():
+            assert np.max(np.abs(got[k][m] - ref[k][m])) <= 1e-4, k
+    assert (got["stage"] == 4).sum() > 0 and (got["stage"] >= 2).sum() > (got["stage"] == 4).sum()
+
+
+def test_large_models_in_a_batch(large):
+    """Chunked models (M = 2600, 5000) mixed with one-pass ones in a batch, from
+    the captured graph: each equals its pipeline run alone."""
+    oms = [p7gpu.OProfile.from_file(large[M][0]) for M in LARGE]
+    path, hh, _, _ = large[2600]
+    db = synth.make_db(3000, [hh], frac_homolog=0.05, seed_db=9, seed_plant=10)
+    sdb = p7gpu.SeqDB(db.residues, db.offsets, db.lengths)
+    F = (0.5, 0.2, 0.05)
+    alone = [p7gpu.Pipeline(*F).run(om, sdb) for om in oms]
+    pli = p7gpu.Pipeline(*F)
+    for eager in (True, False, False):
+        outs, _ = pli.run_batch(oms + oms[:1], sdb, eager=eager)
+        for j, i in enumerate([0, 1, 2, 0]):
+            for k in alone[i]:
+                x, y = outs[j][k], alone[i][k]
+                assert np.array_equal(np.isnan(x), np.isnan(y)) if x.dtype != np.uint8 else True, k
+                mm = ~np.isnan(x) if x.dtype != np.uint8 else slice(None)
+                assert np.array_equal(x[mm], y[mm]), (j, k)
