@@ -78,7 +78,12 @@ struct DevSeqs {
   const int32_t *len;    // [slot]
   const int32_t *perm;   // [slot] -> the caller's index
   int64_t nslot;
+  // the ungapped MSV pass cuts targets longer than P7G_SEG_ROWS into row
+  // segments (slot, first row); they cover slots 0 .. nseg_slots-1
+  const int2 *seg;
+  int64_t nseg, nseg_slots;
 };
+#define P7G_SEG_ROWS 4096
 
 // Per-slot results written by the stage kernels.
 struct DevResults {

@@ -380,6 +380,9 @@ struct p7g_seqdb_s {
   char    *names;          // sequence names in caller order (when read from a file), NUL-separated
   int64_t *name_off;       // [caller index] offset into names
   unsigned long long serial;
+  int2    *d_seg;          // row segments of the targets longer than P7G_SEG_ROWS (slot, first row)
+  int64_t  nseg;           // their number
+  int64_t  nseg_slots;     // the leading slots they cover
 };
 
 extern "C" void
@@ -390,9 +393,30 @@ p7_gpu_seqdb_Destroy(P7_GPU_SEQDB *db)
   if (db->d_off) (void) hipFree(db->d_off);
   if (db->d_len) (void) hipFree(db->d_len);
   if (db->d_perm) (void) hipFree(db->d_perm);
+  if (db->d_seg) (void) hipFree(db->d_seg);
   free(db->h_len); free(db->h_perm); free(db->h_inv);
   free(db->names); free(db->name_off);
   free(db);
+}
+
+// Row segments of the long targets for the ungapped MSV pass: a target of
+// more than P7G_SEG_ROWS residues is cut into pieces of P7G_SEG_ROWS rows,
+// which waves take like whole targets (each starting a model length earlier,
+// see k_ssv). Slots run longest first, so the long targets are the leading ones.
+static hipError_t
+seqdb_segments(P7_GPU_SEQDB *db)
+{
+  std::vector<int2> seg;
+  int64_t s = 0;
+  for (; s < db->nseq && db->h_len[s] > P7G_SEG_ROWS; s++)
+    for (int r0 = 0; r0 < db->h_len[s]; r0 += P7G_SEG_ROWS) seg.push_back(make_int2((int) s, r0));
+  db->nseg = (int64_t) seg.size();
+  db->nseg_slots = s;
+  if (seg.empty()) return hipSuccess;
+  hipError_t e = hipMalloc((void **) &db->d_seg, sizeof(int2) * seg.size());
+  if (e == hipSuccess) e = hipMemcpyAsync(db->d_seg, seg.data(), sizeof(int2) * seg.size(), hipMemcpyHostToDevice, g_stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(g_stream);
+  return e;
 }
 
 extern "C" int
@@ -454,6 +478,7 @@ p7_gpu_seqdb_Create(const uint8_t *residues, const int64_t *offset, const int32_
   if (e == hipSuccess) e = hipMemcpyAsync(db->d_len, db->h_len, sizeof(int32_t) * (size_t) nseq, hipMemcpyHostToDevice, g_stream);
   if (e == hipSuccess) e = hipMemcpyAsync(db->d_perm, h_perm32.data(), sizeof(int32_t) * (size_t) nseq, hipMemcpyHostToDevice, g_stream);
   if (e == hipSuccess) e = hipStreamSynchronize(g_stream);
+  if (e == hipSuccess) e = seqdb_segments(db);
   free(h_res); free(h_off);
   if (e != hipSuccess) { p7g_set_error("database upload failed: %s", hipGetErrorString(e)); p7_gpu_seqdb_Destroy(db); return p7G_EDEVICE; }
   *ret_db = db;
@@ -638,6 +663,7 @@ p7_gpu_seqdb_Load(const char *path, P7_GPU_SEQDB **ret_db)
   if (e == hipSuccess) e = hipMemcpyAsync(db->d_len, db->h_len, sizeof(int32_t) * (size_t) db->nseq, hipMemcpyHostToDevice, g_stream);
   if (e == hipSuccess) e = hipMemcpyAsync(db->d_perm, perm32.data(), sizeof(int32_t) * (size_t) db->nseq, hipMemcpyHostToDevice, g_stream);
   if (e == hipSuccess) e = hipStreamSynchronize(g_stream);
+  if (e == hipSuccess) e = seqdb_segments(db);
   if (e != hipSuccess) { p7g_set_error("database upload failed: %s", hipGetErrorString(e)); p7_gpu_seqdb_Destroy(db); return p7G_EDEVICE; }
   *ret_db = db;
   return p7G_OK;
@@ -649,6 +675,7 @@ extern "C" int64_t p7_gpu_seqdb_NRes(const P7_GPU_SEQDB *db) { return db ? db->n
 static DevSeqs seqs_of(const P7_GPU_SEQDB *db)
 {
   DevSeqs s; s.res = db->d_res; s.off = db->d_off; s.len = db->d_len; s.perm = db->d_perm; s.nslot = db->nseq;
+  s.seg = db->d_seg; s.nseg = db->nseg; s.nseg_slots = db->nseg_slots;
   return s;
 }
 
@@ -1653,6 +1680,7 @@ single_stage(const uint8_t *dsq, int L, DevSeqs *s)
   // the work buffers' stream-ordered use keeps `h` alive until the copies complete
   HIPCHK(hipStreamSynchronize(g_stream));
   s->res = o.res; s->off = o.off; s->len = o.len; s->nslot = 1;
+  s->perm = nullptr; s->seg = nullptr; s->nseg = 0; s->nseg_slots = 0;
   return p7G_OK;
 }
 

@@ -70,9 +70,16 @@ __global__ __launch_bounds__(SSV_BLOCK)
 void k_ssv(const SsvJobs jobs, int njobs, int tbl_dwords,
            const uint8_t *__restrict__ res, const int64_t *__restrict__ off, const int32_t *__restrict__ len,
            const int32_t *__restrict__ slots, int64_t n, uint16_t *__restrict__ zmax, int64_t zstride,
-           int32_t *queues, int grab, const uint16_t *__restrict__ bin, uint16_t *__restrict__ bout)
+           int32_t *queues, int grab, const uint16_t *__restrict__ bin, uint16_t *__restrict__ bout,
+           const int2 *__restrict__ segs, int64_t nsegs, int64_t nseg_slots)
 {
   static_assert(! BND || G == 64, "chunked models take one target per wave");
+  // A row segment starting at row r0 is computed from row r0 - WARM on, from
+  // the floor: every diagonal of the ungapped recurrence reaching a row >= r0
+  // starts at most M - 1 <= WARM rows earlier, so the segment's own rows are
+  // exact and the rows before them lower bounds, and the maximum over all
+  // segments is the target's.
+  constexpr int WARM = 2 * ND * G;
   constexpr int NQ   = ND / 4;                        // 16-byte reads per row
   constexpr int NT   = ND % 4;                        // 4-byte reads per row
   constexpr int NQa  = NQ ? NQ : 1, NTa = NT ? NT : 1;
@@ -87,7 +94,8 @@ void k_ssv(const SsvJobs jobs, int njobs, int tbl_dwords,
   const int z    = lane % G;
   const int4 *qrow = (const int4 *) __builtin_assume_aligned(lds4, 16) + z;     // + (x*NQ + q)*G
   const int  *trow = (const int *) lds4 + TAIL0 + z;                           // + (x*NT + t)*G
-  const int64_t ngroups = (n + TPW - 1) / TPW;
+  const int64_t nitems = n - nseg_slots + nsegs;      // row segments of the long targets, then whole targets
+  const int64_t ngroups = (nitems + TPW - 1) / TPW;
 
   for (int j = 0;; j++) {
     // the first profile from j on whose queue still holds work
@@ -108,10 +116,21 @@ void k_ssv(const SsvJobs jobs, int njobs, int tbl_dwords,
       if (w >= ngroups) break;
       const int64_t t = w * TPW + grp;
       w++;
-      const bool valid = t < n;
-      const int64_t slot = valid ? (slots ? (int64_t) slots[t] : t) : 0;
-      const int L = valid ? len[slot] : 0;
-      const uint8_t *sq = res + (valid ? off[slot] : 0);
+      const bool valid = t < nitems;
+      const bool part = valid && t < nsegs;
+      int64_t slot = 0;
+      int rs = 0, L = 0;                              // rows rs .. rs+L-1 of the target
+      if (part) {
+        const int2 sg = segs[t];
+        slot = sg.x;
+        rs = max(0, sg.y - WARM) & ~15;
+        L = min(len[slot], sg.y + P7G_SEG_ROWS) - rs;
+      } else if (valid) {
+        const int64_t u = t - nsegs + nseg_slots;
+        slot = slots ? (int64_t) slots[u] : u;
+        L = len[slot];
+      }
+      const uint8_t *sq = res + (valid ? off[slot] + rs : 0);
       int Lw = L;
 #pragma unroll
       for (int o = G; o < 64; o <<= 1) Lw = max(Lw, __shfl_xor(Lw, o, 64));
@@ -224,7 +243,18 @@ void k_ssv(const SsvJobs jobs, int njobs, int tbl_dwords,
       for (int o = G/2; o > 0; o >>= 1) m = max(m, __shfl_xor(m, o, 64));
       if (valid && z == 0) {
         const uint16_t v = (uint16_t) (m + 32768);
-        zm[slot] = (BND && bin) ? (uint16_t) max((int) zm[slot], (int) v) : v;
+        if (part) {
+          // segments of one target fold their maxima into its (zeroed) entry:
+          // a compare-and-swap on the 32-bit word that holds it
+          unsigned *wp = (unsigned *) ((uintptr_t) &zm[slot] & ~(uintptr_t) 3);
+          const int sh = ((uintptr_t) &zm[slot] & 2) ? 16 : 0;
+          unsigned old = __hip_atomic_load(wp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT), seen;
+          do {
+            seen = old;
+            if (((seen >> sh) & 0xffffu) >= v) break;
+            old = atomicCAS(wp, seen, (seen & ~(0xffffu << sh)) | ((unsigned) v << sh));
+          } while (old != seen);
+        } else zm[slot] = (BND && bin) ? (uint16_t) max((int) zm[slot], (int) v) : v;
       }
     }
     __syncthreads();                                  // every wave is done with this table
@@ -261,18 +291,41 @@ static void *ssv_kernel(int G, int ND, bool bnd)
 int ssv_shape_built(int G, int ND) { return ssv_kernel(G, ND, false) != nullptr; }
 int ssv_chunk_built(int ND) { return ssv_kernel(64, ND, true) != nullptr; }
 
+// zmax[j * stride + i] = 0 for i < count, j < njobs: the entries the row
+// segments of the long targets fold their maxima into
+__global__ void k_zero_u16(uint16_t *__restrict__ z, int64_t stride, int64_t count, int njobs)
+{
+  const int64_t total = count * njobs;
+  for (int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (int64_t) gridDim.x * blockDim.x)
+    z[(i / count) * stride + i % count] = 0;
+}
+
 static hipError_t ssv_run(int G, int ND, const SsvJobs &jobs, int njobs, const DevSeqs &s, const int32_t *slots,
                           int64_t n, uint16_t *zmax, int64_t zstride, int32_t *queues, int grab,
                           const uint16_t *bin, uint16_t *bout, bool bnd, hipStream_t st)
 {
   void *k = ssv_kernel(G, ND, bnd);
   if (! k || njobs < 1 || njobs > P7G_SSV_MAXJOBS || grab < 1 || (bnd && njobs != 1)) return hipErrorInvalidValue;
+  // long targets in row segments: over every slot, when the warm-up (the
+  // shape's positions) is at most half a segment
+  const int2 *segs = nullptr;
+  int64_t nsegs = 0, nseg_slots = 0;
+  if (! slots && ! bnd && s.nseg > 0 && n == s.nslot && 2 * ND * G <= P7G_SEG_ROWS / 2) {
+    segs = s.seg; nsegs = s.nseg; nseg_slots = s.nseg_slots;
+    const int64_t total = nseg_slots * njobs;
+    const unsigned zg = (unsigned) std::min<int64_t>((total + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_zero_u16, dim3(zg), dim3(256), 0, st, zmax, zstride, nseg_slots, njobs);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
   const size_t lds = (size_t) P7G_NROWS * G * ND * 4;
-  const int64_t nwaves = (int64_t) njobs * ((n + (64 / G) - 1) / (64 / G));
+  const int64_t nitems = n - nseg_slots + nsegs;
+  const int64_t nwaves = (int64_t) njobs * ((nitems + (64 / G) - 1) / (64 / G));
   const unsigned grid = grid_for_waves_blk(k, nwaves, lds, SSV_BLOCK / 64);
   int tbl_dwords = (int) (lds / 4);
   void *args[] = { (void *) &jobs, &njobs, &tbl_dwords, (void *) &s.res, (void *) &s.off, (void *) &s.len,
-                   (void *) &slots, &n, &zmax, &zstride, &queues, &grab, (void *) &bin, (void *) &bout };
+                   (void *) &slots, &n, &zmax, &zstride, &queues, &grab, (void *) &bin, (void *) &bout,
+                   (void *) &segs, &nsegs, &nseg_slots };
   return launch(k, grid, SSV_BLOCK, args, lds, st);
 }
 

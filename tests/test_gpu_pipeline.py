@@ -73,3 +73,25 @@ def test_pipeline_max_mode_passes_everything(gpu):
     got, ref, st, db = run_both(gpu, "synth_M50", 500, 0.0, do_bias=False, F=(1.0, 1.0, 1.0))
     check(got, ref)
     assert (got["stage"] == 4).all()
+
+
+@pytest.mark.parametrize("name,M", [("synth_M400", 400), ("SMC_N", 1142)])
+def test_pipeline_long_targets_in_row_segments(gpu, name, M):
+    """Targets longer than 4096 residues run the ungapped MSV pass in row
+    segments (each started a model length early) whose maxima are folded
+    together; the pass sets and scores stay the oracle's."""
+    from conftest import golden
+    path = synth_profile(name) if name.startswith("synth") else golden(name + ".hmm")
+    om = gpu.OProfile.from_file(path)
+    hmm = synth.read_hmm_probs(synth_profile("synth_M400"))
+    db = synth.make_db(3000, [hmm], frac_homolog=0.03, seed_db=M, seed_plant=M + 1, long_frac=0.05,
+                       long_range=(4097, 30000))
+    assert (db.lengths > 4096).sum() >= 100
+    sdb = gpu.SeqDB(db.residues, db.offsets, db.lengths)
+    F = (0.3, 0.05, 0.01)
+    got = gpu.Pipeline(*F).run(om, sdb)
+    t = om.tables()
+    prof, keep = oracle.make_profile(t, oracle.bias_hmm(t), t["evparam"], F1=F[0], F2=F[1], F3=F[2],
+                                     Lmax=int(db.lengths.max()) + 1)
+    ref = oracle.pipeline_db(prof, db.residues, db.offsets, db.lengths, nthreads=8)
+    check(got, ref)
