@@ -1180,7 +1180,8 @@ p7_gpu_Pipeline(P7_GPU_PIPELINE *pli, const P7_GPU_OPROFILE *om, const P7_GPU_SE
 /*****************************************************************
  * Batches: many profiles against one database in one call
  *****************************************************************/
-#define P7G_BATCH_CHAINS 4          // concurrent per-profile stage chains
+#define P7G_BATCH_CHAINS 8          // concurrent streams of a batch launched directly
+#define P7G_GRAPH_CHAINS 4          // concurrent per-profile stage chains of a captured batch
 
 struct Batch {
   Work     ch[P7G_BATCH_CHAINS];   // each chain's stream and work area
@@ -1194,7 +1195,7 @@ struct Batch {
   int32_t  *counts = nullptr;            // [profile][16] list counts at the end of the run
   uint8_t  *o_stage = nullptr;           // [profile][n] outputs in the caller's order, as asked for
   float    *o_f[6] = { nullptr, nullptr, nullptr, nullptr, nullptr, nullptr };
-  std::vector<hipEvent_t> evs;           // fork (one per shape) and join (one per chain) events
+  std::vector<hipEvent_t> evs;           // per shape (its launch done), per chain (join), then the fork
   hipEvent_t t[2];
   bool     t_ready = false;
   hipGraph_t     graph = nullptr;
@@ -1275,22 +1276,32 @@ batch_reserve(Batch &b, int nprof, int64_t n, unsigned mask)
 }
 
 // Enqueues one batch on st0 and the chain streams: the ungapped pass of every
-// profile, one persistent launch per kernel shape on st0, each followed by a
-// fork event; then every profile's later stages on chain (row mod nchain),
-// which waits only for its own shape's launch; then st0 joins every chain.
+// profile, one persistent launch per kernel shape, each followed by its
+// event; then every profile's later stages on chain (row mod nchain), which
+// waits only for its own shape's launch; then st0 joins every chain. The
+// shape launches run on st0, or with `spread` on stream (shape mod nchain)
+// after a fork event, so that the long targets of different launches overlap
+// (used for direct launches; a captured batch keeps them on st0).
 static int
 batch_enqueue(Batch &b, hipStream_t st0, int nchain, const std::vector<BatchItem> &it, int nshape,
-              const DevSeqs &s, int64_t n, unsigned mask, uint16_t *bnd, int64_t bnd_stride)
+              const DevSeqs &s, int64_t n, unsigned mask, uint16_t *bnd, int64_t bnd_stride, bool spread)
 {
   const int nit = (int) it.size();
   HIPCHK(launch_fill32(b.queues, 0u, (1 + P7G_SSV_MAXCHUNK) * (int64_t) nit, st0));
+  if (spread) {
+    hipEvent_t fork = b.evs[nshape + nchain];
+    HIPCHK(hipEventRecord(fork, st0));
+    for (int k = 0; k < nchain; k++) HIPCHK(hipStreamWaitEvent(b.ch[k].st, fork, 0));
+  }
   for (int r0 = 0; r0 < nit; ) {
     int r1 = r0;
     while (r1 < nit && it[r1].shape == it[r0].shape) r1++;
-    if (it[r0].v < 0) {                  // a chunked model: its own two launches (one row per shape)
+    // chunked models share one pair of boundary columns: their launches stay on st0
+    hipStream_t sst = (spread && it[r0].v >= 0) ? b.ch[it[r0].shape % nchain].st : st0;
+    if (it[r0].v < 0) {                  // a chunked model: its own launches (one row per shape)
       HIPCHK(launch_ssv_chunked(*it[r0].p, s, nullptr, n, b.zmax + (size_t) r0 * n,
-                                b.queues + nit + P7G_SSV_MAXCHUNK * r0, it[r0].grab, bnd, bnd_stride, st0));
-      HIPCHK(hipEventRecord(b.evs[it[r0].shape], st0));
+                                b.queues + nit + P7G_SSV_MAXCHUNK * r0, it[r0].grab, bnd, bnd_stride, sst));
+      HIPCHK(hipEventRecord(b.evs[it[r0].shape], sst));
       r0 = r1;
       continue;
     }
@@ -1298,8 +1309,8 @@ batch_enqueue(Batch &b, hipStream_t st0, int nchain, const std::vector<BatchItem
     for (int r = r0; r < r1; r++) tbl.push_back(it[r].p->ssv[it[r].v].tbl);
     const SsvVariant &sv = it[r0].p->ssv[it[r0].v];
     HIPCHK(launch_ssv_multi(sv.G, sv.ND, tbl.data(), r1 - r0, s, n, b.zmax + (size_t) r0 * n, n, b.queues + r0,
-                            it[r0].grab, st0));
-    HIPCHK(hipEventRecord(b.evs[it[r0].shape], st0));
+                            it[r0].grab, sst));
+    HIPCHK(hipEventRecord(b.evs[it[r0].shape], sst));
     r0 = r1;
   }
   int status;
@@ -1373,7 +1384,8 @@ p7_gpu_PipelineBatch(P7_GPU_PIPELINE *pli, const P7_GPU_OPROFILE *const *oms, in
   nshape++;
   bool any_chunked = false;
   for (const BatchItem &x : it) any_chunked |= (x.v < 0);
-  const int nchain = std::min(nprof, P7G_BATCH_CHAINS);
+  const bool graph = (flags & P7_GPU_BATCH_GRAPH) != 0;
+  const int nchain = std::min(nprof, graph ? P7G_GRAPH_CHAINS : P7G_BATCH_CHAINS);
 
   // work areas
   const unsigned mask = hits_mask(hits, nprof);
@@ -1386,7 +1398,7 @@ p7_gpu_PipelineBatch(P7_GPU_PIPELINE *pli, const P7_GPU_OPROFILE *const *oms, in
   for (const BatchItem &x : it)
     if (x.v < 0 && (status = work_bnd(pli->w, *x.p, db->bytes)) != p7G_OK) return status;
   uint16_t *bnd = any_chunked ? pli->w.bnd : nullptr;
-  while ((int) b.evs.size() < nshape + nchain) {
+  while ((int) b.evs.size() < nshape + nchain + 1) {
     hipEvent_t e;
     HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     b.evs.push_back(e);
@@ -1409,11 +1421,11 @@ p7_gpu_PipelineBatch(P7_GPU_PIPELINE *pli, const P7_GPU_OPROFILE *const *oms, in
                             bits(x.g.T1m), bits(x.g.T2m), bits(x.g.T2v), bits(x.g.T3f),
                             (unsigned long long) (x.g.do_bias * 2 + x.g.do_max) });
   }
-  const bool eager = (flags & P7_GPU_BATCH_EAGER) != 0;
-  if (! eager && (! b.exec || key != b.key)) {
+  const bool eager = ! graph;
+  if (graph && (! b.exec || key != b.key)) {
     batch_drop_graph(b);
     HIPCHK(hipStreamBeginCapture(st0, hipStreamCaptureModeThreadLocal));
-    status = batch_enqueue(b, st0, nchain, it, nshape, s, n, mask, bnd, pli->w.bnd_cap);
+    status = batch_enqueue(b, st0, nchain, it, nshape, s, n, mask, bnd, pli->w.bnd_cap, false);
     hipGraph_t gr = nullptr;
     const hipError_t ec = hipStreamEndCapture(st0, &gr);
     if (status != p7G_OK) { if (gr) (void) hipGraphDestroy(gr); return status; }
@@ -1424,7 +1436,7 @@ p7_gpu_PipelineBatch(P7_GPU_PIPELINE *pli, const P7_GPU_OPROFILE *const *oms, in
   }
 
   HIPCHK(hipEventRecord(b.t[0], st0));
-  if (eager) { if ((status = batch_enqueue(b, st0, nchain, it, nshape, s, n, mask, bnd, pli->w.bnd_cap)) != p7G_OK) return status; }
+  if (eager) { if ((status = batch_enqueue(b, st0, nchain, it, nshape, s, n, mask, bnd, pli->w.bnd_cap, true)) != p7G_OK) return status; }
   else HIPCHK(hipGraphLaunch(b.exec, st0));
   HIPCHK(hipEventRecord(b.t[1], st0));
   b.last_graph = eager ? 0 : 1;

@@ -18,7 +18,7 @@ OK, FAIL, EOF, EMEM, ENOTFOUND, EFORMAT, EINCOMPAT, EINVAL, ESYS, ERANGE, ENORES
     0, 1, 3, 5, 6, 7, 10, 11, 12, 16, 19, 64
 RNA, DNA, AMINO = 1, 2, 3
 LOCAL, GLOCAL, UNILOCAL = 1, 2, 3
-BATCH_EAGER = 1              # p7_gpu_PipelineBatch flag: launch directly, no graph
+BATCH_EAGER, BATCH_GRAPH = 1, 2   # p7_gpu_PipelineBatch flags: launch directly / capture and replay a HIP graph
 XE, XN, XJ, XC = 0, 1, 2, 3
 MOVE, LOOP = 0, 1
 BM, MM, IM, DM, MD, MI, II, DD = range(8)
@@ -532,11 +532,11 @@ class Pipeline:
         _check(lib.p7_gpu_Pipeline(self.h, om.h, db.h, ctypes.byref(hits)), "Pipeline")
         return out
 
-    def run_batch(self, oms, db, scores=True, eager=False):
+    def run_batch(self, oms, db, scores=True, graph=False):
         """p7_gpu_PipelineBatch: every profile of `oms` over the database in one
         call (one launch per kernel shape for the ungapped pass, concurrent
-        per-profile chains, replayed from a captured HIP graph unless eager).
-        Returns one output dict and one stats dict per profile."""
+        per-profile chains; captured once and replayed as a HIP graph with
+        graph=True). Returns one output dict and one stats dict per profile."""
         n = len(oms)
         arr = (c_void_p * n)(*[om.h for om in oms])
         outs, hits = [], (Hits * n)()
@@ -545,7 +545,7 @@ class Pipeline:
             outs.append(o)
             hits[j] = h
         st = (Stats * n)()
-        _check(lib.p7_gpu_PipelineBatch(self.h, arr, n, db.h, hits, st, BATCH_EAGER if eager else 0), "PipelineBatch")
+        _check(lib.p7_gpu_PipelineBatch(self.h, arr, n, db.h, hits, st, BATCH_GRAPH if graph else 0), "PipelineBatch")
         return outs, [st[j].as_dict() for j in range(n)]
 
     @property

@@ -244,16 +244,20 @@ int   p7_gpu_pipeline_GetStats(const P7_GPU_PIPELINE *pli, P7_GPU_STATS *st);
  * sequence in one call. The ungapped MSV pass of all profiles that share a
  * kernel shape runs as one persistent launch, with a device work queue per
  * profile; each profile's later stages (MSV decision, bias filter, Viterbi,
- * Forward, Backward) follow on one of up to four concurrent streams as soon as
- * its shape's launch is done. The whole sequence is captured once as a HIP
- * graph and replayed while the same profiles, database, thresholds and output
- * arrays are asked for (flags P7_GPU_BATCH_EAGER: launch it directly).
+ * Forward, Backward) follow on one of up to eight concurrent streams as soon
+ * as its shape's launch is done, and the shape launches themselves spread over
+ * the streams. With flags P7_GPU_BATCH_GRAPH the sequence (four streams, the
+ * shape launches on the first) is captured once as a HIP graph and replayed
+ * while the same profiles, database, thresholds and output arrays are asked
+ * for; on ROCm 7.2 the direct launches overlap the streams' work better, so
+ * they are the default.
  * hits and stats are arrays of nprof entries; either may be NULL, and so may
  * any pointer in hits. Each profile's results equal a p7_gpu_Pipeline call
  * with the same pipeline. Counters add to the pipeline's totals; ms_total of
  * every stats entry, and of the totals, is the device time of the whole batch.
  * No counterpart in the reference, whose hmmsearch runs one query at a time. */
-#define P7_GPU_BATCH_EAGER 1
+#define P7_GPU_BATCH_EAGER 1   /* launch directly (the default)          */
+#define P7_GPU_BATCH_GRAPH 2   /* capture as a HIP graph, replay it      */
 int   p7_gpu_PipelineBatch(P7_GPU_PIPELINE *pli, const P7_GPU_OPROFILE *const *oms, int nprof,
                            const P7_GPU_SEQDB *db, P7_GPU_HITS *hits, P7_GPU_STATS *stats, int flags);
 /* 1 if the last batch of this pipeline ran from the captured graph. */

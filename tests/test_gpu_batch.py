@@ -53,9 +53,9 @@ def test_batch_equals_each_profile_alone(setup):
     order = [0, 1, 2, 3, 4, 5, 6, 7, 8, 2, 0]                 # repeats: one profile twice in a batch
     batch = [oms[i] for i in order]
     pli = p7gpu.Pipeline()
-    for eager in (True, False, False):                      # launched directly, captured, replayed
-        outs, sts = pli.run_batch(batch, sdb, eager=eager)
-        assert pli.batch_graphed == (not eager)
+    for graph in (False, True, True):                       # launched directly, captured, replayed
+        outs, sts = pli.run_batch(batch, sdb, graph=graph)
+        assert pli.batch_graphed == graph
         for j, i in enumerate(order):
             same(outs[j], alone[i])
             for k in COUNTERS:
@@ -67,15 +67,15 @@ def test_batch_equals_each_profile_alone(setup):
 
 
 def test_graph_replays_with_other_work_between(setup):
-    """The captured graph replayed several times, with eager work and
-    synchronisations between the replays: every replay equals the eager run."""
+    """The captured graph replayed several times, with other device work and
+    synchronisations between the replays: every replay equals the direct run."""
     sdb, oms, alone, alone_st = setup
     batch = [oms[1], oms[2], oms[7]]
     pli = p7gpu.Pipeline()
-    ref, _ = pli.run_batch(batch, sdb, eager=True)
+    ref, _ = pli.run_batch(batch, sdb)
     other = p7gpu.Pipeline()
     for rep in range(4):
-        outs, _ = pli.run_batch(batch, sdb)
+        outs, _ = pli.run_batch(batch, sdb, graph=True)
         assert pli.batch_graphed
         for j in range(len(batch)):
             same(outs[j], ref[j])

@@ -138,8 +138,8 @@ def test_large_models_in_a_batch(large):
     F = (0.5, 0.2, 0.05)
     alone = [p7gpu.Pipeline(*F).run(om, sdb) for om in oms]
     pli = p7gpu.Pipeline(*F)
-    for eager in (True, False, False):
-        outs, _ = pli.run_batch(oms + oms[:1], sdb, eager=eager)
+    for graph in (False, True, True):
+        outs, _ = pli.run_batch(oms + oms[:1], sdb, graph=graph)
         for j, i in enumerate([0, 1, 2, 0]):
             for k in alone[i]:
                 x, y = outs[j][k], alone[i][k]
