@@ -193,7 +193,8 @@ void k_bias(const DevProfile p, const uint8_t *__restrict__ res, const int64_t *
     // room for L row normalisers and the final sum: one bump of the cursor per wave
     int64_t base = -1;
     if (rowmax) {
-      const int64_t need = valid ? (((int64_t) L + 1 + 3) & ~(int64_t) 3) : 0;   // 16-byte aligned blocks
+      // whole 16-row blocks and the closing sum, in 16-byte aligned pieces
+      const int64_t need = valid ? 16 * (((int64_t) L + 15) >> 4) + 4 : 0;
       int64_t incl = need;
 #pragma unroll
       for (int o = 1; o < 64; o <<= 1) { const int64_t v = __shfl_up(incl, o, 64); if (lane >= o) incl += v; }
@@ -210,39 +211,81 @@ void k_bias(const DevProfile p, const uint8_t *__restrict__ res, const int64_t *
     const float t10 = p.ft1_0, t11 = p.ft1_1;
     float *rm = (base >= 0) ? rowmax + base : nullptr;
     float logsc = 0.0f;
-    float d0, d1, mx;
-    // residues 16 at a time, the next chunk fetched ahead (targets are 16-byte
-    // aligned and followed by at least 16 readable bytes); the odds of row
-    // i+1 are read while row i computes
+    // Row 0 runs the same arithmetic as the others with the identity for the
+    // transitions and (pi0, pi1) for the previous row: ((0 + pi0*1) + pi1*0)
+    // * e0 is exactly pi0 * e0, the reference's first row.
+    float d0 = p.pi0, d1 = p.pi1;
+    float T00 = 1.0f, T01 = 0.0f, T10 = 0.0f, T11 = 1.0f;
+    float f0 = d0, f1 = d1;                      // (d0, d1) after the target's last row
     const uint4 *pc = (const uint4 *) sq;
-    uint4 cur = pc[0], nxt = pc[1];
-    const float2 e0 = eo2[chunk_byte(cur, 0)];
-    float2 en = eo2[L > 1 ? chunk_byte(cur, 1) : 0];
-    d0 = e0.x * p.pi0;
-    d1 = e0.y * p.pi1;
-    mx = 0.0f;
-    mx = (d0 > mx) ? d0 : mx;
-    mx = (d1 > mx) ? d1 : mx;
-    d0 = __fdiv_rn(d0, mx); d1 = __fdiv_rn(d1, mx);
-    if (rm) rm[0] = mx; else logsc += (float) log((double) mx);
-    for (int i = 1; i < L; i++) {
-      const float2 e = en;
-      const int j = i + 1;
-      if (j < L) {
-        if ((j & 15) == 0) { cur = nxt; nxt = pc[(j >> 4) + 1]; }
-        en = eo2[chunk_byte(cur, j)];
+    const int nblk = (L + 15) >> 4;
+    if (rm) {
+      // Straight-line blocks of 16 rows, so that the compiler can count the
+      // memory operations in flight. At the top of block bk every access in
+      // flight was issued a whole block earlier: the block's residue chunk,
+      // and the stores of block bk-2's normalisers. The block then reads its
+      // odds pairs from LDS, requests the next chunk, stores block bk-1's
+      // normalisers (block 0 writes its own slot, rewritten a block later),
+      // and runs its rows. Rows past the end compute values no one reads,
+      // into the target's own whole-block room.
+      uint4 cn = pc[0];
+      float qp[16];
+#pragma unroll
+      for (int u = 0; u < 16; u++) qp[u] = 0.0f;
+      for (int bk = 0; bk < nblk; bk++) {
+        float2 eb[16];
+#pragma unroll
+        for (int u = 0; u < 16; u++) eb[u] = eo2[chunk_byte(cn, u)];
+        cn = pc[min(bk + 1, nblk)];             // the chunk after the target is still readable
+        {
+          float4 *o = (float4 *) (rm + max(bk - 1, 0) * 16);
+          o[0] = make_float4(qp[0], qp[1], qp[2], qp[3]);     o[1] = make_float4(qp[4], qp[5], qp[6], qp[7]);
+          o[2] = make_float4(qp[8], qp[9], qp[10], qp[11]);   o[3] = make_float4(qp[12], qp[13], qp[14], qp[15]);
+        }
+#pragma unroll
+        for (int u = 0; u < 16; u++) {
+          const float2 e = eb[u];
+          float n0 = 0.0f, n1 = 0.0f, mx;
+          n0 += d0 * T00; n0 += d1 * T10; n0 *= e.x;
+          mx = 0.0f; mx = (n0 > mx) ? n0 : mx;
+          n1 += d0 * T01; n1 += d1 * T11; n1 *= e.y;
+          mx = (n1 > mx) ? n1 : mx;
+          d0 = __fdiv_rn(n0, mx); d1 = __fdiv_rn(n1, mx);
+          qp[u] = mx;
+          if (u == 0) { T00 = t00; T01 = t01; T10 = t10; T11 = t11; }
+          const bool last = (bk * 16 + u == L - 1);
+          f0 = last ? d0 : f0; f1 = last ? d1 : f1;
+        }
       }
-      float n0 = 0.0f, n1 = 0.0f;
-      n0 += d0 * t00; n0 += d1 * t10; n0 *= e.x;
-      mx = 0.0f; mx = (n0 > mx) ? n0 : mx;
-      n1 += d0 * t01; n1 += d1 * t11; n1 *= e.y;
-      mx = (n1 > mx) ? n1 : mx;
-      d0 = __fdiv_rn(n0, mx); d1 = __fdiv_rn(n1, mx);
-      if (rm) rm[i] = mx; else logsc += (float) log((double) mx);
+      if (nblk > 0) {
+        float4 *o = (float4 *) (rm + (nblk - 1) * 16);
+        o[0] = make_float4(qp[0], qp[1], qp[2], qp[3]);     o[1] = make_float4(qp[4], qp[5], qp[6], qp[7]);
+        o[2] = make_float4(qp[8], qp[9], qp[10], qp[11]);   o[3] = make_float4(qp[12], qp[13], qp[14], qp[15]);
+      }
+    } else {
+      // no room for the normalisers (pool exhausted): their logs are summed here
+      for (int i = 0; i < L; i++) {
+        const float2 e = eo2[chunk_byte(pc[i >> 4], i)];
+        float n0 = 0.0f, n1 = 0.0f, mx;
+        n0 += d0 * T00; n0 += d1 * T10; n0 *= e.x;
+        mx = 0.0f; mx = (n0 > mx) ? n0 : mx;
+        n1 += d0 * T01; n1 += d1 * T11; n1 *= e.y;
+        mx = (n1 > mx) ? n1 : mx;
+        d0 = __fdiv_rn(n0, mx); d1 = __fdiv_rn(n1, mx);
+        logsc += (float) log((double) mx);
+        T00 = t00; T01 = t01; T10 = t10; T11 = t11;
+      }
+      f0 = d0; f1 = d1;
     }
+    d0 = f0; d1 = f1;
+    // the closing sum; an empty target scores log(pi_end) = log(0) (esl_hmm.c:360-364)
     float sfin = 0.0f;
-    sfin += d0 * 1.0f; sfin += d1 * 1.0f;
-    if (rm) { rm[L] = sfin; defer_off[slot] = base; continue; }   // k_bias_log + k_bias_sum finish it
+    if (L > 0) { sfin += d0 * 1.0f; sfin += d1 * 1.0f; }
+    if (rm) {
+      rm[L] = sfin;                            // after the block stores of the same lane
+      defer_off[slot] = base;                  // k_bias_log + k_bias_sum finish it
+      continue;
+    }
     if (rowmax) defer_off[slot] = -1;
     logsc += (float) log((double) sfin);
     bias_finish(p, r, g, gating, lc, slot, L, logsc);
