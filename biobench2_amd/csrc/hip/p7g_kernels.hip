@@ -524,8 +524,10 @@ void k_vit(const DevProfile p, const uint8_t *__restrict__ res, const int64_t *_
       xC = (int16_t) max(xC + p.xwC_loop, xEs + p.xwE_move);
       xJ = (int16_t) max(xJ + p.xwJ_loop, xEs + p.xwE_loop);
       xB = (int16_t) max(xJ + xmove, xN + xmove);
-      const int Dmax = uni(wave_max_i(max(lo16(Dm), hi16(Dm))));
-      if (Dmax + p.ddbound_w > xB) dd_closure<ND>(D, ddc, z);
+      // the lazy-F test Dmax + ddbound > xB (vitfilter.c:198-199) holds if it
+      // holds on any lane: a ballot instead of a wave maximum
+      const int dml = max(lo16(Dm), hi16(Dm));
+      if (__ballot(dml + p.ddbound_w > xB) != 0ull) dd_closure<ND>(D, ddc, z);
     }
     if (z == 0) {
       float sc;
