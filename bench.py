@@ -189,6 +189,75 @@ def batched_leg(p7gpu, synth, nprof, nseq, reps=3, seed=42):
             "identical_pass_sets": same}
 
 
+def precision_leg(p7gpu, synth, Ms=(2000, 3000, 4000, 5000), nseq=2000, nsub=100, seed=91):
+    """Precision stress: profiles of M = 2000..5000 (calibrated by this library)
+    against sequences of L = 2k..35k with 20% indel-rich homologs. The GPU
+    pipeline runs over every sequence; on a length-stratified subset the
+    oracle's pipeline (its fp32 restatement of the reference) runs too, and
+    for the twelve shortest sequences the FP64 log-space Forward. Reports per
+    model: identical pass sets on the subset, the max and mean
+    |Forward - oracle| and |Backward - oracle| in nats, and the max and mean
+    |Forward - FP64| in nats."""
+    import tempfile
+    import shutil
+    import oracle
+    tmp = tempfile.mkdtemp(prefix="p7p_")
+    rng = np.random.default_rng(seed)
+    out = []
+    for M in Ms:
+        h = synth.random_profile(M, rng, f"prec_M{M}")
+        path = os.path.join(tmp, h.name + ".hmm")
+        synth.write_hmm(path, [h])
+        h.stats = synth.calibrate_with(p7gpu, path, seed=seed)
+        synth.write_hmm(path, [h])
+        hh = synth.read_hmm_probs(path)
+        om = p7gpu.OProfile.from_file(path)
+        t = om.tables()
+        db = synth.make_db(nseq, [hh], frac_homolog=0.2, seed_db=seed + M, seed_plant=seed + M + 1,
+                           median=8000.0, sigma=0.8, lo=2000, hi=35000, indel_rich=True)
+        sdb = p7gpu.SeqDB(db.residues, db.offsets, db.lengths)
+        pli = p7gpu.Pipeline()
+        t0 = time.perf_counter()
+        got = pli.run(om, sdb)
+        t_gpu = time.perf_counter() - t0
+        order = np.argsort(db.lengths)
+        sub_idx = np.sort(order[np.linspace(0, nseq - 1, nsub).astype(int)])
+        sub = db.subset(sub_idx)
+        prof, keep = oracle.make_profile(t, oracle.bias_hmm(t), t["evparam"], Lmax=int(db.lengths.max()) + 1)
+        t0 = time.perf_counter()
+        ref = oracle.pipeline_db(prof, sub.residues, sub.offsets, sub.lengths, nthreads=min(16, os.cpu_count() or 1))
+        t_orc = time.perf_counter() - t0
+        g = {k: v[sub_idx] for k, v in got.items()}
+        same = bool(np.array_equal(g["stage"], ref["stage"]))
+        errs = {}
+        for k in ("fwdsc", "bcksc"):
+            m = ~np.isnan(ref[k]) & ~np.isnan(g[k])
+            d = np.abs(g[k][m].astype(np.float64) - ref[k][m])
+            errs[k] = (float(d.max()) if d.size else None, float(d.mean()) if d.size else None, int(m.sum()))
+        # the FP64 log-space Forward of the twelve shortest sequences against the
+        # device Forward of the same sequences (each length-configured as in the pipeline)
+        short = order[:12]
+        fsc, _ = p7gpu.forward_batch(om, sdb, short.astype(np.int64))
+        e64 = []
+        for j, i in enumerate(short):
+            om.gm.reconfig_length(int(db.lengths[i]))
+            tsc, rsc, xsc = om.gm.scores()
+            e64.append(abs(float(fsc[j]) - oracle.forward_log64(db.seq(int(i)), tsc, rsc, xsc)))
+        st = pli.stats()
+        out.append({"M": M, "nseq": nseq, "nres": int(db.nres), "subset": nsub, "gpu_s": round(t_gpu, 3),
+                    "oracle_subset_s": round(t_orc, 1), "pass_sets_identical_on_subset": same,
+                    "n_past_fwd": st["n_past_fwd"],
+                    "fwd_vs_oracle_fp32_max": errs["fwdsc"][0], "fwd_vs_oracle_fp32_mean": errs["fwdsc"][1],
+                    "fwd_compared": errs["fwdsc"][2],
+                    "bck_vs_oracle_fp32_max": errs["bcksc"][0], "bck_vs_oracle_fp32_mean": errs["bcksc"][1],
+                    "bck_compared": errs["bcksc"][2],
+                    "fwd_vs_fp64_max": max(e64) if e64 else None,
+                    "fwd_vs_fp64_mean": float(np.mean(e64)) if e64 else None, "fp64_compared": len(e64)})
+        log(f"precision: {out[-1]}")
+    shutil.rmtree(tmp, ignore_errors=True)
+    return out
+
+
 def calibrate_copy(mib=256, reps=3):
     """Plain device-to-device copies of a known size: what FETCH_SIZE and WRITE_SIZE
     report for them gives the factor from counter KiB to HBM bytes."""
@@ -216,11 +285,18 @@ def main():
     ap.add_argument("--serve-slots", type=int, default=4)
     ap.add_argument("--batch-profiles", type=int, default=64, help="profiles in the batched leg (0: skip it)")
     ap.add_argument("--batch-nseq", type=int, default=500_000, help="database sequences of the batched leg")
+    ap.add_argument("--precision", action="store_true",
+                    help="only run the precision stress (M = 2000..5000, L = 2k..35k) and print its JSON line")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run three plain 256 MiB device copies (to scale the FETCH_SIZE/WRITE_SIZE counters)")
     args = ap.parse_args()
     if args.calibrate_copy:
         calibrate_copy()
+        return
+    if args.precision:
+        from biobench2_amd import p7gpu, synth
+        p7gpu.init(0)
+        print(json.dumps({"precision_stress": precision_leg(p7gpu, synth)}), flush=True)
         return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: --gpus {args.gpus} is not supported by this build "
