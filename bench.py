@@ -285,6 +285,7 @@ def main():
     ap.add_argument("--serve-slots", type=int, default=4)
     ap.add_argument("--batch-profiles", type=int, default=64, help="profiles in the batched leg (0: skip it)")
     ap.add_argument("--batch-nseq", type=int, default=500_000, help="database sequences of the batched leg")
+    ap.add_argument("--no-tune", action="store_true", help="skip the tuner (keep the built-in ungapped-pass shape)")
     ap.add_argument("--precision", action="store_true",
                     help="only run the precision stress (M = 2000..5000, L = 2k..35k) and print its JSON line")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -314,10 +315,12 @@ def main():
     log(f"db: {db.nseq} sequences, {db.nres} residues, generated in {time.time() - t0:.1f} s")
     sdb = p7gpu.SeqDB(db.residues, db.offsets, db.lengths)
     pli = p7gpu.Pipeline()
-    t0 = time.perf_counter()
-    p7gpu.autotune([om], sdb)                     # the ungapped pass's shape for M = 400 on this device
-    lanes, grab, us, _ = p7gpu.tune_get(M)
-    log(f"tuner: {lanes} lanes per target, grab {grab} ({time.perf_counter() - t0:.2f} s; us per shape {us.tolist()})")
+    if not args.no_tune:
+        t0 = time.perf_counter()
+        p7gpu.autotune([om], sdb)                 # the ungapped pass's shape for M = 400 on this device
+        lanes, grab, us, _ = p7gpu.tune_get(M)
+        log(f"tuner: {lanes} lanes per target, grab {grab} ({time.perf_counter() - t0:.2f} s; us per shape {us.tolist()})")
+    lanes, grab, _, _ = p7gpu.tune_get(M)
 
     for w in range(args.warmup):
         pli.run(om, sdb, scores=False)
