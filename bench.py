@@ -51,17 +51,27 @@ def cpu_baseline(om, db, M, seconds_target=15.0, max_threads=16):
                       f"(oracle/p7_oracle.c restatement, scalar C), {nthreads} threads, {dt:.1f} s"}
 
 
-def serving_leg(p7gpu, synth, sdb, nq, rate, nslots, seed=11):
-    """Queries arrive as a Poisson stream against the resident database and are
-    served by p7_gpu_server_* (several in flight); the same queries run one by
-    one through a single pipeline give the plain per-query rate. Reports
-    p50/p99 latency, aggregate GCUPS of both, and whether every query's pass
-    set is identical in both."""
+def serving_leg(p7gpu, synth, nseq, nq, rate, nslots, seed=11):
+    """A resident database of nseq i.i.d. sequences, 0.5% of them planted
+    homologs of a hidden set of 1000 profiles (M ~ lognormal, median 250,
+    clipped to [50, 3000]). Queries, drawn uniformly from 16 calibrated
+    members of the set and 4 decoy profiles outside it, arrive as a Poisson
+    stream and are served by p7_gpu_server_* (several in flight); the same
+    queries run one by one through a single pipeline give the plain
+    per-query rate. Reports p50/p99 latency, aggregate GCUPS of both, and
+    whether every query's pass set is identical in both."""
     import tempfile
     import threading
     tmp = tempfile.mkdtemp(prefix="p7q_")
-    paths = synth.query_family(p7gpu, tmp, nprof=8, seed=seed)
-    oms = [p7gpu.OProfile.from_file(pth) for pth in paths]
+    t0 = time.perf_counter()
+    fam, qpaths, dpaths = synth.serving_family(p7gpu, tmp, nfam=1000, nquery=16, ndecoy=4, seed=seed)
+    db = synth.make_db(nseq, fam, frac_homolog=0.005, seed_db=seed + 1, seed_plant=seed + 2)
+    del fam
+    sdb = p7gpu.SeqDB(db.residues, db.offsets, db.lengths)
+    log(f"serving leg: resident database of {db.nseq} sequences, {db.nres} residues "
+        f"({time.perf_counter() - t0:.1f} s to generate and upload)")
+    del db
+    oms = [p7gpu.OProfile.from_file(pth) for pth in qpaths + dpaths]
     Ms = [om.tables()["M"] for om in oms]
     rng = np.random.default_rng(seed)
     which = rng.integers(0, len(oms), nq)
@@ -110,7 +120,9 @@ def serving_leg(p7gpu, synth, sdb, nq, rate, nslots, seed=11):
     import shutil
     shutil.rmtree(tmp, ignore_errors=True)
     lat = np.array(lat)
-    return {"queries": nq, "arrival_rate_qps": rate, "slots": slots, "profiles_M": Ms,
+    return {"nseq": sdb.nseq, "nres": sdb.nres, "homologs": "0.5% planted from a hidden set of 1000 profiles",
+            "query_profiles": "16 members of the set + 4 decoys, drawn uniformly",
+            "queries": nq, "arrival_rate_qps": rate, "slots": slots, "profiles_M": Ms,
             "p50_ms": round(float(np.percentile(lat, 50)), 3), "p99_ms": round(float(np.percentile(lat, 99)), 3),
             "gcups_served_stream": round(cells / t_served / 1e9, 1),
             "gcups_served_saturated": round(cells / t_sat / 1e9, 1),
@@ -280,8 +292,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "k_ssv_traffic.json"),
                     help="JSON with the HBM bytes of one k_ssv launch, from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE")
-    ap.add_argument("--serve-queries", type=int, default=32, help="queries in the serving leg (0: skip it)")
-    ap.add_argument("--serve-rate", type=float, default=100.0, help="Poisson arrival rate of the serving leg, queries/s")
+    ap.add_argument("--serve-queries", type=int, default=100, help="queries in the serving leg (0: skip it)")
+    ap.add_argument("--serve-rate", type=float, default=20.0, help="Poisson arrival rate of the serving leg, queries/s")
+    ap.add_argument("--serve-nseq", type=int, default=10_000_000, help="sequences of the serving leg's resident database")
     ap.add_argument("--serve-slots", type=int, default=4)
     ap.add_argument("--batch-profiles", type=int, default=64, help="profiles in the batched leg (0: skip it)")
     ap.add_argument("--batch-nseq", type=int, default=500_000, help="database sequences of the batched leg")
@@ -383,7 +396,8 @@ def main():
         line["batched"] = batched_leg(p7gpu, synth, args.batch_profiles, args.batch_nseq)
     if args.serve_queries > 0:
         log("serving leg ...")
-        line["serving"] = serving_leg(p7gpu, synth, sdb, args.serve_queries, args.serve_rate, args.serve_slots)
+        line["serving"] = serving_leg(p7gpu, synth, args.serve_nseq, args.serve_queries, args.serve_rate,
+                                      args.serve_slots)
     if not args.no_cpu:
         log("cpu baseline ...")
         line["cpu_baseline"] = cpu_baseline(om, db, M, args.cpu_seconds)

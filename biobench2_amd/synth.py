@@ -376,3 +376,26 @@ def batch_family(gpu, dirpath, nprof=64, Mmin=50, Mmax=2000, seed=42):
         write_hmm(path, [h])
         paths.append(path)
     return paths
+
+
+def serving_family(gpu, dirpath, nfam=1000, nquery=16, ndecoy=4, median_M=250, sigma=0.6, Mmin=50, Mmax=3000,
+                   seed=11):
+    """A hidden family set of nfam profiles with M ~ lognormal(median, sigma)
+    clipped to [Mmin, Mmax], nquery of them drawn as queries, plus ndecoy
+    profiles from the same generator that are not in the set. The queries and
+    decoys are calibrated and written to dirpath. Returns (family, query
+    paths, decoy paths); the family's profiles are the ones to plant."""
+    import os
+    rng = np.random.default_rng(seed)
+    Ms = np.clip(np.rint(rng.lognormal(math.log(median_M), sigma, nfam + ndecoy)), Mmin, Mmax).astype(int)
+    fam = [random_profile(int(M), rng, f"fam{i}_M{M}") for i, M in enumerate(Ms[:nfam])]
+    dec = [random_profile(int(M), rng, f"decoy{i}_M{M}") for i, M in enumerate(Ms[nfam:])]
+    pick = np.sort(rng.choice(nfam, size=min(nquery, nfam), replace=False))
+    paths = []
+    for j, h in enumerate([fam[i] for i in pick] + dec):
+        path = os.path.join(dirpath, h.name + ".hmm")
+        write_hmm(path, [h])
+        h.stats = calibrate_with(gpu, path, seed=seed + j)
+        write_hmm(path, [h])
+        paths.append(path)
+    return fam, paths[:len(pick)], paths[len(pick):]
