@@ -101,10 +101,16 @@ void k_msv_full(const DevProfile p, const int *__restrict__ tbl, int tbl_dwords,
         E  = max2(E, tv);
         mp[d] = tv;
       }
-      const int xE = uni(wave_max_i(max(lo16(E), hi16(E))));
-      if (xE + p.bias >= 255) { overflow = true; break; }
-      xJ = max(xJ, max(xE - p.tec, 0));
-      xB = max(max(p.base, xJ) - tjbm, 0);
+      // xJ (and so xB) only changes, and the overflow test only fires, when
+      // the row's xE exceeds min(xJ + tec, 254 - bias): a ballot on the lanes'
+      // maxima decides whether the wave maximum is needed at all
+      const int el = max(lo16(E), hi16(E));
+      if (__ballot(el > min(xJ + p.tec, 254 - p.bias)) != 0ull) {
+        const int xE = uni(wave_max_i(el));
+        if (xE + p.bias >= 255) { overflow = true; break; }
+        xJ = max(xJ, max(xE - p.tec, 0));
+        xB = max(max(p.base, xJ) - tjbm, 0);
+      }
     }
     if (z == 0) xj[slot] = (int16_t) (overflow ? -1 : xJ);
   }
@@ -467,6 +473,7 @@ void k_vit(const DevProfile p, const uint8_t *__restrict__ res, const int64_t *_
   }
   DDConst<ND> ddc;
   dd_setup<ND>(ddc, tr + VT_DDS*rowd, z);
+  const bool loops0 = (p.xwN_loop == 0 && p.xwC_loop == 0 && p.xwJ_loop == 0);
   const int64_t n = *count;
 
   int32_t *queue = (int32_t *) count + P7G_QOFF;
@@ -518,12 +525,20 @@ void k_vit(const DevProfile p, const uint8_t *__restrict__ res, const int64_t *_
 #pragma unroll
       for (int d = 0; d < ND; d++) { D[d] = adds2(d ? M[d-1] : ms0, VTR(tMDS, VT_MDS, d)); Dm = max2(Dm, D[d]); }
 #undef VTR
-      const int xEs = uni(wave_max_i(max(lo16(xE), hi16(xE))));
-      if (xEs >= 32767) { status = 16; break; }
-      xN = (int16_t) (xN + p.xwN_loop);
-      xC = (int16_t) max(xC + p.xwC_loop, xEs + p.xwE_move);
-      xJ = (int16_t) max(xJ + p.xwJ_loop, xEs + p.xwE_loop);
-      xB = (int16_t) max(xJ + xmove, xN + xmove);
+      // With the N, C and J loops at zero (as the conversion sets them,
+      // p7_oprofile.c:888-897) the specials only change, and the overflow test
+      // only fires, when the row's xE exceeds th; that holds if it holds on
+      // any lane, so a ballot stands in for the wave maximum on other rows.
+      const int xel = max(lo16(xE), hi16(xE));
+      const int th = loops0 ? min(min(xC - p.xwE_move, xJ - p.xwE_loop), 32766) : -(1 << 30);
+      if (__ballot(xel > th) != 0ull) {
+        const int xEs = uni(wave_max_i(xel));
+        if (xEs >= 32767) { status = 16; break; }
+        xN = (int16_t) (xN + p.xwN_loop);
+        xC = (int16_t) max(xC + p.xwC_loop, xEs + p.xwE_move);
+        xJ = (int16_t) max(xJ + p.xwJ_loop, xEs + p.xwE_loop);
+        xB = (int16_t) max(xJ + xmove, xN + xmove);
+      }
       // the lazy-F test Dmax + ddbound > xB (vitfilter.c:198-199) holds if it
       // holds on any lane: a ballot instead of a wave maximum
       const int dml = max(lo16(Dm), hi16(Dm));
