@@ -421,6 +421,33 @@ __device__ __forceinline__ void dd_closure(int (&D)[ND], const DDConst<ND> &c, i
   for (int d = 0; d < ND; d++) { Q = adds2(Q, c.t[d]); D[d] = max2(P[d], Q); }
 }
 
+// The same closure, found by passes that stop at the fixed point (the lazy-F
+// passes of vitfilter.c:200-225, over half-lanes instead of SSE segments).
+// Each half of a lane is first closed on its own; then the carries into the
+// halves (the previous lane's last position into the low half, the lane's own
+// low half into its high half) are applied while any of them raises a first
+// position. A chain closed inside each half is raised somewhere only if it is
+// raised at its first position, so that one comparison per half decides.
+// Rows whose D->D paths reach past DD_LAZY_PASSES half-lane boundaries finish
+// with the scan above, which gives the same closure from any partly closed row.
+#define DD_LAZY_PASSES 2
+template <int ND>
+__device__ __forceinline__ void dd_closure_lazy(int (&D)[ND], const DDConst<ND> &c, int z)
+{
+#pragma unroll
+  for (int d = 1; d < ND; d++) D[d] = max2(D[d], adds2(D[d-1], c.t[d]));
+  for (int pass = 0; ; pass++) {
+    const int nb = dppi<WAVE_SHR1>((int) NEG_PAIR, D[ND-1]);       // lane 0: no lane before, -infinity
+    int Q = adds2((int) __builtin_amdgcn_alignbit((unsigned) D[ND-1], (unsigned) nb, 16), c.t[0]);
+    const int m0 = max2(D[0], Q);
+    if (__ballot(m0 != D[0]) == 0ull) return;
+    if (pass == DD_LAZY_PASSES) { dd_closure<ND>(D, c, z); return; }
+    D[0] = m0;
+#pragma unroll
+    for (int d = 1; d < ND; d++) { Q = adds2(Q, c.t[d]); D[d] = max2(D[d], Q); }
+  }
+}
+
 // Where a list kernel finds its profile tables (template argument TM):
 //   TM_LDS   the whole table in LDS, transitions pinned in registers
 //   TM_PIN   the table in global memory (L2-resident), transitions pinned
@@ -542,7 +569,7 @@ void k_vit(const DevProfile p, const uint8_t *__restrict__ res, const int64_t *_
       // the lazy-F test Dmax + ddbound > xB (vitfilter.c:198-199) holds if it
       // holds on any lane: a ballot instead of a wave maximum
       const int dml = max(lo16(Dm), hi16(Dm));
-      if (__ballot(dml + p.ddbound_w > xB) != 0ull) dd_closure<ND>(D, ddc, z);
+      if (__ballot(dml + p.ddbound_w > xB) != 0ull) dd_closure_lazy<ND>(D, ddc, z);
     }
     if (z == 0) {
       float sc;
