@@ -51,15 +51,16 @@ def cpu_baseline(om, db, M, seconds_target=15.0, max_threads=16):
                       f"(oracle/p7_oracle.c restatement, scalar C), {nthreads} threads, {dt:.1f} s"}
 
 
-def serving_leg(p7gpu, synth, nseq, nq, rate, nslots, seed=11):
+def serving_leg(p7gpu, synth, nseq, nq, rate, nslots, seed=11, tune=True):
     """A resident database of nseq i.i.d. sequences, 0.5% of them planted
     homologs of a hidden set of 1000 profiles (M ~ lognormal, median 250,
     clipped to [50, 3000]). Queries, drawn uniformly from 16 calibrated
     members of the set and 4 decoy profiles outside it, arrive as a Poisson
     stream and are served by p7_gpu_server_* (several in flight); the same
     queries run one by one through a single pipeline give the plain
-    per-query rate. Reports p50/p99 latency, aggregate GCUPS of both, and
-    whether every query's pass set is identical in both."""
+    per-query rate. Both use the tuner's shapes for the queries' length
+    classes, measured first. Reports p50/p99 latency, aggregate GCUPS of
+    both, and whether every query's pass set is identical in both."""
     import tempfile
     import threading
     tmp = tempfile.mkdtemp(prefix="p7q_")
@@ -73,6 +74,8 @@ def serving_leg(p7gpu, synth, nseq, nq, rate, nslots, seed=11):
     del db
     oms = [p7gpu.OProfile.from_file(pth) for pth in qpaths + dpaths]
     Ms = [om.tables()["M"] for om in oms]
+    if tune:
+        p7gpu.autotune(oms, sdb)                 # as a server would load its device's tuning file
     rng = np.random.default_rng(seed)
     which = rng.integers(0, len(oms), nq)
     arrivals = np.cumsum(rng.exponential(1.0 / rate, nq))
@@ -397,7 +400,7 @@ def main():
     if args.serve_queries > 0:
         log("serving leg ...")
         line["serving"] = serving_leg(p7gpu, synth, args.serve_nseq, args.serve_queries, args.serve_rate,
-                                      args.serve_slots)
+                                      args.serve_slots, tune=not args.no_tune)
     if not args.no_cpu:
         log("cpu baseline ...")
         line["cpu_baseline"] = cpu_baseline(om, db, M, args.cpu_seconds)

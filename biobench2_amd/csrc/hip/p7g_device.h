@@ -25,7 +25,7 @@ struct SsvVariant {
   int   *tbl;
   size_t dwords;
 };
-#define P7G_SSV_NVAR 3   // G = 16, 32, 64
+#define P7G_SSV_NVAR 4   // G = 16, 32, 64, 8
 #define P7G_LDS_MAX (160 * 1024)   // LDS of one workgroup on gfx950
 #define P7G_SSV_MAXCHUNK 3
 

@@ -144,8 +144,11 @@ dev_profile_build(P7_GPU_OPROFILE *om, int Lmax_need, struct p7g_dev_profile_s *
     if (x >= Kp || k < 1 || k > M) return -32768;
     return (int) om->bias_b - (int) om->rbv[(size_t) x * M + k - 1];
   };
-  // the shapes of the ungapped pass this model fits: 16, 32 and 64 lanes per target
-  const int ssvG[P7G_SSV_NVAR] = { 16, 32, 64 };
+  // the shapes of the ungapped pass this model fits: 16, 32, 64 and 8 lanes per
+  // target. With 8 lanes two targets share each 16-lane DPP row, so the first
+  // lane of the second one takes the last position of the first: the shape is
+  // sized to leave that position past M, where every row holds the floor.
+  const int ssvG[P7G_SSV_NVAR] = { 16, 32, 64, 8 };
   std::vector<int> ssv[P7G_SSV_NVAR];
   for (int v = 0; v < P7G_SSV_NVAR; v++) {
     SsvVariant &sv = p.ssv[v];
@@ -153,7 +156,8 @@ dev_profile_build(P7_GPU_OPROFILE *om, int Lmax_need, struct p7g_dev_profile_s *
     int ND = -1;
     if (G == 16)      ND = (M <= 1024) ? std::max(1, cdiv(M, 32)) : -1;
     else if (G == 32) ND = (M <= 2048) ? pick(ssv32_set, (int) (sizeof(ssv32_set)/sizeof(int)), cdiv(M, 64)) : -1;
-    else              ND = pick(ssv64_set, (int) (sizeof(ssv64_set)/sizeof(int)), cdiv(M, 128));
+    else if (G == 64) ND = pick(ssv64_set, (int) (sizeof(ssv64_set)/sizeof(int)), cdiv(M, 128));
+    else              ND = (M <= 511) ? cdiv(M + 1, 16) : -1;
     if (ND > 0 && (size_t) NR * G * ND * 4 + 256 > P7G_LDS_MAX) ND = -1;   // the table (and a word) must fit in LDS
     if (ND < 0) { sv.G = 0; sv.ND = 0; sv.dwords = 0; continue; }
     sv.G = G; sv.ND = ND;
@@ -194,8 +198,10 @@ dev_profile_build(P7_GPU_OPROFILE *om, int Lmax_need, struct p7g_dev_profile_s *
           }
     }
   }
-  // 16 lanes up to M = 1024, else 64; chunked beyond one LDS table
-  p.ssv_def = (p.ssv[0].G != 0) ? 0 : (p.ssv[2].G != 0) ? 2 : -1;
+  // 8 lanes up to M = 352, 16 up to M = 1024, else 64; chunked beyond one LDS
+  // table (on MI355X the 8-lane shape measured 1.03-1.36x faster than 16 lanes
+  // for M = 40..300, even at 400-500)
+  p.ssv_def = (p.ssv[3].G != 0 && M <= 352) ? 3 : (p.ssv[0].G != 0) ? 0 : (p.ssv[2].G != 0) ? 2 : -1;
   p.mf_ND = pick(vf_nd_set, (int) (sizeof(vf_nd_set)/sizeof(int)), std::max(1, cdiv(cdiv(M, 64), 2)));
   if (p.mf_ND < 0) { delete d; p7g_set_error("model too long for the device MSV stage (M=%d)", M); return p7G_EINVAL; }
   p.mf_dwords = (size_t) NR * 64 * p.mf_ND;
@@ -309,10 +315,10 @@ dev_profile_build(P7_GPU_OPROFILE *om, int Lmax_need, struct p7g_dev_profile_s *
   size_t sz[] = { p.ssv[0].dwords * 4, p.ssv[1].dwords * 4, p.ssv[2].dwords * 4, p.mf_dwords * 4, p.vf_dwords * 4,
                   p.fb_floats * 4, L1, L1 * 2, L1 * 4, L1 * 4, L1 * 4, L1 * 4, L1 * 4, eo.size() * 4,
                   p.ssv_nchunk > 0 ? p.ssvc[0].dwords * 4 : 0, p.ssv_nchunk > 1 ? p.ssvc[1].dwords * 4 : 0,
-                  p.ssv_nchunk > 2 ? p.ssvc[2].dwords * 4 : 0 };
+                  p.ssv_nchunk > 2 ? p.ssvc[2].dwords * 4 : 0, p.ssv[3].dwords * 4 };
   const void *src[] = { ssv[0].data(), ssv[1].data(), ssv[2].data(), mf.data(), vf.data(), fb.data(), lc.tjb_b,
                         lc.xw_move, lc.xf_move, lc.xf_loop, lc.nullsc, lc.flen_a, lc.flen_b, eo.data(),
-                        ssvc[0].data(), ssvc[1].data(), ssvc[2].data() };
+                        ssvc[0].data(), ssvc[1].data(), ssvc[2].data(), ssv[3].data() };
   const int NB = (int) (sizeof(sz) / sizeof(sz[0]));
   size_t offs[20], total = 0;
   for (int i = 0; i < NB; i++) { offs[i] = total; total += (sz[i] + 255) & ~(size_t) 255; }
@@ -331,7 +337,8 @@ dev_profile_build(P7_GPU_OPROFILE *om, int Lmax_need, struct p7g_dev_profile_s *
     return p7G_EDEVICE;
   }
   p.blob = blob;
-  for (int v = 0; v < P7G_SSV_NVAR; v++) p.ssv[v].tbl = p.ssv[v].G ? (int *) (blob + offs[v]) : nullptr;
+  const int ssv_blob[P7G_SSV_NVAR] = { 0, 1, 2, 17 };          // blob parts of the variants' tables
+  for (int v = 0; v < P7G_SSV_NVAR; v++) p.ssv[v].tbl = p.ssv[v].G ? (int *) (blob + offs[ssv_blob[v]]) : nullptr;
   p.mf_tbl = (int *) (blob + offs[3]);
   p.vf_tbl = (int *) (blob + offs[4]);  p.fb_tbl = (float *) (blob + offs[5]);
   p.tjb = (uint8_t *) (blob + offs[6]); p.xw_move = (int16_t *) (blob + offs[7]);
@@ -683,9 +690,10 @@ static DevSeqs seqs_of(const P7_GPU_SEQDB *db)
  * Tuned shape of the ungapped MSV pass
  *****************************************************************/
 #define SSV_GRAB_DEFAULT 4          // target groups per work-queue atomic
-// A model's length class, M rounded up to a multiple of 32, fixes every shape
-// of the ungapped pass the model fits, so the models of one class share one
-// measurement.
+#define TUNE_MARGIN 0.97f           // a tuned shape must beat the built-in one by 3%
+// A model's length class, M rounded up to a multiple of 32, fixes the 16-,
+// 32- and 64-lane shapes of the ungapped pass (and the 8-lane one to within
+// one dword per lane), so the models of one class share one measurement.
 static int tune_class(int M) { return cdiv(M, 32); }
 struct TuneEntry { int v, grab; float us[P7G_SSV_NVAR]; };
 static std::mutex g_tunelock;
@@ -1563,16 +1571,27 @@ p7_gpu_Autotune(const P7_GPU_OPROFILE *const *oms, int nprof, const P7_GPU_SEQDB
     for (int v = 0; v < P7G_SSV_NVAR; v++) if (p->ssv[v].G) minpos = std::min(minpos, 2 * p->ssv[v].G * p->ssv[v].ND);
     TuneEntry e;
     e.v = p->ssv_def; e.grab = SSV_GRAB_DEFAULT;
-    float best = 1e30f;
+    int vgrab[P7G_SSV_NVAR];
     for (int v = 0; v < P7G_SSV_NVAR; v++) {
-      e.us[v] = -1.0f;
+      e.us[v] = -1.0f; vgrab[v] = SSV_GRAB_DEFAULT;
       if (! p->ssv[v].G || 2 * p->ssv[v].G * p->ssv[v].ND > 2 * minpos) continue;
       for (int gi = 0; gi < 4; gi++) {
         float us;
         if ((status = tune_time(*p, v, grabs[gi], s, d_slots, ns, zmax, queue, ts.st, ts.e0, ts.e1, &us)) != p7G_OK) return status;
-        if (e.us[v] < 0.0f || us < e.us[v]) e.us[v] = us;
-        if (us < best) { best = us; e.v = v; e.grab = grabs[gi]; }
+        if (e.us[v] < 0.0f || us < e.us[v]) { e.us[v] = us; vgrab[v] = grabs[gi]; }
       }
+    }
+    // another shape replaces the built-in one only when it is clearly faster:
+    // the sample runs last about a millisecond, and a full pass has been seen
+    // to rank shapes within a few percent of each other the other way round
+    int bv = -1;
+    for (int v = 0; v < P7G_SSV_NVAR; v++)
+      if (e.us[v] > 0.0f && (bv < 0 || e.us[v] < e.us[bv])) bv = v;
+    if (bv >= 0) {
+      const int dv = p->ssv_def;
+      const bool keep = dv >= 0 && e.us[dv] > 0.0f && e.us[bv] >= TUNE_MARGIN * e.us[dv];
+      e.v = keep ? dv : bv;
+      e.grab = vgrab[e.v];
     }
     {
       std::lock_guard<std::mutex> lk(g_tunelock);
@@ -1586,8 +1605,8 @@ p7_gpu_Autotune(const P7_GPU_OPROFILE *const *oms, int nprof, const P7_GPU_SEQDB
 extern "C" int
 p7_gpu_tune_Get(int M, int *ret_lanes, int *ret_grab, float *ret_us)
 {
-  const int G[P7G_SSV_NVAR] = { 16, 32, 64 };
-  int lanes = (M <= 1024) ? 16 : 64, grab = SSV_GRAB_DEFAULT, tuned = 0;
+  const int G[P7G_SSV_NVAR] = { 16, 32, 64, 8 };
+  int lanes = (M <= 352) ? 8 : (M <= 1024) ? 16 : 64, grab = SSV_GRAB_DEFAULT, tuned = 0;
   if (ret_us) for (int v = 0; v < P7G_SSV_NVAR; v++) ret_us[v] = -1.0f;
   {
     std::lock_guard<std::mutex> lk(g_tunelock);
@@ -1618,12 +1637,12 @@ p7_gpu_tune_Save(const char *path)
   if ((status = arch_name(arch, sizeof(arch))) != p7G_OK) return status;
   FILE *fp = fopen(path, "w");
   if (! fp) { p7g_set_error("cannot write %s", path); return p7G_EFORMAT; }
-  fprintf(fp, "p7gpu-tune 1 %s\n", arch);
+  fprintf(fp, "p7gpu-tune 2 %s\n", arch);
   {
     std::lock_guard<std::mutex> lk(g_tunelock);
     for (const auto &kv : g_tune)
-      fprintf(fp, "%d %d %d %.3f %.3f %.3f\n", kv.first, kv.second.v, kv.second.grab,
-              kv.second.us[0], kv.second.us[1], kv.second.us[2]);
+      fprintf(fp, "%d %d %d %.3f %.3f %.3f %.3f\n", kv.first, kv.second.v, kv.second.grab,
+              kv.second.us[0], kv.second.us[1], kv.second.us[2], kv.second.us[3]);
   }
   if (fclose(fp) != 0) { p7g_set_error("cannot write %s", path); return p7G_EFORMAT; }
   return p7G_OK;
@@ -1639,7 +1658,8 @@ p7_gpu_tune_Load(const char *path)
   FILE *fp = fopen(path, "r");
   if (! fp) { p7g_set_error("cannot open %s", path); return p7G_ENOTFOUND; }
   int ver = 0;
-  if (fscanf(fp, "p7gpu-tune %d %255s", &ver, farch) != 2 || ver != 1) {
+  // version 1 files hold the times of three shapes (16, 32, 64 lanes), version 2 of four (and 8)
+  if (fscanf(fp, "p7gpu-tune %d %255s", &ver, farch) != 2 || (ver != 1 && ver != 2)) {
     fclose(fp); p7g_set_error("%s is not a tuning file", path); return p7G_EFORMAT;
   }
   if (strcmp(arch, farch) != 0) {
@@ -1647,11 +1667,14 @@ p7_gpu_tune_Load(const char *path)
   }
   std::map<int, TuneEntry> got;
   int cls, v, grab;
-  float u0, u1, u2;
+  float u[4] = { -1.0f, -1.0f, -1.0f, -1.0f };
+  const int nfields = (ver == 1) ? 6 : 7;
   int nf;
-  while ((nf = fscanf(fp, "%d %d %d %f %f %f", &cls, &v, &grab, &u0, &u1, &u2)) == 6) {
-    if (cls < 1 || v < 0 || v >= P7G_SSV_NVAR || (grab != 1 && grab != 2 && grab != 4 && grab != 8)) { nf = -2; break; }
-    TuneEntry e; e.v = v; e.grab = grab; e.us[0] = u0; e.us[1] = u1; e.us[2] = u2;
+  while ((nf = (ver == 1) ? fscanf(fp, "%d %d %d %f %f %f", &cls, &v, &grab, &u[0], &u[1], &u[2])
+                          : fscanf(fp, "%d %d %d %f %f %f %f", &cls, &v, &grab, &u[0], &u[1], &u[2], &u[3])) == nfields) {
+    if (cls < 1 || v < 0 || v >= nfields - 3 || (grab != 1 && grab != 2 && grab != 4 && grab != 8)) { nf = -2; break; }
+    TuneEntry e; e.v = v; e.grab = grab;
+    for (int i = 0; i < P7G_SSV_NVAR; i++) e.us[i] = u[i];
     got[cls] = e;
   }
   fclose(fp);

@@ -9,10 +9,11 @@
 // maximum, below). The pass records max Z per target; k_msv_classify decides
 // from it which targets are finished, overflow, or need the full recurrence.
 //
-// A target is spread over G = 16, 32 or 64 lanes (4, 2 or 1 targets per wave),
-// lane z holding model positions z*2ND+1 .. z*2ND+2ND as ND int16 pairs (dword
-// d: positions d and d + ND). Which G serves a model best is measured by the
-// runtime's tuner; every shape gives the same result.
+// A target is spread over G = 8, 16, 32 or 64 lanes (8, 4, 2 or 1 targets per
+// wave), lane z holding model positions z*2ND+1 .. z*2ND+2ND as ND int16 pairs
+// (dword d: positions d and d + ND). Which G serves a model best is measured by
+// the runtime's tuner; every shape gives the same result. Small models gain
+// from 8 lanes: twice the positions per lane for the same per-row overhead.
 //
 // The kernel serves one or many profiles of one shape: each workgroup stages
 // a profile's table in LDS, its waves take groups of targets from that
@@ -48,11 +49,14 @@ __device__ __forceinline__ int zmax2(int acc, int a) { return zmax3(acc, a, a); 
 // The row shift's lane move: lane z of a target takes lane z-1's last dword;
 // the first lane of each target keeps `fill`. G = 32 needs two moves: lanes
 // 16 and 48 take lanes 15 and 47 by row_bcast:15, then row_shr:1 fills the
-// rest of each 16-lane row, leaving lanes 0 and 32 with the fill.
+// rest of each 16-lane row, leaving lanes 0 and 32 with the fill. With G = 8
+// the second target of a 16-lane row takes the first one's last position: the
+// 8-lane tables leave that position past M (score -32768), so it is at the
+// floor on every row, which is the fill.
 template <int G>
 __device__ __forceinline__ int lane_in(int fill, int v)
 {
-  if (G == 16) return __builtin_amdgcn_update_dpp(fill, v, ROW_SHR1, 0xf, 0xf, false);
+  if (G == 8 || G == 16) return __builtin_amdgcn_update_dpp(fill, v, ROW_SHR1, 0xf, 0xf, false);
   if (G == 64) return __builtin_amdgcn_update_dpp(fill, v, WAVE_SHR1, 0xf, 0xf, false);
   const int b = __builtin_amdgcn_update_dpp(fill, v, ROW_BCAST15, 0xa, 0xf, false);
   return __builtin_amdgcn_update_dpp(b, v, ROW_SHR1, 0xf, 0xf, false);
@@ -264,6 +268,9 @@ void k_ssv(const SsvJobs jobs, int njobs, int tbl_dwords,
 /*****************************************************************
  * Launchers
  *****************************************************************/
+#define SSV_ND8(X) X(8,1) X(8,2) X(8,3) X(8,4) X(8,5) X(8,6) X(8,7) X(8,8) X(8,9) X(8,10) X(8,11) X(8,12) X(8,13) \
+  X(8,14) X(8,15) X(8,16) X(8,17) X(8,18) X(8,19) X(8,20) X(8,21) X(8,22) X(8,23) X(8,24) X(8,25) X(8,26) \
+  X(8,27) X(8,28) X(8,29) X(8,30) X(8,31) X(8,32)
 #define SSV_ND16(X) X(16,1) X(16,2) X(16,3) X(16,4) X(16,5) X(16,6) X(16,7) X(16,8) X(16,9) X(16,10) X(16,11) \
   X(16,12) X(16,13) X(16,14) X(16,15) X(16,16) X(16,17) X(16,18) X(16,19) X(16,20) X(16,21) X(16,22) X(16,23) \
   X(16,24) X(16,25) X(16,26) X(16,27) X(16,28) X(16,29) X(16,30) X(16,31) X(16,32)
@@ -283,7 +290,7 @@ static void *ssv_kernel(int G, int ND, bool bnd)
     return nullptr;
   }
   switch (G * 256 + ND) {
-    SSV_ND16(SSV_CASE) SSV_ND32(SSV_CASE) SSV_ND64(SSV_CASE)
+    SSV_ND8(SSV_CASE) SSV_ND16(SSV_CASE) SSV_ND32(SSV_CASE) SSV_ND64(SSV_CASE)
   }
   return nullptr;
 }

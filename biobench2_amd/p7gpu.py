@@ -570,9 +570,9 @@ def autotune(oms, db):
 
 
 def tune_get(M):
-    """(lanes per target, queue grab, best us per shape [16, 32, 64 lanes], tuned?) for models of length M."""
+    """(lanes per target, queue grab, best us per shape [16, 32, 64, 8 lanes], tuned?) for models of length M."""
     lanes, grab = c_int(), c_int()
-    us = np.zeros(3, np.float32)
+    us = np.zeros(4, np.float32)
     t = lib.p7_gpu_tune_Get(int(M), ctypes.byref(lanes), ctypes.byref(grab), _ptr(us, c_float))
     return lanes.value, grab.value, us, bool(t)
 

@@ -265,8 +265,8 @@ int   p7_gpu_pipeline_BatchGraphed(const P7_GPU_PIPELINE *pli);
 
 /* ---- tuner of the MSV stage's ungapped pass --------------------------------
  * For the length class of each model (M rounded up to a multiple of 32), times
- * every shape of the ungapped pass the model fits -- 16, 32 or 64 lanes per
- * target, i.e. 4, 2 or 1 targets per wavefront, each with its
+ * every shape of the ungapped pass the model fits -- 8, 16, 32 or 64 lanes per
+ * target, i.e. 8, 4, 2 or 1 targets per wavefront, each with its
  * positions-per-lane tile -- with work-queue grabs of 1, 2, 4 and 8 target
  * groups, on every k-th sequence of db (so the sample keeps the length mix),
  * and keeps the fastest for this device. Shapes that compute more than twice
@@ -274,9 +274,9 @@ int   p7_gpu_pipeline_BatchGraphed(const P7_GPU_PIPELINE *pli);
  * server use the kept choice from then on; no result depends on it. */
 int   p7_gpu_Autotune(const P7_GPU_OPROFILE *const *oms, int nprof, const P7_GPU_SEQDB *db);
 /* The choice for models of length M: lanes per target, queue grab, and the
- * best time of each shape (16, 32, 64 lanes; -1 if not tried). Returns 1 if
- * M's class was measured, 0 if the built-in default applies. */
-int   p7_gpu_tune_Get(int M, int *ret_lanes, int *ret_grab, float *ret_us3);
+ * best time of each shape (ret_us[4]: 16, 32, 64, 8 lanes; -1 if not tried).
+ * Returns 1 if M's class was measured, 0 if the built-in default applies. */
+int   p7_gpu_tune_Get(int M, int *ret_lanes, int *ret_grab, float *ret_us);
 void  p7_gpu_tune_Clear(void);
 /* A text file of the measured choices, for the device's architecture. */
 int   p7_gpu_tune_Save(const char *path);

@@ -1,7 +1,7 @@
 """Batches (p7_gpu_PipelineBatch) and the tuner of the MSV stage's ungapped
 pass: many profiles in one call, from a captured HIP graph or launched
 directly, give each profile's p7_gpu_Pipeline result bit for bit; every
-shape of the ungapped pass (16, 32 or 64 lanes per target, any queue grab)
+shape of the ungapped pass (8, 16, 32 or 64 lanes per target, any queue grab)
 gives the same results; the tuner's choices survive a save/load round trip."""
 import numpy as np
 import pytest
@@ -102,16 +102,16 @@ def forced_tuning(tmp_path, entries):
     p7gpu.tune_save(path)                      # header with this device's architecture
     with open(path, "a") as f:
         for cls, v, grab in entries:
-            f.write(f"{cls} {v} {grab} -1 -1 -1\n")
+            f.write(f"{cls} {v} {grab} -1 -1 -1 -1\n")
     p7gpu.tune_load(path)
 
 
-@pytest.mark.parametrize("v", [0, 1, 2])
+@pytest.mark.parametrize("v", [0, 1, 2, 3])
 @pytest.mark.parametrize("grab", [1, 8])
 def test_every_shape_of_the_ungapped_pass_gives_the_same_results(setup, tmp_path, v, grab):
     sdb, oms, alone, alone_st = setup
-    lanes = (16, 32, 64)[v]
-    limit = {16: 1024, 32: 2048, 64: 5120}[lanes]
+    lanes = (16, 32, 64, 8)[v]
+    limit = {8: 511, 16: 1024, 32: 2048, 64: 5120}[lanes]
     idx = [i for i, om in enumerate(oms) if om.tables()["M"] <= limit]
     try:
         forced_tuning(tmp_path, [((oms[i].tables()["M"] + 31) // 32, v, grab) for i in idx])
@@ -134,8 +134,8 @@ def test_tuner_measures_keeps_results_and_round_trips(setup, tmp_path):
         for i in (0, 2, 3, 6):
             M = oms[i].tables()["M"]
             lanes, grab, us, tuned = p7gpu.tune_get(M)
-            assert tuned and lanes in (16, 32, 64) and grab in (1, 2, 4, 8)
-            assert us[(16, 32, 64).index(lanes)] > 0.0
+            assert tuned and lanes in (16, 32, 64, 8) and grab in (1, 2, 4, 8)
+            assert us[(16, 32, 64, 8).index(lanes)] > 0.0
             assert us.max() > 0.0
             same(p7gpu.Pipeline().run(oms[i], sdb), alone[i])
         assert not p7gpu.tune_get(300)[3]                    # class of M = 300 not measured
@@ -149,8 +149,15 @@ def test_tuner_measures_keeps_results_and_round_trips(setup, tmp_path):
             a = p7gpu.tune_get(oms[i].tables()["M"])
             assert a[:2] == b[:2] and a[3]
             assert np.allclose(a[2], b[2], atol=1e-3)
+        arch = open(path).readline().split()[2]
+        old = tmp_path / "v1.txt"                        # the three-shape format of version 1 still loads
+        old.write_text(f"p7gpu-tune 1 {arch}\n{(oms[2].tables()['M'] + 31) // 32} 1 2 5.0 4.0 6.0\n")
+        p7gpu.tune_clear()
+        p7gpu.tune_load(old)
+        lanes, grab, us, tuned = p7gpu.tune_get(oms[2].tables()["M"])
+        assert (lanes, grab, tuned) == (32, 2, True) and us[3] == -1.0
         bad = tmp_path / "bad.txt"
-        bad.write_text("p7gpu-tune 1 some_other_device\n13 0 4 1 1 1\n")
+        bad.write_text("p7gpu-tune 2 some_other_device\n13 0 4 1 1 1 1\n")
         with pytest.raises(p7gpu.P7GpuError) as e:
             p7gpu.tune_load(bad)
         assert e.value.status == p7gpu.EINCOMPAT
