@@ -269,7 +269,8 @@ int   p7_gpu_pipeline_BatchGraphed(const P7_GPU_PIPELINE *pli);
  * target, i.e. 8, 4, 2 or 1 targets per wavefront, each with its
  * positions-per-lane tile -- with work-queue grabs of 1, 2, 4 and 8 target
  * groups, on every k-th sequence of db (so the sample keeps the length mix),
- * and keeps the fastest for this device. Shapes that compute more than twice
+ * and keeps the fastest for this device unless the built-in shape is within
+ * 3% of it. Shapes that compute more than twice
  * the positions of the tightest one are not tried. Pipelines, batches and the
  * server use the kept choice from then on; no result depends on it. */
 int   p7_gpu_Autotune(const P7_GPU_OPROFILE *const *oms, int nprof, const P7_GPU_SEQDB *db);
