@@ -50,6 +50,14 @@ __device__ __forceinline__ float dppf(float old, float v)
 {
   return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(__builtin_bit_cast(int, old), __builtin_bit_cast(int, v), CTRL, RM, 0xf, false));
 }
+// The same move for an additive identity: lanes without a source lane read 0
+// (bound_ctrl), which lets the compiler fold the move into the consuming
+// add or multiply-add as its DPP operand.
+template <int CTRL, int RM = 0xf>
+__device__ __forceinline__ float dppz(float v)
+{
+  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), CTRL, RM, 0xf, true));
+}
 __device__ __forceinline__ float readlanef(float v, int l) { return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), l)); }
 enum { ROW_SHR1 = 0x111, ROW_SHR2 = 0x112, ROW_SHR4 = 0x114, ROW_SHR8 = 0x118,
        ROW_SHL1 = 0x101, ROW_SHL2 = 0x102, ROW_SHL4 = 0x104, ROW_SHL8 = 0x108,
@@ -72,12 +80,12 @@ __device__ __forceinline__ int wave_max_i(int v)
 }
 __device__ __forceinline__ float wave_sum_f(float v)
 {
-  v += dppf<ROW_SHR1>(0.0f, v);
-  v += dppf<ROW_SHR2>(0.0f, v);
-  v += dppf<ROW_SHR4>(0.0f, v);
-  v += dppf<ROW_SHR8>(0.0f, v);
-  v += dppf<ROW_BCAST15, 0xa>(0.0f, v);
-  v += dppf<ROW_BCAST31, 0xc>(0.0f, v);
+  v += dppz<ROW_SHR1>(v);
+  v += dppz<ROW_SHR2>(v);
+  v += dppz<ROW_SHR4>(v);
+  v += dppz<ROW_SHR8>(v);
+  v += dppz<ROW_BCAST15, 0xa>(v);
+  v += dppz<ROW_BCAST31, 0xc>(v);
   return readlanef(v, 63);
 }
 __device__ __forceinline__ int   uni(int v)   { return __builtin_amdgcn_readfirstlane(v); }

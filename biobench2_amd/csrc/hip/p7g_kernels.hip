@@ -637,10 +637,10 @@ __device__ __forceinline__ void affine_fwd_scan(float (&D)[B], const float (&a)[
   float C = 0.0f;
 #pragma unroll
   for (int j = 0; j < B; j++) C = D[j] + a[j] * C;
-  C = C + k.w1 * dppf<ROW_SHR1>(0.0f, C);
-  C = C + k.w2 * dppf<ROW_SHR2>(0.0f, C);
-  C = C + k.w4 * dppf<ROW_SHR4>(0.0f, C);
-  C = C + k.w8 * dppf<ROW_SHR8>(0.0f, C);
+  C = C + k.w1 * dppz<ROW_SHR1>(C);
+  C = C + k.w2 * dppz<ROW_SHR2>(C);
+  C = C + k.w4 * dppz<ROW_SHR4>(C);
+  C = C + k.w8 * dppz<ROW_SHR8>(C);
   {
     const float c1 = readlanef(C, 15);
     const float c2 = k.pa * c1 + readlanef(C, 31);
@@ -649,7 +649,7 @@ __device__ __forceinline__ void affine_fwd_scan(float (&D)[B], const float (&a)[
     const float rc = (row == 0) ? 0.0f : (row == 1) ? c1 : (row == 2) ? c2 : c3;
     C = C + k.prow * rc;
   }
-  const float cin = dppf<WAVE_SHR1>(0.0f, C);
+  const float cin = dppz<WAVE_SHR1>(C);
   float cur = cin;
 #pragma unroll
   for (int j = 0; j < B; j++) { cur = D[j] + a[j] * cur; D[j] = cur; }
@@ -664,10 +664,10 @@ __device__ __forceinline__ void affine_bck_scan(float (&D)[B], A a, const AffCon
   float C = 0.0f;
 #pragma unroll
   for (int j = B-1; j >= 0; j--) C = D[j] + a(j) * C;
-  C = C + k.w1 * dppf<ROW_SHL1>(0.0f, C);
-  C = C + k.w2 * dppf<ROW_SHL2>(0.0f, C);
-  C = C + k.w4 * dppf<ROW_SHL4>(0.0f, C);
-  C = C + k.w8 * dppf<ROW_SHL8>(0.0f, C);
+  C = C + k.w1 * dppz<ROW_SHL1>(C);
+  C = C + k.w2 * dppz<ROW_SHL2>(C);
+  C = C + k.w4 * dppz<ROW_SHL4>(C);
+  C = C + k.w8 * dppz<ROW_SHL8>(C);
   {
     const float c2 = readlanef(C, 48);
     const float c1 = k.pa * c2 + readlanef(C, 32);
@@ -676,7 +676,7 @@ __device__ __forceinline__ void affine_bck_scan(float (&D)[B], A a, const AffCon
     const float rc = (row == 3) ? 0.0f : (row == 2) ? c2 : (row == 1) ? c1 : c0;
     C = C + k.prow * rc;
   }
-  const float cin = dppf<WAVE_SHL1>(0.0f, C);
+  const float cin = dppz<WAVE_SHL1>(C);
   float cur = cin;
 #pragma unroll
   for (int j = B-1; j >= 0; j--) { cur = D[j] + a(j) * cur; D[j] = cur; }
@@ -747,7 +747,7 @@ void k_fwd(const DevProfile p, const uint8_t *__restrict__ res, const int64_t *_
       const float *e = tbl + (size_t) x * rowf + z;
       const float *trr = tr + (PIN ? 0 : opaque_zero());
 #define FTR(A, T, j) (PIN ? A[j] : trr[(T) * rowf + (j) * 64])
-      const float iL = dppf<WAVE_SHR1>(0.0f, I[B-1]), dL = dppf<WAVE_SHR1>(0.0f, D[B-1]);
+      const float iL = dppz<WAVE_SHR1>(I[B-1]), dL = dppz<WAVE_SHR1>(D[B-1]);
       float xEl = 0.0f;
 #pragma unroll
       for (int j = B-1; j >= 0; j--) {
@@ -758,7 +758,7 @@ void k_fwd(const DevProfile p, const uint8_t *__restrict__ res, const int64_t *_
         M[j] = sv; I[j] = In;
         xEl += sv;
       }
-      const float mS = dppf<WAVE_SHR1>(0.0f, M[B-1]);
+      const float mS = dppz<WAVE_SHR1>(M[B-1]);
 #pragma unroll
       for (int j = 0; j < B; j++) D[j] = (j ? M[j-1] : mS) * FTR(tMDS, VT_MDS, j);
 #undef FTR
@@ -787,7 +787,7 @@ void k_fwd(const DevProfile p, const uint8_t *__restrict__ res, const int64_t *_
         totscale = (float) ((double) totscale + log((double) xE));
         xE = 1.0f;
       }
-      mL = dppf<WAVE_SHR1>(0.0f, M[B-1]);
+      mL = dppz<WAVE_SHR1>(M[B-1]);
       if (scl && z == 0) scl[i] = scale;
       if (xmx_out && z == 0) {
         float *xr = xmx_out + (size_t) i * 6;
