@@ -62,6 +62,9 @@ lib.orc_generic_viterbi.argtypes = [pu8, c_int, c_int, pf, pf, pf, c_int, pf]
 lib.orc_forward_log64.argtypes = [pu8, c_int, c_int, pf, pf, pf, pd]
 lib.orc_backward_log64.argtypes = [pu8, c_int, c_int, pf, pf, pf, pd]
 lib.orc_brute_force.argtypes = [pu8, c_int, c_int, pf, pf, pf, pd, pd]
+lib.orc_brute_posterior.argtypes = [pu8, c_int, c_int, pf, pf, pf, pd, pd, pd]
+lib.orc_domain_decoding.argtypes = [c_int, pf, pf, c_int, c_float, c_float, c_float, pf, pf, pf]
+lib.orc_domain_regions.argtypes = [c_int, pf, pf, pf, c_float, c_float, c_float, pi, pi, pi, c_int]
 lib.orc_gumbel_surv.argtypes = [c_double, c_double, c_double]
 lib.orc_gumbel_surv.restype = c_double
 lib.orc_exp_surv.argtypes = [c_double, c_double, c_double]
@@ -181,6 +184,48 @@ def brute_force(residues, tsc, rsc, xsc):
     lib.orc_brute_force(_p(dsq, c_uint8), dsq.size - 2, tsc.shape[0], _p(tsc, c_float), _p(rsc, c_float),
                         _p(xsc, c_float), ctypes.byref(f), ctypes.byref(v))
     return f.value, v.value
+
+
+def brute_posterior(residues, tsc, rsc, xsc):
+    """(pB[0..L], pE[0..L], pNJC[0..L]) by enumeration of every state path (L <= 8): the posterior
+    probabilities of entering B after r residues, leaving through E after r residues, and residue i
+    being emitted by N, J or C."""
+    dsq = dsq_of(residues)
+    L = dsq.size - 2
+    pB, pE, pN = (np.zeros(L + 1) for _ in range(3))
+    st = lib.orc_brute_posterior(_p(dsq, c_uint8), L, tsc.shape[0], _p(tsc, c_float), _p(rsc, c_float),
+                                 _p(xsc, c_float), _p(pB, c_double), _p(pE, c_double), _p(pN, c_double))
+    assert st == 0
+    return pB, pE, pN
+
+
+def domain_decoding(fwd_xmx, bck_xmx, bck_own, xf):
+    """(status, btot, etot, mocc) of p7_DomainDecoding from the parsers' special rows ([L+1, 6],
+    E N J B C SCALE) and the N/J/C loop odds of xf[4][2]."""
+    fx = np.ascontiguousarray(fwd_xmx, np.float32).reshape(-1)
+    bx = np.ascontiguousarray(bck_xmx, np.float32).reshape(-1)
+    L = fx.size // 6 - 1
+    xfa = np.asarray(xf, np.float32).reshape(4, 2)
+    btot, etot, mocc = (np.zeros(L + 1, np.float32) for _ in range(3))
+    st = lib.orc_domain_decoding(L, _p(fx, c_float), _p(bx, c_float), int(bck_own), xfa[1, 1], xfa[2, 1], xfa[3, 1],
+                                 _p(btot, c_float), _p(etot, c_float), _p(mocc, c_float))
+    return st, btot, etot, mocc
+
+
+RT1, RT2, RT3 = 0.25, 0.10, 0.20          # p7_domaindef.c:109-111
+
+
+def domain_regions(btot, etot, mocc, rt1=RT1, rt2=RT2, rt3=RT3):
+    """[(i, j, multidomain)] of the region loop of p7_domaindef_ByPosteriorHeuristics."""
+    L = btot.size - 1
+    cap = L // 2 + 2
+    ri, rj, rm = (np.zeros(cap, np.int32) for _ in range(3))
+    n = lib.orc_domain_regions(L, _p(np.ascontiguousarray(btot, np.float32), c_float),
+                               _p(np.ascontiguousarray(etot, np.float32), c_float),
+                               _p(np.ascontiguousarray(mocc, np.float32), c_float), rt1, rt2, rt3,
+                               _p(ri, c_int), _p(rj, c_int), _p(rm, c_int), cap)
+    assert n <= cap
+    return [(int(ri[k]), int(rj[k]), bool(rm[k])) for k in range(n)]
 
 
 def gumbel_surv(x, mu, lam):

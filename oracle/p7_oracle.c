@@ -18,6 +18,11 @@
  *                      in double over the same profile (generic_fwdback.c:50-253
  *                      structure, exact log-sum-exp instead of the table)
  *   orc_brute_force    enumeration of every state path (itest_brute.c idea)
+ *   orc_brute_posterior the same enumeration, weighting each path's B and E
+ *                      visits and N/J/C emissions by its probability
+ *   orc_domain_decoding HMMER/src/impl_sse/decoding.c:166-197 (fp32)
+ *   orc_domain_regions HMMER/src/p7_domaindef.c:409-489 (the region loop) and
+ *                      :534-547 (is_multidomain_region)
  *   orc_pipeline_db    the stage gates of p7_Pipeline (p7_pipeline.c:687-731)
  * The Forward/Backward restatements keep the reference's four-lane striping,
  * operation order and FTZ/DAZ mode, so they reproduce its fp32 values.
@@ -580,7 +585,15 @@ orc_backward_log64(const uint8_t *dsq, int L, int M, const float *tsc, const flo
  * N/C/J loops (score 0 relative to the null). Returns the log of the summed
  * path probabilities and the best path score.
  *****************************************************************/
-typedef struct { const uint8_t *dsq; int L, M; const float *tsc, *rsc, *xsc; double sum; double best; } BRUTE;
+#define BRUTE_LMAX 8
+typedef struct {
+  const uint8_t *dsq; int L, M; const float *tsc, *rsc, *xsc; double sum; double best;
+  /* posterior mode: visits of B (by row before the entry) and E (by row), and
+   * residues emitted by N, J or C, along the path being walked; their
+   * probability-weighted sums over all complete paths */
+  int post; int nB[BRUTE_LMAX+1], nE[BRUTE_LMAX+1], njc[BRUTE_LMAX+1];
+  double wsum, wB[BRUTE_LMAX+1], wE[BRUTE_LMAX+1], wN[BRUTE_LMAX+1];
+} BRUTE;
 
 static void
 brute_walk(BRUTE *b, int st, int k, int i, double sc)
@@ -591,19 +604,24 @@ brute_walk(BRUTE *b, int st, int k, int i, double sc)
   if (sc == -INFINITY) return;
   switch (st) {
   case 0: /* N */
-    if (i < L) brute_walk(b, 0, 0, i+1, sc + xsc[XN*2 + GL]);
-    brute_walk(b, 1, 0, i, sc + xsc[XN*2 + GM]);
+    if (i < L) { b->njc[i+1]++; brute_walk(b, 0, 0, i+1, sc + xsc[XN*2 + GL]); b->njc[i+1]--; }
+    b->nB[i]++; brute_walk(b, 1, 0, i, sc + xsc[XN*2 + GM]); b->nB[i]--;
     break;
   case 6: /* J */
-    if (i < L) brute_walk(b, 6, 0, i+1, sc + xsc[XJ*2 + GL]);
-    brute_walk(b, 1, 0, i, sc + xsc[XJ*2 + GM]);
+    if (i < L) { b->njc[i+1]++; brute_walk(b, 6, 0, i+1, sc + xsc[XJ*2 + GL]); b->njc[i+1]--; }
+    b->nB[i]++; brute_walk(b, 1, 0, i, sc + xsc[XJ*2 + GM]); b->nB[i]--;
     break;
   case 7: /* C */
-    if (i < L) brute_walk(b, 7, 0, i+1, sc + xsc[XC*2 + GL]);
+    if (i < L) { b->njc[i+1]++; brute_walk(b, 7, 0, i+1, sc + xsc[XC*2 + GL]); b->njc[i+1]--; }
     if (i == L) {
       double t = sc + xsc[XC*2 + GM];
       b->sum = lse(b->sum, t);
       if (t > b->best) b->best = t;
+      if (b->post) {
+        const double w = exp(t);
+        b->wsum += w;
+        for (int r = 0; r <= L; r++) { b->wB[r] += w * b->nB[r]; b->wE[r] += w * b->nE[r]; b->wN[r] += w * b->njc[r]; }
+      }
     }
     break;
   case 1: /* B -> M_k (local entry) */
@@ -613,7 +631,7 @@ brute_walk(BRUTE *b, int st, int k, int i, double sc)
     }
     break;
   case 2: /* M_k */
-    brute_walk(b, 5, 0, i, sc);                                   /* M_k -> E (local exit, 0) */
+    b->nE[i]++; brute_walk(b, 5, 0, i, sc); b->nE[i]--;           /* M_k -> E (local exit, 0) */
     if (k < M) {
       if (i < L) {
         const float *rs = b->rsc + (size_t) b->dsq[i+1] * (M+1) * 2;
@@ -631,7 +649,7 @@ brute_walk(BRUTE *b, int st, int k, int i, double sc)
     }
     break;
   case 4: /* D_k */
-    brute_walk(b, 5, 0, i, sc);                                   /* D_k -> E (local exit, 0) */
+    b->nE[i]++; brute_walk(b, 5, 0, i, sc); b->nE[i]--;           /* D_k -> E (local exit, 0) */
     if (k < M) {
       if (i < L) {
         const float *rs = b->rsc + (size_t) b->dsq[i+1] * (M+1) * 2;
@@ -652,12 +670,91 @@ orc_brute_force(const uint8_t *dsq, int L, int M, const float *tsc, const float 
                 double *ret_fwd, double *ret_vit)
 {
   BRUTE b;
+  if (L > BRUTE_LMAX) return 11;
+  memset(&b, 0, sizeof(b));
   b.dsq = dsq; b.L = L; b.M = M; b.tsc = tsc; b.rsc = rsc; b.xsc = xsc;
   b.sum = -INFINITY; b.best = -INFINITY;
   brute_walk(&b, 0, 0, 0, 0.0);
   *ret_fwd = b.sum;
   *ret_vit = b.best;
   return 0;
+}
+
+/* Posterior expectations by enumeration: pB[r] = P(the path enters B after r
+ * residues), r = 0..L-1; pE[r] = P(it leaves the core model through E after r
+ * residues), r = 1..L; pNJC[i] = P(residue i is emitted by N, J or C). These
+ * are what the special-state decoding computes from Forward x Backward. */
+int
+orc_brute_posterior(const uint8_t *dsq, int L, int M, const float *tsc, const float *rsc, const float *xsc,
+                    double *pB, double *pE, double *pNJC)
+{
+  BRUTE b;
+  if (L > BRUTE_LMAX) return 11;
+  memset(&b, 0, sizeof(b));
+  b.dsq = dsq; b.L = L; b.M = M; b.tsc = tsc; b.rsc = rsc; b.xsc = xsc;
+  b.sum = -INFINITY; b.best = -INFINITY; b.post = 1;
+  brute_walk(&b, 0, 0, 0, 0.0);
+  for (int r = 0; r <= L; r++) { pB[r] = b.wB[r] / b.wsum; pE[r] = b.wE[r] / b.wsum; pNJC[r] = b.wN[r] / b.wsum; }
+  return 0;
+}
+
+/*****************************************************************
+ * Domain decoding (impl_sse/decoding.c:166-197): from the special states of
+ * the Forward and Backward parsers, btot[i] / etot[i] = expected number of
+ * B / E visits up to row i, mocc[i] = probability that residue i is emitted
+ * by the core model. fx, bx: (L+1) rows of [E N J B C SCALE] (p7X_* order).
+ * bown: the Backward parser had to use its own scale factors.
+ *****************************************************************/
+enum { X_E = 0, X_N, X_J, X_B, X_C, X_SC };
+
+int
+orc_domain_decoding(int L, const float *fx, const float *bx, int bown, float nloop, float jloop, float cloop,
+                    float *btot, float *etot, float *mocc)
+{
+  float sp = 1.0 / bx[X_N];
+  float njcp;
+  btot[0] = 0.0f; etot[0] = 0.0f; mocc[0] = 0.0f;
+  for (int i = 1; i <= L; i++) {
+    btot[i] = btot[i-1] + (fx[(i-1)*6 + X_B] * bx[(i-1)*6 + X_B] * fx[(i-1)*6 + X_SC] * sp);
+    if (bown) sp *= fx[(i-1)*6 + X_SC] / bx[(i-1)*6 + X_SC];
+    etot[i] = etot[i-1] + (fx[i*6 + X_E] * bx[i*6 + X_E] * fx[i*6 + X_SC] * sp);
+    njcp  = fx[(i-1)*6 + X_N] * bx[i*6 + X_N] * nloop * sp;
+    njcp += fx[(i-1)*6 + X_J] * bx[i*6 + X_J] * jloop * sp;
+    njcp += fx[(i-1)*6 + X_C] * bx[i*6 + X_C] * cloop * sp;
+    mocc[i] = 1. - njcp;
+  }
+  return isinf(sp) ? 16 : 0;
+}
+
+/* The region loop of p7_domaindef_ByPosteriorHeuristics (p7_domaindef.c:
+ * 409-489) with is_multidomain_region (:534-547): regions i..j where the
+ * core-model occupancy rises to rt1, bounded where occupancy not explained
+ * by B (start) or E (end) entries falls below rt2; a region is "multidomain"
+ * if some split point expects at least rt3 domains on both sides. Writes up
+ * to maxreg regions; returns how many there are. */
+int
+orc_domain_regions(int L, const float *btot, const float *etot, const float *mocc, float rt1, float rt2, float rt3,
+                   int *ri, int *rj, int *rmulti, int maxreg)
+{
+  int i = -1, triggered = 0, n = 0;
+  for (int j = 1; j <= L; j++) {
+    if (! triggered) {
+      if      (mocc[j] - (btot[j] - btot[j-1]) < rt2) i = j;
+      else if (i == -1)                              i = j;
+      if      (mocc[j] >= rt1)                       triggered = 1;
+    } else if (mocc[j] - (etot[j] - etot[j-1]) < rt2) {
+      float max = -1.0f;
+      for (int z = i; z <= j; z++) {
+        const float a = etot[z] - etot[i-1], b = btot[j] - btot[z-1];
+        const float en = (a < b) ? a : b;
+        if (en > max) max = en;
+      }
+      if (n < maxreg) { ri[n] = i; rj[n] = j; rmulti[n] = (max >= rt3); }
+      n++;
+      i = -1; triggered = 0;
+    }
+  }
+  return n;
 }
 
 /*****************************************************************

@@ -1,0 +1,103 @@
+"""The oracle's domain decoding and region heuristics (p7_DomainDecoding,
+HMMER/src/impl_sse/decoding.c:166-197; the region loop of
+p7_domaindef_ByPosteriorHeuristics, HMMER/src/p7_domaindef.c:409-489, with
+is_multidomain_region :534-547), pinned three ways:
+
+  * on tiny models, btot / etot / mocc from the fp32 parsers' special states
+    equal the posterior expectations found by enumerating every state path;
+  * btot[L] = etot[L] (every B entry has its E exit), for every target;
+  * tutorial/fn3.out: fn3 vs 7LESS_DROME reports exp = 9.4 expected domains,
+    which is btot[L] of this decoding (p7_domaindef.c:406, p7_pipeline.c:817).
+
+The region loop is pinned by the same tutorial run: its regions are the
+envelopes fn3.out reports. No GPU."""
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+from biobench2_amd import p7gpu, seqio, synth
+from conftest import golden, random_seqs
+from test_oracle import configured, tiny_models
+
+
+def parsers(s, t):
+    st, fsc, fx, _ = oracle.fwd_parser(s, t)
+    assert st == 0
+    st2, bsc, bx, own = oracle.bck_parser(s, t, fx)
+    assert st2 == 0
+    return fx, bx, own
+
+
+def test_decoding_equals_path_enumeration(tmp_path):
+    rng = np.random.default_rng(23)
+    worst = 0.0
+    for path in tiny_models(str(tmp_path)):
+        for L in (1, 2, 3, 4):
+            om, gm = configured(path, 0, L)
+            t = om.tables()
+            tsc, rsc, xsc = gm.scores()
+            for s in random_seqs(rng, [L] * 3):
+                fx, bx, own = parsers(s, t)
+                st, btot, etot, mocc = oracle.domain_decoding(fx, bx, own, t["xf"])
+                assert st == 0
+                pB, pE, pN = oracle.brute_posterior(s, tsc, rsc, xsc)
+                ref_b = np.concatenate([[0.0], np.cumsum(pB[:L])])
+                ref_e = np.concatenate([[0.0], np.cumsum(pE[1:L + 1])])
+                ref_m = 1.0 - pN
+                for got, ref in ((btot, ref_b), (etot, ref_e), (mocc[1:], ref_m[1:])):
+                    err = float(np.max(np.abs(got.astype(np.float64) - ref)))
+                    worst = max(worst, err)
+                    assert err < 2e-5, (path, L, got, ref)
+    assert worst > 0.0          # the two computations are independent, not the same numbers
+
+
+def test_a_planted_mistake_in_the_decoding_is_caught(tmp_path):
+    """The enumeration check fails for a decoding that drops the scale factor."""
+    path = tiny_models(str(tmp_path))[2]
+    om, gm = configured(path, 0, 3)
+    t = om.tables()
+    tsc, rsc, xsc = gm.scores()
+    s = random_seqs(np.random.default_rng(1), [3])[0]
+    fx, bx, own = parsers(s, t)
+    bad = bx.copy()
+    bad[0, 1] *= 2.0                            # Backward's N at row 0: the total it normalises by
+    _, btot, _, _ = oracle.domain_decoding(fx, bad, own, t["xf"])
+    pB, _, _ = oracle.brute_posterior(s, tsc, rsc, xsc)
+    assert abs(btot[3] - pB[:3].sum()) > 1e-2
+
+
+def test_expected_b_and_e_visits_agree():
+    rng = np.random.default_rng(4)
+    for name, which in (("fn3.hmm", 0), ("minifam", 0), ("minifam", 2)):
+        om = p7gpu.OProfile.from_file(golden(name), which)
+        for L in (10, 300, 1500):
+            om.reconfig_length(L)
+            t = om.tables()
+            for s in random_seqs(rng, [L] * 2):
+                fx, bx, own = parsers(s, t)
+                st, btot, etot, mocc = oracle.domain_decoding(fx, bx, own, t["xf"])
+                assert st == 0 and abs(btot[L] - etot[L]) <= 1e-4 * max(1.0, btot[L])
+                assert np.all(mocc >= -1e-5) and np.all(mocc <= 1.0 + 1e-5)
+
+
+def test_fn3_tutorial_expected_domains():
+    (name, s), = seqio.read_uniprot(golden("7LESS_DROME"))
+    L = len(s)
+    om = p7gpu.OProfile.from_file(golden("fn3.hmm"))
+    om.reconfig_length(L)
+    t = om.tables()
+    fx, bx, own = parsers(s, t)
+    st, btot, etot, mocc = oracle.domain_decoding(fx, bx, own, t["xf"])
+    assert st == 0
+    assert round(float(btot[L]), 1) == 9.4, btot[L]           # tutorial/fn3.out: "exp 9.4"
+    regions = oracle.domain_regions(btot, etot, mocc)
+    # tutorial/fn3.out, envfrom..envto of the nine reported domains. A region that is not
+    # multidomain becomes one envelope with the region's own coordinates
+    # (p7_domaindef.c:481-485); the last region is sent to trace clustering, which the
+    # tutorial run resolved into one domain with the same coordinates.
+    env = [(395, 411), (437, 521), (826, 914), (1203, 1259), (1304, 1386), (1739, 1769), (1799, 1891),
+           (1901, 1976), (1993, 2107)]
+    assert [(i, j) for i, j, _ in regions] == env
+    assert [m for _, _, m in regions[:8]] == [False] * 8
