@@ -99,12 +99,33 @@ struct DevResults {
 // host round trips.
 struct DevLists {
   int32_t *full, *bias, *vit, *fwd, *bck;
-  // [0] full, [1] bias, [2] vit, [3] fwd, [4] bck: list lengths; [6] scale-pool
-  // overflow flag; the work-queue cursor of list k sits at [k + 8]; [13] is the
-  // cursor of the ungapped pass over all targets ([14], [15]: the later
-  // launches of a chunked model). Zeroed before every call.
+  // [0] full, [1] bias, [2] vit, [3] fwd, [4] bck: list lengths; [5] domain
+  // regions written; [6] scale-pool overflow flag; [7] domain-decoding
+  // overflow flags (1: row pool, 2: region list); the work-queue cursor of list
+  // k sits at [k + 8]; [13] is the cursor of the ungapped pass over all targets
+  // ([14], [15]: the later launches of a chunked model). Zeroed before every call.
   int32_t *counts;
-  unsigned long long *cursor;   // next free float in the Forward scale-factor pool
+  // [0] next free float in the Forward scale-factor pool, [1] in the domain
+  // decoding pool
+  unsigned long long *cursor;
+};
+
+// Domain decoding buffers (pool == nullptr: off). The Forward parser keeps the
+// special-state rows (E N J B C SCALE) of every target it scores at doff[slot];
+// the Backward parser keeps its own rows, followed by btot, etot and mocc, at
+// boff[slot] (9 (L+1) floats); -1 where the pool was full.
+struct DomBufs {
+  float   *pool = nullptr;
+  int64_t  size = 0;
+  int64_t *doff = nullptr, *boff = nullptr;
+};
+// Per-target results of the decoding (slot order) and the region records
+// {slot, i, j, multidomain} in the order they were found.
+struct DomOut {
+  float   *nexp = nullptr;
+  int32_t *nreg = nullptr, *nclu = nullptr;
+  int4    *regs = nullptr;
+  int64_t  reg_cap = 0;
 };
 
 // Length model of one call. on == 0: every target uses its own length
@@ -176,7 +197,8 @@ hipError_t launch_vit(const DevProfile &p, const DevSeqs &s, const int32_t *list
                       DevResults r, DevLists l, GateThresh g, int gating, LenCfg lc, hipStream_t st);
 hipError_t launch_fwd(const DevProfile &p, const DevSeqs &s, const int32_t *list, const int32_t *count, int64_t maxn,
                       DevResults r, DevLists l, GateThresh g, int gating, LenCfg lc,
-                      float *scale_pool, int64_t pool_size, int64_t *scale_off, float *xmx_out, hipStream_t st);
+                      float *scale_pool, int64_t pool_size, int64_t *scale_off, float *xmx_out, hipStream_t st,
+                      const DomBufs &dom = DomBufs());
 #define P7G_Q_SSV 13        // counts[] index of the ungapped pass's work-queue cursor
 #define P7G_QOFF  8         // a list's work-queue cursor is counts[list + P7G_QOFF]
 
@@ -197,5 +219,11 @@ hipError_t launch_scatter32(const DevSeqs &s, const void *src, void *dst, int64_
 hipError_t launch_scatter8(const DevSeqs &s, const uint8_t *src, uint8_t *dst, int64_t n, hipStream_t st);
 hipError_t launch_bck(const DevProfile &p, const DevSeqs &s, const int32_t *list, const int32_t *count, int64_t maxn,
                       DevResults r, const float *scale_pool, const int64_t *scale_off, const float *fwd_xmx,
-                      float *xmx_out, LenCfg lc, hipStream_t st);
+                      float *xmx_out, LenCfg lc, hipStream_t st, const DomBufs &dom = DomBufs(),
+                      DevLists l = DevLists());
+// Posterior decoding of the special states and the region heuristics over the
+// Backward list (the F3 survivors), one target per thread.
+hipError_t launch_domains(const DevProfile &p, const DevSeqs &s, const int32_t *list, const int32_t *count, int64_t maxn,
+                          const DevResults &r, DevLists l, LenCfg lc, const DomBufs &dom, const DomOut &out,
+                          hipStream_t st);
 #endif

@@ -687,7 +687,8 @@ __global__ __launch_bounds__(FWD_THREADS(B))
 void k_fwd(const DevProfile p, const uint8_t *__restrict__ res, const int64_t *__restrict__ off,
            const int32_t *__restrict__ len, const int32_t *__restrict__ list, const int32_t *__restrict__ count,
            DevResults r, DevLists l, GateThresh g, int gating, LenCfg lc,
-           float *__restrict__ pool, int64_t pool_size, int64_t *__restrict__ scale_off, float *__restrict__ xmx_out)
+           float *__restrict__ pool, int64_t pool_size, int64_t *__restrict__ scale_off, float *__restrict__ xmx_out,
+           DomBufs dom)
 {
   constexpr bool PIN = (TM == TM_LDS || TM == TM_PIN);
   if ((int64_t) blockIdx.x * (blockDim.x >> 6) >= *count) return;
@@ -729,13 +730,24 @@ void k_fwd(const DevProfile p, const uint8_t *__restrict__ res, const int64_t *_
       if (at + L + 1 <= pool_size) { scl = pool + at; if (z == 0) scale_off[slot] = at; }
       else { if (z == 0) { scale_off[slot] = -1; atomicExch(&l.counts[6], 1); } }
     }
+    // the special-state rows go to the caller's matrix (one target) or, for
+    // domain decoding, to the target's entry in the decoding pool
+    float *xo = xmx_out;
+    if (dom.pool) {
+      const int64_t need = 6 * (int64_t) (L + 1);
+      unsigned long long a64 = 0;
+      if (z == 0) a64 = atomicAdd(l.cursor + 1, (unsigned long long) need);
+      const int64_t at = ((int64_t) (unsigned) __shfl((int) (a64 >> 32), 0, 64) << 32) | (unsigned) __shfl((int) a64, 0, 64);
+      if (at + need <= dom.size) { xo = dom.pool + at; if (z == 0) dom.doff[slot] = at; }
+      else { xo = nullptr; if (z == 0) { dom.doff[slot] = -1; atomicOr(&l.counts[7], 1); } }
+    }
     float M[B], I[B], D[B];
 #pragma unroll
     for (int j = 0; j < B; j++) M[j] = I[j] = D[j] = 0.0f;
     float xN = 1.0f, xJ = 0.0f, xC = 0.0f, xB = fmove, xE = 0.0f;
     float totscale = 0.0f;
-    if (xmx_out && z == 0) {
-      xmx_out[0] = 0.0f; xmx_out[1] = 1.0f; xmx_out[2] = 0.0f; xmx_out[3] = xB; xmx_out[4] = 0.0f; xmx_out[5] = 1.0f;
+    if (xo && z == 0) {
+      xo[0] = 0.0f; xo[1] = 1.0f; xo[2] = 0.0f; xo[3] = xB; xo[4] = 0.0f; xo[5] = 1.0f;
     }
     if (scl && z == 0) scl[0] = 1.0f;
 
@@ -789,8 +801,8 @@ void k_fwd(const DevProfile p, const uint8_t *__restrict__ res, const int64_t *_
       }
       mL = dppz<WAVE_SHR1>(M[B-1]);
       if (scl && z == 0) scl[i] = scale;
-      if (xmx_out && z == 0) {
-        float *xr = xmx_out + (size_t) i * 6;
+      if (xo && z == 0) {
+        float *xr = xo + (size_t) i * 6;
         xr[0] = xE; xr[1] = xN; xr[2] = xJ; xr[3] = xB; xr[4] = xC; xr[5] = scale;
       }
     }
@@ -818,7 +830,7 @@ __global__ __launch_bounds__(BCK_THREADS(B))
 void k_bck(const DevProfile p, const uint8_t *__restrict__ res, const int64_t *__restrict__ off,
            const int32_t *__restrict__ len, const int32_t *__restrict__ list, const int32_t *__restrict__ count,
            DevResults r, const float *__restrict__ pool, const int64_t *__restrict__ scale_off,
-           const float *__restrict__ fwd_xmx, float *__restrict__ xmx_out, LenCfg lc)
+           const float *__restrict__ fwd_xmx, float *__restrict__ xmx_out, LenCfg lc, DomBufs dom, DevLists l)
 {
   constexpr bool PIN = (TM == TM_LDS || TM == TM_PIN);
   if ((int64_t) blockIdx.x * (blockDim.x >> 6) >= *count) return;
@@ -858,8 +870,22 @@ void k_bck(const DevProfile p, const uint8_t *__restrict__ res, const int64_t *_
     if (fwd_xmx) { fsc = fwd_xmx + 5; fstride = 6; }
     else {
       const int64_t so = scale_off[slot];
-      if (so < 0) { if (z == 0) { r.bcksc[slot] = NAN; r.fstatus[slot] = 5; } continue; }
+      if (so < 0) {
+        if (z == 0) { r.bcksc[slot] = NAN; r.fstatus[slot] = 5; if (dom.pool) dom.boff[slot] = -1; }
+        continue;
+      }
       fsc = pool + so;
+    }
+    // the special-state rows: the caller's matrix (one target), or the
+    // target's decoding entry (rows, then room for btot, etot and mocc)
+    float *xo = xmx_out;
+    if (dom.pool) {
+      const int64_t need = 9 * (int64_t) (L + 1);
+      unsigned long long a64 = 0;
+      if (z == 0) a64 = atomicAdd(l.cursor + 1, (unsigned long long) need);
+      const int64_t at = ((int64_t) (unsigned) __shfl((int) (a64 >> 32), 0, 64) << 32) | (unsigned) __shfl((int) a64, 0, 64);
+      if (at + need <= dom.size) { xo = dom.pool + at; if (z == 0) dom.boff[slot] = at; }
+      else { xo = nullptr; if (z == 0) { dom.boff[slot] = -1; atomicOr(&l.counts[7], 1); } }
     }
     float M[B], I[B], D[B];
     float xJ = 0.0f, xB = 0.0f, xN = 0.0f;
@@ -885,8 +911,8 @@ void k_bck(const DevProfile p, const uint8_t *__restrict__ res, const int64_t *_
         for (int j = 0; j < B; j++) { M[j] *= inv; D[j] *= inv; I[j] *= inv; }
       }
       totscale = (float) log((double) s);
-      if (xmx_out && z == 0) {
-        float *xr = xmx_out + (size_t) L * 6;
+      if (xo && z == 0) {
+        float *xr = xo + (size_t) L * 6;
         xr[0] = xE; xr[1] = xN; xr[2] = xJ; xr[3] = xB; xr[4] = xC; xr[5] = s;
       }
     }
@@ -931,8 +957,8 @@ void k_bck(const DevProfile p, const uint8_t *__restrict__ res, const int64_t *_
         for (int j = 0; j < B; j++) { M[j] *= inv; D[j] *= inv; I[j] *= inv; }
         totscale = (float) ((double) totscale + log((double) s));
       }
-      if (xmx_out && z == 0) {
-        float *xr = xmx_out + (size_t) i * 6;
+      if (xo && z == 0) {
+        float *xr = xo + (size_t) i * 6;
         xr[0] = xE; xr[1] = xN; xr[2] = xJ; xr[3] = xB; xr[4] = xC; xr[5] = s;
       }
     }
@@ -947,8 +973,8 @@ void k_bck(const DevProfile p, const uint8_t *__restrict__ res, const int64_t *_
       xN = (xB * fmove) + (xN * floop);
     }
     if (z == 0) {
-      if (xmx_out) {
-        xmx_out[0] = 0.0f; xmx_out[1] = xN; xmx_out[2] = 0.0f; xmx_out[3] = xB; xmx_out[4] = 0.0f; xmx_out[5] = 1.0f;
+      if (xo) {
+        xo[0] = 0.0f; xo[1] = xN; xo[2] = 0.0f; xo[3] = xB; xo[4] = 0.0f; xo[5] = 1.0f;
       }
       int status = 0;
       float sc;
@@ -960,6 +986,79 @@ void k_bck(const DevProfile p, const uint8_t *__restrict__ res, const int64_t *_
   }
 }
 #undef BTR
+
+/*****************************************************************
+ * 9. Domain decoding and regions (F3 survivors, one target per thread)
+ *****************************************************************/
+// p7_DomainDecoding (HMMER/src/impl_sse/decoding.c:166-197) from the two
+// parsers' special rows: btot[i] / etot[i], the expected B / E visits up to
+// row i, and mocc[i], the probability that residue i is emitted by the core
+// model; in the reference's float operations and order. Then the region loop
+// of p7_domaindef_ByPosteriorHeuristics (p7_domaindef.c:409-489) with
+// is_multidomain_region (:534-547), thresholds rt1/rt2/rt3 of
+// p7_domaindef_Create (:109-111). Regions are appended as {slot, i, j,
+// multidomain} records.
+#define DOM_RT1 0.25f
+#define DOM_RT2 0.10f
+#define DOM_RT3 0.20f
+__global__ __launch_bounds__(64)
+void k_domains(const DevProfile p, const int32_t *__restrict__ len, const int32_t *__restrict__ list,
+               const int32_t *__restrict__ count, const int *__restrict__ fstatus, DevLists l, LenCfg lc,
+               DomBufs dom, DomOut out)
+{
+#pragma clang fp contract(off)
+  enum { E = 0, N = 1, J = 2, Bx = 3, C = 4, S = 5 };
+  const int64_t n = *count;
+  for (int64_t t = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (int64_t) gridDim.x * blockDim.x) {
+    const int slot = list[t];
+    const int L = len[slot];
+    const int64_t fo = dom.doff[slot], bo = dom.boff[slot];
+    if (fo < 0 || bo < 0) { out.nexp[slot] = NAN; out.nreg[slot] = -1; out.nclu[slot] = -1; continue; }
+    const float *fx = dom.pool + fo;
+    const float *bx = dom.pool + bo;
+    float *btot = dom.pool + bo + 6 * (int64_t) (L + 1), *etot = btot + (L + 1), *mocc = etot + (L + 1);
+    const bool own = (fstatus[slot] & 0x100) != 0;
+    const float loop = lc.on ? lc.floop : p.xf_loop[L];     // N, J and C loops are equal after ReconfigLength
+    float sp = __fdiv_rn(1.0f, bx[N]);
+    float bt = 0.0f, et = 0.0f;
+    btot[0] = 0.0f; etot[0] = 0.0f; mocc[0] = 0.0f;
+    for (int i = 1; i <= L; i++) {
+      const float *f0 = fx + (size_t) (i - 1) * 6, *f1 = fx + (size_t) i * 6;
+      const float *b0 = bx + (size_t) (i - 1) * 6, *b1 = bx + (size_t) i * 6;
+      bt = bt + (f0[Bx] * b0[Bx] * f0[S] * sp);
+      if (own) sp *= __fdiv_rn(f0[S], b0[S]);
+      et = et + (f1[E] * b1[E] * f1[S] * sp);
+      float njcp = f0[N] * b1[N] * loop * sp;
+      njcp += f0[J] * b1[J] * loop * sp;
+      njcp += f0[C] * b1[C] * loop * sp;
+      btot[i] = bt; etot[i] = et; mocc[i] = 1.0f - njcp;
+    }
+    out.nexp[slot] = bt;
+    int nreg = 0, nclu = 0, i0 = -1;
+    bool triggered = false;
+    for (int j = 1; j <= L; j++) {
+      if (! triggered) {
+        if      (mocc[j] - (btot[j] - btot[j-1]) < DOM_RT2) i0 = j;
+        else if (i0 == -1)                                   i0 = j;
+        if      (mocc[j] >= DOM_RT1)                         triggered = true;
+      } else if (mocc[j] - (etot[j] - etot[j-1]) < DOM_RT2) {
+        float mx = -1.0f;
+        for (int zz = i0; zz <= j; zz++) {
+          const float a = etot[zz] - etot[i0-1], b = btot[j] - btot[zz-1];
+          const float en = (a < b) ? a : b;
+          mx = (mx > en) ? mx : en;
+        }
+        const int multi = (mx >= DOM_RT3) ? 1 : 0;
+        const int k = atomicAdd(&l.counts[5], 1);
+        if (k < out.reg_cap) out.regs[k] = make_int4(slot, i0, j, multi);
+        else atomicOr(&l.counts[7], 2);
+        nreg++; nclu += multi;
+        i0 = -1; triggered = false;
+      }
+    }
+    out.nreg[slot] = nreg; out.nclu[slot] = nclu;
+  }
+}
 
 __global__ void k_fill32(uint32_t *__restrict__ p, uint32_t v, int64_t n)
 {
@@ -1097,7 +1196,8 @@ static size_t fb_lds(const DevProfile &p, int tm)
 
 hipError_t launch_fwd(const DevProfile &p, const DevSeqs &s, const int32_t *list, const int32_t *count, int64_t maxn,
                       DevResults r, DevLists l, GateThresh g, int gating, LenCfg lc,
-                      float *scale_pool, int64_t pool_size, int64_t *scale_off, float *xmx_out, hipStream_t st)
+                      float *scale_pool, int64_t pool_size, int64_t *scale_off, float *xmx_out, hipStream_t st,
+                      const DomBufs &dom)
 {
   if (maxn <= 0) return hipSuccess;
   const int tm = fb_mode(p);
@@ -1110,13 +1210,13 @@ hipError_t launch_fwd(const DevProfile &p, const DevSeqs &s, const int32_t *list
   const size_t lds = fb_lds(p, tm);
   const unsigned grid = grid_for_waves_blk(k, maxn, lds, nthr / 64);
   void *args[] = { (void *) &p, (void *) &s.res, (void *) &s.off, (void *) &s.len, (void *) &list, (void *) &count,
-                   &r, &l, &g, &gating, &lc, &scale_pool, &pool_size, &scale_off, &xmx_out };
+                   &r, &l, &g, &gating, &lc, &scale_pool, &pool_size, &scale_off, &xmx_out, (void *) &dom };
   return launch(k, grid, nthr, args, lds, st);
 }
 
 hipError_t launch_bck(const DevProfile &p, const DevSeqs &s, const int32_t *list, const int32_t *count, int64_t maxn,
                       DevResults r, const float *scale_pool, const int64_t *scale_off, const float *fwd_xmx,
-                      float *xmx_out, LenCfg lc, hipStream_t st)
+                      float *xmx_out, LenCfg lc, hipStream_t st, const DomBufs &dom, DevLists l)
 {
   if (maxn <= 0) return hipSuccess;
   const int tm = fb_mode(p);
@@ -1129,8 +1229,19 @@ hipError_t launch_bck(const DevProfile &p, const DevSeqs &s, const int32_t *list
   const size_t lds = fb_lds(p, tm);
   const unsigned grid = grid_for_waves_blk(k, maxn, lds, nthr / 64);
   void *args[] = { (void *) &p, (void *) &s.res, (void *) &s.off, (void *) &s.len, (void *) &list, (void *) &count,
-                   &r, (void *) &scale_pool, (void *) &scale_off, (void *) &fwd_xmx, &xmx_out, &lc };
+                   &r, (void *) &scale_pool, (void *) &scale_off, (void *) &fwd_xmx, &xmx_out, &lc, (void *) &dom, &l };
   return launch(k, grid, nthr, args, lds, st);
+}
+
+hipError_t launch_domains(const DevProfile &p, const DevSeqs &s, const int32_t *list, const int32_t *count, int64_t maxn,
+                          const DevResults &r, DevLists l, LenCfg lc, const DomBufs &dom, const DomOut &out,
+                          hipStream_t st)
+{
+  if (maxn <= 0) return hipSuccess;
+  if (! dom.pool || ! dom.doff || ! dom.boff || ! out.nexp || ! out.nreg || ! out.nclu || ! out.regs) return hipErrorInvalidValue;
+  const unsigned grid = (unsigned) std::min<int64_t>((maxn + 63) / 64, 1024);
+  hipLaunchKernelGGL(k_domains, dim3(grid), dim3(64), 0, st, p, s.len, list, count, (const int *) r.fstatus, l, lc, dom, out);
+  return hipGetLastError();
 }
 
 /*****************************************************************

@@ -14,6 +14,8 @@
 #include <map>
 #include <atomic>
 #include <set>
+#include <algorithm>
+#include <vector>
 #include <vector>
 #include <algorithm>
 
@@ -47,6 +49,10 @@ struct Work {
   void       *mem = nullptr;
   hipEvent_t  ev[8];
   bool        ev_ready = false;
+  DomBufs     dom;                  // domain decoding: row pool and per-slot offsets (made on first use)
+  DomOut      dout;                 // its per-slot results and region records
+  void       *dom_mem = nullptr;
+  int64_t     dom_cap = 0;          // slots the per-slot arrays hold
 };
 static Work g_work;
 
@@ -793,10 +799,43 @@ work_bnd(Work &w, const DevProfile &p, int64_t bytes)
   return p7G_OK;
 }
 
+// Domain decoding buffers: per slot two pool offsets and three results, the
+// region records, and a row pool of 15 floats per row of an F3 survivor (6 per
+// row of a target that only reaches Forward), at most 2^28 floats.
+#define DOM_REG_CAP ((int64_t) 1 << 20)
+static int64_t dom_pool_floats(int64_t nseq, int64_t nres) { return std::min<int64_t>(16 * (nres + nseq), (int64_t) 1 << 28); }
+
+static int
+dom_reserve(Work &w, int64_t n, int64_t pool_floats)
+{
+  pool_floats = std::max<int64_t>(pool_floats, 1 << 20);
+  if (n <= w.dom_cap && pool_floats <= w.dom.size) return p7G_OK;
+  if (w.dom_mem || w.dom.pool) HIPCHK(hipStreamSynchronize(w.st));
+  if (w.dom_mem) { (void) hipFree(w.dom_mem); w.dom_mem = nullptr; w.dom_cap = 0; }
+  if (w.dom.pool) { (void) hipFree(w.dom.pool); w.dom.pool = nullptr; w.dom.size = 0; }
+  const int64_t cap = std::max<int64_t>(n, 1024);
+  const size_t total = (size_t) cap * (8 + 8 + 4 + 4 + 4) + 16 * (size_t) DOM_REG_CAP + 4 * 256;
+  char *m = nullptr;
+  HIPCHK(hipMalloc((void **) &m, total));
+  size_t o = 0;
+  auto take = [&](size_t bytes) { char *p = m + o; o += (bytes + 255) & ~(size_t) 255; return (void *) p; };
+  w.dom.doff = (int64_t *) take(8 * cap); w.dom.boff = (int64_t *) take(8 * cap);
+  w.dout.nexp = (float *) take(4 * cap); w.dout.nreg = (int32_t *) take(4 * cap); w.dout.nclu = (int32_t *) take(4 * cap);
+  w.dout.regs = (int4 *) take(16 * (size_t) DOM_REG_CAP); w.dout.reg_cap = DOM_REG_CAP;
+  if (o > total) { (void) hipFree(m); p7g_set_error("internal: domain buffer overrun"); return p7G_EINVAL; }
+  w.dom_mem = m; w.dom_cap = cap;
+  HIPCHK(hipMalloc((void **) &w.dom.pool, sizeof(float) * (size_t) pool_floats));
+  w.dom.size = pool_floats;
+  return p7G_OK;
+}
+
 static void
 work_release(Work &w)
 {
   if (w.st) (void) hipStreamSynchronize(w.st);
+  if (w.dom_mem) (void) hipFree(w.dom_mem);
+  if (w.dom.pool) (void) hipFree(w.dom.pool);
+  w.dom_mem = nullptr; w.dom = DomBufs(); w.dout = DomOut(); w.dom_cap = 0;
   if (w.mem) (void) hipFree(w.mem);
   if (w.pool) (void) hipFree(w.pool);
   if (w.bnd) (void) hipFree(w.bnd);
@@ -948,6 +987,9 @@ static void batch_free(Batch *b);
 struct p7g_pipeline_s {
   double F1, F2, F3;
   int    do_biasfilter, do_max;
+  int    do_domains = 0;      // posterior decoding and regions of the F3 survivors (p7_gpu_Pipeline)
+  std::vector<int4> regs;     // the last run's region records, in the caller's sequence order
+  bool   regs_valid = false;
   P7_GPU_STATS st;
   Work       w;             // this pipeline's stream and device work area
   Batch     *b = nullptr;   // work areas and graph of p7_gpu_PipelineBatch, made on first use
@@ -1062,8 +1104,9 @@ chain_reset(Work &w, int64_t n)
 // Forward and Backward stages.
 static int
 chain_stages(Work &w, const DevProfile &p, const DevSeqs &s, int64_t n, const uint16_t *zmax, const GateThresh &g,
-             bool timed)
+             bool timed, bool domains = false)
 {
+  const DomBufs dom = domains ? w.dom : DomBufs();
   LenCfg lc = lencfg_per_target();
   HIPCHK(launch_msv_classify(p, s, nullptr, n, zmax, w.r.xj, w.l.full, &w.l.counts[0], lc, w.st));
   HIPCHK(launch_compact_i16(w.r.xj, -2, n, w.l.full, &w.l.counts[0], w.tilecnt, w.st));
@@ -1079,11 +1122,16 @@ chain_stages(Work &w, const DevProfile &p, const DevSeqs &s, int64_t n, const ui
   HIPCHK(launch_vit(p, s, w.l.vit, &w.l.counts[2], n, w.r, w.l, g, 1, lc, w.st));
   if (timed) HIPCHK(hipEventRecord(w.ev[3], w.st));
   HIPCHK(launch_compact_u8(w.r.stage, 3, n, w.l.fwd, &w.l.counts[3], w.tilecnt, w.st));
-  HIPCHK(launch_fwd(p, s, w.l.fwd, &w.l.counts[3], n, w.r, w.l, g, 1, lc, w.pool, w.pool_size, w.scale_off, nullptr, w.st));
+  HIPCHK(launch_fwd(p, s, w.l.fwd, &w.l.counts[3], n, w.r, w.l, g, 1, lc, w.pool, w.pool_size, w.scale_off, nullptr, w.st,
+                    dom));
   if (timed) HIPCHK(hipEventRecord(w.ev[4], w.st));
   HIPCHK(launch_compact_u8(w.r.stage, 4, n, w.l.bck, &w.l.counts[4], w.tilecnt, w.st));
-  HIPCHK(launch_bck(p, s, w.l.bck, &w.l.counts[4], n, w.r, w.pool, w.scale_off, nullptr, nullptr, lc, w.st));
+  HIPCHK(launch_bck(p, s, w.l.bck, &w.l.counts[4], n, w.r, w.pool, w.scale_off, nullptr, nullptr, lc, w.st, dom, w.l));
   if (timed) HIPCHK(hipEventRecord(w.ev[5], w.st));
+  if (domains) {
+    HIPCHK(launch_domains(p, s, w.l.bck, &w.l.counts[4], n, w.r, w.l, lc, dom, w.dout, w.st));
+    if (timed) HIPCHK(hipEventRecord(w.ev[7], w.st));
+  }
   return p7G_OK;
 }
 
@@ -1138,15 +1186,23 @@ p7_gpu_Pipeline(P7_GPU_PIPELINE *pli, const P7_GPU_OPROFILE *om, const P7_GPU_SE
   // total residues of every target that may reach the Forward stage
   Work &w = pli->w;
   if ((status = work_reserve(w, n, pipeline_pool_floats(n, db->nres))) != p7G_OK) return status;
+  const bool dom = pli->do_domains != 0;
+  if (dom && (status = dom_reserve(w, n, dom_pool_floats(n, db->nres))) != p7G_OK) return status;
+  pli->regs_valid = false;
   DevSeqs s = seqs_of(db);
   const GateThresh g = gates_of(pli, om);
 
   if ((status = chain_reset(w, n)) != p7G_OK) return status;
+  if (dom) {                              // targets that do not reach decoding: NaN, no regions
+    HIPCHK(launch_fill32(w.dout.nexp, 0x7fc00000u, n, w.st));
+    HIPCHK(launch_fill32(w.dout.nreg, 0u, n, w.st));
+    HIPCHK(launch_fill32(w.dout.nclu, 0u, n, w.st));
+  }
   HIPCHK(hipEventRecord(w.ev[0], w.st));
   if ((status = work_bnd(w, *p, db->bytes)) != p7G_OK) return status;
   HIPCHK(launch_ssv_chosen(*p, s, nullptr, n, w.r.zmax, &w.l.counts[P7G_Q_SSV], w.bnd, w.bnd_cap, w.st));
   HIPCHK(hipEventRecord(w.ev[6], w.st));
-  if ((status = chain_stages(w, *p, s, n, w.r.zmax, g, true)) != p7G_OK) return status;
+  if ((status = chain_stages(w, *p, s, n, w.r.zmax, g, true, dom)) != p7G_OK) return status;
 
   // counters on the device, then the requested per-target arrays in the caller's order
   HIPCHK(launch_fill32(w.stats, 0u, 16, w.st));
@@ -1168,8 +1224,27 @@ p7_gpu_Pipeline(P7_GPU_PIPELINE *pli, const P7_GPU_OPROFILE *om, const P7_GPU_SE
     HIPCHK(launch_scatter32(s, fl[a].src, w.outbuf, n, w.st));
     HIPCHK(hipMemcpyAsync(fl[a].dst, w.outbuf, sizeof(float) * (size_t) n, hipMemcpyDeviceToHost, w.st));
   }
+  if (dom && hits) {
+    struct { void *dst; const void *src; } dl[] = {
+      { hits->nexpected, w.dout.nexp }, { hits->nregions, w.dout.nreg }, { hits->nclustered, w.dout.nclu } };
+    for (int a = 0; a < 3; a++) if (dl[a].dst) {
+      HIPCHK(launch_scatter32(s, dl[a].src, w.outbuf, n, w.st));
+      HIPCHK(hipMemcpyAsync(dl[a].dst, w.outbuf, 4 * (size_t) n, hipMemcpyDeviceToHost, w.st));
+    }
+  }
   HIPCHK(hipStreamSynchronize(w.st));
   if (counts[6]) { p7g_set_error("Forward scale-factor pool exhausted"); return p7G_EMEM; }
+  int64_t nregs = 0;
+  if (dom) {
+    if (counts[7]) { p7g_set_error("domain decoding %s exhausted", (counts[7] & 1) ? "row pool" : "region list"); return p7G_EMEM; }
+    nregs = counts[5];
+    std::vector<int4> rec((size_t) nregs);
+    if (nregs) HIPCHK(hipMemcpy(rec.data(), w.dout.regs, sizeof(int4) * (size_t) nregs, hipMemcpyDeviceToHost));
+    for (auto &r : rec) r.x = (int) db->h_perm[r.x];        // slot -> the caller's sequence index
+    std::sort(rec.begin(), rec.end(), [](const int4 &a, const int4 &b) { return a.x != b.x ? a.x < b.x : a.y < b.y; });
+    pli->regs.swap(rec);
+    pli->regs_valid = true;
+  }
 
   P7_GPU_STATS one;
   stats_of_run(one, sums, counts, om, db, pli->do_biasfilter);
@@ -1182,6 +1257,39 @@ p7_gpu_Pipeline(P7_GPU_PIPELINE *pli, const P7_GPU_OPROFILE *om, const P7_GPU_SE
   HIPCHK(hipEventElapsedTime(&mtot, w.ev[0], w.ev[6]));
   st.ms_ssv = mtot;
   st.ms_msv = ms[0]; st.ms_bias = ms[1]; st.ms_vit = ms[2]; st.ms_fwd = ms[3]; st.ms_bck = ms[4];
+  st.ms_domains = 0.0; st.n_regions = 0; st.n_clustered = 0;
+  if (dom) {
+    HIPCHK(hipEventElapsedTime(&mtot, w.ev[5], w.ev[7]));
+    st.ms_domains = mtot;
+    st.n_regions = nregs;
+    for (const auto &r : pli->regs) st.n_clustered += r.w;
+  }
+  return p7G_OK;
+}
+
+extern "C" int
+p7_gpu_pipeline_SetDomains(P7_GPU_PIPELINE *pli, int on)
+{
+  if (! pli) return p7G_EINVAL;
+  std::lock_guard<std::mutex> lk(pli->mu);
+  pli->do_domains = on ? 1 : 0;
+  return p7G_OK;
+}
+
+extern "C" int
+p7_gpu_pipeline_GetRegions(const P7_GPU_PIPELINE *pli, int64_t *seq, int32_t *ienv, int32_t *jenv, int32_t *multi,
+                           int64_t cap, int64_t *ret_n)
+{
+  if (! pli || ! pli->regs_valid) { p7g_set_error("no domain regions: the last p7_gpu_Pipeline call did not decode"); return p7G_EINVAL; }
+  const int64_t n = (int64_t) pli->regs.size();
+  if (ret_n) *ret_n = n;
+  for (int64_t k = 0; k < std::min(n, cap); k++) {
+    const int4 &r = pli->regs[(size_t) k];
+    if (seq) seq[k] = r.x;
+    if (ienv) ienv[k] = r.y;
+    if (jenv) jenv[k] = r.z;
+    if (multi) multi[k] = r.w;
+  }
   return p7G_OK;
 }
 

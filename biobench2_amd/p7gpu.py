@@ -48,14 +48,17 @@ class Tables(ctypes.Structure):
 
 class Hits(ctypes.Structure):
     _fields_ = [(n, POINTER(c_float)) for n in ("nullsc", "usc", "filtersc", "vfsc", "fwdsc", "bcksc")] + \
-               [("stage", POINTER(c_uint8))]
+               [("stage", POINTER(c_uint8)), ("nexpected", POINTER(c_float)),
+                ("nregions", POINTER(c_int32)), ("nclustered", POINTER(c_int32))]
 
 
 class Stats(ctypes.Structure):
     _fields_ = [(n, c_int64) for n in ("nmodels", "nseqs", "nres", "nnodes", "n_past_msv", "n_past_bias",
                                        "n_past_vit", "n_past_fwd", "n_msv_full")] + \
                [(n, c_double) for n in ("cells_msv", "cells_bias", "cells_vit", "cells_fwd", "cells_bck",
-                                        "ms_msv", "ms_bias", "ms_vit", "ms_fwd", "ms_bck", "ms_total", "ms_ssv")]
+                                        "ms_msv", "ms_bias", "ms_vit", "ms_fwd", "ms_bck", "ms_total", "ms_ssv",
+                                        "ms_domains")] + \
+               [(n, c_int64) for n in ("n_regions", "n_clustered")]
 
     def as_dict(self):
         return {n: getattr(self, n) for n, _ in self._fields_}
@@ -97,6 +100,9 @@ def _load():
         "p7_gpu_pipeline_Create": (vp, [d, d, d, i, i]), "p7_gpu_pipeline_Destroy": (None, [vp]),
         "p7_gpu_Pipeline": (i, [vp, vp, vp, POINTER(Hits)]),
         "p7_gpu_pipeline_GetStats": (i, [vp, POINTER(Stats)]),
+        "p7_gpu_pipeline_SetDomains": (i, [vp, i]),
+        "p7_gpu_pipeline_GetRegions": (i, [vp, POINTER(c_int64), POINTER(c_int32), POINTER(c_int32), POINTER(c_int32),
+                                           c_int64, POINTER(c_int64)]),
         "p7_gpu_ScoreThreshold": (f, [i, d, d, d]),
         "p7_gpu_oprofile_ReadPressed": (i, [c_char_p, c_char_p, i, POINTER(vp)]),
         "p7_gpu_seqdb_ReadFasta": (i, [c_char_p, i, POINTER(vp)]),
@@ -473,7 +479,7 @@ def filter_score_batch(om, db, idx=None):
     return _batch(lib.p7_gpu_FilterScoreBatch, om, db, idx, "FilterScoreBatch", with_status=False)
 
 
-def _hits(n, scores):
+def _hits(n, scores, domains=False):
     out = {"stage": np.zeros(n, np.uint8)}
     hits = Hits()
     hits.stage = _ptr(out["stage"], c_uint8)
@@ -481,6 +487,12 @@ def _hits(n, scores):
         for k in ("nullsc", "usc", "filtersc", "vfsc", "fwdsc", "bcksc"):
             out[k] = np.zeros(n, np.float32)
             setattr(hits, k, _ptr(out[k], c_float))
+    if domains:
+        out["nexpected"] = np.zeros(n, np.float32)
+        hits.nexpected = _ptr(out["nexpected"], c_float)
+        for k in ("nregions", "nclustered"):
+            out[k] = np.zeros(n, np.int32)
+            setattr(hits, k, _ptr(out[k], c_int32))
     return out, hits
 
 
@@ -524,13 +536,27 @@ class Server:
 class Pipeline:
     """P7_PIPELINE thresholds and counters; run() is p7_Pipeline over a whole database."""
 
-    def __init__(self, F1=0.02, F2=1e-3, F3=1e-5, do_biasfilter=True, do_max=False):
+    def __init__(self, F1=0.02, F2=1e-3, F3=1e-5, do_biasfilter=True, do_max=False, domains=False):
         self.h = lib.p7_gpu_pipeline_Create(F1, F2, F3, int(do_biasfilter), int(do_max))
+        self.domains = bool(domains)
+        _check(lib.p7_gpu_pipeline_SetDomains(self.h, int(self.domains)), "pipeline_SetDomains")
 
     def run(self, om, db, scores=True):
-        out, hits = _hits(db.nseq, scores)
+        """p7_gpu_Pipeline; with domains=True the outputs also hold nexpected,
+        nregions and nclustered of the F3 survivors."""
+        out, hits = _hits(db.nseq, scores, self.domains)
         _check(lib.p7_gpu_Pipeline(self.h, om.h, db.h, ctypes.byref(hits)), "Pipeline")
         return out
+
+    def regions(self):
+        """[(sequence index, i, j, multidomain)] of the last run with domain decoding."""
+        n = c_int64()
+        _check(lib.p7_gpu_pipeline_GetRegions(self.h, None, None, None, None, 0, ctypes.byref(n)), "GetRegions")
+        seq = np.zeros(n.value, np.int64)
+        i, j, m = (np.zeros(n.value, np.int32) for _ in range(3))
+        _check(lib.p7_gpu_pipeline_GetRegions(self.h, _ptr(seq, c_int64), _ptr(i, c_int32), _ptr(j, c_int32),
+                                              _ptr(m, c_int32), n.value, ctypes.byref(n)), "GetRegions")
+        return [(int(seq[k]), int(i[k]), int(j[k]), bool(m[k])) for k in range(n.value)]
 
     def run_batch(self, oms, db, scores=True, graph=False):
         """p7_gpu_PipelineBatch: every profile of `oms` over the database in one

@@ -110,6 +110,13 @@ typedef struct p7g_hits_s {
   float   *bcksc;      /* Backward parser score (F3 survivors)              */
   uint8_t *stage;      /* 0 failed MSV, 1 failed bias, 2 failed Vit,
                           3 failed Fwd, 4 passed all filters               */
+  /* Domain decoding of the F3 survivors (p7_gpu_Pipeline with
+   * p7_gpu_pipeline_SetDomains on; other calls leave these untouched):
+   * P7_DOMAINDEF's nexpected, nregions and nclustered (NaN / 0 / 0 for
+   * targets that did not pass every filter). */
+  float   *nexpected;  /* posterior expected number of domains, btot[L]   */
+  int32_t *nregions;   /* regions found by the posterior heuristics        */
+  int32_t *nclustered; /* of which flagged for trace clustering            */
 } P7_GPU_HITS;
 
 /* Counters of P7_PIPELINE (HMMER/src/hmmer.h:1223-1237) plus device timings. */
@@ -120,6 +127,8 @@ typedef struct p7g_stats_s {
   double  cells_msv, cells_bias, cells_vit, cells_fwd, cells_bck;
   double  ms_msv, ms_bias, ms_vit, ms_fwd, ms_bck, ms_total;   /* device time, last call */
   double  ms_ssv;              /* the ungapped MSV pass alone (k_ssv), last call    */
+  double  ms_domains;          /* domain decoding and regions, last call (0 if off) */
+  int64_t n_regions, n_clustered;   /* regions of the last call, and multidomain ones */
 } P7_GPU_STATS;
 
 /* ---- device ------------------------------------------------------------- */
@@ -239,6 +248,21 @@ void  p7_gpu_pipeline_Destroy(P7_GPU_PIPELINE *pli);
  * the counters like p7_pli_NewModel/NewSeq/p7_Pipeline. */
 int   p7_gpu_Pipeline(P7_GPU_PIPELINE *pli, const P7_GPU_OPROFILE *om, const P7_GPU_SEQDB *db, P7_GPU_HITS *hits);
 int   p7_gpu_pipeline_GetStats(const P7_GPU_PIPELINE *pli, P7_GPU_STATS *st);
+/* Domain decoding for the F3 survivors of p7_gpu_Pipeline: the first part of
+ * p7_domaindef_ByPosteriorHeuristics (HMMER/src/p7_domaindef.c:386-489) as
+ * p7_Pipeline calls it (p7_pipeline.c:737): p7_DomainDecoding
+ * (impl_sse/decoding.c:166-197) from the two parsers' special states, then
+ * the region heuristics with is_multidomain_region (rt1 0.25, rt2 0.10, rt3
+ * 0.20). The stochastic trace clustering of multidomain regions, envelope
+ * rescoring, null2 and the alignments that follow in the reference are not
+ * computed. Off by default; batches and the server do not decode. */
+int   p7_gpu_pipeline_SetDomains(P7_GPU_PIPELINE *pli, int on);
+/* The regions of the last p7_gpu_Pipeline call with decoding on, ordered by
+ * sequence (the database's order) then start: sequence index, first and last
+ * residue (1-based, inclusive), and 1 if flagged multidomain. Writes up to cap
+ * entries (any array may be NULL); *ret_n = the number of regions. */
+int   p7_gpu_pipeline_GetRegions(const P7_GPU_PIPELINE *pli, int64_t *seq, int32_t *ienv, int32_t *jenv,
+                                 int32_t *multi, int64_t cap, int64_t *ret_n);
 /* ---- batches: many profiles against one resident database ----------------
  * Runs the p7_gpu_Pipeline filter chain of nprof profiles over every database
  * sequence in one call. The ungapped MSV pass of all profiles that share a

@@ -1,0 +1,115 @@
+"""Domain decoding of the F3 survivors on the device (p7_gpu_pipeline_SetDomains):
+p7_DomainDecoding (HMMER/src/impl_sse/decoding.c:166-197) and the region
+heuristics of p7_domaindef_ByPosteriorHeuristics (HMMER/src/p7_domaindef.c:
+409-489, is_multidomain_region :534-547), against the oracle on the same
+targets and against the tutorial's fn3 vs 7LESS_DROME run (tutorial/fn3.out:
+exp 9.4, nine domain envelopes).
+
+The decoding reads the device parsers' special states, which equal the
+reference's to within float rounding, not bit for bit; so nexpected is
+compared within 1e-4 (relative), and a region list may differ from the
+oracle's only where some occupancy test lies within 1e-4 of its threshold."""
+import numpy as np
+import pytest
+
+import oracle
+from biobench2_amd import p7gpu, seqio, synth
+from conftest import golden, synth_profile
+
+pytestmark = pytest.mark.gpu
+
+FN3_ENV = [(395, 411), (437, 521), (826, 914), (1203, 1259), (1304, 1386), (1739, 1769), (1799, 1891),
+           (1901, 1976), (1993, 2107)]
+
+
+def test_fn3_tutorial_regions_on_the_device(gpu):
+    (name, s), = seqio.read_uniprot(golden("7LESS_DROME"))
+    rng = np.random.default_rng(3)
+    seqs = [synth.iid_residues(int(L), rng) for L in (300, 2554, 50)]
+    seqs.insert(1, s)
+    db = gpu.SeqDB.from_list(seqs)
+    om = gpu.OProfile.from_file(golden("fn3.hmm"))
+    pli = gpu.Pipeline(domains=True)
+    out = pli.run(om, db)
+    assert out["stage"][1] == 4
+    assert round(float(out["nexpected"][1]), 1) == 9.4
+    assert out["nregions"][1] == 9
+    regs = [r for r in pli.regions() if r[0] == 1]
+    assert [(i, j) for _, i, j, _ in regs] == FN3_ENV
+    assert [m for _, _, _, m in regs[:8]] == [False] * 8
+    assert out["nclustered"][1] == sum(m for _, _, _, m in regs)
+    for k in (0, 2, 3):                       # targets that did not pass every filter
+        if out["stage"][k] != 4:
+            assert np.isnan(out["nexpected"][k]) and out["nregions"][k] == 0
+    st = pli.stats()
+    assert st["n_regions"] == len(pli.regions()) and st["ms_domains"] > 0.0
+
+
+def per_target_xf(t, L, lm):
+    xf = t["xf"].copy()
+    xf[1, 0] = xf[2, 0] = xf[3, 0] = lm.xfmove[L]
+    xf[1, 1] = xf[2, 1] = xf[3, 1] = lm.xfloop[L]
+    return xf
+
+
+def margin(btot, etot, mocc, regions):
+    """The smallest distance of any occupancy test of the region loop to its threshold."""
+    f = np.float32
+    db = (btot[1:] - btot[:-1]).astype(np.float32)
+    de = (etot[1:] - etot[:-1]).astype(np.float32)
+    m = mocc[1:]
+    d = min(np.min(np.abs(m - db - f(oracle.RT2))), np.min(np.abs(m - f(oracle.RT1))), np.min(np.abs(m - de - f(oracle.RT2))))
+    for i, j, _ in regions:
+        best = max(min(etot[z] - etot[i - 1], btot[j] - btot[z - 1]) for z in range(i, j + 1))
+        d = min(d, abs(best - f(oracle.RT3)))
+    return float(d)
+
+
+@pytest.mark.parametrize("name,indel", [("synth_M200", False), ("synth_M400", True)])
+def test_device_decoding_equals_the_oracle(gpu, name, indel):
+    path = synth_profile(name)
+    hmm = synth.read_hmm_probs(path)
+    db = synth.make_db(3000, [hmm], frac_homolog=0.05, seed_db=31, seed_plant=32, long_frac=0.01,
+                       long_range=(3000, 6000), indel_rich=indel)
+    sdb = gpu.SeqDB(db.residues, db.offsets, db.lengths)
+    om = gpu.OProfile.from_file(path)
+    plain = gpu.Pipeline().run(om, sdb)
+    pli = gpu.Pipeline(domains=True)
+    out = pli.run(om, sdb)
+    for k in plain:                                   # decoding changes no filter result
+        x, y = out[k], plain[k]
+        assert np.array_equal(np.isnan(x), np.isnan(y)) if x.dtype != np.uint8 else True
+        m = ~np.isnan(x) if x.dtype != np.uint8 else slice(None)
+        assert np.array_equal(x[m], y[m]), k
+    t = om.tables()
+    lm = oracle.LengthModel(t, int(db.lengths.max()) + 1)
+    regs = {}
+    for sq, i, j, mu in pli.regions():
+        regs.setdefault(sq, []).append((i, j, mu))
+    passed = np.nonzero(out["stage"] == 4)[0]
+    assert passed.size >= 50
+    worst, differ = 0.0, 0
+    for q in passed:
+        s = db.seq(int(q))
+        L = s.size
+        xf = per_target_xf(t, L, lm)
+        st, fsc, fx, _ = oracle.fwd_parser(s, t, xf)
+        st2, bsc, bx, own = oracle.bck_parser(s, t, fx, xf)
+        st3, btot, etot, mocc = oracle.domain_decoding(fx, bx, own, xf)
+        assert st == st2 == st3 == 0
+        worst = max(worst, abs(out["nexpected"][q] - btot[L]) / max(1.0, btot[L]))
+        ref = oracle.domain_regions(btot, etot, mocc)
+        got = regs.get(int(q), [])
+        assert out["nregions"][q] == len(got) and out["nclustered"][q] == sum(m for _, _, m in got)
+        if got != ref:
+            differ += 1
+            assert margin(btot, etot, mocc, ref) < 1e-4, (q, got, ref)
+    assert worst <= 1e-4, worst
+    assert differ <= max(1, passed.size // 100)
+    assert sum(len(v) for v in regs.values()) == pli.stats()["n_regions"] > 0
+
+
+def test_regions_need_a_decoding_run(gpu):
+    pli = gpu.Pipeline()
+    with pytest.raises(gpu.P7GpuError):
+        pli.regions()
