@@ -204,6 +204,25 @@ def batched_leg(p7gpu, synth, nprof, nseq, reps=3, seed=42):
             "identical_pass_sets": same}
 
 
+def domain_leg(p7gpu, om, sdb, steps, ms_plain):
+    """The same pipeline with domain decoding of the F3 survivors on
+    (posterior decoding of the special states and the region heuristics):
+    the decoding kernel's time, the whole step, and what it found."""
+    pli = p7gpu.Pipeline(domains=True)
+    pli.run(om, sdb, scores=False)
+    st = pli.stats()                                 # counters of one call
+    p7gpu.lib.p7_gpu_Synchronize()
+    ms_dom, t0 = [], time.perf_counter()
+    for _ in range(steps):
+        pli.run(om, sdb, scores=False)
+        ms_dom.append(pli.stats()["ms_domains"])
+    p7gpu.lib.p7_gpu_Synchronize()
+    ms_step = (time.perf_counter() - t0) * 1000.0 / steps
+    return {"steps": steps, "ms_decoding": round(float(np.mean(ms_dom)), 4), "ms_per_step": round(ms_step, 4),
+            "ms_per_step_without": round(ms_plain, 4), "targets_decoded": st["n_past_fwd"],
+            "regions": st["n_regions"], "multidomain_regions": st["n_clustered"]}
+
+
 def precision_leg(p7gpu, synth, Ms=(2000, 3000, 4000, 5000), nseq=2000, nsub=100, seed=91):
     """Precision stress: profiles of M = 2000..5000 (calibrated by this library)
     against sequences of L = 2k..35k with 20% indel-rich homologs. The GPU
@@ -299,6 +318,7 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=20.0, help="Poisson arrival rate of the serving leg, queries/s")
     ap.add_argument("--serve-nseq", type=int, default=10_000_000, help="sequences of the serving leg's resident database")
     ap.add_argument("--serve-slots", type=int, default=4)
+    ap.add_argument("--domain-steps", type=int, default=5, help="steps of the pipeline with domain decoding (0: skip)")
     ap.add_argument("--batch-profiles", type=int, default=64, help="profiles in the batched leg (0: skip it)")
     ap.add_argument("--batch-nseq", type=int, default=500_000, help="database sequences of the batched leg")
     ap.add_argument("--no-tune", action="store_true", help="skip the tuner (keep the built-in ungapped-pass shape)")
@@ -394,6 +414,8 @@ def main():
                         "vit": int((stage >= 3).sum()), "fwd": int((stage >= 4).sum()),
                         "msv_full_rescored": st["n_msv_full"]},
     }
+    if args.domain_steps > 0:
+        line["domain_decoding"] = domain_leg(p7gpu, om, sdb, args.domain_steps, ms_per_step)
     if args.batch_profiles > 0:
         log("batched leg ...")
         line["batched"] = batched_leg(p7gpu, synth, args.batch_profiles, args.batch_nseq)
