@@ -212,13 +212,15 @@ def domain_leg(p7gpu, om, sdb, steps, ms_plain):
     pli.run(om, sdb, scores=False)
     st = pli.stats()                                 # counters of one call
     p7gpu.lib.p7_gpu_Synchronize()
-    ms_dom, t0 = [], time.perf_counter()
+    ms_dom, ms_fwd, ms_bck, t0 = [], [], [], time.perf_counter()
     for _ in range(steps):
         pli.run(om, sdb, scores=False)
-        ms_dom.append(pli.stats()["ms_domains"])
+        sx = pli.stats()
+        ms_dom.append(sx["ms_domains"]); ms_fwd.append(sx["ms_fwd"]); ms_bck.append(sx["ms_bck"])
     p7gpu.lib.p7_gpu_Synchronize()
     ms_step = (time.perf_counter() - t0) * 1000.0 / steps
     return {"steps": steps, "ms_decoding": round(float(np.mean(ms_dom)), 4), "ms_per_step": round(ms_step, 4),
+            "ms_fwd": round(float(np.mean(ms_fwd)), 4), "ms_bck": round(float(np.mean(ms_bck)), 4),
             "ms_per_step_without": round(ms_plain, 4), "targets_decoded": st["n_past_fwd"],
             "regions": st["n_regions"], "multidomain_regions": st["n_clustered"]}
 

@@ -88,6 +88,17 @@ __device__ __forceinline__ float wave_sum_f(float v)
   v += dppz<ROW_BCAST31, 0xc>(v);
   return readlanef(v, 63);
 }
+// The same moves leave every lane with the inclusive prefix sum over lanes 0..lane.
+__device__ __forceinline__ float wave_scan_f(float v)
+{
+  v += dppz<ROW_SHR1>(v);
+  v += dppz<ROW_SHR2>(v);
+  v += dppz<ROW_SHR4>(v);
+  v += dppz<ROW_SHR8>(v);
+  v += dppz<ROW_BCAST15, 0xa>(v);
+  v += dppz<ROW_BCAST31, 0xc>(v);
+  return v;
+}
 __device__ __forceinline__ int   uni(int v)   { return __builtin_amdgcn_readfirstlane(v); }
 __device__ __forceinline__ float unif(float v){ return __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, v))); }
 

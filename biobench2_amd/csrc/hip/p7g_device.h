@@ -126,6 +126,7 @@ struct DomOut {
   int32_t *nreg = nullptr, *nclu = nullptr;
   int4    *regs = nullptr;
   int64_t  reg_cap = 0;
+  int32_t *nclu_total = nullptr;   // regions flagged multidomain (zeroed before a run)
 };
 
 // Length model of one call. on == 0: every target uses its own length

@@ -682,7 +682,9 @@ __device__ __forceinline__ void affine_bck_scan(float (&D)[B], A a, const AffCon
   for (int j = B-1; j >= 0; j--) { cur = D[j] + a(j) * cur; D[j] = cur; }
 }
 
-template <int B, int TM>
+// DOM: the rows also go to the domain decoding pool (a separate instance, so
+// that the plain parser keeps its register budget)
+template <int B, int TM, bool DOM>
 __global__ __launch_bounds__(FWD_THREADS(B))
 void k_fwd(const DevProfile p, const uint8_t *__restrict__ res, const int64_t *__restrict__ off,
            const int32_t *__restrict__ len, const int32_t *__restrict__ list, const int32_t *__restrict__ count,
@@ -726,18 +728,18 @@ void k_fwd(const DevProfile p, const uint8_t *__restrict__ res, const int64_t *_
     if (pool) {
       unsigned long long a64 = 0;
       if (z == 0) a64 = atomicAdd(l.cursor, (unsigned long long) (L + 1));
-      const int64_t at = ((int64_t) (unsigned) __shfl((int) (a64 >> 32), 0, 64) << 32) | (unsigned) __shfl((int) a64, 0, 64);
+      const int64_t at = ((int64_t) (unsigned) uni((int) (a64 >> 32)) << 32) | (unsigned) uni((int) a64);
       if (at + L + 1 <= pool_size) { scl = pool + at; if (z == 0) scale_off[slot] = at; }
       else { if (z == 0) { scale_off[slot] = -1; atomicExch(&l.counts[6], 1); } }
     }
     // the special-state rows go to the caller's matrix (one target) or, for
     // domain decoding, to the target's entry in the decoding pool
     float *xo = xmx_out;
-    if (dom.pool) {
+    if (DOM) {
       const int64_t need = 6 * (int64_t) (L + 1);
       unsigned long long a64 = 0;
       if (z == 0) a64 = atomicAdd(l.cursor + 1, (unsigned long long) need);
-      const int64_t at = ((int64_t) (unsigned) __shfl((int) (a64 >> 32), 0, 64) << 32) | (unsigned) __shfl((int) a64, 0, 64);
+      const int64_t at = ((int64_t) (unsigned) uni((int) (a64 >> 32)) << 32) | (unsigned) uni((int) a64);
       if (at + need <= dom.size) { xo = dom.pool + at; if (z == 0) dom.doff[slot] = at; }
       else { xo = nullptr; if (z == 0) { dom.doff[slot] = -1; atomicOr(&l.counts[7], 1); } }
     }
@@ -813,10 +815,22 @@ void k_fwd(const DevProfile p, const uint8_t *__restrict__ res, const int64_t *_
       else sc = (float) ((double) totscale + log((double) (xC * fmove)));
       r.fwdsc[slot] = sc;
       r.fstatus[slot] = status;
+      bool pass = false;
       if (gating) {
         const float seq = (float) __ddiv_rn((double) (sc - r.filtersc[slot]), LOG2_D);
-        if (status == 0 && seq >= g.T3f) r.stage[slot] = 4;
-        else r.stage[slot] = 3;
+        pass = (status == 0 && seq >= g.T3f);
+        r.stage[slot] = pass ? 4 : 3;
+      }
+      // an F3 survivor's decoding entry: the Backward rows, then btot, etot, mocc
+      if (DOM) {
+        int64_t bo = -1;
+        if (pass && xo) {
+          const int64_t need = 9 * (int64_t) (L + 1);
+          const int64_t at = (int64_t) atomicAdd(l.cursor + 1, (unsigned long long) need);
+          if (at + need <= dom.size) bo = at;
+          else atomicOr(&l.counts[7], 1);
+        }
+        dom.boff[slot] = bo;
       }
     }
   }
@@ -825,7 +839,9 @@ void k_fwd(const DevProfile p, const uint8_t *__restrict__ res, const int64_t *_
 /*****************************************************************
  * 8. Backward parser (one target per wave), with Forward's scale factors
  *****************************************************************/
-template <int B, int TM>
+// DOM: the rows also go to the domain decoding pool (a separate instance, so
+// that the plain parser keeps its register budget)
+template <int B, int TM, bool DOM>
 __global__ __launch_bounds__(BCK_THREADS(B))
 void k_bck(const DevProfile p, const uint8_t *__restrict__ res, const int64_t *__restrict__ off,
            const int32_t *__restrict__ len, const int32_t *__restrict__ list, const int32_t *__restrict__ count,
@@ -871,21 +887,18 @@ void k_bck(const DevProfile p, const uint8_t *__restrict__ res, const int64_t *_
     else {
       const int64_t so = scale_off[slot];
       if (so < 0) {
-        if (z == 0) { r.bcksc[slot] = NAN; r.fstatus[slot] = 5; if (dom.pool) dom.boff[slot] = -1; }
+        if (z == 0) { r.bcksc[slot] = NAN; r.fstatus[slot] = 5; }
         continue;
       }
       fsc = pool + so;
     }
     // the special-state rows: the caller's matrix (one target), or the
-    // target's decoding entry (rows, then room for btot, etot and mocc)
+    // target's decoding entry that the Forward parser reserved
     float *xo = xmx_out;
-    if (dom.pool) {
-      const int64_t need = 9 * (int64_t) (L + 1);
-      unsigned long long a64 = 0;
-      if (z == 0) a64 = atomicAdd(l.cursor + 1, (unsigned long long) need);
-      const int64_t at = ((int64_t) (unsigned) __shfl((int) (a64 >> 32), 0, 64) << 32) | (unsigned) __shfl((int) a64, 0, 64);
-      if (at + need <= dom.size) { xo = dom.pool + at; if (z == 0) dom.boff[slot] = at; }
-      else { xo = nullptr; if (z == 0) { dom.boff[slot] = -1; atomicOr(&l.counts[7], 1); } }
+    if (DOM) {
+      const int64_t bo = dom.boff[slot];
+      const int64_t bu = ((int64_t) (unsigned) uni((int) (bo >> 32)) << 32) | (unsigned) uni((int) bo);
+      xo = (bu >= 0) ? dom.pool + bu : nullptr;
     }
     float M[B], I[B], D[B];
     float xJ = 0.0f, xB = 0.0f, xN = 0.0f;
@@ -988,75 +1001,138 @@ void k_bck(const DevProfile p, const uint8_t *__restrict__ res, const int64_t *_
 #undef BTR
 
 /*****************************************************************
- * 9. Domain decoding and regions (F3 survivors, one target per thread)
+ * 9. Domain decoding and regions (F3 survivors, one target per wave)
  *****************************************************************/
 // p7_DomainDecoding (HMMER/src/impl_sse/decoding.c:166-197) from the two
 // parsers' special rows: btot[i] / etot[i], the expected B / E visits up to
 // row i, and mocc[i], the probability that residue i is emitted by the core
-// model; in the reference's float operations and order. Then the region loop
-// of p7_domaindef_ByPosteriorHeuristics (p7_domaindef.c:409-489) with
-// is_multidomain_region (:534-547), thresholds rt1/rt2/rt3 of
+// model. Each row's terms are the reference's float products; the running
+// sums over rows are taken 64 rows at a time by a wave scan (the reference
+// adds them one by one, so the sums may differ in the last bits). When the
+// Backward parser used its own scale factors the scale product changes from
+// row to row, and the first lane walks the rows in the reference's order.
+// Then the region loop of p7_domaindef_ByPosteriorHeuristics
+// (p7_domaindef.c:409-489): its three occupancy tests are evaluated for 64
+// rows at once into ballots, and the loop's state machine steps through
+// their bits; a region's is_multidomain_region test (:534-547) is a wave
+// maximum over its split points. Thresholds rt1/rt2/rt3 of
 // p7_domaindef_Create (:109-111). Regions are appended as {slot, i, j,
 // multidomain} records.
 #define DOM_RT1 0.25f
 #define DOM_RT2 0.10f
 #define DOM_RT3 0.20f
-__global__ __launch_bounds__(64)
+__global__ __launch_bounds__(256)
 void k_domains(const DevProfile p, const int32_t *__restrict__ len, const int32_t *__restrict__ list,
                const int32_t *__restrict__ count, const int *__restrict__ fstatus, DevLists l, LenCfg lc,
                DomBufs dom, DomOut out)
 {
 #pragma clang fp contract(off)
   enum { E = 0, N = 1, J = 2, Bx = 3, C = 4, S = 5 };
+  const int lane = threadIdx.x & 63;
   const int64_t n = *count;
-  for (int64_t t = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (int64_t) gridDim.x * blockDim.x) {
-    const int slot = list[t];
-    const int L = len[slot];
+  for (int64_t t = wave_id(); t < n; t += wave_cnt()) {
+    const int slot = uni(list[t]);
+    const int L = uni(len[slot]);
     const int64_t fo = dom.doff[slot], bo = dom.boff[slot];
-    if (fo < 0 || bo < 0) { out.nexp[slot] = NAN; out.nreg[slot] = -1; out.nclu[slot] = -1; continue; }
+    if (fo < 0 || bo < 0) {
+      if (lane == 0) { out.nexp[slot] = NAN; out.nreg[slot] = -1; out.nclu[slot] = -1; }
+      continue;
+    }
     const float *fx = dom.pool + fo;
     const float *bx = dom.pool + bo;
     float *btot = dom.pool + bo + 6 * (int64_t) (L + 1), *etot = btot + (L + 1), *mocc = etot + (L + 1);
     const bool own = (fstatus[slot] & 0x100) != 0;
     const float loop = lc.on ? lc.floop : p.xf_loop[L];     // N, J and C loops are equal after ReconfigLength
-    float sp = __fdiv_rn(1.0f, bx[N]);
-    float bt = 0.0f, et = 0.0f;
-    btot[0] = 0.0f; etot[0] = 0.0f; mocc[0] = 0.0f;
-    for (int i = 1; i <= L; i++) {
-      const float *f0 = fx + (size_t) (i - 1) * 6, *f1 = fx + (size_t) i * 6;
-      const float *b0 = bx + (size_t) (i - 1) * 6, *b1 = bx + (size_t) i * 6;
-      bt = bt + (f0[Bx] * b0[Bx] * f0[S] * sp);
-      if (own) sp *= __fdiv_rn(f0[S], b0[S]);
-      et = et + (f1[E] * b1[E] * f1[S] * sp);
-      float njcp = f0[N] * b1[N] * loop * sp;
-      njcp += f0[J] * b1[J] * loop * sp;
-      njcp += f0[C] * b1[C] * loop * sp;
-      btot[i] = bt; etot[i] = et; mocc[i] = 1.0f - njcp;
-    }
-    out.nexp[slot] = bt;
-    int nreg = 0, nclu = 0, i0 = -1;
-    bool triggered = false;
-    for (int j = 1; j <= L; j++) {
-      if (! triggered) {
-        if      (mocc[j] - (btot[j] - btot[j-1]) < DOM_RT2) i0 = j;
-        else if (i0 == -1)                                   i0 = j;
-        if      (mocc[j] >= DOM_RT1)                         triggered = true;
-      } else if (mocc[j] - (etot[j] - etot[j-1]) < DOM_RT2) {
-        float mx = -1.0f;
-        for (int zz = i0; zz <= j; zz++) {
-          const float a = etot[zz] - etot[i0-1], b = btot[j] - btot[zz-1];
-          const float en = (a < b) ? a : b;
-          mx = (mx > en) ? mx : en;
+    const float sp0 = __fdiv_rn(1.0f, bx[N]);
+    if (lane == 0) { btot[0] = 0.0f; etot[0] = 0.0f; mocc[0] = 0.0f; }
+    if (! own) {
+      float cb = 0.0f, ce = 0.0f;                           // btot, etot at the row before the block
+      for (int base = 0; base < L; base += 64) {
+        const int i = base + lane + 1;
+        float db = 0.0f, de = 0.0f, mo = 0.0f;
+        if (i <= L) {
+          const float *f0 = fx + (size_t) (i - 1) * 6, *f1 = fx + (size_t) i * 6;
+          const float *b0 = bx + (size_t) (i - 1) * 6, *b1 = bx + (size_t) i * 6;
+          db = f0[Bx] * b0[Bx] * f0[S] * sp0;
+          de = f1[E] * b1[E] * f1[S] * sp0;
+          float njcp = f0[N] * b1[N] * loop * sp0;
+          njcp += f0[J] * b1[J] * loop * sp0;
+          njcp += f0[C] * b1[C] * loop * sp0;
+          mo = 1.0f - njcp;
         }
-        const int multi = (mx >= DOM_RT3) ? 1 : 0;
-        const int k = atomicAdd(&l.counts[5], 1);
-        if (k < out.reg_cap) out.regs[k] = make_int4(slot, i0, j, multi);
-        else atomicOr(&l.counts[7], 2);
-        nreg++; nclu += multi;
-        i0 = -1; triggered = false;
+        const float bt = cb + wave_scan_f(db), et = ce + wave_scan_f(de);
+        if (i <= L) { btot[i] = bt; etot[i] = et; mocc[i] = mo; }
+        cb = readlanef(bt, 63); ce = readlanef(et, 63);
+      }
+    } else if (lane == 0) {
+      float sp = sp0, bt = 0.0f, et = 0.0f;
+      for (int i = 1; i <= L; i++) {
+        const float *f0 = fx + (size_t) (i - 1) * 6, *f1 = fx + (size_t) i * 6;
+        const float *b0 = bx + (size_t) (i - 1) * 6, *b1 = bx + (size_t) i * 6;
+        bt = bt + (f0[Bx] * b0[Bx] * f0[S] * sp);
+        sp *= __fdiv_rn(f0[S], b0[S]);
+        et = et + (f1[E] * b1[E] * f1[S] * sp);
+        float njcp = f0[N] * b1[N] * loop * sp;
+        njcp += f0[J] * b1[J] * loop * sp;
+        njcp += f0[C] * b1[C] * loop * sp;
+        btot[i] = bt; etot[i] = et; mocc[i] = 1.0f - njcp;
       }
     }
-    out.nreg[slot] = nreg; out.nclu[slot] = nclu;
+    __threadfence_block();                                  // the rows above are read back by other lanes
+    if (lane == 0) out.nexp[slot] = btot[L];
+
+    int nreg = 0, nclu = 0, i0 = -1;
+    bool triggered = false;
+    for (int base = 0; base < L; base += 64) {
+      const int j = base + lane + 1;
+      bool cs = false, ct = false, ce = false;
+      if (j <= L) {
+        const float mo = mocc[j];
+        cs = mo - (btot[j] - btot[j-1]) < DOM_RT2;
+        ct = mo >= DOM_RT1;
+        ce = mo - (etot[j] - etot[j-1]) < DOM_RT2;
+      }
+      const unsigned long long ms = __ballot(cs), mt = __ballot(ct), me = __ballot(ce);
+      const int nr = min(64, L - base);
+      for (int r = 0; r < nr; r++) {                       // the loop's steps, on wave-uniform values
+        const int jj = base + r + 1;
+        if (! triggered) {
+          if      ((ms >> r) & 1ull) i0 = jj;
+          else if (i0 == -1)         i0 = jj;
+          if      ((mt >> r) & 1ull) triggered = true;
+        } else if ((me >> r) & 1ull) {
+          // is_multidomain_region(i0, jj): max over split points z of
+          // min(etot[z] - etot[i0-1], btot[jj] - btot[z-1])
+          const float e0 = etot[i0 - 1], bj = btot[jj];
+          float mx = -1.0f;
+          for (int z0 = i0; z0 <= jj; z0 += 64) {
+            const int z = z0 + lane;
+            float en = -1.0f;
+            if (z <= jj) {
+              const float a = etot[z] - e0, b = bj - btot[z - 1];
+              en = (a < b) ? a : b;
+            }
+            mx = fmaxf(mx, en);
+          }
+          {
+            float v = mx;
+            v = fmaxf(v, __shfl_xor(v, 1, 64)); v = fmaxf(v, __shfl_xor(v, 2, 64)); v = fmaxf(v, __shfl_xor(v, 4, 64));
+            v = fmaxf(v, __shfl_xor(v, 8, 64)); v = fmaxf(v, __shfl_xor(v, 16, 64)); v = fmaxf(v, __shfl_xor(v, 32, 64));
+            mx = unif(v);
+          }
+          const int multi = (mx >= DOM_RT3) ? 1 : 0;
+          if (lane == 0) {
+            const int k = atomicAdd(&l.counts[5], 1);
+            if (k < out.reg_cap) out.regs[k] = make_int4(slot, i0, jj, multi);
+            else atomicOr(&l.counts[7], 2);
+            if (multi) atomicAdd(out.nclu_total, 1);
+          }
+          nreg++; nclu += multi;
+          i0 = -1; triggered = false;
+        }
+      }
+    }
+    if (lane == 0) { out.nreg[slot] = nreg; out.nclu[slot] = nclu; }
   }
 }
 
@@ -1191,8 +1267,10 @@ static size_t fb_lds(const DevProfile &p, int tm)
   X(1, TM_LDS) X(2, TM_LDS) X(3, TM_LDS) X(4, TM_LDS) X(5, TM_LDS) X(6, TM_LDS) X(7, TM_LDS) X(8, TM_LDS) \
   X(10, TM_LDS) X(12, TM_LDS) X(14, TM_PIN) X(16, TM_TLDS) X(20, TM_TLDS) X(24, TM_TLDS) X(28, TM_TLDS) \
   X(32, TM_TLDS) X(40, TM_TLDS) X(48, TM_TLDS) X(64, TM_GLOB) X(80, TM_GLOB)
-#define FWD_CASE(N, T) case N * 4 + T: k = (void *) k_fwd<N, T>; nthr = FWD_THREADS(N); break;
-#define BCK_CASE(N, T) case N * 4 + T: k = (void *) k_bck<N, T>; nthr = BCK_THREADS(N); break;
+#define FWD_CASE(N, T) case N * 4 + T: k = dom.pool ? (void *) k_fwd<N, T, true> : (void *) k_fwd<N, T, false>; \
+                                          nthr = FWD_THREADS(N); break;
+#define BCK_CASE(N, T) case N * 4 + T: k = dom.pool ? (void *) k_bck<N, T, true> : (void *) k_bck<N, T, false>; \
+                                          nthr = BCK_THREADS(N); break;
 
 hipError_t launch_fwd(const DevProfile &p, const DevSeqs &s, const int32_t *list, const int32_t *count, int64_t maxn,
                       DevResults r, DevLists l, GateThresh g, int gating, LenCfg lc,
@@ -1239,8 +1317,8 @@ hipError_t launch_domains(const DevProfile &p, const DevSeqs &s, const int32_t *
 {
   if (maxn <= 0) return hipSuccess;
   if (! dom.pool || ! dom.doff || ! dom.boff || ! out.nexp || ! out.nreg || ! out.nclu || ! out.regs) return hipErrorInvalidValue;
-  const unsigned grid = (unsigned) std::min<int64_t>((maxn + 63) / 64, 1024);
-  hipLaunchKernelGGL(k_domains, dim3(grid), dim3(64), 0, st, p, s.len, list, count, (const int *) r.fstatus, l, lc, dom, out);
+  const unsigned grid = (unsigned) std::min<int64_t>((maxn + 3) / 4, 8192);
+  hipLaunchKernelGGL(k_domains, dim3(grid), dim3(256), 0, st, p, s.len, list, count, (const int *) r.fstatus, l, lc, dom, out);
   return hipGetLastError();
 }
 

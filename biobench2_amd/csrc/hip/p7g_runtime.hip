@@ -163,7 +163,7 @@ dev_profile_build(P7_GPU_OPROFILE *om, int Lmax_need, struct p7g_dev_profile_s *
     if (G == 16)      ND = (M <= 1024) ? std::max(1, cdiv(M, 32)) : -1;
     else if (G == 32) ND = (M <= 2048) ? pick(ssv32_set, (int) (sizeof(ssv32_set)/sizeof(int)), cdiv(M, 64)) : -1;
     else if (G == 64) ND = pick(ssv64_set, (int) (sizeof(ssv64_set)/sizeof(int)), cdiv(M, 128));
-    else              ND = (M <= 511) ? cdiv(M + 1, 16) : -1;
+    else              ND = (M <= 352) ? cdiv(M + 1, 16) : -1;   // where it measured faster (see ssv_def)
     if (ND > 0 && (size_t) NR * G * ND * 4 + 256 > P7G_LDS_MAX) ND = -1;   // the table (and a word) must fit in LDS
     if (ND < 0) { sv.G = 0; sv.ND = 0; sv.dwords = 0; continue; }
     sv.G = G; sv.ND = ND;
@@ -206,7 +206,8 @@ dev_profile_build(P7_GPU_OPROFILE *om, int Lmax_need, struct p7g_dev_profile_s *
   }
   // 8 lanes up to M = 352, 16 up to M = 1024, else 64; chunked beyond one LDS
   // table (on MI355X the 8-lane shape measured 1.03-1.36x faster than 16 lanes
-  // for M = 40..300, even at 400-500)
+  // for M = 40..300 in the tuner's sample runs; at M = 400 the samples were
+  // even, but the full pass ran 10% slower, so larger models are not offered it)
   p.ssv_def = (p.ssv[3].G != 0 && M <= 352) ? 3 : (p.ssv[0].G != 0) ? 0 : (p.ssv[2].G != 0) ? 2 : -1;
   p.mf_ND = pick(vf_nd_set, (int) (sizeof(vf_nd_set)/sizeof(int)), std::max(1, cdiv(cdiv(M, 64), 2)));
   if (p.mf_ND < 0) { delete d; p7g_set_error("model too long for the device MSV stage (M=%d)", M); return p7G_EINVAL; }
@@ -822,6 +823,7 @@ dom_reserve(Work &w, int64_t n, int64_t pool_floats)
   w.dom.doff = (int64_t *) take(8 * cap); w.dom.boff = (int64_t *) take(8 * cap);
   w.dout.nexp = (float *) take(4 * cap); w.dout.nreg = (int32_t *) take(4 * cap); w.dout.nclu = (int32_t *) take(4 * cap);
   w.dout.regs = (int4 *) take(16 * (size_t) DOM_REG_CAP); w.dout.reg_cap = DOM_REG_CAP;
+  w.dout.nclu_total = (int32_t *) take(4);
   if (o > total) { (void) hipFree(m); p7g_set_error("internal: domain buffer overrun"); return p7G_EINVAL; }
   w.dom_mem = m; w.dom_cap = cap;
   HIPCHK(hipMalloc((void **) &w.dom.pool, sizeof(float) * (size_t) pool_floats));
@@ -988,8 +990,15 @@ struct p7g_pipeline_s {
   double F1, F2, F3;
   int    do_biasfilter, do_max;
   int    do_domains = 0;      // posterior decoding and regions of the F3 survivors (p7_gpu_Pipeline)
-  std::vector<int4> regs;     // the last run's region records, in the caller's sequence order
+  // the last decoding run's region records: on the device until asked for,
+  // then in the caller's sequence order
+  mutable std::mutex regs_mu;
+  mutable std::vector<int4> regs;
+  mutable bool regs_ready = false;
   bool   regs_valid = false;
+  int64_t nregs = 0;
+  const int4 *regs_dev = nullptr;
+  const int64_t *regs_perm = nullptr;
   P7_GPU_STATS st;
   Work       w;             // this pipeline's stream and device work area
   Batch     *b = nullptr;   // work areas and graph of p7_gpu_PipelineBatch, made on first use
@@ -1197,6 +1206,7 @@ p7_gpu_Pipeline(P7_GPU_PIPELINE *pli, const P7_GPU_OPROFILE *om, const P7_GPU_SE
     HIPCHK(launch_fill32(w.dout.nexp, 0x7fc00000u, n, w.st));
     HIPCHK(launch_fill32(w.dout.nreg, 0u, n, w.st));
     HIPCHK(launch_fill32(w.dout.nclu, 0u, n, w.st));
+    HIPCHK(launch_fill32(w.dout.nclu_total, 0u, 1, w.st));
   }
   HIPCHK(hipEventRecord(w.ev[0], w.st));
   if ((status = work_bnd(w, *p, db->bytes)) != p7G_OK) return status;
@@ -1232,17 +1242,15 @@ p7_gpu_Pipeline(P7_GPU_PIPELINE *pli, const P7_GPU_OPROFILE *om, const P7_GPU_SE
       HIPCHK(hipMemcpyAsync(dl[a].dst, w.outbuf, 4 * (size_t) n, hipMemcpyDeviceToHost, w.st));
     }
   }
+  int32_t nclu_total = 0;
+  if (dom) HIPCHK(hipMemcpyAsync(&nclu_total, w.dout.nclu_total, 4, hipMemcpyDeviceToHost, w.st));
   HIPCHK(hipStreamSynchronize(w.st));
   if (counts[6]) { p7g_set_error("Forward scale-factor pool exhausted"); return p7G_EMEM; }
-  int64_t nregs = 0;
   if (dom) {
     if (counts[7]) { p7g_set_error("domain decoding %s exhausted", (counts[7] & 1) ? "row pool" : "region list"); return p7G_EMEM; }
-    nregs = counts[5];
-    std::vector<int4> rec((size_t) nregs);
-    if (nregs) HIPCHK(hipMemcpy(rec.data(), w.dout.regs, sizeof(int4) * (size_t) nregs, hipMemcpyDeviceToHost));
-    for (auto &r : rec) r.x = (int) db->h_perm[r.x];        // slot -> the caller's sequence index
-    std::sort(rec.begin(), rec.end(), [](const int4 &a, const int4 &b) { return a.x != b.x ? a.x < b.x : a.y < b.y; });
-    pli->regs.swap(rec);
+    std::lock_guard<std::mutex> rl(pli->regs_mu);
+    pli->nregs = counts[5]; pli->regs_dev = w.dout.regs; pli->regs_perm = db->h_perm;
+    pli->regs_ready = false; pli->regs.clear();
     pli->regs_valid = true;
   }
 
@@ -1261,8 +1269,8 @@ p7_gpu_Pipeline(P7_GPU_PIPELINE *pli, const P7_GPU_OPROFILE *om, const P7_GPU_SE
   if (dom) {
     HIPCHK(hipEventElapsedTime(&mtot, w.ev[5], w.ev[7]));
     st.ms_domains = mtot;
-    st.n_regions = nregs;
-    for (const auto &r : pli->regs) st.n_clustered += r.w;
+    st.n_regions = pli->nregs;
+    st.n_clustered = nclu_total;
   }
   return p7G_OK;
 }
@@ -1281,6 +1289,16 @@ p7_gpu_pipeline_GetRegions(const P7_GPU_PIPELINE *pli, int64_t *seq, int32_t *ie
                            int64_t cap, int64_t *ret_n)
 {
   if (! pli || ! pli->regs_valid) { p7g_set_error("no domain regions: the last p7_gpu_Pipeline call did not decode"); return p7G_EINVAL; }
+  std::lock_guard<std::mutex> rl(pli->regs_mu);
+  if (! pli->regs_ready) {
+    // slot -> the caller's sequence index, then sequence and start order
+    std::vector<int4> rec((size_t) pli->nregs);
+    if (pli->nregs) HIPCHK(hipMemcpy(rec.data(), pli->regs_dev, sizeof(int4) * (size_t) pli->nregs, hipMemcpyDeviceToHost));
+    for (auto &r : rec) r.x = (int) pli->regs_perm[r.x];
+    std::sort(rec.begin(), rec.end(), [](const int4 &a, const int4 &b) { return a.x != b.x ? a.x < b.x : a.y < b.y; });
+    pli->regs.swap(rec);
+    pli->regs_ready = true;
+  }
   const int64_t n = (int64_t) pli->regs.size();
   if (ret_n) *ret_n = n;
   for (int64_t k = 0; k < std::min(n, cap); k++) {

@@ -542,9 +542,10 @@ class Pipeline:
         _check(lib.p7_gpu_pipeline_SetDomains(self.h, int(self.domains)), "pipeline_SetDomains")
 
     def run(self, om, db, scores=True):
-        """p7_gpu_Pipeline; with domains=True the outputs also hold nexpected,
-        nregions and nclustered of the F3 survivors."""
-        out, hits = _hits(db.nseq, scores, self.domains)
+        """p7_gpu_Pipeline; with domains=True (and scores) the outputs also hold
+        nexpected, nregions and nclustered of the F3 survivors; regions() has
+        the regions either way."""
+        out, hits = _hits(db.nseq, scores, self.domains and scores)
         _check(lib.p7_gpu_Pipeline(self.h, om.h, db.h, ctypes.byref(hits)), "Pipeline")
         return out
 

@@ -260,7 +260,8 @@ int   p7_gpu_pipeline_SetDomains(P7_GPU_PIPELINE *pli, int on);
 /* The regions of the last p7_gpu_Pipeline call with decoding on, ordered by
  * sequence (the database's order) then start: sequence index, first and last
  * residue (1-based, inclusive), and 1 if flagged multidomain. Writes up to cap
- * entries (any array may be NULL); *ret_n = the number of regions. */
+ * entries (any array may be NULL); *ret_n = the number of regions. Valid until
+ * the pipeline's next call, while that call's database lives. */
 int   p7_gpu_pipeline_GetRegions(const P7_GPU_PIPELINE *pli, int64_t *seq, int32_t *ienv, int32_t *jenv,
                                  int32_t *multi, int64_t cap, int64_t *ret_n);
 /* ---- batches: many profiles against one resident database ----------------
