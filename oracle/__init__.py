@@ -215,6 +215,26 @@ def domain_decoding(fwd_xmx, bck_xmx, bck_own, xf):
 RT1, RT2, RT3 = 0.25, 0.10, 0.20          # p7_domaindef.c:109-111
 
 
+def unihit_xf(t, L):
+    """xf[4][2] of p7_oprofile_ReconfigUnihit(om, L) (impl_sse/p7_oprofile.c:1176-1186, 1109-1118):
+    E move 1, E loop 0, nj 0, so N/J/C move = 2/(L+2) and loop = 1 - move, in float."""
+    f = np.float32
+    xf = np.asarray(t["xf"], np.float32).reshape(4, 2).copy()
+    pmove = f(f(2.0) + f(0.0)) / f(f(L) + f(2.0) + f(0.0))
+    ploop = f(1.0) - pmove
+    xf[0, 0], xf[0, 1] = f(1.0), f(0.0)
+    for k in (1, 2, 3):
+        xf[k, 0], xf[k, 1] = pmove, ploop
+    return xf
+
+
+def envelope_score(residues, t, i, j):
+    """envsc of rescore_isolated_domain (p7_domaindef.c:812, 841): the Forward score of residues
+    i..j (1-based) with the profile in unihit mode, length-configured to the whole target."""
+    st, sc, _, _ = fwd_parser(np.asarray(residues)[i - 1:j], t, unihit_xf(t, len(residues)))
+    return st, sc
+
+
 def domain_regions(btot, etot, mocc, rt1=RT1, rt2=RT2, rt3=RT3):
     """[(i, j, multidomain)] of the region loop of p7_domaindef_ByPosteriorHeuristics."""
     L = btot.size - 1

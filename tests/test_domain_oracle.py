@@ -101,3 +101,26 @@ def test_fn3_tutorial_expected_domains():
            (1901, 1976), (1993, 2107)]
     assert [(i, j) for i, j, _ in regions] == env
     assert [m for _, _, m in regions[:8]] == [False] * 8
+
+
+def test_fn3_tutorial_envelope_scores():
+    """tutorial/fn3.out: each domain's score + bias (bits, one decimal each) is
+    (envsc + (L - Ld) log(L / (L + 3)) - nullsc) / ln 2 (p7_pipeline.c:845-849,
+    before the null2 correction that the bias column carries)."""
+    (name, s), = seqio.read_uniprot(golden("7LESS_DROME"))
+    L = len(s)
+    om = p7gpu.OProfile.from_file(golden("fn3.hmm"))
+    om.reconfig_length(L)
+    t = om.tables()
+    nullsc = oracle.null_one(L)
+    env = [(395, 411), (437, 521), (826, 914), (1203, 1259), (1304, 1386), (1739, 1769), (1799, 1891),
+           (1901, 1976), (1993, 2107)]
+    score_plus_bias = [-1.3, 40.7, 14.4, 5.1, 24.3, 0.0, 47.2 + 0.9, 17.8, 12.8]
+    for (i, j), ref in zip(env, score_plus_bias):
+        st, envsc = oracle.envelope_score(s, t, i, j)
+        assert st == 0
+        Ld = j - i + 1
+        raw = envsc + (L - Ld) * math.log(np.float32(L) / np.float32(L + 3))
+        bits = (raw - nullsc) / math.log(2.0)
+        assert abs(bits - ref) <= 0.1, (i, j, bits, ref)
+
