@@ -206,8 +206,9 @@ def batched_leg(p7gpu, synth, nprof, nseq, reps=3, seed=42):
 
 def domain_leg(p7gpu, om, sdb, steps, ms_plain):
     """The same pipeline with domain decoding of the F3 survivors on
-    (posterior decoding of the special states and the region heuristics):
-    the decoding kernel's time, the whole step, and what it found."""
+    (posterior decoding of the special states, the region heuristics and the
+    regions' envelope scores): their device time, the whole step, and what
+    they found."""
     pli = p7gpu.Pipeline(domains=True)
     pli.run(om, sdb, scores=False)
     st = pli.stats()                                 # counters of one call
@@ -219,7 +220,7 @@ def domain_leg(p7gpu, om, sdb, steps, ms_plain):
         ms_dom.append(sx["ms_domains"]); ms_fwd.append(sx["ms_fwd"]); ms_bck.append(sx["ms_bck"])
     p7gpu.lib.p7_gpu_Synchronize()
     ms_step = (time.perf_counter() - t0) * 1000.0 / steps
-    return {"steps": steps, "ms_decoding": round(float(np.mean(ms_dom)), 4), "ms_per_step": round(ms_step, 4),
+    return {"steps": steps, "ms_domains": round(float(np.mean(ms_dom)), 4), "ms_per_step": round(ms_step, 4),
             "ms_fwd": round(float(np.mean(ms_fwd)), 4), "ms_bck": round(float(np.mean(ms_bck)), 4),
             "ms_per_step_without": round(ms_plain, 4), "targets_decoded": st["n_past_fwd"],
             "regions": st["n_regions"], "multidomain_regions": st["n_clustered"]}

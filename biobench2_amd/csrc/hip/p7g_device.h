@@ -99,14 +99,14 @@ struct DevResults {
 // host round trips.
 struct DevLists {
   int32_t *full, *bias, *vit, *fwd, *bck;
-  // [0] full, [1] bias, [2] vit, [3] fwd, [4] bck: list lengths; [5] domain
-  // regions written; [6] scale-pool overflow flag; [7] domain-decoding
-  // overflow flags (1: row pool, 2: region list); the work-queue cursor of list
+  // [0] full, [1] bias, [2] vit, [3] fwd, [4] bck: list lengths; [6]
+  // scale-pool overflow flag; [7] domain-decoding overflow flags (1: row
+  // pool, 2: region list, 4: region residues); the work-queue cursor of list
   // k sits at [k + 8]; [13] is the cursor of the ungapped pass over all targets
   // ([14], [15]: the later launches of a chunked model). Zeroed before every call.
   int32_t *counts;
   // [0] next free float in the Forward scale-factor pool, [1] in the domain
-  // decoding pool
+  // decoding pool, [2] next free byte of the region residue copies
   unsigned long long *cursor;
 };
 
@@ -118,6 +118,10 @@ struct DomBufs {
   float   *pool = nullptr;
   int64_t  size = 0;
   int64_t *doff = nullptr, *boff = nullptr;
+  // envelope scoring (the Forward parser over region copies, unihit): the
+  // N/J/C loop and move odds of each region, and how many regions there are room for
+  const float *env_loop = nullptr, *env_move = nullptr;
+  int64_t  env_cap = 0;
 };
 // Per-target results of the decoding (slot order) and the region records
 // {slot, i, j, multidomain} in the order they were found.
@@ -127,6 +131,17 @@ struct DomOut {
   int4    *regs = nullptr;
   int64_t  reg_cap = 0;
   int32_t *nclu_total = nullptr;   // regions flagged multidomain (zeroed before a run)
+  // [0] regions written, [8] the envelope-scoring work-queue cursor (the list
+  // kernels' count/queue layout); zeroed before a run
+  int32_t *ecount = nullptr;
+  // per region k: its residues copied to res + v_off[k] (16-byte aligned,
+  // padded), v_len[k] residues, the unihit N/J/C odds for the whole target's
+  // length, v_list[k] = k, and the envelope score and status
+  uint8_t *res = nullptr;
+  int64_t  res_cap = 0;
+  int64_t *v_off = nullptr;
+  int32_t *v_len = nullptr, *v_list = nullptr, *env_st = nullptr;
+  float   *env_loop = nullptr, *env_move = nullptr, *envsc = nullptr;
 };
 
 // Length model of one call. on == 0: every target uses its own length

@@ -682,9 +682,11 @@ __device__ __forceinline__ void affine_bck_scan(float (&D)[B], A a, const AffCon
   for (int j = B-1; j >= 0; j--) { cur = D[j] + a(j) * cur; D[j] = cur; }
 }
 
-// DOM: the rows also go to the domain decoding pool (a separate instance, so
-// that the plain parser keeps its register budget)
-template <int B, int TM, bool DOM>
+// MODE 0: the parser. MODE 1: the rows also go to the domain decoding pool
+// (a separate instance, so that the plain parser keeps its register budget).
+// MODE 2: envelope scores: the list is of region copies, each scored in unihit
+// mode (E move 1, E loop 0) with the N/J/C odds of its whole target's length.
+template <int B, int TM, int MODE>
 __global__ __launch_bounds__(FWD_THREADS(B))
 void k_fwd(const DevProfile p, const uint8_t *__restrict__ res, const int64_t *__restrict__ off,
            const int32_t *__restrict__ len, const int32_t *__restrict__ list, const int32_t *__restrict__ count,
@@ -716,14 +718,17 @@ void k_fwd(const DevProfile p, const uint8_t *__restrict__ res, const int64_t *_
     for (int j = 0; j < B; j++) tdd[j] = tr[VT_DDS*rowf + j*64];
     aff_fwd_setup<B>(akc, tdd);
   }
-  const int64_t n = *count;
+  constexpr bool DOM = (MODE == 1);
+  const int64_t n = (MODE == 2) ? min((int64_t) *count, dom.env_cap) : (int64_t) *count;
+  const float eMove = (MODE == 2) ? 1.0f : p.xfE_move, eLoop = (MODE == 2) ? 0.0f : p.xfE_loop;
 
   int32_t *queue = (int32_t *) count + P7G_QOFF;
   for (int64_t t = queue_next(queue, 1); t < n; t = queue_next(queue, 1)) {
     const int slot = uni(list[t]);
     const int L = uni(len[slot]);
     const uint8_t *sq = res + off[slot];
-    const float floop = lc.on ? lc.floop : p.xf_loop[L], fmove = lc.on ? lc.fmove : p.xf_move[L];
+    const float floop = (MODE == 2) ? dom.env_loop[slot] : lc.on ? lc.floop : p.xf_loop[L];
+    const float fmove = (MODE == 2) ? dom.env_move[slot] : lc.on ? lc.fmove : p.xf_move[L];
     float *scl = nullptr;
     if (pool) {
       unsigned long long a64 = 0;
@@ -788,8 +793,8 @@ void k_fwd(const DevProfile p, const uint8_t *__restrict__ res, const int64_t *_
       for (int j = 0; j < B; j++) xEl += D[j];
       xE = unif(wave_sum_f(xEl));
       xN = xN * floop;
-      xC = (xC * floop) + (xE * p.xfE_move);
-      xJ = (xJ * floop) + (xE * p.xfE_loop);
+      xC = (xC * floop) + (xE * eMove);
+      xJ = (xJ * floop) + (xE * eLoop);
       xB = (xJ * fmove) + (xN * fmove);
       float scale = 1.0f;
       if (xE > 1.0e4f) {
@@ -1017,12 +1022,15 @@ void k_bck(const DevProfile p, const uint8_t *__restrict__ res, const int64_t *_
 // their bits; a region's is_multidomain_region test (:534-547) is a wave
 // maximum over its split points. Thresholds rt1/rt2/rt3 of
 // p7_domaindef_Create (:109-111). Regions are appended as {slot, i, j,
-// multidomain} records.
+// multidomain} records, each with a 16-byte aligned copy of its residues and
+// the unihit N/J/C odds of its target's length, for the envelope scores that
+// k_fwd (MODE 2) computes next.
 #define DOM_RT1 0.25f
 #define DOM_RT2 0.10f
 #define DOM_RT3 0.20f
 __global__ __launch_bounds__(256)
-void k_domains(const DevProfile p, const int32_t *__restrict__ len, const int32_t *__restrict__ list,
+void k_domains(const DevProfile p, const uint8_t *__restrict__ res, const int64_t *__restrict__ off,
+               const int32_t *__restrict__ len, const int32_t *__restrict__ list,
                const int32_t *__restrict__ count, const int *__restrict__ fstatus, DevLists l, LenCfg lc,
                DomBufs dom, DomOut out)
 {
@@ -1043,6 +1051,9 @@ void k_domains(const DevProfile p, const int32_t *__restrict__ len, const int32_
     float *btot = dom.pool + bo + 6 * (int64_t) (L + 1), *etot = btot + (L + 1), *mocc = etot + (L + 1);
     const bool own = (fstatus[slot] & 0x100) != 0;
     const float loop = lc.on ? lc.floop : p.xf_loop[L];     // N, J and C loops are equal after ReconfigLength
+    // N/J/C odds of p7_oprofile_ReconfigUnihit(om, L) (impl_sse/p7_oprofile.c:1176-1186,
+    // 1113-1114: nj = 0) for the envelope scores of this target's regions
+    const float umove = __fdiv_rn(2.0f + 0.0f, (float) L + 2.0f + 0.0f), uloop = 1.0f - umove;
     const float sp0 = __fdiv_rn(1.0f, bx[N]);
     if (lane == 0) { btot[0] = 0.0f; etot[0] = 0.0f; mocc[0] = 0.0f; }
     if (! own) {
@@ -1121,11 +1132,31 @@ void k_domains(const DevProfile p, const int32_t *__restrict__ len, const int32_
             mx = unif(v);
           }
           const int multi = (mx >= DOM_RT3) ? 1 : 0;
+          // record the region, and copy its residues for the envelope score
+          const int Ld = jj - i0 + 1;
+          const int64_t need = (((int64_t) Ld + 15) & ~(int64_t) 15) + 16;
+          int k = 0;
+          unsigned long long ro64 = 0;
           if (lane == 0) {
-            const int k = atomicAdd(&l.counts[5], 1);
-            if (k < out.reg_cap) out.regs[k] = make_int4(slot, i0, jj, multi);
-            else atomicOr(&l.counts[7], 2);
+            k = atomicAdd(&out.ecount[0], 1);
             if (multi) atomicAdd(out.nclu_total, 1);
+            if (k < out.reg_cap) ro64 = atomicAdd(l.cursor + 2, (unsigned long long) need);
+          }
+          k = uni(k);
+          const int64_t ro = ((int64_t) (unsigned) uni((int) (ro64 >> 32)) << 32) | (unsigned) uni((int) ro64);
+          if (k >= out.reg_cap) { if (lane == 0) atomicOr(&l.counts[7], 2); }
+          else {
+            const bool fits = ro + need <= out.res_cap;
+            if (fits) {
+              const uint8_t *src = res + off[slot] + (i0 - 1);
+              for (int64_t b = lane; b < need; b += 64) out.res[ro + b] = (b < Ld) ? src[b] : (uint8_t) P7G_PADCODE;
+            }
+            if (lane == 0) {
+              out.regs[k] = make_int4(slot, i0, jj, multi);
+              out.v_off[k] = fits ? ro : 0; out.v_len[k] = fits ? Ld : 0; out.v_list[k] = k;
+              out.env_loop[k] = uloop; out.env_move[k] = umove;
+              if (! fits) atomicOr(&l.counts[7], 4);
+            }
           }
           nreg++; nclu += multi;
           i0 = -1; triggered = false;
@@ -1267,8 +1298,8 @@ static size_t fb_lds(const DevProfile &p, int tm)
   X(1, TM_LDS) X(2, TM_LDS) X(3, TM_LDS) X(4, TM_LDS) X(5, TM_LDS) X(6, TM_LDS) X(7, TM_LDS) X(8, TM_LDS) \
   X(10, TM_LDS) X(12, TM_LDS) X(14, TM_PIN) X(16, TM_TLDS) X(20, TM_TLDS) X(24, TM_TLDS) X(28, TM_TLDS) \
   X(32, TM_TLDS) X(40, TM_TLDS) X(48, TM_TLDS) X(64, TM_GLOB) X(80, TM_GLOB)
-#define FWD_CASE(N, T) case N * 4 + T: k = dom.pool ? (void *) k_fwd<N, T, true> : (void *) k_fwd<N, T, false>; \
-                                          nthr = FWD_THREADS(N); break;
+#define FWD_CASE(N, T) case N * 4 + T: k = dom.env_loop ? (void *) k_fwd<N, T, 2> : dom.pool ? (void *) k_fwd<N, T, 1> \
+                                          : (void *) k_fwd<N, T, 0>; nthr = FWD_THREADS(N); break;
 #define BCK_CASE(N, T) case N * 4 + T: k = dom.pool ? (void *) k_bck<N, T, true> : (void *) k_bck<N, T, false>; \
                                           nthr = BCK_THREADS(N); break;
 
@@ -1316,9 +1347,11 @@ hipError_t launch_domains(const DevProfile &p, const DevSeqs &s, const int32_t *
                           hipStream_t st)
 {
   if (maxn <= 0) return hipSuccess;
-  if (! dom.pool || ! dom.doff || ! dom.boff || ! out.nexp || ! out.nreg || ! out.nclu || ! out.regs) return hipErrorInvalidValue;
+  if (! dom.pool || ! dom.doff || ! dom.boff || ! out.nexp || ! out.nreg || ! out.nclu || ! out.regs || ! out.ecount ||
+      ! out.res || ! out.v_off || ! out.v_len || ! out.v_list || ! out.env_loop || ! out.env_move) return hipErrorInvalidValue;
   const unsigned grid = (unsigned) std::min<int64_t>((maxn + 3) / 4, 8192);
-  hipLaunchKernelGGL(k_domains, dim3(grid), dim3(256), 0, st, p, s.len, list, count, (const int *) r.fstatus, l, lc, dom, out);
+  hipLaunchKernelGGL(k_domains, dim3(grid), dim3(256), 0, st, p, s.res, s.off, s.len, list, count, (const int *) r.fstatus,
+                     l, lc, dom, out);
   return hipGetLastError();
 }
 

@@ -801,9 +801,11 @@ work_bnd(Work &w, const DevProfile &p, int64_t bytes)
 }
 
 // Domain decoding buffers: per slot two pool offsets and three results, the
-// region records, and a row pool of 15 floats per row of an F3 survivor (6 per
-// row of a target that only reaches Forward), at most 2^28 floats.
+// region records with their envelope-scoring arrays and residue copies (at
+// most 2^26 bytes), and a row pool of 15 floats per row of an F3 survivor (6
+// per row of a target that only reaches Forward), at most 2^28 floats.
 #define DOM_REG_CAP ((int64_t) 1 << 20)
+#define DOM_RES_CAP ((int64_t) 1 << 26)
 static int64_t dom_pool_floats(int64_t nseq, int64_t nres) { return std::min<int64_t>(16 * (nres + nseq), (int64_t) 1 << 28); }
 
 static int
@@ -815,7 +817,8 @@ dom_reserve(Work &w, int64_t n, int64_t pool_floats)
   if (w.dom_mem) { (void) hipFree(w.dom_mem); w.dom_mem = nullptr; w.dom_cap = 0; }
   if (w.dom.pool) { (void) hipFree(w.dom.pool); w.dom.pool = nullptr; w.dom.size = 0; }
   const int64_t cap = std::max<int64_t>(n, 1024);
-  const size_t total = (size_t) cap * (8 + 8 + 4 + 4 + 4) + 16 * (size_t) DOM_REG_CAP + 4 * 256;
+  const size_t total = (size_t) cap * (8 + 8 + 4 + 4 + 4) + (16 + 8 + 4 * 6) * (size_t) DOM_REG_CAP + (size_t) DOM_RES_CAP
+                       + 16 * 256;
   char *m = nullptr;
   HIPCHK(hipMalloc((void **) &m, total));
   size_t o = 0;
@@ -824,6 +827,13 @@ dom_reserve(Work &w, int64_t n, int64_t pool_floats)
   w.dout.nexp = (float *) take(4 * cap); w.dout.nreg = (int32_t *) take(4 * cap); w.dout.nclu = (int32_t *) take(4 * cap);
   w.dout.regs = (int4 *) take(16 * (size_t) DOM_REG_CAP); w.dout.reg_cap = DOM_REG_CAP;
   w.dout.nclu_total = (int32_t *) take(4);
+  w.dout.ecount = (int32_t *) take(64);
+  w.dout.v_off = (int64_t *) take(8 * (size_t) DOM_REG_CAP);
+  w.dout.v_len = (int32_t *) take(4 * (size_t) DOM_REG_CAP); w.dout.v_list = (int32_t *) take(4 * (size_t) DOM_REG_CAP);
+  w.dout.env_st = (int32_t *) take(4 * (size_t) DOM_REG_CAP);
+  w.dout.env_loop = (float *) take(4 * (size_t) DOM_REG_CAP); w.dout.env_move = (float *) take(4 * (size_t) DOM_REG_CAP);
+  w.dout.envsc = (float *) take(4 * (size_t) DOM_REG_CAP);
+  w.dout.res = (uint8_t *) take((size_t) DOM_RES_CAP); w.dout.res_cap = DOM_RES_CAP;
   if (o > total) { (void) hipFree(m); p7g_set_error("internal: domain buffer overrun"); return p7G_EINVAL; }
   w.dom_mem = m; w.dom_cap = cap;
   HIPCHK(hipMalloc((void **) &w.dom.pool, sizeof(float) * (size_t) pool_floats));
@@ -994,10 +1004,12 @@ struct p7g_pipeline_s {
   // then in the caller's sequence order
   mutable std::mutex regs_mu;
   mutable std::vector<int4> regs;
+  mutable std::vector<float> envsc;
   mutable bool regs_ready = false;
   bool   regs_valid = false;
   int64_t nregs = 0;
   const int4 *regs_dev = nullptr;
+  const float *envsc_dev = nullptr;
   const int64_t *regs_perm = nullptr;
   P7_GPU_STATS st;
   Work       w;             // this pipeline's stream and device work area
@@ -1139,6 +1151,16 @@ chain_stages(Work &w, const DevProfile &p, const DevSeqs &s, int64_t n, const ui
   if (timed) HIPCHK(hipEventRecord(w.ev[5], w.st));
   if (domains) {
     HIPCHK(launch_domains(p, s, w.l.bck, &w.l.counts[4], n, w.r, w.l, lc, dom, w.dout, w.st));
+    // envelope scores: the Forward parser in unihit mode over the region copies
+    DevSeqs vs = s;
+    vs.res = w.dout.res; vs.off = w.dout.v_off; vs.len = w.dout.v_len; vs.perm = nullptr;
+    vs.seg = nullptr; vs.nseg = 0; vs.nseg_slots = 0; vs.nslot = w.dout.reg_cap;
+    DevResults vr = w.r;
+    vr.fwdsc = w.dout.envsc; vr.fstatus = w.dout.env_st;
+    DomBufs eb;
+    eb.env_loop = w.dout.env_loop; eb.env_move = w.dout.env_move; eb.env_cap = w.dout.reg_cap;
+    HIPCHK(launch_fwd(p, vs, w.dout.v_list, &w.dout.ecount[0], w.dout.reg_cap, vr, w.l, g, 0, lc, nullptr, 0, nullptr,
+                      nullptr, w.st, eb));
     if (timed) HIPCHK(hipEventRecord(w.ev[7], w.st));
   }
   return p7G_OK;
@@ -1207,6 +1229,7 @@ p7_gpu_Pipeline(P7_GPU_PIPELINE *pli, const P7_GPU_OPROFILE *om, const P7_GPU_SE
     HIPCHK(launch_fill32(w.dout.nreg, 0u, n, w.st));
     HIPCHK(launch_fill32(w.dout.nclu, 0u, n, w.st));
     HIPCHK(launch_fill32(w.dout.nclu_total, 0u, 1, w.st));
+    HIPCHK(launch_fill32(w.dout.ecount, 0u, 16, w.st));
   }
   HIPCHK(hipEventRecord(w.ev[0], w.st));
   if ((status = work_bnd(w, *p, db->bytes)) != p7G_OK) return status;
@@ -1242,14 +1265,20 @@ p7_gpu_Pipeline(P7_GPU_PIPELINE *pli, const P7_GPU_OPROFILE *om, const P7_GPU_SE
       HIPCHK(hipMemcpyAsync(dl[a].dst, w.outbuf, 4 * (size_t) n, hipMemcpyDeviceToHost, w.st));
     }
   }
-  int32_t nclu_total = 0;
-  if (dom) HIPCHK(hipMemcpyAsync(&nclu_total, w.dout.nclu_total, 4, hipMemcpyDeviceToHost, w.st));
+  int32_t nclu_total = 0, nregs_dev = 0;
+  if (dom) {
+    HIPCHK(hipMemcpyAsync(&nclu_total, w.dout.nclu_total, 4, hipMemcpyDeviceToHost, w.st));
+    HIPCHK(hipMemcpyAsync(&nregs_dev, w.dout.ecount, 4, hipMemcpyDeviceToHost, w.st));
+  }
   HIPCHK(hipStreamSynchronize(w.st));
   if (counts[6]) { p7g_set_error("Forward scale-factor pool exhausted"); return p7G_EMEM; }
   if (dom) {
-    if (counts[7]) { p7g_set_error("domain decoding %s exhausted", (counts[7] & 1) ? "row pool" : "region list"); return p7G_EMEM; }
+    if (counts[7]) {
+      p7g_set_error("domain decoding %s exhausted", (counts[7] & 1) ? "row pool" : (counts[7] & 2) ? "region list" : "region residue pool");
+      return p7G_EMEM;
+    }
     std::lock_guard<std::mutex> rl(pli->regs_mu);
-    pli->nregs = counts[5]; pli->regs_dev = w.dout.regs; pli->regs_perm = db->h_perm;
+    pli->nregs = nregs_dev; pli->regs_dev = w.dout.regs; pli->envsc_dev = w.dout.envsc; pli->regs_perm = db->h_perm;
     pli->regs_ready = false; pli->regs.clear();
     pli->regs_valid = true;
   }
@@ -1286,17 +1315,26 @@ p7_gpu_pipeline_SetDomains(P7_GPU_PIPELINE *pli, int on)
 
 extern "C" int
 p7_gpu_pipeline_GetRegions(const P7_GPU_PIPELINE *pli, int64_t *seq, int32_t *ienv, int32_t *jenv, int32_t *multi,
-                           int64_t cap, int64_t *ret_n)
+                           float *envsc, int64_t cap, int64_t *ret_n)
 {
   if (! pli || ! pli->regs_valid) { p7g_set_error("no domain regions: the last p7_gpu_Pipeline call did not decode"); return p7G_EINVAL; }
   std::lock_guard<std::mutex> rl(pli->regs_mu);
   if (! pli->regs_ready) {
     // slot -> the caller's sequence index, then sequence and start order
     std::vector<int4> rec((size_t) pli->nregs);
-    if (pli->nregs) HIPCHK(hipMemcpy(rec.data(), pli->regs_dev, sizeof(int4) * (size_t) pli->nregs, hipMemcpyDeviceToHost));
+    std::vector<float> sc((size_t) pli->nregs);
+    if (pli->nregs) {
+      HIPCHK(hipMemcpy(rec.data(), pli->regs_dev, sizeof(int4) * (size_t) pli->nregs, hipMemcpyDeviceToHost));
+      HIPCHK(hipMemcpy(sc.data(), pli->envsc_dev, sizeof(float) * (size_t) pli->nregs, hipMemcpyDeviceToHost));
+    }
     for (auto &r : rec) r.x = (int) pli->regs_perm[r.x];
-    std::sort(rec.begin(), rec.end(), [](const int4 &a, const int4 &b) { return a.x != b.x ? a.x < b.x : a.y < b.y; });
-    pli->regs.swap(rec);
+    std::vector<int64_t> ord((size_t) pli->nregs);
+    for (int64_t k = 0; k < pli->nregs; k++) ord[(size_t) k] = k;
+    std::sort(ord.begin(), ord.end(), [&](int64_t a, int64_t b) {
+      const int4 &x = rec[(size_t) a], &y = rec[(size_t) b];
+      return x.x != y.x ? x.x < y.x : x.y < y.y; });
+    pli->regs.resize((size_t) pli->nregs); pli->envsc.resize((size_t) pli->nregs);
+    for (int64_t k = 0; k < pli->nregs; k++) { pli->regs[(size_t) k] = rec[(size_t) ord[(size_t) k]]; pli->envsc[(size_t) k] = sc[(size_t) ord[(size_t) k]]; }
     pli->regs_ready = true;
   }
   const int64_t n = (int64_t) pli->regs.size();
@@ -1307,6 +1345,7 @@ p7_gpu_pipeline_GetRegions(const P7_GPU_PIPELINE *pli, int64_t *seq, int32_t *ie
     if (ienv) ienv[k] = r.y;
     if (jenv) jenv[k] = r.z;
     if (multi) multi[k] = r.w;
+    if (envsc) envsc[k] = pli->envsc[(size_t) k];
   }
   return p7G_OK;
 }

@@ -102,7 +102,7 @@ def _load():
         "p7_gpu_pipeline_GetStats": (i, [vp, POINTER(Stats)]),
         "p7_gpu_pipeline_SetDomains": (i, [vp, i]),
         "p7_gpu_pipeline_GetRegions": (i, [vp, POINTER(c_int64), POINTER(c_int32), POINTER(c_int32), POINTER(c_int32),
-                                           c_int64, POINTER(c_int64)]),
+                                           POINTER(c_float), c_int64, POINTER(c_int64)]),
         "p7_gpu_ScoreThreshold": (f, [i, d, d, d]),
         "p7_gpu_oprofile_ReadPressed": (i, [c_char_p, c_char_p, i, POINTER(vp)]),
         "p7_gpu_seqdb_ReadFasta": (i, [c_char_p, i, POINTER(vp)]),
@@ -550,14 +550,16 @@ class Pipeline:
         return out
 
     def regions(self):
-        """[(sequence index, i, j, multidomain)] of the last run with domain decoding."""
+        """[(sequence index, i, j, multidomain, envelope score)] of the last run with domain decoding."""
         n = c_int64()
-        _check(lib.p7_gpu_pipeline_GetRegions(self.h, None, None, None, None, 0, ctypes.byref(n)), "GetRegions")
+        _check(lib.p7_gpu_pipeline_GetRegions(self.h, None, None, None, None, None, 0, ctypes.byref(n)), "GetRegions")
         seq = np.zeros(n.value, np.int64)
         i, j, m = (np.zeros(n.value, np.int32) for _ in range(3))
+        sc = np.zeros(n.value, np.float32)
         _check(lib.p7_gpu_pipeline_GetRegions(self.h, _ptr(seq, c_int64), _ptr(i, c_int32), _ptr(j, c_int32),
-                                              _ptr(m, c_int32), n.value, ctypes.byref(n)), "GetRegions")
-        return [(int(seq[k]), int(i[k]), int(j[k]), bool(m[k])) for k in range(n.value)]
+                                              _ptr(m, c_int32), _ptr(sc, c_float), n.value, ctypes.byref(n)),
+               "GetRegions")
+        return [(int(seq[k]), int(i[k]), int(j[k]), bool(m[k]), float(sc[k])) for k in range(n.value)]
 
     def run_batch(self, oms, db, scores=True, graph=False):
         """p7_gpu_PipelineBatch: every profile of `oms` over the database in one

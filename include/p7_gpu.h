@@ -259,11 +259,15 @@ int   p7_gpu_pipeline_GetStats(const P7_GPU_PIPELINE *pli, P7_GPU_STATS *st);
 int   p7_gpu_pipeline_SetDomains(P7_GPU_PIPELINE *pli, int on);
 /* The regions of the last p7_gpu_Pipeline call with decoding on, ordered by
  * sequence (the database's order) then start: sequence index, first and last
- * residue (1-based, inclusive), and 1 if flagged multidomain. Writes up to cap
- * entries (any array may be NULL); *ret_n = the number of regions. Valid until
- * the pipeline's next call, while that call's database lives. */
+ * residue (1-based, inclusive), 1 if flagged multidomain, and the envelope
+ * score of the region (nats): the Forward score of residues i..j with the
+ * profile in unihit mode for the whole target's length, as
+ * rescore_isolated_domain computes envsc (p7_domaindef.c:812, 841) for a
+ * region that is one envelope (not multidomain). Writes up to cap entries
+ * (any array may be NULL); *ret_n = the number of regions. Valid until the
+ * pipeline's next call, while that call's database lives. */
 int   p7_gpu_pipeline_GetRegions(const P7_GPU_PIPELINE *pli, int64_t *seq, int32_t *ienv, int32_t *jenv,
-                                 int32_t *multi, int64_t cap, int64_t *ret_n);
+                                 int32_t *multi, float *envsc, int64_t cap, int64_t *ret_n);
 /* ---- batches: many profiles against one resident database ----------------
  * Runs the p7_gpu_Pipeline filter chain of nprof profiles over every database
  * sequence in one call. The ungapped MSV pass of all profiles that share a

@@ -111,7 +111,7 @@ def forced_tuning(tmp_path, entries):
 def test_every_shape_of_the_ungapped_pass_gives_the_same_results(setup, tmp_path, v, grab):
     sdb, oms, alone, alone_st = setup
     lanes = (16, 32, 64, 8)[v]
-    limit = {8: 511, 16: 1024, 32: 2048, 64: 5120}[lanes]
+    limit = {8: 352, 16: 1024, 32: 2048, 64: 5120}[lanes]
     idx = [i for i, om in enumerate(oms) if om.tables()["M"] <= limit]
     try:
         forced_tuning(tmp_path, [((oms[i].tables()["M"] + 31) // 32, v, grab) for i in idx])

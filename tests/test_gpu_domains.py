@@ -3,12 +3,14 @@ p7_DomainDecoding (HMMER/src/impl_sse/decoding.c:166-197) and the region
 heuristics of p7_domaindef_ByPosteriorHeuristics (HMMER/src/p7_domaindef.c:
 409-489, is_multidomain_region :534-547), against the oracle on the same
 targets and against the tutorial's fn3 vs 7LESS_DROME run (tutorial/fn3.out:
-exp 9.4, nine domain envelopes).
+exp 9.4, nine domain envelopes and their scores).
 
 The decoding reads the device parsers' special states, which equal the
 reference's to within float rounding, not bit for bit; so nexpected is
 compared within 1e-4 (relative), and a region list may differ from the
 oracle's only where some occupancy test lies within 1e-4 of its threshold."""
+import math
+
 import numpy as np
 import pytest
 
@@ -20,6 +22,7 @@ pytestmark = pytest.mark.gpu
 
 FN3_ENV = [(395, 411), (437, 521), (826, 914), (1203, 1259), (1304, 1386), (1739, 1769), (1799, 1891),
            (1901, 1976), (1993, 2107)]
+FN3_SCORE_PLUS_BIAS = [-1.3, 40.7, 14.4, 5.1, 24.3, 0.0, 47.2 + 0.9, 17.8, 12.8]
 
 
 def test_fn3_tutorial_regions_on_the_device(gpu):
@@ -35,9 +38,18 @@ def test_fn3_tutorial_regions_on_the_device(gpu):
     assert round(float(out["nexpected"][1]), 1) == 9.4
     assert out["nregions"][1] == 9
     regs = [r for r in pli.regions() if r[0] == 1]
-    assert [(i, j) for _, i, j, _ in regs] == FN3_ENV
-    assert [m for _, _, _, m in regs[:8]] == [False] * 8
-    assert out["nclustered"][1] == sum(m for _, _, _, m in regs)
+    assert [(i, j) for _, i, j, _, _ in regs] == FN3_ENV
+    assert [m for _, _, _, m, _ in regs[:8]] == [False] * 8
+    assert out["nclustered"][1] == sum(m for _, _, _, m, _ in regs)
+    # envelope scores: the oracle's, and tutorial/fn3.out's score + bias of each domain
+    t = om.tables()
+    L = len(s)
+    nullsc = oracle.null_one(L)
+    for (_, i, j, _, envsc), ref in zip(regs, FN3_SCORE_PLUS_BIAS):
+        st, osc = oracle.envelope_score(s, t, i, j)
+        assert st == 0 and abs(envsc - osc) <= 1e-4 * max(1.0, abs(osc)), (i, j, envsc, osc)
+        raw = envsc + (L - (j - i + 1)) * math.log(np.float32(L) / np.float32(L + 3))
+        assert abs((raw - nullsc) / math.log(2.0) - ref) <= 0.1, (i, j)
     for k in (0, 2, 3):                       # targets that did not pass every filter
         if out["stage"][k] != 4:
             assert np.isnan(out["nexpected"][k]) and out["nregions"][k] == 0
@@ -83,12 +95,13 @@ def test_device_decoding_equals_the_oracle(gpu, name, indel):
         assert np.array_equal(x[m], y[m]), k
     t = om.tables()
     lm = oracle.LengthModel(t, int(db.lengths.max()) + 1)
-    regs = {}
-    for sq, i, j, mu in pli.regions():
+    regs, envs = {}, {}
+    for sq, i, j, mu, esc in pli.regions():
         regs.setdefault(sq, []).append((i, j, mu))
+        envs[(sq, i, j)] = esc
     passed = np.nonzero(out["stage"] == 4)[0]
     assert passed.size >= 50
-    worst, differ = 0.0, 0
+    worst, differ, env_worst, nenv = 0.0, 0, 0.0, 0
     for q in passed:
         s = db.seq(int(q))
         L = s.size
@@ -104,7 +117,14 @@ def test_device_decoding_equals_the_oracle(gpu, name, indel):
         if got != ref:
             differ += 1
             assert margin(btot, etot, mocc, ref) < 1e-4, (q, got, ref)
+        for i, j, _ in got:                           # envelope scores against the oracle's
+            st4, osc = oracle.envelope_score(s, t, i, j)
+            esc = envs[(int(q), i, j)]
+            if st4 == 0:
+                env_worst = max(env_worst, abs(esc - osc) / max(1.0, abs(osc)))
+                nenv += 1
     assert worst <= 1e-4, worst
+    assert nenv > 0 and env_worst <= 1e-4, (nenv, env_worst)
     assert differ <= max(1, passed.size // 100)
     assert sum(len(v) for v in regs.values()) == pli.stats()["n_regions"] > 0
 
