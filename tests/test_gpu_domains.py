@@ -133,3 +133,25 @@ def test_regions_need_a_decoding_run(gpu):
     pli = gpu.Pipeline()
     with pytest.raises(gpu.P7GpuError):
         pli.regions()
+
+
+def test_decoding_repeats_and_reuses_its_buffers(gpu):
+    """Two calls give the same regions; a call over a smaller database after a
+    larger one (buffers reused, not cleared) gives that database's own regions."""
+    path = synth_profile("synth_M200")
+    hmm = synth.read_hmm_probs(path)
+    om = gpu.OProfile.from_file(path)
+    big = synth.make_db(4000, [hmm], frac_homolog=0.05, seed_db=41, seed_plant=42)
+    small = synth.make_db(600, [hmm], frac_homolog=0.05, seed_db=43, seed_plant=44)
+    sbig = gpu.SeqDB(big.residues, big.offsets, big.lengths)
+    ssmall = gpu.SeqDB(small.residues, small.offsets, small.lengths)
+    fresh = gpu.Pipeline(domains=True)
+    fresh.run(om, ssmall)
+    ref_small = fresh.regions()
+    pli = gpu.Pipeline(domains=True)
+    a = pli.run(om, sbig)
+    ra = pli.regions()
+    b = pli.run(om, sbig)
+    assert pli.regions() == ra and np.array_equal(a["nregions"], b["nregions"])
+    pli.run(om, ssmall)
+    assert pli.regions() == ref_small and len(ref_small) > 0
