@@ -17,7 +17,6 @@ typedef short s2v __attribute__((ext_vector_type(2)));
 // copy of the profile.
 #define LIST_BLOCK 512
 // 16-byte loads the bias filter's ordered sum keeps in flight.
-#define BIAS_AHEAD 8
 
 __device__ __forceinline__ int   s2i(s2v v) { return __builtin_bit_cast(int, v); }
 __device__ __forceinline__ s2v   i2s(int v) { return __builtin_bit_cast(s2v, v); }

@@ -307,44 +307,40 @@ void k_bias_log(float *__restrict__ rowmax, const unsigned long long *__restrict
     rowmax[i] = (float) log((double) rowmax[i]);
 }
 
-// One target per thread: its L+1 logs added in row order, as esl_hmm_Forward
-// sums sc[1..L+1] (esl_hmm.c:403-405), four at a time from aligned loads
-// issued a block ahead.
-__global__ __launch_bounds__(64)
+// One target per wave: its L+1 logs added in row order, as esl_hmm_Forward
+// sums sc[1..L+1] (esl_hmm.c:403-405). The wave loads 64 terms at a time
+// (coalesced) and the sum takes them lane by lane, so the order of the float
+// additions is the reference's; a long target no longer waits on its own
+// dependent loads, one per four terms.
+__global__ __launch_bounds__(256)
 void k_bias_sum(const DevProfile p, const int32_t *__restrict__ len, const int32_t *__restrict__ list,
                 const int32_t *__restrict__ count, DevResults r, GateThresh g, int gating, LenCfg lc,
                 const float *__restrict__ rowmax, const int64_t *__restrict__ defer_off)
 {
 #pragma clang fp contract(off)
+  const int lane = threadIdx.x & 63;
   const int64_t n = *count;
-  for (int64_t t = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (int64_t) gridDim.x * blockDim.x) {
-    const int slot = list[t];
-    const int64_t o = defer_off[slot];
+  for (int64_t t = wave_id(); t < n; t += wave_cnt()) {
+    const int slot = uni(list[t]);
+    const int64_t o0 = defer_off[slot];
+    const int64_t o = ((int64_t) (unsigned) uni((int) (o0 >> 32)) << 32) | (unsigned) uni((int) o0);
     if (o < 0) continue;                       // finished in the first pass
-    const int L = len[slot];
+    const int L = uni(len[slot]);
     const int nv = L + 1;                      // L row terms and the closing sum
-    const float4 *q = (const float4 *) (rowmax + o);
-    const int nq = (nv + 3) >> 2;
     float logsc = 0.0f;
-    // a ring of BIAS_AHEAD loads in flight: the ordered sum itself is a chain of dependent adds
-    float4 ring[BIAS_AHEAD];
+    float v = (lane < nv) ? rowmax[o + lane] : 0.0f;
+    for (int base = 0; base < nv; base += 64) {
+      const float nxt = (base + 64 + lane < nv) ? rowmax[o + base + 64 + lane] : 0.0f;    // the next block in flight
+      const int cnt = min(64, nv - base);
+      int k = 0;
+      for (; k + 8 <= cnt; k += 8) {
 #pragma unroll
-    for (int u = 0; u < BIAS_AHEAD; u++) ring[u] = (u < nq) ? q[u] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    for (int k0 = 0; k0 < nq; k0 += BIAS_AHEAD) {
-#pragma unroll
-      for (int u = 0; u < BIAS_AHEAD; u++) {
-        const int k = k0 + u;
-        if (k >= nq) break;
-        const float4 v = ring[u];
-        if (k + BIAS_AHEAD < nq) ring[u] = q[k + BIAS_AHEAD];
-        const int rem = nv - 4 * k;
-        logsc += v.x;
-        if (rem > 1) logsc += v.y;
-        if (rem > 2) logsc += v.z;
-        if (rem > 3) logsc += v.w;
+        for (int u = 0; u < 8; u++) logsc += readlanef(v, k + u);
       }
+      for (; k < cnt; k++) logsc += readlanef(v, k);
+      v = nxt;
     }
-    bias_finish(p, r, g, gating, lc, slot, L, logsc);
+    if (lane == 0) bias_finish(p, r, g, gating, lc, slot, L, logsc);
   }
 }
 
@@ -1248,7 +1244,8 @@ hipError_t launch_bias(const DevProfile &p, const DevSeqs &s, const int32_t *lis
   dev_props();
   hipLaunchKernelGGL(k_bias_log, dim3(ncu() * 8), dim3(256), 0, st, rowmax, (const unsigned long long *) l.cursor, rowmax_size);
   if ((e = hipGetLastError()) != hipSuccess) return e;
-  hipLaunchKernelGGL(k_bias_sum, dim3(grid), dim3(64), 0, st, p, s.len, list, count, r, g, gating, lc,
+  const unsigned wgrid = (unsigned) std::min<int64_t>((maxn + 3) / 4, 8192);
+  hipLaunchKernelGGL(k_bias_sum, dim3(wgrid), dim3(256), 0, st, p, s.len, list, count, r, g, gating, lc,
                      (const float *) rowmax, (const int64_t *) defer_off);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   // the scale-factor pool's cursor starts from zero again for the Forward stage
