@@ -837,6 +837,200 @@ void k_fwd(const DevProfile p, const uint8_t *__restrict__ res, const int64_t *_
   }
 }
 
+// Forward parser for large models, W waves per target. Each lane's B table
+// positions are split into W slices of B/W; wave w owns slice w of every lane,
+// with its transitions pinned in registers (one wave alone would have to read
+// them from memory every row). Along the chain (lane by lane, slice by slice)
+// a slice's first position takes its predecessor from the previous slice
+// through LDS. The D->D chain is composed per slice (constants of the row),
+// per lane (the W slices in order), then scanned across the lanes; the row sum
+// is the sum of the waves' parts. Four barriers per row order the exchanges.
+template <int B>
+__device__ __forceinline__ float lane_carry_fwd(float C, const AffConst &k, int z)
+{
+  // C: the lane's map constant (carry-in 0); returns D at the end of lane z-1
+  C = C + k.w1 * dppz<ROW_SHR1>(C);
+  C = C + k.w2 * dppz<ROW_SHR2>(C);
+  C = C + k.w4 * dppz<ROW_SHR4>(C);
+  C = C + k.w8 * dppz<ROW_SHR8>(C);
+  {
+    const float c1 = readlanef(C, 15);
+    const float c2 = k.pa * c1 + readlanef(C, 31);
+    const float c3 = k.pb * c2 + readlanef(C, 47);
+    const int row = z >> 4;
+    const float rc = (row == 0) ? 0.0f : (row == 1) ? c1 : (row == 2) ? c2 : c3;
+    C = C + k.prow * rc;
+  }
+  return dppz<WAVE_SHR1>(C);
+}
+
+__device__ __forceinline__ void aff_setup_lane(AffConst &k, float P)
+{
+  k.w1 = P; P = P * dppf<ROW_SHR1>(1.0f, P);
+  k.w2 = P; P = P * dppf<ROW_SHR2>(1.0f, P);
+  k.w4 = P; P = P * dppf<ROW_SHR4>(1.0f, P);
+  k.w8 = P; P = P * dppf<ROW_SHR8>(1.0f, P);
+  k.prow = P; k.pa = readlanef(P, 31); k.pb = readlanef(P, 47);
+}
+
+template <int B, int W>
+__global__ __launch_bounds__(64 * W)
+void k_fwd_wg(const DevProfile p, const uint8_t *__restrict__ res, const int64_t *__restrict__ off,
+              const int32_t *__restrict__ len, const int32_t *__restrict__ list, const int32_t *__restrict__ count,
+              DevResults r, DevLists l, GateThresh g, int gating, LenCfg lc,
+              float *__restrict__ pool, int64_t pool_size, int64_t *__restrict__ scale_off)
+{
+  static_assert(B % W == 0, "slices of equal width");
+  constexpr int BW = B / W;
+  __shared__ float sh_pm[W][64], sh_pi[W][64], sh_pd[W][64];  // previous row, each slice's last position
+  __shared__ float sh_m[W][64];                               // this row's M at each slice's last position
+  __shared__ float sh_c[W][64];                               // this row's slice D-chain constants
+  __shared__ float sh_e[W];                                   // the waves' parts of the row sum
+  __shared__ int64_t sh_t, sh_at;
+  if ((int64_t) blockIdx.x >= *count) return;
+  const int wv = threadIdx.x >> 6, z = threadIdx.x & 63;
+  const int j0 = wv * BW;
+  const size_t rowf = (size_t) 64 * B;
+  const float *tbl = p.fb_tbl;
+  const float *tr = tbl + (size_t) p.nrows * rowf + z;
+  float tBM[BW], tMM[BW], tIM[BW], tDM[BW], tMDS[BW], tMI[BW], tII[BW], tDD[BW];
+#pragma unroll
+  for (int j = 0; j < BW; j++) {
+    const size_t c = (size_t) (j0 + j) * 64;
+    tBM[j] = tr[VT_BM*rowf + c]; tMM[j] = tr[VT_MM*rowf + c]; tIM[j] = tr[VT_IM*rowf + c]; tDM[j] = tr[VT_DM*rowf + c];
+    tMDS[j] = tr[VT_MDS*rowf + c]; tMI[j] = tr[VT_MI*rowf + c]; tII[j] = tr[VT_II*rowf + c]; tDD[j] = tr[VT_DDS*rowf + c];
+  }
+  // the D->D products of every slice of this lane, and of the whole lane
+  float Pv[W];
+#pragma unroll
+  for (int v = 0; v < W; v++) {
+    float P = 1.0f;
+    for (int j = 0; j < BW; j++) P *= tr[VT_DDS*rowf + (size_t) (v * BW + j) * 64];
+    Pv[v] = P;
+  }
+  AffConst akl;
+  {
+    float P = 1.0f;
+#pragma unroll
+    for (int v = 0; v < W; v++) P *= Pv[v];
+    aff_setup_lane(akl, P);
+  }
+  const int64_t n = *count;
+  int32_t *queue = (int32_t *) count + P7G_QOFF;
+
+  for (;;) {
+    if (threadIdx.x == 0) sh_t = (int64_t) atomicAdd(queue, 1);
+    __syncthreads();
+    const int64_t t = sh_t;
+    if (t >= n) break;
+    const int slot = list[t];
+    const int L = len[slot];
+    const uint8_t *sq = res + off[slot];
+    const float floop = lc.on ? lc.floop : p.xf_loop[L], fmove = lc.on ? lc.fmove : p.xf_move[L];
+    if (threadIdx.x == 0) {
+      int64_t at = -1;
+      if (pool) {
+        at = (int64_t) atomicAdd(l.cursor, (unsigned long long) (L + 1));
+        if (at + L + 1 > pool_size) { at = -1; atomicExch(&l.counts[6], 1); }
+        scale_off[slot] = at;
+      }
+      sh_at = at;
+    }
+    sh_pm[wv][z] = 0.0f; sh_pi[wv][z] = 0.0f; sh_pd[wv][z] = 0.0f;
+    __syncthreads();
+    float *scl = (pool && sh_at >= 0) ? pool + sh_at : nullptr;
+    float M[BW], I[BW], D[BW];
+#pragma unroll
+    for (int j = 0; j < BW; j++) M[j] = I[j] = D[j] = 0.0f;
+    float xN = 1.0f, xJ = 0.0f, xC = 0.0f, xB = fmove, xE = 0.0f;
+    float totscale = 0.0f;
+    if (scl && threadIdx.x == 0) scl[0] = 1.0f;
+    ResAsc rd;
+    if (L > 0) rd.init(sq);
+    for (int i = 1; i <= L; i++) {
+      const int x = rd.at(i-1);
+      const float *e = tbl + (size_t) x * rowf + z;
+      __syncthreads();                                  // the previous row's slice ends are written
+      float mL = 0.0f, iL = 0.0f, dL = 0.0f;
+      if (wv > 0)     { mL = sh_pm[wv-1][z];  iL = sh_pi[wv-1][z];  dL = sh_pd[wv-1][z]; }
+      else if (z > 0) { mL = sh_pm[W-1][z-1]; iL = sh_pi[W-1][z-1]; dL = sh_pd[W-1][z-1]; }
+      float xEl = 0.0f;
+#pragma unroll
+      for (int j = BW-1; j >= 0; j--) {
+        const float mp = j ? M[j-1] : mL, ip = j ? I[j-1] : iL, dp = j ? D[j-1] : dL;
+        const float In = M[j] * tMI[j] + I[j] * tII[j];
+        float sv = xB * tBM[j] + mp * tMM[j] + ip * tIM[j] + dp * tDM[j];
+        sv = sv * e[(size_t) (j0 + j) * 64];
+        M[j] = sv; I[j] = In;
+        xEl += sv;
+      }
+      sh_m[wv][z] = M[BW-1];
+      __syncthreads();                                  // this row's slice-end M values are written
+      const float mS = (wv > 0) ? sh_m[wv-1][z] : (z > 0) ? sh_m[W-1][z-1] : 0.0f;
+#pragma unroll
+      for (int j = 0; j < BW; j++) D[j] = (j ? M[j-1] : mS) * tMDS[j];
+      {
+        float C = 0.0f;
+#pragma unroll
+        for (int j = 0; j < BW; j++) C = D[j] + tDD[j] * C;
+        sh_c[wv][z] = C;
+      }
+      __syncthreads();                                  // the slices' chain constants are written
+      {
+        float Cl = 0.0f;
+#pragma unroll
+        for (int v = 0; v < W; v++) Cl = sh_c[v][z] + Pv[v] * Cl;
+        float c = lane_carry_fwd<B>(Cl, akl, z);
+        for (int v = 0; v < wv; v++) c = sh_c[v][z] + Pv[v] * c;
+        float cur = c;
+#pragma unroll
+        for (int j = 0; j < BW; j++) { cur = D[j] + tDD[j] * cur; D[j] = cur; }
+      }
+#pragma unroll
+      for (int j = 0; j < BW; j++) xEl += D[j];
+      {
+        const float ws = wave_sum_f(xEl);
+        if (z == 0) sh_e[wv] = ws;
+      }
+      __syncthreads();                                  // the waves' parts of the row sum are written
+      xE = 0.0f;
+#pragma unroll
+      for (int v = 0; v < W; v++) xE += sh_e[v];
+      xN = xN * floop;
+      xC = (xC * floop) + (xE * p.xfE_move);
+      xJ = (xJ * floop) + (xE * p.xfE_loop);
+      xB = (xJ * fmove) + (xN * fmove);
+      float scale = 1.0f;
+      if (xE > 1.0e4f) {
+        xN = xN / xE; xC = xC / xE; xJ = xJ / xE; xB = xB / xE;
+        const float inv = (float) (1.0 / (double) xE);
+#pragma unroll
+        for (int j = 0; j < BW; j++) { M[j] *= inv; D[j] *= inv; I[j] *= inv; }
+        scale = xE;
+        totscale = (float) ((double) totscale + log((double) xE));
+        xE = 1.0f;
+      }
+      if (scl && threadIdx.x == 0) scl[i] = scale;
+      sh_pm[wv][z] = M[BW-1]; sh_pi[wv][z] = I[BW-1]; sh_pd[wv][z] = D[BW-1];
+    }
+    if (threadIdx.x == 0) {
+      int status = 0;
+      float sc;
+      if (isnan(xC) || (L > 0 && xC == 0.0f) || isinf(xC)) { status = 16; sc = NAN; }
+      else sc = (float) ((double) totscale + log((double) (xC * fmove)));
+      r.fwdsc[slot] = sc;
+      r.fstatus[slot] = status;
+      bool pass = false;
+      if (gating) {
+        const float seq = (float) __ddiv_rn((double) (sc - r.filtersc[slot]), LOG2_D);
+        pass = (status == 0 && seq >= g.T3f);
+        r.stage[slot] = pass ? 4 : 3;
+      }
+    }
+    __syncthreads();                                    // the target is done before the next one is taken
+  }
+}
+
 /*****************************************************************
  * 8. Backward parser (one target per wave), with Forward's scale factors
  *****************************************************************/
@@ -1306,6 +1500,14 @@ hipError_t launch_fwd(const DevProfile &p, const DevSeqs &s, const int32_t *list
                       const DomBufs &dom)
 {
   if (maxn <= 0) return hipSuccess;
+  // the largest models run the parser with eight waves per target
+  if (! dom.pool && ! dom.env_loop && ! xmx_out && (p.fb_B == 64 || p.fb_B == 80)) {
+    void *kw = (p.fb_B == 64) ? (void *) k_fwd_wg<64, 8> : (void *) k_fwd_wg<80, 8>;
+    const unsigned grid = grid_for_waves_blk(kw, maxn * 8, 0, 8);
+    void *args[] = { (void *) &p, (void *) &s.res, (void *) &s.off, (void *) &s.len, (void *) &list, (void *) &count,
+                     &r, &l, &g, &gating, &lc, &scale_pool, &pool_size, &scale_off };
+    return launch(kw, grid, 512, args, 0, st);
+  }
   const int tm = fb_mode(p);
   void *k = nullptr;
   int nthr = LIST_BLOCK;
