@@ -1357,6 +1357,216 @@ void k_domains(const DevProfile p, const uint8_t *__restrict__ res, const int64_
   }
 }
 
+// Backward parser for large models, W waves per target: the slices of
+// k_fwd_wg, traversed the other way. A slice's last position takes its
+// successor (the next slice's first position) through LDS; the D->D chain runs
+// towards smaller k (slice constants, lanes composed from their last slice,
+// scanned across lanes from lane 63 down).
+__device__ __forceinline__ float lane_carry_bck(float C, const AffConst &k, int z)
+{
+  // C: the lane's map constant (carry-in 0 from the right); returns D at the
+  // first position of lane z+1
+  C = C + k.w1 * dppz<ROW_SHL1>(C);
+  C = C + k.w2 * dppz<ROW_SHL2>(C);
+  C = C + k.w4 * dppz<ROW_SHL4>(C);
+  C = C + k.w8 * dppz<ROW_SHL8>(C);
+  {
+    const float c2 = readlanef(C, 48);
+    const float c1 = k.pa * c2 + readlanef(C, 32);
+    const float c0 = k.pb * c1 + readlanef(C, 16);
+    const int row = z >> 4;
+    const float rc = (row == 3) ? 0.0f : (row == 2) ? c2 : (row == 1) ? c1 : c0;
+    C = C + k.prow * rc;
+  }
+  return dppz<WAVE_SHL1>(C);
+}
+
+__device__ __forceinline__ void aff_setup_lane_bck(AffConst &k, float P)
+{
+  k.w1 = P; P = P * dppf<ROW_SHL1>(1.0f, P);
+  k.w2 = P; P = P * dppf<ROW_SHL2>(1.0f, P);
+  k.w4 = P; P = P * dppf<ROW_SHL4>(1.0f, P);
+  k.w8 = P; P = P * dppf<ROW_SHL8>(1.0f, P);
+  k.prow = P; k.pa = readlanef(P, 32); k.pb = readlanef(P, 16);
+}
+
+template <int B, int W>
+__global__ __launch_bounds__(64 * W)
+void k_bck_wg(const DevProfile p, const uint8_t *__restrict__ res, const int64_t *__restrict__ off,
+              const int32_t *__restrict__ len, const int32_t *__restrict__ list, const int32_t *__restrict__ count,
+              DevResults r, const float *__restrict__ pool, const int64_t *__restrict__ scale_off, LenCfg lc)
+{
+  static_assert(B % W == 0, "slices of equal width");
+  constexpr int BW = B / W;
+  __shared__ float sh_mn[W][64];     // M * e at each slice's first position (row being formed)
+  __shared__ float sh_c[W][64];      // the slices' D-chain constants
+  __shared__ float sh_d0[W][64];     // D at each slice's first position after the chain
+  __shared__ float sh_x[W];          // the waves' parts of the B sum
+  __shared__ int64_t sh_t;
+  if ((int64_t) blockIdx.x >= *count) return;
+  const int wv = threadIdx.x >> 6, z = threadIdx.x & 63;
+  const int j0 = wv * BW;
+  const size_t rowf = (size_t) 64 * B;
+  const float *tbl = p.fb_tbl;
+  const float *tr = tbl + (size_t) p.nrows * rowf + z;
+  float tBM[BW], tMMN[BW], tIMN[BW], tDMN[BW], tMD[BW], tDD[BW], tMI[BW], tII[BW];
+#pragma unroll
+  for (int j = 0; j < BW; j++) {
+    const size_t c = (size_t) (j0 + j) * 64;
+    tBM[j] = tr[VT_BM*rowf + c]; tMMN[j] = tr[FT_MMN*rowf + c]; tIMN[j] = tr[FT_IMN*rowf + c]; tDMN[j] = tr[FT_DMN*rowf + c];
+    tMD[j] = tr[FT_MD*rowf + c]; tDD[j] = tr[FT_DD*rowf + c]; tMI[j] = tr[VT_MI*rowf + c]; tII[j] = tr[VT_II*rowf + c];
+  }
+  float Pv[W];
+#pragma unroll
+  for (int v = 0; v < W; v++) {
+    float P = 1.0f;
+    for (int j = BW-1; j >= 0; j--) P *= tr[FT_DD*rowf + (size_t) (v * BW + j) * 64];
+    Pv[v] = P;
+  }
+  AffConst akl;
+  {
+    float P = 1.0f;
+#pragma unroll
+    for (int v = W-1; v >= 0; v--) P *= Pv[v];
+    aff_setup_lane_bck(akl, P);
+  }
+  const int64_t n = *count;
+  int32_t *queue = (int32_t *) count + P7G_QOFF;
+
+  // the D->D chain of the row in D (towards smaller k), then M += (D of the
+  // next position) * tMD; two barriers
+  auto dchain_and_m = [&](float (&M)[BW], float (&D)[BW], float add) {
+    {
+      float C = 0.0f;
+#pragma unroll
+      for (int j = BW-1; j >= 0; j--) C = D[j] + tDD[j] * C;
+      sh_c[wv][z] = C;
+    }
+    __syncthreads();
+    {
+      float Cl = 0.0f;
+#pragma unroll
+      for (int v = W-1; v >= 0; v--) Cl = sh_c[v][z] + Pv[v] * Cl;
+      float c = lane_carry_bck(Cl, akl, z);
+      for (int v = W-1; v > wv; v--) c = sh_c[v][z] + Pv[v] * c;
+      float cur = c;
+#pragma unroll
+      for (int j = BW-1; j >= 0; j--) { cur = D[j] + tDD[j] * cur; D[j] = cur; }
+      sh_d0[wv][z] = D[0];
+    }
+    __syncthreads();
+    const float dS = (wv < W-1) ? sh_d0[wv+1][z] : (z < 63) ? sh_d0[0][z+1] : 0.0f;
+#pragma unroll
+    for (int j = 0; j < BW; j++) M[j] = M[j] + add + (j < BW-1 ? D[j+1] : dS) * tMD[j];
+  };
+
+  for (;;) {
+    if (threadIdx.x == 0) sh_t = (int64_t) atomicAdd(queue, 1);
+    __syncthreads();
+    const int64_t t = sh_t;
+    if (t >= n) break;
+    const int slot = list[t];
+    const int L = len[slot];
+    const uint8_t *sq = res + off[slot];
+    const float floop = lc.on ? lc.floop : p.xf_loop[L], fmove = lc.on ? lc.fmove : p.xf_move[L];
+    const int64_t so = scale_off[slot];
+    if (so < 0) {
+      if (threadIdx.x == 0) { r.bcksc[slot] = NAN; r.fstatus[slot] = 5; }
+      __syncthreads();
+      continue;
+    }
+    const float *fsc = pool + so;
+    float M[BW], I[BW], D[BW];
+    float xJ = 0.0f, xB = 0.0f, xN = 0.0f;
+    float xC = fmove;
+    float xE = xC * p.xfE_move;
+#pragma unroll
+    for (int j = 0; j < BW; j++) { M[j] = xE; D[j] = xE; I[j] = 0.0f; }
+    dchain_and_m(M, D, 0.0f);
+    int own = 0;
+    float totscale;
+    {
+      const float s = fsc[L];
+      if (s > 1.0f) {
+        xE = xE / s; xN = xN / s; xC = xC / s; xJ = xJ / s; xB = xB / s;
+        const float inv = (float) (1.0 / (double) s);
+#pragma unroll
+        for (int j = 0; j < BW; j++) { M[j] *= inv; D[j] *= inv; I[j] *= inv; }
+      }
+      totscale = (float) log((double) s);
+    }
+    ResDesc rd;
+    if (L > 0) rd.init(sq, L);
+    for (int i = L-1; i >= 1; i--) {
+      const int x = rd.at(i);                         // residue i+1 (1-based)
+      const float *e = tbl + (size_t) x * rowf + z;
+      // M(j) * e(j) of the slice's first position, for the previous slice's last one
+      sh_mn[wv][z] = M[0] * e[(size_t) j0 * 64];
+      __syncthreads();
+      const float mS = (wv < W-1) ? sh_mn[wv+1][z] : (z < 63) ? sh_mn[0][z+1] : 0.0f;
+      float mn = M[0] * e[(size_t) j0 * 64];
+      float xBl = 0.0f;
+      xBl += mn * tBM[0];
+#pragma unroll
+      for (int j = 0; j < BW; j++) {
+        float ms = mS;
+        if (j < BW-1) { ms = M[j+1] * e[(size_t) (j0 + j + 1) * 64]; xBl += ms * tBM[j+1]; }
+        const float ip = I[j];
+        I[j] = ip * tII[j] + ms * tIMN[j];
+        D[j] = ms * tDMN[j];
+        M[j] = ip * tMI[j] + ms * tMMN[j];
+      }
+      {
+        const float ws = wave_sum_f(xBl);
+        if (z == 0) sh_x[wv] = ws;
+      }
+      __syncthreads();
+      xB = 0.0f;
+#pragma unroll
+      for (int v = 0; v < W; v++) xB += sh_x[v];
+      xC = xC * floop;
+      xJ = (xB * fmove) + (xJ * floop);
+      xN = (xB * fmove) + (xN * floop);
+      xE = (xC * p.xfE_move) + (xJ * p.xfE_loop);
+#pragma unroll
+      for (int j = 0; j < BW; j++) D[j] = D[j] + xE;
+      dchain_and_m(M, D, xE);
+      if (xB > 1.0e16f) own = 1;
+      const float s = own ? ((xB > 1.0e4f) ? xB : 1.0f) : fsc[i];
+      if (s > 1.0f) {
+        xE /= s; xN /= s; xJ /= s; xB /= s; xC /= s;
+        const float inv = (float) (1.0 / (double) s);
+#pragma unroll
+        for (int j = 0; j < BW; j++) { M[j] *= inv; D[j] *= inv; I[j] *= inv; }
+        totscale = (float) ((double) totscale + log((double) s));
+      }
+    }
+    {
+      const int x = (L > 0) ? rd.at(0) : P7G_PADCODE;
+      const float *e = tbl + (size_t) x * rowf + z;
+      float xBl = 0.0f;
+#pragma unroll
+      for (int j = 0; j < BW; j++) xBl += (M[j] * e[(size_t) (j0 + j) * 64]) * tBM[j];
+      const float ws = wave_sum_f(xBl);
+      if (z == 0) sh_x[wv] = ws;
+      __syncthreads();
+      xB = 0.0f;
+#pragma unroll
+      for (int v = 0; v < W; v++) xB += sh_x[v];
+      xN = (xB * fmove) + (xN * floop);
+    }
+    if (threadIdx.x == 0) {
+      int status = 0;
+      float sc;
+      if (isnan(xN) || (L > 0 && xN == 0.0f) || isinf(xN)) { status = 16; sc = NAN; }
+      else sc = (float) ((double) totscale + log((double) xN));
+      r.bcksc[slot] = sc;
+      r.fstatus[slot] = status | (own ? 0x100 : 0);
+    }
+    __syncthreads();                                    // the target is done before the next one is taken
+  }
+}
+
 __global__ void k_fill32(uint32_t *__restrict__ p, uint32_t v, int64_t n)
 {
   for (int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t) gridDim.x * blockDim.x) p[i] = v;
@@ -1527,6 +1737,14 @@ hipError_t launch_bck(const DevProfile &p, const DevSeqs &s, const int32_t *list
                       float *xmx_out, LenCfg lc, hipStream_t st, const DomBufs &dom, DevLists l)
 {
   if (maxn <= 0) return hipSuccess;
+  // the largest models run the parser with eight waves per target
+  if (! fwd_xmx && ! xmx_out && ! dom.pool && (p.fb_B == 64 || p.fb_B == 80)) {
+    void *kw = (p.fb_B == 64) ? (void *) k_bck_wg<64, 8> : (void *) k_bck_wg<80, 8>;
+    const unsigned grid = grid_for_waves_blk(kw, maxn * 8, 0, 8);
+    void *args[] = { (void *) &p, (void *) &s.res, (void *) &s.off, (void *) &s.len, (void *) &list, (void *) &count,
+                     &r, (void *) &scale_pool, (void *) &scale_off, &lc };
+    return launch(kw, grid, 512, args, 0, st);
+  }
   const int tm = fb_mode(p);
   void *k = nullptr;
   int nthr = LIST_BLOCK;
