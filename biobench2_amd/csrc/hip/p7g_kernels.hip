@@ -1710,13 +1710,25 @@ hipError_t launch_fwd(const DevProfile &p, const DevSeqs &s, const int32_t *list
                       const DomBufs &dom)
 {
   if (maxn <= 0) return hipSuccess;
-  // the largest models run the parser with eight waves per target
-  if (! dom.pool && ! dom.env_loop && ! xmx_out && (p.fb_B == 64 || p.fb_B == 80)) {
-    void *kw = (p.fb_B == 64) ? (void *) k_fwd_wg<64, 8> : (void *) k_fwd_wg<80, 8>;
-    const unsigned grid = grid_for_waves_blk(kw, maxn * 8, 0, 8);
+  // large models run the parser with several waves per target (4 up to 48
+  // positions per lane, 8 above)
+  void *kw = nullptr;
+  int ww = 0;
+  if (! dom.pool && ! dom.env_loop && ! xmx_out) {
+    switch (p.fb_B) {
+      case 32: kw = (void *) k_fwd_wg<32, 4>; ww = 4; break;
+      case 40: kw = (void *) k_fwd_wg<40, 4>; ww = 4; break;
+      case 48: kw = (void *) k_fwd_wg<48, 4>; ww = 4; break;
+      case 64: kw = (void *) k_fwd_wg<64, 8>; ww = 8; break;
+      case 80: kw = (void *) k_fwd_wg<80, 8>; ww = 8; break;
+      default: break;
+    }
+  }
+  if (kw) {
+    const unsigned grid = grid_for_waves_blk(kw, maxn * ww, 0, ww);
     void *args[] = { (void *) &p, (void *) &s.res, (void *) &s.off, (void *) &s.len, (void *) &list, (void *) &count,
                      &r, &l, &g, &gating, &lc, &scale_pool, &pool_size, &scale_off };
-    return launch(kw, grid, 512, args, 0, st);
+    return launch(kw, grid, 64 * ww, args, 0, st);
   }
   const int tm = fb_mode(p);
   void *k = nullptr;
@@ -1737,13 +1749,24 @@ hipError_t launch_bck(const DevProfile &p, const DevSeqs &s, const int32_t *list
                       float *xmx_out, LenCfg lc, hipStream_t st, const DomBufs &dom, DevLists l)
 {
   if (maxn <= 0) return hipSuccess;
-  // the largest models run the parser with eight waves per target
-  if (! fwd_xmx && ! xmx_out && ! dom.pool && (p.fb_B == 64 || p.fb_B == 80)) {
-    void *kw = (p.fb_B == 64) ? (void *) k_bck_wg<64, 8> : (void *) k_bck_wg<80, 8>;
-    const unsigned grid = grid_for_waves_blk(kw, maxn * 8, 0, 8);
+  // large models run the parser with several waves per target, as Forward
+  void *kw = nullptr;
+  int ww = 0;
+  if (! fwd_xmx && ! xmx_out && ! dom.pool) {
+    switch (p.fb_B) {
+      case 32: kw = (void *) k_bck_wg<32, 4>; ww = 4; break;
+      case 40: kw = (void *) k_bck_wg<40, 4>; ww = 4; break;
+      case 48: kw = (void *) k_bck_wg<48, 4>; ww = 4; break;
+      case 64: kw = (void *) k_bck_wg<64, 8>; ww = 8; break;
+      case 80: kw = (void *) k_bck_wg<80, 8>; ww = 8; break;
+      default: break;
+    }
+  }
+  if (kw) {
+    const unsigned grid = grid_for_waves_blk(kw, maxn * ww, 0, ww);
     void *args[] = { (void *) &p, (void *) &s.res, (void *) &s.off, (void *) &s.len, (void *) &list, (void *) &count,
                      &r, (void *) &scale_pool, (void *) &scale_off, &lc };
-    return launch(kw, grid, 512, args, 0, st);
+    return launch(kw, grid, 64 * ww, args, 0, st);
   }
   const int tm = fb_mode(p);
   void *k = nullptr;
