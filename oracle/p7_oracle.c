@@ -161,6 +161,7 @@ orc_vit_filter(const uint8_t *dsq, int L, int M, const int16_t *rwv, const int16
  * Forward / Backward parsers: four-lane striped emulation of fwdback.c
  *****************************************************************/
 typedef struct { float v[4]; } V4;
+enum { X_E = 0, X_N, X_J, X_B, X_C, X_SC };     /* special-state row layout, p7X_* order */
 static inline V4 v4_set(float a) { V4 r = {{ a, a, a, a }}; return r; }
 static inline V4 v4_add(V4 a, V4 b) { V4 r; for (int z = 0; z < 4; z++) r.v[z] = a.v[z] + b.v[z]; return r; }
 static inline V4 v4_mul(V4 a, V4 b) { V4 r; for (int z = 0; z < 4; z++) r.v[z] = a.v[z] * b.v[z]; return r; }
@@ -192,9 +193,23 @@ stripe(int M, int Kp, const float *rfv, const float *tfv, STRIPED *s)
 }
 static void unstripe(STRIPED *s) { free(s->rf); free(s->tf); }
 
-int
-orc_fwd_parser(const uint8_t *dsq, int L, int M, int Kp, const float *rfv, const float *tfv, const float *xf,
-               float *xmx, float *ret_totscale, float *ret_sc)
+/* one striped row of M and I values to [M_1..M_M][I_1..I_M] */
+static void
+unstripe_row(int Q, int M, const V4 *MMO, const V4 *IMO, float *out)
+{
+  for (int q = 0; q < Q; q++)
+    for (int z = 0; z < 4; z++) {
+      const int k = q + 1 + z*Q;
+      if (k <= M) { out[k-1] = MMO[q].v[z]; out[M + k-1] = IMO[q].v[z]; }
+    }
+}
+
+/* mi (optional): the M and I values of every row after its rescaling, row i
+ * at mi + i*2*M as [M_1..M_M][I_1..I_M] (p7_Forward keeps the whole matrix,
+ * fwdback.c:98; the parser keeps one row). */
+static int
+fwd_engine(const uint8_t *dsq, int L, int M, int Kp, const float *rfv, const float *tfv, const float *xf,
+           float *xmx, float *ret_totscale, float *ret_sc, float *mi)
 {
   STRIPED s;
   V4 *MMO, *DMO, *IMO, mpv, dpv, ipv, sv, dcv, xEv, xBv, zerov = v4_set(0.0f);
@@ -207,6 +222,7 @@ orc_fwd_parser(const uint8_t *dsq, int L, int M, int Kp, const float *rfv, const
   MMO = calloc((size_t) Q, sizeof(V4)); DMO = calloc((size_t) Q, sizeof(V4)); IMO = calloc((size_t) Q, sizeof(V4));
   xE = 0.; xN = 1.; xJ = 0.; xB = xf[XO(XN,MOVE)]; xC = 0.;
   if (xmx) { xmx[0] = xE; xmx[1] = xN; xmx[2] = xJ; xmx[3] = xB; xmx[4] = xC; xmx[5] = 1.0f; }
+  if (mi) memset(mi, 0, sizeof(float) * 2 * (size_t) M);
 
   for (i = 1; i <= L; i++) {
     const V4 *rp = s.rf + (size_t) dsq[i] * Q;
@@ -268,6 +284,7 @@ orc_fwd_parser(const uint8_t *dsq, int L, int M, int Kp, const float *rfv, const
       xE = 1.0;
     }
     if (xmx) { float *r = xmx + (size_t) i*6; r[0] = xE; r[1] = xN; r[2] = xJ; r[3] = xB; r[4] = xC; r[5] = scale; }
+    if (mi) unstripe_row(Q, M, MMO, IMO, mi + (size_t) i * 2 * M);
   }
   free(MMO); free(DMO); free(IMO); unstripe(&s);
   if (ret_totscale) *ret_totscale = totscale;
@@ -280,8 +297,14 @@ orc_fwd_parser(const uint8_t *dsq, int L, int M, int Kp, const float *rfv, const
 }
 
 int
-orc_bck_parser(const uint8_t *dsq, int L, int M, int Kp, const float *rfv, const float *tfv, const float *xf,
-               const float *fwd_xmx, float *xmx, int *ret_own, float *ret_sc)
+orc_fwd_parser(const uint8_t *dsq, int L, int M, int Kp, const float *rfv, const float *tfv, const float *xf,
+               float *xmx, float *ret_totscale, float *ret_sc)
+{ return fwd_engine(dsq, L, M, Kp, rfv, tfv, xf, xmx, ret_totscale, ret_sc, NULL); }
+
+/* mi (optional): as for fwd_engine, the Backward values of rows 1..L (p7_Backward, fwdback.c:141). */
+static int
+bck_engine(const uint8_t *dsq, int L, int M, int Kp, const float *rfv, const float *tfv, const float *xf,
+           const float *fwd_xmx, float *xmx, int *ret_own, float *ret_sc, float *mi)
 {
   STRIPED s;
   V4 *MMO, *DMO, *IMO, mpv, ipv, dpv, mcv, dcv, tmmv, timv, tdmv, xBv, xEv, zerov = v4_set(0.0f);
@@ -322,6 +345,7 @@ orc_bck_parser(const uint8_t *dsq, int L, int M, int Kp, const float *rfv, const
     }
     totscale = log(sL);
     if (xmx) { float *r = xmx + (size_t) L*6; r[0] = xE; r[1] = xN; r[2] = xJ; r[3] = xB; r[4] = xC; r[5] = sL; }
+    if (mi) { memset(mi, 0, sizeof(float) * 2 * (size_t) M); unstripe_row(Q, M, MMO, IMO, mi + (size_t) L * 2 * M); }
   }
 
   for (i = L-1; i >= 1; i--) {
@@ -379,6 +403,7 @@ orc_bck_parser(const uint8_t *dsq, int L, int M, int Kp, const float *rfv, const
       totscale += log(sc);
     }
     if (xmx) { float *r = xmx + (size_t) i*6; r[0] = xE; r[1] = xN; r[2] = xJ; r[3] = xB; r[4] = xC; r[5] = sc; }
+    if (mi) unstripe_row(Q, M, MMO, IMO, mi + (size_t) i * 2 * M);
   }
 
   tp  = s.tf;
@@ -401,6 +426,92 @@ orc_bck_parser(const uint8_t *dsq, int L, int M, int Kp, const float *rfv, const
   if (ret_own) *ret_own = own;
   if (ret_sc) *ret_sc = status ? NAN : totscale + log(xN);
   _mm_setcsr(csr);
+  return status;
+}
+
+int
+orc_bck_parser(const uint8_t *dsq, int L, int M, int Kp, const float *rfv, const float *tfv, const float *xf,
+               const float *fwd_xmx, float *xmx, int *ret_own, float *ret_sc)
+{ return bck_engine(dsq, L, M, Kp, rfv, tfv, xf, fwd_xmx, xmx, ret_own, ret_sc, NULL); }
+
+/*****************************************************************
+ * Null2 of one envelope (p7_domaindef.c:812-934, rescore_isolated_domain, the
+ * non-long-target branch): full Forward and Backward over the envelope's
+ * residues, posterior decoding (impl_sse/decoding.c:82-143), the null2 odds
+ * by expectation (impl_sse/null2.c:47-121), and the envelope's correction
+ * sum_pos logf(null2[dsq[pos]]) (p7_domaindef.c:925-931).
+ *
+ * dsq: the envelope's residues, dsq[1..L]. xf: the unihit configuration.
+ * degen[x] (x < Kp): bit y set if code x stands for canonical residue y;
+ * ndegen[x] their number (esl_alphabet.c:1331-1341, 1462-1468).
+ * Outputs: null2[0..Kp-1], *ret_dc (nats); w (optional, 2*M+3 floats): the
+ * posterior weights Mw_1..Mw_M, Iw_1..Iw_M, and N, C, J, after the 1/L
+ * normalisation. Returns 16 if the decoding overflows (eslERANGE).
+ *****************************************************************/
+int
+orc_null2_envelope(const uint8_t *dsq, int L, int M, int K, int Kp, const float *rfv, const float *tfv,
+                   const float *xf, const uint32_t *degen, const int *ndegen, float *null2, float *ret_dc, float *w)
+{
+  float *fx = malloc(sizeof(float) * 6 * (size_t) (L+1)), *bx = malloc(sizeof(float) * 6 * (size_t) (L+1));
+  float *fm = malloc(sizeof(float) * 2 * (size_t) M * (L+1)), *bm = malloc(sizeof(float) * 2 * (size_t) M * (L+1));
+  float *pm = calloc(2 * (size_t) M, sizeof(float));
+  float sc, pN = 0.0f, pC = 0.0f, pJ = 0.0f;
+  int own = 0, status = 0, Q = (M - 1) / 4 + 1;
+  if (Q < 2) Q = 2;
+  if (! fx || ! bx || ! fm || ! bm || ! pm || L < 1) { status = 5; goto done; }
+  if ((status = fwd_engine(dsq, L, M, Kp, rfv, tfv, xf, fx, NULL, &sc, fm)) != 0) goto done;
+  if ((status = bck_engine(dsq, L, M, Kp, rfv, tfv, xf, fx, bx, &own, &sc, bm)) != 0) goto done;
+  {
+    /* p7_Decoding: row i's posteriors are (f * b) * (scaleproduct * fscale_i);
+     * the special states use row i-1 of Forward and row i of Backward.
+     * p7_Null2_ByExpectation then sums rows 1..L in order. */
+    float sp = 1.0 / bx[X_N];
+    for (int i = 1; i <= L; i++) {
+      const float totr = sp * fx[i*6 + X_SC];
+      const float *f = fm + (size_t) i * 2 * M, *b = bm + (size_t) i * 2 * M;
+      for (int c = 0; c < 2 * M; c++) {
+        const float pp = (f[c] * b[c]) * totr;
+        pm[c] = (i == 1) ? pp : pp + pm[c];
+      }
+      const float n = fx[(i-1)*6 + X_N] * bx[i*6 + X_N] * xf[XO(XN,LOOP)] * sp;
+      const float j = fx[(i-1)*6 + X_J] * bx[i*6 + X_J] * xf[XO(XJ,LOOP)] * sp;
+      const float cc = fx[(i-1)*6 + X_C] * bx[i*6 + X_C] * xf[XO(XC,LOOP)] * sp;
+      if (i == 1) { pN = n; pC = cc; pJ = j; } else { pN += n; pC += cc; pJ += j; }
+      if (own) sp *= fx[i*6 + X_SC] / bx[i*6 + X_SC];
+    }
+    if (isinf(sp)) { status = 16; goto done; }
+  }
+  {
+    const float norm = 1.0 / (float) L;
+    for (int c = 0; c < 2 * M; c++) pm[c] *= norm;
+    pN *= norm; pC *= norm; pJ *= norm;
+    const float xfactor = pN + pC + pJ;
+    for (int x = 0; x < K; x++) {
+      /* four striped lanes, k = q + 1 + z*Q, summed over q, then added across lanes */
+      V4 sv = v4_set(0.0f);
+      for (int q = 0; q < Q; q++)
+        for (int z = 0; z < 4; z++) {
+          const int k = q + 1 + z * Q;
+          const float mw = (k <= M) ? pm[k-1] : 0.0f, iw = (k <= M) ? pm[M + k-1] : 0.0f;
+          const float e = (k <= M) ? rfv[(size_t) x * M + k-1] : 0.0f;
+          sv.v[z] = sv.v[z] + mw * e;
+          sv.v[z] = sv.v[z] + iw;
+        }
+      null2[x] = v4_hsum(sv) + xfactor;
+    }
+    for (int x = K + 1; x <= Kp - 3; x++) {
+      float r = 0.0f;
+      for (int y = 0; y < K; y++) if (degen[x] >> y & 1u) r += null2[y];
+      null2[x] = r / (float) ndegen[x];
+    }
+    null2[K] = 1.0f; null2[Kp-2] = 1.0f; null2[Kp-1] = 1.0f;
+    float dc = 0.0f;
+    for (int pos = 1; pos <= L; pos++) dc += logf(null2[dsq[pos]]);
+    *ret_dc = dc;
+    if (w) { memcpy(w, pm, sizeof(float) * 2 * (size_t) M); w[2*M] = pN; w[2*M+1] = pC; w[2*M+2] = pJ; }
+  }
+done:
+  free(fx); free(bx); free(fm); free(bm); free(pm);
   return status;
 }
 
@@ -586,6 +697,7 @@ orc_backward_log64(const uint8_t *dsq, int L, int M, const float *tsc, const flo
  * path probabilities and the best path score.
  *****************************************************************/
 #define BRUTE_LMAX 8
+#define BRUTE_MMAX 8
 typedef struct {
   const uint8_t *dsq; int L, M; const float *tsc, *rsc, *xsc; double sum; double best;
   /* posterior mode: visits of B (by row before the entry) and E (by row), and
@@ -593,6 +705,10 @@ typedef struct {
    * probability-weighted sums over all complete paths */
   int post; int nB[BRUTE_LMAX+1], nE[BRUTE_LMAX+1], njc[BRUTE_LMAX+1];
   double wsum, wB[BRUTE_LMAX+1], wE[BRUTE_LMAX+1], wN[BRUTE_LMAX+1];
+  /* residues emitted by each emitting state along the path: [0] N, [1] C,
+   * [2] J, [3+k-1] M_k, [3+M+k-1] I_k; and their weighted sums */
+  int ce[3 + 2*BRUTE_MMAX];
+  double we[3 + 2*BRUTE_MMAX];
 } BRUTE;
 
 static void
@@ -604,15 +720,15 @@ brute_walk(BRUTE *b, int st, int k, int i, double sc)
   if (sc == -INFINITY) return;
   switch (st) {
   case 0: /* N */
-    if (i < L) { b->njc[i+1]++; brute_walk(b, 0, 0, i+1, sc + xsc[XN*2 + GL]); b->njc[i+1]--; }
+    if (i < L) { b->njc[i+1]++; b->ce[0]++; brute_walk(b, 0, 0, i+1, sc + xsc[XN*2 + GL]); b->ce[0]--; b->njc[i+1]--; }
     b->nB[i]++; brute_walk(b, 1, 0, i, sc + xsc[XN*2 + GM]); b->nB[i]--;
     break;
   case 6: /* J */
-    if (i < L) { b->njc[i+1]++; brute_walk(b, 6, 0, i+1, sc + xsc[XJ*2 + GL]); b->njc[i+1]--; }
+    if (i < L) { b->njc[i+1]++; b->ce[2]++; brute_walk(b, 6, 0, i+1, sc + xsc[XJ*2 + GL]); b->ce[2]--; b->njc[i+1]--; }
     b->nB[i]++; brute_walk(b, 1, 0, i, sc + xsc[XJ*2 + GM]); b->nB[i]--;
     break;
   case 7: /* C */
-    if (i < L) { b->njc[i+1]++; brute_walk(b, 7, 0, i+1, sc + xsc[XC*2 + GL]); b->njc[i+1]--; }
+    if (i < L) { b->njc[i+1]++; b->ce[1]++; brute_walk(b, 7, 0, i+1, sc + xsc[XC*2 + GL]); b->ce[1]--; b->njc[i+1]--; }
     if (i == L) {
       double t = sc + xsc[XC*2 + GM];
       b->sum = lse(b->sum, t);
@@ -621,13 +737,14 @@ brute_walk(BRUTE *b, int st, int k, int i, double sc)
         const double w = exp(t);
         b->wsum += w;
         for (int r = 0; r <= L; r++) { b->wB[r] += w * b->nB[r]; b->wE[r] += w * b->nE[r]; b->wN[r] += w * b->njc[r]; }
+        for (int c = 0; c < 3 + 2*M; c++) b->we[c] += w * b->ce[c];
       }
     }
     break;
   case 1: /* B -> M_k (local entry) */
     if (i < L) for (int kk = 1; kk <= M; kk++) {
       const float *rs = b->rsc + (size_t) b->dsq[i+1] * (M+1) * 2;
-      brute_walk(b, 2, kk, i+1, sc + GTSC(kk-1, P_BM) + rs[kk*2]);
+      b->ce[3+kk-1]++; brute_walk(b, 2, kk, i+1, sc + GTSC(kk-1, P_BM) + rs[kk*2]); b->ce[3+kk-1]--;
     }
     break;
   case 2: /* M_k */
@@ -635,8 +752,8 @@ brute_walk(BRUTE *b, int st, int k, int i, double sc)
     if (k < M) {
       if (i < L) {
         const float *rs = b->rsc + (size_t) b->dsq[i+1] * (M+1) * 2;
-        brute_walk(b, 2, k+1, i+1, sc + GTSC(k, P_MM) + rs[(k+1)*2]);
-        brute_walk(b, 3, k,   i+1, sc + GTSC(k, P_MI) + rs[k*2+1]);
+        b->ce[3+k]++;     brute_walk(b, 2, k+1, i+1, sc + GTSC(k, P_MM) + rs[(k+1)*2]); b->ce[3+k]--;
+        b->ce[3+M+k-1]++; brute_walk(b, 3, k,   i+1, sc + GTSC(k, P_MI) + rs[k*2+1]);  b->ce[3+M+k-1]--;
       }
       brute_walk(b, 4, k+1, i, sc + GTSC(k, P_MD));
     }
@@ -644,8 +761,8 @@ brute_walk(BRUTE *b, int st, int k, int i, double sc)
   case 3: /* I_k */
     if (i < L) {
       const float *rs = b->rsc + (size_t) b->dsq[i+1] * (M+1) * 2;
-      brute_walk(b, 2, k+1, i+1, sc + GTSC(k, P_IM) + rs[(k+1)*2]);
-      brute_walk(b, 3, k,   i+1, sc + GTSC(k, P_II) + rs[k*2+1]);
+      b->ce[3+k]++;     brute_walk(b, 2, k+1, i+1, sc + GTSC(k, P_IM) + rs[(k+1)*2]); b->ce[3+k]--;
+      b->ce[3+M+k-1]++; brute_walk(b, 3, k,   i+1, sc + GTSC(k, P_II) + rs[k*2+1]);  b->ce[3+M+k-1]--;
     }
     break;
   case 4: /* D_k */
@@ -653,7 +770,7 @@ brute_walk(BRUTE *b, int st, int k, int i, double sc)
     if (k < M) {
       if (i < L) {
         const float *rs = b->rsc + (size_t) b->dsq[i+1] * (M+1) * 2;
-        brute_walk(b, 2, k+1, i+1, sc + GTSC(k, P_DM) + rs[(k+1)*2]);
+        b->ce[3+k]++; brute_walk(b, 2, k+1, i+1, sc + GTSC(k, P_DM) + rs[(k+1)*2]); b->ce[3+k]--;
       }
       brute_walk(b, 4, k+1, i, sc + GTSC(k, P_DD));
     }
@@ -698,6 +815,23 @@ orc_brute_posterior(const uint8_t *dsq, int L, int M, const float *tsc, const fl
   return 0;
 }
 
+/* Expected residues emitted by each emitting state, by enumeration, divided
+ * by L: the posterior weights of p7_Null2_ByExpectation (impl_sse/null2.c:
+ * 59-90). w: M_1..M_M, I_1..I_M, then N, C, J. */
+int
+orc_brute_emitters(const uint8_t *dsq, int L, int M, const float *tsc, const float *rsc, const float *xsc, double *w)
+{
+  BRUTE b;
+  if (L > BRUTE_LMAX || M > BRUTE_MMAX || L < 1) return 11;
+  memset(&b, 0, sizeof(b));
+  b.dsq = dsq; b.L = L; b.M = M; b.tsc = tsc; b.rsc = rsc; b.xsc = xsc;
+  b.sum = -INFINITY; b.best = -INFINITY; b.post = 1;
+  brute_walk(&b, 0, 0, 0, 0.0);
+  for (int k = 0; k < 2*M; k++) w[k] = b.we[3+k] / b.wsum / L;
+  for (int c = 0; c < 3; c++) w[2*M + c] = b.we[c] / b.wsum / L;
+  return 0;
+}
+
 /*****************************************************************
  * Domain decoding (impl_sse/decoding.c:166-197): from the special states of
  * the Forward and Backward parsers, btot[i] / etot[i] = expected number of
@@ -705,7 +839,6 @@ orc_brute_posterior(const uint8_t *dsq, int L, int M, const float *tsc, const fl
  * by the core model. fx, bx: (L+1) rows of [E N J B C SCALE] (p7X_* order).
  * bown: the Backward parser had to use its own scale factors.
  *****************************************************************/
-enum { X_E = 0, X_N, X_J, X_B, X_C, X_SC };
 
 int
 orc_domain_decoding(int L, const float *fx, const float *bx, int bown, float nloop, float jloop, float cloop,
