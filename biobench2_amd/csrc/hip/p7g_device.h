@@ -31,6 +31,10 @@ struct SsvVariant {
 
 struct DevProfile {
   int M, Kp;
+  int K;                 // canonical residues (20 amino, 4 DNA)
+  // per code x < 32: bit y set if x stands for canonical residue y, and the
+  // number of such residues in bits 24..31 (Easel's degen/ndegen)
+  const uint32_t *degen;
   // Every table is lane-minor: the values one read instruction fetches for
   // consecutive lanes are consecutive in memory, so LDS reads are free of
   // bank conflicts for any per-lane block size.
@@ -122,7 +126,24 @@ struct DomBufs {
   // N/J/C loop and move odds of each region, and how many regions there are room for
   const float *env_loop = nullptr, *env_move = nullptr;
   int64_t  env_cap = 0;
+  // null2 of single-domain regions (rescore_isolated_domain): each region's
+  // area at n2pool + n2off[k] holds the Forward row scales, the Backward
+  // special rows and the Backward M and I rows (n2_area_floats). n2mode 1:
+  // the envelope Forward writes its scales; 2: the Forward recomputed with
+  // the posterior sums, null2 and the correction n2dc[k] (nats). n2st[k]:
+  // the Backward's status and own-scales flag (0x100), then 0 or 16
+  float   *n2pool = nullptr;
+  const int64_t *n2off = nullptr;
+  float   *n2dc = nullptr;
+  int32_t *n2st = nullptr;
+  int      n2mode = 0;
 };
+
+// the parts of a region's null2 area, in floats, each a multiple of 64
+__host__ __device__ inline int64_t n2_scales_floats(int L) { return ((int64_t) L + 1 + 63) & ~(int64_t) 63; }
+__host__ __device__ inline int64_t n2_xrows_floats(int L) { return (6 * ((int64_t) L + 1) + 63) & ~(int64_t) 63; }
+__host__ __device__ inline int64_t n2_area_floats(int L, int B)
+{ return n2_scales_floats(L) + n2_xrows_floats(L) + (int64_t) L * 128 * B; }
 // Per-target results of the decoding (slot order) and the region records
 // {slot, i, j, multidomain} in the order they were found.
 struct DomOut {

@@ -678,10 +678,69 @@ __device__ __forceinline__ void affine_bck_scan(float (&D)[B], A a, const AffCon
   for (int j = B-1; j >= 0; j--) { cur = D[j] + a(j) * cur; D[j] = cur; }
 }
 
+// The end of an envelope's null2 pass (MODE 3 of k_fwd), one wave: the
+// posterior sums accM/accI (this lane's B positions) and pN/pJ/pC over the
+// envelope's L rows become weights (times 1/L); null2[x] for canonical x is
+// the weighted sum of the positions' match odds, plus the insert weights
+// (odds 1) and the N, C, J weights (impl_sse/null2.c:80-107); a degenerate
+// code's value is the average over the residues it stands for
+// (esl_alphabet.c:1331-1341), and gap, '*' and '~' get 1. Lane x holds
+// null2[x]. The correction is sum over the envelope's residues of
+// logf(null2[x]) (p7_domaindef.c:925-931). A scale product that ended
+// infinite is the decoding's range error: status 16, no correction.
+template <int B>
+__device__ __forceinline__ void null2_epilogue(const DevProfile &p, const float *tbl, size_t rowf, int z,
+                                               float (&accM)[B], float (&accI)[B], float pN, float pJ, float pC,
+                                               float sp, const uint8_t *sq, int L, float *dc_out, int32_t *st_out)
+{
+  if (isinf(sp)) {
+    if (z == 0) { *dc_out = NAN; *st_out = 16; }
+    return;
+  }
+  const float norm = (float) (1.0 / (double) L);
+  float iw = 0.0f;
+#pragma unroll
+  for (int j = 0; j < B; j++) { accM[j] = __fmul_rn(accM[j], norm); iw = __fadd_rn(iw, __fmul_rn(accI[j], norm)); }
+  const float xfac = __fadd_rn(__fadd_rn(__fmul_rn(pN, norm), __fmul_rn(pC, norm)), __fmul_rn(pJ, norm));
+  float v = 1.0f;
+  for (int x = 0; x < p.K; x++) {
+    const float *e = tbl + (size_t) x * rowf + z;
+    float part = iw;
+#pragma unroll
+    for (int j = 0; j < B; j++) part = __fadd_rn(part, __fmul_rn(accM[j], e[j * 64]));
+    const float s = wave_sum_f(part);
+    if (z == x) v = __fadd_rn(s, xfac);
+  }
+  {
+    const uint32_t dg = (z < 32) ? p.degen[z] : 0u;
+    float r = 0.0f;
+    for (int y = 0; y < p.K; y++) {
+      const float vy = __shfl(v, y);
+      if ((dg >> y) & 1u) r = __fadd_rn(r, vy);
+    }
+    if (z > p.K && z <= p.Kp - 3) v = __fdiv_rn(r, (float) (dg >> 24));
+  }
+  float part = 0.0f;
+  for (int b0 = 0; b0 < L; b0 += 64) {
+    const int pos = b0 + z;
+    const int x = (pos < L) ? sq[pos] : 0;
+    const float vx = __shfl(v, x & 63);
+    if (pos < L) part = __fadd_rn(part, logf(vx));
+  }
+  const float dc = wave_sum_f(part);
+  if (z == 0) { *dc_out = dc; *st_out = 0; }
+}
+
 // MODE 0: the parser. MODE 1: the rows also go to the domain decoding pool
 // (a separate instance, so that the plain parser keeps its register budget).
 // MODE 2: envelope scores: the list is of region copies, each scored in unihit
-// mode (E move 1, E loop 0) with the N/J/C odds of its whole target's length.
+// mode (E move 1, E loop 0) with the N/J/C odds of its whole target's length;
+// with a null2 area (dom.n2off) the row scales go there too.
+// MODE 3: the same Forward pass again for null2: each row's posteriors
+// (p7_Decoding, impl_sse/decoding.c:109-139) from this row and the stored
+// Backward row, summed over the rows; then the null2 odds
+// (p7_Null2_ByExpectation, impl_sse/null2.c:59-118) and the envelope's
+// correction sum_pos logf(null2[x_pos]) (p7_domaindef.c:925-931).
 template <int B, int TM, int MODE>
 __global__ __launch_bounds__(FWD_THREADS(B))
 void k_fwd(const DevProfile p, const uint8_t *__restrict__ res, const int64_t *__restrict__ off,
@@ -715,17 +774,42 @@ void k_fwd(const DevProfile p, const uint8_t *__restrict__ res, const int64_t *_
     aff_fwd_setup<B>(akc, tdd);
   }
   constexpr bool DOM = (MODE == 1);
-  const int64_t n = (MODE == 2) ? min((int64_t) *count, dom.env_cap) : (int64_t) *count;
-  const float eMove = (MODE == 2) ? 1.0f : p.xfE_move, eLoop = (MODE == 2) ? 0.0f : p.xfE_loop;
+  constexpr bool ENV = (MODE >= 2);
+  const int64_t n = ENV ? min((int64_t) *count, dom.env_cap) : (int64_t) *count;
+  const float eMove = ENV ? 1.0f : p.xfE_move, eLoop = ENV ? 0.0f : p.xfE_loop;
 
   int32_t *queue = (int32_t *) count + P7G_QOFF;
   for (int64_t t = queue_next(queue, 1); t < n; t = queue_next(queue, 1)) {
     const int slot = uni(list[t]);
     const int L = uni(len[slot]);
     const uint8_t *sq = res + off[slot];
-    const float floop = (MODE == 2) ? dom.env_loop[slot] : lc.on ? lc.floop : p.xf_loop[L];
-    const float fmove = (MODE == 2) ? dom.env_move[slot] : lc.on ? lc.fmove : p.xf_move[L];
+    const float floop = ENV ? dom.env_loop[slot] : lc.on ? lc.floop : p.xf_loop[L];
+    const float fmove = ENV ? dom.env_move[slot] : lc.on ? lc.fmove : p.xf_move[L];
     float *scl = nullptr;
+    // null2: the region's area (scales, Backward special rows, Backward M/I rows)
+    float *area = nullptr;
+    if (ENV && dom.n2off) {
+      const int64_t o = dom.n2off[slot];
+      area = dom.n2pool + (((int64_t) (unsigned) uni((int) (o >> 32)) << 32) | (unsigned) uni((int) o));
+      if (MODE == 2) scl = area;
+    }
+    const float *xb = nullptr, *cells = nullptr;
+    float sp = 0.0f, pN = 0.0f, pJ = 0.0f, pC = 0.0f;
+    int own = 0;
+    if (MODE == 3) {
+      const int bs = uni(dom.n2st[slot]);
+      if ((bs & 0xff) != 0 || L < 1) {            // the envelope's Backward failed
+        if (z == 0) { dom.n2dc[slot] = NAN; dom.n2st[slot] = 16; }
+        continue;
+      }
+      own = (bs & 0x100) != 0;
+      xb = area + n2_scales_floats(L);
+      cells = xb + n2_xrows_floats(L) + z;
+      sp = (float) (1.0 / (double) xb[1]);        // 1 / Backward's N at row 0
+    }
+    float accM[B], accI[B];
+#pragma unroll
+    for (int j = 0; j < B; j++) accM[j] = accI[j] = 0.0f;
     if (pool) {
       unsigned long long a64 = 0;
       if (z == 0) a64 = atomicAdd(l.cursor, (unsigned long long) (L + 1));
@@ -761,6 +845,7 @@ void k_fwd(const DevProfile p, const uint8_t *__restrict__ res, const int64_t *_
       const int x = rd.at(i-1);
       const float *e = tbl + (size_t) x * rowf + z;
       const float *trr = tr + (PIN ? 0 : opaque_zero());
+      const float pxN = xN, pxJ = xJ, pxC = xC;          // row i-1
 #define FTR(A, T, j) (PIN ? A[j] : trr[(T) * rowf + (j) * 64])
       const float iL = dppz<WAVE_SHR1>(I[B-1]), dL = dppz<WAVE_SHR1>(D[B-1]);
       float xEl = 0.0f;
@@ -808,6 +893,26 @@ void k_fwd(const DevProfile p, const uint8_t *__restrict__ res, const int64_t *_
         float *xr = xo + (size_t) i * 6;
         xr[0] = xE; xr[1] = xN; xr[2] = xJ; xr[3] = xB; xr[4] = xC; xr[5] = scale;
       }
+      if (MODE == 3) {
+        // posteriors of row i: (f * b) * (scaleproduct * scale_i); N, J, C
+        // from Forward's row i-1 and Backward's row i
+        const float totr = sp * scale;
+        const float *cr = cells + (size_t) (i-1) * 2 * rowf;
+#pragma unroll
+        for (int j = 0; j < B; j++) {
+          accM[j] = __fadd_rn(__fmul_rn(__fmul_rn(M[j], cr[j*64]), totr), accM[j]);
+          accI[j] = __fadd_rn(__fmul_rn(__fmul_rn(I[j], cr[rowf + j*64]), totr), accI[j]);
+        }
+        const float *xr = xb + (size_t) i * 6;
+        pN = __fadd_rn(pN, __fmul_rn(__fmul_rn(__fmul_rn(pxN, xr[1]), floop), sp));
+        pJ = __fadd_rn(pJ, __fmul_rn(__fmul_rn(__fmul_rn(pxJ, xr[2]), floop), sp));
+        pC = __fadd_rn(pC, __fmul_rn(__fmul_rn(__fmul_rn(pxC, xr[4]), floop), sp));
+        if (own) sp = __fmul_rn(sp, __fdiv_rn(scale, xr[5]));
+      }
+    }
+    if (MODE == 3) {
+      null2_epilogue<B>(p, tbl, rowf, z, accM, accI, pN, pJ, pC, sp, sq, L, dom.n2dc + slot, dom.n2st + slot);
+      continue;
     }
     if (z == 0) {
       int status = 0;
@@ -1036,7 +1141,7 @@ void k_fwd_wg(const DevProfile p, const uint8_t *__restrict__ res, const int64_t
  *****************************************************************/
 // DOM: the rows also go to the domain decoding pool (a separate instance, so
 // that the plain parser keeps its register budget)
-template <int B, int TM, bool DOM>
+template <int B, int TM, int MODE>
 __global__ __launch_bounds__(BCK_THREADS(B))
 void k_bck(const DevProfile p, const uint8_t *__restrict__ res, const int64_t *__restrict__ off,
            const int32_t *__restrict__ len, const int32_t *__restrict__ list, const int32_t *__restrict__ count,
@@ -1068,17 +1173,28 @@ void k_bck(const DevProfile p, const uint8_t *__restrict__ res, const int64_t *_
     aff_bck_setup<B>(akc, tdd);
   }
 #define BTR(A, T, j) (PIN ? A[j] : trr[(T) * rowf + (j) * 64])
-  const int64_t n = *count;
+  constexpr bool DOM = (MODE == 1), ENV = (MODE == 2);
+  const int64_t n = ENV ? min((int64_t) *count, dom.env_cap) : (int64_t) *count;
+  const float eMove = ENV ? 1.0f : p.xfE_move, eLoop = ENV ? 0.0f : p.xfE_loop;
 
   int32_t *queue = (int32_t *) count + P7G_QOFF;
   for (int64_t t = queue_next(queue, 1); t < n; t = queue_next(queue, 1)) {
     const int slot = uni(list[t]);
     const int L = uni(len[slot]);
     const uint8_t *sq = res + off[slot];
-    const float floop = lc.on ? lc.floop : p.xf_loop[L], fmove = lc.on ? lc.fmove : p.xf_move[L];
+    const float floop = ENV ? dom.env_loop[slot] : lc.on ? lc.floop : p.xf_loop[L];
+    const float fmove = ENV ? dom.env_move[slot] : lc.on ? lc.fmove : p.xf_move[L];
     const float *fsc = nullptr;       // forward row scale factors, fsc[i], i = 0..L
     int fstride = 1;
-    if (fwd_xmx) { fsc = fwd_xmx + 5; fstride = 6; }
+    float *cells = nullptr;           // envelope Backward for null2: its M and I rows
+    float *xo = xmx_out;
+    if (ENV) {
+      const int64_t o = dom.n2off[slot];
+      float *area = dom.n2pool + (((int64_t) (unsigned) uni((int) (o >> 32)) << 32) | (unsigned) uni((int) o));
+      fsc = area;
+      xo = area + n2_scales_floats(L);
+      cells = xo + n2_xrows_floats(L) + z;
+    } else if (fwd_xmx) { fsc = fwd_xmx + 5; fstride = 6; }
     else {
       const int64_t so = scale_off[slot];
       if (so < 0) {
@@ -1089,7 +1205,6 @@ void k_bck(const DevProfile p, const uint8_t *__restrict__ res, const int64_t *_
     }
     // the special-state rows: the caller's matrix (one target), or the
     // target's decoding entry that the Forward parser reserved
-    float *xo = xmx_out;
     if (DOM) {
       const int64_t bo = dom.boff[slot];
       const int64_t bu = ((int64_t) (unsigned) uni((int) (bo >> 32)) << 32) | (unsigned) uni((int) bo);
@@ -1098,7 +1213,7 @@ void k_bck(const DevProfile p, const uint8_t *__restrict__ res, const int64_t *_
     float M[B], I[B], D[B];
     float xJ = 0.0f, xB = 0.0f, xN = 0.0f;
     float xC = fmove;
-    float xE = xC * p.xfE_move;
+    float xE = xC * eMove;
 #pragma unroll
     for (int j = 0; j < B; j++) { M[j] = xE; D[j] = xE; I[j] = 0.0f; }
     {
@@ -1122,6 +1237,11 @@ void k_bck(const DevProfile p, const uint8_t *__restrict__ res, const int64_t *_
       if (xo && z == 0) {
         float *xr = xo + (size_t) L * 6;
         xr[0] = xE; xr[1] = xN; xr[2] = xJ; xr[3] = xB; xr[4] = xC; xr[5] = s;
+      }
+      if (ENV && L > 0) {
+        float *cr = cells + (size_t) (L-1) * 2 * rowf;
+#pragma unroll
+        for (int j = 0; j < B; j++) { cr[j*64] = M[j]; cr[rowf + j*64] = I[j]; }
       }
     }
     ResDesc rd;
@@ -1149,7 +1269,7 @@ void k_bck(const DevProfile p, const uint8_t *__restrict__ res, const int64_t *_
       xC = xC * floop;
       xJ = (xB * fmove) + (xJ * floop);
       xN = (xB * fmove) + (xN * floop);
-      xE = (xC * p.xfE_move) + (xJ * p.xfE_loop);
+      xE = (xC * eMove) + (xJ * eLoop);
 #pragma unroll
       for (int j = 0; j < B; j++) D[j] = D[j] + xE;
       affine_bck_scan<B>(D, [&](int j) { return BTR(tDD, FT_DD, j); }, akc, z);
@@ -1168,6 +1288,11 @@ void k_bck(const DevProfile p, const uint8_t *__restrict__ res, const int64_t *_
       if (xo && z == 0) {
         float *xr = xo + (size_t) i * 6;
         xr[0] = xE; xr[1] = xN; xr[2] = xJ; xr[3] = xB; xr[4] = xC; xr[5] = s;
+      }
+      if (ENV) {
+        float *cr = cells + (size_t) (i-1) * 2 * rowf;
+#pragma unroll
+        for (int j = 0; j < B; j++) { cr[j*64] = M[j]; cr[rowf + j*64] = I[j]; }
       }
     }
     {
@@ -1699,10 +1824,11 @@ static size_t fb_lds(const DevProfile &p, int tm)
   X(1, TM_LDS) X(2, TM_LDS) X(3, TM_LDS) X(4, TM_LDS) X(5, TM_LDS) X(6, TM_LDS) X(7, TM_LDS) X(8, TM_LDS) \
   X(10, TM_LDS) X(12, TM_LDS) X(14, TM_PIN) X(16, TM_TLDS) X(20, TM_TLDS) X(24, TM_TLDS) X(28, TM_TLDS) \
   X(32, TM_TLDS) X(40, TM_TLDS) X(48, TM_TLDS) X(64, TM_GLOB) X(80, TM_GLOB)
-#define FWD_CASE(N, T) case N * 4 + T: k = dom.env_loop ? (void *) k_fwd<N, T, 2> : dom.pool ? (void *) k_fwd<N, T, 1> \
-                                          : (void *) k_fwd<N, T, 0>; nthr = FWD_THREADS(N); break;
-#define BCK_CASE(N, T) case N * 4 + T: k = dom.pool ? (void *) k_bck<N, T, true> : (void *) k_bck<N, T, false>; \
-                                          nthr = BCK_THREADS(N); break;
+#define FWD_CASE(N, T) case N * 4 + T: k = dom.env_loop ? (dom.n2mode == 2 ? (void *) k_fwd<N, T, 3> : (void *) k_fwd<N, T, 2>) \
+                                          : dom.pool ? (void *) k_fwd<N, T, 1> : (void *) k_fwd<N, T, 0>; \
+                                          nthr = FWD_THREADS(N); break;
+#define BCK_CASE(N, T) case N * 4 + T: k = dom.env_loop ? (void *) k_bck<N, T, 2> : dom.pool ? (void *) k_bck<N, T, 1> \
+                                          : (void *) k_bck<N, T, 0>; nthr = BCK_THREADS(N); break;
 
 hipError_t launch_fwd(const DevProfile &p, const DevSeqs &s, const int32_t *list, const int32_t *count, int64_t maxn,
                       DevResults r, DevLists l, GateThresh g, int gating, LenCfg lc,
@@ -1752,7 +1878,7 @@ hipError_t launch_bck(const DevProfile &p, const DevSeqs &s, const int32_t *list
   // large models run the parser with several waves per target, as Forward
   void *kw = nullptr;
   int ww = 0;
-  if (! fwd_xmx && ! xmx_out && ! dom.pool) {
+  if (! fwd_xmx && ! xmx_out && ! dom.pool && ! dom.env_loop) {
     switch (p.fb_B) {
       case 32: kw = (void *) k_bck_wg<32, 4>; ww = 4; break;
       case 40: kw = (void *) k_bck_wg<40, 4>; ww = 4; break;

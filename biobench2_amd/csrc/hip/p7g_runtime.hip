@@ -47,10 +47,17 @@ struct Work {
   uint16_t   *bnd = nullptr;        // boundary column of the chunked ungapped pass
   int64_t     bnd_cap = 0;          // its length in entries (residue bytes)
   void       *mem = nullptr;
-  hipEvent_t  ev[8];
+  hipEvent_t  ev[10];
   bool        ev_ready = false;
   DomBufs     dom;                  // domain decoding: row pool and per-slot offsets (made on first use)
   DomOut      dout;                 // its per-slot results and region records
+  // null2 of the regions: per region its area offset, correction and status,
+  // the list of regions decoded, count blocks, and the pool (grown on use)
+  int64_t    *n2off = nullptr;
+  float      *n2dc = nullptr;
+  int32_t    *n2st = nullptr, *n2list = nullptr, *n2cnt = nullptr;
+  float      *n2pool = nullptr;
+  int64_t     n2size = 0;
   void       *dom_mem = nullptr;
   int64_t     dom_cap = 0;          // slots the per-slot arrays hold
 };
@@ -143,7 +150,7 @@ dev_profile_build(P7_GPU_OPROFILE *om, int Lmax_need, struct p7g_dev_profile_s *
   d->older = nullptr;
   DevProfile &p = d->p;
   memset(&p, 0, sizeof(p));
-  p.M = M; p.Kp = Kp; p.nrows = NR;
+  p.M = M; p.Kp = Kp; p.nrows = NR; p.K = om->abc->K;
 
   // ---- SSV / MSV-full tables: int16 (bias - rbv) ----
   auto rsv = [&](int x, int k) -> int {               // score at node k (1-based) or "impossible"
@@ -317,15 +324,23 @@ dev_profile_build(P7_GPU_OPROFILE *om, int Lmax_need, struct p7g_dev_profile_s *
   p.xfE_move = om->xf[p7G_XE][p7G_MOVE]; p.xfE_loop = om->xf[p7G_XE][p7G_LOOP];
   p.nj = om->nj;
 
+  // ---- degenerate codes: canonical residues of each code, and how many ----
+  std::vector<uint32_t> degen(32, 0u);
+  for (int x = 0; x < Kp && x < 32; x++) {
+    uint32_t m = 0;
+    for (int y = 0; y < om->abc->K; y++) if (om->abc->degen[x][y]) m |= 1u << y;
+    degen[x] = m | ((uint32_t) om->abc->ndegen[x] << 24);
+  }
+
   // ---- one allocation ----
   const size_t L1 = (size_t) Lmax + 1;
   size_t sz[] = { p.ssv[0].dwords * 4, p.ssv[1].dwords * 4, p.ssv[2].dwords * 4, p.mf_dwords * 4, p.vf_dwords * 4,
                   p.fb_floats * 4, L1, L1 * 2, L1 * 4, L1 * 4, L1 * 4, L1 * 4, L1 * 4, eo.size() * 4,
                   p.ssv_nchunk > 0 ? p.ssvc[0].dwords * 4 : 0, p.ssv_nchunk > 1 ? p.ssvc[1].dwords * 4 : 0,
-                  p.ssv_nchunk > 2 ? p.ssvc[2].dwords * 4 : 0, p.ssv[3].dwords * 4 };
+                  p.ssv_nchunk > 2 ? p.ssvc[2].dwords * 4 : 0, p.ssv[3].dwords * 4, degen.size() * 4 };
   const void *src[] = { ssv[0].data(), ssv[1].data(), ssv[2].data(), mf.data(), vf.data(), fb.data(), lc.tjb_b,
                         lc.xw_move, lc.xf_move, lc.xf_loop, lc.nullsc, lc.flen_a, lc.flen_b, eo.data(),
-                        ssvc[0].data(), ssvc[1].data(), ssvc[2].data(), ssv[3].data() };
+                        ssvc[0].data(), ssvc[1].data(), ssvc[2].data(), ssv[3].data(), degen.data() };
   const int NB = (int) (sizeof(sz) / sizeof(sz[0]));
   size_t offs[20], total = 0;
   for (int i = 0; i < NB; i++) { offs[i] = total; total += (sz[i] + 255) & ~(size_t) 255; }
@@ -352,6 +367,7 @@ dev_profile_build(P7_GPU_OPROFILE *om, int Lmax_need, struct p7g_dev_profile_s *
   p.xf_move = (float *) (blob + offs[8]); p.xf_loop = (float *) (blob + offs[9]);
   p.nullsc = (float *) (blob + offs[10]); p.flen_a = (float *) (blob + offs[11]); p.flen_b = (float *) (blob + offs[12]);
   p.eo = (float *) (blob + offs[13]);
+  p.degen = (const uint32_t *) (blob + offs[18]);
   for (int c = 0; c < P7G_SSV_MAXCHUNK; c++) p.ssvc[c].tbl = (c < p.ssv_nchunk) ? (int *) (blob + offs[14 + c]) : nullptr;
   d->Lmax = Lmax;
   d->serial = ++g_serial;
@@ -745,7 +761,7 @@ static int64_t pipeline_pool_floats(int64_t nseq, int64_t nres) { return std::mi
 static int
 work_reserve(Work &w, int64_t n, int64_t pool_floats)
 {
-  if (! w.ev_ready) { for (int i = 0; i < 8; i++) HIPCHK(hipEventCreate(&w.ev[i])); w.ev_ready = true; }
+  if (! w.ev_ready) { for (int i = 0; i < 10; i++) HIPCHK(hipEventCreate(&w.ev[i])); w.ev_ready = true; }
   if (n <= w.cap && pool_floats <= w.pool_size) return p7G_OK;
   if (w.mem || w.pool) HIPCHK(hipStreamSynchronize(w.st));
   if (w.mem) { (void) hipFree(w.mem); w.mem = nullptr; w.cap = 0; }
@@ -805,6 +821,12 @@ work_bnd(Work &w, const DevProfile &p, int64_t bytes)
 // most 2^26 bytes), and a row pool of 15 floats per row of an F3 survivor (6
 // per row of a target that only reaches Forward), at most 2^28 floats.
 #define DOM_REG_CAP ((int64_t) 1 << 20)
+// null2 of single-domain regions: regions go in batches whose areas fit a
+// pool of at most N2_POOL_CAP floats (2 GiB); each batch takes three list
+// launches, each with its own count/queue block of 16 ints
+#define N2_POOL_CAP ((int64_t) 1 << 29)
+#define N2_MAX_BATCH 256
+#define N2_CNT_INTS (N2_MAX_BATCH * 3 * 16)
 #define DOM_RES_CAP ((int64_t) 1 << 26)
 static int64_t dom_pool_floats(int64_t nseq, int64_t nres) { return std::min<int64_t>(16 * (nres + nseq), (int64_t) 1 << 28); }
 
@@ -817,8 +839,8 @@ dom_reserve(Work &w, int64_t n, int64_t pool_floats)
   if (w.dom_mem) { (void) hipFree(w.dom_mem); w.dom_mem = nullptr; w.dom_cap = 0; }
   if (w.dom.pool) { (void) hipFree(w.dom.pool); w.dom.pool = nullptr; w.dom.size = 0; }
   const int64_t cap = std::max<int64_t>(n, 1024);
-  const size_t total = (size_t) cap * (8 + 8 + 4 + 4 + 4) + (16 + 8 + 4 * 6) * (size_t) DOM_REG_CAP + (size_t) DOM_RES_CAP
-                       + 16 * 256;
+  const size_t total = (size_t) cap * (8 + 8 + 4 + 4 + 4) + (16 + 8 + 8 + 4 * 9) * (size_t) DOM_REG_CAP
+                       + (size_t) DOM_RES_CAP + 4 * (size_t) N2_CNT_INTS + 20 * 256;
   char *m = nullptr;
   HIPCHK(hipMalloc((void **) &m, total));
   size_t o = 0;
@@ -834,6 +856,10 @@ dom_reserve(Work &w, int64_t n, int64_t pool_floats)
   w.dout.env_loop = (float *) take(4 * (size_t) DOM_REG_CAP); w.dout.env_move = (float *) take(4 * (size_t) DOM_REG_CAP);
   w.dout.envsc = (float *) take(4 * (size_t) DOM_REG_CAP);
   w.dout.res = (uint8_t *) take((size_t) DOM_RES_CAP); w.dout.res_cap = DOM_RES_CAP;
+  w.n2off = (int64_t *) take(8 * (size_t) DOM_REG_CAP);
+  w.n2dc = (float *) take(4 * (size_t) DOM_REG_CAP); w.n2st = (int32_t *) take(4 * (size_t) DOM_REG_CAP);
+  w.n2list = (int32_t *) take(4 * (size_t) DOM_REG_CAP);
+  w.n2cnt = (int32_t *) take(4 * (size_t) N2_CNT_INTS);
   if (o > total) { (void) hipFree(m); p7g_set_error("internal: domain buffer overrun"); return p7G_EINVAL; }
   w.dom_mem = m; w.dom_cap = cap;
   HIPCHK(hipMalloc((void **) &w.dom.pool, sizeof(float) * (size_t) pool_floats));
@@ -847,12 +873,15 @@ work_release(Work &w)
   if (w.st) (void) hipStreamSynchronize(w.st);
   if (w.dom_mem) (void) hipFree(w.dom_mem);
   if (w.dom.pool) (void) hipFree(w.dom.pool);
+  if (w.n2pool) (void) hipFree(w.n2pool);
   w.dom_mem = nullptr; w.dom = DomBufs(); w.dout = DomOut(); w.dom_cap = 0;
+  w.n2off = nullptr; w.n2dc = nullptr; w.n2st = nullptr; w.n2list = nullptr; w.n2cnt = nullptr;
+  w.n2pool = nullptr; w.n2size = 0;
   if (w.mem) (void) hipFree(w.mem);
   if (w.pool) (void) hipFree(w.pool);
   if (w.bnd) (void) hipFree(w.bnd);
   w.bnd = nullptr; w.bnd_cap = 0;
-  if (w.ev_ready) for (int i = 0; i < 8; i++) (void) hipEventDestroy(w.ev[i]);
+  if (w.ev_ready) for (int i = 0; i < 10; i++) (void) hipEventDestroy(w.ev[i]);
   w.mem = nullptr; w.pool = nullptr; w.cap = 0; w.pool_size = 0; w.ev_ready = false;
 }
 
@@ -1000,16 +1029,20 @@ struct p7g_pipeline_s {
   double F1, F2, F3;
   int    do_biasfilter, do_max;
   int    do_domains = 0;      // posterior decoding and regions of the F3 survivors (p7_gpu_Pipeline)
+  int    do_null2 = 1;        // and the null2 correction of single-domain regions (p7_pipeline.c:208, do_null2)
   // the last decoding run's region records: on the device until asked for,
   // then in the caller's sequence order
   mutable std::mutex regs_mu;
   mutable std::vector<int4> regs;
-  mutable std::vector<float> envsc;
+  mutable std::vector<float> envsc, n2dc;
+  mutable std::vector<int32_t> n2st;
   mutable bool regs_ready = false;
   bool   regs_valid = false;
   int64_t nregs = 0;
   const int4 *regs_dev = nullptr;
   const float *envsc_dev = nullptr;
+  const float *n2dc_dev = nullptr;
+  const int32_t *n2st_dev = nullptr;
   const int64_t *regs_perm = nullptr;
   P7_GPU_STATS st;
   Work       w;             // this pipeline's stream and device work area
@@ -1202,6 +1235,81 @@ stats_accumulate(P7_GPU_STATS &tot, const P7_GPU_STATS &one, bool first_of_call)
   tot.cells_fwd += one.cells_fwd; tot.cells_bck += one.cells_bck;
 }
 
+// The null2 correction of every region that is not multidomain
+// (rescore_isolated_domain, p7_domaindef.c:812-931), after a decoding run
+// left its nregs region records on the device. The host reads the records,
+// gives each such region an area in the null2 pool and groups the regions in
+// batches whose areas fit the pool; per batch, three list launches over the
+// region copies: the envelope Forward writing its row scales, the envelope
+// Backward writing its special and M/I rows, and the Forward again summing
+// the posteriors into null2 and the correction. Multidomain regions (whose
+// null2 comes from sampled traces) get status 2 and no correction.
+static int
+null2_regions(Work &w, const DevProfile &p, const DevSeqs &s, int64_t nregs, const GateThresh &g)
+{
+  if (nregs <= 0) return p7G_OK;
+  HIPCHK(launch_fill32((uint32_t *) w.n2st, 2u, nregs, w.st));
+  HIPCHK(launch_fill32((uint32_t *) w.n2dc, 0x7fc00000u, nregs, w.st));
+  std::vector<int4> rec((size_t) nregs);
+  std::vector<int32_t> vlen((size_t) nregs);
+  HIPCHK(hipMemcpyAsync(rec.data(), w.dout.regs, sizeof(int4) * (size_t) nregs, hipMemcpyDeviceToHost, w.st));
+  HIPCHK(hipMemcpyAsync(vlen.data(), w.dout.v_len, sizeof(int32_t) * (size_t) nregs, hipMemcpyDeviceToHost, w.st));
+  HIPCHK(hipStreamSynchronize(w.st));
+  std::vector<int64_t> off((size_t) nregs, 0);
+  std::vector<int32_t> list;
+  std::vector<std::pair<size_t, size_t>> batches;
+  int64_t used = 0, maxused = 0;
+  size_t start = 0;
+  for (int64_t k = 0; k < nregs; k++) {
+    if (rec[(size_t) k].w) continue;
+    // the kernels size a region by its copy's length, which is its i..j
+    const int Ld = vlen[(size_t) k];
+    if (Ld != rec[(size_t) k].z - rec[(size_t) k].y + 1) { p7g_set_error("internal: region copy length"); return p7G_EINVAL; }
+    const int64_t need = n2_area_floats(Ld, p.fb_B);
+    if (need > N2_POOL_CAP) { p7g_set_error("region of %d residues too long for the null2 pool", Ld); return p7G_EMEM; }
+    if (used + need > N2_POOL_CAP) { batches.push_back({ start, list.size() }); start = list.size(); used = 0; }
+    off[(size_t) k] = used; used += need; maxused = std::max(maxused, used);
+    list.push_back((int32_t) k);
+  }
+  if (list.size() > start) batches.push_back({ start, list.size() });
+  if (batches.empty()) return p7G_OK;
+  if ((int64_t) batches.size() > N2_MAX_BATCH) { p7g_set_error("too many null2 batches (%zu)", batches.size()); return p7G_EMEM; }
+  if (w.n2size < maxused) {
+    if (w.n2pool) { (void) hipFree(w.n2pool); w.n2pool = nullptr; w.n2size = 0; }
+    HIPCHK(hipMalloc((void **) &w.n2pool, sizeof(float) * (size_t) maxused));
+    w.n2size = maxused;
+  }
+  std::vector<int32_t> cnt(batches.size() * 3 * 16, 0);
+  for (size_t b = 0; b < batches.size(); b++)
+    for (int a = 0; a < 3; a++) cnt[(b * 3 + a) * 16] = (int32_t) (batches[b].second - batches[b].first);
+  HIPCHK(hipMemcpyAsync(w.n2off, off.data(), sizeof(int64_t) * (size_t) nregs, hipMemcpyHostToDevice, w.st));
+  HIPCHK(hipMemcpyAsync(w.n2list, list.data(), sizeof(int32_t) * list.size(), hipMemcpyHostToDevice, w.st));
+  HIPCHK(hipMemcpyAsync(w.n2cnt, cnt.data(), sizeof(int32_t) * cnt.size(), hipMemcpyHostToDevice, w.st));
+  DevSeqs vs = s;
+  vs.res = w.dout.res; vs.off = w.dout.v_off; vs.len = w.dout.v_len; vs.perm = nullptr;
+  vs.seg = nullptr; vs.nseg = 0; vs.nseg_slots = 0; vs.nslot = w.dout.reg_cap;
+  DevResults vf = w.r;
+  vf.fwdsc = w.dout.envsc; vf.fstatus = w.dout.env_st;
+  DevResults vb = w.r;
+  vb.bcksc = w.n2dc; vb.fstatus = w.n2st;
+  DomBufs eb;
+  eb.env_loop = w.dout.env_loop; eb.env_move = w.dout.env_move; eb.env_cap = w.dout.reg_cap;
+  eb.n2pool = w.n2pool; eb.n2off = w.n2off; eb.n2dc = w.n2dc; eb.n2st = w.n2st;
+  const LenCfg lc = lencfg_per_target();
+  for (size_t b = 0; b < batches.size(); b++) {
+    const int32_t *bl = w.n2list + batches[b].first;
+    const int64_t nb = (int64_t) (batches[b].second - batches[b].first);
+    const int32_t *c = w.n2cnt + b * 3 * 16;
+    eb.n2mode = 1;
+    HIPCHK(launch_fwd(p, vs, bl, c, nb, vf, w.l, g, 0, lc, nullptr, 0, nullptr, nullptr, w.st, eb));
+    HIPCHK(launch_bck(p, vs, bl, c + 16, nb, vb, nullptr, nullptr, nullptr, nullptr, lc, w.st, eb, w.l));
+    eb.n2mode = 2;
+    HIPCHK(launch_fwd(p, vs, bl, c + 32, nb, vf, w.l, g, 0, lc, nullptr, 0, nullptr, nullptr, w.st, eb));
+  }
+  HIPCHK(hipStreamSynchronize(w.st));          // the host vectors above are the copies' sources
+  return p7G_OK;
+}
+
 extern "C" int
 p7_gpu_Pipeline(P7_GPU_PIPELINE *pli, const P7_GPU_OPROFILE *om, const P7_GPU_SEQDB *db, P7_GPU_HITS *hits)
 {
@@ -1277,8 +1385,15 @@ p7_gpu_Pipeline(P7_GPU_PIPELINE *pli, const P7_GPU_OPROFILE *om, const P7_GPU_SE
       p7g_set_error("domain decoding %s exhausted", (counts[7] & 1) ? "row pool" : (counts[7] & 2) ? "region list" : "region residue pool");
       return p7G_EMEM;
     }
+    if (pli->do_null2) {
+      HIPCHK(hipEventRecord(w.ev[8], w.st));
+      if ((status = null2_regions(w, *p, s, nregs_dev, g)) != p7G_OK) return status;
+      HIPCHK(hipEventRecord(w.ev[9], w.st));
+      HIPCHK(hipEventSynchronize(w.ev[9]));
+    }
     std::lock_guard<std::mutex> rl(pli->regs_mu);
     pli->nregs = nregs_dev; pli->regs_dev = w.dout.regs; pli->envsc_dev = w.dout.envsc; pli->regs_perm = db->h_perm;
+    pli->n2dc_dev = pli->do_null2 ? w.n2dc : nullptr; pli->n2st_dev = w.n2st;
     pli->regs_ready = false; pli->regs.clear();
     pli->regs_valid = true;
   }
@@ -1298,6 +1413,8 @@ p7_gpu_Pipeline(P7_GPU_PIPELINE *pli, const P7_GPU_OPROFILE *om, const P7_GPU_SE
   if (dom) {
     HIPCHK(hipEventElapsedTime(&mtot, w.ev[5], w.ev[7]));
     st.ms_domains = mtot;
+    st.ms_null2 = 0.0;
+    if (pli->do_null2) { HIPCHK(hipEventElapsedTime(&mtot, w.ev[8], w.ev[9])); st.ms_null2 = mtot; }
     st.n_regions = pli->nregs;
     st.n_clustered = nclu_total;
   }
@@ -1314,18 +1431,32 @@ p7_gpu_pipeline_SetDomains(P7_GPU_PIPELINE *pli, int on)
 }
 
 extern "C" int
-p7_gpu_pipeline_GetRegions(const P7_GPU_PIPELINE *pli, int64_t *seq, int32_t *ienv, int32_t *jenv, int32_t *multi,
-                           float *envsc, int64_t cap, int64_t *ret_n)
+p7_gpu_pipeline_SetNull2(P7_GPU_PIPELINE *pli, int on)
 {
-  if (! pli || ! pli->regs_valid) { p7g_set_error("no domain regions: the last p7_gpu_Pipeline call did not decode"); return p7G_EINVAL; }
-  std::lock_guard<std::mutex> rl(pli->regs_mu);
+  if (! pli) return p7G_EINVAL;
+  std::lock_guard<std::mutex> lk(pli->mu);
+  pli->do_null2 = on ? 1 : 0;
+  return p7G_OK;
+}
+
+// The last decoding run's region records on the host, in sequence then start
+// order (called with regs_mu held).
+static int
+regs_fetch(const P7_GPU_PIPELINE *pli)
+{
   if (! pli->regs_ready) {
     // slot -> the caller's sequence index, then sequence and start order
-    std::vector<int4> rec((size_t) pli->nregs);
-    std::vector<float> sc((size_t) pli->nregs);
-    if (pli->nregs) {
-      HIPCHK(hipMemcpy(rec.data(), pli->regs_dev, sizeof(int4) * (size_t) pli->nregs, hipMemcpyDeviceToHost));
-      HIPCHK(hipMemcpy(sc.data(), pli->envsc_dev, sizeof(float) * (size_t) pli->nregs, hipMemcpyDeviceToHost));
+    const size_t nr = (size_t) pli->nregs;
+    std::vector<int4> rec(nr);
+    std::vector<float> sc(nr), dc(nr, NAN);
+    std::vector<int32_t> n2(nr, -1);
+    if (nr) {
+      HIPCHK(hipMemcpy(rec.data(), pli->regs_dev, sizeof(int4) * nr, hipMemcpyDeviceToHost));
+      HIPCHK(hipMemcpy(sc.data(), pli->envsc_dev, sizeof(float) * nr, hipMemcpyDeviceToHost));
+      if (pli->n2dc_dev) {
+        HIPCHK(hipMemcpy(dc.data(), pli->n2dc_dev, sizeof(float) * nr, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(n2.data(), pli->n2st_dev, sizeof(int32_t) * nr, hipMemcpyDeviceToHost));
+      }
     }
     for (auto &r : rec) r.x = (int) pli->regs_perm[r.x];
     std::vector<int64_t> ord((size_t) pli->nregs);
@@ -1333,10 +1464,24 @@ p7_gpu_pipeline_GetRegions(const P7_GPU_PIPELINE *pli, int64_t *seq, int32_t *ie
     std::sort(ord.begin(), ord.end(), [&](int64_t a, int64_t b) {
       const int4 &x = rec[(size_t) a], &y = rec[(size_t) b];
       return x.x != y.x ? x.x < y.x : x.y < y.y; });
-    pli->regs.resize((size_t) pli->nregs); pli->envsc.resize((size_t) pli->nregs);
-    for (int64_t k = 0; k < pli->nregs; k++) { pli->regs[(size_t) k] = rec[(size_t) ord[(size_t) k]]; pli->envsc[(size_t) k] = sc[(size_t) ord[(size_t) k]]; }
+    pli->regs.resize(nr); pli->envsc.resize(nr); pli->n2dc.resize(nr); pli->n2st.resize(nr);
+    for (size_t k = 0; k < nr; k++) {
+      const size_t o = (size_t) ord[k];
+      pli->regs[k] = rec[o]; pli->envsc[k] = sc[o]; pli->n2dc[k] = dc[o]; pli->n2st[k] = n2[o];
+    }
     pli->regs_ready = true;
   }
+  return p7G_OK;
+}
+
+extern "C" int
+p7_gpu_pipeline_GetRegions(const P7_GPU_PIPELINE *pli, int64_t *seq, int32_t *ienv, int32_t *jenv, int32_t *multi,
+                           float *envsc, int64_t cap, int64_t *ret_n)
+{
+  if (! pli || ! pli->regs_valid) { p7g_set_error("no domain regions: the last p7_gpu_Pipeline call did not decode"); return p7G_EINVAL; }
+  std::lock_guard<std::mutex> rl(pli->regs_mu);
+  int status;
+  if ((status = regs_fetch(pli)) != p7G_OK) return status;
   const int64_t n = (int64_t) pli->regs.size();
   if (ret_n) *ret_n = n;
   for (int64_t k = 0; k < std::min(n, cap); k++) {
@@ -1346,6 +1491,23 @@ p7_gpu_pipeline_GetRegions(const P7_GPU_PIPELINE *pli, int64_t *seq, int32_t *ie
     if (jenv) jenv[k] = r.z;
     if (multi) multi[k] = r.w;
     if (envsc) envsc[k] = pli->envsc[(size_t) k];
+  }
+  return p7G_OK;
+}
+
+extern "C" int
+p7_gpu_pipeline_GetRegionNull2(const P7_GPU_PIPELINE *pli, float *domcorr, int32_t *status, int64_t cap, int64_t *ret_n)
+{
+  if (! pli || ! pli->regs_valid) { p7g_set_error("no domain regions: the last p7_gpu_Pipeline call did not decode"); return p7G_EINVAL; }
+  if (! pli->n2dc_dev) { p7g_set_error("the last p7_gpu_Pipeline call ran without null2"); return p7G_EINVAL; }
+  std::lock_guard<std::mutex> rl(pli->regs_mu);
+  int st;
+  if ((st = regs_fetch(pli)) != p7G_OK) return st;
+  const int64_t n = (int64_t) pli->regs.size();
+  if (ret_n) *ret_n = n;
+  for (int64_t k = 0; k < std::min(n, cap); k++) {
+    if (domcorr) domcorr[k] = pli->n2dc[(size_t) k];
+    if (status) status[k] = pli->n2st[(size_t) k];
   }
   return p7G_OK;
 }

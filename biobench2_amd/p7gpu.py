@@ -58,7 +58,7 @@ class Stats(ctypes.Structure):
                [(n, c_double) for n in ("cells_msv", "cells_bias", "cells_vit", "cells_fwd", "cells_bck",
                                         "ms_msv", "ms_bias", "ms_vit", "ms_fwd", "ms_bck", "ms_total", "ms_ssv",
                                         "ms_domains")] + \
-               [(n, c_int64) for n in ("n_regions", "n_clustered")]
+               [(n, c_int64) for n in ("n_regions", "n_clustered")] + [("ms_null2", c_double)]
 
     def as_dict(self):
         return {n: getattr(self, n) for n, _ in self._fields_}
@@ -100,7 +100,8 @@ def _load():
         "p7_gpu_pipeline_Create": (vp, [d, d, d, i, i]), "p7_gpu_pipeline_Destroy": (None, [vp]),
         "p7_gpu_Pipeline": (i, [vp, vp, vp, POINTER(Hits)]),
         "p7_gpu_pipeline_GetStats": (i, [vp, POINTER(Stats)]),
-        "p7_gpu_pipeline_SetDomains": (i, [vp, i]),
+        "p7_gpu_pipeline_SetDomains": (i, [vp, i]), "p7_gpu_pipeline_SetNull2": (i, [vp, i]),
+        "p7_gpu_pipeline_GetRegionNull2": (i, [vp, POINTER(c_float), POINTER(c_int32), c_int64, POINTER(c_int64)]),
         "p7_gpu_pipeline_GetRegions": (i, [vp, POINTER(c_int64), POINTER(c_int32), POINTER(c_int32), POINTER(c_int32),
                                            POINTER(c_float), c_int64, POINTER(c_int64)]),
         "p7_gpu_ScoreThreshold": (f, [i, d, d, d]),
@@ -536,10 +537,11 @@ class Server:
 class Pipeline:
     """P7_PIPELINE thresholds and counters; run() is p7_Pipeline over a whole database."""
 
-    def __init__(self, F1=0.02, F2=1e-3, F3=1e-5, do_biasfilter=True, do_max=False, domains=False):
+    def __init__(self, F1=0.02, F2=1e-3, F3=1e-5, do_biasfilter=True, do_max=False, domains=False, null2=True):
         self.h = lib.p7_gpu_pipeline_Create(F1, F2, F3, int(do_biasfilter), int(do_max))
         self.domains = bool(domains)
         _check(lib.p7_gpu_pipeline_SetDomains(self.h, int(self.domains)), "pipeline_SetDomains")
+        _check(lib.p7_gpu_pipeline_SetNull2(self.h, int(bool(null2))), "pipeline_SetNull2")
 
     def run(self, om, db, scores=True):
         """p7_gpu_Pipeline; with domains=True (and scores) the outputs also hold
@@ -560,6 +562,17 @@ class Pipeline:
                                               _ptr(m, c_int32), _ptr(sc, c_float), n.value, ctypes.byref(n)),
                "GetRegions")
         return [(int(seq[k]), int(i[k]), int(j[k]), bool(m[k]), float(sc[k])) for k in range(n.value)]
+
+    def region_null2(self):
+        """(domcorr[n] nats, status[n]) of the last run's regions, in regions() order: status 0
+        computed, 2 multidomain (not computed), 16 the decoding overflowed."""
+        n = c_int64()
+        _check(lib.p7_gpu_pipeline_GetRegionNull2(self.h, None, None, 0, ctypes.byref(n)), "GetRegionNull2")
+        dc = np.zeros(n.value, np.float32)
+        st = np.zeros(n.value, np.int32)
+        _check(lib.p7_gpu_pipeline_GetRegionNull2(self.h, _ptr(dc, c_float), _ptr(st, c_int32), n.value,
+                                                  ctypes.byref(n)), "GetRegionNull2")
+        return dc, st
 
     def run_batch(self, oms, db, scores=True, graph=False):
         """p7_gpu_PipelineBatch: every profile of `oms` over the database in one

@@ -129,6 +129,7 @@ typedef struct p7g_stats_s {
   double  ms_ssv;              /* the ungapped MSV pass alone (k_ssv), last call    */
   double  ms_domains;          /* domain decoding and regions, last call (0 if off) */
   int64_t n_regions, n_clustered;   /* regions of the last call, and multidomain ones */
+  double  ms_null2;            /* null2 of the single-domain regions, last call (0 if off) */
 } P7_GPU_STATS;
 
 /* ---- device ------------------------------------------------------------- */
@@ -253,10 +254,16 @@ int   p7_gpu_pipeline_GetStats(const P7_GPU_PIPELINE *pli, P7_GPU_STATS *st);
  * p7_Pipeline calls it (p7_pipeline.c:737): p7_DomainDecoding
  * (impl_sse/decoding.c:166-197) from the two parsers' special states, then
  * the region heuristics with is_multidomain_region (rt1 0.25, rt2 0.10, rt3
- * 0.20). The stochastic trace clustering of multidomain regions, envelope
- * rescoring, null2 and the alignments that follow in the reference are not
- * computed. Off by default; batches and the server do not decode. */
+ * 0.20), each region's envelope score, and (p7_gpu_pipeline_SetNull2) the
+ * null2 correction of each region that is not multidomain. The stochastic
+ * trace clustering of multidomain regions and the alignments that follow in
+ * the reference are not computed. Off by default; batches and the server do
+ * not decode. */
 int   p7_gpu_pipeline_SetDomains(P7_GPU_PIPELINE *pli, int on);
+/* The null2 correction of decoded regions (pli->do_null2, p7_pipeline.c:208;
+ * hmmsearch --nonull2 turns it off). On by default; it applies when decoding
+ * is on. */
+int   p7_gpu_pipeline_SetNull2(P7_GPU_PIPELINE *pli, int on);
 /* The regions of the last p7_gpu_Pipeline call with decoding on, ordered by
  * sequence (the database's order) then start: sequence index, first and last
  * residue (1-based, inclusive), 1 if flagged multidomain, and the envelope
@@ -268,6 +275,17 @@ int   p7_gpu_pipeline_SetDomains(P7_GPU_PIPELINE *pli, int on);
  * pipeline's next call, while that call's database lives. */
 int   p7_gpu_pipeline_GetRegions(const P7_GPU_PIPELINE *pli, int64_t *seq, int32_t *ienv, int32_t *jenv,
                                  int32_t *multi, float *envsc, int64_t cap, int64_t *ret_n);
+/* The null2 correction of each region, in p7_gpu_pipeline_GetRegions' order:
+ * domcorr = sum over the envelope's residues of log null2(x) (nats), as
+ * rescore_isolated_domain computes it for a one-envelope region
+ * (p7_domaindef.c:841-931: p7_Forward and p7_Backward over residues i..j in
+ * unihit mode, p7_Decoding, p7_Null2_ByExpectation). status: 0 computed;
+ * 2 multidomain region (the reference takes its null2 from sampled traces:
+ * not computed, domcorr NaN); 16 the decoding overflowed (eslERANGE, for
+ * which the reference drops the domain). p7G_EINVAL if the last call ran
+ * without null2. */
+int   p7_gpu_pipeline_GetRegionNull2(const P7_GPU_PIPELINE *pli, float *domcorr, int32_t *status, int64_t cap,
+                                     int64_t *ret_n);
 /* ---- batches: many profiles against one resident database ----------------
  * Runs the p7_gpu_Pipeline filter chain of nprof profiles over every database
  * sequence in one call. The ungapped MSV pass of all profiles that share a
