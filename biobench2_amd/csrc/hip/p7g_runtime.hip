@@ -1035,7 +1035,7 @@ struct p7g_pipeline_s {
   mutable std::mutex regs_mu;
   mutable std::vector<int4> regs;
   mutable std::vector<float> envsc, n2dc;
-  mutable std::vector<int32_t> n2st;
+  mutable std::vector<int32_t> n2st, regs_slot;
   mutable bool regs_ready = false;
   bool   regs_valid = false;
   int64_t nregs = 0;
@@ -1043,6 +1043,12 @@ struct p7g_pipeline_s {
   const float *envsc_dev = nullptr;
   const float *n2dc_dev = nullptr;
   const int32_t *n2st_dev = nullptr;
+  // for the hit-list scores: the run's Forward scores (by slot), target
+  // lengths (by slot) and the profile's Forward tail parameters
+  const float *fwdsc_dev = nullptr;
+  const int32_t *regs_len = nullptr;
+  int64_t nslots = 0;
+  float ftau = 0.0f, flambda = 0.0f;
   const int64_t *regs_perm = nullptr;
   P7_GPU_STATS st;
   Work       w;             // this pipeline's stream and device work area
@@ -1394,6 +1400,8 @@ p7_gpu_Pipeline(P7_GPU_PIPELINE *pli, const P7_GPU_OPROFILE *om, const P7_GPU_SE
     std::lock_guard<std::mutex> rl(pli->regs_mu);
     pli->nregs = nregs_dev; pli->regs_dev = w.dout.regs; pli->envsc_dev = w.dout.envsc; pli->regs_perm = db->h_perm;
     pli->n2dc_dev = pli->do_null2 ? w.n2dc : nullptr; pli->n2st_dev = w.n2st;
+    pli->fwdsc_dev = w.r.fwdsc; pli->regs_len = db->h_len; pli->nslots = n;
+    pli->ftau = om->evparam[p7G_FTAU]; pli->flambda = om->evparam[p7G_FLAMBDA];
     pli->regs_ready = false; pli->regs.clear();
     pli->regs_valid = true;
   }
@@ -1458,16 +1466,18 @@ regs_fetch(const P7_GPU_PIPELINE *pli)
         HIPCHK(hipMemcpy(n2.data(), pli->n2st_dev, sizeof(int32_t) * nr, hipMemcpyDeviceToHost));
       }
     }
-    for (auto &r : rec) r.x = (int) pli->regs_perm[r.x];
+    std::vector<int32_t> slot_of(nr);
+    for (size_t k = 0; k < nr; k++) { slot_of[k] = rec[k].x; rec[k].x = (int) pli->regs_perm[rec[k].x]; }
     std::vector<int64_t> ord((size_t) pli->nregs);
     for (int64_t k = 0; k < pli->nregs; k++) ord[(size_t) k] = k;
     std::sort(ord.begin(), ord.end(), [&](int64_t a, int64_t b) {
       const int4 &x = rec[(size_t) a], &y = rec[(size_t) b];
       return x.x != y.x ? x.x < y.x : x.y < y.y; });
-    pli->regs.resize(nr); pli->envsc.resize(nr); pli->n2dc.resize(nr); pli->n2st.resize(nr);
+    pli->regs.resize(nr); pli->envsc.resize(nr); pli->n2dc.resize(nr); pli->n2st.resize(nr); pli->regs_slot.resize(nr);
     for (size_t k = 0; k < nr; k++) {
       const size_t o = (size_t) ord[k];
       pli->regs[k] = rec[o]; pli->envsc[k] = sc[o]; pli->n2dc[k] = dc[o]; pli->n2st[k] = n2[o];
+      pli->regs_slot[k] = slot_of[o];
     }
     pli->regs_ready = true;
   }
@@ -1508,6 +1518,93 @@ p7_gpu_pipeline_GetRegionNull2(const P7_GPU_PIPELINE *pli, float *domcorr, int32
   for (int64_t k = 0; k < std::min(n, cap); k++) {
     if (domcorr) domcorr[k] = pli->n2dc[(size_t) k];
     if (status) status[k] = pli->n2st[(size_t) k];
+  }
+  return p7G_OK;
+}
+
+// Hit-list scores of the last decoding run with null2: per target with
+// regions, the p7_Pipeline scores after domain definition
+// (p7_pipeline.c:743-850, host_hits.c), and per region its domain's.
+static int
+hit_scores(const P7_GPU_PIPELINE *pli, std::vector<int64_t> &tseq, std::vector<P7G_TSCORE> &ts,
+           std::vector<int32_t> &tnd, std::vector<int32_t> &tst, std::vector<P7G_DSCORE> &ds)
+{
+  const size_t nr = pli->regs.size();
+  std::vector<float> fwd((size_t) pli->nslots);
+  if (pli->nslots) HIPCHK(hipMemcpy(fwd.data(), pli->fwdsc_dev, sizeof(float) * (size_t) pli->nslots, hipMemcpyDeviceToHost));
+  const float omega = (float) (1. / 256.);                    // p7_bg.c:75
+  ds.assign(nr, P7G_DSCORE{ NAN, NAN, NAN });
+  for (size_t a = 0; a < nr; ) {
+    size_t b = a;
+    while (b < nr && pli->regs[b].x == pli->regs[a].x) b++;    // regions a..b-1: one target
+    const int slot = pli->regs_slot[a];
+    const int L = pli->regs_len[slot];
+    bool multi = false;
+    std::vector<P7G_DOMIN> dom;
+    std::vector<size_t> which;
+    for (size_t k = a; k < b; k++) {
+      if (pli->regs[k].w) { multi = true; continue; }
+      if (pli->n2st[k] != 0) continue;                      // rescoring failed: no domain (p7_domaindef.c:841-843)
+      dom.push_back(P7G_DOMIN{ pli->envsc[k], pli->n2dc[k], pli->regs[k].y, pli->regs[k].z });
+      which.push_back(k);
+    }
+    P7G_TSCORE t{ NAN, NAN, NAN, NAN, NAN, NAN, -1 };
+    std::vector<P7G_DSCORE> d(dom.size());
+    p7g_target_scores(L, fwd[(size_t) slot], p7g_null_one(L), omega, pli->ftau, pli->flambda, dom.data(), (int) dom.size(),
+                      &t, d.data());
+    for (size_t q = 0; q < dom.size(); q++) ds[which[q]] = d[q];
+    if (multi) t = P7G_TSCORE{ NAN, NAN, NAN, NAN, NAN, NAN, -1 };
+    else if (! dom.empty()) t.best = (int) (which[(size_t) t.best] - a);
+    else t.best = -1;
+    tseq.push_back(pli->regs[a].x); ts.push_back(t); tnd.push_back((int32_t) dom.size()); tst.push_back(multi ? 2 : 0);
+    a = b;
+  }
+  return p7G_OK;
+}
+
+extern "C" int
+p7_gpu_pipeline_GetTargetScores(const P7_GPU_PIPELINE *pli, int64_t *seq, float *score, float *pre_score, float *sum_score,
+                                double *lnP, int32_t *ndom, int32_t *best, int32_t *status, int64_t cap, int64_t *ret_n)
+{
+  if (! pli || ! pli->regs_valid) { p7g_set_error("no domain regions: the last p7_gpu_Pipeline call did not decode"); return p7G_EINVAL; }
+  if (! pli->n2dc_dev) { p7g_set_error("the last p7_gpu_Pipeline call ran without null2"); return p7G_EINVAL; }
+  std::lock_guard<std::mutex> rl(pli->regs_mu);
+  int st;
+  if ((st = regs_fetch(pli)) != p7G_OK) return st;
+  std::vector<int64_t> tseq; std::vector<P7G_TSCORE> ts; std::vector<int32_t> tnd, tst; std::vector<P7G_DSCORE> ds;
+  if ((st = hit_scores(pli, tseq, ts, tnd, tst, ds)) != p7G_OK) return st;
+  const int64_t n = (int64_t) tseq.size();
+  if (ret_n) *ret_n = n;
+  for (int64_t k = 0; k < std::min(n, cap); k++) {
+    if (seq) seq[k] = tseq[(size_t) k];
+    if (score) score[k] = ts[(size_t) k].score;
+    if (pre_score) pre_score[k] = ts[(size_t) k].pre_score;
+    if (sum_score) sum_score[k] = ts[(size_t) k].sum_score;
+    if (lnP) lnP[k] = ts[(size_t) k].lnP;
+    if (ndom) ndom[k] = tnd[(size_t) k];
+    if (best) best[k] = ts[(size_t) k].best;
+    if (status) status[k] = tst[(size_t) k];
+  }
+  return p7G_OK;
+}
+
+extern "C" int
+p7_gpu_pipeline_GetDomainScores(const P7_GPU_PIPELINE *pli, float *bitscore, float *dombias, double *lnP, int64_t cap,
+                                int64_t *ret_n)
+{
+  if (! pli || ! pli->regs_valid) { p7g_set_error("no domain regions: the last p7_gpu_Pipeline call did not decode"); return p7G_EINVAL; }
+  if (! pli->n2dc_dev) { p7g_set_error("the last p7_gpu_Pipeline call ran without null2"); return p7G_EINVAL; }
+  std::lock_guard<std::mutex> rl(pli->regs_mu);
+  int st;
+  if ((st = regs_fetch(pli)) != p7G_OK) return st;
+  std::vector<int64_t> tseq; std::vector<P7G_TSCORE> ts; std::vector<int32_t> tnd, tst; std::vector<P7G_DSCORE> ds;
+  if ((st = hit_scores(pli, tseq, ts, tnd, tst, ds)) != p7G_OK) return st;
+  const int64_t n = (int64_t) ds.size();
+  if (ret_n) *ret_n = n;
+  for (int64_t k = 0; k < std::min(n, cap); k++) {
+    if (bitscore) bitscore[k] = ds[(size_t) k].bitscore;
+    if (dombias) dombias[k] = ds[(size_t) k].dombias;
+    if (lnP) lnP[k] = ds[(size_t) k].lnP;
   }
   return p7G_OK;
 }

@@ -126,6 +126,16 @@ double p7g_exp_surv(double x, double mu, double lambda);
 float  p7g_gumbel_threshold(double mu, double lambda, double P);
 float  p7g_exp_threshold(double mu, double lambda, double P);
 
+/* hit-list scores (host_hits.c, p7_pipeline.c:743-850) */
+typedef struct { float envsc, domcorr; int ienv, jenv; } P7G_DOMIN;       /* one domain: nats, 1-based coords */
+typedef struct { float score, pre_score, sum_score; double lnP, pre_lnP, sum_lnP; int best; } P7G_TSCORE;
+typedef struct { float bitscore, dombias; double lnP; } P7G_DSCORE;        /* bits, nats, log P */
+float  p7g_flogsum(float a, float b);
+double p7g_exp_logsurv(double x, double mu, double lambda);
+float  p7g_null_one(int L);
+void   p7g_target_scores(int L, float fwdsc, float nullsc, float omega, float tau, float lambda,
+                         const P7G_DOMIN *dom, int ndom, P7G_TSCORE *ts, P7G_DSCORE *ds);
+
 /* error text */
 void p7g_set_error(const char *fmt, ...);
 

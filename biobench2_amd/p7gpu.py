@@ -102,6 +102,11 @@ def _load():
         "p7_gpu_pipeline_GetStats": (i, [vp, POINTER(Stats)]),
         "p7_gpu_pipeline_SetDomains": (i, [vp, i]), "p7_gpu_pipeline_SetNull2": (i, [vp, i]),
         "p7_gpu_pipeline_GetRegionNull2": (i, [vp, POINTER(c_float), POINTER(c_int32), c_int64, POINTER(c_int64)]),
+        "p7_gpu_pipeline_GetTargetScores": (i, [vp, POINTER(c_int64), POINTER(c_float), POINTER(c_float), POINTER(c_float),
+                                                POINTER(c_double), POINTER(c_int32), POINTER(c_int32), POINTER(c_int32),
+                                                c_int64, POINTER(c_int64)]),
+        "p7_gpu_pipeline_GetDomainScores": (i, [vp, POINTER(c_float), POINTER(c_float), POINTER(c_double), c_int64,
+                                                POINTER(c_int64)]),
         "p7_gpu_pipeline_GetRegions": (i, [vp, POINTER(c_int64), POINTER(c_int32), POINTER(c_int32), POINTER(c_int32),
                                            POINTER(c_float), c_int64, POINTER(c_int64)]),
         "p7_gpu_ScoreThreshold": (f, [i, d, d, d]),
@@ -573,6 +578,31 @@ class Pipeline:
         _check(lib.p7_gpu_pipeline_GetRegionNull2(self.h, _ptr(dc, c_float), _ptr(st, c_int32), n.value,
                                                   ctypes.byref(n)), "GetRegionNull2")
         return dc, st
+
+    def target_scores(self):
+        """Per target with regions (sequence order): dict of arrays seq, score, pre_score,
+        sum_score (bits), lnP, ndom, best (region index within the target, -1 none) and
+        status (0 scored, 2 has a multidomain region)."""
+        n = c_int64()
+        _check(lib.p7_gpu_pipeline_GetTargetScores(self.h, *([None] * 8), 0, ctypes.byref(n)), "GetTargetScores")
+        o = {"seq": np.zeros(n.value, np.int64), "score": np.zeros(n.value, np.float32),
+             "pre_score": np.zeros(n.value, np.float32), "sum_score": np.zeros(n.value, np.float32),
+             "lnP": np.zeros(n.value, np.float64), "ndom": np.zeros(n.value, np.int32),
+             "best": np.zeros(n.value, np.int32), "status": np.zeros(n.value, np.int32)}
+        _check(lib.p7_gpu_pipeline_GetTargetScores(
+            self.h, _ptr(o["seq"], c_int64), _ptr(o["score"], c_float), _ptr(o["pre_score"], c_float),
+            _ptr(o["sum_score"], c_float), _ptr(o["lnP"], c_double), _ptr(o["ndom"], c_int32),
+            _ptr(o["best"], c_int32), _ptr(o["status"], c_int32), n.value, ctypes.byref(n)), "GetTargetScores")
+        return o
+
+    def domain_scores(self):
+        """(bitscore, dombias nats, lnP) arrays per region, in regions() order."""
+        n = c_int64()
+        _check(lib.p7_gpu_pipeline_GetDomainScores(self.h, None, None, None, 0, ctypes.byref(n)), "GetDomainScores")
+        b, d, p = np.zeros(n.value, np.float32), np.zeros(n.value, np.float32), np.zeros(n.value, np.float64)
+        _check(lib.p7_gpu_pipeline_GetDomainScores(self.h, _ptr(b, c_float), _ptr(d, c_float), _ptr(p, c_double),
+                                                   n.value, ctypes.byref(n)), "GetDomainScores")
+        return b, d, p
 
     def run_batch(self, oms, db, scores=True, graph=False):
         """p7_gpu_PipelineBatch: every profile of `oms` over the database in one

@@ -286,6 +286,28 @@ int   p7_gpu_pipeline_GetRegions(const P7_GPU_PIPELINE *pli, int64_t *seq, int32
  * without null2. */
 int   p7_gpu_pipeline_GetRegionNull2(const P7_GPU_PIPELINE *pli, float *domcorr, int32_t *status, int64_t cap,
                                      int64_t *ret_n);
+/* Hit-list scores of the last p7_gpu_Pipeline call with decoding and null2,
+ * as p7_Pipeline computes them after domain definition (p7_pipeline.c:743-850):
+ * one entry per target with at least one region, in sequence order: the
+ * null2-corrected bit score (the reconstruction score when that is higher
+ * and counts a domain), the uncorrected one, the reconstruction score, the
+ * natural-log P-value of the score under the Forward exponential tail
+ * (esl_exp_logsurv with the profile's tau, lambda), the number of domains
+ * (regions whose rescoring succeeded), the best domain's index among the
+ * target's regions (GetRegions order, -1 without domains) and a status: 0
+ * scored; 2 the target has a multidomain region (not scored: NaN). The
+ * E-value of a score is exp(lnP) * Z for a search space of Z targets. */
+int   p7_gpu_pipeline_GetTargetScores(const P7_GPU_PIPELINE *pli, int64_t *seq, float *score, float *pre_score,
+                                      float *sum_score, double *lnP, int32_t *ndom, int32_t *best, int32_t *status,
+                                      int64_t cap, int64_t *ret_n);
+/* Per region (GetRegions order), its domain's scores (p7_pipeline.c:843-850):
+ * the bit score after the null2 correction and the length correction of the
+ * rest of the target, dombias (nats) = p7_FLogsum(0, log(1/256) + domcorr),
+ * and the natural-log P-value of the bit score (the i-Evalue is exp(lnP) * Z,
+ * the c-Evalue exp(lnP) * domZ). NaN for multidomain regions and regions whose
+ * rescoring failed. */
+int   p7_gpu_pipeline_GetDomainScores(const P7_GPU_PIPELINE *pli, float *bitscore, float *dombias, double *lnP,
+                                      int64_t cap, int64_t *ret_n);
 /* ---- batches: many profiles against one resident database ----------------
  * Runs the p7_gpu_Pipeline filter chain of nprof profiles over every database
  * sequence in one call. The ungapped MSV pass of all profiles that share a

@@ -9,6 +9,7 @@ Tables are the unstriped P7_GPU_TABLES layout as returned by
 biobench2_amd.p7gpu.OProfile.tables().
 """
 import ctypes
+import math
 import os
 import subprocess
 from ctypes import POINTER, c_double, c_float, c_int, c_int16, c_int32, c_int64, c_uint8, c_uint32
@@ -455,3 +456,49 @@ def flogsum(a, b):
     if mn == -np.inf or np.float32(mx - mn) >= np.float32(15.7):
         return mx
     return np.float32(mx + _FLOGSUM[int(np.float32(np.float32(mx - mn) * np.float32(1000.0)))])
+
+
+LOG2 = 0.69314718055994529          # eslCONST_LOG2
+OMEGA = np.float32(1.0 / 256.0)     # p7_bg.c:75, bg->omega
+
+
+def exp_logsurv(x, mu, lam):
+    """esl_exp_logsurv (easel/esl_exponential.c:141-145)."""
+    return 0.0 if x < mu else -lam * (x - mu)
+
+
+def target_scores(L, fwdsc, nullsc, doms, tau, lam):
+    """p7_Pipeline's scores after domain definition (p7_pipeline.c:743-850), in its float and
+    double arithmetic. doms: [(envsc, domcorrection, ienv, jenv)] of the domains (nats).
+    Returns (seq_score, pre_score, sum_score, lnP, best, [(bitscore, dombias, lnP)])."""
+    f = np.float32
+    fwdsc, nullsc = f(fwdsc), f(nullsc)
+    tau, lam = float(f(tau)), float(f(lam))          # om->evparam[] is float
+    seqbias = f(0.0)
+    for _, dc, _, _ in doms:
+        seqbias = f(seqbias + f(dc))
+    seqbias = flogsum(0.0, math.log(OMEGA) + float(seqbias))
+    pre_score = f(float(f(fwdsc - nullsc)) / LOG2)
+    seq_score = f(float(f(fwdsc - f(nullsc + seqbias))) / LOG2)
+    sum_score, Ld, seqbias = f(0.0), 0, f(0.0)
+    for envsc, dc, i, j in doms:
+        if float(f(f(envsc) - f(dc))) > 0.0:
+            sum_score = f(sum_score + f(envsc))
+            Ld += j - i + 1
+            seqbias = f(seqbias + f(dc))
+    seqbias = flogsum(0.0, math.log(OMEGA) + float(seqbias))
+    lratio = math.log(float(f(f(L) / f(L + 3))))
+    sum_score = f(float(sum_score) + (L - Ld) * lratio)
+    pre2_score = f(float(f(sum_score - nullsc)) / LOG2)
+    sum_score = f(float(f(sum_score - f(nullsc + seqbias))) / LOG2)
+    if Ld > 0 and sum_score > seq_score:
+        seq_score, pre_score = sum_score, pre2_score
+    dres, best = [], 0
+    for d, (envsc, dc, i, j) in enumerate(doms):
+        bits = f(float(f(envsc)) + (L - (j - i + 1)) * lratio)
+        dombias = flogsum(0.0, math.log(OMEGA) + float(f(dc)))
+        bits = f(float(f(bits - f(nullsc + dombias))) / LOG2)
+        dres.append((bits, dombias, exp_logsurv(float(bits), tau, lam)))
+        if bits > dres[best][0]:
+            best = d
+    return seq_score, pre_score, sum_score, exp_logsurv(float(seq_score), tau, lam), best, dres

@@ -1,0 +1,100 @@
+"""Hit-list scores after domain definition on the device pipeline
+(p7_gpu_pipeline_GetTargetScores / GetDomainScores; HMMER/src/p7_pipeline.c:
+743-850): against tutorial/fn3.out's domain table, and against the oracle's
+whole chain (its parsers, decoding, regions, envelope scores, null2 and
+target_scores) on synthetic databases.
+
+The device's inputs (Forward and envelope scores, corrections) equal the
+oracle's within float rounding, and p7_FLogsum's table steps by 0.001 nats;
+so scores are compared within (1e-4 max(1, |fwdsc|) + 1e-5 per envelope
+residue + 2e-3) nats, in bits."""
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+from biobench2_amd import p7gpu, seqio, synth
+from conftest import golden, synth_profile
+from test_gpu_domains import FN3_ENV, per_target_xf
+from test_hits_oracle import FN3_BIAS, FN3_IEVALUE, FN3_SCORE, printed_equal
+
+pytestmark = pytest.mark.gpu
+
+
+def test_fn3_tutorial_domain_table_on_the_device(gpu):
+    (name, s), = seqio.read_uniprot(golden("7LESS_DROME"))
+    rng = np.random.default_rng(3)
+    seqs = [synth.iid_residues(int(L), rng) for L in (300, 2554, 50)]
+    seqs.insert(1, s)
+    db = gpu.SeqDB.from_list(seqs)
+    om = gpu.OProfile.from_file(golden("fn3.hmm"))
+    pli = gpu.Pipeline(domains=True)
+    pli.run(om, db)
+    regs = pli.regions()
+    bits, dombias, lnP = pli.domain_scores()
+    mine = [k for k, r in enumerate(regs) if r[0] == 1]
+    for k, sc, bias, ie in zip(mine[:8], FN3_SCORE, FN3_BIAS, FN3_IEVALUE):
+        assert abs(float(bits[k]) - sc) <= 0.051, (regs[k], bits[k], sc)
+        assert abs(float(dombias[k]) / oracle.LOG2 - bias) <= 0.051
+        assert printed_equal(math.exp(lnP[k]), ie), (math.exp(lnP[k]), ie)   # the tutorial searched Z = 1 target
+    assert np.isnan(bits[mine[8]])                                          # multidomain: not scored here
+    t = pli.target_scores()
+    k = int(np.nonzero(t["seq"] == 1)[0][0])
+    assert t["status"][k] == 2 and np.isnan(t["score"][k]) and t["ndom"][k] == 8
+
+
+@pytest.mark.parametrize("name,indel", [("synth_M200", False), ("synth_M400", True)])
+def test_device_hit_scores_equal_the_oracle(gpu, name, indel):
+    path = synth_profile(name)
+    hmm = synth.read_hmm_probs(path)
+    db = synth.make_db(2000, [hmm], frac_homolog=0.05, seed_db=91, seed_plant=92, indel_rich=indel)
+    sdb = gpu.SeqDB(db.residues, db.offsets, db.lengths)
+    om = gpu.OProfile.from_file(path)
+    pli = gpu.Pipeline(domains=True)
+    out = pli.run(om, sdb)
+    regs = pli.regions()
+    bits, dombias, dlnP = pli.domain_scores()
+    ts = pli.target_scores()
+    t = om.tables()
+    tau, lam = float(t["evparam"][4]), float(t["evparam"][5])
+    lm = oracle.LengthModel(t, int(db.lengths.max()) + 1)
+    by_seq = {}
+    for k, r in enumerate(regs):
+        by_seq.setdefault(r[0], []).append(k)
+    assert sorted(by_seq) == list(ts["seq"])
+    nchk = 0
+    for row, q in enumerate(ts["seq"]):
+        s = db.seq(int(q))
+        L = s.size
+        xf = per_target_xf(t, L, lm)
+        st, fsc, fx, _ = oracle.fwd_parser(s, t, xf)
+        st2, _, bx, own = oracle.bck_parser(s, t, fx, xf)
+        _, btot, etot, mocc = oracle.domain_decoding(fx, bx, own, xf)
+        ref_regs = oracle.domain_regions(btot, etot, mocc)
+        got_regs = [(regs[k][1], regs[k][2], regs[k][3]) for k in by_seq[int(q)]]
+        if got_regs != ref_regs:
+            continue                                  # a borderline region test; covered by the decoding tests
+        if any(m for _, _, m in ref_regs):
+            assert ts["status"][row] == 2 and np.isnan(ts["score"][row])
+            continue
+        doms, kk = [], []
+        for k, (i, j, _) in zip(by_seq[int(q)], ref_regs):
+            st3, envsc = oracle.envelope_score(s, t, i, j)
+            st4, _, dc, _ = oracle.null2_envelope(s[i - 1:j], t, oracle.unihit_xf(t, L))
+            if st3 == 0 and st4 == 0:
+                doms.append((envsc, dc, i, j))
+                kk.append(k)
+        seq, pre, sm, lnP, best, dres = oracle.target_scores(L, fsc, oracle.null_one(L), doms, tau, lam)
+        tot = sum(j - i + 1 for _, _, i, j in doms)
+        tol = (1e-4 * max(1.0, abs(fsc)) + 1e-5 * tot + 2e-3) / oracle.LOG2
+        assert ts["status"][row] == 0 and ts["ndom"][row] == len(doms)
+        assert abs(float(ts["score"][row]) - float(seq)) <= tol, (q, ts["score"][row], seq, tol)
+        assert abs(ts["lnP"][row] - lnP) <= lam * tol + 1e-9
+        for k, (b, db_, dl) in zip(kk, dres):
+            assert abs(float(bits[k]) - float(b)) <= tol and abs(float(dombias[k]) - float(db_)) <= 2e-3 + 1e-5 * tot
+        if len(dres) > 1 and abs(float(dres[best][0]) - max(float(x[0]) for i, x in enumerate(dres) if i != best)) > 2 * tol:
+            assert ts["best"][row] == best
+        nchk += 1
+    assert nchk >= 20, nchk
+    print(f"{name}: {nchk} targets compared of {ts['seq'].size}")
