@@ -822,9 +822,10 @@ work_bnd(Work &w, const DevProfile &p, int64_t bytes)
 // per row of a target that only reaches Forward), at most 2^28 floats.
 #define DOM_REG_CAP ((int64_t) 1 << 20)
 // null2 of single-domain regions: regions go in batches whose areas fit a
-// pool of at most N2_POOL_CAP floats (2 GiB); each batch takes three list
-// launches, each with its own count/queue block of 16 ints
-#define N2_POOL_CAP ((int64_t) 1 << 29)
+// pool of at most a quarter of the free device memory, and at least
+// N2_POOL_MIN floats (2 GiB); each batch takes three list launches, each with
+// its own count/queue block of 16 ints
+#define N2_POOL_MIN ((int64_t) 1 << 29)
 #define N2_MAX_BATCH 256
 #define N2_CNT_INTS (N2_MAX_BATCH * 3 * 16)
 #define DOM_RES_CAP ((int64_t) 1 << 26)
@@ -1036,6 +1037,12 @@ struct p7g_pipeline_s {
   mutable std::vector<int4> regs;
   mutable std::vector<float> envsc, n2dc;
   mutable std::vector<int32_t> n2st, regs_slot;
+  // hit-list scores of the last run, computed on first request
+  mutable bool hits_ready = false;
+  mutable std::vector<int64_t> h_tseq;
+  mutable std::vector<P7G_TSCORE> h_ts;
+  mutable std::vector<int32_t> h_tnd, h_tst;
+  mutable std::vector<P7G_DSCORE> h_ds;
   mutable bool regs_ready = false;
   bool   regs_valid = false;
   int64_t nregs = 0;
@@ -1261,6 +1268,14 @@ null2_regions(Work &w, const DevProfile &p, const DevSeqs &s, int64_t nregs, con
   HIPCHK(hipMemcpyAsync(rec.data(), w.dout.regs, sizeof(int4) * (size_t) nregs, hipMemcpyDeviceToHost, w.st));
   HIPCHK(hipMemcpyAsync(vlen.data(), w.dout.v_len, sizeof(int32_t) * (size_t) nregs, hipMemcpyDeviceToHost, w.st));
   HIPCHK(hipStreamSynchronize(w.st));
+  // the pool: what the regions need, bounded by a quarter of the free memory
+  // (the pool already held counts as free)
+  int64_t need_all = 0;
+  for (int64_t k = 0; k < nregs; k++)
+    if (! rec[(size_t) k].w) need_all += n2_area_floats(vlen[(size_t) k], p.fb_B);
+  size_t fr = 0, tot = 0;
+  HIPCHK(hipMemGetInfo(&fr, &tot));
+  const int64_t cap = std::max<int64_t>(N2_POOL_MIN, std::min<int64_t>(need_all, (int64_t) (fr / 4 / sizeof(float)) + w.n2size));
   std::vector<int64_t> off((size_t) nregs, 0);
   std::vector<int32_t> list;
   std::vector<std::pair<size_t, size_t>> batches;
@@ -1272,8 +1287,8 @@ null2_regions(Work &w, const DevProfile &p, const DevSeqs &s, int64_t nregs, con
     const int Ld = vlen[(size_t) k];
     if (Ld != rec[(size_t) k].z - rec[(size_t) k].y + 1) { p7g_set_error("internal: region copy length"); return p7G_EINVAL; }
     const int64_t need = n2_area_floats(Ld, p.fb_B);
-    if (need > N2_POOL_CAP) { p7g_set_error("region of %d residues too long for the null2 pool", Ld); return p7G_EMEM; }
-    if (used + need > N2_POOL_CAP) { batches.push_back({ start, list.size() }); start = list.size(); used = 0; }
+    if (need > cap) { p7g_set_error("region of %d residues too long for the null2 pool", Ld); return p7G_EMEM; }
+    if (used + need > cap) { batches.push_back({ start, list.size() }); start = list.size(); used = 0; }
     off[(size_t) k] = used; used += need; maxused = std::max(maxused, used);
     list.push_back((int32_t) k);
   }
@@ -1403,6 +1418,7 @@ p7_gpu_Pipeline(P7_GPU_PIPELINE *pli, const P7_GPU_OPROFILE *om, const P7_GPU_SE
     pli->fwdsc_dev = w.r.fwdsc; pli->regs_len = db->h_len; pli->nslots = n;
     pli->ftau = om->evparam[p7G_FTAU]; pli->flambda = om->evparam[p7G_FLAMBDA];
     pli->regs_ready = false; pli->regs.clear();
+    pli->hits_ready = false;
     pli->regs_valid = true;
   }
 
@@ -1526,9 +1542,14 @@ p7_gpu_pipeline_GetRegionNull2(const P7_GPU_PIPELINE *pli, float *domcorr, int32
 // regions, the p7_Pipeline scores after domain definition
 // (p7_pipeline.c:743-850, host_hits.c), and per region its domain's.
 static int
-hit_scores(const P7_GPU_PIPELINE *pli, std::vector<int64_t> &tseq, std::vector<P7G_TSCORE> &ts,
-           std::vector<int32_t> &tnd, std::vector<int32_t> &tst, std::vector<P7G_DSCORE> &ds)
+hit_scores(const P7_GPU_PIPELINE *pli)
 {
+  if (pli->hits_ready) return p7G_OK;
+  std::vector<int64_t> &tseq = pli->h_tseq;
+  std::vector<P7G_TSCORE> &ts = pli->h_ts;
+  std::vector<int32_t> &tnd = pli->h_tnd, &tst = pli->h_tst;
+  std::vector<P7G_DSCORE> &ds = pli->h_ds;
+  tseq.clear(); ts.clear(); tnd.clear(); tst.clear();
   const size_t nr = pli->regs.size();
   std::vector<float> fwd((size_t) pli->nslots);
   if (pli->nslots) HIPCHK(hipMemcpy(fwd.data(), pli->fwdsc_dev, sizeof(float) * (size_t) pli->nslots, hipMemcpyDeviceToHost));
@@ -1559,6 +1580,7 @@ hit_scores(const P7_GPU_PIPELINE *pli, std::vector<int64_t> &tseq, std::vector<P
     tseq.push_back(pli->regs[a].x); ts.push_back(t); tnd.push_back((int32_t) dom.size()); tst.push_back(multi ? 2 : 0);
     a = b;
   }
+  pli->hits_ready = true;
   return p7G_OK;
 }
 
@@ -1571,8 +1593,10 @@ p7_gpu_pipeline_GetTargetScores(const P7_GPU_PIPELINE *pli, int64_t *seq, float 
   std::lock_guard<std::mutex> rl(pli->regs_mu);
   int st;
   if ((st = regs_fetch(pli)) != p7G_OK) return st;
-  std::vector<int64_t> tseq; std::vector<P7G_TSCORE> ts; std::vector<int32_t> tnd, tst; std::vector<P7G_DSCORE> ds;
-  if ((st = hit_scores(pli, tseq, ts, tnd, tst, ds)) != p7G_OK) return st;
+  if ((st = hit_scores(pli)) != p7G_OK) return st;
+  const std::vector<int64_t> &tseq = pli->h_tseq;
+  const std::vector<P7G_TSCORE> &ts = pli->h_ts;
+  const std::vector<int32_t> &tnd = pli->h_tnd, &tst = pli->h_tst;
   const int64_t n = (int64_t) tseq.size();
   if (ret_n) *ret_n = n;
   for (int64_t k = 0; k < std::min(n, cap); k++) {
@@ -1597,8 +1621,8 @@ p7_gpu_pipeline_GetDomainScores(const P7_GPU_PIPELINE *pli, float *bitscore, flo
   std::lock_guard<std::mutex> rl(pli->regs_mu);
   int st;
   if ((st = regs_fetch(pli)) != p7G_OK) return st;
-  std::vector<int64_t> tseq; std::vector<P7G_TSCORE> ts; std::vector<int32_t> tnd, tst; std::vector<P7G_DSCORE> ds;
-  if ((st = hit_scores(pli, tseq, ts, tnd, tst, ds)) != p7G_OK) return st;
+  if ((st = hit_scores(pli)) != p7G_OK) return st;
+  const std::vector<P7G_DSCORE> &ds = pli->h_ds;
   const int64_t n = (int64_t) ds.size();
   if (ret_n) *ret_n = n;
   for (int64_t k = 0; k < std::min(n, cap); k++) {
