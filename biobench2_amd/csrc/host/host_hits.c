@@ -7,9 +7,12 @@
  * domain order (the reference sums its per-residue null2 scores, which are
  * zero outside the envelopes, in residue order).
  */
+#include <inttypes.h>
 #include <math.h>
 #include <pthread.h>
 #include <stdint.h>
+#include <stdio.h>
+#include <string.h>
 
 #include "p7g_internal.h"
 
@@ -89,4 +92,41 @@ p7g_target_scores(int L, float fwdsc, float nullsc, float omega, float tau, floa
     ds[d].lnP = p7g_exp_logsurv(bits, tau, lambda);
     if (ds[d].bitscore > ds[ts->best].bitscore) ts->best = d;
   }
+}
+
+/* The "Internal pipeline statistics summary" text of p7_pli_Statistics for a
+ * search of models against sequences (p7_pipeline.c:1710-1796, the branch for
+ * p7_SEARCH_SEQS without long targets, without the stopwatch lines), written
+ * into buf (NUL-terminated, truncated at cap). Returns the full length. */
+size_t
+p7_gpu_StatisticsText(const P7_GPU_STATS *st, double F1, double F2, double F3, double Z, int z_by_option, double domZ,
+                    int domz_by_option, char *buf, size_t cap)
+{
+  char tmp[2048];
+  const double ntargets = (double) st->nseqs;
+  int n = snprintf(tmp, sizeof(tmp),
+    "Internal pipeline statistics summary:\n"
+    "-------------------------------------\n"
+    "Query model(s):                %15" PRId64 "  (%" PRId64 " nodes)\n"
+    "Target sequences:              %15" PRId64 "  (%" PRId64 " residues searched)\n"
+    "Passed MSV filter:           %15" PRId64 "  (%.6g); expected %.1f (%.6g)\n"
+    "Passed bias filter:          %15" PRId64 "  (%.6g); expected %.1f (%.6g)\n"
+    "Passed Vit filter:           %15" PRId64 "  (%.6g); expected %.1f (%.6g)\n"
+    "Passed Fwd filter:           %15" PRId64 "  (%.6g); expected %.1f (%.6g)\n"
+    "Initial search space (Z):    %15.0f  %s\n"
+    "Domain search space  (domZ): %15.0f  %s\n",
+    st->nmodels, st->nnodes, st->nseqs, st->nres,
+    st->n_past_msv, (double) st->n_past_msv / ntargets, F1 * ntargets, F1,
+    st->n_past_bias, (double) st->n_past_bias / ntargets, F1 * ntargets, F1,
+    st->n_past_vit, (double) st->n_past_vit / ntargets, F2 * ntargets, F2,
+    st->n_past_fwd, (double) st->n_past_fwd / ntargets, F3 * ntargets, F3,
+    Z, z_by_option ? "[as set by --Z on cmdline]" : "[actual number of targets]",
+    domZ, domz_by_option ? "[as set by --domZ on cmdline]" : "[number of targets reported over threshold]");
+  if (n < 0) n = 0;
+  if (buf && cap) {
+    const size_t m = ((size_t) n < cap - 1) ? (size_t) n : cap - 1;
+    memcpy(buf, tmp, m);
+    buf[m] = '\0';
+  }
+  return (size_t) n;
 }

@@ -107,6 +107,7 @@ def _load():
                                                 c_int64, POINTER(c_int64)]),
         "p7_gpu_pipeline_GetDomainScores": (i, [vp, POINTER(c_float), POINTER(c_float), POINTER(c_double), c_int64,
                                                 POINTER(c_int64)]),
+        "p7_gpu_StatisticsText": (ctypes.c_size_t, [POINTER(Stats), d, d, d, d, i, d, i, c_char_p, ctypes.c_size_t]),
         "p7_gpu_pipeline_GetRegions": (i, [vp, POINTER(c_int64), POINTER(c_int32), POINTER(c_int32), POINTER(c_int32),
                                            POINTER(c_float), c_int64, POINTER(c_int64)]),
         "p7_gpu_ScoreThreshold": (f, [i, d, d, d]),
@@ -539,6 +540,16 @@ class Server:
         self.close()
 
 
+def statistics_text(st, F, Z, domZ, z_by_option=False, domz_by_option=False):
+    """p7_gpu_StatisticsText for a Stats structure."""
+    n = lib.p7_gpu_StatisticsText(ctypes.byref(st), F[0], F[1], F[2], float(Z), int(z_by_option), float(domZ),
+                                  int(domz_by_option), None, 0)
+    buf = ctypes.create_string_buffer(n + 1)
+    lib.p7_gpu_StatisticsText(ctypes.byref(st), F[0], F[1], F[2], float(Z), int(z_by_option), float(domZ),
+                              int(domz_by_option), buf, n + 1)
+    return buf.value.decode()
+
+
 class Pipeline:
     """P7_PIPELINE thresholds and counters; run() is p7_Pipeline over a whole database."""
 
@@ -628,6 +639,13 @@ class Pipeline:
         st = Stats()
         _check(lib.p7_gpu_pipeline_GetStats(self.h, ctypes.byref(st)), "GetStats")
         return st.as_dict()
+
+    def statistics_text(self, F=(0.02, 1e-3, 1e-5), Z=None, domZ=0.0, z_by_option=False, domz_by_option=False):
+        """p7_pli_Statistics' summary text for this pipeline's counters (Z defaults to the
+        number of targets searched)."""
+        st = Stats()
+        _check(lib.p7_gpu_pipeline_GetStats(self.h, ctypes.byref(st)), "GetStats")
+        return statistics_text(st, F, st.nseqs if Z is None else Z, domZ, z_by_option, domz_by_option)
 
     def __del__(self):
         if getattr(self, "h", None):

@@ -308,6 +308,16 @@ int   p7_gpu_pipeline_GetTargetScores(const P7_GPU_PIPELINE *pli, int64_t *seq, 
  * rescoring failed. */
 int   p7_gpu_pipeline_GetDomainScores(const P7_GPU_PIPELINE *pli, float *bitscore, float *dombias, double *lnP,
                                       int64_t cap, int64_t *ret_n);
+/* The "Internal pipeline statistics summary" text of p7_pli_Statistics
+ * (p7_pipeline.c:1710-1796) for a search of models against sequences, from
+ * the counters of p7_gpu_pipeline_GetStats: the query and target counts, the
+ * four filter lines with their expected numbers for F1/F2/F3, and the search
+ * spaces Z and domZ with their notes (z_by_option / domz_by_option: set on
+ * the command line, else the number of targets / of targets reported). The
+ * stopwatch lines are the caller's. Writes at most cap bytes including the
+ * terminating NUL; returns the text's full length. */
+size_t p7_gpu_StatisticsText(const P7_GPU_STATS *st, double F1, double F2, double F3, double Z, int z_by_option,
+                             double domZ, int domz_by_option, char *buf, size_t cap);
 /* ---- batches: many profiles against one resident database ----------------
  * Runs the p7_gpu_Pipeline filter chain of nprof profiles over every database
  * sequence in one call. The ungapped MSV pass of all profiles that share a

@@ -98,3 +98,16 @@ def test_device_hit_scores_equal_the_oracle(gpu, name, indel):
         nchk += 1
     assert nchk >= 20, nchk
     print(f"{name}: {nchk} targets compared of {ts['seq'].size}")
+
+
+def test_fn3_tutorial_statistics_text_on_the_device(gpu):
+    """tutorial/fn3.out's pipeline statistics block, from the device pipeline's
+    counters for fn3 against 7LESS_DROME (Z = 1 target, domZ = 1 reported)."""
+    from test_hits_oracle import tutorial_stats_block
+    (name, s), = seqio.read_uniprot(golden("7LESS_DROME"))
+    db = gpu.SeqDB.from_list([s])
+    om = gpu.OProfile.from_file(golden("fn3.hmm"))
+    pli = gpu.Pipeline()
+    out = pli.run(om, db)
+    assert out["stage"][0] == 4
+    assert pli.statistics_text(domZ=1) == tutorial_stats_block("fn3.out")

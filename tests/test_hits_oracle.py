@@ -125,3 +125,28 @@ def test_library_null_score_and_flogsum_equal_the_oracle():
     rng = np.random.default_rng(9)
     for a, b in rng.normal(0, 6, (400, 2)).astype(np.float32):
         assert np.float32(lib.p7g_flogsum(float(a), float(b))) == oracle.flogsum(a, b)
+
+
+def tutorial_stats_block(name):
+    """The statistics block of a tutorial output, without its stopwatch lines."""
+    text = open(golden(name)).read()
+    i = text.index("Internal pipeline statistics summary:")
+    out = []
+    for ln in text[i:].splitlines():
+        if ln.startswith("# CPU time") or ln.startswith("# Mc/sec") or ln.startswith("//"):
+            break
+        out.append(ln)
+    return "\n".join(out) + "\n"
+
+
+def test_statistics_text_equals_the_tutorial_globins4_search():
+    """tutorial/globins4.out: globins4 (M = 149) against 539,165 sequences; its
+    counters give the reference's summary text byte for byte (the fixture is
+    that file's statistics block, up to its stopwatch line)."""
+    st = p7gpu.Stats()
+    st.nmodels, st.nnodes, st.nseqs, st.nres = 1, 149, 539165, 191456931
+    st.n_past_msv, st.n_past_bias, st.n_past_vit, st.n_past_fwd = 20801, 17061, 2321, 1109
+    got = p7gpu.statistics_text(st, (0.02, 1e-3, 1e-5), 539165, 1108)
+    assert got == tutorial_stats_block("globins4_stats.txt")
+    got = p7gpu.statistics_text(st, (0.02, 1e-3, 1e-5), 1e6, 3, z_by_option=True, domz_by_option=True)
+    assert "1000000  [as set by --Z on cmdline]" in got and "3  [as set by --domZ on cmdline]" in got
