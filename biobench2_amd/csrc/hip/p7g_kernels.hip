@@ -586,6 +586,196 @@ void k_vit(const DevProfile p, const uint8_t *__restrict__ res, const int64_t *_
   }
 }
 
+// Viterbi filter for large models, W waves per target. Each lane's ND dwords
+// (positions d and d+ND of the lane as int16 pairs) are split into W slices
+// of ND/W; wave w owns slice w of every lane, with its transitions pinned in
+// registers. A slice's first dword takes its predecessor from the previous
+// slice's last dword (both halves at once), and slice 0 from slice W-1 of the
+// lane before (low half) and of its own lane (high half), through LDS. The
+// D->D closure composes per-slice maps x -> max(C, x + A) (int32 max-plus,
+// floored at -32768 as the saturating chain is) into the lane's map, scans
+// the lanes as the one-wave kernel does, then closes each slice from its
+// carries. The row's xE and the lazy-F test's D maximum are workgroup maxima.
+// Three barriers per row; results equal the one-wave kernel's bit for bit.
+template <int ND, int W>
+__global__ __launch_bounds__(64 * W)
+void k_vit_wg(const DevProfile p, const uint8_t *__restrict__ res, const int64_t *__restrict__ off,
+              const int32_t *__restrict__ len, const int32_t *__restrict__ list, const int32_t *__restrict__ count,
+              DevResults r, DevLists l, GateThresh g, int gating, LenCfg lc)
+{
+#pragma clang fp contract(off)
+  static_assert(ND % W == 0, "slices of equal width");
+  constexpr int NW = ND / W;
+  const int NEG = -(1 << 28);
+  __shared__ int sh_pm[W][64], sh_pi[W][64], sh_pd[W][64];   // previous row: each slice's last dword
+  __shared__ int sh_m[W][64];                                // this row's M at each slice's last dword
+  __shared__ int sh_plo[W][64], sh_phi[W][64];               // this row's slice chain ends (int32, per half)
+  __shared__ int sh_e[W], sh_dm[W];                          // the waves' xE and D maxima
+  __shared__ int64_t sh_t;
+  if ((int64_t) blockIdx.x >= *count) return;
+  const int wv = threadIdx.x >> 6, z = threadIdx.x & 63;
+  const int d0 = wv * NW;
+  const size_t rowd = (size_t) 64 * ND;
+  const int *tbl = p.vf_tbl;
+  const int *tr = tbl + (size_t) p.nrows * rowd + z;
+  int tBM[NW], tMM[NW], tIM[NW], tDM[NW], tMDS[NW], tMI[NW], tII[NW], tDD[NW];
+#pragma unroll
+  for (int j = 0; j < NW; j++) {
+    const size_t c = (size_t) (d0 + j) * 64;
+    tBM[j] = tr[VT_BM*rowd + c]; tMM[j] = tr[VT_MM*rowd + c]; tIM[j] = tr[VT_IM*rowd + c]; tDM[j] = tr[VT_DM*rowd + c];
+    tMDS[j] = tr[VT_MDS*rowd + c]; tMI[j] = tr[VT_MI*rowd + c]; tII[j] = tr[VT_II*rowd + c]; tDD[j] = tr[VT_DDS*rowd + c];
+  }
+  // int32 sums of the D->D steps of every slice of this lane, per half, and
+  // the lane-scan constants of the whole lane (as the one-wave kernel's)
+  int alo_v[W], ahi_v[W];
+#pragma unroll
+  for (int v = 0; v < W; v++) {
+    int a = 0, b = 0;
+    for (int j = 0; j < NW; j++) { const int t = tr[VT_DDS*rowd + (size_t) (v * NW + j) * 64]; a += lo16(t); b += hi16(t); }
+    alo_v[v] = a; ahi_v[v] = b;
+  }
+  DDConst<ND> ddc;
+  dd_setup<ND>(ddc, tr + VT_DDS*rowd, z);
+  const bool loops0 = (p.xwN_loop == 0 && p.xwC_loop == 0 && p.xwJ_loop == 0);
+  const int64_t n = *count;
+  int32_t *queue = (int32_t *) count + P7G_QOFF;
+
+  for (;;) {
+    if (threadIdx.x == 0) sh_t = (int64_t) atomicAdd(queue, 1);
+    __syncthreads();
+    const int64_t t = sh_t;
+    if (t >= n) break;
+    const int slot = list[t];
+    const int L = len[slot];
+    const uint8_t *sq = res + off[slot];
+    const int xmove = lc.on ? lc.xmove : (int) p.xw_move[L];
+    int xN = p.base_w;
+    int xB = (int16_t) (xN + xmove);
+    int xJ = -32768, xC = -32768;
+    int M[NW], I[NW], D[NW];
+#pragma unroll
+    for (int j = 0; j < NW; j++) { M[j] = I[j] = D[j] = (int) NEG_PAIR; }
+    sh_pm[wv][z] = (int) NEG_PAIR; sh_pi[wv][z] = (int) NEG_PAIR; sh_pd[wv][z] = (int) NEG_PAIR;
+    int status = 0;
+    ResAsc rd;
+    if (L > 0) rd.init(sq);
+    for (int i = 0; i < L; i++) {
+      const int x = rd.at(i);
+      const int *e = tbl + (size_t) x * rowd + z;
+      __syncthreads();                                   // the previous row's slice ends are written
+      int mS, iS, dS;                                    // the previous row's predecessors of the slice's first dword
+      if (wv > 0) { mS = sh_pm[wv-1][z]; iS = sh_pi[wv-1][z]; dS = sh_pd[wv-1][z]; }
+      else {
+        const int pm = (z > 0) ? sh_pm[W-1][z-1] : (int) NEG_PAIR, pi = (z > 0) ? sh_pi[W-1][z-1] : (int) NEG_PAIR;
+        const int pd = (z > 0) ? sh_pd[W-1][z-1] : (int) NEG_PAIR;
+        mS = (int) __builtin_amdgcn_alignbit((unsigned) sh_pm[W-1][z], (unsigned) pm, 16);
+        iS = (int) __builtin_amdgcn_alignbit((unsigned) sh_pi[W-1][z], (unsigned) pi, 16);
+        dS = (int) __builtin_amdgcn_alignbit((unsigned) sh_pd[W-1][z], (unsigned) pd, 16);
+      }
+      const int xB2 = pair16(xB);
+      int xE = (int) NEG_PAIR;
+#pragma unroll
+      for (int j = NW-1; j >= 0; j--) {
+        const int mp = j ? M[j-1] : mS, ip = j ? I[j-1] : iS, dp = j ? D[j-1] : dS;
+        int sv = adds2(xB2, tBM[j]);
+        sv = max2(sv, adds2(mp, tMM[j]));
+        sv = max2(sv, adds2(ip, tIM[j]));
+        sv = max2(sv, adds2(dp, tDM[j]));
+        sv = adds2(sv, e[(size_t) (d0 + j) * 64]);
+        xE = max2(xE, sv);
+        I[j] = max2(adds2(M[j], tMI[j]), adds2(I[j], tII[j]));
+        M[j] = sv;
+      }
+      sh_m[wv][z] = M[NW-1];
+      __syncthreads();                                   // this row's slice-end M values are written
+      int mN;
+      if (wv > 0) mN = sh_m[wv-1][z];
+      else {
+        const int pm = (z > 0) ? sh_m[W-1][z-1] : (int) NEG_PAIR;
+        mN = (int) __builtin_amdgcn_alignbit((unsigned) sh_m[W-1][z], (unsigned) pm, 16);
+      }
+      int Dm = (int) NEG_PAIR;
+#pragma unroll
+      for (int j = 0; j < NW; j++) { D[j] = adds2(j ? M[j-1] : mN, tMDS[j]); Dm = max2(Dm, D[j]); }
+      // the slice's chains without carries in
+      int P[NW];
+      P[0] = D[0];
+#pragma unroll
+      for (int j = 1; j < NW; j++) P[j] = max2(D[j], adds2(P[j-1], tDD[j]));
+      sh_plo[wv][z] = lo16(P[NW-1]); sh_phi[wv][z] = hi16(P[NW-1]);
+      {
+        const int xel = wave_max_i(max(lo16(xE), hi16(xE)));
+        const int dml = wave_max_i(max(lo16(Dm), hi16(Dm)));
+        if (z == 0) { sh_e[wv] = xel; sh_dm[wv] = dml; }
+      }
+      __syncthreads();                                   // slice chains and the waves' maxima are written
+      int xEs = sh_e[0], dmax = sh_dm[0];
+#pragma unroll
+      for (int v = 1; v < W; v++) { xEs = max(xEs, sh_e[v]); dmax = max(dmax, sh_dm[v]); }
+      const int th = loops0 ? min(min(xC - p.xwE_move, xJ - p.xwE_loop), 32766) : -(1 << 30);
+      if (xEs > th) {
+        if (xEs >= 32767) { status = 16; break; }
+        xN = (int16_t) (xN + p.xwN_loop);
+        xC = (int16_t) max(xC + p.xwC_loop, xEs + p.xwE_move);
+        xJ = (int16_t) max(xJ + p.xwJ_loop, xEs + p.xwE_loop);
+        xB = (int16_t) max(xJ + xmove, xN + xmove);
+      }
+      if (dmax + p.ddbound_w > xB) {
+        // the lane's map: the low halves of the W slices in order, then the high halves
+        int Clo = NEG, Alo = 0, Chi = NEG, Ahi = 0;
+#pragma unroll
+        for (int v = 0; v < W; v++) {
+          Clo = max(sh_plo[v][z], Clo + alo_v[v]); Alo += alo_v[v];
+          Chi = max(sh_phi[v][z], Chi + ahi_v[v]); Ahi += ahi_v[v];
+        }
+        int C = max(Chi, Clo + Ahi);
+        C = max(C, dpp0<ROW_SHR1>(C) + ddc.k1);
+        C = max(C, dpp0<ROW_SHR2>(C) + ddc.k2);
+        C = max(C, dpp0<ROW_SHR4>(C) + ddc.k4);
+        C = max(C, dpp0<ROW_SHR8>(C) + ddc.k8);
+        {
+          const int c1 = __builtin_amdgcn_readlane(C, 15);
+          const int c2 = max(c1 + ddc.arow31, __builtin_amdgcn_readlane(C, 31));
+          const int c3 = max(c2 + ddc.arow47, __builtin_amdgcn_readlane(C, 47));
+          const int row = z >> 4;
+          const int rc = (row == 0) ? NEG : (row == 1) ? c1 : (row == 2) ? c2 : c3;
+          C = max(C, rc + ddc.arow);
+        }
+        const int cin = dppi<WAVE_SHR1>(NEG, C);        // the end of the lane before
+        // carries into this slice's low half (through the slices before it) and
+        // high half (the lane's whole low half, then the slices before it)
+        int clo = cin, chi = max(Clo, cin + Alo);
+        for (int v = 0; v < wv; v++) {
+          clo = max(sh_plo[v][z], clo + alo_v[v]);
+          chi = max(sh_phi[v][z], chi + ahi_v[v]);
+        }
+        clo = max(clo, -32768); chi = max(chi, -32768);
+        int Q = (int) (((unsigned) clo & 0xffffu) | ((unsigned) chi << 16));
+#pragma unroll
+        for (int j = 0; j < NW; j++) { Q = adds2(Q, tDD[j]); D[j] = max2(P[j], Q); }
+      }
+      sh_pm[wv][z] = M[NW-1]; sh_pi[wv][z] = I[NW-1]; sh_pd[wv][z] = D[NW-1];
+    }
+    if (threadIdx.x == 0) {
+      float sc;
+      if (status == 16) sc = INFINITY;
+      else if (xC > -32768) {
+        sc = (float) xC + (float) xmove - (float) p.base_w;
+        sc = __fdiv_rn(sc, p.scale_w);
+        sc = (float) ((double) sc - 3.0);
+      } else sc = -INFINITY;
+      r.vfsc[slot] = sc;
+      r.vstatus[slot] = status;
+      if (gating) {
+        const float seq = (float) __ddiv_rn((double) (sc - r.filtersc[slot]), LOG2_D);
+        if (seq >= g.T2v) r.stage[slot] = 3;
+        else r.stage[slot] = 2;
+      }
+    }
+    __syncthreads();                                     // sh_t and the slice ends are free for the next target
+  }
+}
+
 /*****************************************************************
  * 7. Forward parser (one target per wave, fp32, B positions per lane)
  *****************************************************************/
@@ -1783,12 +1973,36 @@ hipError_t launch_bias(const DevProfile &p, const DevSeqs &s, const int32_t *lis
 
 // Viterbi: the whole table in LDS while it fits (ND <= 16 at 160 KB), else the
 // transitions in LDS and emission rows from global memory.
+// The cooperative Viterbi kernel is the default for ND >= 20 (at ND = 16 it
+// measured no faster than the one-wave kernel with its whole table in LDS);
+// P7G_VIT_WG=0 selects the one-wave kernel (for comparisons).
+static bool vit_wg_on()
+{
+  static const bool on = [] { const char *v = getenv("P7G_VIT_WG"); return ! (v && v[0] == '0'); }();
+  return on;
+}
 #define VIT_CASE(N, T) case N: k = (void *) k_vit<N, T>; lds = (T == TM_LDS) ? p.vf_dwords * 4 : (size_t) P7G_VF_NT * 64 * N * 4; \
   nthr = VIT_THREADS(N); break;
 hipError_t launch_vit(const DevProfile &p, const DevSeqs &s, const int32_t *list, const int32_t *count, int64_t maxn,
                       DevResults r, DevLists l, GateThresh g, int gating, LenCfg lc, hipStream_t st)
 {
   if (maxn <= 0) return hipSuccess;
+  // large models run the filter with several waves per target
+  void *kw = nullptr;
+  int ww = 0;
+  switch (p.vf_ND) {
+    case 20: kw = (void *) k_vit_wg<20, 4>; ww = 4; break;
+    case 24: kw = (void *) k_vit_wg<24, 4>; ww = 4; break;
+    case 32: kw = (void *) k_vit_wg<32, 8>; ww = 8; break;
+    case 40: kw = (void *) k_vit_wg<40, 8>; ww = 8; break;
+    default: break;
+  }
+  if (kw && vit_wg_on()) {
+    const unsigned grid = grid_for_waves_blk(kw, maxn * ww, 0, ww);
+    void *args[] = { (void *) &p, (void *) &s.res, (void *) &s.off, (void *) &s.len, (void *) &list, (void *) &count,
+                     &r, &l, &g, &gating, &lc };
+    return launch(kw, grid, 64 * ww, args, 0, st);
+  }
   void *k = nullptr;
   size_t lds = 0;
   int nthr = LIST_BLOCK;

@@ -22,6 +22,9 @@ from conftest import random_seqs
 pytestmark = pytest.mark.gpu
 
 LARGE = [2100, 2600, 5000]
+# every shape of the cooperative Viterbi kernel (ND = 20, 24, 32 at M = 3500, 40) and the one-wave
+# kernel with its whole table in LDS at its largest (ND = 16, M = 2000)
+VIT_LARGE = [2000, 2100, 2600, 3500, 5000]
 STATS = (-9.0, 0.6931, -9.5, 0.6931, -5.0, 0.6931)
 
 
@@ -30,7 +33,7 @@ def large(gpu, tmp_path_factory):
     d = tmp_path_factory.mktemp("large")
     rng = np.random.default_rng(77)
     out = {}
-    for M in LARGE:
+    for M in VIT_LARGE:
         h = synth.random_profile(M, rng, f"large_M{M}")
         h.stats = STATS
         path = str(d / f"large_M{M}.hmm")
@@ -57,7 +60,7 @@ def same(a, b):
     return a == b or (np.isinf(a) and np.isinf(b) and np.sign(a) == np.sign(b))
 
 
-@pytest.mark.parametrize("M", LARGE)
+@pytest.mark.parametrize("M", VIT_LARGE)
 def test_large_msv_and_viterbi_bit_exact(large, M):
     path, hh, seqs, db = large[M]
     om = p7gpu.OProfile.from_file(path)
