@@ -116,6 +116,96 @@ void k_msv_full(const DevProfile p, const int *__restrict__ tbl, int tbl_dwords,
   }
 }
 
+// The full MSV recurrence for large models, W waves per target: wave w owns
+// dwords [w·ND/W, (w+1)·ND/W) of every lane. A slice's first dword takes the
+// previous row's last dword of the slice before it (slice 0: of the last
+// slice, joined across lanes as shift_in does) from LDS, double-buffered by
+// row parity, so one barrier per row orders the exchange. The waves' row
+// maxima travel in the same buffers, and xJ and xB are updated from them at
+// the top of the next row (or after the last), in the same order as the
+// one-wave kernel; results are bit-identical.
+template <int ND, int W>
+__global__ __launch_bounds__(64 * W)
+void k_msv_full_wg(const DevProfile p, const int *__restrict__ tbl, const uint8_t *__restrict__ res,
+                   const int64_t *__restrict__ off, const int32_t *__restrict__ len, const int32_t *__restrict__ list,
+                   const int32_t *__restrict__ count, int16_t *__restrict__ xj, LenCfg lc)
+{
+  static_assert(ND % W == 0, "slices of equal width");
+  constexpr int NW = ND / W;
+  __shared__ int sh_end[2][W][64];       // each slice's last dword, by row parity
+  __shared__ int sh_el[2][W];            // each wave's row maximum, by row parity
+  __shared__ int64_t sh_t;
+  if ((int64_t) blockIdx.x >= *count) return;
+  const int wv = threadIdx.x >> 6, z = threadIdx.x & 63;
+  const int d0 = wv * NW;
+  const int64_t n = *count;
+  int32_t *queue = (int32_t *) count + P7G_QOFF;
+
+  for (;;) {
+    if (threadIdx.x == 0) sh_t = (int64_t) atomicAdd(queue, 1);
+    __syncthreads();
+    const int64_t t = sh_t;
+    if (t >= n) break;
+    const int slot = list[t];
+    const int L = len[slot];
+    const uint8_t *sq = res + off[slot];
+    const int tjbm = ((lc.on ? lc.tjb : (int) p.tjb[L]) + p.tbm) & 0xff;
+    int xJ = 0;
+    int xB = max(p.base - tjbm, 0);
+    int mp[NW];
+#pragma unroll
+    for (int j = 0; j < NW; j++) mp[j] = 0;
+    sh_end[1][wv][z] = 0;                              // row -1
+    bool overflow = false;
+    // the update of xJ and xB from the row maximum the waves left in buffer b
+    auto update = [&](int b) {
+      int xE = sh_el[b][0];
+#pragma unroll
+      for (int v = 1; v < W; v++) xE = max(xE, sh_el[b][v]);
+      if (xE > min(xJ + p.tec, 254 - p.bias)) {
+        if (xE + p.bias >= 255) { overflow = true; return; }
+        xJ = max(xJ, max(xE - p.tec, 0));
+        xB = max(max(p.base, xJ) - tjbm, 0);
+      }
+    };
+    ResAsc rd;
+    if (L > 0) rd.init(sq);
+    for (int i = 0; i < L; i++) {
+      const int x = rd.at(i);
+      const int *row = tbl + (size_t) x * (ND * 64) + z;
+      const int pb = (i + 1) & 1;                      // the previous row's buffer
+      __syncthreads();                                 // the previous row's slice ends and maxima are written
+      if (i > 0) { update(pb); if (overflow) break; }
+      int s0;
+      if (wv > 0) s0 = sh_end[pb][wv-1][z];
+      else {
+        const int nb = (z > 0) ? sh_end[pb][W-1][z-1] : 0;
+        s0 = (int) __builtin_amdgcn_alignbit((unsigned) sh_end[pb][W-1][z], (unsigned) nb, 16);
+      }
+      const int xB2 = pair16(xB);
+      int E = 0;
+#pragma unroll
+      for (int j = NW-1; j >= 0; j--) {
+        int src = j ? mp[j-1] : s0;
+        int tv = max2(src, xB2);
+        tv = adds2(tv, row[(size_t) (d0 + j) * 64]);
+        tv = max2(tv, 0);
+        E  = max2(E, tv);
+        mp[j] = tv;
+      }
+      sh_end[i & 1][wv][z] = mp[NW-1];
+      const int el = wave_max_i(max(lo16(E), hi16(E)));
+      if (z == 0) sh_el[i & 1][wv] = el;
+    }
+    if (L > 0 && ! overflow) {
+      __syncthreads();
+      update((L - 1) & 1);
+    }
+    if (threadIdx.x == 0) xj[slot] = (int16_t) (overflow ? -1 : xJ);
+    __syncthreads();                                   // sh_t and the buffers are free for the next target
+  }
+}
+
 /*****************************************************************
  * 4. MSV gate: score, null score, F1 (and F2 when the bias filter is off)
  *****************************************************************/
@@ -1926,6 +2016,22 @@ hipError_t launch_msv_full(const DevProfile &p, const DevSeqs &s, const int32_t 
                            int64_t maxn, int16_t *xj, LenCfg lc, hipStream_t st)
 {
   if (maxn <= 0) return hipSuccess;
+  // large models run the recurrence with several waves per target
+  void *kw = nullptr;
+  int ww = 0;
+  switch (p.mf_ND) {
+    case 20: kw = (void *) k_msv_full_wg<20, 4>; ww = 4; break;
+    case 24: kw = (void *) k_msv_full_wg<24, 4>; ww = 4; break;
+    case 32: kw = (void *) k_msv_full_wg<32, 8>; ww = 8; break;
+    case 40: kw = (void *) k_msv_full_wg<40, 8>; ww = 8; break;
+    default: break;
+  }
+  if (kw) {
+    const unsigned grid = grid_for_waves_blk(kw, maxn * ww, 0, ww);
+    void *args[] = { (void *) &p, (void *) &p.mf_tbl, (void *) &s.res, (void *) &s.off, (void *) &s.len,
+                     (void *) &list, (void *) &count, &xj, &lc };
+    return launch(kw, grid, 64 * ww, args, 0, st);
+  }
   void *k = nullptr;
   switch (p.mf_ND) {
     MF_CASE(1, true) MF_CASE(2, true) MF_CASE(3, true) MF_CASE(4, true) MF_CASE(5, true) MF_CASE(6, true)
