@@ -1633,6 +1633,62 @@ p7_gpu_pipeline_GetDomainScores(const P7_GPU_PIPELINE *pli, float *bitscore, flo
   return p7G_OK;
 }
 
+// The hit list of the scored targets (status 0): p7_tophits_SortBySortkey
+// with the sort key of E-value inclusion (-lnP, p7_pipeline.c:827; ties by
+// sequence index), then p7_tophits_Threshold (p7_tophits.c:866-935) with
+// E-value thresholds: a target is reported if exp(lnP)·Z <= E and included
+// if also <= incE; domZ = the number of reported targets; a domain of a
+// reported target is reported if exp(lnP_d)·domZ <= domE, and included if
+// its target is and exp(lnP_d)·domZ <= incdomE.
+extern "C" int
+p7_gpu_pipeline_GetHitList(const P7_GPU_PIPELINE *pli, double Z, double E, double incE, double domE, double incdomE,
+                           int64_t *order, int32_t *flags, int32_t *dflags, double *ret_domZ, int64_t cap, int64_t *ret_n)
+{
+  if (! pli || ! pli->regs_valid) { p7g_set_error("no domain regions: the last p7_gpu_Pipeline call did not decode"); return p7G_EINVAL; }
+  if (! pli->n2dc_dev) { p7g_set_error("the last p7_gpu_Pipeline call ran without null2"); return p7G_EINVAL; }
+  std::lock_guard<std::mutex> rl(pli->regs_mu);
+  int st;
+  if ((st = regs_fetch(pli)) != p7G_OK) return st;
+  if ((st = hit_scores(pli)) != p7G_OK) return st;
+  const size_t nt = pli->h_tseq.size(), nr = pli->regs.size();
+  std::vector<int64_t> ord;
+  for (size_t k = 0; k < nt; k++) if (pli->h_tst[k] == 0) ord.push_back((int64_t) k);
+  std::stable_sort(ord.begin(), ord.end(), [&](int64_t a, int64_t b) {
+    const double ka = -pli->h_ts[(size_t) a].lnP, kb = -pli->h_ts[(size_t) b].lnP;
+    return ka > kb; });
+  std::vector<int32_t> tf(nt, 0);
+  int64_t nrep = 0;
+  for (int64_t k : ord) {
+    const double lnP = pli->h_ts[(size_t) k].lnP;
+    if (exp(lnP) * Z <= E) {
+      tf[(size_t) k] |= 1; nrep++;
+      if (exp(lnP) * Z <= incE) tf[(size_t) k] |= 2;
+    }
+  }
+  const double domZ = (double) nrep;
+  if (dflags) {
+    for (size_t q = 0; q < nr; q++) dflags[q] = 0;
+    size_t a = 0;
+    for (size_t k = 0; k < nt; k++) {                 // targets and regions are both in sequence order
+      while (a < nr && (int64_t) pli->regs[a].x != pli->h_tseq[k]) a++;
+      for (size_t q = a; q < nr && (int64_t) pli->regs[q].x == pli->h_tseq[k]; q++) {
+        if (! (tf[k] & 1) || std::isnan(pli->h_ds[q].bitscore)) continue;
+        const double e = exp(pli->h_ds[q].lnP) * domZ;
+        if (e <= domE) dflags[q] |= 1;
+        if ((tf[k] & 2) && e <= incdomE) dflags[q] |= 2;
+      }
+    }
+  }
+  if (ret_domZ) *ret_domZ = domZ;
+  const int64_t n = (int64_t) ord.size();
+  if (ret_n) *ret_n = n;
+  for (int64_t k = 0; k < std::min(n, cap); k++) {
+    if (order) order[k] = ord[(size_t) k];
+    if (flags) flags[k] = tf[(size_t) ord[(size_t) k]];
+  }
+  return p7G_OK;
+}
+
 /*****************************************************************
  * Batches: many profiles against one database in one call
  *****************************************************************/

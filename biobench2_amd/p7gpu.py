@@ -107,6 +107,8 @@ def _load():
                                                 c_int64, POINTER(c_int64)]),
         "p7_gpu_pipeline_GetDomainScores": (i, [vp, POINTER(c_float), POINTER(c_float), POINTER(c_double), c_int64,
                                                 POINTER(c_int64)]),
+        "p7_gpu_pipeline_GetHitList": (i, [vp, d, d, d, d, d, POINTER(c_int64), POINTER(c_int32), POINTER(c_int32),
+                                           POINTER(c_double), c_int64, POINTER(c_int64)]),
         "p7_gpu_StatisticsText": (ctypes.c_size_t, [POINTER(Stats), d, d, d, d, i, d, i, c_char_p, ctypes.c_size_t]),
         "p7_gpu_pipeline_GetRegions": (i, [vp, POINTER(c_int64), POINTER(c_int32), POINTER(c_int32), POINTER(c_int32),
                                            POINTER(c_float), c_int64, POINTER(c_int64)]),
@@ -605,6 +607,20 @@ class Pipeline:
             _ptr(o["sum_score"], c_float), _ptr(o["lnP"], c_double), _ptr(o["ndom"], c_int32),
             _ptr(o["best"], c_int32), _ptr(o["status"], c_int32), n.value, ctypes.byref(n)), "GetTargetScores")
         return o
+
+    def hit_list(self, Z, E=10.0, incE=0.01, domE=10.0, incdomE=0.01):
+        """(order, flags, dflags, domZ): the scored targets ranked (indices into target_scores()
+        arrays), their reported (1) / included (2) flags, per-region flags, and domZ."""
+        n, dz = c_int64(), c_double()
+        _check(lib.p7_gpu_pipeline_GetHitList(self.h, Z, E, incE, domE, incdomE, None, None, None, None, 0,
+                                              ctypes.byref(n)), "GetHitList")
+        nr = len(self.regions())
+        order, flags = np.zeros(n.value, np.int64), np.zeros(n.value, np.int32)
+        dflags = np.zeros(max(nr, 1), np.int32)
+        _check(lib.p7_gpu_pipeline_GetHitList(self.h, Z, E, incE, domE, incdomE, _ptr(order, c_int64),
+                                              _ptr(flags, c_int32), _ptr(dflags, c_int32), ctypes.byref(dz), n.value,
+                                              ctypes.byref(n)), "GetHitList")
+        return order, flags, dflags[:nr], dz.value
 
     def domain_scores(self):
         """(bitscore, dombias nats, lnP) arrays per region, in regions() order."""

@@ -308,6 +308,20 @@ int   p7_gpu_pipeline_GetTargetScores(const P7_GPU_PIPELINE *pli, int64_t *seq, 
  * rescoring failed. */
 int   p7_gpu_pipeline_GetDomainScores(const P7_GPU_PIPELINE *pli, float *bitscore, float *dombias, double *lnP,
                                       int64_t cap, int64_t *ret_n);
+/* The hit list of the scored targets (GetTargetScores status 0), ranked as
+ * p7_tophits_SortBySortkey ranks them under E-value inclusion (by -lnP, ties
+ * by sequence index) and flagged as p7_tophits_Threshold flags them with
+ * E-value thresholds (p7_tophits.c:866-935, p7_pipeline.c:407-460): order[k]
+ * is an index into GetTargetScores' arrays, flags[k] bit 0 reported
+ * (exp(lnP)·Z <= E), bit 1 included (also <= incE). domZ = the number of
+ * reported targets. dflags (NULL, or one entry per region in GetRegions
+ * order): bit 0 reported (its target reported, exp(lnP)·domZ <= domE), bit 1
+ * included (its target included, <= incdomE). Targets with a multidomain
+ * region are not in the list and do not count in domZ. hmmsearch's defaults:
+ * E = 10, incE = 0.01, domE = 10, incdomE = 0.01. */
+int   p7_gpu_pipeline_GetHitList(const P7_GPU_PIPELINE *pli, double Z, double E, double incE, double domE,
+                                 double incdomE, int64_t *order, int32_t *flags, int32_t *dflags, double *ret_domZ,
+                                 int64_t cap, int64_t *ret_n);
 /* The "Internal pipeline statistics summary" text of p7_pli_Statistics
  * (p7_pipeline.c:1710-1796) for a search of models against sequences, from
  * the counters of p7_gpu_pipeline_GetStats: the query and target counts, the

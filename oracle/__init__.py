@@ -502,3 +502,33 @@ def target_scores(L, fwdsc, nullsc, doms, tau, lam):
         if bits > dres[best][0]:
             best = d
     return seq_score, pre_score, sum_score, exp_logsurv(float(seq_score), tau, lam), best, dres
+
+
+def hit_threshold(targets, Z, E=10.0, incE=0.01, domE=10.0, incdomE=0.01):
+    """p7_tophits_SortBySortkey with E-value inclusion (sort key -lnP, p7_pipeline.c:827; ties by
+    position in `targets`) and p7_tophits_Threshold with E-value thresholds (p7_tophits.c:866-935,
+    p7_pipeline.c:407-460). targets: [(lnP, [domain lnP, ...])]. Returns (order, flags, dflags, domZ):
+    flags bit 0 reported, bit 1 included; dflags[t][d] the same for the domains."""
+    order = sorted(range(len(targets)), key=lambda k: (-(-targets[k][0]), k))
+    flags = [0] * len(targets)
+    for k in order:
+        e = math.exp(targets[k][0]) * Z
+        if e <= E:
+            flags[k] |= 1
+            if e <= incE:
+                flags[k] |= 2
+    domZ = float(sum(f & 1 for f in flags))
+    dflags = []
+    for k, (_, dl) in enumerate(targets):
+        row = []
+        for lp in dl:
+            f = 0
+            if flags[k] & 1:
+                e = math.exp(lp) * domZ
+                if e <= domE:
+                    f |= 1
+                if (flags[k] & 2) and e <= incdomE:
+                    f |= 2
+            row.append(f)
+        dflags.append(row)
+    return order, flags, dflags, domZ

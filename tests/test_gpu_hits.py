@@ -111,3 +111,34 @@ def test_fn3_tutorial_statistics_text_on_the_device(gpu):
     out = pli.run(om, db)
     assert out["stage"][0] == 4
     assert pli.statistics_text(domZ=1) == tutorial_stats_block("fn3.out")
+
+
+def test_device_hit_list_ranks_and_flags(gpu):
+    """The device's ranking and flags equal the oracle's restatement of
+    p7_tophits_SortBySortkey/Threshold applied to the device's own scores."""
+    path = synth_profile("synth_M200")
+    hmm = synth.read_hmm_probs(path)
+    db = synth.make_db(3000, [hmm], frac_homolog=0.03, seed_db=95, seed_plant=96)
+    sdb = gpu.SeqDB(db.residues, db.offsets, db.lengths)
+    om = gpu.OProfile.from_file(path)
+    pli = gpu.Pipeline(domains=True)
+    pli.run(om, sdb)
+    regs = pli.regions()
+    ts = pli.target_scores()
+    _, _, dlnP = pli.domain_scores()
+    scored = [k for k in range(ts["seq"].size) if ts["status"][k] == 0]
+    by_seq = {}
+    for q, r in enumerate(regs):
+        by_seq.setdefault(r[0], []).append(q)
+    for Z, E in ((3000, 10.0), (3000, 1e-3), (1e9, 10.0)):
+        order, flags, dflags, domZ = pli.hit_list(Z, E=E)
+        targets = [(float(ts["lnP"][k]), [float(dlnP[q]) for q in by_seq[int(ts["seq"][k])] if not np.isnan(dlnP[q])])
+                   for k in scored]
+        o_ord, o_fl, o_dfl, o_domZ = oracle.hit_threshold(targets, Z, E=E)
+        assert list(order) == [scored[k] for k in o_ord]
+        assert [int(f) for f in flags] == [o_fl[k] for k in o_ord]
+        assert domZ == o_domZ
+        for k, fl in zip(scored, o_dfl):
+            got = [int(dflags[q]) for q in by_seq[int(ts["seq"][k])] if not np.isnan(dlnP[q])]
+            assert got == fl
+        assert 0 < sum(f & 1 for f in flags) or E < 1e-2

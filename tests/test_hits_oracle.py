@@ -150,3 +150,17 @@ def test_statistics_text_equals_the_tutorial_globins4_search():
     assert got == tutorial_stats_block("globins4_stats.txt")
     got = p7gpu.statistics_text(st, (0.02, 1e-3, 1e-5), 1e6, 3, z_by_option=True, domz_by_option=True)
     assert "1000000  [as set by --Z on cmdline]" in got and "3  [as set by --domZ on cmdline]" in got
+
+
+def test_fn3_tutorial_domain_marks():
+    """tutorial/fn3.out marks each domain '!' (included) or '?' (reported, not
+    included): domains 1 and 6 (c-Evalue 0.17 and 0.063 > incdomE 0.01) are '?'.
+    The domain P-values are the oracle's for the eight single-domain envelopes,
+    the ninth's and the sequence's are the tutorial's (Z = domZ = 1)."""
+    L, fwdsc, nullsc, doms, tau, lam = fn3_inputs()
+    dres = oracle.target_scores(L, fwdsc, nullsc, doms[:8], tau, lam)[5]
+    dl = [lp for _, _, lp in dres] + [math.log(6.6e-06)]
+    order, flags, dflags, domZ = oracle.hit_threshold([(math.log(1.9e-57), dl)], Z=1)
+    assert flags == [3] and domZ == 1.0
+    marks = "".join("!" if f & 2 else "?" if f & 1 else " " for f in dflags[0])
+    assert marks == "?!!!!?!!!", marks
