@@ -249,7 +249,7 @@ void  p7_gpu_pipeline_Destroy(P7_GPU_PIPELINE *pli);
  * the counters like p7_pli_NewModel/NewSeq/p7_Pipeline. */
 int   p7_gpu_Pipeline(P7_GPU_PIPELINE *pli, const P7_GPU_OPROFILE *om, const P7_GPU_SEQDB *db, P7_GPU_HITS *hits);
 int   p7_gpu_pipeline_GetStats(const P7_GPU_PIPELINE *pli, P7_GPU_STATS *st);
-/* Domain decoding for the F3 survivors of p7_gpu_Pipeline: the first part of
+/* Domain decoding for the F3 survivors of p7_gpu_Pipeline: the part of
  * p7_domaindef_ByPosteriorHeuristics (HMMER/src/p7_domaindef.c:386-489) as
  * p7_Pipeline calls it (p7_pipeline.c:737): p7_DomainDecoding
  * (impl_sse/decoding.c:166-197) from the two parsers' special states, then
