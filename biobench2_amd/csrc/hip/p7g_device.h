@@ -127,23 +127,24 @@ struct DomBufs {
   const float *env_loop = nullptr, *env_move = nullptr;
   int64_t  env_cap = 0;
   // null2 of single-domain regions (rescore_isolated_domain): each region's
-  // area at n2pool + n2off[k] holds the Forward row scales, the Backward
-  // special rows and the Backward M and I rows (n2_area_floats). n2mode 1:
-  // the envelope Forward writes its scales; 2: the Forward recomputed with
-  // the posterior sums, null2 and the correction n2dc[k] (nats). n2st[k]:
-  // the Backward's status and own-scales flag (0x100), then 0 or 16
+  // area at n2pool + n2off[k] holds the Backward special rows and M and I
+  // rows (n2_area_floats); its Forward row scales are at n2scl + n2soff[k]
+  // (written by the envelope-score pass). n2mode 2: the Forward recomputed
+  // with the posterior sums, null2 and the correction n2dc[k] (nats).
+  // n2st[k]: the Backward's status and own-scales flag (0x100), then 0 or 16
   float   *n2pool = nullptr;
   const int64_t *n2off = nullptr;
+  const float *n2scl = nullptr;
+  const int64_t *n2soff = nullptr;
   float   *n2dc = nullptr;
   int32_t *n2st = nullptr;
   int      n2mode = 0;
 };
 
 // the parts of a region's null2 area, in floats, each a multiple of 64
-__host__ __device__ inline int64_t n2_scales_floats(int L) { return ((int64_t) L + 1 + 63) & ~(int64_t) 63; }
 __host__ __device__ inline int64_t n2_xrows_floats(int L) { return (6 * ((int64_t) L + 1) + 63) & ~(int64_t) 63; }
 __host__ __device__ inline int64_t n2_area_floats(int L, int B)
-{ return n2_scales_floats(L) + n2_xrows_floats(L) + (int64_t) L * 128 * B; }
+{ return n2_xrows_floats(L) + (int64_t) L * 128 * B; }
 // Per-target results of the decoding (slot order) and the region records
 // {slot, i, j, multidomain} in the order they were found.
 struct DomOut {
@@ -163,6 +164,10 @@ struct DomOut {
   int64_t *v_off = nullptr;
   int32_t *v_len = nullptr, *v_list = nullptr, *env_st = nullptr;
   float   *env_loop = nullptr, *env_move = nullptr, *envsc = nullptr;
+  // the envelope-score pass's row scales: region k's at rscale + rsoff[k]
+  float   *rscale = nullptr;
+  int64_t *rsoff = nullptr;
+  int64_t  rscale_cap = 0;
 };
 
 // Length model of one call. on == 0: every target uses its own length

@@ -1015,7 +1015,7 @@ __device__ __forceinline__ void null2_epilogue(const DevProfile &p, const float 
 // (a separate instance, so that the plain parser keeps its register budget).
 // MODE 2: envelope scores: the list is of region copies, each scored in unihit
 // mode (E move 1, E loop 0) with the N/J/C odds of its whole target's length;
-// with a null2 area (dom.n2off) the row scales go there too.
+// its row scales go to the scale pool when one is given.
 // MODE 3: the same Forward pass again for null2: each row's posteriors
 // (p7_Decoding, impl_sse/decoding.c:109-139) from this row and the stored
 // Backward row, summed over the rows; then the null2 odds
@@ -1066,12 +1066,11 @@ void k_fwd(const DevProfile p, const uint8_t *__restrict__ res, const int64_t *_
     const float floop = ENV ? dom.env_loop[slot] : lc.on ? lc.floop : p.xf_loop[L];
     const float fmove = ENV ? dom.env_move[slot] : lc.on ? lc.fmove : p.xf_move[L];
     float *scl = nullptr;
-    // null2: the region's area (scales, Backward special rows, Backward M/I rows)
+    // null2: the region's area (Backward special rows, Backward M/I rows)
     float *area = nullptr;
-    if (ENV && dom.n2off) {
+    if (MODE == 3) {
       const int64_t o = dom.n2off[slot];
       area = dom.n2pool + (((int64_t) (unsigned) uni((int) (o >> 32)) << 32) | (unsigned) uni((int) o));
-      if (MODE == 2) scl = area;
     }
     const float *xb = nullptr, *cells = nullptr;
     float sp = 0.0f, pN = 0.0f, pJ = 0.0f, pC = 0.0f;
@@ -1083,7 +1082,7 @@ void k_fwd(const DevProfile p, const uint8_t *__restrict__ res, const int64_t *_
         continue;
       }
       own = (bs & 0x100) != 0;
-      xb = area + n2_scales_floats(L);
+      xb = area;
       cells = xb + n2_xrows_floats(L) + z;
       sp = (float) (1.0 / (double) xb[1]);        // 1 / Backward's N at row 0
     }
@@ -1469,10 +1468,15 @@ void k_bck(const DevProfile p, const uint8_t *__restrict__ res, const int64_t *_
     float *cells = nullptr;           // envelope Backward for null2: its M and I rows
     float *xo = xmx_out;
     if (ENV) {
+      const int64_t so = dom.n2soff[slot];
+      if (so < 0) {                   // the envelope-score pass found no room for its scales
+        if (z == 0) { r.bcksc[slot] = NAN; r.fstatus[slot] = 5; }
+        continue;
+      }
+      fsc = dom.n2scl + (((int64_t) (unsigned) uni((int) (so >> 32)) << 32) | (unsigned) uni((int) so));
       const int64_t o = dom.n2off[slot];
       float *area = dom.n2pool + (((int64_t) (unsigned) uni((int) (o >> 32)) << 32) | (unsigned) uni((int) o));
-      fsc = area;
-      xo = area + n2_scales_floats(L);
+      xo = area;
       cells = xo + n2_xrows_floats(L) + z;
     } else if (fwd_xmx) { fsc = fwd_xmx + 5; fstride = 6; }
     else {

@@ -841,7 +841,8 @@ dom_reserve(Work &w, int64_t n, int64_t pool_floats)
   if (w.dom.pool) { (void) hipFree(w.dom.pool); w.dom.pool = nullptr; w.dom.size = 0; }
   const int64_t cap = std::max<int64_t>(n, 1024);
   const size_t total = (size_t) cap * (8 + 8 + 4 + 4 + 4) + (16 + 8 + 8 + 4 * 9) * (size_t) DOM_REG_CAP
-                       + (size_t) DOM_RES_CAP + 4 * (size_t) N2_CNT_INTS + 20 * 256;
+                       + (size_t) DOM_RES_CAP + 4 * (size_t) N2_CNT_INTS
+                       + 4 * (size_t) (DOM_RES_CAP + DOM_REG_CAP) + 8 * (size_t) DOM_REG_CAP + 24 * 256;
   char *m = nullptr;
   HIPCHK(hipMalloc((void **) &m, total));
   size_t o = 0;
@@ -861,6 +862,8 @@ dom_reserve(Work &w, int64_t n, int64_t pool_floats)
   w.n2dc = (float *) take(4 * (size_t) DOM_REG_CAP); w.n2st = (int32_t *) take(4 * (size_t) DOM_REG_CAP);
   w.n2list = (int32_t *) take(4 * (size_t) DOM_REG_CAP);
   w.n2cnt = (int32_t *) take(4 * (size_t) N2_CNT_INTS);
+  w.dout.rscale = (float *) take(4 * (size_t) (DOM_RES_CAP + DOM_REG_CAP)); w.dout.rscale_cap = DOM_RES_CAP + DOM_REG_CAP;
+  w.dout.rsoff = (int64_t *) take(8 * (size_t) DOM_REG_CAP);
   if (o > total) { (void) hipFree(m); p7g_set_error("internal: domain buffer overrun"); return p7G_EINVAL; }
   w.dom_mem = m; w.dom_cap = cap;
   HIPCHK(hipMalloc((void **) &w.dom.pool, sizeof(float) * (size_t) pool_floats));
@@ -1205,8 +1208,11 @@ chain_stages(Work &w, const DevProfile &p, const DevSeqs &s, int64_t n, const ui
     vr.fwdsc = w.dout.envsc; vr.fstatus = w.dout.env_st;
     DomBufs eb;
     eb.env_loop = w.dout.env_loop; eb.env_move = w.dout.env_move; eb.env_cap = w.dout.reg_cap;
-    HIPCHK(launch_fwd(p, vs, w.dout.v_list, &w.dout.ecount[0], w.dout.reg_cap, vr, w.l, g, 0, lc, nullptr, 0, nullptr,
-                      nullptr, w.st, eb));
+    // the row scales go to their own pool (cursor 3), for the null2 Backward
+    DevLists lr = w.l;
+    lr.cursor = w.l.cursor + 3;
+    HIPCHK(launch_fwd(p, vs, w.dout.v_list, &w.dout.ecount[0], w.dout.reg_cap, vr, lr, g, 0, lc, w.dout.rscale,
+                      w.dout.rscale_cap, w.dout.rsoff, nullptr, w.st, eb));
     if (timed) HIPCHK(hipEventRecord(w.ev[7], w.st));
   }
   return p7G_OK;
@@ -1316,13 +1322,13 @@ null2_regions(Work &w, const DevProfile &p, const DevSeqs &s, int64_t nregs, con
   DomBufs eb;
   eb.env_loop = w.dout.env_loop; eb.env_move = w.dout.env_move; eb.env_cap = w.dout.reg_cap;
   eb.n2pool = w.n2pool; eb.n2off = w.n2off; eb.n2dc = w.n2dc; eb.n2st = w.n2st;
+  eb.n2scl = w.dout.rscale; eb.n2soff = w.dout.rsoff;
   const LenCfg lc = lencfg_per_target();
   for (size_t b = 0; b < batches.size(); b++) {
     const int32_t *bl = w.n2list + batches[b].first;
     const int64_t nb = (int64_t) (batches[b].second - batches[b].first);
     const int32_t *c = w.n2cnt + b * 3 * 16;
-    eb.n2mode = 1;
-    HIPCHK(launch_fwd(p, vs, bl, c, nb, vf, w.l, g, 0, lc, nullptr, 0, nullptr, nullptr, w.st, eb));
+    eb.n2mode = 0;
     HIPCHK(launch_bck(p, vs, bl, c + 16, nb, vb, nullptr, nullptr, nullptr, nullptr, lc, w.st, eb, w.l));
     eb.n2mode = 2;
     HIPCHK(launch_fwd(p, vs, bl, c + 32, nb, vf, w.l, g, 0, lc, nullptr, 0, nullptr, nullptr, w.st, eb));
