@@ -149,3 +149,22 @@ def test_large_model_null2(gpu, tmp_path):
     pli = gpu.Pipeline(0.5, 0.2, 0.05, domains=True)
     pli.run(om, sdb)
     check_regions(pli, om, lambda q: db.seq(int(q)), min_checked=3)
+
+
+def test_null2_of_the_shortest_envelopes(gpu):
+    """With every filter off (--max), targets of 1 to 40 residues all reach
+    domain definition; regions as short as one residue get the oracle's
+    correction too."""
+    path = synth_profile("synth_M50")
+    hmm = synth.read_hmm_probs(path)
+    rng = np.random.default_rng(101)
+    seqs = [synth.iid_residues(int(L), rng) for L in list(range(1, 41)) * 3]
+    for core in synth.emit_homologs(hmm, 20, rng):
+        seqs.append(core[: int(rng.integers(1, max(2, core.size)))])
+    db = gpu.SeqDB.from_list(seqs)
+    om = gpu.OProfile.from_file(path)
+    pli = gpu.Pipeline(do_max=True, domains=True)
+    pli.run(om, db)
+    regs = pli.regions()
+    assert min(j - i + 1 for _, i, j, _, _ in regs) <= 3
+    check_regions(pli, om, lambda q: seqs[q], min_checked=20)
