@@ -2,9 +2,10 @@
 precision stress: long targets, indel-rich homologs).
 
 For these models the ungapped MSV pass runs in two chunks of model positions,
-joined by a boundary column in HBM (above M = 2560); Viterbi keeps its
-transitions in LDS and reads emission rows from global memory (above
-M = 2047); Forward and Backward read their transitions per row (B >= 16).
+joined by a boundary column in HBM (above M = 2560); the full MSV recurrence
+and Viterbi (above M = 2047) and Forward and Backward (B >= 32) run with four
+or eight waves per target, each wave owning a slice of every lane's
+positions; M = 2000 keeps the one-wave Viterbi with its whole table in LDS.
 MSV and Viterbi must stay bit-exact against the oracle, Forward and Backward
 within 1e-4 nats of the oracle's fp32 restatement, and the whole pipeline
 must give the oracle's pass sets. The error of the fp32 Forward against the
