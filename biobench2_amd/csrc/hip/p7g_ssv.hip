@@ -124,10 +124,12 @@ void k_ssv(const SsvJobs jobs, int njobs, int tbl_dwords,
       const bool part = valid && t < nsegs;
       int64_t slot = 0;
       int rs = 0, L = 0;                              // rows rs .. rs+L-1 of the target
+      int own0 = 0;                                   // the segment's own rows start at rs + own0
       if (part) {
         const int2 sg = segs[t];
         slot = sg.x;
         rs = max(0, sg.y - WARM) & ~15;
+        own0 = sg.y - rs;
         L = min(len[slot], sg.y + P7G_SEG_ROWS) - rs;
       } else if (valid) {
         const int64_t u = t - nsegs + nseg_slots;
@@ -150,9 +152,13 @@ void k_ssv(const SsvJobs jobs, int njobs, int tbl_dwords,
 
       const uint16_t *bi = nullptr;
       uint16_t *bo = nullptr;
+      // a chunk's boundary columns, from the item's first row; a segment writes
+      // only its own rows (its warm-up rows are lower bounds): the previous
+      // chunk's column is exact on every row, so a warm-up of the chunk's width
+      // makes the segment's own rows exact
       if (BND) {
-        if (bin) bi = bin + off[slot];
-        if (bout) bo = bout + off[slot];
+        if (bin) bi = bin + off[slot] + rs;
+        if (bout) bo = bout + off[slot] + rs;
       }
       // boundary values of rows i0 .. i0+15 (bq) and of row i0-1 (pv); the floor before row 0
       uint4 bq0 = make_uint4(0x80008000u, 0x80008000u, 0x80008000u, 0x80008000u), bq1 = bq0;
@@ -164,7 +170,7 @@ void k_ssv(const SsvJobs jobs, int njobs, int tbl_dwords,
 #define BND_IN(u) do { if (BND && (u) < 16) {                                                   \
         const unsigned v_ = ((u) == 0) ? pv : BQ_HALF((u) - 1);                                  \
         nb = (z == 0) ? (int) (v_ << 16) : nb; } } while (0)
-#define BND_OUT(r) do { if (BND && bo && z == 63) bo[r] = (uint16_t) ((unsigned) zr[ND-1] >> 16); } while (0)
+#define BND_OUT(r) do { if (BND && bo && z == 63 && (r) >= own0) bo[r] = (uint16_t) ((unsigned) zr[ND-1] >> 16); } while (0)
 #define BND_LOAD(i) do { if (BND && bi) {                                                        \
         pv = bq1.w >> 16; bq0 = *(const uint4 *) (bi + (i)); bq1 = *(const uint4 *) (bi + (i) + 8); } } while (0)
 
@@ -314,16 +320,20 @@ static hipError_t ssv_run(int G, int ND, const SsvJobs &jobs, int njobs, const D
   void *k = ssv_kernel(G, ND, bnd);
   if (! k || njobs < 1 || njobs > P7G_SSV_MAXJOBS || grab < 1 || (bnd && njobs != 1)) return hipErrorInvalidValue;
   // long targets in row segments: over every slot, when the warm-up (the
-  // shape's positions) is at most half a segment
+  // shape's positions: a chunk's, for a chunked model) is at most half a
+  // segment. Their entries start from zero, except for a later chunk, whose
+  // segments fold into the maxima of the chunks before it.
   const int2 *segs = nullptr;
   int64_t nsegs = 0, nseg_slots = 0;
-  if (! slots && ! bnd && s.nseg > 0 && n == s.nslot && 2 * ND * G <= P7G_SEG_ROWS / 2) {
+  if (! slots && s.nseg > 0 && n == s.nslot && 2 * ND * G <= P7G_SEG_ROWS / 2) {
     segs = s.seg; nsegs = s.nseg; nseg_slots = s.nseg_slots;
-    const int64_t total = nseg_slots * njobs;
-    const unsigned zg = (unsigned) std::min<int64_t>((total + 255) / 256, 4096);
-    hipLaunchKernelGGL(k_zero_u16, dim3(zg), dim3(256), 0, st, zmax, zstride, nseg_slots, njobs);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
+    if (! bin) {
+      const int64_t total = nseg_slots * njobs;
+      const unsigned zg = (unsigned) std::min<int64_t>((total + 255) / 256, 4096);
+      hipLaunchKernelGGL(k_zero_u16, dim3(zg), dim3(256), 0, st, zmax, zstride, nseg_slots, njobs);
+      hipError_t e = hipGetLastError();
+      if (e != hipSuccess) return e;
+    }
   }
   const size_t lds = (size_t) P7G_NROWS * G * ND * 4;
   const int64_t nitems = n - nseg_slots + nsegs;
