@@ -2166,6 +2166,10 @@ hipError_t launch_fwd(const DevProfile &p, const DevSeqs &s, const int32_t *list
   int ww = 0;
   if (! dom.pool && ! dom.env_loop && ! xmx_out) {
     switch (p.fb_B) {
+      case 16: kw = (void *) k_fwd_wg<16, 4>; ww = 4; break;
+      case 20: kw = (void *) k_fwd_wg<20, 4>; ww = 4; break;
+      case 24: kw = (void *) k_fwd_wg<24, 4>; ww = 4; break;
+      case 28: kw = (void *) k_fwd_wg<28, 4>; ww = 4; break;
       case 32: kw = (void *) k_fwd_wg<32, 4>; ww = 4; break;
       case 40: kw = (void *) k_fwd_wg<40, 4>; ww = 4; break;
       case 48: kw = (void *) k_fwd_wg<48, 4>; ww = 4; break;
@@ -2204,6 +2208,10 @@ hipError_t launch_bck(const DevProfile &p, const DevSeqs &s, const int32_t *list
   int ww = 0;
   if (! fwd_xmx && ! xmx_out && ! dom.pool && ! dom.env_loop) {
     switch (p.fb_B) {
+      case 16: kw = (void *) k_bck_wg<16, 4>; ww = 4; break;
+      case 20: kw = (void *) k_bck_wg<20, 4>; ww = 4; break;
+      case 24: kw = (void *) k_bck_wg<24, 4>; ww = 4; break;
+      case 28: kw = (void *) k_bck_wg<28, 4>; ww = 4; break;
       case 32: kw = (void *) k_bck_wg<32, 4>; ww = 4; break;
       case 40: kw = (void *) k_bck_wg<40, 4>; ww = 4; break;
       case 48: kw = (void *) k_bck_wg<48, 4>; ww = 4; break;
