@@ -30,8 +30,34 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(om, db, M, seconds_target=15.0, max_threads=16):
-    """The oracle's pipeline on a bounded sample, on the host cores; GCUPS."""
+def host_cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+# The reference's own SSE build (HMMER 3.1b2, -O3 -msse2), timed on one core of
+# an 8-core Intel Xeon with LuxC.hmm (M = 400): per-stage filter rates and
+# hmmsearch --cpu 1 / --cpu 8 over 200k sequences of L = 300. Quoted beside the
+# port, not measured here: the reference's binary is not on this host.
+REFERENCE_SSE_PER_CORE = {"host": "8-core Intel Xeon (build container), 1 thread per core",
+                          "workload": "LuxC.hmm (M = 400) vs i.i.d. sequences of L = 400 (filters), "
+                                      "200k x L = 300 (hmmsearch)",
+                          "msv_gcups": "37-40", "ssv_gcups": 40.0, "viterbi_gcups": 2.1,
+                          "forward_gcups": 0.19, "backward_gcups": 0.16,
+                          "hmmsearch_cpu1_gcups": 17.9, "hmmsearch_cpu8_gcups": 57.1}
+
+
+def cpu_baseline(om, db, M, gpu_stage, gpu_usc=None, seconds_target=15.0, max_threads=16):
+    """The oracle's pipeline on a bounded sample, on the host cores; GCUPS.
+    The sample's stages are also compared with the device run's stages
+    (gpu_stage, in database order): oracle_mismatches counts the targets whose
+    stage differs, usc_mismatches those whose MSV score differs."""
     import oracle
     t = om.tables()
     prof, keep = oracle.make_profile(t, oracle.bias_hmm(t), t["evparam"], Lmax=int(db.lengths.max()) + 1)
@@ -40,15 +66,23 @@ def cpu_baseline(om, db, M, seconds_target=15.0, max_threads=16):
     while True:
         sub = db.subset(np.arange(n))
         t0 = time.perf_counter()
-        oracle.pipeline_db(prof, sub.residues, sub.offsets, sub.lengths, nthreads)
+        ref = oracle.pipeline_db(prof, sub.residues, sub.offsets, sub.lengths, nthreads)
         dt = time.perf_counter() - t0
         if dt >= seconds_target / 4 or n >= db.nseq:
             break
         n = min(db.nseq, int(n * max(2.0, seconds_target / 4 / max(dt, 1e-3))))
     cells = float(sub.nres) * M
+    mism = int(np.sum(np.asarray(gpu_stage[:n]) != ref["stage"].astype(np.uint8)))
+    usc_mism = None if gpu_usc is None else int(np.sum(np.asarray(gpu_usc[:n]) != ref["usc"]))
     return {"value": cells / dt / 1e9, "unit": "GCUPS", "cores": nthreads, "kind": "port",
+            "cpu_model": host_cpu_model(),
             "sample": f"first {n} of {db.nseq} database sequences ({sub.nres} residues), full filter pipeline "
-                      f"(oracle/p7_oracle.c restatement, scalar C), {nthreads} threads, {dt:.1f} s"}
+                      f"(oracle/p7_oracle.c restatement, scalar C), {nthreads} threads, {dt:.1f} s",
+            "per_thread_gcups": round(cells / dt / 1e9 / nthreads, 3),
+            "oracle_compared": n, "oracle_mismatches": mism, "oracle_usc_mismatches": usc_mism,
+            "oracle_pass_counts": {"msv": int((ref["stage"] >= 1).sum()), "bias": int((ref["stage"] >= 2).sum()),
+                                   "vit": int((ref["stage"] >= 3).sum()), "fwd": int((ref["stage"] >= 4).sum())},
+            "reference_sse_per_core_quoted": REFERENCE_SSE_PER_CORE}
 
 
 def serving_leg(p7gpu, synth, nseq, nq, rate, nslots, seed=11, tune=True):
@@ -436,7 +470,9 @@ def main():
                                       args.serve_slots, tune=not args.no_tune)
     if not args.no_cpu:
         log("cpu baseline ...")
-        line["cpu_baseline"] = cpu_baseline(om, db, M, args.cpu_seconds)
+        chk = pli.run(om, sdb, scores=True)          # outside the timed region: the scores to compare
+        line["cpu_baseline"] = cpu_baseline(om, db, M, chk["stage"], chk["usc"], args.cpu_seconds)
+        line["oracle_mismatches"] = line["cpu_baseline"]["oracle_mismatches"]
     else:
         line["cpu_baseline"] = None
     print(json.dumps(line), flush=True)

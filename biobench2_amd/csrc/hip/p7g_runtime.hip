@@ -636,7 +636,6 @@ p7_gpu_seqdb_Load(const char *path, P7_GPU_SEQDB **ret_db)
 {
   int status;
   *ret_db = nullptr;
-  if ((status = ensure_init()) != p7G_OK) return status;
   FILE *fp = fopen(path, "rb");
   if (! fp) { p7g_set_error("can't open %s", path); return p7G_ENOTFOUND; }
   P7_GPU_SEQDB *db = (P7_GPU_SEQDB *) calloc(1, sizeof(P7_GPU_SEQDB));
@@ -645,6 +644,7 @@ p7_gpu_seqdb_Load(const char *path, P7_GPU_SEQDB **ret_db)
   int64_t namebytes = 0;
   std::vector<uint8_t> h;
   std::vector<int64_t> h_off;
+  const char *why = "truncated or bad header";
   bool ok = db && fread(&magic, 4, 1, fp) == 1 && magic == P7G_DBMAGIC &&
             fread(&db->nseq, 8, 1, fp) == 1 && fread(&db->nres, 8, 1, fp) == 1 && fread(&db->bytes, 8, 1, fp) == 1 &&
             fread(&db->Lmax, 4, 1, fp) == 1 && fread(&db->maxcode, 4, 1, fp) == 1 &&
@@ -656,20 +656,63 @@ p7_gpu_seqdb_Load(const char *path, P7_GPU_SEQDB **ret_db)
     ok = db->h_len && db->h_perm && db->h_inv &&
          fread(db->h_len, 4, (size_t) db->nseq, fp) == (size_t) db->nseq &&
          fread(db->h_perm, 8, (size_t) db->nseq, fp) == (size_t) db->nseq;
+    if (! ok) why = "truncated length or permutation table";
   }
   if (ok) {
-    // the layout follows from the lengths: each target 16-byte aligned, padded to 16
+    // The layout follows from the lengths: each target 16-byte aligned, padded
+    // to 16, longest first. The kernels size their per-length tables from Lmax
+    // and index them by each target's length, and the slot <-> caller maps must
+    // be a permutation, so none of this is taken on trust.
     h_off.resize((size_t) db->nseq);
+    for (int64_t i = 0; i < db->nseq; i++) db->h_inv[i] = -1;
     int64_t pos = 0, nres = 0;
+    int Lmax = 0;
     for (int64_t s = 0; s < db->nseq && ok; s++) {
       const int64_t i = db->h_perm[s];
-      ok = db->h_len[s] >= 0 && i >= 0 && i < db->nseq;
-      if (ok) { db->h_inv[i] = (int32_t) s; h_off[(size_t) s] = pos; pos += ((int64_t) db->h_len[s] + 15) & ~(int64_t) 15; nres += db->h_len[s]; }
+      const int L = db->h_len[s];
+      if (L < 0 || L > 100000000)               { ok = false; why = "bad sequence length"; break; }
+      if (s > 0 && L > db->h_len[s - 1])         { ok = false; why = "lengths not in decreasing order"; break; }
+      if (i < 0 || i >= db->nseq)                { ok = false; why = "permutation entry out of range"; break; }
+      if (db->h_inv[i] != -1)                    { ok = false; why = "permutation has a repeated entry"; break; }
+      db->h_inv[i] = (int32_t) s;
+      h_off[(size_t) s] = pos;
+      pos += ((int64_t) L + 15) & ~(int64_t) 15;
+      nres += L;
+      if (L > Lmax) Lmax = L;
     }
-    ok = ok && pos + 16 == db->bytes && nres == db->nres;
+    if (ok && (pos + 16 != db->bytes || nres != db->nres)) { ok = false; why = "residue count or size does not match the lengths"; }
+    if (ok && Lmax != db->Lmax) { ok = false; why = "header's longest length does not match the lengths"; }
   }
-  if (ok) { h.resize((size_t) db->bytes); ok = fread(h.data(), 1, (size_t) db->bytes, fp) == (size_t) db->bytes; }
-  if (ok) ok = fread(&namebytes, 8, 1, fp) == 1 && namebytes >= 0;
+  if (ok) {
+    h.resize((size_t) db->bytes);
+    ok = fread(h.data(), 1, (size_t) db->bytes, fp) == (size_t) db->bytes;
+    if (! ok) why = "truncated residues";
+  }
+  if (ok) {
+    // residues are digital codes of the alphabet (< 29), padding is P7G_PADCODE,
+    // and the header's largest code is the largest residue present
+    int maxcode = 0;
+    int64_t pos = 0;
+    for (int64_t s = 0; s < db->nseq && ok; s++) {
+      const int L = db->h_len[s];
+      const int64_t end = pos + (((int64_t) L + 15) & ~(int64_t) 15);
+      for (int64_t b = pos; b < pos + L; b++) {
+        const int c = h[(size_t) b];
+        if (c >= 29) { ok = false; why = "residue code outside the digital alphabets"; break; }
+        if (c > maxcode) maxcode = c;
+      }
+      for (int64_t b = pos + L; ok && b < end; b++)
+        if (h[(size_t) b] != P7G_PADCODE) { ok = false; why = "bad padding byte"; }
+      pos = end;
+    }
+    for (int64_t b = pos; ok && b < db->bytes; b++)
+      if (h[(size_t) b] != P7G_PADCODE) { ok = false; why = "bad padding byte"; }
+    if (ok && maxcode != db->maxcode) { ok = false; why = "header's largest residue code does not match the residues"; }
+  }
+  if (ok) {
+    ok = fread(&namebytes, 8, 1, fp) == 1 && namebytes >= 0;
+    if (! ok) why = "truncated name block";
+  }
   if (ok && namebytes > 0) {
     db->names = (char *) malloc((size_t) namebytes);
     db->name_off = (int64_t *) malloc(8 * (size_t) db->nseq);
@@ -677,11 +720,15 @@ p7_gpu_seqdb_Load(const char *path, P7_GPU_SEQDB **ret_db)
          fread(db->name_off, 8, (size_t) db->nseq, fp) == (size_t) db->nseq;
     for (int64_t i = 0; ok && i < db->nseq; i++) ok = db->name_off[i] >= 0 && db->name_off[i] < namebytes;
     ok = ok && db->names[namebytes - 1] == '\0';
+    if (! ok) why = "bad name block";
   }
-  if (ok) ok = fread(&magic, 4, 1, fp) == 1 && magic == P7G_DBMAGIC;
+  if (ok) {
+    ok = fread(&magic, 4, 1, fp) == 1 && magic == P7G_DBMAGIC;
+    if (! ok) why = "bad trailer";
+  }
   fclose(fp);
-  if (! ok) { p7g_set_error("%s is not a valid database image", path); p7_gpu_seqdb_Destroy(db); return p7G_EFORMAT; }
-  for (int64_t b = 0; b < db->bytes; b++) if (h[(size_t) b] >= 32) { p7g_set_error("%s: bad residue code", path); p7_gpu_seqdb_Destroy(db); return p7G_EFORMAT; }
+  if (! ok) { p7g_set_error("%s is not a valid database image: %s", path, why); p7_gpu_seqdb_Destroy(db); return p7G_EFORMAT; }
+  if ((status = ensure_init()) != p7G_OK) { p7_gpu_seqdb_Destroy(db); return status; }
   std::vector<int32_t> perm32((size_t) db->nseq);
   for (int64_t s = 0; s < db->nseq; s++) perm32[(size_t) s] = (int32_t) db->h_perm[s];
   hipError_t e = hipMalloc((void **) &db->d_res, (size_t) db->bytes);
@@ -755,8 +802,12 @@ static hipError_t launch_ssv_chosen(const DevProfile &p, const DevSeqs &s, const
 #define WORK_BYTES_PER_SLOT (2 + 2 + 4*6 + 1 + 4 + 4 + 4*5 + 4 + 8 + 4 + 1)
 #define WORK_BYTES_FIXED    (64 * 1024)     // alignment of the arrays, counters, compaction tiles
 // Forward scale-factor pool of a pipeline: one float per row of every target
-// that may reach Forward, at most 2^28 floats
-static int64_t pipeline_pool_floats(int64_t nseq, int64_t nres) { return std::min<int64_t>(nres + nseq, (int64_t) 1 << 28); }
+// that reaches Forward. F3 is only known after Forward, so every target that
+// reaches it takes L+1 floats. A pool starts at min(rows of every target,
+// 2^28) floats; a run that finds it full (loose thresholds, --max) grows it
+// to the rows of every target (pool_rows_all) and runs again.
+static int64_t pool_rows_all(int64_t nseq, int64_t nres) { return nres + nseq; }
+static int64_t pipeline_pool_floats(int64_t nseq, int64_t nres) { return std::min<int64_t>(pool_rows_all(nseq, nres), (int64_t) 1 << 28); }
 
 static int
 work_reserve(Work &w, int64_t n, int64_t pool_floats)
@@ -1358,6 +1409,10 @@ p7_gpu_Pipeline(P7_GPU_PIPELINE *pli, const P7_GPU_OPROFILE *om, const P7_GPU_SE
   DevSeqs s = seqs_of(db);
   const GateThresh g = gates_of(pli, om);
 
+  unsigned long long sums[8];
+  int32_t counts[16];
+  int32_t nclu_total = 0, nregs_dev = 0;
+  for (int attempt = 0; ; attempt++) {
   if ((status = chain_reset(w, n)) != p7G_OK) return status;
   if (dom) {                              // targets that do not reach decoding: NaN, no regions
     HIPCHK(launch_fill32(w.dout.nexp, 0x7fc00000u, n, w.st));
@@ -1375,8 +1430,6 @@ p7_gpu_Pipeline(P7_GPU_PIPELINE *pli, const P7_GPU_OPROFILE *om, const P7_GPU_SE
   // counters on the device, then the requested per-target arrays in the caller's order
   HIPCHK(launch_fill32(w.stats, 0u, 16, w.st));
   HIPCHK(launch_stats(s, w.r.stage, n, w.l.vit, w.l.fwd, w.l.counts, w.stats, w.st));
-  unsigned long long sums[8];
-  int32_t counts[16];
   HIPCHK(hipMemcpyAsync(sums, w.stats, sizeof(sums), hipMemcpyDeviceToHost, w.st));
   HIPCHK(hipMemcpyAsync(counts, w.l.counts, sizeof(counts), hipMemcpyDeviceToHost, w.st));
   if (hits && hits->stage) {
@@ -1400,13 +1453,16 @@ p7_gpu_Pipeline(P7_GPU_PIPELINE *pli, const P7_GPU_OPROFILE *om, const P7_GPU_SE
       HIPCHK(hipMemcpyAsync(dl[a].dst, w.outbuf, 4 * (size_t) n, hipMemcpyDeviceToHost, w.st));
     }
   }
-  int32_t nclu_total = 0, nregs_dev = 0;
   if (dom) {
     HIPCHK(hipMemcpyAsync(&nclu_total, w.dout.nclu_total, 4, hipMemcpyDeviceToHost, w.st));
     HIPCHK(hipMemcpyAsync(&nregs_dev, w.dout.ecount, 4, hipMemcpyDeviceToHost, w.st));
   }
   HIPCHK(hipStreamSynchronize(w.st));
-  if (counts[6]) { p7g_set_error("Forward scale-factor pool exhausted"); return p7G_EMEM; }
+  if (! counts[6]) break;
+  // the scale-factor pool was full: grow it to the rows of every target, run again
+  if (attempt > 0 || w.pool_size >= pool_rows_all(n, db->nres)) { p7g_set_error("Forward scale-factor pool exhausted"); return p7G_EMEM; }
+  if ((status = work_reserve(w, n, pool_rows_all(n, db->nres))) != p7G_OK) return status;
+  }
   if (dom) {
     if (counts[7]) {
       p7g_set_error("domain decoding %s exhausted", (counts[7] & 1) ? "row pool" : (counts[7] & 2) ? "region list" : "region residue pool");
@@ -1905,12 +1961,17 @@ p7_gpu_PipelineBatch(P7_GPU_PIPELINE *pli, const P7_GPU_OPROFILE *const *oms, in
   const bool graph = (flags & P7_GPU_BATCH_GRAPH) != 0;
   const int nchain = std::min(nprof, graph ? P7G_GRAPH_CHAINS : P7G_BATCH_CHAINS);
 
-  // work areas
+  // work areas (a run that finds a scale-factor pool full grows the pools to
+  // the rows of every target and runs again)
   const unsigned mask = hits_mask(hits, nprof);
+  std::vector<unsigned long long> sums;
+  std::vector<int32_t> counts;
+  int64_t pool_floats = pipeline_pool_floats(n, db->nres);
+  for (int attempt = 0; ; attempt++) {
   for (int k = 0; k < nchain; k++) {
     Work &w = b.ch[k];
     if (! w.st) HIPCHK(hipStreamCreateWithFlags(&w.st, hipStreamNonBlocking));
-    if ((status = work_reserve(w, n, pipeline_pool_floats(n, db->nres))) != p7G_OK) return status;
+    if ((status = work_reserve(w, n, pool_floats)) != p7G_OK) return status;
   }
   if ((status = batch_reserve(b, nprof, n, mask)) != p7G_OK) return status;
   for (const BatchItem &x : it)
@@ -1959,8 +2020,8 @@ p7_gpu_PipelineBatch(P7_GPU_PIPELINE *pli, const P7_GPU_OPROFILE *const *oms, in
   HIPCHK(hipEventRecord(b.t[1], st0));
   b.last_graph = eager ? 0 : 1;
 
-  std::vector<unsigned long long> sums((size_t) nprof * 8);
-  std::vector<int32_t> counts((size_t) nprof * 16);
+  sums.assign((size_t) nprof * 8, 0);
+  counts.assign((size_t) nprof * 16, 0);
   HIPCHK(hipMemcpyAsync(sums.data(), b.sums, sizeof(unsigned long long) * sums.size(), hipMemcpyDeviceToHost, st0));
   HIPCHK(hipMemcpyAsync(counts.data(), b.counts, sizeof(int32_t) * counts.size(), hipMemcpyDeviceToHost, st0));
   for (int j = 0; hits && j < nprof; j++) {
@@ -1970,10 +2031,17 @@ p7_gpu_PipelineBatch(P7_GPU_PIPELINE *pli, const P7_GPU_OPROFILE *const *oms, in
       if (dst[a]) HIPCHK(hipMemcpyAsync(dst[a], b.o_f[a] + (size_t) j * n, sizeof(float) * (size_t) n, hipMemcpyDeviceToHost, st0));
   }
   HIPCHK(hipStreamSynchronize(st0));
+  int full = -1;
+  for (int j = 0; j < nprof && full < 0; j++) if (counts[(size_t) j * 16 + 6]) full = j;
+  if (full < 0) break;
+  if (attempt > 0 || pool_floats >= pool_rows_all(n, db->nres)) {
+    p7g_set_error("Forward scale-factor pool exhausted (batch profile %d)", full);
+    return p7G_EMEM;
+  }
+  pool_floats = pool_rows_all(n, db->nres);
+  }
   float ms = 0.0f;
   HIPCHK(hipEventElapsedTime(&ms, b.t[0], b.t[1]));
-  for (int j = 0; j < nprof; j++)
-    if (counts[(size_t) j * 16 + 6]) { p7g_set_error("Forward scale-factor pool exhausted (batch profile %d)", j); return p7G_EMEM; }
   for (int j = 0; j < nprof; j++) {
     P7_GPU_STATS one;
     stats_of_run(one, &sums[(size_t) j * 8], &counts[(size_t) j * 16], oms[j], db, pli->do_biasfilter);

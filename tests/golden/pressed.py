@@ -127,6 +127,10 @@ def read_pressed(h3f_path, h3p_path):
             raise ValueError("bad .h3p sentinel")
         m["rwv"] = _destripe(rwv, M, Q8, 8)
         m["rfv"] = _destripe(rfv, M, Q4, 4)
+        # the striped vectors as the file holds them (for a de-striping shim to read)
+        m["rbv_striped"], m["rwv_striped"], m["twv_striped"] = rbv, rwv, twv
+        m["rfv_striped"], m["tfv_striped"] = rfv, tfv
+        m["Q"] = (Q16, Q8, Q4)
         # transitions: per q, seven vectors BM MM IM DM MD MI II, then Q DD vectors
         tw = np.zeros((8, Q8, 8), np.int16)
         tf = np.zeros((8, Q4, 4), np.float32)

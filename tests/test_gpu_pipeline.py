@@ -95,3 +95,32 @@ def test_pipeline_long_targets_in_row_segments(gpu, name, M):
                                      Lmax=int(db.lengths.max()) + 1)
     ref = oracle.pipeline_db(prof, db.residues, db.offsets, db.lengths, nthreads=8)
     check(got, ref)
+
+
+def test_pipeline_max_mode_grows_the_scale_pool(gpu):
+    """--max over a database with more rows than the Forward scale-factor
+    pool's first size (2^28 floats): every target reaches Forward and takes L+1
+    floats, the run finds the pool full, grows it to the rows of every target
+    and runs again. Every target passes; Forward/Backward scores on a sample
+    are the oracle's."""
+    path = synth_profile("synth_M50")
+    om = gpu.OProfile.from_file(path)
+    hmm = synth.read_hmm_probs(path)
+    db = synth.make_db(800_000, [hmm], frac_homolog=0.001, seed_db=81, seed_plant=82)
+    assert db.nres + db.nseq > (1 << 28)
+    sdb = gpu.SeqDB(db.residues, db.offsets, db.lengths)
+    pli = gpu.Pipeline(do_max=True)
+    got = pli.run(om, sdb)
+    assert (got["stage"] == 4).all()
+    assert pli.stats()["n_past_fwd"] == db.nseq
+    sub = np.random.default_rng(83).choice(db.nseq, 3000, replace=False)
+    sdbs = db.subset(sub)
+    t = om.tables()
+    prof, keep = oracle.make_profile(t, oracle.bias_hmm(t), t["evparam"], F1=1.0, F2=1.0, F3=1.0, do_bias=False,
+                                     Lmax=int(db.lengths.max()) + 1)
+    ref = oracle.pipeline_db(prof, sdbs.residues, sdbs.offsets, sdbs.lengths, nthreads=8)
+    for k in ("fwdsc", "bcksc"):
+        assert np.max(np.abs(got[k][sub].astype(np.float64) - ref[k])) <= 1e-4, k
+    # a second call keeps the grown pool
+    again = pli.run(om, sdb)
+    assert np.array_equal(again["fwdsc"], got["fwdsc"])
