@@ -227,6 +227,10 @@ hipError_t launch_msv_classify(const DevProfile &p, const DevSeqs &s, const int3
                                LenCfg lc, hipStream_t st);
 hipError_t launch_msv_full(const DevProfile &p, const DevSeqs &s, const int32_t *list, const int32_t *count,
                            int64_t maxn, int16_t *xj, LenCfg lc, hipStream_t st);
+// MSV score (nats) and status (0, or 16 on overflow) of listed slots from xj,
+// each with its own length's tjb.
+hipError_t launch_msv_scores(const DevProfile &p, const DevSeqs &s, const int32_t *slots, int64_t n,
+                             const int16_t *xj, float *sc, int *st, hipStream_t stream);
 hipError_t launch_msv_gate(const DevProfile &p, const DevSeqs &s, int64_t n, const int16_t *xj, DevResults r,
                            DevLists l, GateThresh g, hipStream_t st);
 // The bias filter; with `rowmax` (room for L+1 floats per listed target, taken

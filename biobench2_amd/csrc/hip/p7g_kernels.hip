@@ -206,6 +206,25 @@ void k_msv_full_wg(const DevProfile p, const int *__restrict__ tbl, const uint8_
   }
 }
 
+// MSV scores of listed targets from their final xJ (msvfilter.c:203-207,
+// with the per-target length model): ((xJ - tjb) - base) / scale_b - 3 nats
+// in float, the last step in double as the reference's `-= 3.0`; +inf and
+// status 16 (eslERANGE) where the recurrence overflowed.
+__global__ void k_msv_scores(const DevProfile p, const int32_t *__restrict__ len, const int32_t *__restrict__ slots,
+                             int64_t n, const int16_t *__restrict__ xj, float *__restrict__ sc, int *__restrict__ st)
+{
+#pragma clang fp contract(off)
+  for (int64_t t = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (int64_t) gridDim.x * blockDim.x) {
+    const int slot = slots[t];
+    const int v = xj[slot];
+    if (v < 0) { sc[t] = __int_as_float(0x7f800000); st[t] = 16; continue; }
+    float f = (float) (v - (int) p.tjb[len[slot]]) - (float) p.base;
+    f = __fdiv_rn(f, p.scale_b);
+    sc[t] = (float) ((double) f - 3.0);
+    st[t] = 0;
+  }
+}
+
 /*****************************************************************
  * 4. MSV gate: score, null score, F1 (and F2 when the bias filter is off)
  *****************************************************************/
@@ -2002,6 +2021,15 @@ hipError_t launch_copy32(void *dst, const void *src, int64_t nwords, hipStream_t
   if (nwords <= 0) return hipSuccess;
   const unsigned grid = (unsigned) std::min<int64_t>((nwords + 255) / 256, 4096);
   hipLaunchKernelGGL(k_copy32, dim3(grid), dim3(256), 0, st, (uint32_t *) dst, (const uint32_t *) src, nwords);
+  return hipGetLastError();
+}
+
+hipError_t launch_msv_scores(const DevProfile &p, const DevSeqs &s, const int32_t *slots, int64_t n,
+                             const int16_t *xj, float *sc, int *st, hipStream_t stream)
+{
+  if (n <= 0) return hipSuccess;
+  unsigned grid = (unsigned) ((n + 255) / 256); if (grid > 65536) grid = 65536;
+  hipLaunchKernelGGL(k_msv_scores, dim3(grid), dim3(256), 0, stream, p, s.len, slots, n, xj, sc, st);
   return hipGetLastError();
 }
 

@@ -63,6 +63,20 @@ struct Work {
 };
 static Work g_work;
 
+// Scratch of p7_gpu_MSVFilterBatchNoAlloc, for callers that cannot allocate:
+// per database slot the ungapped pass's maximum and the final xJ (2 + 2
+// bytes), per listed target its slot, the J-state list, the score and the
+// status (4 x 4 bytes), and a block of 16 counters. The array is a host-side
+// name only: kernels and copies take the address hipGetSymbolAddress gives on
+// each device (g_scratch, taken by p7_gpu_Init), never the name itself.
+#define P7G_NOALLOC_MAXSEQ ((int64_t) 1 << 20)
+#define NOALLOC_PART(bytes) ((((size_t) (bytes)) + 255) & ~(size_t) 255)
+#define P7G_NOALLOC_BYTES (2 * NOALLOC_PART(2 * P7G_NOALLOC_MAXSEQ) + 4 * NOALLOC_PART(4 * P7G_NOALLOC_MAXSEQ) + 256)
+static __device__ __align__(256) unsigned char scratch[P7G_NOALLOC_BYTES];
+#define P7G_MAXDEV 64
+static void *g_scratch[P7G_MAXDEV];               // scratch's address on each device
+static int32_t g_na_slots[P7G_NOALLOC_MAXSEQ];   // host staging of the slot list (under g_lock)
+
 #define HIPCHK(call) do { hipError_t e_ = (call); if (e_ != hipSuccess) { \
       p7g_set_error("HIP error %s at %s:%d (%s)", hipGetErrorString(e_), __FILE__, __LINE__, #call); \
       return p7G_EDEVICE; } } while (0)
@@ -76,6 +90,8 @@ p7_gpu_Init(int device)
   HIPCHK(hipGetDeviceCount(&n));
   if (device < 0 || device >= n) { p7g_set_error("no HIP device %d (%d present)", device, n); return p7G_EINVAL; }
   HIPCHK(hipSetDevice(device));
+  if (device < P7G_MAXDEV && ! g_scratch[device])
+    HIPCHK(hipGetSymbolAddress(&g_scratch[device], HIP_SYMBOL(scratch)));
   if (g_stream) { (void) hipStreamDestroy(g_stream); g_stream = nullptr; }
   HIPCHK(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking));
   g_work.st = g_stream;
@@ -1002,6 +1018,70 @@ p7_gpu_MSVFilterBatch(const P7_GPU_OPROFILE *om, const P7_GPU_SEQDB *db, const i
     else { ret_sc[t] = msv_score_from_xj(om, v, lcf.tjb_b[db->h_len[slot]]); if (ret_status) ret_status[t] = p7G_OK; }
   }
   p7g_lencfg_free(&lcf);
+  return p7G_OK;
+}
+
+// p7_gpu_MSVFilterBatch without allocating (see p7_gpu.h): the same kernels
+// over a slot list, with every buffer in this device's part of `scratch`.
+// The profile's device copy must already cover the database's lengths.
+extern "C" int
+p7_gpu_MSVFilterBatchNoAlloc(const P7_GPU_OPROFILE *om, const P7_GPU_SEQDB *db, const int64_t *idx, int64_t n,
+                             float *ret_sc, int *ret_status)
+{
+  std::lock_guard<std::mutex> lk(g_lock);
+  if (! g_stream || g_device < 0 || g_device >= P7G_MAXDEV || ! g_scratch[g_device]) {
+    p7g_set_error("p7_gpu_MSVFilterBatchNoAlloc: call p7_gpu_Init first");
+    return p7G_EINVAL;
+  }
+  if (! om || ! db || ! ret_sc) { p7g_set_error("p7_gpu_MSVFilterBatchNoAlloc: NULL argument"); return p7G_EINVAL; }
+  if (db->maxcode >= om->abc->Kp) { p7g_set_error("database holds residue codes outside the model alphabet"); return p7G_EINCOMPAT; }
+  if (idx == nullptr) n = db->nseq;
+  if (n < 0 || n > P7G_NOALLOC_MAXSEQ || db->nseq > P7G_NOALLOC_MAXSEQ) {
+    p7g_set_error("p7_gpu_MSVFilterBatchNoAlloc: %lld targets of a %lld-sequence database exceed its scratch "
+                  "(at most %lld of each)", (long long) n, (long long) db->nseq, (long long) P7G_NOALLOC_MAXSEQ);
+    return p7G_EINVAL;
+  }
+  const DevProfile *p = nullptr;
+  {
+    std::lock_guard<std::mutex> dl(g_devlock);
+    if (om->dev && om->dev->Lmax >= db->Lmax) p = &om->dev->p;
+  }
+  if (! p) {
+    p7g_set_error("p7_gpu_MSVFilterBatchNoAlloc: the profile has no device copy for this database's lengths "
+                  "(any other batched call or p7_gpu_Pipeline with this profile and database makes it)");
+    return p7G_EINVAL;
+  }
+  if (p->ssv_nchunk) {
+    p7g_set_error("p7_gpu_MSVFilterBatchNoAlloc: models longer than one LDS table (M = %d) need boundary columns "
+                  "sized by the database; use p7_gpu_MSVFilterBatch", p->M);
+    return p7G_EINVAL;
+  }
+  if (n == 0) return p7G_OK;
+  for (int64_t t = 0; t < n; t++) {
+    const int64_t i = idx ? idx[t] : t;
+    if (i < 0 || i >= db->nseq) { p7g_set_error("target index %lld out of range", (long long) i); return p7G_EINVAL; }
+    g_na_slots[t] = db->h_inv[i];
+  }
+  char *m = (char *) g_scratch[g_device];
+  uint16_t *zmax = (uint16_t *) m;                m += NOALLOC_PART(2 * P7G_NOALLOC_MAXSEQ);
+  int16_t  *xj   = (int16_t *) m;                 m += NOALLOC_PART(2 * P7G_NOALLOC_MAXSEQ);
+  int32_t  *slots = (int32_t *) m;                m += NOALLOC_PART(4 * P7G_NOALLOC_MAXSEQ);
+  int32_t  *full  = (int32_t *) m;                m += NOALLOC_PART(4 * P7G_NOALLOC_MAXSEQ);
+  float    *sc    = (float *) m;                  m += NOALLOC_PART(4 * P7G_NOALLOC_MAXSEQ);
+  int      *st    = (int *) m;                    m += NOALLOC_PART(4 * P7G_NOALLOC_MAXSEQ);
+  int32_t  *counts = (int32_t *) m;
+  DevSeqs s = seqs_of(db);
+  LenCfg lc = lencfg_per_target();
+  // list counts and queue cursors start at zero (the last call left its own)
+  HIPCHK(launch_fill32(counts, 0u, 16, g_stream));
+  HIPCHK(hipMemcpyAsync(slots, g_na_slots, sizeof(int32_t) * (size_t) n, hipMemcpyHostToDevice, g_stream));
+  HIPCHK(launch_ssv_chosen(*p, s, slots, n, zmax, &counts[P7G_Q_SSV], nullptr, 0, g_stream));
+  HIPCHK(launch_msv_classify(*p, s, slots, n, zmax, xj, full, &counts[0], lc, g_stream));
+  HIPCHK(launch_msv_full(*p, s, full, &counts[0], n, xj, lc, g_stream));
+  HIPCHK(launch_msv_scores(*p, s, slots, n, xj, sc, st, g_stream));
+  HIPCHK(hipMemcpyAsync(ret_sc, sc, sizeof(float) * (size_t) n, hipMemcpyDeviceToHost, g_stream));
+  if (ret_status) HIPCHK(hipMemcpyAsync(ret_status, st, sizeof(int) * (size_t) n, hipMemcpyDeviceToHost, g_stream));
+  HIPCHK(hipStreamSynchronize(g_stream));
   return p7G_OK;
 }
 

@@ -97,6 +97,7 @@ def _load():
         "p7_gpu_ForwardParserBatch": (i, [vp, vp, pi64, i64, pf, pi]),
         "p7_gpu_BackwardParserBatch": (i, [vp, vp, pi64, i64, pf, pi]),
         "p7_gpu_FilterScoreBatch": (i, [vp, vp, pi64, i64, pf]),
+        "p7_gpu_MSVFilterBatchNoAlloc": (i, [vp, vp, pi64, i64, pf, pi]),
         "p7_gpu_pipeline_Create": (vp, [d, d, d, i, i]), "p7_gpu_pipeline_Destroy": (None, [vp]),
         "p7_gpu_Pipeline": (i, [vp, vp, vp, POINTER(Hits)]),
         "p7_gpu_pipeline_GetStats": (i, [vp, POINTER(Stats)]),
@@ -470,6 +471,12 @@ def _batch(fn, om, db, idx, name, with_status=True):
 
 def msv_batch(om, db, idx=None):
     return _batch(lib.p7_gpu_MSVFilterBatch, om, db, idx, "MSVFilterBatch")
+
+
+def msv_batch_noalloc(om, db, idx=None):
+    """p7_gpu_MSVFilterBatchNoAlloc: MSVFilterBatch with the library's static
+    device scratch (no allocation; the profile's device copy must exist)."""
+    return _batch(lib.p7_gpu_MSVFilterBatchNoAlloc, om, db, idx, "MSVFilterBatchNoAlloc")
 
 
 def viterbi_batch(om, db, idx=None):

@@ -238,6 +238,25 @@ int   p7_gpu_BackwardParserBatch(const P7_GPU_OPROFILE *om, const P7_GPU_SEQDB *
                                  float *ret_sc, int *ret_status);
 int   p7_gpu_FilterScoreBatch  (const P7_GPU_OPROFILE *om, const P7_GPU_SEQDB *db, const int64_t *idx, int64_t n,
                                 float *ret_sc);
+/* p7_gpu_MSVFilterBatch for callers that cannot allocate memory (code under
+ * HIP graph capture, serving loops, harnesses that forbid hipMalloc): no
+ * counterpart in the reference. Same arguments and results, bit for bit; it
+ * calls no hipMalloc/hipFree and no allocating routine. Its buffers live in
+ * a static device array of the library (about 20 MiB per device, held until
+ * the process ends), so:
+ *   - db and n are limited to 1,048,576 sequences each (P7G_NOALLOC_MAXSEQ);
+ *     larger ones are p7G_EINVAL;
+ *   - p7_gpu_Init must have run on the device, and the profile's device copy
+ *     must already cover db's lengths (any earlier batched call or
+ *     p7_gpu_Pipeline with this profile and database makes it): else p7G_EINVAL;
+ *   - models longer than one LDS table (M > 2432, the chunked ungapped pass)
+ *     are p7G_EINVAL;
+ *   - every call on one device shares that device's array (it keeps the last
+ *     call's contents; each call clears its counters first). The library
+ *     serialises these calls on one lock and stream; calls that overlapped on
+ *     one device would race on the array. */
+int   p7_gpu_MSVFilterBatchNoAlloc(const P7_GPU_OPROFILE *om, const P7_GPU_SEQDB *db, const int64_t *idx, int64_t n,
+                                   float *ret_sc, int *ret_status);
 
 /* ---- pipeline (p7_pipeline.c:99, :666-731) -------------------------------- */
 /* F1/F2/F3 are P-value thresholds (defaults 0.02, 1e-3, 1e-5; p7_pipeline.c:210-212);
