@@ -57,7 +57,7 @@ def cpu_baseline(om, db, M, gpu_stage, gpu_usc=None, seconds_target=15.0, max_th
     """The oracle's pipeline on a bounded sample, on the host cores; GCUPS.
     The sample's stages are also compared with the device run's stages
     (gpu_stage, in database order): oracle_mismatches counts the targets whose
-    stage differs, usc_mismatches those whose MSV score differs."""
+    stage differs, oracle_usc_mismatches those whose MSV score differs."""
     import oracle
     t = om.tables()
     prof, keep = oracle.make_profile(t, oracle.bias_hmm(t), t["evparam"], Lmax=int(db.lengths.max()) + 1)
@@ -167,7 +167,7 @@ def serving_leg(p7gpu, synth, nseq, nq, rate, nslots, seed=11, tune=True):
             "identical_pass_sets": same}
 
 
-def batched_leg(p7gpu, synth, nprof, nseq, reps=3, seed=42):
+def batched_leg(p7gpu, synth, nprof, nseq, reps=3, seed=42, tune=True):
     """Many profiles of uneven M against one database with a long-sequence tail:
     the tuner picks each length class's ungapped-pass shape, then the same
     profiles run one by one through a pipeline (plain per-query drop-in) and
@@ -194,7 +194,8 @@ def batched_leg(p7gpu, synth, nprof, nseq, reps=3, seed=42):
     cells = float(sum(Ms)) * db.nres
 
     t0 = time.perf_counter()
-    p7gpu.autotune(oms, sdb)
+    if tune:
+        p7gpu.autotune(oms, sdb)
     t_tune = time.perf_counter() - t0
     shapes = {}
     for M in Ms:
@@ -367,6 +368,8 @@ def main():
     ap.add_argument("--batch-profiles", type=int, default=64, help="profiles in the batched leg (0: skip it)")
     ap.add_argument("--batch-nseq", type=int, default=500_000, help="database sequences of the batched leg")
     ap.add_argument("--no-tune", action="store_true", help="skip the tuner (keep the built-in ungapped-pass shape)")
+    ap.add_argument("--only-batched", action="store_true",
+                    help="only run the batched leg and print its JSON line (for a kernel trace of that leg alone)")
     ap.add_argument("--precision", action="store_true",
                     help="only run the precision stress (M = 2000..5000, L = 2k..35k) and print its JSON line")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -379,6 +382,12 @@ def main():
         from biobench2_amd import p7gpu, synth
         p7gpu.init(0)
         print(json.dumps({"precision_stress": precision_leg(p7gpu, synth)}), flush=True)
+        return
+    if args.only_batched:
+        from biobench2_amd import p7gpu, synth
+        p7gpu.init(0)
+        print(json.dumps({"batched": batched_leg(p7gpu, synth, args.batch_profiles, args.batch_nseq,
+                                                 tune=not args.no_tune)}), flush=True)
         return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: --gpus {args.gpus} is not supported by this build "
@@ -463,7 +472,7 @@ def main():
         line["domain_decoding"] = domain_leg(p7gpu, om, sdb, args.domain_steps, ms_per_step)
     if args.batch_profiles > 0:
         log("batched leg ...")
-        line["batched"] = batched_leg(p7gpu, synth, args.batch_profiles, args.batch_nseq)
+        line["batched"] = batched_leg(p7gpu, synth, args.batch_profiles, args.batch_nseq, tune=not args.no_tune)
     if args.serve_queries > 0:
         log("serving leg ...")
         line["serving"] = serving_leg(p7gpu, synth, args.serve_nseq, args.serve_queries, args.serve_rate,

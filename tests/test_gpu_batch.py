@@ -167,3 +167,40 @@ def test_tuner_measures_keeps_results_and_round_trips(setup, tmp_path):
         assert e.value.status == p7gpu.EFORMAT
     finally:
         p7gpu.tune_clear()
+
+
+def test_batch_with_a_long_tail_against_the_oracle(gpu, tmp_path):
+    """The batched configuration's shape at test size: 16 calibrated profiles
+    with M log-uniform in [50, 2000] (synth.batch_family, the generator of the
+    batched benchmark leg) against a database whose 2% tail has L in
+    [5000, 35000], run as one batch launched directly and from the captured
+    graph. Every profile's stages and MSV/bias/Viterbi scores equal the
+    oracle's pipeline bit for bit, Forward/Backward within 1e-4 nats
+    (p7_pipeline.c:687-735 per profile)."""
+    import oracle
+    paths = synth.batch_family(gpu, str(tmp_path), nprof=16, seed=23)
+    oms = [gpu.OProfile.from_file(p) for p in paths]
+    hmms = [synth.read_hmm_probs(p) for p in paths[:6]]
+    db = synth.make_db(5000, hmms, frac_homolog=0.02, seed_db=24, seed_plant=25, long_frac=0.02,
+                       long_range=(5000, 35000))
+    assert (db.lengths >= 5000).sum() == 100
+    sdb = gpu.SeqDB(db.residues, db.offsets, db.lengths)
+    pli = gpu.Pipeline()
+    runs = [pli.run_batch(oms, sdb, graph=g)[0] for g in (False, True)]
+    assert pli.batch_graphed
+    reached = np.zeros(5, int)
+    for j, om in enumerate(oms):
+        t = om.tables()
+        prof, keep = oracle.make_profile(t, oracle.bias_hmm(t), t["evparam"], Lmax=int(db.lengths.max()) + 1)
+        ref = oracle.pipeline_db(prof, db.residues, db.offsets, db.lengths, nthreads=8)
+        for got in runs:
+            assert np.array_equal(got[j]["stage"], ref["stage"].astype(np.uint8)), (j, t["M"])
+            for k in ("nullsc", "usc", "filtersc", "vfsc"):
+                assert np.array_equal(got[j][k], ref[k], equal_nan=True), (j, k)
+            for k in ("fwdsc", "bcksc"):
+                m = ~np.isnan(ref[k])
+                assert np.array_equal(np.isnan(got[j][k]), ~m), (j, k)
+                if m.any():
+                    assert np.max(np.abs(got[j][k][m] - ref[k][m])) <= 1e-4, (j, k)
+        reached += np.bincount(ref["stage"], minlength=5)
+    assert (reached[1:] > 0).all(), reached

@@ -93,12 +93,13 @@ def test_forward_backward_within_1e4_nats(gpu, targets, path, which):
         assert st2 == bst[i] == 0
         err_f = max(err_f, abs(sc - fsc[i]))
         err_b = max(err_b, abs(sc2 - bsc[i]))
-        if L <= 1000:
-            om.gm.reconfig_length(L)
-            tsc, rsc, xsc = om.gm.scores()
-            err64 = max(err64, abs(fsc[i] - oracle.forward_log64(s, tsc, rsc, xsc)))
+        om.gm.reconfig_length(L)
+        tsc, rsc, xsc = om.gm.scores()
+        err64 = max(err64, abs(fsc[i] - oracle.forward_log64(s, tsc, rsc, xsc)))
     assert err_f <= 1e-4 and err_b <= 1e-4, (err_f, err_b)
-    assert err64 <= 1e-3, err64            # fp32 vs the FP64 log-space Forward (Cephes odds, sum order)
+    # fp32 against the FP64 log-space Forward over every target (L up to 9000):
+    # the oracle's fp32 restatement measures at most 9.4e-6 nats on this set
+    assert err64 <= 1e-4, err64
 
 
 @pytest.mark.parametrize("path,which", MODELS[:4] + MODELS[9:10])

@@ -102,7 +102,9 @@ def test_large_forward_backward_within_1e4_nats(large, M):
             tsc, rsc, xsc = om.gm.scores()
             err64 = max(err64, abs(fsc[i] - oracle.forward_log64(s, tsc, rsc, xsc)))
     assert err_f <= 1e-4 and err_b <= 1e-4, (M, err_f, err_b)
-    assert err64 <= 2e-3, (M, err64)
+    # against the FP64 log-space Forward (targets up to 3000 residues): measured
+    # at most 6.8e-6 nats for M = 2000-5000 in the precision stress
+    assert err64 <= 1e-4, (M, err64)
 
 
 @pytest.mark.parametrize("M", [2600, 5000])
@@ -150,3 +152,40 @@ def test_large_models_in_a_batch(large):
                 assert np.array_equal(np.isnan(x), np.isnan(y)) if x.dtype != np.uint8 else True, k
                 mm = ~np.isnan(x) if x.dtype != np.uint8 else slice(None)
                 assert np.array_equal(x[mm], y[mm]), (j, k)
+
+
+def test_large_targets_of_24k_and_35k_residues(large):
+    """The precision stress's longest targets against its largest model
+    (M = 5000): a 24,000-residue target with an indel-rich homolog inside and a
+    35,000-residue background target. MSV and Viterbi bit-exact; Forward and
+    Backward within 1e-4 nats of the oracle, with rows rescaled along the
+    homolog (31 of them, fwdback.c:419-437) and Backward reusing Forward's
+    scale factors over 24,000 rows (:661-664)."""
+    path, hh, _, _ = large[5000]
+    om = p7gpu.OProfile.from_file(path)
+    t = om.tables()
+    rng = np.random.default_rng(35)
+    core = synth.emit_homologs(hh, 1, rng, indel_rich=True)[0]
+    s1 = synth.iid_residues(24000, rng)
+    s1[9000:9000 + core.size] = core
+    seqs = [s1, synth.iid_residues(35000, rng)]
+    db = p7gpu.SeqDB.from_list(seqs)
+    lm = oracle.LengthModel(t, 35001)
+    msc, mst = p7gpu.msv_batch(om, db)
+    vsc, vst = p7gpu.viterbi_batch(om, db)
+    fsc, fst = p7gpu.forward_batch(om, db)
+    bsc, bst = p7gpu.backward_batch(om, db)
+    for i, s in enumerate(seqs):
+        L = len(s)
+        st, sc, _ = oracle.msv_filter(s, t, tjb=int(lm.tjb[L]))
+        assert st == mst[i] and same(sc, msc[i]), ("msv", i, st, sc, mst[i], msc[i])
+        xw, xf = per_target_config(t, L, lm)
+        st, sc, _ = oracle.vit_filter(s, t, xw)
+        assert st == vst[i] and same(sc, vsc[i]), ("vit", i, st, sc, vst[i], vsc[i])
+        st, sc, xmx, _ = oracle.fwd_parser(s, t, xf)
+        assert st == fst[i] == 0 and abs(sc - fsc[i]) <= 1e-4, ("fwd", i, sc, fsc[i])
+        if i == 0:
+            assert np.sum(xmx[:, 5] != 1.0) > 10        # rows rescaled along the homolog
+        st, sc, _, _ = oracle.bck_parser(s, t, xmx, xf)
+        assert st == bst[i] == 0 and abs(sc - bsc[i]) <= 1e-4, ("bck", i, sc, bsc[i])
+    assert fsc[0] > fsc[1] + 50.0                       # the homolog is found inside the long target

@@ -95,3 +95,40 @@ def test_calibration_with_the_library_equals_the_oracle_calibration(gpu):
     assert abs(got[0] - ref[0]) < 1e-9 and abs(got[2] - ref[2]) < 1e-9, (got, ref)
     assert got[1] == ref[1] and got[3] == ref[3] and got[5] == ref[5]
     assert abs(got[4] - ref[4]) < 1e-2, (got[4], ref[4])
+
+
+def test_served_queries_against_the_oracle(gpu, tmp_path):
+    """The serving configuration's shape at test size: a resident database with
+    planted homologs of a hidden family set (synth.serving_family, the
+    generator of the serving benchmark leg; 5% here against the leg's 0.5%, so
+    that a 20k-sequence database holds hits), queries drawn from the set plus
+    decoys, several in flight on three slots. Each served query's stages and
+    MSV/bias/Viterbi scores equal the oracle's pipeline bit for bit,
+    Forward/Backward within 1e-4 nats."""
+    import oracle
+    fam, qpaths, dpaths = synth.serving_family(gpu, str(tmp_path), nfam=40, nquery=5, ndecoy=2, seed=31)
+    db = synth.make_db(20000, fam, frac_homolog=0.05, seed_db=32, seed_plant=33)
+    sdb = gpu.SeqDB(db.residues, db.offsets, db.lengths)
+    oms = [gpu.OProfile.from_file(p) for p in qpaths + dpaths]
+    srv = p7gpu.Server(sdb, nslots=3)
+    order = [0, 1, 2, 3, 4, 5, 6, 2]
+    tickets = [(i, srv.submit(oms[i])) for i in order]
+    refs = {}
+    for i, tk in tickets:
+        out, st, lat = srv.wait(tk)
+        if i not in refs:
+            t = oms[i].tables()
+            prof, keep = oracle.make_profile(t, oracle.bias_hmm(t), t["evparam"], Lmax=int(db.lengths.max()) + 1)
+            refs[i] = oracle.pipeline_db(prof, db.residues, db.offsets, db.lengths, nthreads=8)
+        ref = refs[i]
+        assert np.array_equal(out["stage"], ref["stage"].astype(np.uint8)), i
+        for k in ("nullsc", "usc", "filtersc", "vfsc"):
+            assert np.array_equal(out[k], ref[k], equal_nan=True), (i, k)
+        for k in ("fwdsc", "bcksc"):
+            m = ~np.isnan(ref[k])
+            assert np.array_equal(np.isnan(out[k]), ~m), (i, k)
+            if m.any():
+                assert np.max(np.abs(out[k][m] - ref[k][m])) <= 1e-4, (i, k)
+        assert st["n_past_fwd"] == int((ref["stage"] == 4).sum())
+    srv.close()
+    assert sum(int((r["stage"] == 4).sum()) for r in refs.values()) > 0
