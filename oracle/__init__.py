@@ -66,6 +66,7 @@ lib.orc_brute_force.argtypes = [pu8, c_int, c_int, pf, pf, pf, pd, pd]
 lib.orc_brute_posterior.argtypes = [pu8, c_int, c_int, pf, pf, pf, pd, pd, pd]
 lib.orc_brute_emitters.argtypes = [pu8, c_int, c_int, pf, pf, pf, pd]
 lib.orc_null2_envelope.argtypes = [pu8, c_int, c_int, c_int, c_int, pf, pf, pf, POINTER(c_uint32), pi, pf, pf, pf]
+lib.orc_oa_envelope.argtypes = [pu8, c_int, c_int, c_int, pf, pf, pf, pf, pi, pi, pi, pi]
 lib.orc_domain_decoding.argtypes = [c_int, pf, pf, c_int, c_float, c_float, c_float, pf, pf, pf]
 lib.orc_domain_regions.argtypes = [c_int, pf, pf, pf, c_float, c_float, c_float, pi, pi, pi, c_int]
 lib.orc_gumbel_surv.argtypes = [c_double, c_double, c_double]
@@ -297,6 +298,23 @@ def envelope_score(residues, t, i, j):
     i..j (1-based) with the profile in unihit mode, length-configured to the whole target."""
     st, sc, _, _ = fwd_parser(np.asarray(residues)[i - 1:j], t, unihit_xf(t, len(residues)))
     return st, sc
+
+
+def oa_envelope(residues, t, i, j):
+    """(status, oasc, hmmfrom, hmmto, alifrom, alito) of rescore_isolated_domain's optimal
+    accuracy alignment of residues i..j (1-based) in unihit mode, length-configured to the
+    whole target (p7_domaindef.c:841-855; impl_sse/optacc.c): the OA score and the model and
+    sequence coordinates (in the whole target) of the trace's first and last M states."""
+    env = np.ascontiguousarray(np.asarray(residues)[i - 1:j], np.uint8)
+    dsq = dsq_of(env)
+    xf = np.ascontiguousarray(unihit_xf(t, len(residues)), np.float32).reshape(8)
+    rfv = np.ascontiguousarray(t["rfv"], np.float32)
+    tfv = np.ascontiguousarray(t["tfv"], np.float32)
+    oasc, kf, kt, i0, i1 = c_float(), c_int(), c_int(), c_int(), c_int()
+    st = lib.orc_oa_envelope(_p(dsq, c_uint8), env.size, t["M"], t["Kp"], _p(rfv, c_float), _p(tfv, c_float),
+                             _p(xf, c_float), ctypes.byref(oasc), ctypes.byref(kf), ctypes.byref(kt),
+                             ctypes.byref(i0), ctypes.byref(i1))
+    return st, oasc.value, kf.value, kt.value, i0.value + i - 1, i1.value + i - 1
 
 
 def domain_regions(btot, etot, mocc, rt1=RT1, rt2=RT2, rt3=RT3):

@@ -516,6 +516,173 @@ done:
 }
 
 /*****************************************************************
+ * Optimal accuracy alignment of an envelope: p7_Forward, p7_Backward and
+ * p7_Decoding in unihit mode (as rescore_isolated_domain runs them,
+ * p7_domaindef.c:841-847), then p7_OptimalAccuracy and p7_OATrace
+ * (impl_sse/optacc.c:60-175, 227-423), reduced to what the domain's
+ * alignment coordinates need: the OA score (expected number of correctly
+ * aligned residues, dom->oasc) and the model and sequence positions of the
+ * first and last M states of the trace (p7_alidisplay_Create's hmmfrom/hmmto/
+ * sqfrom/sqto, p7_alidisplay.c:74-145).
+ *
+ * The fill follows the SSE code's values exactly: a predecessor behind a
+ * transition of probability 0 contributes +0.0 (the _mm_and_ps mask), not
+ * -infinity; D(1) is -infinity; the padding cells of the striped rows (model
+ * positions past M, 4Q > M) hold 0.0 and take part in E's maximum. The
+ * traceback masks with -infinity as select_m/d/i do, breaks ties in the
+ * reference's orders (M I D B by first maximum; M over D, M over I; E over
+ * C and J; J over N), and select_e scans row i in the striped order
+ * (k = r*Q + q + 1, q outer), an M replacing on >= and a D on >.
+ * Coordinates are 1-based within the envelope. Returns 16 if the decoding
+ * overflows (eslERANGE), 1 if the traceback fails.
+ *****************************************************************/
+/* _mm_max_ps(a, b): a if a > b, else b */
+static inline float oa_max(float a, float b) { return a > b ? a : b; }
+
+int
+orc_oa_envelope(const uint8_t *dsq, int L, int M, int Kp, const float *rfv, const float *tfv, const float *xf,
+                float *ret_oasc, int *ret_kfrom, int *ret_kto, int *ret_ifrom, int *ret_ito)
+{
+  enum { OS = 0, ON, OB, OM, OI, OD, OE, OJ, OC };
+  float *fx = malloc(sizeof(float) * 6 * (size_t) (L+1)), *bx = malloc(sizeof(float) * 6 * (size_t) (L+1));
+  float *fm = malloc(sizeof(float) * 2 * (size_t) M * (L+1)), *bm = malloc(sizeof(float) * 2 * (size_t) M * (L+1));
+  float *pp = calloc(2 * (size_t) M * (L+1), sizeof(float));          /* [i][M_1..M_M, I_1..I_M] */
+  float *px = calloc(3 * (size_t) (L+1), sizeof(float));              /* [i][N, J, C]            */
+  float *oa = malloc(sizeof(float) * 3 * (size_t) M * (L+1));          /* [i][M.., I.., D..]      */
+  float *ox = malloc(sizeof(float) * 5 * (size_t) (L+1));              /* [i][E, N, J, B, C]      */
+  float sc;
+  int own = 0, status = 0, Q = (M - 1) / 4 + 1;
+  if (Q < 2) Q = 2;
+  const int pad = (4 * Q > M);
+  const float NI = -INFINITY;
+  if (! fx || ! bx || ! fm || ! bm || ! pp || ! px || ! oa || ! ox || L < 1) { status = 5; goto done; }
+  if ((status = fwd_engine(dsq, L, M, Kp, rfv, tfv, xf, fx, NULL, &sc, fm)) != 0) goto done;
+  if ((status = bck_engine(dsq, L, M, Kp, rfv, tfv, xf, fx, bx, &own, &sc, bm)) != 0) goto done;
+  {
+    float sp = 1.0 / bx[X_N];
+    for (int i = 1; i <= L; i++) {
+      const float totr = sp * fx[i*6 + X_SC];
+      const float *f = fm + (size_t) i * 2 * M, *b = bm + (size_t) i * 2 * M;
+      float *o = pp + (size_t) i * 2 * M;
+      for (int c = 0; c < 2 * M; c++) o[c] = (f[c] * b[c]) * totr;
+      px[i*3 + 0] = fx[(i-1)*6 + X_N] * bx[i*6 + X_N] * xf[XO(XN,LOOP)] * sp;
+      px[i*3 + 1] = fx[(i-1)*6 + X_J] * bx[i*6 + X_J] * xf[XO(XJ,LOOP)] * sp;
+      px[i*3 + 2] = fx[(i-1)*6 + X_C] * bx[i*6 + X_C] * xf[XO(XC,LOOP)] * sp;
+      if (own) sp *= fx[i*6 + X_SC] / bx[i*6 + X_SC];
+    }
+    if (isinf(sp)) { status = 16; goto done; }
+  }
+#define T(t, k)   tfv[(size_t) (t) * M + (k) - 1]
+#define MK(x, t, k) ((T(t, k) > 0.0f) ? (x) : 0.0f)
+#define OAM(i, k) oa[(size_t) (i) * 3 * M + (k) - 1]
+#define OAI(i, k) oa[(size_t) (i) * 3 * M + M + (k) - 1]
+#define OAD(i, k) oa[(size_t) (i) * 3 * M + 2 * M + (k) - 1]
+#define OX(i, s)  ox[(size_t) (i) * 5 + (s)]
+  for (int k = 1; k <= M; k++) OAM(0, k) = OAI(0, k) = OAD(0, k) = NI;
+  OX(0, 0) = NI; OX(0, 1) = 0.0f; OX(0, 2) = NI; OX(0, 3) = 0.0f; OX(0, 4) = NI;
+  for (int i = 1; i <= L; i++) {
+    const float *p = pp + (size_t) i * 2 * M;
+    float xe = pad ? 0.0f : NI;
+    for (int k = 1; k <= M; k++) {
+      const float mp = (k > 1) ? OAM(i-1, k-1) : NI, ip = (k > 1) ? OAI(i-1, k-1) : NI, dp = (k > 1) ? OAD(i-1, k-1) : NI;
+      float v = MK(OX(i-1, 3), T_BM, k);
+      v = oa_max(v, MK(mp, T_MM, k));
+      v = oa_max(v, MK(ip, T_IM, k));
+      v = oa_max(v, MK(dp, T_DM, k));
+      OAM(i, k) = v + p[k-1];
+      OAI(i, k) = oa_max(MK(OAM(i-1, k), T_MI, k), MK(OAI(i-1, k), T_II, k)) + p[M + k-1];
+    }
+    OAD(i, 1) = NI;
+    for (int k = 2; k <= M; k++) OAD(i, k) = oa_max(MK(OAM(i, k-1), T_MD, k-1), MK(OAD(i, k-1), T_DD, k-1));
+    for (int k = 1; k <= M; k++) xe = oa_max(xe, oa_max(OAM(i, k), OAD(i, k)));
+    OX(i, 0) = xe;
+    float t1 = (xf[XO(XJ,LOOP)] == 0.0f) ? 0.0f : OX(i-1, 2) + px[i*3 + 1];
+    float t2 = (xf[XO(XE,LOOP)] == 0.0f) ? 0.0f : xe;
+    OX(i, 2) = oa_max(t1, t2);
+    t1 = (xf[XO(XC,LOOP)] == 0.0f) ? 0.0f : OX(i-1, 4) + px[i*3 + 2];
+    t2 = (xf[XO(XE,MOVE)] == 0.0f) ? 0.0f : xe;
+    OX(i, 4) = oa_max(t1, t2);
+    OX(i, 1) = (xf[XO(XN,LOOP)] == 0.0f) ? 0.0f : OX(i-1, 1) + px[i*3 + 0];
+    t1 = (xf[XO(XN,MOVE)] == 0.0f) ? 0.0f : OX(i, 1);
+    t2 = (xf[XO(XJ,MOVE)] == 0.0f) ? 0.0f : OX(i, 2);
+    OX(i, 3) = oa_max(t1, t2);
+  }
+  *ret_oasc = OX(L, 4);
+  {
+    int i = L, k = 0, s0 = OC, s1, nm = 0, guard = 0;
+#define ZM(x, t, k) ((T(t, k) == 0.0f) ? NI : (x))
+    while (s0 != OS) {
+      if (++guard > 4 * (L + M + 4)) { status = 1; goto done; }
+      switch (s0) {
+      case OM: {
+        const float path[4] = { ZM((k > 1) ? OAM(i-1, k-1) : 0.0f, T_MM, k), ZM((k > 1) ? OAI(i-1, k-1) : 0.0f, T_IM, k),
+                                ZM((k > 1) ? OAD(i-1, k-1) : 0.0f, T_DM, k), ZM(OX(i-1, 3), T_BM, k) };
+        const int st[4] = { OM, OI, OD, OB };
+        int best = 0;
+        for (int a = 1; a < 4; a++) if (path[a] > path[best]) best = a;
+        s1 = st[best]; k--; i--; break;
+      }
+      case OD: {
+        const float a0 = (k > 1) ? ZM(OAM(i, k-1), T_MD, k-1) : NI, a1 = (k > 1) ? ZM(OAD(i, k-1), T_DD, k-1) : NI;
+        s1 = (a0 >= a1) ? OM : OD; k--; break;
+      }
+      case OI: {
+        const float a0 = ZM(OAM(i-1, k), T_MI, k), a1 = ZM(OAI(i-1, k), T_II, k);
+        s1 = (a0 >= a1) ? OM : OI; i--; break;
+      }
+      case ON: s1 = (i == 0) ? OS : ON; break;
+      case OC: {
+        const float a0 = (xf[XO(XC,LOOP)] == 0.0f) ? NI : OX(i-1, 4) + px[i*3 + 2];
+        const float a1 = (xf[XO(XE,MOVE)] == 0.0f) ? NI : OX(i, 0);
+        s1 = (a0 > a1) ? OC : OE; break;
+      }
+      case OJ: {
+        const float a0 = (xf[XO(XJ,LOOP)] == 0.0f) ? NI : OX(i-1, 2) + px[i*3 + 1];
+        const float a1 = (xf[XO(XE,LOOP)] == 0.0f) ? NI : OX(i, 0);
+        s1 = (a0 > a1) ? OJ : OE; break;
+      }
+      case OE: {
+        float mx = NI;
+        int smax = -1, kmax = 0;
+        for (int q = 0; q < Q; q++) {
+          for (int r = 0; r < 4; r++) { const int kk = r*Q + q + 1; const float v = (kk <= M) ? OAM(i, kk) : 0.0f;
+                                        if (v >= mx) { mx = v; smax = OM; kmax = kk; } }
+          for (int r = 0; r < 4; r++) { const int kk = r*Q + q + 1; const float v = (kk <= M) ? OAD(i, kk) : 0.0f;
+                                        if (v > mx)  { mx = v; smax = OD; kmax = kk; } }
+        }
+        if (smax < 0 || kmax > M) { status = 1; goto done; }
+        s1 = smax; k = kmax; break;
+      }
+      case OB: {
+        const float a0 = (xf[XO(XN,MOVE)] == 0.0f) ? NI : OX(i, 1);
+        const float a1 = (xf[XO(XJ,MOVE)] == 0.0f) ? NI : OX(i, 2);
+        s1 = (a0 > a1) ? ON : OJ; break;
+      }
+      default: status = 1; goto done;
+      }
+      if (s1 == OM) {                       /* walking back: the first M is the domain's last */
+        if (nm == 0) { *ret_kto = k; *ret_ito = i; }
+        *ret_kfrom = k; *ret_ifrom = i;
+        nm++;
+      }
+      if ((s1 == ON || s1 == OJ || s1 == OC) && s1 == s0) i--;
+      s0 = s1;
+    }
+    if (nm == 0) status = 1;
+#undef ZM
+  }
+#undef T
+#undef MK
+#undef OAM
+#undef OAI
+#undef OAD
+#undef OX
+done:
+  free(fx); free(bx); free(fm); free(bm); free(pp); free(px); free(oa); free(ox);
+  return status;
+}
+
+/*****************************************************************
  * Null model and bias-composition filter
  *****************************************************************/
 float

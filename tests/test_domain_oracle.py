@@ -124,3 +124,49 @@ def test_fn3_tutorial_envelope_scores():
         bits = (raw - nullsc) / math.log(2.0)
         assert abs(bits - ref) <= 0.1, (i, j, bits, ref)
 
+
+
+# tutorial/fn3.out's domain table: hmmfrom, hmmto, alifrom, alito, acc of the nine domains
+FN3_ALI = [(61, 74, 396, 409, 0.85), (2, 84, 439, 520, 0.95), (13, 85, 836, 913, 0.73), (10, 36, 1209, 1235, 0.82),
+           (14, 80, 1313, 1380, 0.82), (58, 72, 1754, 1768, 0.89), (1, 85, 1799, 1890, 0.91), (6, 74, 1904, 1966, 0.90),
+           (1, 86, 1993, 2107, 0.89)]
+FN3_ENV = [(395, 411), (437, 521), (826, 914), (1203, 1259), (1304, 1386), (1739, 1769), (1799, 1891), (1901, 1976),
+           (1993, 2107)]
+
+
+def test_fn3_tutorial_optimal_accuracy_alignments():
+    """The oracle's optimal accuracy alignment (p7_OptimalAccuracy + p7_OATrace,
+    impl_sse/optacc.c, on rescore_isolated_domain's unihit Forward/Backward/
+    Decoding of each envelope) gives tutorial/fn3.out's alignment columns for
+    all nine domains: hmm from/to, ali from/to (the trace's first and last M
+    states) and acc = oasc / (1 + |env to - env from|) to its two printed
+    decimals (p7_tophits.c:1684)."""
+    (name, s), = seqio.read_uniprot(golden("7LESS_DROME"))
+    om = p7gpu.OProfile.from_file(golden("fn3.hmm"))
+    om.reconfig_length(len(s))
+    t = om.tables()
+    for (i, j), (kf, kt, a, b, acc) in zip(FN3_ENV, FN3_ALI):
+        st, oasc, gkf, gkt, ga, gb = oracle.oa_envelope(s, t, i, j)
+        assert st == 0
+        assert (gkf, gkt, ga, gb) == (kf, kt, a, b), ((i, j), (gkf, gkt, ga, gb))
+        assert "%4.2f" % (oasc / (1.0 + abs(j - i))) == "%4.2f" % acc, ((i, j), oasc)
+        assert 0.0 < oasc <= j - i + 1                 # expected number of correctly aligned residues
+
+
+def test_optimal_accuracy_on_random_envelopes_stays_inside():
+    """On random envelopes of several models the OA alignment lies inside the
+    envelope and the model, starts and ends in M states, and its score is at
+    most the envelope length."""
+    rng = np.random.default_rng(41)
+    for name, which in (("fn3.hmm", 0), ("minifam", 0), ("minifam", 2), ("RRM_1.hmm", 0)):
+        om = p7gpu.OProfile.from_file(golden(name), which)
+        hm_len = om.tables()["M"]
+        for L in (30, 120, 400):
+            s = random_seqs(rng, [L])[0]
+            om.reconfig_length(L)
+            t = om.tables()
+            i, j = 1 + L // 10, L - L // 10
+            st, oasc, kf, kt, a, b = oracle.oa_envelope(s, t, i, j)
+            assert st == 0
+            assert 1 <= kf <= kt <= hm_len and i <= a <= b <= j, (name, L, kf, kt, a, b)
+            assert 0.0 <= oasc <= j - i + 1
