@@ -20,6 +20,15 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
+# The batched and serving legs run up to sixteen stage chains (and several
+# query slots) on their own HIP streams. HIP maps streams onto
+# GPU_MAX_HW_QUEUES hardware queues per process (4 by default); streams that
+# share a queue run their kernels in queue order, so one chain's long-target
+# tail holds up the chains behind it. The runtime reads the variable when it
+# initialises, so main() sets it from --hw-queues before the library (or
+# torch) touches the GPU. 32 is the most this build asks for.
+MAX_HW_QUEUES = 32
+
 PROFILE = os.path.join(ROOT, "tests", "golden", "synth", "synth_M400.hmm")
 PEAK_PK16_TOPS = 78.6      # MI355X packed 16-bit VALU ops/s (256 CU x 64 lanes x 2 x 2.4 GHz)
 PEAK_HBM_GBS = 8000.0
@@ -370,11 +379,17 @@ def main():
     ap.add_argument("--no-tune", action="store_true", help="skip the tuner (keep the built-in ungapped-pass shape)")
     ap.add_argument("--only-batched", action="store_true",
                     help="only run the batched leg and print its JSON line (for a kernel trace of that leg alone)")
+    ap.add_argument("--only-serving", action="store_true", help="only run the serving leg and print its JSON line")
     ap.add_argument("--precision", action="store_true",
                     help="only run the precision stress (M = 2000..5000, L = 2k..35k) and print its JSON line")
+    ap.add_argument("--hw-queues", type=int, default=MAX_HW_QUEUES,
+                    help="GPU_MAX_HW_QUEUES for this process: hardware queues the HIP streams map onto")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run three plain 256 MiB device copies (to scale the FETCH_SIZE/WRITE_SIZE counters)")
     args = ap.parse_args()
+    if not 1 <= args.hw_queues <= MAX_HW_QUEUES:
+        ap.error(f"--hw-queues must be in 1..{MAX_HW_QUEUES}")
+    os.environ["GPU_MAX_HW_QUEUES"] = str(args.hw_queues)
     if args.calibrate_copy:
         calibrate_copy()
         return
@@ -387,7 +402,15 @@ def main():
         from biobench2_amd import p7gpu, synth
         p7gpu.init(0)
         print(json.dumps({"batched": batched_leg(p7gpu, synth, args.batch_profiles, args.batch_nseq,
-                                                 tune=not args.no_tune)}), flush=True)
+                                                 tune=not args.no_tune),
+                          "hw_queues": os.environ.get("GPU_MAX_HW_QUEUES")}), flush=True)
+        return
+    if args.only_serving:
+        from biobench2_amd import p7gpu, synth
+        p7gpu.init(0)
+        print(json.dumps({"serving": serving_leg(p7gpu, synth, args.serve_nseq, args.serve_queries, args.serve_rate,
+                                                 args.serve_slots, tune=not args.no_tune),
+                          "hw_queues": os.environ.get("GPU_MAX_HW_QUEUES")}), flush=True)
         return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: --gpus {args.gpus} is not supported by this build "
@@ -464,6 +487,7 @@ def main():
                      "ms_per_launch": round(ssv_ms, 4), "ops_per_cell": SSV_OPS_PER_CELL,
                      "residue_stream_GBs": round(db.nres / (ssv_ms / 1000.0) / 1e9, 2)},
         "stages": per_stage,
+        "hw_queues": os.environ.get("GPU_MAX_HW_QUEUES"),
         "pass_counts": {"msv": int((stage >= 1).sum()), "bias": int((stage >= 2).sum()),
                         "vit": int((stage >= 3).sum()), "fwd": int((stage >= 4).sum()),
                         "msv_full_rescored": st["n_msv_full"]},

@@ -139,12 +139,30 @@ struct DomBufs {
   float   *n2dc = nullptr;
   int32_t *n2st = nullptr;
   int      n2mode = 0;
+  // optimal accuracy alignment of each region (the n2mode 2 pass fills the
+  // traceback choices in its area and the OA score; k_oa_trace walks them):
+  // n2oa[k] the score, n2ali[k] = {hmmfrom, hmmto, alifrom, alito} (1-based,
+  // within the region copy; 0 where no trace)
+  float   *n2oa = nullptr;
+  int4    *n2ali = nullptr;
 };
 
-// the parts of a region's null2 area, in floats, each a multiple of 64
+// the parts of a region's null2 area, in floats, each a multiple of 64: the
+// Backward special rows, the Backward M and I rows, then the optimal accuracy
+// traceback's choices (n2_oa_words per row, words [row][w][64 lanes], four
+// bits per model position) and one word per row for the special states
 __host__ __device__ inline int64_t n2_xrows_floats(int L) { return (6 * ((int64_t) L + 1) + 63) & ~(int64_t) 63; }
+__host__ __device__ inline int n2_oa_words(int B) { return (B + 7) / 8; }
+__host__ __device__ inline int64_t n2_oa_bits_at(int L, int B) { return n2_xrows_floats(L) + (int64_t) L * 128 * B; }
+__host__ __device__ inline int64_t n2_oa_rows_at(int L, int B) { return n2_oa_bits_at(L, B) + (int64_t) L * 64 * n2_oa_words(B); }
 __host__ __device__ inline int64_t n2_area_floats(int L, int B)
-{ return n2_xrows_floats(L) + (int64_t) L * 128 * B; }
+{ return n2_oa_rows_at(L, B) + (((int64_t) L + 1 + 63) & ~(int64_t) 63); }
+// optimal accuracy traceback choices: per cell, bits 0-1 the state before M
+// (0 M, 1 I, 2 D, 3 B), bit 2 before I (1: I), bit 3 before D (1: D); per
+// row, bits 0-15 the node E leaves from, bit 16 set if from D, bit 17 C from
+// C (else E), bit 18 J from J (else E), bit 19 B from N (else J), bit 20 the
+// E choice fell on a striped padding cell (no trace)
+enum { OA_ESTATE = 16, OA_CBIT = 17, OA_JBIT = 18, OA_BBIT = 19, OA_EFAIL = 20 };
 // Per-target results of the decoding (slot order) and the region records
 // {slot, i, j, multidomain} in the order they were found.
 struct DomOut {
@@ -267,6 +285,11 @@ hipError_t launch_bck(const DevProfile &p, const DevSeqs &s, const int32_t *list
                       DevResults r, const float *scale_pool, const int64_t *scale_off, const float *fwd_xmx,
                       float *xmx_out, LenCfg lc, hipStream_t st, const DomBufs &dom = DomBufs(),
                       DevLists l = DevLists());
+// The optimal accuracy traceback of each listed region (p7_OATrace,
+// impl_sse/optacc.c:227-423) from the choices the null2 pass stored: one
+// lane per region; the alignment's first and last M states go to dom.n2ali.
+hipError_t launch_oa_trace(const DevProfile &p, const DevSeqs &s, const int32_t *list, int64_t n, const DomBufs &dom,
+                           hipStream_t st);
 // Posterior decoding of the special states and the region heuristics over the
 // Backward list (the F3 survivors), one target per thread.
 hipError_t launch_domains(const DevProfile &p, const DevSeqs &s, const int32_t *list, const int32_t *count, int64_t maxn,

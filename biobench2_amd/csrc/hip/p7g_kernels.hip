@@ -1030,6 +1030,152 @@ __device__ __forceinline__ void null2_epilogue(const DevProfile &p, const float 
   if (z == 0) { *dc_out = dc; *st_out = 0; }
 }
 
+// One row of the optimal accuracy fill (p7_OptimalAccuracy, impl_sse/
+// optacc.c:60-175) inside the null2 Forward pass, from the row's posteriors,
+// with the traceback's choices (select_m/d/i/e/c/j/b, optacc.c:288-423) made
+// here from the same values and stored for k_oa_trace: four bits per cell
+// (lane-minor words), one word per row. As in the SSE code, a predecessor
+// behind a transition of probability 0 contributes +0.0 to the fill's maximum
+// and -infinity to the traceback's choice; D(1) is -infinity; the reference's
+// striped padding cells (model positions M+1..4Q) hold 0.0 and take part in
+// E's maximum and in select_e's scan, which runs in the striped order
+// (k = r*Q + q + 1, q outer), an M replacing on >= and a D on >. Values are
+// -infinity or >= 0, so their bit patterns order as integers. The profile is
+// in unihit mode (E move 1, E loop 0; N, J, C loop floop and move fmove).
+template <int B, int NW, bool PIN>
+__device__ __forceinline__ void oa_row(const DevProfile &p, int z, int i,
+                                       const float (&tBM)[B], const float (&tMM)[B], const float (&tIM)[B],
+                                       const float (&tDM)[B], const float (&tMDS)[B], const float (&tMI)[B],
+                                       const float (&tII)[B], const float *trr, size_t rowf,
+                                       const float (&fM)[B], const float (&fI)[B], const float *cr, float totr,
+                                       float (&accM)[B], float (&accI)[B], float ppN, float ppJ, float ppC,
+                                       float floop, float fmove, float (&oaM)[B], float (&oaI)[B], float (&oaD)[B],
+                                       float &oaN, float &oaJ, float &oaB, float &oaC, uint32_t *obits, uint32_t *orow)
+{
+#define OTR(A, T, j) (PIN ? A[j] : trr[(T) * rowf + (j) * 64])
+  const float NI = -INFINITY;
+  const int M = p.M;
+  const int Q = max(2, (M - 1) / 4 + 1);
+  uint32_t nib[NW];
+#pragma unroll
+  for (int w = 0; w < NW; w++) nib[w] = 0u;
+  // row i-1's values at the lane's first position's predecessor (lane z-1's last)
+  const float mpL = dppf<WAVE_SHR1>(NI, oaM[B-1]), ipL = dppf<WAVE_SHR1>(NI, oaI[B-1]), dpL = dppf<WAVE_SHR1>(NI, oaD[B-1]);
+  // M and I in place, last position first: position j reads row i-1 at j-1 and j
+#pragma unroll
+  for (int j = B - 1; j >= 0; j--) {
+    const int k = z * B + j + 1;
+    const bool first = (k == 1);
+    // the posteriors of row i: (f * b) * (scaleproduct * scale_i), summed for null2
+    const float pm = __fmul_rn(__fmul_rn(fM[j], cr[j*64]), totr);
+    const float pi = __fmul_rn(__fmul_rn(fI[j], cr[rowf + j*64]), totr);
+    accM[j] = __fadd_rn(pm, accM[j]);
+    accI[j] = __fadd_rn(pi, accI[j]);
+    const float mp = j ? oaM[j-1] : mpL, ip = j ? oaI[j-1] : ipL, dp = j ? oaD[j-1] : dpL;
+    const bool tb = OTR(tBM, VT_BM, j) > 0.0f, tm = OTR(tMM, VT_MM, j) > 0.0f;
+    const bool ti = OTR(tIM, VT_IM, j) > 0.0f, td = OTR(tDM, VT_DM, j) > 0.0f;
+    // select_m: paths M, I, D, B; the first maximum wins; node 0 reads 0.0
+    const float p0 = tm ? (first ? 0.0f : mp) : NI, p1 = ti ? (first ? 0.0f : ip) : NI;
+    const float p2 = td ? (first ? 0.0f : dp) : NI, p3 = tb ? oaB : NI;
+    int c = 0;
+    float bv = p0;
+    if (p1 > bv) { c = 1; bv = p1; }
+    if (p2 > bv) { c = 2; bv = p2; }
+    if (p3 > bv) { c = 3; }
+    float v = tb ? oaB : 0.0f;
+    v = fmaxf(v, tm ? mp : 0.0f);
+    v = fmaxf(v, ti ? ip : 0.0f);
+    v = fmaxf(v, td ? dp : 0.0f);
+    // select_i: M over I on ties
+    const bool tmi = OTR(tMI, VT_MI, j) > 0.0f, tii = OTR(tII, VT_II, j) > 0.0f;
+    const int ci = ((tmi ? oaM[j] : NI) >= (tii ? oaI[j] : NI)) ? 0 : 1;
+    oaI[j] = __fadd_rn(fmaxf(tmi ? oaM[j] : 0.0f, tii ? oaI[j] : 0.0f), pi);
+    oaM[j] = __fadd_rn(v, pm);
+    nib[j / 8] |= (uint32_t) (c | (ci << 2)) << ((j % 8) * 4);
+  }
+  // D(k) = max(tMD(k-1) ? M(k-1) : 0, tDD(k-1) ? D(k-1) : 0), D(1) = -infinity:
+  // per position the map x -> max(c, p ? x : -inf), composed along the lane,
+  // scanned across the lanes, then applied from the carry of the lanes before
+  const float mL = dppf<WAVE_SHR1>(NI, oaM[B-1]);
+  float C = NI;
+  int P = 1;
+#pragma unroll
+  for (int j = 0; j < B; j++) {
+    const int k = z * B + j + 1;
+    const bool tmd = OTR(tMDS, VT_MDS, j) > 0.0f, tdd = trr[VT_DDS * rowf + j * 64] > 0.0f;
+    const float A = tmd ? (j ? oaM[j-1] : mL) : 0.0f;
+    const int pj = (k != 1 && tdd) ? 1 : 0;
+    const float cj = (k == 1) ? NI : (tdd ? A : fmaxf(A, 0.0f));
+    C = fmaxf(cj, pj ? C : NI);
+    P &= pj;
+  }
+#pragma unroll
+  for (int s = 1; s < 64; s <<= 1) {
+    const float cO = __shfl_up(C, s);
+    const int pO = __shfl_up(P, s);
+    if (z >= s) { C = fmaxf(C, P ? cO : NI); P &= pO; }
+  }
+  float cur = __shfl_up(C, 1);
+  if (z == 0) cur = NI;
+#pragma unroll
+  for (int j = 0; j < B; j++) {
+    const int k = z * B + j + 1;
+    const float mprev = j ? oaM[j-1] : mL;
+    const bool tmd = OTR(tMDS, VT_MDS, j) > 0.0f, tdd = trr[VT_DDS * rowf + j * 64] > 0.0f;
+    // select_d: M over D on ties; node 1 has neither
+    const float a0 = (k == 1) ? NI : (tmd ? mprev : NI), a1 = (k == 1) ? NI : (tdd ? cur : NI);
+    nib[j / 8] |= (uint32_t) ((a0 >= a1) ? 0 : 1) << ((j % 8) * 4 + 3);
+    const float A = tmd ? mprev : 0.0f;
+    const float cj = (k == 1) ? NI : (tdd ? A : fmaxf(A, 0.0f));
+    cur = fmaxf(cj, (k != 1 && tdd) ? cur : NI);
+    oaD[j] = cur;
+  }
+  // E(i): the row maximum over the model's cells (and the striped padding's 0.0)
+  float el = NI;
+#pragma unroll
+  for (int j = 0; j < B; j++) if (z * B + j + 1 <= M) el = fmaxf(el, fmaxf(oaM[j], oaD[j]));
+  const bool refpad = 4 * Q > M;
+  float V = __int_as_float(wave_max_i(__float_as_int(el)));
+  if (refpad) V = fmaxf(V, 0.0f);
+  // select_e: the last M cell of value V in the striped order, else the first D cell
+  auto key = [Q](int k) { return ((k - 1) % Q) * 4 + (k - 1) / Q; };
+  int km = -1, kd = 0x7fffffff;
+#pragma unroll
+  for (int j = 0; j < B; j++) {
+    const int k = z * B + j + 1;
+    if (k <= M && oaM[j] == V) km = max(km, key(k));
+    if (k <= M && oaD[j] == V) kd = min(kd, key(k));
+  }
+  int kmax = wave_max_i(km);
+  int fail = 0, es = 0, ek = 0;
+  if (refpad && V == 0.0f)                       // padding M cells hold V too
+    for (int k = M + 1; k <= 4 * Q; k++) if (key(k) > kmax) fail = 1;
+  if (kmax >= 0) { ek = (kmax & 3) * Q + (kmax >> 2) + 1; }
+  else {
+    const int kmin = -wave_max_i(-kd);
+    if (kmin == 0x7fffffff) fail = 1;
+    else { es = 1; ek = (kmin & 3) * Q + (kmin >> 2) + 1; }
+  }
+  // specials (unihit: E loop 0, E move 1), with select_j/c/b's choices
+  const float eLoop = 0.0f, eMove = 1.0f;
+  const float j0 = (floop == 0.0f) ? 0.0f : __fadd_rn(oaJ, ppJ), j1 = (eLoop == 0.0f) ? 0.0f : V;
+  const int bj = (((floop == 0.0f) ? NI : __fadd_rn(oaJ, ppJ)) > ((eLoop == 0.0f) ? NI : V)) ? 1 : 0;
+  const float nJ = fmaxf(j0, j1);
+  const float c0 = (floop == 0.0f) ? 0.0f : __fadd_rn(oaC, ppC), c1 = (eMove == 0.0f) ? 0.0f : V;
+  const int bc = (((floop == 0.0f) ? NI : __fadd_rn(oaC, ppC)) > ((eMove == 0.0f) ? NI : V)) ? 1 : 0;
+  const float nC = fmaxf(c0, c1);
+  const float nN = (floop == 0.0f) ? 0.0f : __fadd_rn(oaN, ppN);
+  const float b0 = (fmove == 0.0f) ? 0.0f : nN, b1 = (fmove == 0.0f) ? 0.0f : nJ;
+  const int bb = (((fmove == 0.0f) ? NI : nN) > ((fmove == 0.0f) ? NI : nJ)) ? 1 : 0;
+  oaJ = nJ; oaC = nC; oaN = nN; oaB = fmaxf(b0, b1);
+#pragma unroll
+  for (int w = 0; w < NW; w++) obits[((size_t) (i - 1) * NW + w) * 64 + z] = nib[w];
+  if (z == 0)
+    orow[i] = (uint32_t) ek | ((uint32_t) es << OA_ESTATE) | ((uint32_t) bc << OA_CBIT) | ((uint32_t) bj << OA_JBIT) |
+              ((uint32_t) bb << OA_BBIT) | ((uint32_t) fail << OA_EFAIL);
+#undef OTR
+}
+
 // MODE 0: the parser. MODE 1: the rows also go to the domain decoding pool
 // (a separate instance, so that the plain parser keeps its register budget).
 // MODE 2: envelope scores: the list is of region copies, each scored in unihit
@@ -1108,6 +1254,20 @@ void k_fwd(const DevProfile p, const uint8_t *__restrict__ res, const int64_t *_
     float accM[B], accI[B];
 #pragma unroll
     for (int j = 0; j < B; j++) accM[j] = accI[j] = 0.0f;
+    // optimal accuracy alignment (MODE 3): the OA rows (row 0: -infinity in
+    // the cells; N 0, J -infinity, B 0, C -infinity) and where the traceback's
+    // choices go
+    constexpr int NW = (B + 7) / 8;
+    float oaM[B], oaI[B], oaD[B];
+    float oaN = 0.0f, oaJ = -INFINITY, oaB = 0.0f, oaC = -INFINITY;
+    uint32_t *obits = nullptr, *orow = nullptr;
+    if (MODE == 3) {
+#pragma unroll
+      for (int j = 0; j < B; j++) oaM[j] = oaI[j] = oaD[j] = -INFINITY;
+      obits = (uint32_t *) (area + n2_oa_bits_at(L, B));
+      orow  = (uint32_t *) (area + n2_oa_rows_at(L, B));
+      if (z == 0) orow[0] = 1u << OA_BBIT;        // B(0) comes from N(0)
+    }
     if (pool) {
       unsigned long long a64 = 0;
       if (z == 0) a64 = atomicAdd(l.cursor, (unsigned long long) (L + 1));
@@ -1196,19 +1356,22 @@ void k_fwd(const DevProfile p, const uint8_t *__restrict__ res, const int64_t *_
         // from Forward's row i-1 and Backward's row i
         const float totr = sp * scale;
         const float *cr = cells + (size_t) (i-1) * 2 * rowf;
-#pragma unroll
-        for (int j = 0; j < B; j++) {
-          accM[j] = __fadd_rn(__fmul_rn(__fmul_rn(M[j], cr[j*64]), totr), accM[j]);
-          accI[j] = __fadd_rn(__fmul_rn(__fmul_rn(I[j], cr[rowf + j*64]), totr), accI[j]);
-        }
         const float *xr = xb + (size_t) i * 6;
-        pN = __fadd_rn(pN, __fmul_rn(__fmul_rn(__fmul_rn(pxN, xr[1]), floop), sp));
-        pJ = __fadd_rn(pJ, __fmul_rn(__fmul_rn(__fmul_rn(pxJ, xr[2]), floop), sp));
-        pC = __fadd_rn(pC, __fmul_rn(__fmul_rn(__fmul_rn(pxC, xr[4]), floop), sp));
+        const float ppN = __fmul_rn(__fmul_rn(__fmul_rn(pxN, xr[1]), floop), sp);
+        const float ppJ = __fmul_rn(__fmul_rn(__fmul_rn(pxJ, xr[2]), floop), sp);
+        const float ppC = __fmul_rn(__fmul_rn(__fmul_rn(pxC, xr[4]), floop), sp);
+        pN = __fadd_rn(pN, ppN);
+        pJ = __fadd_rn(pJ, ppJ);
+        pC = __fadd_rn(pC, ppC);
         if (own) sp = __fmul_rn(sp, __fdiv_rn(scale, xr[5]));
+        // the row's posteriors (summed for null2) and its optimal accuracy row
+        oa_row<B, NW, PIN>(p, z, i, tBM, tMM, tIM, tDM, tMDS, tMI, tII, tr + (PIN ? 0 : opaque_zero()), rowf,
+                           M, I, cr, totr, accM, accI, ppN, ppJ, ppC, floop, fmove, oaM, oaI, oaD, oaN, oaJ, oaB, oaC,
+                           obits, orow);
       }
     }
     if (MODE == 3) {
+      if (z == 0) dom.n2oa[slot] = oaC;          // p7_OptimalAccuracy's *ret_e = C(L)
       null2_epilogue<B>(p, tbl, rowf, z, accM, accI, pN, pJ, pC, sp, sq, L, dom.n2dc + slot, dom.n2st + slot);
       continue;
     }
@@ -2021,6 +2184,85 @@ hipError_t launch_copy32(void *dst, const void *src, int64_t nwords, hipStream_t
   if (nwords <= 0) return hipSuccess;
   const unsigned grid = (unsigned) std::min<int64_t>((nwords + 255) / 256, 4096);
   hipLaunchKernelGGL(k_copy32, dim3(grid), dim3(256), 0, st, (uint32_t *) dst, (const uint32_t *) src, nwords);
+  return hipGetLastError();
+}
+
+// p7_OATrace (impl_sse/optacc.c:227-276) over the choices the null2 pass
+// stored (oa_row), one lane per region, from C at the region's last residue
+// back to S; keeps the first and last M states (p7_alidisplay_Create's
+// hmmfrom/hmmto/sqfrom/sqto, p7_alidisplay.c:74-145).
+__global__ void k_oa_trace(const DevProfile p, const int32_t *__restrict__ list, int64_t n,
+                           const int32_t *__restrict__ vlen, DomBufs dom)
+{
+  enum { S_S = 0, S_N, S_B, S_M, S_I, S_D, S_E, S_J, S_C };
+  const int64_t t = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const int slot = list[t];
+  int4 out = make_int4(0, 0, 0, 0);
+  if (dom.n2st[slot] != 0) { dom.n2ali[slot] = out; return; }       // no decoding, no trace
+  const int L = vlen[slot], B = p.fb_B, NW = n2_oa_words(B);
+  const float *area = dom.n2pool + dom.n2off[slot];
+  const uint32_t *bits = (const uint32_t *) (area + n2_oa_bits_at(L, B));
+  const uint32_t *rows = (const uint32_t *) (area + n2_oa_rows_at(L, B));
+  auto cell = [&](int i, int k) -> uint32_t {
+    const int z = (k - 1) / B, j = (k - 1) % B;
+    return (bits[((size_t) (i - 1) * NW + j / 8) * 64 + z] >> ((j % 8) * 4)) & 15u;
+  };
+  int i = L, k = 0, s0 = S_C, nm = 0, ok = 1;
+  const int guard = 4 * (L + p.M) + 8;
+  for (int step = 0; s0 != S_S; step++) {
+    int s1;
+    if (step > guard || i < 0) { ok = 0; break; }
+    switch (s0) {
+      case S_C: s1 = ((rows[i] >> OA_CBIT) & 1u) ? S_C : S_E; break;
+      case S_E: {
+        const uint32_t r = rows[i];
+        if ((r >> OA_EFAIL) & 1u) { ok = 0; break; }
+        s1 = ((r >> OA_ESTATE) & 1u) ? S_D : S_M;
+        k = (int) (r & 0xffffu);
+        break;
+      }
+      case S_M: {
+        if (i < 1 || k < 1 || k > p.M) { ok = 0; break; }
+        const int c = (int) (cell(i, k) & 3u);
+        s1 = (c == 0) ? S_M : (c == 1) ? S_I : (c == 2) ? S_D : S_B;
+        k--; i--;
+        break;
+      }
+      case S_D:
+        if (i < 1 || k < 1 || k > p.M) { ok = 0; break; }
+        s1 = ((cell(i, k) >> 3) & 1u) ? S_D : S_M;
+        k--;
+        break;
+      case S_I:
+        if (i < 1 || k < 1 || k > p.M) { ok = 0; break; }
+        s1 = ((cell(i, k) >> 2) & 1u) ? S_I : S_M;
+        i--;
+        break;
+      case S_B: s1 = ((rows[i] >> OA_BBIT) & 1u) ? S_N : S_J; break;
+      case S_N: s1 = (i == 0) ? S_S : S_N; break;
+      case S_J: s1 = ((rows[i] >> OA_JBIT) & 1u) ? S_J : S_E; break;
+      default: ok = 0; s1 = S_S; break;
+    }
+    if (! ok) break;
+    if (s1 == S_M) {                     // walking back: the first M is the alignment's last
+      if (nm == 0) { out.y = k; out.w = i; }
+      out.x = k; out.z = i;
+      nm++;
+    }
+    if ((s1 == S_N || s1 == S_J || s1 == S_C) && s1 == s0) i--;
+    s0 = s1;
+  }
+  if (! ok || nm == 0) out = make_int4(0, 0, 0, 0);
+  dom.n2ali[slot] = out;
+}
+
+hipError_t launch_oa_trace(const DevProfile &p, const DevSeqs &s, const int32_t *list, int64_t n, const DomBufs &dom,
+                           hipStream_t st)
+{
+  if (n <= 0) return hipSuccess;
+  const unsigned grid = (unsigned) ((n + 63) / 64);
+  hipLaunchKernelGGL(k_oa_trace, dim3(grid), dim3(64), 0, st, p, list, n, s.len, dom);
   return hipGetLastError();
 }
 

@@ -58,6 +58,8 @@ struct Work {
   int32_t    *n2st = nullptr, *n2list = nullptr, *n2cnt = nullptr;
   float      *n2pool = nullptr;
   int64_t     n2size = 0;
+  float      *n2oa = nullptr;           // optimal accuracy score and alignment of each region
+  int4       *n2ali = nullptr;
   void       *dom_mem = nullptr;
   int64_t     dom_cap = 0;          // slots the per-slot arrays hold
 };
@@ -909,7 +911,8 @@ dom_reserve(Work &w, int64_t n, int64_t pool_floats)
   const int64_t cap = std::max<int64_t>(n, 1024);
   const size_t total = (size_t) cap * (8 + 8 + 4 + 4 + 4) + (16 + 8 + 8 + 4 * 9) * (size_t) DOM_REG_CAP
                        + (size_t) DOM_RES_CAP + 4 * (size_t) N2_CNT_INTS
-                       + 4 * (size_t) (DOM_RES_CAP + DOM_REG_CAP) + 8 * (size_t) DOM_REG_CAP + 24 * 256;
+                       + 4 * (size_t) (DOM_RES_CAP + DOM_REG_CAP) + 8 * (size_t) DOM_REG_CAP
+                       + (4 + 16) * (size_t) DOM_REG_CAP + 26 * 256;
   char *m = nullptr;
   HIPCHK(hipMalloc((void **) &m, total));
   size_t o = 0;
@@ -931,6 +934,8 @@ dom_reserve(Work &w, int64_t n, int64_t pool_floats)
   w.n2cnt = (int32_t *) take(4 * (size_t) N2_CNT_INTS);
   w.dout.rscale = (float *) take(4 * (size_t) (DOM_RES_CAP + DOM_REG_CAP)); w.dout.rscale_cap = DOM_RES_CAP + DOM_REG_CAP;
   w.dout.rsoff = (int64_t *) take(8 * (size_t) DOM_REG_CAP);
+  w.n2oa = (float *) take(4 * (size_t) DOM_REG_CAP);
+  w.n2ali = (int4 *) take(16 * (size_t) DOM_REG_CAP);
   if (o > total) { (void) hipFree(m); p7g_set_error("internal: domain buffer overrun"); return p7G_EINVAL; }
   w.dom_mem = m; w.dom_cap = cap;
   HIPCHK(hipMalloc((void **) &w.dom.pool, sizeof(float) * (size_t) pool_floats));
@@ -947,6 +952,7 @@ work_release(Work &w)
   if (w.n2pool) (void) hipFree(w.n2pool);
   w.dom_mem = nullptr; w.dom = DomBufs(); w.dout = DomOut(); w.dom_cap = 0;
   w.n2off = nullptr; w.n2dc = nullptr; w.n2st = nullptr; w.n2list = nullptr; w.n2cnt = nullptr;
+  w.n2oa = nullptr; w.n2ali = nullptr;
   w.n2pool = nullptr; w.n2size = 0;
   if (w.mem) (void) hipFree(w.mem);
   if (w.pool) (void) hipFree(w.pool);
@@ -1169,7 +1175,8 @@ struct p7g_pipeline_s {
   // then in the caller's sequence order
   mutable std::mutex regs_mu;
   mutable std::vector<int4> regs;
-  mutable std::vector<float> envsc, n2dc;
+  mutable std::vector<float> envsc, n2dc, oasc;
+  mutable std::vector<int4> ali;
   mutable std::vector<int32_t> n2st, regs_slot;
   // hit-list scores of the last run, computed on first request
   mutable bool hits_ready = false;
@@ -1184,6 +1191,8 @@ struct p7g_pipeline_s {
   const float *envsc_dev = nullptr;
   const float *n2dc_dev = nullptr;
   const int32_t *n2st_dev = nullptr;
+  const float *oa_dev = nullptr;
+  const int4 *ali_dev = nullptr;
   // for the hit-list scores: the run's Forward scores (by slot), target
   // lengths (by slot) and the profile's Forward tail parameters
   const float *fwdsc_dev = nullptr;
@@ -1400,6 +1409,8 @@ null2_regions(Work &w, const DevProfile &p, const DevSeqs &s, int64_t nregs, con
   if (nregs <= 0) return p7G_OK;
   HIPCHK(launch_fill32((uint32_t *) w.n2st, 2u, nregs, w.st));
   HIPCHK(launch_fill32((uint32_t *) w.n2dc, 0x7fc00000u, nregs, w.st));
+  HIPCHK(launch_fill32((uint32_t *) w.n2oa, 0x7fc00000u, nregs, w.st));
+  HIPCHK(launch_fill32((uint32_t *) w.n2ali, 0u, 4 * nregs, w.st));
   std::vector<int4> rec((size_t) nregs);
   std::vector<int32_t> vlen((size_t) nregs);
   HIPCHK(hipMemcpyAsync(rec.data(), w.dout.regs, sizeof(int4) * (size_t) nregs, hipMemcpyDeviceToHost, w.st));
@@ -1454,6 +1465,7 @@ null2_regions(Work &w, const DevProfile &p, const DevSeqs &s, int64_t nregs, con
   eb.env_loop = w.dout.env_loop; eb.env_move = w.dout.env_move; eb.env_cap = w.dout.reg_cap;
   eb.n2pool = w.n2pool; eb.n2off = w.n2off; eb.n2dc = w.n2dc; eb.n2st = w.n2st;
   eb.n2scl = w.dout.rscale; eb.n2soff = w.dout.rsoff;
+  eb.n2oa = w.n2oa; eb.n2ali = w.n2ali;
   const LenCfg lc = lencfg_per_target();
   for (size_t b = 0; b < batches.size(); b++) {
     const int32_t *bl = w.n2list + batches[b].first;
@@ -1463,6 +1475,7 @@ null2_regions(Work &w, const DevProfile &p, const DevSeqs &s, int64_t nregs, con
     HIPCHK(launch_bck(p, vs, bl, c + 16, nb, vb, nullptr, nullptr, nullptr, nullptr, lc, w.st, eb, w.l));
     eb.n2mode = 2;
     HIPCHK(launch_fwd(p, vs, bl, c + 32, nb, vf, w.l, g, 0, lc, nullptr, 0, nullptr, nullptr, w.st, eb));
+    HIPCHK(launch_oa_trace(p, vs, bl, nb, eb, w.st));
   }
   HIPCHK(hipStreamSynchronize(w.st));          // the host vectors above are the copies' sources
   return p7G_OK;
@@ -1557,6 +1570,7 @@ p7_gpu_Pipeline(P7_GPU_PIPELINE *pli, const P7_GPU_OPROFILE *om, const P7_GPU_SE
     std::lock_guard<std::mutex> rl(pli->regs_mu);
     pli->nregs = nregs_dev; pli->regs_dev = w.dout.regs; pli->envsc_dev = w.dout.envsc; pli->regs_perm = db->h_perm;
     pli->n2dc_dev = pli->do_null2 ? w.n2dc : nullptr; pli->n2st_dev = w.n2st;
+    pli->oa_dev = pli->do_null2 ? w.n2oa : nullptr; pli->ali_dev = pli->do_null2 ? w.n2ali : nullptr;
     pli->fwdsc_dev = w.r.fwdsc; pli->regs_len = db->h_len; pli->nslots = n;
     pli->ftau = om->evparam[p7G_FTAU]; pli->flambda = om->evparam[p7G_FLAMBDA];
     pli->regs_ready = false; pli->regs.clear();
@@ -1616,12 +1630,18 @@ regs_fetch(const P7_GPU_PIPELINE *pli)
     std::vector<int4> rec(nr);
     std::vector<float> sc(nr), dc(nr, NAN);
     std::vector<int32_t> n2(nr, -1);
+    std::vector<float> oa(nr, NAN);
+    std::vector<int4> al(nr, make_int4(0, 0, 0, 0));
     if (nr) {
       HIPCHK(hipMemcpy(rec.data(), pli->regs_dev, sizeof(int4) * nr, hipMemcpyDeviceToHost));
       HIPCHK(hipMemcpy(sc.data(), pli->envsc_dev, sizeof(float) * nr, hipMemcpyDeviceToHost));
       if (pli->n2dc_dev) {
         HIPCHK(hipMemcpy(dc.data(), pli->n2dc_dev, sizeof(float) * nr, hipMemcpyDeviceToHost));
         HIPCHK(hipMemcpy(n2.data(), pli->n2st_dev, sizeof(int32_t) * nr, hipMemcpyDeviceToHost));
+      }
+      if (pli->oa_dev) {
+        HIPCHK(hipMemcpy(oa.data(), pli->oa_dev, sizeof(float) * nr, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(al.data(), pli->ali_dev, sizeof(int4) * nr, hipMemcpyDeviceToHost));
       }
     }
     std::vector<int32_t> slot_of(nr);
@@ -1632,10 +1652,16 @@ regs_fetch(const P7_GPU_PIPELINE *pli)
       const int4 &x = rec[(size_t) a], &y = rec[(size_t) b];
       return x.x != y.x ? x.x < y.x : x.y < y.y; });
     pli->regs.resize(nr); pli->envsc.resize(nr); pli->n2dc.resize(nr); pli->n2st.resize(nr); pli->regs_slot.resize(nr);
+    pli->oasc.resize(nr); pli->ali.resize(nr);
     for (size_t k = 0; k < nr; k++) {
       const size_t o = (size_t) ord[k];
       pli->regs[k] = rec[o]; pli->envsc[k] = sc[o]; pli->n2dc[k] = dc[o]; pli->n2st[k] = n2[o];
       pli->regs_slot[k] = slot_of[o];
+      // alignment coordinates in the target: the region copy starts at residue ienv
+      const int4 a = al[o];
+      pli->oasc[k] = (n2[o] == 0 && a.x > 0) ? oa[o] : NAN;
+      pli->ali[k] = (n2[o] == 0 && a.x > 0) ? make_int4(a.x, a.y, a.z + rec[o].y - 1, a.w + rec[o].y - 1)
+                                            : make_int4(0, 0, 0, 0);
     }
     pli->regs_ready = true;
   }
@@ -1676,6 +1702,28 @@ p7_gpu_pipeline_GetRegionNull2(const P7_GPU_PIPELINE *pli, float *domcorr, int32
   for (int64_t k = 0; k < std::min(n, cap); k++) {
     if (domcorr) domcorr[k] = pli->n2dc[(size_t) k];
     if (status) status[k] = pli->n2st[(size_t) k];
+  }
+  return p7G_OK;
+}
+
+extern "C" int
+p7_gpu_pipeline_GetDomainAlignments(const P7_GPU_PIPELINE *pli, float *oasc, int32_t *hmmfrom, int32_t *hmmto,
+                                    int32_t *alifrom, int32_t *alito, int64_t cap, int64_t *ret_n)
+{
+  if (! pli || ! pli->regs_valid) { p7g_set_error("no domain regions: the last p7_gpu_Pipeline call did not decode"); return p7G_EINVAL; }
+  if (! pli->oa_dev) { p7g_set_error("the last p7_gpu_Pipeline call ran without null2 (and its alignments)"); return p7G_EINVAL; }
+  std::lock_guard<std::mutex> rl(pli->regs_mu);
+  int st;
+  if ((st = regs_fetch(pli)) != p7G_OK) return st;
+  const int64_t n = (int64_t) pli->regs.size();
+  if (ret_n) *ret_n = n;
+  for (int64_t k = 0; k < std::min(n, cap); k++) {
+    const int4 &a = pli->ali[(size_t) k];
+    if (oasc) oasc[k] = pli->oasc[(size_t) k];
+    if (hmmfrom) hmmfrom[k] = a.x;
+    if (hmmto) hmmto[k] = a.y;
+    if (alifrom) alifrom[k] = a.z;
+    if (alito) alito[k] = a.w;
   }
   return p7G_OK;
 }
@@ -1834,7 +1882,14 @@ p7_gpu_pipeline_GetHitList(const P7_GPU_PIPELINE *pli, double Z, double E, doubl
 /*****************************************************************
  * Batches: many profiles against one database in one call
  *****************************************************************/
-#define P7G_BATCH_CHAINS 8          // concurrent streams of a batch launched directly
+// Concurrent streams of a batch launched directly. Each chain takes one
+// profile's stage chain at a time; with 16 chains the chains of short-target
+// profiles keep the device busy while others run long-target tails, provided
+// the process maps its streams onto enough hardware queues (GPU_MAX_HW_QUEUES:
+// 17.2k GCUPS at 32 queues on the 64-profile batch against 7.9k at 4).
+#ifndef P7G_BATCH_CHAINS
+#define P7G_BATCH_CHAINS 16
+#endif
 #define P7G_GRAPH_CHAINS 4          // concurrent per-profile stage chains of a captured batch
 
 struct Batch {

@@ -103,6 +103,8 @@ def _load():
         "p7_gpu_pipeline_GetStats": (i, [vp, POINTER(Stats)]),
         "p7_gpu_pipeline_SetDomains": (i, [vp, i]), "p7_gpu_pipeline_SetNull2": (i, [vp, i]),
         "p7_gpu_pipeline_GetRegionNull2": (i, [vp, POINTER(c_float), POINTER(c_int32), c_int64, POINTER(c_int64)]),
+        "p7_gpu_pipeline_GetDomainAlignments": (i, [vp, POINTER(c_float), POINTER(c_int32), POINTER(c_int32),
+                                                    POINTER(c_int32), POINTER(c_int32), c_int64, POINTER(c_int64)]),
         "p7_gpu_pipeline_GetTargetScores": (i, [vp, POINTER(c_int64), POINTER(c_float), POINTER(c_float), POINTER(c_float),
                                                 POINTER(c_double), POINTER(c_int32), POINTER(c_int32), POINTER(c_int32),
                                                 c_int64, POINTER(c_int64)]),
@@ -598,6 +600,22 @@ class Pipeline:
         _check(lib.p7_gpu_pipeline_GetRegionNull2(self.h, _ptr(dc, c_float), _ptr(st, c_int32), n.value,
                                                   ctypes.byref(n)), "GetRegionNull2")
         return dc, st
+
+    def domain_alignments(self):
+        """Per region (regions() order), the optimal accuracy alignment: dict of arrays oasc (expected
+        correctly aligned residues; NaN where not computed), hmmfrom, hmmto, alifrom, alito (target
+        coordinates; 0 where not computed)."""
+        n = c_int64()
+        _check(lib.p7_gpu_pipeline_GetDomainAlignments(self.h, None, None, None, None, None, 0, ctypes.byref(n)),
+               "GetDomainAlignments")
+        out = {"oasc": np.zeros(n.value, np.float32)}
+        for k in ("hmmfrom", "hmmto", "alifrom", "alito"):
+            out[k] = np.zeros(n.value, np.int32)
+        _check(lib.p7_gpu_pipeline_GetDomainAlignments(self.h, _ptr(out["oasc"], c_float), _ptr(out["hmmfrom"], c_int32),
+                                                       _ptr(out["hmmto"], c_int32), _ptr(out["alifrom"], c_int32),
+                                                       _ptr(out["alito"], c_int32), n.value, ctypes.byref(n)),
+               "GetDomainAlignments")
+        return out
 
     def target_scores(self):
         """Per target with regions (sequence order): dict of arrays seq, score, pre_score,

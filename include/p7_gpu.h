@@ -305,6 +305,19 @@ int   p7_gpu_pipeline_GetRegions(const P7_GPU_PIPELINE *pli, int64_t *seq, int32
  * without null2. */
 int   p7_gpu_pipeline_GetRegionNull2(const P7_GPU_PIPELINE *pli, float *domcorr, int32_t *status, int64_t cap,
                                      int64_t *ret_n);
+/* The optimal accuracy alignment of each region, in p7_gpu_pipeline_GetRegions'
+ * order, as rescore_isolated_domain computes it for a one-envelope region
+ * (p7_domaindef.c:846-855: p7_OptimalAccuracy and p7_OATrace,
+ * impl_sse/optacc.c:60, 227, on the envelope's unihit posterior decoding):
+ * oasc, the expected number of correctly aligned residues (dom->oasc; the
+ * domain table's acc column is oasc / (1 + |jenv - ienv|)), and the model and
+ * target coordinates of the alignment's first and last M states
+ * (p7_alidisplay_Create's hmmfrom, hmmto, sqfrom, sqto, p7_alidisplay.c:
+ * 140-145). NaN / 0 for multidomain regions and regions whose decoding
+ * failed. p7G_EINVAL if the last call ran without null2 (the alignments are
+ * computed in the same pass). */
+int   p7_gpu_pipeline_GetDomainAlignments(const P7_GPU_PIPELINE *pli, float *oasc, int32_t *hmmfrom, int32_t *hmmto,
+                                          int32_t *alifrom, int32_t *alito, int64_t cap, int64_t *ret_n);
 /* Hit-list scores of the last p7_gpu_Pipeline call with decoding and null2,
  * as p7_Pipeline computes them after domain definition (p7_pipeline.c:743-850):
  * one entry per target with at least one region, in sequence order: the

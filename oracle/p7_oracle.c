@@ -23,9 +23,24 @@
  *   orc_domain_decoding HMMER/src/impl_sse/decoding.c:166-197 (fp32)
  *   orc_domain_regions HMMER/src/p7_domaindef.c:409-489 (the region loop) and
  *                      :534-547 (is_multidomain_region)
+ *   orc_null2_envelope HMMER/src/p7_domaindef.c:841-931 with impl_sse/decoding.c:82-143
+ *                      and impl_sse/null2.c:47-121 (full-matrix Forward/Backward of one
+ *                      envelope, posterior decoding, null2 by expectation)
+ *   orc_oa_envelope    HMMER/src/impl_sse/optacc.c:60-175, 227-423 (p7_OptimalAccuracy
+ *                      and p7_OATrace) on the same envelope matrices
+ *   orc_brute_emitters the path enumeration, weighting the residues each state emits
+ *   orc_gumbel_surv / orc_exp_surv  easel/esl_gumbel.c:129, esl_exponential.c:127
  *   orc_pipeline_db    the stage gates of p7_Pipeline (p7_pipeline.c:687-731)
  * The Forward/Backward restatements keep the reference's four-lane striping,
  * operation order and FTZ/DAZ mode, so they reproduce its fp32 values.
+ *
+ * Parity unpinned: the constants of the bias filter's two-state HMM
+ * (p7_bg.c:431-453: expected length M/8 in the model state, initial
+ * probabilities 0.999/0.001), restated in oracle/__init__.py's bias_hmm. The
+ * filter's forward algorithm is pinned by brute-force enumeration and its
+ * compo = background closed form, but no vector the reference's tests or
+ * tutorial hold fixes those three numbers, and none of the checks above fails
+ * if one of them is changed.
  */
 #include <math.h>
 #include <float.h>
