@@ -317,6 +317,125 @@ def oa_envelope(residues, t, i, j):
     return st, oasc.value, kf.value, kt.value, i0.value + i - 1, i1.value + i - 1
 
 
+def region_multidomain(residues, t, i, j, seed=42, nsamples=200):
+    """(status, clusters, n2sc) of region_trace_ensemble for residues i..j (1-based) of a target,
+    with t configured in multihit mode for the whole target's length (p7_domaindef.c:440-476,
+    600-690): the clusters' envelopes [(i2, j2)] in target coordinates, start order, and the
+    per-residue null2 scores n2sc[0..j-i] of the region's residues, from nsamples sampled traces
+    with the random number generator reset to seed."""
+    reg = np.ascontiguousarray(np.asarray(residues)[i - 1:j], np.uint8)
+    dsq = dsq_of(reg)
+    K, Kp, mask, nd = degeneracy_masks(t["abctype"])
+    rfv = np.ascontiguousarray(t["rfv"], np.float32)
+    tfv = np.ascontiguousarray(t["tfv"], np.float32)
+    xf = np.ascontiguousarray(t["xf"], np.float32).reshape(8)
+    n2 = np.zeros(reg.size + 1, np.float32)
+    cap = reg.size + 1
+    ci, cj = np.zeros(cap, np.int32), np.zeros(cap, np.int32)
+    nc, nseg = c_int(), c_int()
+    st = lib.orc_region_multidomain(_p(dsq, c_uint8), reg.size, t["M"], K, Kp, _p(rfv, c_float), _p(tfv, c_float),
+                                    _p(xf, c_float), _p(mask, c_uint32), _p(nd, c_int), ctypes.c_uint32(seed),
+                                    nsamples, _p(n2, c_float), ctypes.byref(nc), _p(ci, c_int), _p(cj, c_int), cap,
+                                    ctypes.byref(nseg))
+    clusters = [(int(ci[d]) + i - 1, int(cj[d]) + i - 1) for d in range(min(nc.value, cap))]
+    return st, clusters, n2[1:]
+
+
+def single_linkage_1d(x, thr):
+    """(nclusters, assignment) of the oracle's single-linkage routine over points x, linked when
+    |x_a - x_b| <= thr."""
+    x = np.ascontiguousarray(x, np.float64)
+    asg = np.zeros(x.size, np.int32)
+    n = lib.orc_single_linkage_1d(_p(x, c_double), x.size, c_double(thr), _p(asg, c_int))
+    return n, asg
+
+
+def sample_emissions(residues, t, ntraces, seed=42):
+    """(status, w[2M+3], ndom): mean residues emitted per sampled trace by M_1..M_M, I_1..I_M, N,
+    C, J (divided by L) and mean number of domains, over ntraces stochastic traces of the
+    whole sequence with t's configuration."""
+    dsq = dsq_of(residues)
+    L = dsq.size - 2
+    rfv = np.ascontiguousarray(t["rfv"], np.float32)
+    tfv = np.ascontiguousarray(t["tfv"], np.float32)
+    xf = np.ascontiguousarray(t["xf"], np.float32).reshape(8)
+    w = np.zeros(2 * t["M"] + 3, np.float64)
+    nd = c_double()
+    st = lib.orc_sample_emissions(_p(dsq, c_uint8), L, t["M"], t["Kp"], _p(rfv, c_float), _p(tfv, c_float),
+                                  _p(xf, c_float), ctypes.c_uint32(seed), ntraces, _p(w, c_double), ctypes.byref(nd))
+    return st, w, nd.value
+
+
+def domain_definition(residues, t, seed=42, nsamples=200):
+    """p7_domaindef_ByPosteriorHeuristics (p7_domaindef.c:386-489) for one target, with t
+    configured in multihit mode for the target's length: decoding, regions, and per region one
+    envelope (single-domain) or the rescored clusters of its sampled traces (multidomain), each
+    domain rescored as rescore_isolated_domain does (:812-947). Returns a dict: nexpected,
+    regions [(i, j, multi)], domains [dict ienv jenv envsc domcorr oasc hmmfrom hmmto alifrom alito
+    region], n2sc[0..L] (the per-residue null2 scores the sequence bias sums), nclustered,
+    nenvelopes, noverlaps."""
+    f32 = np.float32
+    res = np.asarray(residues)
+    L = res.size
+    st, fsc, fx, _ = fwd_parser(res, t)
+    st2, bsc, bx, own = bck_parser(res, t, fx)
+    if st or st2:
+        raise ValueError("parsers failed")
+    st, btot, etot, mocc = domain_decoding(fx, bx, own, t["xf"])
+    if st:
+        raise ValueError("decoding overflow")
+    regs = domain_regions(btot, etot, mocc)
+    n2sc = np.zeros(L + 1, np.float32)
+    xfu = unihit_xf(t, L)
+    out = {"nexpected": float(btot[L]), "regions": regs, "domains": [], "n2sc": n2sc,
+           "nclustered": 0, "nenvelopes": 0, "noverlaps": 0}
+
+    def rescore(i, j, region, n2_done):
+        st, envsc = envelope_score(res, t, i, j)
+        st2, _, dc, _ = null2_envelope(res[i - 1:j], t, xfu)
+        if st2 == 16:
+            return False                                   # p7_Decoding's eslERANGE: no domain
+        if st or st2:
+            raise ValueError("envelope rescoring failed")
+        if not n2_done:
+            n2sc[i:j + 1] = null2_residue_scores(res[i - 1:j], t, xfu)
+        dcor = f32(0.0)
+        for pos in range(i, j + 1):
+            dcor = f32(dcor + n2sc[pos])
+        sto, oasc, kf, kt, a0, a1 = oa_envelope(res, t, i, j)
+        out["domains"].append({"ienv": i, "jenv": j, "envsc": float(envsc), "domcorr": float(dcor),
+                               "oasc": oasc if sto == 0 else float("nan"), "hmmfrom": kf, "hmmto": kt,
+                               "alifrom": a0, "alito": a1, "region": region})
+        return True
+
+    for r, (i, j, multi) in enumerate(regs):
+        if multi:
+            out["nclustered"] += 1
+            st, clusters, rn2 = region_multidomain(res, t, i, j, seed, nsamples)
+            if st:
+                raise ValueError(f"trace ensemble failed ({st})")
+            n2sc[i:j + 1] = rn2
+            last_j2 = 0
+            for i2, j2 in clusters:
+                if i2 <= last_j2:
+                    out["noverlaps"] += 1
+                out["nenvelopes"] += 1
+                if rescore(i2, j2, r, True):
+                    last_j2 = j2
+        else:
+            out["nenvelopes"] += 1
+            rescore(i, j, r, False)
+    return out
+
+
+def null2_residue_scores(env_residues, t, xf):
+    """logf(null2[x_pos]) for each residue of one envelope (p7_domaindef.c:937-940)."""
+    st, null2, _, _ = null2_envelope(env_residues, t, xf)
+    if st:
+        raise ValueError("null2 failed")
+    return np.array([_libm.logf(null2[int(x)]) for x in env_residues], np.float32)
+
+
 def domain_regions(btot, etot, mocc, rt1=RT1, rt2=RT2, rt3=RT3):
     """[(i, j, multidomain)] of the region loop of p7_domaindef_ByPosteriorHeuristics."""
     L = btot.size - 1
@@ -485,16 +604,23 @@ def exp_logsurv(x, mu, lam):
     return 0.0 if x < mu else -lam * (x - mu)
 
 
-def target_scores(L, fwdsc, nullsc, doms, tau, lam):
+def target_scores(L, fwdsc, nullsc, doms, tau, lam, n2sc=None):
     """p7_Pipeline's scores after domain definition (p7_pipeline.c:743-850), in its float and
     double arithmetic. doms: [(envsc, domcorrection, ienv, jenv)] of the domains (nats).
+    n2sc: the per-residue null2 scores [0..L] whose float sum in residue order is the sequence
+    bias (esl_vec_FSum, p7_pipeline.c:760); without it the domains' corrections are summed,
+    which is the same set of terms when no region is multidomain.
     Returns (seq_score, pre_score, sum_score, lnP, best, [(bitscore, dombias, lnP)])."""
     f = np.float32
     fwdsc, nullsc = f(fwdsc), f(nullsc)
     tau, lam = float(f(tau)), float(f(lam))          # om->evparam[] is float
     seqbias = f(0.0)
-    for _, dc, _, _ in doms:
-        seqbias = f(seqbias + f(dc))
+    if n2sc is not None:
+        for v in np.asarray(n2sc, np.float32):
+            seqbias = f(seqbias + v)
+    else:
+        for _, dc, _, _ in doms:
+            seqbias = f(seqbias + f(dc))
     seqbias = flogsum(0.0, math.log(OMEGA) + float(seqbias))
     pre_score = f(float(f(fwdsc - nullsc)) / LOG2)
     seq_score = f(float(f(fwdsc - f(nullsc + seqbias))) / LOG2)

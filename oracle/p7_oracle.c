@@ -29,6 +29,10 @@
  *   orc_oa_envelope    HMMER/src/impl_sse/optacc.c:60-175, 227-423 (p7_OptimalAccuracy
  *                      and p7_OATrace) on the same envelope matrices
  *   orc_brute_emitters the path enumeration, weighting the residues each state emits
+ *   orc_region_multidomain HMMER/src/p7_domaindef.c:600-690 (region_trace_ensemble):
+ *                      p7_Forward kept whole, p7_StochasticTrace (impl_sse/stotrace.c),
+ *                      p7_trace_Index, p7_Null2_ByTrace (impl_sse/null2.c:134-205),
+ *                      esl_cluster_SingleLinkage, p7_spensemble_Cluster
  *   orc_gumbel_surv / orc_exp_surv  easel/esl_gumbel.c:129, esl_exponential.c:127
  *   orc_pipeline_db    the stage gates of p7_Pipeline (p7_pipeline.c:687-731)
  * The Forward/Backward restatements keep the reference's four-lane striping,
@@ -41,6 +45,12 @@
  * compo = background closed form, but no vector the reference's tests or
  * tutorial hold fixes those three numbers, and none of the checks above fails
  * if one of them is changed.
+ *
+ * Parity unpinned: the random number stream of orc_region_multidomain's
+ * sampled traces (the Jenkins mix of the seed, then x <- 69069 x + 1). The
+ * sampling it drives is pinned against path enumeration, and the tutorial's
+ * multidomain region gives the same clusters and printed bias for other
+ * seeds, so no check fixes the stream itself.
  */
 #include <math.h>
 #include <float.h>
@@ -222,9 +232,13 @@ unstripe_row(int Q, int M, const V4 *MMO, const V4 *IMO, float *out)
 /* mi (optional): the M and I values of every row after its rescaling, row i
  * at mi + i*2*M as [M_1..M_M][I_1..I_M] (p7_Forward keeps the whole matrix,
  * fwdback.c:98; the parser keeps one row). */
+/* full (optional): the whole striped matrix as p7_Forward keeps it
+ * (fwdback.c:98, impl_sse.h:178): rows 0..L, row i at full + i*Q*12, cell
+ * (q, s, r) at [(q*3 + s)*4 + r] with s = 0 M, 1 D, 2 I, model position
+ * k = r*Q + q + 1; row 0 is zero. */
 static int
 fwd_engine(const uint8_t *dsq, int L, int M, int Kp, const float *rfv, const float *tfv, const float *xf,
-           float *xmx, float *ret_totscale, float *ret_sc, float *mi)
+           float *xmx, float *ret_totscale, float *ret_sc, float *mi, float *full)
 {
   STRIPED s;
   V4 *MMO, *DMO, *IMO, mpv, dpv, ipv, sv, dcv, xEv, xBv, zerov = v4_set(0.0f);
@@ -238,6 +252,7 @@ fwd_engine(const uint8_t *dsq, int L, int M, int Kp, const float *rfv, const flo
   xE = 0.; xN = 1.; xJ = 0.; xB = xf[XO(XN,MOVE)]; xC = 0.;
   if (xmx) { xmx[0] = xE; xmx[1] = xN; xmx[2] = xJ; xmx[3] = xB; xmx[4] = xC; xmx[5] = 1.0f; }
   if (mi) memset(mi, 0, sizeof(float) * 2 * (size_t) M);
+  if (full) memset(full, 0, sizeof(float) * 12 * (size_t) Q);
 
   for (i = 1; i <= L; i++) {
     const V4 *rp = s.rf + (size_t) dsq[i] * Q;
@@ -300,6 +315,11 @@ fwd_engine(const uint8_t *dsq, int L, int M, int Kp, const float *rfv, const flo
     }
     if (xmx) { float *r = xmx + (size_t) i*6; r[0] = xE; r[1] = xN; r[2] = xJ; r[3] = xB; r[4] = xC; r[5] = scale; }
     if (mi) unstripe_row(Q, M, MMO, IMO, mi + (size_t) i * 2 * M);
+    if (full)
+      for (q = 0; q < Q; q++) {
+        V4 *row = (V4 *) (full + ((size_t) i * Q + q) * 12);
+        row[0] = MMO[q]; row[1] = DMO[q]; row[2] = IMO[q];
+      }
   }
   free(MMO); free(DMO); free(IMO); unstripe(&s);
   if (ret_totscale) *ret_totscale = totscale;
@@ -314,7 +334,7 @@ fwd_engine(const uint8_t *dsq, int L, int M, int Kp, const float *rfv, const flo
 int
 orc_fwd_parser(const uint8_t *dsq, int L, int M, int Kp, const float *rfv, const float *tfv, const float *xf,
                float *xmx, float *ret_totscale, float *ret_sc)
-{ return fwd_engine(dsq, L, M, Kp, rfv, tfv, xf, xmx, ret_totscale, ret_sc, NULL); }
+{ return fwd_engine(dsq, L, M, Kp, rfv, tfv, xf, xmx, ret_totscale, ret_sc, NULL, NULL); }
 
 /* mi (optional): as for fwd_engine, the Backward values of rows 1..L (p7_Backward, fwdback.c:141). */
 static int
@@ -474,7 +494,7 @@ orc_null2_envelope(const uint8_t *dsq, int L, int M, int K, int Kp, const float 
   int own = 0, status = 0, Q = (M - 1) / 4 + 1;
   if (Q < 2) Q = 2;
   if (! fx || ! bx || ! fm || ! bm || ! pm || L < 1) { status = 5; goto done; }
-  if ((status = fwd_engine(dsq, L, M, Kp, rfv, tfv, xf, fx, NULL, &sc, fm)) != 0) goto done;
+  if ((status = fwd_engine(dsq, L, M, Kp, rfv, tfv, xf, fx, NULL, &sc, fm, NULL)) != 0) goto done;
   if ((status = bck_engine(dsq, L, M, Kp, rfv, tfv, xf, fx, bx, &own, &sc, bm)) != 0) goto done;
   {
     /* p7_Decoding: row i's posteriors are (f * b) * (scaleproduct * fscale_i);
@@ -571,7 +591,7 @@ orc_oa_envelope(const uint8_t *dsq, int L, int M, int Kp, const float *rfv, cons
   const int pad = (4 * Q > M);
   const float NI = -INFINITY;
   if (! fx || ! bx || ! fm || ! bm || ! pp || ! px || ! oa || ! ox || L < 1) { status = 5; goto done; }
-  if ((status = fwd_engine(dsq, L, M, Kp, rfv, tfv, xf, fx, NULL, &sc, fm)) != 0) goto done;
+  if ((status = fwd_engine(dsq, L, M, Kp, rfv, tfv, xf, fx, NULL, &sc, fm, NULL)) != 0) goto done;
   if ((status = bck_engine(dsq, L, M, Kp, rfv, tfv, xf, fx, bx, &own, &sc, bm)) != 0) goto done;
   {
     float sp = 1.0 / bx[X_N];
@@ -696,6 +716,474 @@ done:
   free(fx); free(bx); free(fm); free(bm); free(pp); free(px); free(oa); free(ox);
   return status;
 }
+
+/*****************************************************************
+ * Multidomain regions: region_trace_ensemble (p7_domaindef.c:600-690) as
+ * p7_domaindef_ByPosteriorHeuristics calls it (:431-476).
+ *
+ * The region's residues are scored by p7_Forward with the profile in multihit
+ * mode, length-configured to the whole target (xf), keeping the striped
+ * matrix. With the random number generator reset to its seed (the Easel
+ * "fast" generator, esl_random.c:148-160, 215-230, 289-310: a Jenkins mix of
+ * the seed, then x <- 69069 x + 1 mod 2^32, x / 2^32), nsamples traces are
+ * drawn by p7_StochasticTrace (impl_sse/stotrace.c:76-329) and indexed into
+ * domains (p7_trace_Index, p7_trace.c:1173-1206). Each sampled domain is
+ * added to the segment ensemble and given the null2 odds of its own trace
+ * segment (p7_Null2_ByTrace, impl_sse/null2.c:134-205); per residue the
+ * samples' odds (1 outside domains) are averaged and logged. The ensemble is
+ * clustered by single linkage (esl_cluster.c:136-185 with link_spsamples,
+ * p7_spensemble.c:191-222), each cluster of posterior >= 0.25 gets the widest
+ * endpoints with >= 2% support (p7_spensemble.c:282-411), and clusters
+ * dominated by an overlapping one of higher posterior are removed
+ * (p7_domaindef.c:650-682).
+ *
+ * dsq: the region's residues dsq[1..Lr]; degen/ndegen as for
+ * orc_null2_envelope. Outputs: n2sc[1..Lr] (log null2 per residue), and
+ * *ret_nc clusters ci[]/cj[] (1-based in the region, in start order), at most
+ * cap. *ret_nsegs: the ensemble's size. Returns 0, 5 on allocation failure,
+ * 1 if a traceback fails (the reference raises an exception), 16 if the
+ * Forward overflows.
+ *****************************************************************/
+enum { TS_M = 1, TS_D, TS_I, TS_S, TS_N, TS_B, TS_E, TS_C, TS_T, TS_J };
+
+typedef struct { uint32_t x; } ORC_RNG;
+
+/* Bob Jenkins' 96-bit mix, as esl_random.c disperses a seed */
+static uint32_t
+orc_mix3(uint32_t a, uint32_t b, uint32_t c)
+{
+  a = (a - b - c) ^ (c >> 13);
+  b = (b - c - a) ^ (a << 8);
+  c = (c - a - b) ^ (b >> 13);
+  a = (a - b - c) ^ (c >> 12);
+  b = (b - c - a) ^ (a << 16);
+  c = (c - a - b) ^ (b >> 5);
+  a = (a - b - c) ^ (c >> 3);
+  b = (b - c - a) ^ (a << 10);
+  c = (c - a - b) ^ (b >> 15);
+  return c;
+}
+static void   rng_seed(ORC_RNG *g, uint32_t seed) { g->x = orc_mix3(seed, 87654321u, 12345678u); if (g->x == 0) g->x = 42; }
+static double rng_uniform(ORC_RNG *g)              { g->x = g->x * 69069u + 1u; return (double) g->x / 4294967296.0; }
+
+/* esl_vec_FNorm then esl_rnd_FChoose: float normalisation, double roll */
+static int
+rng_choose(ORC_RNG *g, float *p, int n)
+{
+  float s = 0.0f;
+  for (int a = 0; a < n; a++) s += p[a];
+  if (s != 0.0f) for (int a = 0; a < n; a++) p[a] /= s;
+  else           for (int a = 0; a < n; a++) p[a] = (float) (1. / (float) n);
+  double norm = 0.0, sum = 0.0, roll = rng_uniform(g);
+  for (int a = 0; a < n; a++) norm += p[a];
+  for (int a = 0; a < n; a++) { sum += p[a]; if (roll < sum / norm) return a; }
+  return -1;
+}
+
+typedef struct {
+  int Q;
+  const float *full, *xmx, *xf;
+  const STRIPED *sp;
+} TRACE_CTX;
+
+#define FV(c,i,q,s,r) ((c)->full[(((size_t) (i) * (c)->Q + (q)) * 3 + (s)) * 4 + (r)])
+#define TV(c,q,t,r)   ((c)->sp->tf[(size_t) (q) * 7 + (t)].v[r])
+#define DDV(c,q,r)    ((c)->sp->tf[(size_t) 7 * (c)->Q + (q)].v[r])
+#define XV(c,i,x)     ((c)->xmx[(size_t) (i) * 6 + (x)])
+
+/* the cell before (q, r) in the striped order, as esl_sse_rightshift_ps
+ * gives it for q = 0: lane r-1 of vector Q-1, or nothing for r = 0 */
+static int prev_cell(int Q, int q, int r, int *pq, int *pr)
+{
+  if (q > 0) { *pq = q - 1; *pr = r; return 1; }
+  if (r == 0) return 0;
+  *pq = Q - 1; *pr = r - 1; return 1;
+}
+
+static int
+trace_step(ORC_RNG *g, const TRACE_CTX *c, int s0, int i, int *k)
+{
+  const int Q = c->Q;
+  float path[4];
+  int q = (*k - 1) % Q, r = (*k - 1) / Q, pq, pr, a;
+  switch (s0) {
+  case TS_M: {                                    /* from B(i-1), M/I/D(i-1, k-1) */
+    const int ok = prev_cell(Q, q, r, &pq, &pr);
+    path[0] = XV(c, i-1, X_B) * TV(c, q, T_BM, r);
+    path[1] = (ok ? FV(c, i-1, pq, 0, pr) : 0.0f) * TV(c, q, T_MM, r);
+    path[2] = (ok ? FV(c, i-1, pq, 2, pr) : 0.0f) * TV(c, q, T_IM, r);
+    path[3] = (ok ? FV(c, i-1, pq, 1, pr) : 0.0f) * TV(c, q, T_DM, r);
+    a = rng_choose(g, path, 4);
+    return a < 0 ? -1 : (int []){ TS_B, TS_M, TS_I, TS_D }[a];
+  }
+  case TS_D: {                                    /* from M/D(i, k-1) */
+    const int ok = prev_cell(Q, q, r, &pq, &pr);
+    path[0] = ok ? FV(c, i, pq, 0, pr) * TV(c, pq, T_MD, pr) : 0.0f * 0.0f;
+    path[1] = ok ? FV(c, i, pq, 1, pr) * DDV(c, pq, pr)      : 0.0f * 0.0f;
+    a = rng_choose(g, path, 2);
+    return a < 0 ? -1 : (a == 0 ? TS_M : TS_D);
+  }
+  case TS_I:                                      /* from M/I(i-1, k) */
+    path[0] = FV(c, i-1, q, 0, r) * TV(c, q, T_MI, r);
+    path[1] = FV(c, i-1, q, 2, r) * TV(c, q, T_II, r);
+    a = rng_choose(g, path, 2);
+    return a < 0 ? -1 : (a == 0 ? TS_M : TS_I);
+  case TS_N: return i == 0 ? TS_S : TS_N;
+  case TS_C:
+    path[0] = XV(c, i-1, X_C) * c->xf[XO(XC,LOOP)];
+    path[1] = XV(c, i, X_E) * c->xf[XO(XE,MOVE)] * XV(c, i, X_SC);
+    a = rng_choose(g, path, 2);
+    return a < 0 ? -1 : (a == 0 ? TS_C : TS_E);
+  case TS_J:
+    path[0] = XV(c, i-1, X_J) * c->xf[XO(XJ,LOOP)];
+    path[1] = XV(c, i, X_E) * c->xf[XO(XE,LOOP)] * XV(c, i, X_SC);
+    a = rng_choose(g, path, 2);
+    return a < 0 ? -1 : (a == 0 ? TS_J : TS_E);
+  case TS_B:
+    path[0] = XV(c, i, X_N) * c->xf[XO(XN,MOVE)];
+    path[1] = XV(c, i, X_J) * c->xf[XO(XJ,MOVE)];
+    a = rng_choose(g, path, 2);
+    return a < 0 ? -1 : (a == 0 ? TS_N : TS_J);
+  case TS_E: {                                    /* any M(i, k), D(i, k): one roll, scanned in striped order */
+    const double roll = rng_uniform(g), norm = 1.0 / XV(c, i, X_E);
+    const float nf = (float) norm;
+    double sum = 0.0;
+    for (int pass = 0; pass < 4; pass++)          /* the reference repeats the scan until it lands */
+      for (q = 0; q < Q; q++) {
+        for (r = 0; r < 4; r++) { sum += FV(c, i, q, 0, r) * nf; if (roll < sum) { *k = r*Q + q + 1; return TS_M; } }
+        for (r = 0; r < 4; r++) { sum += FV(c, i, q, 1, r) * nf; if (roll < sum) { *k = r*Q + q + 1; return TS_D; } }
+      }
+    return -1;
+  }
+  }
+  return -1;
+}
+
+/* p7_StochasticTrace: one trace of the region, built backwards from T with
+ * p7_trace_Append's rules for i and k, then p7_trace_Reverse (N/C/J residues
+ * pulled back by one, the arrays reversed). The arrays grow as needed. */
+static int
+sample_trace(ORC_RNG *g, const TRACE_CTX *c, int L, int **pst, int **pk, int **pi, int *pcap, int *ret_N)
+{
+  int N = 0, i = L, k = 0, s0, s1, *tst = *pst, *tk = *pk, *ti = *pi, tcap = *pcap, status = 0;
+#define APPEND(st_, k_, i_) do { \
+    if (N == tcap) { \
+      int *a = realloc(tst, sizeof(int) * 2 * tcap), *b = realloc(tk, sizeof(int) * 2 * tcap), *d = realloc(ti, sizeof(int) * 2 * tcap); \
+      if (a) { tst = a; } \
+      if (b) { tk = b; } \
+      if (d) { ti = d; } \
+      if (! a || ! b || ! d) { status = 5; break; } \
+      tcap *= 2; } \
+    tst[N] = (st_); \
+    if ((st_) == TS_N || (st_) == TS_C || (st_) == TS_J) { tk[N] = 0; ti[N] = (N > 0 && tst[N-1] == (st_)) ? (i_) : 0; } \
+    else if ((st_) == TS_M || (st_) == TS_I) { tk[N] = (k_); ti[N] = (i_); } \
+    else if ((st_) == TS_D) { tk[N] = (k_); ti[N] = 0; } \
+    else { tk[N] = 0; ti[N] = 0; } \
+    N++; } while (0)
+  APPEND(TS_T, 0, i);
+  APPEND(TS_C, 0, i);
+  s0 = TS_C;
+  while (s0 != TS_S && ! status) {
+    s1 = trace_step(g, c, s0, i, &k);
+    if (s1 < 0) { status = 1; break; }
+    if (s0 == TS_M) { k--; i--; }
+    else if (s0 == TS_D) k--;
+    else if (s0 == TS_I) i--;
+    APPEND(s1, k, i);
+    if ((s1 == TS_N || s1 == TS_J || s1 == TS_C) && s1 == s0) i--;
+    s0 = s1;
+  }
+#undef APPEND
+  *pst = tst; *pk = tk; *pi = ti; *pcap = tcap;
+  if (status) return status;
+  for (int z = 0; z + 1 < N; z++)
+    if ((tst[z] == TS_N || tst[z] == TS_C || tst[z] == TS_J) && tst[z+1] == tst[z] && ti[z] == 0 && ti[z+1] > 0) {
+      ti[z] = ti[z+1]; ti[z+1] = 0;
+    }
+  for (int z = 0; z < N / 2; z++) {
+    int y = N - 1 - z, u;
+    u = tst[z]; tst[z] = tst[y]; tst[y] = u;
+    u = tk[z];  tk[z]  = tk[y];  tk[y]  = u;
+    u = ti[z];  ti[z]  = ti[y];  ti[y]  = u;
+  }
+  *ret_N = N;
+  return 0;
+}
+
+/* esl_cluster_SingleLinkage over n vertices with a symmetric link rule: a
+ * stack of unlinked vertices (pushed backwards, so vertex 0 is popped
+ * first; a linked vertex is deleted by moving the stack's top into its
+ * slot), and a stack of linked vertices still to extend. work: 2n ints.
+ * Returns the number of clusters; asg[v] is v's cluster. */
+static int
+single_linkage(int n, int (*linked)(int, int, const void *), const void *prm, int *work, int *asg)
+{
+  int *av = work, *bv = work + n, na = n, nb = 0, ncl = 0;
+  for (int v = 0; v < n; v++) av[v] = n - v - 1;
+  while (na > 0) {
+    bv[nb++] = av[--na];
+    while (nb > 0) {
+      const int v = bv[--nb];
+      asg[v] = ncl;
+      for (int a = na - 1; a >= 0; a--)
+        if (linked(v, av[a], prm)) { bv[nb++] = av[a]; av[a] = av[--na]; }
+    }
+    ncl++;
+  }
+  return ncl;
+}
+
+typedef struct { int idx, i, j, k, m; float prob; } SPSEG;
+
+/* link_spsamples with min_overlap 0.8 of the smaller segment, max_diagdiff 4 */
+static int
+sp_linked(const SPSEG *a, const SPSEG *b)
+{
+  int nov = (a->j < b->j ? a->j : b->j) - (a->i > b->i ? a->i : b->i) + 1;
+  int n   = (a->j - a->i < b->j - b->i ? a->j - a->i : b->j - b->i) + 1;
+  if ((float) nov / (float) n < 0.8f) return 0;
+  nov = (a->m < b->m ? a->m : b->m) - (a->k > b->k ? a->k : b->k);      /* no +1 in the reference's model overlap */
+  n   = (a->m - a->k < b->m - b->k ? a->m - a->k : b->m - b->k) + 1;
+  if ((float) nov / (float) n < 0.8f) return 0;
+  if (abs((a->i - a->k) - (b->i - b->k)) <= 4) return 1;
+  if (abs((a->j - a->m) - (b->j - b->m)) <= 4) return 1;
+  return 0;
+}
+
+static int seg_linked(int a, int b, const void *prm) { const SPSEG *s = prm; return sp_linked(&s[a], &s[b]); }
+
+int
+orc_region_multidomain(const uint8_t *dsq, int Lr, int M, int K, int Kp, const float *rfv, const float *tfv,
+                       const float *xf, const uint32_t *degen, const int *ndegen, uint32_t seed, int nsamples,
+                       float *n2sc, int *ret_nc, int *ci, int *cj, int cap, int *ret_nsegs)
+{
+  STRIPED sp;
+  TRACE_CTX c;
+  ORC_RNG g;
+  int Q = (M - 1) / 4 + 1, status = 0, nseg = 0, segcap = 64, nc = 0;
+  if (Q < 2) Q = 2;
+  float *fx = malloc(sizeof(float) * 6 * (size_t) (Lr + 1));
+  float *full = malloc(sizeof(float) * 12 * (size_t) Q * (Lr + 1));
+  float *acc = calloc((size_t) Lr + 1, sizeof(float));
+  float *cnt = malloc(sizeof(float) * 4 * (size_t) Q);
+  SPSEG *seg = malloc(sizeof(SPSEG) * (size_t) segcap);
+  int tcap = 1024, *tst = malloc(sizeof(int) * tcap), *tk = malloc(sizeof(int) * tcap), *ti = malloc(sizeof(int) * tcap);
+  int *work = NULL, *asg = NULL;
+  SPSEG *sig = NULL;
+  float sc, null2[32];
+  *ret_nc = 0;
+  if (ret_nsegs) *ret_nsegs = 0;
+  if (! fx || ! full || ! acc || ! cnt || ! seg || ! tst || ! tk || ! ti || Lr < 1 || Kp > 32) { status = 5; goto done; }
+  if ((status = fwd_engine(dsq, Lr, M, Kp, rfv, tfv, xf, fx, NULL, &sc, NULL, full)) != 0) goto done;
+  if (stripe(M, Kp, rfv, tfv, &sp)) { status = 5; goto done; }
+  c.Q = Q; c.full = full; c.xmx = fx; c.xf = xf; c.sp = &sp;
+  rng_seed(&g, seed);
+  {
+    unsigned csr = _mm_getcsr();
+    _mm_setcsr(csr | 0x8040);
+    for (int t = 0; t < nsamples && ! status; t++) {
+      int N = 0;
+      if ((status = sample_trace(&g, &c, Lr, &tst, &tk, &ti, &tcap, &N)) != 0) break;
+      /* p7_trace_Index, then per domain: ensemble entry, null2 of the segment, residue odds */
+      int pos = 1, tfrom = 0, sqfrom = 0, sqto = 0, hmmfrom = 0, hmmto = 0;
+      for (int z = 0; z < N; z++) {
+        if (tst[z] == TS_B) { tfrom = z; sqfrom = 0; hmmfrom = 0; }
+        else if (tst[z] == TS_M) {
+          if (sqfrom == 0) sqfrom = ti[z];
+          if (hmmfrom == 0) hmmfrom = tk[z];
+          sqto = ti[z]; hmmto = tk[z];
+        }
+        else if (tst[z] == TS_E) {
+          if (nseg == segcap) { segcap *= 2; seg = realloc(seg, sizeof(SPSEG) * (size_t) segcap); if (! seg) { status = 5; break; } }
+          seg[nseg++] = (SPSEG){ t, sqfrom, sqto, hmmfrom, hmmto, 0.0f };
+          /* p7_Null2_ByTrace over trace elements tfrom..z: every emitting
+           * M or I state counts at the M cell of its node */
+          int Ld = 0;
+          memset(cnt, 0, sizeof(float) * 4 * (size_t) Q);
+          for (int y = tfrom; y <= z; y++) {
+            if (ti[y] == 0) continue;
+            Ld++;
+            if (tk[y] > 0) cnt[((tk[y] - 1) % Q) * 4 + (tk[y] - 1) / Q] += 1.0f;
+          }
+          const float norm = 1.0 / (float) Ld;
+          for (int a = 0; a < 4 * Q; a++) cnt[a] = cnt[a] * norm;
+          for (int x = 0; x < K; x++) {
+            V4 sv = v4_set(0.0f);
+            for (int q = 0; q < Q; q++)
+              for (int r = 0; r < 4; r++) {
+                sv.v[r] = sv.v[r] + cnt[q*4 + r] * sp.rf[(size_t) x * Q + q].v[r];
+                sv.v[r] = sv.v[r] + 0.0f;                  /* the I cell: never counted */
+              }
+            null2[x] = v4_hsum(sv) + 0.0f;                 /* no N, C or J inside a domain */
+          }
+          for (int x = K + 1; x <= Kp - 3; x++) {
+            float rr = 0.0f;
+            for (int y = 0; y < K; y++) if (degen[x] >> y & 1u) rr += null2[y];
+            null2[x] = rr / (float) ndegen[x];
+          }
+          null2[K] = 1.0f; null2[Kp-2] = 1.0f; null2[Kp-1] = 1.0f;
+          for (; pos <= sqfrom; pos++) acc[pos] += 1.0f;
+          for (; pos <= sqto; pos++)   acc[pos] += null2[dsq[pos]];
+        }
+      }
+      for (; pos <= Lr; pos++) acc[pos] += 1.0f;
+    }
+    _mm_setcsr(csr);
+  }
+  if (status) goto done_sp;
+  for (int pos = 1; pos <= Lr; pos++) n2sc[pos] = logf(acc[pos] / (float) nsamples);
+  if (ret_nsegs) *ret_nsegs = nseg;
+
+  /* esl_cluster_SingleLinkage: a stack of unlinked vertices (pushed
+   * backwards), and a stack of linked ones still to extend */
+  work = malloc(sizeof(int) * 2 * (size_t) (nseg + 1)); asg = malloc(sizeof(int) * (size_t) (nseg + 1));
+  sig = malloc(sizeof(SPSEG) * (size_t) (nseg + 1));
+  if (! work || ! asg || ! sig) { status = 5; goto done_sp; }
+  {
+    const int ncl = single_linkage(nseg, seg_linked, seg, work, asg);
+    /* p7_spensemble_Cluster: significant clusters and their endpoints */
+    int nsig = 0;
+    for (int cl = 0; cl < ncl; cl++) {
+      int ninc = 0, last = -1;
+      for (int h = 0; h < nseg; h++) if (asg[h] == cl) { if (seg[h].idx != last) ninc++; last = seg[h].idx; }
+      if ((float) ninc / (float) nsamples < 0.25f) continue;
+      int lo[4] = { 0 }, hi[4] = { 0 }, first = 1;
+      for (int h = 0; h < nseg; h++) if (asg[h] == cl) {
+        const int v4[4] = { seg[h].i, seg[h].j, seg[h].k, seg[h].m };
+        for (int e = 0; e < 4; e++) {
+          if (first || v4[e] < lo[e]) lo[e] = v4[e];
+          if (first || v4[e] > hi[e]) hi[e] = v4[e];
+        }
+        first = 0;
+      }
+      const int thr = (int) ceilf((float) ninc * 0.02f);
+      int best[4];
+      for (int e = 0; e < 4; e++) {                 /* i, j, k, m: leftmost start, rightmost end with support */
+        const int w = hi[e] - lo[e] + 1;
+        int *epc = calloc((size_t) w, sizeof(int));
+        if (! epc) { status = 5; goto done_sp; }
+        for (int h = 0; h < nseg; h++) if (asg[h] == cl) {
+          const int v = e == 0 ? seg[h].i : e == 1 ? seg[h].j : e == 2 ? seg[h].k : seg[h].m;
+          epc[v - lo[e]]++;
+        }
+        int b;
+        if (e == 0 || e == 2) { for (b = lo[e]; b <= hi[e]; b++) if (epc[b - lo[e]] >= thr) break; }
+        else                  { for (b = hi[e]; b >= lo[e]; b--) if (epc[b - lo[e]] >= thr) break; }
+        if (b < lo[e] || b > hi[e]) {                /* esl_vec_IArgMax: the first maximum */
+          int am = 0;
+          for (int a = 1; a < w; a++) if (epc[a] > epc[am]) am = a;
+          b = lo[e] + am;
+        }
+        best[e] = b;
+        free(epc);
+      }
+      if (best[0] > best[1] || best[2] > best[3]) continue;
+      sig[nsig++] = (SPSEG){ cl, best[0], best[1], best[2], best[3], (float) ninc / (float) nsamples };
+    }
+    for (int a = 1; a < nsig; a++) {               /* in start order; equal starts keep their order */
+      SPSEG v = sig[a];
+      int b = a - 1;
+      while (b >= 0 && sig[b].i > v.i) { sig[b+1] = sig[b]; b--; }
+      sig[b+1] = v;
+    }
+    /* domination: of two overlapping by >= 80% of the shorter, the less probable goes */
+    int *dom = calloc((size_t) nsig + 1, sizeof(int));
+    if (! dom) { status = 5; goto done_sp; }
+    for (int d = 0; d < nsig; d++)
+      for (int d2 = d + 1; d2 < nsig; d2++) {
+        const int nov = (sig[d].j < sig[d2].j ? sig[d].j : sig[d2].j) - (sig[d].i > sig[d2].i ? sig[d].i : sig[d2].i) + 1;
+        if (nov == 0) break;
+        const int n1 = sig[d].j - sig[d].i + 1, n2 = sig[d2].j - sig[d2].i + 1;
+        const int n = n1 < n2 ? n1 : n2;
+        if ((double) ((float) nov / (float) n) >= 0.8) {
+          if (sig[d].prob > sig[d2].prob) dom[d2] = 1; else dom[d] = 1;
+        }
+      }
+    for (int d = 0; d < nsig; d++) {
+      if (dom[d]) continue;
+      if (nc < cap) { ci[nc] = sig[d].i; cj[nc] = sig[d].j; }
+      nc++;
+    }
+    free(dom);
+  }
+  *ret_nc = nc;
+done_sp:
+  unstripe(&sp);
+done:
+  free(fx); free(full); free(acc); free(cnt); free(seg); free(tst); free(tk); free(ti); free(work); free(asg); free(sig);
+  return status;
+}
+
+/* Test entry points of the pieces above. orc_single_linkage_1d: the
+ * clustering routine over points x[0..n-1] linked when |x_a - x_b| <= thr
+ * (the rule of esl_cluster.c's own test driver). orc_sample_emissions: ntraces
+ * stochastic traces of a whole sequence (rows 1..L of p7_Forward with xf),
+ * returning the mean residues emitted per trace by M_1..M_M, I_1..I_M, N, C,
+ * J, each divided by L, and the mean number of domains (B states). */
+typedef struct { const double *x; double thr; } PT1D;
+static int pt1d_linked(int a, int b, const void *prm) { const PT1D *p = prm; return fabs(p->x[a] - p->x[b]) <= p->thr; }
+
+int
+orc_single_linkage_1d(const double *x, int n, double thr, int *asg)
+{
+  PT1D p = { x, thr };
+  int *work = malloc(sizeof(int) * 2 * (size_t) (n + 1));
+  if (! work) return -1;
+  const int nc = single_linkage(n, pt1d_linked, &p, work, asg);
+  free(work);
+  return nc;
+}
+
+int
+orc_sample_emissions(const uint8_t *dsq, int L, int M, int Kp, const float *rfv, const float *tfv, const float *xf,
+                     uint32_t seed, int ntraces, double *w, double *ret_ndom)
+{
+  STRIPED sp;
+  TRACE_CTX c;
+  ORC_RNG g;
+  int Q = (M - 1) / 4 + 1, status = 0, tcap = 256;
+  if (Q < 2) Q = 2;
+  float *fx = malloc(sizeof(float) * 6 * (size_t) (L + 1)), sc;
+  float *full = malloc(sizeof(float) * 12 * (size_t) Q * (L + 1));
+  int *tst = malloc(sizeof(int) * tcap), *tk = malloc(sizeof(int) * tcap), *ti = malloc(sizeof(int) * tcap);
+  double ndom = 0.0;
+  memset(w, 0, sizeof(double) * (2 * (size_t) M + 3));
+  if (! fx || ! full || ! tst || ! tk || ! ti || L < 1) { status = 5; goto done; }
+  if ((status = fwd_engine(dsq, L, M, Kp, rfv, tfv, xf, fx, NULL, &sc, NULL, full)) != 0) goto done;
+  if (stripe(M, Kp, rfv, tfv, &sp)) { status = 5; goto done; }
+  c.Q = Q; c.full = full; c.xmx = fx; c.xf = xf; c.sp = &sp;
+  rng_seed(&g, seed);
+  {
+    unsigned csr = _mm_getcsr();
+    _mm_setcsr(csr | 0x8040);
+    for (int t = 0; t < ntraces && ! status; t++) {
+      int N = 0;
+      if ((status = sample_trace(&g, &c, L, &tst, &tk, &ti, &tcap, &N)) != 0) break;
+      for (int z = 0; z < N; z++) {
+        if (tst[z] == TS_B) ndom += 1.0;
+        if (ti[z] == 0) continue;
+        switch (tst[z]) {
+        case TS_M: w[tk[z] - 1] += 1.0; break;
+        case TS_I: w[M + tk[z] - 1] += 1.0; break;
+        case TS_N: w[2*M] += 1.0; break;
+        case TS_C: w[2*M + 1] += 1.0; break;
+        case TS_J: w[2*M + 2] += 1.0; break;
+        }
+      }
+    }
+    _mm_setcsr(csr);
+  }
+  for (int a = 0; a < 2 * M + 3; a++) w[a] /= (double) ntraces * L;
+  *ret_ndom = ndom / ntraces;
+  unstripe(&sp);
+done:
+  free(fx); free(full); free(tst); free(tk); free(ti);
+  return status;
+}
+#undef FV
+#undef TV
+#undef DDV
+#undef XV
 
 /*****************************************************************
  * Null model and bias-composition filter
