@@ -98,6 +98,18 @@ __device__ __forceinline__ float wave_scan_f(float v)
   v += dppz<ROW_BCAST31, 0xc>(v);
   return v;
 }
+// Inclusive scan over the lanes of the maps x -> max(C, x + W), composed in
+// lane order (each lane's map applied after those of the lanes before), for
+// W in {0, -infinity}: the sums are exact. Identity: (-infinity, 0).
+__device__ __forceinline__ void wave_maxplus_scan(float &C, float &W)
+{
+#define P7G_MPS(CTRL, RM) { const float c1 = dppf<CTRL, RM>(-INFINITY, C), w1 = dppf<CTRL, RM>(0.0f, W); \
+                            C = fmaxf(C, W + c1); W = W + w1; }
+  P7G_MPS(ROW_SHR1, 0xf) P7G_MPS(ROW_SHR2, 0xf) P7G_MPS(ROW_SHR4, 0xf) P7G_MPS(ROW_SHR8, 0xf)
+  P7G_MPS(ROW_BCAST15, 0xa) P7G_MPS(ROW_BCAST31, 0xc)
+#undef P7G_MPS
+}
+
 __device__ __forceinline__ int   uni(int v)   { return __builtin_amdgcn_readfirstlane(v); }
 __device__ __forceinline__ float unif(float v){ return __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, v))); }
 

@@ -188,6 +188,33 @@ struct DomOut {
   int64_t  rscale_cap = 0;
 };
 
+// Multidomain regions (p7g_multidomain.hip): one job per region, its copy at
+// res + res_off (Lr residues), the whole target's length, and its area in
+// the pool (md_area_floats); segcap bounds the sampled-domain ensemble.
+#define MD_TSEG   512     // sampled domains in one trace
+#define MD_MAXSIG 64      // significant clusters of one region
+#define MD_MAXC   32      // envelopes one region may give
+struct MdJob {
+  int64_t res_off;
+  int64_t area;
+  int32_t Lr, Ltarget, segcap, pad;
+};
+// status 0, 1 (a traceback choice failed: the reference raises an exception),
+// or 5 (a capacity above was exceeded); nc envelopes ci..cj, 1-based in the
+// region, in start order; nseg the ensemble's size. The region's residues'
+// null2 scores are left at pool + area + md_n2sc_at(M, Lr), residues 1..Lr.
+struct MdOut {
+  int32_t status, nc, nseg, pad;
+  int32_t ci[MD_MAXC], cj[MD_MAXC];
+};
+__host__ __device__ inline int64_t md_n2sc_at(int M, int Lr)
+{ const int64_t Q = (M - 1) / 4 + 1 < 2 ? 2 : (M - 1) / 4 + 1; return (int64_t) (Lr + 1) * 12 * Q + 6 * (int64_t) (Lr + 1); }
+size_t  md_lds_bytes(int M, int segcap);
+int64_t md_area_floats(int M, int Lr, int segcap);
+hipError_t launch_md_regions(const DevProfile &p, const uint8_t *res, const MdJob *jobs, MdOut *out, int njobs,
+                             float *pool, int segcap, int nsamples, uint32_t seed, hipStream_t st);
+hipError_t launch_md_copy(uint8_t *res, const int64_t *src, const int64_t *dst, const int32_t *len, int n, hipStream_t st);
+
 // Length model of one call. on == 0: every target uses its own length
 // (hmmsearch's per-target ReconfigLength); on == 1: the caller's current
 // oprofile/bg scalars are used as they are (single-target drop-ins).

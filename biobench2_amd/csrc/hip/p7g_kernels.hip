@@ -1109,14 +1109,10 @@ __device__ __forceinline__ void oa_row(const DevProfile &p, int z, int i,
     C = fmaxf(cj, pj ? C : NI);
     P &= pj;
   }
-#pragma unroll
-  for (int s = 1; s < 64; s <<= 1) {
-    const float cO = __shfl_up(C, s);
-    const int pO = __shfl_up(P, s);
-    if (z >= s) { C = fmaxf(C, P ? cO : NI); P &= pO; }
-  }
-  float cur = __shfl_up(C, 1);
-  if (z == 0) cur = NI;
+  // x -> max(C, x + W) with W = 0 (pass) or -infinity: an exact max-plus scan
+  float W = P ? 0.0f : NI;
+  wave_maxplus_scan(C, W);
+  float cur = dppf<WAVE_SHR1>(NI, C);
 #pragma unroll
   for (int j = 0; j < B; j++) {
     const int k = z * B + j + 1;
@@ -1138,7 +1134,9 @@ __device__ __forceinline__ void oa_row(const DevProfile &p, int z, int i,
   float V = __int_as_float(wave_max_i(__float_as_int(el)));
   if (refpad) V = fmaxf(V, 0.0f);
   // select_e: the last M cell of value V in the striped order, else the first D cell
-  auto key = [Q](int k) { return ((k - 1) % Q) * 4 + (k - 1) / Q; };
+  auto key = [Q](int k) {                       // k - 1 < 4Q: the lane r by three compares
+    const int r = (k - 1 >= Q) + (k - 1 >= 2 * Q) + (k - 1 >= 3 * Q);
+    return (k - 1 - r * Q) * 4 + r; };
   int km = -1, kd = 0x7fffffff;
 #pragma unroll
   for (int j = 0; j < B; j++) {
@@ -1148,8 +1146,9 @@ __device__ __forceinline__ void oa_row(const DevProfile &p, int z, int i,
   }
   int kmax = wave_max_i(km);
   int fail = 0, es = 0, ek = 0;
-  if (refpad && V == 0.0f)                       // padding M cells hold V too
-    for (int k = M + 1; k <= 4 * Q; k++) if (key(k) > kmax) fail = 1;
+  // padding M cells hold V too, and the last of them (k = 4Q, key 4Q - 1)
+  // comes after every model cell in the striped order
+  if (refpad && V == 0.0f) fail = 1;
   if (kmax >= 0) { ek = (kmax & 3) * Q + (kmax >> 2) + 1; }
   else {
     const int kmin = -wave_max_i(-kd);

@@ -1,0 +1,563 @@
+// Multidomain regions: region_trace_ensemble (HMMER/src/p7_domaindef.c:600-690)
+// as p7_domaindef_ByPosteriorHeuristics runs it for a region that looks like
+// more than one domain (:431-476). One 64-lane workgroup per region:
+//
+//  1. p7_Forward over the region's residues with the profile in multihit
+//     mode, length-configured to the whole target, keeping every row
+//     (fwdback.c:258-465 with do_full). The sampled traces below are chaotic
+//     in these values: a last-bit difference moves a choice, the random
+//     stream then falls out of step, and every later trace differs. So the
+//     matrix is computed in the reference's own operation order: its four
+//     striped lanes (model position k = r*Q + q + 1) are lanes 0..3 here for
+//     the sums that run along q (the row's E sum, the D->D passes), while the
+//     cells' M and I values, which depend only on the row before, are spread
+//     over the 64 lanes. The file is compiled without contraction into fused
+//     multiply-adds and with flushed fp32 denormals, as the reference runs
+//     with FTZ/DAZ set; each operation is written out in the reference's
+//     order.
+//  2. nsamples stochastic traces (impl_sse/stotrace.c:76-329) with Easel's
+//     fast generator reset to the seed (esl_random.c:148-160, 215-230,
+//     289-310). A step's candidate paths are lanes 0..3: one product each,
+//     esl_vec_FNorm's float sum and division, esl_rnd_FChoose's double sums
+//     and divisions, and the first lane whose threshold exceeds the roll.
+//     select_e scans row i in the striped order with a double running sum.
+//     Each sampled domain (p7_trace_Index, p7_trace.c:1173-1206) gets the null2
+//     odds of its own trace segment (p7_Null2_ByTrace, impl_sse/null2.c:134-
+//     205: every emitting M or I state counts at the M cell of its node), and
+//     every residue of the region adds, per trace, that odds or 1 outside
+//     domains; n2sc = logf(sum / nsamples).
+//  3. The sampled domains are clustered by single linkage (esl_cluster.c:
+//     136-185 with link_spsamples, p7_spensemble.c:191-222), clusters with
+//     posterior >= 0.25 get the widest endpoints with >= 2% support
+//     (p7_spensemble.c:282-411), sorted by start, and clusters dominated by an
+//     overlapping one of higher posterior are dropped (p7_domaindef.c:650-682).
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdint.h>
+
+#include <algorithm>
+
+#include "p7g_device.h"
+#include "p7g_devfn.h"
+
+// (also stated here, so that no build of this file contracts)
+#pragma clang fp contract(off)
+
+enum { MS_M = 1, MS_D, MS_I, MS_S, MS_N, MS_B, MS_E, MS_C, MS_T, MS_J };
+
+// Bob Jenkins' 96-bit mix, as esl_random.c disperses a seed
+__device__ __forceinline__ uint32_t md_mix3(uint32_t a, uint32_t b, uint32_t c)
+{
+  a = (a - b - c) ^ (c >> 13);
+  b = (b - c - a) ^ (a << 8);
+  c = (c - a - b) ^ (b >> 13);
+  a = (a - b - c) ^ (c >> 12);
+  b = (b - c - a) ^ (a << 16);
+  c = (c - a - b) ^ (b >> 5);
+  a = (a - b - c) ^ (c >> 3);
+  b = (b - c - a) ^ (a << 10);
+  c = (c - a - b) ^ (b >> 15);
+  return c;
+}
+
+// the device Forward table's value of row `row` at model position k (0 past M)
+__device__ __forceinline__ float md_tv(const DevProfile &p, int row, int k)
+{
+  if (k < 1 || k > p.M) return 0.0f;
+  const int B = p.fb_B, zl = (k - 1) / B, j = (k - 1) - zl * B;
+  return p.fb_tbl[((size_t) row * B + j) * 64 + zl];
+}
+
+// esl_vec_FNorm then esl_rnd_FChoose over the values lanes 0..n-1 hold; the
+// chosen index (uniform), or -1 if no threshold exceeds the roll
+__device__ __forceinline__ int md_choose(float v, int n, uint32_t &rng)
+{
+  const int z = threadIdx.x;
+  float s = 0.0f;
+  for (int a = 0; a < n; a++) s = __fadd_rn(s, readlanef(v, a));
+  const float pn = (s != 0.0f) ? __fdiv_rn(v, s) : (float) (1.0 / (double) (float) n);
+  double norm = 0.0, mine = 0.0;
+  for (int a = 0; a < n; a++) {
+    norm = norm + (double) readlanef(pn, a);
+    if (z == a) mine = norm;
+  }
+  rng = rng * 69069u + 1u;
+  const double roll = (double) rng / 4294967296.0;
+  const unsigned long long m = __ballot(z < n && roll < mine / norm);
+  return m ? (int) __ffsll((long long) m) - 1 : -1;
+}
+
+struct MdSeg { int idx, i, j, k, m; };
+
+// link_spsamples: overlap >= 0.8 of the smaller segment in the sequence and in
+// the model (the model overlap without the +1, as the reference has it), and
+// start or end diagonals within 4
+__device__ __forceinline__ bool md_linked(const MdSeg &a, const MdSeg &b)
+{
+  int nov = min(a.j, b.j) - max(a.i, b.i) + 1;
+  int n = min(a.j - a.i + 1, b.j - b.i + 1);
+  if (__fdiv_rn((float) nov, (float) n) < 0.8f) return false;
+  nov = min(a.m, b.m) - max(a.k, b.k);
+  n = min(a.m - a.k + 1, b.m - b.k + 1);
+  if (__fdiv_rn((float) nov, (float) n) < 0.8f) return false;
+  if (abs((a.i - a.k) - (b.i - b.k)) <= 4) return true;
+  if (abs((a.j - a.m) - (b.j - b.m)) <= 4) return true;
+  return false;
+}
+
+struct MdSig { int c, i, j, k, m; float prob; };
+
+__global__ __launch_bounds__(64) void
+k_md_region(const DevProfile p, const uint8_t *__restrict__ res, const MdJob *__restrict__ jobs, MdOut *__restrict__ out,
+            float *__restrict__ pool, int nsamples, uint32_t seed)
+{
+  extern __shared__ float lds[];
+  const int z = threadIdx.x;
+  const MdJob J = jobs[blockIdx.x];
+  MdOut *O = out + blockIdx.x;
+  const int M = p.M, Q = max(2, (M - 1) / 4 + 1), Lr = J.Lr, L = J.Ltarget;
+  const int TR = p.nrows;                                  // first transition row of the table
+  const float xEmove = p.xfE_move, xEloop = p.xfE_loop, floop = p.xf_loop[L], fmove = p.xf_move[L];
+  const uint8_t *dsq = res + J.res_off;                    // residue i (1-based) at dsq[i-1]
+  float *mat = pool + J.area;                              // rows 0..Lr, [q][s][r], s = 0 M, 1 D, 2 I
+  float *xmx = mat + (size_t) (Lr + 1) * 12 * Q;           // rows 0..Lr: E N J B C SCALE
+  float *acc = xmx + (size_t) 6 * (Lr + 1);                // residues 0..Lr
+  int   *hist = (int *) (acc + (Lr + 1));                  // max(Lr, M) + 2 endpoint counts
+  MdSeg *segs = (MdSeg *) (hist + (max(Lr, M) + 2));       // the ensemble, J.segcap entries
+  int status = 0;
+#define CELL(i_, q_, s_, r_) mat[(((size_t) (i_) * Q + (q_)) * 3 + (s_)) * 4 + (r_)]
+
+  // ---------------- 1. the region's Forward matrix ----------------
+  {
+    float *prev = lds, *cur = lds + 12 * Q, *tdd = lds + 24 * Q;
+    for (int c = z; c < 12 * Q; c += 64) { prev[c] = 0.0f; mat[c] = 0.0f; }
+    for (int c = z; c < 4 * Q; c += 64) tdd[c] = md_tv(p, TR + FT_DD, (c & 3) * Q + (c >> 2) + 1);
+    float xN = 1.0f, xJ = 0.0f, xB = fmove, xC = 0.0f, xE = 0.0f;
+    if (z == 0) { xmx[0] = 0.0f; xmx[1] = xN; xmx[2] = xJ; xmx[3] = xB; xmx[4] = xC; xmx[5] = 1.0f; }
+    __syncthreads();
+    for (int i = 1; i <= Lr; i++) {
+      const int x = dsq[i - 1];
+      for (int c = z; c < 4 * Q; c += 64) {
+        const int q = c >> 2, r = c & 3, k = r * Q + q + 1;
+        float mp = 0.0f, dp = 0.0f, ip = 0.0f;           // row i-1 at k-1 (esl_sse_rightshift_ps at q = 0)
+        if (q > 0)      { mp = prev[((q-1)*3+0)*4 + r]; dp = prev[((q-1)*3+1)*4 + r]; ip = prev[((q-1)*3+2)*4 + r]; }
+        else if (r > 0) { mp = prev[((Q-1)*3+0)*4 + r-1]; dp = prev[((Q-1)*3+1)*4 + r-1]; ip = prev[((Q-1)*3+2)*4 + r-1]; }
+        float sv = __fmul_rn(xB, md_tv(p, TR + VT_BM, k));
+        sv = __fadd_rn(sv, __fmul_rn(mp, md_tv(p, TR + VT_MM, k)));
+        sv = __fadd_rn(sv, __fmul_rn(ip, md_tv(p, TR + VT_IM, k)));
+        sv = __fadd_rn(sv, __fmul_rn(dp, md_tv(p, TR + VT_DM, k)));
+        sv = __fmul_rn(sv, md_tv(p, x, k));
+        cur[(q*3+0)*4 + r] = sv;
+        cur[(q*3+2)*4 + r] = __fadd_rn(__fmul_rn(prev[(q*3+0)*4 + r], md_tv(p, TR + VT_MI, k)),
+                                       __fmul_rn(prev[(q*3+2)*4 + r], md_tv(p, TR + VT_II, k)));
+      }
+      __syncthreads();
+      // D before the D->D passes: cell q holds M(q-1)·tMD (the q loop's dcv); q = 0 holds 0
+      for (int c = z; c < 4 * Q; c += 64) {
+        const int q = c >> 2, r = c & 3, k = r * Q + q + 1;
+        cur[(q*3+1)*4 + r] = q ? __fmul_rn(cur[((q-1)*3+0)*4 + r], md_tv(p, TR + FT_MD, k - 1)) : 0.0f;
+      }
+      __syncthreads();
+      float xe = 0.0f;
+      if (z < 4) {                                         // lane z is SSE lane r
+        const int r = z;
+        for (int q = 0; q < Q; q++) xe = __fadd_rn(xe, cur[(q*3+0)*4 + r]);
+        float dcv = __fmul_rn(cur[((Q-1)*3+0)*4 + r], md_tv(p, TR + FT_MD, r * Q + Q));
+        float sh = __shfl(dcv, (z + 3) & 3, 4);
+        dcv = r ? sh : 0.0f;
+        for (int q = 0; q < Q; q++) {
+          const float d = __fadd_rn(dcv, cur[(q*3+1)*4 + r]);
+          cur[(q*3+1)*4 + r] = d;
+          dcv = __fmul_rn(d, tdd[q*4 + r]);
+        }
+        for (int pass = 1; pass < 4; pass++) {
+          sh = __shfl(dcv, (z + 3) & 3, 4);
+          dcv = r ? sh : 0.0f;
+          int changed = 0;
+          for (int q = 0; q < Q; q++) {
+            const float old = cur[(q*3+1)*4 + r], d = __fadd_rn(dcv, old);
+            changed |= (d > old);
+            cur[(q*3+1)*4 + r] = d;
+            dcv = __fmul_rn(dcv, tdd[q*4 + r]);
+          }
+          const bool any = (__ballot(changed) & 0xfull) != 0ull;
+          if (M >= 100 && ! any) break;
+        }
+        for (int q = 0; q < Q; q++) xe = __fadd_rn(cur[(q*3+1)*4 + r], xe);
+      }
+      // esl_sse_hsum_ps: (e0 + e1) + (e2 + e3)
+      xE = __fadd_rn(__fadd_rn(readlanef(xe, 0), readlanef(xe, 1)), __fadd_rn(readlanef(xe, 2), readlanef(xe, 3)));
+      xN = __fmul_rn(xN, floop);
+      xC = __fadd_rn(__fmul_rn(xC, floop), __fmul_rn(xE, xEmove));
+      xJ = __fadd_rn(__fmul_rn(xJ, floop), __fmul_rn(xE, xEloop));
+      xB = __fadd_rn(__fmul_rn(xJ, fmove), __fmul_rn(xN, fmove));
+      float scale = 1.0f;
+      __syncthreads();
+      if (xE > 1.0e4f) {
+        xN = __fdiv_rn(xN, xE); xC = __fdiv_rn(xC, xE); xJ = __fdiv_rn(xJ, xE); xB = __fdiv_rn(xB, xE);
+        const float inv = (float) (1.0 / (double) xE);
+        for (int c = z; c < 12 * Q; c += 64) cur[c] = __fmul_rn(cur[c], inv);
+        scale = xE;
+        xE = 1.0f;
+        __syncthreads();
+      }
+      float *row = mat + (size_t) i * 12 * Q;
+      for (int c = z; c < 12 * Q; c += 64) row[c] = cur[c];
+      if (z == 0) {
+        float *xr = xmx + (size_t) i * 6;
+        xr[0] = xE; xr[1] = xN; xr[2] = xJ; xr[3] = xB; xr[4] = xC; xr[5] = scale;
+      }
+      float *t = prev; prev = cur; cur = t;
+      __syncthreads();
+    }
+  }
+
+  // ---------------- 2. the sampled traces ----------------
+  float *cnt = lds;                                        // [q][r] counts of one sampled domain
+  float *n2v = lds + 4 * Q;                                // its null2 odds, 32 codes
+  int   *tseg = (int *) (n2v + 32);                        // this trace's domains, last first
+  for (int c = z; c < 4 * Q; c += 64) cnt[c] = 0.0f;
+  for (int pos = z; pos <= Lr; pos += 64) acc[pos] = 0.0f;
+  __syncthreads();
+  uint32_t rng = md_mix3(seed, 87654321u, 12345678u);
+  if (rng == 0) rng = 42;
+  int nseg = 0;
+  const int64_t max_steps = 64 * ((int64_t) Lr + M + 16);
+  auto xv = [&](int i, int c) { return xmx[(size_t) i * 6 + c]; };
+  auto cellk = [&](int i, int k, int s) {                  // the cell of model position k (1..4Q) in row i
+    const int r = (k - 1) / Q, q = (k - 1) - r * Q;
+    return CELL(i, q, s, r); };
+  for (int t = 0; t < nsamples && ! status; t++) {
+    int s0 = MS_C, i = Lr, k = 0, hi = Lr, nd = 0;
+    int sqfrom = 0, sqto = 0, hmmfrom = 0, hmmto = 0, Ld = 0;
+    int64_t steps = 0;
+    while (s0 != MS_S) {
+      if (++steps > max_steps) { status = 1; break; }
+      int s1 = -1;
+      switch (s0) {
+      case MS_M: {                                         // from B(i-1), M/I/D(i-1, k-1)
+        float v = 0.0f;
+        if (z == 0)      v = __fmul_rn(xv(i - 1, 3), md_tv(p, TR + VT_BM, k));
+        else if (z < 4) {
+          const int s = (z == 1) ? 0 : (z == 2) ? 2 : 1;  // M, I, D
+          const int row = (z == 1) ? VT_MM : (z == 2) ? VT_IM : VT_DM;
+          v = __fmul_rn(k > 1 ? cellk(i - 1, k - 1, s) : 0.0f, md_tv(p, TR + row, k));
+        }
+        const int a = md_choose(v, 4, rng);
+        s1 = (a < 0) ? -1 : (a == 0) ? MS_B : (a == 1) ? MS_M : (a == 2) ? MS_I : MS_D;
+        break;
+      }
+      case MS_D: {                                         // from M/D(i, k-1)
+        float v = 0.0f;
+        if (z < 2) v = (k > 1) ? __fmul_rn(cellk(i, k - 1, z), md_tv(p, TR + (z ? VT_DDS : VT_MDS), k)) : 0.0f;
+        const int a = md_choose(v, 2, rng);
+        s1 = (a < 0) ? -1 : a ? MS_D : MS_M;
+        break;
+      }
+      case MS_I: {                                         // from M/I(i-1, k)
+        float v = 0.0f;
+        if (z < 2) v = __fmul_rn(cellk(i - 1, k, z ? 2 : 0), md_tv(p, TR + (z ? VT_II : VT_MI), k));
+        const int a = md_choose(v, 2, rng);
+        s1 = (a < 0) ? -1 : a ? MS_I : MS_M;
+        break;
+      }
+      case MS_N: s1 = (i == 0) ? MS_S : MS_N; break;
+      case MS_C: case MS_J: {
+        if (i < 1) break;                                  // only a degenerate matrix gets here: s1 stays -1
+        const int xs = (s0 == MS_C) ? 4 : 2;
+        float v = 0.0f;
+        if (z == 0) v = __fmul_rn(xv(i - 1, xs), floop);
+        else if (z == 1) v = __fmul_rn(__fmul_rn(xv(i, 0), s0 == MS_C ? xEmove : xEloop), xv(i, 5));
+        const int a = md_choose(v, 2, rng);
+        s1 = (a < 0) ? -1 : a ? MS_E : s0;
+        break;
+      }
+      case MS_B: {
+        float v = 0.0f;
+        if (z < 2) v = __fmul_rn(xv(i, z ? 2 : 1), fmove);
+        const int a = md_choose(v, 2, rng);
+        s1 = (a < 0) ? -1 : a ? MS_J : MS_N;
+        break;
+      }
+      case MS_E: {                                         // one roll; M(i, k), D(i, k) in striped order
+        rng = rng * 69069u + 1u;
+        const double roll = (double) rng / 4294967296.0;
+        const double norm = 1.0 / (double) xv(i, 0);
+        const float nf = (float) norm;
+        double sum = 0.0;
+        for (int pass = 0; pass < 4 && s1 < 0; pass++)
+          for (int q0 = 0; q0 < Q && s1 < 0; q0 += 8) {
+            // lane l: q = q0 + l/8, then M lanes r = 0..3, D lanes r = 0..3
+            const int q = q0 + (z >> 3), s = (z >> 2) & 1, r = z & 3;
+            const float u = (q < Q) ? __fmul_rn(CELL(i, q, s, r), nf) : 0.0f;
+            for (int l = 0; l < 64; l++) {
+              sum = sum + (double) readlanef(u, l);
+              if (roll < sum) {
+                k = (l & 3) * Q + q0 + (l >> 3) + 1;
+                s1 = ((l >> 2) & 1) ? MS_D : MS_M;
+                break;
+              }
+            }
+          }
+        break;
+      }
+      }
+      if (s1 < 0) { status = 1; break; }
+      if (s0 == MS_M) { k--; i--; }
+      else if (s0 == MS_D) k--;
+      else if (s0 == MS_I) i--;
+      // the trace element s1 at (i, k), in backward order
+      if (s1 == MS_M || s1 == MS_I) {
+        if (k < 1 || k > M || i < 1) { status = 1; break; }
+        if (z == 0) cnt[((k - 1) % Q) * 4 + (k - 1) / Q] += 1.0f;
+        Ld++;
+        if (s1 == MS_M) {
+          if (sqto == 0) { sqto = i; hmmto = k; }
+          sqfrom = i; hmmfrom = k;
+        }
+      } else if (s1 == MS_D) {
+        if (k < 1 || k > M) { status = 1; break; }
+      } else if (s1 == MS_E) {
+        sqfrom = sqto = hmmfrom = hmmto = 0; Ld = 0;
+      } else if (s1 == MS_B) {
+        // p7_Null2_ByTrace over the domain: four striped sums over q per
+        // canonical residue, their hsum; degenerate codes the mean
+        __syncthreads();
+        const float nrm = (float) (1.0 / (double) (float) Ld);
+        for (int base = 0; base < p.K * 4; base += 64) {
+          const int xr = base + z, x = xr >> 2, r = xr & 3;
+          float sv = 0.0f;
+          if (x < p.K)
+            for (int q = 0; q < Q; q++) {
+              const float cq = cnt[q*4 + r];
+              if (cq != 0.0f) sv = __fadd_rn(sv, __fmul_rn(__fmul_rn(cq, nrm), md_tv(p, x, r * Q + q + 1)));
+            }
+          // lanes 4x..4x+3 hold x's lane sums
+          const float s01 = __fadd_rn(sv, __shfl_down(sv, 1, 4)), s23 = __shfl_down(s01, 2, 4);
+          if (x < p.K && r == 0) n2v[x] = __fadd_rn(__fadd_rn(s01, s23), 0.0f);
+        }
+        __syncthreads();
+        if (z == 0) {
+          for (int xx = p.K + 1; xx <= p.Kp - 3; xx++) {
+            const uint32_t dm = p.degen[xx];
+            float rr = 0.0f;
+            for (int y = 0; y < p.K; y++) if ((dm >> y) & 1u) rr = __fadd_rn(rr, n2v[y]);
+            n2v[xx] = __fdiv_rn(rr, (float) (dm >> 24));
+          }
+          n2v[p.K] = 1.0f; n2v[p.Kp - 2] = 1.0f; n2v[p.Kp - 1] = 1.0f;
+        }
+        __syncthreads();
+        // residues (sqto, hi] are outside domains (1), (sqfrom, sqto] take the odds
+        for (int pos = z + 1; pos <= hi; pos += 64) {
+          // the lane of a residue is fixed (pos mod 64), so its updates stay in order
+          if (pos > sqto)        acc[pos] = __fadd_rn(acc[pos], 1.0f);
+          else if (pos > sqfrom) acc[pos] = __fadd_rn(acc[pos], n2v[dsq[pos - 1]]);
+        }
+        hi = sqfrom;
+        if (nd >= MD_TSEG) { status = 5; break; }
+        if (z == 0) { tseg[nd*4 + 0] = sqfrom; tseg[nd*4 + 1] = sqto; tseg[nd*4 + 2] = hmmfrom; tseg[nd*4 + 3] = hmmto; }
+        nd++;
+        for (int c = z; c < 4 * Q; c += 64) cnt[c] = 0.0f;
+        __syncthreads();
+      }
+      // N, C and J emit on the transition to themselves (p7_StochasticTrace)
+      if ((s1 == MS_N || s1 == MS_J || s1 == MS_C) && s1 == s0) i--;
+      s0 = s1;
+    }
+    if (status) break;
+    for (int pos = z + 1; pos <= hi; pos += 64) acc[pos] = __fadd_rn(acc[pos], 1.0f);
+    // the trace's domains join the ensemble in sequence order
+    if (nseg + nd > J.segcap) { status = 5; break; }
+    __syncthreads();
+    for (int d = z; d < nd; d += 64) {
+      const int e = nd - 1 - d;
+      segs[nseg + d] = MdSeg{ t, tseg[e*4 + 0], tseg[e*4 + 1], tseg[e*4 + 2], tseg[e*4 + 3] };
+    }
+    nseg += nd;
+    __syncthreads();
+  }
+  if (status) { if (z == 0) { O->status = status; O->nc = 0; O->nseg = nseg; } return; }
+  for (int pos = z + 1; pos <= Lr; pos += 64) acc[pos] = logf(__fdiv_rn(acc[pos], (float) nsamples));
+  __syncthreads();
+
+  // ---------------- 3. clustering ----------------
+  int *av = (int *) lds, *bv = av + J.segcap, *asg = bv + J.segcap, *ninc = asg + J.segcap;
+  unsigned long long *lk = (unsigned long long *) (ninc + J.segcap);   // link bits of one pass
+  MdSig *sig = (MdSig *) (lk + (J.segcap + 63) / 64);
+  const int n = nseg;
+  for (int v = z; v < n; v += 64) { av[v] = n - v - 1; ninc[v] = 0; }
+  __syncthreads();
+  int na = n, nb = 0, ncl = 0;
+  while (na > 0) {
+    na--;
+    if (z == 0) bv[nb] = av[na];
+    nb++;
+    __syncthreads();
+    while (nb > 0) {
+      nb--;
+      const int v = bv[nb];
+      if (z == 0) asg[v] = ncl;
+      const MdSeg sv = segs[v];
+      const int nw = (na + 63) / 64;
+      for (int w = 0; w < nw; w++) {
+        const int a = w * 64 + z;
+        const bool l = (a < na) && md_linked(sv, segs[av[a]]);
+        const unsigned long long m = __ballot(l);
+        if (z == 0) lk[w] = m;
+      }
+      __syncthreads();
+      // the linked slots, highest first: each leaves by taking the top's element
+      for (int w = nw - 1; w >= 0; w--) {
+        unsigned long long m = lk[w];
+        while (m) {
+          const int bit = 63 - __clzll((long long) m);
+          m &= ~(1ull << bit);
+          const int a = w * 64 + bit;
+          if (z == 0) { const int we = av[a]; av[a] = av[na - 1]; bv[nb] = we; }
+          na--; nb++;
+          __syncthreads();
+        }
+      }
+      __syncthreads();
+    }
+    ncl++;
+  }
+  __syncthreads();
+  // posterior of each cluster: the traces with a member (members of one trace are adjacent)
+  for (int h = z; h < n; h += 64) {
+    bool first = true;
+    for (int h2 = h - 1; h2 >= 0 && segs[h2].idx == segs[h].idx; h2--) if (asg[h2] == asg[h]) first = false;
+    if (first) atomicAdd(&ninc[asg[h]], 1);
+  }
+  __syncthreads();
+  int nsig = 0;
+  for (int c = 0; c < ncl; c++) {
+    const int ni = ninc[c];
+    if (__fdiv_rn((float) ni, (float) nsamples) < 0.25f) continue;
+    int lo[4], hi4[4];
+    for (int e = 0; e < 4; e++) { lo[e] = 0x7fffffff; hi4[e] = -1; }
+    for (int h = z; h < n; h += 64) if (asg[h] == c) {
+      const int v4[4] = { segs[h].i, segs[h].j, segs[h].k, segs[h].m };
+      for (int e = 0; e < 4; e++) { lo[e] = min(lo[e], v4[e]); hi4[e] = max(hi4[e], v4[e]); }
+    }
+    for (int e = 0; e < 4; e++) { lo[e] = -wave_max_i(-lo[e]); hi4[e] = wave_max_i(hi4[e]); }
+    const int thr = (int) ceilf(__fmul_rn((float) ni, 0.02f));
+    int best[4];
+    for (int e = 0; e < 4; e++) {                          // i, j, k, m: leftmost start, rightmost end with support
+      const int w = hi4[e] - lo[e] + 1;
+      for (int b = z; b < w; b += 64) hist[b] = 0;
+      __syncthreads();
+      for (int h = z; h < n; h += 64) if (asg[h] == c) {
+        const int v = (e == 0) ? segs[h].i : (e == 1) ? segs[h].j : (e == 2) ? segs[h].k : segs[h].m;
+        atomicAdd(&hist[v - lo[e]], 1);
+      }
+      __syncthreads();
+      // the counts were made by atomics at L2: read them there too
+      auto hcount = [&](int o) { return __hip_atomic_load(&hist[o], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); };
+      int b = -1;
+      const bool up = (e == 0 || e == 2);
+      for (int base = 0; base < w && b < 0; base += 64) {
+        const int o = up ? base + z : w - 1 - base - z;
+        const unsigned long long m = __ballot(o >= 0 && o < w && hcount(o) >= thr);
+        if (m) b = up ? base + (int) __ffsll((long long) m) - 1 : w - 1 - base - ((int) __ffsll((long long) m) - 1);
+      }
+      if (b < 0) {                                         // esl_vec_IArgMax: the first maximum
+        int bc = -1, bi = 0x7fffffff;
+        for (int o = z; o < w; o += 64) { const int h = hcount(o); if (h > bc) { bc = h; bi = o; } }
+        const int mc = wave_max_i(bc);
+        b = -wave_max_i(bc == mc ? -bi : -0x7fffffff);
+      }
+      best[e] = lo[e] + b;
+      __syncthreads();
+    }
+    if (best[0] > best[1] || best[2] > best[3]) continue;
+    if (nsig >= MD_MAXSIG) { status = 5; break; }
+    if (z == 0) sig[nsig] = MdSig{ c, best[0], best[1], best[2], best[3], __fdiv_rn((float) ni, (float) nsamples) };
+    nsig++;
+  }
+  __syncthreads();
+  if (status) { if (z == 0) { O->status = status; O->nc = 0; O->nseg = nseg; } return; }
+  if (z == 0) {
+    for (int a = 1; a < nsig; a++) {                       // start order; equal starts keep their order
+      const MdSig v = sig[a];
+      int b = a - 1;
+      while (b >= 0 && sig[b].i > v.i) { sig[b + 1] = sig[b]; b--; }
+      sig[b + 1] = v;
+    }
+    int dom[MD_MAXSIG];
+    for (int d = 0; d < nsig; d++) dom[d] = 0;
+    for (int d = 0; d < nsig; d++)
+      for (int d2 = d + 1; d2 < nsig; d2++) {
+        const int nov = min(sig[d].j, sig[d2].j) - max(sig[d].i, sig[d2].i) + 1;
+        if (nov == 0) break;
+        const int nn = min(sig[d].j - sig[d].i + 1, sig[d2].j - sig[d2].i + 1);
+        if ((double) __fdiv_rn((float) nov, (float) nn) >= 0.8) {
+          if (sig[d].prob > sig[d2].prob) dom[d2] = 1; else dom[d] = 1;
+        }
+      }
+    int nc = 0;
+    for (int d = 0; d < nsig; d++) {
+      if (dom[d]) continue;
+      if (nc < MD_MAXC) { O->ci[nc] = sig[d].i; O->cj[nc] = sig[d].j; }
+      nc++;
+    }
+    O->status = (nc > MD_MAXC) ? 5 : 0;
+    O->nc = min(nc, MD_MAXC);
+    O->nseg = nseg;
+  }
+#undef CELL
+}
+
+// LDS of one region's workgroup: the Forward's two rows and D->D odds, the
+// traces' counts, odds and per-trace domains, or the clustering's stacks
+size_t md_lds_bytes(int M, int segcap)
+{
+  const size_t Q = (size_t) std::max(2, (M - 1) / 4 + 1);
+  const size_t fwd = 28 * Q * 4;
+  const size_t trc = (4 * Q + 32) * 4 + (size_t) MD_TSEG * 4 * 4;
+  const size_t clu = (size_t) 4 * segcap * 4 + (size_t) (segcap + 63) / 64 * 8 + (size_t) MD_MAXSIG * sizeof(MdSig);
+  return std::max(fwd, std::max(trc, clu));
+}
+
+// floats of one region's area in the pool: matrix, specials, residue sums,
+// endpoint counts, then the ensemble
+int64_t md_area_floats(int M, int Lr, int segcap)
+{
+  const int64_t Q = std::max(2, (M - 1) / 4 + 1);
+  const int64_t f = (int64_t) (Lr + 1) * 12 * Q + 6 * (int64_t) (Lr + 1) + (Lr + 1) + (std::max(Lr, M) + 2)
+                  + (int64_t) segcap * (int64_t) (sizeof(MdSeg) / 4);
+  return (f + 63) & ~(int64_t) 63;
+}
+
+hipError_t launch_md_regions(const DevProfile &p, const uint8_t *res, const MdJob *jobs, MdOut *out, int njobs,
+                             float *pool, int segcap, int nsamples, uint32_t seed, hipStream_t st)
+{
+  if (njobs <= 0) return hipSuccess;
+  const size_t lds = md_lds_bytes(p.M, segcap);
+  if (lds > P7G_LDS_MAX) return hipErrorInvalidValue;
+  hipError_t e = hipFuncSetAttribute((const void *) k_md_region, hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_md_region, dim3(njobs), dim3(64), lds, st, p, res, jobs, out, pool, nsamples, seed);
+  return hipGetLastError();
+}
+
+// Residue copies of the envelopes the clusters define, from their regions'
+// copies: one block per envelope, padded to 16 bytes plus 16 with the pad code
+// as the region copies are.
+__global__ void k_md_copy(uint8_t *__restrict__ res, const int64_t *__restrict__ src, const int64_t *__restrict__ dst,
+                          const int32_t *__restrict__ len, int n)
+{
+  const int e = blockIdx.x;
+  if (e >= n) return;
+  const int Ld = len[e];
+  const int64_t need = (((int64_t) Ld + 15) & ~(int64_t) 15) + 16;
+  for (int64_t b = threadIdx.x; b < need; b += blockDim.x)
+    res[dst[e] + b] = (b < Ld) ? res[src[e] + b] : (uint8_t) P7G_PADCODE;
+}
+
+hipError_t launch_md_copy(uint8_t *res, const int64_t *src, const int64_t *dst, const int32_t *len, int n, hipStream_t st)
+{
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_md_copy, dim3(n), dim3(256), 0, st, res, src, dst, len, n);
+  return hipGetLastError();
+}

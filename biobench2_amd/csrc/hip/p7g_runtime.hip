@@ -14,10 +14,9 @@
 #include <map>
 #include <atomic>
 #include <set>
+#include <chrono>
 #include <algorithm>
 #include <vector>
-#include <vector>
-#include <algorithm>
 
 #include "p7_gpu.h"
 #include "p7g_internal.h"
@@ -60,6 +59,13 @@ struct Work {
   int64_t     n2size = 0;
   float      *n2oa = nullptr;           // optimal accuracy score and alignment of each region
   int4       *n2ali = nullptr;
+  // multidomain regions: per region its job and result, the envelope copies'
+  // source/destination/length, one count block (grown on use)
+  MdJob      *mdjobs = nullptr;
+  MdOut      *mdout = nullptr;
+  int64_t    *mdsrc = nullptr, *mddst = nullptr;
+  int32_t    *mdlen = nullptr, *mdcnt = nullptr;
+  int64_t     md_cap = 0;
   void       *dom_mem = nullptr;
   int64_t     dom_cap = 0;          // slots the per-slot arrays hold
 };
@@ -954,6 +960,8 @@ work_release(Work &w)
   w.n2off = nullptr; w.n2dc = nullptr; w.n2st = nullptr; w.n2list = nullptr; w.n2cnt = nullptr;
   w.n2oa = nullptr; w.n2ali = nullptr;
   w.n2pool = nullptr; w.n2size = 0;
+  if (w.mdjobs) (void) hipFree(w.mdjobs);
+  w.mdjobs = nullptr; w.mdout = nullptr; w.mdsrc = w.mddst = nullptr; w.mdlen = w.mdcnt = nullptr; w.md_cap = 0;
   if (w.mem) (void) hipFree(w.mem);
   if (w.pool) (void) hipFree(w.pool);
   if (w.bnd) (void) hipFree(w.bnd);
@@ -1394,44 +1402,217 @@ stats_accumulate(P7_GPU_STATS &tot, const P7_GPU_STATS &one, bool first_of_call)
   tot.cells_fwd += one.cells_fwd; tot.cells_bck += one.cells_bck;
 }
 
-// The null2 correction of every region that is not multidomain
-// (rescore_isolated_domain, p7_domaindef.c:812-931), after a decoding run
-// left its nregs region records on the device. The host reads the records,
-// gives each such region an area in the null2 pool and groups the regions in
-// batches whose areas fit the pool; per batch, three list launches over the
-// region copies: the envelope Forward writing its row scales, the envelope
-// Backward writing its special and M/I rows, and the Forward again summing
-// the posteriors into null2 and the correction. Multidomain regions (whose
-// null2 comes from sampled traces) get status 2 and no correction.
+// Multidomain regions (p7_domaindef.c:431-476): each region's sampled traces
+// and clusters (k_md_region), in batches whose areas fit the null2 pool.
+// Every cluster becomes a record of its own after the nregs regions (w = 2):
+// its residues copied from the region's copy, its envelope score computed as
+// the regions' were; its correction is the sum of the region's sampled null2
+// scores over the envelope (rescore_isolated_domain with null2 done,
+// p7_domaindef.c:941-943), returned in env_dc, record by record. The region's
+// record keeps the sum over all its residues in reg_sum (the sequence bias
+// counts every residue, p7_pipeline.c:760).
+#define MD_NSAMPLES 200        // ddef->nsamples, p7_domaindef.c:112
+#define MD_SEED     42u        // the pipeline's default --seed, p7_pipeline.c:102
+#define MD_SEGCAP   6144       // sampled domains kept per region
 static int
-null2_regions(Work &w, const DevProfile &p, const DevSeqs &s, int64_t nregs, const GateThresh &g)
+multidomain_regions(Work &w, const DevProfile &p, const DevSeqs &s, std::vector<int4> &rec, std::vector<int32_t> &vlen,
+                    int64_t nregs, const int32_t *h_len, const GateThresh &g, std::vector<float> &env_dc,
+                    std::vector<std::pair<int64_t, float>> &reg_sum)
 {
+  std::vector<int64_t> mdk;
+  for (int64_t k = 0; k < nregs; k++) if (rec[(size_t) k].w == 1 && vlen[(size_t) k] > 0) mdk.push_back(k);
+  if (mdk.empty()) return p7G_OK;
+  if (md_lds_bytes(p.M, MD_SEGCAP) > P7G_LDS_MAX) { p7g_set_error("model too long for multidomain regions (M=%d)", p.M); return p7G_EINVAL; }
+  std::vector<int64_t> voff((size_t) nregs);
+  std::vector<float> eloop((size_t) nregs), emove((size_t) nregs);
+  unsigned long long rcur = 0;
+  HIPCHK(hipMemcpyAsync(voff.data(), w.dout.v_off, 8 * (size_t) nregs, hipMemcpyDeviceToHost, w.st));
+  HIPCHK(hipMemcpyAsync(eloop.data(), w.dout.env_loop, 4 * (size_t) nregs, hipMemcpyDeviceToHost, w.st));
+  HIPCHK(hipMemcpyAsync(emove.data(), w.dout.env_move, 4 * (size_t) nregs, hipMemcpyDeviceToHost, w.st));
+  HIPCHK(hipMemcpyAsync(&rcur, w.l.cursor + 2, 8, hipMemcpyDeviceToHost, w.st));
+  HIPCHK(hipStreamSynchronize(w.st));
+  const int64_t nmd = (int64_t) mdk.size();
+  if (w.md_cap < nmd) {
+    if (w.mdjobs) { (void) hipFree(w.mdjobs); w.mdjobs = nullptr; w.md_cap = 0; }
+    const int64_t c = std::max<int64_t>(nmd, 256), ce = c * MD_MAXC;
+    char *m = nullptr;
+    HIPCHK(hipMalloc((void **) &m, (size_t) c * (sizeof(MdJob) + sizeof(MdOut)) + (size_t) ce * 20 + 256 * 4));
+    w.mdjobs = (MdJob *) m; w.mdout = (MdOut *) (w.mdjobs + c);
+    w.mdsrc = (int64_t *) (w.mdout + c); w.mddst = w.mdsrc + ce;
+    w.mdlen = (int32_t *) (w.mddst + ce); w.mdcnt = w.mdlen + ce;
+    w.md_cap = c;
+  }
+  // batches of regions whose areas fit the pool
+  size_t fr = 0, tot = 0;
+  HIPCHK(hipMemGetInfo(&fr, &tot));
+  int64_t need_all = 0;
+  for (int64_t k : mdk) need_all += md_area_floats(p.M, vlen[(size_t) k], MD_SEGCAP);
+  const int64_t cap = std::max<int64_t>(N2_POOL_MIN, std::min<int64_t>(need_all, (int64_t) (fr / 4 / sizeof(float)) + w.n2size));
+  std::vector<MdJob> jobs;
+  std::vector<std::pair<size_t, size_t>> batches;
+  int64_t used = 0, maxused = 0;
+  size_t start = 0;
+  for (int64_t k : mdk) {
+    const int Lr = vlen[(size_t) k];
+    const int64_t need = md_area_floats(p.M, Lr, MD_SEGCAP);
+    if (need > cap) { p7g_set_error("multidomain region of %d residues too long for the pool", Lr); return p7G_EMEM; }
+    if (used + need > cap || (int64_t) (jobs.size() - start) >= w.md_cap) { batches.push_back({ start, jobs.size() }); start = jobs.size(); used = 0; }
+    MdJob J;
+    J.res_off = voff[(size_t) k]; J.area = used; J.Lr = Lr; J.Ltarget = h_len[rec[(size_t) k].x]; J.segcap = MD_SEGCAP; J.pad = 0;
+    jobs.push_back(J);
+    used += need; maxused = std::max(maxused, used);
+  }
+  if (jobs.size() > start) batches.push_back({ start, jobs.size() });
+  if (w.n2size < maxused) {
+    if (w.n2pool) { (void) hipFree(w.n2pool); w.n2pool = nullptr; w.n2size = 0; }
+    HIPCHK(hipMalloc((void **) &w.n2pool, sizeof(float) * (size_t) maxused));
+    w.n2size = maxused;
+  }
+  std::vector<MdOut> outs(jobs.size());
+  std::vector<float> n2;
+  std::vector<int64_t> src, dst;
+  std::vector<int32_t> len;
+  for (const auto &b : batches) {
+    const int nb = (int) (b.second - b.first);
+    int64_t tot_res = 0;
+    for (size_t a = b.first; a < b.second; a++) tot_res += jobs[a].Lr;
+    n2.assign((size_t) tot_res, 0.0f);
+    HIPCHK(hipMemcpyAsync(w.mdjobs, jobs.data() + b.first, sizeof(MdJob) * (size_t) nb, hipMemcpyHostToDevice, w.st));
+    HIPCHK(launch_md_regions(p, w.dout.res, w.mdjobs, w.mdout, nb, w.n2pool, MD_SEGCAP, MD_NSAMPLES, MD_SEED, w.st));
+    HIPCHK(hipMemcpyAsync(outs.data() + b.first, w.mdout, sizeof(MdOut) * (size_t) nb, hipMemcpyDeviceToHost, w.st));
+    int64_t o = 0;
+    for (size_t a = b.first; a < b.second; a++) {
+      HIPCHK(hipMemcpyAsync(n2.data() + o, w.n2pool + jobs[a].area + md_n2sc_at(p.M, jobs[a].Lr) + 1,
+                            sizeof(float) * (size_t) jobs[a].Lr, hipMemcpyDeviceToHost, w.st));
+      o += jobs[a].Lr;
+    }
+    HIPCHK(hipStreamSynchronize(w.st));
+    o = 0;
+    for (size_t a = b.first; a < b.second; a++) {
+      const MdOut &r = outs[a];
+      const int64_t k = mdk[a];
+      const int Lr = jobs[a].Lr;
+      if (r.status == 1) { p7g_set_error("stochastic traceback failed in a multidomain region"); return p7G_EINVAL; }
+      if (r.status != 0) { p7g_set_error("multidomain region exceeds the sampled-domain capacity"); return p7G_EMEM; }
+      const float *rn = n2.data() + o;
+      float rs = 0.0f;
+      for (int pos = 0; pos < Lr; pos++) rs += rn[pos];
+      reg_sum.push_back({ k, rs });
+      for (int c = 0; c < r.nc; c++) {
+        const int ci = r.ci[c], cj = r.cj[c];
+        if (ci < 1 || cj > Lr || ci > cj) { p7g_set_error("internal: cluster outside its region"); return p7G_EINVAL; }
+        float dc = 0.0f;
+        for (int pos = ci; pos <= cj; pos++) dc += rn[pos - 1];
+        const int4 &pr = rec[(size_t) k];
+        rec.push_back(make_int4(pr.x, pr.y + ci - 1, pr.y + cj - 1, 2));
+        vlen.push_back(cj - ci + 1);
+        env_dc.push_back(dc);
+        src.push_back(voff[(size_t) k] + ci - 1);
+        len.push_back(cj - ci + 1);
+        eloop.push_back(eloop[(size_t) k]); emove.push_back(emove[(size_t) k]);
+      }
+      o += Lr;
+    }
+  }
+  const int64_t ne = (int64_t) src.size();
+  if (ne == 0) return p7G_OK;
+  if (nregs + ne > w.dout.reg_cap || ne > w.md_cap * MD_MAXC) { p7g_set_error("too many domain envelopes"); return p7G_EMEM; }
+  // the envelopes' residue copies after the regions'
+  int64_t cur = (int64_t) rcur;
+  for (int64_t e = 0; e < ne; e++) {
+    dst.push_back(cur);
+    cur += ((((int64_t) len[(size_t) e]) + 15) & ~(int64_t) 15) + 16;
+  }
+  if (cur > w.dout.res_cap) { p7g_set_error("domain decoding region residue pool exhausted"); return p7G_EMEM; }
+  std::vector<int32_t> vl((size_t) ne);
+  std::vector<int4> nrec(rec.begin() + nregs, rec.end());
+  for (int64_t e = 0; e < ne; e++) vl[(size_t) e] = (int32_t) (nregs + e);
+  const unsigned long long ncur = (unsigned long long) cur;
+  int32_t cnt[16] = { (int32_t) ne };
+  HIPCHK(hipMemcpyAsync(w.mdsrc, src.data(), 8 * (size_t) ne, hipMemcpyHostToDevice, w.st));
+  HIPCHK(hipMemcpyAsync(w.mddst, dst.data(), 8 * (size_t) ne, hipMemcpyHostToDevice, w.st));
+  HIPCHK(hipMemcpyAsync(w.mdlen, len.data(), 4 * (size_t) ne, hipMemcpyHostToDevice, w.st));
+  HIPCHK(hipMemcpyAsync(w.mdcnt, cnt, sizeof(cnt), hipMemcpyHostToDevice, w.st));
+  HIPCHK(hipMemcpyAsync(w.dout.regs + nregs, nrec.data(), sizeof(int4) * (size_t) ne, hipMemcpyHostToDevice, w.st));
+  HIPCHK(hipMemcpyAsync(w.dout.v_off + nregs, dst.data(), 8 * (size_t) ne, hipMemcpyHostToDevice, w.st));
+  HIPCHK(hipMemcpyAsync(w.dout.v_len + nregs, len.data(), 4 * (size_t) ne, hipMemcpyHostToDevice, w.st));
+  HIPCHK(hipMemcpyAsync(w.dout.v_list + nregs, vl.data(), 4 * (size_t) ne, hipMemcpyHostToDevice, w.st));
+  HIPCHK(hipMemcpyAsync(w.dout.env_loop + nregs, eloop.data() + nregs, 4 * (size_t) ne, hipMemcpyHostToDevice, w.st));
+  HIPCHK(hipMemcpyAsync(w.dout.env_move + nregs, emove.data() + nregs, 4 * (size_t) ne, hipMemcpyHostToDevice, w.st));
+  HIPCHK(hipMemcpyAsync(w.l.cursor + 2, &ncur, 8, hipMemcpyHostToDevice, w.st));
+  HIPCHK(launch_md_copy(w.dout.res, w.mdsrc, w.mddst, w.mdlen, (int) ne, w.st));
+  // their envelope scores and row scales, as the decoding step scores regions
+  DevSeqs vs = s;
+  vs.res = w.dout.res; vs.off = w.dout.v_off; vs.len = w.dout.v_len; vs.perm = nullptr;
+  vs.seg = nullptr; vs.nseg = 0; vs.nseg_slots = 0; vs.nslot = w.dout.reg_cap;
+  DevResults vr = w.r;
+  vr.fwdsc = w.dout.envsc; vr.fstatus = w.dout.env_st;
+  DomBufs eb;
+  eb.env_loop = w.dout.env_loop; eb.env_move = w.dout.env_move; eb.env_cap = w.dout.reg_cap;
+  DevLists lr = w.l;
+  lr.cursor = w.l.cursor + 3;
+  const LenCfg lc = lencfg_per_target();
+  HIPCHK(launch_fwd(p, vs, w.dout.v_list + nregs, w.mdcnt, ne, vr, lr, g, 0, lc, w.dout.rscale,
+                    w.dout.rscale_cap, w.dout.rsoff, nullptr, w.st, eb));
+  int32_t ovf = 0;
+  HIPCHK(hipMemcpyAsync(&ovf, w.l.counts + 6, 4, hipMemcpyDeviceToHost, w.st));
+  HIPCHK(hipStreamSynchronize(w.st));               // the host vectors above are the copies' sources
+  if (ovf) { p7g_set_error("envelope row-scale pool exhausted"); return p7G_EMEM; }
+  return p7G_OK;
+}
+
+// Rescoring of the regions after a decoding run left its nregs region
+// records on the device (rescore_isolated_domain, p7_domaindef.c:812-947):
+// multidomain regions first give their clusters' envelopes as records of
+// their own (multidomain_regions). Then every envelope (a single-domain
+// region, w = 0, or a cluster, w = 2) gets its null2 correction and optimal
+// accuracy alignment: the host gives each an area in the null2 pool and
+// groups them in batches whose areas fit the pool; per batch, three list
+// launches over the copies: the envelope Backward writing its special and M/I
+// rows, the Forward again summing the posteriors into null2 and the
+// correction and filling the alignment, and the alignment's traceback. A
+// cluster's correction is then replaced by the sum of its region's sampled
+// null2 scores; a multidomain region's record keeps status 2 and, as its
+// correction, the sum over all its residues. *ret_nrec: records in all.
+static int
+null2_regions(Work &w, const DevProfile &p, const DevSeqs &s, int64_t nregs, const GateThresh &g, const int32_t *h_len,
+              int64_t *ret_nrec, double *ret_ms_md)
+{
+  *ret_nrec = nregs;
+  *ret_ms_md = 0.0;
   if (nregs <= 0) return p7G_OK;
-  HIPCHK(launch_fill32((uint32_t *) w.n2st, 2u, nregs, w.st));
-  HIPCHK(launch_fill32((uint32_t *) w.n2dc, 0x7fc00000u, nregs, w.st));
-  HIPCHK(launch_fill32((uint32_t *) w.n2oa, 0x7fc00000u, nregs, w.st));
-  HIPCHK(launch_fill32((uint32_t *) w.n2ali, 0u, 4 * nregs, w.st));
   std::vector<int4> rec((size_t) nregs);
   std::vector<int32_t> vlen((size_t) nregs);
   HIPCHK(hipMemcpyAsync(rec.data(), w.dout.regs, sizeof(int4) * (size_t) nregs, hipMemcpyDeviceToHost, w.st));
   HIPCHK(hipMemcpyAsync(vlen.data(), w.dout.v_len, sizeof(int32_t) * (size_t) nregs, hipMemcpyDeviceToHost, w.st));
   HIPCHK(hipStreamSynchronize(w.st));
-  // the pool: what the regions need, bounded by a quarter of the free memory
+  std::vector<float> env_dc;
+  std::vector<std::pair<int64_t, float>> reg_sum;
+  int status;
+  const auto t0 = std::chrono::steady_clock::now();
+  if ((status = multidomain_regions(w, p, s, rec, vlen, nregs, h_len, g, env_dc, reg_sum)) != p7G_OK) return status;
+  *ret_ms_md = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  const int64_t nrec = (int64_t) rec.size();
+  HIPCHK(launch_fill32((uint32_t *) w.n2st, 2u, nrec, w.st));
+  HIPCHK(launch_fill32((uint32_t *) w.n2dc, 0x7fc00000u, nrec, w.st));
+  HIPCHK(launch_fill32((uint32_t *) w.n2oa, 0x7fc00000u, nrec, w.st));
+  HIPCHK(launch_fill32((uint32_t *) w.n2ali, 0u, 4 * nrec, w.st));
+  // the pool: what the envelopes need, bounded by a quarter of the free memory
   // (the pool already held counts as free)
   int64_t need_all = 0;
-  for (int64_t k = 0; k < nregs; k++)
-    if (! rec[(size_t) k].w) need_all += n2_area_floats(vlen[(size_t) k], p.fb_B);
+  for (int64_t k = 0; k < nrec; k++)
+    if (rec[(size_t) k].w != 1) need_all += n2_area_floats(vlen[(size_t) k], p.fb_B);
   size_t fr = 0, tot = 0;
   HIPCHK(hipMemGetInfo(&fr, &tot));
   const int64_t cap = std::max<int64_t>(N2_POOL_MIN, std::min<int64_t>(need_all, (int64_t) (fr / 4 / sizeof(float)) + w.n2size));
-  std::vector<int64_t> off((size_t) nregs, 0);
+  std::vector<int64_t> off((size_t) nrec, 0);
   std::vector<int32_t> list;
   std::vector<std::pair<size_t, size_t>> batches;
   int64_t used = 0, maxused = 0;
   size_t start = 0;
-  for (int64_t k = 0; k < nregs; k++) {
-    if (rec[(size_t) k].w) continue;
-    // the kernels size a region by its copy's length, which is its i..j
+  for (int64_t k = 0; k < nrec; k++) {
+    if (rec[(size_t) k].w == 1) continue;
+    // the kernels size an envelope by its copy's length, which is its i..j
     const int Ld = vlen[(size_t) k];
     if (Ld != rec[(size_t) k].z - rec[(size_t) k].y + 1) { p7g_set_error("internal: region copy length"); return p7G_EINVAL; }
     const int64_t need = n2_area_floats(Ld, p.fb_B);
@@ -1441,8 +1622,8 @@ null2_regions(Work &w, const DevProfile &p, const DevSeqs &s, int64_t nregs, con
     list.push_back((int32_t) k);
   }
   if (list.size() > start) batches.push_back({ start, list.size() });
-  if (batches.empty()) return p7G_OK;
   if ((int64_t) batches.size() > N2_MAX_BATCH) { p7g_set_error("too many null2 batches (%zu)", batches.size()); return p7G_EMEM; }
+  if (! batches.empty()) {
   if (w.n2size < maxused) {
     if (w.n2pool) { (void) hipFree(w.n2pool); w.n2pool = nullptr; w.n2size = 0; }
     HIPCHK(hipMalloc((void **) &w.n2pool, sizeof(float) * (size_t) maxused));
@@ -1451,7 +1632,7 @@ null2_regions(Work &w, const DevProfile &p, const DevSeqs &s, int64_t nregs, con
   std::vector<int32_t> cnt(batches.size() * 3 * 16, 0);
   for (size_t b = 0; b < batches.size(); b++)
     for (int a = 0; a < 3; a++) cnt[(b * 3 + a) * 16] = (int32_t) (batches[b].second - batches[b].first);
-  HIPCHK(hipMemcpyAsync(w.n2off, off.data(), sizeof(int64_t) * (size_t) nregs, hipMemcpyHostToDevice, w.st));
+  HIPCHK(hipMemcpyAsync(w.n2off, off.data(), sizeof(int64_t) * (size_t) nrec, hipMemcpyHostToDevice, w.st));
   HIPCHK(hipMemcpyAsync(w.n2list, list.data(), sizeof(int32_t) * list.size(), hipMemcpyHostToDevice, w.st));
   HIPCHK(hipMemcpyAsync(w.n2cnt, cnt.data(), sizeof(int32_t) * cnt.size(), hipMemcpyHostToDevice, w.st));
   DevSeqs vs = s;
@@ -1477,7 +1658,20 @@ null2_regions(Work &w, const DevProfile &p, const DevSeqs &s, int64_t nregs, con
     HIPCHK(launch_fwd(p, vs, bl, c + 32, nb, vf, w.l, g, 0, lc, nullptr, 0, nullptr, nullptr, w.st, eb));
     HIPCHK(launch_oa_trace(p, vs, bl, nb, eb, w.st));
   }
+  }
+  if (! env_dc.empty() || ! reg_sum.empty()) {
+    // corrections from the sampled traces: clusters' over their envelopes,
+    // multidomain regions' over all their residues
+    std::vector<float> dc((size_t) nrec);
+    HIPCHK(hipMemcpyAsync(dc.data(), w.n2dc, 4 * (size_t) nrec, hipMemcpyDeviceToHost, w.st));
+    HIPCHK(hipStreamSynchronize(w.st));
+    for (size_t e = 0; e < env_dc.size(); e++) dc[(size_t) nregs + e] = env_dc[e];
+    for (const auto &r : reg_sum) dc[(size_t) r.first] = r.second;
+    HIPCHK(hipMemcpyAsync(w.n2dc, dc.data(), 4 * (size_t) nrec, hipMemcpyHostToDevice, w.st));
+    HIPCHK(hipStreamSynchronize(w.st));
+  }
   HIPCHK(hipStreamSynchronize(w.st));          // the host vectors above are the copies' sources
+  *ret_nrec = nrec;
   return p7G_OK;
 }
 
@@ -1505,6 +1699,8 @@ p7_gpu_Pipeline(P7_GPU_PIPELINE *pli, const P7_GPU_OPROFILE *om, const P7_GPU_SE
   unsigned long long sums[8];
   int32_t counts[16];
   int32_t nclu_total = 0, nregs_dev = 0;
+  int64_t nrecs = 0;                      // region records, with the clusters' envelopes
+  double ms_md = 0.0;                     // host time of the multidomain regions (they synchronize)
   for (int attempt = 0; ; attempt++) {
   if ((status = chain_reset(w, n)) != p7G_OK) return status;
   if (dom) {                              // targets that do not reach decoding: NaN, no regions
@@ -1563,12 +1759,15 @@ p7_gpu_Pipeline(P7_GPU_PIPELINE *pli, const P7_GPU_OPROFILE *om, const P7_GPU_SE
     }
     if (pli->do_null2) {
       HIPCHK(hipEventRecord(w.ev[8], w.st));
-      if ((status = null2_regions(w, *p, s, nregs_dev, g)) != p7G_OK) return status;
+      int64_t nrec = nregs_dev;
+      if ((status = null2_regions(w, *p, s, nregs_dev, g, db->h_len, &nrec, &ms_md)) != p7G_OK) return status;
+      nrecs = nrec;
       HIPCHK(hipEventRecord(w.ev[9], w.st));
       HIPCHK(hipEventSynchronize(w.ev[9]));
     }
     std::lock_guard<std::mutex> rl(pli->regs_mu);
-    pli->nregs = nregs_dev; pli->regs_dev = w.dout.regs; pli->envsc_dev = w.dout.envsc; pli->regs_perm = db->h_perm;
+    if (nrecs < nregs_dev) nrecs = nregs_dev;
+    pli->nregs = nrecs; pli->regs_dev = w.dout.regs; pli->envsc_dev = w.dout.envsc; pli->regs_perm = db->h_perm;
     pli->n2dc_dev = pli->do_null2 ? w.n2dc : nullptr; pli->n2st_dev = w.n2st;
     pli->oa_dev = pli->do_null2 ? w.n2oa : nullptr; pli->ali_dev = pli->do_null2 ? w.n2ali : nullptr;
     pli->fwdsc_dev = w.r.fwdsc; pli->regs_len = db->h_len; pli->nslots = n;
@@ -1595,7 +1794,8 @@ p7_gpu_Pipeline(P7_GPU_PIPELINE *pli, const P7_GPU_OPROFILE *om, const P7_GPU_SE
     st.ms_domains = mtot;
     st.ms_null2 = 0.0;
     if (pli->do_null2) { HIPCHK(hipEventElapsedTime(&mtot, w.ev[8], w.ev[9])); st.ms_null2 = mtot; }
-    st.n_regions = pli->nregs;
+    st.ms_multidomain = ms_md;
+    st.n_regions = nregs_dev;
     st.n_clustered = nclu_total;
   }
   return p7G_OK;
@@ -1650,7 +1850,7 @@ regs_fetch(const P7_GPU_PIPELINE *pli)
     for (int64_t k = 0; k < pli->nregs; k++) ord[(size_t) k] = k;
     std::sort(ord.begin(), ord.end(), [&](int64_t a, int64_t b) {
       const int4 &x = rec[(size_t) a], &y = rec[(size_t) b];
-      return x.x != y.x ? x.x < y.x : x.y < y.y; });
+      return x.x != y.x ? x.x < y.x : x.y != y.y ? x.y < y.y : x.w < y.w; });
     pli->regs.resize(nr); pli->envsc.resize(nr); pli->n2dc.resize(nr); pli->n2st.resize(nr); pli->regs_slot.resize(nr);
     pli->oasc.resize(nr); pli->ali.resize(nr);
     for (size_t k = 0; k < nr; k++) {
@@ -1750,24 +1950,29 @@ hit_scores(const P7_GPU_PIPELINE *pli)
     while (b < nr && pli->regs[b].x == pli->regs[a].x) b++;    // regions a..b-1: one target
     const int slot = pli->regs_slot[a];
     const int L = pli->regs_len[slot];
-    bool multi = false;
+    // domains: single-domain regions (w 0) and multidomain regions' clusters
+    // (w 2) whose rescoring succeeded; the sequence bias sums the null2 scores
+    // of every residue: the single-domain envelopes' and, for a multidomain
+    // region (w 1), all of its residues' sampled ones
     std::vector<P7G_DOMIN> dom;
     std::vector<size_t> which;
+    std::vector<float> parts;
     for (size_t k = a; k < b; k++) {
-      if (pli->regs[k].w) { multi = true; continue; }
+      const int kind = pli->regs[k].w;
+      if (kind == 1) { parts.push_back(pli->n2dc[k]); continue; }
       if (pli->n2st[k] != 0) continue;                      // rescoring failed: no domain (p7_domaindef.c:841-843)
+      if (kind == 0) parts.push_back(pli->n2dc[k]);
       dom.push_back(P7G_DOMIN{ pli->envsc[k], pli->n2dc[k], pli->regs[k].y, pli->regs[k].z });
       which.push_back(k);
     }
     P7G_TSCORE t{ NAN, NAN, NAN, NAN, NAN, NAN, -1 };
     std::vector<P7G_DSCORE> d(dom.size());
     p7g_target_scores(L, fwd[(size_t) slot], p7g_null_one(L), omega, pli->ftau, pli->flambda, dom.data(), (int) dom.size(),
-                      &t, d.data());
+                      parts.data(), (int) parts.size(), &t, d.data());
     for (size_t q = 0; q < dom.size(); q++) ds[which[q]] = d[q];
-    if (multi) t = P7G_TSCORE{ NAN, NAN, NAN, NAN, NAN, NAN, -1 };
-    else if (! dom.empty()) t.best = (int) (which[(size_t) t.best] - a);
+    if (! dom.empty()) t.best = (int) (which[(size_t) t.best] - a);
     else t.best = -1;
-    tseq.push_back(pli->regs[a].x); ts.push_back(t); tnd.push_back((int32_t) dom.size()); tst.push_back(multi ? 2 : 0);
+    tseq.push_back(pli->regs[a].x); ts.push_back(t); tnd.push_back((int32_t) dom.size()); tst.push_back(0);
     a = b;
   }
   pli->hits_ready = true;
@@ -2678,5 +2883,57 @@ p7_gpu_bg_FilterScore(P7_GPU_BG *bg, const uint8_t *dsq, int L, float *ret_sc)
     (void) hipFree(d_eo);
   }
   *ret_sc = logsc + (float) L * logf(bg->p1) + logf(1.-bg->p1);
+  return p7G_OK;
+}
+
+// region_trace_ensemble for one region (p7_domaindef.c:600-690, as
+// p7_domaindef_ByPosteriorHeuristics calls it at :440-446): k_md_region over
+// residues dsq[1..Lr] of a target of Ltarget residues.
+extern "C" int
+p7_gpu_RegionTraceEnsemble(const uint8_t *dsq, int Lr, int Ltarget, const P7_GPU_OPROFILE *om, int nsamples,
+                           uint32_t seed, int32_t *ci, int32_t *cj, int cap, int32_t *ret_nc, float *n2sc,
+                           float *opt_mx, float *opt_xmx, int32_t *opt_segs, int segcap, int32_t *ret_nseg)
+{
+  std::lock_guard<std::mutex> lk(g_lock);
+  int status;
+  DevProfile *p;
+  if ((status = ensure_init()) != p7G_OK) return status;
+  if (Lr < 1 || Ltarget < Lr || nsamples < 1 || ! ret_nc || ! n2sc) { p7g_set_error("RegionTraceEnsemble: bad arguments"); return p7G_EINVAL; }
+  if ((status = check_dsq(dsq, Lr, om)) != p7G_OK) return status;
+  if (segcap <= 0) segcap = MD_SEGCAP;
+  if ((status = dev_profile(om, Ltarget, &p)) != p7G_OK) return status;
+  if (md_lds_bytes(p->M, segcap) > P7G_LDS_MAX) { p7g_set_error("model too long for multidomain regions (M=%d)", p->M); return p7G_EINVAL; }
+  const int64_t Q = std::max(2, (p->M - 1) / 4 + 1);
+  const int64_t area = md_area_floats(p->M, Lr, segcap);
+  const size_t resb = ((size_t) Lr + 31) & ~(size_t) 15;
+  char *m = nullptr;
+  HIPCHK(hipMalloc((void **) &m, sizeof(float) * (size_t) area + resb + sizeof(MdJob) + sizeof(MdOut) + 512));
+  float *pool = (float *) m;
+  uint8_t *res = (uint8_t *) (pool + area);
+  MdJob *job = (MdJob *) (res + resb);
+  MdOut *out = (MdOut *) (job + 1);
+  MdJob J;
+  J.res_off = 0; J.area = 0; J.Lr = Lr; J.Ltarget = Ltarget; J.segcap = segcap; J.pad = 0;
+  MdOut O;
+  hipError_t e = hipSuccess;
+  if (e == hipSuccess) e = hipMemcpyAsync(res, dsq + 1, (size_t) Lr, hipMemcpyHostToDevice, g_stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(job, &J, sizeof(J), hipMemcpyHostToDevice, g_stream);
+  if (e == hipSuccess) e = launch_md_regions(*p, res, job, out, 1, pool, segcap, nsamples, seed, g_stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(&O, out, sizeof(O), hipMemcpyDeviceToHost, g_stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(n2sc, pool + md_n2sc_at(p->M, Lr) + 1, sizeof(float) * (size_t) Lr, hipMemcpyDeviceToHost, g_stream);
+  if (e == hipSuccess && opt_mx) e = hipMemcpyAsync(opt_mx, pool, sizeof(float) * (size_t) (Lr + 1) * 12 * Q, hipMemcpyDeviceToHost, g_stream);
+  if (e == hipSuccess && opt_xmx) e = hipMemcpyAsync(opt_xmx, pool + (size_t) (Lr + 1) * 12 * Q, sizeof(float) * 6 * (size_t) (Lr + 1), hipMemcpyDeviceToHost, g_stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(g_stream);
+  if (e == hipSuccess && opt_segs && O.nseg > 0) {
+    const int64_t segs_at = md_n2sc_at(p->M, Lr) + (Lr + 1) + (std::max(Lr, p->M) + 2);
+    e = hipMemcpy(opt_segs, pool + segs_at, sizeof(int32_t) * 5 * (size_t) std::min(O.nseg, segcap), hipMemcpyDeviceToHost);
+  }
+  (void) hipFree(m);
+  if (e != hipSuccess) { p7g_set_error("HIP error %d (%s)", (int) e, hipGetErrorString(e)); return p7G_EDEVICE; }
+  *ret_nc = O.nc;
+  if (ret_nseg) *ret_nseg = O.nseg;
+  for (int c = 0; c < std::min(O.nc, cap); c++) { if (ci) ci[c] = O.ci[c]; if (cj) cj[c] = O.cj[c]; }
+  if (O.status == 1) { p7g_set_error("stochastic traceback failed"); return p7G_EINVAL; }
+  if (O.status != 0) { p7g_set_error("region exceeds the sampled-domain capacity"); return p7G_EMEM; }
   return p7G_OK;
 }

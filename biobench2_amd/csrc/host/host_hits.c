@@ -3,9 +3,9 @@
  * computes them after domain definition (HMMER/src/p7_pipeline.c:743-850),
  * with p7_FLogsum (HMMER/src/logsum.c:59-112) and esl_exp_logsurv
  * (easel/esl_exponential.c:141-145). Scores are float and P-values double,
- * as in the reference; the sequence bias sums the domains' corrections in
- * domain order (the reference sums its per-residue null2 scores, which are
- * zero outside the envelopes, in residue order).
+ * as in the reference; the sequence bias sums the per-residue null2 scores
+ * grouped by envelope (and multidomain region) in sequence order, where the
+ * reference sums them residue by residue (they are zero elsewhere).
  */
 #include <inttypes.h>
 #include <math.h>
@@ -55,10 +55,11 @@ p7g_null_one(int L)
 
 void
 p7g_target_scores(int L, float fwdsc, float nullsc, float omega, float tau, float lambda,
-                  const P7G_DOMIN *dom, int ndom, P7G_TSCORE *ts, P7G_DSCORE *ds)
+                  const P7G_DOMIN *dom, int ndom, const float *n2parts, int nn2, P7G_TSCORE *ts, P7G_DSCORE *ds)
 {
   float seqbias = 0.0f;
-  for (int d = 0; d < ndom; d++) seqbias += dom[d].domcorr;
+  if (n2parts) for (int a = 0; a < nn2; a++) seqbias += n2parts[a];
+  else         for (int d = 0; d < ndom; d++) seqbias += dom[d].domcorr;
   seqbias = p7g_flogsum(0.0f, log(omega) + seqbias);
   float pre_score = (fwdsc - nullsc) / CONST_LOG2;
   float seq_score = (fwdsc - (nullsc + seqbias)) / CONST_LOG2;

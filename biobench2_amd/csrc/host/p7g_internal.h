@@ -133,8 +133,12 @@ typedef struct { float bitscore, dombias; double lnP; } P7G_DSCORE;        /* bi
 float  p7g_flogsum(float a, float b);
 double p7g_exp_logsurv(double x, double mu, double lambda);
 float  p7g_null_one(int L);
+/* n2parts (optional, nn2 values): the target's null2 scores summed per
+ * envelope or multidomain region, in sequence order; the sequence bias is
+ * their float sum. NULL: the domains' corrections (the same terms when no
+ * region is multidomain). */
 void   p7g_target_scores(int L, float fwdsc, float nullsc, float omega, float tau, float lambda,
-                         const P7G_DOMIN *dom, int ndom, P7G_TSCORE *ts, P7G_DSCORE *ds);
+                         const P7G_DOMIN *dom, int ndom, const float *n2parts, int nn2, P7G_TSCORE *ts, P7G_DSCORE *ds);
 
 /* error text */
 void p7g_set_error(const char *fmt, ...);

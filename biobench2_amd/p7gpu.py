@@ -7,7 +7,7 @@ missing, importing this module raises.
 """
 import ctypes
 import os
-from ctypes import POINTER, c_double, c_float, c_int, c_int16, c_int32, c_int64, c_uint8, c_void_p, c_char_p
+from ctypes import POINTER, c_double, c_float, c_int, c_int16, c_int32, c_int64, c_uint8, c_uint32, c_void_p, c_char_p
 
 import numpy as np
 
@@ -58,7 +58,7 @@ class Stats(ctypes.Structure):
                [(n, c_double) for n in ("cells_msv", "cells_bias", "cells_vit", "cells_fwd", "cells_bck",
                                         "ms_msv", "ms_bias", "ms_vit", "ms_fwd", "ms_bck", "ms_total", "ms_ssv",
                                         "ms_domains")] + \
-               [(n, c_int64) for n in ("n_regions", "n_clustered")] + [("ms_null2", c_double)]
+               [(n, c_int64) for n in ("n_regions", "n_clustered")] + [("ms_null2", c_double), ("ms_multidomain", c_double)]
 
     def as_dict(self):
         return {n: getattr(self, n) for n, _ in self._fields_}
@@ -90,6 +90,8 @@ def _load():
         "p7_gpu_MSVFilter": (i, [pu8, i, vp, vp, pf]), "p7_gpu_SSVFilter": (i, [pu8, i, vp, pf]),
         "p7_gpu_ViterbiFilter": (i, [pu8, i, vp, vp, pf]), "p7_gpu_ForwardParser": (i, [pu8, i, vp, vp, pf]),
         "p7_gpu_BackwardParser": (i, [pu8, i, vp, vp, vp, pf]),
+        "p7_gpu_RegionTraceEnsemble": (i, [pu8, i, i, vp, i, c_uint32, pi32, pi32, i, pi32, pf, pf, pf, pi32, i,
+                                           pi32]),
         "p7_gpu_seqdb_Create": (i, [pu8, pi64, pi32, i64, POINTER(vp)]),
         "p7_gpu_seqdb_NSeq": (i64, [vp]), "p7_gpu_seqdb_NRes": (i64, [vp]), "p7_gpu_seqdb_Destroy": (None, [vp]),
         "p7_gpu_MSVFilterBatch": (i, [vp, vp, pi64, i64, pf, pi]),
@@ -389,6 +391,36 @@ def backward_parser(residues, om, fwd, bck=None):
     return st, sc, bck
 
 
+def region_trace_ensemble(residues, Ltarget, om, nsamples=200, seed=42, matrix=False, segcap=0):
+    """p7_gpu_RegionTraceEnsemble over one region's residues of a target of Ltarget residues:
+    dict clusters [(ci, cj)] (1-based in the region), n2sc[Lr], nseg, and with matrix=True the
+    Forward matrix mx[(Lr+1), Q, 3, 4] (M, D, I per striped lane), xmx[(Lr+1), 6] and the sampled
+    domains segs[nseg, 5] (trace, i, j, k, m)."""
+    r = np.asarray(residues, np.uint8)
+    Lr = r.size
+    dsq = dsq_of(r)
+    cap = 64
+    ci, cj = np.zeros(cap, np.int32), np.zeros(cap, np.int32)
+    n2 = np.zeros(Lr, np.float32)
+    nc, ns = c_int32(), c_int32()
+    Q = max(2, (om.tables()["M"] - 1) // 4 + 1)
+    sc = segcap if segcap > 0 else 6144
+    mx = np.zeros((Lr + 1) * 12 * Q, np.float32) if matrix else None
+    xmx = np.zeros((Lr + 1) * 6, np.float32) if matrix else None
+    segs = np.zeros(5 * sc, np.int32) if matrix else None
+    _check(lib.p7_gpu_RegionTraceEnsemble(_ptr(dsq, c_uint8), Lr, int(Ltarget), om.h, nsamples, seed, _ptr(ci, c_int32),
+                                          _ptr(cj, c_int32), cap, ctypes.byref(nc), _ptr(n2, c_float),
+                                          _ptr(mx, c_float) if matrix else None, _ptr(xmx, c_float) if matrix else None,
+                                          _ptr(segs, c_int32) if matrix else None, segcap, ctypes.byref(ns)),
+           "RegionTraceEnsemble")
+    out = {"clusters": [(int(ci[c]), int(cj[c])) for c in range(min(nc.value, cap))], "n2sc": n2, "nseg": ns.value}
+    if matrix:
+        out["mx"] = mx.reshape(Lr + 1, Q, 3, 4)
+        out["xmx"] = xmx.reshape(Lr + 1, 6)
+        out["segs"] = segs[:5 * min(ns.value, sc)].reshape(-1, 5)
+    return out
+
+
 class SeqDB:
     """HBM-resident digital database. residues: uint8 concatenation; offsets/lengths per sequence."""
 
@@ -579,7 +611,9 @@ class Pipeline:
         return out
 
     def regions(self):
-        """[(sequence index, i, j, multidomain, envelope score)] of the last run with domain decoding."""
+        """[(sequence index, i, j, kind, envelope score)] of the last run with domain decoding, by
+        sequence then start: kind 0 a single-domain region (one envelope), 1 a multidomain region,
+        2 an envelope its sampled traces' clusters define (after the region, in start order)."""
         n = c_int64()
         _check(lib.p7_gpu_pipeline_GetRegions(self.h, None, None, None, None, None, 0, ctypes.byref(n)), "GetRegions")
         seq = np.zeros(n.value, np.int64)
@@ -588,11 +622,13 @@ class Pipeline:
         _check(lib.p7_gpu_pipeline_GetRegions(self.h, _ptr(seq, c_int64), _ptr(i, c_int32), _ptr(j, c_int32),
                                               _ptr(m, c_int32), _ptr(sc, c_float), n.value, ctypes.byref(n)),
                "GetRegions")
-        return [(int(seq[k]), int(i[k]), int(j[k]), bool(m[k]), float(sc[k])) for k in range(n.value)]
+        return [(int(seq[k]), int(i[k]), int(j[k]), int(m[k]), float(sc[k])) for k in range(n.value)]
 
     def region_null2(self):
         """(domcorr[n] nats, status[n]) of the last run's regions, in regions() order: status 0
-        computed, 2 multidomain (not computed), 16 the decoding overflowed."""
+        computed (by expectation for kind 0, from the region's sampled traces for kind 2), 16 the
+        decoding overflowed, 2 a multidomain region, whose domcorr is the sum of the sampled null2
+        scores of all its residues."""
         n = c_int64()
         _check(lib.p7_gpu_pipeline_GetRegionNull2(self.h, None, None, 0, ctypes.byref(n)), "GetRegionNull2")
         dc = np.zeros(n.value, np.float32)
@@ -619,8 +655,8 @@ class Pipeline:
 
     def target_scores(self):
         """Per target with regions (sequence order): dict of arrays seq, score, pre_score,
-        sum_score (bits), lnP, ndom, best (region index within the target, -1 none) and
-        status (0 scored, 2 has a multidomain region)."""
+        sum_score (bits), lnP, ndom, best (index within the target's regions() entries, -1
+        none) and status (0 scored)."""
         n = c_int64()
         _check(lib.p7_gpu_pipeline_GetTargetScores(self.h, *([None] * 8), 0, ctypes.byref(n)), "GetTargetScores")
         o = {"seq": np.zeros(n.value, np.int64), "score": np.zeros(n.value, np.float32),

@@ -129,7 +129,8 @@ typedef struct p7g_stats_s {
   double  ms_ssv;              /* the ungapped MSV pass alone (k_ssv), last call    */
   double  ms_domains;          /* domain decoding and regions, last call (0 if off) */
   int64_t n_regions, n_clustered;   /* regions of the last call, and multidomain ones */
-  double  ms_null2;            /* null2 of the single-domain regions, last call (0 if off) */
+  double  ms_null2;            /* rescoring of the regions (null2, alignments, multidomain regions), last call (0 if off) */
+  double  ms_multidomain;      /* of which the multidomain regions' sampled traces and clusters */
 } P7_GPU_STATS;
 
 /* ---- device ------------------------------------------------------------- */
@@ -273,36 +274,63 @@ int   p7_gpu_pipeline_GetStats(const P7_GPU_PIPELINE *pli, P7_GPU_STATS *st);
  * p7_Pipeline calls it (p7_pipeline.c:737): p7_DomainDecoding
  * (impl_sse/decoding.c:166-197) from the two parsers' special states, then
  * the region heuristics with is_multidomain_region (rt1 0.25, rt2 0.10, rt3
- * 0.20), each region's envelope score, and (p7_gpu_pipeline_SetNull2) the
- * null2 correction of each region that is not multidomain. The stochastic
- * trace clustering of multidomain regions and the alignments that follow in
- * the reference are not computed. Off by default; batches and the server do
- * not decode. */
+ * 0.20) and each region's envelope score; with p7_gpu_pipeline_SetNull2 (on
+ * by default) the rest of the domain definition: each multidomain region's
+ * sampled traces and clusters (region_trace_ensemble, p7_domaindef.c:600-690,
+ * 200 samples, seed 42), and for every envelope (a single-domain region, or a
+ * cluster of a multidomain one) rescore_isolated_domain's null2 correction
+ * and optimal accuracy alignment (:812-947). Off by default; batches and the
+ * server do not decode. */
 int   p7_gpu_pipeline_SetDomains(P7_GPU_PIPELINE *pli, int on);
+/* region_trace_ensemble (HMMER/src/p7_domaindef.c:600-690), the analysis of
+ * one region that p7_domaindef_ByPosteriorHeuristics finds multidomain
+ * (:431-446), on residues dsq[1..Lr] of a target of Ltarget residues: the
+ * region's p7_Forward with the profile in multihit mode for the whole target
+ * (fwdback.c:258-465, every row kept), nsamples stochastic traces
+ * (impl_sse/stotrace.c) with Easel's fast generator reset to seed (the
+ * pipeline uses 200 and 42, p7_domaindef.c:112, p7_pipeline.c:102), null2 by
+ * trace (impl_sse/null2.c:134), and the clustering of the sampled domains
+ * (p7_spensemble.c:282-411) without the dominated ones. Out: *ret_nc
+ * envelopes ci..cj (1-based in the region, start order; up to cap written),
+ * n2sc[0..Lr-1] the residues' log null2 scores (ddef->n2sc[ireg..jreg]).
+ * Optional: opt_mx the Forward matrix as P7_OMX's dpf keeps it, rows 0..Lr of
+ * 12*Q floats ([q][M,D,I][4 lanes], Q = max(2, (M-1)/4 + 1)); opt_xmx its
+ * special rows (E N J B C SCALE); opt_segs the sampled domains (trace index,
+ * i, j, k, m per domain, up to segcap) and *ret_nseg their number. segcap <=
+ * 0: the pipeline's 6144. p7G_EINVAL if a traceback choice fails (the
+ * reference raises an exception), p7G_EMEM past segcap sampled domains. */
+int   p7_gpu_RegionTraceEnsemble(const uint8_t *dsq, int Lr, int Ltarget, const P7_GPU_OPROFILE *om, int nsamples,
+                                 uint32_t seed, int32_t *ci, int32_t *cj, int cap, int32_t *ret_nc, float *n2sc,
+                                 float *opt_mx, float *opt_xmx, int32_t *opt_segs, int segcap, int32_t *ret_nseg);
 /* The null2 correction of decoded regions (pli->do_null2, p7_pipeline.c:208;
  * hmmsearch --nonull2 turns it off). On by default; it applies when decoding
  * is on. */
 int   p7_gpu_pipeline_SetNull2(P7_GPU_PIPELINE *pli, int on);
-/* The regions of the last p7_gpu_Pipeline call with decoding on, ordered by
- * sequence (the database's order) then start: sequence index, first and last
- * residue (1-based, inclusive), 1 if flagged multidomain, and the envelope
- * score of the region (nats): the Forward score of residues i..j with the
- * profile in unihit mode for the whole target's length, as
- * rescore_isolated_domain computes envsc (p7_domaindef.c:812, 841) for a
- * region that is one envelope (not multidomain). Writes up to cap entries
- * (any array may be NULL); *ret_n = the number of regions. Valid until the
+/* The region records of the last p7_gpu_Pipeline call with decoding on,
+ * ordered by sequence (the database's order), start, then kind: sequence
+ * index, first and last residue (1-based, inclusive), the kind (multi): 0 a
+ * single-domain region (one envelope), 1 a multidomain region, 2 an envelope
+ * that a multidomain region's clusters define (after that region, in start
+ * order; only when null2 is on); and the envelope score (nats): the Forward
+ * score of residues i..j with the profile in unihit mode for the whole
+ * target's length, as rescore_isolated_domain computes envsc
+ * (p7_domaindef.c:812, 841). The domains of a target, in the reference's
+ * order, are its records of kinds 0 and 2. Writes up to cap entries (any
+ * array may be NULL); *ret_n = the number of records. Valid until the
  * pipeline's next call, while that call's database lives. */
 int   p7_gpu_pipeline_GetRegions(const P7_GPU_PIPELINE *pli, int64_t *seq, int32_t *ienv, int32_t *jenv,
                                  int32_t *multi, float *envsc, int64_t cap, int64_t *ret_n);
-/* The null2 correction of each region, in p7_gpu_pipeline_GetRegions' order:
+/* The null2 correction of each record, in p7_gpu_pipeline_GetRegions' order:
  * domcorr = sum over the envelope's residues of log null2(x) (nats), as
- * rescore_isolated_domain computes it for a one-envelope region
- * (p7_domaindef.c:841-931: p7_Forward and p7_Backward over residues i..j in
- * unihit mode, p7_Decoding, p7_Null2_ByExpectation). status: 0 computed;
- * 2 multidomain region (the reference takes its null2 from sampled traces:
- * not computed, domcorr NaN); 16 the decoding overflowed (eslERANGE, for
- * which the reference drops the domain). p7G_EINVAL if the last call ran
- * without null2. */
+ * rescore_isolated_domain computes it (p7_domaindef.c:841-947): for a
+ * single-domain region by expectation (p7_Forward and p7_Backward over
+ * residues i..j in unihit mode, p7_Decoding, p7_Null2_ByExpectation); for a
+ * cluster's envelope from its region's sampled traces (p7_Null2_ByTrace
+ * averaged over the samples). status: 0 computed; 16 the envelope's decoding
+ * overflowed (eslERANGE, for which the reference drops the domain); 2 a
+ * multidomain region itself, whose domcorr is the sum of the sampled null2
+ * scores of all its residues (what the sequence bias counts for it).
+ * p7G_EINVAL if the last call ran without null2. */
 int   p7_gpu_pipeline_GetRegionNull2(const P7_GPU_PIPELINE *pli, float *domcorr, int32_t *status, int64_t cap,
                                      int64_t *ret_n);
 /* The optimal accuracy alignment of each region, in p7_gpu_pipeline_GetRegions'
@@ -313,9 +341,10 @@ int   p7_gpu_pipeline_GetRegionNull2(const P7_GPU_PIPELINE *pli, float *domcorr,
  * domain table's acc column is oasc / (1 + |jenv - ienv|)), and the model and
  * target coordinates of the alignment's first and last M states
  * (p7_alidisplay_Create's hmmfrom, hmmto, sqfrom, sqto, p7_alidisplay.c:
- * 140-145). NaN / 0 for multidomain regions and regions whose decoding
- * failed. p7G_EINVAL if the last call ran without null2 (the alignments are
- * computed in the same pass). */
+ * 140-145). NaN / 0 for multidomain regions themselves (their clusters'
+ * envelopes have theirs) and envelopes whose decoding failed. p7G_EINVAL if
+ * the last call ran without null2 (the alignments are computed in the same
+ * pass). */
 int   p7_gpu_pipeline_GetDomainAlignments(const P7_GPU_PIPELINE *pli, float *oasc, int32_t *hmmfrom, int32_t *hmmto,
                                           int32_t *alifrom, int32_t *alito, int64_t cap, int64_t *ret_n);
 /* Hit-list scores of the last p7_gpu_Pipeline call with decoding and null2,
@@ -325,10 +354,11 @@ int   p7_gpu_pipeline_GetDomainAlignments(const P7_GPU_PIPELINE *pli, float *oas
  * and counts a domain), the uncorrected one, the reconstruction score, the
  * natural-log P-value of the score under the Forward exponential tail
  * (esl_exp_logsurv with the profile's tau, lambda), the number of domains
- * (regions whose rescoring succeeded), the best domain's index among the
- * target's regions (GetRegions order, -1 without domains) and a status: 0
- * scored; 2 the target has a multidomain region (not scored: NaN). The
- * E-value of a score is exp(lnP) * Z for a search space of Z targets. */
+ * (envelopes whose rescoring succeeded), the best domain's index among the
+ * target's records (GetRegions order, -1 without domains) and a status (0).
+ * The sequence bias sums every residue's null2 score: the single-domain
+ * envelopes' and each multidomain region's sampled ones (p7_pipeline.c:760).
+ * The E-value of a score is exp(lnP) * Z for a search space of Z targets. */
 int   p7_gpu_pipeline_GetTargetScores(const P7_GPU_PIPELINE *pli, int64_t *seq, float *score, float *pre_score,
                                       float *sum_score, double *lnP, int32_t *ndom, int32_t *best, int32_t *status,
                                       int64_t cap, int64_t *ret_n);
@@ -336,8 +366,8 @@ int   p7_gpu_pipeline_GetTargetScores(const P7_GPU_PIPELINE *pli, int64_t *seq, 
  * the bit score after the null2 correction and the length correction of the
  * rest of the target, dombias (nats) = p7_FLogsum(0, log(1/256) + domcorr),
  * and the natural-log P-value of the bit score (the i-Evalue is exp(lnP) * Z,
- * the c-Evalue exp(lnP) * domZ). NaN for multidomain regions and regions whose
- * rescoring failed. */
+ * the c-Evalue exp(lnP) * domZ). NaN for multidomain regions themselves and
+ * envelopes whose rescoring failed. */
 int   p7_gpu_pipeline_GetDomainScores(const P7_GPU_PIPELINE *pli, float *bitscore, float *dombias, double *lnP,
                                       int64_t cap, int64_t *ret_n);
 /* The hit list of the scored targets (GetTargetScores status 0), ranked as
@@ -348,8 +378,7 @@ int   p7_gpu_pipeline_GetDomainScores(const P7_GPU_PIPELINE *pli, float *bitscor
  * (exp(lnP)·Z <= E), bit 1 included (also <= incE). domZ = the number of
  * reported targets. dflags (NULL, or one entry per region in GetRegions
  * order): bit 0 reported (its target reported, exp(lnP)·domZ <= domE), bit 1
- * included (its target included, <= incdomE). Targets with a multidomain
- * region are not in the list and do not count in domZ. hmmsearch's defaults:
+ * included (its target included, <= incdomE). hmmsearch's defaults:
  * E = 10, incE = 0.01, domE = 10, incdomE = 0.01. */
 int   p7_gpu_pipeline_GetHitList(const P7_GPU_PIPELINE *pli, double Z, double E, double incE, double domE,
                                  double incdomE, int64_t *order, int32_t *flags, int32_t *dflags, double *ret_domZ,

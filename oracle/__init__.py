@@ -317,12 +317,13 @@ def oa_envelope(residues, t, i, j):
     return st, oasc.value, kf.value, kt.value, i0.value + i - 1, i1.value + i - 1
 
 
-def region_multidomain(residues, t, i, j, seed=42, nsamples=200):
+def region_multidomain(residues, t, i, j, seed=42, nsamples=200, want_segs=False):
     """(status, clusters, n2sc) of region_trace_ensemble for residues i..j (1-based) of a target,
     with t configured in multihit mode for the whole target's length (p7_domaindef.c:440-476,
     600-690): the clusters' envelopes [(i2, j2)] in target coordinates, start order, and the
     per-residue null2 scores n2sc[0..j-i] of the region's residues, from nsamples sampled traces
-    with the random number generator reset to seed."""
+    with the random number generator reset to seed. want_segs: also the sampled domains
+    [nseg, 5] (trace, i, j, k, m; region coordinates), in the ensemble's order."""
     reg = np.ascontiguousarray(np.asarray(residues)[i - 1:j], np.uint8)
     dsq = dsq_of(reg)
     K, Kp, mask, nd = degeneracy_masks(t["abctype"])
@@ -333,12 +334,32 @@ def region_multidomain(residues, t, i, j, seed=42, nsamples=200):
     cap = reg.size + 1
     ci, cj = np.zeros(cap, np.int32), np.zeros(cap, np.int32)
     nc, nseg = c_int(), c_int()
+    scap = 1 << 16
+    segs = np.zeros(5 * scap, np.int32) if want_segs else None
     st = lib.orc_region_multidomain(_p(dsq, c_uint8), reg.size, t["M"], K, Kp, _p(rfv, c_float), _p(tfv, c_float),
                                     _p(xf, c_float), _p(mask, c_uint32), _p(nd, c_int), ctypes.c_uint32(seed),
                                     nsamples, _p(n2, c_float), ctypes.byref(nc), _p(ci, c_int), _p(cj, c_int), cap,
-                                    ctypes.byref(nseg))
+                                    ctypes.byref(nseg), _p(segs, c_int) if want_segs else None, scap)
     clusters = [(int(ci[d]) + i - 1, int(cj[d]) + i - 1) for d in range(min(nc.value, cap))]
+    if want_segs:
+        return st, clusters, n2[1:], segs[:5 * min(nseg.value, scap)].reshape(-1, 5)
     return st, clusters, n2[1:]
+
+
+def forward_full(residues, t):
+    """(status, mx[(L+1), Q, 3, 4], xmx[(L+1), 6]): p7_Forward's whole striped matrix (M, D, I per
+    lane) and special rows for the residues with t's configuration."""
+    dsq = dsq_of(residues)
+    L = dsq.size - 2
+    M = t["M"]
+    Q = max(2, (M - 1) // 4 + 1)
+    full = np.zeros((L + 1) * 12 * Q, np.float32)
+    xmx = np.zeros((L + 1) * 6, np.float32)
+    st = lib.orc_forward_full(_p(dsq, c_uint8), L, M, t["Kp"], _p(np.ascontiguousarray(t["rfv"], np.float32), c_float),
+                              _p(np.ascontiguousarray(t["tfv"], np.float32), c_float),
+                              _p(np.ascontiguousarray(t["xf"], np.float32).reshape(8), c_float),
+                              _p(full, c_float), _p(xmx, c_float))
+    return st, full.reshape(L + 1, Q, 3, 4), xmx.reshape(L + 1, 6)
 
 
 def single_linkage_1d(x, thr):

@@ -955,7 +955,7 @@ static int seg_linked(int a, int b, const void *prm) { const SPSEG *s = prm; ret
 int
 orc_region_multidomain(const uint8_t *dsq, int Lr, int M, int K, int Kp, const float *rfv, const float *tfv,
                        const float *xf, const uint32_t *degen, const int *ndegen, uint32_t seed, int nsamples,
-                       float *n2sc, int *ret_nc, int *ci, int *cj, int cap, int *ret_nsegs)
+                       float *n2sc, int *ret_nc, int *ci, int *cj, int cap, int *ret_nsegs, int *segs_out, int segs_cap)
 {
   STRIPED sp;
   TRACE_CTX c;
@@ -1033,6 +1033,11 @@ orc_region_multidomain(const uint8_t *dsq, int Lr, int M, int K, int Kp, const f
   if (status) goto done_sp;
   for (int pos = 1; pos <= Lr; pos++) n2sc[pos] = logf(acc[pos] / (float) nsamples);
   if (ret_nsegs) *ret_nsegs = nseg;
+  if (segs_out)
+    for (int h = 0; h < nseg && h < segs_cap; h++) {
+      int *o = segs_out + 5 * (size_t) h;
+      o[0] = seg[h].idx; o[1] = seg[h].i; o[2] = seg[h].j; o[3] = seg[h].k; o[4] = seg[h].m;
+    }
 
   /* esl_cluster_SingleLinkage: a stack of unlinked vertices (pushed
    * backwards), and a stack of linked ones still to extend */
@@ -1184,6 +1189,16 @@ done:
 #undef TV
 #undef DDV
 #undef XV
+
+/* p7_Forward kept whole (the region matrix of orc_region_multidomain): full
+ * as fwd_engine's full output, xmx its special rows. */
+int
+orc_forward_full(const uint8_t *dsq, int L, int M, int Kp, const float *rfv, const float *tfv, const float *xf,
+                 float *full, float *xmx)
+{
+  float sc;
+  return fwd_engine(dsq, L, M, Kp, rfv, tfv, xf, xmx, NULL, &sc, NULL, full);
+}
 
 /*****************************************************************
  * Null model and bias-composition filter

@@ -31,5 +31,6 @@ int main(void)
   F(P7_GPU_STATS, ms_fwd); F(P7_GPU_STATS, ms_bck); F(P7_GPU_STATS, ms_total); F(P7_GPU_STATS, ms_ssv);
   F(P7_GPU_STATS, ms_domains); F(P7_GPU_STATS, n_regions); F(P7_GPU_STATS, n_clustered);
   F(P7_GPU_STATS, ms_null2);
+  F(P7_GPU_STATS, ms_multidomain);
   return 0;
 }

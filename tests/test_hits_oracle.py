@@ -85,7 +85,7 @@ def test_library_hit_scores_equal_the_oracle():
     correction exceeds their score and targets where the reconstruction score wins)."""
     lib = p7gpu.lib
     lib.p7g_target_scores.argtypes = [c_int, c_float, c_float, c_float, c_float, c_float, POINTER(DomIn), c_int,
-                                      POINTER(TScore), POINTER(DScore)]
+                                      POINTER(c_float), c_int, POINTER(TScore), POINTER(DScore)]
     lib.p7g_target_scores.restype = None
     rng = np.random.default_rng(5)
     won = 0
@@ -100,10 +100,16 @@ def test_library_hit_scores_equal_the_oracle():
         fwdsc = float(np.float32(rng.normal(30, 40)))
         nullsc = float(oracle.null_one(L))
         tau, lam = float(np.float32(rng.normal(-4, 1))), 0.6931
-        ref = oracle.target_scores(L, fwdsc, nullsc, doms, tau, lam)
+        # half the targets with multidomain-region parts in the sequence bias
+        parts = None
+        if rng.random() < 0.5:
+            parts = [float(np.float32(rng.normal(0, 2))) for _ in range(int(rng.integers(1, 4)))]
+        ref = oracle.target_scores(L, fwdsc, nullsc, doms, tau, lam, n2sc=parts)
         din = (DomIn * max(nd, 1))(*[DomIn(e, d, i, j) for e, d, i, j in doms])
         ts, ds = TScore(), (DScore * max(nd, 1))()
-        lib.p7g_target_scores(L, fwdsc, nullsc, float(oracle.OMEGA), tau, lam, din, nd, ctypes.byref(ts), ds)
+        pa = (c_float * len(parts))(*parts) if parts else None
+        lib.p7g_target_scores(L, fwdsc, nullsc, float(oracle.OMEGA), tau, lam, din, nd, pa, len(parts) if parts else 0,
+                              ctypes.byref(ts), ds)
         assert (np.float32(ts.score), np.float32(ts.pre_score), np.float32(ts.sum_score)) == ref[:3]
         assert ts.lnP == ref[3]
         if nd:
