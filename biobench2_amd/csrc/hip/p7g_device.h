@@ -194,6 +194,16 @@ struct DomOut {
 #define MD_TSEG   512     // sampled domains in one trace
 #define MD_MAXSIG 64      // significant clusters of one region
 #define MD_MAXC   32      // envelopes one region may give
+#define MD_ACC_LDS 8192   // regions up to this many residues + 1 sum their odds in LDS
+__host__ __device__ inline int64_t md_q(int M) { return (M - 1) / 4 + 1 < 2 ? 2 : (M - 1) / 4 + 1; }
+// LDS, in floats: the transitions staged for the Forward (10 rows of M + 2),
+// the Forward's rows, and the traces' buffers (from the start of LDS): a
+// domain's counts (4Q), its null2 odds (32), the trace's domains, the
+// residues' sums
+__host__ __device__ inline int64_t md_stage_floats(int M) { return (10 * ((int64_t) M + 2) + 3) & ~(int64_t) 3; }
+__host__ __device__ inline bool md_stage_t(int M) { return (md_stage_floats(M) + 32 * md_q(M)) * 4 <= P7G_LDS_MAX; }
+__host__ __device__ inline int64_t md_acc_at(int64_t Q) { return 4 * Q + 32 + MD_TSEG * 4; }
+__host__ __device__ inline int64_t md_trace_floats(int M) { return md_acc_at(md_q(M)) + MD_ACC_LDS; }
 struct MdJob {
   int64_t res_off;
   int64_t area;
@@ -208,7 +218,13 @@ struct MdOut {
   int32_t ci[MD_MAXC], cj[MD_MAXC];
 };
 __host__ __device__ inline int64_t md_n2sc_at(int M, int Lr)
-{ const int64_t Q = (M - 1) / 4 + 1 < 2 ? 2 : (M - 1) / 4 + 1; return (int64_t) (Lr + 1) * 12 * Q + 6 * (int64_t) (Lr + 1); }
+{ return (int64_t) (Lr + 1) * 12 * md_q(M) + 6 * (int64_t) (Lr + 1); }
+// then the endpoint counts, the sampled domains (5 ints each), and (doubles)
+// the traces' thresholds: 5 per cell, 3 per row, 8Q running sums per row
+__host__ __device__ inline int64_t md_seg_at(int M, int Lr)
+{ return md_n2sc_at(M, Lr) + (Lr + 1) + ((Lr > M ? Lr : M) + 2); }
+__host__ __device__ inline int64_t md_thr_at(int M, int Lr, int segcap)
+{ return (md_seg_at(M, Lr) + 5 * (int64_t) segcap + 1) & ~(int64_t) 1; }
 size_t  md_lds_bytes(int M, int segcap);
 int64_t md_area_floats(int M, int Lr, int segcap);
 hipError_t launch_md_regions(const DevProfile &p, const uint8_t *res, const MdJob *jobs, MdOut *out, int njobs,

@@ -68,23 +68,30 @@ __device__ __forceinline__ float md_tv(const DevProfile &p, int row, int k)
   return p.fb_tbl[((size_t) row * B + j) * 64 + zl];
 }
 
-// esl_vec_FNorm then esl_rnd_FChoose over the values lanes 0..n-1 hold; the
-// chosen index (uniform), or -1 if no threshold exceeds the roll
-__device__ __forceinline__ int md_choose(float v, int n, uint32_t &rng)
+// the device table row of the i-th staged transition row
+__device__ __forceinline__ int md_trow(int i)
 {
-  const int z = threadIdx.x;
-  float s = 0.0f;
-  for (int a = 0; a < n; a++) s = __fadd_rn(s, readlanef(v, a));
-  const float pn = (s != 0.0f) ? __fdiv_rn(v, s) : (float) (1.0 / (double) (float) n);
-  double norm = 0.0, mine = 0.0;
-  for (int a = 0; a < n; a++) {
-    norm = norm + (double) readlanef(pn, a);
-    if (z == a) mine = norm;
+  switch (i) {
+  case 0: return VT_BM; case 1: return VT_MM; case 2: return VT_IM; case 3: return VT_DM;
+  case 4: return VT_MDS; case 5: return VT_DDS; case 6: return VT_MI; case 7: return VT_II;
+  case 8: return FT_MD; default: return FT_DD;
   }
-  rng = rng * 69069u + 1u;
-  const double roll = (double) rng / 4294967296.0;
-  const unsigned long long m = __ballot(z < n && roll < mine / norm);
-  return m ? (int) __ffsll((long long) m) - 1 : -1;
+}
+
+// esl_vec_FNorm then esl_rnd_FChoose's thresholds for n choices of weights
+// v: the float sum and divisions, then the double running sums each divided
+// by their total; the first n - 1 of them go to th (the last is 1)
+__device__ __forceinline__ void md_thresholds(const float *v, int n, double *th)
+{
+  float s = 0.0f;
+  for (int a = 0; a < n; a++) s = __fadd_rn(s, v[a]);
+  double c[4], tot = 0.0;
+  for (int a = 0; a < n; a++) {
+    const float pa = (s != 0.0f) ? __fdiv_rn(v[a], s) : (float) (1.0 / (double) (float) n);
+    tot = tot + (double) pa;
+    c[a] = tot;
+  }
+  for (int a = 0; a < n - 1; a++) th[a] = c[a] / tot;
 }
 
 struct MdSeg { int idx, i, j, k, m; };
@@ -123,67 +130,100 @@ k_md_region(const DevProfile p, const uint8_t *__restrict__ res, const MdJob *__
   float *xmx = mat + (size_t) (Lr + 1) * 12 * Q;           // rows 0..Lr: E N J B C SCALE
   float *acc = xmx + (size_t) 6 * (Lr + 1);                // residues 0..Lr
   int   *hist = (int *) (acc + (Lr + 1));                  // max(Lr, M) + 2 endpoint counts
-  MdSeg *segs = (MdSeg *) (hist + (max(Lr, M) + 2));       // the ensemble, J.segcap entries
+  MdSeg *segs = (MdSeg *) (pool + J.area + md_seg_at(M, Lr));   // the ensemble, J.segcap entries
   int status = 0;
 #define CELL(i_, q_, s_, r_) mat[(((size_t) (i_) * Q + (q_)) * 3 + (s_)) * 4 + (r_)]
 
+  // The profile's transitions, staged in LDS when they fit (md_stage_t): rows
+  // BM MM IM DM (into k), MDS DDS (into D_k), MI II, MD DD (out of k), each
+  // [k = 0..M+1] with 0 outside 1..M
+  const bool stage = md_stage_t(M);
+  const int TW = M + 2;
+  float *tl = lds;
+  float *ph = lds + (stage ? md_stage_floats(M) : 0);       // the phases' own LDS
+  if (stage)
+    for (int c = z; c < 10 * TW; c += 64) tl[c] = md_tv(p, TR + md_trow(c / TW), c % TW);
+  auto tv = [&](int row, int k) -> float {                 // row: 0..9 as staged
+    if (stage) return tl[row * TW + k];
+    return md_tv(p, TR + md_trow(row), k);
+  };
+  enum { T_BM = 0, T_MM, T_IM, T_DM, T_MDS, T_DDS, T_MI, T_II, T_MD, T_DD };
+
   // ---------------- 1. the region's Forward matrix ----------------
   {
-    float *prev = lds, *cur = lds + 12 * Q, *tdd = lds + 24 * Q;
+    float *prev = ph, *cur = ph + 12 * Q, *tdd = ph + 24 * Q, *erow = ph + 28 * Q;   // erow: 4Q emissions
+    __syncthreads();
     for (int c = z; c < 12 * Q; c += 64) { prev[c] = 0.0f; mat[c] = 0.0f; }
-    for (int c = z; c < 4 * Q; c += 64) tdd[c] = md_tv(p, TR + FT_DD, (c & 3) * Q + (c >> 2) + 1);
+    for (int c = z; c < 4 * Q; c += 64) tdd[c] = tv(T_DD, min((c & 3) * Q + (c >> 2) + 1, M + 1));
     float xN = 1.0f, xJ = 0.0f, xB = fmove, xC = 0.0f, xE = 0.0f;
     if (z == 0) { xmx[0] = 0.0f; xmx[1] = xN; xmx[2] = xJ; xmx[3] = xB; xmx[4] = xC; xmx[5] = 1.0f; }
     __syncthreads();
     for (int i = 1; i <= Lr; i++) {
       const int x = dsq[i - 1];
+      for (int c0 = 0; c0 < 4 * Q; c0 += 64 * 8) {           // this residue's odds, striped order
+        float e[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) { const int c = c0 + u * 64 + z; e[u] = md_tv(p, x, (c & 3) * Q + (c >> 2) + 1); }
+#pragma unroll
+        for (int u = 0; u < 8; u++) { const int c = c0 + u * 64 + z; if (c < 4 * Q) erow[c] = e[u]; }
+      }
+      __syncthreads();
       for (int c = z; c < 4 * Q; c += 64) {
-        const int q = c >> 2, r = c & 3, k = r * Q + q + 1;
+        const int q = c >> 2, r = c & 3, k = r * Q + q + 1, kk = min(k, M + 1);
         float mp = 0.0f, dp = 0.0f, ip = 0.0f;           // row i-1 at k-1 (esl_sse_rightshift_ps at q = 0)
         if (q > 0)      { mp = prev[((q-1)*3+0)*4 + r]; dp = prev[((q-1)*3+1)*4 + r]; ip = prev[((q-1)*3+2)*4 + r]; }
         else if (r > 0) { mp = prev[((Q-1)*3+0)*4 + r-1]; dp = prev[((Q-1)*3+1)*4 + r-1]; ip = prev[((Q-1)*3+2)*4 + r-1]; }
-        float sv = __fmul_rn(xB, md_tv(p, TR + VT_BM, k));
-        sv = __fadd_rn(sv, __fmul_rn(mp, md_tv(p, TR + VT_MM, k)));
-        sv = __fadd_rn(sv, __fmul_rn(ip, md_tv(p, TR + VT_IM, k)));
-        sv = __fadd_rn(sv, __fmul_rn(dp, md_tv(p, TR + VT_DM, k)));
-        sv = __fmul_rn(sv, md_tv(p, x, k));
+        float sv = __fmul_rn(xB, tv(T_BM, kk));
+        sv = __fadd_rn(sv, __fmul_rn(mp, tv(T_MM, kk)));
+        sv = __fadd_rn(sv, __fmul_rn(ip, tv(T_IM, kk)));
+        sv = __fadd_rn(sv, __fmul_rn(dp, tv(T_DM, kk)));
+        sv = __fmul_rn(sv, erow[c]);
         cur[(q*3+0)*4 + r] = sv;
-        cur[(q*3+2)*4 + r] = __fadd_rn(__fmul_rn(prev[(q*3+0)*4 + r], md_tv(p, TR + VT_MI, k)),
-                                       __fmul_rn(prev[(q*3+2)*4 + r], md_tv(p, TR + VT_II, k)));
+        cur[(q*3+2)*4 + r] = __fadd_rn(__fmul_rn(prev[(q*3+0)*4 + r], tv(T_MI, kk)),
+                                       __fmul_rn(prev[(q*3+2)*4 + r], tv(T_II, kk)));
       }
       __syncthreads();
       // D before the D->D passes: cell q holds M(q-1)·tMD (the q loop's dcv); q = 0 holds 0
       for (int c = z; c < 4 * Q; c += 64) {
         const int q = c >> 2, r = c & 3, k = r * Q + q + 1;
-        cur[(q*3+1)*4 + r] = q ? __fmul_rn(cur[((q-1)*3+0)*4 + r], md_tv(p, TR + FT_MD, k - 1)) : 0.0f;
+        cur[(q*3+1)*4 + r] = q ? __fmul_rn(cur[((q-1)*3+0)*4 + r], tv(T_MD, min(k - 1, M + 1))) : 0.0f;
       }
       __syncthreads();
       float xe = 0.0f;
       if (z < 4) {                                         // lane z is SSE lane r
+        // the sums along q, eight cells' loads issued before their dependent chain
         const int r = z;
-        for (int q = 0; q < Q; q++) xe = __fadd_rn(xe, cur[(q*3+0)*4 + r]);
-        float dcv = __fmul_rn(cur[((Q-1)*3+0)*4 + r], md_tv(p, TR + FT_MD, r * Q + Q));
+#define MD_CHUNK(body) for (int q0 = 0; q0 < Q; q0 += 8) { \
+          float mv[8], tv8[8]; \
+          _Pragma("unroll") for (int u = 0; u < 8; u++) { const int qq = min(q0 + u, Q - 1); \
+            mv[u] = cur[(qq*3+SL)*4 + r]; tv8[u] = tdd[qq*4 + r]; } \
+          _Pragma("unroll") for (int u = 0; u < 8; u++) if (q0 + u < Q) { body } \
+          _Pragma("unroll") for (int u = 0; u < 8; u++) if (q0 + u < Q && WB) cur[((q0+u)*3+SL)*4 + r] = mv[u]; }
+#define SL 0
+#define WB false
+        MD_CHUNK(xe = __fadd_rn(xe, mv[u]);)
+#undef SL
+#undef WB
+        float dcv = __fmul_rn(cur[((Q-1)*3+0)*4 + r], tv(T_MD, min(r * Q + Q, M + 1)));
         float sh = __shfl(dcv, (z + 3) & 3, 4);
         dcv = r ? sh : 0.0f;
-        for (int q = 0; q < Q; q++) {
-          const float d = __fadd_rn(dcv, cur[(q*3+1)*4 + r]);
-          cur[(q*3+1)*4 + r] = d;
-          dcv = __fmul_rn(d, tdd[q*4 + r]);
-        }
+#define SL 1
+#define WB true
+        MD_CHUNK(const float d = __fadd_rn(dcv, mv[u]); mv[u] = d; dcv = __fmul_rn(d, tv8[u]);)
         for (int pass = 1; pass < 4; pass++) {
           sh = __shfl(dcv, (z + 3) & 3, 4);
           dcv = r ? sh : 0.0f;
           int changed = 0;
-          for (int q = 0; q < Q; q++) {
-            const float old = cur[(q*3+1)*4 + r], d = __fadd_rn(dcv, old);
-            changed |= (d > old);
-            cur[(q*3+1)*4 + r] = d;
-            dcv = __fmul_rn(dcv, tdd[q*4 + r]);
-          }
+          MD_CHUNK(const float d = __fadd_rn(dcv, mv[u]); changed |= (d > mv[u]); mv[u] = d; dcv = __fmul_rn(dcv, tv8[u]);)
           const bool any = (__ballot(changed) & 0xfull) != 0ull;
           if (M >= 100 && ! any) break;
         }
-        for (int q = 0; q < Q; q++) xe = __fadd_rn(cur[(q*3+1)*4 + r], xe);
+#undef WB
+#define WB false
+        MD_CHUNK(xe = __fadd_rn(mv[u], xe);)
+#undef SL
+#undef WB
+#undef MD_CHUNK
       }
       // esl_sse_hsum_ps: (e0 + e1) + (e2 + e3)
       xE = __fadd_rn(__fadd_rn(readlanef(xe, 0), readlanef(xe, 1)), __fadd_rn(readlanef(xe, 2), readlanef(xe, 3)));
@@ -213,94 +253,187 @@ k_md_region(const DevProfile p, const uint8_t *__restrict__ res, const MdJob *__
   }
 
   // ---------------- 2. the sampled traces ----------------
+  // Every choice a trace can make depends only on the finished matrix, so
+  // its thresholds are computed first for every cell and row, in parallel,
+  // with the same operations p7_StochasticTrace performs at the step
+  // (md_thresholds); a step is then one draw and one to three comparisons.
+  // thr[(i, k)]: M's three (B | M | I | D), D's one (M | D), I's one (M | I);
+  // xthr[i]: C's (C | E), J's (J | E), B's (N | J); ecum[i]: select_e's
+  // running sum over row i in the striped order (q outer; M lanes, D lanes).
+  double *thr = (double *) (pool + J.area + md_thr_at(M, Lr, J.segcap));
+  double *xthr = thr + (size_t) 5 * (Lr + 1) * (M + 1);
+  double *ecum = xthr + (size_t) 3 * (Lr + 1);
+  auto cellk = [&](int i, int k, int s) {                  // the cell of model position k (1..4Q) in row i
+    const int r = (k - 1 >= Q) + (k - 1 >= 2 * Q) + (k - 1 >= 3 * Q), q = k - 1 - r * Q;
+    return CELL(i, q, s, r); };
+  for (int64_t c = z; c < (int64_t) Lr * M; c += 64) {
+    const int i = 1 + (int) (c / M), k = 1 + (int) (c - (int64_t) (i - 1) * M);
+    double *o = thr + ((size_t) i * (M + 1) + k) * 5;
+    float v[4];
+    v[0] = __fmul_rn(xmx[(size_t) (i - 1) * 6 + 3], tv(T_BM, k));
+    v[1] = __fmul_rn(k > 1 ? cellk(i - 1, k - 1, 0) : 0.0f, tv(T_MM, k));
+    v[2] = __fmul_rn(k > 1 ? cellk(i - 1, k - 1, 2) : 0.0f, tv(T_IM, k));
+    v[3] = __fmul_rn(k > 1 ? cellk(i - 1, k - 1, 1) : 0.0f, tv(T_DM, k));
+    md_thresholds(v, 4, o);
+    v[0] = (k > 1) ? __fmul_rn(cellk(i, k - 1, 0), tv(T_MDS, k)) : 0.0f;
+    v[1] = (k > 1) ? __fmul_rn(cellk(i, k - 1, 1), tv(T_DDS, k)) : 0.0f;
+    md_thresholds(v, 2, o + 3);
+    v[0] = __fmul_rn(cellk(i - 1, k, 0), tv(T_MI, k));
+    v[1] = __fmul_rn(cellk(i - 1, k, 2), tv(T_II, k));
+    md_thresholds(v, 2, o + 4);
+  }
+  for (int i = z; i <= Lr; i += 64) {
+    float v[2];
+    if (i >= 1) {
+      v[0] = __fmul_rn(xmx[(size_t) (i - 1) * 6 + 4], floop);
+      v[1] = __fmul_rn(__fmul_rn(xmx[(size_t) i * 6 + 0], xEmove), xmx[(size_t) i * 6 + 5]);
+      md_thresholds(v, 2, xthr + (size_t) i * 3 + 0);
+      v[0] = __fmul_rn(xmx[(size_t) (i - 1) * 6 + 2], floop);
+      v[1] = __fmul_rn(__fmul_rn(xmx[(size_t) i * 6 + 0], xEloop), xmx[(size_t) i * 6 + 5]);
+      md_thresholds(v, 2, xthr + (size_t) i * 3 + 1);
+      // select_e: norm = 1/E(i) in double, the products in float, summed in double
+      const float nf = (float) (1.0 / (double) xmx[(size_t) i * 6 + 0]);
+      double sum = 0.0, *er = ecum + (size_t) i * 8 * Q;
+      for (int q = 0; q < Q; q++) {
+        const float *cq = mat + ((size_t) i * Q + q) * 12;
+        float u[8];
+#pragma unroll
+        for (int e = 0; e < 8; e++) u[e] = cq[e];
+#pragma unroll
+        for (int e = 0; e < 8; e++) { sum = sum + (double) __fmul_rn(u[e], nf); er[q * 8 + e] = sum; }
+      }
+    }
+    v[0] = __fmul_rn(xmx[(size_t) i * 6 + 1], fmove);
+    v[1] = __fmul_rn(xmx[(size_t) i * 6 + 2], fmove);
+    md_thresholds(v, 2, xthr + (size_t) i * 3 + 2);
+  }
+  __syncthreads();
+
+  // LDS: a domain's counts, its null2 odds, the trace's domains, and the
+  // residues' odds sums when the region is short
   float *cnt = lds;                                        // [q][r] counts of one sampled domain
   float *n2v = lds + 4 * Q;                                // its null2 odds, 32 codes
   int   *tseg = (int *) (n2v + 32);                        // this trace's domains, last first
+  const bool acc_lds = (Lr + 1) <= MD_ACC_LDS;
+  float *accp = acc_lds ? lds + md_acc_at(Q) : acc;        // residue sums
+  auto cnt_at = [&](int k) {                               // the count cell of model position k
+    const int r = (k - 1 >= Q) + (k - 1 >= 2 * Q) + (k - 1 >= 3 * Q);
+    return (k - 1 - r * Q) * 4 + r; };
   for (int c = z; c < 4 * Q; c += 64) cnt[c] = 0.0f;
-  for (int pos = z; pos <= Lr; pos += 64) acc[pos] = 0.0f;
+  for (int pos = z; pos <= Lr; pos += 64) accp[pos] = 0.0f;
   __syncthreads();
   uint32_t rng = md_mix3(seed, 87654321u, 12345678u);
   if (rng == 0) rng = 42;
+  // lane l's jump: l + 1 draws of x -> 69069 x + 1 are x -> jA x + jC
+  uint32_t jA = 1u, jC = 0u;
+  for (int l = 0; l <= z; l++) { jA *= 69069u; jC = jC * 69069u + 1u; }
   int nseg = 0;
   const int64_t max_steps = 64 * ((int64_t) Lr + M + 16);
-  auto xv = [&](int i, int c) { return xmx[(size_t) i * 6 + c]; };
-  auto cellk = [&](int i, int k, int s) {                  // the cell of model position k (1..4Q) in row i
-    const int r = (k - 1) / Q, q = (k - 1) - r * Q;
-    return CELL(i, q, s, r); };
   for (int t = 0; t < nsamples && ! status; t++) {
     int s0 = MS_C, i = Lr, k = 0, hi = Lr, nd = 0;
-    int sqfrom = 0, sqto = 0, hmmfrom = 0, hmmto = 0, Ld = 0;
+    int sqfrom = 0, sqto = 0, hmmfrom = 0, hmmto = 0, Ld = 0, kmin = 0x7fffffff, kmax = 0;
     int64_t steps = 0;
     while (s0 != MS_S) {
-      if (++steps > max_steps) { status = 1; break; }
       int s1 = -1;
-      switch (s0) {
-      case MS_M: {                                         // from B(i-1), M/I/D(i-1, k-1)
-        float v = 0.0f;
-        if (z == 0)      v = __fmul_rn(xv(i - 1, 3), md_tv(p, TR + VT_BM, k));
-        else if (z < 4) {
-          const int s = (z == 1) ? 0 : (z == 2) ? 2 : 1;  // M, I, D
-          const int row = (z == 1) ? VT_MM : (z == 2) ? VT_IM : VT_DM;
-          v = __fmul_rn(k > 1 ? cellk(i - 1, k - 1, s) : 0.0f, md_tv(p, TR + row, k));
+      if (s0 == MS_M || s0 == MS_D || s0 == MS_I || s0 == MS_C || s0 == MS_J) {
+        // A state that can repeat: M along its diagonal, D along its row, I,
+        // C and J down their column. Lane l takes the step that follows l
+        // repeats, with the (l+1)-th draw from here and that cell's
+        // thresholds; the first lane whose step leaves the state (or fails)
+        // is the step the walk takes after those repeats.
+        const uint32_t xl = jA * rng + jC;
+        const double roll = (double) xl / 4294967296.0;
+        const int li = (s0 == MS_D) ? i : i - z, lk = (s0 == MS_M) ? k - z : (s0 == MS_D) ? k - z : k;
+        int sl = -1;
+        if (s0 == MS_C || s0 == MS_J) {
+          if (li >= 1) {
+            const double th = xthr[(size_t) li * 3 + (s0 == MS_J)];
+            sl = (roll < th) ? s0 : (th == th) ? MS_E : -1;
+          }
+        } else if (li >= 1 && lk >= 1) {
+          const double *th = thr + ((size_t) li * (M + 1) + lk) * 5;
+          if (s0 == MS_M) {
+            const double t0 = th[0], t1 = th[1], t2 = th[2];
+            sl = (roll < t0) ? MS_B : (roll < t1) ? MS_M : (roll < t2) ? MS_I : (t0 == t0) ? MS_D : -1;
+          } else {
+            const double tt = th[(s0 == MS_D) ? 3 : 4];
+            sl = (roll < tt) ? MS_M : (tt == tt) ? s0 : -1;
+          }
         }
-        const int a = md_choose(v, 4, rng);
-        s1 = (a < 0) ? -1 : (a == 0) ? MS_B : (a == 1) ? MS_M : (a == 2) ? MS_I : MS_D;
-        break;
-      }
-      case MS_D: {                                         // from M/D(i, k-1)
-        float v = 0.0f;
-        if (z < 2) v = (k > 1) ? __fmul_rn(cellk(i, k - 1, z), md_tv(p, TR + (z ? VT_DDS : VT_MDS), k)) : 0.0f;
-        const int a = md_choose(v, 2, rng);
-        s1 = (a < 0) ? -1 : a ? MS_D : MS_M;
-        break;
-      }
-      case MS_I: {                                         // from M/I(i-1, k)
-        float v = 0.0f;
-        if (z < 2) v = __fmul_rn(cellk(i - 1, k, z ? 2 : 0), md_tv(p, TR + (z ? VT_II : VT_MI), k));
-        const int a = md_choose(v, 2, rng);
-        s1 = (a < 0) ? -1 : a ? MS_I : MS_M;
-        break;
-      }
-      case MS_N: s1 = (i == 0) ? MS_S : MS_N; break;
-      case MS_C: case MS_J: {
-        if (i < 1) break;                                  // only a degenerate matrix gets here: s1 stays -1
-        const int xs = (s0 == MS_C) ? 4 : 2;
-        float v = 0.0f;
-        if (z == 0) v = __fmul_rn(xv(i - 1, xs), floop);
-        else if (z == 1) v = __fmul_rn(__fmul_rn(xv(i, 0), s0 == MS_C ? xEmove : xEloop), xv(i, 5));
-        const int a = md_choose(v, 2, rng);
-        s1 = (a < 0) ? -1 : a ? MS_E : s0;
-        break;
-      }
-      case MS_B: {
-        float v = 0.0f;
-        if (z < 2) v = __fmul_rn(xv(i, z ? 2 : 1), fmove);
-        const int a = md_choose(v, 2, rng);
-        s1 = (a < 0) ? -1 : a ? MS_J : MS_N;
-        break;
-      }
-      case MS_E: {                                         // one roll; M(i, k), D(i, k) in striped order
-        rng = rng * 69069u + 1u;
-        const double roll = (double) rng / 4294967296.0;
-        const double norm = 1.0 / (double) xv(i, 0);
-        const float nf = (float) norm;
-        double sum = 0.0;
-        for (int pass = 0; pass < 4 && s1 < 0; pass++)
-          for (int q0 = 0; q0 < Q && s1 < 0; q0 += 8) {
-            // lane l: q = q0 + l/8, then M lanes r = 0..3, D lanes r = 0..3
-            const int q = q0 + (z >> 3), s = (z >> 2) & 1, r = z & 3;
-            const float u = (q < Q) ? __fmul_rn(CELL(i, q, s, r), nf) : 0.0f;
-            for (int l = 0; l < 64; l++) {
-              sum = sum + (double) readlanef(u, l);
-              if (roll < sum) {
-                k = (l & 3) * Q + q0 + (l >> 3) + 1;
-                s1 = ((l >> 2) & 1) ? MS_D : MS_M;
-                break;
-              }
+        const unsigned long long stop = __ballot(sl != s0);
+        const int f = stop ? (int) __ffsll((long long) stop) - 1 : 64;
+        if (f > 0) {                                       // the repeats: M and I states emit
+          if (s0 == MS_M) {
+            if (z < f) atomicAdd(&cnt[cnt_at(k - 1 - z)], 1.0f);
+            kmin = min(kmin, k - f); kmax = max(kmax, k - 1);
+            Ld += f;
+            if (sqto == 0) { sqto = i - 1; hmmto = k - 1; }
+            sqfrom = i - f; hmmfrom = k - f;
+            i -= f; k -= f;
+          } else if (s0 == MS_I) {
+            if (z == 0) atomicAdd(&cnt[cnt_at(k)], (float) f);
+            kmin = min(kmin, k); kmax = max(kmax, k);
+            Ld += f;
+            i -= f;
+          } else if (s0 == MS_D) k -= f;
+          else i -= f;
+        }
+        steps += f;
+        if (f == 64) {
+          rng = (uint32_t) __builtin_amdgcn_readlane((int) xl, 63);
+          if (steps > max_steps) { status = 1; break; }
+          continue;
+        }
+        rng = (uint32_t) __builtin_amdgcn_readlane((int) xl, f);
+        s1 = __builtin_amdgcn_readlane(sl, f);
+        if (++steps > max_steps) { status = 1; break; }
+      } else {
+        if (++steps > max_steps) { status = 1; break; }
+        switch (s0) {
+        case MS_N: s1 = MS_S; i = 0; break;                // N repeats down to row 0 without draws
+        case MS_B: {
+          rng = rng * 69069u + 1u;
+          const double roll = (double) rng / 4294967296.0, th = xthr[(size_t) i * 3 + 2];
+          s1 = (roll < th) ? MS_N : (th == th) ? MS_J : -1;
+          break;
+        }
+        case MS_E: {                                       // one roll; the first running sum above it
+          if (i < 1) break;
+          rng = rng * 69069u + 1u;
+          const double roll = (double) rng / 4294967296.0;
+          const double *er = ecum + (size_t) i * 8 * Q;
+          int found = -1;
+          for (int j0 = 0; j0 < 8 * Q && found < 0; j0 += 64 * 8) {
+            double v[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++) { const int j = j0 + u * 64 + z; v[u] = er[min(j, 8 * Q - 1)]; }
+#pragma unroll
+            for (int u = 0; u < 8; u++) {
+              const int j = j0 + u * 64 + z;
+              const unsigned long long m = __ballot(j < 8 * Q && roll < v[u]);
+              if (m && found < 0) found = j0 + u * 64 + (int) __ffsll((long long) m) - 1;
             }
           }
-        break;
-      }
+          if (found >= 0) {
+            k = (found & 3) * Q + (found >> 3) + 1;
+            s1 = ((found >> 2) & 1) ? MS_D : MS_M;
+          } else {
+            // the reference scans the row again, the sum going on from its total
+            const float nf = (float) (1.0 / (double) xmx[(size_t) i * 6 + 0]);
+            double sum = er[8 * Q - 1];
+            for (int pass = 1; pass < 4 && s1 < 0; pass++)
+              for (int q0 = 0; q0 < Q && s1 < 0; q0 += 8) {
+                const int q = q0 + (z >> 3), sm = (z >> 2) & 1, r = z & 3;
+                const float u = (q < Q) ? __fmul_rn(CELL(i, q, sm, r), nf) : 0.0f;
+                for (int l = 0; l < 64; l++) {
+                  sum = sum + (double) readlanef(u, l);
+                  if (roll < sum) { k = (l & 3) * Q + q0 + (l >> 3) + 1; s1 = ((l >> 2) & 1) ? MS_D : MS_M; break; }
+                }
+              }
+          }
+          break;
+        }
+        }
       }
       if (s1 < 0) { status = 1; break; }
       if (s0 == MS_M) { k--; i--; }
@@ -309,16 +442,17 @@ k_md_region(const DevProfile p, const uint8_t *__restrict__ res, const MdJob *__
       // the trace element s1 at (i, k), in backward order
       if (s1 == MS_M || s1 == MS_I) {
         if (k < 1 || k > M || i < 1) { status = 1; break; }
-        if (z == 0) cnt[((k - 1) % Q) * 4 + (k - 1) / Q] += 1.0f;
+        if (z == 0) atomicAdd(&cnt[cnt_at(k)], 1.0f);      // the node's M cell
+        kmin = min(kmin, k); kmax = max(kmax, k);
         Ld++;
         if (s1 == MS_M) {
           if (sqto == 0) { sqto = i; hmmto = k; }
           sqfrom = i; hmmfrom = k;
         }
       } else if (s1 == MS_D) {
-        if (k < 1 || k > M) { status = 1; break; }
+        if (k < 1 || k > M || i < 1) { status = 1; break; }
       } else if (s1 == MS_E) {
-        sqfrom = sqto = hmmfrom = hmmto = 0; Ld = 0;
+        sqfrom = sqto = hmmfrom = hmmto = 0; Ld = 0; kmin = 0x7fffffff; kmax = 0;
       } else if (s1 == MS_B) {
         // p7_Null2_ByTrace over the domain: four striped sums over q per
         // canonical residue, their hsum; degenerate codes the mean
@@ -327,45 +461,57 @@ k_md_region(const DevProfile p, const uint8_t *__restrict__ res, const MdJob *__
         for (int base = 0; base < p.K * 4; base += 64) {
           const int xr = base + z, x = xr >> 2, r = xr & 3;
           float sv = 0.0f;
-          if (x < p.K)
-            for (int q = 0; q < Q; q++) {
-              const float cq = cnt[q*4 + r];
-              if (cq != 0.0f) sv = __fadd_rn(sv, __fmul_rn(__fmul_rn(cq, nrm), md_tv(p, x, r * Q + q + 1)));
+          if (x < p.K) {
+            // positions kmin..kmax in lane r: the other cells' counts are 0, and
+            // adding their products (+0) leaves the striped sum unchanged
+            const int q0 = max(0, kmin - 1 - r * Q), q1 = min(Q - 1, kmax - 1 - r * Q);
+            for (int qa = q0; qa <= q1; qa += 8) {
+              float e[8];
+#pragma unroll
+              for (int u = 0; u < 8; u++) e[u] = md_tv(p, x, min(r * Q + qa + u + 1, M + 1));
+#pragma unroll
+              for (int u = 0; u < 8; u++)
+                if (qa + u <= q1) {
+                  const float cq = cnt[(qa + u) * 4 + r];
+                  if (cq != 0.0f) sv = __fadd_rn(sv, __fmul_rn(__fmul_rn(cq, nrm), e[u]));
+                }
             }
+          }
           // lanes 4x..4x+3 hold x's lane sums
           const float s01 = __fadd_rn(sv, __shfl_down(sv, 1, 4)), s23 = __shfl_down(s01, 2, 4);
           if (x < p.K && r == 0) n2v[x] = __fadd_rn(__fadd_rn(s01, s23), 0.0f);
         }
         __syncthreads();
-        if (z == 0) {
-          for (int xx = p.K + 1; xx <= p.Kp - 3; xx++) {
+        {                                                  // degenerate codes: one lane each
+          const int xx = p.K + 1 + z;
+          if (xx <= p.Kp - 3) {
             const uint32_t dm = p.degen[xx];
             float rr = 0.0f;
             for (int y = 0; y < p.K; y++) if ((dm >> y) & 1u) rr = __fadd_rn(rr, n2v[y]);
             n2v[xx] = __fdiv_rn(rr, (float) (dm >> 24));
           }
-          n2v[p.K] = 1.0f; n2v[p.Kp - 2] = 1.0f; n2v[p.Kp - 1] = 1.0f;
+          if (z == 0) { n2v[p.K] = 1.0f; n2v[p.Kp - 2] = 1.0f; n2v[p.Kp - 1] = 1.0f; }
         }
         __syncthreads();
         // residues (sqto, hi] are outside domains (1), (sqfrom, sqto] take the odds
-        for (int pos = z + 1; pos <= hi; pos += 64) {
+        for (int pos = z + 1 + ((sqfrom >> 6) << 6); pos <= hi; pos += 64) {
           // the lane of a residue is fixed (pos mod 64), so its updates stay in order
-          if (pos > sqto)        acc[pos] = __fadd_rn(acc[pos], 1.0f);
-          else if (pos > sqfrom) acc[pos] = __fadd_rn(acc[pos], n2v[dsq[pos - 1]]);
+          if (pos > sqto)        accp[pos] = __fadd_rn(accp[pos], 1.0f);
+          else if (pos > sqfrom) accp[pos] = __fadd_rn(accp[pos], n2v[dsq[pos - 1]]);
         }
         hi = sqfrom;
         if (nd >= MD_TSEG) { status = 5; break; }
         if (z == 0) { tseg[nd*4 + 0] = sqfrom; tseg[nd*4 + 1] = sqto; tseg[nd*4 + 2] = hmmfrom; tseg[nd*4 + 3] = hmmto; }
         nd++;
         for (int c = z; c < 4 * Q; c += 64) cnt[c] = 0.0f;
+        kmin = 0x7fffffff; kmax = 0;
         __syncthreads();
       }
-      // N, C and J emit on the transition to themselves (p7_StochasticTrace)
-      if ((s1 == MS_N || s1 == MS_J || s1 == MS_C) && s1 == s0) i--;
+      // (N, C and J emit on the transition to themselves: the repeats above)
       s0 = s1;
     }
     if (status) break;
-    for (int pos = z + 1; pos <= hi; pos += 64) acc[pos] = __fadd_rn(acc[pos], 1.0f);
+    for (int pos = z + 1; pos <= hi; pos += 64) accp[pos] = __fadd_rn(accp[pos], 1.0f);
     // the trace's domains join the ensemble in sequence order
     if (nseg + nd > J.segcap) { status = 5; break; }
     __syncthreads();
@@ -377,7 +523,7 @@ k_md_region(const DevProfile p, const uint8_t *__restrict__ res, const MdJob *__
     __syncthreads();
   }
   if (status) { if (z == 0) { O->status = status; O->nc = 0; O->nseg = nseg; } return; }
-  for (int pos = z + 1; pos <= Lr; pos += 64) acc[pos] = logf(__fdiv_rn(acc[pos], (float) nsamples));
+  for (int pos = z + 1; pos <= Lr; pos += 64) acc[pos] = logf(__fdiv_rn(accp[pos], (float) nsamples));
   __syncthreads();
 
   // ---------------- 3. clustering ----------------
@@ -512,20 +658,17 @@ k_md_region(const DevProfile p, const uint8_t *__restrict__ res, const MdJob *__
 // traces' counts, odds and per-trace domains, or the clustering's stacks
 size_t md_lds_bytes(int M, int segcap)
 {
-  const size_t Q = (size_t) std::max(2, (M - 1) / 4 + 1);
-  const size_t fwd = 28 * Q * 4;
-  const size_t trc = (4 * Q + 32) * 4 + (size_t) MD_TSEG * 4 * 4;
+  const int64_t fwd = (md_stage_t(M) ? md_stage_floats(M) : 0) + 32 * md_q(M);
   const size_t clu = (size_t) 4 * segcap * 4 + (size_t) (segcap + 63) / 64 * 8 + (size_t) MD_MAXSIG * sizeof(MdSig);
-  return std::max(fwd, std::max(trc, clu));
+  return std::max((size_t) std::max(fwd, md_trace_floats(M)) * 4, clu);
 }
 
 // floats of one region's area in the pool: matrix, specials, residue sums,
-// endpoint counts, then the ensemble
+// endpoint counts, the ensemble, then the traces' thresholds (doubles)
 int64_t md_area_floats(int M, int Lr, int segcap)
 {
-  const int64_t Q = std::max(2, (M - 1) / 4 + 1);
-  const int64_t f = (int64_t) (Lr + 1) * 12 * Q + 6 * (int64_t) (Lr + 1) + (Lr + 1) + (std::max(Lr, M) + 2)
-                  + (int64_t) segcap * (int64_t) (sizeof(MdSeg) / 4);
+  const int64_t Q = md_q(M), R = (int64_t) Lr + 1;
+  const int64_t f = md_thr_at(M, Lr, segcap) + 2 * (5 * R * ((int64_t) M + 1) + 3 * R + 8 * Q * R);
   return (f + 63) & ~(int64_t) 63;
 }
 

@@ -2925,8 +2925,7 @@ p7_gpu_RegionTraceEnsemble(const uint8_t *dsq, int Lr, int Ltarget, const P7_GPU
   if (e == hipSuccess && opt_xmx) e = hipMemcpyAsync(opt_xmx, pool + (size_t) (Lr + 1) * 12 * Q, sizeof(float) * 6 * (size_t) (Lr + 1), hipMemcpyDeviceToHost, g_stream);
   if (e == hipSuccess) e = hipStreamSynchronize(g_stream);
   if (e == hipSuccess && opt_segs && O.nseg > 0) {
-    const int64_t segs_at = md_n2sc_at(p->M, Lr) + (Lr + 1) + (std::max(Lr, p->M) + 2);
-    e = hipMemcpy(opt_segs, pool + segs_at, sizeof(int32_t) * 5 * (size_t) std::min(O.nseg, segcap), hipMemcpyDeviceToHost);
+    e = hipMemcpy(opt_segs, pool + md_seg_at(p->M, Lr), sizeof(int32_t) * 5 * (size_t) std::min(O.nseg, segcap), hipMemcpyDeviceToHost);
   }
   (void) hipFree(m);
   if (e != hipSuccess) { p7g_set_error("HIP error %d (%s)", (int) e, hipGetErrorString(e)); return p7G_EDEVICE; }
