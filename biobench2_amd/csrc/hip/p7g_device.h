@@ -170,6 +170,7 @@ struct DomOut {
   int32_t *nreg = nullptr, *nclu = nullptr;
   int4    *regs = nullptr;
   int64_t  reg_cap = 0;
+  int32_t *rseq = nullptr, *rlen = nullptr;   // per record: the caller's index and the target's length
   int32_t *nclu_total = nullptr;   // regions flagged multidomain (zeroed before a run)
   // [0] regions written, [8] the envelope-scoring work-queue cursor (the list
   // kernels' count/queue layout); zeroed before a run
@@ -323,6 +324,7 @@ hipError_t launch_stats(const DevSeqs &s, const uint8_t *stage, int64_t n, const
                         const int32_t *counts, unsigned long long *out, hipStream_t st);
 // dst[perm[slot]] = src[slot] for 4-byte and 1-byte per-slot arrays.
 hipError_t launch_scatter32(const DevSeqs &s, const void *src, void *dst, int64_t n, hipStream_t st);
+hipError_t launch_rec_gather(const DevSeqs &s, const int4 *regs, int64_t n, int32_t *rseq, int32_t *rlen, hipStream_t st);
 hipError_t launch_scatter8(const DevSeqs &s, const uint8_t *src, uint8_t *dst, int64_t n, hipStream_t st);
 hipError_t launch_bck(const DevProfile &p, const DevSeqs &s, const int32_t *list, const int32_t *count, int64_t maxn,
                       DevResults r, const float *scale_pool, const int64_t *scale_off, const float *fwd_xmx,

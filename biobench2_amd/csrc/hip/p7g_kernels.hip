@@ -2577,6 +2577,24 @@ hipError_t launch_stats(const DevSeqs &s, const uint8_t *stage, int64_t n, const
   hipLaunchKernelGGL(k_stats, dim3(grid), dim3(256), 0, st, s.len, stage, n, vit, fwd, counts, out);
   return hipGetLastError();
 }
+// each region record's caller index and target length, taken while the
+// database is at hand, so that the run's results outlive it
+__global__ void k_rec_gather(const int4 *__restrict__ regs, int64_t n, const int32_t *__restrict__ perm,
+                             const int32_t *__restrict__ len, int32_t *__restrict__ rseq, int32_t *__restrict__ rlen)
+{
+  for (int64_t r = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; r < n; r += (int64_t) gridDim.x * blockDim.x) {
+    const int slot = regs[r].x;
+    rseq[r] = perm[slot];
+    rlen[r] = len[slot];
+  }
+}
+hipError_t launch_rec_gather(const DevSeqs &s, const int4 *regs, int64_t n, int32_t *rseq, int32_t *rlen, hipStream_t st)
+{
+  if (n <= 0) return hipSuccess;
+  unsigned grid = (unsigned) std::min<int64_t>((n + 255) / 256, 4096);
+  hipLaunchKernelGGL(k_rec_gather, dim3(grid), dim3(256), 0, st, regs, n, s.perm, s.len, rseq, rlen);
+  return hipGetLastError();
+}
 hipError_t launch_scatter32(const DevSeqs &s, const void *src, void *dst, int64_t n, hipStream_t st)
 {
   if (n <= 0) return hipSuccess;

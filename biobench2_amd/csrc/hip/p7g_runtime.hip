@@ -915,10 +915,10 @@ dom_reserve(Work &w, int64_t n, int64_t pool_floats)
   if (w.dom_mem) { (void) hipFree(w.dom_mem); w.dom_mem = nullptr; w.dom_cap = 0; }
   if (w.dom.pool) { (void) hipFree(w.dom.pool); w.dom.pool = nullptr; w.dom.size = 0; }
   const int64_t cap = std::max<int64_t>(n, 1024);
-  const size_t total = (size_t) cap * (8 + 8 + 4 + 4 + 4) + (16 + 8 + 8 + 4 * 9) * (size_t) DOM_REG_CAP
+  const size_t total = (size_t) cap * (8 + 8 + 4 + 4 + 4) + (16 + 8 + 8 + 4 * 11) * (size_t) DOM_REG_CAP
                        + (size_t) DOM_RES_CAP + 4 * (size_t) N2_CNT_INTS
                        + 4 * (size_t) (DOM_RES_CAP + DOM_REG_CAP) + 8 * (size_t) DOM_REG_CAP
-                       + (4 + 16) * (size_t) DOM_REG_CAP + 26 * 256;
+                       + (4 + 16) * (size_t) DOM_REG_CAP + 28 * 256;
   char *m = nullptr;
   HIPCHK(hipMalloc((void **) &m, total));
   size_t o = 0;
@@ -926,6 +926,7 @@ dom_reserve(Work &w, int64_t n, int64_t pool_floats)
   w.dom.doff = (int64_t *) take(8 * cap); w.dom.boff = (int64_t *) take(8 * cap);
   w.dout.nexp = (float *) take(4 * cap); w.dout.nreg = (int32_t *) take(4 * cap); w.dout.nclu = (int32_t *) take(4 * cap);
   w.dout.regs = (int4 *) take(16 * (size_t) DOM_REG_CAP); w.dout.reg_cap = DOM_REG_CAP;
+  w.dout.rseq = (int32_t *) take(4 * (size_t) DOM_REG_CAP); w.dout.rlen = (int32_t *) take(4 * (size_t) DOM_REG_CAP);
   w.dout.nclu_total = (int32_t *) take(4);
   w.dout.ecount = (int32_t *) take(64);
   w.dout.v_off = (int64_t *) take(8 * (size_t) DOM_REG_CAP);
@@ -1185,13 +1186,15 @@ struct p7g_pipeline_s {
   mutable std::vector<int4> regs;
   mutable std::vector<float> envsc, n2dc, oasc;
   mutable std::vector<int4> ali;
-  mutable std::vector<int32_t> n2st, regs_slot;
+  mutable std::vector<int32_t> n2st, regs_slot, regs_L;
   // hit-list scores of the last run, computed on first request
   mutable bool hits_ready = false;
   mutable std::vector<int64_t> h_tseq;
   mutable std::vector<P7G_TSCORE> h_ts;
   mutable std::vector<int32_t> h_tnd, h_tst;
   mutable std::vector<P7G_DSCORE> h_ds;
+  mutable std::vector<float> h_nexp;           // per target: P7_DOMAINDEF's nexpected,
+  mutable std::vector<int4> h_tcnt;            // nregions, nclustered, noverlaps, nenvelopes
   mutable bool regs_ready = false;
   bool   regs_valid = false;
   int64_t nregs = 0;
@@ -1204,10 +1207,10 @@ struct p7g_pipeline_s {
   // for the hit-list scores: the run's Forward scores (by slot), target
   // lengths (by slot) and the profile's Forward tail parameters
   const float *fwdsc_dev = nullptr;
-  const int32_t *regs_len = nullptr;
+  const float *nexp_dev = nullptr;             // nexpected by slot
+  const int32_t *rseq_dev = nullptr, *rlen_dev = nullptr;   // per record: caller index, target length
   int64_t nslots = 0;
   float ftau = 0.0f, flambda = 0.0f;
-  const int64_t *regs_perm = nullptr;
   P7_GPU_STATS st;
   Work       w;             // this pipeline's stream and device work area
   Batch     *b = nullptr;   // work areas and graph of p7_gpu_PipelineBatch, made on first use
@@ -1765,12 +1768,17 @@ p7_gpu_Pipeline(P7_GPU_PIPELINE *pli, const P7_GPU_OPROFILE *om, const P7_GPU_SE
       HIPCHK(hipEventRecord(w.ev[9], w.st));
       HIPCHK(hipEventSynchronize(w.ev[9]));
     }
-    std::lock_guard<std::mutex> rl(pli->regs_mu);
     if (nrecs < nregs_dev) nrecs = nregs_dev;
-    pli->nregs = nrecs; pli->regs_dev = w.dout.regs; pli->envsc_dev = w.dout.envsc; pli->regs_perm = db->h_perm;
+    // the records' caller indices and target lengths, so that the results
+    // do not read the database after the call
+    HIPCHK(launch_rec_gather(s, w.dout.regs, nrecs, w.dout.rseq, w.dout.rlen, w.st));
+    HIPCHK(hipStreamSynchronize(w.st));
+    std::lock_guard<std::mutex> rl(pli->regs_mu);
+    pli->nregs = nrecs; pli->regs_dev = w.dout.regs; pli->envsc_dev = w.dout.envsc;
+    pli->rseq_dev = w.dout.rseq; pli->rlen_dev = w.dout.rlen;
     pli->n2dc_dev = pli->do_null2 ? w.n2dc : nullptr; pli->n2st_dev = w.n2st;
     pli->oa_dev = pli->do_null2 ? w.n2oa : nullptr; pli->ali_dev = pli->do_null2 ? w.n2ali : nullptr;
-    pli->fwdsc_dev = w.r.fwdsc; pli->regs_len = db->h_len; pli->nslots = n;
+    pli->fwdsc_dev = w.r.fwdsc; pli->nexp_dev = w.dout.nexp; pli->nslots = n;
     pli->ftau = om->evparam[p7G_FTAU]; pli->flambda = om->evparam[p7G_FLAMBDA];
     pli->regs_ready = false; pli->regs.clear();
     pli->hits_ready = false;
@@ -1832,8 +1840,11 @@ regs_fetch(const P7_GPU_PIPELINE *pli)
     std::vector<int32_t> n2(nr, -1);
     std::vector<float> oa(nr, NAN);
     std::vector<int4> al(nr, make_int4(0, 0, 0, 0));
+    std::vector<int32_t> rseq(nr), rlen(nr);
     if (nr) {
       HIPCHK(hipMemcpy(rec.data(), pli->regs_dev, sizeof(int4) * nr, hipMemcpyDeviceToHost));
+      HIPCHK(hipMemcpy(rseq.data(), pli->rseq_dev, sizeof(int32_t) * nr, hipMemcpyDeviceToHost));
+      HIPCHK(hipMemcpy(rlen.data(), pli->rlen_dev, sizeof(int32_t) * nr, hipMemcpyDeviceToHost));
       HIPCHK(hipMemcpy(sc.data(), pli->envsc_dev, sizeof(float) * nr, hipMemcpyDeviceToHost));
       if (pli->n2dc_dev) {
         HIPCHK(hipMemcpy(dc.data(), pli->n2dc_dev, sizeof(float) * nr, hipMemcpyDeviceToHost));
@@ -1845,18 +1856,19 @@ regs_fetch(const P7_GPU_PIPELINE *pli)
       }
     }
     std::vector<int32_t> slot_of(nr);
-    for (size_t k = 0; k < nr; k++) { slot_of[k] = rec[k].x; rec[k].x = (int) pli->regs_perm[rec[k].x]; }
+    for (size_t k = 0; k < nr; k++) { slot_of[k] = rec[k].x; rec[k].x = rseq[k]; }
     std::vector<int64_t> ord((size_t) pli->nregs);
     for (int64_t k = 0; k < pli->nregs; k++) ord[(size_t) k] = k;
     std::sort(ord.begin(), ord.end(), [&](int64_t a, int64_t b) {
       const int4 &x = rec[(size_t) a], &y = rec[(size_t) b];
       return x.x != y.x ? x.x < y.x : x.y != y.y ? x.y < y.y : x.w < y.w; });
     pli->regs.resize(nr); pli->envsc.resize(nr); pli->n2dc.resize(nr); pli->n2st.resize(nr); pli->regs_slot.resize(nr);
+    pli->regs_L.resize(nr);
     pli->oasc.resize(nr); pli->ali.resize(nr);
     for (size_t k = 0; k < nr; k++) {
       const size_t o = (size_t) ord[k];
       pli->regs[k] = rec[o]; pli->envsc[k] = sc[o]; pli->n2dc[k] = dc[o]; pli->n2st[k] = n2[o];
-      pli->regs_slot[k] = slot_of[o];
+      pli->regs_slot[k] = slot_of[o]; pli->regs_L[k] = rlen[o];
       // alignment coordinates in the target: the region copy starts at residue ienv
       const int4 a = al[o];
       pli->oasc[k] = (n2[o] == 0 && a.x > 0) ? oa[o] : NAN;
@@ -1939,17 +1951,20 @@ hit_scores(const P7_GPU_PIPELINE *pli)
   std::vector<P7G_TSCORE> &ts = pli->h_ts;
   std::vector<int32_t> &tnd = pli->h_tnd, &tst = pli->h_tst;
   std::vector<P7G_DSCORE> &ds = pli->h_ds;
-  tseq.clear(); ts.clear(); tnd.clear(); tst.clear();
+  tseq.clear(); ts.clear(); tnd.clear(); tst.clear(); pli->h_nexp.clear(); pli->h_tcnt.clear();
   const size_t nr = pli->regs.size();
-  std::vector<float> fwd((size_t) pli->nslots);
-  if (pli->nslots) HIPCHK(hipMemcpy(fwd.data(), pli->fwdsc_dev, sizeof(float) * (size_t) pli->nslots, hipMemcpyDeviceToHost));
+  std::vector<float> fwd((size_t) pli->nslots), nexp((size_t) pli->nslots);
+  if (pli->nslots) {
+    HIPCHK(hipMemcpy(fwd.data(), pli->fwdsc_dev, sizeof(float) * (size_t) pli->nslots, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(nexp.data(), pli->nexp_dev, sizeof(float) * (size_t) pli->nslots, hipMemcpyDeviceToHost));
+  }
   const float omega = (float) (1. / 256.);                    // p7_bg.c:75
   ds.assign(nr, P7G_DSCORE{ NAN, NAN, NAN });
   for (size_t a = 0; a < nr; ) {
     size_t b = a;
     while (b < nr && pli->regs[b].x == pli->regs[a].x) b++;    // regions a..b-1: one target
     const int slot = pli->regs_slot[a];
-    const int L = pli->regs_len[slot];
+    const int L = pli->regs_L[a];
     // domains: single-domain regions (w 0) and multidomain regions' clusters
     // (w 2) whose rescoring succeeded; the sequence bias sums the null2 scores
     // of every residue: the single-domain envelopes' and, for a multidomain
@@ -1957,6 +1972,23 @@ hit_scores(const P7_GPU_PIPELINE *pli)
     std::vector<P7G_DOMIN> dom;
     std::vector<size_t> which;
     std::vector<float> parts;
+    // P7_DOMAINDEF's counters (p7_domaindef.c:426-483): a cluster overlaps
+    // when it starts at or before the end of the last cluster of its region
+    // whose rescoring succeeded
+    int4 cnt = make_int4(0, 0, 0, 0);
+    int last_j2 = 0;
+    for (size_t k = a; k < b; k++) {
+      const int kind = pli->regs[k].w;
+      if (kind != 2) { cnt.x++; last_j2 = 0; }
+      if (kind == 1) cnt.y++;
+      else {
+        cnt.w++;
+        if (kind == 2) {
+          if (pli->regs[k].y <= last_j2) cnt.z++;
+          if (pli->n2st[k] == 0) last_j2 = pli->regs[k].z;
+        }
+      }
+    }
     for (size_t k = a; k < b; k++) {
       const int kind = pli->regs[k].w;
       if (kind == 1) { parts.push_back(pli->n2dc[k]); continue; }
@@ -1972,7 +2004,10 @@ hit_scores(const P7_GPU_PIPELINE *pli)
     for (size_t q = 0; q < dom.size(); q++) ds[which[q]] = d[q];
     if (! dom.empty()) t.best = (int) (which[(size_t) t.best] - a);
     else t.best = -1;
-    tseq.push_back(pli->regs[a].x); ts.push_back(t); tnd.push_back((int32_t) dom.size()); tst.push_back(0);
+    // no envelope (every region multidomain, no significant cluster): not a
+    // hit (p7_pipeline.c:739-740)
+    tseq.push_back(pli->regs[a].x); ts.push_back(t); tnd.push_back((int32_t) dom.size()); tst.push_back(cnt.w ? 0 : 1);
+    pli->h_nexp.push_back(nexp[(size_t) slot]); pli->h_tcnt.push_back(cnt);
     a = b;
   }
   pli->hits_ready = true;
@@ -2035,23 +2070,22 @@ p7_gpu_pipeline_GetDomainScores(const P7_GPU_PIPELINE *pli, float *bitscore, flo
 // if also <= incE; domZ = the number of reported targets; a domain of a
 // reported target is reported if exp(lnP_d)·domZ <= domE, and included if
 // its target is and exp(lnP_d)·domZ <= incdomE.
-extern "C" int
-p7_gpu_pipeline_GetHitList(const P7_GPU_PIPELINE *pli, double Z, double E, double incE, double domE, double incdomE,
-                           int64_t *order, int32_t *flags, int32_t *dflags, double *ret_domZ, int64_t cap, int64_t *ret_n)
+// p7_tophits_SortBySortkey and p7_tophits_Threshold with E-value thresholds
+// (p7_tophits.c:866-935): the scored targets ranked by -lnP (ties in
+// sequence order), their flags (bit 0 reported, bit 1 included), the regions'
+// domain flags, and domZ: domZ_set when it is >= 0 (--domZ), else the number
+// of reported targets
+static void
+hit_list(const P7_GPU_PIPELINE *pli, double Z, double E, double incE, double domE, double incdomE, double domZ_set,
+         std::vector<int64_t> &ord, std::vector<int32_t> &tf, std::vector<int32_t> &df, double &domZ)
 {
-  if (! pli || ! pli->regs_valid) { p7g_set_error("no domain regions: the last p7_gpu_Pipeline call did not decode"); return p7G_EINVAL; }
-  if (! pli->n2dc_dev) { p7g_set_error("the last p7_gpu_Pipeline call ran without null2"); return p7G_EINVAL; }
-  std::lock_guard<std::mutex> rl(pli->regs_mu);
-  int st;
-  if ((st = regs_fetch(pli)) != p7G_OK) return st;
-  if ((st = hit_scores(pli)) != p7G_OK) return st;
   const size_t nt = pli->h_tseq.size(), nr = pli->regs.size();
-  std::vector<int64_t> ord;
+  ord.clear();
   for (size_t k = 0; k < nt; k++) if (pli->h_tst[k] == 0) ord.push_back((int64_t) k);
   std::stable_sort(ord.begin(), ord.end(), [&](int64_t a, int64_t b) {
     const double ka = -pli->h_ts[(size_t) a].lnP, kb = -pli->h_ts[(size_t) b].lnP;
     return ka > kb; });
-  std::vector<int32_t> tf(nt, 0);
+  tf.assign(nt, 0);
   int64_t nrep = 0;
   for (int64_t k : ord) {
     const double lnP = pli->h_ts[(size_t) k].lnP;
@@ -2060,20 +2094,58 @@ p7_gpu_pipeline_GetHitList(const P7_GPU_PIPELINE *pli, double Z, double E, doubl
       if (exp(lnP) * Z <= incE) tf[(size_t) k] |= 2;
     }
   }
-  const double domZ = (double) nrep;
-  if (dflags) {
-    for (size_t q = 0; q < nr; q++) dflags[q] = 0;
-    size_t a = 0;
-    for (size_t k = 0; k < nt; k++) {                 // targets and regions are both in sequence order
-      while (a < nr && (int64_t) pli->regs[a].x != pli->h_tseq[k]) a++;
-      for (size_t q = a; q < nr && (int64_t) pli->regs[q].x == pli->h_tseq[k]; q++) {
-        if (! (tf[k] & 1) || std::isnan(pli->h_ds[q].bitscore)) continue;
-        const double e = exp(pli->h_ds[q].lnP) * domZ;
-        if (e <= domE) dflags[q] |= 1;
-        if ((tf[k] & 2) && e <= incdomE) dflags[q] |= 2;
-      }
+  domZ = (domZ_set >= 0.0) ? domZ_set : (double) nrep;
+  df.assign(nr, 0);
+  size_t a = 0;
+  for (size_t k = 0; k < nt; k++) {                   // targets and regions are both in sequence order
+    while (a < nr && (int64_t) pli->regs[a].x != pli->h_tseq[k]) a++;
+    if (pli->h_tst[k] != 0) continue;
+    for (size_t q = a; q < nr && (int64_t) pli->regs[q].x == pli->h_tseq[k]; q++) {
+      if (! (tf[k] & 1) || std::isnan(pli->h_ds[q].bitscore)) continue;
+      const double e = exp(pli->h_ds[q].lnP) * domZ;
+      if (e <= domE) df[q] |= 1;
+      if ((tf[k] & 2) && e <= incdomE) df[q] |= 2;
+    }
+    // workaround_bug_h74 (p7_tophits.c:678-698): in a target with
+    // overlapping clusters, of two domains with the same alignment start and
+    // end only the higher bit score stays reported and included
+    if (pli->h_tcnt[k].z > 0) {
+      std::vector<size_t> dq;
+      for (size_t q = a; q < nr && (int64_t) pli->regs[q].x == pli->h_tseq[k]; q++)
+        if (! std::isnan(pli->h_ds[q].bitscore)) dq.push_back(q);
+      for (size_t d1 = 0; d1 < dq.size(); d1++)
+        for (size_t d2 = d1 + 1; d2 < dq.size(); d2++) {
+          const int4 &x = pli->ali[dq[d1]], &y = pli->ali[dq[d2]];
+          if (x.z == y.z && x.w == y.w)
+            df[pli->h_ds[dq[d1]].bitscore >= pli->h_ds[dq[d2]].bitscore ? dq[d2] : dq[d1]] = 0;
+        }
     }
   }
+}
+
+static int
+hits_ready(const P7_GPU_PIPELINE *pli)
+{
+  if (! pli || ! pli->regs_valid) { p7g_set_error("no domain regions: the last p7_gpu_Pipeline call did not decode"); return p7G_EINVAL; }
+  if (! pli->n2dc_dev) { p7g_set_error("the last p7_gpu_Pipeline call ran without null2"); return p7G_EINVAL; }
+  int st;
+  if ((st = regs_fetch(pli)) != p7G_OK) return st;
+  return hit_scores(pli);
+}
+
+extern "C" int
+p7_gpu_pipeline_GetHitList(const P7_GPU_PIPELINE *pli, double Z, double E, double incE, double domE, double incdomE,
+                           int64_t *order, int32_t *flags, int32_t *dflags, double *ret_domZ, int64_t cap, int64_t *ret_n)
+{
+  if (! pli) { p7g_set_error("no pipeline"); return p7G_EINVAL; }
+  std::lock_guard<std::mutex> rl(pli->regs_mu);
+  int st;
+  if ((st = hits_ready(pli)) != p7G_OK) return st;
+  std::vector<int64_t> ord;
+  std::vector<int32_t> tf, df;
+  double domZ;
+  hit_list(pli, Z, E, incE, domE, incdomE, -1.0, ord, tf, df, domZ);
+  if (dflags) for (size_t q = 0; q < df.size(); q++) dflags[q] = df[q];
   if (ret_domZ) *ret_domZ = domZ;
   const int64_t n = (int64_t) ord.size();
   if (ret_n) *ret_n = n;
@@ -2083,6 +2155,134 @@ p7_gpu_pipeline_GetHitList(const P7_GPU_PIPELINE *pli, double Z, double E, doubl
   }
   return p7G_OK;
 }
+
+extern "C" int
+p7_gpu_pipeline_GetTargetCounts(const P7_GPU_PIPELINE *pli, float *nexpected, int32_t *nregions, int32_t *nclustered,
+                                int32_t *noverlaps, int32_t *nenvelopes, int64_t cap, int64_t *ret_n)
+{
+  if (! pli) { p7g_set_error("no pipeline"); return p7G_EINVAL; }
+  std::lock_guard<std::mutex> rl(pli->regs_mu);
+  int st;
+  if ((st = hits_ready(pli)) != p7G_OK) return st;
+  const int64_t n = (int64_t) pli->h_tseq.size();
+  if (ret_n) *ret_n = n;
+  for (int64_t k = 0; k < std::min(n, cap); k++) {
+    const int4 &c = pli->h_tcnt[(size_t) k];
+    if (nexpected) nexpected[k] = pli->h_nexp[(size_t) k];
+    if (nregions) nregions[k] = c.x;
+    if (nclustered) nclustered[k] = c.y;
+    if (noverlaps) noverlaps[k] = c.z;
+    if (nenvelopes) nenvelopes[k] = c.w;
+  }
+  return p7G_OK;
+}
+
+// The reported list as the text writers take it: targets in rank order with
+// their domains (rescored envelopes, in sequence order) and flags
+static int
+report_list(const P7_GPU_PIPELINE *pli, const P7_GPU_REPORT *rep, std::vector<P7G_RHIT> &hits,
+            std::vector<P7G_RDOM> &doms, P7G_RLIST &lst)
+{
+  if (! rep || ! rep->qname || ! rep->tname) { p7g_set_error("the report needs the query's name and the targets' names"); return p7G_EINVAL; }
+  int st;
+  if ((st = hits_ready(pli)) != p7G_OK) return st;
+  std::vector<int64_t> ord;
+  std::vector<int32_t> tf, df;
+  double domZ;
+  hit_list(pli, rep->Z, rep->E, rep->incE, rep->domE, rep->incdomE, rep->domZ, ord, tf, df, domZ);
+  // first region of each target
+  const size_t nt = pli->h_tseq.size(), nr = pli->regs.size();
+  std::vector<size_t> first(nt, nr);
+  for (size_t k = 0, a = 0; k < nt; k++) {
+    while (a < nr && (int64_t) pli->regs[a].x != pli->h_tseq[k]) a++;
+    first[k] = a;
+  }
+  hits.clear(); doms.clear();
+  std::vector<size_t> dstart;
+  for (int64_t k64 : ord) {
+    const size_t k = (size_t) k64;
+    const int64_t s = pli->h_tseq[k];
+    P7G_RHIT h;
+    memset(&h, 0, sizeof(h));
+    h.name = rep->tname[s];
+    h.acc = (rep->tacc && rep->tacc[s] && rep->tacc[s][0]) ? rep->tacc[s] : nullptr;
+    h.desc = (rep->tdesc && rep->tdesc[s] && rep->tdesc[s][0]) ? rep->tdesc[s] : nullptr;
+    h.L = pli->regs_L[first[k]];
+    h.flags = tf[k];
+    h.score = pli->h_ts[k].score; h.pre_score = pli->h_ts[k].pre_score; h.lnP = pli->h_ts[k].lnP;
+    h.nexpected = pli->h_nexp[k];
+    const int4 &c = pli->h_tcnt[k];
+    h.nregions = c.x; h.nclustered = c.y; h.noverlaps = c.z; h.nenvelopes = c.w;
+    h.best = 0;
+    dstart.push_back(doms.size());
+    for (size_t q = first[k]; q < nr && pli->regs[q].x == s; q++) {
+      if (std::isnan(pli->h_ds[q].bitscore)) continue;
+      P7G_RDOM d;
+      d.bitscore = pli->h_ds[q].bitscore; d.dombias = pli->h_ds[q].dombias; d.lnP = pli->h_ds[q].lnP;
+      d.oasc = pli->oasc[q];
+      d.hmmfrom = pli->ali[q].x; d.hmmto = pli->ali[q].y; d.alifrom = pli->ali[q].z; d.alito = pli->ali[q].w;
+      d.ienv = pli->regs[q].y; d.jenv = pli->regs[q].z;
+      d.flags = df[q];
+      if ((int) (q - first[k]) == pli->h_ts[k].best) h.best = (int) (doms.size() - dstart.back());
+      if (d.flags & 1) h.nreported++;
+      if (d.flags & 2) h.nincluded++;
+      doms.push_back(d);
+    }
+    h.ndom = (int) (doms.size() - dstart.back());
+    hits.push_back(h);
+  }
+  for (size_t h = 0; h < hits.size(); h++) hits[h].dom = doms.data() + dstart[h];
+  memset(&lst, 0, sizeof(lst));
+  lst.hit = hits.data(); lst.N = (int) hits.size();
+  lst.qname = rep->qname;
+  lst.qacc = (rep->qacc && rep->qacc[0]) ? rep->qacc : nullptr;
+  lst.M = rep->M;
+  lst.Z = rep->Z; lst.domZ = domZ;
+  lst.textw = rep->textw; lst.show_accessions = rep->show_accessions;
+  return p7G_OK;
+}
+
+// the four writers share their arguments: which = 0 targets, 1 domains,
+// 2 tabular targets, 3 tabular domains
+static int
+report_text(const P7_GPU_PIPELINE *pli, const P7_GPU_REPORT *rep, int which, int show_header, char *buf, size_t cap,
+            size_t *ret_len)
+{
+  if (! pli) { p7g_set_error("no pipeline"); return p7G_EINVAL; }
+  std::lock_guard<std::mutex> rl(pli->regs_mu);
+  std::vector<P7G_RHIT> hits;
+  std::vector<P7G_RDOM> doms;
+  P7G_RLIST lst;
+  int st;
+  if ((st = report_list(pli, rep, hits, doms, lst)) != p7G_OK) return st;
+  size_t n = 0;
+  switch (which) {
+  case 0: n = p7g_targets_text(&lst, buf, cap); break;
+  case 1: n = p7g_domains_text(&lst, buf, cap); break;
+  case 2: n = p7g_tabular_targets(&lst, show_header, buf, cap); break;
+  default: n = p7g_tabular_domains(&lst, show_header, buf, cap); break;
+  }
+  if (ret_len) *ret_len = n;
+  return p7G_OK;
+}
+
+extern "C" int
+p7_gpu_pipeline_TargetsText(const P7_GPU_PIPELINE *pli, const P7_GPU_REPORT *rep, char *buf, size_t cap, size_t *ret_len)
+{ return report_text(pli, rep, 0, 0, buf, cap, ret_len); }
+
+extern "C" int
+p7_gpu_pipeline_DomainsText(const P7_GPU_PIPELINE *pli, const P7_GPU_REPORT *rep, char *buf, size_t cap, size_t *ret_len)
+{ return report_text(pli, rep, 1, 0, buf, cap, ret_len); }
+
+extern "C" int
+p7_gpu_pipeline_TabularTargets(const P7_GPU_PIPELINE *pli, const P7_GPU_REPORT *rep, int show_header, char *buf,
+                               size_t cap, size_t *ret_len)
+{ return report_text(pli, rep, 2, show_header, buf, cap, ret_len); }
+
+extern "C" int
+p7_gpu_pipeline_TabularDomains(const P7_GPU_PIPELINE *pli, const P7_GPU_REPORT *rep, int show_header, char *buf,
+                               size_t cap, size_t *ret_len)
+{ return report_text(pli, rep, 3, show_header, buf, cap, ret_len); }
 
 /*****************************************************************
  * Batches: many profiles against one database in one call

@@ -140,6 +140,36 @@ float  p7g_null_one(int L);
 void   p7g_target_scores(int L, float fwdsc, float nullsc, float omega, float tau, float lambda,
                          const P7G_DOMIN *dom, int ndom, const float *n2parts, int nn2, P7G_TSCORE *ts, P7G_DSCORE *ds);
 
+/* the hit list as the text writers take it (host_report.c): targets in rank
+ * order, each with its rescored domains in sequence order */
+typedef struct {
+  float  bitscore, dombias;      /* bits, nats */
+  double lnP;
+  float  oasc;
+  int    hmmfrom, hmmto, alifrom, alito, ienv, jenv;
+  int    flags;                  /* bit 0 reported, bit 1 included */
+} P7G_RDOM;
+typedef struct {
+  const char *name, *acc, *desc; /* acc, desc NULL when empty */
+  int    L, flags, ndom, best, nreported, nincluded;
+  float  score, pre_score, nexpected;
+  double lnP;
+  int    nregions, nclustered, noverlaps, nenvelopes;
+  const P7G_RDOM *dom;
+} P7G_RHIT;
+typedef struct {
+  const P7G_RHIT *hit;
+  int    N;
+  const char *qname, *qacc;      /* qacc NULL when empty */
+  int    M;
+  double Z, domZ;
+  int    textw, show_accessions;
+} P7G_RLIST;
+size_t p7g_targets_text(const P7G_RLIST *l, char *buf, size_t cap);
+size_t p7g_domains_text(const P7G_RLIST *l, char *buf, size_t cap);
+size_t p7g_tabular_targets(const P7G_RLIST *l, int show_header, char *buf, size_t cap);
+size_t p7g_tabular_domains(const P7G_RLIST *l, int show_header, char *buf, size_t cap);
+
 /* error text */
 void p7g_set_error(const char *fmt, ...);
 

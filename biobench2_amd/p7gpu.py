@@ -64,6 +64,14 @@ class Stats(ctypes.Structure):
         return {n: getattr(self, n) for n, _ in self._fields_}
 
 
+class Report(ctypes.Structure):
+    """P7_GPU_REPORT: what the hit-list text needs beyond the run's results."""
+    _fields_ = [("qname", c_char_p), ("qacc", c_char_p), ("M", c_int),
+                ("tname", POINTER(c_char_p)), ("tacc", POINTER(c_char_p)), ("tdesc", POINTER(c_char_p)),
+                ("Z", c_double), ("domZ", c_double), ("E", c_double), ("incE", c_double), ("domE", c_double),
+                ("incdomE", c_double), ("textw", c_int), ("show_accessions", c_int)]
+
+
 def _load():
     if not os.path.exists(LIBPATH):
         raise ImportError(f"{LIBPATH} is missing: build it with __graft_entry__.build() "
@@ -114,6 +122,16 @@ def _load():
                                                 POINTER(c_int64)]),
         "p7_gpu_pipeline_GetHitList": (i, [vp, d, d, d, d, d, POINTER(c_int64), POINTER(c_int32), POINTER(c_int32),
                                            POINTER(c_double), c_int64, POINTER(c_int64)]),
+        "p7_gpu_pipeline_GetTargetCounts": (i, [vp, POINTER(c_float), POINTER(c_int32), POINTER(c_int32),
+                                                POINTER(c_int32), POINTER(c_int32), c_int64, POINTER(c_int64)]),
+        "p7_gpu_pipeline_TargetsText": (i, [vp, POINTER(Report), c_char_p, ctypes.c_size_t,
+                                            POINTER(ctypes.c_size_t)]),
+        "p7_gpu_pipeline_DomainsText": (i, [vp, POINTER(Report), c_char_p, ctypes.c_size_t,
+                                            POINTER(ctypes.c_size_t)]),
+        "p7_gpu_pipeline_TabularTargets": (i, [vp, POINTER(Report), i, c_char_p, ctypes.c_size_t,
+                                               POINTER(ctypes.c_size_t)]),
+        "p7_gpu_pipeline_TabularDomains": (i, [vp, POINTER(Report), i, c_char_p, ctypes.c_size_t,
+                                               POINTER(ctypes.c_size_t)]),
         "p7_gpu_StatisticsText": (ctypes.c_size_t, [POINTER(Stats), d, d, d, d, i, d, i, c_char_p, ctypes.c_size_t]),
         "p7_gpu_pipeline_GetRegions": (i, [vp, POINTER(c_int64), POINTER(c_int32), POINTER(c_int32), POINTER(c_int32),
                                            POINTER(c_float), c_int64, POINTER(c_int64)]),
@@ -656,7 +674,7 @@ class Pipeline:
     def target_scores(self):
         """Per target with regions (sequence order): dict of arrays seq, score, pre_score,
         sum_score (bits), lnP, ndom, best (index within the target's regions() entries, -1
-        none) and status (0 scored)."""
+        none) and status (0 scored; 1 no envelope, not a hit)."""
         n = c_int64()
         _check(lib.p7_gpu_pipeline_GetTargetScores(self.h, *([None] * 8), 0, ctypes.byref(n)), "GetTargetScores")
         o = {"seq": np.zeros(n.value, np.int64), "score": np.zeros(n.value, np.float32),
@@ -682,6 +700,46 @@ class Pipeline:
                                               _ptr(flags, c_int32), _ptr(dflags, c_int32), ctypes.byref(dz), n.value,
                                               ctypes.byref(n)), "GetHitList")
         return order, flags, dflags[:nr], dz.value
+
+    def target_counts(self):
+        """Per target (target_scores() order): dict of arrays nexpected, nregions, nclustered,
+        noverlaps, nenvelopes (P7_DOMAINDEF's counters)."""
+        n = c_int64()
+        _check(lib.p7_gpu_pipeline_GetTargetCounts(self.h, *([None] * 5), 0, ctypes.byref(n)), "GetTargetCounts")
+        o = {"nexpected": np.zeros(n.value, np.float32)}
+        for k in ("nregions", "nclustered", "noverlaps", "nenvelopes"):
+            o[k] = np.zeros(n.value, np.int32)
+        _check(lib.p7_gpu_pipeline_GetTargetCounts(
+            self.h, _ptr(o["nexpected"], c_float), _ptr(o["nregions"], c_int32), _ptr(o["nclustered"], c_int32),
+            _ptr(o["noverlaps"], c_int32), _ptr(o["nenvelopes"], c_int32), n.value, ctypes.byref(n)),
+            "GetTargetCounts")
+        return o
+
+    def report_text(self, kind, qname, M, names, accs=None, descs=None, qacc=None, Z=None, domZ=-1.0, E=10.0,
+                    incE=0.01, domE=10.0, incdomE=0.01, textw=120, show_accessions=False, show_header=True):
+        """The last run's hit list as hmmsearch writes it: kind "targets" (p7_tophits_Targets),
+        "domains" (p7_tophits_Domains without alignments), "tblout" or "domtblout". names (and
+        accs, descs: None or lists with None / "" for none) are indexed by database sequence;
+        Z defaults to the number of names."""
+        enc = lambda v: None if v is None else str(v).encode()
+        arr = lambda vals: None if vals is None else (c_char_p * len(vals))(*[enc(v) for v in vals])
+        tn, ta, td = arr(names), arr(accs), arr(descs)
+        r = Report(enc(qname), enc(qacc), int(M), ctypes.cast(tn, POINTER(c_char_p)),
+                   ctypes.cast(ta, POINTER(c_char_p)) if ta is not None else None,
+                   ctypes.cast(td, POINTER(c_char_p)) if td is not None else None,
+                   float(len(names) if Z is None else Z), float(domZ), E, incE, domE, incdomE, int(textw),
+                   int(bool(show_accessions)))
+        fn = {"targets": lambda b, c, n: lib.p7_gpu_pipeline_TargetsText(self.h, ctypes.byref(r), b, c, n),
+              "domains": lambda b, c, n: lib.p7_gpu_pipeline_DomainsText(self.h, ctypes.byref(r), b, c, n),
+              "tblout": lambda b, c, n: lib.p7_gpu_pipeline_TabularTargets(self.h, ctypes.byref(r), int(show_header),
+                                                                           b, c, n),
+              "domtblout": lambda b, c, n: lib.p7_gpu_pipeline_TabularDomains(self.h, ctypes.byref(r),
+                                                                              int(show_header), b, c, n)}[kind]
+        n = ctypes.c_size_t()
+        _check(fn(None, 0, ctypes.byref(n)), kind)
+        buf = ctypes.create_string_buffer(n.value + 1)
+        _check(fn(buf, n.value + 1, ctypes.byref(n)), kind)
+        return buf.value.decode()
 
     def domain_scores(self):
         """(bitscore, dombias nats, lnP) arrays per region, in regions() order."""

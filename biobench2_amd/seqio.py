@@ -71,3 +71,29 @@ def read_uniprot(path, abctype=3):
             elif inseq:
                 buf.append(line.strip())
     return out
+
+
+def read_uniprot_records(path, abctype=3):
+    """List of (name, accession, description, digital residues) from
+    UniProt/EMBL text records, with the header fields as Easel's EMBL parser
+    keeps them (easel/esl_sqio_ascii.c:2610-2680): the name from ID, the first
+    accession of the first AC line (up to its ';'), and the DE lines' text
+    after their five-character tag, right-trimmed and joined by one space."""
+    out, name, acc, desc, buf, inseq = [], None, "", [], [], False
+    with open(path) as fh:
+        for line in fh:
+            if line.startswith("ID "):
+                name, acc, desc = line.split()[1], "", []
+            elif line.startswith("AC   ") and not acc:
+                acc = line[5:].strip().lstrip(";").split(";")[0].strip()
+            elif line.startswith("DE   "):
+                desc.append(line[5:].rstrip())
+            elif line.startswith("SQ "):
+                inseq, buf = True, []
+            elif line.startswith("//"):
+                if name is not None:
+                    out.append((name, acc, " ".join(desc), digitize("".join(buf), abctype)))
+                name, inseq = None, False
+            elif inseq:
+                buf.append(line.strip())
+    return out

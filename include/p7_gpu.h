@@ -355,7 +355,10 @@ int   p7_gpu_pipeline_GetDomainAlignments(const P7_GPU_PIPELINE *pli, float *oas
  * natural-log P-value of the score under the Forward exponential tail
  * (esl_exp_logsurv with the profile's tau, lambda), the number of domains
  * (envelopes whose rescoring succeeded), the best domain's index among the
- * target's records (GetRegions order, -1 without domains) and a status (0).
+ * target's records (GetRegions order, -1 without domains) and a status: 0,
+ * or 1 for a target without envelopes (each of its regions multidomain
+ * without a significant cluster), which the reference does not make a hit
+ * (p7_pipeline.c:739-740) and the hit list leaves out.
  * The sequence bias sums every residue's null2 score: the single-domain
  * envelopes' and each multidomain region's sampled ones (p7_pipeline.c:760).
  * The E-value of a score is exp(lnP) * Z for a search space of Z targets. */
@@ -383,6 +386,56 @@ int   p7_gpu_pipeline_GetDomainScores(const P7_GPU_PIPELINE *pli, float *bitscor
 int   p7_gpu_pipeline_GetHitList(const P7_GPU_PIPELINE *pli, double Z, double E, double incE, double domE,
                                  double incdomE, int64_t *order, int32_t *flags, int32_t *dflags, double *ret_domZ,
                                  int64_t cap, int64_t *ret_n);
+/* P7_DOMAINDEF's counters of each target, in GetTargetScores' order
+ * (p7_domaindef.c:406-483, copied into the hit at p7_pipeline.c:816-821):
+ * nexpected (btot[L]), the regions the posterior heuristics found, of which
+ * multidomain, the clusters that start at or before the end of the previous
+ * successfully rescored cluster of their region (noverlaps), and the
+ * envelopes (single-domain regions plus clusters, rescored or not). */
+int   p7_gpu_pipeline_GetTargetCounts(const P7_GPU_PIPELINE *pli, float *nexpected, int32_t *nregions,
+                                      int32_t *nclustered, int32_t *noverlaps, int32_t *nenvelopes, int64_t cap,
+                                      int64_t *ret_n);
+/* ---- the hit list as hmmsearch writes it -----------------------------------
+ * What the text needs beyond the last run's results: the query profile's
+ * name, accession (NULL or "" for none) and length M; the targets' names,
+ * accessions and descriptions as arrays indexed by database sequence (the
+ * caller's order; tacc, tdesc or any of their entries may be NULL, and ""
+ * counts as none, as an ESL_SQ's empty acc and desc do); the search spaces and
+ * E-value thresholds (hmmsearch: Z = the number of targets searched unless
+ * --Z, domZ < 0 for the number of reported targets unless --domZ, E = 10,
+ * incE = 0.01, domE = 10, incdomE = 0.01); --textw (120; 0 for --notextw)
+ * and --acc. */
+typedef struct p7g_report_s {
+  const char        *qname;
+  const char        *qacc;
+  int                M;
+  const char *const *tname;
+  const char *const *tacc;
+  const char *const *tdesc;
+  double             Z, domZ;
+  double             E, incE, domE, incdomE;
+  int                textw;
+  int                show_accessions;
+} P7_GPU_REPORT;
+/* p7_tophits_Targets (p7_tophits.c:1055-1180): "Scores for complete
+ * sequences", one line per reported target in rank order, the inclusion
+ * threshold line, or the no-hits line. p7_tophits_Domains (:1200-1380)
+ * without the alignments (hmmsearch --noali): per reported target its name
+ * and description line and its reported domains' table. The tabular forms,
+ * --tblout (p7_tophits_TabularTargets, :1519-1590) and --domtblout
+ * (p7_tophits_TabularDomains, :1610-1700), with their three header lines when
+ * show_header is set (hmmsearch sets it for the first query). Each writes at
+ * most cap bytes of text including the terminating NUL into buf (buf may be
+ * NULL with cap 0 to size it) and the text's full length to *ret_len.
+ * p7G_EINVAL without a decoded run with null2 or without names. */
+int   p7_gpu_pipeline_TargetsText(const P7_GPU_PIPELINE *pli, const P7_GPU_REPORT *rep, char *buf, size_t cap,
+                                  size_t *ret_len);
+int   p7_gpu_pipeline_DomainsText(const P7_GPU_PIPELINE *pli, const P7_GPU_REPORT *rep, char *buf, size_t cap,
+                                  size_t *ret_len);
+int   p7_gpu_pipeline_TabularTargets(const P7_GPU_PIPELINE *pli, const P7_GPU_REPORT *rep, int show_header,
+                                     char *buf, size_t cap, size_t *ret_len);
+int   p7_gpu_pipeline_TabularDomains(const P7_GPU_PIPELINE *pli, const P7_GPU_REPORT *rep, int show_header,
+                                     char *buf, size_t cap, size_t *ret_len);
 /* The "Internal pipeline statistics summary" text of p7_pli_Statistics
  * (p7_pipeline.c:1710-1796) for a search of models against sequences, from
  * the counters of p7_gpu_pipeline_GetStats: the query and target counts, the

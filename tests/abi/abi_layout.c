@@ -32,5 +32,10 @@ int main(void)
   F(P7_GPU_STATS, ms_domains); F(P7_GPU_STATS, n_regions); F(P7_GPU_STATS, n_clustered);
   F(P7_GPU_STATS, ms_null2);
   F(P7_GPU_STATS, ms_multidomain);
+  Z(P7_GPU_REPORT);
+  F(P7_GPU_REPORT, qname); F(P7_GPU_REPORT, qacc); F(P7_GPU_REPORT, M); F(P7_GPU_REPORT, tname);
+  F(P7_GPU_REPORT, tacc); F(P7_GPU_REPORT, tdesc); F(P7_GPU_REPORT, Z); F(P7_GPU_REPORT, domZ);
+  F(P7_GPU_REPORT, E); F(P7_GPU_REPORT, incE); F(P7_GPU_REPORT, domE); F(P7_GPU_REPORT, incdomE);
+  F(P7_GPU_REPORT, textw); F(P7_GPU_REPORT, show_accessions);
   return 0;
 }

@@ -41,7 +41,8 @@ def test_struct_layout_matches_the_ctypes_mirrors(tmp_path):
     exe = str(tmp_path / "abi_layout")
     gcc(exe, [os.path.join(ABI, "abi_layout.c")])
     lines = subprocess.run([exe], capture_output=True, text=True, check=True).stdout.split("\n")
-    mirrors = {"P7_GPU_TABLES": p7gpu.Tables, "P7_GPU_HITS": p7gpu.Hits, "P7_GPU_STATS": p7gpu.Stats}
+    mirrors = {"P7_GPU_TABLES": p7gpu.Tables, "P7_GPU_HITS": p7gpu.Hits, "P7_GPU_STATS": p7gpu.Stats,
+               "P7_GPU_REPORT": p7gpu.Report}
     seen = {k: set() for k in mirrors}
     for line in filter(None, lines):
         sname, field, val = line.split()

@@ -1,0 +1,142 @@
+"""The hit list as hmmsearch writes it (p7_gpu_pipeline_TargetsText,
+DomainsText, TabularTargets, TabularDomains; HMMER/src/p7_tophits.c:
+1055-1380, 1519-1700) and P7_DOMAINDEF's counters behind its columns
+(p7_gpu_pipeline_GetTargetCounts; p7_domaindef.c:406-483).
+
+tutorial/fn3.out holds hmmsearch's per-sequence and per-domain tables for fn3
+against 7LESS_DROME (searched with Z = 1): the device's text must equal those
+lines byte for byte (the domain header line without " (and alignments)",
+which this library does not draw). The tabular forms carry the same numbers
+in other columns, and the domain-number columns, which no reference file
+shows, are checked against the oracle's whole domain definition on
+synthetic multidomain targets."""
+import gc
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+from biobench2_amd import seqio, synth
+from conftest import golden, synth_profile
+from test_gpu_multidomain import multidomain_db, per_length_tables
+
+pytestmark = pytest.mark.gpu
+
+
+def fn3_run(gpu):
+    (name, acc, desc, s), = seqio.read_uniprot_records(golden("7LESS_DROME"))
+    rng = np.random.default_rng(3)
+    seqs = [synth.iid_residues(int(L), rng) for L in (300, 2554, 50)]
+    seqs.insert(1, s)
+    names = ["rnd1", name, "rnd2", "rnd3"]
+    accs, descs = [None, acc, None, None], [None, desc, "", None]
+    db = gpu.SeqDB.from_list(seqs)
+    om = gpu.OProfile.from_file(golden("fn3.hmm"))
+    pli = gpu.Pipeline(domains=True)
+    pli.run(om, db)
+    del db                      # the run's results do not read the database after the call
+    gc.collect()
+    return pli, names, accs, descs
+
+
+def tutorial_lines():
+    with open(golden("fn3.out")) as fh:
+        return fh.read().split("\n")
+
+
+def test_fn3_target_and_domain_tables_are_the_tutorials(gpu):
+    pli, names, accs, descs = fn3_run(gpu)
+    ref = tutorial_lines()
+    kw = dict(qacc="PF00041.13", Z=1.0)
+    t = pli.report_text("targets", "fn3", 86, names, accs, descs, **kw).split("\n")
+    a = ref.index("Scores for complete sequences (score includes all domains):")
+    assert t[:5] == ref[a:a + 5], "\n".join(t[:5])
+    assert t[5:] == [""]
+    d = pli.report_text("domains", "fn3", 86, names, accs, descs, **kw).split("\n")
+    b = ref.index("Domain annotation for each sequence (and alignments):")
+    assert d[0] == "Domain annotation for each sequence:"
+    assert d[1:13] == ref[b + 1:b + 13], "\n".join(d[1:13])
+    assert d[13:] == ["", ""]
+    # --acc shows the accession in place of the name, and the header width follows it
+    ta = pli.report_text("targets", "fn3", 86, names, accs, descs, show_accessions=True, **kw).split("\n")
+    assert ta[4].split()[8] == "P13368" and ta[4].index("P13368") == t[4].index("7LESS_DROME")
+    # --notextw: the full description
+    tw = pli.report_text("targets", "fn3", 86, names, accs, descs, textw=0, **kw).split("\n")
+    assert tw[4].endswith("EC=2.7.10.1;") and tw[4].startswith(t[4].rstrip())
+
+
+def test_fn3_tabular_outputs_carry_the_tables_numbers(gpu):
+    pli, names, accs, descs = fn3_run(gpu)
+    kw = dict(qacc="PF00041.13", Z=1.0)
+    tbl = pli.report_text("tblout", "fn3", 86, names, accs, descs, **kw).split("\n")
+    assert tbl[0].startswith("#") and tbl[1].startswith("# target name") and tbl[2].startswith("#---")
+    row = tbl[3].split()
+    assert tbl[4:] == [""]
+    assert row[:4] == ["7LESS_DROME", "P13368", "fn3", "PF00041.13"]
+    # full sequence: E-value, score, bias; best domain: E-value, score, bias; exp reg clu ov env dom rep inc
+    assert row[4:18] == ["1.9e-57", "178.0", "0.4", "1.2e-16", "47.2", "0.9", "9.4",
+                         "9", "1", "0", "9", "9", "9", "7"], row
+    assert " ".join(row[18:]) == "RecName: Full=Protein sevenless; EC=2.7.10.1;"
+    # the header's columns line up with the row's fields
+    assert tbl[1].index("query name") == tbl[3].index("fn3")
+    dom = pli.report_text("domtblout", "fn3", 86, names, accs, descs, **kw).split("\n")
+    assert dom[1].startswith("# target name") and dom[12:] == [""]
+    ref = tutorial_lines()
+    b = ref.index("Domain annotation for each sequence (and alignments):")
+    for k, line in enumerate(dom[3:12]):
+        f = line.split()
+        r = ref[b + 4 + k].split()        # #, !/?, score, bias, c-Evalue, i-Evalue, hmm, .., ali, .., env, .., acc
+        assert f[:6] == ["7LESS_DROME", "P13368", "2554", "fn3", "PF00041.13", "86"]
+        assert f[6:9] == ["1.9e-57", "178.0", "0.4"]
+        assert f[9:11] == [r[0], "9"]
+        assert f[11:15] == [r[4], r[5], r[2], r[3]]
+        assert f[15:21] == [r[6], r[7], r[9], r[10], r[12], r[13]]
+        assert f[21] == r[15]
+    # with every target above the thresholds, none: the no-hits lines
+    none = pli.report_text("targets", "fn3", 86, names, accs, descs, E=1e-80, Z=1.0)
+    assert none.endswith("\n   [No hits detected that satisfy reporting thresholds]\n")
+    assert pli.report_text("tblout", "fn3", 86, names, accs, descs, E=1e-80, Z=1.0, show_header=False) == ""
+
+
+@pytest.mark.parametrize("name,n,seed", [("synth_M200", 30, 12)])
+def test_tabular_counts_equal_the_oracles_domain_definition(gpu, name, n, seed):
+    path = synth_profile(name)
+    seqs = multidomain_db(synth.read_hmm_probs(path), n, seed)
+    db = gpu.SeqDB.from_list(seqs)
+    om = gpu.OProfile.from_file(path)
+    pli = gpu.Pipeline(domains=True)
+    pli.run(om, db)
+    ts, tc = pli.target_scores(), pli.target_counts()
+    om2 = gpu.OProfile.from_file(path)
+    names = ["t%03d" % q for q in range(len(seqs))]
+    checked = 0
+    for row, q in enumerate(ts["seq"]):
+        s = seqs[int(q)]
+        ref = oracle.domain_definition(s, per_length_tables(om2, s.size))
+        if [(i, j, m) for i, j, m in ref["regions"]] != [(r[1], r[2], r[3] == 1) for r in pli.regions()
+                                                         if r[0] == q and r[3] != 2]:
+            continue                    # an occupancy test within float rounding of a threshold
+        assert tc["nregions"][row] == len(ref["regions"])
+        assert tc["nclustered"][row] == ref["nclustered"]
+        assert tc["noverlaps"][row] == ref["noverlaps"]
+        assert tc["nenvelopes"][row] == ref["nenvelopes"]
+        assert abs(float(tc["nexpected"][row]) - ref["nexpected"]) <= 1e-3 * max(1.0, ref["nexpected"])
+        checked += 1
+    assert checked >= 20, checked
+    Z = float(len(seqs))
+    order, flags, dflags, domZ = pli.hit_list(Z)
+    tbl = [ln.split() for ln in pli.report_text("tblout", name, 200, names, Z=Z).split("\n")[3:] if ln]
+    assert len(tbl) == int(np.sum(flags & 1))
+    rank = [int(ts["seq"][k]) for k, f in zip(order, flags) if f & 1]
+    assert [r[0] for r in tbl] == [names[q] for q in rank]
+    regs = pli.regions()
+    for r, k in zip(tbl, [k for k, f in zip(order, flags) if f & 1]):
+        q = int(ts["seq"][k])
+        mine = [x for x, g in enumerate(regs) if g[0] == q]
+        assert r[4] == "%.2g" % (math.exp(ts["lnP"][k]) * Z) and r[5] == "%.1f" % ts["score"][k]
+        assert [int(v) for v in r[11:18]] == [tc["nregions"][k], tc["nclustered"][k], tc["noverlaps"][k],
+                                               tc["nenvelopes"][k], ts["ndom"][k],
+                                               sum(dflags[x] & 1 for x in mine), sum(dflags[x] >> 1 & 1 for x in mine)]
+    dom = [ln.split() for ln in pli.report_text("domtblout", name, 200, names, Z=Z).split("\n")[3:] if ln]
+    assert len(dom) == sum(int(f & 1) for f in dflags)

@@ -170,3 +170,13 @@ def test_fn3_tutorial_domain_marks():
     assert flags == [3] and domZ == 1.0
     marks = "".join("!" if f & 2 else "?" if f & 1 else " " for f in dflags[0])
     assert marks == "?!!!!?!!!", marks
+
+
+def test_uniprot_header_fields_are_the_tutorials():
+    """The target's name and description as Easel's EMBL parser keeps them:
+    fn3.out prints them in its domain section's '>>' line."""
+    (name, acc, desc, s), = seqio.read_uniprot_records(golden("7LESS_DROME"))
+    with open(golden("fn3.out")) as fh:
+        line = next(ln for ln in fh if ln.startswith(">> "))
+    assert line.rstrip("\n") == ">> %s  %s" % (name, desc)
+    assert acc == "P13368" and s.size == 2554
