@@ -2295,7 +2295,6 @@ p7_gpu_pipeline_TabularDomains(const P7_GPU_PIPELINE *pli, const P7_GPU_REPORT *
 #ifndef P7G_BATCH_CHAINS
 #define P7G_BATCH_CHAINS 16
 #endif
-#define P7G_GRAPH_CHAINS 4          // concurrent per-profile stage chains of a captured batch
 
 struct Batch {
   Work     ch[P7G_BATCH_CHAINS];   // each chain's stream and work area
@@ -2312,9 +2311,11 @@ struct Batch {
   std::vector<hipEvent_t> evs;           // per shape (its launch done), per chain (join), then the fork
   hipEvent_t t[2];
   bool     t_ready = false;
-  hipGraph_t     graph = nullptr;
-  hipGraphExec_t exec = nullptr;
-  std::vector<unsigned long long> key;   // what the instantiated graph was captured for
+  // captured form: one graph per profile row (its stage chain), launched on
+  // the row's chain stream after its shape's event
+  std::vector<hipGraph_t>     graphs;
+  std::vector<hipGraphExec_t> execs;
+  std::vector<unsigned long long> key;   // what the instantiated graphs were captured for
   int      last_graph = 0;
 };
 
@@ -2329,9 +2330,9 @@ struct BatchItem {                 // one profile of a batch, in shape order
 static void
 batch_drop_graph(Batch &b)
 {
-  if (b.exec) (void) hipGraphExecDestroy(b.exec);
-  if (b.graph) (void) hipGraphDestroy(b.graph);
-  b.exec = nullptr; b.graph = nullptr; b.key.clear();
+  for (hipGraphExec_t e : b.execs) if (e) (void) hipGraphExecDestroy(e);
+  for (hipGraph_t g : b.graphs) if (g) (void) hipGraphDestroy(g);
+  b.execs.clear(); b.graphs.clear(); b.key.clear();
 }
 
 static void
@@ -2389,29 +2390,70 @@ batch_reserve(Batch &b, int nprof, int64_t n, unsigned mask)
   return p7G_OK;
 }
 
-// Enqueues one batch on st0 and the chain streams: the ungapped pass of every
-// profile, one persistent launch per kernel shape, each followed by its
-// event; then every profile's later stages on chain (row mod nchain), which
-// waits only for its own shape's launch; then st0 joins every chain. The
-// shape launches run on st0, or with `spread` on stream (shape mod nchain)
-// after a fork event, so that the long targets of different launches overlap
-// (used for direct launches; a captured batch keeps them on st0).
+// One profile row's work after its shape's ungapped pass, on its chain's
+// stream: the stage chain, its counters and the outputs asked for.
+static int
+batch_row(Batch &b, Work &w, const BatchItem &x, const DevSeqs &s, int64_t n, unsigned mask, int r)
+{
+  const int j = x.prof;
+  int status;
+  if ((status = chain_reset(w, n)) != p7G_OK) return status;
+  if ((status = chain_stages(w, *x.p, s, n, b.zmax + (size_t) r * n, x.g, false)) != p7G_OK) return status;
+  HIPCHK(launch_fill32(b.sums + 8 * (size_t) j, 0u, 16, w.st));
+  HIPCHK(launch_stats(s, w.r.stage, n, w.l.vit, w.l.fwd, w.l.counts, b.sums + 8 * (size_t) j, w.st));
+  HIPCHK(launch_copy32(b.counts + 16 * (size_t) j, w.l.counts, 16, w.st));
+  if (mask & 1u) HIPCHK(launch_scatter8(s, w.r.stage, b.o_stage + (size_t) j * n, n, w.st));
+  const float *src[6] = { w.r.nullsc, w.r.usc, w.r.filtersc, w.r.vfsc, w.r.fwdsc, w.r.bcksc };
+  for (int a = 0; a < 6; a++)
+    if (mask & (2u << a)) HIPCHK(launch_scatter32(s, src[a], b.o_f[a] + (size_t) j * n, n, w.st));
+  return p7G_OK;
+}
+
+// Captures every row's batch_row as its own graph, on the row's chain stream
+// (row r on chain r mod nchain, as batch_enqueue runs it).
+static int
+batch_capture(Batch &b, int nchain, const std::vector<BatchItem> &it, const DevSeqs &s, int64_t n, unsigned mask)
+{
+  batch_drop_graph(b);
+  const int nit = (int) it.size();
+  b.graphs.assign((size_t) nit, nullptr);
+  b.execs.assign((size_t) nit, nullptr);
+  for (int r = 0; r < nit; r++) {
+    Work &w = b.ch[r % nchain];
+    HIPCHK(hipStreamBeginCapture(w.st, hipStreamCaptureModeThreadLocal));
+    const int status = batch_row(b, w, it[(size_t) r], s, n, mask, r);
+    hipGraph_t gr = nullptr;
+    const hipError_t ec = hipStreamEndCapture(w.st, &gr);
+    if (status != p7G_OK) { if (gr) (void) hipGraphDestroy(gr); batch_drop_graph(b); return status; }
+    if (ec != hipSuccess) { batch_drop_graph(b); HIPCHK(ec); }
+    b.graphs[(size_t) r] = gr;
+    const hipError_t ei = hipGraphInstantiateWithFlags(&b.execs[(size_t) r], gr, 0);
+    if (ei != hipSuccess) { batch_drop_graph(b); HIPCHK(ei); }
+  }
+  return p7G_OK;
+}
+
+// Enqueues one batch on st0 and the chain streams: after a fork event, the
+// ungapped pass of every profile, one persistent launch per kernel shape on
+// stream (shape mod nchain), so that the long targets of different launches
+// overlap, each followed by its event; then every profile row's later
+// stages on chain (row mod nchain), which waits only for its own shape's
+// launch, launched directly or as the row's captured graph; then st0 joins
+// every chain. Chunked models share one pair of boundary columns, so their
+// launches stay on st0.
 static int
 batch_enqueue(Batch &b, hipStream_t st0, int nchain, const std::vector<BatchItem> &it, int nshape,
-              const DevSeqs &s, int64_t n, unsigned mask, uint16_t *bnd, int64_t bnd_stride, bool spread)
+              const DevSeqs &s, int64_t n, unsigned mask, uint16_t *bnd, int64_t bnd_stride, bool graphs)
 {
   const int nit = (int) it.size();
   HIPCHK(launch_fill32(b.queues, 0u, (1 + P7G_SSV_MAXCHUNK) * (int64_t) nit, st0));
-  if (spread) {
-    hipEvent_t fork = b.evs[nshape + nchain];
-    HIPCHK(hipEventRecord(fork, st0));
-    for (int k = 0; k < nchain; k++) HIPCHK(hipStreamWaitEvent(b.ch[k].st, fork, 0));
-  }
+  hipEvent_t fork = b.evs[nshape + nchain];
+  HIPCHK(hipEventRecord(fork, st0));
+  for (int k = 0; k < nchain; k++) HIPCHK(hipStreamWaitEvent(b.ch[k].st, fork, 0));
   for (int r0 = 0; r0 < nit; ) {
     int r1 = r0;
     while (r1 < nit && it[r1].shape == it[r0].shape) r1++;
-    // chunked models share one pair of boundary columns: their launches stay on st0
-    hipStream_t sst = (spread && it[r0].v >= 0) ? b.ch[it[r0].shape % nchain].st : st0;
+    hipStream_t sst = (it[r0].v >= 0) ? b.ch[it[r0].shape % nchain].st : st0;
     if (it[r0].v < 0) {                  // a chunked model: its own launches (one row per shape)
       HIPCHK(launch_ssv_chunked(*it[r0].p, s, nullptr, n, b.zmax + (size_t) r0 * n,
                                 b.queues + nit + P7G_SSV_MAXCHUNK * r0, it[r0].grab, bnd, bnd_stride, sst));
@@ -2430,17 +2472,9 @@ batch_enqueue(Batch &b, hipStream_t st0, int nchain, const std::vector<BatchItem
   int status;
   for (int r = 0; r < nit; r++) {
     Work &w = b.ch[r % nchain];
-    const int j = it[r].prof;
     HIPCHK(hipStreamWaitEvent(w.st, b.evs[it[r].shape], 0));
-    if ((status = chain_reset(w, n)) != p7G_OK) return status;
-    if ((status = chain_stages(w, *it[r].p, s, n, b.zmax + (size_t) r * n, it[r].g, false)) != p7G_OK) return status;
-    HIPCHK(launch_fill32(b.sums + 8 * (size_t) j, 0u, 16, w.st));
-    HIPCHK(launch_stats(s, w.r.stage, n, w.l.vit, w.l.fwd, w.l.counts, b.sums + 8 * (size_t) j, w.st));
-    HIPCHK(launch_copy32(b.counts + 16 * (size_t) j, w.l.counts, 16, w.st));
-    if (mask & 1u) HIPCHK(launch_scatter8(s, w.r.stage, b.o_stage + (size_t) j * n, n, w.st));
-    const float *src[6] = { w.r.nullsc, w.r.usc, w.r.filtersc, w.r.vfsc, w.r.fwdsc, w.r.bcksc };
-    for (int a = 0; a < 6; a++)
-      if (mask & (2u << a)) HIPCHK(launch_scatter32(s, src[a], b.o_f[a] + (size_t) j * n, n, w.st));
+    if (graphs) HIPCHK(hipGraphLaunch(b.execs[(size_t) r], w.st));
+    else if ((status = batch_row(b, w, it[(size_t) r], s, n, mask, r)) != p7G_OK) return status;
   }
   for (int k = 0; k < nchain; k++) {
     HIPCHK(hipEventRecord(b.evs[nshape + k], b.ch[k].st));
@@ -2499,7 +2533,7 @@ p7_gpu_PipelineBatch(P7_GPU_PIPELINE *pli, const P7_GPU_OPROFILE *const *oms, in
   bool any_chunked = false;
   for (const BatchItem &x : it) any_chunked |= (x.v < 0);
   const bool graph = (flags & P7_GPU_BATCH_GRAPH) != 0;
-  const int nchain = std::min(nprof, graph ? P7G_GRAPH_CHAINS : P7G_BATCH_CHAINS);
+  const int nchain = std::min(nprof, P7G_BATCH_CHAINS);
 
   // work areas (a run that finds a scale-factor pool full grows the pools to
   // the rows of every target and runs again)
@@ -2540,25 +2574,15 @@ p7_gpu_PipelineBatch(P7_GPU_PIPELINE *pli, const P7_GPU_OPROFILE *const *oms, in
                             bits(x.g.T1m), bits(x.g.T2m), bits(x.g.T2v), bits(x.g.T3f),
                             (unsigned long long) (x.g.do_bias * 2 + x.g.do_max) });
   }
-  const bool eager = ! graph;
-  if (graph && (! b.exec || key != b.key)) {
-    batch_drop_graph(b);
-    HIPCHK(hipStreamBeginCapture(st0, hipStreamCaptureModeThreadLocal));
-    status = batch_enqueue(b, st0, nchain, it, nshape, s, n, mask, bnd, pli->w.bnd_cap, false);
-    hipGraph_t gr = nullptr;
-    const hipError_t ec = hipStreamEndCapture(st0, &gr);
-    if (status != p7G_OK) { if (gr) (void) hipGraphDestroy(gr); return status; }
-    HIPCHK(ec);
-    b.graph = gr;
-    HIPCHK(hipGraphInstantiateWithFlags(&b.exec, b.graph, 0));
+  if (graph && (b.execs.size() != (size_t) nprof || key != b.key)) {
+    if ((status = batch_capture(b, nchain, it, s, n, mask)) != p7G_OK) return status;
     b.key = key;
   }
 
   HIPCHK(hipEventRecord(b.t[0], st0));
-  if (eager) { if ((status = batch_enqueue(b, st0, nchain, it, nshape, s, n, mask, bnd, pli->w.bnd_cap, true)) != p7G_OK) return status; }
-  else HIPCHK(hipGraphLaunch(b.exec, st0));
+  if ((status = batch_enqueue(b, st0, nchain, it, nshape, s, n, mask, bnd, pli->w.bnd_cap, graph)) != p7G_OK) return status;
   HIPCHK(hipEventRecord(b.t[1], st0));
-  b.last_graph = eager ? 0 : 1;
+  b.last_graph = graph ? 1 : 0;
 
   sums.assign((size_t) nprof * 8, 0);
   counts.assign((size_t) nprof * 16, 0);

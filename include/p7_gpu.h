@@ -451,13 +451,12 @@ size_t p7_gpu_StatisticsText(const P7_GPU_STATS *st, double F1, double F2, doubl
  * sequence in one call. The ungapped MSV pass of all profiles that share a
  * kernel shape runs as one persistent launch, with a device work queue per
  * profile; each profile's later stages (MSV decision, bias filter, Viterbi,
- * Forward, Backward) follow on one of up to eight concurrent streams as soon
+ * Forward, Backward) follow on one of up to sixteen concurrent streams as soon
  * as its shape's launch is done, and the shape launches themselves spread over
- * the streams. With flags P7_GPU_BATCH_GRAPH the sequence (four streams, the
- * shape launches on the first) is captured once as a HIP graph and replayed
- * while the same profiles, database, thresholds and output arrays are asked
- * for; on ROCm 7.2 the direct launches overlap the streams' work better, so
- * they are the default.
+ * the streams. With flags P7_GPU_BATCH_GRAPH each profile's stage chain is
+ * captured once as a HIP graph of its own and replayed, on its stream after
+ * its shape's launch, while the same profiles, database, thresholds and
+ * output arrays are asked for.
  * hits and stats are arrays of nprof entries; either may be NULL, and so may
  * any pointer in hits. Each profile's results equal a p7_gpu_Pipeline call
  * with the same pipeline. Counters add to the pipeline's totals; ms_total of
