@@ -450,10 +450,10 @@ size_t p7_gpu_StatisticsText(const P7_GPU_STATS *st, double F1, double F2, doubl
  * Runs the p7_gpu_Pipeline filter chain of nprof profiles over every database
  * sequence in one call. The ungapped MSV pass of all profiles that share a
  * kernel shape runs as one persistent launch, with a device work queue per
- * profile; each profile's later stages (MSV decision, bias filter, Viterbi,
- * Forward, Backward) follow on one of up to sixteen concurrent streams as soon
- * as its shape's launch is done, and the shape launches themselves spread over
- * the streams. With flags P7_GPU_BATCH_GRAPH each profile's stage chain is
+ * profile, the shapes one after another, largest models first; each
+ * profile's later stages (MSV decision, bias filter, Viterbi, Forward,
+ * Backward) follow on one of up to sixteen concurrent streams as soon as its
+ * shape's launch is done. With flags P7_GPU_BATCH_GRAPH each profile's stage chain is
  * captured once as a HIP graph of its own and replayed, on its stream after
  * its shape's launch, while the same profiles, database, thresholds and
  * output arrays are asked for.
