@@ -204,7 +204,13 @@ __host__ __device__ inline int64_t md_q(int M) { return (M - 1) / 4 + 1 < 2 ? 2 
 __host__ __device__ inline int64_t md_stage_floats(int M) { return (10 * ((int64_t) M + 2) + 3) & ~(int64_t) 3; }
 __host__ __device__ inline bool md_stage_t(int M) { return (md_stage_floats(M) + 32 * md_q(M)) * 4 <= P7G_LDS_MAX; }
 __host__ __device__ inline int64_t md_acc_at(int64_t Q) { return 4 * Q + 32 + MD_TSEG * 4; }
-__host__ __device__ inline int64_t md_trace_floats(int M) { return md_acc_at(md_q(M)) + MD_ACC_LDS; }
+// then, when it fits, the emission odds of 20 residues in striped order
+// ([x][q][r]) for the sampled domains' null2
+__host__ __device__ inline int64_t md_em_floats(int M) { return 20 * 4 * md_q(M); }
+__host__ __device__ inline bool md_em_lds(int M)
+{ return (md_acc_at(md_q(M)) + MD_ACC_LDS + md_em_floats(M)) * 4 <= P7G_LDS_MAX; }
+__host__ __device__ inline int64_t md_trace_floats(int M)
+{ return md_acc_at(md_q(M)) + MD_ACC_LDS + (md_em_lds(M) ? md_em_floats(M) : 0); }
 struct MdJob {
   int64_t res_off;
   int64_t area;

@@ -94,6 +94,16 @@ __device__ __forceinline__ void md_thresholds(const float *v, int n, double *th)
   for (int a = 0; a < n - 1; a++) th[a] = c[a] / tot;
 }
 
+// The workgroup is one wavefront, whose LDS operations complete in the order
+// it issues them: between lanes that only exchange data through LDS it is
+// enough that the compiler moves no memory operation across this point (a
+// workgroup barrier would also wait for the outstanding global stores).
+__device__ __forceinline__ void md_sync()
+{
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+}
+
 struct MdSeg { int idx, i, j, k, m; };
 
 // link_spsamples: overlap >= 0.8 of the smaller segment in the sequence and in
@@ -158,16 +168,33 @@ k_md_region(const DevProfile p, const uint8_t *__restrict__ res, const MdJob *__
     float xN = 1.0f, xJ = 0.0f, xB = fmove, xC = 0.0f, xE = 0.0f;
     if (z == 0) { xmx[0] = 0.0f; xmx[1] = xN; xmx[2] = xJ; xmx[3] = xB; xmx[4] = xC; xmx[5] = 1.0f; }
     __syncthreads();
+    // a row's odds in striped order, eight per lane: loaded one row ahead
+    // while the row before is computed (models up to M = 512; larger ones load
+    // theirs at the row)
+    const bool pre = 4 * Q <= 64 * 8;
+    float enext[8];
+    auto load_row = [&](int i1) {
+      const int x = dsq[i1 - 1];
+#pragma unroll
+      for (int u = 0; u < 8; u++) { const int c = u * 64 + z; enext[u] = md_tv(p, x, (c & 3) * Q + (c >> 2) + 1); }
+    };
+    if (pre) load_row(1);
     for (int i = 1; i <= Lr; i++) {
-      const int x = dsq[i - 1];
-      for (int c0 = 0; c0 < 4 * Q; c0 += 64 * 8) {           // this residue's odds, striped order
-        float e[8];
+      if (pre) {
 #pragma unroll
-        for (int u = 0; u < 8; u++) { const int c = c0 + u * 64 + z; e[u] = md_tv(p, x, (c & 3) * Q + (c >> 2) + 1); }
+        for (int u = 0; u < 8; u++) { const int c = u * 64 + z; if (c < 4 * Q) erow[c] = enext[u]; }
+        if (i < Lr) load_row(i + 1);
+      } else {
+        const int x = dsq[i - 1];
+        for (int c0 = 0; c0 < 4 * Q; c0 += 64 * 8) {         // this residue's odds, striped order
+          float e[8];
 #pragma unroll
-        for (int u = 0; u < 8; u++) { const int c = c0 + u * 64 + z; if (c < 4 * Q) erow[c] = e[u]; }
+          for (int u = 0; u < 8; u++) { const int c = c0 + u * 64 + z; e[u] = md_tv(p, x, (c & 3) * Q + (c >> 2) + 1); }
+#pragma unroll
+          for (int u = 0; u < 8; u++) { const int c = c0 + u * 64 + z; if (c < 4 * Q) erow[c] = e[u]; }
+        }
       }
-      __syncthreads();
+      md_sync();
       for (int c = z; c < 4 * Q; c += 64) {
         const int q = c >> 2, r = c & 3, k = r * Q + q + 1, kk = min(k, M + 1);
         float mp = 0.0f, dp = 0.0f, ip = 0.0f;           // row i-1 at k-1 (esl_sse_rightshift_ps at q = 0)
@@ -182,13 +209,13 @@ k_md_region(const DevProfile p, const uint8_t *__restrict__ res, const MdJob *__
         cur[(q*3+2)*4 + r] = __fadd_rn(__fmul_rn(prev[(q*3+0)*4 + r], tv(T_MI, kk)),
                                        __fmul_rn(prev[(q*3+2)*4 + r], tv(T_II, kk)));
       }
-      __syncthreads();
+      md_sync();
       // D before the D->D passes: cell q holds M(q-1)·tMD (the q loop's dcv); q = 0 holds 0
       for (int c = z; c < 4 * Q; c += 64) {
         const int q = c >> 2, r = c & 3, k = r * Q + q + 1;
         cur[(q*3+1)*4 + r] = q ? __fmul_rn(cur[((q-1)*3+0)*4 + r], tv(T_MD, min(k - 1, M + 1))) : 0.0f;
       }
-      __syncthreads();
+      md_sync();
       float xe = 0.0f;
       if (z < 4) {                                         // lane z is SSE lane r
         // the sums along q, eight cells' loads issued before their dependent chain
@@ -232,14 +259,14 @@ k_md_region(const DevProfile p, const uint8_t *__restrict__ res, const MdJob *__
       xJ = __fadd_rn(__fmul_rn(xJ, floop), __fmul_rn(xE, xEloop));
       xB = __fadd_rn(__fmul_rn(xJ, fmove), __fmul_rn(xN, fmove));
       float scale = 1.0f;
-      __syncthreads();
+      md_sync();
       if (xE > 1.0e4f) {
         xN = __fdiv_rn(xN, xE); xC = __fdiv_rn(xC, xE); xJ = __fdiv_rn(xJ, xE); xB = __fdiv_rn(xB, xE);
         const float inv = (float) (1.0 / (double) xE);
         for (int c = z; c < 12 * Q; c += 64) cur[c] = __fmul_rn(cur[c], inv);
         scale = xE;
         xE = 1.0f;
-        __syncthreads();
+        md_sync();
       }
       float *row = mat + (size_t) i * 12 * Q;
       for (int c = z; c < 12 * Q; c += 64) row[c] = cur[c];
@@ -248,8 +275,9 @@ k_md_region(const DevProfile p, const uint8_t *__restrict__ res, const MdJob *__
         xr[0] = xE; xr[1] = xN; xr[2] = xJ; xr[3] = xB; xr[4] = xC; xr[5] = scale;
       }
       float *t = prev; prev = cur; cur = t;
-      __syncthreads();
+      md_sync();
     }
+    __syncthreads();                                       // the matrix in global memory, for every lane
   }
 
   // ---------------- 2. the sampled traces ----------------
@@ -266,21 +294,43 @@ k_md_region(const DevProfile p, const uint8_t *__restrict__ res, const MdJob *__
   auto cellk = [&](int i, int k, int s) {                  // the cell of model position k (1..4Q) in row i
     const int r = (k - 1 >= Q) + (k - 1 >= 2 * Q) + (k - 1 >= 3 * Q), q = k - 1 - r * Q;
     return CELL(i, q, s, r); };
-  for (int64_t c = z; c < (int64_t) Lr * M; c += 64) {
-    const int i = 1 + (int) (c / M), k = 1 + (int) (c - (int64_t) (i - 1) * M);
-    double *o = thr + ((size_t) i * (M + 1) + k) * 5;
-    float v[4];
-    v[0] = __fmul_rn(xmx[(size_t) (i - 1) * 6 + 3], tv(T_BM, k));
-    v[1] = __fmul_rn(k > 1 ? cellk(i - 1, k - 1, 0) : 0.0f, tv(T_MM, k));
-    v[2] = __fmul_rn(k > 1 ? cellk(i - 1, k - 1, 2) : 0.0f, tv(T_IM, k));
-    v[3] = __fmul_rn(k > 1 ? cellk(i - 1, k - 1, 1) : 0.0f, tv(T_DM, k));
-    md_thresholds(v, 4, o);
-    v[0] = (k > 1) ? __fmul_rn(cellk(i, k - 1, 0), tv(T_MDS, k)) : 0.0f;
-    v[1] = (k > 1) ? __fmul_rn(cellk(i, k - 1, 1), tv(T_DDS, k)) : 0.0f;
-    md_thresholds(v, 2, o + 3);
-    v[0] = __fmul_rn(cellk(i - 1, k, 0), tv(T_MI, k));
-    v[1] = __fmul_rn(cellk(i - 1, k, 2), tv(T_II, k));
-    md_thresholds(v, 2, o + 4);
+  // four cells per lane at a time, their matrix values loaded before the
+  // threshold arithmetic of any of them
+  const int64_t LM = (int64_t) Lr * M;
+  for (int64_t c0 = 0; c0 < LM; c0 += 4 * 64) {
+    float in[4][9];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const int64_t c = min(c0 + u * 64 + z, LM - 1);
+      const int i = 1 + (int) (c / M), k = 1 + (int) (c - (int64_t) (i - 1) * M);
+      in[u][0] = xmx[(size_t) (i - 1) * 6 + 3];
+      in[u][1] = k > 1 ? cellk(i - 1, k - 1, 0) : 0.0f;
+      in[u][2] = k > 1 ? cellk(i - 1, k - 1, 2) : 0.0f;
+      in[u][3] = k > 1 ? cellk(i - 1, k - 1, 1) : 0.0f;
+      in[u][4] = k > 1 ? cellk(i, k - 1, 0) : 0.0f;
+      in[u][5] = k > 1 ? cellk(i, k - 1, 1) : 0.0f;
+      in[u][6] = cellk(i - 1, k, 0);
+      in[u][7] = cellk(i - 1, k, 2);
+    }
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const int64_t c = c0 + u * 64 + z;
+      if (c >= LM) break;
+      const int i = 1 + (int) (c / M), k = 1 + (int) (c - (int64_t) (i - 1) * M);
+      double *o = thr + ((size_t) i * (M + 1) + k) * 5;
+      float v[4];
+      v[0] = __fmul_rn(in[u][0], tv(T_BM, k));
+      v[1] = __fmul_rn(in[u][1], tv(T_MM, k));
+      v[2] = __fmul_rn(in[u][2], tv(T_IM, k));
+      v[3] = __fmul_rn(in[u][3], tv(T_DM, k));
+      md_thresholds(v, 4, o);
+      v[0] = (k > 1) ? __fmul_rn(in[u][4], tv(T_MDS, k)) : 0.0f;
+      v[1] = (k > 1) ? __fmul_rn(in[u][5], tv(T_DDS, k)) : 0.0f;
+      md_thresholds(v, 2, o + 3);
+      v[0] = __fmul_rn(in[u][6], tv(T_MI, k));
+      v[1] = __fmul_rn(in[u][7], tv(T_II, k));
+      md_thresholds(v, 2, o + 4);
+    }
   }
   for (int i = z; i <= Lr; i += 64) {
     float v[2];
@@ -294,13 +344,17 @@ k_md_region(const DevProfile p, const uint8_t *__restrict__ res, const MdJob *__
       // select_e: norm = 1/E(i) in double, the products in float, summed in double
       const float nf = (float) (1.0 / (double) xmx[(size_t) i * 6 + 0]);
       double sum = 0.0, *er = ecum + (size_t) i * 8 * Q;
-      for (int q = 0; q < Q; q++) {
-        const float *cq = mat + ((size_t) i * Q + q) * 12;
-        float u[8];
+      float u[8], un[8];
 #pragma unroll
-        for (int e = 0; e < 8; e++) u[e] = cq[e];
+      for (int e = 0; e < 8; e++) u[e] = mat[(size_t) i * Q * 12 + e];
+      for (int q = 0; q < Q; q++) {
+        const float *cn = mat + ((size_t) i * Q + min(q + 1, Q - 1)) * 12;    // the next q's, in flight
+#pragma unroll
+        for (int e = 0; e < 8; e++) un[e] = cn[e];
 #pragma unroll
         for (int e = 0; e < 8; e++) { sum = sum + (double) __fmul_rn(u[e], nf); er[q * 8 + e] = sum; }
+#pragma unroll
+        for (int e = 0; e < 8; e++) u[e] = un[e];
       }
     }
     v[0] = __fmul_rn(xmx[(size_t) i * 6 + 1], fmove);
@@ -319,6 +373,13 @@ k_md_region(const DevProfile p, const uint8_t *__restrict__ res, const MdJob *__
   auto cnt_at = [&](int k) {                               // the count cell of model position k
     const int r = (k - 1 >= Q) + (k - 1 >= 2 * Q) + (k - 1 >= 3 * Q);
     return (k - 1 - r * Q) * 4 + r; };
+  const bool em_lds = md_em_lds(M) && p.K <= 20;
+  float *em = lds + md_acc_at(Q) + MD_ACC_LDS;             // [x][q][r] odds, when em_lds
+  if (em_lds)
+    for (int c = z; c < p.K * 4 * Q; c += 64) {
+      const int x = c / (4 * Q), rem = c - x * 4 * Q;
+      em[c] = md_tv(p, x, min((rem & 3) * Q + (rem >> 2) + 1, M + 1));
+    }
   for (int c = z; c < 4 * Q; c += 64) cnt[c] = 0.0f;
   for (int pos = z; pos <= Lr; pos += 64) accp[pos] = 0.0f;
   __syncthreads();
@@ -456,7 +517,7 @@ k_md_region(const DevProfile p, const uint8_t *__restrict__ res, const MdJob *__
       } else if (s1 == MS_B) {
         // p7_Null2_ByTrace over the domain: four striped sums over q per
         // canonical residue, their hsum; degenerate codes the mean
-        __syncthreads();
+        md_sync();
         const float nrm = (float) (1.0 / (double) (float) Ld);
         for (int base = 0; base < p.K * 4; base += 64) {
           const int xr = base + z, x = xr >> 2, r = xr & 3;
@@ -467,8 +528,13 @@ k_md_region(const DevProfile p, const uint8_t *__restrict__ res, const MdJob *__
             const int q0 = max(0, kmin - 1 - r * Q), q1 = min(Q - 1, kmax - 1 - r * Q);
             for (int qa = q0; qa <= q1; qa += 8) {
               float e[8];
+              if (em_lds) {
 #pragma unroll
-              for (int u = 0; u < 8; u++) e[u] = md_tv(p, x, min(r * Q + qa + u + 1, M + 1));
+                for (int u = 0; u < 8; u++) e[u] = em[(x * Q + min(qa + u, Q - 1)) * 4 + r];
+              } else {
+#pragma unroll
+                for (int u = 0; u < 8; u++) e[u] = md_tv(p, x, min(r * Q + qa + u + 1, M + 1));
+              }
 #pragma unroll
               for (int u = 0; u < 8; u++)
                 if (qa + u <= q1) {
@@ -481,7 +547,7 @@ k_md_region(const DevProfile p, const uint8_t *__restrict__ res, const MdJob *__
           const float s01 = __fadd_rn(sv, __shfl_down(sv, 1, 4)), s23 = __shfl_down(s01, 2, 4);
           if (x < p.K && r == 0) n2v[x] = __fadd_rn(__fadd_rn(s01, s23), 0.0f);
         }
-        __syncthreads();
+        md_sync();
         {                                                  // degenerate codes: one lane each
           const int xx = p.K + 1 + z;
           if (xx <= p.Kp - 3) {
@@ -492,7 +558,7 @@ k_md_region(const DevProfile p, const uint8_t *__restrict__ res, const MdJob *__
           }
           if (z == 0) { n2v[p.K] = 1.0f; n2v[p.Kp - 2] = 1.0f; n2v[p.Kp - 1] = 1.0f; }
         }
-        __syncthreads();
+        md_sync();
         // residues (sqto, hi] are outside domains (1), (sqfrom, sqto] take the odds
         for (int pos = z + 1 + ((sqfrom >> 6) << 6); pos <= hi; pos += 64) {
           // the lane of a residue is fixed (pos mod 64), so its updates stay in order
@@ -505,7 +571,7 @@ k_md_region(const DevProfile p, const uint8_t *__restrict__ res, const MdJob *__
         nd++;
         for (int c = z; c < 4 * Q; c += 64) cnt[c] = 0.0f;
         kmin = 0x7fffffff; kmax = 0;
-        __syncthreads();
+        md_sync();
       }
       // (N, C and J emit on the transition to themselves: the repeats above)
       s0 = s1;
@@ -514,13 +580,13 @@ k_md_region(const DevProfile p, const uint8_t *__restrict__ res, const MdJob *__
     for (int pos = z + 1; pos <= hi; pos += 64) accp[pos] = __fadd_rn(accp[pos], 1.0f);
     // the trace's domains join the ensemble in sequence order
     if (nseg + nd > J.segcap) { status = 5; break; }
-    __syncthreads();
+    md_sync();
     for (int d = z; d < nd; d += 64) {
       const int e = nd - 1 - d;
       segs[nseg + d] = MdSeg{ t, tseg[e*4 + 0], tseg[e*4 + 1], tseg[e*4 + 2], tseg[e*4 + 3] };
     }
     nseg += nd;
-    __syncthreads();
+    md_sync();
   }
   if (status) { if (z == 0) { O->status = status; O->nc = 0; O->nseg = nseg; } return; }
   for (int pos = z + 1; pos <= Lr; pos += 64) acc[pos] = logf(__fdiv_rn(accp[pos], (float) nsamples));
