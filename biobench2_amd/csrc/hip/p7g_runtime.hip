@@ -134,6 +134,7 @@ static std::atomic<unsigned long long> g_serial(0);   // identifies device profi
 
 struct p7g_dev_profile_s {
   DevProfile p;
+  size_t     bytes;                  // device memory of the tables
   int        Lmax;
   unsigned long long serial;
   struct p7g_dev_profile_s *older;   // copies replaced by a longer length range, kept until the profile goes
@@ -369,6 +370,7 @@ dev_profile_build(P7_GPU_OPROFILE *om, int Lmax_need, struct p7g_dev_profile_s *
   size_t offs[20], total = 0;
   for (int i = 0; i < NB; i++) { offs[i] = total; total += (sz[i] + 255) & ~(size_t) 255; }
   char *blob = nullptr;
+  d->bytes = total;
   hipError_t e = hipMalloc((void **) &blob, total);
   if (e == hipSuccess) {
     for (int i = 0; i < NB && e == hipSuccess; i++)
@@ -771,6 +773,21 @@ p7_gpu_seqdb_Load(const char *path, P7_GPU_SEQDB **ret_db)
 }
 
 extern "C" int64_t p7_gpu_seqdb_NSeq(const P7_GPU_SEQDB *db) { return db ? db->nseq : 0; }
+
+static int check_query(const P7_GPU_OPROFILE *om, const P7_GPU_SEQDB *db);
+
+extern "C" int
+p7_gpu_oprofile_DeviceBytes(const P7_GPU_OPROFILE *om, const P7_GPU_SEQDB *db, size_t *ret_bytes)
+{
+  int status;
+  if ((status = ensure_init()) != p7G_OK) return status;
+  if ((status = check_query(om, db)) != p7G_OK) return status;
+  DevProfile *p;
+  if ((status = dev_profile(om, db->Lmax, &p)) != p7G_OK) return status;
+  const p7g_dev_profile_s *d = (const p7g_dev_profile_s *) ((const char *) p - offsetof(p7g_dev_profile_s, p));
+  *ret_bytes = d->bytes;
+  return p7G_OK;
+}
 extern "C" int64_t p7_gpu_seqdb_NRes(const P7_GPU_SEQDB *db) { return db ? db->nres : 0; }
 
 static DevSeqs seqs_of(const P7_GPU_SEQDB *db)

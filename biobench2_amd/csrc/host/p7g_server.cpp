@@ -3,12 +3,20 @@
 //
 // The reference's resident-database search (hmmpgmd: HMMER/src/hmmdmstr.c:701-790,
 // hmmdwrkr.c:184-644) pops one query at a time and gives it the whole machine.
-// Here a fixed set of slots, each a P7_GPU_PIPELINE with its own HIP stream and
-// device work area, takes queries from a FIFO as slots free up, so the stage
-// kernels of several queries overlap on the device. The number of slots is
-// bounded by a device-memory budget (admission by memory). Every query's
-// result is the same as a lone p7_gpu_Pipeline call: the slots share only
-// read-only data (the database and the device copies of the profiles).
+// Here a fixed set of slots, each a P7_GPU_PIPELINE with its own HIP streams
+// and device work areas, takes queries from a FIFO as slots free up. A slot
+// that finds several queries waiting takes up to `coalesce` of them at once
+// and runs them as one p7_gpu_PipelineBatch call: one persistent ungapped-pass
+// launch per kernel shape for all of them, then their later stages on
+// concurrent streams; with one query waiting it runs p7_gpu_Pipeline. Memory
+// admits the queries: the slots' work areas are set aside when the server is
+// made (the budget bounds their number), and each query in flight holds its
+// profile's device tables, which must fit in what the budget has left (a
+// query that does not fit waits for others to finish; one alone is always
+// admitted). Every query's result is the same as a lone p7_gpu_Pipeline call:
+// queries share only read-only data (the database and the profiles' device
+// copies).
+#include <algorithm>
 #include <chrono>
 #include <condition_variable>
 #include <deque>
@@ -22,6 +30,8 @@
 
 #include "p7_gpu.h"
 #include "p7g_internal.h"
+
+#define SERVER_COALESCE_MAX 4      // queries one slot runs as one batch
 
 namespace {
 
@@ -37,6 +47,7 @@ struct Request {
   int                    status = p7G_OK;
   char                   errmsg[256] = { 0 };
   P7_GPU_STATS           st;
+  size_t                 bytes = 0;          // its profile's device tables, while in flight
 };
 
 }  // namespace
@@ -51,33 +62,90 @@ struct p7g_server_s {
   std::map<int64_t, std::shared_ptr<Request>> live;
   int64_t next_ticket = 1;
   bool stop = false;
+  int coalesce = SERVER_COALESCE_MAX;
+  size_t slot_bytes = 0;                     // device memory set aside per slot
+  size_t query_budget = 0;                   // device memory for the profiles of queries in flight
+  size_t in_flight_bytes = 0;
+  int in_flight = 0;
 };
+
+// The queries a free slot takes, under the lock: the queue's front ones, up
+// to `coalesce`, while their profiles' tables fit in the memory left. The
+// sizes come from the profiles' device copies, made on first use.
+static std::vector<std::shared_ptr<Request>>
+take_queries(P7_GPU_SERVER *srv, std::unique_lock<std::mutex> &lk)
+{
+  std::vector<std::shared_ptr<Request>> take;
+  for (;;) {
+    srv->cv_work.wait(lk, [&] { return srv->stop || ! srv->queue.empty(); });
+    if (srv->queue.empty()) return take;                // stopping, nothing left
+    while (! srv->queue.empty() && (int) take.size() < srv->coalesce) {
+      std::shared_ptr<Request> rq = srv->queue.front();
+      if (rq->bytes == 0) {
+        size_t b = 0;
+        lk.unlock();
+        const int st = p7_gpu_oprofile_DeviceBytes(rq->om, srv->db, &b);
+        lk.lock();
+        if (st != p7G_OK) b = 1;                         // the run itself reports the error
+        rq->bytes = b;
+        if (srv->queue.empty() || srv->queue.front() != rq) continue;   // another slot took it meanwhile
+      }
+      const bool alone = srv->in_flight == 0 && take.empty();
+      if (! alone && srv->in_flight_bytes + rq->bytes > srv->query_budget) break;
+      srv->queue.pop_front();
+      srv->in_flight_bytes += rq->bytes;
+      srv->in_flight++;
+      take.push_back(rq);
+    }
+    if (! take.empty()) return take;
+    if (srv->queue.empty()) continue;                    // others took them: wait for more
+    srv->cv_work.wait(lk);                               // the front waits for memory to free up
+  }
+}
 
 static void
 worker_main(P7_GPU_SERVER *srv, P7_GPU_PIPELINE *pli)
 {
   for (;;) {
-    std::shared_ptr<Request> rq;
+    std::vector<std::shared_ptr<Request>> take;
     {
       std::unique_lock<std::mutex> lk(srv->mu);
-      srv->cv_work.wait(lk, [&] { return srv->stop || ! srv->queue.empty(); });
-      if (srv->queue.empty()) return;                 // stopping, nothing left
-      rq = srv->queue.front();
-      srv->queue.pop_front();
+      take = take_queries(srv, lk);
+      if (take.empty()) return;
     }
+    const int n = (int) take.size();
     p7g_pipeline_ResetStats(pli);
-    const int status = p7_gpu_Pipeline(pli, rq->om, srv->db, rq->has_hits ? &rq->hits : nullptr);
-    P7_GPU_STATS st;
-    p7_gpu_pipeline_GetStats(pli, &st);
+    std::vector<P7_GPU_STATS> st((size_t) n);
+    int status;
+    if (n == 1) {
+      status = p7_gpu_Pipeline(pli, take[0]->om, srv->db, take[0]->has_hits ? &take[0]->hits : nullptr);
+      p7_gpu_pipeline_GetStats(pli, &st[0]);
+    } else {
+      std::vector<const P7_GPU_OPROFILE *> oms((size_t) n);
+      std::vector<P7_GPU_HITS> hits((size_t) n);
+      for (int j = 0; j < n; j++) {
+        oms[(size_t) j] = take[(size_t) j]->om;
+        if (take[(size_t) j]->has_hits) hits[(size_t) j] = take[(size_t) j]->hits;
+        else memset(&hits[(size_t) j], 0, sizeof(P7_GPU_HITS));
+      }
+      status = p7_gpu_PipelineBatch(pli, oms.data(), n, srv->db, hits.data(), st.data(), P7_GPU_BATCH_EAGER);
+    }
     {
       std::lock_guard<std::mutex> lk(srv->mu);
-      rq->status = status;
-      if (status != p7G_OK) { strncpy(rq->errmsg, p7_gpu_LastError(), sizeof(rq->errmsg) - 1); }
-      rq->st = st;
-      rq->t_done = Clock::now();
-      rq->done = true;
+      const Clock::time_point now = Clock::now();
+      for (int j = 0; j < n; j++) {
+        Request &rq = *take[(size_t) j];
+        rq.status = status;
+        if (status != p7G_OK) { strncpy(rq.errmsg, p7_gpu_LastError(), sizeof(rq.errmsg) - 1); }
+        rq.st = st[(size_t) j];
+        rq.t_done = now;
+        rq.done = true;
+        srv->in_flight_bytes -= rq.bytes;
+        srv->in_flight--;
+      }
     }
     srv->cv_done.notify_all();
+    srv->cv_work.notify_all();                           // memory freed: a waiting front may fit now
   }
 }
 
@@ -87,8 +155,13 @@ p7_gpu_server_Create(const P7_GPU_SEQDB *db, int nslots, size_t hbm_budget,
 {
   if (! db || nslots < 1) { p7g_set_error("server needs a database and at least one slot"); return nullptr; }
   if (p7_gpu_Init(p7g_current_device()) != p7G_OK) return nullptr;
-  // admission by device memory: a slot's work area is fixed by the database size
-  const size_t per_slot = p7g_pipeline_work_bytes(p7_gpu_seqdb_NSeq(db), p7_gpu_seqdb_NRes(db));
+  // admission by device memory: a slot holds its pipeline's work area and, for
+  // a batch of up to SERVER_COALESCE_MAX queries, one chain work area and the
+  // caller-order outputs (stage and six scores per sequence) per query; the
+  // rest of the budget holds the profiles of the queries in flight
+  const int64_t nseq = p7_gpu_seqdb_NSeq(db);
+  const size_t work = p7g_pipeline_work_bytes(nseq, p7_gpu_seqdb_NRes(db));
+  const size_t per_slot = work * (1 + SERVER_COALESCE_MAX) + (size_t) SERVER_COALESCE_MAX * (size_t) nseq * 25;
   size_t avail = 0;
   if (p7g_device_free_bytes(&avail) != p7G_OK) return nullptr;
   size_t budget = (size_t) (0.9 * (double) avail);
@@ -103,6 +176,8 @@ p7_gpu_server_Create(const P7_GPU_SEQDB *db, int nslots, size_t hbm_budget,
   P7_GPU_SERVER *srv = new (std::nothrow) p7g_server_s();
   if (! srv) return nullptr;
   srv->db = db;
+  srv->slot_bytes = per_slot;
+  srv->query_budget = budget - (size_t) nslots * per_slot;
   for (int i = 0; i < nslots; i++) {
     P7_GPU_PIPELINE *pli = p7_gpu_pipeline_Create(F1, F2, F3, do_biasfilter, 0);
     if (! pli || p7g_pipeline_Reserve(pli, db) != p7G_OK) {
@@ -114,6 +189,27 @@ p7_gpu_server_Create(const P7_GPU_SEQDB *db, int nslots, size_t hbm_budget,
   }
   for (P7_GPU_PIPELINE *pli : srv->slots) srv->workers.emplace_back(worker_main, srv, pli);
   return srv;
+}
+
+extern "C" int
+p7_gpu_server_Memory(const P7_GPU_SERVER *srv, size_t *ret_slot_bytes, size_t *ret_query_budget)
+{
+  if (! srv) { p7g_set_error("server_Memory: null server"); return p7G_EINVAL; }
+  if (ret_slot_bytes) *ret_slot_bytes = srv->slot_bytes;
+  if (ret_query_budget) *ret_query_budget = srv->query_budget;
+  return p7G_OK;
+}
+
+extern "C" int
+p7_gpu_server_SetCoalesce(P7_GPU_SERVER *srv, int n)
+{
+  if (! srv || n < 1 || n > SERVER_COALESCE_MAX) {
+    p7g_set_error("server_SetCoalesce: n must be 1..%d", SERVER_COALESCE_MAX);
+    return p7G_EINVAL;
+  }
+  std::lock_guard<std::mutex> lk(srv->mu);
+  srv->coalesce = n;
+  return p7G_OK;
 }
 
 extern "C" int

@@ -143,6 +143,9 @@ def _load():
         "p7_gpu_seqdb_Load": (i, [c_char_p, POINTER(vp)]),
         "p7_gpu_server_Create": (vp, [vp, i, ctypes.c_size_t, d, d, d, i]),
         "p7_gpu_server_Slots": (i, [vp]),
+        "p7_gpu_server_SetCoalesce": (i, [vp, i]),
+        "p7_gpu_server_Memory": (i, [vp, POINTER(ctypes.c_size_t), POINTER(ctypes.c_size_t)]),
+        "p7_gpu_oprofile_DeviceBytes": (i, [vp, vp, POINTER(ctypes.c_size_t)]),
         "p7_gpu_server_Submit": (i, [vp, vp, POINTER(Hits), POINTER(c_int64)]),
         "p7_gpu_server_Wait": (i, [vp, c_int64, POINTER(Stats), POINTER(c_double)]),
         "p7_gpu_server_Destroy": (None, [vp]),
@@ -305,6 +308,12 @@ class OProfile:
         om.h = h.value
         om.abctype = om.tables()["abctype"]
         return om
+
+    def device_bytes(self, db):
+        """Device memory of this profile's tables for searches of `db` (p7_gpu_oprofile_DeviceBytes)."""
+        b = ctypes.c_size_t()
+        _check(lib.p7_gpu_oprofile_DeviceBytes(self.h, db.h, ctypes.byref(b)), "oprofile_DeviceBytes")
+        return b.value
 
     def tables(self):
         t = Tables()
@@ -577,6 +586,16 @@ class Server:
     @property
     def slots(self):
         return lib.p7_gpu_server_Slots(self.h)
+
+    def set_coalesce(self, n):
+        """Queries one slot may take as one batch (1..4)."""
+        _check(lib.p7_gpu_server_SetCoalesce(self.h, int(n)), "server_SetCoalesce")
+
+    def memory(self):
+        """(bytes set aside per slot, bytes left for the profiles of queries in flight)."""
+        a, b = ctypes.c_size_t(), ctypes.c_size_t()
+        _check(lib.p7_gpu_server_Memory(self.h, ctypes.byref(a), ctypes.byref(b)), "server_Memory")
+        return a.value, b.value
 
     def submit(self, om, scores=True):
         out, hits = _hits(self.db.nseq, scores)

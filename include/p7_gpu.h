@@ -495,18 +495,34 @@ int   p7_gpu_tune_Load(const char *path);
  * "P > F: reject" (p7_pipeline.c:692, 702, 720, 729) is "s < threshold". */
 float p7_gpu_ScoreThreshold(int which, double mu, double lambda, double P);
 
+/* The device memory of a profile's tables for searches of this database
+ * (its device copy is made here if it has none for the database's longest
+ * target yet): what a served query holds while in flight. No counterpart in
+ * the reference. */
+int   p7_gpu_oprofile_DeviceBytes(const P7_GPU_OPROFILE *om, const P7_GPU_SEQDB *db, size_t *ret_bytes);
 /* ---- serving: queries against a resident database, admitted as they arrive ----
  * Stands in for hmmpgmd's search loop (HMMER/src/hmmdmstr.c:701-790,
  * hmmdwrkr.c:535-644), which serves one query at a time: here up to `nslots`
- * queries run at once, each in its own pipeline (stream and device work
- * area), taken from a FIFO as slots free up. The slot count is cut to what
- * `hbm_budget` bytes of device memory hold (0: 90% of the free memory).
- * Each query's result equals a lone p7_gpu_Pipeline call with the same
- * thresholds. Every call may come from any host thread. */
+ * slots (pipelines with their streams and device work areas) take queries
+ * from a FIFO as they free up, and a slot that finds several queries waiting
+ * takes up to four of them as one p7_gpu_PipelineBatch (their ungapped passes
+ * in one persistent launch per kernel shape). The slots' work areas, for
+ * batches of four, are set aside within `hbm_budget` bytes of device memory
+ * (0: 90% of the free memory), which cuts the slot count; what is left admits
+ * the queries in flight by their profiles' device tables (a query waits until
+ * its tables fit; one alone always runs). Each query's result equals a lone
+ * p7_gpu_Pipeline call with the same thresholds. Every call may come from any
+ * host thread. */
 typedef struct p7g_server_s P7_GPU_SERVER;
 P7_GPU_SERVER *p7_gpu_server_Create(const P7_GPU_SEQDB *db, int nslots, size_t hbm_budget,
                                     double F1, double F2, double F3, int do_biasfilter);
 int   p7_gpu_server_Slots(const P7_GPU_SERVER *srv);
+/* Queries one slot may take as one batch: 1 (each query alone) to 4 (the
+ * default). */
+int   p7_gpu_server_SetCoalesce(P7_GPU_SERVER *srv, int n);
+/* The device memory set aside per slot, and what the budget leaves for the
+ * profiles of the queries in flight. */
+int   p7_gpu_server_Memory(const P7_GPU_SERVER *srv, size_t *ret_slot_bytes, size_t *ret_query_budget);
 /* Queues a query and returns at once; `om` and `hits` (may be NULL) must stay
  * valid until the matching Wait returns. */
 int   p7_gpu_server_Submit(P7_GPU_SERVER *srv, const P7_GPU_OPROFILE *om, P7_GPU_HITS *hits, int64_t *ret_ticket);

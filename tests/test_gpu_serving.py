@@ -62,6 +62,40 @@ def test_admission_is_bounded_by_the_memory_budget(setup):
     big.close()
 
 
+def test_coalesced_queries_and_admission_by_their_memory(setup):
+    """A slot that finds several queries waiting runs up to four as one batch
+    (one persistent ungapped-pass launch per kernel shape): every result
+    equals the query alone, with one query per launch or four. With a budget
+    that leaves room for only the largest profile's tables, the queries run
+    one at a time and all finish."""
+    sdb, oms = setup
+    alone = [p7gpu.Pipeline().run(om, sdb, scores=False)["stage"] for om in oms]
+    order = [0, 1, 2, 3, 4, 5, 1, 0, 2, 5, 4, 3]
+    for c in (1, 4):
+        srv = p7gpu.Server(sdb, nslots=1)
+        srv.set_coalesce(c)
+        tickets = [(i, srv.submit(oms[i], scores=False)) for i in order]
+        for i, t in tickets:
+            out, st, _ = srv.wait(t)
+            assert np.array_equal(out["stage"], alone[i]), (c, i)
+            assert st["nseqs"] == sdb.nseq
+        srv.close()
+    probe = p7gpu.Server(sdb, nslots=1)
+    with pytest.raises(p7gpu.P7GpuError):
+        probe.set_coalesce(5)
+    per_slot, left = probe.memory()
+    probe.close()
+    assert per_slot > 0 and left > 0
+    need = max(om.device_bytes(sdb) for om in oms)
+    tight = p7gpu.Server(sdb, nslots=1, hbm_budget=per_slot + need)
+    assert tight.slots == 1 and tight.memory()[1] == need
+    tickets = [(i, tight.submit(oms[i], scores=False)) for i in order]
+    for i, t in tickets:
+        out, _, _ = tight.wait(t)
+        assert np.array_equal(out["stage"], alone[i]), i
+    tight.close()
+
+
 def test_calibration_with_the_library_equals_the_oracle_calibration(gpu):
     """synth.calibrate_with fits the E-value parameters to the library's own MSV,
     Viterbi and Forward scores; fitted to the oracle's scores of the same
