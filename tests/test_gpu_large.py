@@ -1,5 +1,7 @@
 """Models longer than one LDS table holds (M = 2100, 2600 and 5000, as in the
-precision stress: long targets, indel-rich homologs).
+precision stress: long targets, indel-rich homologs), and the lengths at which
+the cooperative parsers take each of their other shapes (M = 1300, 1600,
+2000, 3500).
 
 For these models the ungapped MSV pass runs in two chunks of model positions,
 joined by a boundary column in HBM (above M = 2560); the full MSV recurrence
@@ -26,6 +28,9 @@ LARGE = [2100, 2600, 5000]
 # every shape of the cooperative Viterbi kernel (ND = 20, 24, 32 at M = 3500, 40) and the one-wave
 # kernel with its whole table in LDS at its largest (ND = 16, M = 2000)
 VIT_LARGE = [2000, 2100, 2600, 3500, 5000]
+# every instantiated shape of the cooperative Forward and Backward parsers (positions per lane
+# B = 24, 28, 32, 40, 48, 64, 80)
+FB_LARGE = [1300, 1600, 2000, 2100, 2600, 3500, 5000]
 STATS = (-9.0, 0.6931, -9.5, 0.6931, -5.0, 0.6931)
 
 
@@ -34,7 +39,7 @@ def large(gpu, tmp_path_factory):
     d = tmp_path_factory.mktemp("large")
     rng = np.random.default_rng(77)
     out = {}
-    for M in VIT_LARGE:
+    for M in sorted(set(VIT_LARGE) | set(FB_LARGE)):
         h = synth.random_profile(M, rng, f"large_M{M}")
         h.stats = STATS
         path = str(d / f"large_M{M}.hmm")
@@ -79,7 +84,7 @@ def test_large_msv_and_viterbi_bit_exact(large, M):
         assert st == vst[i] and same(sc, vsc[i]), ("vit", M, i, L, st, sc, vst[i], vsc[i])
 
 
-@pytest.mark.parametrize("M", LARGE)
+@pytest.mark.parametrize("M", FB_LARGE)
 def test_large_forward_backward_within_1e4_nats(large, M):
     path, hh, seqs, db = large[M]
     om = p7gpu.OProfile.from_file(path)
