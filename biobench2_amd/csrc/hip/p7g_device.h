@@ -145,6 +145,11 @@ struct DomBufs {
   // within the region copy; 0 where no trace)
   float   *n2oa = nullptr;
   int4    *n2ali = nullptr;
+  // the alignments themselves (k_oa_trace): region k's states at
+  // n2path + n2poff[k], at most its length + M + 2 words, n2pn[k] of them
+  uint32_t *n2path = nullptr;
+  const int64_t *n2poff = nullptr;
+  int32_t *n2pn = nullptr;
 };
 
 // the parts of a region's null2 area, in floats, each a multiple of 64: the
@@ -155,8 +160,12 @@ __host__ __device__ inline int64_t n2_xrows_floats(int L) { return (6 * ((int64_
 __host__ __device__ inline int n2_oa_words(int B) { return (B + 7) / 8; }
 __host__ __device__ inline int64_t n2_oa_bits_at(int L, int B) { return n2_xrows_floats(L) + (int64_t) L * 128 * B; }
 __host__ __device__ inline int64_t n2_oa_rows_at(int L, int B) { return n2_oa_bits_at(L, B) + (int64_t) L * 64 * n2_oa_words(B); }
+// then, for the alignment display, each cell's posterior probability codes
+// (p7_alidisplay_EncodePostProb: 0-9, 10 for '*'), M in the low and I in the
+// high four bits of a byte, bytes [row][j][64 lanes]
+__host__ __device__ inline int64_t n2_pp_at(int L, int B) { return n2_oa_rows_at(L, B) + (((int64_t) L + 1 + 63) & ~(int64_t) 63); }
 __host__ __device__ inline int64_t n2_area_floats(int L, int B)
-{ return n2_oa_rows_at(L, B) + (((int64_t) L + 1 + 63) & ~(int64_t) 63); }
+{ return n2_pp_at(L, B) + (((int64_t) L * 16 * B + 63) & ~(int64_t) 63); }
 // optimal accuracy traceback choices: per cell, bits 0-1 the state before M
 // (0 M, 1 I, 2 D, 3 B), bit 2 before I (1: I), bit 3 before D (1: D); per
 // row, bits 0-15 the node E leaves from, bit 16 set if from D, bit 17 C from
@@ -341,6 +350,7 @@ hipError_t launch_bck(const DevProfile &p, const DevSeqs &s, const int32_t *list
 // lane per region; the alignment's first and last M states go to dom.n2ali.
 hipError_t launch_oa_trace(const DevProfile &p, const DevSeqs &s, const int32_t *list, int64_t n, const DomBufs &dom,
                            hipStream_t st);
+
 // Posterior decoding of the special states and the region heuristics over the
 // Backward list (the F3 survivors), one target per thread.
 hipError_t launch_domains(const DevProfile &p, const DevSeqs &s, const int32_t *list, const int32_t *count, int64_t maxn,

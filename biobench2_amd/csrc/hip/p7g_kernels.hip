@@ -1030,6 +1030,13 @@ __device__ __forceinline__ void null2_epilogue(const DevProfile &p, const float 
   if (z == 0) { *dc_out = dc; *st_out = 0; }
 }
 
+// p7_alidisplay_EncodePostProb (p7_alidisplay.c:520-524) as a number: '*' is 10
+__device__ __forceinline__ int pp_code(float p)
+{
+  const double q = (double) p + 0.05;
+  return (q >= 1.0) ? 10 : (int) (q * 10.0);
+}
+
 // One row of the optimal accuracy fill (p7_OptimalAccuracy, impl_sse/
 // optacc.c:60-175) inside the null2 Forward pass, from the row's posteriors,
 // with the traceback's choices (select_m/d/i/e/c/j/b, optacc.c:288-423) made
@@ -1050,7 +1057,8 @@ __device__ __forceinline__ void oa_row(const DevProfile &p, int z, int i,
                                        const float (&fM)[B], const float (&fI)[B], const float *cr, float totr,
                                        float (&accM)[B], float (&accI)[B], float ppN, float ppJ, float ppC,
                                        float floop, float fmove, float (&oaM)[B], float (&oaI)[B], float (&oaD)[B],
-                                       float &oaN, float &oaJ, float &oaB, float &oaC, uint32_t *obits, uint32_t *orow)
+                                       float &oaN, float &oaJ, float &oaB, float &oaC, uint32_t *obits, uint32_t *orow,
+                                       uint8_t *opp)
 {
 #define OTR(A, T, j) (PIN ? A[j] : trr[(T) * rowf + (j) * 64])
   const float NI = -INFINITY;
@@ -1071,6 +1079,8 @@ __device__ __forceinline__ void oa_row(const DevProfile &p, int z, int i,
     const float pi = __fmul_rn(__fmul_rn(fI[j], cr[rowf + j*64]), totr);
     accM[j] = __fadd_rn(pm, accM[j]);
     accI[j] = __fadd_rn(pi, accI[j]);
+    // the cell's posterior probability codes for the alignment display
+    opp[((size_t) (i - 1) * B + j) * 64 + z] = (uint8_t) (pp_code(pm) | (pp_code(pi) << 4));
     const float mp = j ? oaM[j-1] : mpL, ip = j ? oaI[j-1] : ipL, dp = j ? oaD[j-1] : dpL;
     const bool tb = OTR(tBM, VT_BM, j) > 0.0f, tm = OTR(tMM, VT_MM, j) > 0.0f;
     const bool ti = OTR(tIM, VT_IM, j) > 0.0f, td = OTR(tDM, VT_DM, j) > 0.0f;
@@ -1260,11 +1270,13 @@ void k_fwd(const DevProfile p, const uint8_t *__restrict__ res, const int64_t *_
     float oaM[B], oaI[B], oaD[B];
     float oaN = 0.0f, oaJ = -INFINITY, oaB = 0.0f, oaC = -INFINITY;
     uint32_t *obits = nullptr, *orow = nullptr;
+    uint8_t *opp = nullptr;
     if (MODE == 3) {
 #pragma unroll
       for (int j = 0; j < B; j++) oaM[j] = oaI[j] = oaD[j] = -INFINITY;
       obits = (uint32_t *) (area + n2_oa_bits_at(L, B));
       orow  = (uint32_t *) (area + n2_oa_rows_at(L, B));
+      opp   = (uint8_t *) (area + n2_pp_at(L, B));
       if (z == 0) orow[0] = 1u << OA_BBIT;        // B(0) comes from N(0)
     }
     if (pool) {
@@ -1366,7 +1378,7 @@ void k_fwd(const DevProfile p, const uint8_t *__restrict__ res, const int64_t *_
         // the row's posteriors (summed for null2) and its optimal accuracy row
         oa_row<B, NW, PIN>(p, z, i, tBM, tMM, tIM, tDM, tMDS, tMI, tII, tr + (PIN ? 0 : opaque_zero()), rowf,
                            M, I, cr, totr, accM, accI, ppN, ppJ, ppC, floop, fmove, oaM, oaI, oaD, oaN, oaJ, oaB, oaC,
-                           obits, orow);
+                           obits, orow, opp);
       }
     }
     if (MODE == 3) {
@@ -2189,25 +2201,40 @@ hipError_t launch_copy32(void *dst, const void *src, int64_t nwords, hipStream_t
 // p7_OATrace (impl_sse/optacc.c:227-276) over the choices the null2 pass
 // stored (oa_row), one lane per region, from C at the region's last residue
 // back to S; keeps the first and last M states (p7_alidisplay_Create's
-// hmmfrom/hmmto/sqfrom/sqto, p7_alidisplay.c:74-145).
+// hmmfrom/hmmto/sqfrom/sqto, p7_alidisplay.c:74-145), and writes the
+// alignment itself for the display, from its last M state back to its first
+// (p7_alidisplay_Create's z2 down to z1), one word per state: state (0 M,
+// 1 I, 2 D) in bits 0-1, model position in bits 2-18, the cell's posterior
+// probability code in bits 19-22, the residue code in bits 23-27; at most
+// n2pcap words at n2path + n2poff[region], their number to n2pn[region]
+// (0 without an alignment).
 __global__ void k_oa_trace(const DevProfile p, const int32_t *__restrict__ list, int64_t n,
-                           const int32_t *__restrict__ vlen, DomBufs dom)
+                           const int32_t *__restrict__ vlen, const uint8_t *__restrict__ vres,
+                           const int64_t *__restrict__ voff, DomBufs dom)
 {
   enum { S_S = 0, S_N, S_B, S_M, S_I, S_D, S_E, S_J, S_C };
   const int64_t t = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
   const int slot = list[t];
   int4 out = make_int4(0, 0, 0, 0);
-  if (dom.n2st[slot] != 0) { dom.n2ali[slot] = out; return; }       // no decoding, no trace
+  if (dom.n2st[slot] != 0) { dom.n2ali[slot] = out; dom.n2pn[slot] = 0; return; }   // no decoding, no trace
   const int L = vlen[slot], B = p.fb_B, NW = n2_oa_words(B);
   const float *area = dom.n2pool + dom.n2off[slot];
   const uint32_t *bits = (const uint32_t *) (area + n2_oa_bits_at(L, B));
   const uint32_t *rows = (const uint32_t *) (area + n2_oa_rows_at(L, B));
+  const uint8_t *pp = (const uint8_t *) (area + n2_pp_at(L, B));
+  const uint8_t *sq = vres + voff[slot];               // residue i at sq[i-1]
+  uint32_t *path = dom.n2path + dom.n2poff[slot];
+  const int pcap = L + p.M + 2;
   auto cell = [&](int i, int k) -> uint32_t {
     const int z = (k - 1) / B, j = (k - 1) % B;
     return (bits[((size_t) (i - 1) * NW + j / 8) * 64 + z] >> ((j % 8) * 4)) & 15u;
   };
-  int i = L, k = 0, s0 = S_C, nm = 0, ok = 1;
+  auto ppat = [&](int i, int k) -> uint32_t {
+    const int z = (k - 1) / B, j = (k - 1) % B;
+    return pp[((size_t) (i - 1) * B + j) * 64 + z];
+  };
+  int i = L, k = 0, s0 = S_C, nm = 0, ok = 1, nw = 0, last = 0;
   const int guard = 4 * (L + p.M) + 8;
   for (int step = 0; s0 != S_S; step++) {
     int s1;
@@ -2249,11 +2276,23 @@ __global__ void k_oa_trace(const DevProfile p, const int32_t *__restrict__ list,
       out.x = k; out.z = i;
       nm++;
     }
+    // the display's states: from the first M walked back to (states before
+    // it are the E -> D end, not shown) to the last (B precedes it)
+    if ((s1 == S_M || ((s1 == S_I || s1 == S_D) && nm > 0))) {
+      if (i < 1 || k < 1 || k > p.M || nw >= pcap) { ok = 0; break; }
+      const uint32_t b = (s1 == S_D) ? 0u : ppat(i, k);
+      const uint32_t st = (s1 == S_M) ? 0u : (s1 == S_I) ? 1u : 2u;
+      const uint32_t code = (s1 == S_M) ? (b & 15u) : (s1 == S_I) ? (b >> 4) : 0u;
+      const uint32_t x = (s1 == S_D) ? 0u : (uint32_t) sq[i - 1];
+      path[nw++] = st | ((uint32_t) k << 2) | (code << 19) | (x << 23);
+      if (s1 == S_M) last = nw;
+    }
     if ((s1 == S_N || s1 == S_J || s1 == S_C) && s1 == s0) i--;
     s0 = s1;
   }
-  if (! ok || nm == 0) out = make_int4(0, 0, 0, 0);
+  if (! ok || nm == 0) { out = make_int4(0, 0, 0, 0); last = 0; }
   dom.n2ali[slot] = out;
+  dom.n2pn[slot] = last;
 }
 
 hipError_t launch_oa_trace(const DevProfile &p, const DevSeqs &s, const int32_t *list, int64_t n, const DomBufs &dom,
@@ -2261,7 +2300,7 @@ hipError_t launch_oa_trace(const DevProfile &p, const DevSeqs &s, const int32_t 
 {
   if (n <= 0) return hipSuccess;
   const unsigned grid = (unsigned) ((n + 63) / 64);
-  hipLaunchKernelGGL(k_oa_trace, dim3(grid), dim3(64), 0, st, p, list, n, s.len, dom);
+  hipLaunchKernelGGL(k_oa_trace, dim3(grid), dim3(64), 0, st, p, list, n, s.len, s.res, s.off, dom);
   return hipGetLastError();
 }
 

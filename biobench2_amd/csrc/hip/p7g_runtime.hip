@@ -16,7 +16,9 @@
 #include <set>
 #include <chrono>
 #include <algorithm>
+#include <string>
 #include <vector>
+#include <ctype.h>
 
 #include "p7_gpu.h"
 #include "p7g_internal.h"
@@ -59,6 +61,10 @@ struct Work {
   int64_t     n2size = 0;
   float      *n2oa = nullptr;           // optimal accuracy score and alignment of each region
   int4       *n2ali = nullptr;
+  int64_t    *n2poff = nullptr;         // the alignments' states: per region offset and count,
+  int32_t    *n2pn = nullptr;
+  uint32_t   *n2path = nullptr;         // and the words (grown on use)
+  int64_t     n2path_cap = 0;
   // multidomain regions: per region its job and result, the envelope copies'
   // source/destination/length, one count block (grown on use)
   MdJob      *mdjobs = nullptr;
@@ -932,10 +938,10 @@ dom_reserve(Work &w, int64_t n, int64_t pool_floats)
   if (w.dom_mem) { (void) hipFree(w.dom_mem); w.dom_mem = nullptr; w.dom_cap = 0; }
   if (w.dom.pool) { (void) hipFree(w.dom.pool); w.dom.pool = nullptr; w.dom.size = 0; }
   const int64_t cap = std::max<int64_t>(n, 1024);
-  const size_t total = (size_t) cap * (8 + 8 + 4 + 4 + 4) + (16 + 8 + 8 + 4 * 11) * (size_t) DOM_REG_CAP
+  const size_t total = (size_t) cap * (8 + 8 + 4 + 4 + 4) + (16 + 8 + 8 + 8 + 4 * 12) * (size_t) DOM_REG_CAP
                        + (size_t) DOM_RES_CAP + 4 * (size_t) N2_CNT_INTS
                        + 4 * (size_t) (DOM_RES_CAP + DOM_REG_CAP) + 8 * (size_t) DOM_REG_CAP
-                       + (4 + 16) * (size_t) DOM_REG_CAP + 28 * 256;
+                       + (4 + 16) * (size_t) DOM_REG_CAP + 30 * 256;
   char *m = nullptr;
   HIPCHK(hipMalloc((void **) &m, total));
   size_t o = 0;
@@ -960,6 +966,7 @@ dom_reserve(Work &w, int64_t n, int64_t pool_floats)
   w.dout.rsoff = (int64_t *) take(8 * (size_t) DOM_REG_CAP);
   w.n2oa = (float *) take(4 * (size_t) DOM_REG_CAP);
   w.n2ali = (int4 *) take(16 * (size_t) DOM_REG_CAP);
+  w.n2poff = (int64_t *) take(8 * (size_t) DOM_REG_CAP); w.n2pn = (int32_t *) take(4 * (size_t) DOM_REG_CAP);
   if (o > total) { (void) hipFree(m); p7g_set_error("internal: domain buffer overrun"); return p7G_EINVAL; }
   w.dom_mem = m; w.dom_cap = cap;
   HIPCHK(hipMalloc((void **) &w.dom.pool, sizeof(float) * (size_t) pool_floats));
@@ -974,9 +981,10 @@ work_release(Work &w)
   if (w.dom_mem) (void) hipFree(w.dom_mem);
   if (w.dom.pool) (void) hipFree(w.dom.pool);
   if (w.n2pool) (void) hipFree(w.n2pool);
+  if (w.n2path) (void) hipFree(w.n2path);
   w.dom_mem = nullptr; w.dom = DomBufs(); w.dout = DomOut(); w.dom_cap = 0;
   w.n2off = nullptr; w.n2dc = nullptr; w.n2st = nullptr; w.n2list = nullptr; w.n2cnt = nullptr;
-  w.n2oa = nullptr; w.n2ali = nullptr;
+  w.n2oa = nullptr; w.n2ali = nullptr; w.n2poff = nullptr; w.n2pn = nullptr; w.n2path = nullptr; w.n2path_cap = 0;
   w.n2pool = nullptr; w.n2size = 0;
   if (w.mdjobs) (void) hipFree(w.mdjobs);
   w.mdjobs = nullptr; w.mdout = nullptr; w.mdsrc = w.mddst = nullptr; w.mdlen = w.mdcnt = nullptr; w.md_cap = 0;
@@ -1204,6 +1212,7 @@ struct p7g_pipeline_s {
   mutable std::vector<float> envsc, n2dc, oasc;
   mutable std::vector<int4> ali;
   mutable std::vector<int32_t> n2st, regs_slot, regs_L;
+  mutable std::vector<int64_t> regs_didx;      // each sorted record's index on the device
   // hit-list scores of the last run, computed on first request
   mutable bool hits_ready = false;
   mutable std::vector<int64_t> h_tseq;
@@ -1226,6 +1235,16 @@ struct p7g_pipeline_s {
   const float *fwdsc_dev = nullptr;
   const float *nexp_dev = nullptr;             // nexpected by slot
   const int32_t *rseq_dev = nullptr, *rlen_dev = nullptr;   // per record: caller index, target length
+  const uint32_t *path_dev = nullptr;          // the alignments' states (k_oa_trace)
+  const int64_t *poff_dev = nullptr;
+  const int32_t *pn_dev = nullptr;
+  // what the alignment display needs of the last run's profile, copied so
+  // that the results outlive it: alphabet, length, annotation, match odds
+  const P7G_ALPHABET *d_abc = nullptr;
+  int         d_M = 0;
+  std::string d_cons, d_cs, d_rf;
+  bool        d_has_cs = false, d_has_rf = false;
+  std::vector<float> d_rfv;                    // [Kp][M]
   int64_t nslots = 0;
   float ftau = 0.0f, flambda = 0.0f;
   P7_GPU_STATS st;
@@ -1617,6 +1636,24 @@ null2_regions(Work &w, const DevProfile &p, const DevSeqs &s, int64_t nregs, con
   HIPCHK(launch_fill32((uint32_t *) w.n2dc, 0x7fc00000u, nrec, w.st));
   HIPCHK(launch_fill32((uint32_t *) w.n2oa, 0x7fc00000u, nrec, w.st));
   HIPCHK(launch_fill32((uint32_t *) w.n2ali, 0u, 4 * nrec, w.st));
+  HIPCHK(launch_fill32((uint32_t *) w.n2pn, 0u, nrec, w.st));
+  // the alignments' words: an envelope's length + M + 2 each
+  {
+    std::vector<int64_t> poff((size_t) nrec, 0);
+    int64_t ptot = 0;
+    for (int64_t k = 0; k < nrec; k++) {
+      poff[(size_t) k] = ptot;
+      if (rec[(size_t) k].w != 1) ptot += (int64_t) vlen[(size_t) k] + p.M + 2;
+    }
+    if (w.n2path_cap < ptot) {
+      HIPCHK(hipStreamSynchronize(w.st));
+      if (w.n2path) { (void) hipFree(w.n2path); w.n2path = nullptr; w.n2path_cap = 0; }
+      HIPCHK(hipMalloc((void **) &w.n2path, sizeof(uint32_t) * (size_t) std::max<int64_t>(ptot, 1)));
+      w.n2path_cap = ptot;
+    }
+    HIPCHK(hipMemcpyAsync(w.n2poff, poff.data(), sizeof(int64_t) * (size_t) nrec, hipMemcpyHostToDevice, w.st));
+    HIPCHK(hipStreamSynchronize(w.st));          // poff is the copy's source
+  }
   // the pool: what the envelopes need, bounded by a quarter of the free memory
   // (the pool already held counts as free)
   int64_t need_all = 0;
@@ -1667,6 +1704,7 @@ null2_regions(Work &w, const DevProfile &p, const DevSeqs &s, int64_t nregs, con
   eb.n2pool = w.n2pool; eb.n2off = w.n2off; eb.n2dc = w.n2dc; eb.n2st = w.n2st;
   eb.n2scl = w.dout.rscale; eb.n2soff = w.dout.rsoff;
   eb.n2oa = w.n2oa; eb.n2ali = w.n2ali;
+  eb.n2path = w.n2path; eb.n2poff = w.n2poff; eb.n2pn = w.n2pn;
   const LenCfg lc = lencfg_per_target();
   for (size_t b = 0; b < batches.size(); b++) {
     const int32_t *bl = w.n2list + batches[b].first;
@@ -1795,6 +1833,12 @@ p7_gpu_Pipeline(P7_GPU_PIPELINE *pli, const P7_GPU_OPROFILE *om, const P7_GPU_SE
     pli->rseq_dev = w.dout.rseq; pli->rlen_dev = w.dout.rlen;
     pli->n2dc_dev = pli->do_null2 ? w.n2dc : nullptr; pli->n2st_dev = w.n2st;
     pli->oa_dev = pli->do_null2 ? w.n2oa : nullptr; pli->ali_dev = pli->do_null2 ? w.n2ali : nullptr;
+    pli->path_dev = pli->do_null2 ? w.n2path : nullptr; pli->poff_dev = w.n2poff; pli->pn_dev = w.n2pn;
+    pli->d_abc = om->abc; pli->d_M = om->M;
+    pli->d_cons = om->cons ? om->cons : "";
+    pli->d_has_cs = om->cs != nullptr; pli->d_cs = om->cs ? om->cs : "";
+    pli->d_has_rf = om->rf != nullptr; pli->d_rf = om->rf ? om->rf : "";
+    pli->d_rfv.assign(om->rfv, om->rfv + (size_t) om->abc->Kp * om->M);
     pli->fwdsc_dev = w.r.fwdsc; pli->nexp_dev = w.dout.nexp; pli->nslots = n;
     pli->ftau = om->evparam[p7G_FTAU]; pli->flambda = om->evparam[p7G_FLAMBDA];
     pli->regs_ready = false; pli->regs.clear();
@@ -1880,12 +1924,12 @@ regs_fetch(const P7_GPU_PIPELINE *pli)
       const int4 &x = rec[(size_t) a], &y = rec[(size_t) b];
       return x.x != y.x ? x.x < y.x : x.y != y.y ? x.y < y.y : x.w < y.w; });
     pli->regs.resize(nr); pli->envsc.resize(nr); pli->n2dc.resize(nr); pli->n2st.resize(nr); pli->regs_slot.resize(nr);
-    pli->regs_L.resize(nr);
+    pli->regs_L.resize(nr); pli->regs_didx.resize(nr);
     pli->oasc.resize(nr); pli->ali.resize(nr);
     for (size_t k = 0; k < nr; k++) {
       const size_t o = (size_t) ord[k];
       pli->regs[k] = rec[o]; pli->envsc[k] = sc[o]; pli->n2dc[k] = dc[o]; pli->n2st[k] = n2[o];
-      pli->regs_slot[k] = slot_of[o]; pli->regs_L[k] = rlen[o];
+      pli->regs_slot[k] = slot_of[o]; pli->regs_L[k] = rlen[o]; pli->regs_didx[k] = (int64_t) o;
       // alignment coordinates in the target: the region copy starts at residue ienv
       const int4 a = al[o];
       pli->oasc[k] = (n2[o] == 0 && a.x > 0) ? oa[o] : NAN;
@@ -2196,9 +2240,50 @@ p7_gpu_pipeline_GetTargetCounts(const P7_GPU_PIPELINE *pli, float *nexpected, in
 
 // The reported list as the text writers take it: targets in rank order with
 // their domains (rescored envelopes, in sequence order) and flags
+// The alignment display's lines of one domain (p7_alidisplay_Create,
+// p7_alidisplay.c:147-210) from its states in alignment order: consensus
+// residue or '.', match line (the residue when it is the consensus residue,
+// '+' when its match odds exceed 1, else ' '), the residue (upper case for
+// M, lower case for I, '-' for D), its posterior probability code ('.' for
+// D), and the CS and RF annotation ('.' under I).
+static void
+display_lines(const P7_GPU_PIPELINE *pli, const uint32_t *w, int n, std::string &model, std::string &mline,
+              std::string &aseq, std::string &pp, std::string &cs, std::string &rf)
+{
+  const P7G_ALPHABET *abc = pli->d_abc;
+  const int M = pli->d_M;
+  const bool has_cons = (int) pli->d_cons.size() > M;
+  model.assign((size_t) n, ' '); mline.assign((size_t) n, ' '); aseq.assign((size_t) n, ' '); pp.assign((size_t) n, ' ');
+  cs.clear(); rf.clear();
+  if (pli->d_has_cs) cs.assign((size_t) n, ' ');
+  if (pli->d_has_rf) rf.assign((size_t) n, ' ');
+  for (int z = 0; z < n; z++) {
+    const uint32_t v = w[z];
+    const int st = (int) (v & 3u), k = (int) ((v >> 2) & 0x1ffffu), code = (int) ((v >> 19) & 15u), x = (int) ((v >> 23) & 31u);
+    const bool kin = k >= 1 && k <= M && x < abc->Kp;
+    const char cons = (has_cons && kin) ? pli->d_cons[(size_t) k] : 'x';
+    if (st == 0) {
+      model[z] = cons;
+      const int cx = abc->inmap[(unsigned char) cons & 127];
+      if (cx == x) mline[z] = cons;
+      else if (kin && pli->d_rfv[(size_t) x * M + k - 1] > 1.0f) mline[z] = '+';
+      else mline[z] = ' ';
+      aseq[z] = (char) toupper(abc->sym[x]);
+    } else if (st == 1) {
+      model[z] = '.'; mline[z] = ' ';
+      aseq[z] = (char) tolower(abc->sym[x]);
+    } else {
+      model[z] = cons; mline[z] = ' '; aseq[z] = '-';
+    }
+    pp[z] = (st == 2) ? '.' : (code >= 10) ? '*' : (char) ('0' + code);
+    if (pli->d_has_cs) cs[z] = (st == 1 || ! kin || (int) pli->d_cs.size() <= M) ? '.' : pli->d_cs[(size_t) k];
+    if (pli->d_has_rf) rf[z] = (st == 1 || ! kin || (int) pli->d_rf.size() <= M) ? '.' : pli->d_rf[(size_t) k];
+  }
+}
+
 static int
 report_list(const P7_GPU_PIPELINE *pli, const P7_GPU_REPORT *rep, std::vector<P7G_RHIT> &hits,
-            std::vector<P7G_RDOM> &doms, P7G_RLIST &lst)
+            std::vector<P7G_RDOM> &doms, P7G_RLIST &lst, std::vector<std::string> &text)
 {
   if (! rep || ! rep->qname || ! rep->tname) { p7g_set_error("the report needs the query's name and the targets' names"); return p7G_EINVAL; }
   int st;
@@ -2214,7 +2299,18 @@ report_list(const P7_GPU_PIPELINE *pli, const P7_GPU_REPORT *rep, std::vector<P7
     while (a < nr && (int64_t) pli->regs[a].x != pli->h_tseq[k]) a++;
     first[k] = a;
   }
-  hits.clear(); doms.clear();
+  hits.clear(); doms.clear(); text.clear();
+  // the alignments' states, when drawn: every record's count and offset, then
+  // the words of the reported domains
+  const bool ali = rep->show_alignments && pli->path_dev && pli->d_abc;
+  std::vector<int32_t> pn;
+  std::vector<int64_t> poff;
+  if (ali && pli->nregs > 0) {
+    pn.resize((size_t) pli->nregs); poff.resize((size_t) pli->nregs);
+    HIPCHK(hipMemcpy(pn.data(), pli->pn_dev, sizeof(int32_t) * pn.size(), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(poff.data(), pli->poff_dev, sizeof(int64_t) * poff.size(), hipMemcpyDeviceToHost));
+  }
+  std::vector<int> dtext;                  // per domain: its first line in `text`, or -1
   std::vector<size_t> dstart;
   for (int64_t k64 : ord) {
     const size_t k = (size_t) k64;
@@ -2240,6 +2336,25 @@ report_list(const P7_GPU_PIPELINE *pli, const P7_GPU_REPORT *rep, std::vector<P7
       d.hmmfrom = pli->ali[q].x; d.hmmto = pli->ali[q].y; d.alifrom = pli->ali[q].z; d.alito = pli->ali[q].w;
       d.ienv = pli->regs[q].y; d.jenv = pli->regs[q].z;
       d.flags = df[q];
+      d.model = d.mline = d.aseq = d.ppline = d.csline = d.rfline = nullptr; d.N = 0;
+      int ti = -1;
+      if (ali && (d.flags & 1)) {
+        const int64_t o = pli->regs_didx[q];
+        const int n = pn[(size_t) o];
+        if (n > 0) {
+          std::vector<uint32_t> w((size_t) n);
+          HIPCHK(hipMemcpy(w.data(), pli->path_dev + poff[(size_t) o], sizeof(uint32_t) * (size_t) n, hipMemcpyDeviceToHost));
+          std::reverse(w.begin(), w.end());                  // the kernel writes them last first
+          std::string a[6];
+          display_lines(pli, w.data(), n, a[0], a[1], a[2], a[3], a[4], a[5]);
+          ti = (int) text.size();
+          for (int e = 0; e < 6; e++) text.push_back(a[e]);
+          d.N = n;
+          d.hmmfrom = (int) ((w[0] >> 2) & 0x1ffffu);
+          d.hmmto = (int) ((w[(size_t) n - 1] >> 2) & 0x1ffffu);
+        }
+      }
+      dtext.push_back(ti);
       if ((int) (q - first[k]) == pli->h_ts[k].best) h.best = (int) (doms.size() - dstart.back());
       if (d.flags & 1) h.nreported++;
       if (d.flags & 2) h.nincluded++;
@@ -2249,13 +2364,21 @@ report_list(const P7_GPU_PIPELINE *pli, const P7_GPU_REPORT *rep, std::vector<P7
     hits.push_back(h);
   }
   for (size_t h = 0; h < hits.size(); h++) hits[h].dom = doms.data() + dstart[h];
+  for (size_t e = 0; e < doms.size(); e++)            // text no longer grows: its strings stay put
+    if (dtext[e] >= 0) {
+      const size_t t0 = (size_t) dtext[e];
+      doms[e].model = text[t0].c_str(); doms[e].mline = text[t0 + 1].c_str(); doms[e].aseq = text[t0 + 2].c_str();
+      doms[e].ppline = text[t0 + 3].c_str();
+      doms[e].csline = pli->d_has_cs ? text[t0 + 4].c_str() : nullptr;
+      doms[e].rfline = pli->d_has_rf ? text[t0 + 5].c_str() : nullptr;
+    }
   memset(&lst, 0, sizeof(lst));
   lst.hit = hits.data(); lst.N = (int) hits.size();
   lst.qname = rep->qname;
   lst.qacc = (rep->qacc && rep->qacc[0]) ? rep->qacc : nullptr;
   lst.M = rep->M;
   lst.Z = rep->Z; lst.domZ = domZ;
-  lst.textw = rep->textw; lst.show_accessions = rep->show_accessions;
+  lst.textw = rep->textw; lst.show_accessions = rep->show_accessions; lst.show_alignments = ali;
   return p7G_OK;
 }
 
@@ -2269,9 +2392,10 @@ report_text(const P7_GPU_PIPELINE *pli, const P7_GPU_REPORT *rep, int which, int
   std::lock_guard<std::mutex> rl(pli->regs_mu);
   std::vector<P7G_RHIT> hits;
   std::vector<P7G_RDOM> doms;
+  std::vector<std::string> text;
   P7G_RLIST lst;
   int st;
-  if ((st = report_list(pli, rep, hits, doms, lst)) != p7G_OK) return st;
+  if ((st = report_list(pli, rep, hits, doms, lst, text)) != p7G_OK) return st;
   size_t n = 0;
   switch (which) {
   case 0: n = p7g_targets_text(&lst, buf, cap); break;

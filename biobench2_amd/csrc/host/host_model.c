@@ -163,6 +163,7 @@ p7_gpu_hmm_Destroy(P7_GPU_HMM *hmm)
 {
   if (! hmm) return;
   free(hmm->t); free(hmm->mat); free(hmm->ins);
+  free(hmm->cons); free(hmm->cs); free(hmm->rf);
   free(hmm);
 }
 
@@ -175,6 +176,7 @@ read_one_hmm(LINEREADER *lr, P7_GPU_HMM **ret)
   char *tag, *tok;
   const P7G_ALPHABET *abc = NULL;
   int   stats = 0;
+  int   has_cons = 0, has_cs = 0, has_rf = 0;
 
   *ret = NULL;
   if (! hmm) return p7G_EMEM;
@@ -193,6 +195,11 @@ read_one_hmm(LINEREADER *lr, P7_GPU_HMM **ret)
     if (strcmp(tag, "NAME") == 0) {
       if (! (tok = lr_token(lr))) goto FORMAT;
       snprintf(hmm->name, sizeof(hmm->name), "%s", tok);
+    } else if (strcmp(tag, "CONS") == 0 || strcmp(tag, "CS") == 0 || strcmp(tag, "RF") == 0) {
+      const int yes = (tok = lr_token(lr)) && strcasecmp(tok, "yes") == 0;
+      if      (tag[0] == 'C' && tag[1] == 'O') has_cons = yes;
+      else if (tag[0] == 'C')                  has_cs = yes;
+      else                                     has_rf = yes;
     } else if (strcmp(tag, "LENG") == 0) {
       if (! (tok = lr_token(lr)) || (hmm->M = atoi(tok)) <= 0) goto FORMAT;
     } else if (strcmp(tag, "MAXL") == 0) {
@@ -224,7 +231,15 @@ read_one_hmm(LINEREADER *lr, P7_GPU_HMM **ret)
   hmm->t   = calloc((size_t) (hmm->M+1) * 7, sizeof(float));
   hmm->mat = calloc((size_t) (hmm->M+1) * K, sizeof(float));
   hmm->ins = calloc((size_t) (hmm->M+1) * K, sizeof(float));
-  if (! hmm->t || ! hmm->mat || ! hmm->ins) { p7_gpu_hmm_Destroy(hmm); return p7G_EMEM; }
+  hmm->cons = calloc((size_t) hmm->M + 2, 1);
+  if (has_cs) hmm->cs = calloc((size_t) hmm->M + 2, 1);
+  if (has_rf) hmm->rf = calloc((size_t) hmm->M + 2, 1);
+  if (! hmm->t || ! hmm->mat || ! hmm->ins || ! hmm->cons || (has_cs && ! hmm->cs) || (has_rf && ! hmm->rf)) {
+    p7_gpu_hmm_Destroy(hmm); return p7G_EMEM;
+  }
+  hmm->cons[0] = ' ';
+  if (hmm->cs) hmm->cs[0] = ' ';
+  if (hmm->rf) hmm->rf[0] = ' ';
 
   if (lr_next_line(lr) != p7G_OK) goto FORMAT;            /* m->m m->i ... header */
   if (lr_next_line(lr) != p7G_OK) goto FORMAT;
@@ -246,7 +261,19 @@ read_one_hmm(LINEREADER *lr, P7_GPU_HMM **ret)
     if (lr_next_line(lr) != p7G_OK) goto FORMAT;
     if (! (tok = lr_token(lr)) || atoi(tok) != k) goto FORMAT;
     for (x = 0; x < K; x++) { if (! (tok = lr_token(lr))) goto FORMAT; hmm->mat[(size_t) k*K + x] = parse_prob(tok); }
-    for (x = 0; x < nfields; x++) if (! lr_token(lr)) goto FORMAT;
+    /* MAP, CONS, RF, [MM,] CS: one token each */
+    for (x = 0; x < nfields; x++) {
+      if (! (tok = lr_token(lr))) goto FORMAT;
+      if (x == 1 && has_cons) hmm->cons[k] = tok[0];
+      if (x == 2 && hmm->rf) hmm->rf[k] = tok[0];
+      if (x == nfields - 1 && hmm->cs) hmm->cs[k] = tok[0];
+    }
+    if (! has_cons) {                         /* p7_hmm_SetConsensus (p7_hmm.c:704-734) */
+      const float mthresh = (hmm->abctype == p7G_AMINO) ? 0.5f : 0.9f;
+      int best = 0;
+      for (x = 1; x < K; x++) if (hmm->mat[(size_t) k*K + x] > hmm->mat[(size_t) k*K + best]) best = x;
+      hmm->cons[k] = (char) ((hmm->mat[(size_t) k*K + best] >= mthresh) ? toupper(abc->sym[best]) : tolower(abc->sym[best]));
+    }
     if (lr_next_line(lr) != p7G_OK) goto FORMAT;
     for (x = 0; x < K; x++) { if (! (tok = lr_token(lr))) goto FORMAT; hmm->ins[(size_t) k*K + x] = parse_prob(tok); }
     if (lr_next_line(lr) != p7G_OK) goto FORMAT;
@@ -399,7 +426,9 @@ void
 p7_gpu_profile_Destroy(P7_GPU_PROFILE *gm)
 {
   if (! gm) return;
-  free(gm->tsc); free(gm->rsc); free(gm);
+  free(gm->tsc); free(gm->rsc);
+  free(gm->cons); free(gm->cs); free(gm->rf);
+  free(gm);
 }
 
 #define TSC(gm,k,s)  ((gm)->tsc[(size_t)(k) * P_NTRANS + (s)])
@@ -446,6 +475,10 @@ p7_gpu_ProfileConfig(const P7_GPU_HMM *hmm, const P7_GPU_BG *bg, int L, int mode
   for (x = 0; x < Kp; x++) { MSC(gm, 0, x) = -INFINITY; ISC(gm, 0, x) = -INFINITY; }
   for (k = 0; k <= M; k++) { MSC(gm, k, K) = ISC(gm, k, K) = -INFINITY; MSC(gm, k, Kp-1) = ISC(gm, k, Kp-1) = -INFINITY; }
   for (x = 0; x < 6; x++) gm->evparam[x] = hmm->evparam[x];
+  /* the annotation goes along (modelconfig.c:80-81) */
+  if (hmm->cons) gm->cons = strdup(hmm->cons);
+  if (hmm->cs)   gm->cs   = strdup(hmm->cs);
+  if (hmm->rf)   gm->rf   = strdup(hmm->rf);
   for (x = 0; x < P7G_MAXABET; x++) gm->compo[x] = hmm->has_compo ? hmm->compo[x] : -1.0f;
 
   /* entry scores */
@@ -642,6 +675,7 @@ p7_gpu_oprofile_Destroy(P7_GPU_OPROFILE *om)
   if (! om) return;
   if (om->dev) p7g_dev_profile_free(om->dev);
   free(om->rbv); free(om->rwv); free(om->twv); free(om->rfv); free(om->tfv);
+  free(om->cons); free(om->cs); free(om->rf);
   free(om);
 }
 
@@ -740,6 +774,10 @@ p7_gpu_oprofile_Convert(const P7_GPU_PROFILE *gm, P7_GPU_OPROFILE *om)
 
   for (z = 0; z < 6; z++) om->evparam[z] = gm->evparam[z];
   for (z = 0; z < P7G_MAXABET; z++) om->compo[z] = gm->compo[z];
+  free(om->cons); free(om->cs); free(om->rf);
+  om->cons = gm->cons ? strdup(gm->cons) : NULL;
+  om->cs   = gm->cs   ? strdup(gm->cs)   : NULL;
+  om->rf   = gm->rf   ? strdup(gm->rf)   : NULL;
   return p7G_OK;
 }
 

@@ -1,6 +1,7 @@
 /* The hit list as text: hmmsearch's per-sequence and per-domain tables
- * (p7_tophits_Targets and p7_tophits_Domains without the alignments,
- * HMMER/src/p7_tophits.c:1055-1380 with pli->show_alignments off) and its
+ * (p7_tophits_Targets and p7_tophits_Domains, HMMER/src/p7_tophits.c:
+ * 1055-1380, the domains' alignments drawn as p7_alidisplay_Print draws them,
+ * p7_alidisplay.c:572-641) and its
  * parsable tables (--tblout and --domtblout: p7_tophits_TabularTargets and
  * p7_tophits_TabularDomains, p7_tophits.c:1519-1700), for a search of one
  * profile against sequences (p7_SEARCH_SEQS, protein targets). The column
@@ -108,11 +109,57 @@ p7g_targets_text(const P7G_RLIST *l, char *buf, size_t cap)
   return sb_end(&s);
 }
 
+static int
+text_width(long v)
+{
+  char tmp[32];
+  return snprintf(tmp, sizeof(tmp), "%ld", v);
+}
+
+/* p7_alidisplay_Print for one domain: blocks of aliwidth columns (at least
+ * min_aliwidth when textw leaves less), each with the CS and RF lines, the
+ * model line with its first and last node, the match line, the target line
+ * with its first and last residue ('-' for a block of deletions only), and
+ * the posterior probability line. */
+static void
+print_alignment(SBUF *s, const P7G_RDOM *d, const char *hmmname, const char *seqname, int textw)
+{
+  const int min_aliwidth = 40;
+  const int namewidth = MAXI((int) strlen(hmmname), (int) strlen(seqname));
+  const int coordwidth = MAXI(MAXI(text_width(d->hmmfrom), text_width(d->hmmto)),
+                              MAXI(text_width(d->alifrom), text_width(d->alito)));
+  int aliwidth = (textw > 0) ? textw - namewidth - 2 * coordwidth - 5 : d->N;
+  if (aliwidth < d->N && aliwidth < min_aliwidth) aliwidth = min_aliwidth;
+  if (aliwidth < 1) aliwidth = 1;
+  long i1 = d->alifrom;
+  int k1 = d->hmmfrom;
+  for (int pos = 0; pos < d->N; pos += aliwidth) {
+    if (pos > 0) sb_printf(s, "\n");
+    const int w = MAXI(0, (d->N - pos < aliwidth) ? d->N - pos : aliwidth);
+    int ni = 0, nk = 0;
+    for (int z = pos; z < pos + w; z++) {
+      if (d->model[z] != '.') nk++;
+      if (d->aseq[z] != '-') ni++;
+    }
+    const int k2 = k1 + nk - 1;
+    const long i2 = i1 + ni - 1;
+    if (d->csline) sb_printf(s, "  %*s %.*s CS\n", namewidth + coordwidth + 1, "", w, d->csline + pos);
+    if (d->rfline) sb_printf(s, "  %*s %.*s RF\n", namewidth + coordwidth + 1, "", w, d->rfline + pos);
+    sb_printf(s, "  %*s %*d %.*s %-*d\n", namewidth, hmmname, coordwidth, k1, w, d->model + pos, coordwidth, k2);
+    sb_printf(s, "  %*s %.*s\n", namewidth + coordwidth + 1, " ", w, d->mline + pos);
+    if (ni > 0) sb_printf(s, "  %*s %*ld %.*s %-*ld\n", namewidth, seqname, coordwidth, i1, w, d->aseq + pos, coordwidth, i2);
+    else        sb_printf(s, "  %*s %*s %.*s %*s\n", namewidth, seqname, coordwidth, "-", w, d->aseq + pos, coordwidth, "-");
+    if (d->ppline) sb_printf(s, "  %*s %.*s PP\n", namewidth + coordwidth + 1, "", w, d->ppline + pos);
+    k1 += nk;
+    i1 += ni;
+  }
+}
+
 size_t
 p7g_domains_text(const P7G_RLIST *l, char *buf, size_t cap)
 {
   SBUF s = { buf, cap, 0 };
-  sb_printf(&s, "Domain annotation for each sequence:\n");
+  sb_printf(&s, "Domain annotation for each sequence%s:\n", l->show_alignments ? " (and alignments)" : "");
   for (int h = 0; h < l->N; h++) {
     const P7G_RHIT *t = &l->hit[h];
     if (! (t->flags & 1)) continue;
@@ -144,7 +191,22 @@ p7g_domains_text(const P7G_RLIST *l, char *buf, size_t cap)
                 (long) d->alifrom, (long) d->alito, d->alifrom == 1 ? '[' : '.', d->alito == t->L ? ']' : '.',
                 d->ienv, d->jenv, d->ienv == 1 ? '[' : '.', d->jenv == t->L ? ']' : '.', dom_acc(d));
     }
-    sb_printf(&s, "\n");
+    if (l->show_alignments) {
+      sb_printf(&s, "\n  Alignments for each domain:\n");
+      const char *hmmname = (l->show_accessions && l->qacc) ? l->qacc : l->qname;
+      nd = 0;
+      for (int k = 0; k < t->ndom; k++) {
+        const P7G_RDOM *d = &t->dom[k];
+        if (! (d->flags & 1)) continue;
+        nd++;
+        sb_printf(&s, "  == domain %d", nd);
+        sb_printf(&s, "  score: %.1f bits", d->bitscore);
+        sb_printf(&s, ";  conditional E-value: %.2g\n", exp(d->lnP) * l->domZ);
+        if (d->model) print_alignment(&s, d, hmmname, shown, l->textw);
+        sb_printf(&s, "\n");
+      }
+    } else
+      sb_printf(&s, "\n");
   }
   if (n_reported(l) == 0) sb_printf(&s, "\n   [No targets detected that satisfy reporting thresholds]\n");
   return sb_end(&s);

@@ -44,6 +44,10 @@ struct p7g_hmm_s {
   float   compo[P7G_MAXABET];
   float   evparam[6];
   int     has_compo, has_stats;
+  /* per-node annotation, [0] unused, NUL-terminated at M+1: the consensus
+   * residues (from the file's CONS column, else p7_hmm_SetConsensus's
+   * rule), and the CS and RF lines when the file has them (else NULL) */
+  char   *cons, *cs, *rf;
 };
 
 struct p7g_bg_s {
@@ -66,6 +70,7 @@ struct p7g_profile_s {
   float   xsc[4][2];  /* [E,N,J,C][LOOP,MOVE]                              */
   float   evparam[6];
   float   compo[P7G_MAXABET];
+  char   *cons, *cs, *rf;  /* the model's annotation, as P7_HMM holds it (or NULL) */
 };
 
 struct p7g_dev_profile_s;   /* device copy; defined in the HIP layer */
@@ -88,6 +93,7 @@ struct p7g_oprofile_s {
   float     xf[4][2];
   float     evparam[6];
   float     compo[P7G_MAXABET];
+  char     *cons, *cs, *rf;   /* the model's annotation for the alignment display (or NULL) */
   /* device side, built on first use, dropped when the tables change */
   struct p7g_dev_profile_s *dev;
   uint64_t  version;
@@ -148,6 +154,10 @@ typedef struct {
   float  oasc;
   int    hmmfrom, hmmto, alifrom, alito, ienv, jenv;
   int    flags;                  /* bit 0 reported, bit 1 included */
+  /* the alignment display's lines (p7_alidisplay_Create), N columns each,
+   * or NULL without one; cs and rf NULL when the model has no such line */
+  const char *model, *mline, *aseq, *ppline, *csline, *rfline;
+  int    N;
 } P7G_RDOM;
 typedef struct {
   const char *name, *acc, *desc; /* acc, desc NULL when empty */
@@ -163,7 +173,7 @@ typedef struct {
   const char *qname, *qacc;      /* qacc NULL when empty */
   int    M;
   double Z, domZ;
-  int    textw, show_accessions;
+  int    textw, show_accessions, show_alignments;
 } P7G_RLIST;
 size_t p7g_targets_text(const P7G_RLIST *l, char *buf, size_t cap);
 size_t p7g_domains_text(const P7G_RLIST *l, char *buf, size_t cap);

@@ -69,7 +69,7 @@ class Report(ctypes.Structure):
     _fields_ = [("qname", c_char_p), ("qacc", c_char_p), ("M", c_int),
                 ("tname", POINTER(c_char_p)), ("tacc", POINTER(c_char_p)), ("tdesc", POINTER(c_char_p)),
                 ("Z", c_double), ("domZ", c_double), ("E", c_double), ("incE", c_double), ("domE", c_double),
-                ("incdomE", c_double), ("textw", c_int), ("show_accessions", c_int)]
+                ("incdomE", c_double), ("textw", c_int), ("show_accessions", c_int), ("show_alignments", c_int)]
 
 
 def _load():
@@ -735,7 +735,8 @@ class Pipeline:
         return o
 
     def report_text(self, kind, qname, M, names, accs=None, descs=None, qacc=None, Z=None, domZ=-1.0, E=10.0,
-                    incE=0.01, domE=10.0, incdomE=0.01, textw=120, show_accessions=False, show_header=True):
+                    incE=0.01, domE=10.0, incdomE=0.01, textw=120, show_accessions=False, show_header=True,
+                    show_alignments=False):
         """The last run's hit list as hmmsearch writes it: kind "targets" (p7_tophits_Targets),
         "domains" (p7_tophits_Domains without alignments), "tblout" or "domtblout". names (and
         accs, descs: None or lists with None / "" for none) are indexed by database sequence;
@@ -747,7 +748,7 @@ class Pipeline:
                    ctypes.cast(ta, POINTER(c_char_p)) if ta is not None else None,
                    ctypes.cast(td, POINTER(c_char_p)) if td is not None else None,
                    float(len(names) if Z is None else Z), float(domZ), E, incE, domE, incdomE, int(textw),
-                   int(bool(show_accessions)))
+                   int(bool(show_accessions)), int(bool(show_alignments)))
         fn = {"targets": lambda b, c, n: lib.p7_gpu_pipeline_TargetsText(self.h, ctypes.byref(r), b, c, n),
               "domains": lambda b, c, n: lib.p7_gpu_pipeline_DomainsText(self.h, ctypes.byref(r), b, c, n),
               "tblout": lambda b, c, n: lib.p7_gpu_pipeline_TabularTargets(self.h, ctypes.byref(r), int(show_header),

@@ -404,7 +404,8 @@ int   p7_gpu_pipeline_GetTargetCounts(const P7_GPU_PIPELINE *pli, float *nexpect
  * E-value thresholds (hmmsearch: Z = the number of targets searched unless
  * --Z, domZ < 0 for the number of reported targets unless --domZ, E = 10,
  * incE = 0.01, domE = 10, incdomE = 0.01); --textw (120; 0 for --notextw)
- * and --acc. */
+ * and --acc; show_alignments draws each reported domain's alignment in the
+ * domain section (hmmsearch's default; 0 is --noali). */
 typedef struct p7g_report_s {
   const char        *qname;
   const char        *qacc;
@@ -416,12 +417,18 @@ typedef struct p7g_report_s {
   double             E, incE, domE, incdomE;
   int                textw;
   int                show_accessions;
+  int                show_alignments;   /* DomainsText: each domain's alignment (0: --noali) */
 } P7_GPU_REPORT;
 /* p7_tophits_Targets (p7_tophits.c:1055-1180): "Scores for complete
  * sequences", one line per reported target in rank order, the inclusion
- * threshold line, or the no-hits line. p7_tophits_Domains (:1200-1380)
- * without the alignments (hmmsearch --noali): per reported target its name
- * and description line and its reported domains' table. The tabular forms,
+ * threshold line, or the no-hits line. p7_tophits_Domains (:1200-1380): per
+ * reported target its name and description line, its reported domains'
+ * table and, with show_alignments, each reported domain's optimal accuracy
+ * alignment as p7_alidisplay_Print draws it (p7_alidisplay.c:59-210,
+ * 572-641: CS and RF lines when the model has them, the consensus, match,
+ * target and posterior probability lines, in blocks as wide as textw
+ * allows; the consensus and CS/RF lines are the annotation the profile's
+ * .hmm file gave, an 'x' consensus for profiles loaded from tables). The tabular forms,
  * --tblout (p7_tophits_TabularTargets, :1519-1590) and --domtblout
  * (p7_tophits_TabularDomains, :1610-1700), with their three header lines when
  * show_header is set (hmmsearch sets it for the first query). Each writes at

@@ -36,6 +36,6 @@ int main(void)
   F(P7_GPU_REPORT, qname); F(P7_GPU_REPORT, qacc); F(P7_GPU_REPORT, M); F(P7_GPU_REPORT, tname);
   F(P7_GPU_REPORT, tacc); F(P7_GPU_REPORT, tdesc); F(P7_GPU_REPORT, Z); F(P7_GPU_REPORT, domZ);
   F(P7_GPU_REPORT, E); F(P7_GPU_REPORT, incE); F(P7_GPU_REPORT, domE); F(P7_GPU_REPORT, incdomE);
-  F(P7_GPU_REPORT, textw); F(P7_GPU_REPORT, show_accessions);
+  F(P7_GPU_REPORT, textw); F(P7_GPU_REPORT, show_accessions); F(P7_GPU_REPORT, show_alignments);
   return 0;
 }

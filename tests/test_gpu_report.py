@@ -35,7 +35,7 @@ def fn3_run(gpu):
     om = gpu.OProfile.from_file(golden("fn3.hmm"))
     pli = gpu.Pipeline(domains=True)
     pli.run(om, db)
-    del db                      # the run's results do not read the database after the call
+    del db, om                  # the run's results read neither the database nor the profile after the call
     gc.collect()
     return pli, names, accs, descs
 
@@ -64,6 +64,24 @@ def test_fn3_target_and_domain_tables_are_the_tutorials(gpu):
     # --notextw: the full description
     tw = pli.report_text("targets", "fn3", 86, names, accs, descs, textw=0, **kw).split("\n")
     assert tw[4].endswith("EC=2.7.10.1;") and tw[4].startswith(t[4].rstrip())
+
+
+def test_fn3_domain_alignments_are_the_tutorials(gpu):
+    """hmmsearch's domain section with its alignments (p7_alidisplay_Create
+    and p7_alidisplay_Print): for each of fn3's nine domains in 7LESS_DROME
+    the CS, consensus, match, target and posterior probability lines equal
+    tutorial/fn3.out's byte for byte, blocks wrapped at --textw 120 as there
+    (domain 9 takes two)."""
+    pli, names, accs, descs = fn3_run(gpu)
+    ref = tutorial_lines()
+    d = pli.report_text("domains", "fn3", 86, names, accs, descs, qacc="PF00041.13", Z=1.0,
+                        show_alignments=True).split("\n")
+    b = ref.index("Domain annotation for each sequence (and alignments):")
+    e = ref.index("Internal pipeline statistics summary:")
+    want = ref[b:e - 2] + [""]
+    for k, (x, y) in enumerate(zip(d, want)):
+        assert x == y, (k, x, y)
+    assert len(d) == len(want), (len(d), len(want))
 
 
 def test_fn3_tabular_outputs_carry_the_tables_numbers(gpu):
