@@ -2531,6 +2531,27 @@ batch_reserve(Batch &b, int nprof, int64_t n, unsigned mask)
   return p7G_OK;
 }
 
+// Sets aside what p7_gpu_PipelineBatch needs for batches of up to nprof
+// profiles over this database, with every output: the chains' streams and
+// work areas and the batch's arrays (the server does this for its slots, so
+// that no query waits for their allocation).
+extern "C" int
+p7g_pipeline_ReserveBatch(P7_GPU_PIPELINE *pli, const P7_GPU_SEQDB *db, int nprof)
+{
+  std::lock_guard<std::mutex> lk(pli->mu);
+  int status;
+  if ((status = ensure_init()) != p7G_OK) return status;
+  if (! pli->b) { pli->b = new (std::nothrow) Batch(); if (! pli->b) return p7G_EMEM; }
+  Batch &b = *pli->b;
+  const int nchain = std::min(nprof, P7G_BATCH_CHAINS);
+  for (int k = 0; k < nchain; k++) {
+    Work &w = b.ch[k];
+    if (! w.st) HIPCHK(hipStreamCreateWithFlags(&w.st, hipStreamNonBlocking));
+    if ((status = work_reserve(w, db->nseq, pipeline_pool_floats(db->nseq, db->nres))) != p7G_OK) return status;
+  }
+  return batch_reserve(b, nprof, db->nseq, 0x7fu);
+}
+
 // One profile row's work after its shape's ungapped pass, on its chain's
 // stream: the stage chain, its counters and the outputs asked for.
 static int

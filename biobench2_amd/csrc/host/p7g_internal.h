@@ -191,6 +191,7 @@ int    p7g_device_free_bytes(size_t *ret_bytes);
 size_t p7g_pipeline_work_bytes(int64_t nseq, int64_t nres);
 /* Creates the pipeline's stream and allocates its work area for `db` now. */
 int    p7g_pipeline_Reserve(P7_GPU_PIPELINE *pli, const P7_GPU_SEQDB *db);
+int    p7g_pipeline_ReserveBatch(P7_GPU_PIPELINE *pli, const P7_GPU_SEQDB *db, int nprof);
 void   p7g_pipeline_ResetStats(P7_GPU_PIPELINE *pli);
 
 #ifdef __cplusplus

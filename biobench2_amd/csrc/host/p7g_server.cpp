@@ -4,11 +4,12 @@
 // The reference's resident-database search (hmmpgmd: HMMER/src/hmmdmstr.c:701-790,
 // hmmdwrkr.c:184-644) pops one query at a time and gives it the whole machine.
 // Here a fixed set of slots, each a P7_GPU_PIPELINE with its own HIP streams
-// and device work areas, takes queries from a FIFO as slots free up. A slot
-// that finds several queries waiting takes up to `coalesce` of them at once
-// and runs them as one p7_gpu_PipelineBatch call: one persistent ungapped-pass
-// launch per kernel shape for all of them, then their later stages on
-// concurrent streams; with one query waiting it runs p7_gpu_Pipeline. Memory
+// and device work areas, takes queries from a FIFO as slots free up. With
+// coalescing on (p7_gpu_server_SetCoalesce), a slot that finds several
+// queries waiting takes up to `coalesce` of them at once and runs them as one
+// p7_gpu_PipelineBatch call: one persistent ungapped-pass launch per kernel
+// shape for all of them, then their later stages on concurrent streams;
+// otherwise, and with one query waiting, it runs p7_gpu_Pipeline. Memory
 // admits the queries: the slots' work areas are set aside when the server is
 // made (the budget bounds their number), and each query in flight holds its
 // profile's device tables, which must fit in what the budget has left (a
@@ -62,7 +63,7 @@ struct p7g_server_s {
   std::map<int64_t, std::shared_ptr<Request>> live;
   int64_t next_ticket = 1;
   bool stop = false;
-  int coalesce = SERVER_COALESCE_MAX;
+  int coalesce = 1;
   size_t slot_bytes = 0;                     // device memory set aside per slot
   size_t query_budget = 0;                   // device memory for the profiles of queries in flight
   size_t in_flight_bytes = 0;
@@ -156,12 +157,13 @@ p7_gpu_server_Create(const P7_GPU_SEQDB *db, int nslots, size_t hbm_budget,
   if (! db || nslots < 1) { p7g_set_error("server needs a database and at least one slot"); return nullptr; }
   if (p7_gpu_Init(p7g_current_device()) != p7G_OK) return nullptr;
   // admission by device memory: a slot holds its pipeline's work area and, for
-  // a batch of up to SERVER_COALESCE_MAX queries, one chain work area and the
-  // caller-order outputs (stage and six scores per sequence) per query; the
-  // rest of the budget holds the profiles of the queries in flight
+  // a batch of up to SERVER_COALESCE_MAX queries, one chain work area, the
+  // ungapped pass's results and the caller-order outputs (stage and six scores
+  // per sequence) per query, all made here; the rest of the budget holds the
+  // profiles of the queries in flight
   const int64_t nseq = p7_gpu_seqdb_NSeq(db);
   const size_t work = p7g_pipeline_work_bytes(nseq, p7_gpu_seqdb_NRes(db));
-  const size_t per_slot = work * (1 + SERVER_COALESCE_MAX) + (size_t) SERVER_COALESCE_MAX * (size_t) nseq * 25;
+  const size_t per_slot = work * (1 + SERVER_COALESCE_MAX) + (size_t) SERVER_COALESCE_MAX * (size_t) nseq * 27;
   size_t avail = 0;
   if (p7g_device_free_bytes(&avail) != p7G_OK) return nullptr;
   size_t budget = (size_t) (0.9 * (double) avail);
@@ -207,6 +209,11 @@ p7_gpu_server_SetCoalesce(P7_GPU_SERVER *srv, int n)
     p7g_set_error("server_SetCoalesce: n must be 1..%d", SERVER_COALESCE_MAX);
     return p7G_EINVAL;
   }
+  // the slots' batch work areas (set aside in the budget) are made now, so
+  // that no query waits for them; each slot is taken when it is idle
+  if (n > 1)
+    for (P7_GPU_PIPELINE *pli : srv->slots)
+      if (p7g_pipeline_ReserveBatch(pli, srv->db, SERVER_COALESCE_MAX) != p7G_OK) return p7G_EMEM;
   std::lock_guard<std::mutex> lk(srv->mu);
   srv->coalesce = n;
   return p7G_OK;

@@ -511,9 +511,10 @@ int   p7_gpu_oprofile_DeviceBytes(const P7_GPU_OPROFILE *om, const P7_GPU_SEQDB 
  * Stands in for hmmpgmd's search loop (HMMER/src/hmmdmstr.c:701-790,
  * hmmdwrkr.c:535-644), which serves one query at a time: here up to `nslots`
  * slots (pipelines with their streams and device work areas) take queries
- * from a FIFO as they free up, and a slot that finds several queries waiting
- * takes up to four of them as one p7_gpu_PipelineBatch (their ungapped passes
- * in one persistent launch per kernel shape). The slots' work areas, for
+ * from a FIFO as they free up; with p7_gpu_server_SetCoalesce, a slot that
+ * finds several queries waiting takes up to four of them as one
+ * p7_gpu_PipelineBatch (their ungapped passes in one persistent launch per
+ * kernel shape). The slots' work areas, for
  * batches of four, are set aside within `hbm_budget` bytes of device memory
  * (0: 90% of the free memory), which cuts the slot count; what is left admits
  * the queries in flight by their profiles' device tables (a query waits until
@@ -524,8 +525,8 @@ typedef struct p7g_server_s P7_GPU_SERVER;
 P7_GPU_SERVER *p7_gpu_server_Create(const P7_GPU_SEQDB *db, int nslots, size_t hbm_budget,
                                     double F1, double F2, double F3, int do_biasfilter);
 int   p7_gpu_server_Slots(const P7_GPU_SERVER *srv);
-/* Queries one slot may take as one batch: 1 (each query alone) to 4 (the
- * default). */
+/* Queries one slot may take as one batch: 1 (each query alone, the default)
+ * to 4; turning it on makes the slots' batch work areas. */
 int   p7_gpu_server_SetCoalesce(P7_GPU_SERVER *srv, int n);
 /* The device memory set aside per slot, and what the budget leaves for the
  * profiles of the queries in flight. */
