@@ -162,10 +162,12 @@ __host__ __device__ inline int64_t n2_oa_bits_at(int L, int B) { return n2_xrows
 __host__ __device__ inline int64_t n2_oa_rows_at(int L, int B) { return n2_oa_bits_at(L, B) + (int64_t) L * 64 * n2_oa_words(B); }
 // then, for the alignment display, each cell's posterior probability codes
 // (p7_alidisplay_EncodePostProb: 0-9, 10 for '*'), M in the low and I in the
-// high four bits of a byte, bytes [row][j][64 lanes]
+// high four bits of a byte, four positions per word (position j in byte j % 4
+// of word j / 4), words [row][j / 4][64 lanes]
+__host__ __device__ inline int n2_pp_words(int B) { return (B + 3) / 4; }
 __host__ __device__ inline int64_t n2_pp_at(int L, int B) { return n2_oa_rows_at(L, B) + (((int64_t) L + 1 + 63) & ~(int64_t) 63); }
 __host__ __device__ inline int64_t n2_area_floats(int L, int B)
-{ return n2_pp_at(L, B) + (((int64_t) L * 16 * B + 63) & ~(int64_t) 63); }
+{ return n2_pp_at(L, B) + (int64_t) L * 64 * n2_pp_words(B); }
 // optimal accuracy traceback choices: per cell, bits 0-1 the state before M
 // (0 M, 1 I, 2 D, 3 B), bit 2 before I (1: I), bit 3 before D (1: D); per
 // row, bits 0-15 the node E leaves from, bit 16 set if from D, bit 17 C from

@@ -1041,7 +1041,11 @@ __device__ __forceinline__ int pp_code(float p)
 // optacc.c:60-175) inside the null2 Forward pass, from the row's posteriors,
 // with the traceback's choices (select_m/d/i/e/c/j/b, optacc.c:288-423) made
 // here from the same values and stored for k_oa_trace: four bits per cell
-// (lane-minor words), one word per row. As in the SSE code, a predecessor
+// (lane-minor words), one word per row. Backward's M and I values of the row
+// come in registers (pbM, pbI: PRE, loaded by the caller before the row's
+// Forward part) or are loaded here from cr, eight positions at a time, so
+// that the row waits on memory once per eight positions, not once per cell.
+// As in the SSE code, a predecessor
 // behind a transition of probability 0 contributes +0.0 to the fill's maximum
 // and -infinity to the traceback's choice; D(1) is -infinity; the reference's
 // striped padding cells (model positions M+1..4Q) hold 0.0 and take part in
@@ -1049,12 +1053,13 @@ __device__ __forceinline__ int pp_code(float p)
 // (k = r*Q + q + 1, q outer), an M replacing on >= and a D on >. Values are
 // -infinity or >= 0, so their bit patterns order as integers. The profile is
 // in unihit mode (E move 1, E loop 0; N, J, C loop floop and move fmove).
-template <int B, int NW, bool PIN>
+template <int B, int NW, bool PIN, bool PRE>
 __device__ __forceinline__ void oa_row(const DevProfile &p, int z, int i,
                                        const float (&tBM)[B], const float (&tMM)[B], const float (&tIM)[B],
                                        const float (&tDM)[B], const float (&tMDS)[B], const float (&tMI)[B],
                                        const float (&tII)[B], const float *trr, size_t rowf,
-                                       const float (&fM)[B], const float (&fI)[B], const float *cr, float totr,
+                                       const float (&fM)[B], const float (&fI)[B], const float *cr,
+                                       const float *pbM, const float *pbI, float totr,
                                        float (&accM)[B], float (&accI)[B], float ppN, float ppJ, float ppC,
                                        float floop, float fmove, float (&oaM)[B], float (&oaI)[B], float (&oaD)[B],
                                        float &oaN, float &oaJ, float &oaB, float &oaC, uint32_t *obits, uint32_t *orow,
@@ -1067,6 +1072,10 @@ __device__ __forceinline__ void oa_row(const DevProfile &p, int z, int i,
   uint32_t nib[NW];
 #pragma unroll
   for (int w = 0; w < NW; w++) nib[w] = 0u;
+  constexpr int CH = 8;
+  float cbM[CH], cbI[CH];
+  uint32_t ppw = 0u;
+  uint32_t *opw = (uint32_t *) opp + (size_t) (i - 1) * n2_pp_words(B) * 64 + z;
   // row i-1's values at the lane's first position's predecessor (lane z-1's last)
   const float mpL = dppf<WAVE_SHR1>(NI, oaM[B-1]), ipL = dppf<WAVE_SHR1>(NI, oaI[B-1]), dpL = dppf<WAVE_SHR1>(NI, oaD[B-1]);
   // M and I in place, last position first: position j reads row i-1 at j-1 and j
@@ -1074,13 +1083,21 @@ __device__ __forceinline__ void oa_row(const DevProfile &p, int z, int i,
   for (int j = B - 1; j >= 0; j--) {
     const int k = z * B + j + 1;
     const bool first = (k == 1);
+    const int c8 = (B - 1 - j) % CH;
+    if (! PRE && c8 == 0) {
+#pragma unroll
+      for (int u = 0; u < CH; u++)
+        if (j - u >= 0) { cbM[u] = cr[(j - u) * 64]; cbI[u] = cr[rowf + (j - u) * 64]; }
+    }
+    const float bm = PRE ? pbM[j] : cbM[c8], bi = PRE ? pbI[j] : cbI[c8];
     // the posteriors of row i: (f * b) * (scaleproduct * scale_i), summed for null2
-    const float pm = __fmul_rn(__fmul_rn(fM[j], cr[j*64]), totr);
-    const float pi = __fmul_rn(__fmul_rn(fI[j], cr[rowf + j*64]), totr);
+    const float pm = __fmul_rn(__fmul_rn(fM[j], bm), totr);
+    const float pi = __fmul_rn(__fmul_rn(fI[j], bi), totr);
     accM[j] = __fadd_rn(pm, accM[j]);
     accI[j] = __fadd_rn(pi, accI[j]);
     // the cell's posterior probability codes for the alignment display
-    opp[((size_t) (i - 1) * B + j) * 64 + z] = (uint8_t) (pp_code(pm) | (pp_code(pi) << 4));
+    ppw |= (uint32_t) (pp_code(pm) | (pp_code(pi) << 4)) << ((j % 4) * 8);
+    if (j % 4 == 0) { opw[(j / 4) * 64] = ppw; ppw = 0u; }
     const float mp = j ? oaM[j-1] : mpL, ip = j ? oaI[j-1] : ipL, dp = j ? oaD[j-1] : dpL;
     const bool tb = OTR(tBM, VT_BM, j) > 0.0f, tm = OTR(tMM, VT_MM, j) > 0.0f;
     const bool ti = OTR(tIM, VT_IM, j) > 0.0f, td = OTR(tDM, VT_DM, j) > 0.0f;
@@ -1315,6 +1332,22 @@ void k_fwd(const DevProfile p, const uint8_t *__restrict__ res, const int64_t *_
       const float *e = tbl + (size_t) x * rowf + z;
       const float *trr = tr + (PIN ? 0 : opaque_zero());
       const float pxN = xN, pxJ = xJ, pxC = xC;          // row i-1
+      // null2 (MODE 3): Backward's specials and, for up to eight positions
+      // per lane, its M and I values of row i, loaded before the row's
+      // Forward part so that their latency overlaps it
+      constexpr bool PRE = (MODE == 3 && B <= 8);
+      float bx1 = 0.0f, bx2 = 0.0f, bx4 = 0.0f, bx5 = 1.0f;
+      float pbM[PRE ? B : 1], pbI[PRE ? B : 1];
+      if (MODE == 3) {
+        const float *xr = xb + (size_t) i * 6;
+        bx1 = xr[1]; bx2 = xr[2]; bx4 = xr[4]; bx5 = xr[5];
+        if (PRE) {
+          const float *cr = cells + (size_t) (i-1) * 2 * rowf;
+#pragma unroll
+          for (int j = 0; j < (PRE ? B : 1); j++) { pbM[j] = cr[j*64]; pbI[j] = cr[rowf + j*64]; }
+        }
+      }
+
 #define FTR(A, T, j) (PIN ? A[j] : trr[(T) * rowf + (j) * 64])
       const float iL = dppz<WAVE_SHR1>(I[B-1]), dL = dppz<WAVE_SHR1>(D[B-1]);
       float xEl = 0.0f;
@@ -1367,18 +1400,17 @@ void k_fwd(const DevProfile p, const uint8_t *__restrict__ res, const int64_t *_
         // from Forward's row i-1 and Backward's row i
         const float totr = sp * scale;
         const float *cr = cells + (size_t) (i-1) * 2 * rowf;
-        const float *xr = xb + (size_t) i * 6;
-        const float ppN = __fmul_rn(__fmul_rn(__fmul_rn(pxN, xr[1]), floop), sp);
-        const float ppJ = __fmul_rn(__fmul_rn(__fmul_rn(pxJ, xr[2]), floop), sp);
-        const float ppC = __fmul_rn(__fmul_rn(__fmul_rn(pxC, xr[4]), floop), sp);
+        const float ppN = __fmul_rn(__fmul_rn(__fmul_rn(pxN, bx1), floop), sp);
+        const float ppJ = __fmul_rn(__fmul_rn(__fmul_rn(pxJ, bx2), floop), sp);
+        const float ppC = __fmul_rn(__fmul_rn(__fmul_rn(pxC, bx4), floop), sp);
         pN = __fadd_rn(pN, ppN);
         pJ = __fadd_rn(pJ, ppJ);
         pC = __fadd_rn(pC, ppC);
-        if (own) sp = __fmul_rn(sp, __fdiv_rn(scale, xr[5]));
+        if (own) sp = __fmul_rn(sp, __fdiv_rn(scale, bx5));
         // the row's posteriors (summed for null2) and its optimal accuracy row
-        oa_row<B, NW, PIN>(p, z, i, tBM, tMM, tIM, tDM, tMDS, tMI, tII, tr + (PIN ? 0 : opaque_zero()), rowf,
-                           M, I, cr, totr, accM, accI, ppN, ppJ, ppC, floop, fmove, oaM, oaI, oaD, oaN, oaJ, oaB, oaC,
-                           obits, orow, opp);
+        oa_row<B, NW, PIN, PRE>(p, z, i, tBM, tMM, tIM, tDM, tMDS, tMI, tII, tr + (PIN ? 0 : opaque_zero()), rowf,
+                                M, I, cr, pbM, pbI, totr, accM, accI, ppN, ppJ, ppC, floop, fmove, oaM, oaI, oaD,
+                                oaN, oaJ, oaB, oaC, obits, orow, opp);
       }
     }
     if (MODE == 3) {
@@ -2222,7 +2254,7 @@ __global__ void k_oa_trace(const DevProfile p, const int32_t *__restrict__ list,
   const float *area = dom.n2pool + dom.n2off[slot];
   const uint32_t *bits = (const uint32_t *) (area + n2_oa_bits_at(L, B));
   const uint32_t *rows = (const uint32_t *) (area + n2_oa_rows_at(L, B));
-  const uint8_t *pp = (const uint8_t *) (area + n2_pp_at(L, B));
+  const uint32_t *pp = (const uint32_t *) (area + n2_pp_at(L, B));
   const uint8_t *sq = vres + voff[slot];               // residue i at sq[i-1]
   uint32_t *path = dom.n2path + dom.n2poff[slot];
   const int pcap = L + p.M + 2;
@@ -2230,9 +2262,10 @@ __global__ void k_oa_trace(const DevProfile p, const int32_t *__restrict__ list,
     const int z = (k - 1) / B, j = (k - 1) % B;
     return (bits[((size_t) (i - 1) * NW + j / 8) * 64 + z] >> ((j % 8) * 4)) & 15u;
   };
+  const int NP = n2_pp_words(B);
   auto ppat = [&](int i, int k) -> uint32_t {
     const int z = (k - 1) / B, j = (k - 1) % B;
-    return pp[((size_t) (i - 1) * B + j) * 64 + z];
+    return (pp[((size_t) (i - 1) * NP + j / 4) * 64 + z] >> ((j % 4) * 8)) & 0xffu;
   };
   int i = L, k = 0, s0 = S_C, nm = 0, ok = 1, nw = 0, last = 0;
   const int guard = 4 * (L + p.M) + 8;
