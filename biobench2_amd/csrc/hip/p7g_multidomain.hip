@@ -218,39 +218,85 @@ k_md_region(const DevProfile p, const uint8_t *__restrict__ res, const MdJob *__
       md_sync();
       float xe = 0.0f;
       if (z < 4) {                                         // lane z is SSE lane r
-        // the sums along q, eight cells' loads issued before their dependent chain
+        // The row's sums along q and its D->D passes, each a chain in q order
+        // (esl_sse_hsum_ps's lanes, fwdback.c's D passes): M(q) at cm[12 q],
+        // D(q) at cd[12 q], tDD(q) at tq[4 q]. Eight cells at a time, the next
+        // eight's loads issued before this eight's chain. The M sum and the
+        // first pass are independent chains and run together; every later
+        // pass also sums its new D values onto the M sum, so that the last
+        // pass leaves the row's total (its D values are the final ones).
         const int r = z;
-#define MD_CHUNK(body) for (int q0 = 0; q0 < Q; q0 += 8) { \
-          float mv[8], tv8[8]; \
-          _Pragma("unroll") for (int u = 0; u < 8; u++) { const int qq = min(q0 + u, Q - 1); \
-            mv[u] = cur[(qq*3+SL)*4 + r]; tv8[u] = tdd[qq*4 + r]; } \
-          _Pragma("unroll") for (int u = 0; u < 8; u++) if (q0 + u < Q) { body } \
-          _Pragma("unroll") for (int u = 0; u < 8; u++) if (q0 + u < Q && WB) cur[((q0+u)*3+SL)*4 + r] = mv[u]; }
-#define SL 0
-#define WB false
-        MD_CHUNK(xe = __fadd_rn(xe, mv[u]);)
-#undef SL
-#undef WB
+        float *cm = cur + r, *cd = cur + 4 + r;
+        const float *tq = tdd + r;
         float dcv = __fmul_rn(cur[((Q-1)*3+0)*4 + r], tv(T_MD, min(r * Q + Q, M + 1)));
         float sh = __shfl(dcv, (z + 3) & 3, 4);
         dcv = r ? sh : 0.0f;
-#define SL 1
-#define WB true
-        MD_CHUNK(const float d = __fadd_rn(dcv, mv[u]); mv[u] = d; dcv = __fmul_rn(d, tv8[u]);)
+        // Reads past the row's last q stay inside the phase's LDS (the next
+        // buffers) and feed no result.
+        float xm = 0.0f;
+        {
+          float nm[8], nd[8], nt[8];
+#pragma unroll
+          for (int u = 0; u < 8; u++) { nm[u] = cm[12*u]; nd[u] = cd[12*u]; nt[u] = tq[4*u]; }
+          for (int q0 = 0; q0 < Q; q0 += 8) {
+            float am[8], ad[8], at[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++) { am[u] = nm[u]; ad[u] = nd[u]; at[u] = nt[u]; }
+            if (q0 + 8 < Q) {
+#pragma unroll
+              for (int u = 0; u < 8; u++) { nm[u] = cm[12*(q0+8+u)]; nd[u] = cd[12*(q0+8+u)]; nt[u] = tq[4*(q0+8+u)]; }
+            }
+            auto step = [&](int u) {
+              xm = __fadd_rn(xm, am[u]);
+              const float d = __fadd_rn(dcv, ad[u]);
+              ad[u] = d;
+              dcv = __fmul_rn(d, at[u]);
+              cd[12*(q0+u)] = d;
+            };
+            if (q0 + 8 <= Q) {
+#pragma unroll
+              for (int u = 0; u < 8; u++) step(u);
+            } else {
+#pragma unroll
+              for (int u = 0; u < 8; u++) if (q0 + u < Q) step(u);
+            }
+          }
+        }
         for (int pass = 1; pass < 4; pass++) {
           sh = __shfl(dcv, (z + 3) & 3, 4);
           dcv = r ? sh : 0.0f;
           int changed = 0;
-          MD_CHUNK(const float d = __fadd_rn(dcv, mv[u]); changed |= (d > mv[u]); mv[u] = d; dcv = __fmul_rn(dcv, tv8[u]);)
+          float xd = xm;
+          float nd[8], nt[8];
+#pragma unroll
+          for (int u = 0; u < 8; u++) { nd[u] = cd[12*u]; nt[u] = tq[4*u]; }
+          for (int q0 = 0; q0 < Q; q0 += 8) {
+            float ad[8], at[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++) { ad[u] = nd[u]; at[u] = nt[u]; }
+            if (q0 + 8 < Q) {
+#pragma unroll
+              for (int u = 0; u < 8; u++) { nd[u] = cd[12*(q0+8+u)]; nt[u] = tq[4*(q0+8+u)]; }
+            }
+            auto step = [&](int u) {
+              const float d = __fadd_rn(dcv, ad[u]);
+              changed |= (d > ad[u]);
+              dcv = __fmul_rn(dcv, at[u]);
+              xd = __fadd_rn(d, xd);
+              cd[12*(q0+u)] = d;
+            };
+            if (q0 + 8 <= Q) {
+#pragma unroll
+              for (int u = 0; u < 8; u++) step(u);
+            } else {
+#pragma unroll
+              for (int u = 0; u < 8; u++) if (q0 + u < Q) step(u);
+            }
+          }
+          xe = xd;
           const bool any = (__ballot(changed) & 0xfull) != 0ull;
           if (M >= 100 && ! any) break;
         }
-#undef WB
-#define WB false
-        MD_CHUNK(xe = __fadd_rn(mv[u], xe);)
-#undef SL
-#undef WB
-#undef MD_CHUNK
       }
       // esl_sse_hsum_ps: (e0 + e1) + (e2 + e3)
       xE = __fadd_rn(__fadd_rn(readlanef(xe, 0), readlanef(xe, 1)), __fadd_rn(readlanef(xe, 2), readlanef(xe, 3)));
