@@ -237,12 +237,16 @@ struct MdOut {
 };
 __host__ __device__ inline int64_t md_n2sc_at(int M, int Lr)
 { return (int64_t) (Lr + 1) * 12 * md_q(M) + 6 * (int64_t) (Lr + 1); }
-// then the endpoint counts, the sampled domains (5 ints each), and (doubles)
-// the traces' thresholds: 5 per cell, 3 per row, 8Q running sums per row
+// then the endpoint counts, the sampled domains (5 ints each), the
+// clustering's element clusters and cluster counts (segcap ints each), and
+// (doubles) the traces' thresholds: 5 per cell, 3 per row, 8Q running sums
+// per row
 __host__ __device__ inline int64_t md_seg_at(int M, int Lr)
 { return md_n2sc_at(M, Lr) + (Lr + 1) + ((Lr > M ? Lr : M) + 2); }
+__host__ __device__ inline int64_t md_clu_at(int M, int Lr, int segcap)
+{ return md_seg_at(M, Lr) + 5 * (int64_t) segcap; }
 __host__ __device__ inline int64_t md_thr_at(int M, int Lr, int segcap)
-{ return (md_seg_at(M, Lr) + 5 * (int64_t) segcap + 1) & ~(int64_t) 1; }
+{ return (md_clu_at(M, Lr, segcap) + 2 * (int64_t) segcap + 1) & ~(int64_t) 1; }
 size_t  md_lds_bytes(int M, int segcap);
 int64_t md_area_floats(int M, int Lr, int segcap);
 hipError_t launch_md_regions(const DevProfile &p, const uint8_t *res, const MdJob *jobs, MdOut *out, int njobs,

@@ -2532,6 +2532,9 @@ hipError_t launch_fwd(const DevProfile &p, const DevSeqs &s, const int32_t *list
     FB_CASES(FWD_CASE)
     default: return hipErrorInvalidValue;
   }
+  // the null2 pass in workgroups of four waves (one per SIMD), so that one of
+  // them fits on a compute unit beside a multidomain region's wave
+  if (dom.env_loop && dom.n2mode == 2) nthr = std::min(nthr, 256);
   const size_t lds = fb_lds(p, tm);
   const unsigned grid = grid_for_waves_blk(k, maxn, lds, nthr / 64);
   void *args[] = { (void *) &p, (void *) &s.res, (void *) &s.off, (void *) &s.len, (void *) &list, (void *) &count,

@@ -639,8 +639,11 @@ k_md_region(const DevProfile p, const uint8_t *__restrict__ res, const MdJob *__
   __syncthreads();
 
   // ---------------- 3. clustering ----------------
-  int *av = (int *) lds, *bv = av + J.segcap, *asg = bv + J.segcap, *ninc = asg + J.segcap;
-  unsigned long long *lk = (unsigned long long *) (ninc + J.segcap);   // link bits of one pass
+  // the two stacks of the linkage in LDS; each element's cluster and each
+  // cluster's trace count in the region's area
+  int *av = (int *) lds, *bv = av + J.segcap;
+  int *asg = (int *) (pool + J.area + md_clu_at(M, Lr, J.segcap)), *ninc = asg + J.segcap;
+  unsigned long long *lk = (unsigned long long *) (bv + J.segcap);   // link bits of one pass
   MdSig *sig = (MdSig *) (lk + (J.segcap + 63) / 64);
   const int n = nseg;
   for (int v = z; v < n; v += 64) { av[v] = n - v - 1; ninc[v] = 0; }
@@ -690,7 +693,7 @@ k_md_region(const DevProfile p, const uint8_t *__restrict__ res, const MdJob *__
   __syncthreads();
   int nsig = 0;
   for (int c = 0; c < ncl; c++) {
-    const int ni = ninc[c];
+    const int ni = __hip_atomic_load(&ninc[c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);   // counted at L2
     if (__fdiv_rn((float) ni, (float) nsamples) < 0.25f) continue;
     int lo[4], hi4[4];
     for (int e = 0; e < 4; e++) { lo[e] = 0x7fffffff; hi4[e] = -1; }
@@ -771,7 +774,7 @@ k_md_region(const DevProfile p, const uint8_t *__restrict__ res, const MdJob *__
 size_t md_lds_bytes(int M, int segcap)
 {
   const int64_t fwd = (md_stage_t(M) ? md_stage_floats(M) : 0) + 32 * md_q(M);
-  const size_t clu = (size_t) 4 * segcap * 4 + (size_t) (segcap + 63) / 64 * 8 + (size_t) MD_MAXSIG * sizeof(MdSig);
+  const size_t clu = (size_t) 2 * segcap * 4 + (size_t) (segcap + 63) / 64 * 8 + (size_t) MD_MAXSIG * sizeof(MdSig);
   return std::max((size_t) std::max(fwd, md_trace_floats(M)) * 4, clu);
 }
 

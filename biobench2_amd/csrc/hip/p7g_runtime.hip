@@ -72,6 +72,9 @@ struct Work {
   int64_t    *mdsrc = nullptr, *mddst = nullptr;
   int32_t    *mdlen = nullptr, *mdcnt = nullptr;
   int64_t     md_cap = 0;
+  float      *mdpool = nullptr;           // the multidomain regions' areas (grown on use)
+  int64_t     mdsize = 0;
+  hipStream_t st2 = nullptr;              // their stream, beside st (made on first use)
   void       *dom_mem = nullptr;
   int64_t     dom_cap = 0;          // slots the per-slot arrays hold
 };
@@ -986,8 +989,11 @@ work_release(Work &w)
   w.n2off = nullptr; w.n2dc = nullptr; w.n2st = nullptr; w.n2list = nullptr; w.n2cnt = nullptr;
   w.n2oa = nullptr; w.n2ali = nullptr; w.n2poff = nullptr; w.n2pn = nullptr; w.n2path = nullptr; w.n2path_cap = 0;
   w.n2pool = nullptr; w.n2size = 0;
+  if (w.st2) (void) hipStreamSynchronize(w.st2);
   if (w.mdjobs) (void) hipFree(w.mdjobs);
   w.mdjobs = nullptr; w.mdout = nullptr; w.mdsrc = w.mddst = nullptr; w.mdlen = w.mdcnt = nullptr; w.md_cap = 0;
+  if (w.mdpool) (void) hipFree(w.mdpool);
+  w.mdpool = nullptr; w.mdsize = 0;
   if (w.mem) (void) hipFree(w.mem);
   if (w.pool) (void) hipFree(w.pool);
   if (w.bnd) (void) hipFree(w.bnd);
@@ -1308,6 +1314,7 @@ p7_gpu_pipeline_Destroy(P7_GPU_PIPELINE *pli)
   batch_free(pli->b);
   work_release(pli->w);
   if (pli->w.st) (void) hipStreamDestroy(pli->w.st);
+  if (pli->w.st2) (void) hipStreamDestroy(pli->w.st2);
   delete pli;
 }
 
@@ -1442,7 +1449,7 @@ stats_accumulate(P7_GPU_STATS &tot, const P7_GPU_STATS &one, bool first_of_call)
 }
 
 // Multidomain regions (p7_domaindef.c:431-476): each region's sampled traces
-// and clusters (k_md_region), in batches whose areas fit the null2 pool.
+// and clusters (k_md_region), in batches whose areas fit the multidomain pool.
 // Every cluster becomes a record of its own after the nregs regions (w = 2):
 // its residues copied from the region's copy, its envelope score computed as
 // the regions' were; its correction is the sum of the region's sampled null2
@@ -1450,33 +1457,45 @@ stats_accumulate(P7_GPU_STATS &tot, const P7_GPU_STATS &one, bool first_of_call)
 // p7_domaindef.c:941-943), returned in env_dc, record by record. The region's
 // record keeps the sum over all its residues in reg_sum (the sequence bias
 // counts every residue, p7_pipeline.c:760).
+//
+// md_start launches the regions on the work area's second stream when one
+// batch holds them all, so that they run while the null2 pass of the
+// single-domain regions runs on the first (their workgroups fit on a compute
+// unit beside a region's wave); md_finish waits for them (or runs every
+// batch when there are several) and makes the clusters' records.
 #define MD_NSAMPLES 200        // ddef->nsamples, p7_domaindef.c:112
 #define MD_SEED     42u        // the pipeline's default --seed, p7_pipeline.c:102
 #define MD_SEGCAP   6144       // sampled domains kept per region
-static int
-multidomain_regions(Work &w, const DevProfile &p, const DevSeqs &s, std::vector<int4> &rec, std::vector<int32_t> &vlen,
-                    int64_t nregs, const int32_t *h_len, const GateThresh &g, std::vector<float> &env_dc,
-                    std::vector<std::pair<int64_t, float>> &reg_sum)
-{
-  std::vector<int64_t> mdk;
-  for (int64_t k = 0; k < nregs; k++) if (rec[(size_t) k].w == 1 && vlen[(size_t) k] > 0) mdk.push_back(k);
-  if (mdk.empty()) return p7G_OK;
-  if (md_lds_bytes(p.M, MD_SEGCAP) > P7G_LDS_MAX) { p7g_set_error("model too long for multidomain regions (M=%d)", p.M); return p7G_EINVAL; }
-  std::vector<int64_t> voff((size_t) nregs);
-  std::vector<float> eloop((size_t) nregs), emove((size_t) nregs);
+struct MdRun {
+  std::vector<int64_t> mdk;            // the multidomain regions' records
+  std::vector<int64_t> voff;
+  std::vector<float> eloop, emove;
   unsigned long long rcur = 0;
-  HIPCHK(hipMemcpyAsync(voff.data(), w.dout.v_off, 8 * (size_t) nregs, hipMemcpyDeviceToHost, w.st));
-  HIPCHK(hipMemcpyAsync(eloop.data(), w.dout.env_loop, 4 * (size_t) nregs, hipMemcpyDeviceToHost, w.st));
-  HIPCHK(hipMemcpyAsync(emove.data(), w.dout.env_move, 4 * (size_t) nregs, hipMemcpyDeviceToHost, w.st));
-  HIPCHK(hipMemcpyAsync(&rcur, w.l.cursor + 2, 8, hipMemcpyDeviceToHost, w.st));
+  std::vector<MdJob> jobs;
+  std::vector<std::pair<size_t, size_t>> batches;
+  bool launched = false;               // the one batch is on w.st2
+};
+
+static int
+md_start(Work &w, const DevProfile &p, const std::vector<int4> &rec, const std::vector<int32_t> &vlen, int64_t nregs,
+         const int32_t *h_len, MdRun &m)
+{
+  for (int64_t k = 0; k < nregs; k++) if (rec[(size_t) k].w == 1 && vlen[(size_t) k] > 0) m.mdk.push_back(k);
+  if (m.mdk.empty()) return p7G_OK;
+  if (md_lds_bytes(p.M, MD_SEGCAP) > P7G_LDS_MAX) { p7g_set_error("model too long for multidomain regions (M=%d)", p.M); return p7G_EINVAL; }
+  m.voff.resize((size_t) nregs); m.eloop.resize((size_t) nregs); m.emove.resize((size_t) nregs);
+  HIPCHK(hipMemcpyAsync(m.voff.data(), w.dout.v_off, 8 * (size_t) nregs, hipMemcpyDeviceToHost, w.st));
+  HIPCHK(hipMemcpyAsync(m.eloop.data(), w.dout.env_loop, 4 * (size_t) nregs, hipMemcpyDeviceToHost, w.st));
+  HIPCHK(hipMemcpyAsync(m.emove.data(), w.dout.env_move, 4 * (size_t) nregs, hipMemcpyDeviceToHost, w.st));
+  HIPCHK(hipMemcpyAsync(&m.rcur, w.l.cursor + 2, 8, hipMemcpyDeviceToHost, w.st));
   HIPCHK(hipStreamSynchronize(w.st));
-  const int64_t nmd = (int64_t) mdk.size();
+  const int64_t nmd = (int64_t) m.mdk.size();
   if (w.md_cap < nmd) {
     if (w.mdjobs) { (void) hipFree(w.mdjobs); w.mdjobs = nullptr; w.md_cap = 0; }
     const int64_t c = std::max<int64_t>(nmd, 256), ce = c * MD_MAXC;
-    char *m = nullptr;
-    HIPCHK(hipMalloc((void **) &m, (size_t) c * (sizeof(MdJob) + sizeof(MdOut)) + (size_t) ce * 20 + 256 * 4));
-    w.mdjobs = (MdJob *) m; w.mdout = (MdOut *) (w.mdjobs + c);
+    char *mem = nullptr;
+    HIPCHK(hipMalloc((void **) &mem, (size_t) c * (sizeof(MdJob) + sizeof(MdOut)) + (size_t) ce * 20 + 256 * 4));
+    w.mdjobs = (MdJob *) mem; w.mdout = (MdOut *) (w.mdjobs + c);
     w.mdsrc = (int64_t *) (w.mdout + c); w.mddst = w.mdsrc + ce;
     w.mdlen = (int32_t *) (w.mddst + ce); w.mdcnt = w.mdlen + ce;
     w.md_cap = c;
@@ -1485,52 +1504,71 @@ multidomain_regions(Work &w, const DevProfile &p, const DevSeqs &s, std::vector<
   size_t fr = 0, tot = 0;
   HIPCHK(hipMemGetInfo(&fr, &tot));
   int64_t need_all = 0;
-  for (int64_t k : mdk) need_all += md_area_floats(p.M, vlen[(size_t) k], MD_SEGCAP);
-  const int64_t cap = std::max<int64_t>(N2_POOL_MIN, std::min<int64_t>(need_all, (int64_t) (fr / 4 / sizeof(float)) + w.n2size));
-  std::vector<MdJob> jobs;
-  std::vector<std::pair<size_t, size_t>> batches;
+  for (int64_t k : m.mdk) need_all += md_area_floats(p.M, vlen[(size_t) k], MD_SEGCAP);
+  const int64_t cap = std::max<int64_t>(N2_POOL_MIN, std::min<int64_t>(need_all, (int64_t) (fr / 4 / sizeof(float)) + w.mdsize));
   int64_t used = 0, maxused = 0;
   size_t start = 0;
-  for (int64_t k : mdk) {
+  for (int64_t k : m.mdk) {
     const int Lr = vlen[(size_t) k];
     const int64_t need = md_area_floats(p.M, Lr, MD_SEGCAP);
     if (need > cap) { p7g_set_error("multidomain region of %d residues too long for the pool", Lr); return p7G_EMEM; }
-    if (used + need > cap || (int64_t) (jobs.size() - start) >= w.md_cap) { batches.push_back({ start, jobs.size() }); start = jobs.size(); used = 0; }
+    if (used + need > cap || (int64_t) (m.jobs.size() - start) >= w.md_cap) { m.batches.push_back({ start, m.jobs.size() }); start = m.jobs.size(); used = 0; }
     MdJob J;
-    J.res_off = voff[(size_t) k]; J.area = used; J.Lr = Lr; J.Ltarget = h_len[rec[(size_t) k].x]; J.segcap = MD_SEGCAP; J.pad = 0;
-    jobs.push_back(J);
+    J.res_off = m.voff[(size_t) k]; J.area = used; J.Lr = Lr; J.Ltarget = h_len[rec[(size_t) k].x]; J.segcap = MD_SEGCAP; J.pad = 0;
+    m.jobs.push_back(J);
     used += need; maxused = std::max(maxused, used);
   }
-  if (jobs.size() > start) batches.push_back({ start, jobs.size() });
-  if (w.n2size < maxused) {
-    if (w.n2pool) { (void) hipFree(w.n2pool); w.n2pool = nullptr; w.n2size = 0; }
-    HIPCHK(hipMalloc((void **) &w.n2pool, sizeof(float) * (size_t) maxused));
-    w.n2size = maxused;
+  if (m.jobs.size() > start) m.batches.push_back({ start, m.jobs.size() });
+  if (w.mdsize < maxused) {
+    if (w.mdpool) { (void) hipFree(w.mdpool); w.mdpool = nullptr; w.mdsize = 0; }
+    HIPCHK(hipMalloc((void **) &w.mdpool, sizeof(float) * (size_t) maxused));
+    w.mdsize = maxused;
   }
-  std::vector<MdOut> outs(jobs.size());
+  if (m.batches.size() == 1) {
+    if (! w.st2) HIPCHK(hipStreamCreateWithFlags(&w.st2, hipStreamNonBlocking));
+    const int nb = (int) m.jobs.size();
+    HIPCHK(hipMemcpyAsync(w.mdjobs, m.jobs.data(), sizeof(MdJob) * (size_t) nb, hipMemcpyHostToDevice, w.st2));
+    HIPCHK(launch_md_regions(p, w.dout.res, w.mdjobs, w.mdout, nb, w.mdpool, MD_SEGCAP, MD_NSAMPLES, MD_SEED, w.st2));
+    m.launched = true;
+  }
+  return p7G_OK;
+}
+
+static int
+md_finish(Work &w, const DevProfile &p, const DevSeqs &s, std::vector<int4> &rec, std::vector<int32_t> &vlen,
+          int64_t nregs, const GateThresh &g, std::vector<float> &env_dc,
+          std::vector<std::pair<int64_t, float>> &reg_sum, MdRun &m)
+{
+  if (m.mdk.empty()) return p7G_OK;
+  hipStream_t ms = m.launched ? w.st2 : w.st;
+  std::vector<float> &eloop = m.eloop, &emove = m.emove;
+  std::vector<MdOut> outs(m.jobs.size());
   std::vector<float> n2;
   std::vector<int64_t> src, dst;
   std::vector<int32_t> len;
-  for (const auto &b : batches) {
+  for (size_t bi = 0; bi < m.batches.size(); bi++) {
+    const auto &b = m.batches[bi];
     const int nb = (int) (b.second - b.first);
     int64_t tot_res = 0;
-    for (size_t a = b.first; a < b.second; a++) tot_res += jobs[a].Lr;
+    for (size_t a = b.first; a < b.second; a++) tot_res += m.jobs[a].Lr;
     n2.assign((size_t) tot_res, 0.0f);
-    HIPCHK(hipMemcpyAsync(w.mdjobs, jobs.data() + b.first, sizeof(MdJob) * (size_t) nb, hipMemcpyHostToDevice, w.st));
-    HIPCHK(launch_md_regions(p, w.dout.res, w.mdjobs, w.mdout, nb, w.n2pool, MD_SEGCAP, MD_NSAMPLES, MD_SEED, w.st));
-    HIPCHK(hipMemcpyAsync(outs.data() + b.first, w.mdout, sizeof(MdOut) * (size_t) nb, hipMemcpyDeviceToHost, w.st));
+    if (! (m.launched && bi == 0)) {
+      HIPCHK(hipMemcpyAsync(w.mdjobs, m.jobs.data() + b.first, sizeof(MdJob) * (size_t) nb, hipMemcpyHostToDevice, ms));
+      HIPCHK(launch_md_regions(p, w.dout.res, w.mdjobs, w.mdout, nb, w.mdpool, MD_SEGCAP, MD_NSAMPLES, MD_SEED, ms));
+    }
+    HIPCHK(hipMemcpyAsync(outs.data() + b.first, w.mdout, sizeof(MdOut) * (size_t) nb, hipMemcpyDeviceToHost, ms));
     int64_t o = 0;
     for (size_t a = b.first; a < b.second; a++) {
-      HIPCHK(hipMemcpyAsync(n2.data() + o, w.n2pool + jobs[a].area + md_n2sc_at(p.M, jobs[a].Lr) + 1,
-                            sizeof(float) * (size_t) jobs[a].Lr, hipMemcpyDeviceToHost, w.st));
-      o += jobs[a].Lr;
+      HIPCHK(hipMemcpyAsync(n2.data() + o, w.mdpool + m.jobs[a].area + md_n2sc_at(p.M, m.jobs[a].Lr) + 1,
+                            sizeof(float) * (size_t) m.jobs[a].Lr, hipMemcpyDeviceToHost, ms));
+      o += m.jobs[a].Lr;
     }
-    HIPCHK(hipStreamSynchronize(w.st));
+    HIPCHK(hipStreamSynchronize(ms));
     o = 0;
     for (size_t a = b.first; a < b.second; a++) {
       const MdOut &r = outs[a];
-      const int64_t k = mdk[a];
-      const int Lr = jobs[a].Lr;
+      const int64_t k = m.mdk[a];
+      const int Lr = m.jobs[a].Lr;
       if (r.status == 1) { p7g_set_error("stochastic traceback failed in a multidomain region"); return p7G_EINVAL; }
       if (r.status != 0) { p7g_set_error("multidomain region exceeds the sampled-domain capacity"); return p7G_EMEM; }
       const float *rn = n2.data() + o;
@@ -1546,7 +1584,7 @@ multidomain_regions(Work &w, const DevProfile &p, const DevSeqs &s, std::vector<
         rec.push_back(make_int4(pr.x, pr.y + ci - 1, pr.y + cj - 1, 2));
         vlen.push_back(cj - ci + 1);
         env_dc.push_back(dc);
-        src.push_back(voff[(size_t) k] + ci - 1);
+        src.push_back(m.voff[(size_t) k] + ci - 1);
         len.push_back(cj - ci + 1);
         eloop.push_back(eloop[(size_t) k]); emove.push_back(emove[(size_t) k]);
       }
@@ -1557,7 +1595,7 @@ multidomain_regions(Work &w, const DevProfile &p, const DevSeqs &s, std::vector<
   if (ne == 0) return p7G_OK;
   if (nregs + ne > w.dout.reg_cap || ne > w.md_cap * MD_MAXC) { p7g_set_error("too many domain envelopes"); return p7G_EMEM; }
   // the envelopes' residue copies after the regions'
-  int64_t cur = (int64_t) rcur;
+  int64_t cur = (int64_t) m.rcur;
   for (int64_t e = 0; e < ne; e++) {
     dst.push_back(cur);
     cur += ((((int64_t) len[(size_t) e]) + 15) & ~(int64_t) 15) + 16;
@@ -1600,74 +1638,33 @@ multidomain_regions(Work &w, const DevProfile &p, const DevSeqs &s, std::vector<
   return p7G_OK;
 }
 
-// Rescoring of the regions after a decoding run left its nregs region
-// records on the device (rescore_isolated_domain, p7_domaindef.c:812-947):
-// multidomain regions first give their clusters' envelopes as records of
-// their own (multidomain_regions). Then every envelope (a single-domain
-// region, w = 0, or a cluster, w = 2) gets its null2 correction and optimal
-// accuracy alignment: the host gives each an area in the null2 pool and
-// groups them in batches whose areas fit the pool; per batch, three list
-// launches over the copies: the envelope Backward writing its special and M/I
-// rows, the Forward again summing the posteriors into null2 and the
-// correction and filling the alignment, and the alignment's traceback. A
-// cluster's correction is then replaced by the sum of its region's sampled
-// null2 scores; a multidomain region's record keeps status 2 and, as its
-// correction, the sum over all its residues. *ret_nrec: records in all.
+
+// The null2 pass over records k0..k1-1 that are envelopes (w != 1): the host
+// gives each an area in the null2 pool and groups them in batches whose areas
+// fit the pool; per batch, three list launches over the copies: the envelope
+// Backward writing its special and M/I rows, the Forward again summing the
+// posteriors into null2 and the correction and filling the alignment, and the
+// alignment's traceback. Returns after the pass has run (the host vectors are
+// the copies' sources).
 static int
-null2_regions(Work &w, const DevProfile &p, const DevSeqs &s, int64_t nregs, const GateThresh &g, const int32_t *h_len,
-              int64_t *ret_nrec, double *ret_ms_md)
+null2_pass(Work &w, const DevProfile &p, const DevSeqs &s, const GateThresh &g, const std::vector<int4> &rec,
+           const std::vector<int32_t> &vlen, int64_t k0, int64_t k1)
 {
-  *ret_nrec = nregs;
-  *ret_ms_md = 0.0;
-  if (nregs <= 0) return p7G_OK;
-  std::vector<int4> rec((size_t) nregs);
-  std::vector<int32_t> vlen((size_t) nregs);
-  HIPCHK(hipMemcpyAsync(rec.data(), w.dout.regs, sizeof(int4) * (size_t) nregs, hipMemcpyDeviceToHost, w.st));
-  HIPCHK(hipMemcpyAsync(vlen.data(), w.dout.v_len, sizeof(int32_t) * (size_t) nregs, hipMemcpyDeviceToHost, w.st));
-  HIPCHK(hipStreamSynchronize(w.st));
-  std::vector<float> env_dc;
-  std::vector<std::pair<int64_t, float>> reg_sum;
-  int status;
-  const auto t0 = std::chrono::steady_clock::now();
-  if ((status = multidomain_regions(w, p, s, rec, vlen, nregs, h_len, g, env_dc, reg_sum)) != p7G_OK) return status;
-  *ret_ms_md = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-  const int64_t nrec = (int64_t) rec.size();
-  HIPCHK(launch_fill32((uint32_t *) w.n2st, 2u, nrec, w.st));
-  HIPCHK(launch_fill32((uint32_t *) w.n2dc, 0x7fc00000u, nrec, w.st));
-  HIPCHK(launch_fill32((uint32_t *) w.n2oa, 0x7fc00000u, nrec, w.st));
-  HIPCHK(launch_fill32((uint32_t *) w.n2ali, 0u, 4 * nrec, w.st));
-  HIPCHK(launch_fill32((uint32_t *) w.n2pn, 0u, nrec, w.st));
-  // the alignments' words: an envelope's length + M + 2 each
-  {
-    std::vector<int64_t> poff((size_t) nrec, 0);
-    int64_t ptot = 0;
-    for (int64_t k = 0; k < nrec; k++) {
-      poff[(size_t) k] = ptot;
-      if (rec[(size_t) k].w != 1) ptot += (int64_t) vlen[(size_t) k] + p.M + 2;
-    }
-    if (w.n2path_cap < ptot) {
-      HIPCHK(hipStreamSynchronize(w.st));
-      if (w.n2path) { (void) hipFree(w.n2path); w.n2path = nullptr; w.n2path_cap = 0; }
-      HIPCHK(hipMalloc((void **) &w.n2path, sizeof(uint32_t) * (size_t) std::max<int64_t>(ptot, 1)));
-      w.n2path_cap = ptot;
-    }
-    HIPCHK(hipMemcpyAsync(w.n2poff, poff.data(), sizeof(int64_t) * (size_t) nrec, hipMemcpyHostToDevice, w.st));
-    HIPCHK(hipStreamSynchronize(w.st));          // poff is the copy's source
-  }
   // the pool: what the envelopes need, bounded by a quarter of the free memory
   // (the pool already held counts as free)
   int64_t need_all = 0;
-  for (int64_t k = 0; k < nrec; k++)
+  for (int64_t k = k0; k < k1; k++)
     if (rec[(size_t) k].w != 1) need_all += n2_area_floats(vlen[(size_t) k], p.fb_B);
+  if (need_all == 0) return p7G_OK;
   size_t fr = 0, tot = 0;
   HIPCHK(hipMemGetInfo(&fr, &tot));
   const int64_t cap = std::max<int64_t>(N2_POOL_MIN, std::min<int64_t>(need_all, (int64_t) (fr / 4 / sizeof(float)) + w.n2size));
-  std::vector<int64_t> off((size_t) nrec, 0);
+  std::vector<int64_t> off((size_t) (k1 - k0), 0);
   std::vector<int32_t> list;
   std::vector<std::pair<size_t, size_t>> batches;
   int64_t used = 0, maxused = 0;
   size_t start = 0;
-  for (int64_t k = 0; k < nrec; k++) {
+  for (int64_t k = k0; k < k1; k++) {
     if (rec[(size_t) k].w == 1) continue;
     // the kernels size an envelope by its copy's length, which is its i..j
     const int Ld = vlen[(size_t) k];
@@ -1675,13 +1672,13 @@ null2_regions(Work &w, const DevProfile &p, const DevSeqs &s, int64_t nregs, con
     const int64_t need = n2_area_floats(Ld, p.fb_B);
     if (need > cap) { p7g_set_error("region of %d residues too long for the null2 pool", Ld); return p7G_EMEM; }
     if (used + need > cap) { batches.push_back({ start, list.size() }); start = list.size(); used = 0; }
-    off[(size_t) k] = used; used += need; maxused = std::max(maxused, used);
+    off[(size_t) (k - k0)] = used; used += need; maxused = std::max(maxused, used);
     list.push_back((int32_t) k);
   }
   if (list.size() > start) batches.push_back({ start, list.size() });
   if ((int64_t) batches.size() > N2_MAX_BATCH) { p7g_set_error("too many null2 batches (%zu)", batches.size()); return p7G_EMEM; }
-  if (! batches.empty()) {
   if (w.n2size < maxused) {
+    HIPCHK(hipStreamSynchronize(w.st));
     if (w.n2pool) { (void) hipFree(w.n2pool); w.n2pool = nullptr; w.n2size = 0; }
     HIPCHK(hipMalloc((void **) &w.n2pool, sizeof(float) * (size_t) maxused));
     w.n2size = maxused;
@@ -1689,7 +1686,7 @@ null2_regions(Work &w, const DevProfile &p, const DevSeqs &s, int64_t nregs, con
   std::vector<int32_t> cnt(batches.size() * 3 * 16, 0);
   for (size_t b = 0; b < batches.size(); b++)
     for (int a = 0; a < 3; a++) cnt[(b * 3 + a) * 16] = (int32_t) (batches[b].second - batches[b].first);
-  HIPCHK(hipMemcpyAsync(w.n2off, off.data(), sizeof(int64_t) * (size_t) nrec, hipMemcpyHostToDevice, w.st));
+  HIPCHK(hipMemcpyAsync(w.n2off + k0, off.data(), sizeof(int64_t) * (size_t) (k1 - k0), hipMemcpyHostToDevice, w.st));
   HIPCHK(hipMemcpyAsync(w.n2list, list.data(), sizeof(int32_t) * list.size(), hipMemcpyHostToDevice, w.st));
   HIPCHK(hipMemcpyAsync(w.n2cnt, cnt.data(), sizeof(int32_t) * cnt.size(), hipMemcpyHostToDevice, w.st));
   DevSeqs vs = s;
@@ -1716,6 +1713,82 @@ null2_regions(Work &w, const DevProfile &p, const DevSeqs &s, int64_t nregs, con
     HIPCHK(launch_fwd(p, vs, bl, c + 32, nb, vf, w.l, g, 0, lc, nullptr, 0, nullptr, nullptr, w.st, eb));
     HIPCHK(launch_oa_trace(p, vs, bl, nb, eb, w.st));
   }
+  HIPCHK(hipStreamSynchronize(w.st));
+  return p7G_OK;
+}
+
+// Rescoring of the regions after a decoding run left its nregs region
+// records on the device (rescore_isolated_domain, p7_domaindef.c:812-947):
+// the multidomain regions' sampled traces start on the second stream; the
+// single-domain regions (w = 0) get their null2 correction and optimal
+// accuracy alignment meanwhile; then the multidomain regions' clusters become
+// envelope records of their own (w = 2), which get theirs. A cluster's
+// correction is then replaced by the sum of its region's sampled null2
+// scores; a multidomain region's record keeps status 2 and, as its
+// correction, the sum over all its residues. *ret_nrec: records in all.
+static int
+null2_regions(Work &w, const DevProfile &p, const DevSeqs &s, int64_t nregs, const GateThresh &g, const int32_t *h_len,
+              int64_t *ret_nrec, double *ret_ms_md)
+{
+  *ret_nrec = nregs;
+  *ret_ms_md = 0.0;
+  if (nregs <= 0) return p7G_OK;
+  std::vector<int4> rec((size_t) nregs);
+  std::vector<int32_t> vlen((size_t) nregs);
+  HIPCHK(hipMemcpyAsync(rec.data(), w.dout.regs, sizeof(int4) * (size_t) nregs, hipMemcpyDeviceToHost, w.st));
+  HIPCHK(hipMemcpyAsync(vlen.data(), w.dout.v_len, sizeof(int32_t) * (size_t) nregs, hipMemcpyDeviceToHost, w.st));
+  HIPCHK(hipStreamSynchronize(w.st));
+  std::vector<float> env_dc;
+  std::vector<std::pair<int64_t, float>> reg_sum;
+  int status;
+  const auto t0 = std::chrono::steady_clock::now();
+  MdRun m;
+  if ((status = md_start(w, p, rec, vlen, nregs, h_len, m)) != p7G_OK) return status;
+  // every record's status, correction and alignment start as "not rescored"
+  auto clear = [&](int64_t k0, int64_t k1) -> int {
+    const int64_t n = k1 - k0;
+    if (n <= 0) return p7G_OK;
+    HIPCHK(launch_fill32((uint32_t *) (w.n2st + k0), 2u, n, w.st));
+    HIPCHK(launch_fill32((uint32_t *) (w.n2dc + k0), 0x7fc00000u, n, w.st));
+    HIPCHK(launch_fill32((uint32_t *) (w.n2oa + k0), 0x7fc00000u, n, w.st));
+    HIPCHK(launch_fill32((uint32_t *) (w.n2ali + k0), 0u, 4 * n, w.st));
+    HIPCHK(launch_fill32((uint32_t *) (w.n2pn + k0), 0u, n, w.st));
+    return p7G_OK;
+  };
+  if ((status = clear(0, nregs)) != p7G_OK) return status;
+  // the alignments' words: an envelope's length + M + 2 each; room for the
+  // clusters' envelopes too (at most MD_MAXC per multidomain region, each
+  // within its region)
+  std::vector<int64_t> poff((size_t) nregs, 0);
+  int64_t ptot = 0, pbound = 0;
+  for (int64_t k = 0; k < nregs; k++) {
+    poff[(size_t) k] = ptot;
+    if (rec[(size_t) k].w != 1) ptot += (int64_t) vlen[(size_t) k] + p.M + 2;
+  }
+  for (int64_t k : m.mdk) pbound += (int64_t) MD_MAXC * ((int64_t) vlen[(size_t) k] + p.M + 2);
+  if (w.n2path_cap < ptot + pbound) {
+    HIPCHK(hipStreamSynchronize(w.st));
+    if (w.n2path) { (void) hipFree(w.n2path); w.n2path = nullptr; w.n2path_cap = 0; }
+    HIPCHK(hipMalloc((void **) &w.n2path, sizeof(uint32_t) * (size_t) std::max<int64_t>(ptot + pbound, 1)));
+    w.n2path_cap = ptot + pbound;
+  }
+  HIPCHK(hipMemcpyAsync(w.n2poff, poff.data(), sizeof(int64_t) * (size_t) nregs, hipMemcpyHostToDevice, w.st));
+  HIPCHK(hipStreamSynchronize(w.st));          // poff is the copy's source
+  if ((status = null2_pass(w, p, s, g, rec, vlen, 0, nregs)) != p7G_OK) return status;
+  if ((status = md_finish(w, p, s, rec, vlen, nregs, g, env_dc, reg_sum, m)) != p7G_OK) return status;
+  *ret_ms_md = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  const int64_t nrec = (int64_t) rec.size();
+  if (nrec > nregs) {
+    if ((status = clear(nregs, nrec)) != p7G_OK) return status;
+    std::vector<int64_t> cpoff((size_t) (nrec - nregs), 0);
+    for (int64_t k = nregs; k < nrec; k++) {
+      cpoff[(size_t) (k - nregs)] = ptot;
+      ptot += (int64_t) vlen[(size_t) k] + p.M + 2;
+    }
+    if (ptot > w.n2path_cap) { p7g_set_error("internal: alignment words of the clusters' envelopes"); return p7G_EINVAL; }
+    HIPCHK(hipMemcpyAsync(w.n2poff + nregs, cpoff.data(), sizeof(int64_t) * (size_t) (nrec - nregs), hipMemcpyHostToDevice, w.st));
+    HIPCHK(hipStreamSynchronize(w.st));        // cpoff is the copy's source
+    if ((status = null2_pass(w, p, s, g, rec, vlen, nregs, nrec)) != p7G_OK) return status;
   }
   if (! env_dc.empty() || ! reg_sum.empty()) {
     // corrections from the sampled traces: clusters' over their envelopes,
@@ -2484,6 +2557,7 @@ batch_free(Batch *b)
   for (int k = 0; k < P7G_BATCH_CHAINS; k++) {
     work_release(b->ch[k]);
     if (b->ch[k].st) (void) hipStreamDestroy(b->ch[k].st);
+    if (b->ch[k].st2) (void) hipStreamDestroy(b->ch[k].st2);
   }
   if (b->mem) (void) hipFree(b->mem);
   for (hipEvent_t e : b->evs) (void) hipEventDestroy(e);
