@@ -208,6 +208,9 @@ struct DomOut {
 #define MD_MAXC   32      // envelopes one region may give
 #define MD_ACC_LDS 8192   // regions up to this many residues + 1 sum their odds in LDS
 __host__ __device__ inline int64_t md_q(int M) { return (M - 1) / 4 + 1 < 2 ? 2 : (M - 1) / 4 + 1; }
+// checkpoints per row of the E state's running sum over the 8Q M and D
+// cells (one at the end of every 64 of them, and the last)
+__host__ __device__ inline int md_ncp(int64_t Q) { return (int) ((8 * Q + 63) / 64); }
 // LDS, in floats: the transitions staged for the Forward (10 rows of M + 2),
 // the Forward's rows, and the traces' buffers (from the start of LDS): a
 // domain's counts (4Q), its null2 odds (32), the trace's domains, the
@@ -239,8 +242,8 @@ __host__ __device__ inline int64_t md_n2sc_at(int M, int Lr)
 { return (int64_t) (Lr + 1) * 12 * md_q(M) + 6 * (int64_t) (Lr + 1); }
 // then the endpoint counts, the sampled domains (5 ints each), the
 // clustering's element clusters and cluster counts (segcap ints each), and
-// (doubles) the traces' thresholds: 5 per cell, 3 per row, 8Q running sums
-// per row
+// (doubles) the traces' thresholds: 5 per cell, 3 per row, md_ncp running-sum
+// checkpoints per row
 __host__ __device__ inline int64_t md_seg_at(int M, int Lr)
 { return md_n2sc_at(M, Lr) + (Lr + 1) + ((Lr > M ? Lr : M) + 2); }
 __host__ __device__ inline int64_t md_clu_at(int M, int Lr, int segcap)
@@ -249,8 +252,10 @@ __host__ __device__ inline int64_t md_thr_at(int M, int Lr, int segcap)
 { return (md_clu_at(M, Lr, segcap) + 2 * (int64_t) segcap + 1) & ~(int64_t) 1; }
 size_t  md_lds_bytes(int M, int segcap);
 int64_t md_area_floats(int M, int Lr, int segcap);
+// thr_done (optional): recorded on st once the traces' thresholds are filled
 hipError_t launch_md_regions(const DevProfile &p, const uint8_t *res, const MdJob *jobs, MdOut *out, int njobs,
-                             float *pool, int segcap, int nsamples, uint32_t seed, hipStream_t st);
+                             float *pool, int segcap, int nsamples, uint32_t seed, hipStream_t st,
+                             hipEvent_t thr_done = nullptr);
 hipError_t launch_md_copy(uint8_t *res, const int64_t *src, const int64_t *dst, const int32_t *len, int n, hipStream_t st);
 
 // Length model of one call. on == 0: every target uses its own length

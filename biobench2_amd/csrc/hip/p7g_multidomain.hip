@@ -126,7 +126,7 @@ struct MdSig { int c, i, j, k, m; float prob; };
 
 __global__ __launch_bounds__(64) void
 k_md_region(const DevProfile p, const uint8_t *__restrict__ res, const MdJob *__restrict__ jobs, MdOut *__restrict__ out,
-            float *__restrict__ pool, int nsamples, uint32_t seed)
+            float *__restrict__ pool, int nsamples, uint32_t seed, int phase)
 {
   extern __shared__ float lds[];
   const int z = threadIdx.x;
@@ -151,7 +151,7 @@ k_md_region(const DevProfile p, const uint8_t *__restrict__ res, const MdJob *__
   const int TW = M + 2;
   float *tl = lds;
   float *ph = lds + (stage ? md_stage_floats(M) : 0);       // the phases' own LDS
-  if (stage)
+  if (stage && phase == 0)
     for (int c = z; c < 10 * TW; c += 64) tl[c] = md_tv(p, TR + md_trow(c / TW), c % TW);
   auto tv = [&](int row, int k) -> float {                 // row: 0..9 as staged
     if (stage) return tl[row * TW + k];
@@ -160,7 +160,7 @@ k_md_region(const DevProfile p, const uint8_t *__restrict__ res, const MdJob *__
   enum { T_BM = 0, T_MM, T_IM, T_DM, T_MDS, T_DDS, T_MI, T_II, T_MD, T_DD };
 
   // ---------------- 1. the region's Forward matrix ----------------
-  {
+  if (phase == 0) {
     float *prev = ph, *cur = ph + 12 * Q, *tdd = ph + 24 * Q, *erow = ph + 28 * Q;   // erow: 4Q emissions
     __syncthreads();
     for (int c = z; c < 12 * Q; c += 64) { prev[c] = 0.0f; mat[c] = 0.0f; }
@@ -324,91 +324,14 @@ k_md_region(const DevProfile p, const uint8_t *__restrict__ res, const MdJob *__
       md_sync();
     }
     __syncthreads();                                       // the matrix in global memory, for every lane
+    return;
   }
 
   // ---------------- 2. the sampled traces ----------------
-  // Every choice a trace can make depends only on the finished matrix, so
-  // its thresholds are computed first for every cell and row, in parallel,
-  // with the same operations p7_StochasticTrace performs at the step
-  // (md_thresholds); a step is then one draw and one to three comparisons.
-  // thr[(i, k)]: M's three (B | M | I | D), D's one (M | D), I's one (M | I);
-  // xthr[i]: C's (C | E), J's (J | E), B's (N | J); ecum[i]: select_e's
-  // running sum over row i in the striped order (q outer; M lanes, D lanes).
+  // (their thresholds were filled by k_md_thr, md_thr_fill)
   double *thr = (double *) (pool + J.area + md_thr_at(M, Lr, J.segcap));
   double *xthr = thr + (size_t) 5 * (Lr + 1) * (M + 1);
   double *ecum = xthr + (size_t) 3 * (Lr + 1);
-  auto cellk = [&](int i, int k, int s) {                  // the cell of model position k (1..4Q) in row i
-    const int r = (k - 1 >= Q) + (k - 1 >= 2 * Q) + (k - 1 >= 3 * Q), q = k - 1 - r * Q;
-    return CELL(i, q, s, r); };
-  // four cells per lane at a time, their matrix values loaded before the
-  // threshold arithmetic of any of them
-  const int64_t LM = (int64_t) Lr * M;
-  for (int64_t c0 = 0; c0 < LM; c0 += 4 * 64) {
-    float in[4][9];
-#pragma unroll
-    for (int u = 0; u < 4; u++) {
-      const int64_t c = min(c0 + u * 64 + z, LM - 1);
-      const int i = 1 + (int) (c / M), k = 1 + (int) (c - (int64_t) (i - 1) * M);
-      in[u][0] = xmx[(size_t) (i - 1) * 6 + 3];
-      in[u][1] = k > 1 ? cellk(i - 1, k - 1, 0) : 0.0f;
-      in[u][2] = k > 1 ? cellk(i - 1, k - 1, 2) : 0.0f;
-      in[u][3] = k > 1 ? cellk(i - 1, k - 1, 1) : 0.0f;
-      in[u][4] = k > 1 ? cellk(i, k - 1, 0) : 0.0f;
-      in[u][5] = k > 1 ? cellk(i, k - 1, 1) : 0.0f;
-      in[u][6] = cellk(i - 1, k, 0);
-      in[u][7] = cellk(i - 1, k, 2);
-    }
-#pragma unroll
-    for (int u = 0; u < 4; u++) {
-      const int64_t c = c0 + u * 64 + z;
-      if (c >= LM) break;
-      const int i = 1 + (int) (c / M), k = 1 + (int) (c - (int64_t) (i - 1) * M);
-      double *o = thr + ((size_t) i * (M + 1) + k) * 5;
-      float v[4];
-      v[0] = __fmul_rn(in[u][0], tv(T_BM, k));
-      v[1] = __fmul_rn(in[u][1], tv(T_MM, k));
-      v[2] = __fmul_rn(in[u][2], tv(T_IM, k));
-      v[3] = __fmul_rn(in[u][3], tv(T_DM, k));
-      md_thresholds(v, 4, o);
-      v[0] = (k > 1) ? __fmul_rn(in[u][4], tv(T_MDS, k)) : 0.0f;
-      v[1] = (k > 1) ? __fmul_rn(in[u][5], tv(T_DDS, k)) : 0.0f;
-      md_thresholds(v, 2, o + 3);
-      v[0] = __fmul_rn(in[u][6], tv(T_MI, k));
-      v[1] = __fmul_rn(in[u][7], tv(T_II, k));
-      md_thresholds(v, 2, o + 4);
-    }
-  }
-  for (int i = z; i <= Lr; i += 64) {
-    float v[2];
-    if (i >= 1) {
-      v[0] = __fmul_rn(xmx[(size_t) (i - 1) * 6 + 4], floop);
-      v[1] = __fmul_rn(__fmul_rn(xmx[(size_t) i * 6 + 0], xEmove), xmx[(size_t) i * 6 + 5]);
-      md_thresholds(v, 2, xthr + (size_t) i * 3 + 0);
-      v[0] = __fmul_rn(xmx[(size_t) (i - 1) * 6 + 2], floop);
-      v[1] = __fmul_rn(__fmul_rn(xmx[(size_t) i * 6 + 0], xEloop), xmx[(size_t) i * 6 + 5]);
-      md_thresholds(v, 2, xthr + (size_t) i * 3 + 1);
-      // select_e: norm = 1/E(i) in double, the products in float, summed in double
-      const float nf = (float) (1.0 / (double) xmx[(size_t) i * 6 + 0]);
-      double sum = 0.0, *er = ecum + (size_t) i * 8 * Q;
-      float u[8], un[8];
-#pragma unroll
-      for (int e = 0; e < 8; e++) u[e] = mat[(size_t) i * Q * 12 + e];
-      for (int q = 0; q < Q; q++) {
-        const float *cn = mat + ((size_t) i * Q + min(q + 1, Q - 1)) * 12;    // the next q's, in flight
-#pragma unroll
-        for (int e = 0; e < 8; e++) un[e] = cn[e];
-#pragma unroll
-        for (int e = 0; e < 8; e++) { sum = sum + (double) __fmul_rn(u[e], nf); er[q * 8 + e] = sum; }
-#pragma unroll
-        for (int e = 0; e < 8; e++) u[e] = un[e];
-      }
-    }
-    v[0] = __fmul_rn(xmx[(size_t) i * 6 + 1], fmove);
-    v[1] = __fmul_rn(xmx[(size_t) i * 6 + 2], fmove);
-    md_thresholds(v, 2, xthr + (size_t) i * 3 + 2);
-  }
-  __syncthreads();
-
   // LDS: a domain's counts, its null2 odds, the trace's domains, and the
   // residues' odds sums when the region is short
   float *cnt = lds;                                        // [q][r] counts of one sampled domain
@@ -508,17 +431,27 @@ k_md_region(const DevProfile p, const uint8_t *__restrict__ res, const MdJob *__
           if (i < 1) break;
           rng = rng * 69069u + 1u;
           const double roll = (double) rng / 4294967296.0;
-          const double *er = ecum + (size_t) i * 8 * Q;
+          // the row's running sums are kept at every 64th element (ecum,
+          // md_ncp per row): the first checkpoint above the roll names the
+          // chunk, whose sums are formed again from the checkpoint before it
+          const int NCP = md_ncp(Q);
+          const double *cp = ecum + (size_t) i * NCP;
+          int c = -1;
+          for (int c0 = 0; c0 < NCP && c < 0; c0 += 64) {
+            const int cc = c0 + z;
+            const double v = cp[min(cc, NCP - 1)];
+            const unsigned long long m = __ballot(cc < NCP && roll < v);
+            if (m) c = c0 + (int) __ffsll((long long) m) - 1;
+          }
           int found = -1;
-          for (int j0 = 0; j0 < 8 * Q && found < 0; j0 += 64 * 8) {
-            double v[8];
-#pragma unroll
-            for (int u = 0; u < 8; u++) { const int j = j0 + u * 64 + z; v[u] = er[min(j, 8 * Q - 1)]; }
-#pragma unroll
-            for (int u = 0; u < 8; u++) {
-              const int j = j0 + u * 64 + z;
-              const unsigned long long m = __ballot(j < 8 * Q && roll < v[u]);
-              if (m && found < 0) found = j0 + u * 64 + (int) __ffsll((long long) m) - 1;
+          const float nf = (float) (1.0 / (double) xmx[(size_t) i * 6 + 0]);
+          if (c >= 0) {
+            const int j = 64 * c + z, q = j >> 3;
+            const float u = (j < 8 * Q) ? __fmul_rn(mat[((size_t) i * Q + q) * 12 + (j & 7)], nf) : 0.0f;
+            double sum = c ? cp[c - 1] : 0.0;
+            for (int l = 0; l < 64; l++) {
+              sum = sum + (double) readlanef(u, l);
+              if (roll < sum) { found = 64 * c + l; break; }
             }
           }
           if (found >= 0) {
@@ -526,8 +459,7 @@ k_md_region(const DevProfile p, const uint8_t *__restrict__ res, const MdJob *__
             s1 = ((found >> 2) & 1) ? MS_D : MS_M;
           } else {
             // the reference scans the row again, the sum going on from its total
-            const float nf = (float) (1.0 / (double) xmx[(size_t) i * 6 + 0]);
-            double sum = er[8 * Q - 1];
+            double sum = cp[NCP - 1];
             for (int pass = 1; pass < 4 && s1 < 0; pass++)
               for (int q0 = 0; q0 < Q && s1 < 0; q0 += 8) {
                 const int q = q0 + (z >> 3), sm = (z >> 2) & 1, r = z & 3;
@@ -773,9 +705,162 @@ k_md_region(const DevProfile p, const uint8_t *__restrict__ res, const MdJob *__
 // traces' counts, odds and per-trace domains, or the clustering's stacks
 size_t md_lds_bytes(int M, int segcap)
 {
-  const int64_t fwd = (md_stage_t(M) ? md_stage_floats(M) : 0) + 32 * md_q(M);
+  const int64_t fwd = (md_stage_t(M) ? md_stage_floats(M) : 0) + 32 * md_q(M) + 128;
   const size_t clu = (size_t) 2 * segcap * 4 + (size_t) (segcap + 63) / 64 * 8 + (size_t) MD_MAXSIG * sizeof(MdSig);
   return std::max((size_t) std::max(fwd, md_trace_floats(M)) * 4, clu);
+}
+
+// The traces' thresholds of every region (md_thresholds over each cell and
+// row of its finished Forward matrix): a grid of gridDim.x workgroups of
+// MD_THR_NT threads per region (blockIdx.y). Every choice a trace can make
+// depends only on the finished matrix, so its thresholds are computed first
+// for every cell and row, with the same operations p7_StochasticTrace
+// performs at the step (md_thresholds); a step is then one draw and one to
+// three comparisons. thr[(i, k)]: M's three (B | M | I | D), D's one (M | D),
+// I's one (M | I); xthr[i]: C's (C | E), J's (J | E), B's (N | J); ecum[i]:
+// select_e's running sum over row i in the striped order (q outer; M lanes,
+// D lanes), kept at the end of every 64 cells and the last (md_ncp per
+// row). A cell's five thresholds go out through LDS, so that the stores of
+// a wave are contiguous; a row's sums are a chain of 8Q that one thread
+// computes.
+#define MD_THR_NT 256
+#define MD_THR_CELLS 2                                     // cells per thread per pass
+#define MD_THR_LDS_OUT ((size_t) 8 * (MD_THR_CELLS * MD_THR_NT * 5 > MD_THR_NT * 8 ? MD_THR_CELLS * MD_THR_NT * 5 : MD_THR_NT * 8))
+static __host__ __device__ inline size_t md_thr_tl_bytes(int M)      // the staged transitions, when they fit
+{
+  const size_t b = ((size_t) 10 * (M + 2) * 4 + 7) & ~(size_t) 7;
+  return (b + MD_THR_LDS_OUT <= P7G_LDS_MAX) ? b : 0;
+}
+
+__global__ __launch_bounds__(MD_THR_NT) void
+k_md_thr(const DevProfile p, const MdJob *__restrict__ jobs, float *__restrict__ pool)
+{
+  const int t = threadIdx.x, NT = MD_THR_NT;
+  const MdJob J = jobs[blockIdx.y];
+  const int M = p.M, Q = max(2, (M - 1) / 4 + 1), Lr = J.Lr, L = J.Ltarget;
+  const int TR = p.nrows;
+  const float xEmove = p.xfE_move, xEloop = p.xfE_loop, floop = p.xf_loop[L], fmove = p.xf_move[L];
+  float *mat = pool + J.area;
+  float *xmx = mat + (size_t) (Lr + 1) * 12 * Q;
+  extern __shared__ double ldsd[];
+  const int TW = M + 2;
+  const size_t tlb = md_thr_tl_bytes(M);
+  float *tl = (float *) ldsd;
+  double *ob = ldsd + tlb / 8;                             // the outgoing values
+  if (tlb) {
+    for (int c = t; c < 10 * TW; c += NT) tl[c] = md_tv(p, TR + md_trow(c / TW), c % TW);
+    __syncthreads();
+  }
+  auto tv = [&](int row, int k) -> float { return tlb ? tl[row * TW + k] : md_tv(p, TR + md_trow(row), k); };
+  enum { T_BM = 0, T_MM, T_IM, T_DM, T_MDS, T_DDS, T_MI, T_II, T_MD, T_DD };
+#define CELL(i_, q_, s_, r_) mat[(((size_t) (i_) * Q + (q_)) * 3 + (s_)) * 4 + (r_)]
+  double *thr = (double *) (pool + J.area + md_thr_at(M, Lr, J.segcap));
+  double *xthr = thr + (size_t) 5 * (Lr + 1) * (M + 1);
+  double *ecum = xthr + (size_t) 3 * (Lr + 1);
+  auto cellk = [&](int i, int k, int s) {                  // the cell of model position k (1..4Q) in row i
+    const int r = (k - 1 >= Q) + (k - 1 >= 2 * Q) + (k - 1 >= 3 * Q), q = k - 1 - r * Q;
+    return CELL(i, q, s, r); };
+  // cells c = (i - 1) M + (k - 1): each pass MD_THR_CELLS * NT of them, their
+  // matrix values loaded before the threshold arithmetic of any of them
+  const int64_t LM = (int64_t) Lr * M;
+  constexpr int CU = MD_THR_CELLS;
+  // a cell's row and column from the pass's first cell (one 64-bit division
+  // per pass) and its offset o < CU * NT: x = r0 + o < M + CU * NT, divided
+  // by M through a float reciprocal and corrected by one
+  const float invM = 1.0f / (float) M;
+  for (int64_t c0 = (int64_t) blockIdx.x * CU * NT; c0 < LM; c0 += (int64_t) gridDim.x * CU * NT) {
+    const int a0 = (int) (c0 / M), r0 = (int) (c0 - (int64_t) a0 * M);
+    const int olast = (int) min((int64_t) CU * NT - 1, LM - 1 - c0);
+    auto rowcol = [&](int o, int &i, int &k) {
+      const int x = r0 + o;
+      int qd = (int) ((float) x * invM);
+      if (qd * M > x) qd--; else if ((qd + 1) * M <= x) qd++;
+      i = 1 + a0 + qd; k = 1 + x - qd * M;
+    };
+    float in[CU][8];
+#pragma unroll
+    for (int u = 0; u < CU; u++) {
+      int i, k;
+      rowcol(min(u * NT + t, olast), i, k);
+      in[u][0] = xmx[(size_t) (i - 1) * 6 + 3];
+      in[u][1] = k > 1 ? cellk(i - 1, k - 1, 0) : 0.0f;
+      in[u][2] = k > 1 ? cellk(i - 1, k - 1, 2) : 0.0f;
+      in[u][3] = k > 1 ? cellk(i - 1, k - 1, 1) : 0.0f;
+      in[u][4] = k > 1 ? cellk(i, k - 1, 0) : 0.0f;
+      in[u][5] = k > 1 ? cellk(i, k - 1, 1) : 0.0f;
+      in[u][6] = cellk(i - 1, k, 0);
+      in[u][7] = cellk(i - 1, k, 2);
+    }
+#pragma unroll
+    for (int u = 0; u < CU; u++) {
+      int i, k;
+      rowcol(min(u * NT + t, olast), i, k);
+      double *o = ob + (size_t) (u * NT + t) * 5;
+      float v[4];
+      v[0] = __fmul_rn(in[u][0], tv(T_BM, k));
+      v[1] = __fmul_rn(in[u][1], tv(T_MM, k));
+      v[2] = __fmul_rn(in[u][2], tv(T_IM, k));
+      v[3] = __fmul_rn(in[u][3], tv(T_DM, k));
+      md_thresholds(v, 4, o);
+      v[0] = (k > 1) ? __fmul_rn(in[u][4], tv(T_MDS, k)) : 0.0f;
+      v[1] = (k > 1) ? __fmul_rn(in[u][5], tv(T_DDS, k)) : 0.0f;
+      md_thresholds(v, 2, o + 3);
+      v[0] = __fmul_rn(in[u][6], tv(T_MI, k));
+      v[1] = __fmul_rn(in[u][7], tv(T_II, k));
+      md_thresholds(v, 2, o + 4);
+    }
+    __syncthreads();
+    // the pass's cells are consecutive in row-major order, each row's cells at
+    // (i (M + 1) + k) 5: the stores run along them
+    for (int d = t; d < CU * NT * 5; d += NT) {
+      const int o = d / 5;
+      if (o <= olast) {
+        int i, k;
+        rowcol(o, i, k);
+        thr[((size_t) i * (M + 1) + k) * 5 + (d - o * 5)] = ob[d];
+      }
+    }
+    __syncthreads();
+  }
+  // rows: one thread each; the running sum's checkpoints (every 64th
+  // element and the last) go out as they are reached
+  const int NCP = md_ncp(Q);
+  for (int i = blockIdx.x * NT + t; i <= Lr; i += gridDim.x * NT) {
+    float v[2];
+    if (i >= 1) {
+      v[0] = __fmul_rn(xmx[(size_t) (i - 1) * 6 + 4], floop);
+      v[1] = __fmul_rn(__fmul_rn(xmx[(size_t) i * 6 + 0], xEmove), xmx[(size_t) i * 6 + 5]);
+      md_thresholds(v, 2, xthr + (size_t) i * 3 + 0);
+      v[0] = __fmul_rn(xmx[(size_t) (i - 1) * 6 + 2], floop);
+      v[1] = __fmul_rn(__fmul_rn(xmx[(size_t) i * 6 + 0], xEloop), xmx[(size_t) i * 6 + 5]);
+      md_thresholds(v, 2, xthr + (size_t) i * 3 + 1);
+      // select_e: norm = 1/E(i) in double, the products in float, summed in double
+      const float nf = (float) (1.0 / (double) xmx[(size_t) i * 6 + 0]);
+      double sum = 0.0, *cp = ecum + (size_t) i * NCP;
+      // one checkpoint's eight q at a time, their 64 values loaded together
+      for (int q0 = 0; q0 < Q; q0 += 8) {
+        float u[8][8];
+#pragma unroll
+        for (int a = 0; a < 8; a++) {
+          const float4 *cn = (const float4 *) (mat + ((size_t) i * Q + min(q0 + a, Q - 1)) * 12);
+          const float4 x0 = cn[0], x1 = cn[1];
+          u[a][0] = x0.x; u[a][1] = x0.y; u[a][2] = x0.z; u[a][3] = x0.w;
+          u[a][4] = x1.x; u[a][5] = x1.y; u[a][6] = x1.z; u[a][7] = x1.w;
+        }
+#pragma unroll
+        for (int a = 0; a < 8; a++)
+          if (q0 + a < Q) {
+#pragma unroll
+            for (int e = 0; e < 8; e++) sum = sum + (double) __fmul_rn(u[a][e], nf);
+          }
+        cp[q0 >> 3] = sum;                                       // after element min(8 (q0 + 8), 8Q) - 1
+      }
+    }
+    v[0] = __fmul_rn(xmx[(size_t) i * 6 + 1], fmove);
+    v[1] = __fmul_rn(xmx[(size_t) i * 6 + 2], fmove);
+    md_thresholds(v, 2, xthr + (size_t) i * 3 + 2);
+  }
+#undef CELL
 }
 
 // floats of one region's area in the pool: matrix, specials, residue sums,
@@ -783,19 +868,31 @@ size_t md_lds_bytes(int M, int segcap)
 int64_t md_area_floats(int M, int Lr, int segcap)
 {
   const int64_t Q = md_q(M), R = (int64_t) Lr + 1;
-  const int64_t f = md_thr_at(M, Lr, segcap) + 2 * (5 * R * ((int64_t) M + 1) + 3 * R + 8 * Q * R);
+  const int64_t f = md_thr_at(M, Lr, segcap) + 2 * (5 * R * ((int64_t) M + 1) + 3 * R + md_ncp(Q) * R);
   return (f + 63) & ~(int64_t) 63;
 }
 
 hipError_t launch_md_regions(const DevProfile &p, const uint8_t *res, const MdJob *jobs, MdOut *out, int njobs,
-                             float *pool, int segcap, int nsamples, uint32_t seed, hipStream_t st)
+                             float *pool, int segcap, int nsamples, uint32_t seed, hipStream_t st, hipEvent_t thr_done)
 {
   if (njobs <= 0) return hipSuccess;
   const size_t lds = md_lds_bytes(p.M, segcap);
   if (lds > P7G_LDS_MAX) return hipErrorInvalidValue;
   hipError_t e = hipFuncSetAttribute((const void *) k_md_region, hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_md_region, dim3(njobs), dim3(64), lds, st, p, res, jobs, out, pool, nsamples, seed);
+  // the Forward matrices (one wave per region, the Forward's LDS only), the
+  // thresholds (the whole device), then the traces and the clustering
+  // (128 floats past the rows: the D->D chains read up to seven cells past q = Q - 1)
+  const size_t lds0 = (size_t) ((md_stage_t(p.M) ? md_stage_floats(p.M) : 0) + 32 * md_q(p.M) + 128) * 4;
+  hipLaunchKernelGGL(k_md_region, dim3(njobs), dim3(64), lds0, st, p, res, jobs, out, pool, nsamples, seed, 0);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  for (int j0 = 0; j0 < njobs; j0 += 65535) {
+    const int nj = std::min(njobs - j0, 65535);
+    hipLaunchKernelGGL(k_md_thr, dim3(32, nj), dim3(MD_THR_NT), md_thr_tl_bytes(p.M) + MD_THR_LDS_OUT, st, p, jobs + j0, pool);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+  }
+  if (thr_done && (e = hipEventRecord(thr_done, st)) != hipSuccess) return e;
+  hipLaunchKernelGGL(k_md_region, dim3(njobs), dim3(64), lds, st, p, res, jobs, out, pool, nsamples, seed, 1);
   return hipGetLastError();
 }
 

@@ -75,6 +75,7 @@ struct Work {
   float      *mdpool = nullptr;           // the multidomain regions' areas (grown on use)
   int64_t     mdsize = 0;
   hipStream_t st2 = nullptr;              // their stream, beside st (made on first use)
+  hipEvent_t  md_thr = nullptr;           // on st2: the regions' traces' thresholds are filled
   void       *dom_mem = nullptr;
   int64_t     dom_cap = 0;          // slots the per-slot arrays hold
 };
@@ -1315,6 +1316,7 @@ p7_gpu_pipeline_Destroy(P7_GPU_PIPELINE *pli)
   work_release(pli->w);
   if (pli->w.st) (void) hipStreamDestroy(pli->w.st);
   if (pli->w.st2) (void) hipStreamDestroy(pli->w.st2);
+  if (pli->w.md_thr) (void) hipEventDestroy(pli->w.md_thr);
   delete pli;
 }
 
@@ -1525,10 +1527,14 @@ md_start(Work &w, const DevProfile &p, const std::vector<int4> &rec, const std::
     w.mdsize = maxused;
   }
   if (m.batches.size() == 1) {
-    if (! w.st2) HIPCHK(hipStreamCreateWithFlags(&w.st2, hipStreamNonBlocking));
+    if (! w.st2) {
+      HIPCHK(hipStreamCreateWithFlags(&w.st2, hipStreamNonBlocking));
+      HIPCHK(hipEventCreateWithFlags(&w.md_thr, hipEventDisableTiming));
+    }
     const int nb = (int) m.jobs.size();
     HIPCHK(hipMemcpyAsync(w.mdjobs, m.jobs.data(), sizeof(MdJob) * (size_t) nb, hipMemcpyHostToDevice, w.st2));
-    HIPCHK(launch_md_regions(p, w.dout.res, w.mdjobs, w.mdout, nb, w.mdpool, MD_SEGCAP, MD_NSAMPLES, MD_SEED, w.st2));
+    HIPCHK(launch_md_regions(p, w.dout.res, w.mdjobs, w.mdout, nb, w.mdpool, MD_SEGCAP, MD_NSAMPLES, MD_SEED, w.st2,
+                             w.md_thr));
     m.launched = true;
   }
   return p7G_OK;
@@ -1774,6 +1780,9 @@ null2_regions(Work &w, const DevProfile &p, const DevSeqs &s, int64_t nregs, con
   }
   HIPCHK(hipMemcpyAsync(w.n2poff, poff.data(), sizeof(int64_t) * (size_t) nregs, hipMemcpyHostToDevice, w.st));
   HIPCHK(hipStreamSynchronize(w.st));          // poff is the copy's source
+  // the null2 pass's persistent workgroups would hold the compute units the
+  // thresholds' grid needs: it starts once they are filled, beside the traces
+  if (m.launched) HIPCHK(hipStreamWaitEvent(w.st, w.md_thr, 0));
   if ((status = null2_pass(w, p, s, g, rec, vlen, 0, nregs)) != p7G_OK) return status;
   if ((status = md_finish(w, p, s, rec, vlen, nregs, g, env_dc, reg_sum, m)) != p7G_OK) return status;
   *ret_ms_md = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
@@ -2558,6 +2567,7 @@ batch_free(Batch *b)
     work_release(b->ch[k]);
     if (b->ch[k].st) (void) hipStreamDestroy(b->ch[k].st);
     if (b->ch[k].st2) (void) hipStreamDestroy(b->ch[k].st2);
+    if (b->ch[k].md_thr) (void) hipEventDestroy(b->ch[k].md_thr);
   }
   if (b->mem) (void) hipFree(b->mem);
   for (hipEvent_t e : b->evs) (void) hipEventDestroy(e);
