@@ -206,7 +206,7 @@ struct DomOut {
 #define MD_TSEG   512     // sampled domains in one trace
 #define MD_MAXSIG 64      // significant clusters of one region
 #define MD_MAXC   32      // envelopes one region may give
-#define MD_ACC_LDS 8192   // regions up to this many residues + 1 sum their odds in LDS
+#define MD_ACC_LDS 8192   // regions up to this many residues + 1 sum their odds, and keep their residues, in LDS
 __host__ __device__ inline int64_t md_q(int M) { return (M - 1) / 4 + 1 < 2 ? 2 : (M - 1) / 4 + 1; }
 // checkpoints per row of the E state's running sum over the 8Q M and D
 // cells (one at the end of every 64 of them, and the last)
@@ -221,10 +221,14 @@ __host__ __device__ inline int64_t md_acc_at(int64_t Q) { return 4 * Q + 32 + MD
 // then, when it fits, the emission odds of 20 residues in striped order
 // ([x][q][r]) for the sampled domains' null2
 __host__ __device__ inline int64_t md_em_floats(int M) { return 20 * 4 * md_q(M); }
-__host__ __device__ inline bool md_em_lds(int M)
-{ return (md_acc_at(md_q(M)) + MD_ACC_LDS + md_em_floats(M)) * 4 <= P7G_LDS_MAX; }
-__host__ __device__ inline int64_t md_trace_floats(int M)
-{ return md_acc_at(md_q(M)) + MD_ACC_LDS + (md_em_lds(M) ? md_em_floats(M) : 0); }
+// a launch's room for the residue sums (floats) and the residues (bytes):
+// its longest region's length + 1, rounded up, at most MD_ACC_LDS
+__host__ __device__ inline int md_acc_cap(int max_lr)
+{ const int c = (max_lr + 1 + 63) & ~63; return c < MD_ACC_LDS ? c : MD_ACC_LDS; }
+__host__ __device__ inline bool md_em_lds(int M, int acc_cap)
+{ return (md_acc_at(md_q(M)) + acc_cap + acc_cap / 4 + md_em_floats(M)) * 4 <= P7G_LDS_MAX; }
+__host__ __device__ inline int64_t md_trace_floats(int M, int acc_cap)
+{ return md_acc_at(md_q(M)) + acc_cap + acc_cap / 4 + (md_em_lds(M, acc_cap) ? md_em_floats(M) : 0); }
 struct MdJob {
   int64_t res_off;
   int64_t area;
@@ -250,11 +254,11 @@ __host__ __device__ inline int64_t md_clu_at(int M, int Lr, int segcap)
 { return md_seg_at(M, Lr) + 5 * (int64_t) segcap; }
 __host__ __device__ inline int64_t md_thr_at(int M, int Lr, int segcap)
 { return (md_clu_at(M, Lr, segcap) + 2 * (int64_t) segcap + 1) & ~(int64_t) 1; }
-size_t  md_lds_bytes(int M, int segcap);
+size_t  md_lds_bytes(int M, int segcap, int acc_cap);
 int64_t md_area_floats(int M, int Lr, int segcap);
 // thr_done (optional): recorded on st once the traces' thresholds are filled
 hipError_t launch_md_regions(const DevProfile &p, const uint8_t *res, const MdJob *jobs, MdOut *out, int njobs,
-                             float *pool, int segcap, int nsamples, uint32_t seed, hipStream_t st,
+                             float *pool, int segcap, int nsamples, uint32_t seed, int max_lr, hipStream_t st,
                              hipEvent_t thr_done = nullptr);
 hipError_t launch_md_copy(uint8_t *res, const int64_t *src, const int64_t *dst, const int32_t *len, int n, hipStream_t st);
 

@@ -126,7 +126,7 @@ struct MdSig { int c, i, j, k, m; float prob; };
 
 __global__ __launch_bounds__(64) void
 k_md_region(const DevProfile p, const uint8_t *__restrict__ res, const MdJob *__restrict__ jobs, MdOut *__restrict__ out,
-            float *__restrict__ pool, int nsamples, uint32_t seed, int phase)
+            float *__restrict__ pool, int nsamples, uint32_t seed, int phase, int acc_cap)
 {
   extern __shared__ float lds[];
   const int z = threadIdx.x;
@@ -337,13 +337,15 @@ k_md_region(const DevProfile p, const uint8_t *__restrict__ res, const MdJob *__
   float *cnt = lds;                                        // [q][r] counts of one sampled domain
   float *n2v = lds + 4 * Q;                                // its null2 odds, 32 codes
   int   *tseg = (int *) (n2v + 32);                        // this trace's domains, last first
-  const bool acc_lds = (Lr + 1) <= MD_ACC_LDS;
+  const bool acc_lds = (Lr + 1) <= acc_cap;
   float *accp = acc_lds ? lds + md_acc_at(Q) : acc;        // residue sums
+  uint8_t *rl = (uint8_t *) (lds + md_acc_at(Q) + acc_cap);   // and the residues, with them in LDS
+  const uint8_t *rsq = acc_lds ? rl : dsq;
   auto cnt_at = [&](int k) {                               // the count cell of model position k
     const int r = (k - 1 >= Q) + (k - 1 >= 2 * Q) + (k - 1 >= 3 * Q);
     return (k - 1 - r * Q) * 4 + r; };
-  const bool em_lds = md_em_lds(M) && p.K <= 20;
-  float *em = lds + md_acc_at(Q) + MD_ACC_LDS;             // [x][q][r] odds, when em_lds
+  const bool em_lds = md_em_lds(M, acc_cap) && p.K <= 20;
+  float *em = lds + md_acc_at(Q) + acc_cap + acc_cap / 4;  // [x][q][r] odds, when em_lds
   if (em_lds)
     for (int c = z; c < p.K * 4 * Q; c += 64) {
       const int x = c / (4 * Q), rem = c - x * 4 * Q;
@@ -351,6 +353,7 @@ k_md_region(const DevProfile p, const uint8_t *__restrict__ res, const MdJob *__
     }
   for (int c = z; c < 4 * Q; c += 64) cnt[c] = 0.0f;
   for (int pos = z; pos <= Lr; pos += 64) accp[pos] = 0.0f;
+  if (acc_lds) for (int pos = z; pos < Lr; pos += 64) rl[pos] = dsq[pos];
   __syncthreads();
   uint32_t rng = md_mix3(seed, 87654321u, 12345678u);
   if (rng == 0) rng = 42;
@@ -541,7 +544,7 @@ k_md_region(const DevProfile p, const uint8_t *__restrict__ res, const MdJob *__
         for (int pos = z + 1 + ((sqfrom >> 6) << 6); pos <= hi; pos += 64) {
           // the lane of a residue is fixed (pos mod 64), so its updates stay in order
           if (pos > sqto)        accp[pos] = __fadd_rn(accp[pos], 1.0f);
-          else if (pos > sqfrom) accp[pos] = __fadd_rn(accp[pos], n2v[dsq[pos - 1]]);
+          else if (pos > sqfrom) accp[pos] = __fadd_rn(accp[pos], n2v[rsq[pos - 1]]);
         }
         hi = sqfrom;
         if (nd >= MD_TSEG) { status = 5; break; }
@@ -703,11 +706,11 @@ k_md_region(const DevProfile p, const uint8_t *__restrict__ res, const MdJob *__
 
 // LDS of one region's workgroup: the Forward's two rows and D->D odds, the
 // traces' counts, odds and per-trace domains, or the clustering's stacks
-size_t md_lds_bytes(int M, int segcap)
+size_t md_lds_bytes(int M, int segcap, int acc_cap)
 {
   const int64_t fwd = (md_stage_t(M) ? md_stage_floats(M) : 0) + 32 * md_q(M) + 128;
   const size_t clu = (size_t) 2 * segcap * 4 + (size_t) (segcap + 63) / 64 * 8 + (size_t) MD_MAXSIG * sizeof(MdSig);
-  return std::max((size_t) std::max(fwd, md_trace_floats(M)) * 4, clu);
+  return std::max((size_t) std::max(fwd, md_trace_floats(M, acc_cap)) * 4, clu);
 }
 
 // The traces' thresholds of every region (md_thresholds over each cell and
@@ -873,10 +876,12 @@ int64_t md_area_floats(int M, int Lr, int segcap)
 }
 
 hipError_t launch_md_regions(const DevProfile &p, const uint8_t *res, const MdJob *jobs, MdOut *out, int njobs,
-                             float *pool, int segcap, int nsamples, uint32_t seed, hipStream_t st, hipEvent_t thr_done)
+                             float *pool, int segcap, int nsamples, uint32_t seed, int max_lr, hipStream_t st,
+                             hipEvent_t thr_done)
 {
   if (njobs <= 0) return hipSuccess;
-  const size_t lds = md_lds_bytes(p.M, segcap);
+  const int acc_cap = md_acc_cap(max_lr);
+  const size_t lds = md_lds_bytes(p.M, segcap, acc_cap);
   if (lds > P7G_LDS_MAX) return hipErrorInvalidValue;
   hipError_t e = hipFuncSetAttribute((const void *) k_md_region, hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds);
   if (e != hipSuccess) return e;
@@ -884,7 +889,7 @@ hipError_t launch_md_regions(const DevProfile &p, const uint8_t *res, const MdJo
   // thresholds (the whole device), then the traces and the clustering
   // (128 floats past the rows: the D->D chains read up to seven cells past q = Q - 1)
   const size_t lds0 = (size_t) ((md_stage_t(p.M) ? md_stage_floats(p.M) : 0) + 32 * md_q(p.M) + 128) * 4;
-  hipLaunchKernelGGL(k_md_region, dim3(njobs), dim3(64), lds0, st, p, res, jobs, out, pool, nsamples, seed, 0);
+  hipLaunchKernelGGL(k_md_region, dim3(njobs), dim3(64), lds0, st, p, res, jobs, out, pool, nsamples, seed, 0, acc_cap);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   for (int j0 = 0; j0 < njobs; j0 += 65535) {
     const int nj = std::min(njobs - j0, 65535);
@@ -892,7 +897,7 @@ hipError_t launch_md_regions(const DevProfile &p, const uint8_t *res, const MdJo
     if ((e = hipGetLastError()) != hipSuccess) return e;
   }
   if (thr_done && (e = hipEventRecord(thr_done, st)) != hipSuccess) return e;
-  hipLaunchKernelGGL(k_md_region, dim3(njobs), dim3(64), lds, st, p, res, jobs, out, pool, nsamples, seed, 1);
+  hipLaunchKernelGGL(k_md_region, dim3(njobs), dim3(64), lds, st, p, res, jobs, out, pool, nsamples, seed, 1, acc_cap);
   return hipGetLastError();
 }
 

@@ -1476,6 +1476,7 @@ struct MdRun {
   std::vector<MdJob> jobs;
   std::vector<std::pair<size_t, size_t>> batches;
   bool launched = false;               // the one batch is on w.st2
+  int max_lr = 0;                      // the longest region
 };
 
 static int
@@ -1484,7 +1485,8 @@ md_start(Work &w, const DevProfile &p, const std::vector<int4> &rec, const std::
 {
   for (int64_t k = 0; k < nregs; k++) if (rec[(size_t) k].w == 1 && vlen[(size_t) k] > 0) m.mdk.push_back(k);
   if (m.mdk.empty()) return p7G_OK;
-  if (md_lds_bytes(p.M, MD_SEGCAP) > P7G_LDS_MAX) { p7g_set_error("model too long for multidomain regions (M=%d)", p.M); return p7G_EINVAL; }
+  for (int64_t k : m.mdk) m.max_lr = std::max(m.max_lr, (int) vlen[(size_t) k]);
+  if (md_lds_bytes(p.M, MD_SEGCAP, md_acc_cap(m.max_lr)) > P7G_LDS_MAX) { p7g_set_error("model too long for multidomain regions (M=%d)", p.M); return p7G_EINVAL; }
   m.voff.resize((size_t) nregs); m.eloop.resize((size_t) nregs); m.emove.resize((size_t) nregs);
   HIPCHK(hipMemcpyAsync(m.voff.data(), w.dout.v_off, 8 * (size_t) nregs, hipMemcpyDeviceToHost, w.st));
   HIPCHK(hipMemcpyAsync(m.eloop.data(), w.dout.env_loop, 4 * (size_t) nregs, hipMemcpyDeviceToHost, w.st));
@@ -1533,7 +1535,7 @@ md_start(Work &w, const DevProfile &p, const std::vector<int4> &rec, const std::
     }
     const int nb = (int) m.jobs.size();
     HIPCHK(hipMemcpyAsync(w.mdjobs, m.jobs.data(), sizeof(MdJob) * (size_t) nb, hipMemcpyHostToDevice, w.st2));
-    HIPCHK(launch_md_regions(p, w.dout.res, w.mdjobs, w.mdout, nb, w.mdpool, MD_SEGCAP, MD_NSAMPLES, MD_SEED, w.st2,
+    HIPCHK(launch_md_regions(p, w.dout.res, w.mdjobs, w.mdout, nb, w.mdpool, MD_SEGCAP, MD_NSAMPLES, MD_SEED, m.max_lr, w.st2,
                              w.md_thr));
     m.launched = true;
   }
@@ -1560,7 +1562,7 @@ md_finish(Work &w, const DevProfile &p, const DevSeqs &s, std::vector<int4> &rec
     n2.assign((size_t) tot_res, 0.0f);
     if (! (m.launched && bi == 0)) {
       HIPCHK(hipMemcpyAsync(w.mdjobs, m.jobs.data() + b.first, sizeof(MdJob) * (size_t) nb, hipMemcpyHostToDevice, ms));
-      HIPCHK(launch_md_regions(p, w.dout.res, w.mdjobs, w.mdout, nb, w.mdpool, MD_SEGCAP, MD_NSAMPLES, MD_SEED, ms));
+      HIPCHK(launch_md_regions(p, w.dout.res, w.mdjobs, w.mdout, nb, w.mdpool, MD_SEGCAP, MD_NSAMPLES, MD_SEED, m.max_lr, ms));
     }
     HIPCHK(hipMemcpyAsync(outs.data() + b.first, w.mdout, sizeof(MdOut) * (size_t) nb, hipMemcpyDeviceToHost, ms));
     int64_t o = 0;
@@ -3375,7 +3377,7 @@ p7_gpu_RegionTraceEnsemble(const uint8_t *dsq, int Lr, int Ltarget, const P7_GPU
   if ((status = check_dsq(dsq, Lr, om)) != p7G_OK) return status;
   if (segcap <= 0) segcap = MD_SEGCAP;
   if ((status = dev_profile(om, Ltarget, &p)) != p7G_OK) return status;
-  if (md_lds_bytes(p->M, segcap) > P7G_LDS_MAX) { p7g_set_error("model too long for multidomain regions (M=%d)", p->M); return p7G_EINVAL; }
+  if (md_lds_bytes(p->M, segcap, md_acc_cap(Lr)) > P7G_LDS_MAX) { p7g_set_error("model too long for multidomain regions (M=%d)", p->M); return p7G_EINVAL; }
   const int64_t Q = std::max(2, (p->M - 1) / 4 + 1);
   const int64_t area = md_area_floats(p->M, Lr, segcap);
   const size_t resb = ((size_t) Lr + 31) & ~(size_t) 15;
@@ -3391,7 +3393,7 @@ p7_gpu_RegionTraceEnsemble(const uint8_t *dsq, int Lr, int Ltarget, const P7_GPU
   hipError_t e = hipSuccess;
   if (e == hipSuccess) e = hipMemcpyAsync(res, dsq + 1, (size_t) Lr, hipMemcpyHostToDevice, g_stream);
   if (e == hipSuccess) e = hipMemcpyAsync(job, &J, sizeof(J), hipMemcpyHostToDevice, g_stream);
-  if (e == hipSuccess) e = launch_md_regions(*p, res, job, out, 1, pool, segcap, nsamples, seed, g_stream);
+  if (e == hipSuccess) e = launch_md_regions(*p, res, job, out, 1, pool, segcap, nsamples, seed, Lr, g_stream);
   if (e == hipSuccess) e = hipMemcpyAsync(&O, out, sizeof(O), hipMemcpyDeviceToHost, g_stream);
   if (e == hipSuccess) e = hipMemcpyAsync(n2sc, pool + md_n2sc_at(p->M, Lr) + 1, sizeof(float) * (size_t) Lr, hipMemcpyDeviceToHost, g_stream);
   if (e == hipSuccess && opt_mx) e = hipMemcpyAsync(opt_mx, pool, sizeof(float) * (size_t) (Lr + 1) * 12 * Q, hipMemcpyDeviceToHost, g_stream);
