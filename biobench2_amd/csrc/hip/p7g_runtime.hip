@@ -1652,11 +1652,12 @@ md_finish(Work &w, const DevProfile &p, const DevSeqs &s, std::vector<int4> &rec
 // fit the pool; per batch, three list launches over the copies: the envelope
 // Backward writing its special and M/I rows, the Forward again summing the
 // posteriors into null2 and the correction and filling the alignment, and the
-// alignment's traceback. Returns after the pass has run (the host vectors are
-// the copies' sources).
+// alignment's traceback; the first batch's Forward waits for fwd_after when
+// one is given. Returns after the pass has run (the host vectors are the
+// copies' sources).
 static int
 null2_pass(Work &w, const DevProfile &p, const DevSeqs &s, const GateThresh &g, const std::vector<int4> &rec,
-           const std::vector<int32_t> &vlen, int64_t k0, int64_t k1)
+           const std::vector<int32_t> &vlen, int64_t k0, int64_t k1, hipEvent_t fwd_after = nullptr)
 {
   // the pool: what the envelopes need, bounded by a quarter of the free memory
   // (the pool already held counts as free)
@@ -1718,6 +1719,7 @@ null2_pass(Work &w, const DevProfile &p, const DevSeqs &s, const GateThresh &g, 
     eb.n2mode = 0;
     HIPCHK(launch_bck(p, vs, bl, c + 16, nb, vb, nullptr, nullptr, nullptr, nullptr, lc, w.st, eb, w.l));
     eb.n2mode = 2;
+    if (b == 0 && fwd_after) HIPCHK(hipStreamWaitEvent(w.st, fwd_after, 0));
     HIPCHK(launch_fwd(p, vs, bl, c + 32, nb, vf, w.l, g, 0, lc, nullptr, 0, nullptr, nullptr, w.st, eb));
     HIPCHK(launch_oa_trace(p, vs, bl, nb, eb, w.st));
   }
@@ -1783,9 +1785,10 @@ null2_regions(Work &w, const DevProfile &p, const DevSeqs &s, int64_t nregs, con
   HIPCHK(hipMemcpyAsync(w.n2poff, poff.data(), sizeof(int64_t) * (size_t) nregs, hipMemcpyHostToDevice, w.st));
   HIPCHK(hipStreamSynchronize(w.st));          // poff is the copy's source
   // the null2 pass's persistent workgroups would hold the compute units the
-  // thresholds' grid needs: it starts once they are filled, beside the traces
-  if (m.launched) HIPCHK(hipStreamWaitEvent(w.st, w.md_thr, 0));
-  if ((status = null2_pass(w, p, s, g, rec, vlen, 0, nregs)) != p7G_OK) return status;
+  // thresholds' grid needs: its Backward runs beside the regions' Forward
+  // (one wave per region), its Forward once the thresholds are filled,
+  // beside the traces
+  if ((status = null2_pass(w, p, s, g, rec, vlen, 0, nregs, m.launched ? w.md_thr : nullptr)) != p7G_OK) return status;
   if ((status = md_finish(w, p, s, rec, vlen, nregs, g, env_dc, reg_sum, m)) != p7G_OK) return status;
   *ret_ms_md = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   const int64_t nrec = (int64_t) rec.size();
