@@ -500,33 +500,47 @@ k_md_region(const DevProfile p, const uint8_t *__restrict__ res, const MdJob *__
         // canonical residue, their hsum; degenerate codes the mean
         md_sync();
         const float nrm = (float) (1.0 / (double) (float) Ld);
-        for (int base = 0; base < p.K * 4; base += 64) {
-          const int xr = base + z, x = xr >> 2, r = xr & 3;
-          float sv = 0.0f;
-          if (x < p.K) {
+        // lane z sums x = (base + z) / 4 and, for base + 64 + z, x + 16 (the
+        // same lane r, so the same counts): two chains per lane, one pass
+        for (int base = 0; base < p.K * 4; base += 128) {
+          const int xr = base + z, x = xr >> 2, r = xr & 3, x2 = x + 16;
+          const bool h1 = x < p.K, h2 = x2 < p.K;
+          float sv = 0.0f, sv2 = 0.0f;
+          if (h1) {
             // positions kmin..kmax in lane r: the other cells' counts are 0, and
             // adding their products (+0) leaves the striped sum unchanged
             const int q0 = max(0, kmin - 1 - r * Q), q1 = min(Q - 1, kmax - 1 - r * Q);
             for (int qa = q0; qa <= q1; qa += 8) {
-              float e[8];
+              float e[8], e2[8], cq[8];
+#pragma unroll
+              for (int u = 0; u < 8; u++) cq[u] = cnt[min(qa + u, Q - 1) * 4 + r];
               if (em_lds) {
 #pragma unroll
-                for (int u = 0; u < 8; u++) e[u] = em[(x * Q + min(qa + u, Q - 1)) * 4 + r];
+                for (int u = 0; u < 8; u++) {
+                  e[u] = em[(x * Q + min(qa + u, Q - 1)) * 4 + r];
+                  e2[u] = h2 ? em[(x2 * Q + min(qa + u, Q - 1)) * 4 + r] : 0.0f;
+                }
               } else {
 #pragma unroll
-                for (int u = 0; u < 8; u++) e[u] = md_tv(p, x, min(r * Q + qa + u + 1, M + 1));
+                for (int u = 0; u < 8; u++) {
+                  e[u] = md_tv(p, x, min(r * Q + qa + u + 1, M + 1));
+                  e2[u] = h2 ? md_tv(p, x2, min(r * Q + qa + u + 1, M + 1)) : 0.0f;
+                }
               }
 #pragma unroll
               for (int u = 0; u < 8; u++)
-                if (qa + u <= q1) {
-                  const float cq = cnt[(qa + u) * 4 + r];
-                  if (cq != 0.0f) sv = __fadd_rn(sv, __fmul_rn(__fmul_rn(cq, nrm), e[u]));
+                if (qa + u <= q1 && cq[u] != 0.0f) {
+                  const float w = __fmul_rn(cq[u], nrm);
+                  sv = __fadd_rn(sv, __fmul_rn(w, e[u]));
+                  sv2 = __fadd_rn(sv2, __fmul_rn(w, e2[u]));
                 }
             }
           }
           // lanes 4x..4x+3 hold x's lane sums
           const float s01 = __fadd_rn(sv, __shfl_down(sv, 1, 4)), s23 = __shfl_down(s01, 2, 4);
-          if (x < p.K && r == 0) n2v[x] = __fadd_rn(__fadd_rn(s01, s23), 0.0f);
+          if (h1 && r == 0) n2v[x] = __fadd_rn(__fadd_rn(s01, s23), 0.0f);
+          const float t01 = __fadd_rn(sv2, __shfl_down(sv2, 1, 4)), t23 = __shfl_down(t01, 2, 4);
+          if (h2 && r == 0) n2v[x2] = __fadd_rn(__fadd_rn(t01, t23), 0.0f);
         }
         md_sync();
         {                                                  // degenerate codes: one lane each
