@@ -527,13 +527,14 @@ k_md_region(const DevProfile p, const uint8_t *__restrict__ res, const MdJob *__
                   e2[u] = h2 ? md_tv(p, x2, min(r * Q + qa + u + 1, M + 1)) : 0.0f;
                 }
               }
+              // a zero count adds +0.0, which leaves the non-negative sums as
+              // they are: no branch per cell
 #pragma unroll
-              for (int u = 0; u < 8; u++)
-                if (qa + u <= q1 && cq[u] != 0.0f) {
-                  const float w = __fmul_rn(cq[u], nrm);
-                  sv = __fadd_rn(sv, __fmul_rn(w, e[u]));
-                  sv2 = __fadd_rn(sv2, __fmul_rn(w, e2[u]));
-                }
+              for (int u = 0; u < 8; u++) {
+                const float w = __fmul_rn(qa + u <= q1 ? cq[u] : 0.0f, nrm);
+                sv = __fadd_rn(sv, __fmul_rn(w, e[u]));
+                sv2 = __fadd_rn(sv2, __fmul_rn(w, e2[u]));
+              }
             }
           }
           // lanes 4x..4x+3 hold x's lane sums
