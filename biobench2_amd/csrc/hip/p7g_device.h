@@ -232,7 +232,8 @@ __host__ __device__ inline int64_t md_trace_floats(int M, int acc_cap)
 struct MdJob {
   int64_t res_off;
   int64_t area;
-  int32_t Lr, Ltarget, segcap, pad;
+  int32_t Lr, Ltarget, segcap;
+  int32_t n2o;          // the region's first residue in its launch's gathered null2 scores
 };
 // status 0, 1 (a traceback choice failed: the reference raises an exception),
 // or 5 (a capacity above was exceeded); nc envelopes ci..cj, 1-based in the
@@ -261,6 +262,8 @@ hipError_t launch_md_regions(const DevProfile &p, const uint8_t *res, const MdJo
                              float *pool, int segcap, int nsamples, uint32_t seed, int max_lr, hipStream_t st,
                              hipEvent_t thr_done = nullptr);
 hipError_t launch_md_copy(uint8_t *res, const int64_t *src, const int64_t *dst, const int32_t *len, int n, hipStream_t st);
+// the residue null2 scores of the launch's regions, job j's Lr of them at dst + jobs[j].n2o
+hipError_t launch_md_n2_gather(const MdJob *jobs, int njobs, const float *pool, int M, float *dst, hipStream_t st);
 
 // Length model of one call. on == 0: every target uses its own length
 // (hmmsearch's per-target ReconfigLength); on == 1: the caller's current

@@ -930,6 +930,23 @@ __global__ void k_md_copy(uint8_t *__restrict__ res, const int64_t *__restrict__
     res[dst[e] + b] = (b < Ld) ? res[src[e] + b] : (uint8_t) P7G_PADCODE;
 }
 
+// The regions' residue null2 scores (their areas' residues 1..Lr), gathered
+// in job order for one copy to the host: one workgroup per region.
+__global__ void k_md_n2_gather(const MdJob *__restrict__ jobs, const float *__restrict__ pool, int M,
+                               float *__restrict__ dst)
+{
+  const MdJob J = jobs[blockIdx.x];
+  const float *src = pool + J.area + md_n2sc_at(M, J.Lr) + 1;
+  for (int i = threadIdx.x; i < J.Lr; i += blockDim.x) dst[(size_t) J.n2o + i] = src[i];
+}
+
+hipError_t launch_md_n2_gather(const MdJob *jobs, int njobs, const float *pool, int M, float *dst, hipStream_t st)
+{
+  if (njobs <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_md_n2_gather, dim3(njobs), dim3(256), 0, st, jobs, pool, M, dst);
+  return hipGetLastError();
+}
+
 hipError_t launch_md_copy(uint8_t *res, const int64_t *src, const int64_t *dst, const int32_t *len, int n, hipStream_t st)
 {
   if (n <= 0) return hipSuccess;

@@ -74,6 +74,8 @@ struct Work {
   int64_t     md_cap = 0;
   float      *mdpool = nullptr;           // the multidomain regions' areas (grown on use)
   int64_t     mdsize = 0;
+  float      *mdn2 = nullptr;             // their residue null2 scores, gathered for the host
+  int64_t     mdn2_cap = 0;
   hipStream_t st2 = nullptr;              // their stream, beside st (made on first use)
   hipEvent_t  md_thr = nullptr;           // on st2: the regions' traces' thresholds are filled
   void       *dom_mem = nullptr;
@@ -995,6 +997,8 @@ work_release(Work &w)
   w.mdjobs = nullptr; w.mdout = nullptr; w.mdsrc = w.mddst = nullptr; w.mdlen = w.mdcnt = nullptr; w.md_cap = 0;
   if (w.mdpool) (void) hipFree(w.mdpool);
   w.mdpool = nullptr; w.mdsize = 0;
+  if (w.mdn2) (void) hipFree(w.mdn2);
+  w.mdn2 = nullptr; w.mdn2_cap = 0;
   if (w.mem) (void) hipFree(w.mem);
   if (w.pool) (void) hipFree(w.pool);
   if (w.bnd) (void) hipFree(w.bnd);
@@ -1510,23 +1514,32 @@ md_start(Work &w, const DevProfile &p, const std::vector<int4> &rec, const std::
   int64_t need_all = 0;
   for (int64_t k : m.mdk) need_all += md_area_floats(p.M, vlen[(size_t) k], MD_SEGCAP);
   const int64_t cap = std::max<int64_t>(N2_POOL_MIN, std::min<int64_t>(need_all, (int64_t) (fr / 4 / sizeof(float)) + w.mdsize));
-  int64_t used = 0, maxused = 0;
+  int64_t used = 0, maxused = 0, nres = 0, maxres = 0;
   size_t start = 0;
   for (int64_t k : m.mdk) {
     const int Lr = vlen[(size_t) k];
     const int64_t need = md_area_floats(p.M, Lr, MD_SEGCAP);
     if (need > cap) { p7g_set_error("multidomain region of %d residues too long for the pool", Lr); return p7G_EMEM; }
-    if (used + need > cap || (int64_t) (m.jobs.size() - start) >= w.md_cap) { m.batches.push_back({ start, m.jobs.size() }); start = m.jobs.size(); used = 0; }
+    if (used + need > cap || (int64_t) (m.jobs.size() - start) >= w.md_cap || nres + Lr > INT32_MAX) {
+      m.batches.push_back({ start, m.jobs.size() }); start = m.jobs.size(); used = 0; nres = 0;
+    }
     MdJob J;
-    J.res_off = m.voff[(size_t) k]; J.area = used; J.Lr = Lr; J.Ltarget = h_len[rec[(size_t) k].x]; J.segcap = MD_SEGCAP; J.pad = 0;
+    J.res_off = m.voff[(size_t) k]; J.area = used; J.Lr = Lr; J.Ltarget = h_len[rec[(size_t) k].x]; J.segcap = MD_SEGCAP;
+    J.n2o = (int32_t) nres;
     m.jobs.push_back(J);
     used += need; maxused = std::max(maxused, used);
+    nres += Lr; maxres = std::max(maxres, nres);
   }
   if (m.jobs.size() > start) m.batches.push_back({ start, m.jobs.size() });
   if (w.mdsize < maxused) {
     if (w.mdpool) { (void) hipFree(w.mdpool); w.mdpool = nullptr; w.mdsize = 0; }
     HIPCHK(hipMalloc((void **) &w.mdpool, sizeof(float) * (size_t) maxused));
     w.mdsize = maxused;
+  }
+  if (w.mdn2_cap < maxres) {
+    if (w.mdn2) { (void) hipFree(w.mdn2); w.mdn2 = nullptr; w.mdn2_cap = 0; }
+    HIPCHK(hipMalloc((void **) &w.mdn2, sizeof(float) * (size_t) maxres));
+    w.mdn2_cap = maxres;
   }
   if (m.batches.size() == 1) {
     if (! w.st2) {
@@ -1564,15 +1577,11 @@ md_finish(Work &w, const DevProfile &p, const DevSeqs &s, std::vector<int4> &rec
       HIPCHK(hipMemcpyAsync(w.mdjobs, m.jobs.data() + b.first, sizeof(MdJob) * (size_t) nb, hipMemcpyHostToDevice, ms));
       HIPCHK(launch_md_regions(p, w.dout.res, w.mdjobs, w.mdout, nb, w.mdpool, MD_SEGCAP, MD_NSAMPLES, MD_SEED, m.max_lr, ms));
     }
+    HIPCHK(launch_md_n2_gather(w.mdjobs, nb, w.mdpool, p.M, w.mdn2, ms));
     HIPCHK(hipMemcpyAsync(outs.data() + b.first, w.mdout, sizeof(MdOut) * (size_t) nb, hipMemcpyDeviceToHost, ms));
-    int64_t o = 0;
-    for (size_t a = b.first; a < b.second; a++) {
-      HIPCHK(hipMemcpyAsync(n2.data() + o, w.mdpool + m.jobs[a].area + md_n2sc_at(p.M, m.jobs[a].Lr) + 1,
-                            sizeof(float) * (size_t) m.jobs[a].Lr, hipMemcpyDeviceToHost, ms));
-      o += m.jobs[a].Lr;
-    }
+    HIPCHK(hipMemcpyAsync(n2.data(), w.mdn2, sizeof(float) * (size_t) tot_res, hipMemcpyDeviceToHost, ms));
     HIPCHK(hipStreamSynchronize(ms));
-    o = 0;
+    int64_t o = 0;
     for (size_t a = b.first; a < b.second; a++) {
       const MdOut &r = outs[a];
       const int64_t k = m.mdk[a];
@@ -3391,7 +3400,7 @@ p7_gpu_RegionTraceEnsemble(const uint8_t *dsq, int Lr, int Ltarget, const P7_GPU
   MdJob *job = (MdJob *) (res + resb);
   MdOut *out = (MdOut *) (job + 1);
   MdJob J;
-  J.res_off = 0; J.area = 0; J.Lr = Lr; J.Ltarget = Ltarget; J.segcap = segcap; J.pad = 0;
+  J.res_off = 0; J.area = 0; J.Lr = Lr; J.Ltarget = Ltarget; J.segcap = segcap; J.n2o = 0;
   MdOut O;
   hipError_t e = hipSuccess;
   if (e == hipSuccess) e = hipMemcpyAsync(res, dsq + 1, (size_t) Lr, hipMemcpyHostToDevice, g_stream);
