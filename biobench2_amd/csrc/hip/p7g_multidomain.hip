@@ -235,23 +235,19 @@ k_md_region(const DevProfile p, const uint8_t *__restrict__ res, const MdJob *__
         // buffers) and feed no result.
         float xm = 0.0f;
         {
-          float nm[8], nd[8], nt[8];
+          // two chunk buffers in turn: the next chunk loads into one while
+          // the other's chain runs
+          float am[8], ad[8], at[8], bm[8], bd[8], bt[8];
+          auto ld = [&](int q0, float (&m)[8], float (&d)[8], float (&t)[8]) {
 #pragma unroll
-          for (int u = 0; u < 8; u++) { nm[u] = cm[12*u]; nd[u] = cd[12*u]; nt[u] = tq[4*u]; }
-          for (int q0 = 0; q0 < Q; q0 += 8) {
-            float am[8], ad[8], at[8];
-#pragma unroll
-            for (int u = 0; u < 8; u++) { am[u] = nm[u]; ad[u] = nd[u]; at[u] = nt[u]; }
-            if (q0 + 8 < Q) {
-#pragma unroll
-              for (int u = 0; u < 8; u++) { nm[u] = cm[12*(q0+8+u)]; nd[u] = cd[12*(q0+8+u)]; nt[u] = tq[4*(q0+8+u)]; }
-            }
+            for (int u = 0; u < 8; u++) { m[u] = cm[12*(q0+u)]; d[u] = cd[12*(q0+u)]; t[u] = tq[4*(q0+u)]; }
+          };
+          auto run = [&](int q0, const float (&m)[8], const float (&d)[8], const float (&t)[8]) {
             auto step = [&](int u) {
-              xm = __fadd_rn(xm, am[u]);
-              const float d = __fadd_rn(dcv, ad[u]);
-              ad[u] = d;
-              dcv = __fmul_rn(d, at[u]);
-              cd[12*(q0+u)] = d;
+              xm = __fadd_rn(xm, m[u]);
+              const float dd = __fadd_rn(dcv, d[u]);
+              dcv = __fmul_rn(dd, t[u]);
+              cd[12*(q0+u)] = dd;
             };
             if (q0 + 8 <= Q) {
 #pragma unroll
@@ -260,6 +256,14 @@ k_md_region(const DevProfile p, const uint8_t *__restrict__ res, const MdJob *__
 #pragma unroll
               for (int u = 0; u < 8; u++) if (q0 + u < Q) step(u);
             }
+          };
+          ld(0, am, ad, at);
+          for (int q0 = 0; q0 < Q; q0 += 16) {
+            if (q0 + 8 < Q) ld(q0 + 8, bm, bd, bt);
+            run(q0, am, ad, at);
+            if (q0 + 8 >= Q) break;
+            if (q0 + 16 < Q) ld(q0 + 16, am, ad, at);
+            run(q0 + 8, bm, bd, bt);
           }
         }
         for (int pass = 1; pass < 4; pass++) {
@@ -267,23 +271,18 @@ k_md_region(const DevProfile p, const uint8_t *__restrict__ res, const MdJob *__
           dcv = r ? sh : 0.0f;
           int changed = 0;
           float xd = xm;
-          float nd[8], nt[8];
+          float ad[8], at[8], bd[8], bt[8];
+          auto ld = [&](int q0, float (&d)[8], float (&t)[8]) {
 #pragma unroll
-          for (int u = 0; u < 8; u++) { nd[u] = cd[12*u]; nt[u] = tq[4*u]; }
-          for (int q0 = 0; q0 < Q; q0 += 8) {
-            float ad[8], at[8];
-#pragma unroll
-            for (int u = 0; u < 8; u++) { ad[u] = nd[u]; at[u] = nt[u]; }
-            if (q0 + 8 < Q) {
-#pragma unroll
-              for (int u = 0; u < 8; u++) { nd[u] = cd[12*(q0+8+u)]; nt[u] = tq[4*(q0+8+u)]; }
-            }
+            for (int u = 0; u < 8; u++) { d[u] = cd[12*(q0+u)]; t[u] = tq[4*(q0+u)]; }
+          };
+          auto run = [&](int q0, const float (&d)[8], const float (&t)[8]) {
             auto step = [&](int u) {
-              const float d = __fadd_rn(dcv, ad[u]);
-              changed |= (d > ad[u]);
-              dcv = __fmul_rn(dcv, at[u]);
-              xd = __fadd_rn(d, xd);
-              cd[12*(q0+u)] = d;
+              const float dd = __fadd_rn(dcv, d[u]);
+              changed |= (dd > d[u]);
+              dcv = __fmul_rn(dcv, t[u]);
+              xd = __fadd_rn(dd, xd);
+              cd[12*(q0+u)] = dd;
             };
             if (q0 + 8 <= Q) {
 #pragma unroll
@@ -292,6 +291,14 @@ k_md_region(const DevProfile p, const uint8_t *__restrict__ res, const MdJob *__
 #pragma unroll
               for (int u = 0; u < 8; u++) if (q0 + u < Q) step(u);
             }
+          };
+          ld(0, ad, at);
+          for (int q0 = 0; q0 < Q; q0 += 16) {
+            if (q0 + 8 < Q) ld(q0 + 8, bd, bt);
+            run(q0, ad, at);
+            if (q0 + 8 >= Q) break;
+            if (q0 + 16 < Q) ld(q0 + 16, ad, at);
+            run(q0 + 8, bd, bt);
           }
           xe = xd;
           const bool any = (__ballot(changed) & 0xfull) != 0ull;

@@ -1762,6 +1762,8 @@ null2_regions(Work &w, const DevProfile &p, const DevSeqs &s, int64_t nregs, con
   int status;
   const auto t0 = std::chrono::steady_clock::now();
   MdRun m;
+  // an early return leaves no region launch running on st2 behind it
+  struct Drain { Work &w; const MdRun &m; ~Drain() { if (m.launched && w.st2) (void) hipStreamSynchronize(w.st2); } } drain{ w, m };
   if ((status = md_start(w, p, rec, vlen, nregs, h_len, m)) != p7G_OK) return status;
   // every record's status, correction and alignment start as "not rescored"
   auto clear = [&](int64_t k0, int64_t k1) -> int {
