@@ -735,19 +735,15 @@ size_t md_lds_bytes(int M, int segcap, int acc_cap)
   return std::max((size_t) std::max(fwd, md_trace_floats(M, acc_cap)) * 4, clu);
 }
 
-// The traces' thresholds of every region (md_thresholds over each cell and
-// row of its finished Forward matrix): a grid of gridDim.x workgroups of
+// The traces' cell thresholds of every region (md_thresholds over each cell
+// of its finished Forward matrix): a grid of gridDim.x workgroups of
 // MD_THR_NT threads per region (blockIdx.y). Every choice a trace can make
-// depends only on the finished matrix, so its thresholds are computed first
-// for every cell and row, with the same operations p7_StochasticTrace
-// performs at the step (md_thresholds); a step is then one draw and one to
-// three comparisons. thr[(i, k)]: M's three (B | M | I | D), D's one (M | D),
-// I's one (M | I); xthr[i]: C's (C | E), J's (J | E), B's (N | J); ecum[i]:
-// select_e's running sum over row i in the striped order (q outer; M lanes,
-// D lanes), kept at the end of every 64 cells and the last (md_ncp per
-// row). A cell's five thresholds go out through LDS, so that the stores of
-// a wave are contiguous; a row's sums are a chain of 8Q that one thread
-// computes.
+// depends only on the finished matrix, so its thresholds are computed first,
+// with the same operations p7_StochasticTrace performs at the step
+// (md_thresholds); a step is then one draw and one to three comparisons.
+// thr[(i, k)]: M's three (B | M | I | D), D's one (M | D), I's one (M | I),
+// as doubles. A pass's five-double records go out through LDS, so that the
+// stores of a wave are contiguous. The rows' thresholds are k_md_rows'.
 #define MD_THR_NT 256
 #define MD_THR_CELLS 2                                     // cells per thread per pass
 #define MD_THR_LDS_OUT ((size_t) 8 * (MD_THR_CELLS * MD_THR_NT * 5 > MD_THR_NT * 8 ? MD_THR_CELLS * MD_THR_NT * 5 : MD_THR_NT * 8))
@@ -762,9 +758,8 @@ k_md_thr(const DevProfile p, const MdJob *__restrict__ jobs, float *__restrict__
 {
   const int t = threadIdx.x, NT = MD_THR_NT;
   const MdJob J = jobs[blockIdx.y];
-  const int M = p.M, Q = max(2, (M - 1) / 4 + 1), Lr = J.Lr, L = J.Ltarget;
+  const int M = p.M, Q = max(2, (M - 1) / 4 + 1), Lr = J.Lr;
   const int TR = p.nrows;
-  const float xEmove = p.xfE_move, xEloop = p.xfE_loop, floop = p.xf_loop[L], fmove = p.xf_move[L];
   float *mat = pool + J.area;
   float *xmx = mat + (size_t) (Lr + 1) * 12 * Q;
   extern __shared__ double ldsd[];
@@ -780,8 +775,6 @@ k_md_thr(const DevProfile p, const MdJob *__restrict__ jobs, float *__restrict__
   enum { T_BM = 0, T_MM, T_IM, T_DM, T_MDS, T_DDS, T_MI, T_II, T_MD, T_DD };
 #define CELL(i_, q_, s_, r_) mat[(((size_t) (i_) * Q + (q_)) * 3 + (s_)) * 4 + (r_)]
   double *thr = (double *) (pool + J.area + md_thr_at(M, Lr, J.segcap));
-  double *xthr = thr + (size_t) 5 * (Lr + 1) * (M + 1);
-  double *ecum = xthr + (size_t) 3 * (Lr + 1);
   auto cellk = [&](int i, int k, int s) {                  // the cell of model position k (1..4Q) in row i
     const int r = (k - 1 >= Q) + (k - 1 >= 2 * Q) + (k - 1 >= 3 * Q), q = k - 1 - r * Q;
     return CELL(i, q, s, r); };
@@ -847,10 +840,34 @@ k_md_thr(const DevProfile p, const MdJob *__restrict__ jobs, float *__restrict__
     }
     __syncthreads();
   }
-  // rows: one thread each; the running sum's checkpoints (every 64th
-  // element and the last) go out as they are reached
-  const int NCP = md_ncp(Q);
-  for (int i = blockIdx.x * NT + t; i <= Lr; i += gridDim.x * NT) {
+#undef CELL
+}
+
+// The rows' thresholds of every region: C's (C | E), J's (J | E) and B's
+// (N | J), and the E state's running-sum checkpoints (every 64th cell and
+// the last, md_ncp per row). One wave per 64 rows (blockIdx.x) of a region
+// (blockIdx.y). A row's sum is a chain of 8Q double additions that its lane
+// runs; the wave loads the 64 rows' M and D values eight q at a time into
+// LDS, so that its loads run along the rows.
+__global__ __launch_bounds__(64) void
+k_md_rows(const DevProfile p, const MdJob *__restrict__ jobs, float *__restrict__ pool)
+{
+  const int t = threadIdx.x;
+  const MdJob J = jobs[blockIdx.y];
+  const int M = p.M, Q = max(2, (M - 1) / 4 + 1), Lr = J.Lr, L = J.Ltarget;
+  const int r0 = blockIdx.x * 64;
+  if (r0 > Lr) return;
+  const float xEmove = p.xfE_move, xEloop = p.xfE_loop, floop = p.xf_loop[L], fmove = p.xf_move[L];
+  const float *mat = pool + J.area;
+  const float *xmx = mat + (size_t) (Lr + 1) * 12 * Q;
+  double *thr = (double *) (pool + J.area + md_thr_at(M, Lr, J.segcap));
+  double *xthr = thr + (size_t) 5 * (Lr + 1) * (M + 1);
+  double *ecum = xthr + (size_t) 3 * (Lr + 1);
+  __shared__ float4 tile[64 * 17];                         // [row][q][M, D halves], 17 float4 a row
+  const int i = r0 + t;
+  const bool row = (i >= 1 && i <= Lr);
+  float nf = 0.0f;
+  if (i <= Lr) {
     float v[2];
     if (i >= 1) {
       v[0] = __fmul_rn(xmx[(size_t) (i - 1) * 6 + 4], floop);
@@ -860,32 +877,37 @@ k_md_thr(const DevProfile p, const MdJob *__restrict__ jobs, float *__restrict__
       v[1] = __fmul_rn(__fmul_rn(xmx[(size_t) i * 6 + 0], xEloop), xmx[(size_t) i * 6 + 5]);
       md_thresholds(v, 2, xthr + (size_t) i * 3 + 1);
       // select_e: norm = 1/E(i) in double, the products in float, summed in double
-      const float nf = (float) (1.0 / (double) xmx[(size_t) i * 6 + 0]);
-      double sum = 0.0, *cp = ecum + (size_t) i * NCP;
-      // one checkpoint's eight q at a time, their 64 values loaded together
-      for (int q0 = 0; q0 < Q; q0 += 8) {
-        float u[8][8];
-#pragma unroll
-        for (int a = 0; a < 8; a++) {
-          const float4 *cn = (const float4 *) (mat + ((size_t) i * Q + min(q0 + a, Q - 1)) * 12);
-          const float4 x0 = cn[0], x1 = cn[1];
-          u[a][0] = x0.x; u[a][1] = x0.y; u[a][2] = x0.z; u[a][3] = x0.w;
-          u[a][4] = x1.x; u[a][5] = x1.y; u[a][6] = x1.z; u[a][7] = x1.w;
-        }
-#pragma unroll
-        for (int a = 0; a < 8; a++)
-          if (q0 + a < Q) {
-#pragma unroll
-            for (int e = 0; e < 8; e++) sum = sum + (double) __fmul_rn(u[a][e], nf);
-          }
-        cp[q0 >> 3] = sum;                                       // after element min(8 (q0 + 8), 8Q) - 1
-      }
+      nf = (float) (1.0 / (double) xmx[(size_t) i * 6 + 0]);
     }
     v[0] = __fmul_rn(xmx[(size_t) i * 6 + 1], fmove);
     v[1] = __fmul_rn(xmx[(size_t) i * 6 + 2], fmove);
     md_thresholds(v, 2, xthr + (size_t) i * 3 + 2);
   }
-#undef CELL
+  const int NCP = md_ncp(Q);
+  double sum = 0.0;
+  for (int q0 = 0; q0 < Q; q0 += 8) {
+    // lane t loads parts j = t + 64 m: row j / 16, q q0 + (j % 16) / 2, half j % 2
+#pragma unroll
+    for (int m = 0; m < 16; m++) {
+      const int j = t + 64 * m, rw = j >> 4, part = j & 15;
+      const int ii = min(r0 + rw, Lr), q = min(q0 + (part >> 1), Q - 1);   // past the ends: a valid cell, unused
+      tile[rw * 17 + part] = *(const float4 *) (mat + ((size_t) ii * Q + q) * 12 + (part & 1) * 4);
+    }
+    __syncthreads();
+    if (row) {
+#pragma unroll
+      for (int a = 0; a < 8; a++)
+        if (q0 + a < Q) {
+          const float4 x0 = tile[t * 17 + 2 * a], x1 = tile[t * 17 + 2 * a + 1];
+          sum = sum + (double) __fmul_rn(x0.x, nf); sum = sum + (double) __fmul_rn(x0.y, nf);
+          sum = sum + (double) __fmul_rn(x0.z, nf); sum = sum + (double) __fmul_rn(x0.w, nf);
+          sum = sum + (double) __fmul_rn(x1.x, nf); sum = sum + (double) __fmul_rn(x1.y, nf);
+          sum = sum + (double) __fmul_rn(x1.z, nf); sum = sum + (double) __fmul_rn(x1.w, nf);
+        }
+      ecum[(size_t) i * NCP + (q0 >> 3)] = sum;                 // after element min(8 (q0 + 8), 8Q) - 1
+    }
+    __syncthreads();
+  }
 }
 
 // floats of one region's area in the pool: matrix, specials, residue sums,
@@ -916,6 +938,8 @@ hipError_t launch_md_regions(const DevProfile &p, const uint8_t *res, const MdJo
   for (int j0 = 0; j0 < njobs; j0 += 65535) {
     const int nj = std::min(njobs - j0, 65535);
     hipLaunchKernelGGL(k_md_thr, dim3(32, nj), dim3(MD_THR_NT), md_thr_tl_bytes(p.M) + MD_THR_LDS_OUT, st, p, jobs + j0, pool);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    hipLaunchKernelGGL(k_md_rows, dim3((max_lr + 1 + 63) / 64, nj), dim3(64), 0, st, p, jobs + j0, pool);
     if ((e = hipGetLastError()) != hipSuccess) return e;
   }
   if (thr_done && (e = hipEventRecord(thr_done, st)) != hipSuccess) return e;
