@@ -178,11 +178,26 @@ k_md_region(const DevProfile p, const uint8_t *__restrict__ res, const MdJob *__
 #pragma unroll
       for (int u = 0; u < 8; u++) { const int c = u * 64 + z; enext[u] = md_tv(p, x, (c & 3) * Q + (c >> 2) + 1); }
     };
-    if (pre) load_row(1);
+    // with the row's odds in registers (pre), lane z's cells are c = 64 u + z
+    // for u < 8, and their transitions are the row loop's constants: in
+    // registers too
+    float ecur[8], tBMr[8], tMMr[8], tIMr[8], tDMr[8], tMIr[8], tIIr[8], tMDr[8];
+    if (pre) {
+#pragma unroll
+      for (int u = 0; u < 8; u++) {
+        const int c = u * 64 + z, q = c >> 2, r = c & 3, k = r * Q + q + 1, kk = min(k, M + 1);
+        const bool in = c < 4 * Q;
+        tBMr[u] = in ? tv(T_BM, kk) : 0.0f; tMMr[u] = in ? tv(T_MM, kk) : 0.0f;
+        tIMr[u] = in ? tv(T_IM, kk) : 0.0f; tDMr[u] = in ? tv(T_DM, kk) : 0.0f;
+        tMIr[u] = in ? tv(T_MI, kk) : 0.0f; tIIr[u] = in ? tv(T_II, kk) : 0.0f;
+        tMDr[u] = in ? tv(T_MD, min(k - 1, M + 1)) : 0.0f;
+      }
+      load_row(1);
+    }
     for (int i = 1; i <= Lr; i++) {
       if (pre) {
 #pragma unroll
-        for (int u = 0; u < 8; u++) { const int c = u * 64 + z; if (c < 4 * Q) erow[c] = enext[u]; }
+        for (int u = 0; u < 8; u++) ecur[u] = enext[u];
         if (i < Lr) load_row(i + 1);
       } else {
         const int x = dsq[i - 1];
@@ -195,6 +210,32 @@ k_md_region(const DevProfile p, const uint8_t *__restrict__ res, const MdJob *__
         }
       }
       md_sync();
+      if (pre) {
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+          const int c = u * 64 + z, q = c >> 2, r = c & 3;
+          if (c < 4 * Q) {
+            float mp = 0.0f, dp = 0.0f, ip = 0.0f;       // row i-1 at k-1 (esl_sse_rightshift_ps at q = 0)
+            if (q > 0)      { mp = prev[((q-1)*3+0)*4 + r]; dp = prev[((q-1)*3+1)*4 + r]; ip = prev[((q-1)*3+2)*4 + r]; }
+            else if (r > 0) { mp = prev[((Q-1)*3+0)*4 + r-1]; dp = prev[((Q-1)*3+1)*4 + r-1]; ip = prev[((Q-1)*3+2)*4 + r-1]; }
+            float sv = __fmul_rn(xB, tBMr[u]);
+            sv = __fadd_rn(sv, __fmul_rn(mp, tMMr[u]));
+            sv = __fadd_rn(sv, __fmul_rn(ip, tIMr[u]));
+            sv = __fadd_rn(sv, __fmul_rn(dp, tDMr[u]));
+            sv = __fmul_rn(sv, ecur[u]);
+            cur[(q*3+0)*4 + r] = sv;
+            cur[(q*3+2)*4 + r] = __fadd_rn(__fmul_rn(prev[(q*3+0)*4 + r], tMIr[u]),
+                                           __fmul_rn(prev[(q*3+2)*4 + r], tIIr[u]));
+          }
+        }
+        md_sync();
+        // D before the D->D passes: cell q holds M(q-1)·tMD (the q loop's dcv); q = 0 holds 0
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+          const int c = u * 64 + z, q = c >> 2, r = c & 3;
+          if (c < 4 * Q) cur[(q*3+1)*4 + r] = q ? __fmul_rn(cur[((q-1)*3+0)*4 + r], tMDr[u]) : 0.0f;
+        }
+      } else {
       for (int c = z; c < 4 * Q; c += 64) {
         const int q = c >> 2, r = c & 3, k = r * Q + q + 1, kk = min(k, M + 1);
         float mp = 0.0f, dp = 0.0f, ip = 0.0f;           // row i-1 at k-1 (esl_sse_rightshift_ps at q = 0)
@@ -214,6 +255,7 @@ k_md_region(const DevProfile p, const uint8_t *__restrict__ res, const MdJob *__
       for (int c = z; c < 4 * Q; c += 64) {
         const int q = c >> 2, r = c & 3, k = r * Q + q + 1;
         cur[(q*3+1)*4 + r] = q ? __fmul_rn(cur[((q-1)*3+0)*4 + r], tv(T_MD, min(k - 1, M + 1))) : 0.0f;
+      }
       }
       md_sync();
       float xe = 0.0f;
