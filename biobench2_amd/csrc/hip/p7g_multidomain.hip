@@ -417,6 +417,7 @@ k_md_region(const DevProfile p, const uint8_t *__restrict__ res, const MdJob *__
     int64_t steps = 0;
     while (s0 != MS_S) {
       int s1 = -1;
+      bool back = false;                                   // s1 (I or D) steps back to M at once
       if (s0 == MS_M || s0 == MS_D || s0 == MS_I || s0 == MS_C || s0 == MS_J) {
         // A state that can repeat: M along its diagonal, D along its row, I,
         // C and J down their column. Lane l takes the step that follows l
@@ -427,20 +428,37 @@ k_md_region(const DevProfile p, const uint8_t *__restrict__ res, const MdJob *__
         const double roll = (double) xl / 4294967296.0;
         const int li = (s0 == MS_D) ? i : i - z, lk = (s0 == MS_M) ? k - z : (s0 == MS_D) ? k - z : k;
         int sl = -1;
+        // An M run's step to I or D is most often followed by one step back
+        // to M. Lane l also takes that step ahead, as if it stopped here:
+        // the I or D cell is lane l+1's, whose thresholds that lane loads
+        // anyway, and the draw the next one. spec: that step leads to M
+        // (and its M cell is inside the matrix).
+        bool spec = false;
         if (s0 == MS_C || s0 == MS_J) {
           if (li >= 1) {
             const double th = xthr[(size_t) li * 3 + (s0 == MS_J)];
             sl = (roll < th) ? s0 : (th == th) ? MS_E : -1;
           }
+        } else if (s0 == MS_M) {
+          const bool in = (li >= 1 && lk >= 1);
+          double t0 = 0.0, t1 = 0.0, t2 = 0.0, t3 = 0.0, t4 = 0.0;
+          if (in) {
+            const double *th = thr + ((size_t) li * (M + 1) + lk) * 5;
+            t0 = th[0]; t1 = th[1]; t2 = th[2]; t3 = th[3]; t4 = th[4];
+            sl = (roll < t0) ? MS_B : (roll < t1) ? MS_M : (roll < t2) ? MS_I : (t0 == t0) ? MS_D : -1;
+          }
+          const double n3 = __shfl_down(t3, 1, 64), n4 = __shfl_down(t4, 1, 64);   // cell (li - 1, lk - 1)
+          const bool nin = __shfl_down((int) in, 1, 64) != 0 && z < 63;
+          if (nin && (sl == MS_I || sl == MS_D)) {
+            const double tt = (sl == MS_I) ? n4 : n3;
+            const double roll2 = (double) (69069u * xl + 1u) / 4294967296.0;
+            // back to M at (li - 2, lk - 1) from I, (li - 1, lk - 2) from D
+            spec = (roll2 < tt) && ((sl == MS_I) ? li - 2 >= 1 : lk - 2 >= 1);
+          }
         } else if (li >= 1 && lk >= 1) {
           const double *th = thr + ((size_t) li * (M + 1) + lk) * 5;
-          if (s0 == MS_M) {
-            const double t0 = th[0], t1 = th[1], t2 = th[2];
-            sl = (roll < t0) ? MS_B : (roll < t1) ? MS_M : (roll < t2) ? MS_I : (t0 == t0) ? MS_D : -1;
-          } else {
-            const double tt = th[(s0 == MS_D) ? 3 : 4];
-            sl = (roll < tt) ? MS_M : (tt == tt) ? s0 : -1;
-          }
+          const double tt = th[(s0 == MS_D) ? 3 : 4];
+          sl = (roll < tt) ? MS_M : (tt == tt) ? s0 : -1;
         }
         const unsigned long long stop = __ballot(sl != s0);
         const int f = stop ? (int) __ffsll((long long) stop) - 1 : 64;
@@ -468,6 +486,7 @@ k_md_region(const DevProfile p, const uint8_t *__restrict__ res, const MdJob *__
         }
         rng = (uint32_t) __builtin_amdgcn_readlane((int) xl, f);
         s1 = __builtin_amdgcn_readlane(sl, f);
+        back = (s0 == MS_M) && __builtin_amdgcn_readlane((int) spec, f) != 0;
         if (++steps > max_steps) { status = 1; break; }
       } else {
         if (++steps > max_steps) { status = 1; break; }
@@ -617,6 +636,20 @@ k_md_region(const DevProfile p, const uint8_t *__restrict__ res, const MdJob *__
         for (int c = z; c < 4 * Q; c += 64) cnt[c] = 0.0f;
         kmin = 0x7fffffff; kmax = 0;
         md_sync();
+      }
+      // the I or D state's step back to M, taken ahead in the M run's ballot:
+      // its draw, then the M element at (i - 1, k) from I, (i, k - 1) from D
+      if (back) {
+        rng = rng * 69069u + 1u;
+        if (++steps > max_steps) { status = 1; break; }
+        if (s1 == MS_I) i--; else k--;
+        if (k < 1 || k > M || i < 1) { status = 1; break; }
+        if (z == 0) atomicAdd(&cnt[cnt_at(k)], 1.0f);
+        kmin = min(kmin, k); kmax = max(kmax, k);
+        Ld++;
+        if (sqto == 0) { sqto = i; hmmto = k; }
+        sqfrom = i; hmmfrom = k;
+        s1 = MS_M;
       }
       // (N, C and J emit on the transition to themselves: the repeats above)
       s0 = s1;
