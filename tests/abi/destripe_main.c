@@ -1,5 +1,5 @@
-/* Runs the de-striping shim of INTEGRATION.md (p7_oprofile_ToGPU, compiled
- * from the document's own code block against tests/abi/striped_oprofile.h)
+/* Runs the de-striping shim of the integration guide (p7_oprofile_ToGPU,
+ * compiled from the guide's own code block against tests/abi/striped_oprofile.h)
  * on the striped vectors of one model of the reference's pressed profiles,
  * and compares the library tables it loads (p7_gpu_oprofile_Load) with the
  * library's own reading of the same pressed files

@@ -1,5 +1,5 @@
-/* A test double of the P7_OPROFILE fields that the de-striping shim in
- * INTEGRATION.md reads (HMMER/src/impl_sse/impl_sse.h:76-143: the vector
+/* A test double of the P7_OPROFILE fields that the de-striping shim of the
+ * integration guide reads (HMMER/src/impl_sse/impl_sse.h:76-143: the vector
  * types, the striped table pointers, the scalars), filled here from the
  * striped vectors of the reference's pressed profiles, so that the shim's
  * code compiles and runs as written. Only these fields; nothing else of the

@@ -3,8 +3,8 @@
 * a C program built against include/p7_gpu.h prints sizeof and offsetof of
   P7_GPU_TABLES, P7_GPU_HITS and P7_GPU_STATS, which must equal the ctypes
   mirrors in biobench2_amd/p7gpu.py;
-* the de-striping shim of INTEGRATION.md (the code block itself, extracted
-  from the document) is compiled against a test double of the P7_OPROFILE
+* the de-striping shim of the integration guide (the code block itself,
+  extracted from the guide) is compiled against a test double of the P7_OPROFILE
   fields it reads (tests/abi/striped_oprofile.h, SSE vector types) and linked
   with libp7gpu.so. It runs on the striped vectors of the reference's pressed
   profiles (tests/golden/ref/minifam.h3f/.h3p: globins4, fn3, Pkinase;
@@ -29,6 +29,7 @@ from golden.pressed import read_pressed
 ABI = os.path.join(ROOT, "tests", "abi")
 INC = os.path.join(ROOT, "include")
 LIBDIR = os.path.join(ROOT, "biobench2_amd")
+GUIDE = "INTEGRATION.md"          # the integration guide, whose shim code block is compiled here
 
 
 def gcc(out, sources, extra=()):
@@ -62,10 +63,10 @@ def ctypes_sizeof(cls):
 
 
 def shim_source():
-    """The shim's code block from INTEGRATION.md, its HMMER includes replaced by the test double."""
-    text = open(os.path.join(ROOT, "INTEGRATION.md")).read()
+    """The shim's code block from the integration guide, its HMMER includes replaced by the test double."""
+    text = open(os.path.join(ROOT, GUIDE)).read()
     m = re.search(r"```c\n(/\* hmmer side: src/impl_sse/p7_gpu_shim\.c.*?)```", text, re.S)
-    assert m, "INTEGRATION.md has no shim code block"
+    assert m, "the integration guide has no shim code block"
     body = "\n".join(line for line in m.group(1).split("\n") if not line.startswith("#include"))
     return '#include "striped_oprofile.h"\n#include "p7_gpu.h"\n' + body
 
