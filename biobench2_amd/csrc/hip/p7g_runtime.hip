@@ -76,6 +76,7 @@ struct Work {
   int64_t     mdsize = 0;
   float      *mdn2 = nullptr;             // their residue null2 scores, gathered for the host
   int64_t     mdn2_cap = 0;
+  int64_t     pool_cap = 0;               // the caller's cap on each of the two pools (floats; 0 none)
   hipStream_t st2 = nullptr;              // their stream, beside st (made on first use)
   hipEvent_t  md_thr = nullptr;           // on st2: the regions' traces' thresholds are filled
   void       *dom_mem = nullptr;
@@ -1513,7 +1514,8 @@ md_start(Work &w, const DevProfile &p, const std::vector<int4> &rec, const std::
   HIPCHK(hipMemGetInfo(&fr, &tot));
   int64_t need_all = 0;
   for (int64_t k : m.mdk) need_all += md_area_floats(p.M, vlen[(size_t) k], MD_SEGCAP);
-  const int64_t cap = std::max<int64_t>(N2_POOL_MIN, std::min<int64_t>(need_all, (int64_t) (fr / 4 / sizeof(float)) + w.mdsize));
+  int64_t cap = std::max<int64_t>(N2_POOL_MIN, std::min<int64_t>(need_all, (int64_t) (fr / 4 / sizeof(float)) + w.mdsize));
+  if (w.pool_cap > 0) cap = std::min(cap, w.pool_cap);
   int64_t used = 0, maxused = 0, nres = 0, maxres = 0;
   size_t start = 0;
   for (int64_t k : m.mdk) {
@@ -1676,7 +1678,8 @@ null2_pass(Work &w, const DevProfile &p, const DevSeqs &s, const GateThresh &g, 
   if (need_all == 0) return p7G_OK;
   size_t fr = 0, tot = 0;
   HIPCHK(hipMemGetInfo(&fr, &tot));
-  const int64_t cap = std::max<int64_t>(N2_POOL_MIN, std::min<int64_t>(need_all, (int64_t) (fr / 4 / sizeof(float)) + w.n2size));
+  int64_t cap = std::max<int64_t>(N2_POOL_MIN, std::min<int64_t>(need_all, (int64_t) (fr / 4 / sizeof(float)) + w.n2size));
+  if (w.pool_cap > 0) cap = std::min(cap, w.pool_cap);
   std::vector<int64_t> off((size_t) (k1 - k0), 0);
   std::vector<int32_t> list;
   std::vector<std::pair<size_t, size_t>> batches;
@@ -1983,6 +1986,15 @@ p7_gpu_pipeline_SetNull2(P7_GPU_PIPELINE *pli, int on)
   if (! pli) return p7G_EINVAL;
   std::lock_guard<std::mutex> lk(pli->mu);
   pli->do_null2 = on ? 1 : 0;
+  return p7G_OK;
+}
+
+extern "C" int
+p7_gpu_pipeline_SetPoolCap(P7_GPU_PIPELINE *pli, int64_t floats)
+{
+  if (! pli || floats < 0) return p7G_EINVAL;
+  std::lock_guard<std::mutex> lk(pli->mu);
+  pli->w.pool_cap = floats;
   return p7G_OK;
 }
 

@@ -112,6 +112,7 @@ def _load():
         "p7_gpu_Pipeline": (i, [vp, vp, vp, POINTER(Hits)]),
         "p7_gpu_pipeline_GetStats": (i, [vp, POINTER(Stats)]),
         "p7_gpu_pipeline_SetDomains": (i, [vp, i]), "p7_gpu_pipeline_SetNull2": (i, [vp, i]),
+        "p7_gpu_pipeline_SetPoolCap": (i, [vp, i64]),
         "p7_gpu_pipeline_GetRegionNull2": (i, [vp, POINTER(c_float), POINTER(c_int32), c_int64, POINTER(c_int64)]),
         "p7_gpu_pipeline_GetDomainAlignments": (i, [vp, POINTER(c_float), POINTER(c_int32), POINTER(c_int32),
                                                     POINTER(c_int32), POINTER(c_int32), c_int64, POINTER(c_int64)]),
@@ -638,6 +639,10 @@ class Pipeline:
         self.domains = bool(domains)
         _check(lib.p7_gpu_pipeline_SetDomains(self.h, int(self.domains)), "pipeline_SetDomains")
         _check(lib.p7_gpu_pipeline_SetNull2(self.h, int(bool(null2))), "pipeline_SetNull2")
+
+    def set_pool_cap(self, floats):
+        """p7_gpu_pipeline_SetPoolCap: each domain-definition pool at most floats floats (0: no cap)."""
+        _check(lib.p7_gpu_pipeline_SetPoolCap(self.h, int(floats)), "pipeline_SetPoolCap")
 
     def run(self, om, db, scores=True):
         """p7_gpu_Pipeline; with domains=True (and scores) the outputs also hold

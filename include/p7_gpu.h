@@ -306,6 +306,16 @@ int   p7_gpu_RegionTraceEnsemble(const uint8_t *dsq, int Lr, int Ltarget, const 
  * hmmsearch --nonull2 turns it off). On by default; it applies when decoding
  * is on. */
 int   p7_gpu_pipeline_SetNull2(P7_GPU_PIPELINE *pli, int on);
+/* No counterpart in the reference (its domain definition works one target
+ * at a time in host memory): caps each of the device pools that the domain
+ * definition's regions take their working areas from (the envelopes' null2
+ * and alignment areas, the multidomain regions' matrices and thresholds) at
+ * floats floats. 0 (the default): the smaller of what the regions need and
+ * a quarter of the free device memory, at least 2 GiB. Regions whose areas
+ * exceed one pool run in several batches, with the same results; a cap below
+ * one region's area makes p7_gpu_Pipeline fail with p7G_EMEM. For callers
+ * with a device memory budget. */
+int   p7_gpu_pipeline_SetPoolCap(P7_GPU_PIPELINE *pli, int64_t floats);
 /* The region records of the last p7_gpu_Pipeline call with decoding on,
  * ordered by sequence (the database's order), start, then kind: sequence
  * index, first and last residue (1-based, inclusive), the kind (multi): 0 a

@@ -194,3 +194,56 @@ def test_region_ensemble_is_the_oracles_bit_for_bit(gpu, name, n, seed):
             assert float(np.max(np.abs(n2 - g["n2sc"]))) <= 1e-6, (i, j)
             nreg += 1
     assert nreg >= {"fn3": 1, "synth_M50": 10, "synth_M200": 20}[name], nreg
+
+
+def md_area_floats(M, Lr, segcap=6144):
+    """The device's area of one multidomain region, in floats (md_area_floats)."""
+    Q = max(2, (M - 1) // 4 + 1)
+    R = Lr + 1
+    seg_at = R * 12 * Q + 6 * R + R + (max(Lr, M) + 2)
+    thr_at = (seg_at + 7 * segcap + 1) & ~1
+    f = thr_at + 2 * (5 * R * (M + 1) + 3 * R + ((8 * Q + 63) // 64) * R)
+    return (f + 63) & ~63
+
+
+def test_domain_definition_in_several_batches_equals_one_batch(gpu):
+    """With each domain-definition pool capped at about one multidomain
+    region's area (p7_gpu_pipeline_SetPoolCap), the regions run one batch
+    after another on the pipeline's own stream instead of beside the
+    single-domain null2 pass, and the envelopes' null2 areas in several
+    batches too: every region, correction, alignment and score equals the
+    uncapped run's, and a cap below one region's area is p7G_EMEM."""
+    path = synth_profile("synth_M200")
+    hmm = synth.read_hmm_probs(path)
+    seqs = multidomain_db(hmm, 30, 12)
+    rng = np.random.default_rng(13)
+    seqs += [np.concatenate([synth.iid_residues(40, rng), s, synth.iid_residues(40, rng)]).astype(np.uint8)
+             for s in synth.emit_homologs(hmm, 20, rng)]
+    db = gpu.SeqDB.from_list(seqs)
+    om = gpu.OProfile.from_file(path)
+    M = om.tables()["M"]
+
+    def outputs(cap):
+        pli = gpu.Pipeline(domains=True)
+        pli.set_pool_cap(cap)
+        pli.run(om, db)
+        dc, st = pli.region_null2()
+        return pli.regions(), dc, st, pli.domain_alignments(), pli.domain_scores(), pli.target_scores()
+
+    regs, dc, st, al, ds, ts = outputs(0)
+    md = [r for r in regs if r[3] == 1]
+    assert len(md) >= 4 and sum(r[3] == 0 for r in regs) >= 10
+    big = max(md_area_floats(M, r[2] - r[1] + 1) for r in md)
+    regs2, dc2, st2, al2, ds2, ts2 = outputs(big + big // 2)
+    assert regs2 == regs
+    assert np.array_equal(st2, st) and np.array_equal(dc2, dc, equal_nan=True)
+    for k in al:
+        assert np.array_equal(al2[k], al[k], equal_nan=True), k
+    for a, b in zip(ds2, ds):
+        assert np.array_equal(a, b, equal_nan=True)
+    for k in ts:
+        assert np.array_equal(ts2[k], ts[k], equal_nan=True), k
+    pli = gpu.Pipeline(domains=True)
+    pli.set_pool_cap(big // 4)
+    with pytest.raises(gpu.P7GpuError):
+        pli.run(om, db)
