@@ -2268,10 +2268,24 @@ __global__ void k_oa_trace(const DevProfile p, const int32_t *__restrict__ list,
     return (pp[((size_t) (i - 1) * NP + j / 4) * 64 + z] >> ((j % 4) * 8)) & 0xffu;
   };
   int i = L, k = 0, s0 = S_C, nm = 0, ok = 1, nw = 0, last = 0;
+  bool rec = false;                    // s0 is a display state still to be written
   const int guard = 4 * (L + p.M) + 8;
   for (int step = 0; s0 != S_S; step++) {
     int s1;
     if (step > guard || i < 0) { ok = 0; break; }
+    // the display word of s0 (entered last step) is written here, so that its
+    // loads and s0's choice loads go out together
+    const bool mid = (s0 == S_M || s0 == S_I || s0 == S_D) && i >= 1 && k >= 1 && k <= p.M;
+    const uint32_t cw = mid ? cell(i, k) : 0u;
+    if (rec) {
+      const uint32_t b = (s0 == S_D) ? 0u : ppat(i, k);
+      const uint32_t st = (s0 == S_M) ? 0u : (s0 == S_I) ? 1u : 2u;
+      const uint32_t code = (s0 == S_M) ? (b & 15u) : (s0 == S_I) ? (b >> 4) : 0u;
+      const uint32_t x = (s0 == S_D) ? 0u : (uint32_t) sq[i - 1];
+      path[nw++] = st | ((uint32_t) k << 2) | (code << 19) | (x << 23);
+      if (s0 == S_M) last = nw;
+      rec = false;
+    }
     switch (s0) {
       case S_C: s1 = ((rows[i] >> OA_CBIT) & 1u) ? S_C : S_E; break;
       case S_E: {
@@ -2283,19 +2297,19 @@ __global__ void k_oa_trace(const DevProfile p, const int32_t *__restrict__ list,
       }
       case S_M: {
         if (i < 1 || k < 1 || k > p.M) { ok = 0; break; }
-        const int c = (int) (cell(i, k) & 3u);
+        const int c = (int) (cw & 3u);
         s1 = (c == 0) ? S_M : (c == 1) ? S_I : (c == 2) ? S_D : S_B;
         k--; i--;
         break;
       }
       case S_D:
         if (i < 1 || k < 1 || k > p.M) { ok = 0; break; }
-        s1 = ((cell(i, k) >> 3) & 1u) ? S_D : S_M;
+        s1 = ((cw >> 3) & 1u) ? S_D : S_M;
         k--;
         break;
       case S_I:
         if (i < 1 || k < 1 || k > p.M) { ok = 0; break; }
-        s1 = ((cell(i, k) >> 2) & 1u) ? S_I : S_M;
+        s1 = ((cw >> 2) & 1u) ? S_I : S_M;
         i--;
         break;
       case S_B: s1 = ((rows[i] >> OA_BBIT) & 1u) ? S_N : S_J; break;
@@ -2313,12 +2327,7 @@ __global__ void k_oa_trace(const DevProfile p, const int32_t *__restrict__ list,
     // it are the E -> D end, not shown) to the last (B precedes it)
     if ((s1 == S_M || ((s1 == S_I || s1 == S_D) && nm > 0))) {
       if (i < 1 || k < 1 || k > p.M || nw >= pcap) { ok = 0; break; }
-      const uint32_t b = (s1 == S_D) ? 0u : ppat(i, k);
-      const uint32_t st = (s1 == S_M) ? 0u : (s1 == S_I) ? 1u : 2u;
-      const uint32_t code = (s1 == S_M) ? (b & 15u) : (s1 == S_I) ? (b >> 4) : 0u;
-      const uint32_t x = (s1 == S_D) ? 0u : (uint32_t) sq[i - 1];
-      path[nw++] = st | ((uint32_t) k << 2) | (code << 19) | (x << 23);
-      if (s1 == S_M) last = nw;
+      rec = true;                        // its word, at the next step
     }
     if ((s1 == S_N || s1 == S_J || s1 == S_C) && s1 == s0) i--;
     s0 = s1;
