@@ -2231,7 +2231,7 @@ hipError_t launch_copy32(void *dst, const void *src, int64_t nwords, hipStream_t
 }
 
 // p7_OATrace (impl_sse/optacc.c:227-276) over the choices the null2 pass
-// stored (oa_row), one lane per region, from C at the region's last residue
+// stored (oa_row), one wave per region, from C at the region's last residue
 // back to S; keeps the first and last M states (p7_alidisplay_Create's
 // hmmfrom/hmmto/sqfrom/sqto, p7_alidisplay.c:74-145), and writes the
 // alignment itself for the display, from its last M state back to its first
@@ -2240,16 +2240,27 @@ hipError_t launch_copy32(void *dst, const void *src, int64_t nwords, hipStream_t
 // probability code in bits 19-22, the residue code in bits 23-27; at most
 // n2pcap words at n2path + n2poff[region], their number to n2pn[region]
 // (0 without an alignment).
-__global__ void k_oa_trace(const DevProfile p, const int32_t *__restrict__ list, int64_t n,
+//
+// The walk is one thread's, kept uniform over the wave; the wave's lanes
+// speculate on runs: in M, lane l loads the choice at (i-l, k-l), and the
+// leading lanes whose choice is M -> M take their steps at once, each writing
+// its state's word; in C, lane l reads row i-l's C -> C bit likewise. A run's
+// lanes stop wherever the step-by-step walk would fail (bounds, the word
+// cap, the step guard), so that the walk itself meets every failure.
+__global__ void __launch_bounds__(64) k_oa_trace(const DevProfile p, const int32_t *__restrict__ list, int64_t n,
                            const int32_t *__restrict__ vlen, const uint8_t *__restrict__ vres,
                            const int64_t *__restrict__ voff, DomBufs dom)
 {
   enum { S_S = 0, S_N, S_B, S_M, S_I, S_D, S_E, S_J, S_C };
-  const int64_t t = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t t = (int64_t) blockIdx.x;
+  const int lane = (int) threadIdx.x;
   if (t >= n) return;
   const int slot = list[t];
   int4 out = make_int4(0, 0, 0, 0);
-  if (dom.n2st[slot] != 0) { dom.n2ali[slot] = out; dom.n2pn[slot] = 0; return; }   // no decoding, no trace
+  if (dom.n2st[slot] != 0) {                           // no decoding, no trace
+    if (lane == 0) { dom.n2ali[slot] = out; dom.n2pn[slot] = 0; }
+    return;
+  }
   const int L = vlen[slot], B = p.fb_B, NW = n2_oa_words(B);
   const float *area = dom.n2pool + dom.n2off[slot];
   const uint32_t *bits = (const uint32_t *) (area + n2_oa_bits_at(L, B));
@@ -2269,10 +2280,51 @@ __global__ void k_oa_trace(const DevProfile p, const int32_t *__restrict__ list,
   };
   int i = L, k = 0, s0 = S_C, nm = 0, ok = 1, nw = 0, last = 0;
   bool rec = false;                    // s0 is a display state still to be written
+  bool spec = true;                    // a run may start here (false just after a broken run)
   const int guard = 4 * (L + p.M) + 8;
   for (int step = 0; s0 != S_S; step++) {
     int s1;
     if (step > guard || i < 0) { ok = 0; break; }
+    if (spec && s0 == S_M && rec) {
+      // M -> M steps from (i, k): the one from (i-l, k-l) needs both cells in
+      // the model and the matrix, its record below the cap, its step within
+      // the guard; each writes the word of the state it leaves
+      const int lim = min(min(64, pcap - nw - 1), guard - step + 1);
+      const int ii = i - lane, kk = k - lane;
+      bool go = false;
+      uint32_t wd = 0;
+      if (lane < lim && ii >= 2 && kk >= 2 && kk <= p.M) {
+        const uint32_t c = cell(ii, kk), b = ppat(ii, kk), x = (uint32_t) sq[ii - 1];
+        go = (c & 3u) == 0u;
+        wd = ((uint32_t) kk << 2) | ((b & 15u) << 19) | (x << 23);
+      }
+      const unsigned long long stop = __ballot(! go);
+      const int r = stop ? (int) __ffsll((long long) stop) - 1 : 64;
+      if (r > 0) {
+        if (lane < r) path[nw + lane] = wd;
+        nw += r; last = nw; nm += r; i -= r; k -= r;
+        out.x = k; out.z = i;
+        spec = (r == 64);
+        step += r - 1;
+        continue;
+      }
+    }
+    if (spec && s0 == S_C) {
+      // C -> C steps from row i down: row i-l's C bit, the next row in the
+      // matrix, the step within the guard
+      const int lim = min(64, guard - step + 1);
+      const int ii = i - lane;
+      const bool go = lane < lim && ii >= 1 && ((rows[ii] >> OA_CBIT) & 1u);
+      const unsigned long long stop = __ballot(! go);
+      const int r = stop ? (int) __ffsll((long long) stop) - 1 : 64;
+      if (r > 0) {
+        i -= r;
+        spec = (r == 64);
+        step += r - 1;
+        continue;
+      }
+    }
+    spec = true;
     // the display word of s0 (entered last step) is written here, so that its
     // loads and s0's choice loads go out together
     const bool mid = (s0 == S_M || s0 == S_I || s0 == S_D) && i >= 1 && k >= 1 && k <= p.M;
@@ -2282,7 +2334,8 @@ __global__ void k_oa_trace(const DevProfile p, const int32_t *__restrict__ list,
       const uint32_t st = (s0 == S_M) ? 0u : (s0 == S_I) ? 1u : 2u;
       const uint32_t code = (s0 == S_M) ? (b & 15u) : (s0 == S_I) ? (b >> 4) : 0u;
       const uint32_t x = (s0 == S_D) ? 0u : (uint32_t) sq[i - 1];
-      path[nw++] = st | ((uint32_t) k << 2) | (code << 19) | (x << 23);
+      if (lane == 0) path[nw] = st | ((uint32_t) k << 2) | (code << 19) | (x << 23);
+      nw++;
       if (s0 == S_M) last = nw;
       rec = false;
     }
@@ -2333,16 +2386,15 @@ __global__ void k_oa_trace(const DevProfile p, const int32_t *__restrict__ list,
     s0 = s1;
   }
   if (! ok || nm == 0) { out = make_int4(0, 0, 0, 0); last = 0; }
-  dom.n2ali[slot] = out;
-  dom.n2pn[slot] = last;
+  if (lane == 0) { dom.n2ali[slot] = out; dom.n2pn[slot] = last; }
 }
 
 hipError_t launch_oa_trace(const DevProfile &p, const DevSeqs &s, const int32_t *list, int64_t n, const DomBufs &dom,
                            hipStream_t st)
 {
   if (n <= 0) return hipSuccess;
-  const unsigned grid = (unsigned) ((n + 63) / 64);
-  hipLaunchKernelGGL(k_oa_trace, dim3(grid), dim3(64), 0, st, p, list, n, s.len, s.res, s.off, dom);
+  if (n > 0x7fffffff) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_oa_trace, dim3((unsigned) n), dim3(64), 0, st, p, list, n, s.len, s.res, s.off, dom);
   return hipGetLastError();
 }
 
