@@ -365,7 +365,7 @@ hipError_t launch_bck(const DevProfile &p, const DevSeqs &s, const int32_t *list
                       DevLists l = DevLists());
 // The optimal accuracy traceback of each listed region (p7_OATrace,
 // impl_sse/optacc.c:227-423) from the choices the null2 pass stored: one
-// lane per region; the alignment's first and last M states go to dom.n2ali.
+// wave per region; the alignment's first and last M states go to dom.n2ali.
 hipError_t launch_oa_trace(const DevProfile &p, const DevSeqs &s, const int32_t *list, int64_t n, const DomBufs &dom,
                            hipStream_t st);
 
